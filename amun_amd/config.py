@@ -1,0 +1,232 @@
+"""Run configuration of the AMUN per-block update, keyed by the reference's
+parameter names.
+
+``Params`` holds the keys that select the hot path (``equation_system``,
+``reconstruction``, ``limiter``, ``riemann_solver``, ``time_advance``, ``cfl``,
+``gamma``, ``ncells``, ``nghosts``, ``xblocks``... exactly as they appear in an
+AMUN ``params.in``, parsed by parameters.F90:99-686) and converts them to the
+``amun_cfg`` C struct of ``include/amun_dev.h``.  Unknown names raise the same
+integer error codes as the reference initialisers (equations.F90:399-830,
+interpolations.F90:382).
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from dataclasses import dataclass, field, fields
+
+# constants mirrored from include/amun_dev.h
+EQSYS = {"hd": 1, "hydrodynamic": 1, "mhd": 2, "magnetohydrodynamic": 2}
+EOS = {"adi": 1, "adiabatic": 1}
+RECON = {"tvd": 1, "weno5z": 5, "weno5-z": 5}
+LIMITER = {"zero": 0, "mm": 1, "minmod": 1, "mc": 2, "monotonized_central": 2,
+           "sb": 3, "superbee": 3, "vl": 4, "vanleer": 4, "va": 5,
+           "vanalbada": 5}
+RIEMANN = {"hll": 1, "hllc": 2, "hlld": 3}
+INTEGRATOR = {"euler": 0, "rk2": 1, "rk3": 2}
+PRECISION = {"fp64": 0, "mixed": 1}
+
+
+class AmunCfg(ctypes.Structure):
+    """ctypes mirror of ``amun_cfg`` (include/amun_dev.h)."""
+    _fields_ = [
+        ("ndims", ctypes.c_int32), ("eqsys", ctypes.c_int32),
+        ("eos", ctypes.c_int32), ("nv", ctypes.c_int32),
+        ("ncells", ctypes.c_int32), ("nghosts", ctypes.c_int32),
+        ("xblocks", ctypes.c_int32), ("yblocks", ctypes.c_int32),
+        ("zblocks", ctypes.c_int32),
+        ("reconstruction", ctypes.c_int32), ("limiter", ctypes.c_int32),
+        ("riemann_solver", ctypes.c_int32), ("time_advance", ctypes.c_int32),
+        ("precision", ctypes.c_int32),
+        ("fix_positivity", ctypes.c_int32), ("clip_extrema", ctypes.c_int32),
+        ("gamma", ctypes.c_double), ("cfl", ctypes.c_double),
+        ("eps", ctypes.c_double), ("alpha", ctypes.c_double),
+        ("xmin", ctypes.c_double), ("xmax", ctypes.c_double),
+        ("ymin", ctypes.c_double), ("ymax", ctypes.c_double),
+        ("zmin", ctypes.c_double), ("zmax", ctypes.c_double),
+    ]
+
+
+class ConfigError(ValueError):
+    """Invalid parameter, carrying the reference's integer error code."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+def _onoff(v) -> int:
+    if isinstance(v, bool):
+        return int(v)
+    return int(str(v).strip().lower() in ("on", "t", "y", "true", "yes", "1"))
+
+
+@dataclass
+class Params:
+    """The hot-path subset of an AMUN parameter file (defaults = reference
+    defaults: equations.F90:130-200, interpolations.F90:80-118, 220-238,
+    evolution.F90:65-80, coordinates.F90:42-80)."""
+    ndims: int = 3
+    equation_system: str = "hd"
+    equation_of_state: str = "adi"
+    reconstruction: str = "tvd"
+    limiter: str = "mm"
+    riemann_solver: str = "hlld"
+    time_advance: str = "rk2"
+    precision: str = "fp64"
+    fix_positivity: str = "off"
+    clip_extrema: str = "off"
+    gamma: float = 5.0 / 3.0
+    cfl: float = 0.5
+    eps: float = 2.220446049250313e-16
+    alpha: float = 2.0
+    ncells: int = 8
+    nghosts: int = 2
+    xblocks: int = 1
+    yblocks: int = 1
+    zblocks: int = 1
+    xmin: float = 0.0
+    xmax: float = 1.0
+    ymin: float = 0.0
+    ymax: float = 1.0
+    zmin: float = 0.0
+    zmax: float = 1.0
+    extra: dict = field(default_factory=dict)
+
+    # ---------------------------------------------------------------- derived
+    @property
+    def nv(self) -> int:
+        return 5 if EQSYS[self.equation_system.lower()] == 1 else 9
+
+    @property
+    def shape(self):
+        """Global grid shape (nz, ny, nx) of the base level."""
+        nx = self.xblocks * self.ncells
+        ny = self.yblocks * self.ncells
+        nz = self.zblocks * self.ncells if self.ndims == 3 else 1
+        return (nz, ny, nx)
+
+    @property
+    def nblocks(self) -> int:
+        return self.xblocks * self.yblocks * (self.zblocks if self.ndims == 3 else 1)
+
+    @property
+    def ncells_total(self) -> int:
+        nz, ny, nx = self.shape
+        return nz * ny * nx
+
+    def to_cfg(self) -> AmunCfg:
+        def pick(table, key, name, code):
+            k = str(key).strip().lower()
+            if k not in table:
+                raise ConfigError(code, f"unsupported {name}: {key}")
+            return table[k]
+
+        c = AmunCfg()
+        c.ndims = int(self.ndims)
+        c.eqsys = pick(EQSYS, self.equation_system, "equation_system", 100)
+        c.eos = pick(EOS, self.equation_of_state, "equation_of_state", 110)
+        c.nv = self.nv
+        c.ncells = int(self.ncells)
+        c.nghosts = int(self.nghosts)
+        c.xblocks, c.yblocks, c.zblocks = int(self.xblocks), int(self.yblocks), int(self.zblocks)
+        c.reconstruction = pick(RECON, self.reconstruction, "reconstruction", 200)
+        # unknown limiter names select limiter_zero (interpolations.F90:394-397)
+        c.limiter = LIMITER.get(str(self.limiter).strip().lower(), 0)
+        c.riemann_solver = pick(RIEMANN, self.riemann_solver, "riemann_solver", 300)
+        c.time_advance = pick(INTEGRATOR, self.time_advance, "time_advance", 400)
+        c.precision = pick(PRECISION, self.precision, "precision", 500)
+        c.fix_positivity = _onoff(self.fix_positivity)
+        c.clip_extrema = _onoff(self.clip_extrema)
+        c.gamma, c.cfl, c.eps, c.alpha = self.gamma, self.cfl, self.eps, self.alpha
+        c.xmin, c.xmax, c.ymin, c.ymax = self.xmin, self.xmax, self.ymin, self.ymax
+        c.zmin, c.zmax = self.zmin, self.zmax
+        return c
+
+    def dx(self):
+        nz, ny, nx = self.shape
+        d = [(self.xmax - self.xmin) / nx, (self.ymax - self.ymin) / ny]
+        if self.ndims == 3:
+            d.append((self.zmax - self.zmin) / nz)
+        return d
+
+    def to_params_in(self) -> str:
+        """Text of an AMUN params.in with these settings (periodic, one level)."""
+        eq = {1: "hd", 2: "mhd"}[EQSYS[self.equation_system.lower()]]
+        lines = [
+            f"equation_system = {eq}", "equation_of_state = adi",
+            f"reconstruction = {self.reconstruction}", f"limiter = {self.limiter}",
+            f"riemann_solver = {self.riemann_solver}",
+            f"time_advance = {self.time_advance}",
+            f"fix_positivity = {self.fix_positivity}",
+            f"clip_extrema = {self.clip_extrema}",
+            f"gamma = {self.gamma!r}", f"cfl = {self.cfl!r}", f"eps = {self.eps!r}",
+            f"alpha = {self.alpha!r}",
+            f"ncells = {self.ncells}", f"nghosts = {self.nghosts}",
+            f"xblocks = {self.xblocks}", f"yblocks = {self.yblocks}",
+            f"zblocks = {self.zblocks}",
+            f"xmin = {self.xmin!r}", f"xmax = {self.xmax!r}",
+            f"ymin = {self.ymin!r}", f"ymax = {self.ymax!r}",
+            f"zmin = {self.zmin!r}", f"zmax = {self.zmax!r}",
+            "minlev = 1", "maxlev = 1",
+            "xlbndry = periodic", "xubndry = periodic",
+            "ylbndry = periodic", "yubndry = periodic",
+            "zlbndry = periodic", "zubndry = periodic",
+            "problem = blast", "snapshot_interval = 0", "restart_interval = 0",
+            "integrals_interval = 1000000", "tmax = 1.0e+30", "nmax = 1000000000",
+        ]
+        for k, v in self.extra.items():
+            lines.append(f"{k} = {v}")
+        return "\n".join(lines) + "\n"
+
+    @classmethod
+    def from_text(cls, text: str, ndims: int = 3) -> "Params":
+        """Parse ``key = value`` lines the way parameters.F90 does: ``#``
+        comments, optional quotes, last occurrence wins, unknown keys kept in
+        ``extra``."""
+        p = cls(ndims=ndims)
+        names = {f.name for f in fields(cls)} - {"extra", "ndims"}
+        for raw in text.splitlines():
+            line = raw.split("#", 1)[0].strip()
+            if "=" not in line:
+                continue
+            k, v = (s.strip() for s in line.split("=", 1))
+            v = v.strip("'\"")
+            if k in names:
+                cur = getattr(p, k)
+                if isinstance(cur, bool) or isinstance(cur, str):
+                    setattr(p, k, v)
+                elif isinstance(cur, int):
+                    setattr(p, k, int(v))
+                else:
+                    setattr(p, k, float(v.replace("d", "e").replace("D", "e")))
+            else:
+                p.extra[k] = v
+        return p
+
+
+def c1_params() -> Params:
+    """2D adiabatic hydro blast, 128^2 as 8x8 blocks of 16^2 (+2 ghosts),
+    TVD-MC + HLLC + SSPRK(2,2), CFL 0.4, periodic, gamma 1.4."""
+    return Params(ndims=2, equation_system="hd", reconstruction="tvd",
+                  limiter="mc", riemann_solver="hllc", time_advance="rk2",
+                  gamma=1.4, cfl=0.4, ncells=16, nghosts=2, xblocks=8,
+                  yblocks=8, zblocks=1, xmin=-0.5, xmax=0.5, ymin=-0.5,
+                  ymax=0.5)
+
+
+def c2_params(n: int = 256, precision: str = "fp64") -> Params:
+    """3D ideal MHD with GLM cleaning, n^3 as (n/16)^3 blocks of 16^3
+    (+4 ghosts), WENO5-Z + HLLD + SSPRK(3,3), CFL 0.3, periodic, gamma 5/3.
+    The TVD limiter of the WENO5-Z edge points keeps the reference default
+    (minmod)."""
+    nb = n // 16
+    return Params(ndims=3, equation_system="mhd", reconstruction="weno5z",
+                  limiter="mm", riemann_solver="hlld", time_advance="rk3",
+                  precision=precision, gamma=5.0 / 3.0, cfl=0.3, ncells=16,
+                  nghosts=4, xblocks=nb, yblocks=nb, zblocks=nb, xmin=-0.5,
+                  xmax=0.5, ymin=-0.5, ymax=0.5, zmin=-0.5, zmax=0.5)
+
+
+def psi_decay(p: Params) -> float:
+    return math.exp(-p.alpha * p.cfl)

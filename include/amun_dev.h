@@ -1,0 +1,153 @@
+/*
+ * amun_dev.h -- C-ABI of the MI355X-native AMUN per-block finite-volume update.
+ *
+ * The library (amun_amd/libamun_hip.so) keeps the block arrays resident in
+ * HBM in structure-of-arrays layout and runs the whole time step -- CFL
+ * reduction, reconstruction, Riemann fluxes, flux divergence, SSPRK stage
+ * combine, primitive recovery and the ghost-zone exchange between blocks --
+ * as hand-written gfx950 kernels.  Every entry point takes plain pointers and
+ * sizes; a Fortran host binds them with iso_c_binding (amun_amd/host/
+ * amun_dev_mod.F90), Python with ctypes (amun_amd/_lib.py).
+ *
+ * Return codes: 0 = success.  Configuration errors reuse the reference's
+ * integer codes (100/110 equation system / equation of state,
+ * equations.F90:399-830; 200-202 reconstruction, interpolations.F90:382),
+ * 300 = Riemann solver / equation system mismatch, 400 = integrator,
+ * 500 = bad sizes or arguments, 600 = unsupported combination on the device
+ * path (for instance too few ghost zones for the stencil); HIP runtime errors
+ * are reported as 1000 + hipError_t.  amun_dev_last_error() returns a message.
+ *
+ * Host array layout.  The reference stores every block as u0/u1/q(nv,im,jm,km)
+ * in Fortran order with im = ncells + 2*nghosts (blocks.F90:1120-1191); the
+ * *_blocks transfer calls take exactly that layout, blocks one after another.
+ * The *_global calls take a whole uniform grid as U[nv][nz][ny][nx] (C order,
+ * x fastest), which is what tests and benchmarks use.
+ */
+#ifndef AMUN_DEV_H
+#define AMUN_DEV_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* equation systems: equations.F90:319-830 (parameter "equation_system") */
+#define AMUN_EQSYS_HD      1
+#define AMUN_EQSYS_MHD     2
+/* equations of state: equations.F90:345-395 (parameter "equation_of_state") */
+#define AMUN_EOS_ADI       1
+
+/* reconstructions: interpolations.F90:237-376 (parameter "reconstruction") */
+#define AMUN_REC_TVD       1
+#define AMUN_REC_WENO5Z    5
+
+/* TVD limiters: interpolations.F90:379-398 (parameter "limiter") */
+#define AMUN_LIM_ZERO      0
+#define AMUN_LIM_MINMOD    1
+#define AMUN_LIM_MC        2
+#define AMUN_LIM_SUPERBEE  3
+#define AMUN_LIM_VANLEER   4
+#define AMUN_LIM_VANALBADA 5
+
+/* Riemann solvers: schemes.F90:114-522 (parameter "riemann_solver") */
+#define AMUN_RS_HLL        1
+#define AMUN_RS_HLLC       2
+#define AMUN_RS_HLLD       3
+
+/* integrators: evolution.F90:156-200 (parameter "time_advance") */
+#define AMUN_TA_EULER      0
+#define AMUN_TA_RK2        1   /* SSPRK(2,2), evolve_rk2 evolution.F90:628 */
+#define AMUN_TA_RK3        2   /* SSPRK(3,3), evolve_rk3 evolution.F90:1005 */
+
+/* arithmetic of reconstruction + fluxes; the conserved state is always FP64 */
+#define AMUN_PREC_FP64     0
+#define AMUN_PREC_MIXED    1   /* FP32 reconstruction/fluxes, FP64 Kahan state */
+
+/* Run configuration.  Field names follow the reference parameter names. */
+typedef struct amun_cfg {
+  int32_t ndims;            /* 2 or 3 (compile-time NDIMS in the reference) */
+  int32_t eqsys;            /* AMUN_EQSYS_*                                  */
+  int32_t eos;              /* AMUN_EOS_*                                    */
+  int32_t nv;               /* filled in by amun_dev_init                    */
+  int32_t ncells;           /* "ncells": cells per block edge                */
+  int32_t nghosts;          /* "nghosts": ghost layers of host block arrays  */
+  int32_t xblocks, yblocks, zblocks;  /* "xblocks" ...: base-level blocks    */
+  int32_t reconstruction;   /* AMUN_REC_*                                    */
+  int32_t limiter;          /* AMUN_LIM_*  ("limiter")                       */
+  int32_t riemann_solver;   /* AMUN_RS_*                                     */
+  int32_t time_advance;     /* AMUN_TA_*                                     */
+  int32_t precision;        /* AMUN_PREC_*                                   */
+  int32_t fix_positivity;   /* "fix_positivity" (0 = off, default)           */
+  int32_t clip_extrema;     /* "clip_extrema"   (0 = off, default)           */
+  double  gamma;            /* "gamma"                                       */
+  double  cfl;              /* "cfl"                                         */
+  double  eps;              /* "eps" of the WENO weights (default epsilon)   */
+  double  alpha;            /* "alpha": GLM psi decay, psi *= exp(-alpha*cfl)*/
+  double  xmin, xmax, ymin, ymax, zmin, zmax;   /* domain bounds             */
+} amun_cfg;
+
+typedef struct amun_ctx amun_ctx;
+
+/* Fill *cfg with the reference defaults (equations.F90:166-200,
+ * interpolations.F90:80-118, evolution.F90:65-80, coordinates.F90:42-80). */
+void amun_dev_default_cfg(amun_cfg* cfg);
+
+/* Create a device context for one run: validates the configuration like the
+ * initialize_* routines (equations.F90:257, interpolations.F90:144,
+ * schemes.F90:114, evolution.F90:115), builds the base-level periodic block
+ * topology of setup_domain_default (domains.F90:136-500) and allocates the
+ * device block pool.  Replaces the module state set up by those routines. */
+int  amun_dev_init(const amun_cfg* cfg, amun_ctx** ctx);
+
+/* Release every device allocation of the context (finalize_* routines). */
+void amun_dev_finalize(amun_ctx* ctx);
+
+/* Upload conserved variables.  _blocks: nblk host blocks u0(nv,im,jm,km)
+ * (blocks.F90:1120-1191 layout; ghosts are ignored).  _global: U[nv][nz][ny][nx].
+ * Both recompute the primitive variables (update_primitive_variables,
+ * equations.F90:1088) on the device. */
+int  amun_dev_upload_blocks(amun_ctx* ctx, const double* u);
+int  amun_dev_upload_global(amun_ctx* ctx, const double* u);
+
+/* Download conserved (and optionally primitive, q may be NULL) variables.
+ * _blocks fills the ghost layers too, as after boundary_variables
+ * (boundaries.F90:372-479). */
+int  amun_dev_download_blocks(amun_ctx* ctx, double* u, double* q);
+int  amun_dev_download_global(amun_ctx* ctx, double* u, double* q);
+
+/* new_time_step (evolution.F90:379-493): cmax reduction over all interior
+ * cells and dt = min(cfl*dx_min/cmax, dtnext) when dtnext > 0. */
+int  amun_dev_new_time_step(amun_ctx* ctx, double dtnext, double* dt,
+                            double* cmax);
+
+/* advance (evolution.F90:292-363) for a single-level mesh: new_time_step,
+ * then every stage of the selected integrator (update_fluxes, update_increment,
+ * stage combine, psi decay, update_variables), all on the device. */
+int  amun_dev_step(amun_ctx* ctx, double dtnext, double* dt, double* cmax);
+
+/* Run nsteps steps back to back on the device stream (the host reads dt only
+ * at the end); dts (may be NULL) receives the dt of every step. */
+int  amun_dev_run(amun_ctx* ctx, int32_t nsteps, double dtnext, double* dts);
+
+/* Volume integrals of store_integrals (integrals.F90:254-485), FP64 on the
+ * device: out[0]=mass, [1..3]=momentum, [4]=total energy, [5]=kinetic,
+ * [6]=magnetic energy, [7]=internal energy. */
+int  amun_dev_integrals(amun_ctx* ctx, double out[8]);
+
+/* Simulation time and step counter kept by the context (evolution.F90:75). */
+int  amun_dev_get_time(amun_ctx* ctx, double* time, int64_t* step);
+
+/* Number of blocks, and the base-level block coordinates (nblk x 3 int32,
+ * 0-based) in the order used by the *_blocks transfers. */
+int  amun_dev_nblocks(amun_ctx* ctx, int32_t* nblk);
+int  amun_dev_block_coords(amun_ctx* ctx, int32_t* coords);
+
+/* Message of the last failing call of this thread. */
+const char* amun_dev_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AMUN_DEV_H */

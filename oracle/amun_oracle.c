@@ -1,0 +1,1115 @@
+/*
+ * amun_oracle.c -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C, single-threaded CPU restatement of the AMUN per-block
+ * finite-volume update, written for this project to serve as the checker of
+ * the HIP kernels.  Only tests/, __graft_entry__.smoke() and the cpu_baseline
+ * leg of bench.py load it (liboracle.so, through ctypes); the product path
+ * never does.
+ *
+ * It follows the reference structure and operation order closely, so that on
+ * the x86-64 baseline ISA (no FMA, -ffp-contract=off) it reproduces the
+ * reference binaries built by oracle/build_ref.sh to round-off:
+ *   - block arrays u0, u1, q (nv,im,jm,km) and f (NDIMS,nv,im,jm,km) in
+ *     Fortran order with ghost layers (blocks.F90:1120-1191);
+ *   - update_flux_hd_adi / update_flux_mhd_adi (schemes.F90:1080-1223,
+ *     2970-3138) over full pencils with the component rotation;
+ *   - interfaces_tvd (interpolations.F90:729-848), interfaces_dir (867-959)
+ *     with reconstruct_weno5z (1937-2083), limiters (4684-4877);
+ *   - riemann_hd_adi_hll (schemes.F90:1248), _hllc (1366-1513),
+ *     riemann_mhd_adi_hll (3163), _hlld (3596-4203);
+ *   - prim2cons / cons2prim / fluxspeed / maxspeed (equations.F90:1762-2044,
+ *     2815-3131), update_primitive_variables (1088-1148);
+ *   - update_increment (evolution.F90:1898-1977), evolve_euler/rk2/rk3
+ *     (517, 628-770, 1005-1208), new_time_step (379-493);
+ *   - same-level ghost exchange of q (boundaries.F90:372-479 with the face,
+ *     edge and corner copies 1273, 2447, 3867) on the periodic base-level
+ *     block grid of setup_domain_default (domains.F90:136-500), followed by
+ *     update_ghost_cells (boundaries.F90:6633).
+ *
+ * Pins (see tests/): golden vectors written by the reference binaries
+ * (tests/golden/), round trips, consistency F(u,u)=F(u), exact discrete
+ * conservation, uniform-state preservation, polynomial exactness of WENO5-Z.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../include/amun_dev.h"
+
+/* ------------------------------------------------------------------------ */
+/* configuration derived from amun_cfg, mirroring the module variables       */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+  amun_cfg c;
+  int nd, nv, nc, ng;
+  int im, jm, km;          /* block sizes incl. ghosts                       */
+  int ib, ie, jb, je, kb, ke, ibl, ieu, jbl, jeu, kbl, keu;  /* 0-based      */
+  int idn, ivx, ivy, ivz, imx, imy, imz, ibx, iby, ibz, ibp, ipr, ien;
+  double gammam1, gammam1i;
+  double cflx;             /* cfl as adjusted by initialize_evolution        */
+  double decay;            /* exp(-alpha*cfl), evolution.F90:211-213         */
+  double adx, ady, adz, adxi, adyi, adzi;
+  double cmax, cmax2;      /* equations module state                         */
+  long   npos_events;      /* "Positive variable is not positive" prints     */
+} ocfg;
+
+static int oc_setup(ocfg* o, const amun_cfg* c)
+{
+  memset(o, 0, sizeof(*o));
+  o->c  = *c;
+  o->nd = c->ndims;
+  o->nc = c->ncells;
+  o->ng = c->nghosts;
+  if (o->nd != 2 && o->nd != 3) return 500;
+  if (c->eos != AMUN_EOS_ADI) return 110;
+  if (c->eqsys == AMUN_EQSYS_HD) {
+    o->nv = 5;
+    o->idn = 0; o->ivx = 1; o->ivy = 2; o->ivz = 3;
+    o->imx = 1; o->imy = 2; o->imz = 3;
+    o->ibx = o->iby = o->ibz = o->ibp = -1;
+    o->ipr = o->ien = 4;
+  } else if (c->eqsys == AMUN_EQSYS_MHD) {
+    o->nv = 9;
+    o->idn = 0; o->ivx = 1; o->ivy = 2; o->ivz = 3;
+    o->imx = 1; o->imy = 2; o->imz = 3;
+    o->ibx = 4; o->iby = 5; o->ibz = 6; o->ibp = 7;
+    o->ipr = o->ien = 8;
+  } else {
+    return 100;
+  }
+  o->im = o->nc + 2 * o->ng;
+  o->jm = o->nc + 2 * o->ng;
+  o->km = (o->nd == 3) ? o->nc + 2 * o->ng : 1;
+  o->ib = o->ng; o->ie = o->ng + o->nc - 1;
+  o->jb = o->ng; o->je = o->ng + o->nc - 1;
+  o->ibl = o->ib - 1; o->ieu = o->ie + 1;
+  o->jbl = o->jb - 1; o->jeu = o->je + 1;
+  if (o->nd == 3) {
+    o->kb = o->ng; o->ke = o->ng + o->nc - 1;
+    o->kbl = o->kb - 1; o->keu = o->ke + 1;
+  } else {
+    o->kb = o->ke = o->kbl = o->keu = 0;
+  }
+  o->gammam1  = c->gamma - 1.0;
+  o->gammam1i = 1.0 / o->gammam1;
+  o->cflx  = c->cfl;       /* euler, rk2 and rk3 keep the CFL unchanged */
+  o->decay = exp(-c->alpha * o->cflx);
+  {
+    int ir = c->xblocks, jr = c->yblocks, kr = (o->nd == 3) ? c->zblocks : 1;
+    double xlen = c->xmax - c->xmin, ylen = c->ymax - c->ymin;
+    double zlen = c->zmax - c->zmin;
+    o->adx = xlen / (ir * o->nc);
+    o->ady = ylen / (jr * o->nc);
+    o->adz = (o->nd == 3) ? zlen / (kr * o->nc) : 1.0;
+    o->adxi = 1.0 / o->adx;
+    o->adyi = 1.0 / o->ady;
+    o->adzi = (o->nd == 3) ? 1.0 / o->adz : 1.0;
+  }
+  return 0;
+}
+
+#define QI(o, p, i, j, k) ((((size_t)(k) * (o)->jm + (j)) * (o)->im + (i)) * (o)->nv + (p))
+
+/* ------------------------------------------------------------------------ */
+/* limiters, interpolations.F90:4684-4877                                     */
+/* ------------------------------------------------------------------------ */
+static double lim(int kind, double x, double a, double b)
+{
+  double c;
+  switch (kind) {
+  case AMUN_LIM_MINMOD:     /* 4731 */
+    return 0.5 * (copysign(x, a) + copysign(x, b)) * fmin(fabs(a), fabs(b));
+  case AMUN_LIM_MC:         /* 4765 */
+    return (copysign(x, a) + copysign(x, b))
+         * fmin(fmin(fabs(a), fabs(b)), 0.25 * fabs(a + b));
+  case AMUN_LIM_SUPERBEE:   /* 4799 */
+    return 0.5 * (copysign(x, a) + copysign(x, b))
+         * fmax(fmin(2.0 * fabs(a), fabs(b)), fmin(fabs(a), 2.0 * fabs(b)));
+  case AMUN_LIM_VANLEER:    /* 4834 */
+    c = a * b;
+    if (c > 0.0) return 2.0 * x * c / (a + b);
+    return 0.0;
+  case AMUN_LIM_VANALBADA:  /* 4873: eps = epsilon(1d0) default */
+    return x * a * b * (a + b) / fmax(2.220446049250313e-16, a * a + b * b);
+  default:                  /* limiter_zero 4700 */
+    return 0.0;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* reconstruct_weno5z, interpolations.F90:1937-2083, one pencil of length n   */
+/* ------------------------------------------------------------------------ */
+static void weno5z(const ocfg* o, int n, const double* f, double* fl,
+                   double* fr, double* dfm, double* dfp, double* df2)
+{
+  const double c1 = 13.0 / 12.0, c2 = 0.25;
+  const double dl = 0.1, dc = 0.6, dr = 0.3;
+  const double a11 = 2.0 / 6.0, a12 = -7.0 / 6.0, a13 = 11.0 / 6.0;
+  const double a21 = -1.0 / 6.0, a22 = 5.0 / 6.0, a23 = 2.0 / 6.0;
+  const double a31 = 2.0 / 6.0, a32 = 5.0 / 6.0, a33 = -1.0 / 6.0;
+  const double eps = o->c.eps;
+  int i;
+  for (i = 0; i < n - 1; i++) {
+    dfp[i]     = f[i + 1] - f[i];
+    dfm[i + 1] = dfp[i];
+  }
+  dfm[0]     = dfp[0];
+  dfp[n - 1] = dfm[n - 1];
+  for (i = 0; i < n; i++) {
+    double d = dfp[i] - dfm[i];
+    df2[i] = c1 * (d * d);
+  }
+  for (i = 2; i < n - 2; i++) {
+    double bl, bc, br, tt, al, ac, ar, wl, wc, wr, ww, ql, qc, qr, t;
+    t  = 3.0 * dfm[i] - dfm[i - 1];
+    bl = df2[i - 1] + c2 * (t * t);
+    t  = dfp[i] + dfm[i];
+    bc = df2[i] + c2 * (t * t);
+    t  = 3.0 * dfp[i] - dfp[i + 1];
+    br = df2[i + 1] + c2 * (t * t);
+    tt = fabs(br - bl);
+    al = 1.0 + tt / (bl + eps);
+    ac = 1.0 + tt / (bc + eps);
+    ar = 1.0 + tt / (br + eps);
+
+    wl = dl * al; wc = dc * ac; wr = dr * ar;
+    ww = (wl + wr) + wc;
+    wl = wl / ww; wr = wr / ww; wc = 1.0 - (wl + wr);
+    ql = a11 * f[i - 2] + a12 * f[i - 1] + a13 * f[i];
+    qc = a21 * f[i - 1] + a22 * f[i] + a23 * f[i + 1];
+    qr = a31 * f[i] + a32 * f[i + 1] + a33 * f[i + 2];
+    fl[i] = (wl * ql + wr * qr) + wc * qc;
+
+    wl = dl * ar; wc = dc * ac; wr = dr * al;
+    ww = (wl + wr) + wc;
+    wl = wl / ww; wr = wr / ww; wc = 1.0 - (wl + wr);
+    ql = a11 * f[i + 2] + a12 * f[i + 1] + a13 * f[i];
+    qc = a21 * f[i + 1] + a22 * f[i] + a23 * f[i - 1];
+    qr = a31 * f[i] + a32 * f[i - 1] + a33 * f[i - 2];
+    fr[i - 1] = (wl * ql + wr * qr) + wc * qc;
+  }
+  {
+    double df;
+    fl[0] = 0.5 * (f[0] + f[1]);
+    df    = lim(o->c.limiter, 0.5, dfm[1], dfp[1]);
+    fr[0] = f[1] - df;
+    fl[1] = f[1] + df;
+    i     = n - 2;
+    df    = lim(o->c.limiter, 0.5, dfm[i], dfp[i]);
+    fr[i - 1] = f[i] - df;
+    fl[i]     = f[i] + df;
+    fr[i]     = 0.5 * (f[i] + f[n - 1]);
+    fl[n - 1] = f[n - 1];
+    fr[n - 1] = f[n - 1];
+  }
+}
+
+/* fix_positivity, interpolations.F90:4987-5064 (active if fix_positivity) */
+static void fix_positivity(int n, const double* f, double* fl, double* fr)
+{
+  int i;
+  for (i = 0; i < n; i++) {
+    if (fl[i] <= 0.0) {
+      int im1 = (i - 1 > 0) ? i - 1 : 0;
+      fl[i]   = f[i];
+      fr[im1] = f[i];
+    }
+    if (fr[i] <= 0.0) {
+      int ip1 = (i + 1 < n - 1) ? i + 1 : n - 1;
+      fl[ip1] = f[ip1];
+      fr[i]   = f[ip1];
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* reconstruct_interfaces (schemes.F90:591-633) for one block                 */
+/* qs layout: [(d*2 + s)][k][j][i][p]                                         */
+/* ------------------------------------------------------------------------ */
+static void reconstruct_interfaces(ocfg* o, const double* q, double* qs)
+{
+  const int im = o->im, jm = o->jm, km = o->km, nv = o->nv, nd = o->nd;
+  const size_t ncell = (size_t)im * jm * km;
+  int p, d, s, i, j, k;
+#define QS(d, s, i, j, k, p) qs[((size_t)((d) * 2 + (s)) * ncell + ((size_t)(k) * jm + (j)) * im + (i)) * nv + (p)]
+  /* copy ghost zones: qi = q everywhere first, the interior region is
+   * overwritten below (interpolations.F90:767-778) */
+  for (d = 0; d < nd; d++)
+    for (s = 0; s < 2; s++)
+      memcpy(&qs[(size_t)(d * 2 + s) * ncell * nv], q, ncell * nv * sizeof(double));
+
+  for (p = 0; p < nv; p++) {
+    int positive = (p == o->idn || p == o->ipr);
+    if (o->c.reconstruction == AMUN_REC_TVD) {
+      /* interfaces_tvd, interpolations.F90:729-848 */
+      for (k = o->kbl; k <= o->keu; k++)
+        for (j = o->jbl; j <= o->jeu; j++)
+          for (i = o->ibl; i <= o->ieu; i++) {
+            double dql, dqr, dq[3] = {0.0, 0.0, 0.0};
+            double qc = q[QI(o, p, i, j, k)];
+            dql = qc - q[QI(o, p, i - 1, j, k)];
+            dqr = q[QI(o, p, i + 1, j, k)] - qc;
+            dq[0] = lim(o->c.limiter, 0.5, dql, dqr);
+            dql = qc - q[QI(o, p, i, j - 1, k)];
+            dqr = q[QI(o, p, i, j + 1, k)] - qc;
+            dq[1] = lim(o->c.limiter, 0.5, dql, dqr);
+            if (nd == 3) {
+              dql = qc - q[QI(o, p, i, j, k - 1)];
+              dqr = q[QI(o, p, i, j, k + 1)] - qc;
+              dq[2] = lim(o->c.limiter, 0.5, dql, dqr);
+            }
+            if (positive) {
+              if (qc > 0.0) {
+                for (;;) {
+                  double sa = fabs(dq[0]) + fabs(dq[1]);
+                  if (nd == 3) sa += fabs(dq[2]);
+                  if (!(qc <= sa)) break;
+                  dq[0] *= 0.5; dq[1] *= 0.5; dq[2] *= 0.5;
+                }
+              } else {
+                o->npos_events++;
+                dq[0] = dq[1] = dq[2] = 0.0;
+              }
+            }
+            QS(0, 0, i, j, k, p)     = qc + dq[0];
+            QS(0, 1, i - 1, j, k, p) = qc - dq[0];
+            QS(1, 0, i, j, k, p)     = qc + dq[1];
+            QS(1, 1, i, j - 1, k, p) = qc - dq[1];
+            if (nd == 3) {
+              QS(2, 0, i, j, k, p)     = qc + dq[2];
+              QS(2, 1, i, j, k - 1, p) = qc - dq[2];
+            }
+          }
+    } else {
+      /* interfaces_dir, interpolations.F90:867-959 */
+      int nmax = im > jm ? im : jm;
+      double *fp, *fl, *fr, *w1, *w2, *w3;
+      if (km > nmax) nmax = km;
+      fp = (double*)malloc(6 * (size_t)nmax * sizeof(double));
+      fl = fp + nmax; fr = fl + nmax; w1 = fr + nmax; w2 = w1 + nmax; w3 = w2 + nmax;
+      for (k = o->kbl; k <= o->keu; k++) {
+        for (j = o->jbl; j <= o->jeu; j++) {
+          for (i = 0; i < im; i++) fp[i] = q[QI(o, p, i, j, k)];
+          weno5z(o, im, fp, fl, fr, w1, w2, w3);
+          if (positive && o->c.fix_positivity) fix_positivity(im, fp, fl, fr);
+          for (i = 0; i < im; i++) { QS(0, 0, i, j, k, p) = fl[i]; QS(0, 1, i, j, k, p) = fr[i]; }
+        }
+        for (i = o->ibl; i <= o->ieu; i++) {
+          for (j = 0; j < jm; j++) fp[j] = q[QI(o, p, i, j, k)];
+          weno5z(o, jm, fp, fl, fr, w1, w2, w3);
+          if (positive && o->c.fix_positivity) fix_positivity(jm, fp, fl, fr);
+          for (j = 0; j < jm; j++) { QS(1, 0, i, j, k, p) = fl[j]; QS(1, 1, i, j, k, p) = fr[j]; }
+        }
+      }
+      if (nd == 3) {
+        for (j = o->jbl; j <= o->jeu; j++)
+          for (i = o->ibl; i <= o->ieu; i++) {
+            for (k = 0; k < km; k++) fp[k] = q[QI(o, p, i, j, k)];
+            weno5z(o, km, fp, fl, fr, w1, w2, w3);
+            if (positive && o->c.fix_positivity) fix_positivity(km, fp, fl, fr);
+            for (k = 0; k < km; k++) { QS(2, 0, i, j, k, p) = fl[k]; QS(2, 1, i, j, k, p) = fr[k]; }
+          }
+      }
+      free(fp);
+    }
+  }
+#undef QS
+}
+
+/* ------------------------------------------------------------------------ */
+/* equation of state, equations.F90:1762-1958 (HD) and 2815-3045 (MHD)        */
+/* ------------------------------------------------------------------------ */
+static void prim2cons(const ocfg* o, const double* q, double* u)
+{
+  double ek, ei, em;
+  u[o->idn] = q[o->idn];
+  u[o->imx] = q[o->idn] * q[o->ivx];
+  u[o->imy] = q[o->idn] * q[o->ivy];
+  u[o->imz] = q[o->idn] * q[o->ivz];
+  if (o->c.eqsys == AMUN_EQSYS_HD) {
+    ek = 0.5 * (u[o->imx] * q[o->ivx] + u[o->imy] * q[o->ivy] + u[o->imz] * q[o->ivz]);
+    ei = o->gammam1i * q[o->ipr];
+    u[o->ien] = ei + ek;
+  } else {
+    u[o->ibx] = q[o->ibx];
+    u[o->iby] = q[o->iby];
+    u[o->ibz] = q[o->ibz];
+    u[o->ibp] = q[o->ibp];
+    ei = o->gammam1i * q[o->ipr];
+    ek = 0.5 * (u[o->imx] * q[o->ivx] + u[o->imy] * q[o->ivy] + u[o->imz] * q[o->ivz]);
+    em = 0.5 * ((q[o->ibx] * q[o->ibx] + q[o->iby] * q[o->iby]) + q[o->ibz] * q[o->ibz]);
+    u[o->ien] = (ei + ek) + em;
+  }
+}
+
+static void cons2prim(const ocfg* o, const double* u, double* q)
+{
+  double ek, ei, em;
+  q[o->idn] = u[o->idn];
+  q[o->ivx] = u[o->imx] / u[o->idn];
+  q[o->ivy] = u[o->imy] / u[o->idn];
+  q[o->ivz] = u[o->imz] / u[o->idn];
+  if (o->c.eqsys == AMUN_EQSYS_HD) {
+    ek = 0.5 * (u[o->imx] * q[o->ivx] + u[o->imy] * q[o->ivy] + u[o->imz] * q[o->ivz]);
+    ei = u[o->ien] - ek;
+    q[o->ipr] = o->gammam1 * ei;
+  } else {
+    q[o->ibx] = u[o->ibx];
+    q[o->iby] = u[o->iby];
+    q[o->ibz] = u[o->ibz];
+    q[o->ibp] = u[o->ibp];
+    ek = 0.5 * (u[o->imx] * q[o->ivx] + u[o->imy] * q[o->ivy] + u[o->imz] * q[o->ivz]);
+    em = 0.5 * ((q[o->ibx] * q[o->ibx] + q[o->iby] * q[o->iby]) + q[o->ibz] * q[o->ibz]);
+    ei = u[o->ien] - (ek + em);
+    q[o->ipr] = o->gammam1 * ei;
+  }
+}
+
+static void fluxspeed(const ocfg* o, const double* q, const double* u,
+                      double* f, double* cm, double* cp)
+{
+  const double g = o->c.gamma;
+  if (o->c.eqsys == AMUN_EQSYS_HD) {
+    double cs;
+    f[o->idn] = u[o->imx];
+    f[o->imx] = q[o->ivx] * u[o->imx];
+    f[o->imy] = q[o->ivx] * u[o->imy];
+    f[o->imz] = q[o->ivx] * u[o->imz];
+    f[o->imx] = f[o->imx] + q[o->ipr];
+    f[o->ien] = q[o->ivx] * (u[o->ien] + q[o->ipr]);
+    cs  = sqrt(g * q[o->ipr] / q[o->idn]);
+    *cm = q[o->ivx] - cs;
+    *cp = q[o->ivx] + cs;
+  } else {
+    double bx2, by2, bz2, bb, vb, pt, fa, fb, fc, cf;
+    bx2 = q[o->ibx] * q[o->ibx];
+    by2 = q[o->iby] * q[o->iby];
+    bz2 = q[o->ibz] * q[o->ibz];
+    bb  = (bx2 + by2) + bz2;
+    vb  = (q[o->ivx] * q[o->ibx] + q[o->ivy] * q[o->iby]) + q[o->ivz] * q[o->ibz];
+    pt  = q[o->ipr] + 0.5 * bb;
+    f[o->idn] = u[o->imx];
+    f[o->imx] = q[o->ivx] * u[o->imx] - bx2;
+    f[o->imy] = q[o->ivx] * u[o->imy] - q[o->ibx] * q[o->iby];
+    f[o->imz] = q[o->ivx] * u[o->imz] - q[o->ibx] * q[o->ibz];
+    f[o->imx] = f[o->imx] + pt;
+    f[o->ibx] = q[o->ibp];
+    f[o->iby] = q[o->ivx] * q[o->iby] - q[o->ibx] * q[o->ivy];
+    f[o->ibz] = q[o->ivx] * q[o->ibz] - q[o->ibx] * q[o->ivz];
+    f[o->ibp] = o->cmax2 * q[o->ibx];
+    f[o->ien] = q[o->ivx] * (u[o->ien] + pt) - q[o->ibx] * vb;
+    fa = g * q[o->ipr];
+    fb = fa + bb;
+    fc = fb * fb - 4.0 * fa * bx2;
+    if (fc > 0.0) cf = sqrt(0.5 * (fb + sqrt(fc)) / q[o->idn]);
+    else          cf = sqrt(0.5 * fb / q[o->idn]);
+    *cm = q[o->ivx] - cf;
+    *cp = q[o->ivx] + cf;
+  }
+}
+
+/* maxspeed_hd_adi (equations.F90:1974-2044), maxspeed_mhd_adi (3060-3131) */
+static double maxspeed_cell(const ocfg* o, const double* q)
+{
+  double vv, v, c, bb;
+  vv = (q[o->ivx] * q[o->ivx] + q[o->ivy] * q[o->ivy]) + q[o->ivz] * q[o->ivz];
+  v  = sqrt(vv);
+  if (o->c.eqsys == AMUN_EQSYS_HD) {
+    c = sqrt(o->c.gamma * q[o->ipr] / q[o->idn]);
+  } else {
+    bb = (q[o->ibx] * q[o->ibx] + q[o->iby] * q[o->iby]) + q[o->ibz] * q[o->ibz];
+    c  = sqrt((o->c.gamma * q[o->ipr] + bb) / q[o->idn]);
+  }
+  return v + c;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Riemann solvers for one interface; ql, qr are modified (intent(inout))     */
+/* ------------------------------------------------------------------------ */
+#define NVMAX 9
+
+/* HLL: riemann_hd_adi_hll (schemes.F90:1248-1341) and riemann_mhd_adi_hll
+ * (3163-3280, with the GLM pre-step) */
+static void riemann_hll(const ocfg* o, double* ql, double* qr, double* f)
+{
+  double ul[NVMAX], ur[NVMAX], fl[NVMAX], fr[NVMAX], wl[NVMAX], wr[NVMAX];
+  double clm, clp, crm, crp, sl, sr, srml;
+  int p, nv = o->nv;
+  if (o->c.eqsys == AMUN_EQSYS_MHD) {
+    double bx = 0.5 * ((qr[o->ibx] + ql[o->ibx]) - (qr[o->ibp] - ql[o->ibp]) / o->cmax);
+    double bp = 0.5 * ((qr[o->ibp] + ql[o->ibp]) - (qr[o->ibx] - ql[o->ibx]) * o->cmax);
+    ql[o->ibx] = bx; qr[o->ibx] = bx;
+    ql[o->ibp] = bp; qr[o->ibp] = bp;
+  }
+  prim2cons(o, ql, ul);
+  prim2cons(o, qr, ur);
+  fluxspeed(o, ql, ul, fl, &clm, &clp);
+  fluxspeed(o, qr, ur, fr, &crm, &crp);
+  sl = fmin(clm, crm);
+  sr = fmax(clp, crp);
+  if (sl >= 0.0) {
+    for (p = 0; p < nv; p++) f[p] = fl[p];
+  } else if (sr <= 0.0) {
+    for (p = 0; p < nv; p++) f[p] = fr[p];
+  } else {
+    srml = sr - sl;
+    for (p = 0; p < nv; p++) {
+      wl[p] = sl * ul[p] - fl[p];
+      wr[p] = sr * ur[p] - fr[p];
+      f[p]  = (sl * wr[p] - sr * wl[p]) / srml;
+    }
+  }
+}
+
+/* riemann_hd_adi_hllc, schemes.F90:1366-1513 */
+static void riemann_hd_hllc(const ocfg* o, double* ql, double* qr, double* f)
+{
+  double ul[NVMAX], ur[NVMAX], fl[NVMAX], fr[NVMAX], wl[NVMAX], wr[NVMAX], ui[NVMAX];
+  double clm, clp, crm, crp, sl, sr, sm, dn, pr, slmm, srmm;
+  int p, nv = o->nv;
+  prim2cons(o, ql, ul);
+  prim2cons(o, qr, ur);
+  fluxspeed(o, ql, ul, fl, &clm, &clp);
+  fluxspeed(o, qr, ur, fr, &crm, &crp);
+  sl = fmin(clm, crm);
+  sr = fmax(clp, crp);
+  if (sl >= 0.0) {
+    for (p = 0; p < nv; p++) f[p] = fl[p];
+  } else if (sr <= 0.0) {
+    for (p = 0; p < nv; p++) f[p] = fr[p];
+  } else {
+    for (p = 0; p < nv; p++) {
+      wl[p] = sl * ul[p] - fl[p];
+      wr[p] = sr * ur[p] - fr[p];
+    }
+    dn = wr[o->idn] - wl[o->idn];
+    sm = (wr[o->imx] - wl[o->imx]) / dn;
+    pr = (wl[o->idn] * wr[o->imx] - wr[o->idn] * wl[o->imx]) / dn;
+    if (sm > 0.0) {
+      slmm = sl - sm;
+      ui[o->idn] = wl[o->idn] / slmm;
+      ui[o->imx] = ui[o->idn] * sm;
+      ui[o->imy] = ui[o->idn] * ql[o->ivy];
+      ui[o->imz] = ui[o->idn] * ql[o->ivz];
+      ui[o->ien] = (wl[o->ien] + sm * pr) / slmm;
+      for (p = 0; p < nv; p++) f[p] = sl * ui[p] - wl[p];
+    } else if (sm < 0.0) {
+      srmm = sr - sm;
+      ui[o->idn] = wr[o->idn] / srmm;
+      ui[o->imx] = ui[o->idn] * sm;
+      ui[o->imy] = ui[o->idn] * qr[o->ivy];
+      ui[o->imz] = ui[o->idn] * qr[o->ivz];
+      ui[o->ien] = (wr[o->ien] + sm * pr) / srmm;
+      for (p = 0; p < nv; p++) f[p] = sr * ui[p] - wr[p];
+    } else {
+      f[o->idn] = 0.0;
+      f[o->imx] = -wl[o->imx];
+      f[o->imy] = 0.0;
+      f[o->imz] = 0.0;
+      f[o->ien] = 0.0;
+    }
+  }
+}
+
+/* riemann_mhd_adi_hlld, schemes.F90:3596-4203 */
+static void riemann_mhd_hlld(const ocfg* o, double* ql, double* qr, double* f)
+{
+  const int idn = o->idn, imx = o->imx, imy = o->imy, imz = o->imz;
+  const int ibx = o->ibx, iby = o->iby, ibz = o->ibz, ibp = o->ibp, ien = o->ien;
+  double ul[NVMAX], ur[NVMAX], fl[NVMAX], fr[NVMAX], wl[NVMAX], wr[NVMAX];
+  double wcl[NVMAX], wcr[NVMAX], ui[NVMAX];
+  double clm, clp, crm, crp, sl, sr, sm, srml, slmm, srmm;
+  double dn, bx, bp, b2, pt, vy, vz, by, bz, vb, dnl, dnr, cal, car, sml, smr;
+  double dv, dvl, dvr;
+  int p, nv = o->nv;
+
+  bx = 0.5 * ((qr[ibx] + ql[ibx]) - (qr[ibp] - ql[ibp]) / o->cmax);
+  bp = 0.5 * ((qr[ibp] + ql[ibp]) - (qr[ibx] - ql[ibx]) * o->cmax);
+  ql[ibx] = bx; qr[ibx] = bx;
+  ql[ibp] = bp; qr[ibp] = bp;
+
+  prim2cons(o, ql, ul);
+  prim2cons(o, qr, ur);
+  fluxspeed(o, ql, ul, fl, &clm, &clp);
+  fluxspeed(o, qr, ur, fr, &crm, &crp);
+  sl = fmin(clm, crm);
+  sr = fmax(clp, crp);
+
+  if (sl >= 0.0) {
+    for (p = 0; p < nv; p++) f[p] = fl[p];
+    return;
+  }
+  if (sr <= 0.0) {
+    for (p = 0; p < nv; p++) f[p] = fr[p];
+    return;
+  }
+  srml = sr - sl;
+  for (p = 0; p < nv; p++) {
+    wl[p] = sl * ul[p] - fl[p];
+    wr[p] = sr * ur[p] - fr[p];
+  }
+  dn   = wr[idn] - wl[idn];
+  sm   = (wr[imx] - wl[imx]) / dn;
+  slmm = sl - sm;
+  srmm = sr - sm;
+  dnl  = wl[idn] / slmm;
+  dnr  = wr[idn] / srmm;
+  bx   = ql[ibx];
+  b2   = ql[ibx] * qr[ibx];
+  pt   = (wl[idn] * wr[imx] - wr[idn] * wl[imx]) / dn + b2;
+
+#define OUTER_LEFT()                                                           \
+  vy = (slmm * wl[imy] - bx * wl[iby]) / dvl;                                  \
+  vz = (slmm * wl[imz] - bx * wl[ibz]) / dvl;                                  \
+  by = (wl[idn] * wl[iby] - bx * wl[imy]) / dvl;                               \
+  bz = (wl[idn] * wl[ibz] - bx * wl[imz]) / dvl;                               \
+  vb = (sm * bx + vy * by) + vz * bz;                                          \
+  ui[idn] = dnl; ui[imx] = dnl * sm; ui[imy] = dnl * vy; ui[imz] = dnl * vz;   \
+  ui[ibx] = bx; ui[iby] = by; ui[ibz] = bz; ui[ibp] = ul[ibp];                 \
+  ui[ien] = ((wl[ien] + sm * pt) - bx * vb) / slmm;
+#define OUTER_RIGHT()                                                          \
+  vy = (srmm * wr[imy] - bx * wr[iby]) / dvr;                                  \
+  vz = (srmm * wr[imz] - bx * wr[ibz]) / dvr;                                  \
+  by = (wr[idn] * wr[iby] - bx * wr[imy]) / dvr;                               \
+  bz = (wr[idn] * wr[ibz] - bx * wr[imz]) / dvr;                               \
+  vb = (sm * bx + vy * by) + vz * bz;                                          \
+  ui[idn] = dnr; ui[imx] = dnr * sm; ui[imy] = dnr * vy; ui[imz] = dnr * vz;   \
+  ui[ibx] = bx; ui[iby] = by; ui[ibz] = bz; ui[ibp] = ur[ibp];                 \
+  ui[ien] = ((wr[ien] + sm * pt) - bx * vb) / srmm;
+
+  if (b2 > 1.0e-08 * fmax(dnl * (sl * sl), dnr * (sr * sr))) {
+    cal = sqrt(b2 / dnl);
+    car = sqrt(b2 / dnr);
+    sml = sm - cal;
+    smr = sm + car;
+    dvl = slmm * wl[idn] - b2;
+    dvr = srmm * wr[idn] - b2;
+    if (dvl > 0.0 && dvr > 0.0) {
+      if (sml >= 0.0) {
+        OUTER_LEFT();
+        for (p = 0; p < nv; p++) f[p] = sl * ui[p] - wl[p];
+      } else if (smr <= 0.0) {
+        OUTER_RIGHT();
+        for (p = 0; p < nv; p++) f[p] = sr * ui[p] - wr[p];
+      } else {
+        OUTER_LEFT();
+        for (p = 0; p < nv; p++) wcl[p] = (sml - sl) * ui[p] + wl[p];
+        OUTER_RIGHT();
+        for (p = 0; p < nv; p++) wcr[p] = (smr - sr) * ui[p] + wr[p];
+        dv = car * dnr + cal * dnl;
+        vy = (wcr[imy] - wcl[imy]) / dv;
+        vz = (wcr[imz] - wcl[imz]) / dv;
+        dv = car + cal;
+        by = (wcr[iby] - wcl[iby]) / dv;
+        bz = (wcr[ibz] - wcl[ibz]) / dv;
+        vb = (sm * bx + vy * by) + vz * bz;
+        if (sm > 0.0) {
+          ui[idn] = dnl; ui[imx] = dnl * sm; ui[imy] = dnl * vy; ui[imz] = dnl * vz;
+          ui[ibx] = bx; ui[iby] = by; ui[ibz] = bz; ui[ibp] = ul[ibp];
+          ui[ien] = -((wcl[ien] + sm * pt) - bx * vb) / cal;
+          for (p = 0; p < nv; p++) f[p] = sml * ui[p] - wcl[p];
+        } else if (sm < 0.0) {
+          ui[idn] = dnr; ui[imx] = dnr * sm; ui[imy] = dnr * vy; ui[imz] = dnr * vz;
+          ui[ibx] = bx; ui[iby] = by; ui[ibz] = bz; ui[ibp] = ur[ibp];
+          ui[ien] = ((wcr[ien] + sm * pt) - bx * vb) / car;
+          for (p = 0; p < nv; p++) f[p] = smr * ui[p] - wcr[p];
+        } else {
+          for (p = 0; p < nv; p++) f[p] = (sml * wcr[p] - smr * wcl[p]) / (smr - sml);
+        }
+      }
+    } else {
+      if (dvl > 0.0) {
+        OUTER_LEFT();
+        if (sml >= 0.0) {
+          for (p = 0; p < nv; p++) f[p] = sl * ui[p] - wl[p];
+        } else {
+          for (p = 0; p < nv; p++) wcl[p] = (sml - sl) * ui[p] + wl[p];
+          dv = srmm * dnr + cal * dnl;
+          vy = (wr[imy] - wcl[imy]) / dv;
+          vz = (wr[imz] - wcl[imz]) / dv;
+          dv = sr - sml;
+          by = (wr[iby] - wcl[iby]) / dv;
+          bz = (wr[ibz] - wcl[ibz]) / dv;
+          vb = (sm * bx + vy * by) + vz * bz;
+          ui[idn] = dnl; ui[imx] = dnl * sm; ui[imy] = dnl * vy; ui[imz] = dnl * vz;
+          ui[ibx] = bx; ui[iby] = by; ui[ibz] = bz; ui[ibp] = ul[ibp];
+          ui[ien] = -((wcl[ien] + sm * pt) - bx * vb) / cal;
+          if (sm >= 0.0) {
+            for (p = 0; p < nv; p++) f[p] = sml * ui[p] - wcl[p];
+          } else {
+            for (p = 0; p < nv; p++) wcr[p] = (sm - sml) * ui[p] + wcl[p];
+            for (p = 0; p < nv; p++) f[p] = (sm * wr[p] - sr * wcr[p]) / srmm;
+          }
+        }
+      } else if (dvr > 0.0) {
+        OUTER_RIGHT();
+        if (smr <= 0.0) {
+          for (p = 0; p < nv; p++) f[p] = sr * ui[p] - wr[p];
+        } else {
+          for (p = 0; p < nv; p++) wcr[p] = (smr - sr) * ui[p] + wr[p];
+          dv = slmm * dnl - car * dnr;
+          vy = (wl[imy] - wcr[imy]) / dv;
+          vz = (wl[imz] - wcr[imz]) / dv;
+          dv = sl - smr;
+          by = (wl[iby] - wcr[iby]) / dv;
+          bz = (wl[ibz] - wcr[ibz]) / dv;
+          vb = (sm * bx + vy * by) + vz * bz;
+          ui[idn] = dnr; ui[imx] = dnr * sm; ui[imy] = dnr * vy; ui[imz] = dnr * vz;
+          ui[ibx] = bx; ui[iby] = by; ui[ibz] = bz; ui[ibp] = ur[ibp];
+          ui[ien] = ((wcr[ien] + sm * pt) - bx * vb) / car;
+          if (sm <= 0.0) {
+            for (p = 0; p < nv; p++) f[p] = smr * ui[p] - wcr[p];
+          } else {
+            for (p = 0; p < nv; p++) wcl[p] = (sm - smr) * ui[p] + wcr[p];
+            for (p = 0; p < nv; p++) f[p] = (sm * wl[p] - sl * wcl[p]) / slmm;
+          }
+        }
+      } else {
+        for (p = 0; p < nv; p++) f[p] = (sl * wr[p] - sr * wl[p]) / srml;
+      }
+    }
+  } else if (b2 > 0.0) {
+    vy = (wr[imy] - wl[imy]) / dn;
+    vz = (wr[imz] - wl[imz]) / dn;
+    by = (wr[iby] - wl[iby]) / srml;
+    bz = (wr[ibz] - wl[ibz]) / srml;
+    vb = (sm * bx + vy * by) + vz * bz;
+    if (sm > 0.0) {
+      ui[idn] = dnl; ui[imx] = dnl * sm; ui[imy] = dnl * vy; ui[imz] = dnl * vz;
+      ui[ibx] = bx; ui[iby] = by; ui[ibz] = bz; ui[ibp] = ul[ibp];
+      ui[ien] = ((wl[ien] + sm * pt) - bx * vb) / slmm;
+      for (p = 0; p < nv; p++) f[p] = sl * ui[p] - wl[p];
+    } else if (sm < 0.0) {
+      ui[idn] = dnr; ui[imx] = dnr * sm; ui[imy] = dnr * vy; ui[imz] = dnr * vz;
+      ui[ibx] = bx; ui[iby] = by; ui[ibz] = bz; ui[ibp] = ur[ibp];
+      ui[ien] = ((wr[ien] + sm * pt) - bx * vb) / srmm;
+      for (p = 0; p < nv; p++) f[p] = sr * ui[p] - wr[p];
+    } else {
+      for (p = 0; p < nv; p++) f[p] = (sl * wr[p] - sr * wl[p]) / srml;
+    }
+  } else {
+    if (sm > 0.0) {
+      ui[idn] = dnl; ui[imx] = dnl * sm;
+      ui[imy] = wl[imy] / slmm; ui[imz] = wl[imz] / slmm;
+      ui[ibx] = 0.0; ui[iby] = wl[iby] / slmm; ui[ibz] = wl[ibz] / slmm;
+      ui[ibp] = ul[ibp];
+      ui[ien] = (wl[ien] + sm * pt) / slmm;
+      for (p = 0; p < nv; p++) f[p] = sl * ui[p] - wl[p];
+    } else if (sm < 0.0) {
+      ui[idn] = dnr; ui[imx] = dnr * sm;
+      ui[imy] = wr[imy] / srmm; ui[imz] = wr[imz] / srmm;
+      ui[ibx] = 0.0; ui[iby] = wr[iby] / srmm; ui[ibz] = wr[ibz] / srmm;
+      ui[ibp] = ur[ibp];
+      ui[ien] = (wr[ien] + sm * pt) / srmm;
+      for (p = 0; p < nv; p++) f[p] = sr * ui[p] - wr[p];
+    } else {
+      for (p = 0; p < nv; p++) f[p] = (sl * wr[p] - sr * wl[p]) / srml;
+    }
+  }
+#undef OUTER_LEFT
+#undef OUTER_RIGHT
+}
+
+static int riemann_check(const amun_cfg* c)
+{
+  if (c->riemann_solver == AMUN_RS_HLL) return 0;
+  if (c->riemann_solver == AMUN_RS_HLLC && c->eqsys == AMUN_EQSYS_HD) return 0;
+  if (c->riemann_solver == AMUN_RS_HLLD && c->eqsys == AMUN_EQSYS_MHD) return 0;
+  return 300;
+}
+
+static void riemann(const ocfg* o, double* ql, double* qr, double* f)
+{
+  switch (o->c.riemann_solver) {
+  case AMUN_RS_HLLC: riemann_hd_hllc(o, ql, qr, f); break;
+  case AMUN_RS_HLLD: riemann_mhd_hlld(o, ql, qr, f); break;
+  default:           riemann_hll(o, ql, qr, f); break;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* update_flux for one block (schemes.F90:1080-1223 / 2970-3138)              */
+/* f layout: Fortran f(NDIMS,nv,im,jm,km)                                     */
+/* ------------------------------------------------------------------------ */
+#define FI(o, d, p, i, j, k) (((((size_t)(k) * (o)->jm + (j)) * (o)->im + (i)) * (o)->nv + (p)) * (o)->nd + (d))
+
+static void update_flux(ocfg* o, const double* q, double* f, double* qs)
+{
+  const int nv = o->nv, nd = o->nd, im = o->im, jm = o->jm, km = o->km;
+  const size_t ncell = (size_t)im * jm * km;
+  int perm[3][NVMAX];   /* rotated position -> original variable */
+  int d, p, i, j, k;
+  double ql[NVMAX], qr[NVMAX], fx[NVMAX];
+
+  /* component rotation of schemes.F90:3063-3071 / 3100-3108 */
+  for (d = 0; d < 3; d++)
+    for (p = 0; p < nv; p++) perm[d][p] = p;
+  perm[1][o->ivx] = o->ivy; perm[1][o->ivy] = o->ivz; perm[1][o->ivz] = o->ivx;
+  perm[2][o->ivx] = o->ivz; perm[2][o->ivy] = o->ivx; perm[2][o->ivz] = o->ivy;
+  if (o->c.eqsys == AMUN_EQSYS_MHD) {
+    perm[1][o->ibx] = o->iby; perm[1][o->iby] = o->ibz; perm[1][o->ibz] = o->ibx;
+    perm[2][o->ibx] = o->ibz; perm[2][o->iby] = o->ibx; perm[2][o->ibz] = o->iby;
+  }
+
+  memset(f, 0, ncell * nv * nd * sizeof(double));
+  reconstruct_interfaces(o, q, qs);
+
+#define QSV(d, s, i, j, k, p) qs[((size_t)((d) * 2 + (s)) * ncell + ((size_t)(k) * jm + (j)) * im + (i)) * nv + (p)]
+  for (d = 0; d < nd; d++) {
+    int n1lo, n1hi, n2lo, n2hi, a, b, c, len;
+    if (d == 0) { n1lo = o->jbl; n1hi = o->jeu; n2lo = o->kbl; n2hi = o->keu; len = im; }
+    else if (d == 1) { n1lo = o->ibl; n1hi = o->ieu; n2lo = o->kbl; n2hi = o->keu; len = jm; }
+    else { n1lo = o->ibl; n1hi = o->ieu; n2lo = o->jbl; n2hi = o->jeu; len = km; }
+    for (b = n2lo; b <= n2hi; b++)
+      for (a = n1lo; a <= n1hi; a++)
+        for (c = 0; c < len; c++) {
+          if (d == 0) { i = c; j = a; k = b; }
+          else if (d == 1) { i = a; j = c; k = b; }
+          else { i = a; j = b; k = c; }
+          for (p = 0; p < nv; p++) {
+            ql[p] = QSV(d, 0, i, j, k, perm[d][p]);
+            qr[p] = QSV(d, 1, i, j, k, perm[d][p]);
+          }
+          riemann(o, ql, qr, fx);
+          for (p = 0; p < nv; p++) f[FI(o, d, perm[d][p], i, j, k)] = fx[p];
+        }
+  }
+#undef QSV
+}
+
+/* ------------------------------------------------------------------------ */
+/* block grid with ghost exchange                                             */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+  ocfg o;
+  int ir, jr, kr, nblk;
+  double *u0, *u1, *q, *f, *du, *qs;    /* u0,u1,q: nblk blocks */
+  double time, dt, dtn;
+  long step;
+} ogrid;
+
+static size_t bsize(const ocfg* o) { return (size_t)o->im * o->jm * o->km * o->nv; }
+
+static int nbr_block(const ogrid* g, int b, int di, int dj, int dk)
+{
+  int bx = b % g->ir, by = (b / g->ir) % g->jr, bz = b / (g->ir * g->jr);
+  bx = (bx + di + g->ir) % g->ir;
+  by = (by + dj + g->jr) % g->jr;
+  bz = (bz + dk + g->kr) % g->kr;
+  return (bz * g->jr + by) * g->ir + bx;
+}
+
+/* boundary_variables for a single-level periodic mesh: every ghost cell of q
+ * is the neighbour's interior value (face, edge and corner copies), then
+ * update_ghost_cells converts the ghost q to u */
+static void boundary_variables(ogrid* g, double* u)
+{
+  ocfg* o = &g->o;
+  const size_t bs = bsize(o);
+  int b, i, j, k;
+  for (b = 0; b < g->nblk; b++) {
+    double* qb = g->q + b * bs;
+    for (k = 0; k < o->km; k++)
+      for (j = 0; j < o->jm; j++)
+        for (i = 0; i < o->im; i++) {
+          int gi = i - o->ng, gj = j - o->ng, gk = (o->nd == 3) ? k - o->ng : 0;
+          int di = gi < 0 ? -1 : (gi >= o->nc ? 1 : 0);
+          int dj = gj < 0 ? -1 : (gj >= o->nc ? 1 : 0);
+          int dk = (o->nd == 3) ? (gk < 0 ? -1 : (gk >= o->nc ? 1 : 0)) : 0;
+          int nb, li, lj, lk;
+          if (di == 0 && dj == 0 && dk == 0) continue;
+          nb = nbr_block(g, b, di, dj, dk);
+          li = gi - di * o->nc + o->ng;
+          lj = gj - dj * o->nc + o->ng;
+          lk = (o->nd == 3) ? gk - dk * o->nc + o->ng : 0;
+          memcpy(&qb[QI(o, 0, i, j, k)], &g->q[nb * bs + QI(o, 0, li, lj, lk)],
+                 o->nv * sizeof(double));
+          prim2cons(o, &qb[QI(o, 0, i, j, k)], &u[b * bs + QI(o, 0, i, j, k)]);
+        }
+  }
+}
+
+/* update_primitive_variables (equations.F90:1088-1148): interior cons2prim,
+ * the ghost layers are then replaced by the exchange */
+static void update_variables(ogrid* g, double* u)
+{
+  ocfg* o = &g->o;
+  const size_t bs = bsize(o);
+  int b, i, j, k;
+  for (b = 0; b < g->nblk; b++)
+    for (k = o->kb; k <= o->ke; k++)
+      for (j = o->jb; j <= o->je; j++)
+        for (i = o->ib; i <= o->ie; i++)
+          cons2prim(o, &u[b * bs + QI(o, 0, i, j, k)], &g->q[b * bs + QI(o, 0, i, j, k)]);
+  boundary_variables(g, u);
+}
+
+/* update_increment, evolution.F90:1898-1977 */
+static void update_increment(const ocfg* o, const double* f, double* du)
+{
+  int p, i, j, k;
+  memset(du, 0, bsize(o) * sizeof(double));
+  for (k = o->kbl; k <= o->keu; k++)
+    for (j = o->jbl; j <= o->jeu; j++)
+      for (i = o->ibl; i <= o->ieu; i++)
+        for (p = 0; p < o->nv; p++) {
+          double v;
+          v = -o->adxi * (f[FI(o, 0, p, i, j, k)] - f[FI(o, 0, p, i - 1, j, k)])
+              - o->adyi * (f[FI(o, 1, p, i, j, k)] - f[FI(o, 1, p, i, j - 1, k)]);
+          if (o->nd == 3)
+            v = v - o->adzi * (f[FI(o, 2, p, i, j, k)] - f[FI(o, 2, p, i, j, k - 1)]);
+          du[QI(o, p, i, j, k)] = v;
+        }
+}
+
+/* new_time_step, evolution.F90:379-493 (single level, no viscosity) */
+static void new_time_step(ogrid* g, double dtnext)
+{
+  ocfg* o = &g->o;
+  const size_t bs = bsize(o);
+  const double eps = 2.2250738585072014e-308;   /* tiny(1d0) */
+  double dxmin;
+  int b, i, j, k;
+  o->cmax = eps;
+  for (b = 0; b < g->nblk; b++) {
+    double ms = 0.0;
+    for (k = o->kb; k <= o->ke; k++)
+      for (j = o->jb; j <= o->je; j++)
+        for (i = o->ib; i <= o->ie; i++)
+          ms = fmax(ms, maxspeed_cell(o, &g->q[b * bs + QI(o, 0, i, j, k)]));
+    o->cmax = fmax(o->cmax, ms);
+  }
+  o->cmax2 = o->cmax * o->cmax;
+  dxmin = fmin(o->adx, o->ady);
+  if (o->nd == 3) dxmin = fmin(dxmin, o->adz);
+  g->dtn = o->cflx * dxmin / fmax(o->cmax + 2.0 * 0.0 / dxmin, eps);
+  if (dtnext > 0.0) g->dt = fmin(g->dtn, dtnext);
+}
+
+/* one stage: fluxes of u (through q), increment, combine into dst */
+static void stage(ogrid* g, int kind, double ds, double* dst, const double* a0,
+                  const double* a1)
+{
+  ocfg* o = &g->o;
+  const size_t bs = bsize(o);
+  int b;
+  size_t n;
+  for (b = 0; b < g->nblk; b++) {
+    update_flux(o, g->q + b * bs, g->f, g->qs);
+    update_increment(o, g->f, g->du);
+    for (n = 0; n < bs; n++) {
+      double v;
+      if (kind == 0)       /* u1 = u0 + ds du */
+        v = a0[b * bs + n] + ds * g->du[n];
+      else if (kind == 1)  /* rk3 2nd: 3/4 u0 + 1/4 u1 + ds du */
+        v = ((3.0 / 4.0) * a0[b * bs + n] + (1.0 / 4.0) * a1[b * bs + n]) + ds * g->du[n];
+      else if (kind == 2)  /* rk3 3rd: 1/3 u0 + 2/3 u1 + ds du */
+        v = ((1.0 / 3.0) * a0[b * bs + n] + (2.0 / 3.0) * a1[b * bs + n]) + ds * g->du[n];
+      else                 /* rk2 2nd: 1/2 (u0 + u1 + dt du) */
+        v = 0.5 * ((a0[b * bs + n] + a1[b * bs + n]) + ds * g->du[n]);
+      dst[b * bs + n] = v;
+    }
+  }
+}
+
+static void psi_decay(ogrid* g, double* u)
+{
+  ocfg* o = &g->o;
+  size_t c, ncell = (size_t)g->nblk * o->im * o->jm * o->km;
+  if (o->ibp < 0) return;
+  for (c = 0; c < ncell; c++) u[c * o->nv + o->ibp] = o->decay * u[c * o->nv + o->ibp];
+}
+
+static void advance(ogrid* g, double dtnext)
+{
+  ocfg* o = &g->o;
+  new_time_step(g, dtnext);
+  if (o->c.time_advance == AMUN_TA_EULER) {       /* evolve_euler 517 */
+    stage(g, 0, g->dt, g->u0, g->u0, NULL);
+    psi_decay(g, g->u0);
+    update_variables(g, g->u0);
+  } else if (o->c.time_advance == AMUN_TA_RK2) {  /* evolve_rk2 628 */
+    stage(g, 0, g->dt, g->u1, g->u0, NULL);
+    update_variables(g, g->u1);
+    stage(g, 3, g->dt, g->u0, g->u0, g->u1);
+    psi_decay(g, g->u0);
+    update_variables(g, g->u0);
+  } else {                                         /* evolve_rk3 1005 */
+    stage(g, 0, g->dt, g->u1, g->u0, NULL);
+    update_variables(g, g->u1);
+    stage(g, 1, (1.0 / 4.0) * g->dt, g->u1, g->u0, g->u1);
+    update_variables(g, g->u1);
+    stage(g, 2, (2.0 / 3.0) * g->dt, g->u0, g->u0, g->u1);
+    psi_decay(g, g->u0);
+    update_variables(g, g->u0);
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* exported API (ctypes)                                                      */
+/* ------------------------------------------------------------------------ */
+int orc_create(const amun_cfg* c, void** handle)
+{
+  ogrid* g;
+  int r;
+  size_t bs, ncell;
+  *handle = NULL;
+  g = (ogrid*)calloc(1, sizeof(ogrid));
+  if (!g) return 500;
+  r = oc_setup(&g->o, c);
+  if (!r) r = riemann_check(c);
+  if (!r && c->reconstruction != AMUN_REC_TVD && c->reconstruction != AMUN_REC_WENO5Z) r = 200;
+  if (!r && c->time_advance != AMUN_TA_EULER && c->time_advance != AMUN_TA_RK2 &&
+      c->time_advance != AMUN_TA_RK3) r = 400;
+  if (r) { free(g); return r; }
+  g->ir = c->xblocks; g->jr = c->yblocks; g->kr = (c->ndims == 3) ? c->zblocks : 1;
+  g->nblk = g->ir * g->jr * g->kr;
+  bs = bsize(&g->o);
+  ncell = (size_t)g->o.im * g->o.jm * g->o.km;
+  g->u0 = (double*)calloc(bs * g->nblk, sizeof(double));
+  g->u1 = (double*)calloc(bs * g->nblk, sizeof(double));
+  g->q  = (double*)calloc(bs * g->nblk, sizeof(double));
+  g->f  = (double*)calloc(bs * g->o.nd, sizeof(double));
+  g->du = (double*)calloc(bs, sizeof(double));
+  g->qs = (double*)calloc(ncell * g->o.nv * 2 * g->o.nd, sizeof(double));
+  g->dt = 1.0; g->dtn = 1.0;
+  *handle = g;
+  return 0;
+}
+
+void orc_destroy(void* h)
+{
+  ogrid* g = (ogrid*)h;
+  if (!g) return;
+  free(g->u0); free(g->u1); free(g->q); free(g->f); free(g->du); free(g->qs);
+  free(g);
+}
+
+/* global array U[nv][nz][ny][nx] <-> block interiors */
+static void xfer_global(ogrid* g, double* blk, double* glob, int to_blocks)
+{
+  ocfg* o = &g->o;
+  const size_t bs = bsize(o);
+  const size_t nx = (size_t)g->ir * o->nc, ny = (size_t)g->jr * o->nc;
+  const size_t nz = (o->nd == 3) ? (size_t)g->kr * o->nc : 1;
+  int b, i, j, k, p;
+  for (b = 0; b < g->nblk; b++) {
+    int bx = b % g->ir, by = (b / g->ir) % g->jr, bz = b / (g->ir * g->jr);
+    for (k = o->kb; k <= o->ke; k++)
+      for (j = o->jb; j <= o->je; j++)
+        for (i = o->ib; i <= o->ie; i++)
+          for (p = 0; p < o->nv; p++) {
+            size_t gx = (size_t)bx * o->nc + (i - o->ib);
+            size_t gy = (size_t)by * o->nc + (j - o->jb);
+            size_t gz = (o->nd == 3) ? (size_t)bz * o->nc + (k - o->kb) : 0;
+            size_t gi = (((size_t)p * nz + gz) * ny + gy) * nx + gx;
+            if (to_blocks) blk[b * bs + QI(o, p, i, j, k)] = glob[gi];
+            else glob[gi] = blk[b * bs + QI(o, p, i, j, k)];
+          }
+  }
+}
+
+/* Set the state as the reference driver does after problem set-up:
+ * interior u0 given, q = cons2prim(u0), ghost exchange, first dt estimate. */
+int orc_set_state(void* h, const double* uglob, double dtnext)
+{
+  ogrid* g = (ogrid*)h;
+  xfer_global(g, g->u0, (double*)uglob, 1);
+  update_variables(g, g->u0);
+  new_time_step(g, dtnext);
+  return 0;
+}
+
+int orc_get_state(void* h, double* uglob, double* qglob)
+{
+  ogrid* g = (ogrid*)h;
+  xfer_global(g, g->u0, uglob, 0);
+  if (qglob) xfer_global(g, g->q, qglob, 0);
+  return 0;
+}
+
+/* nsteps of advance(dtnext); trace (may be NULL) gets (dt, cmax) per step */
+int orc_run(void* h, int nsteps, double dtnext, double* trace)
+{
+  ogrid* g = (ogrid*)h;
+  int s;
+  for (s = 0; s < nsteps; s++) {
+    advance(g, dtnext);
+    if (trace) { trace[2 * s] = g->dt; trace[2 * s + 1] = g->o.cmax; }
+    g->time += g->dt;
+    g->step += 1;
+  }
+  return 0;
+}
+
+long orc_positivity_events(void* h) { return ((ogrid*)h)->o.npos_events; }
+
+/* unit level: update_flux on one block q(nv,im,jm,km) with a given cmax */
+int orc_update_flux(const amun_cfg* c, const double* q, const double* dx,
+                    double cmax, double* f)
+{
+  ocfg o;
+  double* qs;
+  int r = oc_setup(&o, c);
+  (void)dx;
+  if (!r) r = riemann_check(c);
+  if (r) return r;
+  o.cmax = cmax;
+  o.cmax2 = cmax * cmax;
+  qs = (double*)malloc((size_t)o.im * o.jm * o.km * o.nv * 2 * o.nd * sizeof(double));
+  update_flux(&o, q, f, qs);
+  free(qs);
+  return 0;
+}
+
+/* unit level: reconstruct() on npencil pencils of length n (WENO5-Z) */
+int orc_reconstruct(const amun_cfg* c, int n, int npencil, const double* fp,
+                    double* fl, double* fr)
+{
+  ocfg o;
+  double* w;
+  int p, r = oc_setup(&o, c);
+  if (r) return r;
+  w = (double*)malloc(3 * (size_t)n * sizeof(double));
+  for (p = 0; p < npencil; p++)
+    weno5z(&o, n, fp + (size_t)p * n, fl + (size_t)p * n, fr + (size_t)p * n, w, w + n, w + 2 * n);
+  free(w);
+  return 0;
+}
+
+/* unit level: prim2cons, cons2prim, fluxspeed on n states q(nv,n) */
+int orc_eqs(const amun_cfg* c, int n, double cmax, const double* q, double* u,
+            double* q2, double* f, double* cm, double* cp)
+{
+  ocfg o;
+  int i, r = oc_setup(&o, c);
+  if (r) return r;
+  o.cmax = cmax;
+  o.cmax2 = cmax * cmax;
+  for (i = 0; i < n; i++) {
+    prim2cons(&o, q + (size_t)i * o.nv, u + (size_t)i * o.nv);
+    cons2prim(&o, u + (size_t)i * o.nv, q2 + (size_t)i * o.nv);
+    fluxspeed(&o, q + (size_t)i * o.nv, u + (size_t)i * o.nv, f + (size_t)i * o.nv, cm + i, cp + i);
+  }
+  return 0;
+}
+
+/* unit level: one Riemann problem per state pair, ql/qr (nv,n) primitive */
+int orc_riemann(const amun_cfg* c, int n, double cmax, const double* ql,
+                const double* qr, double* f)
+{
+  ocfg o;
+  int i, p, r = oc_setup(&o, c);
+  double a[NVMAX], b[NVMAX];
+  if (!r) r = riemann_check(c);
+  if (r) return r;
+  o.cmax = cmax;
+  o.cmax2 = cmax * cmax;
+  for (i = 0; i < n; i++) {
+    for (p = 0; p < o.nv; p++) { a[p] = ql[(size_t)i * o.nv + p]; b[p] = qr[(size_t)i * o.nv + p]; }
+    riemann(&o, a, b, f + (size_t)i * o.nv);
+  }
+  return 0;
+}

@@ -1,0 +1,202 @@
+"""ctypes access to the C oracle (oracle/liboracle.so) and the reference
+harness binaries (oracle/_ref/amun_ref_{2,3}d.x).  Test infrastructure: the
+oracle is only ever the checker."""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+
+from amun_amd.config import Params
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_DIR = os.path.join(ROOT, "oracle", "_ref")
+
+_P = ctypes.POINTER(ctypes.c_double)
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(_P)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError("oracle/liboracle.so is not built; run __graft_entry__.build()")
+        L = ctypes.CDLL(ORACLE_SO)
+        L.orc_create.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]
+        L.orc_destroy.argtypes = [ctypes.c_void_p]
+        L.orc_set_state.argtypes = [ctypes.c_void_p, _P, ctypes.c_double]
+        L.orc_get_state.argtypes = [ctypes.c_void_p, _P, _P]
+        L.orc_run.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, _P]
+        L.orc_positivity_events.argtypes = [ctypes.c_void_p]
+        L.orc_positivity_events.restype = ctypes.c_long
+        L.orc_update_flux.argtypes = [ctypes.c_void_p, _P, _P, ctypes.c_double, _P]
+        L.orc_reconstruct.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _P, _P, _P]
+        L.orc_eqs.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, _P, _P, _P, _P, _P, _P]
+        L.orc_riemann.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, _P, _P, _P]
+        _lib = L
+    return _lib
+
+
+class OracleRun:
+    """A block-structured CPU run of the oracle."""
+
+    def __init__(self, p: Params):
+        self.p = p
+        self.cfg = p.to_cfg()
+        h = ctypes.c_void_p()
+        r = lib().orc_create(ctypes.byref(self.cfg), ctypes.byref(h))
+        if r:
+            raise RuntimeError(f"orc_create failed with code {r}")
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_destroy(self.h)
+            self.h = None
+
+    def set_state(self, u: np.ndarray, dtnext: float = 2.0e30):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        lib().orc_set_state(self.h, _dp(u), dtnext)
+
+    def get_state(self, with_q: bool = False):
+        nz, ny, nx = self.p.shape
+        u = np.empty((self.p.nv, nz, ny, nx))
+        q = np.empty_like(u) if with_q else None
+        lib().orc_get_state(self.h, _dp(u), _dp(q) if with_q else None)
+        return (u, q) if with_q else u
+
+    def run(self, nsteps: int, dtnext: float = 2.0e30) -> np.ndarray:
+        tr = np.zeros((nsteps, 2))
+        lib().orc_run(self.h, nsteps, dtnext, _dp(tr))
+        return tr
+
+    def positivity_events(self) -> int:
+        return lib().orc_positivity_events(self.h)
+
+
+def oracle_riemann(p: Params, ql: np.ndarray, qr: np.ndarray, cmax: float) -> np.ndarray:
+    ql = np.ascontiguousarray(ql, dtype=np.float64)
+    qr = np.ascontiguousarray(qr, dtype=np.float64)
+    f = np.empty_like(ql)
+    cfg = p.to_cfg()
+    r = lib().orc_riemann(ctypes.byref(cfg), ql.shape[0], cmax, _dp(ql.copy()), _dp(qr.copy()), _dp(f))
+    assert r == 0, r
+    return f
+
+
+def oracle_update_flux(p: Params, q: np.ndarray, cmax: float) -> np.ndarray:
+    """q: (km, jm, im, nv) C order == Fortran q(nv,im,jm,km); returns f with
+    C shape (km, jm, im, nv, ndims) == Fortran f(NDIMS,nv,im,jm,km)."""
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    f = np.zeros(q.shape + (p.ndims,))
+    dx = np.array(p.dx())
+    cfg = p.to_cfg()
+    r = lib().orc_update_flux(ctypes.byref(cfg), _dp(q), _dp(dx), cmax, _dp(f))
+    assert r == 0, r
+    return f
+
+
+def oracle_reconstruct(p: Params, fp: np.ndarray):
+    fp = np.ascontiguousarray(fp, dtype=np.float64)
+    npen, n = fp.shape
+    fl = np.empty_like(fp)
+    fr = np.empty_like(fp)
+    cfg = p.to_cfg()
+    lib().orc_reconstruct(ctypes.byref(cfg), n, npen, _dp(fp), _dp(fl), _dp(fr))
+    return fl, fr
+
+
+def oracle_eqs(p: Params, q: np.ndarray, cmax: float = 1.0):
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    n = q.shape[0]
+    u = np.empty_like(q); q2 = np.empty_like(q); f = np.empty_like(q)
+    cm = np.empty(n); cp = np.empty(n)
+    cfg = p.to_cfg()
+    lib().orc_eqs(ctypes.byref(cfg), n, cmax, _dp(q), _dp(u), _dp(q2), _dp(f), _dp(cm), _dp(cp))
+    return u, q2, f, cm, cp
+
+
+# ------------------------------------------------------------------ reference
+def ref_available(ndims: int) -> bool:
+    return os.path.exists(os.path.join(REF_DIR, f"amun_ref_{ndims}d.x"))
+
+
+def _ref_exec(p: Params, extra: dict, workdir: str):
+    exe = os.path.join(REF_DIR, f"amun_ref_{p.ndims}d.x")
+    txt = p.to_params_in() + "".join(f"{k} = {v}\n" for k, v in extra.items())
+    with open(os.path.join(workdir, "params.in"), "w") as fh:
+        fh.write(txt)
+    r = subprocess.run([exe, "-i", "params.in"], cwd=workdir, capture_output=True,
+                       text=True, timeout=3600)
+    if r.returncode != 0:
+        raise RuntimeError(f"reference harness failed: {r.stdout}\n{r.stderr}")
+    return r
+
+
+def ref_run(p: Params, u0: np.ndarray, nsteps: int):
+    """Run the reference binary on the global conserved state u0; returns the
+    final state and the (dt, cmax) trace."""
+    with tempfile.TemporaryDirectory() as d:
+        np.ascontiguousarray(u0, dtype=np.float64).tofile(os.path.join(d, "in.bin"))
+        _ref_exec(p, {"harness_mode": "run", "harness_input": "in.bin",
+                      "harness_output": "out", "harness_steps": nsteps}, d)
+        u = np.fromfile(os.path.join(d, "out_u.bin")).reshape(u0.shape)
+        tr = np.fromfile(os.path.join(d, "out_dt.bin")).reshape(nsteps, 2)
+    return u, tr
+
+
+def ref_update_flux(p: Params, qblocks: np.ndarray, cmax: float) -> np.ndarray:
+    """qblocks: (nb, km, jm, im, nv); returns (nb, km, jm, im, nv, ndims)."""
+    nb = qblocks.shape[0]
+    dx = np.array(p.dx())
+    with tempfile.TemporaryDirectory() as d:
+        with open(os.path.join(d, "in.bin"), "wb") as fh:
+            for b in range(nb):
+                np.ascontiguousarray(qblocks[b], dtype=np.float64).tofile(fh)
+                dx.tofile(fh)
+        _ref_exec(p, {"harness_mode": "flux", "harness_input": "in.bin",
+                      "harness_output": "out", "harness_count": nb,
+                      "harness_cmax": repr(float(cmax))}, d)
+        f = np.fromfile(os.path.join(d, "out_f.bin")).reshape(qblocks.shape + (p.ndims,))
+    return f
+
+
+def ref_reconstruct(p: Params, fp: np.ndarray, h: float = 1.0):
+    npen, n = fp.shape
+    with tempfile.TemporaryDirectory() as d:
+        with open(os.path.join(d, "in.bin"), "wb") as fh:
+            np.ascontiguousarray(fp, dtype=np.float64).tofile(fh)
+            np.array([h]).tofile(fh)
+        _ref_exec(p, {"harness_mode": "recon", "harness_input": "in.bin",
+                      "harness_output": "out", "harness_count": npen,
+                      "harness_n": n}, d)
+        fl = np.fromfile(os.path.join(d, "out_fl.bin")).reshape(npen, n)
+        fr = np.fromfile(os.path.join(d, "out_fr.bin")).reshape(npen, n)
+    return fl, fr
+
+
+def ref_eqs(p: Params, q: np.ndarray, cmax: float = 1.0):
+    n = q.shape[0]
+    with tempfile.TemporaryDirectory() as d:
+        np.ascontiguousarray(q, dtype=np.float64).tofile(os.path.join(d, "in.bin"))
+        _ref_exec(p, {"harness_mode": "eqs", "harness_input": "in.bin",
+                      "harness_output": "out", "harness_n": n,
+                      "harness_cmax": repr(float(cmax))}, d)
+        a = np.fromfile(os.path.join(d, "out_eqs.bin"))
+    nv = p.nv
+    u = a[: nv * n].reshape(n, nv)
+    q2 = a[nv * n: 2 * nv * n].reshape(n, nv)
+    f = a[2 * nv * n: 3 * nv * n].reshape(n, nv)
+    cm = a[3 * nv * n: 3 * nv * n + n]
+    cp = a[3 * nv * n + n:]
+    return u, q2, f, cm, cp
