@@ -1,0 +1,84 @@
+"""Build steps of the native parts (used by __graft_entry__.build()).
+
+- amun_amd/libamun_hip.so : the HIP kernels + C-ABI, hipcc for gfx950
+- amun_amd/host/amun_host.x : the Fortran 2003 host driver (amdflang)
+- oracle/liboracle.so      : the C oracle (test infrastructure, gcc)
+- oracle/_ref/*            : the reference binaries, only where the
+                             reference sources exist (this container)
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "amun_amd")
+CSRC = os.path.join(PKG, "csrc")
+HOST = os.path.join(PKG, "host")
+ORACLE = os.path.join(ROOT, "oracle")
+
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+HIPCC = shutil.which("hipcc") or os.path.join(ROCM, "bin", "hipcc")
+FC = shutil.which("amdflang") or os.path.join(ROCM, "bin", "amdflang")
+
+# -Xclang -target-feature -Xclang -packed-fp32-ops: no packed FP32 VALU ops
+HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-Wno-unused-value", "-Wno-unused-result"]
+
+
+def _run(cmd, cwd=None):
+    r = subprocess.run(cmd, cwd=cwd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"build step failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return r
+
+
+def build_hip():
+    out = os.path.join(PKG, "libamun_hip.so")
+    srcs = [os.path.join(CSRC, "amun_dev.hip")]
+    _run([HIPCC, *HIP_FLAGS, "-shared", "-o", out + ".tmp", *srcs])
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build_oracle():
+    out = os.path.join(ORACLE, "liboracle.so")
+    _run(["gcc", "-O2", "-ffp-contract=off", "-fPIC", "-shared", "-o", out,
+          os.path.join(ORACLE, "amun_oracle.c"), "-lm"])
+    return out
+
+
+def build_host():
+    """Fortran host driver linked against libamun_hip.so."""
+    src = [os.path.join(HOST, "amun_dev_mod.F90"), os.path.join(HOST, "amun_host.F90")]
+    if not all(os.path.exists(s) for s in src):
+        return None
+    out = os.path.join(HOST, "amun_host.x")
+    mod = os.path.join(HOST, "_mod")
+    os.makedirs(mod, exist_ok=True)
+    objs = []
+    for s in src:
+        o = os.path.join(mod, os.path.basename(s) + ".o")
+        _run([FC, "-cpp", "-O2", "-module-dir", mod, "-c", s, "-o", o])
+        objs.append(o)
+    _run([FC, "-O2", "-o", out, *objs, "-L" + PKG, "-lamun_hip",
+          "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath," + PKG])
+    return out
+
+
+def build_reference():
+    """Reference binaries for golden vectors; skipped where the reference
+    sources are absent."""
+    ref = os.environ.get("AMUN_REFERENCE_ROOT", "/root/reference")
+    if not os.path.exists(os.path.join(ref, "src", "evolution.F90")):
+        return None
+    _run(["sh", os.path.join(ORACLE, "build_ref.sh"), ref])
+    return os.path.join(ORACLE, "_ref")
+
+
+def build_all():
+    built = {"hip": build_hip(), "oracle": build_oracle(), "host": build_host(),
+             "reference": build_reference()}
+    return built

@@ -1,0 +1,562 @@
+// physics.h -- device functions of the AMUN per-block update for gfx950.
+//
+// Pointwise equation-of-state routines, TVD limiters, the WENO5-Z cell
+// reconstruction and the HLL / HLLC / HLLD Riemann solvers, written for
+// register-resident state vectors (fully unrolled over the variable count).
+// Each routine follows the arithmetic of the reference routine cited next to
+// it; the branch structure is kept exactly, the expressions are evaluated in
+// the reference order (the compiler is still free to contract a*b+c into an
+// FMA, which is the only source of round-off differences against the CPU).
+//
+// T is the arithmetic type of reconstruction and fluxes (double, or float in
+// the mixed-precision mode).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace amun {
+
+// variable layout of the two equation systems (equations.F90:335-485)
+struct HD {
+  static constexpr int nv = 5;
+  static constexpr int idn = 0, ivx = 1, ivy = 2, ivz = 3;
+  static constexpr int imx = 1, imy = 2, imz = 3;
+  static constexpr int ipr = 4, ien = 4;
+  static constexpr int ibx = -1, iby = -1, ibz = -1, ibp = -1;
+  static constexpr bool mhd = false;
+};
+
+struct MHD {
+  static constexpr int nv = 9;
+  static constexpr int idn = 0, ivx = 1, ivy = 2, ivz = 3;
+  static constexpr int imx = 1, imy = 2, imz = 3;
+  static constexpr int ibx = 4, iby = 5, ibz = 6, ibp = 7;
+  static constexpr int ipr = 8, ien = 8;
+  static constexpr bool mhd = true;
+};
+
+// constants that every kernel needs, passed by value
+template <typename T>
+struct EosConst {
+  T gamma, gammam1, gammam1i;
+};
+
+template <typename T> __device__ __forceinline__ T tsqrt(T x);
+template <> __device__ __forceinline__ double tsqrt<double>(double x) { return sqrt(x); }
+template <> __device__ __forceinline__ float tsqrt<float>(float x) { return sqrtf(x); }
+
+template <typename T> __device__ __forceinline__ T tabs(T x) { return x < T(0) ? -x : x; }
+template <typename T> __device__ __forceinline__ T tmin(T a, T b) { return b < a ? b : a; }
+template <typename T> __device__ __forceinline__ T tmax(T a, T b) { return b > a ? b : a; }
+template <typename T> __device__ __forceinline__ T tsign(T x, T a) {
+  return copysign(x, a);
+}
+
+// ---------------------------------------------------------------------------
+// TVD limiters, interpolations.F90:4684-4877 (x is the scaling factor 1/2)
+// ---------------------------------------------------------------------------
+enum { LIM_ZERO = 0, LIM_MINMOD = 1, LIM_MC = 2, LIM_SUPERBEE = 3,
+       LIM_VANLEER = 4, LIM_VANALBADA = 5 };
+
+template <typename T>
+__device__ __forceinline__ T limiter(int kind, T x, T a, T b) {
+  switch (kind) {
+  case LIM_MINMOD:
+    return T(0.5) * (tsign(x, a) + tsign(x, b)) * tmin(tabs(a), tabs(b));
+  case LIM_MC:
+    return (tsign(x, a) + tsign(x, b)) *
+           tmin(tmin(tabs(a), tabs(b)), T(0.25) * tabs(a + b));
+  case LIM_SUPERBEE:
+    return T(0.5) * (tsign(x, a) + tsign(x, b)) *
+           tmax(tmin(T(2) * tabs(a), tabs(b)), tmin(tabs(a), T(2) * tabs(b)));
+  case LIM_VANLEER: {
+    T c = a * b;
+    return c > T(0) ? T(2) * x * c / (a + b) : T(0);
+  }
+  case LIM_VANALBADA:
+    return x * a * b * (a + b) / tmax(T(2.220446049250313e-16), a * a + b * b);
+  default:
+    return T(0);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// WENO5-Z of one cell (reconstruct_weno5z, interpolations.F90:1986-2065):
+// from f[0..4] = f(i-2..i+2) the state at the right face (fl(i), i+1/2) and
+// the state at the left face (fr(i-1), i-1/2).  The smoothness indicators are
+// shared by both faces exactly as in the reference loop body.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ void weno5z_cell(const T f[5], T eps, T& fl, T& fr) {
+  const T c1 = T(13.0 / 12.0), c2 = T(0.25);
+  const T dl = T(0.1), dc = T(0.6), dr = T(0.3);
+  const T a11 = T(2.0 / 6.0), a12 = T(-7.0 / 6.0), a13 = T(11.0 / 6.0);
+  const T a21 = T(-1.0 / 6.0), a22 = T(5.0 / 6.0), a23 = T(2.0 / 6.0);
+  const T a31 = T(2.0 / 6.0), a32 = T(5.0 / 6.0), a33 = T(-1.0 / 6.0);
+  // dfm(i) = f(i)-f(i-1), dfp(i) = f(i+1)-f(i); df2(i) = c1*(dfp-dfm)^2
+  const T dfm_m1 = f[1] - f[0];
+  const T dfm_0 = f[2] - f[1];
+  const T dfp_0 = f[3] - f[2];
+  const T dfp_p1 = f[4] - f[3];
+  const T dfp_m1 = dfm_0, dfm_p1 = dfp_0;
+  T t;
+  t = dfp_m1 - dfm_m1; const T df2_m1 = c1 * (t * t);
+  t = dfp_0 - dfm_0;   const T df2_0 = c1 * (t * t);
+  t = dfp_p1 - dfm_p1; const T df2_p1 = c1 * (t * t);
+  t = T(3) * dfm_0 - dfm_m1; const T bl = df2_m1 + c2 * (t * t);
+  t = dfp_0 + dfm_0;         const T bc = df2_0 + c2 * (t * t);
+  t = T(3) * dfp_0 - dfp_p1; const T br = df2_p1 + c2 * (t * t);
+  const T tt = tabs(br - bl);
+  const T al = T(1) + tt / (bl + eps);
+  const T ac = T(1) + tt / (bc + eps);
+  const T ar = T(1) + tt / (br + eps);
+  T wl, wc, wr, ww, ql, qc, qr;
+  wl = dl * al; wc = dc * ac; wr = dr * ar;
+  ww = (wl + wr) + wc;
+  wl = wl / ww; wr = wr / ww; wc = T(1) - (wl + wr);
+  ql = a11 * f[0] + a12 * f[1] + a13 * f[2];
+  qc = a21 * f[1] + a22 * f[2] + a23 * f[3];
+  qr = a31 * f[2] + a32 * f[3] + a33 * f[4];
+  fl = (wl * ql + wr * qr) + wc * qc;
+  wl = dl * ar; wc = dc * ac; wr = dr * al;
+  ww = (wl + wr) + wc;
+  wl = wl / ww; wr = wr / ww; wc = T(1) - (wl + wr);
+  ql = a11 * f[4] + a12 * f[3] + a13 * f[2];
+  qc = a21 * f[3] + a22 * f[2] + a23 * f[1];
+  qr = a31 * f[2] + a32 * f[1] + a33 * f[0];
+  fr = (wl * ql + wr * qr) + wc * qc;
+}
+
+// ---------------------------------------------------------------------------
+// equation of state: prim2cons / cons2prim (equations.F90:1762-1876 HD,
+// 2815-2939 MHD); fluxes and fast speeds (1896-1958, 2959-3045)
+// ---------------------------------------------------------------------------
+template <class P, typename T>
+__device__ __forceinline__ void prim2cons(const EosConst<T>& e, const T* q, T* u) {
+  u[P::idn] = q[P::idn];
+  u[P::imx] = q[P::idn] * q[P::ivx];
+  u[P::imy] = q[P::idn] * q[P::ivy];
+  u[P::imz] = q[P::idn] * q[P::ivz];
+  if constexpr (!P::mhd) {
+    T ek = T(0.5) * (u[P::imx] * q[P::ivx] + u[P::imy] * q[P::ivy] + u[P::imz] * q[P::ivz]);
+    T ei = e.gammam1i * q[P::ipr];
+    u[P::ien] = ei + ek;
+  } else {
+    u[P::ibx] = q[P::ibx];
+    u[P::iby] = q[P::iby];
+    u[P::ibz] = q[P::ibz];
+    u[P::ibp] = q[P::ibp];
+    T ei = e.gammam1i * q[P::ipr];
+    T ek = T(0.5) * (u[P::imx] * q[P::ivx] + u[P::imy] * q[P::ivy] + u[P::imz] * q[P::ivz]);
+    T em = T(0.5) * ((q[P::ibx] * q[P::ibx] + q[P::iby] * q[P::iby]) + q[P::ibz] * q[P::ibz]);
+    u[P::ien] = (ei + ek) + em;
+  }
+}
+
+template <class P, typename T>
+__device__ __forceinline__ void cons2prim(const EosConst<T>& e, const T* u, T* q) {
+  q[P::idn] = u[P::idn];
+  q[P::ivx] = u[P::imx] / u[P::idn];
+  q[P::ivy] = u[P::imy] / u[P::idn];
+  q[P::ivz] = u[P::imz] / u[P::idn];
+  if constexpr (!P::mhd) {
+    T ek = T(0.5) * (u[P::imx] * q[P::ivx] + u[P::imy] * q[P::ivy] + u[P::imz] * q[P::ivz]);
+    T ei = u[P::ien] - ek;
+    q[P::ipr] = e.gammam1 * ei;
+  } else {
+    q[P::ibx] = u[P::ibx];
+    q[P::iby] = u[P::iby];
+    q[P::ibz] = u[P::ibz];
+    q[P::ibp] = u[P::ibp];
+    T ek = T(0.5) * (u[P::imx] * q[P::ivx] + u[P::imy] * q[P::ivy] + u[P::imz] * q[P::ivz]);
+    T em = T(0.5) * ((q[P::ibx] * q[P::ibx] + q[P::iby] * q[P::iby]) + q[P::ibz] * q[P::ibz]);
+    T ei = u[P::ien] - (ek + em);
+    q[P::ipr] = e.gammam1 * ei;
+  }
+}
+
+// maxspeed_hd_adi (equations.F90:2022-2035) / maxspeed_mhd_adi (3108-3122)
+template <class P, typename T>
+__device__ __forceinline__ T maxspeed(const EosConst<T>& e, const T* q) {
+  T vv = (q[P::ivx] * q[P::ivx] + q[P::ivy] * q[P::ivy]) + q[P::ivz] * q[P::ivz];
+  T v = tsqrt(vv);
+  T c;
+  if constexpr (!P::mhd) {
+    c = tsqrt(e.gamma * q[P::ipr] / q[P::idn]);
+  } else {
+    T bb = (q[P::ibx] * q[P::ibx] + q[P::iby] * q[P::iby]) + q[P::ibz] * q[P::ibz];
+    c = tsqrt((e.gamma * q[P::ipr] + bb) / q[P::idn]);
+  }
+  return v + c;
+}
+
+template <class P, typename T>
+__device__ __forceinline__ void fluxspeed(const EosConst<T>& e, T cmax2, const T* q,
+                                          const T* u, T* f, T& cm, T& cp) {
+  if constexpr (!P::mhd) {
+    f[P::idn] = u[P::imx];
+    f[P::imx] = q[P::ivx] * u[P::imx];
+    f[P::imy] = q[P::ivx] * u[P::imy];
+    f[P::imz] = q[P::ivx] * u[P::imz];
+    f[P::imx] = f[P::imx] + q[P::ipr];
+    f[P::ien] = q[P::ivx] * (u[P::ien] + q[P::ipr]);
+    T cs = tsqrt(e.gamma * q[P::ipr] / q[P::idn]);
+    cm = q[P::ivx] - cs;
+    cp = q[P::ivx] + cs;
+  } else {
+    T bx2 = q[P::ibx] * q[P::ibx];
+    T by2 = q[P::iby] * q[P::iby];
+    T bz2 = q[P::ibz] * q[P::ibz];
+    T bb = (bx2 + by2) + bz2;
+    T vb = (q[P::ivx] * q[P::ibx] + q[P::ivy] * q[P::iby]) + q[P::ivz] * q[P::ibz];
+    T pt = q[P::ipr] + T(0.5) * bb;
+    f[P::idn] = u[P::imx];
+    f[P::imx] = q[P::ivx] * u[P::imx] - bx2;
+    f[P::imy] = q[P::ivx] * u[P::imy] - q[P::ibx] * q[P::iby];
+    f[P::imz] = q[P::ivx] * u[P::imz] - q[P::ibx] * q[P::ibz];
+    f[P::imx] = f[P::imx] + pt;
+    f[P::ibx] = q[P::ibp];
+    f[P::iby] = q[P::ivx] * q[P::iby] - q[P::ibx] * q[P::ivy];
+    f[P::ibz] = q[P::ivx] * q[P::ibz] - q[P::ibx] * q[P::ivz];
+    f[P::ibp] = cmax2 * q[P::ibx];
+    f[P::ien] = q[P::ivx] * (u[P::ien] + pt) - q[P::ibx] * vb;
+    T fa = e.gamma * q[P::ipr];
+    T fb = fa + bb;
+    T fc = fb * fb - T(4) * fa * bx2;
+    T cf;
+    if (fc > T(0)) cf = tsqrt(T(0.5) * (fb + tsqrt(fc)) / q[P::idn]);
+    else           cf = tsqrt(T(0.5) * fb / q[P::idn]);
+    cm = q[P::ivx] - cf;
+    cp = q[P::ivx] + cf;
+  }
+}
+
+// GLM pre-step of every MHD solver (schemes.F90:3640-3652): Bx and psi of
+// both states replaced by their upwinded interface values
+template <class P, typename T>
+__device__ __forceinline__ void glm_upwind(T* ql, T* qr, T cmax) {
+  if constexpr (P::mhd) {
+    T bx = T(0.5) * ((qr[P::ibx] + ql[P::ibx]) - (qr[P::ibp] - ql[P::ibp]) / cmax);
+    T bp = T(0.5) * ((qr[P::ibp] + ql[P::ibp]) - (qr[P::ibx] - ql[P::ibx]) * cmax);
+    ql[P::ibx] = bx; qr[P::ibx] = bx;
+    ql[P::ibp] = bp; qr[P::ibp] = bp;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Riemann solvers of one interface.  ql/qr are overwritten (intent(inout) in
+// the reference).
+// ---------------------------------------------------------------------------
+// riemann_hd_adi_hll (schemes.F90:1248-1341) / riemann_mhd_adi_hll (3163-3280)
+template <class P, typename T>
+__device__ __forceinline__ void riemann_hll(const EosConst<T>& e, T cmax, T cmax2,
+                                            T* ql, T* qr, T* f) {
+  constexpr int nv = P::nv;
+  T ul[nv], ur[nv], fl[nv], fr[nv];
+  T clm, clp, crm, crp;
+  glm_upwind<P>(ql, qr, cmax);
+  prim2cons<P>(e, ql, ul);
+  prim2cons<P>(e, qr, ur);
+  fluxspeed<P>(e, cmax2, ql, ul, fl, clm, clp);
+  fluxspeed<P>(e, cmax2, qr, ur, fr, crm, crp);
+  T sl = tmin(clm, crm), sr = tmax(clp, crp);
+  if (sl >= T(0)) {
+#pragma unroll
+    for (int p = 0; p < nv; p++) f[p] = fl[p];
+  } else if (sr <= T(0)) {
+#pragma unroll
+    for (int p = 0; p < nv; p++) f[p] = fr[p];
+  } else {
+    T srml = sr - sl;
+#pragma unroll
+    for (int p = 0; p < nv; p++) {
+      T wl = sl * ul[p] - fl[p];
+      T wr = sr * ur[p] - fr[p];
+      f[p] = (sl * wr - sr * wl) / srml;
+    }
+  }
+}
+
+// riemann_hd_adi_hllc, schemes.F90:1366-1513
+template <class P, typename T>
+__device__ __forceinline__ void riemann_hllc(const EosConst<T>& e, T cmax, T cmax2,
+                                             T* ql, T* qr, T* f) {
+  static_assert(!P::mhd, "HLLC is the hydrodynamic solver");
+  constexpr int nv = P::nv;
+  T ul[nv], ur[nv], fl[nv], fr[nv], wl[nv], wr[nv], ui[nv];
+  T clm, clp, crm, crp;
+  prim2cons<P>(e, ql, ul);
+  prim2cons<P>(e, qr, ur);
+  fluxspeed<P>(e, cmax2, ql, ul, fl, clm, clp);
+  fluxspeed<P>(e, cmax2, qr, ur, fr, crm, crp);
+  T sl = tmin(clm, crm), sr = tmax(clp, crp);
+  if (sl >= T(0)) {
+#pragma unroll
+    for (int p = 0; p < nv; p++) f[p] = fl[p];
+    return;
+  }
+  if (sr <= T(0)) {
+#pragma unroll
+    for (int p = 0; p < nv; p++) f[p] = fr[p];
+    return;
+  }
+#pragma unroll
+  for (int p = 0; p < nv; p++) {
+    wl[p] = sl * ul[p] - fl[p];
+    wr[p] = sr * ur[p] - fr[p];
+  }
+  T dn = wr[P::idn] - wl[P::idn];
+  T sm = (wr[P::imx] - wl[P::imx]) / dn;
+  T pr = (wl[P::idn] * wr[P::imx] - wr[P::idn] * wl[P::imx]) / dn;
+  if (sm > T(0)) {
+    T slmm = sl - sm;
+    ui[P::idn] = wl[P::idn] / slmm;
+    ui[P::imx] = ui[P::idn] * sm;
+    ui[P::imy] = ui[P::idn] * ql[P::ivy];
+    ui[P::imz] = ui[P::idn] * ql[P::ivz];
+    ui[P::ien] = (wl[P::ien] + sm * pr) / slmm;
+#pragma unroll
+    for (int p = 0; p < nv; p++) f[p] = sl * ui[p] - wl[p];
+  } else if (sm < T(0)) {
+    T srmm = sr - sm;
+    ui[P::idn] = wr[P::idn] / srmm;
+    ui[P::imx] = ui[P::idn] * sm;
+    ui[P::imy] = ui[P::idn] * qr[P::ivy];
+    ui[P::imz] = ui[P::idn] * qr[P::ivz];
+    ui[P::ien] = (wr[P::ien] + sm * pr) / srmm;
+#pragma unroll
+    for (int p = 0; p < nv; p++) f[p] = sr * ui[p] - wr[p];
+  } else {
+    f[P::idn] = T(0);
+    f[P::imx] = -wl[P::imx];
+    f[P::imy] = T(0);
+    f[P::imz] = T(0);
+    f[P::ien] = T(0);
+  }
+}
+
+// star state of the HLLD fan from a wave vector w (outer state, speed diff
+// smm = s - sm, divisor dv = smm*w_rho - Bx^2), schemes.F90:3740-3760
+template <class P, typename T>
+__device__ __forceinline__ void hlld_outer(const T* w, T smm, T dv, T dn, T sm,
+                                           T bx, T pt, T psi, T* ui) {
+  T vy = (smm * w[P::imy] - bx * w[P::iby]) / dv;
+  T vz = (smm * w[P::imz] - bx * w[P::ibz]) / dv;
+  T by = (w[P::idn] * w[P::iby] - bx * w[P::imy]) / dv;
+  T bz = (w[P::idn] * w[P::ibz] - bx * w[P::imz]) / dv;
+  T vb = (sm * bx + vy * by) + vz * bz;
+  ui[P::idn] = dn;
+  ui[P::imx] = dn * sm;
+  ui[P::imy] = dn * vy;
+  ui[P::imz] = dn * vz;
+  ui[P::ibx] = bx;
+  ui[P::iby] = by;
+  ui[P::ibz] = bz;
+  ui[P::ibp] = psi;
+  ui[P::ien] = ((w[P::ien] + sm * pt) - bx * vb) / smm;
+}
+
+template <class P, typename T>
+__device__ __forceinline__ void hlld_inner_state(T dn, T sm, T vy, T vz, T bx, T by,
+                                                 T bz, T psi, T* ui) {
+  ui[P::idn] = dn;
+  ui[P::imx] = dn * sm;
+  ui[P::imy] = dn * vy;
+  ui[P::imz] = dn * vz;
+  ui[P::ibx] = bx;
+  ui[P::iby] = by;
+  ui[P::ibz] = bz;
+  ui[P::ibp] = psi;
+}
+
+// riemann_mhd_adi_hlld, schemes.F90:3596-4203
+template <class P, typename T>
+__device__ __forceinline__ void riemann_hlld(const EosConst<T>& e, T cmax, T cmax2,
+                                             T* ql, T* qr, T* f) {
+  static_assert(P::mhd, "HLLD is the MHD solver");
+  constexpr int nv = P::nv;
+  constexpr int idn = P::idn, imx = P::imx, imy = P::imy, imz = P::imz;
+  constexpr int ibx = P::ibx, iby = P::iby, ibz = P::ibz, ibp = P::ibp, ien = P::ien;
+  T ul[nv], ur[nv], fl[nv], fr[nv], wl[nv], wr[nv], wcl[nv], wcr[nv], ui[nv];
+  T clm, clp, crm, crp;
+  glm_upwind<P>(ql, qr, cmax);
+  prim2cons<P>(e, ql, ul);
+  prim2cons<P>(e, qr, ur);
+  fluxspeed<P>(e, cmax2, ql, ul, fl, clm, clp);
+  fluxspeed<P>(e, cmax2, qr, ur, fr, crm, crp);
+  T sl = tmin(clm, crm), sr = tmax(clp, crp);
+  if (sl >= T(0)) {
+#pragma unroll
+    for (int p = 0; p < nv; p++) f[p] = fl[p];
+    return;
+  }
+  if (sr <= T(0)) {
+#pragma unroll
+    for (int p = 0; p < nv; p++) f[p] = fr[p];
+    return;
+  }
+  T srml = sr - sl;
+#pragma unroll
+  for (int p = 0; p < nv; p++) {
+    wl[p] = sl * ul[p] - fl[p];
+    wr[p] = sr * ur[p] - fr[p];
+  }
+  T dn = wr[idn] - wl[idn];
+  T sm = (wr[imx] - wl[imx]) / dn;
+  T slmm = sl - sm;
+  T srmm = sr - sm;
+  T dnl = wl[idn] / slmm;
+  T dnr = wr[idn] / srmm;
+  T bx = ql[ibx];
+  T b2 = ql[ibx] * qr[ibx];
+  T pt = (wl[idn] * wr[imx] - wr[idn] * wl[imx]) / dn + b2;
+  T psil = ul[ibp], psir = ur[ibp];
+
+  if (b2 > T(1.0e-08) * tmax(dnl * (sl * sl), dnr * (sr * sr))) {
+    T cal = tsqrt(b2 / dnl);
+    T car = tsqrt(b2 / dnr);
+    T sml = sm - cal;
+    T smr = sm + car;
+    T dvl = slmm * wl[idn] - b2;
+    T dvr = srmm * wr[idn] - b2;
+    if (dvl > T(0) && dvr > T(0)) {
+      if (sml >= T(0)) {
+        hlld_outer<P>(wl, slmm, dvl, dnl, sm, bx, pt, psil, ui);
+#pragma unroll
+        for (int p = 0; p < nv; p++) f[p] = sl * ui[p] - wl[p];
+      } else if (smr <= T(0)) {
+        hlld_outer<P>(wr, srmm, dvr, dnr, sm, bx, pt, psir, ui);
+#pragma unroll
+        for (int p = 0; p < nv; p++) f[p] = sr * ui[p] - wr[p];
+      } else {
+        hlld_outer<P>(wl, slmm, dvl, dnl, sm, bx, pt, psil, ui);
+#pragma unroll
+        for (int p = 0; p < nv; p++) wcl[p] = (sml - sl) * ui[p] + wl[p];
+        hlld_outer<P>(wr, srmm, dvr, dnr, sm, bx, pt, psir, ui);
+#pragma unroll
+        for (int p = 0; p < nv; p++) wcr[p] = (smr - sr) * ui[p] + wr[p];
+        T dv = car * dnr + cal * dnl;
+        T vy = (wcr[imy] - wcl[imy]) / dv;
+        T vz = (wcr[imz] - wcl[imz]) / dv;
+        dv = car + cal;
+        T by = (wcr[iby] - wcl[iby]) / dv;
+        T bz = (wcr[ibz] - wcl[ibz]) / dv;
+        T vb = (sm * bx + vy * by) + vz * bz;
+        if (sm > T(0)) {
+          hlld_inner_state<P>(dnl, sm, vy, vz, bx, by, bz, psil, ui);
+          ui[ien] = -((wcl[ien] + sm * pt) - bx * vb) / cal;
+#pragma unroll
+          for (int p = 0; p < nv; p++) f[p] = sml * ui[p] - wcl[p];
+        } else if (sm < T(0)) {
+          hlld_inner_state<P>(dnr, sm, vy, vz, bx, by, bz, psir, ui);
+          ui[ien] = ((wcr[ien] + sm * pt) - bx * vb) / car;
+#pragma unroll
+          for (int p = 0; p < nv; p++) f[p] = smr * ui[p] - wcr[p];
+        } else {
+#pragma unroll
+          for (int p = 0; p < nv; p++) f[p] = (sml * wcr[p] - smr * wcl[p]) / (smr - sml);
+        }
+      }
+    } else if (dvl > T(0)) {
+      hlld_outer<P>(wl, slmm, dvl, dnl, sm, bx, pt, psil, ui);
+      if (sml >= T(0)) {
+#pragma unroll
+        for (int p = 0; p < nv; p++) f[p] = sl * ui[p] - wl[p];
+      } else {
+#pragma unroll
+        for (int p = 0; p < nv; p++) wcl[p] = (sml - sl) * ui[p] + wl[p];
+        T dv = srmm * dnr + cal * dnl;
+        T vy = (wr[imy] - wcl[imy]) / dv;
+        T vz = (wr[imz] - wcl[imz]) / dv;
+        dv = sr - sml;
+        T by = (wr[iby] - wcl[iby]) / dv;
+        T bz = (wr[ibz] - wcl[ibz]) / dv;
+        T vb = (sm * bx + vy * by) + vz * bz;
+        hlld_inner_state<P>(dnl, sm, vy, vz, bx, by, bz, psil, ui);
+        ui[ien] = -((wcl[ien] + sm * pt) - bx * vb) / cal;
+        if (sm >= T(0)) {
+#pragma unroll
+          for (int p = 0; p < nv; p++) f[p] = sml * ui[p] - wcl[p];
+        } else {
+#pragma unroll
+          for (int p = 0; p < nv; p++) wcr[p] = (sm - sml) * ui[p] + wcl[p];
+#pragma unroll
+          for (int p = 0; p < nv; p++) f[p] = (sm * wr[p] - sr * wcr[p]) / srmm;
+        }
+      }
+    } else if (dvr > T(0)) {
+      hlld_outer<P>(wr, srmm, dvr, dnr, sm, bx, pt, psir, ui);
+      if (smr <= T(0)) {
+#pragma unroll
+        for (int p = 0; p < nv; p++) f[p] = sr * ui[p] - wr[p];
+      } else {
+#pragma unroll
+        for (int p = 0; p < nv; p++) wcr[p] = (smr - sr) * ui[p] + wr[p];
+        T dv = slmm * dnl - car * dnr;
+        T vy = (wl[imy] - wcr[imy]) / dv;
+        T vz = (wl[imz] - wcr[imz]) / dv;
+        dv = sl - smr;
+        T by = (wl[iby] - wcr[iby]) / dv;
+        T bz = (wl[ibz] - wcr[ibz]) / dv;
+        T vb = (sm * bx + vy * by) + vz * bz;
+        hlld_inner_state<P>(dnr, sm, vy, vz, bx, by, bz, psir, ui);
+        ui[ien] = ((wcr[ien] + sm * pt) - bx * vb) / car;
+        if (sm <= T(0)) {
+#pragma unroll
+          for (int p = 0; p < nv; p++) f[p] = smr * ui[p] - wcr[p];
+        } else {
+#pragma unroll
+          for (int p = 0; p < nv; p++) wcl[p] = (sm - smr) * ui[p] + wcr[p];
+#pragma unroll
+          for (int p = 0; p < nv; p++) f[p] = (sm * wl[p] - sl * wcl[p]) / slmm;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int p = 0; p < nv; p++) f[p] = (sl * wr[p] - sr * wl[p]) / srml;
+    }
+  } else if (b2 > T(0)) {
+    T vy = (wr[imy] - wl[imy]) / dn;
+    T vz = (wr[imz] - wl[imz]) / dn;
+    T by = (wr[iby] - wl[iby]) / srml;
+    T bz = (wr[ibz] - wl[ibz]) / srml;
+    T vb = (sm * bx + vy * by) + vz * bz;
+    if (sm > T(0)) {
+      hlld_inner_state<P>(dnl, sm, vy, vz, bx, by, bz, psil, ui);
+      ui[ien] = ((wl[ien] + sm * pt) - bx * vb) / slmm;
+#pragma unroll
+      for (int p = 0; p < nv; p++) f[p] = sl * ui[p] - wl[p];
+    } else if (sm < T(0)) {
+      hlld_inner_state<P>(dnr, sm, vy, vz, bx, by, bz, psir, ui);
+      ui[ien] = ((wr[ien] + sm * pt) - bx * vb) / srmm;
+#pragma unroll
+      for (int p = 0; p < nv; p++) f[p] = sr * ui[p] - wr[p];
+    } else {
+#pragma unroll
+      for (int p = 0; p < nv; p++) f[p] = (sl * wr[p] - sr * wl[p]) / srml;
+    }
+  } else {
+    if (sm > T(0)) {
+      ui[idn] = dnl; ui[imx] = dnl * sm;
+      ui[imy] = wl[imy] / slmm; ui[imz] = wl[imz] / slmm;
+      ui[ibx] = T(0); ui[iby] = wl[iby] / slmm; ui[ibz] = wl[ibz] / slmm;
+      ui[ibp] = psil;
+      ui[ien] = (wl[ien] + sm * pt) / slmm;
+#pragma unroll
+      for (int p = 0; p < nv; p++) f[p] = sl * ui[p] - wl[p];
+    } else if (sm < T(0)) {
+      ui[idn] = dnr; ui[imx] = dnr * sm;
+      ui[imy] = wr[imy] / srmm; ui[imz] = wr[imz] / srmm;
+      ui[ibx] = T(0); ui[iby] = wr[iby] / srmm; ui[ibz] = wr[ibz] / srmm;
+      ui[ibp] = psir;
+      ui[ien] = (wr[ien] + sm * pt) / srmm;
+#pragma unroll
+      for (int p = 0; p < nv; p++) f[p] = sr * ui[p] - wr[p];
+    } else {
+#pragma unroll
+      for (int p = 0; p < nv; p++) f[p] = (sl * wr[p] - sr * wl[p]) / srml;
+    }
+  }
+}
+
+}  // namespace amun

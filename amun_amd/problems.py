@@ -118,8 +118,13 @@ def _synth(p: Params, ks: np.ndarray, a: np.ndarray) -> np.ndarray:
         spec[:, kz % nz, ky % ny, kx % nx] += c
         spec[:, (-kz) % nz, (-ky) % ny, (-kx) % nx] += np.conj(c)
     out = np.empty((3, nz, ny, nx))
-    for d in range(3):
-        out[d] = np.fft.ifftn(spec[d]).real * (nx * ny * nz)
+    try:
+        import scipy.fft as sfft
+        for d in range(3):
+            out[d] = sfft.ifftn(spec[d], workers=-1).real * (nx * ny * nz)
+    except ImportError:
+        for d in range(3):
+            out[d] = np.fft.ifftn(spec[d]).real * (nx * ny * nz)
     return out
 
 

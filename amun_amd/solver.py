@@ -1,0 +1,219 @@
+"""ctypes binding of the C-ABI (include/amun_dev.h) of libamun_hip.so.
+
+``Solver`` mirrors the reference's step-level interface: ``new_time_step``
+(evolution.F90:379), ``advance`` (evolution.F90:292), the ``time``/``step``
+counters (evolution.F90:75), ``store_integrals`` (integrals.F90:254) and the
+host transfers of block data.  Every call goes to the HIP library; there is
+no CPU fallback -- a missing library raises ``RuntimeError``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from .config import AmunCfg, Params
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libamun_hip.so")
+_P = ctypes.POINTER(ctypes.c_double)
+
+EXPORTS = (
+    "amun_dev_default_cfg", "amun_dev_init", "amun_dev_finalize",
+    "amun_dev_upload_blocks", "amun_dev_upload_global",
+    "amun_dev_download_blocks", "amun_dev_download_global",
+    "amun_dev_new_time_step", "amun_dev_step", "amun_dev_run",
+    "amun_dev_integrals", "amun_dev_get_time", "amun_dev_nblocks",
+    "amun_dev_block_coords", "amun_dev_last_error", "amun_dev_sync",
+    "amun_dev_positivity_events", "amun_dev_stream", "amun_dev_timed_run",
+)
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libamun_hip.so (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"{path} is missing: the HIP library must be built "
+            "(__graft_entry__.build()); there is no CPU fallback")
+    L = ctypes.CDLL(path)
+    vp = ctypes.c_void_p
+    L.amun_dev_default_cfg.argtypes = [ctypes.POINTER(AmunCfg)]
+    L.amun_dev_default_cfg.restype = None
+    L.amun_dev_init.argtypes = [ctypes.POINTER(AmunCfg), ctypes.POINTER(vp)]
+    L.amun_dev_finalize.argtypes = [vp]
+    L.amun_dev_finalize.restype = None
+    L.amun_dev_upload_blocks.argtypes = [vp, _P]
+    L.amun_dev_upload_global.argtypes = [vp, _P]
+    L.amun_dev_download_blocks.argtypes = [vp, _P, _P]
+    L.amun_dev_download_global.argtypes = [vp, _P, _P]
+    L.amun_dev_new_time_step.argtypes = [vp, ctypes.c_double, _P, _P]
+    L.amun_dev_step.argtypes = [vp, ctypes.c_double, _P, _P]
+    L.amun_dev_run.argtypes = [vp, ctypes.c_int32, ctypes.c_double, _P]
+    L.amun_dev_integrals.argtypes = [vp, _P]
+    L.amun_dev_get_time.argtypes = [vp, _P, ctypes.POINTER(ctypes.c_int64)]
+    L.amun_dev_nblocks.argtypes = [vp, ctypes.POINTER(ctypes.c_int32)]
+    L.amun_dev_block_coords.argtypes = [vp, ctypes.POINTER(ctypes.c_int32)]
+    L.amun_dev_timed_run.argtypes = [vp, ctypes.c_int32, ctypes.c_double, _P, _P,
+                                     ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]
+    L.amun_dev_stream.argtypes = [vp]
+    L.amun_dev_stream.restype = ctypes.c_void_p
+    L.amun_dev_last_error.argtypes = []
+    L.amun_dev_last_error.restype = ctypes.c_char_p
+    for name in ("amun_dev_sync", "amun_dev_positivity_events"):
+        getattr(L, name).argtypes = [vp] + ([ctypes.POINTER(ctypes.c_uint64)]
+                                            if name.endswith("events") else [])
+    _lib = L
+    return L
+
+
+class AmunError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"amun_dev error {code}: {msg}")
+        self.code = code
+
+
+def _dp(a):
+    return None if a is None else a.ctypes.data_as(_P)
+
+
+class Solver:
+    """One device context: the per-block update of one run on ``cuda:0``."""
+
+    def __init__(self, params: Params):
+        self.lib = load_library()
+        self.params = params
+        self.cfg = params.to_cfg()
+        h = ctypes.c_void_p()
+        self._check(self.lib.amun_dev_init(ctypes.byref(self.cfg), ctypes.byref(h)))
+        self.h = h
+
+    def _check(self, rc: int):
+        if rc != 0:
+            raise AmunError(rc, self.lib.amun_dev_last_error().decode(errors="replace"))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.amun_dev_finalize(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ------------------------------------------------------------ transfers
+    def upload(self, u: np.ndarray):
+        """Global conserved state U[nv][nz][ny][nx]."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        nz, ny, nx = self.params.shape
+        if u.shape != (self.params.nv, nz, ny, nx):
+            raise ValueError(f"state shape {u.shape} != {(self.params.nv, nz, ny, nx)}")
+        self._check(self.lib.amun_dev_upload_global(self.h, _dp(u)))
+
+    def download(self, with_q: bool = False):
+        nz, ny, nx = self.params.shape
+        u = np.empty((self.params.nv, nz, ny, nx))
+        q = np.empty_like(u) if with_q else None
+        self._check(self.lib.amun_dev_download_global(self.h, _dp(u), _dp(q)))
+        return (u, q) if with_q else u
+
+    def _block_shape(self):
+        p = self.params
+        im = p.ncells + 2 * p.nghosts
+        km = im if p.ndims == 3 else 1
+        return (p.nblocks, km, im, im, p.nv)
+
+    def upload_blocks(self, u: np.ndarray):
+        """Host blocks u0(nv,im,jm,km) (Fortran order) one after another."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        if u.shape != self._block_shape():
+            raise ValueError(f"block array shape {u.shape} != {self._block_shape()}")
+        self._check(self.lib.amun_dev_upload_blocks(self.h, _dp(u)))
+
+    def download_blocks(self, with_q: bool = False):
+        u = np.empty(self._block_shape())
+        q = np.empty_like(u) if with_q else None
+        self._check(self.lib.amun_dev_download_blocks(self.h, _dp(u), _dp(q)))
+        return (u, q) if with_q else u
+
+    def block_coords(self) -> np.ndarray:
+        n = ctypes.c_int32()
+        self._check(self.lib.amun_dev_nblocks(self.h, ctypes.byref(n)))
+        c = np.empty((n.value, 3), dtype=np.int32)
+        self._check(self.lib.amun_dev_block_coords(
+            self.h, c.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+        return c
+
+    # ------------------------------------------------------------ stepping
+    def new_time_step(self, dtnext: float = 2.0e30):
+        dt, cm = ctypes.c_double(), ctypes.c_double()
+        self._check(self.lib.amun_dev_new_time_step(self.h, dtnext, ctypes.byref(dt),
+                                                    ctypes.byref(cm)))
+        return dt.value, cm.value
+
+    def advance(self, dtnext: float = 2.0e30):
+        """One step; returns (dt, cmax) used by it."""
+        dt, cm = ctypes.c_double(), ctypes.c_double()
+        self._check(self.lib.amun_dev_step(self.h, dtnext, ctypes.byref(dt), ctypes.byref(cm)))
+        return dt.value, cm.value
+
+    def run(self, nsteps: int, dtnext: float = 2.0e30) -> np.ndarray:
+        """nsteps steps queued back to back; returns the dt of each step."""
+        dts = np.zeros(max(nsteps, 1))
+        self._check(self.lib.amun_dev_run(self.h, nsteps, dtnext, _dp(dts)))
+        return dts[:nsteps]
+
+    def launch(self, nsteps: int, dtnext: float = 2.0e30):
+        """Queue nsteps without reading anything back (timing loops)."""
+        self._check(self.lib.amun_dev_run(self.h, nsteps, dtnext, None))
+
+    def timed_run(self, nsteps: int, dtnext: float = 2.0e30, per_kernel: bool = False):
+        """Queue nsteps with HIP events on the solver stream; returns the
+        device time in ms and (per_kernel) the average ms of each kernel of a
+        step in launch order."""
+        ms = ctypes.c_double()
+        nk = ctypes.c_int32()
+        kms = np.zeros(64)
+        self._check(self.lib.amun_dev_timed_run(
+            self.h, nsteps, dtnext, ctypes.byref(ms), _dp(kms) if per_kernel else None,
+            64 if per_kernel else 0, ctypes.byref(nk)))
+        return ms.value, (kms[: nk.value].copy() if per_kernel else None)
+
+    def sync(self):
+        self._check(self.lib.amun_dev_sync(self.h))
+
+    @property
+    def time(self) -> float:
+        t, s = ctypes.c_double(), ctypes.c_int64()
+        self._check(self.lib.amun_dev_get_time(self.h, ctypes.byref(t), ctypes.byref(s)))
+        return t.value
+
+    @property
+    def step(self) -> int:
+        t, s = ctypes.c_double(), ctypes.c_int64()
+        self._check(self.lib.amun_dev_get_time(self.h, ctypes.byref(t), ctypes.byref(s)))
+        return s.value
+
+    def integrals(self) -> dict:
+        out = np.zeros(8)
+        self._check(self.lib.amun_dev_integrals(self.h, _dp(out)))
+        keys = ("mass", "momx", "momy", "momz", "ener", "ekin", "emag", "eint")
+        return dict(zip(keys, out.tolist()))
+
+    def positivity_events(self) -> int:
+        n = ctypes.c_uint64()
+        self._check(self.lib.amun_dev_positivity_events(self.h, ctypes.byref(n)))
+        return n.value

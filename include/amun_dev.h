@@ -7,7 +7,7 @@
  * combine, primitive recovery and the ghost-zone exchange between blocks --
  * as hand-written gfx950 kernels.  Every entry point takes plain pointers and
  * sizes; a Fortran host binds them with iso_c_binding (amun_amd/host/
- * amun_dev_mod.F90), Python with ctypes (amun_amd/_lib.py).
+ * amun_dev_mod.F90), Python with ctypes (amun_amd/solver.py).
  *
  * Return codes: 0 = success.  Configuration errors reuse the reference's
  * integer codes (100/110 equation system / equation of state,
@@ -130,6 +130,15 @@ int  amun_dev_step(amun_ctx* ctx, double dtnext, double* dt, double* cmax);
  * at the end); dts (may be NULL) receives the dt of every step. */
 int  amun_dev_run(amun_ctx* ctx, int32_t nsteps, double dtnext, double* dts);
 
+/* Timing helper: queue nsteps steps on the context stream with HIP events
+ * around them (and, if kernel_ms has room for *nkernel entries, after every
+ * kernel of every step); elapsed_ms = device time of all steps, kernel_ms[k] =
+ * average duration of the k-th kernel of a step.  No reference counterpart
+ * (the reference times steps with system_clock, driver.F90:730-793). */
+int  amun_dev_timed_run(amun_ctx* ctx, int32_t nsteps, double dtnext,
+                        double* elapsed_ms, double* kernel_ms,
+                        int32_t nkernel_max, int32_t* nkernel);
+
 /* Volume integrals of store_integrals (integrals.F90:254-485), FP64 on the
  * device: out[0]=mass, [1..3]=momentum, [4]=total energy, [5]=kinetic,
  * [6]=magnetic energy, [7]=internal energy. */
@@ -142,6 +151,18 @@ int  amun_dev_get_time(amun_ctx* ctx, double* time, int64_t* step);
  * 0-based) in the order used by the *_blocks transfers. */
 int  amun_dev_nblocks(amun_ctx* ctx, int32_t* nblk);
 int  amun_dev_block_coords(amun_ctx* ctx, int32_t* coords);
+
+/* Wait until all work queued on the context's stream has finished. */
+int  amun_dev_sync(amun_ctx* ctx);
+
+/* Number of non-positive density or pressure values met by the TVD
+ * reconstruction so far (the reference prints a warning for each one and
+ * zeroes the slopes, interpolations.F90:814-822). */
+int  amun_dev_positivity_events(amun_ctx* ctx, uint64_t* n);
+
+/* The context's HIP stream (a hipStream_t), so that a caller can order its
+ * own device work after the solver's. */
+void* amun_dev_stream(amun_ctx* ctx);
 
 /* Message of the last failing call of this thread. */
 const char* amun_dev_last_error(void);

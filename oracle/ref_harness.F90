@@ -26,6 +26,9 @@
 !   run   in : U(nx,ny,nz,nv)            conserved variables of the whole grid
 !         out: <prefix>_u.bin  U(nx,ny,nz,nv) after harness_steps steps
 !              <prefix>_dt.bin (dt, cmax) for each step
+!              <prefix>_time.txt wall-clock seconds of the step loop
+!   ghosts in: as run; out: <prefix>_q.bin every block's q with ghost layers
+!              after the initial boundary_variables, preceded by its offsets
 !   flux  in : count x [ q(nv,im,jm,km), dx(NDIMS) ]
 !         out: <prefix>_f.bin  count x f(NDIMS,nv,im,jm,km)
 !   recon in : f(n,count), h
@@ -133,6 +136,7 @@ contains
     real(kind=8), dimension(:,:,:,:), allocatable :: ug
     real(kind=8), dimension(:,:)    , allocatable :: trace
     integer :: nx, ny, nz, i0, j0, k0, p, s, u
+    integer(kind=8) :: c0, c1, crate
 
     nx = ir * nc
     ny = jr * nc
@@ -192,6 +196,7 @@ contains
       return
     end if
 
+    call system_clock(c0, crate)
     do s = 1, nsteps
       call advance(dtnext)
       trace(1,s) = dt
@@ -199,6 +204,12 @@ contains
       time = time + dt
       step = step + 1
     end do
+    call system_clock(c1)
+
+! wall-clock seconds of the step loop only (CPU baseline timing)
+    open(newunit=u, file=trim(fout)//"_time.txt", status="replace")
+    write(u,*) real(c1 - c0, kind=8) / real(crate, kind=8)
+    close(u)
 
     pdata => list_data
     do while (associated(pdata))
