@@ -153,3 +153,38 @@ def initial_state(p: Params, seed: int = DEFAULT_SEED) -> np.ndarray:
     if EQSYS[p.equation_system.lower()] == 1:
         return blast_hd_2d(p, seed)
     return turbulent_mhd_3d(p, seed)
+
+
+def orszag_tang_2d(p: Params, seed: int = DEFAULT_SEED, noise: float = 0.01) -> np.ndarray:
+    """Orszag-Tang-type vortex on [0,1]^2: rho = 25/(36 pi), p = 5/(12 pi),
+    v = (-sin 2 pi y, sin 2 pi x, 0), B = (-sin 2 pi y, sin 4 pi x, 0)/sqrt(4 pi),
+    psi = 0, with a seeded uniform density perturbation rho *= 1 + noise (2 xi - 1)."""
+    nz, ny, nx = p.shape
+    z, y, x = cell_centres(p)
+    Y, X = np.meshgrid(y, x, indexing="ij")
+    q = np.zeros((p.nv, nz, ny, nx))
+    xi = uniform(seed, nz * ny * nx, stream=5).reshape(nz, ny, nx)
+    q[0] = 25.0 / (36.0 * np.pi) * (1.0 + noise * (2.0 * xi - 1.0))
+    q[1] = -np.sin(2.0 * np.pi * Y)
+    q[2] = np.sin(2.0 * np.pi * X)
+    b0 = 1.0 / np.sqrt(4.0 * np.pi)
+    q[4] = -b0 * np.sin(2.0 * np.pi * Y)
+    q[5] = b0 * np.sin(4.0 * np.pi * X)
+    q[8] = 5.0 / (12.0 * np.pi)
+    return prim_to_cons(p, q)
+
+
+def kelvin_helmholtz_2d(p: Params, seed: int = DEFAULT_SEED) -> np.ndarray:
+    """MHD Kelvin-Helmholtz shear layer: vx = 0.5 tanh(y/0.02), seeded vy noise
+    1e-3 U[-1,1], rho = 1 (y > 0) / 2 (y < 0), p = 2.5, uniform Bx = 0.1."""
+    nz, ny, nx = p.shape
+    z, y, x = cell_centres(p)
+    Y = np.broadcast_to(y[None, :, None], (nz, ny, nx))
+    q = np.zeros((p.nv, nz, ny, nx))
+    xi = uniform(seed, nz * ny * nx, stream=3).reshape(nz, ny, nx)
+    q[0] = np.where(Y > 0.0, 1.0, 2.0)
+    q[1] = 0.5 * np.tanh(Y / 0.02)
+    q[2] = 1.0e-3 * (2.0 * xi - 1.0)
+    q[4] = 0.1
+    q[8] = 2.5
+    return prim_to_cons(p, q)

@@ -1,0 +1,136 @@
+#!/usr/bin/env python3
+"""Write the golden fixtures of tests/golden/ from the reference binaries
+(oracle/_ref, built from the unmodified reference sources by
+oracle/build_ref.sh).  Run in the container that holds the reference:
+
+    python tests/golden/make_golden.py
+
+Every fixture stores the configuration (as AMUN parameter text), the seed,
+the reference outputs needed by the tests (sha256 of the exact bytes, the
+(dt, cmax) trace, a strided subsample of the final state, or full small
+arrays for the unit-level vectors).
+"""
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.dirname(HERE))
+
+from amun_amd.config import Params, c1_params, c2_params  # noqa: E402
+from amun_amd import problems  # noqa: E402
+import oracle_lib as ol  # noqa: E402
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a, dtype=np.float64).tobytes()).hexdigest()
+
+
+def run_cases():
+    """Full runs: (name, params, initial-state builder, steps, stride)."""
+    p_c1 = c1_params()
+    p_c2 = c2_params(32)
+    p_ot = Params(ndims=2, equation_system="mhd", reconstruction="tvd", limiter="mc",
+                  riemann_solver="hlld", time_advance="rk3", gamma=5.0 / 3.0, cfl=0.4,
+                  ncells=16, nghosts=2, xblocks=4, yblocks=4)
+    p_kh = Params(ndims=2, equation_system="mhd", reconstruction="tvd", limiter="mc",
+                  riemann_solver="hlld", time_advance="rk3", gamma=5.0 / 3.0, cfl=0.4,
+                  ncells=16, nghosts=2, xblocks=4, yblocks=4, xmin=-0.5, xmax=0.5,
+                  ymin=-0.5, ymax=0.5)
+    p_hd3 = Params(ndims=3, equation_system="hd", reconstruction="weno5z", limiter="mm",
+                   riemann_solver="hllc", time_advance="rk2", gamma=1.4, cfl=0.4,
+                   ncells=8, nghosts=4, xblocks=3, yblocks=2, zblocks=2, xmin=-0.5,
+                   xmax=0.5, ymin=-0.5, ymax=0.5, zmin=-0.5, zmax=0.5)
+    return [
+        ("c1_blast_hd2d_128_200", p_c1, problems.blast_hd_2d, 200, 4),
+        ("c2_mhd3d_weno5z_hlld_32_4", p_c2, problems.turbulent_mhd_3d, 4, 4),
+        ("orszag_tang_mhd2d_64_30", p_ot, problems.orszag_tang_2d, 30, 4),
+        ("kh_mhd2d_64_30", p_kh, problems.kelvin_helmholtz_2d, 30, 4),
+        ("blast_hd3d_weno5z_hllc_4", p_hd3, None, 4, 2),
+    ]
+
+
+def hd3_blast(p):
+    nz, ny, nx = p.shape
+    z, y, x = problems.cell_centres(p)
+    r = np.sqrt(x[None, None, :] ** 2 + y[None, :, None] ** 2 + z[:, None, None] ** 2)
+    q = np.zeros((5, nz, ny, nx))
+    q[0] = 1.0 + 0.01 * (2 * problems.uniform(7, nz * ny * nx).reshape(nz, ny, nx) - 1)
+    q[4] = np.where(r < 0.2, 10.0, 0.1)
+    return problems.prim_to_cons(p, q)
+
+
+def main():
+    if not (ol.ref_available(2) and ol.ref_available(3)):
+        sys.exit("reference binaries missing: run oracle/build_ref.sh first")
+    index = {}
+    for name, p, builder, nsteps, stride in run_cases():
+        u0 = builder(p) if builder else hd3_blast(p)
+        u, tr = ol.ref_run(p, u0, nsteps)
+        sub = u[:, ::stride, ::stride, ::stride] if p.ndims == 3 else u[:, :, ::stride, ::stride]
+        np.savez_compressed(os.path.join(HERE, name + ".npz"),
+                            params=np.array(p.to_params_in()), ndims=p.ndims,
+                            nsteps=nsteps, stride=stride, trace=tr, sub=sub,
+                            sha_u0=sha(u0), sha_u=sha(u))
+        index[name] = {"steps": nsteps, "sha_u": sha(u), "dt_last": float(tr[-1, 0])}
+        print(name, p.shape, "dt", tr[-1, 0])
+
+    rng = np.random.default_rng(20261202)
+    # WENO5-Z pencils (reconstruct_weno5z), including steps, extrema and
+    # smooth data
+    pw = c2_params(32)
+    fp = np.concatenate([rng.normal(size=(8, 24)),
+                         np.cumsum(rng.normal(size=(4, 24)), axis=1),
+                         np.sin(np.linspace(0, 6, 24))[None, :] * np.ones((2, 1)),
+                         np.where(np.arange(24) < 12, 1.0, 0.1)[None, :] * np.ones((2, 1))])
+    fl, fr = ol.ref_reconstruct(pw, fp)
+    np.savez_compressed(os.path.join(HERE, "unit_weno5z.npz"), fp=fp, fl=fl, fr=fr)
+
+    # equation of state vectors, HD and MHD
+    for eq, nv in (("hd", 5), ("mhd", 9)):
+        pe = Params(ndims=3, equation_system=eq, gamma=1.4 if eq == "hd" else 5.0 / 3.0)
+        q = np.zeros((64, nv))
+        q[:, 0] = rng.uniform(0.1, 3.0, 64)
+        q[:, 1:4] = rng.normal(size=(64, 3))
+        if nv == 9:
+            q[:, 4:8] = rng.normal(size=(64, 4))
+            q[:5, 4] = 0.0
+        q[:, -1] = rng.uniform(0.05, 3.0, 64)
+        u, q2, f, cm, cp = ol.ref_eqs(pe, q, cmax=1.7)
+        np.savez_compressed(os.path.join(HERE, f"unit_eqs_{eq}.npz"), q=q, u=u, q2=q2, f=f,
+                            cm=cm, cp=cp, cmax=1.7)
+
+    # Riemann solvers through update_flux with the zero TVD limiter (piecewise
+    # constant states, so every face solves the Riemann problem of its two
+    # cell values): 2D blocks with random states, including weak Bx, sm ~ 0
+    # and strongly supersonic pairs
+    for eq, rs in (("hd", "hll"), ("hd", "hllc"), ("mhd", "hll"), ("mhd", "hlld")):
+        pf = Params(ndims=2, equation_system=eq, reconstruction="tvd", limiter="zero",
+                    riemann_solver=rs, gamma=1.4 if eq == "hd" else 5.0 / 3.0,
+                    ncells=8, nghosts=2)
+        nv = pf.nv
+        nb = 4
+        qb = np.zeros((nb, 1, 12, 12, nv))
+        qb[..., 0] = rng.uniform(0.2, 2.0, qb.shape[:-1])
+        qb[..., 1:4] = rng.normal(scale=1.5, size=qb.shape[:-1] + (3,))
+        qb[..., -1] = rng.uniform(0.1, 2.0, qb.shape[:-1])
+        if nv == 9:
+            qb[..., 4:7] = rng.normal(size=qb.shape[:-1] + (3,))
+            qb[..., 7] = rng.normal(scale=0.1, size=qb.shape[:-1])
+            qb[1, ..., 4] = 1e-6 * rng.normal(size=qb.shape[1:-1])   # weak Bx
+            qb[2, ..., 4] = 0.0                                       # Bx = 0
+            qb[3, ..., 1] = qb[3, ..., 1] * 5.0                       # fast flows
+        f = ol.ref_update_flux(pf, qb, cmax=2.3)
+        np.savez_compressed(os.path.join(HERE, f"unit_flux_{eq}_{rs}.npz"), q=qb, f=f,
+                            cmax=2.3, params=np.array(pf.to_params_in()))
+    with open(os.path.join(HERE, "index.json"), "w") as fh:
+        json.dump(index, fh, indent=1)
+
+
+if __name__ == "__main__":
+    main()

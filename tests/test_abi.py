@@ -1,0 +1,106 @@
+"""The C-ABI boundary without a GPU: the library loads, exports every entry
+point declared in include/amun_dev.h, and rejects bad configurations with
+the reference's error codes before touching the device."""
+import ctypes
+import os
+import re
+import subprocess
+import sys
+
+import pytest
+
+from amun_amd.config import Params, AmunCfg, ConfigError, c1_params, c2_params
+from amun_amd import solver
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_symbols():
+    txt = open(os.path.join(ROOT, "include", "amun_dev.h")).read()
+    return sorted(set(re.findall(r"\b(amun_dev_\w+)\s*\(", txt)))
+
+
+def test_library_exports_every_header_symbol():
+    lib = solver.load_library()
+    syms = header_symbols()
+    assert len(syms) >= 15
+    for s in syms:
+        assert hasattr(lib, s), s
+    assert set(solver.EXPORTS) <= set(syms)
+
+
+def test_default_cfg_matches_reference_defaults():
+    lib = solver.load_library()
+    c = AmunCfg()
+    lib.amun_dev_default_cfg(ctypes.byref(c))
+    assert c.gamma == pytest.approx(5.0 / 3.0)       # equations.F90:167
+    assert c.eps == 2.220446049250313e-16            # interpolations.F90:82
+    assert c.alpha == 2.0                            # evolution.F90:68
+    assert c.limiter == 1                            # "mm" default
+    assert c.ncells == 8 and c.nghosts == 2          # coordinates.F90:42-45
+
+
+def _init_code(p):
+    lib = solver.load_library()
+    cfg = p.to_cfg()
+    h = ctypes.c_void_p()
+    rc = lib.amun_dev_init(ctypes.byref(cfg), ctypes.byref(h))
+    if rc == 0:
+        lib.amun_dev_finalize(h)
+    return rc
+
+
+@pytest.mark.parametrize("field,value,code", [
+    ("eqsys", 7, 100), ("eos", 3, 110), ("reconstruction", 9, 200),
+    ("riemann_solver", 3, 300), ("time_advance", 9, 400), ("nghosts", 1, 600),
+    ("ncells", 2, 500),
+])
+def test_config_errors(field, value, code):
+    p = c1_params()
+    cfg = p.to_cfg()
+    setattr(cfg, field, value)
+    lib = solver.load_library()
+    h = ctypes.c_void_p()
+    assert lib.amun_dev_init(ctypes.byref(cfg), ctypes.byref(h)) == code
+    assert lib.amun_dev_last_error()
+
+
+def test_weno_needs_four_ghosts():
+    p = c2_params(32)
+    p.nghosts = 2
+    assert _init_code(p) == 600
+
+
+def test_valid_config_reaches_the_device_query():
+    rc = _init_code(c1_params())
+    assert rc == 0 or rc >= 1000
+
+
+def test_params_names_and_errors():
+    text = """
+    # AMUN parameter file
+    equation_system = "mhd"
+    reconstruction  = weno5z
+    limiter         = mc
+    riemann_solver  = hlld
+    time_advance    = rk3
+    cfl             = 0.3d0
+    ncells = 16
+    nghosts = 4
+    limiter = mm
+    problem = blast
+    """
+    p = Params.from_text(text)
+    assert p.limiter == "mm" and p.cfl == 0.3 and p.extra["problem"] == "blast"
+    assert p.to_cfg().riemann_solver == 3
+    p.riemann_solver = "roe2"
+    with pytest.raises(ConfigError) as e:
+        p.to_cfg()
+    assert e.value.code == 300
+
+
+def test_bench_refuses_several_gpus():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "4"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
