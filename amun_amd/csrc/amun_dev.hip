@@ -90,7 +90,23 @@ __device__ __forceinline__ double dt_of(const RunConst& rc, const DevState* st,
   return dtnext > 0.0 ? fmin(dtn, dtnext) : st->dt;
 }
 
+// Position of cell (i,j,k) inside a block's variable plane.  Cells are stored
+// in 4x4 x-y bricks: one 128-byte line holds x = 4i'..4i'+3 of four
+// consecutive y, so the x, y and z sweeps (lanes across the two transverse
+// directions, walking the third) each touch few lines per load and reuse
+// them within four steps.  Requires nc % 4 == 0.
+// FP32 arrays (element size 4) of 3D runs use 4x4x2 bricks instead, so that
+// a line (32 floats) again spans four steps of every sweep direction.
+template <int ES>
+__host__ __device__ __forceinline__ int lidx(int nc, int nd, int i, int j, int k) {
+  if (nd == 2) return (((i >> 2) * nc + j) << 2) | (i & 3);
+  if (ES == 8) return ((((i >> 2) * nc + k) * nc + j) << 2) | (i & 3);
+  return (((((i >> 2) * (nc >> 1) + (k >> 1)) * nc + j) * 2 + (k & 1)) << 2) | (i & 3);
+}
+
 // position of (i,j,k), possibly outside the block by at most nc, in the pool
+// (array of element size ES)
+template <int ES>
 __device__ __forceinline__ size_t cell_index(const int* __restrict__ nbr,
                                              const RunConst& rc, int b, int i,
                                              int j, int k) {
@@ -105,7 +121,7 @@ __device__ __forceinline__ size_t cell_index(const int* __restrict__ nbr,
     k -= dk * nc;
   }
   // packed: block in the high word, cell in the low word
-  return ((size_t)b << 32) | (size_t)((k * nc + j) * nc + i);
+  return ((size_t)b << 32) | (size_t)lidx<ES>(nc, rc.nd, i, j, k);
 }
 
 template <int NV>
@@ -155,12 +171,13 @@ __device__ __forceinline__ void riemann(const EosConst<T>& e, T cmax, T cmax2, T
 }
 
 struct SweepArgs {
-  const double* __restrict__ Q;    // primitive state of the stage input
+  const void* Q;                   // primitive state of the stage input (T)
   const double* U0;                // state at the start of the step
   const double* U1;                // intermediate state (stage input)
   double* Uout;                    // stage output (may alias U0 or U1)
-  double* __restrict__ Qout;       // primitive state of Uout
-  double* __restrict__ D;          // increment accumulator
+  void* Qout;                      // primitive state of Uout (T)
+  void* D;                         // increment accumulator (T)
+  float* C;                        // Kahan compensation of U0 (mixed mode)
   const int* __restrict__ nbr;
   DevState* st;
   int slot;                        // cmax slot of this step
@@ -204,7 +221,7 @@ struct Recon<P, REC_WENO5Z, ND, DIR, T> {
 #pragma unroll
       for (int m = 0; m < 4; m++) w[v][m] = w[v][m + 1];
   }
-  __device__ __forceinline__ void cell(const RunConst& rc, T* flc, T* frc, const double*,
+  __device__ __forceinline__ void cell(const RunConst& rc, T* flc, T* frc, const T*,
                                        const int*, int, int, int, int, DevState*) {
     const T eps = T(rc.eps);
 #pragma unroll
@@ -232,7 +249,7 @@ struct Recon<P, REC_TVD, ND, DIR, T> {
     }
   }
   __device__ __forceinline__ void cell(const RunConst& rc, T* flc, T* frc,
-                                       const double* __restrict__ Q,
+                                       const T* __restrict__ Q,
                                        const int* __restrict__ nbr, int b, int i, int j,
                                        int k, DevState* st) {
 #pragma unroll
@@ -247,8 +264,8 @@ struct Recon<P, REC_TVD, ND, DIR, T> {
           if (d == DIR) continue;
           int im = i - (d == 0), jm = j - (d == 1), km = k - (d == 2);
           int ip = i + (d == 0), jp = j + (d == 1), kp = k + (d == 2);
-          T qm = T(Q[var_index<P::nv>(rc, cell_index(nbr, rc, b, im, jm, km), v)]);
-          T qp = T(Q[var_index<P::nv>(rc, cell_index(nbr, rc, b, ip, jp, kp), v)]);
+          T qm = Q[var_index<P::nv>(rc, cell_index<sizeof(T)>(nbr, rc, b, im, jm, km), v)];
+          T qp = Q[var_index<P::nv>(rc, cell_index<sizeof(T)>(nbr, rc, b, ip, jp, kp), v)];
           dqt[d] = limiter<T>(rc.limiter, T(0.5), qc - qm, qp - qc);
         }
         if (qc > T(0)) {
@@ -297,6 +314,9 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a, RunConst rc) {
   const double ds = a.fac * dt;
   const EosConst<T> e{T(rc.gamma), T(rc.gammam1), T(rc.gammam1i)};
   const EosConst<double> ed{rc.gamma, rc.gammam1, rc.gammam1i};
+  const T* __restrict__ Qp = static_cast<const T*>(a.Q);
+  T* __restrict__ Dp = static_cast<T*>(a.D);
+  T* __restrict__ Qo = static_cast<T*>(a.Qout);
   double smax = 0.0;
 
   if (active) {
@@ -310,9 +330,9 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a, RunConst rc) {
     auto loadq = [&](int c, T* q) {
       int i, j, k;
       coords(c, i, j, k);
-      size_t pk = cell_index(a.nbr, rc, b, i, j, k);
+      size_t pk = cell_index<sizeof(T)>(a.nbr, rc, b, i, j, k);
 #pragma unroll
-      for (int v = 0; v < NV; v++) q[v] = T(a.Q[var_index<NV>(rc, pk, v)]);
+      for (int v = 0; v < NV; v++) q[v] = Qp[var_index<NV>(rc, pk, v)];
     };
 
     RC rec;
@@ -325,13 +345,37 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a, RunConst rc) {
     }
     T flp[NV], fprev[NV];
     for (int c = -1; c <= nc; c++) {
+      // operands of the cell finished in this iteration (c-1), loaded before
+      // the reconstruction and Riemann solve so that their latency overlaps
+      // the arithmetic
+      T dpre[NV];
+      double u0pre[NV], u1pre[NV];
+      float cpre[NV];
+      if (c >= 1) {
+        int i, j, k;
+        coords(c - 1, i, j, k);
+        const size_t bb = (size_t)b * NV * (size_t)rc.ncell;
+        const size_t bt = bb + (size_t)lidx<sizeof(T)>(nc, ND, i, j, k);
+        const size_t bd = bb + (size_t)lidx<8>(nc, ND, i, j, k);
+        const size_t bf = bb + (size_t)lidx<4>(nc, ND, i, j, k);
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+          const size_t off = (size_t)v * rc.ncell;
+          if constexpr (DIR > 0) dpre[v] = Dp[bt + off];
+          if constexpr (kLast) {
+            u0pre[v] = a.U0[bd + off];
+            if (a.kind != 0) u1pre[v] = a.U1[bd + off];
+            if constexpr (sizeof(T) == 4) cpre[v] = a.C[bf + off];
+          }
+        }
+      }
       if (c > -1) rec.shift();
       loadq(c + RC::R, qtmp);
       rec.load(2 * RC::R, qtmp);
       T flc[NV], frc[NV];
       int ci, cj, ck;
       coords(c, ci, cj, ck);
-      rec.cell(rc, flc, frc, a.Q, a.nbr, b, ci, cj, ck, a.st);
+      rec.cell(rc, flc, frc, Qp, a.nbr, b, ci, cj, ck, a.st);
       if (c >= 0) {
         // face c-1/2: left state from cell c-1, right state from cell c
         T ql[NV], qr[NV], fr[NV], f[NV];
@@ -347,21 +391,48 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a, RunConst rc) {
           // cell c-1 has both faces
           int i, j, k;
           coords(c - 1, i, j, k);
-          const size_t cidx = ((size_t)k * nc + j) * nc + i;
-          const double dxi = rc.dxi[DIR];
+          const size_t bb = (size_t)b * NV * (size_t)rc.ncell;
+          const size_t bt = bb + (size_t)lidx<sizeof(T)>(nc, ND, i, j, k);
+          const size_t bd = bb + (size_t)lidx<8>(nc, ND, i, j, k);
+          const size_t bf = bb + (size_t)lidx<4>(nc, ND, i, j, k);
+          const T dxi = T(rc.dxi[DIR]);
           double unew[NV];
 #pragma unroll
           for (int v = 0; v < NV; v++) {
-            const size_t gi = ((size_t)b * NV + v) * (size_t)rc.ncell + cidx;
-            const double dfx = double(f[v]) - double(fprev[v]);
-            double du;
+            const size_t off = (size_t)v * rc.ncell;
+            // update_increment order: -dxi*dfx - dyi*dfy - dzi*dfz
+            const T dfx = f[v] - fprev[v];
+            T du;
             if constexpr (DIR == 0) du = -dxi * dfx;
-            else du = a.D[gi] - dxi * dfx;
+            else du = dpre[v] - dxi * dfx;
             if constexpr (!kLast) {
-              a.D[gi] = du;
-            } else {
-              double un = combine(a.kind, a.U0[gi], a.U1[gi], ds, du);
+              Dp[bt + off] = du;
+            } else if constexpr (sizeof(T) == 8) {
+              double un = combine(a.kind, u0pre[v], u1pre[v], ds, du);
               if (a.last && v == P::ibp) un = rc.decay * un;
+              unew[v] = un;
+            } else {
+              // mixed precision: the FP32 increment is added to the FP64
+              // state in increment form with Kahan compensation.  The true
+              // step state is U0 - C; every stage is U0_true + delta, and the
+              // last stage keeps the rounding error of that sum in C.
+              const double u0 = u0pre[v];
+              const double cp = double(cpre[v]);
+              const double dd = ds * double(du);
+              double delta;
+              switch (a.kind) {
+              case 0: delta = dd; break;
+              case 1: delta = 0.25 * ((u1pre[v] - u0) + cp) + dd; break;
+              case 2: delta = (2.0 / 3.0) * ((u1pre[v] - u0) + cp) + dd; break;
+              default: delta = 0.5 * (((u1pre[v] - u0) + cp) + dd); break;
+              }
+              const double y = delta - cp;
+              double un = u0 + y;
+              if (a.last) {
+                double cn = (un - u0) - y;
+                if (v == P::ibp) { un = rc.decay * un; cn = rc.decay * cn; }
+                a.C[bf + off] = float(cn);
+              }
               unew[v] = un;
             }
           }
@@ -370,9 +441,9 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a, RunConst rc) {
             cons2prim<P, double>(ed, unew, qn);
 #pragma unroll
             for (int v = 0; v < NV; v++) {
-              const size_t gi = ((size_t)b * NV + v) * (size_t)rc.ncell + cidx;
-              a.Uout[gi] = unew[v];
-              a.Qout[gi] = qn[v];
+              const size_t off = (size_t)v * rc.ncell;
+              a.Uout[bd + off] = unew[v];
+              Qo[bt + off] = T(qn[v]);
             }
             if (a.last) smax = fmax(smax, maxspeed<P, double>(ed, qn));
           }
@@ -399,28 +470,37 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a, RunConst rc) {
 
 // Q = cons2prim(U) on every interior cell, and the maximum speed into a slot
 // (update_primitive_variables + maxspeed); used after uploads
-template <class P>
+template <class P, typename T>
 __global__ __launch_bounds__(256) void k_prim(const double* __restrict__ U,
-                                              double* __restrict__ Q, RunConst rc,
+                                              void* Qv, RunConst rc,
                                               DevState* st, int slot) {
+  T* __restrict__ Q = static_cast<T*>(Qv);
   constexpr int NV = P::nv;
   const long long n = (long long)rc.nblk * rc.ncell;
   const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const EosConst<double> e{rc.gamma, rc.gammam1, rc.gammam1i};
   double smax = 0.0;
   if (gid < n) {
-    long long b = gid / rc.ncell, c = gid % rc.ncell;
+    const long long b = gid / rc.ncell, c = gid % rc.ncell;
+    const int i = (int)(c % rc.nc), j = (int)((c / rc.nc) % rc.nc), k = (int)(c / ((long long)rc.nc * rc.nc));
+    const long long cu = lidx<8>(rc.nc, rc.nd, i, j, k), cq = lidx<sizeof(T)>(rc.nc, rc.nd, i, j, k);
     double u[NV], q[NV];
 #pragma unroll
-    for (int v = 0; v < NV; v++) u[v] = U[(b * NV + v) * rc.ncell + c];
+    for (int v = 0; v < NV; v++) u[v] = U[(b * NV + v) * rc.ncell + cu];
     cons2prim<P, double>(e, u, q);
 #pragma unroll
-    for (int v = 0; v < NV; v++) Q[(b * NV + v) * rc.ncell + c] = q[v];
+    for (int v = 0; v < NV; v++) Q[(b * NV + v) * rc.ncell + cq] = T(q[v]);
     smax = maxspeed<P, double>(e, q);
   }
   for (int off = 32; off > 0; off >>= 1) smax = fmax(smax, __shfl_xor(smax, off));
   if ((threadIdx.x & 63) == 0 && smax > 0.0)
     atomicMax(&st->cmax_bits[slot], (unsigned long long)__double_as_longlong(smax));
+}
+
+__global__ void k_zero_f32(float* x, long long n) {
+  for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < n;
+       g += (long long)gridDim.x * blockDim.x)
+    x[g] = 0.0f;
 }
 
 __global__ void k_reset_slot(DevState* st, int slot) {
@@ -487,9 +567,10 @@ __global__ __launch_bounds__(256) void k_integrals(const double* __restrict__ U,
 
 // pack interior + ghost layers of every block into the reference host layout
 // u(nv,im,jm,km) (boundary_variables: ghost cells are neighbour copies)
-template <int NV>
-__global__ void k_pack_blocks(const double* __restrict__ X, double* __restrict__ out,
+template <int NV, typename TX>
+__global__ void k_pack_blocks(const void* Xv, double* __restrict__ out,
                               const int* __restrict__ nbr, RunConst rc, int ng) {
+  const TX* __restrict__ X = static_cast<const TX*>(Xv);
   const int nc = rc.nc;
   const int im = nc + 2 * ng, km = (rc.nd == 3) ? im : 1;
   const long long per = (long long)im * im * km;
@@ -500,9 +581,9 @@ __global__ void k_pack_blocks(const double* __restrict__ X, double* __restrict__
     long long r = g % per;
     int i = (int)(r % im), j = (int)((r / im) % im), k = (int)(r / ((long long)im * im));
     int li = i - ng, lj = j - ng, lk = (rc.nd == 3) ? k - ng : 0;
-    size_t pk = cell_index(nbr, rc, b, li, lj, lk);
+    size_t pk = cell_index<sizeof(TX)>(nbr, rc, b, li, lj, lk);
 #pragma unroll
-    for (int v = 0; v < NV; v++) out[(size_t)g * NV + v] = X[var_index<NV>(rc, pk, v)];
+    for (int v = 0; v < NV; v++) out[(size_t)g * NV + v] = double(X[var_index<NV>(rc, pk, v)]);
   }
 }
 
@@ -513,9 +594,10 @@ typedef void (*SweepFn)(SweepArgs, RunConst);
 // kernel instantiations selected for one configuration
 struct Dispatch {
   SweepFn sweep[3] = {nullptr, nullptr, nullptr};
-  void (*prim)(const double*, double*, RunConst, DevState*, int) = nullptr;
+  void (*prim)(const double*, void*, RunConst, DevState*, int) = nullptr;
   void (*integ)(const double*, RunConst, double*) = nullptr;
-  void (*pack)(const double*, double*, const int*, RunConst, int) = nullptr;
+  void (*pack_u)(const void*, double*, const int*, RunConst, int) = nullptr;
+  void (*pack_q)(const void*, double*, const int*, RunConst, int) = nullptr;
 };
 
 // ===========================================================================
@@ -526,7 +608,10 @@ struct amun_ctx {
   RunConst rc;
   int nv;
   size_t nstate;            // doubles per state array
-  double *U0 = nullptr, *U1 = nullptr, *Q[2] = {nullptr, nullptr}, *D = nullptr;
+  double *U0 = nullptr, *U1 = nullptr;
+  void *Q[2] = {nullptr, nullptr}, *D = nullptr;   // element size qsize
+  float* C = nullptr;       // Kahan compensation of U0 (mixed mode)
+  size_t qsize = 8;
   int* nbr = nullptr;
   DevState* st = nullptr;
   double* scratch = nullptr;   // reductions / small transfers
@@ -540,34 +625,40 @@ struct amun_ctx {
 namespace {
 
 
-template <class P, int REC, int RS, int ND>
+template <class P, int REC, int RS, int ND, typename T>
 void fill(Dispatch& d) {
-  d.sweep[0] = k_sweep<P, REC, RS, ND, 0, double>;
-  d.sweep[1] = k_sweep<P, REC, RS, ND, 1, double>;
-  if constexpr (ND == 3) d.sweep[2] = k_sweep<P, REC, RS, ND, 2, double>;
-  d.prim = k_prim<P>;
+  d.sweep[0] = k_sweep<P, REC, RS, ND, 0, T>;
+  d.sweep[1] = k_sweep<P, REC, RS, ND, 1, T>;
+  if constexpr (ND == 3) d.sweep[2] = k_sweep<P, REC, RS, ND, 2, T>;
+  d.prim = k_prim<P, T>;
   d.integ = k_integrals<P>;
-  d.pack = k_pack_blocks<P::nv>;
+  d.pack_u = k_pack_blocks<P::nv, double>;
+  d.pack_q = k_pack_blocks<P::nv, T>;
 }
 
-template <class P, int RS>
+template <class P, int RS, typename T>
 bool pick_nd_rec(Dispatch& d, int nd, int rec) {
-  if (nd == 2 && rec == REC_TVD) { fill<P, REC_TVD, RS, 2>(d); return true; }
-  if (nd == 3 && rec == REC_TVD) { fill<P, REC_TVD, RS, 3>(d); return true; }
-  if (nd == 2 && rec == REC_WENO5Z) { fill<P, REC_WENO5Z, RS, 2>(d); return true; }
-  if (nd == 3 && rec == REC_WENO5Z) { fill<P, REC_WENO5Z, RS, 3>(d); return true; }
+  if (nd == 2 && rec == REC_TVD) { fill<P, REC_TVD, RS, 2, T>(d); return true; }
+  if (nd == 3 && rec == REC_TVD) { fill<P, REC_TVD, RS, 3, T>(d); return true; }
+  if (nd == 2 && rec == REC_WENO5Z) { fill<P, REC_WENO5Z, RS, 2, T>(d); return true; }
+  if (nd == 3 && rec == REC_WENO5Z) { fill<P, REC_WENO5Z, RS, 3, T>(d); return true; }
+  return false;
+}
+
+template <typename T>
+bool pick_t(Dispatch& d, const amun_cfg& c) {
+  if (c.eqsys == AMUN_EQSYS_HD) {
+    if (c.riemann_solver == AMUN_RS_HLL) return pick_nd_rec<HD, RS_HLL, T>(d, c.ndims, c.reconstruction);
+    if (c.riemann_solver == AMUN_RS_HLLC) return pick_nd_rec<HD, RS_HLLC, T>(d, c.ndims, c.reconstruction);
+  } else if (c.eqsys == AMUN_EQSYS_MHD) {
+    if (c.riemann_solver == AMUN_RS_HLL) return pick_nd_rec<MHD, RS_HLL, T>(d, c.ndims, c.reconstruction);
+    if (c.riemann_solver == AMUN_RS_HLLD) return pick_nd_rec<MHD, RS_HLLD, T>(d, c.ndims, c.reconstruction);
+  }
   return false;
 }
 
 bool pick(Dispatch& d, const amun_cfg& c) {
-  if (c.eqsys == AMUN_EQSYS_HD) {
-    if (c.riemann_solver == AMUN_RS_HLL) return pick_nd_rec<HD, RS_HLL>(d, c.ndims, c.reconstruction);
-    if (c.riemann_solver == AMUN_RS_HLLC) return pick_nd_rec<HD, RS_HLLC>(d, c.ndims, c.reconstruction);
-  } else if (c.eqsys == AMUN_EQSYS_MHD) {
-    if (c.riemann_solver == AMUN_RS_HLL) return pick_nd_rec<MHD, RS_HLL>(d, c.ndims, c.reconstruction);
-    if (c.riemann_solver == AMUN_RS_HLLD) return pick_nd_rec<MHD, RS_HLLD>(d, c.ndims, c.reconstruction);
-  }
-  return false;
+  return c.precision == AMUN_PREC_MIXED ? pick_t<float>(d, c) : pick_t<double>(d, c);
 }
 
 int validate(const amun_cfg& c, int* nv) {
@@ -585,10 +676,12 @@ int validate(const amun_cfg& c, int* nv) {
   if (!rs_ok) return fail(300, "riemann_solver does not match the equation system");
   if (c.time_advance < AMUN_TA_EULER || c.time_advance > AMUN_TA_RK3)
     return fail(400, "unsupported time_advance");
-  if (c.precision != AMUN_PREC_FP64) return fail(600, "mixed precision is not built yet");
+  if (c.precision != AMUN_PREC_FP64 && c.precision != AMUN_PREC_MIXED)
+    return fail(500, "unsupported precision");
   if (c.fix_positivity || c.clip_extrema)
     return fail(600, "fix_positivity / clip_extrema are not supported on the device path");
-  if (c.ncells < 4 || c.ncells > 64) return fail(500, "ncells must be in [4, 64]");
+  if (c.ncells < 4 || c.ncells > 64 || c.ncells % 4)
+    return fail(500, "ncells must be a multiple of 4 in [4, 64] (4x4 brick layout)");
   int need = (c.reconstruction == AMUN_REC_WENO5Z) ? 4 : 2;
   if (c.nghosts < need)
     return fail(600, "nghosts too small for the reconstruction stencil (WENO5-Z needs 4, TVD 2)");
@@ -687,6 +780,8 @@ int amun_dev_init(const amun_cfg* cfg, amun_ctx** out) {
 
   c->nstate = (size_t)rc.nblk * nv * rc.ncell;
   const size_t bytes = c->nstate * sizeof(double);
+  c->qsize = (cfg->precision == AMUN_PREC_MIXED) ? sizeof(float) : sizeof(double);
+  const size_t qbytes = c->nstate * c->qsize;
   hipError_t e;
 #define ALLOC(ptr, sz)                                                       \
   e = hipMalloc((void**)&(ptr), (sz));                                       \
@@ -696,9 +791,12 @@ int amun_dev_init(const amun_cfg* cfg, amun_ctx** out) {
   }
   ALLOC(c->U0, bytes);
   ALLOC(c->U1, bytes);
-  ALLOC(c->Q[0], bytes);
-  ALLOC(c->Q[1], bytes);
-  ALLOC(c->D, bytes);
+  ALLOC(c->Q[0], qbytes);
+  ALLOC(c->Q[1], qbytes);
+  ALLOC(c->D, qbytes);
+  if (cfg->precision == AMUN_PREC_MIXED) {
+    ALLOC(c->C, c->nstate * sizeof(float));
+  }
   ALLOC(c->nbr, nbr.size() * sizeof(int32_t));
   ALLOC(c->st, sizeof(DevState));
   ALLOC(c->scratch, 4096 * 8 * sizeof(double));
@@ -739,6 +837,7 @@ void amun_dev_finalize(amun_ctx* c) {
   hipFree(c->Q[0]);
   hipFree(c->Q[1]);
   hipFree(c->D);
+  hipFree(c->C);
   hipFree(c->nbr);
   hipFree(c->st);
   hipFree(c->scratch);
@@ -754,6 +853,9 @@ static int after_upload(amun_ctx* c) {
   long long n = (long long)c->rc.nblk * c->rc.ncell;
   hipLaunchKernelGGL(d.prim, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
                      c->U0, c->Q[0], c->rc, c->st, slot);
+  if (c->C)
+    hipLaunchKernelGGL(k_zero_f32, dim3(2048), dim3(256), 0, c->stream, c->C,
+                       (long long)c->nstate);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(c->stream));
   return 0;
@@ -774,8 +876,8 @@ int amun_dev_upload_global(amun_ctx* c, const double* u) {
       for (int k = 0; k < kn; k++)
         for (int j = 0; j < nc; j++) {
           const double* src = u + ((v * nz + bz * kn + k) * ny + by * nc + j) * nx + bx * nc;
-          double* dst = &h[(((size_t)b * nv + v) * rc.ncell) + ((size_t)k * nc + j) * nc];
-          std::memcpy(dst, src, nc * sizeof(double));
+          double* dst = &h[((size_t)b * nv + v) * rc.ncell];
+          for (int i = 0; i < nc; i++) dst[lidx<8>(nc, rc.nd, i, j, k)] = src[i];
         }
   }
   HIP_TRY(hipMemcpy(c->U0, h.data(), c->nstate * sizeof(double), hipMemcpyHostToDevice));
@@ -795,7 +897,7 @@ int amun_dev_upload_blocks(amun_ctx* c, const double* u) {
       for (int j = 0; j < nc; j++)
         for (int i = 0; i < nc; i++)
           for (int v = 0; v < nv; v++)
-            h[(((size_t)b * nv + v) * rc.ncell) + ((size_t)k * nc + j) * nc + i] =
+            h[(((size_t)b * nv + v) * rc.ncell) + lidx<8>(nc, rc.nd, i, j, k)] =
                 u[b * per + ((((size_t)(k + kg) * im + (j + ng)) * im + (i + ng)) * nv) + v];
   HIP_TRY(hipMemcpy(c->U0, h.data(), c->nstate * sizeof(double), hipMemcpyHostToDevice));
   return after_upload(c);
@@ -814,16 +916,27 @@ int amun_dev_download_global(amun_ctx* c, double* u, double* q) {
   for (int pass = 0; pass < 2; pass++) {
     double* dst_arr = pass == 0 ? u : q;
     if (!dst_arr) continue;
-    HIP_TRY(hipMemcpy(h.data(), pass == 0 ? c->U0 : c->Q[c->qcur],
-                      c->nstate * sizeof(double), hipMemcpyDeviceToHost));
+    if (pass == 0 || c->qsize == sizeof(double)) {
+      HIP_TRY(hipMemcpy(h.data(), pass == 0 ? (void*)c->U0 : c->Q[c->qcur],
+                        c->nstate * sizeof(double), hipMemcpyDeviceToHost));
+    } else {
+      std::vector<float> hf(c->nstate);
+      HIP_TRY(hipMemcpy(hf.data(), c->Q[c->qcur], c->nstate * sizeof(float),
+                        hipMemcpyDeviceToHost));
+      for (size_t n = 0; n < c->nstate; n++) h[n] = hf[n];
+    }
     for (int b = 0; b < rc.nblk; b++) {
       size_t bx = c->coords[b * 3], by = c->coords[b * 3 + 1], bz = c->coords[b * 3 + 2];
       for (int v = 0; v < nv; v++)
         for (int k = 0; k < kn; k++)
-          for (int j = 0; j < nc; j++)
-            std::memcpy(dst_arr + ((v * nz + bz * kn + k) * ny + by * nc + j) * nx + bx * nc,
-                        &h[(((size_t)b * nv + v) * rc.ncell) + ((size_t)k * nc + j) * nc],
-                        nc * sizeof(double));
+          for (int j = 0; j < nc; j++) {
+            double* dst = dst_arr + ((v * nz + bz * kn + k) * ny + by * nc + j) * nx + bx * nc;
+            const double* src = &h[((size_t)b * nv + v) * rc.ncell];
+            if (pass == 0 || c->qsize == sizeof(double))
+              for (int i = 0; i < nc; i++) dst[i] = src[lidx<8>(nc, rc.nd, i, j, k)];
+            else
+              for (int i = 0; i < nc; i++) dst[i] = src[lidx<4>(nc, rc.nd, i, j, k)];
+          }
     }
   }
   return 0;
@@ -840,8 +953,12 @@ int amun_dev_download_blocks(amun_ctx* c, double* u, double* q) {
   for (int pass = 0; pass < 2; pass++) {
     double* dst = pass == 0 ? u : q;
     if (!dst) continue;
-    hipLaunchKernelGGL(d.pack, dim3(2048), dim3(256), 0, c->stream,
-                       pass == 0 ? c->U0 : c->Q[c->qcur], buf, c->nbr, c->rc, ng);
+    if (pass == 0)
+      hipLaunchKernelGGL(d.pack_u, dim3(2048), dim3(256), 0, c->stream, (const void*)c->U0,
+                         buf, c->nbr, c->rc, ng);
+    else
+      hipLaunchKernelGGL(d.pack_q, dim3(2048), dim3(256), 0, c->stream,
+                         (const void*)c->Q[c->qcur], buf, c->nbr, c->rc, ng);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e == hipSuccess) e = hipMemcpy(dst, buf, n * sizeof(double), hipMemcpyDeviceToHost);
@@ -884,6 +1001,7 @@ static int launch_step(amun_ctx* c, double dtnext, hipEvent_t* ev = nullptr) {
     a.Uout = stages[s].out;
     a.Qout = c->Q[c->qcur ^ 1];
     a.D = c->D;
+    a.C = c->C;
     a.nbr = c->nbr;
     a.st = c->st;
     a.slot = slot;

@@ -52,6 +52,25 @@ template <typename T> __device__ __forceinline__ T tsign(T x, T a) {
   return copysign(x, a);
 }
 
+// Reciprocal of the hot loops.  FP64: the hardware estimate v_rcp_f64
+// refined by two Newton-Raphson steps (e = 1 - d r, r += r e), which is
+// within an ulp of 1/d for the normal, finite operands met here; FP32: the
+// v_rcp_f32 estimate (1 ulp).  a * rcp(d) replaces a / d where the divisor is
+// shared, which is the only intended rounding difference against the
+// reference's divisions.
+template <typename T> __device__ __forceinline__ T rcp(T d);
+template <> __device__ __forceinline__ double rcp<double>(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  double e = fma(-d, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-d, r, 1.0);
+  return fma(r, e, r);
+}
+template <> __device__ __forceinline__ float rcp<float>(float d) {
+  return __builtin_amdgcn_rcpf(d);
+}
+template <typename T> __device__ __forceinline__ T dv(T a, T b) { return a * rcp(b); }
+
 // ---------------------------------------------------------------------------
 // TVD limiters, interpolations.F90:4684-4877 (x is the scaling factor 1/2)
 // ---------------------------------------------------------------------------
@@ -107,24 +126,35 @@ __device__ __forceinline__ void weno5z_cell(const T f[5], T eps, T& fl, T& fr) {
   t = dfp_0 + dfm_0;         const T bc = df2_0 + c2 * (t * t);
   t = T(3) * dfp_0 - dfp_p1; const T br = df2_p1 + c2 * (t * t);
   const T tt = tabs(br - bl);
-  const T al = T(1) + tt / (bl + eps);
-  const T ac = T(1) + tt / (bc + eps);
-  const T ar = T(1) + tt / (br + eps);
-  T wl, wc, wr, ww, ql, qc, qr;
-  wl = dl * al; wc = dc * ac; wr = dr * ar;
-  ww = (wl + wr) + wc;
-  wl = wl / ww; wr = wr / ww; wc = T(1) - (wl + wr);
-  ql = a11 * f[0] + a12 * f[1] + a13 * f[2];
-  qc = a21 * f[1] + a22 * f[2] + a23 * f[3];
-  qr = a31 * f[2] + a32 * f[3] + a33 * f[4];
-  fl = (wl * ql + wr * qr) + wc * qc;
-  wl = dl * ar; wc = dc * ac; wr = dr * al;
-  ww = (wl + wr) + wc;
-  wl = wl / ww; wr = wr / ww; wc = T(1) - (wl + wr);
-  ql = a11 * f[4] + a12 * f[3] + a13 * f[2];
-  qc = a21 * f[3] + a22 * f[2] + a23 * f[1];
-  qr = a31 * f[2] + a32 * f[1] + a33 * f[0];
-  fr = (wl * ql + wr * qr) + wc * qc;
+  // the three interpolations of each face
+  const T ql = a11 * f[0] + a12 * f[1] + a13 * f[2];
+  const T qc = a21 * f[1] + a22 * f[2] + a23 * f[3];
+  const T qr = a31 * f[2] + a32 * f[3] + a33 * f[4];
+  const T pl = a11 * f[4] + a12 * f[3] + a13 * f[2];
+  const T pc = a21 * f[3] + a22 * f[2] + a23 * f[1];
+  const T pr = a31 * f[2] + a32 * f[1] + a33 * f[0];
+  // alpha_k = 1 + tau/(beta_k + eps) (eq. 28 of Borges et al. 2008, p = 1);
+  // the normalised weights w_k = d_k alpha_k / sum(d alpha) are formed
+  // without the reference's three alpha divisions: in FP64 all alphas are
+  // scaled by (e_l e_c e_r), e_k = beta_k + eps, which leaves one division
+  // per face; in FP32 the alphas use the hardware reciprocal
+  T gl, gc, gr;
+  const T el = bl + eps, ec = bc + eps, er = br + eps;
+  if constexpr (sizeof(T) == 8) {
+    gl = (el + tt) * (ec * er);
+    gc = (ec + tt) * (el * er);
+    gr = (er + tt) * (el * ec);
+  } else {
+    gl = T(1) + tt * rcp(el);
+    gc = T(1) + tt * rcp(ec);
+    gr = T(1) + tt * rcp(er);
+  }
+  // left state of the upper face (weights dl, dc, dr on gl, gc, gr) and
+  // right state of the lower face (mirrored stencils, gl and gr swapped)
+  const T wl = dl * gl, wc = dc * gc, wr = dr * gr;
+  const T vl = dl * gr, vr = dr * gl;
+  fl = dv((wl * ql + wr * qr) + wc * qc, (wl + wr) + wc);
+  fr = dv((vl * pl + vr * pr) + wc * pc, (vl + vr) + wc);
 }
 
 // ---------------------------------------------------------------------------
@@ -155,10 +185,11 @@ __device__ __forceinline__ void prim2cons(const EosConst<T>& e, const T* q, T* u
 
 template <class P, typename T>
 __device__ __forceinline__ void cons2prim(const EosConst<T>& e, const T* u, T* q) {
+  const T rdn = rcp(u[P::idn]);
   q[P::idn] = u[P::idn];
-  q[P::ivx] = u[P::imx] / u[P::idn];
-  q[P::ivy] = u[P::imy] / u[P::idn];
-  q[P::ivz] = u[P::imz] / u[P::idn];
+  q[P::ivx] = u[P::imx] * rdn;
+  q[P::ivy] = u[P::imy] * rdn;
+  q[P::ivz] = u[P::imz] * rdn;
   if constexpr (!P::mhd) {
     T ek = T(0.5) * (u[P::imx] * q[P::ivx] + u[P::imy] * q[P::ivy] + u[P::imz] * q[P::ivz]);
     T ei = u[P::ien] - ek;
@@ -200,7 +231,7 @@ __device__ __forceinline__ void fluxspeed(const EosConst<T>& e, T cmax2, const T
     f[P::imz] = q[P::ivx] * u[P::imz];
     f[P::imx] = f[P::imx] + q[P::ipr];
     f[P::ien] = q[P::ivx] * (u[P::ien] + q[P::ipr]);
-    T cs = tsqrt(e.gamma * q[P::ipr] / q[P::idn]);
+    T cs = tsqrt(dv(e.gamma * q[P::ipr], q[P::idn]));
     cm = q[P::ivx] - cs;
     cp = q[P::ivx] + cs;
   } else {
@@ -224,8 +255,9 @@ __device__ __forceinline__ void fluxspeed(const EosConst<T>& e, T cmax2, const T
     T fb = fa + bb;
     T fc = fb * fb - T(4) * fa * bx2;
     T cf;
-    if (fc > T(0)) cf = tsqrt(T(0.5) * (fb + tsqrt(fc)) / q[P::idn]);
-    else           cf = tsqrt(T(0.5) * fb / q[P::idn]);
+    const T rdn = rcp(q[P::idn]);
+    if (fc > T(0)) cf = tsqrt(T(0.5) * (fb + tsqrt(fc)) * rdn);
+    else           cf = tsqrt(T(0.5) * fb * rdn);
     cm = q[P::ivx] - cf;
     cp = q[P::ivx] + cf;
   }
@@ -236,7 +268,7 @@ __device__ __forceinline__ void fluxspeed(const EosConst<T>& e, T cmax2, const T
 template <class P, typename T>
 __device__ __forceinline__ void glm_upwind(T* ql, T* qr, T cmax) {
   if constexpr (P::mhd) {
-    T bx = T(0.5) * ((qr[P::ibx] + ql[P::ibx]) - (qr[P::ibp] - ql[P::ibp]) / cmax);
+    T bx = T(0.5) * ((qr[P::ibx] + ql[P::ibx]) - dv(qr[P::ibp] - ql[P::ibp], cmax));
     T bp = T(0.5) * ((qr[P::ibp] + ql[P::ibp]) - (qr[P::ibx] - ql[P::ibx]) * cmax);
     ql[P::ibx] = bx; qr[P::ibx] = bx;
     ql[P::ibp] = bp; qr[P::ibp] = bp;
@@ -267,12 +299,12 @@ __device__ __forceinline__ void riemann_hll(const EosConst<T>& e, T cmax, T cmax
 #pragma unroll
     for (int p = 0; p < nv; p++) f[p] = fr[p];
   } else {
-    T srml = sr - sl;
+    const T rsrml = rcp(sr - sl);
 #pragma unroll
     for (int p = 0; p < nv; p++) {
       T wl = sl * ul[p] - fl[p];
       T wr = sr * ur[p] - fr[p];
-      f[p] = (sl * wr - sr * wl) / srml;
+      f[p] = (sl * wr - sr * wl) * rsrml;
     }
   }
 }
@@ -305,25 +337,25 @@ __device__ __forceinline__ void riemann_hllc(const EosConst<T>& e, T cmax, T cma
     wl[p] = sl * ul[p] - fl[p];
     wr[p] = sr * ur[p] - fr[p];
   }
-  T dn = wr[P::idn] - wl[P::idn];
-  T sm = (wr[P::imx] - wl[P::imx]) / dn;
-  T pr = (wl[P::idn] * wr[P::imx] - wr[P::idn] * wl[P::imx]) / dn;
+  const T rdn = rcp(wr[P::idn] - wl[P::idn]);
+  T sm = (wr[P::imx] - wl[P::imx]) * rdn;
+  T pr = (wl[P::idn] * wr[P::imx] - wr[P::idn] * wl[P::imx]) * rdn;
   if (sm > T(0)) {
-    T slmm = sl - sm;
-    ui[P::idn] = wl[P::idn] / slmm;
+    const T rslmm = rcp(sl - sm);
+    ui[P::idn] = wl[P::idn] * rslmm;
     ui[P::imx] = ui[P::idn] * sm;
     ui[P::imy] = ui[P::idn] * ql[P::ivy];
     ui[P::imz] = ui[P::idn] * ql[P::ivz];
-    ui[P::ien] = (wl[P::ien] + sm * pr) / slmm;
+    ui[P::ien] = (wl[P::ien] + sm * pr) * rslmm;
 #pragma unroll
     for (int p = 0; p < nv; p++) f[p] = sl * ui[p] - wl[p];
   } else if (sm < T(0)) {
-    T srmm = sr - sm;
-    ui[P::idn] = wr[P::idn] / srmm;
+    const T rsrmm = rcp(sr - sm);
+    ui[P::idn] = wr[P::idn] * rsrmm;
     ui[P::imx] = ui[P::idn] * sm;
     ui[P::imy] = ui[P::idn] * qr[P::ivy];
     ui[P::imz] = ui[P::idn] * qr[P::ivz];
-    ui[P::ien] = (wr[P::ien] + sm * pr) / srmm;
+    ui[P::ien] = (wr[P::ien] + sm * pr) * rsrmm;
 #pragma unroll
     for (int p = 0; p < nv; p++) f[p] = sr * ui[p] - wr[p];
   } else {
@@ -338,12 +370,13 @@ __device__ __forceinline__ void riemann_hllc(const EosConst<T>& e, T cmax, T cma
 // star state of the HLLD fan from a wave vector w (outer state, speed diff
 // smm = s - sm, divisor dv = smm*w_rho - Bx^2), schemes.F90:3740-3760
 template <class P, typename T>
-__device__ __forceinline__ void hlld_outer(const T* w, T smm, T dv, T dn, T sm,
+__device__ __forceinline__ void hlld_outer(const T* w, T smm, T rsmm, T dvd, T dn, T sm,
                                            T bx, T pt, T psi, T* ui) {
-  T vy = (smm * w[P::imy] - bx * w[P::iby]) / dv;
-  T vz = (smm * w[P::imz] - bx * w[P::ibz]) / dv;
-  T by = (w[P::idn] * w[P::iby] - bx * w[P::imy]) / dv;
-  T bz = (w[P::idn] * w[P::ibz] - bx * w[P::imz]) / dv;
+  const T rdv = rcp(dvd);
+  T vy = (smm * w[P::imy] - bx * w[P::iby]) * rdv;
+  T vz = (smm * w[P::imz] - bx * w[P::ibz]) * rdv;
+  T by = (w[P::idn] * w[P::iby] - bx * w[P::imy]) * rdv;
+  T bz = (w[P::idn] * w[P::ibz] - bx * w[P::imz]) * rdv;
   T vb = (sm * bx + vy * by) + vz * bz;
   ui[P::idn] = dn;
   ui[P::imx] = dn * sm;
@@ -353,7 +386,7 @@ __device__ __forceinline__ void hlld_outer(const T* w, T smm, T dv, T dn, T sm,
   ui[P::iby] = by;
   ui[P::ibz] = bz;
   ui[P::ibp] = psi;
-  ui[P::ien] = ((w[P::ien] + sm * pt) - bx * vb) / smm;
+  ui[P::ien] = ((w[P::ien] + sm * pt) - bx * vb) * rsmm;
 }
 
 template <class P, typename T>
@@ -395,85 +428,87 @@ __device__ __forceinline__ void riemann_hlld(const EosConst<T>& e, T cmax, T cma
     for (int p = 0; p < nv; p++) f[p] = fr[p];
     return;
   }
-  T srml = sr - sl;
+  const T rsrml = rcp(sr - sl);
 #pragma unroll
   for (int p = 0; p < nv; p++) {
     wl[p] = sl * ul[p] - fl[p];
     wr[p] = sr * ur[p] - fr[p];
   }
-  T dn = wr[idn] - wl[idn];
-  T sm = (wr[imx] - wl[imx]) / dn;
+  const T rdn = rcp(wr[idn] - wl[idn]);
+  T sm = (wr[imx] - wl[imx]) * rdn;
   T slmm = sl - sm;
   T srmm = sr - sm;
-  T dnl = wl[idn] / slmm;
-  T dnr = wr[idn] / srmm;
+  const T rslmm = rcp(slmm), rsrmm = rcp(srmm);
+  T dnl = wl[idn] * rslmm;
+  T dnr = wr[idn] * rsrmm;
   T bx = ql[ibx];
   T b2 = ql[ibx] * qr[ibx];
-  T pt = (wl[idn] * wr[imx] - wr[idn] * wl[imx]) / dn + b2;
+  T pt = (wl[idn] * wr[imx] - wr[idn] * wl[imx]) * rdn + b2;
   T psil = ul[ibp], psir = ur[ibp];
 
   if (b2 > T(1.0e-08) * tmax(dnl * (sl * sl), dnr * (sr * sr))) {
-    T cal = tsqrt(b2 / dnl);
-    T car = tsqrt(b2 / dnr);
+    T cal = tsqrt(b2 * (slmm * rcp(wl[idn])));
+    T car = tsqrt(b2 * (srmm * rcp(wr[idn])));
     T sml = sm - cal;
     T smr = sm + car;
     T dvl = slmm * wl[idn] - b2;
     T dvr = srmm * wr[idn] - b2;
     if (dvl > T(0) && dvr > T(0)) {
       if (sml >= T(0)) {
-        hlld_outer<P>(wl, slmm, dvl, dnl, sm, bx, pt, psil, ui);
+        hlld_outer<P>(wl, slmm, rslmm, dvl, dnl, sm, bx, pt, psil, ui);
 #pragma unroll
         for (int p = 0; p < nv; p++) f[p] = sl * ui[p] - wl[p];
       } else if (smr <= T(0)) {
-        hlld_outer<P>(wr, srmm, dvr, dnr, sm, bx, pt, psir, ui);
+        hlld_outer<P>(wr, srmm, rsrmm, dvr, dnr, sm, bx, pt, psir, ui);
 #pragma unroll
         for (int p = 0; p < nv; p++) f[p] = sr * ui[p] - wr[p];
       } else {
-        hlld_outer<P>(wl, slmm, dvl, dnl, sm, bx, pt, psil, ui);
+        hlld_outer<P>(wl, slmm, rslmm, dvl, dnl, sm, bx, pt, psil, ui);
 #pragma unroll
         for (int p = 0; p < nv; p++) wcl[p] = (sml - sl) * ui[p] + wl[p];
-        hlld_outer<P>(wr, srmm, dvr, dnr, sm, bx, pt, psir, ui);
+        hlld_outer<P>(wr, srmm, rsrmm, dvr, dnr, sm, bx, pt, psir, ui);
 #pragma unroll
         for (int p = 0; p < nv; p++) wcr[p] = (smr - sr) * ui[p] + wr[p];
-        T dv = car * dnr + cal * dnl;
-        T vy = (wcr[imy] - wcl[imy]) / dv;
-        T vz = (wcr[imz] - wcl[imz]) / dv;
-        dv = car + cal;
-        T by = (wcr[iby] - wcl[iby]) / dv;
-        T bz = (wcr[ibz] - wcl[ibz]) / dv;
+        T rdv = rcp(car * dnr + cal * dnl);
+        T vy = (wcr[imy] - wcl[imy]) * rdv;
+        T vz = (wcr[imz] - wcl[imz]) * rdv;
+        rdv = rcp(car + cal);
+        T by = (wcr[iby] - wcl[iby]) * rdv;
+        T bz = (wcr[ibz] - wcl[ibz]) * rdv;
         T vb = (sm * bx + vy * by) + vz * bz;
         if (sm > T(0)) {
           hlld_inner_state<P>(dnl, sm, vy, vz, bx, by, bz, psil, ui);
-          ui[ien] = -((wcl[ien] + sm * pt) - bx * vb) / cal;
+          ui[ien] = -dv((wcl[ien] + sm * pt) - bx * vb, cal);
 #pragma unroll
           for (int p = 0; p < nv; p++) f[p] = sml * ui[p] - wcl[p];
         } else if (sm < T(0)) {
           hlld_inner_state<P>(dnr, sm, vy, vz, bx, by, bz, psir, ui);
-          ui[ien] = ((wcr[ien] + sm * pt) - bx * vb) / car;
+          ui[ien] = dv((wcr[ien] + sm * pt) - bx * vb, car);
 #pragma unroll
           for (int p = 0; p < nv; p++) f[p] = smr * ui[p] - wcr[p];
         } else {
+          const T rd = rcp(smr - sml);
 #pragma unroll
-          for (int p = 0; p < nv; p++) f[p] = (sml * wcr[p] - smr * wcl[p]) / (smr - sml);
+          for (int p = 0; p < nv; p++) f[p] = (sml * wcr[p] - smr * wcl[p]) * rd;
         }
       }
     } else if (dvl > T(0)) {
-      hlld_outer<P>(wl, slmm, dvl, dnl, sm, bx, pt, psil, ui);
+      hlld_outer<P>(wl, slmm, rslmm, dvl, dnl, sm, bx, pt, psil, ui);
       if (sml >= T(0)) {
 #pragma unroll
         for (int p = 0; p < nv; p++) f[p] = sl * ui[p] - wl[p];
       } else {
 #pragma unroll
         for (int p = 0; p < nv; p++) wcl[p] = (sml - sl) * ui[p] + wl[p];
-        T dv = srmm * dnr + cal * dnl;
-        T vy = (wr[imy] - wcl[imy]) / dv;
-        T vz = (wr[imz] - wcl[imz]) / dv;
-        dv = sr - sml;
-        T by = (wr[iby] - wcl[iby]) / dv;
-        T bz = (wr[ibz] - wcl[ibz]) / dv;
+        T rdv = rcp(srmm * dnr + cal * dnl);
+        T vy = (wr[imy] - wcl[imy]) * rdv;
+        T vz = (wr[imz] - wcl[imz]) * rdv;
+        rdv = rcp(sr - sml);
+        T by = (wr[iby] - wcl[iby]) * rdv;
+        T bz = (wr[ibz] - wcl[ibz]) * rdv;
         T vb = (sm * bx + vy * by) + vz * bz;
         hlld_inner_state<P>(dnl, sm, vy, vz, bx, by, bz, psil, ui);
-        ui[ien] = -((wcl[ien] + sm * pt) - bx * vb) / cal;
+        ui[ien] = -dv((wcl[ien] + sm * pt) - bx * vb, cal);
         if (sm >= T(0)) {
 #pragma unroll
           for (int p = 0; p < nv; p++) f[p] = sml * ui[p] - wcl[p];
@@ -481,26 +516,26 @@ __device__ __forceinline__ void riemann_hlld(const EosConst<T>& e, T cmax, T cma
 #pragma unroll
           for (int p = 0; p < nv; p++) wcr[p] = (sm - sml) * ui[p] + wcl[p];
 #pragma unroll
-          for (int p = 0; p < nv; p++) f[p] = (sm * wr[p] - sr * wcr[p]) / srmm;
+          for (int p = 0; p < nv; p++) f[p] = (sm * wr[p] - sr * wcr[p]) * rsrmm;
         }
       }
     } else if (dvr > T(0)) {
-      hlld_outer<P>(wr, srmm, dvr, dnr, sm, bx, pt, psir, ui);
+      hlld_outer<P>(wr, srmm, rsrmm, dvr, dnr, sm, bx, pt, psir, ui);
       if (smr <= T(0)) {
 #pragma unroll
         for (int p = 0; p < nv; p++) f[p] = sr * ui[p] - wr[p];
       } else {
 #pragma unroll
         for (int p = 0; p < nv; p++) wcr[p] = (smr - sr) * ui[p] + wr[p];
-        T dv = slmm * dnl - car * dnr;
-        T vy = (wl[imy] - wcr[imy]) / dv;
-        T vz = (wl[imz] - wcr[imz]) / dv;
-        dv = sl - smr;
-        T by = (wl[iby] - wcr[iby]) / dv;
-        T bz = (wl[ibz] - wcr[ibz]) / dv;
+        T rdv = rcp(slmm * dnl - car * dnr);
+        T vy = (wl[imy] - wcr[imy]) * rdv;
+        T vz = (wl[imz] - wcr[imz]) * rdv;
+        rdv = rcp(sl - smr);
+        T by = (wl[iby] - wcr[iby]) * rdv;
+        T bz = (wl[ibz] - wcr[ibz]) * rdv;
         T vb = (sm * bx + vy * by) + vz * bz;
         hlld_inner_state<P>(dnr, sm, vy, vz, bx, by, bz, psir, ui);
-        ui[ien] = ((wcr[ien] + sm * pt) - bx * vb) / car;
+        ui[ien] = dv((wcr[ien] + sm * pt) - bx * vb, car);
         if (sm <= T(0)) {
 #pragma unroll
           for (int p = 0; p < nv; p++) f[p] = smr * ui[p] - wcr[p];
@@ -508,53 +543,53 @@ __device__ __forceinline__ void riemann_hlld(const EosConst<T>& e, T cmax, T cma
 #pragma unroll
           for (int p = 0; p < nv; p++) wcl[p] = (sm - smr) * ui[p] + wcr[p];
 #pragma unroll
-          for (int p = 0; p < nv; p++) f[p] = (sm * wl[p] - sl * wcl[p]) / slmm;
+          for (int p = 0; p < nv; p++) f[p] = (sm * wl[p] - sl * wcl[p]) * rslmm;
         }
       }
     } else {
 #pragma unroll
-      for (int p = 0; p < nv; p++) f[p] = (sl * wr[p] - sr * wl[p]) / srml;
+      for (int p = 0; p < nv; p++) f[p] = (sl * wr[p] - sr * wl[p]) * rsrml;
     }
   } else if (b2 > T(0)) {
-    T vy = (wr[imy] - wl[imy]) / dn;
-    T vz = (wr[imz] - wl[imz]) / dn;
-    T by = (wr[iby] - wl[iby]) / srml;
-    T bz = (wr[ibz] - wl[ibz]) / srml;
+    T vy = (wr[imy] - wl[imy]) * rdn;
+    T vz = (wr[imz] - wl[imz]) * rdn;
+    T by = (wr[iby] - wl[iby]) * rsrml;
+    T bz = (wr[ibz] - wl[ibz]) * rsrml;
     T vb = (sm * bx + vy * by) + vz * bz;
     if (sm > T(0)) {
       hlld_inner_state<P>(dnl, sm, vy, vz, bx, by, bz, psil, ui);
-      ui[ien] = ((wl[ien] + sm * pt) - bx * vb) / slmm;
+      ui[ien] = ((wl[ien] + sm * pt) - bx * vb) * rslmm;
 #pragma unroll
       for (int p = 0; p < nv; p++) f[p] = sl * ui[p] - wl[p];
     } else if (sm < T(0)) {
       hlld_inner_state<P>(dnr, sm, vy, vz, bx, by, bz, psir, ui);
-      ui[ien] = ((wr[ien] + sm * pt) - bx * vb) / srmm;
+      ui[ien] = ((wr[ien] + sm * pt) - bx * vb) * rsrmm;
 #pragma unroll
       for (int p = 0; p < nv; p++) f[p] = sr * ui[p] - wr[p];
     } else {
 #pragma unroll
-      for (int p = 0; p < nv; p++) f[p] = (sl * wr[p] - sr * wl[p]) / srml;
+      for (int p = 0; p < nv; p++) f[p] = (sl * wr[p] - sr * wl[p]) * rsrml;
     }
   } else {
     if (sm > T(0)) {
       ui[idn] = dnl; ui[imx] = dnl * sm;
-      ui[imy] = wl[imy] / slmm; ui[imz] = wl[imz] / slmm;
-      ui[ibx] = T(0); ui[iby] = wl[iby] / slmm; ui[ibz] = wl[ibz] / slmm;
+      ui[imy] = wl[imy] * rslmm; ui[imz] = wl[imz] * rslmm;
+      ui[ibx] = T(0); ui[iby] = wl[iby] * rslmm; ui[ibz] = wl[ibz] * rslmm;
       ui[ibp] = psil;
-      ui[ien] = (wl[ien] + sm * pt) / slmm;
+      ui[ien] = (wl[ien] + sm * pt) * rslmm;
 #pragma unroll
       for (int p = 0; p < nv; p++) f[p] = sl * ui[p] - wl[p];
     } else if (sm < T(0)) {
       ui[idn] = dnr; ui[imx] = dnr * sm;
-      ui[imy] = wr[imy] / srmm; ui[imz] = wr[imz] / srmm;
-      ui[ibx] = T(0); ui[iby] = wr[iby] / srmm; ui[ibz] = wr[ibz] / srmm;
+      ui[imy] = wr[imy] * rsrmm; ui[imz] = wr[imz] * rsrmm;
+      ui[ibx] = T(0); ui[iby] = wr[iby] * rsrmm; ui[ibz] = wr[ibz] * rsrmm;
       ui[ibp] = psir;
-      ui[ien] = (wr[ien] + sm * pt) / srmm;
+      ui[ien] = (wr[ien] + sm * pt) * rsrmm;
 #pragma unroll
       for (int p = 0; p < nv; p++) f[p] = sr * ui[p] - wr[p];
     } else {
 #pragma unroll
-      for (int p = 0; p < nv; p++) f[p] = (sl * wr[p] - sr * wl[p]) / srml;
+      for (int p = 0; p < nv; p++) f[p] = (sl * wr[p] - sr * wl[p]) * rsrml;
     }
   }
 }

@@ -83,6 +83,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="skip the mixed-precision leg reported beside the FP64 value")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
 
@@ -108,24 +110,38 @@ def main():
     cells = p.ncells_total
     print(f"[bench] building {args.size}^3 initial state", file=sys.stderr, flush=True)
     u0 = initial_state(p)
-    s = Solver(p)
-    s.upload(u0)
+
+    def measure(params):
+        s = Solver(params)
+        s.upload(u0)
+        print(f"[bench] {params.precision}: warmup {args.warmup} steps", file=sys.stderr,
+              flush=True)
+        s.run(args.warmup)
+        s.sync()
+        sync_all()
+        t0 = time.perf_counter()
+        dev_ms, _ = s.timed_run(args.steps)
+        s.sync()
+        sync_all()
+        wall = time.perf_counter() - t0
+        print(f"[bench] {params.precision}: {args.steps} steps: wall {wall:.3f} s, "
+              f"device {dev_ms:.1f} ms", file=sys.stderr, flush=True)
+        # per-kernel averages over a few more steps (same stream, HIP events)
+        _, kms = s.timed_run(3, per_kernel=True)
+        s.close()
+        return wall, dev_ms, kms
+
+    wall, dev_ms, kms = measure(p)
+    mixed = None
+    if args.precision == "fp64" and not args.no_mixed:
+        pm = c2_params(args.size, precision="mixed")
+        wm, dm, km = measure(pm)
+        mixed = {"value": cells * args.steps / wm, "ms_per_step": wm / args.steps * 1e3,
+                 "speedup_vs_fp64": wall / wm,
+                 "roofline_frac": bytes_per_zone_update(pm) * cells / (dm / args.steps * 1e-3)
+                 / 1e9 / HBM_PEAK_GBS,
+                 "kernel_ms_per_step": [round(float(x), 4) for x in km]}
     del u0
-    print(f"[bench] warmup {args.warmup} steps", file=sys.stderr, flush=True)
-    s.run(args.warmup)
-    s.sync()
-    sync_all()
-    t0 = time.perf_counter()
-    dev_ms, _ = s.timed_run(args.steps)
-    s.sync()
-    sync_all()
-    wall = time.perf_counter() - t0
-    print(f"[bench] {args.steps} steps: wall {wall:.3f} s, device {dev_ms:.1f} ms",
-          file=sys.stderr, flush=True)
-    # per-kernel averages over a few more steps (same stream, HIP events)
-    nk_steps = 3
-    _, kms = s.timed_run(nk_steps, per_kernel=True)
-    s.close()
 
     bpz = bytes_per_zone_update(p)
     step_ms = dev_ms / args.steps
@@ -159,6 +175,8 @@ def main():
                      "device_ms_per_step": step_ms,
                      "kernel_ms_per_step": [round(float(x), 4) for x in kms]},
     }
+    if mixed is not None:
+        out["mixed"] = mixed
     if not args.no_cpu_baseline:
         print("[bench] CPU baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(p)
