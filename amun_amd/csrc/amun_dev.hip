@@ -295,6 +295,12 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a, RunConst rc) {
   constexpr int NV = P::nv;
   constexpr bool kLast = (DIR == ND - 1);
   using RC = Recon<P, REC, ND, DIR, T>;
+  // chunked pencil I/O along x and y (see loadq below)
+  constexpr bool kChunk = DIR < 2;
+  constexpr bool kDin = kChunk && DIR > 0;       // increment read in chunks
+  constexpr bool kDout = kChunk && !kLast;       // increment written in chunks
+  __shared__ T sq[kChunk ? 4 * NV * 256 : 1];
+  __shared__ T sd[(kDin || kDout) ? 4 * NV * 256 : 1];
   const int nc = rc.nc;
   const int npen = (ND == 3) ? nc * nc : nc;
   const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -327,12 +333,44 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a, RunConst rc) {
       else if constexpr (DIR == 1) { i = c1; j = c; k = c2; }
       else { i = c1; j = c2; k = c; }
     };
+    // x and y sweeps move their pencil data in chunks of four cells: a
+    // brick line spans four cells along x and along y, so loading (and
+    // storing) four cells at once uses whole lines instead of a quarter of a
+    // line per walk step.  The chunks live in per-lane LDS slots.
+    const int tid = threadIdx.x;
     auto loadq = [&](int c, T* q) {
-      int i, j, k;
-      coords(c, i, j, k);
-      size_t pk = cell_index<sizeof(T)>(a.nbr, rc, b, i, j, k);
+      if constexpr (kChunk) {
+        const int first = -1 - RC::R;
+        if (c == first || (c & 3) == 0) {
+          const int c0 = c & ~3;
 #pragma unroll
-      for (int v = 0; v < NV; v++) q[v] = Qp[var_index<NV>(rc, pk, v)];
+          for (int m = 0; m < 4; m++) {
+            int i, j, k;
+            coords(c0 + m, i, j, k);
+            const size_t pk = cell_index<sizeof(T)>(a.nbr, rc, b, i, j, k);
+#pragma unroll
+            for (int v = 0; v < NV; v++) sq[(m * NV + v) * 256 + tid] = Qp[var_index<NV>(rc, pk, v)];
+          }
+        }
+#pragma unroll
+        for (int v = 0; v < NV; v++) q[v] = sq[((c & 3) * NV + v) * 256 + tid];
+      } else {
+        int i, j, k;
+        coords(c, i, j, k);
+        size_t pk = cell_index<sizeof(T)>(a.nbr, rc, b, i, j, k);
+#pragma unroll
+        for (int v = 0; v < NV; v++) q[v] = Qp[var_index<NV>(rc, pk, v)];
+      }
+    };
+    // global offsets (without the variable term) of the four cells of the
+    // increment chunk that starts at cell c0
+    auto dchunk_base = [&](int c0, size_t* base) {
+#pragma unroll
+      for (int m = 0; m < 4; m++) {
+        int i, j, k;
+        coords(c0 + m, i, j, k);
+        base[m] = (size_t)b * NV * (size_t)rc.ncell + (size_t)lidx<sizeof(T)>(nc, ND, i, j, k);
+      }
     };
 
     RC rec;
@@ -358,14 +396,27 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a, RunConst rc) {
         const size_t bt = bb + (size_t)lidx<sizeof(T)>(nc, ND, i, j, k);
         const size_t bd = bb + (size_t)lidx<8>(nc, ND, i, j, k);
         const size_t bf = bb + (size_t)lidx<4>(nc, ND, i, j, k);
+        if constexpr (kDin) {
+          if (((c - 1) & 3) == 0) {
+            size_t base[4];
+            dchunk_base(c - 1, base);
+#pragma unroll
+            for (int m = 0; m < 4; m++)
+#pragma unroll
+              for (int v = 0; v < NV; v++)
+                sd[(m * NV + v) * 256 + tid] = Dp[base[m] + (size_t)v * rc.ncell];
+          }
+        }
 #pragma unroll
         for (int v = 0; v < NV; v++) {
           const size_t off = (size_t)v * rc.ncell;
-          if constexpr (DIR > 0) dpre[v] = Dp[bt + off];
+          if constexpr (DIR > 0 && !kDin) dpre[v] = Dp[bt + off];
           if constexpr (kLast) {
             u0pre[v] = a.U0[bd + off];
             if (a.kind != 0) u1pre[v] = a.U1[bd + off];
-            if constexpr (sizeof(T) == 4) cpre[v] = a.C[bf + off];
+            // the compensation enters the step update; the transient
+            // stages ignore it (it is below the FP64 rounding of U0)
+            if constexpr (sizeof(T) == 4) cpre[v] = a.last ? a.C[bf + off] : 0.0f;
           }
         }
       }
@@ -404,9 +455,11 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a, RunConst rc) {
             const T dfx = f[v] - fprev[v];
             T du;
             if constexpr (DIR == 0) du = -dxi * dfx;
+            else if constexpr (kDin) du = sd[(((c - 1) & 3) * NV + v) * 256 + tid] - dxi * dfx;
             else du = dpre[v] - dxi * dfx;
             if constexpr (!kLast) {
-              Dp[bt + off] = du;
+              if constexpr (kDout) sd[(((c - 1) & 3) * NV + v) * 256 + tid] = du;
+              else Dp[bt + off] = du;
             } else if constexpr (sizeof(T) == 8) {
               double un = combine(a.kind, u0pre[v], u1pre[v], ds, du);
               if (a.last && v == P::ibp) un = rc.decay * un;
@@ -414,8 +467,9 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a, RunConst rc) {
             } else {
               // mixed precision: the FP32 increment is added to the FP64
               // state in increment form with Kahan compensation.  The true
-              // step state is U0 - C; every stage is U0_true + delta, and the
-              // last stage keeps the rounding error of that sum in C.
+              // step state is U0 - C; the last stage forms U0_true + delta
+              // and keeps the rounding error of that sum in C (the
+              // intermediate stages use U0 alone).
               const double u0 = u0pre[v];
               const double cp = double(cpre[v]);
               const double dd = ds * double(du);
@@ -446,6 +500,17 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a, RunConst rc) {
               Qo[bt + off] = T(qn[v]);
             }
             if (a.last) smax = fmax(smax, maxspeed<P, double>(ed, qn));
+          }
+        }
+        if constexpr (kDout) {
+          if (c >= 1 && ((c - 1) & 3) == 3) {
+            size_t base[4];
+            dchunk_base(c - 4, base);
+#pragma unroll
+            for (int m = 0; m < 4; m++)
+#pragma unroll
+              for (int v = 0; v < NV; v++)
+                Dp[base[m] + (size_t)v * rc.ncell] = sd[(m * NV + v) * 256 + tid];
           }
         }
 #pragma unroll

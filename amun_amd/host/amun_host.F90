@@ -1,0 +1,445 @@
+!===============================================================================
+!
+! amun_host.F90 -- Fortran 2003 host driver of the MI355X block update
+!
+! The host keeps what the reference keeps on the CPU -- parameters, the block
+! list with its u(nv,im,jm,km) arrays, problem set-up and I/O -- and hands the
+! whole time step to the device through the C-ABI (module amun_dev_mod).  It
+! mirrors the reference driver: read the parameter file given with -i
+! (parameters.F90:99-195: "key = value", # comments, quotes, last occurrence
+! wins), set up the problem on every block, then loop
+!   advance(dtnext); time = time + dt        (driver.F90:571-609)
+! for nmax steps or until tmax, and report the time per step
+! (driver.F90:730-793).
+!
+! Problems: "blast" (the seeded 2D/3D hydrodynamic blast: rho = 1, v = 0,
+! p = 10 inside r < 0.1 else 0.1, density * (1 + 0.01 (2 xi - 1))),
+! "orszag_tang" (2D MHD vortex with the same density noise) and "file"
+! (conserved state read from "input" as raw FP64 U(nx,ny,nz,nv)).  The noise
+! xi comes from the counter-based SplitMix64 generator, bit-identical to the
+! Python side (amun_amd/problems.py).
+!
+! Outputs: "output" (raw FP64 U(nx,ny,nz,nv) after the run, optional) and a
+! summary on stdout.
+!
+!===============================================================================
+!
+program amun_host
+
+  use iso_c_binding
+  use iso_fortran_env, only : int64, real64, error_unit
+  use amun_dev_mod
+
+  implicit none
+
+  integer, parameter :: mpar = 256
+  character(len=64)  :: pnames(mpar)
+  character(len=256) :: pvalues(mpar)
+  integer            :: npar = 0
+
+  type(amun_cfg) :: cfg
+  type(c_ptr)    :: ctx
+  integer        :: iret, nv, nc, ng, im, km, nblk, nx, ny, nz, kn
+  integer        :: nmax, s, b, i, j, k, p
+  integer(int64) :: seed, c0, c1, crate
+  real(real64)   :: dt, cmax, time, tmax, dtnext, elapsed, ints(8)
+  character(len=256) :: fin, fout, problem, str
+  real(c_double), allocatable, target :: u(:,:,:,:,:)
+  integer(c_int32_t), allocatable :: coords(:,:)
+  real(real64), allocatable :: ug(:,:,:,:)
+
+  call read_params()
+
+  call amun_dev_default_cfg(cfg)
+  cfg%ndims = get_int("ndims", 3)
+  str = get_str("equation_system", "hd")
+  select case(trim(str))
+  case("hd", "HD", "hydrodynamic", "HYDRODYNAMIC")
+    cfg%eqsys = AMUN_EQSYS_HD
+  case("mhd", "MHD", "magnetohydrodynamic", "MAGNETOHYDRODYNAMIC")
+    cfg%eqsys = AMUN_EQSYS_MHD
+  case default
+    cfg%eqsys = -1
+  end select
+  str = get_str("equation_of_state", "adi")
+  cfg%eos = merge(AMUN_EOS_ADI, -1_c_int32_t, any(trim(str) == &
+                  [character(len=10) :: "adi", "ADI", "adiabatic", "ADIABATIC"]))
+  str = get_str("reconstruction", "tvd")
+  select case(trim(str))
+  case("tvd", "TVD")
+    cfg%reconstruction = AMUN_REC_TVD
+  case("weno5z", "weno5-z", "WENO5Z", "WENO5-Z")
+    cfg%reconstruction = AMUN_REC_WENO5Z
+  case default
+    cfg%reconstruction = -1
+  end select
+  str = get_str("limiter", "mm")
+  select case(trim(str))
+  case("mm", "minmod")
+    cfg%limiter = AMUN_LIM_MINMOD
+  case("mc", "monotonized_central")
+    cfg%limiter = AMUN_LIM_MC
+  case("sb", "superbee")
+    cfg%limiter = AMUN_LIM_SUPERBEE
+  case("vl", "vanleer")
+    cfg%limiter = AMUN_LIM_VANLEER
+  case("va", "vanalbada")
+    cfg%limiter = AMUN_LIM_VANALBADA
+  case default
+    cfg%limiter = AMUN_LIM_ZERO
+  end select
+  str = get_str("riemann_solver", "hlld")
+  select case(trim(str))
+  case("hll", "HLL")
+    cfg%riemann_solver = AMUN_RS_HLL
+  case("hllc", "HLLC")
+    cfg%riemann_solver = AMUN_RS_HLLC
+  case("hlld", "HLLD")
+    cfg%riemann_solver = AMUN_RS_HLLD
+  case default
+    cfg%riemann_solver = -1
+  end select
+  str = get_str("time_advance", "rk2")
+  select case(trim(str))
+  case("euler", "EULER")
+    cfg%time_advance = AMUN_TA_EULER
+  case("rk2", "RK2")
+    cfg%time_advance = AMUN_TA_RK2
+  case("rk3", "RK3")
+    cfg%time_advance = AMUN_TA_RK3
+  case default
+    cfg%time_advance = -1
+  end select
+  str = get_str("precision", "fp64")
+  cfg%precision = merge(AMUN_PREC_MIXED, AMUN_PREC_FP64, trim(str) == "mixed")
+  cfg%gamma   = get_real("gamma", 5.0d0 / 3.0d0)
+  cfg%cfl     = get_real("cfl", 0.5d0)
+  cfg%eps     = get_real("eps", epsilon(1.0d0))
+  cfg%alpha   = get_real("alpha", 2.0d0)
+  cfg%ncells  = get_int("ncells", 8)
+  cfg%nghosts = get_int("nghosts", 2)
+  cfg%xblocks = get_int("xblocks", 1)
+  cfg%yblocks = get_int("yblocks", 1)
+  cfg%zblocks = get_int("zblocks", 1)
+  cfg%xmin = get_real("xmin", 0.0d0)
+  cfg%xmax = get_real("xmax", 1.0d0)
+  cfg%ymin = get_real("ymin", 0.0d0)
+  cfg%ymax = get_real("ymax", 1.0d0)
+  cfg%zmin = get_real("zmin", 0.0d0)
+  cfg%zmax = get_real("zmax", 1.0d0)
+  nmax    = get_int("nmax", 10)
+  tmax    = get_real("tmax", 1.0d30)
+  seed    = int(get_int("seed", 20261202), int64)
+  problem = get_str("problem", "blast")
+  fin     = get_str("input", "")
+  fout    = get_str("output", "")
+
+  nv   = merge(5, 9, cfg%eqsys == AMUN_EQSYS_HD)
+  nc   = cfg%ncells
+  ng   = cfg%nghosts
+  im   = nc + 2 * ng
+  km   = merge(im, 1, cfg%ndims == 3)
+  kn   = merge(nc, 1, cfg%ndims == 3)
+  nblk = cfg%xblocks * cfg%yblocks * merge(cfg%zblocks, 1_c_int32_t, cfg%ndims == 3)
+  nx   = cfg%xblocks * nc
+  ny   = cfg%yblocks * nc
+  nz   = merge(cfg%zblocks * nc, 1, cfg%ndims == 3)
+  allocate(u(nv,im,im,km,nblk), coords(3,nblk), ug(nx,ny,nz,nv))
+
+! problem set-up on the global grid (host side, as setup_problem does per
+! block in the reference)
+  select case(trim(problem))
+  case("file")
+    call read_state(trim(fin))
+  case("orszag_tang")
+    call setup_orszag_tang()
+  case default
+    call setup_blast()
+  end select
+  str = get_str("dump_initial", "")
+  if (len_trim(str) > 0) then
+    call write_state(trim(str))
+    stop
+  end if
+
+  iret = amun_dev_init(cfg, ctx)
+  if (iret /= 0) then
+    write(error_unit, "(a,i0,2a)") "amun_dev_init failed with code ", iret, &
+                                   ": ", trim(amun_dev_last_error())
+    stop 2
+  end if
+  iret = amun_dev_block_coords(ctx, coords)
+  call scatter()
+  iret = amun_dev_upload_blocks(ctx, u)
+  if (iret /= 0) call die("upload")
+
+  time   = 0.0d0
+  dtnext = 2.0d0 * tmax
+  iret = amun_dev_new_time_step(ctx, dtnext, dt, cmax)
+  call system_clock(c0, crate)
+  s = 0
+  do while (s < nmax .and. time < tmax)
+    iret = amun_dev_step(ctx, dtnext, dt, cmax)
+    if (iret /= 0) call die("step")
+    time = time + dt
+    s = s + 1
+  end do
+  call system_clock(c1)
+  elapsed = real(c1 - c0, real64) / real(crate, real64)
+
+  iret = amun_dev_integrals(ctx, ints)
+  iret = amun_dev_download_blocks(ctx, u, c_null_ptr)
+  if (iret /= 0) call die("download")
+  call gather()
+  if (len_trim(fout) > 0) call write_state(trim(fout))
+
+  write(*,"(a,i0,a,es22.15)") "steps = ", s, "  time = ", time
+  write(*,"(a,es22.15)") "last dt = ", dt
+  write(*,"(a,8es22.14)") "integrals = ", ints
+  if (s > 0) then
+    write(*,"(a,es12.5,a)") "TIME PER STEP = ", elapsed / s, " s"
+    write(*,"(a,es12.5)") "zone-updates/s = ", real(nx, real64) * ny * nz * s / elapsed
+  end if
+
+  call amun_dev_finalize(ctx)
+
+contains
+
+  subroutine die(what)
+    character(len=*), intent(in) :: what
+    write(error_unit, "(3a)") what, " failed: ", trim(amun_dev_last_error())
+    call amun_dev_finalize(ctx)
+    stop 3
+  end subroutine die
+
+! ---------------------------------------------------------------- parameters
+  subroutine read_params()
+    character(len=512) :: line, fname
+    integer :: l, u, ios, e, h
+    fname = "params.in"
+    do l = 1, command_argument_count()
+      call get_command_argument(l, line)
+      if (trim(line) == "-i" .or. trim(line) == "--input") then
+        call get_command_argument(l + 1, fname)
+      end if
+    end do
+    open(newunit=u, file=trim(fname), status="old", action="read", iostat=ios)
+    if (ios /= 0) then
+      write(error_unit, "(3a)") "cannot open parameter file '", trim(fname), "'"
+      stop 1
+    end if
+    do
+      read(u, "(a)", iostat=ios) line
+      if (ios /= 0) exit
+      h = index(line, "#")
+      if (h > 0) line = line(1:h-1)
+      e = index(line, "=")
+      if (e == 0) cycle
+      call set_param(adjustl(line(1:e-1)), adjustl(line(e+1:)))
+    end do
+    close(u)
+  end subroutine read_params
+
+  subroutine set_param(name, value)
+    character(len=*), intent(in) :: name, value
+    character(len=256) :: v
+    integer :: n
+    v = trim(value)
+    n = len_trim(v)
+    if (n >= 2) then
+      if ((v(1:1) == '"' .and. v(n:n) == '"') .or. &
+          (v(1:1) == "'" .and. v(n:n) == "'")) v = v(2:n-1)
+    end if
+    do n = 1, npar
+      if (trim(pnames(n)) == trim(name)) then
+        pvalues(n) = v
+        return
+      end if
+    end do
+    if (npar < mpar) then
+      npar = npar + 1
+      pnames(npar)  = trim(name)
+      pvalues(npar) = v
+    end if
+  end subroutine set_param
+
+  function get_str(name, def) result(r)
+    character(len=*), intent(in) :: name, def
+    character(len=256) :: r
+    integer :: n
+    r = def
+    do n = 1, npar
+      if (trim(pnames(n)) == name) r = pvalues(n)
+    end do
+  end function get_str
+
+  integer function get_int(name, def)
+    character(len=*), intent(in) :: name
+    integer, intent(in)          :: def
+    character(len=256) :: r
+    integer :: ios
+    r = get_str(name, "")
+    get_int = def
+    if (len_trim(r) > 0) read(r, *, iostat=ios) get_int
+  end function get_int
+
+  real(real64) function get_real(name, def)
+    character(len=*), intent(in) :: name
+    real(real64), intent(in)     :: def
+    character(len=256) :: r
+    integer :: ios
+    r = get_str(name, "")
+    get_real = def
+    if (len_trim(r) > 0) read(r, *, iostat=ios) get_real
+  end function get_real
+
+! ------------------------------------------------------------ random numbers
+! SplitMix64 (Steele, Lea & Flood 2014) in wrapping 64-bit arithmetic
+  integer(int64) function splitmix64(x)
+    integer(int64), intent(in) :: x
+    integer(int64) :: z
+    z = x + transfer(int(z'9E3779B97F4A7C15', int64), 0_int64)
+    z = ieor(z, ishft(z, -30)) * transfer(int(z'BF58476D1CE4E5B9', int64), 0_int64)
+    z = ieor(z, ishft(z, -27)) * transfer(int(z'94D049BB133111EB', int64), 0_int64)
+    splitmix64 = ieor(z, ishft(z, -31))
+  end function splitmix64
+
+! n uniform doubles in [0,1): the 53 high bits of SplitMix64(key + i)
+  subroutine uniform(stream, n, xi)
+    integer, intent(in)       :: stream, n
+    real(real64), intent(out) :: xi(n)
+    integer(int64) :: key
+    integer :: m
+    key = splitmix64(ieor(seed, ishft(int(stream, int64), 40)))
+    do m = 1, n
+      xi(m) = real(ishft(splitmix64(key + int(m - 1, int64)), -11), real64) &
+              * (1.0d0 / 9007199254740992.0d0)
+    end do
+  end subroutine uniform
+
+! ------------------------------------------------------------------ problems
+  subroutine cons_from_prim(q)
+    real(real64), intent(in) :: q(nx,ny,nz,nv)
+    real(real64) :: ek(nx,ny,nz), g
+    g = cfg%gamma
+    ug(:,:,:,1) = q(:,:,:,1)
+    do p = 2, 4
+      ug(:,:,:,p) = q(:,:,:,1) * q(:,:,:,p)
+    end do
+    ek = 0.5d0 * (ug(:,:,:,2) * q(:,:,:,2) + ug(:,:,:,3) * q(:,:,:,3) &
+                + ug(:,:,:,4) * q(:,:,:,4))
+    if (nv == 5) then
+      ug(:,:,:,5) = q(:,:,:,5) / (g - 1.0d0) + ek
+    else
+      ug(:,:,:,5:8) = q(:,:,:,5:8)
+      ug(:,:,:,9) = (q(:,:,:,9) / (g - 1.0d0) + ek) + 0.5d0 * (q(:,:,:,5)**2 &
+                    + q(:,:,:,6)**2 + q(:,:,:,7)**2)
+    end if
+  end subroutine cons_from_prim
+
+  subroutine setup_blast()
+    real(real64), allocatable :: q(:,:,:,:), xi(:)
+    real(real64) :: x, y, z, r, dx, dy, dz
+    allocate(q(nx,ny,nz,nv), xi(nx*ny*nz))
+    call uniform(0, nx*ny*nz, xi)
+    dx = (cfg%xmax - cfg%xmin) / nx
+    dy = (cfg%ymax - cfg%ymin) / ny
+    dz = (cfg%zmax - cfg%zmin) / max(nz, 1)
+    q = 0.0d0
+    do k = 1, nz
+      do j = 1, ny
+        do i = 1, nx
+          x = cfg%xmin + (i - 0.5d0) * dx
+          y = cfg%ymin + (j - 0.5d0) * dy
+          z = merge(cfg%zmin + (k - 0.5d0) * dz, 0.0d0, cfg%ndims == 3)
+          r = sqrt(x * x + y * y + merge(z * z, 0.0d0, cfg%ndims == 3))
+          q(i,j,k,1) = 1.0d0 * (1.0d0 + 0.01d0 * (2.0d0 * xi(i + nx * (j - 1 + ny * (k - 1))) - 1.0d0))
+          q(i,j,k,nv) = merge(10.0d0, 0.1d0, r < 0.1d0)
+        end do
+      end do
+    end do
+    call cons_from_prim(q)
+  end subroutine setup_blast
+
+  subroutine setup_orszag_tang()
+    real(real64), allocatable :: q(:,:,:,:), xi(:)
+    real(real64) :: x, y, dx, dy, pi, b0
+    pi = acos(-1.0d0)
+    b0 = 1.0d0 / sqrt(4.0d0 * pi)
+    allocate(q(nx,ny,nz,nv), xi(nx*ny*nz))
+    call uniform(5, nx*ny*nz, xi)
+    dx = (cfg%xmax - cfg%xmin) / nx
+    dy = (cfg%ymax - cfg%ymin) / ny
+    q = 0.0d0
+    do k = 1, nz
+      do j = 1, ny
+        do i = 1, nx
+          x = cfg%xmin + (i - 0.5d0) * dx
+          y = cfg%ymin + (j - 0.5d0) * dy
+          q(i,j,k,1) = 25.0d0 / (36.0d0 * pi) * (1.0d0 + 0.01d0 * (2.0d0 &
+                       * xi(i + nx * (j - 1 + ny * (k - 1))) - 1.0d0))
+          q(i,j,k,2) = -sin(2.0d0 * pi * y)
+          q(i,j,k,3) =  sin(2.0d0 * pi * x)
+          if (nv == 9) then
+            q(i,j,k,5) = -b0 * sin(2.0d0 * pi * y)
+            q(i,j,k,6) =  b0 * sin(4.0d0 * pi * x)
+          end if
+          q(i,j,k,nv) = 5.0d0 / (12.0d0 * pi)
+        end do
+      end do
+    end do
+    call cons_from_prim(q)
+  end subroutine setup_orszag_tang
+
+! --------------------------------------------------------------- global I/O
+  subroutine read_state(fname)
+    character(len=*), intent(in) :: fname
+    integer :: un, ios
+    open(newunit=un, file=fname, access="stream", form="unformatted", &
+         status="old", iostat=ios)
+    if (ios /= 0) then
+      write(error_unit, "(3a)") "cannot read state '", fname, "'"
+      stop 4
+    end if
+    read(un) ug
+    close(un)
+  end subroutine read_state
+
+  subroutine write_state(fname)
+    character(len=*), intent(in) :: fname
+    integer :: un
+    open(newunit=un, file=fname, access="stream", form="unformatted", &
+         status="replace")
+    write(un) ug
+    close(un)
+  end subroutine write_state
+
+! global grid <-> block interiors (ghost layers are filled on the device)
+  subroutine scatter()
+    integer :: i0, j0, k0
+    u = 0.0d0
+    do b = 1, nblk
+      i0 = coords(1,b) * nc
+      j0 = coords(2,b) * nc
+      k0 = coords(3,b) * kn
+      do p = 1, nv
+        u(p,ng+1:ng+nc,ng+1:ng+nc,merge(ng+1,1,cfg%ndims==3):merge(ng+nc,1,cfg%ndims==3),b) = &
+          ug(i0+1:i0+nc,j0+1:j0+nc,k0+1:k0+kn,p)
+      end do
+    end do
+  end subroutine scatter
+
+  subroutine gather()
+    integer :: i0, j0, k0
+    do b = 1, nblk
+      i0 = coords(1,b) * nc
+      j0 = coords(2,b) * nc
+      k0 = coords(3,b) * kn
+      do p = 1, nv
+        ug(i0+1:i0+nc,j0+1:j0+nc,k0+1:k0+kn,p) = &
+          u(p,ng+1:ng+nc,ng+1:ng+nc,merge(ng+1,1,cfg%ndims==3):merge(ng+nc,1,cfg%ndims==3),b)
+      end do
+    end do
+  end subroutine gather
+
+end program amun_host
