@@ -3,10 +3,11 @@
 // Pointwise equation-of-state routines, TVD limiters, the WENO5-Z cell
 // reconstruction and the HLL / HLLC / HLLD Riemann solvers, written for
 // register-resident state vectors (fully unrolled over the variable count).
-// Each routine follows the arithmetic of the reference routine cited next to
-// it; the branch structure is kept exactly, the expressions are evaluated in
-// the reference order (the compiler is still free to contract a*b+c into an
-// FMA, which is the only source of round-off differences against the CPU).
+// Each routine follows the reference routine cited next to it: the branch
+// structure and the order of the sums are kept exactly.  Round-off differs
+// from the CPU in two controlled ways: the compiler contracts a*b+c into
+// FMAs, and divisions by a shared divisor use a refined hardware reciprocal
+// (rcp below); WENO5-Z forms its normalised weights with fewer divisions.
 //
 // T is the arithmetic type of reconstruction and fluxes (double, or float in
 // the mixed-precision mode).

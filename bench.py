@@ -147,9 +147,16 @@ def main():
     step_ms = dev_ms / args.steps
     achieved = bpz * cells / (step_ms * 1e-3) / 1e9
     traffic = None
+    valu = None
     tf = os.path.join(ROOT, "profiles", f"traffic_c2_{args.size}_{args.precision}.json")
     if os.path.exists(tf):
-        traffic = json.load(open(tf)).get("hbm_bytes_per_step")
+        prof = json.load(open(tf))
+        traffic = prof.get("hbm_bytes_per_step")
+        if args.precision == "fp64" and prof.get("fp64_flop_per_step"):
+            ach = prof["fp64_flop_per_step"] / (step_ms * 1e-3) / 1e12
+            valu = {"achieved": ach, "peak": FP64_VEC_TFLOPS, "unit": "TFLOP/s",
+                    "frac": ach / FP64_VEC_TFLOPS, "dtype": "f64",
+                    "flop_per_zone_update": prof["fp64_flop_per_step"] / cells}
     value = cells * args.steps / wall
     out = {
         "metric": "zone-updates/s (3D MHD WENO5+HLLD 256^3) FP64 vs mixed, % HBM roofline",
@@ -175,6 +182,8 @@ def main():
                      "device_ms_per_step": step_ms,
                      "kernel_ms_per_step": [round(float(x), 4) for x in kms]},
     }
+    if valu is not None:
+        out["valu"] = valu
     if mixed is not None:
         out["mixed"] = mixed
     if not args.no_cpu_baseline:
