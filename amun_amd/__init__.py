@@ -6,4 +6,4 @@ The compute path is the HIP library ``libamun_hip.so`` built from
 boundary: the parameter mapping (``config``), seeded initial states
 (``problems``), and the ctypes binding of the C-ABI (``solver``).
 """
-from .config import Params, ConfigError, c1_params, c2_params  # noqa: F401
+from .config import Params, ConfigError, c1_params, c2_params, c4_params  # noqa: F401

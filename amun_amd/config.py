@@ -16,7 +16,8 @@ import math
 from dataclasses import dataclass, field, fields
 
 # constants mirrored from include/amun_dev.h
-EQSYS = {"hd": 1, "hydrodynamic": 1, "mhd": 2, "magnetohydrodynamic": 2}
+EQSYS = {"hd": 1, "hydrodynamic": 1, "mhd": 2, "magnetohydrodynamic": 2,
+         "srmhd": 4}
 EOS = {"adi": 1, "adiabatic": 1}
 RECON = {"tvd": 1, "weno5z": 5, "weno5-z": 5}
 LIMITER = {"zero": 0, "mm": 1, "minmod": 1, "mc": 2, "monotonized_central": 2,
@@ -44,6 +45,8 @@ class AmunCfg(ctypes.Structure):
         ("xmin", ctypes.c_double), ("xmax", ctypes.c_double),
         ("ymin", ctypes.c_double), ("ymax", ctypes.c_double),
         ("zmin", ctypes.c_double), ("zmax", ctypes.c_double),
+        ("tolerance", ctypes.c_double), ("pmin", ctypes.c_double),
+        ("nr_maxit", ctypes.c_int32), ("nr_extra", ctypes.c_int32),
     ]
 
 
@@ -91,6 +94,10 @@ class Params:
     ymax: float = 1.0
     zmin: float = 0.0
     zmax: float = 1.0
+    tolerance: float = 1.0e-10
+    pmin: float = 1.0e-16
+    nr_maxit: int = 100
+    nr_extra: int = 2
     extra: dict = field(default_factory=dict)
 
     # ---------------------------------------------------------------- derived
@@ -141,6 +148,8 @@ class Params:
         c.gamma, c.cfl, c.eps, c.alpha = self.gamma, self.cfl, self.eps, self.alpha
         c.xmin, c.xmax, c.ymin, c.ymax = self.xmin, self.xmax, self.ymin, self.ymax
         c.zmin, c.zmax = self.zmin, self.zmax
+        c.tolerance, c.pmin = self.tolerance, self.pmin
+        c.nr_maxit, c.nr_extra = int(self.nr_maxit), int(self.nr_extra)
         return c
 
     def dx(self):
@@ -152,7 +161,7 @@ class Params:
 
     def to_params_in(self) -> str:
         """Text of an AMUN params.in with these settings (periodic, one level)."""
-        eq = {1: "hd", 2: "mhd"}[EQSYS[self.equation_system.lower()]]
+        eq = {1: "hd", 2: "mhd", 4: "srmhd"}[EQSYS[self.equation_system.lower()]]
         lines = [
             f"equation_system = {eq}", "equation_of_state = adi",
             f"reconstruction = {self.reconstruction}", f"limiter = {self.limiter}",
@@ -168,6 +177,8 @@ class Params:
             f"xmin = {self.xmin!r}", f"xmax = {self.xmax!r}",
             f"ymin = {self.ymin!r}", f"ymax = {self.ymax!r}",
             f"zmin = {self.zmin!r}", f"zmax = {self.zmax!r}",
+            f"tolerance = {self.tolerance!r}", f"pmin = {self.pmin!r}",
+            f"nr_maxit = {self.nr_maxit}", f"nr_extra = {self.nr_extra}",
             "minlev = 1", "maxlev = 1",
             "xlbndry = periodic", "xubndry = periodic",
             "ylbndry = periodic", "yubndry = periodic",
@@ -226,6 +237,20 @@ def c2_params(n: int = 256, precision: str = "fp64") -> Params:
                   precision=precision, gamma=5.0 / 3.0, cfl=0.3, ncells=16,
                   nghosts=4, xblocks=nb, yblocks=nb, zblocks=nb, xmin=-0.5,
                   xmax=0.5, ymin=-0.5, ymax=0.5, zmin=-0.5, zmax=0.5)
+
+
+def c4_params(n: int = 1024, precision: str = "fp64") -> Params:
+    """2D special-relativistic MHD blast, n^2 as (n/16)^2 blocks of 16^2
+    (+2 ghosts), TVD minmod + HLL + SSPRK(2,2), CFL 0.25, periodic, primitive
+    solver 1D(W) with tolerance 1e-12 and at most 50 Newton iterations.  The
+    adiabatic index is the reference default 5/3 (equations.F90:167)."""
+    nb = n // 16
+    return Params(ndims=2, equation_system="srmhd", reconstruction="tvd",
+                  limiter="mm", riemann_solver="hll", time_advance="rk2",
+                  precision=precision, gamma=5.0 / 3.0, cfl=0.25, ncells=16,
+                  nghosts=2, xblocks=nb, yblocks=nb, zblocks=1, xmin=-0.5,
+                  xmax=0.5, ymin=-0.5, ymax=0.5, tolerance=1.0e-12,
+                  nr_maxit=50)
 
 
 def psi_decay(p: Params) -> float:

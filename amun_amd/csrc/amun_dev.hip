@@ -35,6 +35,7 @@
 
 #include "../../include/amun_dev.h"
 #include "physics.h"
+#include "srmhd.h"
 
 using namespace amun;
 
@@ -66,6 +67,7 @@ struct DevState {
   double time;
   long long step;
   unsigned long long npos;          // non-positive density/pressure events
+  unsigned long long c2p[3];        // SR primitive solver: failed, p <= 0, no convergence
   double trace[kTrace];             // dt of every step (ring)
 };
 
@@ -76,6 +78,7 @@ struct RunConst {
   double gamma, gammam1, gammam1i;
   double cfl, dxmin, eps, decay;
   double dxi[3];
+  SrConst sc;                       // relativistic primitive solver
 };
 
 __device__ __forceinline__ double cmax_of(const DevState* st, int slot) {
@@ -163,9 +166,10 @@ enum { REC_TVD = AMUN_REC_TVD, REC_WENO5Z = AMUN_REC_WENO5Z };
 enum { RS_HLL = AMUN_RS_HLL, RS_HLLC = AMUN_RS_HLLC, RS_HLLD = AMUN_RS_HLLD };
 
 template <class P, int RS, typename T>
-__device__ __forceinline__ void riemann(const EosConst<T>& e, T cmax, T cmax2, T* ql,
-                                        T* qr, T* f) {
-  if constexpr (RS == RS_HLLD) riemann_hlld<P, T>(e, cmax, cmax2, ql, qr, f);
+__device__ __forceinline__ void riemann(const EosConst<T>& e, const SrConst& sc, T cmax,
+                                        T cmax2, T* ql, T* qr, T* f) {
+  if constexpr (P::sr) riemann_sr_hll<P, T>(sc, cmax, cmax2, ql, qr, f);
+  else if constexpr (RS == RS_HLLD) riemann_hlld<P, T>(e, cmax, cmax2, ql, qr, f);
   else if constexpr (RS == RS_HLLC) riemann_hllc<P, T>(e, cmax, cmax2, ql, qr, f);
   else riemann_hll<P, T>(e, cmax, cmax2, ql, qr, f);
 }
@@ -286,6 +290,26 @@ struct Recon<P, REC_TVD, ND, DIR, T> {
     }
   }
 };
+
+// Counts relativistic primitive-solver events (the reference prints each to
+// stderr and goes on).
+__device__ __forceinline__ void c2p_count(DevState* st, int cs) {
+  if (cs & C2P_FAIL) atomicAdd(&st->c2p[0], 1ull);
+  if (cs & C2P_PNEG) atomicAdd(&st->c2p[1], 1ull);
+  if (cs & C2P_NOCONV) atomicAdd(&st->c2p[2], 1ull);
+}
+
+// A cell whose bracketing failed keeps its previous primitive values (read
+// from Qold at offset base, variable stride ncell); p = 0 is already set.
+template <class P, typename T>
+__device__ __forceinline__ void c2p_keep_old(int cs, double* q, const T* __restrict__ Qold,
+                                             size_t base, int ncell) {
+  if (cs & C2P_FAIL) {
+#pragma unroll
+    for (int v = 0; v < P::nv; v++)
+      if (v != P::ipr) q[v] = double(Qold[base + (size_t)v * ncell]);
+  }
+}
 
 // ---------------------------------------------------------------------------
 // the sweep kernel
@@ -435,7 +459,7 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a, RunConst rc) {
           ql[p] = flp[rot<P, DIR>(p)];
           qr[p] = frc[rot<P, DIR>(p)];
         }
-        riemann<P, RS, T>(e, cmax, cmax2, ql, qr, fr);
+        riemann<P, RS, T>(e, rc.sc, cmax, cmax2, ql, qr, fr);
 #pragma unroll
         for (int p = 0; p < NV; p++) f[rot<P, DIR>(p)] = fr[p];
         if (c >= 1) {
@@ -492,7 +516,15 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a, RunConst rc) {
           }
           if constexpr (kLast) {
             double qn[NV];
-            cons2prim<P, double>(ed, unew, qn);
+            if constexpr (P::sr) {
+              const int cs = cons2prim_sr<P>(rc.sc, unew, qn);
+              if (cs) {
+                c2p_keep_old<P, T>(cs, qn, Qp, bt, rc.ncell);
+                c2p_count(a.st, cs);
+              }
+            } else {
+              cons2prim<P, double>(ed, unew, qn);
+            }
 #pragma unroll
             for (int v = 0; v < NV; v++) {
               const size_t off = (size_t)v * rc.ncell;
@@ -552,7 +584,15 @@ __global__ __launch_bounds__(256) void k_prim(const double* __restrict__ U,
     double u[NV], q[NV];
 #pragma unroll
     for (int v = 0; v < NV; v++) u[v] = U[(b * NV + v) * rc.ncell + cu];
-    cons2prim<P, double>(e, u, q);
+    if constexpr (P::sr) {
+      const int cs = cons2prim_sr<P>(rc.sc, u, q);
+      if (cs) {
+        c2p_keep_old<P, T>(cs, q, Q, (size_t)b * NV * rc.ncell + cq, rc.ncell);
+        c2p_count(st, cs);
+      }
+    } else {
+      cons2prim<P, double>(e, u, q);
+    }
 #pragma unroll
     for (int v = 0; v < NV; v++) Q[(b * NV + v) * rc.ncell + cq] = T(q[v]);
     smax = maxspeed<P, double>(e, q);
@@ -718,6 +758,9 @@ bool pick_t(Dispatch& d, const amun_cfg& c) {
   } else if (c.eqsys == AMUN_EQSYS_MHD) {
     if (c.riemann_solver == AMUN_RS_HLL) return pick_nd_rec<MHD, RS_HLL, T>(d, c.ndims, c.reconstruction);
     if (c.riemann_solver == AMUN_RS_HLLD) return pick_nd_rec<MHD, RS_HLLD, T>(d, c.ndims, c.reconstruction);
+  } else if (c.eqsys == AMUN_EQSYS_SRMHD) {
+    // every solver name but "hllc" selects HLL (schemes.F90:479-498)
+    return pick_nd_rec<SRMHD, RS_HLL, T>(d, c.ndims, c.reconstruction);
   }
   return false;
 }
@@ -729,16 +772,23 @@ bool pick(Dispatch& d, const amun_cfg& c) {
 int validate(const amun_cfg& c, int* nv) {
   if (c.ndims != 2 && c.ndims != 3) return fail(500, "ndims must be 2 or 3");
   if (c.eqsys == AMUN_EQSYS_HD) *nv = 5;
-  else if (c.eqsys == AMUN_EQSYS_MHD) *nv = 9;
+  else if (c.eqsys == AMUN_EQSYS_MHD || c.eqsys == AMUN_EQSYS_SRMHD) *nv = 9;
   else return fail(100, "unsupported equation_system");
   if (c.eos != AMUN_EOS_ADI) return fail(110, "unsupported equation_of_state");
   if (c.reconstruction != AMUN_REC_TVD && c.reconstruction != AMUN_REC_WENO5Z)
     return fail(200, "unsupported reconstruction");
   if (c.limiter < 0 || c.limiter > 5) return fail(200, "unsupported limiter");
-  bool rs_ok = c.riemann_solver == AMUN_RS_HLL ||
-               (c.riemann_solver == AMUN_RS_HLLC && c.eqsys == AMUN_EQSYS_HD) ||
-               (c.riemann_solver == AMUN_RS_HLLD && c.eqsys == AMUN_EQSYS_MHD);
-  if (!rs_ok) return fail(300, "riemann_solver does not match the equation system");
+  if (c.eqsys == AMUN_EQSYS_SRMHD) {
+    if (c.riemann_solver == AMUN_RS_HLLC)
+      return fail(600, "the relativistic HLLC solver is not supported on the device path");
+    if (!(c.tolerance > 0.0) || c.nr_maxit < 1 || c.nr_extra < 0 || !(c.pmin >= 0.0))
+      return fail(500, "tolerance, nr_maxit, nr_extra or pmin out of range");
+  } else {
+    bool rs_ok = c.riemann_solver == AMUN_RS_HLL ||
+                 (c.riemann_solver == AMUN_RS_HLLC && c.eqsys == AMUN_EQSYS_HD) ||
+                 (c.riemann_solver == AMUN_RS_HLLD && c.eqsys == AMUN_EQSYS_MHD);
+    if (!rs_ok) return fail(300, "riemann_solver does not match the equation system");
+  }
   if (c.time_advance < AMUN_TA_EULER || c.time_advance > AMUN_TA_RK3)
     return fail(400, "unsupported time_advance");
   if (c.precision != AMUN_PREC_FP64 && c.precision != AMUN_PREC_MIXED)
@@ -784,6 +834,10 @@ void amun_dev_default_cfg(amun_cfg* c) {
   c->alpha = 2.0;
   c->xmin = c->ymin = c->zmin = 0.0;
   c->xmax = c->ymax = c->zmax = 1.0;
+  c->tolerance = 1.0e-10;
+  c->pmin = 1.0e-16;
+  c->nr_maxit = 100;
+  c->nr_extra = 2;
 }
 
 int amun_dev_init(const amun_cfg* cfg, amun_ctx** out) {
@@ -814,6 +868,13 @@ int amun_dev_init(const amun_cfg* cfg, amun_ctx** out) {
   rc.cfl = cfg->cfl;
   rc.eps = cfg->eps;
   rc.decay = std::exp(-cfg->alpha * cfg->cfl);
+  rc.sc.gamma = cfg->gamma;
+  rc.sc.gammaxi = rc.gammam1 / cfg->gamma;       // equations.F90:843
+  rc.sc.rgammaxi = 1.0 / rc.sc.gammaxi;
+  rc.sc.tol = cfg->tolerance;
+  rc.sc.pmin = cfg->pmin;
+  rc.sc.nrmax = cfg->nr_maxit;
+  rc.sc.nrext = cfg->nr_extra;
   // cell sizes of coordinates.F90:330-348 (one level)
   double adx = (cfg->xmax - cfg->xmin) / (ir * nc);
   double ady = (cfg->ymax - cfg->ymin) / (jr * nc);
@@ -882,6 +943,10 @@ int amun_dev_init(const amun_cfg* cfg, amun_ctx** out) {
     e = hipMemcpy(c->st, &h, sizeof(DevState), hipMemcpyHostToDevice);
   }
   if (e == hipSuccess) e = hipMemset(c->U0, 0, bytes);
+  // primitive arrays start at zero: a relativistic cell whose recovery fails
+  // keeps the previous primitive values, as the reference's q array does
+  if (e == hipSuccess) e = hipMemset(c->Q[0], 0, qbytes);
+  if (e == hipSuccess) e = hipMemset(c->Q[1], 0, qbytes);
   if (e != hipSuccess) {
     amun_dev_finalize(c);
     return fail(1000 + (int)e, "initial copies");
@@ -913,11 +978,12 @@ void amun_dev_finalize(amun_ctx* c) {
 static int after_upload(amun_ctx* c) {
   Dispatch& d = dispatch_of(c);
   const int slot = (int)(c->nsteps & 1);
-  c->qcur = 0;
   hipLaunchKernelGGL(k_reset_slot, dim3(1), dim3(1), 0, c->stream, c->st, slot);
   long long n = (long long)c->rc.nblk * c->rc.ncell;
+  // in place over the current primitive array (update_primitive_variables
+  // overwrites pdata%q)
   hipLaunchKernelGGL(d.prim, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
-                     c->U0, c->Q[0], c->rc, c->st, slot);
+                     c->U0, c->Q[c->qcur], c->rc, c->st, slot);
   if (c->C)
     hipLaunchKernelGGL(k_zero_f32, dim3(2048), dim3(256), 0, c->stream, c->C,
                        (long long)c->nstate);
@@ -1235,6 +1301,17 @@ int amun_dev_positivity_events(amun_ctx* c, uint64_t* n) {
   DevState h;
   HIP_TRY(hipMemcpy(&h, c->st, offsetof(DevState, trace), hipMemcpyDeviceToHost));
   *n = h.npos;
+  return 0;
+}
+
+int amun_dev_c2p_events(amun_ctx* c, uint64_t n[3]) {
+  if (!c || !n) return fail(500, "null argument");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  DevState h;
+  HIP_TRY(hipMemcpy(&h, c->st, offsetof(DevState, trace), hipMemcpyDeviceToHost));
+  n[0] = h.c2p[0];
+  n[1] = h.c2p[1];
+  n[2] = h.c2p[2];
   return 0;
 }
 

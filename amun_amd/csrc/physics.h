@@ -17,7 +17,8 @@
 
 namespace amun {
 
-// variable layout of the two equation systems (equations.F90:335-485)
+// variable layout of the Newtonian equation systems (equations.F90:335-485);
+// the relativistic one is in srmhd.h
 struct HD {
   static constexpr int nv = 5;
   static constexpr int idn = 0, ivx = 1, ivy = 2, ivz = 3;
@@ -25,6 +26,7 @@ struct HD {
   static constexpr int ipr = 4, ien = 4;
   static constexpr int ibx = -1, iby = -1, ibz = -1, ibp = -1;
   static constexpr bool mhd = false;
+  static constexpr bool sr = false;
 };
 
 struct MHD {
@@ -34,6 +36,7 @@ struct MHD {
   static constexpr int ibx = 4, iby = 5, ibz = 6, ibp = 7;
   static constexpr int ipr = 8, ien = 8;
   static constexpr bool mhd = true;
+  static constexpr bool sr = false;
 };
 
 // constants that every kernel needs, passed by value
@@ -207,9 +210,11 @@ __device__ __forceinline__ void cons2prim(const EosConst<T>& e, const T* u, T* q
   }
 }
 
-// maxspeed_hd_adi (equations.F90:2022-2035) / maxspeed_mhd_adi (3108-3122)
+// maxspeed_hd_adi (equations.F90:2022-2035) / maxspeed_mhd_adi (3108-3122);
+// maxspeed_srmhd_adi (5116-5163) is the speed of light
 template <class P, typename T>
 __device__ __forceinline__ T maxspeed(const EosConst<T>& e, const T* q) {
+  if constexpr (P::sr) return T(1);
   T vv = (q[P::ivx] * q[P::ivx] + q[P::ivy] * q[P::ivy]) + q[P::ivz] * q[P::ivz];
   T v = tsqrt(vv);
   T c;

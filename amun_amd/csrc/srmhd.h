@@ -1,0 +1,679 @@
+// srmhd.h -- special-relativistic adiabatic MHD for the gfx950 kernels.
+//
+// The equation system of equations.F90:673-760 (nv = 9, GLM psi, c = 1):
+// prim2cons_srmhd_adi (equations.F90:4590-4653), the Mignone & Bodo (2006)
+// fluxes and characteristic speeds of fluxspeed_srmhd_adi (4821-5101), the
+// 1D(W) primitive solver of Noble et al. (2006) with the brackets of
+// nr_initial_brackets_srmhd_adi and the iteration of nr_iterate_srmhd_adi_1dw
+// (5192-5788), and the polynomial root finders it relies on (algebra.F90:
+// 86-1085).  Branches and the order of the operations follow the reference;
+// the compiler contracts a*b+c into FMAs, a**(1/3) is cbrt and divisions by
+// a shared divisor use the refined reciprocal, which is the whole of the
+// rounding difference.
+//
+// Fluxes run in T (double, or float in the mixed mode); the characteristic
+// speeds always run in double, because the quartic of the fast and slow
+// speeds loses about half of its digits near multiple roots, and the
+// primitive recovery (a Newton iteration to a 1e-10..1e-12 tolerance) is
+// double by construction.
+#pragma once
+
+#include "physics.h"
+
+namespace amun {
+
+struct SRMHD {
+  static constexpr int nv = 9;
+  static constexpr int idn = 0, ivx = 1, ivy = 2, ivz = 3;
+  static constexpr int imx = 1, imy = 2, imz = 3;
+  static constexpr int ibx = 4, iby = 5, ibz = 6, ibp = 7;
+  static constexpr int ipr = 8, ien = 8;
+  static constexpr bool mhd = true;
+  static constexpr bool sr = true;
+};
+
+// constants of the primitive solver ("tolerance", "nr_maxit", "nr_extra",
+// "pmin"; equations.F90:184, 196-201)
+struct SrConst {
+  double gamma, gammaxi, rgammaxi;   // gammaxi = (gamma-1)/gamma
+  double tol, pmin;
+  int nrmax, nrext;
+};
+
+// status bits of one primitive recovery
+enum { C2P_FAIL = 1, C2P_PNEG = 2, C2P_NOCONV = 4 };
+
+// ---------------------------------------------------------------------------
+// polynomial roots, algebra.F90:86-1085 (coefficients in increasing powers)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double sgn1(double b) { return copysign(1.0, b); }
+
+// sort (algebra.F90:1040-1085): a stable ascending sort of x[0..n-1].  The
+// reference's insertion sort is written here as a bubble sort over static
+// indices (both are stable, so the result is the same) so that x stays in
+// registers.
+template <int N>
+__device__ __forceinline__ void alg_sort(int n, double* x) {
+#pragma unroll
+  for (int i = 0; i < N - 1; i++) {
+#pragma unroll
+    for (int j = N - 1; j > i; j--) {
+      if (j < n && x[j - 1] > x[j]) {
+        const double t = x[j - 1];
+        x[j - 1] = x[j];
+        x[j] = t;
+      }
+    }
+  }
+}
+
+// x[k] of a root array for a run-time k < N, without dynamic indexing
+template <int N>
+__device__ __forceinline__ double pick(const double* x, int k) {
+  double r = x[0];
+#pragma unroll
+  for (int i = 1; i < N; i++) r = (k == i) ? x[i] : r;
+  return r;
+}
+
+// quadratic, algebra.F90:86-242
+__device__ __forceinline__ int alg_quadratic(const double* a, double* x) {
+  const double mx = fmax(fmax(fabs(a[0]), fabs(a[1])), fabs(a[2]));
+  const double b0 = a[0] / mx, b1 = a[1] / mx, b2 = a[2] / mx;
+  if (b2 != 0.0) {
+    if (b0 != 0.0) {
+      const double bh = 0.5 * b1;
+      const double dl = bh * bh - b2 * b0;
+      if (dl > 0.0) {
+        const double dr = sqrt(dl);
+        if (b1 > 0.0) {
+          const double tm = -bh - dr;
+          x[0] = tm / b2;
+          x[1] = b0 / tm;
+        } else if (b1 < 0.0) {
+          const double tm = -bh + dr;
+          x[0] = b0 / tm;
+          x[1] = tm / b2;
+        } else {
+          x[1] = dr / b2;
+          x[0] = -x[1];
+        }
+        return 2;
+      } else if (dl == 0.0) {
+        x[0] = -0.5 * b1 / b2;
+        x[1] = x[0];
+        return 2;
+      }
+      x[0] = 0.0;
+      x[1] = 0.0;
+      return 0;
+    }
+    const double tm = -b1 / b2;
+    if (tm >= 0.0) { x[0] = 0.0; x[1] = tm; }
+    else           { x[0] = tm;  x[1] = 0.0; }
+    return 2;
+  }
+  x[1] = 0.0;
+  if (b1 != 0.0) { x[0] = -b0 / b1; return 1; }
+  x[0] = 0.0;
+  return 0;
+}
+
+// quadratic_normalized, algebra.F90:262-372
+__device__ __forceinline__ int alg_quadratic_normalized(const double* a, double* x) {
+  if (a[0] != 0.0) {
+    const double bh = 0.5 * a[1];
+    const double dl = bh * bh - a[0];
+    if (dl > 0.0) {
+      const double dr = sqrt(dl);
+      if (a[1] > 0.0) {
+        const double tm = bh + dr;
+        x[0] = -a[0] / tm;
+        x[1] = -tm;
+      } else if (a[1] < 0.0) {
+        const double tm = bh - dr;
+        x[0] = -tm;
+        x[1] = -a[0] / tm;
+      } else {
+        x[0] = dr;
+        x[1] = -dr;
+      }
+      return 2;
+    } else if (dl == 0.0) {
+      x[0] = -bh;
+      x[1] = -bh;
+      return 2;
+    }
+    x[0] = 0.0;
+    x[1] = 0.0;
+    return 0;
+  }
+  x[0] = 0.0;
+  x[1] = -a[1];
+  return 2;
+}
+
+constexpr double kPi2 = 6.28318530717958647692528676655900;   // constants.F90:40
+constexpr double kC0 = 1.0 / 3.0;
+
+// cubic, algebra.F90:398-614.  The three-real-root branch takes
+// sqrt(qc) = sqrt(bc)/(3|a3|), as the reference does, which is right for the
+// positive leading coefficient of its only caller (quartic: b3 = 8).
+__device__ __forceinline__ int alg_cubic(const double* a, double* x) {
+  int nr;
+  if (a[3] != 0.0) {
+    if (a[0] != 0.0) {
+      double ac = (2.0 * (a[2] * a[2] * a[2]) - 9.0 * a[3] * a[2] * a[1]) +
+                  27.0 * (a[3] * a[3]) * a[0];
+      double bc = a[2] * a[2] - 3.0 * a[3] * a[1];
+      const double a2 = ac * ac;
+      const double b3 = bc * bc * bc;
+      double qc = a2 - 4.0 * b3;
+      if (qc > 0.0) {
+        const double cc = 0.5 * (sqrt(qc) + ac);
+        const double dc = cbrt(cc);
+        double ec;
+        if (cc == 0.0) {
+          ec = b3 / a2;
+          ec = 2.0 * ec * (1.0 / sqrt(1.0 - 4.0 * ec) + 1.0 / ac);
+        } else {
+          ec = b3 / cc;
+        }
+        ec = cbrt(ec);
+        x[0] = -(kC0 * ((a[2] + dc) + ec)) / a[3];
+        x[1] = 0.0;
+        x[2] = 0.0;
+        nr = 1;
+      } else if (qc == 0.0) {
+        if (bc != 0.0) {
+          x[0] = 0.5 * (9.0 * a[3] * a[0] - a[2] * a[1]) / bc;
+          x[1] = x[0];
+          x[2] = ((-(9.0 * a[3] * a[0]) + 4.0 * a[2] * a[1]) - (a[2] * a[2] * a[2]) / a[3]) / bc;
+        } else {
+          x[0] = x[1] = x[2] = -(kC0 * a[2]) / a[3];
+        }
+        nr = 3;
+      } else {
+        double th = 3.0 * a[3];
+        qc = bc / (th * th);
+        const double rc = 0.5 * ac / (th * th * th);
+        const double fb = fabs(bc);
+        const double cs = copysign(0.5, ac * bc) * sqrt((ac * ac) / (fb * fb * fb));
+        if (cs <= 1.0) {
+          th = acos(cs);
+          ac = -2.0 * sqrt(qc);
+          bc = -(kC0 * (a[2] / a[3]));
+          x[0] = ac * cos(kC0 * th) + bc;
+          x[1] = ac * cos(kC0 * (th + kPi2)) + bc;
+          x[2] = ac * cos(kC0 * (th - kPi2)) + bc;
+          nr = 3;
+        } else {
+          ac = -cbrt(copysign(fabs(rc) + sqrt(rc * rc - qc * qc * qc), rc));
+          bc = (ac != 0.0) ? qc / ac : 0.0;
+          x[0] = (ac + bc) - kC0 * (a[2] / a[3]);
+          x[1] = 0.0;
+          x[2] = 0.0;
+          nr = 1;
+        }
+      }
+    } else {
+      x[2] = 0.0;
+      nr = alg_quadratic(a + 1, x) + 1;
+    }
+  } else {
+    x[2] = 0.0;
+    nr = alg_quadratic(a, x);
+  }
+  alg_sort<3>(nr, x);
+  return nr;
+}
+
+// cubic_normalized, algebra.F90:636-796
+__device__ __forceinline__ int alg_cubic_normalized(const double* a, double* x) {
+  constexpr double c1 = 1.0 / 2.7e+01, c2 = 1.0 / 4.0;
+  int nr;
+  if (a[0] != 0.0) {
+    const double ac = kC0 * (a[2] * a[2] - 3.0 * a[1]);
+    const double bc = c1 * ((2.0 * (a[2] * a[2] * a[2]) - 9.0 * a[2] * a[1]) + 2.7e+01 * a[0]);
+    const double a3 = c1 * (ac * ac * ac);
+    const double b2 = c2 * (bc * bc);
+    const double qc = b2 - a3;
+    const double sh = kC0 * a[2];
+    if (qc > 0.0) {
+      const double sq = sqrt(qc);
+      const double aa = cbrt(-0.5 * bc + sq);
+      const double bb = cbrt(-0.5 * bc - sq);
+      x[0] = (aa + bb) - sh;
+      x[1] = 0.0;
+      x[2] = 0.0;
+      nr = 1;
+    } else if (qc < 0.0) {
+      double cs;
+      if (bc > 0.0)      cs = -sqrt(b2 / a3);
+      else if (bc < 0.0) cs = sqrt(b2 / a3);
+      else               cs = 0.0;
+      const double ph = acos(cs);
+      const double aa = 2.0 * sqrt(kC0 * ac);
+      x[0] = aa * cos(kC0 * (ph - kPi2)) - sh;
+      x[1] = aa * cos(kC0 * ph) - sh;
+      x[2] = aa * cos(kC0 * (ph + kPi2)) - sh;
+      nr = 3;
+    } else {
+      if (bc > 0.0) {
+        const double aa = sqrt(kC0 * ac);
+        x[0] = -2.0 * aa; x[1] = aa; x[2] = aa;
+      } else if (bc < 0.0) {
+        const double aa = sqrt(kC0 * ac);
+        x[0] = 2.0 * aa; x[1] = -aa; x[2] = -aa;
+      } else {
+        x[0] = x[1] = x[2] = 0.0;
+      }
+      x[0] -= sh; x[1] -= sh; x[2] -= sh;
+      nr = 3;
+    }
+  } else {
+    x[2] = 0.0;
+    nr = alg_quadratic_normalized(a + 1, x) + 1;
+  }
+  alg_sort<3>(nr, x);
+  return nr;
+}
+
+// quartic, algebra.F90:820-1017: Hacke's method through the resolvent cubic,
+// taking its smallest root z(1)
+__device__ __forceinline__ int alg_quartic(const double* a, double* x) {
+  int nr;
+  if (a[4] != 0.0) {
+    if (a[0] != 0.0) {
+      const double a42 = a[3] * a[3], a52 = a[4] * a[4], a53 = a52 * a[4];
+      const double pc = (-(3.0 * a42) + 8.0 * a[4] * a[2]) / (8.0 * a52);
+      const double qc = ((a42 * a[3] - 4.0 * a[4] * a[3] * a[2]) + 8.0 * a52 * a[1]) / (8.0 * a53);
+      const double rc = (((-(3.0 * (a42 * a[3] * a[3])) + 16.0 * a[4] * a42 * a[2]) -
+                          64.0 * a52 * a[3] * a[1]) + 256.0 * a53 * a[0]) /
+                        (256.0 * (a53 * a[4]));
+      double b[4], z[3];
+      b[3] = 8.0;
+      b[2] = -4.0 * pc;
+      b[1] = -8.0 * rc;
+      b[0] = 4.0 * pc * rc - qc * qc;
+      nr = alg_cubic(b, z);
+      x[0] = x[1] = x[2] = x[3] = 0.0;
+      if (nr > 0) {
+        const double zm = z[0];
+        const double k2 = 2.0 * zm - pc;
+        const double l2 = zm * zm - rc;
+        if (k2 > 0.0 || l2 > 0.0) {
+          double kc, lc;
+          if (k2 > l2) {
+            kc = sqrt(k2);
+            lc = 0.5 * qc / kc;
+          } else if (l2 > k2) {
+            lc = sqrt(l2);
+            kc = 0.5 * qc / lc;
+          } else {
+            kc = sqrt(k2);
+            lc = sqrt(l2);
+          }
+          const double sh = 0.25 * a[3] / a[4];
+          double c[3], y[2], y2[2];
+          c[2] = 1.0; c[1] = -kc; c[0] = zm + lc;
+          const int m1 = alg_quadratic(c, y);
+          c[2] = 1.0; c[1] = kc; c[0] = zm - lc;
+          const int m2 = alg_quadratic(c, y2);
+          // roots of the first factor, then those of the second, then zeros
+          const double r2 = (m2 > 0) ? y2[0] - sh : 0.0;
+          const double r3 = (m2 > 1) ? y2[1] - sh : 0.0;
+          if (m1 == 2) {
+            x[0] = y[0] - sh; x[1] = y[1] - sh; x[2] = r2; x[3] = r3;
+          } else if (m1 == 1) {
+            x[0] = y[0] - sh; x[1] = r2; x[2] = r3;
+          } else {
+            x[0] = r2; x[1] = r3;
+          }
+          nr = m1 + m2;
+        } else {
+          nr = 0;
+        }
+      }
+    } else {
+      x[3] = 0.0;
+      nr = alg_cubic(a + 1, x) + 1;
+    }
+  } else {
+    x[3] = 0.0;
+    nr = alg_cubic(a, x);
+  }
+  alg_sort<4>(nr, x);
+  return nr;
+}
+
+// ---------------------------------------------------------------------------
+// equation of state
+// ---------------------------------------------------------------------------
+// prim2cons_srmhd_adi, equations.F90:4590-4653
+template <class P, typename T>
+__device__ __forceinline__ void prim2cons_sr(T rgammaxi, const T* q, T* u) {
+  const T vv = (q[P::ivx] * q[P::ivx] + q[P::ivy] * q[P::ivy]) + q[P::ivz] * q[P::ivz];
+  const T bb = (q[P::ibx] * q[P::ibx] + q[P::iby] * q[P::iby]) + q[P::ibz] * q[P::ibz];
+  const T vb = (q[P::ivx] * q[P::ibx] + q[P::ivy] * q[P::iby]) + q[P::ivz] * q[P::ibz];
+  const T vm = T(1) - vv;
+  const T vs = tsqrt(vm);
+  const T ww = (q[P::idn] + q[P::ipr] * rgammaxi) * rcp(vm);
+  const T wt = ww + bb;
+  u[P::idn] = q[P::idn] * rcp(vs);
+  u[P::imx] = wt * q[P::ivx] - vb * q[P::ibx];
+  u[P::imy] = wt * q[P::ivy] - vb * q[P::iby];
+  u[P::imz] = wt * q[P::ivz] - vb * q[P::ibz];
+  u[P::ibx] = q[P::ibx];
+  u[P::iby] = q[P::iby];
+  u[P::ibz] = q[P::ibz];
+  u[P::ibp] = q[P::ibp];
+  u[P::ien] = ((wt - q[P::ipr]) - u[P::idn]) - T(0.5) * (vm * bb + vb * vb);
+}
+
+// characteristic speeds of fluxspeed_srmhd_adi (equations.F90:4895-5083), in
+// double: the quartic (Bx != 0), the quadratic (Bx = 0) or the bi-quadratic
+// (v = 0) of the fast/slow magnetosonic waves; -1/+1 when fewer than two real
+// roots come out
+__device__ __forceinline__ void sr_speeds(const SrConst& s, double rho, double vx,
+                                          double vv, double vm, double vb, double b2,
+                                          double bxn, double pr, double& cm, double& cp) {
+  double a[5], x[4] = {0.0, 0.0, 0.0, 0.0};
+  int nr;
+  const double rh = rho + pr * s.rgammaxi;
+  const double vs = sqrt(vm);
+  const double c2 = s.gamma * pr / rh;
+  if (vv > 0.0) {
+    if (bxn != 0.0) {
+      const double v1 = fabs(vx);
+      const double v2 = v1 * v1;
+      const double fa = rh * (1.0 - c2);
+      const double fb = c2 * (rh - vb * vb) + b2;
+      const double fc = sgn1(vx) * bxn * vs;
+      const double fd = c2 * fc * fc;
+      const double fe = 1.0 - v2;
+      const double ff = c2 * vb * fc;
+      const double fg = v1 * vs;
+      a[4] = fa + fb * vm;
+      a[3] = 2.0 * (ff * vm + fb * fg);
+      a[2] = (-(fd * vm) + 4.0 * ff * fg) - fb * fe;
+      a[1] = -(2.0 * (fd * fg + fe * ff));
+      a[0] = fd * fe;
+      nr = alg_quartic(a, x);
+      const double sv = sgn1(vx);
+#pragma unroll
+      for (int n = 0; n < 4; n++)
+        if (n < nr) x[n] = sv * (v1 + x[n] * vs);
+    } else {
+      const double cc = (1.0 - c2) / vm;
+      const double gn = b2 - c2 * vb * vb;
+      a[2] = rh * (c2 + cc) + gn;
+      a[1] = -(2.0 * rh * cc * vx);
+      a[0] = rh * (cc * (vx * vx) - c2) - gn;
+      nr = alg_quadratic(a, x);
+    }
+  } else {
+    const double rt = rh + b2;
+    double ca = bxn * vs + vb * vx / vs;
+    ca = ca * ca;
+    a[2] = 1.0;
+    a[1] = -(((rh + ca) * c2 + b2) / rt);
+    a[0] = c2 * ca / rt;
+    nr = alg_quadratic(a, x);
+    if (nr > 0) {
+      double zm = fmin(x[0], x[1]);
+      double zp = fmax(x[0], x[1]);
+      if (zm >= 0.0) {
+        zm = sqrt(zm);
+        zp = sqrt(zp);
+        x[0] = zp; x[1] = zm; x[2] = -zm; x[3] = -zp;
+        nr = 4;
+      } else if (zp >= 0.0) {
+        zp = sqrt(zp);
+        x[0] = zp; x[1] = -zp;
+        nr = 2;
+      } else {
+        nr = 0;
+      }
+    }
+  }
+  if (nr > 1) {
+    double mn = x[0], mx = x[0];
+#pragma unroll
+    for (int n = 1; n < 4; n++)
+      if (n < nr) { mn = fmin(mn, x[n]); mx = fmax(mx, x[n]); }
+    cm = mn;
+    cp = mx;
+  } else {
+    cm = -1.0;
+    cp = 1.0;
+  }
+}
+
+// fluxspeed_srmhd_adi, equations.F90:4821-5101: fluxes (eq. 13 of Mignone &
+// Bodo 2006) in T, speeds in double
+template <class P, typename T>
+__device__ __forceinline__ void fluxspeed_sr(const SrConst& s, T cmax2, const T* q, const T* u,
+                                             T* f, T& cm, T& cp) {
+  const T vv = (q[P::ivx] * q[P::ivx] + q[P::ivy] * q[P::ivy]) + q[P::ivz] * q[P::ivz];
+  const T bb = (q[P::ibx] * q[P::ibx] + q[P::iby] * q[P::iby]) + q[P::ibz] * q[P::ibz];
+  const T vb = (q[P::ivx] * q[P::ibx] + q[P::ivy] * q[P::iby]) + q[P::ivz] * q[P::ibz];
+  const T vm = T(1) - vv;
+  const T bx = q[P::ibx] * vm + vb * q[P::ivx];
+  const T by = q[P::iby] * vm + vb * q[P::ivy];
+  const T bz = q[P::ibz] * vm + vb * q[P::ivz];
+  const T b2 = bb * vm + vb * vb;
+  const T pt = q[P::ipr] + T(0.5) * b2;
+  f[P::idn] = u[P::idn] * q[P::ivx];
+  f[P::imx] = (u[P::imx] * q[P::ivx] - q[P::ibx] * bx) + pt;
+  f[P::imy] = u[P::imy] * q[P::ivx] - q[P::ibx] * by;
+  f[P::imz] = u[P::imz] * q[P::ivx] - q[P::ibx] * bz;
+  f[P::ibx] = q[P::ibp];
+  f[P::ibp] = cmax2 * q[P::ibx];
+  f[P::iby] = q[P::ivx] * q[P::iby] - q[P::ibx] * q[P::ivy];
+  f[P::ibz] = q[P::ivx] * q[P::ibz] - q[P::ibx] * q[P::ivz];
+  f[P::ien] = u[P::imx] - f[P::idn];
+  double dcm, dcp;
+  if constexpr (sizeof(T) == 8) {
+    sr_speeds(s, q[P::idn], q[P::ivx], vv, vm, vb, b2, q[P::ibx], q[P::ipr], dcm, dcp);
+  } else {
+    // the speed polynomial from the state promoted to double
+    const double dvx = q[P::ivx], dvy = q[P::ivy], dvz = q[P::ivz];
+    const double dbx = q[P::ibx], dby = q[P::iby], dbz = q[P::ibz];
+    const double dvv = (dvx * dvx + dvy * dvy) + dvz * dvz;
+    const double dbb = (dbx * dbx + dby * dby) + dbz * dbz;
+    const double dvb = (dvx * dbx + dvy * dby) + dvz * dbz;
+    const double dvm = 1.0 - dvv;
+    sr_speeds(s, q[P::idn], dvx, dvv, dvm, dvb, dbb * dvm + dvb * dvb, dbx, q[P::ipr], dcm, dcp);
+  }
+  cm = T(dcm);
+  cp = T(dcp);
+}
+
+// ---------------------------------------------------------------------------
+// primitive recovery, 1D(W) (equations.F90:5192-5788)
+// ---------------------------------------------------------------------------
+// nr_function_srmhd_adi_1d: F(W) and dF/dW
+__device__ __forceinline__ void nr_function(const SrConst& s, double mm, double bb, double mb,
+                                            double en, double dn, double w, double& f,
+                                            double* df) {
+  const double ww = w * w;
+  const double wt = w + bb;
+  const double wd = wt * wt;
+  const double ss = mb * mb;
+  const double sw = ss / ww;
+  const double ws = (w + wt) * sw;
+  const double fn = (mm * bb - ss) / wd;
+  const double dvv = (wd - mm) - ws;
+  const double ds = sqrt(dvv);
+  const double gm2 = wd / dvv;
+  const double gm = wt / ds;
+  const double pr = s.gammaxi * (w - gm * dn) / gm2;
+  f = ((w - pr) - en) + 0.5 * (bb + fn);
+  if (df) {
+    const double dg = (1.0 - wt * ((wt - sw) + ws / w) / dvv) / ds;
+    const double dp = s.gammaxi * (1.0 - (2.0 * w / gm - dn) * dg) / gm2;
+    *df = (1.0 - dp) - fn / wt;
+  }
+}
+
+// nr_initial_brackets_srmhd_adi (5459-5635) + nr_iterate_srmhd_adi_1dw
+// (5665-5788).  Returns the status bits; on success w and vv hold W and v^2.
+__device__ __forceinline__ int nr_iterate_1dw(const SrConst& s, double mm, double bb,
+                                              double mb, double en, double dn, double& w,
+                                              double& vv) {
+  const double dd = dn * dn;
+  const double ss = mb * mb;
+  const double ec = en + s.pmin;
+  double wl, wu = en + s.pmin, fl, fu;
+  // lower bracket: the largest roots of the cubic (pressure) and quartic
+  // (1 - v^2) positivity conditions
+  {
+    double a[5], x[4];
+    a[2] = 2.5 * bb - ec;
+    a[1] = 2.0 * (bb - ec) * bb;
+    a[0] = 0.5 * (((bb - 2.0 * ec) * bb + mm) * bb - ss);
+    int nr = alg_cubic_normalized(a, x);
+    if (nr > 0) {
+      wl = pick<3>(x, nr - 1);
+      a[4] = 1.0;
+      a[3] = 2.0 * bb;
+      a[2] = (bb * bb - dd) - mm;
+      a[1] = -(2.0 * (ss + dd * bb));
+      a[0] = -(ss + dd * bb) * bb;
+      nr = alg_quartic(a, x);
+      if (nr > 0) wl = fmax(wl, pick<4>(x, nr - 1));
+    } else {
+      // Newton on the pressure from the upper bound (never reached: the
+      // normalised cubic always returns a root)
+      int it = s.nrmax;
+      bool keep = true;
+      wl = wu;
+      while (keep) {
+        const double wt = wl + bb, fn = (mm * bb - ss) / (wt * wt);
+        const double f = (wl - ec) + 0.5 * (bb + fn), df = 1.0 - fn / wt;
+        const double dw = f / df;
+        wl = wl - dw;
+        const double err = fabs(dw / wl);
+        it = it - 1;
+        keep = (err > s.tol) && it > 0;
+      }
+    }
+  }
+  nr_function(s, mm, bb, mb, en, dn, wl, fl, nullptr);
+  if (!(fl <= 0.0)) return C2P_FAIL;
+  if (wu <= wl) wu = 2.0 * wl;
+  nr_function(s, mm, bb, mb, en, dn, wu, fu, nullptr);
+  {
+    int it = s.nrmax;
+    bool keep = fl * fu > 0.0;
+    while (keep) {
+      it = it - 1;
+      wl = wu;
+      fl = fu;
+      wu = 2.0 * wu;
+      nr_function(s, mm, bb, mb, en, dn, wu, fu, nullptr);
+      keep = (fl * fu > 0.0) && it > 0;
+    }
+    if (!(it > 0 && fu >= 0.0)) return C2P_FAIL;
+  }
+  w = wl - fl * (wu - wl) / (fu - fl);
+  // Newton-Raphson with bracket updates and bisection when outside
+  int it = s.nrmax, cn = s.nrext;
+  bool keep = true;
+  double err = 0.0;
+  while (keep) {
+    double f, df;
+    nr_function(s, mm, bb, mb, en, dn, w, f, &df);
+    if (f > fl && f < 0.0) { wl = w; fl = f; }
+    if (f < fu && f > 0.0) { wu = w; fu = f; }
+    const double dw = f / df;
+    w = w - dw;
+    err = fabs(dw / w);
+    if (err < s.tol) {
+      keep = cn > 0;
+      cn = cn - 1;
+    } else {
+      keep = it > 0;
+    }
+    if (w < wl || w > wu) w = 0.5 * (wl + wu);
+    it = it - 1;
+  }
+  // nr_velocity_srmhd_adi_1d
+  {
+    const double ww = w * w, wt = w + bb;
+    vv = (mm * ww + (w + wt) * ss) / (ww * (wt * wt));
+  }
+  return err >= s.tol ? C2P_NOCONV : 0;
+}
+
+// cons2prim_srmhd_adi, equations.F90:4671-4790.  q must hold the previous
+// primitive state on entry: a cell whose bracketing fails gets p = 0 and
+// keeps the rest, as the reference leaves its q array.
+template <class P>
+__device__ __forceinline__ int cons2prim_sr(const SrConst& s, const double* u, double* q) {
+  const double mm = (u[P::imx] * u[P::imx] + u[P::imy] * u[P::imy]) + u[P::imz] * u[P::imz];
+  const double mb = (u[P::imx] * u[P::ibx] + u[P::imy] * u[P::iby]) + u[P::imz] * u[P::ibz];
+  const double bb = (u[P::ibx] * u[P::ibx] + u[P::iby] * u[P::iby]) + u[P::ibz] * u[P::ibz];
+  const double en = u[P::ien] + u[P::idn];
+  const double dn = u[P::idn];
+  double w = 0.0, vv = 0.0;
+  int st = nr_iterate_1dw(s, mm, bb, mb, en, dn, w, vv);
+  if (st & C2P_FAIL) {
+    q[P::ipr] = 0.0;
+    return st;
+  }
+  const double vm = 1.0 - vv;
+  const double vs = sqrt(vm);
+  const double wt = w + bb;
+  const double fc = mb / w;
+  const double rwt = rcp(wt);
+  q[P::idn] = dn * vs;
+  q[P::ivx] = (u[P::imx] + fc * u[P::ibx]) * rwt;
+  q[P::ivy] = (u[P::imy] + fc * u[P::iby]) * rwt;
+  q[P::ivz] = (u[P::imz] + fc * u[P::ibz]) * rwt;
+  q[P::ibx] = u[P::ibx];
+  q[P::iby] = u[P::iby];
+  q[P::ibz] = u[P::ibz];
+  q[P::ibp] = u[P::ibp];
+  q[P::ipr] = (w - en) + 0.5 * (bb + (bb * mm - mb * mb) * (rwt * rwt));
+  if (q[P::ipr] <= 0.0) {
+    q[P::ipr] = 0.0;
+    st |= C2P_PNEG;
+  }
+  return st;
+}
+
+// riemann_srmhd_adi_hll (schemes.F90:5244-5355): the GLM pre-step, then HLL
+// with the relativistic fluxes and speeds
+template <class P, typename T>
+__device__ __forceinline__ void riemann_sr_hll(const SrConst& s, T cmax, T cmax2, T* ql,
+                                               T* qr, T* f) {
+  constexpr int nv = P::nv;
+  T ul[nv], ur[nv], fl[nv], fr[nv];
+  T clm, clp, crm, crp;
+  glm_upwind<P>(ql, qr, cmax);
+  const T rgx = T(s.rgammaxi);
+  prim2cons_sr<P>(rgx, ql, ul);
+  prim2cons_sr<P>(rgx, qr, ur);
+  fluxspeed_sr<P>(s, cmax2, ql, ul, fl, clm, clp);
+  fluxspeed_sr<P>(s, cmax2, qr, ur, fr, crm, crp);
+  const T sl = tmin(clm, crm), sr = tmax(clp, crp);
+  if (sl >= T(0)) {
+#pragma unroll
+    for (int p = 0; p < nv; p++) f[p] = fl[p];
+  } else if (sr <= T(0)) {
+#pragma unroll
+    for (int p = 0; p < nv; p++) f[p] = fr[p];
+  } else {
+    const T rsrml = rcp(sr - sl);
+#pragma unroll
+    for (int p = 0; p < nv; p++) {
+      const T wl = sl * ul[p] - fl[p];
+      const T wr = sr * ur[p] - fr[p];
+      f[p] = (sl * wr - sr * wl) * rsrml;
+    }
+  }
+}
+
+}  // namespace amun

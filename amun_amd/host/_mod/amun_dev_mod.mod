@@ -1,4 +1,4 @@
-﻿!mod$ v1 sum:4912b25ef60b3bc7
+﻿!mod$ v1 sum:49faf7fc53b1d3c7
 !need$ 0bde2ac47243ead2 i iso_c_binding
 !need$ bb381bf46e508468 i __fortran_builtins
 module amun_dev_mod
@@ -81,6 +81,7 @@ use,intrinsic::__fortran_builtins,only:iso_c_binding$__fortran_builtins$c_associ
 private::iso_c_binding$__fortran_builtins$c_associated_c_ptr
 integer(4),parameter::amun_eqsys_hd=1_4
 integer(4),parameter::amun_eqsys_mhd=2_4
+integer(4),parameter::amun_eqsys_srmhd=4_4
 integer(4),parameter::amun_eos_adi=1_4
 integer(4),parameter::amun_rec_tvd=1_4
 integer(4),parameter::amun_rec_weno5z=5_4
@@ -125,6 +126,10 @@ real(8)::ymin
 real(8)::ymax
 real(8)::zmin
 real(8)::zmax
+real(8)::tolerance
+real(8)::pmin
+integer(4)::nr_maxit
+integer(4)::nr_extra
 end type
 interface
 subroutine amun_dev_default_cfg(cfg) bind(c,name="amun_dev_default_cfg")
@@ -217,6 +222,14 @@ import::c_ptr
 type(c_ptr),value::ctx
 integer(4),intent(out)::coords(1_8:*)
 integer(4)::amun_dev_block_coords
+end
+end interface
+interface
+function amun_dev_c2p_events(ctx,n) bind(c,name="amun_dev_c2p_events")
+import::c_ptr
+type(c_ptr),value::ctx
+integer(8),intent(out)::n(1_8:3_8)
+integer(4)::amun_dev_c2p_events
 end
 end interface
 interface

@@ -16,6 +16,7 @@ module amun_dev_mod
   implicit none
 
   integer(c_int32_t), parameter :: AMUN_EQSYS_HD = 1, AMUN_EQSYS_MHD = 2
+  integer(c_int32_t), parameter :: AMUN_EQSYS_SRMHD = 4
   integer(c_int32_t), parameter :: AMUN_EOS_ADI = 1
   integer(c_int32_t), parameter :: AMUN_REC_TVD = 1, AMUN_REC_WENO5Z = 5
   integer(c_int32_t), parameter :: AMUN_LIM_ZERO = 0, AMUN_LIM_MINMOD = 1
@@ -35,6 +36,8 @@ module amun_dev_mod
     integer(c_int32_t) :: clip_extrema
     real(c_double)     :: gamma, cfl, eps, alpha
     real(c_double)     :: xmin, xmax, ymin, ymax, zmin, zmax
+    real(c_double)     :: tolerance, pmin
+    integer(c_int32_t) :: nr_maxit, nr_extra
   end type amun_cfg
 
   interface
@@ -115,6 +118,13 @@ module amun_dev_mod
       import :: c_int, c_ptr, c_int32_t
       type(c_ptr), value                            :: ctx
       integer(c_int32_t), dimension(*), intent(out) :: coords
+    end function
+
+    integer(c_int) function amun_dev_c2p_events(ctx, n) &
+                                             bind(C, name="amun_dev_c2p_events")
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value                             :: ctx
+      integer(c_int64_t), dimension(3), intent(out)  :: n
     end function
 
     type(c_ptr) function amun_dev_last_error_c() &

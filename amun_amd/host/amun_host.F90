@@ -41,7 +41,7 @@ program amun_host
   type(c_ptr)    :: ctx
   integer        :: iret, nv, nc, ng, im, km, nblk, nx, ny, nz, kn
   integer        :: nmax, s, b, i, j, k, p
-  integer(int64) :: seed, c0, c1, crate
+  integer(int64) :: seed, c0, c1, crate, c2p(3)
   real(real64)   :: dt, cmax, time, tmax, dtnext, elapsed, ints(8)
   character(len=256) :: fin, fout, problem, str
   real(c_double), allocatable, target :: u(:,:,:,:,:)
@@ -58,6 +58,8 @@ program amun_host
     cfg%eqsys = AMUN_EQSYS_HD
   case("mhd", "MHD", "magnetohydrodynamic", "MAGNETOHYDRODYNAMIC")
     cfg%eqsys = AMUN_EQSYS_MHD
+  case("srmhd", "SRMHD")
+    cfg%eqsys = AMUN_EQSYS_SRMHD
   case default
     cfg%eqsys = -1
   end select
@@ -127,6 +129,10 @@ program amun_host
   cfg%ymax = get_real("ymax", 1.0d0)
   cfg%zmin = get_real("zmin", 0.0d0)
   cfg%zmax = get_real("zmax", 1.0d0)
+  cfg%tolerance = get_real("tolerance", 1.0d-10)
+  cfg%pmin      = get_real("pmin", 1.0d-16)
+  cfg%nr_maxit  = get_int("nr_maxit", 100)
+  cfg%nr_extra  = get_int("nr_extra", 2)
   nmax    = get_int("nmax", 10)
   tmax    = get_real("tmax", 1.0d30)
   seed    = int(get_int("seed", 20261202), int64)
@@ -153,6 +159,8 @@ program amun_host
     call read_state(trim(fin))
   case("orszag_tang")
     call setup_orszag_tang()
+  case("srblast")
+    call setup_srblast()
   case default
     call setup_blast()
   end select
@@ -196,6 +204,10 @@ program amun_host
   write(*,"(a,i0,a,es22.15)") "steps = ", s, "  time = ", time
   write(*,"(a,es22.15)") "last dt = ", dt
   write(*,"(a,8es22.14)") "integrals = ", ints
+  if (cfg%eqsys == AMUN_EQSYS_SRMHD) then
+    iret = amun_dev_c2p_events(ctx, c2p)
+    write(*,"(a,3(1x,i0))") "primitive recovery failed / p<=0 / not converged =", c2p
+  end if
   if (s > 0) then
     write(*,"(a,es12.5,a)") "TIME PER STEP = ", elapsed / s, " s"
     write(*,"(a,es12.5)") "zone-updates/s = ", real(nx, real64) * ny * nz * s / elapsed
@@ -322,6 +334,10 @@ contains
     real(real64), intent(in) :: q(nx,ny,nz,nv)
     real(real64) :: ek(nx,ny,nz), g
     g = cfg%gamma
+    if (cfg%eqsys == AMUN_EQSYS_SRMHD) then
+      call cons_from_prim_sr(q)
+      return
+    end if
     ug(:,:,:,1) = q(:,:,:,1)
     do p = 2, 4
       ug(:,:,:,p) = q(:,:,:,1) * q(:,:,:,p)
@@ -336,6 +352,55 @@ contains
                     + q(:,:,:,6)**2 + q(:,:,:,7)**2)
     end if
   end subroutine cons_from_prim
+
+! prim2cons_srmhd_adi (equations.F90:4590-4653) on the global grid
+  subroutine cons_from_prim_sr(q)
+    real(real64), intent(in) :: q(nx,ny,nz,nv)
+    real(real64) :: vv, bb, vb, vm, wt, gx
+    gx = (cfg%gamma - 1.0d0) / cfg%gamma
+    do k = 1, nz
+      do j = 1, ny
+        do i = 1, nx
+          vv = (q(i,j,k,2)**2 + q(i,j,k,3)**2) + q(i,j,k,4)**2
+          bb = (q(i,j,k,5)**2 + q(i,j,k,6)**2) + q(i,j,k,7)**2
+          vb = (q(i,j,k,2) * q(i,j,k,5) + q(i,j,k,3) * q(i,j,k,6)) &
+             + q(i,j,k,4) * q(i,j,k,7)
+          vm = 1.0d0 - vv
+          wt = (q(i,j,k,1) + q(i,j,k,9) / gx) / vm + bb
+          ug(i,j,k,1) = q(i,j,k,1) / sqrt(vm)
+          do p = 2, 4
+            ug(i,j,k,p) = wt * q(i,j,k,p) - vb * q(i,j,k,p+3)
+          end do
+          ug(i,j,k,5:8) = q(i,j,k,5:8)
+          ug(i,j,k,9) = ((wt - q(i,j,k,9)) - ug(i,j,k,1)) &
+                        - 0.5d0 * (vm * bb + vb * vb)
+        end do
+      end do
+    end do
+  end subroutine cons_from_prim_sr
+
+! relativistic magnetised blast: rho = 1, v = 0, p = 1 for r < 0.1 else
+! 0.01, Bx = 0.5, psi = 0
+  subroutine setup_srblast()
+    real(real64), allocatable :: q(:,:,:,:)
+    real(real64) :: x, y, dx, dy
+    allocate(q(nx,ny,nz,nv))
+    dx = (cfg%xmax - cfg%xmin) / nx
+    dy = (cfg%ymax - cfg%ymin) / ny
+    q = 0.0d0
+    do k = 1, nz
+      do j = 1, ny
+        do i = 1, nx
+          x = cfg%xmin + (i - 0.5d0) * dx
+          y = cfg%ymin + (j - 0.5d0) * dy
+          q(i,j,k,1) = 1.0d0
+          q(i,j,k,5) = 0.5d0
+          q(i,j,k,9) = merge(1.0d0, 0.01d0, sqrt(x * x + y * y) < 0.1d0)
+        end do
+      end do
+    end do
+    call cons_from_prim(q)
+  end subroutine setup_srblast
 
   subroutine setup_blast()
     real(real64), allocatable :: q(:,:,:,:), xi(:)

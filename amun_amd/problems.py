@@ -48,9 +48,23 @@ def cell_centres(p: Params):
 
 
 def prim_to_cons(p: Params, q: np.ndarray) -> np.ndarray:
-    """prim2cons of equations.F90:1762 (HD) / 2815 (MHD), array form."""
+    """prim2cons of equations.F90:1762 (HD) / 2815 (MHD) / 4590 (SRMHD),
+    array form."""
     g = p.gamma
     u = np.empty_like(q)
+    if EQSYS[p.equation_system.lower()] == 4:
+        gammaxi = (g - 1.0) / g
+        vv = (q[1] * q[1] + q[2] * q[2]) + q[3] * q[3]
+        bb = (q[4] * q[4] + q[5] * q[5]) + q[6] * q[6]
+        vb = (q[1] * q[4] + q[2] * q[5]) + q[3] * q[6]
+        vm = 1.0 - vv
+        vs = np.sqrt(vm)
+        wt = (q[0] + q[8] / gammaxi) / vm + bb
+        u[0] = q[0] / vs
+        u[1:4] = wt * q[1:4] - vb * q[4:7]
+        u[4:8] = q[4:8]
+        u[8] = ((wt - q[8]) - u[0]) - 0.5 * (vm * bb + vb * vb)
+        return u
     u[0] = q[0]
     u[1:4] = q[0] * q[1:4]
     ek = 0.5 * (u[1] * q[1] + u[2] * q[2] + u[3] * q[3])
@@ -148,10 +162,30 @@ def turbulent_mhd_3d(p: Params, seed: int = DEFAULT_SEED, amplitude: float = 0.1
     return prim_to_cons(p, q)
 
 
+def srmhd_blast_2d(p: Params, seed: int = DEFAULT_SEED, noise: float = 0.0) -> np.ndarray:
+    """Relativistic magnetised cylindrical blast: rho = 1, v = 0, p = 1 where
+    the cell centre has r < 0.1 and 0.01 elsewhere, uniform Bx = 0.5, psi = 0;
+    optionally a seeded uniform density perturbation rho *= 1 + noise (2 xi - 1)."""
+    nz, ny, nx = p.shape
+    z, y, x = cell_centres(p)
+    r = np.sqrt(x[None, None, :] ** 2 + y[None, :, None] ** 2) * np.ones((nz, 1, 1))
+    q = np.zeros((p.nv, nz, ny, nx))
+    q[0] = 1.0
+    if noise:
+        xi = uniform(seed, nz * ny * nx, stream=7).reshape(nz, ny, nx)
+        q[0] = 1.0 + noise * (2.0 * xi - 1.0)
+    q[4] = 0.5
+    q[8] = np.where(r < 0.1, 1.0, 0.01)
+    return prim_to_cons(p, q)
+
+
 def initial_state(p: Params, seed: int = DEFAULT_SEED) -> np.ndarray:
     """Seeded initial conserved state for the configuration ``p``."""
-    if EQSYS[p.equation_system.lower()] == 1:
+    eq = EQSYS[p.equation_system.lower()]
+    if eq == 1:
         return blast_hd_2d(p, seed)
+    if eq == 4:
+        return srmhd_blast_2d(p, seed)
     return turbulent_mhd_3d(p, seed)
 
 

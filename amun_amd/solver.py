@@ -27,6 +27,7 @@ EXPORTS = (
     "amun_dev_integrals", "amun_dev_get_time", "amun_dev_nblocks",
     "amun_dev_block_coords", "amun_dev_last_error", "amun_dev_sync",
     "amun_dev_positivity_events", "amun_dev_stream", "amun_dev_timed_run",
+    "amun_dev_c2p_events",
 )
 
 _lib = None
@@ -62,6 +63,7 @@ def load_library(path: str = LIB_PATH):
     L.amun_dev_timed_run.argtypes = [vp, ctypes.c_int32, ctypes.c_double, _P, _P,
                                      ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]
     L.amun_dev_stream.argtypes = [vp]
+    L.amun_dev_c2p_events.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64)]
     L.amun_dev_stream.restype = ctypes.c_void_p
     L.amun_dev_last_error.argtypes = []
     L.amun_dev_last_error.restype = ctypes.c_char_p
@@ -217,3 +219,10 @@ class Solver:
         n = ctypes.c_uint64()
         self._check(self.lib.amun_dev_positivity_events(self.h, ctypes.byref(n)))
         return n.value
+
+    def c2p_events(self) -> list:
+        """[bracketing failed, p <= 0, not converged] of the relativistic
+        primitive solver so far."""
+        n = (ctypes.c_uint64 * 3)()
+        self._check(self.lib.amun_dev_c2p_events(self.h, n))
+        return list(n)

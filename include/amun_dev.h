@@ -35,6 +35,7 @@ extern "C" {
 /* equation systems: equations.F90:319-830 (parameter "equation_system") */
 #define AMUN_EQSYS_HD      1
 #define AMUN_EQSYS_MHD     2
+#define AMUN_EQSYS_SRMHD   4   /* special-relativistic MHD, equations.F90:673-760 */
 /* equations of state: equations.F90:345-395 (parameter "equation_of_state") */
 #define AMUN_EOS_ADI       1
 
@@ -85,6 +86,13 @@ typedef struct amun_cfg {
   double  eps;              /* "eps" of the WENO weights (default epsilon)   */
   double  alpha;            /* "alpha": GLM psi decay, psi *= exp(-alpha*cfl)*/
   double  xmin, xmax, ymin, ymax, zmin, zmax;   /* domain bounds             */
+  /* primitive-variable solver of the relativistic systems (equations.F90:
+   * 166-201, 303-318): Newton tolerance, iteration limit, extra iterations
+   * after convergence and the pressure floor of the brackets */
+  double  tolerance;        /* "tolerance" (1e-10)                            */
+  double  pmin;             /* "pmin"      (1e-16)                            */
+  int32_t nr_maxit;         /* "nr_maxit"  (100)                              */
+  int32_t nr_extra;         /* "nr_extra"  (2)                                */
 } amun_cfg;
 
 typedef struct amun_ctx amun_ctx;
@@ -159,6 +167,15 @@ int  amun_dev_sync(amun_ctx* ctx);
  * reconstruction so far (the reference prints a warning for each one and
  * zeroes the slopes, interpolations.F90:814-822). */
 int  amun_dev_positivity_events(amun_ctx* ctx, uint64_t* n);
+
+/* Events of the relativistic primitive solver so far, each of which the
+ * reference reports on stderr and survives (cons2prim_srmhd_adi,
+ * equations.F90:4740-4777; nr_iterate_srmhd_adi_1dw 5766-5770):
+ * n[0] = cells whose bracketing failed (pressure set to 0, the other
+ * primitives keep their previous values), n[1] = cells recovered with a
+ * non-positive pressure (set to 0), n[2] = cells whose Newton iteration
+ * ended without reaching the tolerance.  All zero for HD/MHD. */
+int  amun_dev_c2p_events(amun_ctx* ctx, uint64_t n[3]);
 
 /* The context's HIP stream (a hipStream_t), so that a caller can order its
  * own device work after the solver's. */

@@ -20,6 +20,12 @@
  *     riemann_mhd_adi_hll (3163), _hlld (3596-4203);
  *   - prim2cons / cons2prim / fluxspeed / maxspeed (equations.F90:1762-2044,
  *     2815-3131), update_primitive_variables (1088-1148);
+ *   - the special-relativistic MHD system: prim2cons_srmhd_adi,
+ *     cons2prim_srmhd_adi with the 1D(W) Newton solver and its brackets,
+ *     fluxspeed_srmhd_adi, maxspeed_srmhd_adi (equations.F90:4590-5163,
+ *     5249-5788), riemann_srmhd_adi_hll (schemes.F90:5244-5355) and the root
+ *     finders quadratic, quadratic_normalized, cubic, cubic_normalized,
+ *     quartic, sort (algebra.F90:86-1085);
  *   - update_increment (evolution.F90:1898-1977), evolve_euler/rk2/rk3
  *     (517, 628-770, 1005-1208), new_time_step (379-493);
  *   - same-level ghost exchange of q (boundaries.F90:372-479 with the face,
@@ -29,7 +35,9 @@
  *
  * Pins (see tests/): golden vectors written by the reference binaries
  * (tests/golden/), round trips, consistency F(u,u)=F(u), exact discrete
- * conservation, uniform-state preservation, polynomial exactness of WENO5-Z.
+ * conservation, uniform-state preservation, polynomial exactness of WENO5-Z,
+ * polynomial residuals and numpy.roots for the root finders, the
+ * prim -> cons -> prim round trip of the relativistic primitive solver.
  */
 #include <math.h>
 #include <stdint.h>
@@ -47,12 +55,14 @@ typedef struct {
   int im, jm, km;          /* block sizes incl. ghosts                       */
   int ib, ie, jb, je, kb, ke, ibl, ieu, jbl, jeu, kbl, keu;  /* 0-based      */
   int idn, ivx, ivy, ivz, imx, imy, imz, ibx, iby, ibz, ibp, ipr, ien;
-  double gammam1, gammam1i;
+  int    mhd, sr;          /* magnetic field present / relativistic system   */
+  double gammam1, gammam1i, gammaxi;
   double cflx;             /* cfl as adjusted by initialize_evolution        */
   double decay;            /* exp(-alpha*cfl), evolution.F90:211-213         */
   double adx, ady, adz, adxi, adyi, adzi;
   double cmax, cmax2;      /* equations module state                         */
   long   npos_events;      /* "Positive variable is not positive" prints     */
+  long   c2p_fail, c2p_pneg, c2p_noconv;   /* SR primitive solver messages */
 } ocfg;
 
 static int oc_setup(ocfg* o, const amun_cfg* c)
@@ -70,7 +80,9 @@ static int oc_setup(ocfg* o, const amun_cfg* c)
     o->imx = 1; o->imy = 2; o->imz = 3;
     o->ibx = o->iby = o->ibz = o->ibp = -1;
     o->ipr = o->ien = 4;
-  } else if (c->eqsys == AMUN_EQSYS_MHD) {
+  } else if (c->eqsys == AMUN_EQSYS_MHD || c->eqsys == AMUN_EQSYS_SRMHD) {
+    o->mhd = 1;
+    o->sr  = (c->eqsys == AMUN_EQSYS_SRMHD);
     o->nv = 9;
     o->idn = 0; o->ivx = 1; o->ivy = 2; o->ivz = 3;
     o->imx = 1; o->imy = 2; o->imz = 3;
@@ -94,6 +106,7 @@ static int oc_setup(ocfg* o, const amun_cfg* c)
   }
   o->gammam1  = c->gamma - 1.0;
   o->gammam1i = 1.0 / o->gammam1;
+  o->gammaxi  = o->gammam1 / c->gamma;       /* equations.F90:843 */
   o->cflx  = c->cfl;       /* euler, rk2 and rk3 keep the CFL unchanged */
   o->decay = exp(-c->alpha * o->cflx);
   {
@@ -319,11 +332,625 @@ static void reconstruct_interfaces(ocfg* o, const double* q, double* qs)
 }
 
 /* ------------------------------------------------------------------------ */
+/* polynomial roots, algebra.F90:86-1085.  The operation order of every     */
+/* expression follows the Fortran source as compiled (left to right;        */
+/* x**n = ((x*x)*x)..., a**(1/3) = pow(a, 1/3)).                             */
+/* ------------------------------------------------------------------------ */
+static const double alg_pi2 = 6.28318530717958647692528676655900;  /* constants.F90:40 */
+static const double alg_c0  = 1.0 / 3.0;
+
+static double fsign(double a, double b) { return copysign(fabs(a), b); }
+
+/* sort, algebra.F90:1040-1085: insertion sort of x[0..n-1], ascending */
+static void alg_sort(int n, double* x)
+{
+  int k, l;
+  double y;
+  if (n < 2) return;
+  for (k = 1; k < n; k++) {
+    y = x[k];
+    for (l = k - 1; l >= 0; l--) {
+      if (x[l] <= y) break;
+      x[l + 1] = x[l];
+    }
+    x[l + 1] = y;
+  }
+}
+
+/* quadratic, algebra.F90:86-242: a[0] + a[1] x + a[2] x^2 = 0 */
+static int alg_quadratic(const double* a, double* x)
+{
+  double b[3], bh, dl, dr, tm, mx;
+  int nr;
+  mx = fmax(fmax(fabs(a[0]), fabs(a[1])), fabs(a[2]));
+  b[0] = a[0] / mx; b[1] = a[1] / mx; b[2] = a[2] / mx;
+  if (b[2] != 0.0) {
+    if (b[0] != 0.0) {
+      bh = 0.5 * b[1];
+      dl = bh * bh - b[2] * b[0];
+      if (dl > 0.0) {
+        dr = sqrt(dl);
+        if (b[1] > 0.0) {
+          tm   = -bh - dr;
+          x[0] = tm / b[2];
+          x[1] = b[0] / tm;
+        } else if (b[1] < 0.0) {
+          tm   = -bh + dr;
+          x[0] = b[0] / tm;
+          x[1] = tm / b[2];
+        } else {
+          x[1] = dr / b[2];
+          x[0] = -x[1];
+        }
+        nr = 2;
+      } else if (dl == 0.0) {
+        x[0] = -0.5 * b[1] / b[2];
+        x[1] = x[0];
+        nr = 2;
+      } else {
+        x[0] = 0.0; x[1] = 0.0;
+        nr = 0;
+      }
+    } else {
+      tm = -b[1] / b[2];
+      if (tm >= 0.0) { x[0] = 0.0; x[1] = tm; }
+      else           { x[0] = tm;  x[1] = 0.0; }
+      nr = 2;
+    }
+  } else {
+    x[1] = 0.0;
+    if (b[1] != 0.0) { x[0] = -b[0] / b[1]; nr = 1; }
+    else             { x[0] = 0.0;          nr = 0; }
+  }
+  return nr;
+}
+
+/* quadratic_normalized, algebra.F90:262-372: a[0] + a[1] x + x^2 = 0 */
+static int alg_quadratic_normalized(const double* a, double* x)
+{
+  double bh, dl, dr, tm;
+  int nr;
+  if (a[0] != 0.0) {
+    bh = 0.5 * a[1];
+    dl = bh * bh - a[0];
+    if (dl > 0.0) {
+      dr = sqrt(dl);
+      if (a[1] > 0.0) {
+        tm   = bh + dr;
+        x[0] = -a[0] / tm;
+        x[1] = -tm;
+      } else if (a[1] < 0.0) {
+        tm   = bh - dr;
+        x[0] = -tm;
+        x[1] = -a[0] / tm;
+      } else {
+        x[0] =  dr;
+        x[1] = -dr;
+      }
+      nr = 2;
+    } else if (dl == 0.0) {
+      x[0] = -bh; x[1] = -bh;
+      nr = 2;
+    } else {
+      x[0] = 0.0; x[1] = 0.0;
+      nr = 0;
+    }
+  } else {
+    x[0] = 0.0;
+    x[1] = -a[1];
+    nr = 2;
+  }
+  return nr;
+}
+
+/* cubic, algebra.F90:398-614: a[0] + a[1] x + a[2] x^2 + a[3] x^3 = 0 */
+static int alg_cubic(const double* a, double* x)
+{
+  double ac, bc, qc, cc, dc, ec, rc, th, cs, a2, b3;
+  int nr;
+  if (a[3] != 0.0) {
+    if (a[0] != 0.0) {
+      ac = (2.0 * (a[2] * a[2] * a[2]) - 9.0 * a[3] * a[2] * a[1])
+           + 27.0 * (a[3] * a[3]) * a[0];
+      bc = a[2] * a[2] - 3.0 * a[3] * a[1];
+      a2 = ac * ac;
+      b3 = bc * bc * bc;
+      qc = a2 - 4.0 * b3;
+      if (qc > 0.0) {
+        cc = 0.5 * (sqrt(qc) + ac);
+        dc = fsign(1.0, cc) * pow(fabs(cc), alg_c0);
+        if (cc == 0.0) {
+          ec = b3 / a2;
+          ec = 2.0 * ec * (1.0 / sqrt(1.0 - 4.0 * ec) + 1.0 / ac);
+        } else {
+          ec = b3 / cc;
+        }
+        ec = fsign(1.0, ec) * pow(fabs(ec), alg_c0);
+        x[0] = -(alg_c0 * ((a[2] + dc) + ec)) / a[3];
+        x[1] = 0.0;
+        x[2] = 0.0;
+        nr = 1;
+      } else if (qc == 0.0) {
+        if (bc != 0.0) {
+          x[0] = 0.5 * (9.0 * a[3] * a[0] - a[2] * a[1]) / bc;
+          x[1] = x[0];
+          x[2] = ((-(9.0 * a[3] * a[0]) + 4.0 * a[2] * a[1])
+                  - (a[2] * a[2] * a[2]) / a[3]) / bc;
+        } else {
+          x[0] = x[1] = x[2] = -(alg_c0 * a[2]) / a[3];
+        }
+        nr = 3;
+      } else {
+        double fb;
+        th = 3.0 * a[3];
+        qc = bc / (th * th);
+        rc = 0.5 * ac / (th * th * th);
+        fb = fabs(bc);
+        cs = fsign(0.5, ac * bc) * sqrt((ac * ac) / (fb * fb * fb));
+        if (cs <= 1.0) {
+          th = acos(cs);
+          ac = -2.0 * sqrt(qc);
+          bc = -(alg_c0 * (a[2] / a[3]));
+          x[0] = ac * cos(alg_c0 * th) + bc;
+          x[1] = ac * cos(alg_c0 * (th + alg_pi2)) + bc;
+          x[2] = ac * cos(alg_c0 * (th - alg_pi2)) + bc;
+          nr = 3;
+        } else {
+          ac = -(fsign(1.0, rc) * pow(fabs(rc) + sqrt(rc * rc - qc * qc * qc), alg_c0));
+          if (ac != 0.0) bc = qc / ac;
+          else           bc = 0.0;
+          x[0] = (ac + bc) - alg_c0 * (a[2] / a[3]);
+          x[1] = 0.0;
+          x[2] = 0.0;
+          nr = 1;
+        }
+      }
+    } else {
+      x[2] = 0.0;
+      nr = alg_quadratic(a + 1, x) + 1;
+    }
+  } else {
+    x[2] = 0.0;
+    nr = alg_quadratic(a, x);
+  }
+  alg_sort(nr, x);
+  return nr;
+}
+
+/* cubic_normalized, algebra.F90:636-796: a[0] + a[1] x + a[2] x^2 + x^3 = 0 */
+static int alg_cubic_normalized(const double* a, double* x)
+{
+  const double c1 = 1.0 / 2.7e+01, c2 = 1.0 / 4.0;
+  double ac, bc, qc, a3, b2, aa, bb, sq, cs, ph;
+  int nr;
+  if (a[0] != 0.0) {
+    ac = alg_c0 * (a[2] * a[2] - 3.0 * a[1]);
+    bc = c1 * ((2.0 * (a[2] * a[2] * a[2]) - 9.0 * a[2] * a[1]) + 2.7e+01 * a[0]);
+    a3 = c1 * (ac * ac * ac);
+    b2 = c2 * (bc * bc);
+    qc = b2 - a3;
+    if (qc > 0.0) {
+      sq = sqrt(qc);
+      ph = -0.5 * bc + sq;
+      aa = fsign(1.0, ph) * pow(fabs(ph), alg_c0);
+      ph = -0.5 * bc - sq;
+      bb = fsign(1.0, ph) * pow(fabs(ph), alg_c0);
+      x[0] = (aa + bb) - (alg_c0 * a[2]);
+      x[1] = 0.0;
+      x[2] = 0.0;
+      nr = 1;
+    } else if (qc < 0.0) {
+      if (bc > 0.0)      cs = -sqrt(b2 / a3);
+      else if (bc < 0.0) cs =  sqrt(b2 / a3);
+      else               cs = 0.0;
+      ph = acos(cs);
+      aa = 2.0 * sqrt(alg_c0 * ac);
+      x[0] = aa * cos(alg_c0 * (ph - alg_pi2));
+      x[1] = aa * cos(alg_c0 * ph);
+      x[2] = aa * cos(alg_c0 * (ph + alg_pi2));
+      x[0] -= alg_c0 * a[2]; x[1] -= alg_c0 * a[2]; x[2] -= alg_c0 * a[2];
+      nr = 3;
+    } else {
+      if (bc > 0.0) {
+        aa = sqrt(alg_c0 * ac);
+        x[0] = -2.0 * aa; x[1] = aa; x[2] = aa;
+      } else if (bc < 0.0) {
+        aa = sqrt(alg_c0 * ac);
+        x[0] = 2.0 * aa; x[1] = -aa; x[2] = -aa;
+      } else {
+        x[0] = x[1] = x[2] = 0.0;
+      }
+      x[0] -= alg_c0 * a[2]; x[1] -= alg_c0 * a[2]; x[2] -= alg_c0 * a[2];
+      nr = 3;
+    }
+  } else {
+    x[2] = 0.0;
+    nr = alg_quadratic_normalized(a + 1, x) + 1;
+  }
+  alg_sort(nr, x);
+  return nr;
+}
+
+/* quartic, algebra.F90:820-1017 (Hacke's method through the resolvent
+ * cubic; its smallest root z(1) is taken, cubic sorting ascending):
+ * a[0] + a[1] x + ... + a[4] x^4 = 0 */
+static int alg_quartic(const double* a, double* x)
+{
+  double pc, qc, rc, zm, k2, l2, kc, lc, a42, a52;
+  double b[4], y[2], c[3], z[3];
+  int nr, m, n;
+  if (a[4] != 0.0) {
+    if (a[0] != 0.0) {
+      a42 = a[3] * a[3];
+      a52 = a[4] * a[4];
+      pc = (-(3.0 * a42) + 8.0 * a[4] * a[2]) / (8.0 * a52);
+      qc = ((a[3] * a[3] * a[3] - 4.0 * a[4] * a[3] * a[2]) + 8.0 * a52 * a[1])
+           / (8.0 * (a[4] * a[4] * a[4]));
+      rc = (((-(3.0 * (a42 * a[3] * a[3])) + 16.0 * a[4] * a42 * a[2])
+             - 64.0 * a52 * a[3] * a[1]) + 256.0 * (a[4] * a[4] * a[4]) * a[0])
+           / (256.0 * (a52 * a[4] * a[4]));
+      b[3] =  8.0;
+      b[2] = -4.0 * pc;
+      b[1] = -8.0 * rc;
+      b[0] =  4.0 * pc * rc - qc * qc;
+      nr = alg_cubic(b, z);
+      if (nr > 0) {
+        zm = z[0];
+        k2 = 2.0 * zm - pc;
+        l2 = zm * zm - rc;
+        if (k2 > 0.0 || l2 > 0.0) {
+          if (k2 > l2) {
+            kc = sqrt(k2);
+            lc = 0.5 * qc / kc;
+          } else if (l2 > k2) {
+            lc = sqrt(l2);
+            kc = 0.5 * qc / lc;
+          } else {
+            kc = sqrt(k2);
+            lc = sqrt(l2);
+          }
+          c[2] = 1.0; c[1] = -kc; c[0] = zm + lc;
+          m = alg_quadratic(c, y);
+          for (n = 0; n < m; n++) x[n] = y[n] - 0.25 * a[3] / a[4];
+          nr = m;
+          c[2] = 1.0; c[1] = kc; c[0] = zm - lc;
+          m = alg_quadratic(c, y);
+          for (n = 0; n < m; n++) x[nr + n] = y[n] - 0.25 * a[3] / a[4];
+          nr = nr + m;
+          for (n = nr; n < 4; n++) x[n] = 0.0;
+        } else {
+          x[0] = x[1] = x[2] = x[3] = 0.0;
+          nr = 0;
+        }
+      } else {
+        x[0] = x[1] = x[2] = x[3] = 0.0;
+      }
+    } else {
+      x[3] = 0.0;
+      nr = alg_cubic(a + 1, x) + 1;
+    }
+  } else {
+    x[3] = 0.0;
+    nr = alg_cubic(a, x);
+  }
+  alg_sort(nr, x);
+  return nr;
+}
+
+/* ------------------------------------------------------------------------ */
+/* special-relativistic MHD, adiabatic (equations.F90:4590-5163)              */
+/* ------------------------------------------------------------------------ */
+/* prim2cons_srmhd_adi, equations.F90:4590-4653 */
+static void prim2cons_sr(const ocfg* o, const double* q, double* u)
+{
+  double vv, bb, vb, vm, vs, ww, wt;
+  vv = (q[o->ivx] * q[o->ivx] + q[o->ivy] * q[o->ivy]) + q[o->ivz] * q[o->ivz];
+  bb = (q[o->ibx] * q[o->ibx] + q[o->iby] * q[o->iby]) + q[o->ibz] * q[o->ibz];
+  vb = (q[o->ivx] * q[o->ibx] + q[o->ivy] * q[o->iby]) + q[o->ivz] * q[o->ibz];
+  vm = 1.0 - vv;
+  vs = sqrt(vm);
+  ww = (q[o->idn] + q[o->ipr] / o->gammaxi) / vm;
+  wt = ww + bb;
+  u[o->idn] = q[o->idn] / vs;
+  u[o->imx] = wt * q[o->ivx] - vb * q[o->ibx];
+  u[o->imy] = wt * q[o->ivy] - vb * q[o->iby];
+  u[o->imz] = wt * q[o->ivz] - vb * q[o->ibz];
+  u[o->ibx] = q[o->ibx];
+  u[o->iby] = q[o->iby];
+  u[o->ibz] = q[o->ibz];
+  u[o->ibp] = q[o->ibp];
+  u[o->ien] = ((wt - q[o->ipr]) - u[o->idn]) - 0.5 * (vm * bb + vb * vb);
+}
+
+/* nr_pressure_srmhd_adi_1d, equations.F90:5314-5350 */
+static void nr_pressure(double mm, double bb, double mb, double en, double w,
+                        double* p, double* dp)
+{
+  double wt = w + bb, wd = wt * wt, ss = mb * mb, fn = (mm * bb - ss) / wd;
+  *p = (w - en) + 0.5 * (bb + fn);
+  if (dp) *dp = 1.0 - fn / wt;
+}
+
+/* nr_velocity_srmhd_adi_1d, equations.F90:5249-5290 (without dv) */
+static double nr_velocity(double mm, double bb, double mb, double w)
+{
+  double ss = mb * mb, ww = w * w, wt = w + bb, wt2 = wt * wt;
+  return (mm * ww + (w + wt) * ss) / (ww * wt2);
+}
+
+/* nr_function_srmhd_adi_1d, equations.F90:5371-5430 */
+static void nr_function(const ocfg* o, double mm, double bb, double mb, double en,
+                        double dn, double w, double* f, double* df)
+{
+  double pr, dp, gm2, gm, dg, ww, wt, wd, ss, sw, ws, fn, dv, ds;
+  ww  = w * w;
+  wt  = w + bb;
+  wd  = wt * wt;
+  ss  = mb * mb;
+  sw  = ss / ww;
+  ws  = (w + wt) * sw;
+  fn  = (mm * bb - ss) / wd;
+  dv  = (wd - mm) - ws;
+  ds  = sqrt(dv);
+  gm2 = wd / dv;
+  gm  = wt / ds;
+  pr  = o->gammaxi * (w - gm * dn) / gm2;
+  *f  = ((w - pr) - en) + 0.5 * (bb + fn);
+  if (df) {
+    dg  = (1.0 - wt * ((wt - sw) + ws / w) / dv) / ds;
+    dp  = o->gammaxi * (1.0 - (2.0 * w / gm - dn) * dg) / gm2;
+    *df = (1.0 - dp) - fn / wt;
+  }
+}
+
+/* nr_initial_brackets_srmhd_adi, equations.F90:5459-5635 */
+static int nr_initial_brackets(const ocfg* o, double mm, double bb, double mb,
+                               double en, double dn, double* wl, double* wu,
+                               double* wc, double* fl, double* fu)
+{
+  const double tol = o->c.tolerance, pmin = o->c.pmin;
+  const int nrmax = o->c.nr_maxit;
+  double dd, ss, ec, f, df, dw, err, a[5], x[4];
+  int it, nr, keep;
+  dd = dn * dn;
+  ss = mb * mb;
+  ec = en + pmin;
+  *wu = en + pmin;
+  /* lower bracket: largest root of the cubic and quartic positivity
+   * conditions of the pressure and of 1 - v^2 */
+  a[2] = 2.5 * bb - ec;
+  a[1] = 2.0 * (bb - ec) * bb;
+  a[0] = 0.5 * (((bb - 2.0 * ec) * bb + mm) * bb - ss);
+  nr = alg_cubic_normalized(a, x);
+  if (nr > 0) {
+    *wl = x[nr - 1];
+    a[4] = 1.0;
+    a[3] = 2.0 * bb;
+    a[2] = (bb * bb - dd) - mm;
+    a[1] = -(2.0 * (ss + dd * bb));
+    a[0] = -(ss + dd * bb) * bb;
+    nr = alg_quartic(a, x);
+    if (nr > 0) *wl = fmax(*wl, x[nr - 1]);
+  } else {
+    keep = 1;
+    it = nrmax;
+    *wl = *wu;
+    while (keep) {
+      nr_pressure(mm, bb, mb, ec, *wl, &f, &df);
+      dw  = f / df;
+      *wl = *wl - dw;
+      err = fabs(dw / *wl);
+      it  = it - 1;
+      keep = (err > tol) && it > 0;
+    }
+  }
+  nr_function(o, mm, bb, mb, en, dn, *wl, fl, NULL);
+  if (*fl <= 0.0) {
+    if (*wu <= *wl) *wu = 2.0 * *wl;
+    nr_function(o, mm, bb, mb, en, dn, *wu, fu, NULL);
+    it = nrmax;
+    keep = *fl * *fu > 0.0;
+    while (keep) {
+      it  = it - 1;
+      *wl = *wu;
+      *fl = *fu;
+      *wu = 2.0 * *wu;
+      nr_function(o, mm, bb, mb, en, dn, *wu, fu, NULL);
+      keep = (*fl * *fu > 0.0) && it > 0;
+    }
+    if (it > 0 && *fu >= 0.0) {
+      *wc = *wl - *fl * (*wu - *wl) / (*fu - *fl);
+      return 1;
+    }
+    return 0;
+  }
+  return 0;
+}
+
+/* nr_iterate_srmhd_adi_1dw, equations.F90:5665-5788; returns info, and
+ * sets *noconv when the iteration stopped above the tolerance */
+static int nr_iterate_1dw(const ocfg* o, double mm, double bb, double mb,
+                          double en, double dn, double* w, double* vv,
+                          int* noconv)
+{
+  const double tol = o->c.tolerance;
+  double wl, wu, fl, fu, f, df, dw, err = 0.0;
+  int it, cn, keep, info;
+  *noconv = 0;
+  info = nr_initial_brackets(o, mm, bb, mb, en, dn, &wl, &wu, w, &fl, &fu);
+  if (info) {
+    keep = 1;
+    it = o->c.nr_maxit;
+    cn = o->c.nr_extra;
+    while (keep) {
+      nr_function(o, mm, bb, mb, en, dn, *w, &f, &df);
+      if (f > fl && f < 0.0) { wl = *w; fl = f; }
+      if (f < fu && f > 0.0) { wu = *w; fu = f; }
+      dw  = f / df;
+      *w  = *w - dw;
+      err = fabs(dw / *w);
+      if (err < tol) {
+        keep = cn > 0;
+        cn = cn - 1;
+      } else {
+        keep = it > 0;
+      }
+      if (*w < wl || *w > wu) *w = 0.5 * (wl + wu);
+      it = it - 1;
+    }
+    if (err >= tol) *noconv = 1;
+    *vv = nr_velocity(mm, bb, mb, *w);
+  }
+  return info;
+}
+
+/* cons2prim_srmhd_adi, equations.F90:4671-4790.  On failure only the
+ * pressure is written (0), the other entries of q keep what they held. */
+static void cons2prim_sr(ocfg* o, const double* u, double* q)
+{
+  double mm, mb, bb, en, dn, w = 0.0, wt, vv = 0.0, vm, vs, fc;
+  int info, noconv;
+  mm = (u[o->imx] * u[o->imx] + u[o->imy] * u[o->imy]) + u[o->imz] * u[o->imz];
+  mb = (u[o->imx] * u[o->ibx] + u[o->imy] * u[o->iby]) + u[o->imz] * u[o->ibz];
+  bb = (u[o->ibx] * u[o->ibx] + u[o->iby] * u[o->iby]) + u[o->ibz] * u[o->ibz];
+  en = u[o->ien] + u[o->idn];
+  dn = u[o->idn];
+  info = nr_iterate_1dw(o, mm, bb, mb, en, dn, &w, &vv, &noconv);
+  if (noconv) o->c2p_noconv++;
+  if (info) {
+    vm = 1.0 - vv;
+    vs = sqrt(vm);
+    wt = w + bb;
+    fc = mb / w;
+    q[o->idn] = dn * vs;
+    q[o->ivx] = (u[o->imx] + fc * u[o->ibx]) / wt;
+    q[o->ivy] = (u[o->imy] + fc * u[o->iby]) / wt;
+    q[o->ivz] = (u[o->imz] + fc * u[o->ibz]) / wt;
+    q[o->ibx] = u[o->ibx];
+    q[o->iby] = u[o->iby];
+    q[o->ibz] = u[o->ibz];
+    q[o->ibp] = u[o->ibp];
+    q[o->ipr] = (w - en) + 0.5 * (bb + (bb * mm - mb * mb) / (wt * wt));
+    if (q[o->ipr] <= 0.0) {
+      o->c2p_pneg++;
+      q[o->ipr] = 0.0;
+    }
+  } else {
+    o->c2p_fail++;
+    q[o->ipr] = 0.0;
+  }
+}
+
+/* fluxspeed_srmhd_adi, equations.F90:4821-5101 (Mignone & Bodo 2006 fluxes;
+ * speeds from the quartic, quadratic or bi-quadratic characteristic
+ * polynomial) */
+static void fluxspeed_sr(const ocfg* o, const double* q, const double* u,
+                         double* f, double* cm, double* cp)
+{
+  const double g = o->c.gamma;
+  double vv, bb, vb, vm, bx, by, bz, b2, pm, pt;
+  double rh, vs, c2, v1, v2, fa, fb, fc, fd, fe, ff, fg, cc, gn, rt, ca, zm, zp;
+  double a[5], x[4];
+  int nr, n;
+  vv = (q[o->ivx] * q[o->ivx] + q[o->ivy] * q[o->ivy]) + q[o->ivz] * q[o->ivz];
+  bb = (q[o->ibx] * q[o->ibx] + q[o->iby] * q[o->iby]) + q[o->ibz] * q[o->ibz];
+  vb = (q[o->ivx] * q[o->ibx] + q[o->ivy] * q[o->iby]) + q[o->ivz] * q[o->ibz];
+  vm = 1.0 - vv;
+  bx = q[o->ibx] * vm + vb * q[o->ivx];
+  by = q[o->iby] * vm + vb * q[o->ivy];
+  bz = q[o->ibz] * vm + vb * q[o->ivz];
+  b2 = bb * vm + vb * vb;
+  pm = 0.5 * b2;
+  pt = q[o->ipr] + pm;
+  f[o->idn] = u[o->idn] * q[o->ivx];
+  f[o->imx] = (u[o->imx] * q[o->ivx] - q[o->ibx] * bx) + pt;
+  f[o->imy] = u[o->imy] * q[o->ivx] - q[o->ibx] * by;
+  f[o->imz] = u[o->imz] * q[o->ivx] - q[o->ibx] * bz;
+  f[o->ibx] = q[o->ibp];
+  f[o->ibp] = o->cmax2 * q[o->ibx];
+  f[o->iby] = q[o->ivx] * q[o->iby] - q[o->ibx] * q[o->ivy];
+  f[o->ibz] = q[o->ivx] * q[o->ibz] - q[o->ibx] * q[o->ivz];
+  f[o->ien] = u[o->imx] - f[o->idn];
+  if (!cm) return;
+  x[0] = x[1] = x[2] = x[3] = 0.0;
+  if (vv > 0.0) {
+    rh = q[o->idn] + q[o->ipr] / o->gammaxi;
+    vs = sqrt(vm);
+    if (q[o->ibx] != 0.0) {
+      c2 = g * q[o->ipr] / rh;
+      v1 = fabs(q[o->ivx]);
+      v2 = v1 * v1;
+      fa = rh * (1.0 - c2);
+      fb = c2 * (rh - vb * vb) + b2;
+      fc = fsign(1.0, q[o->ivx]) * q[o->ibx] * vs;
+      fd = c2 * fc * fc;
+      fe = 1.0 - v2;
+      ff = c2 * vb * fc;
+      fg = v1 * vs;
+      a[4] = fa + fb * vm;
+      a[3] = 2.0 * (ff * vm + fb * fg);
+      a[2] = (-(fd * vm) + 4.0 * ff * fg) - fb * fe;
+      a[1] = -(2.0 * (fd * fg + fe * ff));
+      a[0] = fd * fe;
+      nr = alg_quartic(a, x);
+      for (n = 0; n < nr; n++)
+        x[n] = fsign(1.0, q[o->ivx]) * (fabs(v1) + x[n] * vs);
+    } else {
+      c2 = g * q[o->ipr] / rh;
+      cc = (1.0 - c2) / vm;
+      gn = b2 - c2 * vb * vb;
+      a[2] = rh * (c2 + cc) + gn;
+      a[1] = -(2.0 * rh * cc * q[o->ivx]);
+      a[0] = rh * (cc * (q[o->ivx] * q[o->ivx]) - c2) - gn;
+      nr = alg_quadratic(a, x);
+    }
+  } else {
+    rh = q[o->idn] + q[o->ipr] / o->gammaxi;
+    vs = sqrt(vm);
+    rt = rh + b2;
+    c2 = g * q[o->ipr] / rh;
+    ca = q[o->ibx] * vs + vb * q[o->ivx] / vs;
+    ca = ca * ca;
+    a[2] = 1.0;
+    a[1] = -(((rh + ca) * c2 + b2) / rt);
+    a[0] = c2 * ca / rt;
+    nr = alg_quadratic(a, x);
+    if (nr > 0) {
+      zm = fmin(x[0], x[1]);
+      zp = fmax(x[0], x[1]);
+      if (zm >= 0.0) {
+        zm = sqrt(zm);
+        zp = sqrt(zp);
+        x[0] = zp; x[1] = zm; x[2] = -zm; x[3] = -zp;
+        nr = 4;
+      } else if (zp >= 0.0) {
+        zp = sqrt(zp);
+        x[0] = zp; x[1] = -zp;
+        nr = 2;
+      } else {
+        x[0] = x[1] = x[2] = x[3] = 0.0;
+        nr = 0;
+      }
+    }
+  }
+  if (nr > 1) {
+    double mn = x[0], mx = x[0];
+    for (n = 1; n < nr; n++) { mn = fmin(mn, x[n]); mx = fmax(mx, x[n]); }
+    *cm = mn;
+    *cp = mx;
+  } else {
+    *cm = -1.0;
+    *cp =  1.0;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
 /* equation of state, equations.F90:1762-1958 (HD) and 2815-3045 (MHD)        */
 /* ------------------------------------------------------------------------ */
 static void prim2cons(const ocfg* o, const double* q, double* u)
 {
   double ek, ei, em;
+  if (o->sr) { prim2cons_sr(o, q, u); return; }
   u[o->idn] = q[o->idn];
   u[o->imx] = q[o->idn] * q[o->ivx];
   u[o->imy] = q[o->idn] * q[o->ivy];
@@ -344,9 +971,10 @@ static void prim2cons(const ocfg* o, const double* q, double* u)
   }
 }
 
-static void cons2prim(const ocfg* o, const double* u, double* q)
+static void cons2prim(ocfg* o, const double* u, double* q)
 {
   double ek, ei, em;
+  if (o->sr) { cons2prim_sr(o, u, q); return; }
   q[o->idn] = u[o->idn];
   q[o->ivx] = u[o->imx] / u[o->idn];
   q[o->ivy] = u[o->imy] / u[o->idn];
@@ -371,6 +999,7 @@ static void fluxspeed(const ocfg* o, const double* q, const double* u,
                       double* f, double* cm, double* cp)
 {
   const double g = o->c.gamma;
+  if (o->sr) { fluxspeed_sr(o, q, u, f, cm, cp); return; }
   if (o->c.eqsys == AMUN_EQSYS_HD) {
     double cs;
     f[o->idn] = u[o->imx];
@@ -410,10 +1039,12 @@ static void fluxspeed(const ocfg* o, const double* q, const double* u,
   }
 }
 
-/* maxspeed_hd_adi (equations.F90:1974-2044), maxspeed_mhd_adi (3060-3131) */
+/* maxspeed_hd_adi (equations.F90:1974-2044), maxspeed_mhd_adi (3060-3131);
+ * maxspeed_srmhd_adi (5116-5163) returns the speed of light, 1 */
 static double maxspeed_cell(const ocfg* o, const double* q)
 {
   double vv, v, c, bb;
+  if (o->sr) return 1.0;
   vv = (q[o->ivx] * q[o->ivx] + q[o->ivy] * q[o->ivy]) + q[o->ivz] * q[o->ivz];
   v  = sqrt(vv);
   if (o->c.eqsys == AMUN_EQSYS_HD) {
@@ -437,7 +1068,7 @@ static void riemann_hll(const ocfg* o, double* ql, double* qr, double* f)
   double ul[NVMAX], ur[NVMAX], fl[NVMAX], fr[NVMAX], wl[NVMAX], wr[NVMAX];
   double clm, clp, crm, crp, sl, sr, srml;
   int p, nv = o->nv;
-  if (o->c.eqsys == AMUN_EQSYS_MHD) {
+  if (o->mhd) {
     double bx = 0.5 * ((qr[o->ibx] + ql[o->ibx]) - (qr[o->ibp] - ql[o->ibp]) / o->cmax);
     double bp = 0.5 * ((qr[o->ibp] + ql[o->ibp]) - (qr[o->ibx] - ql[o->ibx]) * o->cmax);
     ql[o->ibx] = bx; qr[o->ibx] = bx;
@@ -717,11 +1348,15 @@ static int riemann_check(const amun_cfg* c)
   if (c->riemann_solver == AMUN_RS_HLL) return 0;
   if (c->riemann_solver == AMUN_RS_HLLC && c->eqsys == AMUN_EQSYS_HD) return 0;
   if (c->riemann_solver == AMUN_RS_HLLD && c->eqsys == AMUN_EQSYS_MHD) return 0;
+  /* SRMHD: any name other than "hllc" selects HLL (schemes.F90:479-498) */
+  if (c->eqsys == AMUN_EQSYS_SRMHD && c->riemann_solver != AMUN_RS_HLLC) return 0;
+  /* riemann_srmhd_adi_hllc (schemes.F90:5382) is not restated */
   return 300;
 }
 
 static void riemann(const ocfg* o, double* ql, double* qr, double* f)
 {
+  if (o->sr) { riemann_hll(o, ql, qr, f); return; }   /* riemann_srmhd_adi_hll */
   switch (o->c.riemann_solver) {
   case AMUN_RS_HLLC: riemann_hd_hllc(o, ql, qr, f); break;
   case AMUN_RS_HLLD: riemann_mhd_hlld(o, ql, qr, f); break;
@@ -748,7 +1383,7 @@ static void update_flux(ocfg* o, const double* q, double* f, double* qs)
     for (p = 0; p < nv; p++) perm[d][p] = p;
   perm[1][o->ivx] = o->ivy; perm[1][o->ivy] = o->ivz; perm[1][o->ivz] = o->ivx;
   perm[2][o->ivx] = o->ivz; perm[2][o->ivy] = o->ivx; perm[2][o->ivz] = o->ivy;
-  if (o->c.eqsys == AMUN_EQSYS_MHD) {
+  if (o->mhd) {
     perm[1][o->ibx] = o->iby; perm[1][o->iby] = o->ibz; perm[1][o->ibz] = o->ibx;
     perm[2][o->ibx] = o->ibz; perm[2][o->iby] = o->ibx; perm[2][o->ibz] = o->iby;
   }
@@ -1045,6 +1680,59 @@ int orc_run(void* h, int nsteps, double dtnext, double* trace)
 }
 
 long orc_positivity_events(void* h) { return ((ogrid*)h)->o.npos_events; }
+
+/* messages of the relativistic primitive solver so far: bracketing failed,
+ * non-positive pressure, no convergence */
+void orc_c2p_events(void* h, long* n)
+{
+  ogrid* g = (ogrid*)h;
+  n[0] = g->o.c2p_fail;
+  n[1] = g->o.c2p_pneg;
+  n[2] = g->o.c2p_noconv;
+}
+
+/* unit level: the root finders of algebra.F90 on count coefficient sets
+ * a(5) (increasing powers).  Per set, out receives 19 values:
+ * [nr, x1, x2] quadratic(a(1:3)), [nr, x1, x2] quadratic_normalized(a(1:2)),
+ * [nr, x1..x3] cubic(a(1:4)), [nr, x1..x3] cubic_normalized(a(1:3)),
+ * [nr, x1..x4] quartic(a(1:5)); unused roots as the routines leave them */
+void orc_algebra(int count, const double* a, double* out)
+{
+  int i, k;
+  for (i = 0; i < count; i++) {
+    const double* c = a + (size_t)i * 5;
+    double* o = out + (size_t)i * 19;
+    double x[4];
+    x[0] = x[1] = x[2] = x[3] = 0.0;
+    o[0] = alg_quadratic(c, x);
+    o[1] = x[0]; o[2] = x[1];
+    x[0] = x[1] = x[2] = x[3] = 0.0;
+    o[3] = alg_quadratic_normalized(c, x);
+    o[4] = x[0]; o[5] = x[1];
+    x[0] = x[1] = x[2] = x[3] = 0.0;
+    o[6] = alg_cubic(c, x);
+    for (k = 0; k < 3; k++) o[7 + k] = x[k];
+    x[0] = x[1] = x[2] = x[3] = 0.0;
+    o[10] = alg_cubic_normalized(c, x);
+    for (k = 0; k < 3; k++) o[11 + k] = x[k];
+    x[0] = x[1] = x[2] = x[3] = 0.0;
+    o[14] = alg_quartic(c, x);
+    for (k = 0; k < 4; k++) o[15 + k] = x[k];
+  }
+}
+
+/* unit level: cons2prim on n conserved states u(nv,n); q(nv,n) is updated in
+ * place (entries a failing cell does not write keep their values); ev (may be
+ * NULL) receives the three solver message counts */
+int orc_cons2prim(const amun_cfg* c, int n, const double* u, double* q, long* ev)
+{
+  ocfg o;
+  int i, r = oc_setup(&o, c);
+  if (r) return r;
+  for (i = 0; i < n; i++) cons2prim(&o, u + (size_t)i * o.nv, q + (size_t)i * o.nv);
+  if (ev) { ev[0] = o.c2p_fail; ev[1] = o.c2p_pneg; ev[2] = o.c2p_noconv; }
+  return 0;
+}
 
 /* unit level: update_flux on one block q(nv,im,jm,km) with a given cmax */
 int orc_update_flux(const amun_cfg* c, const double* q, const double* dx,

@@ -14,7 +14,7 @@
 ! (parameters.F90:99-195) from the file given with "-i".  Extra keys used only
 ! by this harness (ignored by every reference module):
 !
-!   harness_mode   = run | flux | recon | eqs
+!   harness_mode   = run | ghosts | flux | recon | eqs | c2p | algebra
 !   harness_input  = input file (raw FP64)
 !   harness_output = output file prefix
 !   harness_steps  = number of steps (run)
@@ -35,11 +35,16 @@
 !         out: <prefix>_fl.bin fl(n,count), <prefix>_fr.bin fr(n,count)
 !   eqs   in : q(nv,n)
 !         out: <prefix>_eqs.bin u(nv,n), q2(nv,n), f(nv,n), cm(n), cp(n)
+!   c2p   in : u(nv,n), q(nv,n) (initial content of the output)
+!         out: <prefix>_c2p.bin q(nv,n) after cons2prim
+!   algebra in: a(5,n); out: <prefix>_alg.bin r(19,n) (see mode_algebra)
 !
 !===============================================================================
 !
 program amun_ref_harness
 
+  use algebra       , only : quadratic, quadratic_normalized
+  use algebra       , only : cubic, cubic_normalized, quartic
   use blocks        , only : initialize_blocks, block_meta, block_data
   use blocks        , only : list_data
   use boundaries    , only : initialize_boundaries, boundary_variables
@@ -119,6 +124,10 @@ program amun_ref_harness
     call mode_recon()
   case("eqs")
     call mode_eqs()
+  case("c2p")
+    call mode_c2p()
+  case("algebra")
+    call mode_algebra()
   case default
     write(*,*) "unknown harness_mode: ", trim(mode)
     stop 9
@@ -320,5 +329,73 @@ contains
     close(u)
 
   end subroutine mode_eqs
+
+!-------------------------------------------------------------------------------
+! cons2prim alone on given conserved states; q starts as the input's second
+! half so that entries a failing cell leaves untouched are known
+!-------------------------------------------------------------------------------
+  subroutine mode_c2p()
+
+    real(kind=8), dimension(:,:), allocatable :: uu, q
+    integer :: u
+
+    allocate(uu(nv,n), q(nv,n))
+    open(newunit=u, file=trim(fin), access="stream", form="unformatted",     &
+         status="old")
+    read(u) uu, q
+    close(u)
+    call cons2prim(n, uu, q)
+    open(newunit=u, file=trim(fout)//"_c2p.bin", access="stream",            &
+         form="unformatted", status="replace")
+    write(u) q
+    close(u)
+
+  end subroutine mode_c2p
+
+!-------------------------------------------------------------------------------
+! the polynomial root finders on coefficient sets a(5) (increasing powers);
+! per set 19 values: quadratic(a(1:3)) [nr,x(2)], quadratic_normalized(a(1:2))
+! [nr,x(2)], cubic(a(1:4)) [nr,x(3)], cubic_normalized(a(1:3)) [nr,x(3)],
+! quartic(a(1:5)) [nr,x(4)]; root arrays start at zero
+!-------------------------------------------------------------------------------
+  subroutine mode_algebra()
+
+    real(kind=8), dimension(:,:), allocatable :: a, r
+    real(kind=8), dimension(4) :: x
+    integer :: i, u, nr
+
+    allocate(a(5,n), r(19,n))
+    open(newunit=u, file=trim(fin), access="stream", form="unformatted",     &
+         status="old")
+    read(u) a
+    close(u)
+    do i = 1, n
+      x(:) = 0.0d+00
+      nr = quadratic(a(1:3,i), x(1:2))
+      r(1,i) = nr
+      r(2:3,i) = x(1:2)
+      x(:) = 0.0d+00
+      nr = quadratic_normalized(a(1:2,i), x(1:2))
+      r(4,i) = nr
+      r(5:6,i) = x(1:2)
+      x(:) = 0.0d+00
+      nr = cubic(a(1:4,i), x(1:3))
+      r(7,i) = nr
+      r(8:10,i) = x(1:3)
+      x(:) = 0.0d+00
+      nr = cubic_normalized(a(1:3,i), x(1:3))
+      r(11,i) = nr
+      r(12:14,i) = x(1:3)
+      x(:) = 0.0d+00
+      nr = quartic(a(1:5,i), x(1:4))
+      r(15,i) = nr
+      r(16:19,i) = x(1:4)
+    end do
+    open(newunit=u, file=trim(fout)//"_alg.bin", access="stream",            &
+         form="unformatted", status="replace")
+    write(u) r
+    close(u)
+
+  end subroutine mode_algebra
 
 end program amun_ref_harness

@@ -3,7 +3,8 @@
 (oracle/_ref, built from the unmodified reference sources by
 oracle/build_ref.sh).  Run in the container that holds the reference:
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py        # every fixture
+    python tests/golden/make_golden.py sr     # the relativistic ones only
 
 Every fixture stores the configuration (as AMUN parameter text), the seed,
 the reference outputs needed by the tests (sha256 of the exact bytes, the
@@ -22,7 +23,7 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.dirname(HERE))
 
-from amun_amd.config import Params, c1_params, c2_params  # noqa: E402
+from amun_amd.config import Params, c1_params, c2_params, c4_params  # noqa: E402
 from amun_amd import problems  # noqa: E402
 import oracle_lib as ol  # noqa: E402
 
@@ -65,9 +66,100 @@ def hd3_blast(p):
     return problems.prim_to_cons(p, q)
 
 
+def sr_states(rng, n):
+    """Random special-relativistic primitive states (n, 9): |v| < 0.99,
+    densities and pressures over decades, field strengths from weak to
+    dominant; the first rows have v = 0, then Bx = 0, then both, then B = 0."""
+    q = np.zeros((n, 9))
+    q[:, 0] = 10.0 ** rng.uniform(-2, 1, n)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    q[:, 1:4] = d * rng.uniform(0.0, 0.99, n)[:, None]
+    q[:, 4:7] = rng.normal(size=(n, 3)) * 10.0 ** rng.uniform(-2, 0.5, (n, 1))
+    q[:, 7] = rng.normal(scale=0.1, size=n)
+    q[:, 8] = 10.0 ** rng.uniform(-3, 1, n)
+    k = n // 16
+    q[:k, 1:4] = 0.0
+    q[k:2 * k, 4] = 0.0
+    q[2 * k:3 * k, 1:4] = 0.0
+    q[2 * k:3 * k, 4] = 0.0
+    q[3 * k:3 * k + 4, 4:7] = 0.0
+    return q
+
+
+def algebra_sets(rng, n):
+    """Coefficient sets a(5) (increasing powers) for the root finders:
+    random magnitudes over decades, zero constant / leading terms, and
+    polynomials with known multiple roots."""
+    a = rng.normal(size=(n, 5)) * 10.0 ** rng.integers(-3, 4, (n, 5))
+    a[10:40, 0] = 0.0
+    a[40:70, 4] = 0.0
+    a[70:90, 3:] = 0.0
+    a[90:100, 2:] = 0.0
+    known = [(1, 1, 2, -3), (2, 2, 2, 2), (0.5, -0.5, 1, -1), (1, 2, 3, 4), (-1, -1, -1, 5)]
+    for i, r in enumerate(known):
+        a[i] = np.poly(r)[::-1]
+    a[5, :4] = np.poly([2, 2, 2])[::-1]
+    a[5, 4] = 0.0
+    a[6, :3] = np.poly([3, 3])[::-1]
+    a[6, 3:] = 0.0
+    return a
+
+
+def main_sr():
+    """Fixtures of the special-relativistic MHD path (own generator stream,
+    so the other fixtures are unaffected)."""
+    rng = np.random.default_rng(20261203)
+    a = algebra_sets(rng, 600)
+    np.savez_compressed(os.path.join(HERE, "unit_algebra.npz"), a=a, r=ol.ref_algebra(a))
+
+    pe = Params(ndims=2, equation_system="srmhd", riemann_solver="hll",
+                tolerance=1.0e-12, nr_maxit=50)
+    q = sr_states(rng, 256)
+    u, q2, f, cm, cp = ol.ref_eqs(pe, q, cmax=0.8)
+    # conserved states for which the primitive solver fails (energy below
+    # the rest mass) or copes (large momenta); the output starts at -7
+    ub = u[:64].copy()
+    ub[:16, 8] = -2.0 * ub[:16, 0]
+    ub[16:32, 1:4] *= 50.0
+    q0 = np.full_like(ub, -7.0)
+    qb, _ = ol.ref_cons2prim(pe, ub, q0)
+    np.savez_compressed(os.path.join(HERE, "unit_eqs_srmhd.npz"), q=q, u=u, q2=q2, f=f,
+                        cm=cm, cp=cp, cmax=0.8, ub=ub, q0=q0, qb=qb,
+                        params=np.array(pe.to_params_in()))
+
+    pf = Params(ndims=2, equation_system="srmhd", reconstruction="tvd", limiter="zero",
+                riemann_solver="hll", gamma=5.0 / 3.0, ncells=8, nghosts=2,
+                tolerance=1.0e-12, nr_maxit=50)
+    nb = 3
+    qb = sr_states(rng, nb * 144).reshape(nb, 1, 12, 12, 9)
+    f = ol.ref_update_flux(pf, qb, cmax=1.0)
+    np.savez_compressed(os.path.join(HERE, "unit_flux_srmhd_hll.npz"), q=qb, f=f,
+                        cmax=1.0, params=np.array(pf.to_params_in()))
+
+    p4 = c4_params(64)
+    u0 = problems.srmhd_blast_2d(p4, noise=0.01)
+    nsteps, stride = 100, 4
+    u, tr = ol.ref_run(p4, u0, nsteps)
+    name = "c4_srmhd2d_64_100"
+    np.savez_compressed(os.path.join(HERE, name + ".npz"),
+                        params=np.array(p4.to_params_in()), ndims=2, nsteps=nsteps,
+                        stride=stride, trace=tr, sub=u[:, :, ::stride, ::stride],
+                        sha_u0=sha(u0), sha_u=sha(u))
+    with open(os.path.join(HERE, "index.json")) as fh:
+        index = json.load(fh)
+    index[name] = {"steps": nsteps, "sha_u": sha(u), "dt_last": float(tr[-1, 0])}
+    with open(os.path.join(HERE, "index.json"), "w") as fh:
+        json.dump(index, fh, indent=1)
+    print(name, p4.shape, "dt", tr[-1, 0])
+
+
 def main():
     if not (ol.ref_available(2) and ol.ref_available(3)):
         sys.exit("reference binaries missing: run oracle/build_ref.sh first")
+    if sys.argv[1:] == ["sr"]:
+        main_sr()
+        return
     index = {}
     for name, p, builder, nsteps, stride in run_cases():
         u0 = builder(p) if builder else hd3_blast(p)
@@ -130,6 +222,7 @@ def main():
                             cmax=2.3, params=np.array(pf.to_params_in()))
     with open(os.path.join(HERE, "index.json"), "w") as fh:
         json.dump(index, fh, indent=1)
+    main_sr()
 
 
 if __name__ == "__main__":

@@ -43,6 +43,10 @@ def lib():
         L.orc_reconstruct.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _P, _P, _P]
         L.orc_eqs.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, _P, _P, _P, _P, _P, _P]
         L.orc_riemann.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, _P, _P, _P]
+        L.orc_c2p_events.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_long)]
+        L.orc_algebra.argtypes = [ctypes.c_int, _P, _P]
+        L.orc_cons2prim.argtypes = [ctypes.c_void_p, ctypes.c_int, _P, _P,
+                                    ctypes.POINTER(ctypes.c_long)]
         _lib = L
     return _lib
 
@@ -82,6 +86,11 @@ class OracleRun:
 
     def positivity_events(self) -> int:
         return lib().orc_positivity_events(self.h)
+
+    def c2p_events(self):
+        ev = (ctypes.c_long * 3)()
+        lib().orc_c2p_events(self.h, ev)
+        return list(ev)
 
 
 def oracle_riemann(p: Params, ql: np.ndarray, qr: np.ndarray, cmax: float) -> np.ndarray:
@@ -124,6 +133,26 @@ def oracle_eqs(p: Params, q: np.ndarray, cmax: float = 1.0):
     cfg = p.to_cfg()
     lib().orc_eqs(ctypes.byref(cfg), n, cmax, _dp(q), _dp(u), _dp(q2), _dp(f), _dp(cm), _dp(cp))
     return u, q2, f, cm, cp
+
+
+def oracle_algebra(a: np.ndarray) -> np.ndarray:
+    """Root finders of algebra.F90 on coefficient sets a (n, 5), increasing
+    powers; returns (n, 19) as laid out by orc_algebra."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    out = np.zeros((a.shape[0], 19))
+    lib().orc_algebra(a.shape[0], _dp(a), _dp(out))
+    return out
+
+
+def oracle_cons2prim(p: Params, u: np.ndarray, q0: np.ndarray):
+    """cons2prim of the equation system on u (n, nv); q starts as q0."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    q = np.array(q0, dtype=np.float64, order="C", copy=True)
+    ev = (ctypes.c_long * 3)()
+    cfg = p.to_cfg()
+    r = lib().orc_cons2prim(ctypes.byref(cfg), u.shape[0], _dp(u), _dp(q), ev)
+    assert r == 0, r
+    return q, list(ev)
 
 
 # ------------------------------------------------------------------ reference
@@ -200,3 +229,29 @@ def ref_eqs(p: Params, q: np.ndarray, cmax: float = 1.0):
     cm = a[3 * nv * n: 3 * nv * n + n]
     cp = a[3 * nv * n + n:]
     return u, q2, f, cm, cp
+
+
+def ref_algebra(a: np.ndarray, ndims: int = 2) -> np.ndarray:
+    """The reference's root finders (harness mode "algebra")."""
+    n = a.shape[0]
+    p = Params(ndims=ndims, equation_system="hd", riemann_solver="hllc")
+    with tempfile.TemporaryDirectory() as d:
+        np.ascontiguousarray(a, dtype=np.float64).tofile(os.path.join(d, "in.bin"))
+        _ref_exec(p, {"harness_mode": "algebra", "harness_input": "in.bin",
+                      "harness_output": "out", "harness_n": n}, d)
+        r = np.fromfile(os.path.join(d, "out_alg.bin")).reshape(n, 19)
+    return r
+
+
+def ref_cons2prim(p: Params, u: np.ndarray, q0: np.ndarray):
+    """The reference's cons2prim on u (n, nv) with q starting as q0; also
+    returns the stderr text (the solver's messages)."""
+    n = u.shape[0]
+    with tempfile.TemporaryDirectory() as d:
+        with open(os.path.join(d, "in.bin"), "wb") as fh:
+            np.ascontiguousarray(u, dtype=np.float64).tofile(fh)
+            np.ascontiguousarray(q0, dtype=np.float64).tofile(fh)
+        r = _ref_exec(p, {"harness_mode": "c2p", "harness_input": "in.bin",
+                          "harness_output": "out", "harness_n": n}, d)
+        q = np.fromfile(os.path.join(d, "out_c2p.bin")).reshape(n, p.nv)
+    return q, r.stderr

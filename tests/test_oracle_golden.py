@@ -91,3 +91,52 @@ def test_riemann_via_update_flux(eq, rs):
     for b in range(g["q"].shape[0]):
         f = ol.oracle_update_flux(p, g["q"][b], float(g["cmax"]))
         np.testing.assert_array_equal(f, g["f"][b])
+
+
+# ----------------------------------------------------------------- SRMHD
+def test_c4_srmhd_run_bit_exact():
+    """Relativistic MHD blast (TVD minmod + HLL + SSPRK(2,2), 1D(W) primitive
+    solver) on 64^2 for 100 steps."""
+    g = load("c4_srmhd2d_64_100")
+    p = params_of(g)
+    assert p.equation_system == "srmhd" and p.tolerance == 1e-12 and p.nr_maxit == 50
+    u0 = problems.srmhd_blast_2d(p, noise=0.01)
+    assert sha(u0) == str(g["sha_u0"]), "initial state changed"
+    o = ol.OracleRun(p)
+    o.set_state(u0)
+    tr = o.run(int(g["nsteps"]))
+    np.testing.assert_array_equal(tr, g["trace"])
+    s = int(g["stride"])
+    u = o.get_state()
+    np.testing.assert_array_equal(u[:, :, ::s, ::s], g["sub"])
+    assert sha(u) == str(g["sha_u"])
+    assert o.c2p_events() == [0, 0, 0]
+
+
+def test_algebra_root_finders():
+    """quadratic, quadratic_normalized, cubic, cubic_normalized, quartic."""
+    g = load("unit_algebra")
+    r = ol.oracle_algebra(g["a"])
+    np.testing.assert_array_equal(r, g["r"])
+
+
+def test_srmhd_equation_of_state():
+    g = load("unit_eqs_srmhd")
+    p = Params.from_text(str(g["params"]), ndims=2)
+    u, q2, f, cm, cp = ol.oracle_eqs(p, g["q"], float(g["cmax"]))
+    for a, b in ((u, g["u"]), (q2, g["q2"]), (f, g["f"]), (cm, g["cm"]), (cp, g["cp"])):
+        np.testing.assert_array_equal(a, b)
+    # failing and hard conserved states: the reference writes only p = 0 for
+    # a failed cell and keeps the rest of q
+    qb, ev = ol.oracle_cons2prim(p, g["ub"], g["q0"])
+    np.testing.assert_array_equal(qb, g["qb"])
+    failed = np.all(qb[:, :8] == -7.0, axis=1) & (qb[:, 8] == 0.0)
+    assert failed[:16].all() and ev[0] == failed.sum()
+
+
+def test_srmhd_hll_via_update_flux():
+    g = load("unit_flux_srmhd_hll")
+    p = Params.from_text(str(g["params"]), ndims=2)
+    for b in range(g["q"].shape[0]):
+        f = ol.oracle_update_flux(p, g["q"][b], float(g["cmax"]))
+        np.testing.assert_array_equal(f, g["f"][b])

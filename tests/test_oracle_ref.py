@@ -4,7 +4,7 @@ reference sources)."""
 import numpy as np
 import pytest
 
-from amun_amd.config import Params, c1_params, c2_params
+from amun_amd.config import Params, c1_params, c2_params, c4_params
 from amun_amd import problems
 import oracle_lib as ol
 
@@ -55,3 +55,33 @@ def test_kh_hd_hllc_rk2():
     to = o.run(20)
     np.testing.assert_array_equal(to, tr)
     np.testing.assert_array_equal(o.get_state(), ur)
+
+
+@needs_ref
+@pytest.mark.parametrize("ta,lim", [("rk2", "mm"), ("rk3", "mc"), ("euler", "vl")])
+def test_srmhd_blast(ta, lim):
+    p = c4_params(48)
+    p.time_advance, p.limiter = ta, lim
+    u0 = problems.srmhd_blast_2d(p, seed=3, noise=0.05)
+    ur, tr = ol.ref_run(p, u0, 12)
+    o = ol.OracleRun(p)
+    o.set_state(u0)
+    to = o.run(12)
+    np.testing.assert_array_equal(to, tr)
+    np.testing.assert_array_equal(o.get_state(), ur)
+
+
+@needs_ref
+def test_srmhd_eqs_and_roots_fresh_inputs():
+    rng = np.random.default_rng(99)
+    p = Params(ndims=2, equation_system="srmhd", riemann_solver="hll")
+    q = np.zeros((500, 9))
+    q[:, 0] = 10.0 ** rng.uniform(-2, 1, 500)
+    d = rng.normal(size=(500, 3))
+    q[:, 1:4] = d / np.linalg.norm(d, axis=1)[:, None] * rng.uniform(0, 0.999, (500, 1))
+    q[:, 4:8] = rng.normal(size=(500, 4))
+    q[:, 8] = 10.0 ** rng.uniform(-3, 1, 500)
+    for a, b in zip(ol.oracle_eqs(p, q, 0.9), ol.ref_eqs(p, q, 0.9)):
+        np.testing.assert_array_equal(a, b)
+    co = rng.normal(size=(500, 5)) * 10.0 ** rng.integers(-4, 5, (500, 5))
+    np.testing.assert_array_equal(ol.oracle_algebra(co), ol.ref_algebra(co))
