@@ -253,5 +253,28 @@ def c4_params(n: int = 1024, precision: str = "fp64") -> Params:
                   nr_maxit=50)
 
 
+C3_SIZES = (64, 128, 256, 512)
+
+
+def c3_batch_params(nruns: int = 64, seed: int = 20261202, precision: str = "fp64"):
+    """64 independent 2D MHD Kelvin-Helmholtz runs whose grid sizes are drawn
+    by seed from {64^2, 128^2, 256^2, 512^2} (SplitMix64 stream 11), each on
+    [-0.5, 0.5]^2 in blocks of 16^2 (+2 ghosts), TVD-MC + HLLD + SSPRK(3,3),
+    CFL 0.4.  Returns (common Params, [Params of every run])."""
+    from .problems import uniform
+    xi = uniform(seed, nruns, stream=11)
+    common = Params(ndims=2, equation_system="mhd", reconstruction="tvd", limiter="mc",
+                    riemann_solver="hlld", time_advance="rk3", precision=precision,
+                    gamma=5.0 / 3.0, cfl=0.4, ncells=16, nghosts=2, xmin=-0.5,
+                    xmax=0.5, ymin=-0.5, ymax=0.5)
+    runs = []
+    for x in xi:
+        n = C3_SIZES[min(int(4.0 * x), 3)]
+        p = Params(**{f.name: getattr(common, f.name) for f in fields(Params) if f.name != "extra"})
+        p.xblocks = p.yblocks = n // 16
+        runs.append(p)
+    return common, runs
+
+
 def psi_decay(p: Params) -> float:
     return math.exp(-p.alpha * p.cfl)

@@ -29,6 +29,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -56,41 +57,47 @@ int fail(int code, const std::string& msg) {
   } while (0)
 
 constexpr double kTiny = 2.2250738585072014e-308;   // tiny(1d0)
-constexpr int kTrace = 1 << 16;                      // dt trace ring
+constexpr int kTrace = 1 << 16;                      // dt trace entries (all runs)
 
 // ---------------------------------------------------------------------------
 // device-side step state
 // ---------------------------------------------------------------------------
 struct DevState {
+  unsigned long long npos;          // non-positive density/pressure events
+  unsigned long long c2p[3];        // SR primitive solver: failed, p <= 0, no convergence
+};
+
+// step state of one run (one independent grid of a batch; a single run is a
+// batch of one): what new_time_step and the driver keep per simulation
+struct RunState {
   unsigned long long cmax_bits[2];  // cmax of step s lives in slot s&1
   double dt;                        // last dt (kept when dtnext <= 0)
   double time;
   long long step;
-  unsigned long long npos;          // non-positive density/pressure events
-  unsigned long long c2p[3];        // SR primitive solver: failed, p <= 0, no convergence
-  double trace[kTrace];             // dt of every step (ring)
+  double dxmin;                     // smallest cell size (evolution.F90:467-476)
+  double dxi[3];                    // inverse cell sizes
 };
 
 // constants of one run, passed by value to every kernel
 struct RunConst {
   int nd, nc, nblk, ncell;
+  int nruns;                        // independent grids in the block pool
   int limiter;
   double gamma, gammam1, gammam1i;
-  double cfl, dxmin, eps, decay;
-  double dxi[3];
+  double cfl, eps, decay;
   SrConst sc;                       // relativistic primitive solver
 };
 
-__device__ __forceinline__ double cmax_of(const DevState* st, int slot) {
-  return fmax(__longlong_as_double((long long)st->cmax_bits[slot]), kTiny);
+__device__ __forceinline__ double cmax_of(const RunState* R, int slot) {
+  return fmax(__longlong_as_double((long long)R->cmax_bits[slot]), kTiny);
 }
 
 // dt of new_time_step (evolution.F90:478-483), identical in every thread
-__device__ __forceinline__ double dt_of(const RunConst& rc, const DevState* st,
+__device__ __forceinline__ double dt_of(const RunConst& rc, const RunState* R,
                                        int slot, double dtnext) {
-  double cmax = cmax_of(st, slot);
-  double dtn = rc.cfl * rc.dxmin / fmax(cmax + 2.0 * 0.0 / rc.dxmin, kTiny);
-  return dtnext > 0.0 ? fmin(dtn, dtnext) : st->dt;
+  double cmax = cmax_of(R, slot);
+  double dtn = rc.cfl * R->dxmin / fmax(cmax + 2.0 * 0.0 / R->dxmin, kTiny);
+  return dtnext > 0.0 ? fmin(dtn, dtnext) : R->dt;
 }
 
 // Position of cell (i,j,k) inside a block's variable plane.  Cells are stored
@@ -183,8 +190,11 @@ struct SweepArgs {
   void* D;                         // increment accumulator (T)
   float* C;                        // Kahan compensation of U0 (mixed mode)
   const int* __restrict__ nbr;
-  DevState* st;
+  DevState* st;                    // event counters
+  RunState* rs;                    // per-run step state
+  const int* __restrict__ brun;    // run of every block
   int slot;                        // cmax slot of this step
+  int swz;                         // XCD-aware workgroup order (autotuned)
   int kind;                        // stage combine, see combine()
   int last;                        // last stage of the step
   int first;                       // first sweep of the step
@@ -311,6 +321,34 @@ __device__ __forceinline__ void c2p_keep_old(int cs, double* q, const T* __restr
   }
 }
 
+// Workgroup index of this workgroup.  The dispatcher deals workgroups out to
+// the eight XCDs round robin (workgroup i runs on XCD i mod 8), and each XCD
+// has its own L2.  With swz set, the index is remapped so that every XCD gets
+// one contiguous range of the pencil space (neighbouring pencils, hence the
+// same bricks and the same neighbour blocks, share an L2); the map is a
+// bijection of [0, gridDim.x).
+__device__ __forceinline__ int workgroup_index(int swz) {
+  const int i = blockIdx.x;
+  if (!swz) return i;
+  const int g = gridDim.x, q = g >> 3, r = g & 7, x = i & 7;
+  return x * q + (x < r ? x : r) + (i >> 3);
+}
+
+// Maximum speed of a wave into its run's accumulator: one atomic per wave
+// when every lane belongs to the same run (the common case), one per lane
+// otherwise (a wave straddling two grids of a batch).  Positive doubles
+// order as their bit patterns.
+__device__ __forceinline__ void wave_max_speed(RunState* rs, int run, int slot, double smax) {
+  const int r0 = __shfl(run, 0);
+  if (__all(run == r0)) {
+    for (int off = 32; off > 0; off >>= 1) smax = fmax(smax, __shfl_xor(smax, off));
+    if ((threadIdx.x & 63) == 0 && smax > 0.0)
+      atomicMax(&rs[r0].cmax_bits[slot], (unsigned long long)__double_as_longlong(smax));
+  } else if (smax > 0.0) {
+    atomicMax(&rs[run].cmax_bits[slot], (unsigned long long)__double_as_longlong(smax));
+  }
+}
+
 // ---------------------------------------------------------------------------
 // the sweep kernel
 // ---------------------------------------------------------------------------
@@ -323,24 +361,31 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a, RunConst rc) {
   constexpr bool kChunk = DIR < 2;
   constexpr bool kDin = kChunk && DIR > 0;       // increment read in chunks
   constexpr bool kDout = kChunk && !kLast;       // increment written in chunks
-  __shared__ T sq[kChunk ? 4 * NV * 256 : 1];
-  __shared__ T sd[(kDin || kDout) ? 4 * NV * 256 : 1];
+  // per-lane LDS slots of the chunked pencil I/O: sq holds four cells of the
+  // primitive window, sd four cells of the increment (4 NV values per lane
+  // each); sized by the launch for its workgroup size
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int wg = blockDim.x;
+  T* const sq = reinterpret_cast<T*>(smem);
+  T* const sd = sq + (kChunk ? 4 * NV * wg : 0);
   const int nc = rc.nc;
   const int npen = (ND == 3) ? nc * nc : nc;
-  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long gid = (long long)workgroup_index(a.swz) * blockDim.x + threadIdx.x;
   const int b = (int)(gid / npen);
   const int t = (int)(gid % npen);
   const bool active = b < rc.nblk;
 
-  if (a.first && gid == 0) {
-    // reset the maximum-speed accumulator of the next step (new_time_step:
-    // cmax = tiny before the reduction)
-    a.st->cmax_bits[a.slot ^ 1] = (unsigned long long)__double_as_longlong(kTiny);
+  if (a.first && gid < rc.nruns) {
+    // reset the maximum-speed accumulators of the next step (new_time_step:
+    // cmax = tiny before the reduction), one thread per run
+    a.rs[gid].cmax_bits[a.slot ^ 1] = (unsigned long long)__double_as_longlong(kTiny);
   }
 
-  const double cmaxd = cmax_of(a.st, a.slot);
+  const int run = active ? a.brun[b] : 0;
+  const RunState* R = a.rs + run;
+  const double cmaxd = cmax_of(R, a.slot);
   const T cmax = T(cmaxd), cmax2 = T(cmaxd * cmaxd);
-  const double dt = dt_of(rc, a.st, a.slot, a.dtnext);
+  const double dt = dt_of(rc, R, a.slot, a.dtnext);
   const double ds = a.fac * dt;
   const EosConst<T> e{T(rc.gamma), T(rc.gammam1), T(rc.gammam1i)};
   const EosConst<double> ed{rc.gamma, rc.gammam1, rc.gammam1i};
@@ -373,11 +418,11 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a, RunConst rc) {
             coords(c0 + m, i, j, k);
             const size_t pk = cell_index<sizeof(T)>(a.nbr, rc, b, i, j, k);
 #pragma unroll
-            for (int v = 0; v < NV; v++) sq[(m * NV + v) * 256 + tid] = Qp[var_index<NV>(rc, pk, v)];
+            for (int v = 0; v < NV; v++) sq[(m * NV + v) * wg + tid] = Qp[var_index<NV>(rc, pk, v)];
           }
         }
 #pragma unroll
-        for (int v = 0; v < NV; v++) q[v] = sq[((c & 3) * NV + v) * 256 + tid];
+        for (int v = 0; v < NV; v++) q[v] = sq[((c & 3) * NV + v) * wg + tid];
       } else {
         int i, j, k;
         coords(c, i, j, k);
@@ -428,7 +473,7 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a, RunConst rc) {
             for (int m = 0; m < 4; m++)
 #pragma unroll
               for (int v = 0; v < NV; v++)
-                sd[(m * NV + v) * 256 + tid] = Dp[base[m] + (size_t)v * rc.ncell];
+                sd[(m * NV + v) * wg + tid] = Dp[base[m] + (size_t)v * rc.ncell];
           }
         }
 #pragma unroll
@@ -470,7 +515,7 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a, RunConst rc) {
           const size_t bt = bb + (size_t)lidx<sizeof(T)>(nc, ND, i, j, k);
           const size_t bd = bb + (size_t)lidx<8>(nc, ND, i, j, k);
           const size_t bf = bb + (size_t)lidx<4>(nc, ND, i, j, k);
-          const T dxi = T(rc.dxi[DIR]);
+          const T dxi = T(R->dxi[DIR]);
           double unew[NV];
 #pragma unroll
           for (int v = 0; v < NV; v++) {
@@ -479,10 +524,10 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a, RunConst rc) {
             const T dfx = f[v] - fprev[v];
             T du;
             if constexpr (DIR == 0) du = -dxi * dfx;
-            else if constexpr (kDin) du = sd[(((c - 1) & 3) * NV + v) * 256 + tid] - dxi * dfx;
+            else if constexpr (kDin) du = sd[(((c - 1) & 3) * NV + v) * wg + tid] - dxi * dfx;
             else du = dpre[v] - dxi * dfx;
             if constexpr (!kLast) {
-              if constexpr (kDout) sd[(((c - 1) & 3) * NV + v) * 256 + tid] = du;
+              if constexpr (kDout) sd[(((c - 1) & 3) * NV + v) * wg + tid] = du;
               else Dp[bt + off] = du;
             } else if constexpr (sizeof(T) == 8) {
               double un = combine(a.kind, u0pre[v], u1pre[v], ds, du);
@@ -542,7 +587,7 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a, RunConst rc) {
             for (int m = 0; m < 4; m++)
 #pragma unroll
               for (int v = 0; v < NV; v++)
-                Dp[base[m] + (size_t)v * rc.ncell] = sd[(m * NV + v) * 256 + tid];
+                Dp[base[m] + (size_t)v * rc.ncell] = sd[(m * NV + v) * wg + tid];
           }
         }
 #pragma unroll
@@ -554,14 +599,7 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a, RunConst rc) {
   }
 
   if constexpr (kLast) {
-    if (a.last) {
-      // wave-level max, one atomic per wave (positive doubles order as
-      // their bit patterns)
-      for (int off = 32; off > 0; off >>= 1) smax = fmax(smax, __shfl_xor(smax, off));
-      if ((threadIdx.x & 63) == 0 && smax > 0.0)
-        atomicMax(&a.st->cmax_bits[a.slot ^ 1],
-                  (unsigned long long)__double_as_longlong(smax));
-    }
+    if (a.last) wave_max_speed(a.rs, run, a.slot ^ 1, smax);
   }
 }
 
@@ -569,16 +607,19 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a, RunConst rc) {
 // (update_primitive_variables + maxspeed); used after uploads
 template <class P, typename T>
 __global__ __launch_bounds__(256) void k_prim(const double* __restrict__ U,
-                                              void* Qv, RunConst rc,
-                                              DevState* st, int slot) {
+                                              void* Qv, RunConst rc, DevState* st,
+                                              RunState* rs, const int* __restrict__ brun,
+                                              int slot, int b0, int nb) {
   T* __restrict__ Q = static_cast<T*>(Qv);
   constexpr int NV = P::nv;
-  const long long n = (long long)rc.nblk * rc.ncell;
+  const long long n = (long long)nb * rc.ncell;
   const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const EosConst<double> e{rc.gamma, rc.gammam1, rc.gammam1i};
   double smax = 0.0;
+  int run = 0;
   if (gid < n) {
-    const long long b = gid / rc.ncell, c = gid % rc.ncell;
+    const long long b = b0 + gid / rc.ncell, c = gid % rc.ncell;
+    run = brun[b];
     const int i = (int)(c % rc.nc), j = (int)((c / rc.nc) % rc.nc), k = (int)(c / ((long long)rc.nc * rc.nc));
     const long long cu = lidx<8>(rc.nc, rc.nd, i, j, k), cq = lidx<sizeof(T)>(rc.nc, rc.nd, i, j, k);
     double u[NV], q[NV];
@@ -597,9 +638,7 @@ __global__ __launch_bounds__(256) void k_prim(const double* __restrict__ U,
     for (int v = 0; v < NV; v++) Q[(b * NV + v) * rc.ncell + cq] = T(q[v]);
     smax = maxspeed<P, double>(e, q);
   }
-  for (int off = 32; off > 0; off >>= 1) smax = fmax(smax, __shfl_xor(smax, off));
-  if ((threadIdx.x & 63) == 0 && smax > 0.0)
-    atomicMax(&st->cmax_bits[slot], (unsigned long long)__double_as_longlong(smax));
+  wave_max_speed(rs, run, slot, smax);
 }
 
 __global__ void k_zero_f32(float* x, long long n) {
@@ -608,35 +647,47 @@ __global__ void k_zero_f32(float* x, long long n) {
     x[g] = 0.0f;
 }
 
-__global__ void k_reset_slot(DevState* st, int slot) {
-  st->cmax_bits[slot] = (unsigned long long)__double_as_longlong(kTiny);
+__global__ void k_reset_slot(RunState* rs, int slot, int r0, int nr) {
+  const int r = r0 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (r < r0 + nr) rs[r].cmax_bits[slot] = (unsigned long long)__double_as_longlong(kTiny);
 }
 
-__global__ void k_end_step(DevState* st, RunConst rc, int slot, double dtnext) {
-  double dt = dt_of(rc, st, slot, dtnext);
-  st->trace[st->step & (kTrace - 1)] = dt;
-  st->dt = dt;
-  st->time += dt;
-  st->step += 1;
+// end of a step, one thread per run: dt into the trace ring
+// trace[(step mod tsteps) * nruns + run], time and step advanced
+__global__ void k_end_step(RunState* rs, double* trace, int tsteps, RunConst rc, int slot,
+                           double dtnext) {
+  const int r = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (r >= rc.nruns) return;
+  RunState* R = rs + r;
+  const double dt = dt_of(rc, R, slot, dtnext);
+  trace[(size_t)(R->step % tsteps) * rc.nruns + r] = dt;
+  R->dt = dt;
+  R->time += dt;
+  R->step += 1;
 }
 
-// dt of the current slot without advancing (new_time_step alone)
-__global__ void k_peek_dt(DevState* st, RunConst rc, int slot, double dtnext,
-                          double* out) {
-  double dt = dt_of(rc, st, slot, dtnext);
-  st->dt = dt;
-  out[0] = dt;
-  out[1] = cmax_of(st, slot);
+// dt of the current slot without advancing (new_time_step alone), per run:
+// out[2 r] = dt, out[2 r + 1] = cmax
+__global__ void k_peek_dt(RunState* rs, RunConst rc, int slot, double dtnext, double* out) {
+  const int r = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (r >= rc.nruns) return;
+  RunState* R = rs + r;
+  const double dt = dt_of(rc, R, slot, dtnext);
+  R->dt = dt;
+  out[2 * r] = dt;
+  out[2 * r + 1] = cmax_of(R, slot);
 }
 
 // volume integrals (store_integrals, integrals.F90:331-358): per-workgroup
 // partial sums, then a fixed-order final sum (deterministic)
 template <class P>
 __global__ __launch_bounds__(256) void k_integrals(const double* __restrict__ U,
-                                                   RunConst rc, double* part) {
+                                                   RunConst rc, double* part, int b0,
+                                                   int nb) {
   constexpr int NV = P::nv;
   __shared__ double sh[8][256];
-  const long long n = (long long)rc.nblk * rc.ncell;
+  U += (size_t)b0 * NV * rc.ncell;
+  const long long n = (long long)nb * rc.ncell;
   double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < n;
        g += (long long)gridDim.x * blockDim.x) {
@@ -692,6 +743,77 @@ __global__ void k_pack_blocks(const void* Xv, double* __restrict__ out,
   }
 }
 
+// ---------------------------------------------------------------------------
+// unit-level kernels (parity tests): the pointwise routines of the sweeps on
+// vectors of states, in the arithmetic type of the configuration, one thread
+// per state (the riemann_iface / prim2cons / cons2prim / fluxspeed seams of
+// schemes.F90:66-71 and equations.F90:70-108)
+// ---------------------------------------------------------------------------
+template <class P, int RS, typename T>
+__global__ void k_unit_riemann(int n, double cmaxd, const double* __restrict__ ql,
+                               const double* __restrict__ qr, double* __restrict__ f,
+                               RunConst rc) {
+  constexpr int NV = P::nv;
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i >= n) return;
+  const EosConst<T> e{T(rc.gamma), T(rc.gammam1), T(rc.gammam1i)};
+  T a[NV], b[NV], g[NV];
+#pragma unroll
+  for (int v = 0; v < NV; v++) {
+    a[v] = T(ql[(size_t)i * NV + v]);
+    b[v] = T(qr[(size_t)i * NV + v]);
+  }
+  riemann<P, RS, T>(e, rc.sc, T(cmaxd), T(cmaxd * cmaxd), a, b, g);
+#pragma unroll
+  for (int v = 0; v < NV; v++) f[(size_t)i * NV + v] = double(g[v]);
+}
+
+// u = prim2cons(q), f/cm/cp = fluxspeed(q, u) in T; q2 = cons2prim(u) in
+// double (the kernels recover primitives in FP64 in every mode; q2 enters
+// holding the previous primitives, kept by a failed relativistic recovery)
+template <class P, typename T>
+__global__ void k_unit_eqs(int n, double cmaxd, const double* __restrict__ q,
+                           double* __restrict__ u, double* __restrict__ q2,
+                           double* __restrict__ f, double* __restrict__ cm,
+                           double* __restrict__ cp, RunConst rc, DevState* st) {
+  constexpr int NV = P::nv;
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i >= n) return;
+  const EosConst<T> e{T(rc.gamma), T(rc.gammam1), T(rc.gammam1i)};
+  const EosConst<double> ed{rc.gamma, rc.gammam1, rc.gammam1i};
+  T qq[NV], uu[NV], ff[NV], m, pp;
+#pragma unroll
+  for (int v = 0; v < NV; v++) qq[v] = T(q[(size_t)i * NV + v]);
+  const T cmax2 = T(cmaxd * cmaxd);
+  if constexpr (P::sr) {
+    prim2cons_sr<P>(T(rc.sc.rgammaxi), qq, uu);
+    fluxspeed_sr<P>(rc.sc, cmax2, qq, uu, ff, m, pp);
+  } else {
+    prim2cons<P>(e, qq, uu);
+    fluxspeed<P>(e, cmax2, qq, uu, ff, m, pp);
+  }
+  double ud[NV], qd[NV];
+#pragma unroll
+  for (int v = 0; v < NV; v++) {
+    ud[v] = double(uu[v]);
+    qd[v] = q2[(size_t)i * NV + v];
+  }
+  if constexpr (P::sr) {
+    const int cs = cons2prim_sr<P>(rc.sc, ud, qd);
+    if (cs) c2p_count(st, cs);
+  } else {
+    cons2prim<P, double>(ed, ud, qd);
+  }
+#pragma unroll
+  for (int v = 0; v < NV; v++) {
+    u[(size_t)i * NV + v] = ud[v];
+    q2[(size_t)i * NV + v] = qd[v];
+    f[(size_t)i * NV + v] = double(ff[v]);
+  }
+  cm[i] = double(m);
+  cp[i] = double(pp);
+}
+
 typedef void (*SweepFn)(SweepArgs, RunConst);
 
 }  // namespace
@@ -699,10 +821,14 @@ typedef void (*SweepFn)(SweepArgs, RunConst);
 // kernel instantiations selected for one configuration
 struct Dispatch {
   SweepFn sweep[3] = {nullptr, nullptr, nullptr};
-  void (*prim)(const double*, void*, RunConst, DevState*, int) = nullptr;
-  void (*integ)(const double*, RunConst, double*) = nullptr;
+  void (*prim)(const double*, void*, RunConst, DevState*, RunState*, const int*, int, int,
+               int) = nullptr;
+  void (*integ)(const double*, RunConst, double*, int, int) = nullptr;
   void (*pack_u)(const void*, double*, const int*, RunConst, int) = nullptr;
   void (*pack_q)(const void*, double*, const int*, RunConst, int) = nullptr;
+  void (*unit_riemann)(int, double, const double*, const double*, double*, RunConst) = nullptr;
+  void (*unit_eqs)(int, double, const double*, double*, double*, double*, double*, double*,
+                   RunConst, DevState*) = nullptr;
 };
 
 // ===========================================================================
@@ -712,19 +838,33 @@ struct amun_ctx {
   amun_cfg cfg;
   RunConst rc;
   int nv;
-  size_t nstate;            // doubles per state array
+  size_t nstate;            // doubles per state array (all runs)
   double *U0 = nullptr, *U1 = nullptr;
   void *Q[2] = {nullptr, nullptr}, *D = nullptr;   // element size qsize
   float* C = nullptr;       // Kahan compensation of U0 (mixed mode)
   size_t qsize = 8;
   int* nbr = nullptr;
   DevState* st = nullptr;
+  RunState* rs = nullptr;   // per-run step state
+  int* brun = nullptr;      // run of every block
+  double* trace = nullptr;  // dt ring, [tsteps][nruns]
+  int tsteps = 0;
   double* scratch = nullptr;   // reductions / small transfers
   int qcur = 0;                // Q[qcur] holds the primitive state of U0
   long long nsteps = 0;        // steps launched (slot = nsteps & 1)
   hipStream_t stream = nullptr;
-  std::vector<int32_t> coords;
+  std::vector<amun_grid> grids;     // one per run
+  std::vector<int32_t> run_b0;      // first block of every run (+ total)
+  std::vector<int32_t> coords;      // block coordinates inside its run
   Dispatch disp;
+  // launch shapes of the sweeps (workgroup size, XCD-aware order), per
+  // direction; set by amun_dev_autotune
+  int wg[3] = {256, 256, 256};
+  int swz[3] = {0, 0, 0};
+  // captured steps: one graph per (cmax slot, primitive buffer, dtnext)
+  bool use_graph = false;
+  struct Graph { int slot, qcur; double dtnext; hipGraphExec_t exec; };
+  std::vector<Graph> graphs;
 };
 
 namespace {
@@ -739,6 +879,8 @@ void fill(Dispatch& d) {
   d.integ = k_integrals<P>;
   d.pack_u = k_pack_blocks<P::nv, double>;
   d.pack_q = k_pack_blocks<P::nv, T>;
+  d.unit_riemann = k_unit_riemann<P, RS, T>;
+  d.unit_eqs = k_unit_eqs<P, T>;
 }
 
 template <class P, int RS, typename T>
@@ -801,13 +943,54 @@ int validate(const amun_cfg& c, int* nv) {
   if (c.nghosts < need)
     return fail(600, "nghosts too small for the reconstruction stencil (WENO5-Z needs 4, TVD 2)");
   if (c.ncells < 3) return fail(500, "ncells too small for the stencil");
-  if (c.xblocks < 1 || c.yblocks < 1 || (c.ndims == 3 && c.zblocks < 1))
-    return fail(500, "block counts must be positive");
   if (!(c.gamma > 1.0) || !(c.cfl > 0.0)) return fail(500, "gamma must exceed 1 and cfl be positive");
   return 0;
 }
 
 Dispatch& dispatch_of(amun_ctx* ctx) { return ctx->disp; }
+
+// run constants of a configuration (block counts are filled in by the caller)
+RunConst make_rc(const amun_cfg& cfg) {
+  RunConst rc;
+  std::memset(&rc, 0, sizeof(rc));
+  const int nd = cfg.ndims, nc = cfg.ncells;
+  rc.nd = nd;
+  rc.nc = nc;
+  rc.nruns = 1;
+  rc.ncell = (nd == 3) ? nc * nc * nc : nc * nc;
+  rc.limiter = cfg.limiter;
+  rc.gamma = cfg.gamma;
+  rc.gammam1 = cfg.gamma - 1.0;
+  rc.gammam1i = 1.0 / rc.gammam1;
+  rc.cfl = cfg.cfl;
+  rc.eps = cfg.eps;
+  rc.decay = std::exp(-cfg.alpha * cfg.cfl);
+  rc.sc.gamma = cfg.gamma;
+  rc.sc.gammaxi = rc.gammam1 / cfg.gamma;       // equations.F90:843
+  rc.sc.rgammaxi = 1.0 / rc.sc.gammaxi;
+  rc.sc.tol = cfg.tolerance;
+  rc.sc.pmin = cfg.pmin;
+  rc.sc.nrmax = cfg.nr_maxit;
+  rc.sc.nrext = cfg.nr_extra;
+  return rc;
+}
+
+int validate_grid(const amun_cfg& c, const amun_grid& g) {
+  if (g.xblocks < 1 || g.yblocks < 1 || (c.ndims == 3 && g.zblocks < 1))
+    return fail(500, "block counts must be positive");
+  if (!(g.xmax > g.xmin) || !(g.ymax > g.ymin) || (c.ndims == 3 && !(g.zmax > g.zmin)))
+    return fail(500, "domain bounds must be increasing");
+  return 0;
+}
+
+// LDS bytes of the sweep along dir for a workgroup of wg lanes (chunked
+// pencil I/O of the x and y sweeps: primitive slots, increment slots)
+size_t sweep_lds(const amun_ctx* c, int dir, int wg) {
+  const bool chunk = dir < 2, last = dir == c->rc.nd - 1;
+  const bool din = chunk && dir > 0, dout = chunk && !last;
+  const int nbuf = (chunk ? 1 : 0) + ((din || dout) ? 1 : 0);
+  return (size_t)nbuf * 4 * c->nv * wg * c->qsize;
+}
 
 }  // namespace
 
@@ -840,12 +1023,15 @@ void amun_dev_default_cfg(amun_cfg* c) {
   c->nr_extra = 2;
 }
 
-int amun_dev_init(const amun_cfg* cfg, amun_ctx** out) {
-  if (!cfg || !out) return fail(500, "null argument");
+int amun_dev_init_batch(const amun_cfg* cfg, int32_t nruns, const amun_grid* grids,
+                        amun_ctx** out) {
+  if (!cfg || !out || !grids || nruns < 1) return fail(500, "bad argument");
   *out = nullptr;
   int nv = 0;
   int r = validate(*cfg, &nv);
   if (r) return r;
+  for (int g = 0; g < nruns; g++)
+    if ((r = validate_grid(*cfg, grids[g]))) return r;
   int ndev = 0;
   HIP_TRY(hipGetDeviceCount(&ndev));
   if (ndev < 1) return fail(1000 + (int)hipErrorNoDevice, "no HIP device");
@@ -854,55 +1040,61 @@ int amun_dev_init(const amun_cfg* cfg, amun_ctx** out) {
   c->cfg = *cfg;
   c->cfg.nv = nv;
   c->nv = nv;
+  c->grids.assign(grids, grids + nruns);
   const int nd = cfg->ndims, nc = cfg->ncells;
-  const int ir = cfg->xblocks, jr = cfg->yblocks, kr = (nd == 3) ? cfg->zblocks : 1;
   RunConst& rc = c->rc;
-  rc.nd = nd;
-  rc.nc = nc;
-  rc.nblk = ir * jr * kr;
-  rc.ncell = (nd == 3) ? nc * nc * nc : nc * nc;
-  rc.limiter = cfg->limiter;
-  rc.gamma = cfg->gamma;
-  rc.gammam1 = cfg->gamma - 1.0;
-  rc.gammam1i = 1.0 / rc.gammam1;
-  rc.cfl = cfg->cfl;
-  rc.eps = cfg->eps;
-  rc.decay = std::exp(-cfg->alpha * cfg->cfl);
-  rc.sc.gamma = cfg->gamma;
-  rc.sc.gammaxi = rc.gammam1 / cfg->gamma;       // equations.F90:843
-  rc.sc.rgammaxi = 1.0 / rc.sc.gammaxi;
-  rc.sc.tol = cfg->tolerance;
-  rc.sc.pmin = cfg->pmin;
-  rc.sc.nrmax = cfg->nr_maxit;
-  rc.sc.nrext = cfg->nr_extra;
-  // cell sizes of coordinates.F90:330-348 (one level)
-  double adx = (cfg->xmax - cfg->xmin) / (ir * nc);
-  double ady = (cfg->ymax - cfg->ymin) / (jr * nc);
-  double adz = (nd == 3) ? (cfg->zmax - cfg->zmin) / (kr * nc) : 1.0;
-  rc.dxi[0] = 1.0 / adx;
-  rc.dxi[1] = 1.0 / ady;
-  rc.dxi[2] = (nd == 3) ? 1.0 / adz : 1.0;
-  rc.dxmin = std::fmin(adx, ady);
-  if (nd == 3) rc.dxmin = std::fmin(rc.dxmin, adz);
+  rc = make_rc(*cfg);
+  rc.nruns = nruns;
 
-  // periodic base-level neighbour table (setup_domain_default wiring)
-  std::vector<int32_t> nbr((size_t)rc.nblk * 27, -1);
+  // runs one after another in the pool; every run is a periodic base-level
+  // block grid (setup_domain_default wiring) with its own cell sizes
+  // (coordinates.F90:330-348) and its own step state
+  std::vector<RunState> hrs(nruns);
+  c->run_b0.assign(nruns + 1, 0);
+  for (int g = 0; g < nruns; g++) {
+    const amun_grid& G = grids[g];
+    const int kr = (nd == 3) ? G.zblocks : 1;
+    c->run_b0[g + 1] = c->run_b0[g] + G.xblocks * G.yblocks * kr;
+    RunState& R = hrs[g];
+    std::memset(&R, 0, sizeof(R));
+    std::memcpy(&R.cmax_bits[0], &kTiny, 8);
+    std::memcpy(&R.cmax_bits[1], &kTiny, 8);
+    R.dt = 1.0;
+    const double adx = (G.xmax - G.xmin) / (G.xblocks * nc);
+    const double ady = (G.ymax - G.ymin) / (G.yblocks * nc);
+    const double adz = (nd == 3) ? (G.zmax - G.zmin) / (kr * nc) : 1.0;
+    R.dxi[0] = 1.0 / adx;
+    R.dxi[1] = 1.0 / ady;
+    R.dxi[2] = (nd == 3) ? 1.0 / adz : 1.0;
+    R.dxmin = std::fmin(adx, ady);
+    if (nd == 3) R.dxmin = std::fmin(R.dxmin, adz);
+  }
+  rc.nblk = c->run_b0[nruns];
+  std::vector<int32_t> nbr((size_t)rc.nblk * 27, -1), brun(rc.nblk);
   c->coords.resize((size_t)rc.nblk * 3);
-  for (int bz = 0; bz < kr; bz++)
-    for (int by = 0; by < jr; by++)
-      for (int bx = 0; bx < ir; bx++) {
-        int b = (bz * jr + by) * ir + bx;
-        c->coords[b * 3 + 0] = bx;
-        c->coords[b * 3 + 1] = by;
-        c->coords[b * 3 + 2] = bz;
-        for (int dk = -1; dk <= 1; dk++)
-          for (int dj = -1; dj <= 1; dj++)
-            for (int di = -1; di <= 1; di++) {
-              if (nd == 2 && dk != 0) continue;
-              int nx = (bx + di + ir) % ir, ny = (by + dj + jr) % jr, nz = (bz + dk + kr) % kr;
-              nbr[(size_t)b * 27 + (dk + 1) * 9 + (dj + 1) * 3 + (di + 1)] = (nz * jr + ny) * ir + nx;
-            }
-      }
+  for (int g = 0; g < nruns; g++) {
+    const int ir = grids[g].xblocks, jr = grids[g].yblocks;
+    const int kr = (nd == 3) ? grids[g].zblocks : 1, b0 = c->run_b0[g];
+    for (int bz = 0; bz < kr; bz++)
+      for (int by = 0; by < jr; by++)
+        for (int bx = 0; bx < ir; bx++) {
+          const int b = b0 + (bz * jr + by) * ir + bx;
+          brun[b] = g;
+          c->coords[b * 3 + 0] = bx;
+          c->coords[b * 3 + 1] = by;
+          c->coords[b * 3 + 2] = bz;
+          for (int dk = -1; dk <= 1; dk++)
+            for (int dj = -1; dj <= 1; dj++)
+              for (int di = -1; di <= 1; di++) {
+                if (nd == 2 && dk != 0) continue;
+                const int nx = (bx + di + ir) % ir, ny = (by + dj + jr) % jr;
+                const int nz = (bz + dk + kr) % kr;
+                nbr[(size_t)b * 27 + (dk + 1) * 9 + (dj + 1) * 3 + (di + 1)] =
+                    b0 + (nz * jr + ny) * ir + nx;
+              }
+        }
+  }
+  c->tsteps = (nruns == 1) ? kTrace : std::max(64, kTrace / nruns);
 
   c->nstate = (size_t)rc.nblk * nv * rc.ncell;
   const size_t bytes = c->nstate * sizeof(double);
@@ -924,8 +1116,11 @@ int amun_dev_init(const amun_cfg* cfg, amun_ctx** out) {
     ALLOC(c->C, c->nstate * sizeof(float));
   }
   ALLOC(c->nbr, nbr.size() * sizeof(int32_t));
+  ALLOC(c->brun, brun.size() * sizeof(int32_t));
   ALLOC(c->st, sizeof(DevState));
-  ALLOC(c->scratch, 4096 * 8 * sizeof(double));
+  ALLOC(c->rs, (size_t)nruns * sizeof(RunState));
+  ALLOC(c->trace, (size_t)nruns * c->tsteps * sizeof(double));
+  ALLOC(c->scratch, (size_t)std::max(4096 * 8, 2 * nruns) * sizeof(double));
 #undef ALLOC
   e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
@@ -933,15 +1128,11 @@ int amun_dev_init(const amun_cfg* cfg, amun_ctx** out) {
     return fail(1000 + (int)e, "hipStreamCreate");
   }
   e = hipMemcpy(c->nbr, nbr.data(), nbr.size() * sizeof(int32_t), hipMemcpyHostToDevice);
-  if (e == hipSuccess) {
-    DevState h;
-    std::memset(&h, 0, sizeof(h));
-    h.cmax_bits[0] = h.cmax_bits[1] = 0;
-    std::memcpy(&h.cmax_bits[0], &kTiny, 8);
-    std::memcpy(&h.cmax_bits[1], &kTiny, 8);
-    h.dt = 1.0;
-    e = hipMemcpy(c->st, &h, sizeof(DevState), hipMemcpyHostToDevice);
-  }
+  if (e == hipSuccess)
+    e = hipMemcpy(c->brun, brun.data(), brun.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(c->rs, hrs.data(), hrs.size() * sizeof(RunState), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemset(c->st, 0, sizeof(DevState));
   if (e == hipSuccess) e = hipMemset(c->U0, 0, bytes);
   // primitive arrays start at zero: a relativistic cell whose recovery fails
   // keeps the previous primitive values, as the reference's q array does
@@ -955,13 +1146,45 @@ int amun_dev_init(const amun_cfg* cfg, amun_ctx** out) {
     amun_dev_finalize(c);
     return fail(600, "no kernel instantiation for this configuration");
   }
+  // the x and y sweeps stage pencil chunks in LDS: allow the largest
+  // workgroup's share (above the 64 KiB default of dynamic LDS)
+  for (int dir = 0; dir < nd; dir++) {
+    const size_t lds = sweep_lds(c, dir, 256);
+    if (!lds) continue;
+    e = hipFuncSetAttribute((const void*)c->disp.sweep[dir],
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) {
+      amun_dev_finalize(c);
+      return fail(1000 + (int)e, "hipFuncSetAttribute (LDS)");
+    }
+  }
   *out = c;
   return 0;
+}
+
+int amun_dev_init(const amun_cfg* cfg, amun_ctx** out) {
+  if (!cfg || !out) return fail(500, "null argument");
+  amun_grid g;
+  std::memset(&g, 0, sizeof(g));
+  g.xblocks = cfg->xblocks;
+  g.yblocks = cfg->yblocks;
+  g.zblocks = (cfg->ndims == 3) ? cfg->zblocks : 1;
+  g.xmin = cfg->xmin; g.xmax = cfg->xmax;
+  g.ymin = cfg->ymin; g.ymax = cfg->ymax;
+  g.zmin = cfg->zmin; g.zmax = cfg->zmax;
+  if (cfg->ndims != 3) { g.zmin = 0.0; g.zmax = 1.0; }
+  return amun_dev_init_batch(cfg, 1, &g, out);
+}
+
+static void drop_graphs(amun_ctx* c) {
+  for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.exec);
+  c->graphs.clear();
 }
 
 void amun_dev_finalize(amun_ctx* c) {
   if (!c) return;
   if (c->stream) hipStreamSynchronize(c->stream);
+  drop_graphs(c);
   hipFree(c->U0);
   hipFree(c->U1);
   hipFree(c->Q[0]);
@@ -969,50 +1192,97 @@ void amun_dev_finalize(amun_ctx* c) {
   hipFree(c->D);
   hipFree(c->C);
   hipFree(c->nbr);
+  hipFree(c->brun);
   hipFree(c->st);
+  hipFree(c->rs);
+  hipFree(c->trace);
   hipFree(c->scratch);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
 }
 
-static int after_upload(amun_ctx* c) {
+// primitive variables and maximum speed of runs [r0, r0 + nr) after their
+// conserved state changed on the host side
+static int after_upload(amun_ctx* c, int r0, int nr) {
   Dispatch& d = dispatch_of(c);
   const int slot = (int)(c->nsteps & 1);
-  hipLaunchKernelGGL(k_reset_slot, dim3(1), dim3(1), 0, c->stream, c->st, slot);
-  long long n = (long long)c->rc.nblk * c->rc.ncell;
+  const int b0 = c->run_b0[r0], nb = c->run_b0[r0 + nr] - b0;
+  hipLaunchKernelGGL(k_reset_slot, dim3((unsigned)((nr + 255) / 256)), dim3(256), 0,
+                     c->stream, c->rs, slot, r0, nr);
+  const long long n = (long long)nb * c->rc.ncell;
   // in place over the current primitive array (update_primitive_variables
   // overwrites pdata%q)
   hipLaunchKernelGGL(d.prim, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
-                     c->U0, c->Q[c->qcur], c->rc, c->st, slot);
+                     c->U0, c->Q[c->qcur], c->rc, c->st, c->rs, c->brun, slot, b0, nb);
   if (c->C)
-    hipLaunchKernelGGL(k_zero_f32, dim3(2048), dim3(256), 0, c->stream, c->C,
-                       (long long)c->nstate);
+    hipLaunchKernelGGL(k_zero_f32, dim3(2048), dim3(256), 0, c->stream,
+                       c->C + (size_t)b0 * c->nv * c->rc.ncell,
+                       (long long)nb * c->nv * c->rc.ncell);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(c->stream));
   return 0;
 }
 
-int amun_dev_upload_global(amun_ctx* c, const double* u) {
-  if (!c || !u) return fail(500, "null argument");
+// global array U[nv][nz][ny][nx] of run r <-> its blocks (h: the run's
+// segment of the pool, FP64 layout of lidx<8>)
+static void scatter_run(const amun_ctx* c, int r, const double* u, double* h) {
   const RunConst& rc = c->rc;
-  const int nc = rc.nc, nv = c->nv;
-  const int ir = c->cfg.xblocks, jr = c->cfg.yblocks;
-  const size_t nx = (size_t)ir * nc, ny = (size_t)jr * nc;
-  const size_t nz = (rc.nd == 3) ? (size_t)c->cfg.zblocks * nc : 1;
+  const int nc = rc.nc, nv = c->nv, b0 = c->run_b0[r], b1 = c->run_b0[r + 1];
+  const amun_grid& G = c->grids[r];
+  const size_t nx = (size_t)G.xblocks * nc, ny = (size_t)G.yblocks * nc;
+  const size_t nz = (rc.nd == 3) ? (size_t)G.zblocks * nc : 1;
   const int kn = (rc.nd == 3) ? nc : 1;
-  std::vector<double> h(c->nstate);
-  for (int b = 0; b < rc.nblk; b++) {
-    size_t bx = c->coords[b * 3], by = c->coords[b * 3 + 1], bz = c->coords[b * 3 + 2];
+  for (int b = b0; b < b1; b++) {
+    const size_t bx = c->coords[b * 3], by = c->coords[b * 3 + 1], bz = c->coords[b * 3 + 2];
     for (int v = 0; v < nv; v++)
       for (int k = 0; k < kn; k++)
         for (int j = 0; j < nc; j++) {
           const double* src = u + ((v * nz + bz * kn + k) * ny + by * nc + j) * nx + bx * nc;
-          double* dst = &h[((size_t)b * nv + v) * rc.ncell];
+          double* dst = &h[((size_t)(b - b0) * nv + v) * rc.ncell];
           for (int i = 0; i < nc; i++) dst[lidx<8>(nc, rc.nd, i, j, k)] = src[i];
         }
   }
-  HIP_TRY(hipMemcpy(c->U0, h.data(), c->nstate * sizeof(double), hipMemcpyHostToDevice));
-  return after_upload(c);
+}
+
+static void gather_run(const amun_ctx* c, int r, const double* h, double* u, int es) {
+  const RunConst& rc = c->rc;
+  const int nc = rc.nc, nv = c->nv, b0 = c->run_b0[r], b1 = c->run_b0[r + 1];
+  const amun_grid& G = c->grids[r];
+  const size_t nx = (size_t)G.xblocks * nc, ny = (size_t)G.yblocks * nc;
+  const size_t nz = (rc.nd == 3) ? (size_t)G.zblocks * nc : 1;
+  const int kn = (rc.nd == 3) ? nc : 1;
+  for (int b = b0; b < b1; b++) {
+    const size_t bx = c->coords[b * 3], by = c->coords[b * 3 + 1], bz = c->coords[b * 3 + 2];
+    for (int v = 0; v < nv; v++)
+      for (int k = 0; k < kn; k++)
+        for (int j = 0; j < nc; j++) {
+          double* dst = u + ((v * nz + bz * kn + k) * ny + by * nc + j) * nx + bx * nc;
+          const double* src = &h[((size_t)(b - b0) * nv + v) * rc.ncell];
+          if (es == 8)
+            for (int i = 0; i < nc; i++) dst[i] = src[lidx<8>(nc, rc.nd, i, j, k)];
+          else
+            for (int i = 0; i < nc; i++) dst[i] = src[lidx<4>(nc, rc.nd, i, j, k)];
+        }
+  }
+}
+
+int amun_dev_upload_run(amun_ctx* c, int32_t r, const double* u) {
+  if (!c || !u) return fail(500, "null argument");
+  if (r < 0 || r >= c->rc.nruns) return fail(500, "run index out of range");
+  const size_t per = (size_t)c->nv * c->rc.ncell;
+  const size_t n = (size_t)(c->run_b0[r + 1] - c->run_b0[r]) * per;
+  std::vector<double> h(n);
+  scatter_run(c, r, u, h.data());
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipMemcpy(c->U0 + (size_t)c->run_b0[r] * per, h.data(), n * sizeof(double),
+                    hipMemcpyHostToDevice));
+  return after_upload(c, r, 1);
+}
+
+int amun_dev_upload_global(amun_ctx* c, const double* u) {
+  if (!c || !u) return fail(500, "null argument");
+  if (c->rc.nruns != 1) return fail(500, "a batch of runs is uploaded with amun_dev_upload_run");
+  return amun_dev_upload_run(c, 0, u);
 }
 
 int amun_dev_upload_blocks(amun_ctx* c, const double* u) {
@@ -1030,47 +1300,41 @@ int amun_dev_upload_blocks(amun_ctx* c, const double* u) {
           for (int v = 0; v < nv; v++)
             h[(((size_t)b * nv + v) * rc.ncell) + lidx<8>(nc, rc.nd, i, j, k)] =
                 u[b * per + ((((size_t)(k + kg) * im + (j + ng)) * im + (i + ng)) * nv) + v];
+  HIP_TRY(hipStreamSynchronize(c->stream));
   HIP_TRY(hipMemcpy(c->U0, h.data(), c->nstate * sizeof(double), hipMemcpyHostToDevice));
-  return after_upload(c);
+  return after_upload(c, 0, rc.nruns);
+}
+
+int amun_dev_download_run(amun_ctx* c, int32_t r, double* u, double* q) {
+  if (!c || !u) return fail(500, "null argument");
+  if (r < 0 || r >= c->rc.nruns) return fail(500, "run index out of range");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  const size_t per = (size_t)c->nv * c->rc.ncell;
+  const size_t off = (size_t)c->run_b0[r] * per;
+  const size_t n = (size_t)(c->run_b0[r + 1] - c->run_b0[r]) * per;
+  std::vector<double> h(n);
+  HIP_TRY(hipMemcpy(h.data(), c->U0 + off, n * sizeof(double), hipMemcpyDeviceToHost));
+  gather_run(c, r, h.data(), u, 8);
+  if (q) {
+    if (c->qsize == sizeof(double)) {
+      HIP_TRY(hipMemcpy(h.data(), (const double*)c->Q[c->qcur] + off, n * sizeof(double),
+                        hipMemcpyDeviceToHost));
+      gather_run(c, r, h.data(), q, 8);
+    } else {
+      std::vector<float> hf(n);
+      HIP_TRY(hipMemcpy(hf.data(), (const float*)c->Q[c->qcur] + off, n * sizeof(float),
+                        hipMemcpyDeviceToHost));
+      for (size_t m = 0; m < n; m++) h[m] = hf[m];
+      gather_run(c, r, h.data(), q, 4);
+    }
+  }
+  return 0;
 }
 
 int amun_dev_download_global(amun_ctx* c, double* u, double* q) {
   if (!c || !u) return fail(500, "null argument");
-  HIP_TRY(hipStreamSynchronize(c->stream));
-  const RunConst& rc = c->rc;
-  const int nc = rc.nc, nv = c->nv;
-  const int ir = c->cfg.xblocks, jr = c->cfg.yblocks;
-  const size_t nx = (size_t)ir * nc, ny = (size_t)jr * nc;
-  const size_t nz = (rc.nd == 3) ? (size_t)c->cfg.zblocks * nc : 1;
-  const int kn = (rc.nd == 3) ? nc : 1;
-  std::vector<double> h(c->nstate);
-  for (int pass = 0; pass < 2; pass++) {
-    double* dst_arr = pass == 0 ? u : q;
-    if (!dst_arr) continue;
-    if (pass == 0 || c->qsize == sizeof(double)) {
-      HIP_TRY(hipMemcpy(h.data(), pass == 0 ? (void*)c->U0 : c->Q[c->qcur],
-                        c->nstate * sizeof(double), hipMemcpyDeviceToHost));
-    } else {
-      std::vector<float> hf(c->nstate);
-      HIP_TRY(hipMemcpy(hf.data(), c->Q[c->qcur], c->nstate * sizeof(float),
-                        hipMemcpyDeviceToHost));
-      for (size_t n = 0; n < c->nstate; n++) h[n] = hf[n];
-    }
-    for (int b = 0; b < rc.nblk; b++) {
-      size_t bx = c->coords[b * 3], by = c->coords[b * 3 + 1], bz = c->coords[b * 3 + 2];
-      for (int v = 0; v < nv; v++)
-        for (int k = 0; k < kn; k++)
-          for (int j = 0; j < nc; j++) {
-            double* dst = dst_arr + ((v * nz + bz * kn + k) * ny + by * nc + j) * nx + bx * nc;
-            const double* src = &h[((size_t)b * nv + v) * rc.ncell];
-            if (pass == 0 || c->qsize == sizeof(double))
-              for (int i = 0; i < nc; i++) dst[i] = src[lidx<8>(nc, rc.nd, i, j, k)];
-            else
-              for (int i = 0; i < nc; i++) dst[i] = src[lidx<4>(nc, rc.nd, i, j, k)];
-          }
-    }
-  }
-  return 0;
+  if (c->rc.nruns != 1) return fail(500, "a batch of runs is downloaded with amun_dev_download_run");
+  return amun_dev_download_run(c, 0, u, q);
 }
 
 int amun_dev_download_blocks(amun_ctx* c, double* u, double* q) {
@@ -1102,69 +1366,116 @@ int amun_dev_download_blocks(amun_ctx* c, double* u, double* q) {
   return 0;
 }
 
-// launch one whole step asynchronously; with ev != nullptr an event is
-// recorded after every kernel (ev must hold 3*nd+2 events)
-static int launch_step(amun_ctx* c, double dtnext, hipEvent_t* ev = nullptr) {
+static int stages_per_step(const amun_ctx* c) {
+  return c->cfg.time_advance == AMUN_TA_EULER ? 1 : (c->cfg.time_advance == AMUN_TA_RK2 ? 2 : 3);
+}
+
+static int kernels_per_step(const amun_ctx* c) {
+  return stages_per_step(c) * c->rc.nd + 1;
+}
+
+// Enqueue one whole step for cmax slot `slot` with the primitive state in
+// Q[qcur]; host-side state is left alone.  With ev != nullptr an event is
+// recorded after every kernel.
+static int enqueue_step(amun_ctx* c, double dtnext, int slot, int qcur, hipEvent_t* ev) {
   Dispatch& d = dispatch_of(c);
   const RunConst& rc = c->rc;
-  const int slot = (int)(c->nsteps & 1);
   const int npen = (rc.nd == 3) ? rc.nc * rc.nc : rc.nc;
   const long long nthreads = (long long)rc.nblk * npen;
-  const dim3 grid((unsigned)((nthreads + 255) / 256)), block(256);
 
-  struct Stage { int kind; double fac; int src_is_u0; double* out; };
+  struct Stage { int kind; double fac; double* out; };
   std::vector<Stage> stages;
   switch (c->cfg.time_advance) {
   case AMUN_TA_EULER:
-    stages = {{0, 1.0, 1, c->U0}};
+    stages = {{0, 1.0, c->U0}};
     break;
   case AMUN_TA_RK2:
-    stages = {{0, 1.0, 1, c->U1}, {3, 1.0, 0, c->U0}};
+    stages = {{0, 1.0, c->U1}, {3, 1.0, c->U0}};
     break;
   default:
-    stages = {{0, 1.0, 1, c->U1}, {1, 1.0 / 4.0, 0, c->U1}, {2, 2.0 / 3.0, 0, c->U0}};
+    stages = {{0, 1.0, c->U1}, {1, 1.0 / 4.0, c->U1}, {2, 2.0 / 3.0, c->U0}};
   }
   for (size_t s = 0; s < stages.size(); s++) {
     SweepArgs a;
-    a.Q = c->Q[c->qcur];
+    a.Q = c->Q[qcur];
     a.U0 = c->U0;
     a.U1 = c->U1;
     a.Uout = stages[s].out;
-    a.Qout = c->Q[c->qcur ^ 1];
+    a.Qout = c->Q[qcur ^ 1];
     a.D = c->D;
     a.C = c->C;
     a.nbr = c->nbr;
     a.st = c->st;
+    a.rs = c->rs;
+    a.brun = c->brun;
     a.slot = slot;
     a.kind = stages[s].kind;
     a.last = (s + 1 == stages.size());
     a.fac = stages[s].fac;
     a.dtnext = dtnext;
     for (int dir = 0; dir < rc.nd; dir++) {
+      const int wg = c->wg[dir];
       a.first = (s == 0 && dir == 0);
-      hipLaunchKernelGGL(d.sweep[dir], grid, block, 0, c->stream, a, rc);
+      a.swz = c->swz[dir];
+      const dim3 grid((unsigned)((nthreads + wg - 1) / wg)), block(wg);
+      hipLaunchKernelGGL(d.sweep[dir], grid, block, sweep_lds(c, dir, wg), c->stream, a, rc);
       if (ev) HIP_TRY(hipEventRecord(*ev++, c->stream));
     }
-    c->qcur ^= 1;
+    qcur ^= 1;
   }
-  hipLaunchKernelGGL(k_end_step, dim3(1), dim3(1), 0, c->stream, c->st, rc, slot, dtnext);
+  hipLaunchKernelGGL(k_end_step, dim3((unsigned)((rc.nruns + 255) / 256)), dim3(256), 0,
+                     c->stream, c->rs, c->trace, c->tsteps, rc, slot, dtnext);
   if (ev) HIP_TRY(hipEventRecord(*ev++, c->stream));
   HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+// One step: replayed from a captured graph when graphs are on (and no
+// per-kernel events are wanted), launched kernel by kernel otherwise.
+static int launch_step(amun_ctx* c, double dtnext, hipEvent_t* ev = nullptr) {
+  const int slot = (int)(c->nsteps & 1), qcur = c->qcur;
+  if (c->use_graph && !ev) {
+    hipGraphExec_t exec = nullptr;
+    for (auto& g : c->graphs)
+      if (g.slot == slot && g.qcur == qcur && g.dtnext == dtnext) exec = g.exec;
+    if (!exec) {
+      hipGraph_t graph = nullptr;
+      HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      int r = enqueue_step(c, dtnext, slot, qcur, nullptr);
+      hipError_t e = hipStreamEndCapture(c->stream, &graph);
+      if (r) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return r;
+      }
+      if (e != hipSuccess) return fail(1000 + (int)e, "hipStreamEndCapture");
+      e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (e != hipSuccess) return fail(1000 + (int)e, "hipGraphInstantiate");
+      if (c->graphs.size() >= 8) drop_graphs(c);
+      c->graphs.push_back({slot, qcur, dtnext, exec});
+    }
+    HIP_TRY(hipGraphLaunch(exec, c->stream));
+  } else {
+    int r = enqueue_step(c, dtnext, slot, qcur, ev);
+    if (r) return r;
+  }
+  if (stages_per_step(c) & 1) c->qcur ^= 1;
   c->nsteps++;
   return 0;
 }
 
-static int kernels_per_step(const amun_ctx* c) {
-  int ns = c->cfg.time_advance == AMUN_TA_EULER ? 1 : (c->cfg.time_advance == AMUN_TA_RK2 ? 2 : 3);
-  return ns * c->rc.nd + 1;
+static int peek(amun_ctx* c, double dtnext) {
+  const int slot = (int)(c->nsteps & 1);
+  hipLaunchKernelGGL(k_peek_dt, dim3((unsigned)((c->rc.nruns + 255) / 256)), dim3(256), 0,
+                     c->stream, c->rs, c->rc, slot, dtnext, c->scratch);
+  HIP_TRY(hipGetLastError());
+  return 0;
 }
 
 int amun_dev_new_time_step(amun_ctx* c, double dtnext, double* dt, double* cmax) {
   if (!c) return fail(500, "null argument");
-  const int slot = (int)(c->nsteps & 1);
-  hipLaunchKernelGGL(k_peek_dt, dim3(1), dim3(1), 0, c->stream, c->st, c->rc, slot, dtnext,
-                     c->scratch);
-  HIP_TRY(hipGetLastError());
+  int r = peek(c, dtnext);
+  if (r) return r;
   double h[2];
   HIP_TRY(hipMemcpyAsync(h, c->scratch, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
@@ -1175,10 +1486,8 @@ int amun_dev_new_time_step(amun_ctx* c, double dtnext, double* dt, double* cmax)
 
 int amun_dev_step(amun_ctx* c, double dtnext, double* dt, double* cmax) {
   if (!c) return fail(500, "null argument");
-  const int slot = (int)(c->nsteps & 1);
-  hipLaunchKernelGGL(k_peek_dt, dim3(1), dim3(1), 0, c->stream, c->st, c->rc, slot, dtnext,
-                     c->scratch);
-  int r = launch_step(c, dtnext);
+  int r = peek(c, dtnext);
+  if (!r) r = launch_step(c, dtnext);
   if (r) return r;
   double h[2];
   HIP_TRY(hipMemcpyAsync(h, c->scratch, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -1190,28 +1499,34 @@ int amun_dev_step(amun_ctx* c, double dtnext, double* dt, double* cmax) {
 
 int amun_dev_run(amun_ctx* c, int32_t nsteps, double dtnext, double* dts) {
   if (!c || nsteps < 0) return fail(500, "bad argument");
-  if (nsteps > kTrace) return fail(500, "at most 65536 steps per call");
-  long long s0 = c->nsteps;
+  if (nsteps > c->tsteps) return fail(500, "too many steps for the dt trace in one call");
+  std::vector<long long> s0(c->rc.nruns);
+  {
+    std::vector<RunState> h(c->rc.nruns);
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpy(h.data(), c->rs, h.size() * sizeof(RunState), hipMemcpyDeviceToHost));
+    for (int g = 0; g < c->rc.nruns; g++) s0[g] = h[g].step;
+  }
   for (int s = 0; s < nsteps; s++) {
     int r = launch_step(c, dtnext);
     if (r) return r;
   }
   HIP_TRY(hipStreamSynchronize(c->stream));
   if (dts && nsteps > 0) {
-    std::vector<double> tr(kTrace);
-    HIP_TRY(hipMemcpy(tr.data(), (const char*)c->st + offsetof(DevState, trace),
-                      kTrace * sizeof(double), hipMemcpyDeviceToHost));
-    for (int s = 0; s < nsteps; s++) dts[s] = tr[(s0 + s) & (kTrace - 1)];
+    const int nr = c->rc.nruns;
+    std::vector<double> tr((size_t)nr * c->tsteps);
+    HIP_TRY(hipMemcpy(tr.data(), c->trace, tr.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int s = 0; s < nsteps; s++)
+      for (int g = 0; g < nr; g++)
+        dts[(size_t)s * nr + g] = tr[(size_t)((s0[g] + s) % c->tsteps) * nr + g];
   }
   return 0;
 }
 
-int amun_dev_timed_run(amun_ctx* c, int32_t nsteps, double dtnext, double* elapsed_ms,
-                       double* kernel_ms, int32_t nkernel_max, int32_t* nkernel) {
-  if (!c || nsteps < 1 || !elapsed_ms) return fail(500, "bad argument");
+static int timed_steps(amun_ctx* c, int32_t nsteps, double dtnext, double* elapsed_ms,
+                       double* kernel_ms, int nk_want) {
   const int nk = kernels_per_step(c);
-  if (nkernel) *nkernel = nk;
-  const bool per_kernel = kernel_ms && nkernel_max >= nk;
+  const bool per_kernel = kernel_ms && nk_want >= nk;
   std::vector<hipEvent_t> ev((size_t)(per_kernel ? nk * nsteps : 0) + 2);
   for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
   int r = 0;
@@ -1242,42 +1557,182 @@ int amun_dev_timed_run(amun_ctx* c, int32_t nsteps, double dtnext, double* elaps
   return r;
 }
 
+int amun_dev_timed_run(amun_ctx* c, int32_t nsteps, double dtnext, double* elapsed_ms,
+                       double* kernel_ms, int32_t nkernel_max, int32_t* nkernel) {
+  if (!c || nsteps < 1 || !elapsed_ms) return fail(500, "bad argument");
+  if (nkernel) *nkernel = kernels_per_step(c);
+  return timed_steps(c, nsteps, dtnext, elapsed_ms, kernel_ms, kernel_ms ? nkernel_max : 0);
+}
+
+int amun_dev_set_graph(amun_ctx* c, int32_t enable) {
+  if (!c) return fail(500, "null argument");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (!enable) drop_graphs(c);
+  c->use_graph = enable != 0;
+  return 0;
+}
+
+int amun_dev_autotune(amun_ctx* c, int32_t nsteps, int32_t* choice) {
+  if (!c || nsteps < 1) return fail(500, "bad argument");
+  const int nd = c->rc.nd, nr = c->rc.nruns;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  // the candidates run real steps on the live state, which is restored
+  // afterwards: conserved state, primitive state, compensation, run and
+  // event state, and the host-side step counters
+  const size_t bytes = c->nstate * sizeof(double), qbytes = c->nstate * c->qsize;
+  double *u0 = nullptr, *qs = nullptr;
+  float* cs = nullptr;
+  RunState* rs = nullptr;
+  DevState* st = nullptr;
+  auto release = [&]() {
+    hipFree(u0); hipFree(qs); hipFree(cs); hipFree(rs); hipFree(st);
+  };
+  hipError_t e = hipMalloc((void**)&u0, bytes);
+  if (e == hipSuccess) e = hipMalloc((void**)&qs, qbytes);
+  if (e == hipSuccess && c->C) e = hipMalloc((void**)&cs, c->nstate * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc((void**)&rs, nr * sizeof(RunState));
+  if (e == hipSuccess) e = hipMalloc((void**)&st, sizeof(DevState));
+  if (e != hipSuccess) {
+    release();
+    return fail(1000 + (int)e, "autotune: hipMalloc");
+  }
+  const long long nsteps0 = c->nsteps;
+  const int qcur0 = c->qcur;
+  auto save = [&](bool back) -> hipError_t {
+    hipError_t x = hipSuccess;
+    auto cp = [&](void* a, void* b, size_t n) {
+      if (x == hipSuccess) x = hipMemcpyAsync(back ? a : b, back ? b : a, n,
+                                              hipMemcpyDeviceToDevice, c->stream);
+    };
+    cp(c->U0, u0, bytes);
+    cp(c->Q[qcur0], qs, qbytes);
+    if (c->C) cp(c->C, cs, c->nstate * sizeof(float));
+    cp(c->rs, rs, nr * sizeof(RunState));
+    cp(c->st, st, sizeof(DevState));
+    if (x == hipSuccess) x = hipStreamSynchronize(c->stream);
+    if (back) {
+      c->nsteps = nsteps0;
+      c->qcur = qcur0;
+    }
+    return x;
+  };
+  if ((e = save(false)) != hipSuccess) {
+    release();
+    return fail(1000 + (int)e, "autotune: save");
+  }
+  const bool graphs = c->use_graph;
+  c->use_graph = false;
+  const int wgs[3] = {64, 128, 256};
+  double best[3] = {1e300, 1e300, 1e300};
+  int bwg[3] = {256, 256, 256}, bsw[3] = {0, 0, 0};
+  const int nk = kernels_per_step(c);
+  std::vector<double> kms(nk);
+  int r = 0;
+  for (int w = 0; w < 3 && !r; w++)
+    for (int z = 0; z < 2 && !r; z++) {
+      for (int dir = 0; dir < nd; dir++) { c->wg[dir] = wgs[w]; c->swz[dir] = z; }
+      double ms = 0.0;
+      r = timed_steps(c, 1, 2.0e30, &ms, nullptr, 0);
+      if (!r) r = timed_steps(c, nsteps, 2.0e30, &ms, kms.data(), nk);
+      if (!r && (e = save(true)) != hipSuccess) r = fail(1000 + (int)e, "autotune: restore");
+      if (r) break;
+      // per-direction time summed over the stages of a step
+      for (int dir = 0; dir < nd; dir++) {
+        double t = 0.0;
+        for (int s = 0; s < stages_per_step(c); s++) t += kms[(size_t)s * nd + dir];
+        if (t < best[dir]) { best[dir] = t; bwg[dir] = wgs[w]; bsw[dir] = z; }
+      }
+    }
+  for (int dir = 0; dir < nd; dir++) { c->wg[dir] = bwg[dir]; c->swz[dir] = bsw[dir]; }
+  drop_graphs(c);
+  c->use_graph = graphs;
+  release();
+  if (r) return r;
+  if (choice)
+    for (int dir = 0; dir < nd; dir++) {
+      choice[2 * dir] = bwg[dir];
+      choice[2 * dir + 1] = bsw[dir];
+    }
+  return 0;
+}
+
+int amun_dev_set_launch(amun_ctx* c, const int32_t* choice) {
+  if (!c || !choice) return fail(500, "null argument");
+  for (int dir = 0; dir < c->rc.nd; dir++) {
+    const int wg = choice[2 * dir];
+    if (wg != 64 && wg != 128 && wg != 256) return fail(500, "workgroup size must be 64, 128 or 256");
+  }
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  for (int dir = 0; dir < c->rc.nd; dir++) {
+    c->wg[dir] = choice[2 * dir];
+    c->swz[dir] = choice[2 * dir + 1] != 0;
+  }
+  drop_graphs(c);
+  return 0;
+}
+
 int amun_dev_sync(amun_ctx* c) {
   if (!c) return fail(500, "null argument");
   HIP_TRY(hipStreamSynchronize(c->stream));
   return 0;
 }
 
-int amun_dev_integrals(amun_ctx* c, double out[8]) {
-  if (!c || !out) return fail(500, "null argument");
+static int integrals_range(amun_ctx* c, int r0, int nr, double out[8]) {
   Dispatch& d = dispatch_of(c);
   const int nwg = 1024;
-  hipLaunchKernelGGL(d.integ, dim3(nwg), dim3(256), 0, c->stream, c->U0, c->rc, c->scratch);
-  HIP_TRY(hipGetLastError());
-  std::vector<double> part((size_t)nwg * 8);
-  HIP_TRY(hipMemcpyAsync(part.data(), c->scratch, part.size() * sizeof(double),
-                         hipMemcpyDeviceToHost, c->stream));
+  std::vector<RunState> hrs(c->rc.nruns);
   HIP_TRY(hipStreamSynchronize(c->stream));
-  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  for (int w = 0; w < nwg; w++)
-    for (int m = 0; m < 8; m++) acc[m] += part[(size_t)w * 8 + m];
-  double dvol = 1.0 / (c->rc.dxi[0] * c->rc.dxi[1] * (c->rc.nd == 3 ? c->rc.dxi[2] : 1.0));
-  out[0] = acc[0] * dvol;
-  out[1] = acc[1] * dvol;
-  out[2] = acc[2] * dvol;
-  out[3] = acc[3] * dvol;
-  out[4] = acc[4] * dvol;
-  out[5] = acc[5] * 0.5 * dvol;
-  out[6] = acc[6] * 0.5 * dvol;
-  out[7] = acc[7] * dvol;
+  HIP_TRY(hipMemcpy(hrs.data(), c->rs, hrs.size() * sizeof(RunState), hipMemcpyDeviceToHost));
+  for (int m = 0; m < 8; m++) out[m] = 0.0;
+  for (int g = r0; g < r0 + nr; g++) {
+    const int b0 = c->run_b0[g], nb = c->run_b0[g + 1] - b0;
+    hipLaunchKernelGGL(d.integ, dim3(nwg), dim3(256), 0, c->stream, c->U0, c->rc, c->scratch,
+                       b0, nb);
+    HIP_TRY(hipGetLastError());
+    std::vector<double> part((size_t)nwg * 8);
+    HIP_TRY(hipMemcpyAsync(part.data(), c->scratch, part.size() * sizeof(double),
+                           hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int w = 0; w < nwg; w++)
+      for (int m = 0; m < 8; m++) acc[m] += part[(size_t)w * 8 + m];
+    const RunState& R = hrs[g];
+    const double dvol = 1.0 / (R.dxi[0] * R.dxi[1] * (c->rc.nd == 3 ? R.dxi[2] : 1.0));
+    const double f[8] = {1, 1, 1, 1, 1, 0.5, 0.5, 1};
+    for (int m = 0; m < 8; m++) out[m] += acc[m] * f[m] * dvol;
+  }
+  return 0;
+}
+
+int amun_dev_integrals(amun_ctx* c, double out[8]) {
+  if (!c || !out) return fail(500, "null argument");
+  return integrals_range(c, 0, c->rc.nruns, out);
+}
+
+int amun_dev_integrals_run(amun_ctx* c, int32_t r, double out[8]) {
+  if (!c || !out) return fail(500, "null argument");
+  if (r < 0 || r >= c->rc.nruns) return fail(500, "run index out of range");
+  return integrals_range(c, r, 1, out);
+}
+
+int amun_dev_run_state(amun_ctx* c, double* time, double* dt, int64_t* step) {
+  if (!c) return fail(500, "null argument");
+  std::vector<RunState> h(c->rc.nruns);
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipMemcpy(h.data(), c->rs, h.size() * sizeof(RunState), hipMemcpyDeviceToHost));
+  for (int g = 0; g < c->rc.nruns; g++) {
+    if (time) time[g] = h[g].time;
+    if (dt) dt[g] = h[g].dt;
+    if (step) step[g] = h[g].step;
+  }
   return 0;
 }
 
 int amun_dev_get_time(amun_ctx* c, double* time, int64_t* step) {
   if (!c) return fail(500, "null argument");
+  RunState h;
   HIP_TRY(hipStreamSynchronize(c->stream));
-  DevState h;
-  HIP_TRY(hipMemcpy(&h, c->st, offsetof(DevState, trace), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(&h, c->rs, sizeof(RunState), hipMemcpyDeviceToHost));
   if (time) *time = h.time;
   if (step) *step = h.step;
   return 0;
@@ -1286,6 +1741,20 @@ int amun_dev_get_time(amun_ctx* c, double* time, int64_t* step) {
 int amun_dev_nblocks(amun_ctx* c, int32_t* nblk) {
   if (!c || !nblk) return fail(500, "null argument");
   *nblk = c->rc.nblk;
+  return 0;
+}
+
+int amun_dev_nruns(amun_ctx* c, int32_t* nruns) {
+  if (!c || !nruns) return fail(500, "null argument");
+  *nruns = c->rc.nruns;
+  return 0;
+}
+
+int amun_dev_run_blocks(amun_ctx* c, int32_t r, int32_t* first, int32_t* count) {
+  if (!c) return fail(500, "null argument");
+  if (r < 0 || r >= c->rc.nruns) return fail(500, "run index out of range");
+  if (first) *first = c->run_b0[r];
+  if (count) *count = c->run_b0[r + 1] - c->run_b0[r];
   return 0;
 }
 
@@ -1299,7 +1768,7 @@ int amun_dev_positivity_events(amun_ctx* c, uint64_t* n) {
   if (!c || !n) return fail(500, "null argument");
   HIP_TRY(hipStreamSynchronize(c->stream));
   DevState h;
-  HIP_TRY(hipMemcpy(&h, c->st, offsetof(DevState, trace), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(&h, c->st, sizeof(DevState), hipMemcpyDeviceToHost));
   *n = h.npos;
   return 0;
 }
@@ -1308,7 +1777,7 @@ int amun_dev_c2p_events(amun_ctx* c, uint64_t n[3]) {
   if (!c || !n) return fail(500, "null argument");
   HIP_TRY(hipStreamSynchronize(c->stream));
   DevState h;
-  HIP_TRY(hipMemcpy(&h, c->st, offsetof(DevState, trace), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(&h, c->st, sizeof(DevState), hipMemcpyDeviceToHost));
   n[0] = h.c2p[0];
   n[1] = h.c2p[1];
   n[2] = h.c2p[2];
@@ -1316,5 +1785,94 @@ int amun_dev_c2p_events(amun_ctx* c, uint64_t n[3]) {
 }
 
 void* amun_dev_stream(amun_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// unit level (tests): device routines on host vectors
+// ---------------------------------------------------------------------------
+namespace {
+struct DevBufs {
+  std::vector<void*> p;
+  ~DevBufs() { for (void* x : p) (void)hipFree(x); }
+  template <typename X>
+  hipError_t alloc(X** out, size_t n) {
+    void* v = nullptr;
+    hipError_t e = hipMalloc(&v, n * sizeof(X) + 8);
+    if (e == hipSuccess) { p.push_back(v); *out = static_cast<X*>(v); }
+    return e;
+  }
+};
+
+int unit_setup(const amun_cfg* cfg, Dispatch& d, RunConst& rc) {
+  if (!cfg) return fail(500, "null argument");
+  int nv = 0;
+  int r = validate(*cfg, &nv);
+  if (r) return r;
+  if (!pick(d, *cfg)) return fail(600, "no kernel instantiation for this configuration");
+  rc = make_rc(*cfg);
+  return 0;
+}
+}  // namespace
+
+extern "C" {
+
+int amun_dev_unit_riemann(const amun_cfg* cfg, int32_t n, double cmax, const double* ql,
+                          const double* qr, double* f) {
+  Dispatch d;
+  RunConst rc;
+  int r = unit_setup(cfg, d, rc);
+  if (r) return r;
+  if (n < 1 || !ql || !qr || !f) return fail(500, "bad argument");
+  const int nv = (cfg->eqsys == AMUN_EQSYS_HD) ? 5 : 9;
+  const size_t m = (size_t)n * nv;
+  DevBufs B;
+  double *dl, *dr, *df;
+  HIP_TRY(B.alloc(&dl, m));
+  HIP_TRY(B.alloc(&dr, m));
+  HIP_TRY(B.alloc(&df, m));
+  HIP_TRY(hipMemcpy(dl, ql, m * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(dr, qr, m * sizeof(double), hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(d.unit_riemann, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, 0, n, cmax,
+                     dl, dr, df, rc);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipDeviceSynchronize());
+  HIP_TRY(hipMemcpy(f, df, m * sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int amun_dev_unit_eqs(const amun_cfg* cfg, int32_t n, double cmax, const double* q, double* u,
+                      double* q2, double* f, double* cm, double* cp) {
+  Dispatch d;
+  RunConst rc;
+  int r = unit_setup(cfg, d, rc);
+  if (r) return r;
+  if (n < 1 || !q || !u || !q2 || !f || !cm || !cp) return fail(500, "bad argument");
+  const int nv = (cfg->eqsys == AMUN_EQSYS_HD) ? 5 : 9;
+  const size_t m = (size_t)n * nv;
+  DevBufs B;
+  double *dq, *du, *dq2, *df, *dm, *dp;
+  DevState* st;
+  HIP_TRY(B.alloc(&dq, m));
+  HIP_TRY(B.alloc(&du, m));
+  HIP_TRY(B.alloc(&dq2, m));
+  HIP_TRY(B.alloc(&df, m));
+  HIP_TRY(B.alloc(&dm, (size_t)n));
+  HIP_TRY(B.alloc(&dp, (size_t)n));
+  HIP_TRY(B.alloc(&st, 1));
+  HIP_TRY(hipMemset(st, 0, sizeof(DevState)));
+  HIP_TRY(hipMemcpy(dq, q, m * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(dq2, q2, m * sizeof(double), hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(d.unit_eqs, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, 0, n, cmax,
+                     dq, du, dq2, df, dm, dp, rc, st);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipDeviceSynchronize());
+  HIP_TRY(hipMemcpy(u, du, m * sizeof(double), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(q2, dq2, m * sizeof(double), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(f, df, m * sizeof(double), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(cm, dm, (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(cp, dp, (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
+}
 
 }  // extern "C"

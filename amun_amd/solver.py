@@ -27,8 +27,23 @@ EXPORTS = (
     "amun_dev_integrals", "amun_dev_get_time", "amun_dev_nblocks",
     "amun_dev_block_coords", "amun_dev_last_error", "amun_dev_sync",
     "amun_dev_positivity_events", "amun_dev_stream", "amun_dev_timed_run",
-    "amun_dev_c2p_events",
+    "amun_dev_c2p_events", "amun_dev_init_batch", "amun_dev_nruns",
+    "amun_dev_run_blocks", "amun_dev_upload_run", "amun_dev_download_run",
+    "amun_dev_run_state", "amun_dev_integrals_run", "amun_dev_set_graph",
+    "amun_dev_autotune", "amun_dev_set_launch", "amun_dev_unit_riemann",
+    "amun_dev_unit_eqs",
 )
+
+
+class AmunGrid(ctypes.Structure):
+    """ctypes mirror of ``amun_grid`` (include/amun_dev.h)."""
+    _fields_ = [
+        ("xblocks", ctypes.c_int32), ("yblocks", ctypes.c_int32),
+        ("zblocks", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("xmin", ctypes.c_double), ("xmax", ctypes.c_double),
+        ("ymin", ctypes.c_double), ("ymax", ctypes.c_double),
+        ("zmin", ctypes.c_double), ("zmax", ctypes.c_double),
+    ]
 
 _lib = None
 
@@ -64,6 +79,22 @@ def load_library(path: str = LIB_PATH):
                                      ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]
     L.amun_dev_stream.argtypes = [vp]
     L.amun_dev_c2p_events.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64)]
+    i32p = ctypes.POINTER(ctypes.c_int32)
+    L.amun_dev_init_batch.argtypes = [ctypes.POINTER(AmunCfg), ctypes.c_int32,
+                                      ctypes.POINTER(AmunGrid), ctypes.POINTER(vp)]
+    L.amun_dev_nruns.argtypes = [vp, i32p]
+    L.amun_dev_run_blocks.argtypes = [vp, ctypes.c_int32, i32p, i32p]
+    L.amun_dev_upload_run.argtypes = [vp, ctypes.c_int32, _P]
+    L.amun_dev_download_run.argtypes = [vp, ctypes.c_int32, _P, _P]
+    L.amun_dev_run_state.argtypes = [vp, _P, _P, ctypes.POINTER(ctypes.c_int64)]
+    L.amun_dev_integrals_run.argtypes = [vp, ctypes.c_int32, _P]
+    L.amun_dev_set_graph.argtypes = [vp, ctypes.c_int32]
+    L.amun_dev_autotune.argtypes = [vp, ctypes.c_int32, i32p]
+    L.amun_dev_set_launch.argtypes = [vp, i32p]
+    L.amun_dev_unit_riemann.argtypes = [ctypes.POINTER(AmunCfg), ctypes.c_int32,
+                                        ctypes.c_double, _P, _P, _P]
+    L.amun_dev_unit_eqs.argtypes = [ctypes.POINTER(AmunCfg), ctypes.c_int32, ctypes.c_double,
+                                    _P, _P, _P, _P, _P, _P]
     L.amun_dev_stream.restype = ctypes.c_void_p
     L.amun_dev_last_error.argtypes = []
     L.amun_dev_last_error.restype = ctypes.c_char_p
@@ -86,6 +117,8 @@ def _dp(a):
 
 class Solver:
     """One device context: the per-block update of one run on ``cuda:0``."""
+
+    nruns = 1
 
     def __init__(self, params: Params):
         self.lib = load_library()
@@ -173,10 +206,30 @@ class Solver:
         return dt.value, cm.value
 
     def run(self, nsteps: int, dtnext: float = 2.0e30) -> np.ndarray:
-        """nsteps steps queued back to back; returns the dt of each step."""
-        dts = np.zeros(max(nsteps, 1))
+        """nsteps steps queued back to back; returns the dt of each step
+        (shape (nsteps,) for one run, (nsteps, nruns) for a batch)."""
+        dts = np.zeros((max(nsteps, 1), self.nruns))
         self._check(self.lib.amun_dev_run(self.h, nsteps, dtnext, _dp(dts)))
-        return dts[:nsteps]
+        return dts[:nsteps, 0] if self.nruns == 1 else dts[:nsteps]
+
+    def set_graph(self, enable: bool = True):
+        """Capture steps into HIP graphs and replay them."""
+        self._check(self.lib.amun_dev_set_graph(self.h, int(enable)))
+
+    def autotune(self, nsteps: int = 3):
+        """Time every launch shape on the live state (which is restored) and
+        keep the fastest per sweep direction; returns [(wg, xcd_order)...]."""
+        ch = np.zeros(6, dtype=np.int32)
+        self._check(self.lib.amun_dev_autotune(
+            self.h, nsteps, ch.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+        return [(int(ch[2 * d]), int(ch[2 * d + 1])) for d in range(self.params.ndims)]
+
+    def set_launch(self, choice):
+        ch = np.zeros(6, dtype=np.int32)
+        for d, (wg, z) in enumerate(choice):
+            ch[2 * d], ch[2 * d + 1] = wg, z
+        self._check(self.lib.amun_dev_set_launch(
+            self.h, ch.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
 
     def launch(self, nsteps: int, dtnext: float = 2.0e30):
         """Queue nsteps without reading anything back (timing loops)."""
@@ -226,3 +279,95 @@ class Solver:
         n = (ctypes.c_uint64 * 3)()
         self._check(self.lib.amun_dev_c2p_events(self.h, n))
         return list(n)
+
+
+class Batch(Solver):
+    """Many independent uniform grids in one device context
+    (amun_dev_init_batch): one launch per sweep over all their blocks, each
+    grid with its own cell size, time step and time.  ``grids`` holds one
+    Params per run; only their block counts and bounds may differ from
+    ``params``."""
+
+    def __init__(self, params: Params, grids):
+        self.lib = load_library()
+        self.params = params
+        self.grids = list(grids)
+        self.nruns = len(self.grids)
+        self.cfg = params.to_cfg()
+        arr = (AmunGrid * self.nruns)()
+        for g, p in zip(arr, self.grids):
+            g.xblocks, g.yblocks = p.xblocks, p.yblocks
+            g.zblocks = p.zblocks if p.ndims == 3 else 1
+            g.xmin, g.xmax, g.ymin, g.ymax = p.xmin, p.xmax, p.ymin, p.ymax
+            g.zmin, g.zmax = (p.zmin, p.zmax) if p.ndims == 3 else (0.0, 1.0)
+        h = ctypes.c_void_p()
+        self._check(self.lib.amun_dev_init_batch(ctypes.byref(self.cfg), self.nruns, arr,
+                                                 ctypes.byref(h)))
+        self.h = h
+
+    def upload(self, r: int, u: np.ndarray):
+        """Global conserved state of run r, U[nv][nz][ny][nx]."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        nz, ny, nx = self.grids[r].shape
+        if u.shape != (self.params.nv, nz, ny, nx):
+            raise ValueError(f"state shape {u.shape} != {(self.params.nv, nz, ny, nx)}")
+        self._check(self.lib.amun_dev_upload_run(self.h, r, _dp(u)))
+
+    def download(self, r: int, with_q: bool = False):
+        nz, ny, nx = self.grids[r].shape
+        u = np.empty((self.params.nv, nz, ny, nx))
+        q = np.empty_like(u) if with_q else None
+        self._check(self.lib.amun_dev_download_run(self.h, r, _dp(u), _dp(q)))
+        return (u, q) if with_q else u
+
+    def state(self):
+        """(time, dt, step) arrays over the runs."""
+        t, d = np.zeros(self.nruns), np.zeros(self.nruns)
+        s = np.zeros(self.nruns, dtype=np.int64)
+        self._check(self.lib.amun_dev_run_state(
+            self.h, _dp(t), _dp(d), s.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
+        return t, d, s
+
+    def integrals_run(self, r: int) -> dict:
+        out = np.zeros(8)
+        self._check(self.lib.amun_dev_integrals_run(self.h, r, _dp(out)))
+        keys = ("mass", "momx", "momy", "momz", "ener", "ekin", "emag", "eint")
+        return dict(zip(keys, out.tolist()))
+
+    def run_blocks(self, r: int):
+        f, n = ctypes.c_int32(), ctypes.c_int32()
+        self._check(self.lib.amun_dev_run_blocks(self.h, r, ctypes.byref(f), ctypes.byref(n)))
+        return f.value, n.value
+
+
+def _unit_check(lib, rc):
+    if rc != 0:
+        raise AmunError(rc, lib.amun_dev_last_error().decode(errors="replace"))
+
+
+def unit_riemann(params: Params, ql: np.ndarray, qr: np.ndarray, cmax: float) -> np.ndarray:
+    """Device Riemann solver of the configuration on state pairs (n, nv)."""
+    lib = load_library()
+    ql = np.ascontiguousarray(ql, dtype=np.float64)
+    qr = np.ascontiguousarray(qr, dtype=np.float64)
+    f = np.empty_like(ql)
+    cfg = params.to_cfg()
+    _unit_check(lib, lib.amun_dev_unit_riemann(ctypes.byref(cfg), ql.shape[0], cmax, _dp(ql),
+                                               _dp(qr), _dp(f)))
+    return f
+
+
+def unit_eqs(params: Params, q: np.ndarray, cmax: float = 1.0, q2_init=None):
+    """Device prim2cons / fluxspeed / cons2prim on states q (n, nv):
+    returns u, q2, f, cm, cp."""
+    lib = load_library()
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    n = q.shape[0]
+    u = np.empty_like(q)
+    q2 = np.zeros_like(q) if q2_init is None else np.array(q2_init, dtype=np.float64, copy=True)
+    f = np.empty_like(q)
+    cm, cp = np.empty(n), np.empty(n)
+    cfg = params.to_cfg()
+    _unit_check(lib, lib.amun_dev_unit_eqs(ctypes.byref(cfg), n, cmax, _dp(q), _dp(u), _dp(q2),
+                                           _dp(f), _dp(cm), _dp(cp)))
+    return u, q2, f, cm, cp

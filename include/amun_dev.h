@@ -97,6 +97,13 @@ typedef struct amun_cfg {
 
 typedef struct amun_ctx amun_ctx;
 
+/* One independent uniform grid of a batch: base-level block counts and
+ * domain bounds (the "xblocks".. and "xmin".. parameters of one run). */
+typedef struct amun_grid {
+  int32_t xblocks, yblocks, zblocks, reserved;
+  double  xmin, xmax, ymin, ymax, zmin, zmax;
+} amun_grid;
+
 /* Fill *cfg with the reference defaults (equations.F90:166-200,
  * interpolations.F90:80-118, evolution.F90:65-80, coordinates.F90:42-80). */
 void amun_dev_default_cfg(amun_cfg* cfg);
@@ -108,11 +115,52 @@ void amun_dev_default_cfg(amun_cfg* cfg);
  * device block pool.  Replaces the module state set up by those routines. */
 int  amun_dev_init(const amun_cfg* cfg, amun_ctx** ctx);
 
+/* Batched form of amun_dev_init: nruns independent periodic grids that share
+ * the numerical configuration of cfg (its block-count and bound fields are
+ * ignored) but each have their own block counts, cell sizes, time step, time
+ * and step counter.  All their blocks live in one pool, run after run in the
+ * order of grids, and every sweep of every stage is one launch over the
+ * whole pool.  No reference counterpart: the reference runs one grid per
+ * process (driver.F90:157-657). */
+int  amun_dev_init_batch(const amun_cfg* cfg, int32_t nruns, const amun_grid* grids,
+                         amun_ctx** ctx);
+
+/* Number of runs of the context, and the first pool block and block count
+ * of run r (its blocks in amun_dev_block_coords / *_blocks order). */
+int  amun_dev_nruns(amun_ctx* ctx, int32_t* nruns);
+int  amun_dev_run_blocks(amun_ctx* ctx, int32_t r, int32_t* first, int32_t* count);
+
+/* Upload / download the global array U[nv][nz][ny][nx] of run r alone. */
+int  amun_dev_upload_run(amun_ctx* ctx, int32_t r, const double* u);
+int  amun_dev_download_run(amun_ctx* ctx, int32_t r, double* u, double* q);
+
+/* Time, last dt and step counter of every run (arrays of nruns; any may be
+ * NULL). */
+int  amun_dev_run_state(amun_ctx* ctx, double* time, double* dt, int64_t* step);
+
+/* amun_dev_integrals for run r alone. */
+int  amun_dev_integrals_run(amun_ctx* ctx, int32_t r, double out[8]);
+
+/* With enable != 0, steps are captured once into HIP graphs (one per cmax
+ * slot, primitive buffer and dtnext) and replayed with hipGraphLaunch.  The
+ * captured kernels are the ones launched otherwise; no memset is captured. */
+int  amun_dev_set_graph(amun_ctx* ctx, int32_t enable);
+
+/* Launch-shape autotuner: runs nsteps timed steps of the live state for every
+ * candidate shape (workgroup of 64, 128 or 256 pencils; plain or XCD-grouped
+ * workgroup order), keeps the fastest per sweep direction and restores the
+ * state, so the run continues as if the call had not happened.  choice (may
+ * be NULL) receives ndims pairs (workgroup size, XCD grouping).
+ * amun_dev_set_launch sets such pairs directly. */
+int  amun_dev_autotune(amun_ctx* ctx, int32_t nsteps, int32_t* choice);
+int  amun_dev_set_launch(amun_ctx* ctx, const int32_t* choice);
+
 /* Release every device allocation of the context (finalize_* routines). */
 void amun_dev_finalize(amun_ctx* ctx);
 
 /* Upload conserved variables.  _blocks: nblk host blocks u0(nv,im,jm,km)
- * (blocks.F90:1120-1191 layout; ghosts are ignored).  _global: U[nv][nz][ny][nx].
+ * (blocks.F90:1120-1191 layout; ghosts are ignored), all runs of a batch.
+ * _global: U[nv][nz][ny][nx] of a single-run context.
  * Both recompute the primitive variables (update_primitive_variables,
  * equations.F90:1088) on the device. */
 int  amun_dev_upload_blocks(amun_ctx* ctx, const double* u);
@@ -120,12 +168,13 @@ int  amun_dev_upload_global(amun_ctx* ctx, const double* u);
 
 /* Download conserved (and optionally primitive, q may be NULL) variables.
  * _blocks fills the ghost layers too, as after boundary_variables
- * (boundaries.F90:372-479). */
+ * (boundaries.F90:372-479), all runs of a batch; _global needs a single run. */
 int  amun_dev_download_blocks(amun_ctx* ctx, double* u, double* q);
 int  amun_dev_download_global(amun_ctx* ctx, double* u, double* q);
 
 /* new_time_step (evolution.F90:379-493): cmax reduction over all interior
- * cells and dt = min(cfl*dx_min/cmax, dtnext) when dtnext > 0. */
+ * cells and dt = min(cfl*dx_min/cmax, dtnext) when dtnext > 0.  Every run of
+ * a batch gets its own; dt and cmax returned here are those of run 0. */
 int  amun_dev_new_time_step(amun_ctx* ctx, double dtnext, double* dt,
                             double* cmax);
 
@@ -135,7 +184,8 @@ int  amun_dev_new_time_step(amun_ctx* ctx, double dtnext, double* dt,
 int  amun_dev_step(amun_ctx* ctx, double dtnext, double* dt, double* cmax);
 
 /* Run nsteps steps back to back on the device stream (the host reads dt only
- * at the end); dts (may be NULL) receives the dt of every step. */
+ * at the end); dts (may be NULL) receives the dt of every step of every run,
+ * dts[step * nruns + run]. */
 int  amun_dev_run(amun_ctx* ctx, int32_t nsteps, double dtnext, double* dts);
 
 /* Timing helper: queue nsteps steps on the context stream with HIP events
@@ -149,10 +199,12 @@ int  amun_dev_timed_run(amun_ctx* ctx, int32_t nsteps, double dtnext,
 
 /* Volume integrals of store_integrals (integrals.F90:254-485), FP64 on the
  * device: out[0]=mass, [1..3]=momentum, [4]=total energy, [5]=kinetic,
- * [6]=magnetic energy, [7]=internal energy. */
+ * [6]=magnetic energy, [7]=internal energy (summed over the runs of a
+ * batch). */
 int  amun_dev_integrals(amun_ctx* ctx, double out[8]);
 
-/* Simulation time and step counter kept by the context (evolution.F90:75). */
+/* Simulation time and step counter kept by the context (evolution.F90:75);
+ * run 0 of a batch. */
 int  amun_dev_get_time(amun_ctx* ctx, double* time, int64_t* step);
 
 /* Number of blocks, and the base-level block coordinates (nblk x 3 int32,
@@ -180,6 +232,20 @@ int  amun_dev_c2p_events(amun_ctx* ctx, uint64_t n[3]);
 /* The context's HIP stream (a hipStream_t), so that a caller can order its
  * own device work after the solver's. */
 void* amun_dev_stream(amun_ctx* ctx);
+
+/* Unit level, for parity tests of the device routines on vectors of states
+ * (the seams riemann_iface, schemes.F90:66-71, and prim2cons_iface /
+ * cons2prim_iface / fluxspeed_iface, equations.F90:70-108), in the
+ * arithmetic type of cfg->precision.  States are (nv, n) in Fortran order,
+ * i.e. n consecutive vectors of nv doubles.
+ * _riemann: f = riemann(ql, qr) with the GLM cmax given.
+ * _eqs: u = prim2cons(q), f/cm/cp = fluxspeed(q, u), q2 = cons2prim(u) in
+ * FP64; q2 enters holding the previous primitives (kept by a failed
+ * relativistic recovery). */
+int  amun_dev_unit_riemann(const amun_cfg* cfg, int32_t n, double cmax, const double* ql,
+                           const double* qr, double* f);
+int  amun_dev_unit_eqs(const amun_cfg* cfg, int32_t n, double cmax, const double* q,
+                       double* u, double* q2, double* f, double* cm, double* cp);
 
 /* Message of the last failing call of this thread. */
 const char* amun_dev_last_error(void);

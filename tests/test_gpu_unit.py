@@ -1,0 +1,105 @@
+"""Unit-level parity of the device routines (amun_dev_unit_riemann,
+amun_dev_unit_eqs) on vectors of states: against the golden vectors written
+by the reference binaries and against the oracle on random and degenerate
+states.  FP64: <= 1e-12 of the magnitude of each vector (FMA contraction and
+refined reciprocals are the only differences); mixed: FP32 round-off."""
+import os
+
+import numpy as np
+import pytest
+
+from amun_amd.config import Params
+from amun_amd.solver import unit_eqs, unit_riemann
+import oracle_lib as ol
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def close(a, b, tol):
+    """|a - b| <= tol * (per-row max |b|), rows are state vectors."""
+    scale = np.maximum(np.abs(b).max(axis=-1, keepdims=True), 1e-300)
+    return np.abs(a - b) <= tol * scale
+
+
+def _states(nv, n, rng, degenerate=True):
+    q = np.zeros((n, nv))
+    q[:, 0] = rng.uniform(0.1, 3.0, n)
+    q[:, 1:4] = rng.normal(scale=1.2, size=(n, 3))
+    q[:, -1] = rng.uniform(0.05, 3.0, n)
+    if nv == 9:
+        q[:, 4:7] = rng.normal(size=(n, 3))
+        q[:, 7] = rng.normal(scale=0.1, size=n)
+        if degenerate:
+            k = n // 8
+            q[:k, 4] = 0.0                       # Bx = 0
+            q[k:2 * k, 5:7] = 0.0                # no transverse field
+            q[2 * k:3 * k, 4] = 1e-6 * q[2 * k:3 * k, 4]   # weak Bx
+            q[3 * k:4 * k, 1] *= 8.0             # fast flows
+    return q
+
+
+CASES = [("hd", "hll"), ("hd", "hllc"), ("mhd", "hll"), ("mhd", "hlld"), ("srmhd", "hll")]
+
+
+@pytest.mark.parametrize("eq,rs", CASES)
+def test_riemann_vs_oracle(eq, rs):
+    rng = np.random.default_rng(31)
+    p = Params(ndims=3, equation_system=eq, riemann_solver=rs,
+               gamma=1.4 if eq == "hd" else 5.0 / 3.0, tolerance=1e-12, nr_maxit=50)
+    if eq == "srmhd":
+        from test_oracle_invariants import _sr_states
+        ql, qr = _sr_states(rng, 2000), _sr_states(rng, 2000)
+    else:
+        ql, qr = _states(p.nv, 2000, rng), _states(p.nv, 2000, rng)
+        qr[:200] = ql[:200]                      # identical states
+        qr[200:400] = ql[200:400] * (1 + 1e-9)   # nearly identical
+    fo = ol.oracle_riemann(p, ql, qr, cmax=1.7)
+    fg = unit_riemann(p, ql, qr, cmax=1.7)
+    ok = close(fg, fo, 1e-12)
+    print(f"[case] {eq}/{rs}: max rel {np.abs(fg - fo).max():.3e}")
+    assert ok.all(), np.argwhere(~ok)[:5]
+    pm = Params(**{k: v for k, v in p.__dict__.items() if k != "extra"})
+    pm.precision = "mixed"
+    fm = unit_riemann(pm, ql, qr, cmax=1.7)
+    assert close(fm, fo, 3e-5).mean() >= 0.999
+
+
+@pytest.mark.parametrize("eq", ["hd", "mhd", "srmhd"])
+def test_equation_of_state_golden(eq):
+    """prim2cons, fluxspeed and cons2prim against the reference's own outputs."""
+    g = np.load(os.path.join(GOLD, f"unit_eqs_{eq}.npz"), allow_pickle=False)
+    if eq == "srmhd":
+        p = Params.from_text(str(g["params"]), ndims=2)
+    else:
+        p = Params(ndims=3, equation_system=eq, gamma=1.4 if eq == "hd" else 5.0 / 3.0,
+                   riemann_solver="hllc" if eq == "hd" else "hlld")
+    u, q2, f, cm, cp = unit_eqs(p, g["q"], float(g["cmax"]))
+    for name, a, b in (("u", u, g["u"]), ("q2", q2, g["q2"]), ("f", f, g["f"])):
+        tol = 1e-12 if name != "q2" or eq != "srmhd" else 1e-11
+        ok = close(a, b, tol)
+        print(f"[case] {eq} {name}: max abs {np.abs(a - b).max():.3e}")
+        assert ok.all(), (name, np.argwhere(~ok)[:5])
+    np.testing.assert_allclose(cm, g["cm"], rtol=1e-12, atol=1e-13)
+    np.testing.assert_allclose(cp, g["cp"], rtol=1e-12, atol=1e-13)
+
+
+@pytest.mark.parametrize("eq,rs", [("hd", "hllc"), ("mhd", "hlld"), ("srmhd", "hll")])
+def test_riemann_golden_via_update_flux_states(eq, rs):
+    """The golden update_flux fixtures use the zero limiter, so every face is
+    the Riemann problem of its two cell values: the device solver on those
+    pairs against the reference's fluxes (x faces)."""
+    g = np.load(os.path.join(GOLD, f"unit_flux_{eq}_{rs}.npz"), allow_pickle=False)
+    p = Params.from_text(str(g["params"]), ndims=2)
+    q, fref = g["q"], g["f"]               # (nb, 1, jm, im, nv), (..., nv, nd)
+    # faces whose both cells are reconstructed: cells ibl..ieu (one ghost
+    # layer each side) of rows jbl..jeu
+    lo, hi = p.nghosts - 1, p.nghosts + p.ncells
+    ql = q[:, 0, lo:hi + 1, lo:hi, :].reshape(-1, p.nv)
+    qr = q[:, 0, lo:hi + 1, lo + 1:hi + 1, :].reshape(-1, p.nv)
+    fx = fref[:, 0, lo:hi + 1, lo:hi, :, 0].reshape(-1, p.nv)
+    fg = unit_riemann(p, ql, qr, float(g["cmax"]))
+    ok = close(fg, fx, 1e-12)
+    print(f"[case] golden {eq}/{rs}: max abs {np.abs(fg - fx).max():.3e}")
+    assert ok.all(), np.argwhere(~ok)[:5]
