@@ -31,6 +31,7 @@
 #include <cstdlib>
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 #include <string>
 #include <vector>
 
@@ -76,6 +77,7 @@ struct RunState {
   long long step;
   double dxmin;                     // smallest cell size (evolution.F90:467-476)
   double dxi[3];                    // inverse cell sizes
+  double fix_dt, fix_cmax;          // > 0: dt and cmax held at these values
 };
 
 // constants of one run, passed by value to every kernel
@@ -89,12 +91,14 @@ struct RunConst {
 };
 
 __device__ __forceinline__ double cmax_of(const RunState* R, int slot) {
+  if (R->fix_cmax > 0.0) return R->fix_cmax;
   return fmax(__longlong_as_double((long long)R->cmax_bits[slot]), kTiny);
 }
 
 // dt of new_time_step (evolution.F90:478-483), identical in every thread
 __device__ __forceinline__ double dt_of(const RunConst& rc, const RunState* R,
                                        int slot, double dtnext) {
+  if (R->fix_dt > 0.0) return R->fix_dt;
   double cmax = cmax_of(R, slot);
   double dtn = rc.cfl * R->dxmin / fmax(cmax + 2.0 * 0.0 / R->dxmin, kTiny);
   return dtnext > 0.0 ? fmin(dtn, dtnext) : R->dt;
@@ -818,6 +822,8 @@ typedef void (*SweepFn)(SweepArgs, RunConst);
 
 }  // namespace
 
+#include "adjoint.h"
+
 // kernel instantiations selected for one configuration
 struct Dispatch {
   SweepFn sweep[3] = {nullptr, nullptr, nullptr};
@@ -829,6 +835,18 @@ struct Dispatch {
   void (*unit_riemann)(int, double, const double*, const double*, double*, RunConst) = nullptr;
   void (*unit_eqs)(int, double, const double*, double*, double*, double*, double*, double*,
                    RunConst, DevState*) = nullptr;
+  // tangent-linear / adjoint kernels (FP64, TVD reconstruction, HD and MHD)
+  bool has_adj = false;
+  void (*adj_prim)(const double*, double*, RunConst) = nullptr;
+  void (*adj_face[3])(const double*, const double*, double*, double*, double*, const int*,
+                      RunConst, double, double) = {nullptr, nullptr, nullptr};
+  void (*tlm_face[3])(const double*, const double*, const double*, double, double*, const int*,
+                      RunConst, double) = {nullptr, nullptr, nullptr};
+  void (*adj_cell)(const double*, const double*, const double*, const double*, const double*,
+                   const double*, const double*, const double*, const double*, const double*,
+                   const double*, double*, double*, const int*, RunConst) = nullptr;
+  void (*tlm_cell)(const double*, const double*, const double*, double*, const int*, RunConst,
+                   double, double, double) = nullptr;
 };
 
 // ===========================================================================
@@ -865,6 +883,15 @@ struct amun_ctx {
   bool use_graph = false;
   struct Graph { int slot, qcur; double dtnext; hipGraphExec_t exec; };
   std::vector<Graph> graphs;
+  // forward tape of amun_dev_tape_forward: checkpoints of U every interval
+  // steps, dt and cmax of every step
+  struct Tape {
+    int nsteps = 0, interval = 0;
+    double dtnext = 0.0;
+    std::vector<double> dt, cmax;
+    double* ck = nullptr;       // ceil(nsteps / interval) states
+    double* dtc = nullptr;      // device copy of (dt, cmax) per step
+  } tape;
 };
 
 namespace {
@@ -881,6 +908,20 @@ void fill(Dispatch& d) {
   d.pack_q = k_pack_blocks<P::nv, T>;
   d.unit_riemann = k_unit_riemann<P, RS, T>;
   d.unit_eqs = k_unit_eqs<P, T>;
+  if constexpr (REC == REC_TVD && !P::sr && std::is_same<T, double>::value) {
+    d.has_adj = true;
+    d.adj_prim = k_adj_prim<P>;
+    d.adj_face[0] = k_adj_face<P, RS, ND, 0>;
+    d.adj_face[1] = k_adj_face<P, RS, ND, 1>;
+    d.tlm_face[0] = k_tlm_face<P, RS, ND, 0>;
+    d.tlm_face[1] = k_tlm_face<P, RS, ND, 1>;
+    if constexpr (ND == 3) {
+      d.adj_face[2] = k_adj_face<P, RS, ND, 2>;
+      d.tlm_face[2] = k_tlm_face<P, RS, ND, 2>;
+    }
+    d.adj_cell = k_adj_cell<P, ND>;
+    d.tlm_cell = k_tlm_cell<P, ND>;
+  }
 }
 
 template <class P, int RS, typename T>
@@ -1185,6 +1226,8 @@ void amun_dev_finalize(amun_ctx* c) {
   if (!c) return;
   if (c->stream) hipStreamSynchronize(c->stream);
   drop_graphs(c);
+  hipFree(c->tape.ck);
+  hipFree(c->tape.dtc);
   hipFree(c->U0);
   hipFree(c->U1);
   hipFree(c->Q[0]);
@@ -1377,7 +1420,8 @@ static int kernels_per_step(const amun_ctx* c) {
 // Enqueue one whole step for cmax slot `slot` with the primitive state in
 // Q[qcur]; host-side state is left alone.  With ev != nullptr an event is
 // recorded after every kernel.
-static int enqueue_step(amun_ctx* c, double dtnext, int slot, int qcur, hipEvent_t* ev) {
+static int enqueue_step(amun_ctx* c, double dtnext, int slot, int qcur, hipEvent_t* ev,
+                        double* rec = nullptr) {
   Dispatch& d = dispatch_of(c);
   const RunConst& rc = c->rc;
   const int npen = (rc.nd == 3) ? rc.nc * rc.nc : rc.nc;
@@ -1395,6 +1439,9 @@ static int enqueue_step(amun_ctx* c, double dtnext, int slot, int qcur, hipEvent
   default:
     stages = {{0, 1.0, c->U1}, {1, 1.0 / 4.0, c->U1}, {2, 2.0 / 3.0, c->U0}};
   }
+  if (rec)   // stage inputs: U^n, then every intermediate stage output
+    HIP_TRY(hipMemcpyAsync(rec, c->U0, c->nstate * sizeof(double), hipMemcpyDeviceToDevice,
+                           c->stream));
   for (size_t s = 0; s < stages.size(); s++) {
     SweepArgs a;
     a.Q = c->Q[qcur];
@@ -1422,6 +1469,9 @@ static int enqueue_step(amun_ctx* c, double dtnext, int slot, int qcur, hipEvent
       if (ev) HIP_TRY(hipEventRecord(*ev++, c->stream));
     }
     qcur ^= 1;
+    if (rec && s + 1 < stages.size())
+      HIP_TRY(hipMemcpyAsync(rec + (s + 1) * c->nstate, c->U1, c->nstate * sizeof(double),
+                             hipMemcpyDeviceToDevice, c->stream));
   }
   hipLaunchKernelGGL(k_end_step, dim3((unsigned)((rc.nruns + 255) / 256)), dim3(256), 0,
                      c->stream, c->rs, c->trace, c->tsteps, rc, slot, dtnext);
@@ -1785,6 +1835,314 @@ int amun_dev_c2p_events(amun_ctx* c, uint64_t n[3]) {
 }
 
 void* amun_dev_stream(amun_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+// ---------------------------------------------------------------------------
+// gradients: forward tape, tangent-linear and adjoint runs
+// ---------------------------------------------------------------------------
+static int tape_check(amun_ctx* c) {
+  if (c->rc.nruns != 1) return fail(600, "gradients need a single-run context");
+  if (!c->disp.has_adj)
+    return fail(600, "gradients need FP64, TVD reconstruction and the HD or MHD system");
+  return 0;
+}
+
+int amun_dev_tape_forward(amun_ctx* c, int32_t nsteps, int32_t interval, double dtnext) {
+  if (!c || nsteps < 1 || interval < 1) return fail(500, "bad argument");
+  int r = tape_check(c);
+  if (r) return r;
+  auto& T = c->tape;
+  hipFree(T.ck);
+  hipFree(T.dtc);
+  T.ck = nullptr;
+  T.dtc = nullptr;
+  const int nck = (nsteps + interval - 1) / interval;
+  HIP_TRY(hipMalloc((void**)&T.ck, (size_t)nck * c->nstate * sizeof(double)));
+  HIP_TRY(hipMalloc((void**)&T.dtc, (size_t)2 * nsteps * sizeof(double)));
+  T.nsteps = nsteps;
+  T.interval = interval;
+  T.dtnext = dtnext;
+  for (int s = 0; s < nsteps; s++) {
+    if (s % interval == 0)
+      HIP_TRY(hipMemcpyAsync(T.ck + (size_t)(s / interval) * c->nstate, c->U0,
+                             c->nstate * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    hipLaunchKernelGGL(k_tape, dim3(1), dim3(1), 0, c->stream, c->rs, c->rc,
+                       (int)(c->nsteps & 1), dtnext, T.dtc + 2 * s);
+    r = launch_step(c, dtnext);
+    if (r) return r;
+  }
+  std::vector<double> h((size_t)2 * nsteps);
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipMemcpy(h.data(), T.dtc, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+  T.dt.resize(nsteps);
+  T.cmax.resize(nsteps);
+  for (int s = 0; s < nsteps; s++) {
+    T.dt[s] = h[2 * s];
+    T.cmax[s] = h[2 * s + 1];
+  }
+  return 0;
+}
+
+int amun_dev_tape_steps(amun_ctx* c, double* dt, double* cmax) {
+  if (!c) return fail(500, "null argument");
+  for (int s = 0; s < c->tape.nsteps; s++) {
+    if (dt) dt[s] = c->tape.dt[s];
+    if (cmax) cmax[s] = c->tape.cmax[s];
+  }
+  return 0;
+}
+
+int amun_dev_run_fixed(amun_ctx* c, int32_t nsteps, const double* dt, const double* cmax) {
+  if (!c || nsteps < 0 || !dt || !cmax) return fail(500, "bad argument");
+  for (int s = 0; s < nsteps; s++) {
+    if (!(dt[s] > 0.0) || !(cmax[s] > 0.0)) return fail(500, "dt and cmax must be positive");
+    hipLaunchKernelGGL(k_set_fixed, dim3(1), dim3(1), 0, c->stream, c->rs, dt[s], cmax[s]);
+    int r = launch_step(c, 2.0e30);
+    if (r) return r;
+  }
+  hipLaunchKernelGGL(k_set_fixed, dim3(1), dim3(1), 0, c->stream, c->rs, 0.0, 0.0);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+namespace {
+
+struct GradBufs {
+  std::vector<double*> all;
+  double *seg = nullptr, *Q = nullptr, *MU = nullptr, *LT = nullptr, *GC = nullptr;
+  double *LY = nullptr, *LU0 = nullptr, *LX = nullptr, *GP = nullptr;
+  double *AL[3] = {nullptr, nullptr, nullptr}, *AR[3] = {nullptr, nullptr, nullptr};
+  double *GF[3] = {nullptr, nullptr, nullptr};
+  ~GradBufs() { for (double* p : all) (void)hipFree(p); }
+  hipError_t get(double** p, size_t n) {
+    hipError_t e = hipMalloc((void**)p, n * sizeof(double) + 64);
+    if (e == hipSuccess) all.push_back(*p);
+    return e;
+  }
+};
+
+struct StageCoef { double a, b, c; };
+
+std::vector<StageCoef> stage_coefs(const amun_ctx* c) {
+  switch (c->cfg.time_advance) {
+  case AMUN_TA_EULER: return {{0.0, 1.0, 1.0}};
+  case AMUN_TA_RK2: return {{0.0, 1.0, 1.0}, {0.5, 0.5, 0.5}};
+  default: return {{0.0, 1.0, 1.0}, {0.75, 0.25, 0.25}, {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0}};
+  }
+}
+
+// recompute steps [s0, s1) from the checkpoint of s0, storing the stage
+// inputs of every step into seg ([step - s0][stage][state])
+int recompute_segment(amun_ctx* c, int s0, int s1, double* seg) {
+  auto& T = c->tape;
+  const int ns = stages_per_step(c);
+  HIP_TRY(hipMemcpyAsync(c->U0, T.ck + (size_t)(s0 / T.interval) * c->nstate,
+                         c->nstate * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  int r = after_upload(c, 0, 1);
+  if (r) return r;
+  for (int s = s0; s < s1; s++) {
+    hipLaunchKernelGGL(k_set_fixed, dim3(1), dim3(1), 0, c->stream, c->rs, T.dt[s], T.cmax[s]);
+    r = enqueue_step(c, 2.0e30, (int)(c->nsteps & 1), c->qcur, nullptr,
+                     seg + (size_t)(s - s0) * ns * c->nstate);
+    if (r) return r;
+    if (ns & 1) c->qcur ^= 1;
+    c->nsteps++;
+  }
+  hipLaunchKernelGGL(k_set_fixed, dim3(1), dim3(1), 0, c->stream, c->rs, 0.0, 0.0);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+// global array of the single run <-> pool layout (device buffer dst/src)
+int global_to_pool(amun_ctx* c, const double* g, double* dpool) {
+  std::vector<double> h(c->nstate);
+  scatter_run(c, 0, g, h.data());
+  HIP_TRY(hipMemcpy(dpool, h.data(), c->nstate * sizeof(double), hipMemcpyHostToDevice));
+  return 0;
+}
+
+int pool_to_global(amun_ctx* c, const double* dpool, double* g) {
+  std::vector<double> h(c->nstate);
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipMemcpy(h.data(), dpool, c->nstate * sizeof(double), hipMemcpyDeviceToHost));
+  gather_run(c, 0, h.data(), g, 8);
+  return 0;
+}
+
+}  // namespace
+
+// Save and restore the live state around a gradient computation, which
+// recomputes from the checkpoints inside the context: conserved state, run
+// state (time, step, dt) and the host step counters.
+struct GradKeep {
+  double* u = nullptr;
+  RunState rs;
+  long long nsteps = 0;
+  int qcur = 0;
+};
+
+static int grad_state(amun_ctx* c, GradKeep& k, bool back) {
+  if (!back) {
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMalloc((void**)&k.u, c->nstate * sizeof(double)));
+    HIP_TRY(hipMemcpy(k.u, c->U0, c->nstate * sizeof(double), hipMemcpyDeviceToDevice));
+    HIP_TRY(hipMemcpy(&k.rs, c->rs, sizeof(RunState), hipMemcpyDeviceToHost));
+    k.nsteps = c->nsteps;
+    k.qcur = c->qcur;
+    return 0;
+  }
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipMemcpy(c->U0, k.u, c->nstate * sizeof(double), hipMemcpyDeviceToDevice));
+  HIP_TRY(hipMemcpy(c->rs, &k.rs, sizeof(RunState), hipMemcpyHostToDevice));
+  HIP_TRY(hipFree(k.u));
+  k.u = nullptr;
+  c->nsteps = k.nsteps;
+  c->qcur = k.qcur;
+  return after_upload(c, 0, 1);
+}
+
+int amun_dev_tape_tangent(amun_ctx* c, const double* du0, double dgamma, double* du) {
+  if (!c || !du0 || !du) return fail(500, "null argument");
+  int r = tape_check(c);
+  if (r) return r;
+  auto& T = c->tape;
+  if (T.nsteps < 1) return fail(500, "no forward tape: call amun_dev_tape_forward first");
+  Dispatch& d = c->disp;
+  const RunConst& rc = c->rc;
+  const int ns = stages_per_step(c), nd = rc.nd;
+  const auto co = stage_coefs(c);
+  const size_t n = c->nstate;
+  GradBufs B;
+  HIP_TRY(B.get(&B.seg, (size_t)T.interval * ns * n));
+  HIP_TRY(B.get(&B.Q, n));
+  HIP_TRY(B.get(&B.LX, n));      // tangent of the stage input
+  HIP_TRY(B.get(&B.LU0, n));     // tangent of U^n
+  HIP_TRY(B.get(&B.LT, n));      // tangent of L
+  for (int k = 0; k < nd; k++) HIP_TRY(B.get(&B.AL[k], n));   // face flux tangents
+  GradKeep keep;
+  if ((r = grad_state(c, keep, false))) return r;
+  const long long nn = (long long)n;
+  const dim3 gc((unsigned)(((long long)rc.nblk * rc.ncell + 255) / 256)), bl(256);
+  if ((r = global_to_pool(c, du0, B.LU0))) return r;
+  RunState h;
+  HIP_TRY(hipMemcpy(&h, c->rs, sizeof(RunState), hipMemcpyDeviceToHost));
+  for (int s0 = 0; s0 < T.nsteps && !r; s0 += T.interval) {
+    const int s1 = std::min(T.nsteps, s0 + T.interval);
+    if ((r = recompute_segment(c, s0, s1, B.seg))) break;
+    for (int s = s0; s < s1; s++) {
+      const double* X0 = B.seg + (size_t)(s - s0) * ns * n;
+      // tangent of the stage input: dU^n for the first stage
+      hipLaunchKernelGGL(k_axpby, dim3(2048), bl, 0, c->stream, nn, 1.0, B.LU0, 0.0, B.LX);
+      for (int k = 0; k < ns; k++) {
+        const double* X = X0 + (size_t)k * n;
+        hipLaunchKernelGGL(d.adj_prim, gc, bl, 0, c->stream, X, B.Q, rc);
+        for (int dd = 0; dd < nd; dd++)
+          hipLaunchKernelGGL(d.tlm_face[dd], gc, bl, 0, c->stream, X, B.Q, B.LX, dgamma,
+                             B.AL[dd], c->nbr, rc, T.cmax[s]);
+        hipLaunchKernelGGL(d.tlm_cell, gc, bl, 0, c->stream, B.AL[0], B.AL[1],
+                           nd == 3 ? B.AL[2] : B.AL[1], B.LT, c->nbr, rc, h.dxi[0], h.dxi[1],
+                           h.dxi[2]);
+        // dY = a dU^n + b dX + c dt dL
+        hipLaunchKernelGGL(k_axpby, dim3(2048), bl, 0, c->stream, nn, co[k].b, B.LX, 0.0, B.LX);
+        if (co[k].a != 0.0)
+          hipLaunchKernelGGL(k_axpby, dim3(2048), bl, 0, c->stream, nn, co[k].a, B.LU0, 1.0, B.LX);
+        hipLaunchKernelGGL(k_axpby, dim3(2048), bl, 0, c->stream, nn, co[k].c * T.dt[s], B.LT,
+                           1.0, B.LX);
+      }
+      if (c->cfg.eqsys == AMUN_EQSYS_MHD)
+        hipLaunchKernelGGL(k_scale_var, dim3(2048), bl, 0, c->stream, B.LX, c->nv, 7, rc.decay, rc);
+      hipLaunchKernelGGL(k_axpby, dim3(2048), bl, 0, c->stream, nn, 1.0, B.LX, 0.0, B.LU0);
+    }
+    HIP_TRY(hipGetLastError());
+  }
+  if (!r) r = pool_to_global(c, B.LU0, du);
+  int r2 = grad_state(c, keep, true);
+  return r ? r : r2;
+}
+
+int amun_dev_tape_backward(amun_ctx* c, const double* lam, double* lam0, double* dgamma) {
+  if (!c || !lam || !lam0) return fail(500, "null argument");
+  int r = tape_check(c);
+  if (r) return r;
+  auto& T = c->tape;
+  if (T.nsteps < 1) return fail(500, "no forward tape: call amun_dev_tape_forward first");
+  Dispatch& d = c->disp;
+  const RunConst& rc = c->rc;
+  const int ns = stages_per_step(c), nd = rc.nd;
+  const auto co = stage_coefs(c);
+  const size_t n = c->nstate, ncell = (size_t)rc.nblk * rc.ncell;
+  const int npart = 1024;
+  GradBufs B;
+  HIP_TRY(B.get(&B.seg, (size_t)T.interval * ns * n));
+  HIP_TRY(B.get(&B.Q, n));
+  HIP_TRY(B.get(&B.MU, n));
+  HIP_TRY(B.get(&B.LT, n));
+  HIP_TRY(B.get(&B.LY, n));
+  HIP_TRY(B.get(&B.LU0, n));
+  HIP_TRY(B.get(&B.GC, ncell));
+  HIP_TRY(B.get(&B.GP, npart));
+  for (int k = 0; k < nd; k++) {
+    HIP_TRY(B.get(&B.AL[k], n));
+    HIP_TRY(B.get(&B.AR[k], n));
+    HIP_TRY(B.get(&B.GF[k], ncell));
+  }
+  HIP_TRY(hipMemset(B.GP, 0, npart * sizeof(double)));
+  GradKeep keep;
+  if ((r = grad_state(c, keep, false))) return r;
+  const long long nn = (long long)n;
+  const dim3 gc((unsigned)(((long long)ncell + 255) / 256)), bl(256);
+  if ((r = global_to_pool(c, lam, B.LU0))) return r;     // lambda of U^N
+  RunState h;
+  HIP_TRY(hipMemcpy(&h, c->rs, sizeof(RunState), hipMemcpyDeviceToHost));
+  const int nseg = (T.nsteps + T.interval - 1) / T.interval;
+  for (int g = nseg - 1; g >= 0 && !r; g--) {
+    const int s0 = g * T.interval, s1 = std::min(T.nsteps, s0 + T.interval);
+    if ((r = recompute_segment(c, s0, s1, B.seg))) break;
+    for (int s = s1 - 1; s >= s0; s--) {
+      const double* X0 = B.seg + (size_t)(s - s0) * ns * n;
+      // lambda of the last stage output: psi decay of the step
+      hipLaunchKernelGGL(k_axpby, dim3(2048), bl, 0, c->stream, nn, 1.0, B.LU0, 0.0, B.LY);
+      if (c->cfg.eqsys == AMUN_EQSYS_MHD)
+        hipLaunchKernelGGL(k_scale_var, dim3(2048), bl, 0, c->stream, B.LY, c->nv, 7, rc.decay, rc);
+      hipLaunchKernelGGL(k_axpby, dim3(2048), bl, 0, c->stream, nn, 0.0, B.LY, 0.0, B.LU0);
+      for (int k = ns - 1; k >= 0; k--) {
+        const double* X = X0 + (size_t)k * n;
+        // MU = c dt lambda_Y, LT = (dL/dX)^T MU
+        hipLaunchKernelGGL(k_axpby, dim3(2048), bl, 0, c->stream, nn, co[k].c * T.dt[s], B.LY,
+                           0.0, B.MU);
+        hipLaunchKernelGGL(d.adj_prim, gc, bl, 0, c->stream, X, B.Q, rc);
+        for (int dd = 0; dd < nd; dd++)
+          hipLaunchKernelGGL(d.adj_face[dd], gc, bl, 0, c->stream, B.Q, B.MU, B.AL[dd], B.AR[dd],
+                             B.GF[dd], c->nbr, rc, h.dxi[dd], T.cmax[s]);
+        const int z = nd == 3 ? 2 : 1;
+        hipLaunchKernelGGL(d.adj_cell, gc, bl, 0, c->stream, X, B.Q, B.AL[0], B.AR[0], B.AL[1],
+                           B.AR[1], B.AL[z], B.AR[z], B.GF[0], B.GF[1], B.GF[z], B.LT, B.GC,
+                           c->nbr, rc);
+        hipLaunchKernelGGL(k_sum_acc, dim3(npart), bl, 0, c->stream, B.GC, (long long)ncell, B.GP);
+        // lambda of U^n gets a lambda_Y; lambda of X = b lambda_Y + LT
+        if (co[k].a != 0.0)
+          hipLaunchKernelGGL(k_axpby, dim3(2048), bl, 0, c->stream, nn, co[k].a, B.LY, 1.0, B.LU0);
+        hipLaunchKernelGGL(k_axpby, dim3(2048), bl, 0, c->stream, nn, co[k].b, B.LY, 0.0, B.MU);
+        hipLaunchKernelGGL(k_axpby, dim3(2048), bl, 0, c->stream, nn, 1.0, B.LT, 1.0, B.MU);
+        if (k == 0)
+          hipLaunchKernelGGL(k_axpby, dim3(2048), bl, 0, c->stream, nn, 1.0, B.MU, 1.0, B.LU0);
+        else
+          hipLaunchKernelGGL(k_axpby, dim3(2048), bl, 0, c->stream, nn, 1.0, B.MU, 0.0, B.LY);
+      }
+    }
+    HIP_TRY(hipGetLastError());
+  }
+  if (!r) r = pool_to_global(c, B.LU0, lam0);
+  if (!r && dgamma) {
+    std::vector<double> gp(npart);
+    HIP_TRY(hipMemcpy(gp.data(), B.GP, npart * sizeof(double), hipMemcpyDeviceToHost));
+    double acc = 0.0;
+    for (int w = 0; w < npart; w++) acc += gp[w];
+    *dgamma = acc;
+  }
+  int r2 = grad_state(c, keep, true);
+  return r ? r : r2;
+}
 
 }  // extern "C"
 

@@ -31,7 +31,8 @@ EXPORTS = (
     "amun_dev_run_blocks", "amun_dev_upload_run", "amun_dev_download_run",
     "amun_dev_run_state", "amun_dev_integrals_run", "amun_dev_set_graph",
     "amun_dev_autotune", "amun_dev_set_launch", "amun_dev_unit_riemann",
-    "amun_dev_unit_eqs",
+    "amun_dev_unit_eqs", "amun_dev_tape_forward", "amun_dev_tape_steps",
+    "amun_dev_tape_backward", "amun_dev_tape_tangent", "amun_dev_run_fixed",
 )
 
 
@@ -91,6 +92,11 @@ def load_library(path: str = LIB_PATH):
     L.amun_dev_set_graph.argtypes = [vp, ctypes.c_int32]
     L.amun_dev_autotune.argtypes = [vp, ctypes.c_int32, i32p]
     L.amun_dev_set_launch.argtypes = [vp, i32p]
+    L.amun_dev_tape_forward.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_double]
+    L.amun_dev_tape_steps.argtypes = [vp, _P, _P]
+    L.amun_dev_tape_backward.argtypes = [vp, _P, _P, _P]
+    L.amun_dev_tape_tangent.argtypes = [vp, _P, ctypes.c_double, _P]
+    L.amun_dev_run_fixed.argtypes = [vp, ctypes.c_int32, _P, _P]
     L.amun_dev_unit_riemann.argtypes = [ctypes.POINTER(AmunCfg), ctypes.c_int32,
                                         ctypes.c_double, _P, _P, _P]
     L.amun_dev_unit_eqs.argtypes = [ctypes.POINTER(AmunCfg), ctypes.c_int32, ctypes.c_double,
@@ -223,6 +229,40 @@ class Solver:
         self._check(self.lib.amun_dev_autotune(
             self.h, nsteps, ch.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
         return [(int(ch[2 * d]), int(ch[2 * d + 1])) for d in range(self.params.ndims)]
+
+    # ----------------------------------------------------------- gradients
+    def tape_forward(self, nsteps: int, interval: int = 20, dtnext: float = 2.0e30):
+        """nsteps steps with checkpoints every interval steps (for
+        tape_backward / tape_tangent)."""
+        self._taped = nsteps
+        self._check(self.lib.amun_dev_tape_forward(self.h, nsteps, interval, dtnext))
+
+    def tape_steps(self):
+        """(dt, cmax) of every taped step."""
+        dt, cm = np.zeros(self._taped), np.zeros(self._taped)
+        self._check(self.lib.amun_dev_tape_steps(self.h, _dp(dt), _dp(cm)))
+        return dt, cm
+
+    def tape_backward(self, lam: np.ndarray):
+        """dJ/dU0 and dJ/dgamma from lam = dJ/dU of the final state."""
+        lam = np.ascontiguousarray(lam, dtype=np.float64)
+        lam0 = np.empty_like(lam)
+        dg = ctypes.c_double()
+        self._check(self.lib.amun_dev_tape_backward(self.h, _dp(lam), _dp(lam0), ctypes.byref(dg)))
+        return lam0, dg.value
+
+    def tape_tangent(self, du0: np.ndarray, dgamma: float = 0.0) -> np.ndarray:
+        """dU of the final state for the initial perturbation (du0, dgamma)."""
+        du0 = np.ascontiguousarray(du0, dtype=np.float64)
+        du = np.empty_like(du0)
+        self._check(self.lib.amun_dev_tape_tangent(self.h, _dp(du0), float(dgamma), _dp(du)))
+        return du
+
+    def run_fixed(self, dt: np.ndarray, cmax: np.ndarray):
+        """Ordinary steps with dt and cmax imposed per step."""
+        dt = np.ascontiguousarray(dt, dtype=np.float64)
+        cmax = np.ascontiguousarray(cmax, dtype=np.float64)
+        self._check(self.lib.amun_dev_run_fixed(self.h, len(dt), _dp(dt), _dp(cmax)))
 
     def set_launch(self, choice):
         ch = np.zeros(6, dtype=np.int32)

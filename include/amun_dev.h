@@ -233,6 +233,29 @@ int  amun_dev_c2p_events(amun_ctx* ctx, uint64_t n[3]);
  * own device work after the solver's. */
 void* amun_dev_stream(amun_ctx* ctx);
 
+/* Gradients of a scalar objective of the final state (no reference
+ * counterpart: the reference computes no derivatives).  Single-run FP64
+ * contexts with TVD reconstruction and the HD or MHD system.
+ * _tape_forward: nsteps steps from the current state, as amun_dev_run, with a
+ *   checkpoint of the conserved state every interval steps and the dt and
+ *   GLM cmax of every step recorded (_tape_steps returns them).
+ * _tape_backward: given lam = dJ/dU of the final state (global array
+ *   U[nv][nz][ny][nx]), lam0 = dJ/dU of the taped initial state and
+ *   *dgamma = dJ/dgamma, by the adjoint of every stage (reverse order,
+ *   steps recomputed from the checkpoints).  dt and cmax are held at their
+ *   taped values: the derivative is that of the map with the step sizes
+ *   frozen, as are the limiter branches and positivity halvings.
+ * _tape_tangent: the matching tangent-linear run: du = dU(final) for an
+ *   initial perturbation du0 and dgamma.
+ * _run_fixed: nsteps ordinary steps with dt and cmax imposed per step (the
+ *   nonlinear map that the tangent and adjoint linearise; finite differences).
+ * The live state is left as it was. */
+int  amun_dev_tape_forward(amun_ctx* ctx, int32_t nsteps, int32_t interval, double dtnext);
+int  amun_dev_tape_steps(amun_ctx* ctx, double* dt, double* cmax);
+int  amun_dev_tape_backward(amun_ctx* ctx, const double* lam, double* lam0, double* dgamma);
+int  amun_dev_tape_tangent(amun_ctx* ctx, const double* du0, double dgamma, double* du);
+int  amun_dev_run_fixed(amun_ctx* ctx, int32_t nsteps, const double* dt, const double* cmax);
+
 /* Unit level, for parity tests of the device routines on vectors of states
  * (the seams riemann_iface, schemes.F90:66-71, and prim2cons_iface /
  * cons2prim_iface / fluxspeed_iface, equations.F90:70-108), in the
