@@ -35,10 +35,25 @@ def _run(cmd, cwd=None):
     return r
 
 
+HIP_UNITS = ("amun_dev", "inst_tvd", "inst_weno5z", "inst_gp")
+
+
 def build_hip():
+    """Compile the host unit and the kernel instantiation units in parallel,
+    then link libamun_hip.so."""
+    from concurrent.futures import ThreadPoolExecutor
     out = os.path.join(PKG, "libamun_hip.so")
-    srcs = [os.path.join(CSRC, "amun_dev.hip")]
-    _run([HIPCC, *HIP_FLAGS, "-shared", "-o", out + ".tmp", *srcs])
+    bdir = os.path.join(PKG, "_build")
+    os.makedirs(bdir, exist_ok=True)
+    jobs = []
+    for u in HIP_UNITS:
+        obj = os.path.join(bdir, u + ".o")
+        jobs.append((obj, [HIPCC, *HIP_FLAGS, "-c", "-o", obj, os.path.join(CSRC, u + ".hip")]))
+    with ThreadPoolExecutor(max_workers=len(jobs)) as ex:
+        for f in [ex.submit(_run, cmd) for _, cmd in jobs]:
+            f.result()
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", out + ".tmp",
+          *[obj for obj, _ in jobs]])
     os.replace(out + ".tmp", out)
     return out
 
@@ -46,7 +61,7 @@ def build_hip():
 def build_oracle():
     out = os.path.join(ORACLE, "liboracle.so")
     _run(["gcc", "-O2", "-ffp-contract=off", "-fPIC", "-shared", "-o", out,
-          os.path.join(ORACLE, "amun_oracle.c"), "-lm"])
+          os.path.join(ORACLE, "amun_oracle.c"), "-lquadmath", "-lm"])
     return out
 
 

@@ -19,7 +19,7 @@ from dataclasses import dataclass, field, fields
 EQSYS = {"hd": 1, "hydrodynamic": 1, "mhd": 2, "magnetohydrodynamic": 2,
          "srmhd": 4}
 EOS = {"adi": 1, "adiabatic": 1}
-RECON = {"tvd": 1, "weno5z": 5, "weno5-z": 5}
+RECON = {"tvd": 1, "weno5z": 5, "weno5-z": 5, "gp": 20}
 LIMITER = {"zero": 0, "mm": 1, "minmod": 1, "mc": 2, "monotonized_central": 2,
            "sb": 3, "superbee": 3, "vl": 4, "vanleer": 4, "va": 5,
            "vanalbada": 5}
@@ -47,6 +47,8 @@ class AmunCfg(ctypes.Structure):
         ("zmin", ctypes.c_double), ("zmax", ctypes.c_double),
         ("tolerance", ctypes.c_double), ("pmin", ctypes.c_double),
         ("nr_maxit", ctypes.c_int32), ("nr_extra", ctypes.c_int32),
+        ("sgp", ctypes.c_double), ("kappa", ctypes.c_double), ("kbeta", ctypes.c_double),
+        ("ngp", ctypes.c_int32), ("reserved", ctypes.c_int32),
     ]
 
 
@@ -98,6 +100,10 @@ class Params:
     pmin: float = 1.0e-16
     nr_maxit: int = 100
     nr_extra: int = 2
+    sgp: float = 9.0
+    kappa: float = 1.0
+    kbeta: float = 1.0
+    ngp: int = 5
     extra: dict = field(default_factory=dict)
 
     # ---------------------------------------------------------------- derived
@@ -150,6 +156,7 @@ class Params:
         c.zmin, c.zmax = self.zmin, self.zmax
         c.tolerance, c.pmin = self.tolerance, self.pmin
         c.nr_maxit, c.nr_extra = int(self.nr_maxit), int(self.nr_extra)
+        c.sgp, c.kappa, c.kbeta, c.ngp = self.sgp, self.kappa, self.kbeta, int(self.ngp)
         return c
 
     def dx(self):
@@ -179,6 +186,8 @@ class Params:
             f"zmin = {self.zmin!r}", f"zmax = {self.zmax!r}",
             f"tolerance = {self.tolerance!r}", f"pmin = {self.pmin!r}",
             f"nr_maxit = {self.nr_maxit}", f"nr_extra = {self.nr_extra}",
+            f"sgp = {self.sgp!r}", f"kappa = {self.kappa!r}", f"kbeta = {self.kbeta!r}",
+            f"ngp = {self.ngp}",
             "minlev = 1", "maxlev = 1",
             "xlbndry = periodic", "xubndry = periodic",
             "ylbndry = periodic", "yubndry = periodic",

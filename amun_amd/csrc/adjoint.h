@@ -20,7 +20,7 @@
 
 #include "dual.h"
 
-namespace {
+namespace amun_dev {
 
 constexpr int kAdjNch = 4;          // Jacobian columns per dual pass
 
@@ -400,7 +400,7 @@ __global__ void k_adj_cell(const double* __restrict__ X, const double* __restric
 }
 
 // y = alpha x + beta y (beta = 0: y = alpha x), over n doubles
-__global__ void k_axpby(long long n, double alpha, const double* __restrict__ x, double beta,
+static __global__ void k_axpby(long long n, double alpha, const double* __restrict__ x, double beta,
                         double* __restrict__ y) {
   for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < n;
        g += (long long)gridDim.x * blockDim.x)
@@ -408,7 +408,7 @@ __global__ void k_axpby(long long n, double alpha, const double* __restrict__ x,
 }
 
 // scale variable v of every cell by s (psi decay and its adjoint)
-__global__ void k_scale_var(double* __restrict__ Y, int nv, int v, double s, RunConst rc) {
+static __global__ void k_scale_var(double* __restrict__ Y, int nv, int v, double s, RunConst rc) {
   const long long n = (long long)rc.nblk * rc.ncell;
   for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < n;
        g += (long long)gridDim.x * blockDim.x) {
@@ -419,7 +419,7 @@ __global__ void k_scale_var(double* __restrict__ Y, int nv, int v, double s, Run
 
 // fixed-order partial sums of a per-cell array added to part[workgroup]
 // (deterministic: the same cells always go to the same partial, in order)
-__global__ __launch_bounds__(256) void k_sum_acc(const double* __restrict__ x, long long n,
+static __global__ __launch_bounds__(256) void k_sum_acc(const double* __restrict__ x, long long n,
                                                  double* __restrict__ part) {
   __shared__ double sh[256];
   double acc = 0.0;
@@ -436,15 +436,15 @@ __global__ __launch_bounds__(256) void k_sum_acc(const double* __restrict__ x, l
 }
 
 // dt and cmax of the coming step into the tape
-__global__ void k_tape(const RunState* rs, RunConst rc, int slot, double dtnext, double* out) {
+static __global__ void k_tape(const RunState* rs, RunConst rc, int slot, double dtnext, double* out) {
   const RunState* R = rs;
   out[0] = dt_of(rc, R, slot, dtnext);
   out[1] = cmax_of(R, slot);
 }
 
-__global__ void k_set_fixed(RunState* rs, double dt, double cmax) {
+static __global__ void k_set_fixed(RunState* rs, double dt, double cmax) {
   rs->fix_dt = dt;
   rs->fix_cmax = cmax;
 }
 
-}  // namespace
+}  // namespace amun_dev

@@ -1,0 +1,937 @@
+// kernels.h -- device code of the MI355X (gfx950) AMUN per-block update: data
+// layout, the sweep kernels and their helpers, and the table of kernel
+// instantiations a configuration selects.  Shared by amun_dev.hip (host side
+// of the C-ABI) and the instantiation units inst_*.hip, which are compiled in
+// parallel.
+//
+// Device data (block pool, structure of arrays, interior cells only):
+//   U0, U1   conserved state, double [nblk][nv][nc^nd]   (u0/u1 of blocks.F90)
+//   Q[2]     primitive state ping-pong, double, same layout      (q)
+//   D        flux-divergence accumulator of the current stage     (du)
+//   nbr      int [nblk][27] block index of every same-level neighbour
+//            (face, edge and corner), periodic wrap included
+// Ghost zones are never materialised on the hot path: a stencil that leaves
+// its block reads the neighbour's interior through nbr, which is exactly the
+// value the reference copies into its ghost layers (boundaries.F90:372-479,
+// same-level face/edge/corner copies).
+//
+// One RK stage = one sweep kernel per direction.  A sweep thread owns one
+// pencil of a block and walks it with a sliding window: each cell is
+// reconstructed once (both faces), each face gets one Riemann solve, and the
+// flux difference goes straight into the cell's increment.  The x sweep
+// stores -dxi*dF, the y sweep subtracts dyi*dF, and the last sweep completes
+// du, applies the stage combine of the integrator, the psi decay, recomputes
+// the primitive variables of the new state and (last stage) reduces the
+// maximum speed for the next CFL step -- i.e. update_fluxes, update_increment,
+// the stage update and update_variables of evolution.F90:1005-2094 fused into
+// nd kernels.  dt, cmax and the simulation time stay on the device.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstddef>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <algorithm>
+#include <cstring>
+#include <type_traits>
+#include <string>
+#include <vector>
+
+#pragma once
+
+#include "../../include/amun_dev.h"
+#include "physics.h"
+#include "srmhd.h"
+
+using namespace amun;
+
+namespace amun_dev {
+
+inline thread_local std::string g_last_error;
+
+inline int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                          \
+  do {                                                                         \
+    hipError_t e_ = (expr);                                                    \
+    if (e_ != hipSuccess)                                                      \
+      return fail(1000 + (int)e_, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+constexpr double kTiny = 2.2250738585072014e-308;   // tiny(1d0)
+constexpr int kTrace = 1 << 16;                      // dt trace entries (all runs)
+
+// ---------------------------------------------------------------------------
+// device-side step state
+// ---------------------------------------------------------------------------
+struct DevState {
+  unsigned long long npos;          // non-positive density/pressure events
+  unsigned long long c2p[3];        // SR primitive solver: failed, p <= 0, no convergence
+};
+
+// step state of one run (one independent grid of a batch; a single run is a
+// batch of one): what new_time_step and the driver keep per simulation
+struct RunState {
+  unsigned long long cmax_bits[2];  // cmax of step s lives in slot s&1
+  double dt;                        // last dt (kept when dtnext <= 0)
+  double time;
+  long long step;
+  double dxmin;                     // smallest cell size (evolution.F90:467-476)
+  double dxi[3];                    // inverse cell sizes
+  double fix_dt, fix_cmax;          // > 0: dt and cmax held at these values
+};
+
+// constants of one run, passed by value to every kernel
+struct RunConst {
+  int nd, nc, nblk, ncell;
+  int nruns;                        // independent grids in the block pool
+  int limiter;
+  double gamma, gammam1, gammam1i;
+  double cfl, eps, decay;
+  double cgp[5];                    // Gaussian-process weights (prepare_gp)
+  double kappa, kbeta;              // MP limiter of the GP reconstruction
+  SrConst sc;                       // relativistic primitive solver
+};
+
+__device__ __forceinline__ double cmax_of(const RunState* R, int slot) {
+  if (R->fix_cmax > 0.0) return R->fix_cmax;
+  return fmax(__longlong_as_double((long long)R->cmax_bits[slot]), kTiny);
+}
+
+// dt of new_time_step (evolution.F90:478-483), identical in every thread
+__device__ __forceinline__ double dt_of(const RunConst& rc, const RunState* R,
+                                       int slot, double dtnext) {
+  if (R->fix_dt > 0.0) return R->fix_dt;
+  double cmax = cmax_of(R, slot);
+  double dtn = rc.cfl * R->dxmin / fmax(cmax + 2.0 * 0.0 / R->dxmin, kTiny);
+  return dtnext > 0.0 ? fmin(dtn, dtnext) : R->dt;
+}
+
+// Position of cell (i,j,k) inside a block's variable plane.  Cells are stored
+// in 4x4 x-y bricks: one 128-byte line holds x = 4i'..4i'+3 of four
+// consecutive y, so the x, y and z sweeps (lanes across the two transverse
+// directions, walking the third) each touch few lines per load and reuse
+// them within four steps.  Requires nc % 4 == 0.
+// FP32 arrays (element size 4) of 3D runs use 4x4x2 bricks instead, so that
+// a line (32 floats) again spans four steps of every sweep direction.
+template <int ES>
+__host__ __device__ __forceinline__ int lidx(int nc, int nd, int i, int j, int k) {
+  if (nd == 2) return (((i >> 2) * nc + j) << 2) | (i & 3);
+  if (ES == 8) return ((((i >> 2) * nc + k) * nc + j) << 2) | (i & 3);
+  return (((((i >> 2) * (nc >> 1) + (k >> 1)) * nc + j) * 2 + (k & 1)) << 2) | (i & 3);
+}
+
+// position of (i,j,k), possibly outside the block by at most nc, in the pool
+// (array of element size ES)
+template <int ES>
+__device__ __forceinline__ size_t cell_index(const int* __restrict__ nbr,
+                                             const RunConst& rc, int b, int i,
+                                             int j, int k) {
+  const int nc = rc.nc;
+  int di = (i < 0) ? -1 : (i >= nc ? 1 : 0);
+  int dj = (j < 0) ? -1 : (j >= nc ? 1 : 0);
+  int dk = (k < 0) ? -1 : (k >= nc ? 1 : 0);
+  if (di | dj | dk) {
+    b = nbr[b * 27 + (dk + 1) * 9 + (dj + 1) * 3 + (di + 1)];
+    i -= di * nc;
+    j -= dj * nc;
+    k -= dk * nc;
+  }
+  // packed: block in the high word, cell in the low word
+  return ((size_t)b << 32) | (size_t)lidx<ES>(nc, rc.nd, i, j, k);
+}
+
+template <int NV>
+__device__ __forceinline__ size_t var_index(const RunConst& rc, size_t packed, int v) {
+  size_t b = packed >> 32, c = packed & 0xffffffffull;
+  return (b * NV + v) * (size_t)rc.ncell + c;
+}
+
+// component rotation of the directional solvers (schemes.F90:3063-3071,
+// 3100-3108): rotated slot p takes original variable rot<P,DIR>(p)
+template <class P, int DIR>
+__device__ __forceinline__ constexpr int rot(int p) {
+  if constexpr (DIR == 0) {
+    return p;
+  } else if constexpr (DIR == 1) {
+    if (p == P::ivx) return P::ivy;
+    if (p == P::ivy) return P::ivz;
+    if (p == P::ivz) return P::ivx;
+    if constexpr (P::mhd) {
+      if (p == P::ibx) return P::iby;
+      if (p == P::iby) return P::ibz;
+      if (p == P::ibz) return P::ibx;
+    }
+    return p;
+  } else {
+    if (p == P::ivx) return P::ivz;
+    if (p == P::ivy) return P::ivx;
+    if (p == P::ivz) return P::ivy;
+    if constexpr (P::mhd) {
+      if (p == P::ibx) return P::ibz;
+      if (p == P::iby) return P::ibx;
+      if (p == P::ibz) return P::iby;
+    }
+    return p;
+  }
+}
+
+enum { REC_TVD = AMUN_REC_TVD, REC_WENO5Z = AMUN_REC_WENO5Z, REC_GP = AMUN_REC_GP };
+enum { RS_HLL = AMUN_RS_HLL, RS_HLLC = AMUN_RS_HLLC, RS_HLLD = AMUN_RS_HLLD };
+
+template <class P, int RS, typename T>
+__device__ __forceinline__ void riemann(const EosConst<T>& e, const SrConst& sc, T cmax,
+                                        T cmax2, T* ql, T* qr, T* f) {
+  if constexpr (P::sr) riemann_sr_hll<P, T>(sc, cmax, cmax2, ql, qr, f);
+  else if constexpr (RS == RS_HLLD) riemann_hlld<P, T>(e, cmax, cmax2, ql, qr, f);
+  else if constexpr (RS == RS_HLLC) riemann_hllc<P, T>(e, cmax, cmax2, ql, qr, f);
+  else riemann_hll<P, T>(e, cmax, cmax2, ql, qr, f);
+}
+
+struct SweepArgs {
+  const void* Q;                   // primitive state of the stage input (T)
+  const double* U0;                // state at the start of the step
+  const double* U1;                // intermediate state (stage input)
+  double* Uout;                    // stage output (may alias U0 or U1)
+  void* Qout;                      // primitive state of Uout (T)
+  void* D;                         // increment accumulator (T)
+  float* C;                        // Kahan compensation of U0 (mixed mode)
+  const int* __restrict__ nbr;
+  DevState* st;                    // event counters
+  RunState* rs;                    // per-run step state
+  const int* __restrict__ brun;    // run of every block
+  int slot;                        // cmax slot of this step
+  int swz;                         // XCD-aware workgroup order (autotuned)
+  int kind;                        // stage combine, see combine()
+  int last;                        // last stage of the step
+  int first;                       // first sweep of the step
+  double fac;                      // ds = fac * dt
+  double dtnext;
+};
+
+// stage combines of evolve_euler/rk2/rk3 (evolution.F90:570, 700, 740, 1080,
+// 1130, 1177)
+__device__ __forceinline__ double combine(int kind, double u0, double u1, double ds,
+                                          double du) {
+  switch (kind) {
+  case 0: return u0 + ds * du;
+  case 1: return ((3.0 / 4.0) * u0 + (1.0 / 4.0) * u1) + ds * du;
+  case 2: return ((1.0 / 3.0) * u0 + (2.0 / 3.0) * u1) + ds * du;
+  default: return 0.5 * ((u0 + u1) + ds * du);
+  }
+}
+
+// reconstruction of one cell along DIR from its primitive window:
+// flc = state at the upper face, frc = state at the lower face
+template <class P, int REC, int ND, int DIR, typename T>
+struct Recon;
+
+// WENO5-Z window of five cells (interfaces_dir -> reconstruct_weno5z)
+template <class P, int ND, int DIR, typename T>
+struct Recon<P, REC_WENO5Z, ND, DIR, T> {
+  static constexpr int R = 2;    // window half width
+  static constexpr int H = 3;    // ghost depth needed (faces of cells -1, nc)
+  T w[P::nv][5];
+  __device__ __forceinline__ void load(int slotm, const T* q) {
+#pragma unroll
+    for (int v = 0; v < P::nv; v++) w[v][slotm] = q[v];
+  }
+  __device__ __forceinline__ void shift() {
+#pragma unroll
+    for (int v = 0; v < P::nv; v++)
+#pragma unroll
+      for (int m = 0; m < 4; m++) w[v][m] = w[v][m + 1];
+  }
+  __device__ __forceinline__ void cell(const RunConst& rc, T* flc, T* frc, const T*,
+                                       const int*, int, int, int, int, DevState*) {
+    const T eps = T(rc.eps);
+#pragma unroll
+    for (int v = 0; v < P::nv; v++) weno5z_cell<T>(w[v], eps, flc[v], frc[v]);
+  }
+};
+
+// Gaussian process (interfaces_dir -> reconstruct_gp, mp_limiting): the
+// five-cell window of WENO5-Z
+template <class P, int ND, int DIR, typename T>
+struct Recon<P, REC_GP, ND, DIR, T> : Recon<P, REC_WENO5Z, ND, DIR, T> {
+  __device__ __forceinline__ void cell(const RunConst& rc, T* flc, T* frc, const T*,
+                                       const int*, int, int, int, int, DevState*) {
+    const T cgp[5] = {T(rc.cgp[0]), T(rc.cgp[1]), T(rc.cgp[2]), T(rc.cgp[3]), T(rc.cgp[4])};
+    const T kappa = T(rc.kappa), kbeta = T(rc.kbeta), eps = T(rc.eps);
+#pragma unroll
+    for (int v = 0; v < P::nv; v++) gp_cell<T>(this->w[v], cgp, kappa, kbeta, eps, flc[v], frc[v]);
+  }
+};
+
+// TVD (interfaces_tvd, interpolations.F90:787-833): three-cell window along
+// DIR; density and pressure also need the transverse slopes for the joint
+// positivity halving
+template <class P, int ND, int DIR, typename T>
+struct Recon<P, REC_TVD, ND, DIR, T> {
+  static constexpr int R = 1;
+  static constexpr int H = 2;
+  T w[P::nv][3];
+  __device__ __forceinline__ void load(int slotm, const T* q) {
+#pragma unroll
+    for (int v = 0; v < P::nv; v++) w[v][slotm] = q[v];
+  }
+  __device__ __forceinline__ void shift() {
+#pragma unroll
+    for (int v = 0; v < P::nv; v++) {
+      w[v][0] = w[v][1];
+      w[v][1] = w[v][2];
+    }
+  }
+  __device__ __forceinline__ void cell(const RunConst& rc, T* flc, T* frc,
+                                       const T* __restrict__ Q,
+                                       const int* __restrict__ nbr, int b, int i, int j,
+                                       int k, DevState* st) {
+#pragma unroll
+    for (int v = 0; v < P::nv; v++) {
+      const T qc = w[v][1];
+      T dq = limiter<T>(rc.limiter, T(0.5), qc - w[v][0], w[v][2] - qc);
+      if (v == P::idn || v == P::ipr) {
+        T dqt[3] = {T(0), T(0), T(0)};
+        dqt[DIR] = dq;
+#pragma unroll
+        for (int d = 0; d < ND; d++) {
+          if (d == DIR) continue;
+          int im = i - (d == 0), jm = j - (d == 1), km = k - (d == 2);
+          int ip = i + (d == 0), jp = j + (d == 1), kp = k + (d == 2);
+          T qm = Q[var_index<P::nv>(rc, cell_index<sizeof(T)>(nbr, rc, b, im, jm, km), v)];
+          T qp = Q[var_index<P::nv>(rc, cell_index<sizeof(T)>(nbr, rc, b, ip, jp, kp), v)];
+          dqt[d] = limiter<T>(rc.limiter, T(0.5), qc - qm, qp - qc);
+        }
+        if (qc > T(0)) {
+          for (;;) {
+            T sa = tabs(dqt[0]) + tabs(dqt[1]);
+            if (ND == 3) sa += tabs(dqt[2]);
+            if (!(qc <= sa)) break;
+            dqt[0] *= T(0.5); dqt[1] *= T(0.5); dqt[2] *= T(0.5);
+          }
+        } else {
+          atomicAdd(&st->npos, 1ull);
+          dqt[0] = dqt[1] = dqt[2] = T(0);
+        }
+        dq = dqt[DIR];
+      }
+      flc[v] = qc + dq;
+      frc[v] = qc - dq;
+    }
+  }
+};
+
+// Counts relativistic primitive-solver events (the reference prints each to
+// stderr and goes on).
+__device__ __forceinline__ void c2p_count(DevState* st, int cs) {
+  if (cs & C2P_FAIL) atomicAdd(&st->c2p[0], 1ull);
+  if (cs & C2P_PNEG) atomicAdd(&st->c2p[1], 1ull);
+  if (cs & C2P_NOCONV) atomicAdd(&st->c2p[2], 1ull);
+}
+
+// A cell whose bracketing failed keeps its previous primitive values (read
+// from Qold at offset base, variable stride ncell); p = 0 is already set.
+template <class P, typename T>
+__device__ __forceinline__ void c2p_keep_old(int cs, double* q, const T* __restrict__ Qold,
+                                             size_t base, int ncell) {
+  if (cs & C2P_FAIL) {
+#pragma unroll
+    for (int v = 0; v < P::nv; v++)
+      if (v != P::ipr) q[v] = double(Qold[base + (size_t)v * ncell]);
+  }
+}
+
+// Workgroup index of this workgroup.  The dispatcher deals workgroups out to
+// the eight XCDs round robin (workgroup i runs on XCD i mod 8), and each XCD
+// has its own L2.  With swz set, the index is remapped so that every XCD gets
+// one contiguous range of the pencil space (neighbouring pencils, hence the
+// same bricks and the same neighbour blocks, share an L2); the map is a
+// bijection of [0, gridDim.x).
+__device__ __forceinline__ int workgroup_index(int swz) {
+  const int i = blockIdx.x;
+  if (!swz) return i;
+  const int g = gridDim.x, q = g >> 3, r = g & 7, x = i & 7;
+  return x * q + (x < r ? x : r) + (i >> 3);
+}
+
+// Maximum speed of a wave into its run's accumulator: one atomic per wave
+// when every lane belongs to the same run (the common case), one per lane
+// otherwise (a wave straddling two grids of a batch).  Positive doubles
+// order as their bit patterns.
+__device__ __forceinline__ void wave_max_speed(RunState* rs, int run, int slot, double smax) {
+  const int r0 = __shfl(run, 0);
+  if (__all(run == r0)) {
+    for (int off = 32; off > 0; off >>= 1) smax = fmax(smax, __shfl_xor(smax, off));
+    if ((threadIdx.x & 63) == 0 && smax > 0.0)
+      atomicMax(&rs[r0].cmax_bits[slot], (unsigned long long)__double_as_longlong(smax));
+  } else if (smax > 0.0) {
+    atomicMax(&rs[run].cmax_bits[slot], (unsigned long long)__double_as_longlong(smax));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// the sweep kernel
+// ---------------------------------------------------------------------------
+template <class P, int REC, int RS, int ND, int DIR, typename T>
+__global__ __launch_bounds__(256) void k_sweep(SweepArgs a, RunConst rc) {
+  constexpr int NV = P::nv;
+  constexpr bool kLast = (DIR == ND - 1);
+  using RC = Recon<P, REC, ND, DIR, T>;
+  // chunked pencil I/O along x and y (see loadq below)
+  constexpr bool kChunk = DIR < 2;
+  constexpr bool kDin = kChunk && DIR > 0;       // increment read in chunks
+  constexpr bool kDout = kChunk && !kLast;       // increment written in chunks
+  // per-lane LDS slots of the chunked pencil I/O: sq holds four cells of the
+  // primitive window, sd four cells of the increment (4 NV values per lane
+  // each); sized by the launch for its workgroup size
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int wg = blockDim.x;
+  T* const sq = reinterpret_cast<T*>(smem);
+  T* const sd = sq + (kChunk ? 4 * NV * wg : 0);
+  const int nc = rc.nc;
+  const int npen = (ND == 3) ? nc * nc : nc;
+  const long long gid = (long long)workgroup_index(a.swz) * blockDim.x + threadIdx.x;
+  const int b = (int)(gid / npen);
+  const int t = (int)(gid % npen);
+  const bool active = b < rc.nblk;
+
+  if (a.first && gid < rc.nruns) {
+    // reset the maximum-speed accumulators of the next step (new_time_step:
+    // cmax = tiny before the reduction), one thread per run
+    a.rs[gid].cmax_bits[a.slot ^ 1] = (unsigned long long)__double_as_longlong(kTiny);
+  }
+
+  const int run = active ? a.brun[b] : 0;
+  const RunState* R = a.rs + run;
+  const double cmaxd = cmax_of(R, a.slot);
+  const T cmax = T(cmaxd), cmax2 = T(cmaxd * cmaxd);
+  const double dt = dt_of(rc, R, a.slot, a.dtnext);
+  const double ds = a.fac * dt;
+  const EosConst<T> e{T(rc.gamma), T(rc.gammam1), T(rc.gammam1i)};
+  const EosConst<double> ed{rc.gamma, rc.gammam1, rc.gammam1i};
+  const T* __restrict__ Qp = static_cast<const T*>(a.Q);
+  T* __restrict__ Dp = static_cast<T*>(a.D);
+  T* __restrict__ Qo = static_cast<T*>(a.Qout);
+  double smax = 0.0;
+
+  if (active) {
+    // transverse coordinates of the pencil
+    int c1 = t % nc, c2 = (ND == 3) ? t / nc : 0;
+    auto coords = [&](int c, int& i, int& j, int& k) {
+      if constexpr (DIR == 0) { i = c; j = c1; k = c2; }
+      else if constexpr (DIR == 1) { i = c1; j = c; k = c2; }
+      else { i = c1; j = c2; k = c; }
+    };
+    // x and y sweeps move their pencil data in chunks of four cells: a
+    // brick line spans four cells along x and along y, so loading (and
+    // storing) four cells at once uses whole lines instead of a quarter of a
+    // line per walk step.  The chunks live in per-lane LDS slots.
+    const int tid = threadIdx.x;
+    auto loadq = [&](int c, T* q) {
+      if constexpr (kChunk) {
+        const int first = -1 - RC::R;
+        if (c == first || (c & 3) == 0) {
+          const int c0 = c & ~3;
+#pragma unroll
+          for (int m = 0; m < 4; m++) {
+            int i, j, k;
+            coords(c0 + m, i, j, k);
+            const size_t pk = cell_index<sizeof(T)>(a.nbr, rc, b, i, j, k);
+#pragma unroll
+            for (int v = 0; v < NV; v++) sq[(m * NV + v) * wg + tid] = Qp[var_index<NV>(rc, pk, v)];
+          }
+        }
+#pragma unroll
+        for (int v = 0; v < NV; v++) q[v] = sq[((c & 3) * NV + v) * wg + tid];
+      } else {
+        int i, j, k;
+        coords(c, i, j, k);
+        size_t pk = cell_index<sizeof(T)>(a.nbr, rc, b, i, j, k);
+#pragma unroll
+        for (int v = 0; v < NV; v++) q[v] = Qp[var_index<NV>(rc, pk, v)];
+      }
+    };
+    // global offsets (without the variable term) of the four cells of the
+    // increment chunk that starts at cell c0
+    auto dchunk_base = [&](int c0, size_t* base) {
+#pragma unroll
+      for (int m = 0; m < 4; m++) {
+        int i, j, k;
+        coords(c0 + m, i, j, k);
+        base[m] = (size_t)b * NV * (size_t)rc.ncell + (size_t)lidx<sizeof(T)>(nc, ND, i, j, k);
+      }
+    };
+
+    RC rec;
+    T qtmp[NV];
+    // prime the window for cell c = -1: cells -1-R .. -1+R-1
+#pragma unroll
+    for (int m = 0; m < 2 * RC::R; m++) {
+      loadq(-1 - RC::R + m, qtmp);
+      rec.load(m, qtmp);
+    }
+    T flp[NV], fprev[NV];
+    for (int c = -1; c <= nc; c++) {
+      // operands of the cell finished in this iteration (c-1), loaded before
+      // the reconstruction and Riemann solve so that their latency overlaps
+      // the arithmetic
+      T dpre[NV];
+      double u0pre[NV], u1pre[NV];
+      float cpre[NV];
+      if (c >= 1) {
+        int i, j, k;
+        coords(c - 1, i, j, k);
+        const size_t bb = (size_t)b * NV * (size_t)rc.ncell;
+        const size_t bt = bb + (size_t)lidx<sizeof(T)>(nc, ND, i, j, k);
+        const size_t bd = bb + (size_t)lidx<8>(nc, ND, i, j, k);
+        const size_t bf = bb + (size_t)lidx<4>(nc, ND, i, j, k);
+        if constexpr (kDin) {
+          if (((c - 1) & 3) == 0) {
+            size_t base[4];
+            dchunk_base(c - 1, base);
+#pragma unroll
+            for (int m = 0; m < 4; m++)
+#pragma unroll
+              for (int v = 0; v < NV; v++)
+                sd[(m * NV + v) * wg + tid] = Dp[base[m] + (size_t)v * rc.ncell];
+          }
+        }
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+          const size_t off = (size_t)v * rc.ncell;
+          if constexpr (DIR > 0 && !kDin) dpre[v] = Dp[bt + off];
+          if constexpr (kLast) {
+            u0pre[v] = a.U0[bd + off];
+            if (a.kind != 0) u1pre[v] = a.U1[bd + off];
+            // the compensation enters the step update; the transient
+            // stages ignore it (it is below the FP64 rounding of U0)
+            if constexpr (sizeof(T) == 4) cpre[v] = a.last ? a.C[bf + off] : 0.0f;
+          }
+        }
+      }
+      if (c > -1) rec.shift();
+      loadq(c + RC::R, qtmp);
+      rec.load(2 * RC::R, qtmp);
+      T flc[NV], frc[NV];
+      int ci, cj, ck;
+      coords(c, ci, cj, ck);
+      rec.cell(rc, flc, frc, Qp, a.nbr, b, ci, cj, ck, a.st);
+      if (c >= 0) {
+        // face c-1/2: left state from cell c-1, right state from cell c
+        T ql[NV], qr[NV], fr[NV], f[NV];
+#pragma unroll
+        for (int p = 0; p < NV; p++) {
+          ql[p] = flp[rot<P, DIR>(p)];
+          qr[p] = frc[rot<P, DIR>(p)];
+        }
+        riemann<P, RS, T>(e, rc.sc, cmax, cmax2, ql, qr, fr);
+#pragma unroll
+        for (int p = 0; p < NV; p++) f[rot<P, DIR>(p)] = fr[p];
+        if (c >= 1) {
+          // cell c-1 has both faces
+          int i, j, k;
+          coords(c - 1, i, j, k);
+          const size_t bb = (size_t)b * NV * (size_t)rc.ncell;
+          const size_t bt = bb + (size_t)lidx<sizeof(T)>(nc, ND, i, j, k);
+          const size_t bd = bb + (size_t)lidx<8>(nc, ND, i, j, k);
+          const size_t bf = bb + (size_t)lidx<4>(nc, ND, i, j, k);
+          const T dxi = T(R->dxi[DIR]);
+          double unew[NV];
+#pragma unroll
+          for (int v = 0; v < NV; v++) {
+            const size_t off = (size_t)v * rc.ncell;
+            // update_increment order: -dxi*dfx - dyi*dfy - dzi*dfz
+            const T dfx = f[v] - fprev[v];
+            T du;
+            if constexpr (DIR == 0) du = -dxi * dfx;
+            else if constexpr (kDin) du = sd[(((c - 1) & 3) * NV + v) * wg + tid] - dxi * dfx;
+            else du = dpre[v] - dxi * dfx;
+            if constexpr (!kLast) {
+              if constexpr (kDout) sd[(((c - 1) & 3) * NV + v) * wg + tid] = du;
+              else Dp[bt + off] = du;
+            } else if constexpr (sizeof(T) == 8) {
+              double un = combine(a.kind, u0pre[v], u1pre[v], ds, du);
+              if (a.last && v == P::ibp) un = rc.decay * un;
+              unew[v] = un;
+            } else {
+              // mixed precision: the FP32 increment is added to the FP64
+              // state in increment form with Kahan compensation.  The true
+              // step state is U0 - C; the last stage forms U0_true + delta
+              // and keeps the rounding error of that sum in C (the
+              // intermediate stages use U0 alone).
+              const double u0 = u0pre[v];
+              const double cp = double(cpre[v]);
+              const double dd = ds * double(du);
+              double delta;
+              switch (a.kind) {
+              case 0: delta = dd; break;
+              case 1: delta = 0.25 * ((u1pre[v] - u0) + cp) + dd; break;
+              case 2: delta = (2.0 / 3.0) * ((u1pre[v] - u0) + cp) + dd; break;
+              default: delta = 0.5 * (((u1pre[v] - u0) + cp) + dd); break;
+              }
+              const double y = delta - cp;
+              double un = u0 + y;
+              if (a.last) {
+                double cn = (un - u0) - y;
+                if (v == P::ibp) { un = rc.decay * un; cn = rc.decay * cn; }
+                a.C[bf + off] = float(cn);
+              }
+              unew[v] = un;
+            }
+          }
+          if constexpr (kLast) {
+            double qn[NV];
+            if constexpr (P::sr) {
+              const int cs = cons2prim_sr<P>(rc.sc, unew, qn);
+              if (cs) {
+                c2p_keep_old<P, T>(cs, qn, Qp, bt, rc.ncell);
+                c2p_count(a.st, cs);
+              }
+            } else {
+              cons2prim<P, double>(ed, unew, qn);
+            }
+#pragma unroll
+            for (int v = 0; v < NV; v++) {
+              const size_t off = (size_t)v * rc.ncell;
+              a.Uout[bd + off] = unew[v];
+              Qo[bt + off] = T(qn[v]);
+            }
+            if (a.last) smax = fmax(smax, maxspeed<P, double>(ed, qn));
+          }
+        }
+        if constexpr (kDout) {
+          if (c >= 1 && ((c - 1) & 3) == 3) {
+            size_t base[4];
+            dchunk_base(c - 4, base);
+#pragma unroll
+            for (int m = 0; m < 4; m++)
+#pragma unroll
+              for (int v = 0; v < NV; v++)
+                Dp[base[m] + (size_t)v * rc.ncell] = sd[(m * NV + v) * wg + tid];
+          }
+        }
+#pragma unroll
+        for (int p = 0; p < NV; p++) fprev[p] = f[p];
+      }
+#pragma unroll
+      for (int p = 0; p < NV; p++) flp[p] = flc[p];
+    }
+  }
+
+  if constexpr (kLast) {
+    if (a.last) wave_max_speed(a.rs, run, a.slot ^ 1, smax);
+  }
+}
+
+// Q = cons2prim(U) on every interior cell, and the maximum speed into a slot
+// (update_primitive_variables + maxspeed); used after uploads
+template <class P, typename T>
+__global__ __launch_bounds__(256) void k_prim(const double* __restrict__ U,
+                                              void* Qv, RunConst rc, DevState* st,
+                                              RunState* rs, const int* __restrict__ brun,
+                                              int slot, int b0, int nb) {
+  T* __restrict__ Q = static_cast<T*>(Qv);
+  constexpr int NV = P::nv;
+  const long long n = (long long)nb * rc.ncell;
+  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const EosConst<double> e{rc.gamma, rc.gammam1, rc.gammam1i};
+  double smax = 0.0;
+  int run = 0;
+  if (gid < n) {
+    const long long b = b0 + gid / rc.ncell, c = gid % rc.ncell;
+    run = brun[b];
+    const int i = (int)(c % rc.nc), j = (int)((c / rc.nc) % rc.nc), k = (int)(c / ((long long)rc.nc * rc.nc));
+    const long long cu = lidx<8>(rc.nc, rc.nd, i, j, k), cq = lidx<sizeof(T)>(rc.nc, rc.nd, i, j, k);
+    double u[NV], q[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) u[v] = U[(b * NV + v) * rc.ncell + cu];
+    if constexpr (P::sr) {
+      const int cs = cons2prim_sr<P>(rc.sc, u, q);
+      if (cs) {
+        c2p_keep_old<P, T>(cs, q, Q, (size_t)b * NV * rc.ncell + cq, rc.ncell);
+        c2p_count(st, cs);
+      }
+    } else {
+      cons2prim<P, double>(e, u, q);
+    }
+#pragma unroll
+    for (int v = 0; v < NV; v++) Q[(b * NV + v) * rc.ncell + cq] = T(q[v]);
+    smax = maxspeed<P, double>(e, q);
+  }
+  wave_max_speed(rs, run, slot, smax);
+}
+
+static __global__ void k_zero_f32(float* x, long long n) {
+  for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < n;
+       g += (long long)gridDim.x * blockDim.x)
+    x[g] = 0.0f;
+}
+
+static __global__ void k_reset_slot(RunState* rs, int slot, int r0, int nr) {
+  const int r = r0 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (r < r0 + nr) rs[r].cmax_bits[slot] = (unsigned long long)__double_as_longlong(kTiny);
+}
+
+// end of a step, one thread per run: dt into the trace ring
+// trace[(step mod tsteps) * nruns + run], time and step advanced
+static __global__ void k_end_step(RunState* rs, double* trace, int tsteps, RunConst rc, int slot,
+                           double dtnext) {
+  const int r = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (r >= rc.nruns) return;
+  RunState* R = rs + r;
+  const double dt = dt_of(rc, R, slot, dtnext);
+  trace[(size_t)(R->step % tsteps) * rc.nruns + r] = dt;
+  R->dt = dt;
+  R->time += dt;
+  R->step += 1;
+}
+
+// dt of the current slot without advancing (new_time_step alone), per run:
+// out[2 r] = dt, out[2 r + 1] = cmax
+static __global__ void k_peek_dt(RunState* rs, RunConst rc, int slot, double dtnext, double* out) {
+  const int r = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (r >= rc.nruns) return;
+  RunState* R = rs + r;
+  const double dt = dt_of(rc, R, slot, dtnext);
+  R->dt = dt;
+  out[2 * r] = dt;
+  out[2 * r + 1] = cmax_of(R, slot);
+}
+
+// volume integrals (store_integrals, integrals.F90:331-358): per-workgroup
+// partial sums, then a fixed-order final sum (deterministic)
+template <class P>
+__global__ __launch_bounds__(256) void k_integrals(const double* __restrict__ U,
+                                                   RunConst rc, double* part, int b0,
+                                                   int nb) {
+  constexpr int NV = P::nv;
+  __shared__ double sh[8][256];
+  U += (size_t)b0 * NV * rc.ncell;
+  const long long n = (long long)nb * rc.ncell;
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < n;
+       g += (long long)gridDim.x * blockDim.x) {
+    long long b = g / rc.ncell, c = g % rc.ncell;
+    double u[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) u[v] = U[(b * NV + v) * rc.ncell + c];
+    acc[0] += u[P::idn];
+    acc[1] += u[P::imx];
+    acc[2] += u[P::imy];
+    acc[3] += u[P::imz];
+    acc[4] += u[P::ien];
+    double ek = (u[P::imx] * u[P::imx] + u[P::imy] * u[P::imy] + u[P::imz] * u[P::imz]) / u[P::idn];
+    acc[5] += ek;
+    double em = 0.0;
+    if constexpr (P::mhd) em = u[P::ibx] * u[P::ibx] + u[P::iby] * u[P::iby] + u[P::ibz] * u[P::ibz];
+    acc[6] += em;
+    acc[7] += u[P::ien] - 0.5 * ek - 0.5 * em;
+  }
+#pragma unroll
+  for (int m = 0; m < 8; m++) sh[m][threadIdx.x] = acc[m];
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+#pragma unroll
+      for (int m = 0; m < 8; m++) sh[m][threadIdx.x] += sh[m][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int m = 0; m < 8; m++) part[blockIdx.x * 8 + m] = sh[m][0];
+}
+
+// pack interior + ghost layers of every block into the reference host layout
+// u(nv,im,jm,km) (boundary_variables: ghost cells are neighbour copies)
+template <int NV, typename TX>
+__global__ void k_pack_blocks(const void* Xv, double* __restrict__ out,
+                              const int* __restrict__ nbr, RunConst rc, int ng) {
+  const TX* __restrict__ X = static_cast<const TX*>(Xv);
+  const int nc = rc.nc;
+  const int im = nc + 2 * ng, km = (rc.nd == 3) ? im : 1;
+  const long long per = (long long)im * im * km;
+  const long long n = per * rc.nblk;
+  for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < n;
+       g += (long long)gridDim.x * blockDim.x) {
+    int b = (int)(g / per);
+    long long r = g % per;
+    int i = (int)(r % im), j = (int)((r / im) % im), k = (int)(r / ((long long)im * im));
+    int li = i - ng, lj = j - ng, lk = (rc.nd == 3) ? k - ng : 0;
+    size_t pk = cell_index<sizeof(TX)>(nbr, rc, b, li, lj, lk);
+#pragma unroll
+    for (int v = 0; v < NV; v++) out[(size_t)g * NV + v] = double(X[var_index<NV>(rc, pk, v)]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// unit-level kernels (parity tests): the pointwise routines of the sweeps on
+// vectors of states, in the arithmetic type of the configuration, one thread
+// per state (the riemann_iface / prim2cons / cons2prim / fluxspeed seams of
+// schemes.F90:66-71 and equations.F90:70-108)
+// ---------------------------------------------------------------------------
+template <class P, int RS, typename T>
+__global__ void k_unit_riemann(int n, double cmaxd, const double* __restrict__ ql,
+                               const double* __restrict__ qr, double* __restrict__ f,
+                               RunConst rc) {
+  constexpr int NV = P::nv;
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i >= n) return;
+  const EosConst<T> e{T(rc.gamma), T(rc.gammam1), T(rc.gammam1i)};
+  T a[NV], b[NV], g[NV];
+#pragma unroll
+  for (int v = 0; v < NV; v++) {
+    a[v] = T(ql[(size_t)i * NV + v]);
+    b[v] = T(qr[(size_t)i * NV + v]);
+  }
+  riemann<P, RS, T>(e, rc.sc, T(cmaxd), T(cmaxd * cmaxd), a, b, g);
+#pragma unroll
+  for (int v = 0; v < NV; v++) f[(size_t)i * NV + v] = double(g[v]);
+}
+
+// u = prim2cons(q), f/cm/cp = fluxspeed(q, u) in T; q2 = cons2prim(u) in
+// double (the kernels recover primitives in FP64 in every mode; q2 enters
+// holding the previous primitives, kept by a failed relativistic recovery)
+template <class P, typename T>
+__global__ void k_unit_eqs(int n, double cmaxd, const double* __restrict__ q,
+                           double* __restrict__ u, double* __restrict__ q2,
+                           double* __restrict__ f, double* __restrict__ cm,
+                           double* __restrict__ cp, RunConst rc, DevState* st) {
+  constexpr int NV = P::nv;
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i >= n) return;
+  const EosConst<T> e{T(rc.gamma), T(rc.gammam1), T(rc.gammam1i)};
+  const EosConst<double> ed{rc.gamma, rc.gammam1, rc.gammam1i};
+  T qq[NV], uu[NV], ff[NV], m, pp;
+#pragma unroll
+  for (int v = 0; v < NV; v++) qq[v] = T(q[(size_t)i * NV + v]);
+  const T cmax2 = T(cmaxd * cmaxd);
+  if constexpr (P::sr) {
+    prim2cons_sr<P>(T(rc.sc.rgammaxi), qq, uu);
+    fluxspeed_sr<P>(rc.sc, cmax2, qq, uu, ff, m, pp);
+  } else {
+    prim2cons<P>(e, qq, uu);
+    fluxspeed<P>(e, cmax2, qq, uu, ff, m, pp);
+  }
+  double ud[NV], qd[NV];
+#pragma unroll
+  for (int v = 0; v < NV; v++) {
+    ud[v] = double(uu[v]);
+    qd[v] = q2[(size_t)i * NV + v];
+  }
+  if constexpr (P::sr) {
+    const int cs = cons2prim_sr<P>(rc.sc, ud, qd);
+    if (cs) c2p_count(st, cs);
+  } else {
+    cons2prim<P, double>(ed, ud, qd);
+  }
+#pragma unroll
+  for (int v = 0; v < NV; v++) {
+    u[(size_t)i * NV + v] = ud[v];
+    q2[(size_t)i * NV + v] = qd[v];
+    f[(size_t)i * NV + v] = double(ff[v]);
+  }
+  cm[i] = double(m);
+  cp[i] = double(pp);
+}
+
+typedef void (*SweepFn)(SweepArgs, RunConst);
+
+}  // namespace amun_dev
+
+#include "adjoint.h"
+
+using namespace amun_dev;
+
+// kernel instantiations selected for one configuration
+struct Dispatch {
+  SweepFn sweep[3] = {nullptr, nullptr, nullptr};
+  void (*prim)(const double*, void*, RunConst, DevState*, RunState*, const int*, int, int,
+               int) = nullptr;
+  void (*integ)(const double*, RunConst, double*, int, int) = nullptr;
+  void (*pack_u)(const void*, double*, const int*, RunConst, int) = nullptr;
+  void (*pack_q)(const void*, double*, const int*, RunConst, int) = nullptr;
+  void (*unit_riemann)(int, double, const double*, const double*, double*, RunConst) = nullptr;
+  void (*unit_eqs)(int, double, const double*, double*, double*, double*, double*, double*,
+                   RunConst, DevState*) = nullptr;
+  // tangent-linear / adjoint kernels (FP64, TVD reconstruction, HD and MHD)
+  bool has_adj = false;
+  void (*adj_prim)(const double*, double*, RunConst) = nullptr;
+  void (*adj_face[3])(const double*, const double*, double*, double*, double*, const int*,
+                      RunConst, double, double) = {nullptr, nullptr, nullptr};
+  void (*tlm_face[3])(const double*, const double*, const double*, double, double*, const int*,
+                      RunConst, double) = {nullptr, nullptr, nullptr};
+  void (*adj_cell)(const double*, const double*, const double*, const double*, const double*,
+                   const double*, const double*, const double*, const double*, const double*,
+                   const double*, double*, double*, const int*, RunConst) = nullptr;
+  void (*tlm_cell)(const double*, const double*, const double*, double*, const int*, RunConst,
+                   double, double, double) = nullptr;
+};
+
+
+namespace amun_dev {
+
+template <class P, int REC, int RS, int ND, typename T>
+void fill(Dispatch& d) {
+  d.sweep[0] = k_sweep<P, REC, RS, ND, 0, T>;
+  d.sweep[1] = k_sweep<P, REC, RS, ND, 1, T>;
+  if constexpr (ND == 3) d.sweep[2] = k_sweep<P, REC, RS, ND, 2, T>;
+  d.prim = k_prim<P, T>;
+  d.integ = k_integrals<P>;
+  d.pack_u = k_pack_blocks<P::nv, double>;
+  d.pack_q = k_pack_blocks<P::nv, T>;
+  d.unit_riemann = k_unit_riemann<P, RS, T>;
+  d.unit_eqs = k_unit_eqs<P, T>;
+  if constexpr (REC == REC_TVD && !P::sr && std::is_same<T, double>::value) {
+    d.has_adj = true;
+    d.adj_prim = k_adj_prim<P>;
+    d.adj_face[0] = k_adj_face<P, RS, ND, 0>;
+    d.adj_face[1] = k_adj_face<P, RS, ND, 1>;
+    d.tlm_face[0] = k_tlm_face<P, RS, ND, 0>;
+    d.tlm_face[1] = k_tlm_face<P, RS, ND, 1>;
+    if constexpr (ND == 3) {
+      d.adj_face[2] = k_adj_face<P, RS, ND, 2>;
+      d.tlm_face[2] = k_tlm_face<P, RS, ND, 2>;
+    }
+    d.adj_cell = k_adj_cell<P, ND>;
+    d.tlm_cell = k_tlm_cell<P, ND>;
+  }
+}
+
+template <class P, int RS, int REC, typename T>
+bool pick_nd(Dispatch& d, int nd) {
+  if (nd == 2) { fill<P, REC, RS, 2, T>(d); return true; }
+  if (nd == 3) { fill<P, REC, RS, 3, T>(d); return true; }
+  return false;
+}
+
+template <int REC, typename T>
+bool pick_rec_t(Dispatch& d, const amun_cfg& c) {
+  if (c.eqsys == AMUN_EQSYS_HD) {
+    if (c.riemann_solver == AMUN_RS_HLL) return pick_nd<HD, RS_HLL, REC, T>(d, c.ndims);
+    if (c.riemann_solver == AMUN_RS_HLLC) return pick_nd<HD, RS_HLLC, REC, T>(d, c.ndims);
+  } else if (c.eqsys == AMUN_EQSYS_MHD) {
+    if (c.riemann_solver == AMUN_RS_HLL) return pick_nd<MHD, RS_HLL, REC, T>(d, c.ndims);
+    if (c.riemann_solver == AMUN_RS_HLLD) return pick_nd<MHD, RS_HLLD, REC, T>(d, c.ndims);
+  } else if (c.eqsys == AMUN_EQSYS_SRMHD) {
+    // every solver name but "hllc" selects HLL (schemes.F90:479-498)
+    return pick_nd<SRMHD, RS_HLL, REC, T>(d, c.ndims);
+  }
+  return false;
+}
+
+// kernel table of one reconstruction (instantiated by inst_<rec>.hip)
+template <int REC>
+bool pick_rec(Dispatch& d, const amun_cfg& c) {
+  return c.precision == AMUN_PREC_MIXED ? pick_rec_t<REC, float>(d, c)
+                                        : pick_rec_t<REC, double>(d, c);
+}
+
+bool pick_tvd(Dispatch& d, const amun_cfg& c);
+bool pick_weno5z(Dispatch& d, const amun_cfg& c);
+bool pick_gp(Dispatch& d, const amun_cfg& c);
+
+}  // namespace amun_dev

@@ -162,6 +162,52 @@ __device__ __forceinline__ void weno5z_cell(const T f[5], T eps, T& fl, T& fr) {
 }
 
 // ---------------------------------------------------------------------------
+// Gaussian-process reconstruction of one cell (reconstruct_gp,
+// interpolations.F90:4561-4668) with the monotonicity-preserving limiter of
+// Suresh & Huynh (mp_limiting, 5209-5284): from f[0..4] = f(i-2..i+2) and the
+// weights cgp of prepare_gp, the state at the upper face (fl) and, from the
+// mirrored stencil, at the lower face (fr)
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T mmod(T a, T b) {   // minmod, interpolations.F90:4892
+  return (tsign(T(0.5), a) + tsign(T(0.5), b)) * tmin(tabs(a), tabs(b));
+}
+
+template <typename T>
+__device__ __forceinline__ T gp_face(const T f[5], const T cgp[5], T kappa, T kbeta, T eps) {
+  const T fi = f[2];
+  T u = cgp[0] * (f[0] - fi);
+#pragma unroll
+  for (int m = 1; m < 5; m++) u = u + cgp[m] * (f[m] - fi);
+  u = u + fi;
+  // dm(k) = f(k) - f(k-1) around the cell
+  const T dmm = f[1] - f[0], dm0 = f[2] - f[1], dmp = f[3] - f[2], dmpp = f[4] - f[3];
+  const T df = (dm0 * dmp >= T(0)) ? kappa * dm0 : kbeta * dm0;
+  const T fmp = fi + mmod(dmp, df);
+  const T ds = (u - fi) * (u - fmp);
+  if (ds > eps) {
+    const T dm1 = dm0 - dmm, dc0 = dmp - dm0, dp1 = dmpp - dmp, dc4 = T(4) * dc0;
+    const T dml = T(0.5) * mmod(mmod(dc4 - dm1, T(4) * dm1 - dc0), mmod(dc0, dm1));
+    const T dmr = T(0.5) * mmod(mmod(dc4 - dp1, T(4) * dp1 - dc0), mmod(dc0, dp1));
+    const T fmd = (fi + T(0.5) * dmp) - dmr;
+    const T ful = fi + df;
+    const T flc = (fi + T(0.5) * df) + dml;
+    const T fmx = tmax(tmin(tmin(fi, f[3]), fmd), tmin(tmin(fi, ful), flc));
+    const T fmn = tmin(tmax(tmax(fi, f[3]), fmd), tmax(tmax(fi, ful), flc));
+    u = u + mmod(fmn - u, fmx - u);               // median(u, fmn, fmx)
+  }
+  return u;
+}
+
+template <typename T>
+__device__ __forceinline__ void gp_cell(const T f[5], const T cgp[5], T kappa, T kbeta, T eps,
+                                        T& fl, T& fr) {
+  fl = gp_face<T>(f, cgp, kappa, kbeta, eps);
+  const T g[5] = {f[4], f[3], f[2], f[1], f[0]};
+  fr = gp_face<T>(g, cgp, kappa, kbeta, eps);
+}
+
+// ---------------------------------------------------------------------------
 // equation of state: prim2cons / cons2prim (equations.F90:1762-1876 HD,
 // 2815-2939 MHD); fluxes and fast speeds (1896-1958, 2959-3045)
 // ---------------------------------------------------------------------------

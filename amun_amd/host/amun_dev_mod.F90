@@ -19,6 +19,7 @@ module amun_dev_mod
   integer(c_int32_t), parameter :: AMUN_EQSYS_SRMHD = 4
   integer(c_int32_t), parameter :: AMUN_EOS_ADI = 1
   integer(c_int32_t), parameter :: AMUN_REC_TVD = 1, AMUN_REC_WENO5Z = 5
+  integer(c_int32_t), parameter :: AMUN_REC_GP = 20
   integer(c_int32_t), parameter :: AMUN_LIM_ZERO = 0, AMUN_LIM_MINMOD = 1
   integer(c_int32_t), parameter :: AMUN_LIM_MC = 2, AMUN_LIM_SUPERBEE = 3
   integer(c_int32_t), parameter :: AMUN_LIM_VANLEER = 4, AMUN_LIM_VANALBADA = 5
@@ -38,6 +39,8 @@ module amun_dev_mod
     real(c_double)     :: xmin, xmax, ymin, ymax, zmin, zmax
     real(c_double)     :: tolerance, pmin
     integer(c_int32_t) :: nr_maxit, nr_extra
+    real(c_double)     :: sgp, kappa, kbeta
+    integer(c_int32_t) :: ngp, reserved
   end type amun_cfg
 
   interface

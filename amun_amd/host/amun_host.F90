@@ -72,6 +72,8 @@ program amun_host
     cfg%reconstruction = AMUN_REC_TVD
   case("weno5z", "weno5-z", "WENO5Z", "WENO5-Z")
     cfg%reconstruction = AMUN_REC_WENO5Z
+  case("gp", "GP")
+    cfg%reconstruction = AMUN_REC_GP
   case default
     cfg%reconstruction = -1
   end select
@@ -133,6 +135,10 @@ program amun_host
   cfg%pmin      = get_real("pmin", 1.0d-16)
   cfg%nr_maxit  = get_int("nr_maxit", 100)
   cfg%nr_extra  = get_int("nr_extra", 2)
+  cfg%sgp       = get_real("sgp", 9.0d0)
+  cfg%kappa     = get_real("kappa", 1.0d0)
+  cfg%kbeta     = get_real("kbeta", 1.0d0)
+  cfg%ngp       = get_int("ngp", 5)
   nmax    = get_int("nmax", 10)
   tmax    = get_real("tmax", 1.0d30)
   seed    = int(get_int("seed", 20261202), int64)

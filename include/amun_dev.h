@@ -42,6 +42,7 @@ extern "C" {
 /* reconstructions: interpolations.F90:237-376 (parameter "reconstruction") */
 #define AMUN_REC_TVD       1
 #define AMUN_REC_WENO5Z    5
+#define AMUN_REC_GP        20  /* Gaussian process, reconstruct_gp interpolations.F90:4561 */
 
 /* TVD limiters: interpolations.F90:379-398 (parameter "limiter") */
 #define AMUN_LIM_ZERO      0
@@ -93,6 +94,13 @@ typedef struct amun_cfg {
   double  pmin;             /* "pmin"      (1e-16)                            */
   int32_t nr_maxit;         /* "nr_maxit"  (100)                              */
   int32_t nr_extra;         /* "nr_extra"  (2)                                */
+  /* Gaussian-process reconstruction and its monotonicity-preserving limiter
+   * (interpolations.F90:87-106, 206-221) */
+  double  sgp;              /* "sgp": kernel width in cells (9)               */
+  double  kappa;            /* "kappa" (1), used as min(kappa, (1-cfl)/cfl)   */
+  double  kbeta;            /* "kbeta" (1)                                    */
+  int32_t ngp;              /* "ngp": stencil points (5)                      */
+  int32_t reserved;
 } amun_cfg;
 
 typedef struct amun_ctx amun_ctx;

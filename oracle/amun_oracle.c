@@ -40,6 +40,7 @@
  * prim -> cons -> prim round trip of the relativistic primitive solver.
  */
 #include <math.h>
+#include <quadmath.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -63,7 +64,12 @@ typedef struct {
   double cmax, cmax2;      /* equations module state                         */
   long   npos_events;      /* "Positive variable is not positive" prints     */
   long   c2p_fail, c2p_pneg, c2p_noconv;   /* SR primitive solver messages */
+  double kappa;            /* min(kappa, (1 - cfl)/cfl), interpolations.F90:217 */
+  int    mgp;              /* (ngp - 1) / 2                                  */
+  double cgp[16];          /* Gaussian-process weights                      */
 } ocfg;
+
+static int prepare_gp(ocfg* o);
 
 static int oc_setup(ocfg* o, const amun_cfg* c)
 {
@@ -107,6 +113,11 @@ static int oc_setup(ocfg* o, const amun_cfg* c)
   o->gammam1  = c->gamma - 1.0;
   o->gammam1i = 1.0 / o->gammam1;
   o->gammaxi  = o->gammam1 / c->gamma;       /* equations.F90:843 */
+  o->kappa    = fmin(c->kappa, (1.0 - c->cfl) / c->cfl);   /* interpolations.F90:217 */
+  if (c->reconstruction == AMUN_REC_GP) {
+    int r = prepare_gp(o);
+    if (r) return r;
+  }
   o->cflx  = c->cfl;       /* euler, rk2 and rk3 keep the CFL unchanged */
   o->decay = exp(-c->alpha * o->cflx);
   {
@@ -238,9 +249,188 @@ static void fix_positivity(int n, const double* f, double* fl, double* fr)
 }
 
 /* ------------------------------------------------------------------------ */
+/* Gaussian-process reconstruction                                           */
+/* ------------------------------------------------------------------------ */
+/* invert, algebra.F90:1198-1305: Gauss-Jordan elimination in quad precision
+ * (max_real_kind, algebra.F90:42); returns 0 when the matrix is singular */
+static int invert_q(int n, const __float128* m, __float128* r)
+{
+  __float128 g[16][32], t;
+  int i, j, k, flag;
+  for (i = 0; i < n; i++)
+    for (j = 0; j < 2 * n; j++)
+      g[i][j] = (j < n) ? m[i * n + j] : ((i + n == j) ? 1.0Q : 0.0Q);
+  for (k = 0; k < n - 1; k++) {
+    if (g[k][k] == 0.0Q) {
+      flag = 0;
+      for (i = k + 1; i < n; i++) {
+        if (g[i][k] != 0.0Q) {
+          for (j = 0; j < 2 * n; j++) g[k][j] = g[k][j] + g[i][j];
+          flag = 1;
+          break;
+        }
+      }
+      if (!flag) return 0;
+    }
+    for (j = k + 1; j < n; j++) {
+      t = g[j][k] / g[k][k];
+      for (i = k; i < 2 * n; i++) g[j][i] = g[j][i] - t * g[k][i];
+    }
+  }
+  for (i = 0; i < n; i++)
+    if (g[i][i] == 0.0Q) return 0;
+  for (i = 0; i < n; i++) {
+    t = g[i][i];
+    for (j = i; j < 2 * n; j++) g[i][j] = g[i][j] / t;
+  }
+  for (k = n - 2; k >= 0; k--)
+    for (i = 0; i <= k; i++) {
+      t = g[i][k + 1];
+      for (j = k; j < 2 * n; j++) g[i][j] = g[i][j] - g[k + 1][j] * t;
+    }
+  for (i = 0; i < n; i++)
+    for (j = 0; j < n; j++) r[i * n + j] = g[i][j + n];
+  return 1;
+}
+
+/* prepare_gp, interpolations.F90:4438-4547: the weights cgp = x^T A^-1 of
+ * the Gaussian process with an integrated squared-exponential kernel of width
+ * sgp over ngp cells; quad precision as in the reference, with its mixed
+ * precision constants (sqrt(2) sgp and sqrt(pi)/2 are formed in double) */
+static int prepare_gp(ocfg* o)
+{
+  const int ngp = o->c.ngp, mgp = (ngp - 1) / 2;
+  __float128 cov[256], agp[256], xgp[16], sig, fc, zl, zr;
+  int i, j, ip, jp;
+  if (ngp < 1 || ngp > 15 || ngp % 2 == 0) return 202;
+  sig = (__float128)(sqrt(2.0) * o->c.sgp);
+  fc  = (__float128)(0.5 * sqrt(3.14159265358979323846264338327950)) * sig;
+  for (ip = -mgp, i = 0; ip <= mgp; ip++, i++)
+    for (jp = -mgp, j = 0; jp <= mgp; jp++, j++) {
+      zl = ((__float128)(1.0 * (ip - jp)) - 0.5Q) / sig;
+      zr = ((__float128)(1.0 * (ip - jp)) + 0.5Q) / sig;
+      cov[i * ngp + j] = fc * (erfq(zr) - erfq(zl));
+    }
+  if (!invert_q(ngp, cov, agp)) return 202;
+  for (j = 0; j < ngp; j++)
+    for (i = 0; i < ngp; i++)
+      if (i != j) {
+        fc = 0.5Q * (agp[i * ngp + j] + agp[j * ngp + i]);
+        agp[i * ngp + j] = fc;
+        agp[j * ngp + i] = fc;
+      }
+  for (jp = -mgp, j = 0; jp <= mgp; jp++, j++) {
+    zr = (__float128)(0.5 - 1.0 * jp) / sig;
+    xgp[j] = expq(-(zr * zr));
+  }
+  for (j = 0; j < ngp; j++) {
+    __float128 acc = 0.0Q;
+    for (i = 0; i < ngp; i++) acc = acc + xgp[i] * agp[i * ngp + j];
+    o->cgp[j] = (double)acc;
+  }
+  o->mgp = mgp;
+  return 0;
+}
+
+/* minmod, minmod4, median (interpolations.F90:4892-4975) */
+static double mmod(double a, double b)
+{
+  return (copysign(0.5, a) + copysign(0.5, b)) * fmin(fabs(a), fabs(b));
+}
+static double mmod4(double a, double b, double c, double d) { return mmod(mmod(a, b), mmod(c, d)); }
+static double median3(double a, double b, double c) { return a + mmod(b - a, c - a); }
+
+/* mp_limiting, interpolations.F90:5209-5284 (Suresh & Huynh 1997) on the
+ * left-face values v of pencil f */
+static void mp_limiting(const ocfg* o, int n, const double* f, double* v, double* dm)
+{
+  const double kappa = o->kappa, kbeta = o->c.kbeta, eps = o->c.eps;
+  int i;
+  /* dm[k] holds dm(k) of the reference, k = 0 .. n + 2 */
+  dm[0] = 0.0;
+  dm[1] = 0.0;
+  for (i = 2; i <= n; i++) dm[i] = f[i - 1] - f[i - 2];
+  dm[n + 1] = 0.0;
+  dm[n + 2] = 0.0;
+  for (i = 1; i <= n - 1; i++) {
+    double df, ds, fmp, dm1, dc0, dp1, dc4, dml, dmr, fmd, ful, flc, fmx, fmn;
+    const double fi = f[i - 1];
+    if (dm[i] * dm[i + 1] >= 0.0) df = kappa * dm[i];
+    else                          df = kbeta * dm[i];
+    fmp = fi + mmod(dm[i + 1], df);
+    ds  = (v[i - 1] - fi) * (v[i - 1] - fmp);
+    if (ds > eps) {
+      dm1 = dm[i] - dm[i - 1];
+      dc0 = dm[i + 1] - dm[i];
+      dp1 = dm[i + 2] - dm[i + 1];
+      dc4 = 4.0 * dc0;
+      dml = 0.5 * mmod4(dc4 - dm1, 4.0 * dm1 - dc0, dc0, dm1);
+      dmr = 0.5 * mmod4(dc4 - dp1, 4.0 * dp1 - dc0, dc0, dp1);
+      fmd = (fi + 0.5 * dm[i + 1]) - dmr;
+      ful = fi + df;
+      flc = (fi + 0.5 * df) + dml;
+      fmx = fmax(fmin(fmin(fi, f[i]), fmd), fmin(fmin(fi, ful), flc));
+      fmn = fmin(fmax(fmax(fi, f[i]), fmd), fmax(fmax(fi, ful), flc));
+      v[i - 1] = median3(v[i - 1], fmn, fmx);
+    }
+  }
+}
+
+/* one direction of reconstruct_gp: u(1..n) of pencil f, mp-limited */
+static void gp_pass(const ocfg* o, int n, const double* f, double* u, double* dm)
+{
+  static const double ce5[5] = {2.0 / 6.0e+01, -1.3e+01 / 6.0e+01, 4.70e+01 / 6.0e+01,
+                                2.70e+01 / 6.0e+01, -3.0 / 6.0e+01};
+  static const double ce3[3] = {-1.0 / 6.0, 5.0 / 6.0, 2.0 / 6.0};
+  const int mgp = o->mgp, ngp = o->c.ngp;
+  int i, m;
+  u[0] = 5.0e-01 * f[0] + 5.0e-01 * f[1];
+  u[1] = (ce3[0] * f[0] + ce3[1] * f[1]) + ce3[2] * f[2];
+  for (i = 3; i <= mgp; i++)
+    u[i - 1] = (((ce5[0] * f[i - 3] + ce5[1] * f[i - 2]) + ce5[2] * f[i - 1]) + ce5[3] * f[i])
+               + ce5[4] * f[i + 1];
+  for (i = n + 1 - mgp; i <= n - 2; i++)
+    u[i - 1] = (((ce5[0] * f[i - 3] + ce5[1] * f[i - 2]) + ce5[2] * f[i - 1]) + ce5[3] * f[i])
+               + ce5[4] * f[i + 1];
+  u[n - 2] = (ce3[0] * f[n - 3] + ce3[1] * f[n - 2]) + ce3[2] * f[n - 1];
+  u[n - 1] = f[n - 1];
+  for (i = 1 + mgp; i <= n - mgp; i++) {
+    const double fi = f[i - 1];
+    double acc = o->cgp[0] * (f[i - 1 - mgp] - fi);
+    for (m = 1; m < ngp; m++) acc = acc + o->cgp[m] * (f[i - 1 - mgp + m] - fi);
+    u[i - 1] = acc + fi;
+  }
+  mp_limiting(o, n, f, u, dm);
+}
+
+/* reconstruct_gp, interpolations.F90:4561-4668 */
+static void reconstruct_gp(const ocfg* o, int n, const double* f, double* fl, double* fr,
+                           double* w1, double* w2, double* w3)
+{
+  int i;
+  gp_pass(o, n, f, fl, w3);
+  for (i = 0; i < n; i++) w1[i] = f[n - 1 - i];
+  gp_pass(o, n, w1, w2, w3);
+  for (i = 0; i < n - 1; i++) fr[i] = w2[n - 2 - i];
+  i = n - 2;
+  fl[0] = 0.5 * (f[0] + f[1]);
+  fr[i] = 0.5 * (f[i] + f[n - 1]);
+  fl[n - 1] = f[n - 1];
+  fr[n - 1] = f[n - 1];
+}
+
+/* ------------------------------------------------------------------------ */
 /* reconstruct_interfaces (schemes.F90:591-633) for one block                 */
 /* qs layout: [(d*2 + s)][k][j][i][p]                                         */
 /* ------------------------------------------------------------------------ */
+/* reconstruct_states of interfaces_dir: the selected 1D scheme */
+static void recon_dir(const ocfg* o, int n, const double* f, double* fl, double* fr,
+                      double* w1, double* w2, double* w3)
+{
+  if (o->c.reconstruction == AMUN_REC_GP) reconstruct_gp(o, n, f, fl, fr, w1, w2, w3);
+  else weno5z(o, n, f, fl, fr, w1, w2, w3);
+}
+
 static void reconstruct_interfaces(ocfg* o, const double* q, double* qs)
 {
   const int im = o->im, jm = o->jm, km = o->km, nv = o->nv, nd = o->nd;
@@ -300,18 +490,18 @@ static void reconstruct_interfaces(ocfg* o, const double* q, double* qs)
       int nmax = im > jm ? im : jm;
       double *fp, *fl, *fr, *w1, *w2, *w3;
       if (km > nmax) nmax = km;
-      fp = (double*)malloc(6 * (size_t)nmax * sizeof(double));
+      fp = (double*)malloc(8 * (size_t)(nmax + 4) * sizeof(double));
       fl = fp + nmax; fr = fl + nmax; w1 = fr + nmax; w2 = w1 + nmax; w3 = w2 + nmax;
       for (k = o->kbl; k <= o->keu; k++) {
         for (j = o->jbl; j <= o->jeu; j++) {
           for (i = 0; i < im; i++) fp[i] = q[QI(o, p, i, j, k)];
-          weno5z(o, im, fp, fl, fr, w1, w2, w3);
+          recon_dir(o, im, fp, fl, fr, w1, w2, w3);
           if (positive && o->c.fix_positivity) fix_positivity(im, fp, fl, fr);
           for (i = 0; i < im; i++) { QS(0, 0, i, j, k, p) = fl[i]; QS(0, 1, i, j, k, p) = fr[i]; }
         }
         for (i = o->ibl; i <= o->ieu; i++) {
           for (j = 0; j < jm; j++) fp[j] = q[QI(o, p, i, j, k)];
-          weno5z(o, jm, fp, fl, fr, w1, w2, w3);
+          recon_dir(o, jm, fp, fl, fr, w1, w2, w3);
           if (positive && o->c.fix_positivity) fix_positivity(jm, fp, fl, fr);
           for (j = 0; j < jm; j++) { QS(1, 0, i, j, k, p) = fl[j]; QS(1, 1, i, j, k, p) = fr[j]; }
         }
@@ -320,7 +510,7 @@ static void reconstruct_interfaces(ocfg* o, const double* q, double* qs)
         for (j = o->jbl; j <= o->jeu; j++)
           for (i = o->ibl; i <= o->ieu; i++) {
             for (k = 0; k < km; k++) fp[k] = q[QI(o, p, i, j, k)];
-            weno5z(o, km, fp, fl, fr, w1, w2, w3);
+            recon_dir(o, km, fp, fl, fr, w1, w2, w3);
             if (positive && o->c.fix_positivity) fix_positivity(km, fp, fl, fr);
             for (k = 0; k < km; k++) { QS(2, 0, i, j, k, p) = fl[k]; QS(2, 1, i, j, k, p) = fr[k]; }
           }
@@ -1595,7 +1785,8 @@ int orc_create(const amun_cfg* c, void** handle)
   if (!g) return 500;
   r = oc_setup(&g->o, c);
   if (!r) r = riemann_check(c);
-  if (!r && c->reconstruction != AMUN_REC_TVD && c->reconstruction != AMUN_REC_WENO5Z) r = 200;
+  if (!r && c->reconstruction != AMUN_REC_TVD && c->reconstruction != AMUN_REC_WENO5Z &&
+      c->reconstruction != AMUN_REC_GP) r = 200;
   if (!r && c->time_advance != AMUN_TA_EULER && c->time_advance != AMUN_TA_RK2 &&
       c->time_advance != AMUN_TA_RK3) r = 400;
   if (r) { free(g); return r; }
@@ -1691,6 +1882,19 @@ void orc_c2p_events(void* h, long* n)
   n[2] = g->o.c2p_noconv;
 }
 
+/* unit level: the Gaussian-process weights cgp(1..ngp) of prepare_gp */
+int orc_gp_weights(const amun_cfg* c, double* cgp)
+{
+  ocfg o;
+  amun_cfg cc = *c;
+  int i, r;
+  cc.reconstruction = AMUN_REC_GP;
+  r = oc_setup(&o, &cc);
+  if (r) return r;
+  for (i = 0; i < cc.ngp; i++) cgp[i] = o.cgp[i];
+  return 0;
+}
+
 /* unit level: the root finders of algebra.F90 on count coefficient sets
  * a(5) (increasing powers).  Per set, out receives 19 values:
  * [nr, x1, x2] quadratic(a(1:3)), [nr, x1, x2] quadratic_normalized(a(1:2)),
@@ -1752,7 +1956,7 @@ int orc_update_flux(const amun_cfg* c, const double* q, const double* dx,
   return 0;
 }
 
-/* unit level: reconstruct() on npencil pencils of length n (WENO5-Z) */
+/* unit level: reconstruct() on npencil pencils of length n (WENO5-Z, GP) */
 int orc_reconstruct(const amun_cfg* c, int n, int npencil, const double* fp,
                     double* fl, double* fr)
 {
@@ -1760,9 +1964,10 @@ int orc_reconstruct(const amun_cfg* c, int n, int npencil, const double* fp,
   double* w;
   int p, r = oc_setup(&o, c);
   if (r) return r;
-  w = (double*)malloc(3 * (size_t)n * sizeof(double));
+  w = (double*)malloc(3 * (size_t)(n + 4) * sizeof(double));
   for (p = 0; p < npencil; p++)
-    weno5z(&o, n, fp + (size_t)p * n, fl + (size_t)p * n, fr + (size_t)p * n, w, w + n, w + 2 * n);
+    recon_dir(&o, n, fp + (size_t)p * n, fl + (size_t)p * n, fr + (size_t)p * n, w, w + n + 4,
+              w + 2 * (n + 4));
   free(w);
   return 0;
 }

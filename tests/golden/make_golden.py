@@ -5,6 +5,7 @@ oracle/build_ref.sh).  Run in the container that holds the reference:
 
     python tests/golden/make_golden.py        # every fixture
     python tests/golden/make_golden.py sr     # the relativistic ones only
+    python tests/golden/make_golden.py gp     # the Gaussian-process ones only
 
 Every fixture stores the configuration (as AMUN parameter text), the seed,
 the reference outputs needed by the tests (sha256 of the exact bytes, the
@@ -154,11 +155,46 @@ def main_sr():
     print(name, p4.shape, "dt", tr[-1, 0])
 
 
+def main_gp():
+    """Fixtures of the Gaussian-process reconstruction (own generator)."""
+    rng = np.random.default_rng(20261204)
+    pg = Params(ndims=3, equation_system="mhd", reconstruction="gp", limiter="mm",
+                ncells=16, nghosts=4, cfl=0.3)
+    fp = np.concatenate([rng.normal(size=(8, 24)),
+                         np.cumsum(rng.normal(size=(4, 24)), axis=1),
+                         np.sin(np.linspace(0, 6, 24))[None, :] * np.ones((2, 1)),
+                         np.where(np.arange(24) < 12, 1.0, 0.1)[None, :] * np.ones((2, 1)),
+                         np.linspace(0, 3, 24)[None, :] ** 2])
+    fl, fr = ol.ref_reconstruct(pg, fp)
+    np.savez_compressed(os.path.join(HERE, "unit_gp.npz"), fp=fp, fl=fl, fr=fr,
+                        params=np.array(pg.to_params_in()))
+    p3 = c2_params(32)
+    p3.reconstruction = "gp"
+    p3.xblocks, p3.yblocks, p3.zblocks, p3.ncells = 3, 2, 2, 8
+    u0 = problems.turbulent_mhd_3d(p3, seed=12)
+    nsteps, stride = 3, 2
+    u, tr = ol.ref_run(p3, u0, nsteps)
+    name = "gp_mhd3d_hlld_3"
+    np.savez_compressed(os.path.join(HERE, name + ".npz"),
+                        params=np.array(p3.to_params_in()), ndims=3, nsteps=nsteps,
+                        stride=stride, trace=tr, sub=u[:, ::stride, ::stride, ::stride],
+                        sha_u0=sha(u0), sha_u=sha(u))
+    with open(os.path.join(HERE, "index.json")) as fh:
+        index = json.load(fh)
+    index[name] = {"steps": nsteps, "sha_u": sha(u), "dt_last": float(tr[-1, 0])}
+    with open(os.path.join(HERE, "index.json"), "w") as fh:
+        json.dump(index, fh, indent=1)
+    print(name, p3.shape, "dt", tr[-1, 0])
+
+
 def main():
     if not (ol.ref_available(2) and ol.ref_available(3)):
         sys.exit("reference binaries missing: run oracle/build_ref.sh first")
     if sys.argv[1:] == ["sr"]:
         main_sr()
+        return
+    if sys.argv[1:] == ["gp"]:
+        main_gp()
         return
     index = {}
     for name, p, builder, nsteps, stride in run_cases():
@@ -223,6 +259,7 @@ def main():
     with open(os.path.join(HERE, "index.json"), "w") as fh:
         json.dump(index, fh, indent=1)
     main_sr()
+    main_gp()
 
 
 if __name__ == "__main__":

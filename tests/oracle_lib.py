@@ -45,6 +45,7 @@ def lib():
         L.orc_riemann.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, _P, _P, _P]
         L.orc_c2p_events.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_long)]
         L.orc_algebra.argtypes = [ctypes.c_int, _P, _P]
+        L.orc_gp_weights.argtypes = [ctypes.c_void_p, _P]
         L.orc_cons2prim.argtypes = [ctypes.c_void_p, ctypes.c_int, _P, _P,
                                     ctypes.POINTER(ctypes.c_long)]
         _lib = L
@@ -142,6 +143,14 @@ def oracle_algebra(a: np.ndarray) -> np.ndarray:
     out = np.zeros((a.shape[0], 19))
     lib().orc_algebra(a.shape[0], _dp(a), _dp(out))
     return out
+
+
+def oracle_gp_weights(p: Params) -> np.ndarray:
+    w = np.zeros(p.ngp)
+    cfg = p.to_cfg()
+    r = lib().orc_gp_weights(ctypes.byref(cfg), _dp(w))
+    assert r == 0, r
+    return w
 
 
 def oracle_cons2prim(p: Params, u: np.ndarray, q0: np.ndarray):

@@ -439,3 +439,42 @@ def test_c4_1024_400_steps_mixed_vs_fp64():
           f"recovery events fp64 {ev64}, mixed {evmx}")
     assert ev64 == [0, 0, 0]
     assert l1 <= 1e-5
+
+
+# ------------------------------------------------------- Gaussian process
+@pytest.mark.parametrize("nd", [2, 3])
+@pytest.mark.parametrize("eq,rs", [("hd", "hllc"), ("mhd", "hlld"), ("mhd", "hll")])
+@pytest.mark.parametrize("ta", ["rk2", "rk3"])
+def test_gp_one_step(nd, eq, rs, ta):
+    p = Params(ndims=nd, equation_system=eq, riemann_solver=rs, reconstruction="gp",
+               limiter="mm", time_advance=ta, ncells=8, nghosts=3, xblocks=2, yblocks=3,
+               zblocks=2, cfl=0.3, gamma=1.4 if eq == "hd" else 5.0 / 3.0, xmin=-0.5,
+               xmax=0.5, ymin=-0.75, ymax=0.75, zmin=0.0, zmax=1.0)
+    u0 = initial_state(p) if eq == "hd" else _mhd_state(p)
+    dt, cmax, ug, tr, uo = one_step(p, u0)
+    assert abs(dt - tr[0]) <= 1e-13 * tr[0]
+    assert relerr(ug, uo).max() <= 1e-12
+
+
+def test_gp_256_mixed_vs_fp64():
+    """The 256^3 MHD case with GP reconstruction, 10 steps: mixed against FP64
+    (density L1), and the FP64 GP run against a 32^3 oracle run first."""
+    p = c2_params(32)
+    p.reconstruction = "gp"
+    u0 = initial_state(p)
+    dt, cmax, ug, tr, uo = one_step(p, u0)
+    assert relerr(ug, uo).max() <= 1e-12
+    p64 = c2_params(256)
+    p64.reconstruction = "gp"
+    pmx = c2_params(256, precision="mixed")
+    pmx.reconstruction = "gp"
+    u0 = initial_state(p64)
+    res = []
+    for q in (p64, pmx):
+        with Solver(q) as s:
+            s.upload(u0)
+            s.run(10)
+            res.append(s.download())
+    l1 = _l1(res[1][0], res[0][0])
+    print(f"GP 256^3 x 10 mixed vs fp64: density L1 {l1:.3e}")
+    assert l1 <= 1e-6

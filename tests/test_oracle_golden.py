@@ -140,3 +140,26 @@ def test_srmhd_hll_via_update_flux():
     for b in range(g["q"].shape[0]):
         f = ol.oracle_update_flux(p, g["q"][b], float(g["cmax"]))
         np.testing.assert_array_equal(f, g["f"][b])
+
+
+# ------------------------------------------------------- Gaussian process
+def test_gp_pencils():
+    """reconstruct_gp with its quad-precision weights and mp_limiting."""
+    g = load("unit_gp")
+    p = Params.from_text(str(g["params"]), ndims=3)
+    fl, fr = ol.oracle_reconstruct(p, g["fp"])
+    np.testing.assert_array_equal(fl, g["fl"])
+    np.testing.assert_array_equal(fr, g["fr"])
+
+
+def test_gp_run_bit_exact():
+    g = load("gp_mhd3d_hlld_3")
+    p = params_of(g)
+    assert p.reconstruction == "gp"
+    u0 = problems.turbulent_mhd_3d(p, seed=12)
+    assert sha(u0) == str(g["sha_u0"])
+    o = ol.OracleRun(p)
+    o.set_state(u0)
+    tr = o.run(int(g["nsteps"]))
+    np.testing.assert_array_equal(tr, g["trace"])
+    assert sha(o.get_state()) == str(g["sha_u"])

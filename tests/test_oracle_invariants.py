@@ -392,3 +392,28 @@ def test_srmhd_discrete_conservation():
         s0, s1 = u0[v].sum(), u[v].sum()
         assert abs(s1 - s0) <= 1e-12 * np.abs(u0[v]).sum() + 1e-14, v
     assert o.c2p_events() == [0, 0, 0]
+
+
+# ------------------------------------------------------- Gaussian process
+@pytest.mark.parametrize("sgp,ngp", [(9.0, 5), (3.0, 5), (2.0, 3), (6.0, 7)])
+def test_gp_weights_solve_the_covariance_system(sgp, ngp):
+    """cgp = x^T A^-1 with A_ij = sqrt(pi)/2 s (erf((i-j+1/2)/s) - erf((i-j-1/2)/s)),
+    s = sqrt(2) sgp, x_j = exp(-((1/2 - j)/s)^2): the weights solve A^T c = x,
+    reproduce constants, and for a wide kernel approach the fifth-order
+    polynomial face stencil (2, -13, 47, 27, -3)/60."""
+    from scipy.special import erf
+    p = Params(ndims=2, equation_system="hd", reconstruction="gp", sgp=sgp, ngp=ngp)
+    c = ol.oracle_gp_weights(p)
+    m = (ngp - 1) // 2
+    s = np.sqrt(2.0) * sgp
+    k = np.arange(-m, m + 1)
+    d = k[:, None] - k[None, :]
+    A = 0.5 * np.sqrt(np.pi) * s * (erf((d + 0.5) / s) - erf((d - 0.5) / s))
+    x = np.exp(-((0.5 - k) / s) ** 2)
+    np.testing.assert_allclose(A.T @ c, x, rtol=1e-10, atol=1e-12)
+    if ngp == 5 and sgp == 9.0:
+        poly = np.array([2, -13, 47, 27, -3]) / 60.0
+        errs = [np.abs(ol.oracle_gp_weights(Params(ndims=2, equation_system="hd",
+                                                   reconstruction="gp", sgp=w)) - poly).max()
+                for w in (9.0, 30.0, 100.0)]
+        assert errs[0] > errs[1] > errs[2] and errs[2] < 1e-4

@@ -85,3 +85,24 @@ def test_srmhd_eqs_and_roots_fresh_inputs():
         np.testing.assert_array_equal(a, b)
     co = rng.normal(size=(500, 5)) * 10.0 ** rng.integers(-4, 5, (500, 5))
     np.testing.assert_array_equal(ol.oracle_algebra(co), ol.ref_algebra(co))
+
+
+@needs_ref
+@pytest.mark.parametrize("sgp,kappa", [(9.0, 1.0), (3.0, 0.7), (1.5, 2.0)])
+def test_gp_fresh_inputs(sgp, kappa):
+    """GP weights for other kernel widths and MP limiter constants, on fresh
+    pencils and a short 2D run."""
+    rng = np.random.default_rng(int(sgp * 10))
+    p = Params(ndims=2, equation_system="mhd", reconstruction="gp", limiter="mm",
+               riemann_solver="hlld", time_advance="rk3", ncells=8, nghosts=3, xblocks=3,
+               yblocks=2, cfl=0.3, sgp=sgp, kappa=kappa, kbeta=0.8)
+    fp = np.cumsum(rng.normal(size=(20, 14)), axis=1)
+    a, b = ol.oracle_reconstruct(p, fp), ol.ref_reconstruct(p, fp)
+    np.testing.assert_array_equal(a[0], b[0])
+    np.testing.assert_array_equal(a[1], b[1])
+    u0 = problems.orszag_tang_2d(p, seed=4)
+    ur, tr = ol.ref_run(p, u0, 3)
+    o = ol.OracleRun(p)
+    o.set_state(u0)
+    np.testing.assert_array_equal(o.run(3), tr)
+    np.testing.assert_array_equal(o.get_state(), ur)
