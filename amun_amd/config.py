@@ -24,7 +24,8 @@ LIMITER = {"zero": 0, "mm": 1, "minmod": 1, "mc": 2, "monotonized_central": 2,
            "sb": 3, "superbee": 3, "vl": 4, "vanleer": 4, "va": 5,
            "vanalbada": 5}
 RIEMANN = {"hll": 1, "hllc": 2, "hlld": 3}
-INTEGRATOR = {"euler": 0, "rk2": 1, "rk3": 2}
+INTEGRATOR = {"euler": 0, "rk2": 1, "rk3": 2, "ssprk(m,2)": 3, "rk3.4": 4, "ssprk(4,3)": 4,
+              "rk3.5": 5, "ssprk(5,3)": 5}
 PRECISION = {"fp64": 0, "mixed": 1}
 
 
@@ -48,7 +49,7 @@ class AmunCfg(ctypes.Structure):
         ("tolerance", ctypes.c_double), ("pmin", ctypes.c_double),
         ("nr_maxit", ctypes.c_int32), ("nr_extra", ctypes.c_int32),
         ("sgp", ctypes.c_double), ("kappa", ctypes.c_double), ("kbeta", ctypes.c_double),
-        ("ngp", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("ngp", ctypes.c_int32), ("stages", ctypes.c_int32),
     ]
 
 
@@ -104,6 +105,7 @@ class Params:
     kappa: float = 1.0
     kbeta: float = 1.0
     ngp: int = 5
+    stages: int = 2
     extra: dict = field(default_factory=dict)
 
     # ---------------------------------------------------------------- derived
@@ -157,6 +159,7 @@ class Params:
         c.tolerance, c.pmin = self.tolerance, self.pmin
         c.nr_maxit, c.nr_extra = int(self.nr_maxit), int(self.nr_extra)
         c.sgp, c.kappa, c.kbeta, c.ngp = self.sgp, self.kappa, self.kbeta, int(self.ngp)
+        c.stages = int(self.stages)
         return c
 
     def dx(self):
@@ -187,7 +190,7 @@ class Params:
             f"tolerance = {self.tolerance!r}", f"pmin = {self.pmin!r}",
             f"nr_maxit = {self.nr_maxit}", f"nr_extra = {self.nr_extra}",
             f"sgp = {self.sgp!r}", f"kappa = {self.kappa!r}", f"kbeta = {self.kbeta!r}",
-            f"ngp = {self.ngp}",
+            f"ngp = {self.ngp}", f"stages = {self.stages}",
             "minlev = 1", "maxlev = 1",
             "xlbndry = periodic", "xubndry = periodic",
             "ylbndry = periodic", "yubndry = periodic",

@@ -86,7 +86,7 @@ int validate(const amun_cfg& c, int* nv) {
                  (c.riemann_solver == AMUN_RS_HLLD && c.eqsys == AMUN_EQSYS_MHD);
     if (!rs_ok) return fail(300, "riemann_solver does not match the equation system");
   }
-  if (c.time_advance < AMUN_TA_EULER || c.time_advance > AMUN_TA_RK3)
+  if (c.time_advance < AMUN_TA_EULER || c.time_advance > AMUN_TA_SSPRK53)
     return fail(400, "unsupported time_advance");
   if (c.precision != AMUN_PREC_FP64 && c.precision != AMUN_PREC_MIXED)
     return fail(500, "unsupported precision");
@@ -157,6 +157,8 @@ void gp_weights(double sgp, double* cgp) {
   }
 }
 
+int ssprk_stages(const amun_cfg& cfg) { return std::max(2, std::min(9, (int)cfg.stages)); }
+
 // run constants of a configuration (block counts are filled in by the caller)
 RunConst make_rc(const amun_cfg& cfg) {
   RunConst rc;
@@ -170,9 +172,16 @@ RunConst make_rc(const amun_cfg& cfg) {
   rc.gamma = cfg.gamma;
   rc.gammam1 = cfg.gamma - 1.0;
   rc.gammam1i = 1.0 / rc.gammam1;
-  rc.cfl = cfg.cfl;
+  // initialize_evolution (evolution.F90:166-200): SSPRK(m,2) with m clamped
+  // to [2, 9], SSPRK(4,3) and SSPRK(5,3) step at a multiple of the CFL
+  // number, and the psi decay follows the scaled number
+  double cfl = cfg.cfl;
+  if (cfg.time_advance == AMUN_TA_SSPRK_M2) cfl = (ssprk_stages(cfg) - 1) * cfg.cfl;
+  if (cfg.time_advance == AMUN_TA_SSPRK43) cfl = 2.0 * cfg.cfl;
+  if (cfg.time_advance == AMUN_TA_SSPRK53) cfl = 2.65062919143939 * cfg.cfl;
+  rc.cfl = cfl;
   rc.eps = cfg.eps;
-  rc.decay = std::exp(-cfg.alpha * cfg.cfl);
+  rc.decay = std::exp(-cfg.alpha * cfl);
   rc.sc.gamma = cfg.gamma;
   rc.sc.gammaxi = rc.gammam1 / cfg.gamma;       // equations.F90:843
   rc.sc.rgammaxi = 1.0 / rc.sc.gammaxi;
@@ -232,6 +241,11 @@ void amun_dev_default_cfg(amun_cfg* c) {
   c->pmin = 1.0e-16;
   c->nr_maxit = 100;
   c->nr_extra = 2;
+  c->sgp = 9.0;
+  c->kappa = 1.0;
+  c->kbeta = 1.0;
+  c->ngp = 5;
+  c->stages = 2;
 }
 
 int amun_dev_init_batch(const amun_cfg* cfg, int32_t nruns, const amun_grid* grids,
@@ -580,7 +594,53 @@ int amun_dev_download_blocks(amun_ctx* c, double* u, double* q) {
 }
 
 static int stages_per_step(const amun_ctx* c) {
-  return c->cfg.time_advance == AMUN_TA_EULER ? 1 : (c->cfg.time_advance == AMUN_TA_RK2 ? 2 : 3);
+  switch (c->cfg.time_advance) {
+  case AMUN_TA_EULER: return 1;
+  case AMUN_TA_RK2: return 2;
+  case AMUN_TA_SSPRK_M2: return ssprk_stages(c->cfg);
+  case AMUN_TA_SSPRK43: return 4;
+  case AMUN_TA_SSPRK53: return 5;
+  default: return 3;
+  }
+}
+
+// The stages of one step: combine kind (see combine() in kernels.h), ds =
+// fac * dt, the state weights ca, cb of kinds 5-7 and whether the stage
+// writes U0 (the step state) or U1.
+struct StageDef { int kind; double fac, ca, cb; bool to_u0; };
+
+static std::vector<StageDef> stage_table(const amun_ctx* c) {
+  switch (c->cfg.time_advance) {
+  case AMUN_TA_EULER:                               // evolve_euler
+    return {{0, 1.0, 0, 0, true}};
+  case AMUN_TA_RK2:                                 // evolve_rk2
+    return {{0, 1.0, 0, 0, false}, {3, 1.0, 0, 0, true}};
+  case AMUN_TA_SSPRK_M2: {                          // evolve_ssprk2
+    // U1 = U0 + ds L(U0) is the reference's copy U1 = U0 then U1 + ds L(U1)
+    const int m = ssprk_stages(c->cfg);
+    const double ft = 1.0 / (m - 1), fl = 1.0 / m, fr = 1.0 - fl;
+    std::vector<StageDef> t = {{0, ft, 0, 0, false}};
+    for (int n = 2; n < m; n++) t.push_back({4, ft, 0, 0, false});
+    t.push_back({5, ft, fl, fr, true});
+    return t;
+  }
+  case AMUN_TA_SSPRK43: {                           // evolve_ssprk34
+    const double b1 = 1.0 / 2.0, b3 = 1.0 / 6.0, a31 = 2.0 / 3.0, a33 = 1.0 / 3.0;
+    return {{0, b1, 0, 0, false}, {4, b1, 0, 0, false}, {6, b3, a31, a33, false},
+            {4, b1, 0, 0, true}};
+  }
+  case AMUN_TA_SSPRK53: {                           // evolve_ssprk35
+    const double b1 = 3.77268915331368e-01, b3 = 2.42995220537396e-01;
+    const double b4 = 2.38458932846290e-01, b5 = 2.87632146308408e-01;
+    const double a31 = 3.55909775063327e-01, a33 = 6.44090224936673e-01;
+    const double a41 = 3.67933791638137e-01, a44 = 6.32066208361863e-01;
+    const double a53 = 2.37593836598569e-01, a55 = 7.62406163401431e-01;
+    return {{0, b1, 0, 0, false}, {4, b1, 0, 0, false}, {6, b3, a31, a33, false},
+            {6, b4, a41, a44, true}, {7, b5, a53, a55, true}};
+  }
+  default:                                          // evolve_rk3
+    return {{0, 1.0, 0, 0, false}, {1, 1.0 / 4.0, 0, 0, false}, {2, 2.0 / 3.0, 0, 0, true}};
+  }
 }
 
 static int kernels_per_step(const amun_ctx* c) {
@@ -597,18 +657,7 @@ static int enqueue_step(amun_ctx* c, double dtnext, int slot, int qcur, hipEvent
   const int npen = (rc.nd == 3) ? rc.nc * rc.nc : rc.nc;
   const long long nthreads = (long long)rc.nblk * npen;
 
-  struct Stage { int kind; double fac; double* out; };
-  std::vector<Stage> stages;
-  switch (c->cfg.time_advance) {
-  case AMUN_TA_EULER:
-    stages = {{0, 1.0, c->U0}};
-    break;
-  case AMUN_TA_RK2:
-    stages = {{0, 1.0, c->U1}, {3, 1.0, c->U0}};
-    break;
-  default:
-    stages = {{0, 1.0, c->U1}, {1, 1.0 / 4.0, c->U1}, {2, 2.0 / 3.0, c->U0}};
-  }
+  const std::vector<StageDef> stages = stage_table(c);
   if (rec)   // stage inputs: U^n, then every intermediate stage output
     HIP_TRY(hipMemcpyAsync(rec, c->U0, c->nstate * sizeof(double), hipMemcpyDeviceToDevice,
                            c->stream));
@@ -617,7 +666,7 @@ static int enqueue_step(amun_ctx* c, double dtnext, int slot, int qcur, hipEvent
     a.Q = c->Q[qcur];
     a.U0 = c->U0;
     a.U1 = c->U1;
-    a.Uout = stages[s].out;
+    a.Uout = stages[s].to_u0 ? c->U0 : c->U1;
     a.Qout = c->Q[qcur ^ 1];
     a.D = c->D;
     a.C = c->C;
@@ -628,7 +677,10 @@ static int enqueue_step(amun_ctx* c, double dtnext, int slot, int qcur, hipEvent
     a.slot = slot;
     a.kind = stages[s].kind;
     a.last = (s + 1 == stages.size());
+    a.comp = stages[s].to_u0;
     a.fac = stages[s].fac;
+    a.ca = stages[s].ca;
+    a.cb = stages[s].cb;
     a.dtnext = dtnext;
     for (int dir = 0; dir < rc.nd; dir++) {
       const int wg = c->wg[dir];
@@ -640,8 +692,8 @@ static int enqueue_step(amun_ctx* c, double dtnext, int slot, int qcur, hipEvent
     }
     qcur ^= 1;
     if (rec && s + 1 < stages.size())
-      HIP_TRY(hipMemcpyAsync(rec + (s + 1) * c->nstate, c->U1, c->nstate * sizeof(double),
-                             hipMemcpyDeviceToDevice, c->stream));
+      HIP_TRY(hipMemcpyAsync(rec + (s + 1) * c->nstate, stages[s].to_u0 ? c->U0 : c->U1,
+                             c->nstate * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
   }
   hipLaunchKernelGGL(k_end_step, dim3((unsigned)((rc.nruns + 255) / 256)), dim3(256), 0,
                      c->stream, c->rs, c->trace, c->tsteps, rc, slot, dtnext);
@@ -1081,6 +1133,7 @@ struct GradBufs {
   std::vector<double*> all;
   double *seg = nullptr, *Q = nullptr, *MU = nullptr, *LT = nullptr, *GC = nullptr;
   double *LY = nullptr, *LU0 = nullptr, *LX = nullptr, *GP = nullptr;
+  double *LS = nullptr;          // tangent / adjoint of a later stage's X_src
   double *AL[3] = {nullptr, nullptr, nullptr}, *AR[3] = {nullptr, nullptr, nullptr};
   double *GF[3] = {nullptr, nullptr, nullptr};
   ~GradBufs() { for (double* p : all) (void)hipFree(p); }
@@ -1091,14 +1144,34 @@ struct GradBufs {
   }
 };
 
-struct StageCoef { double a, b, c; };
+// Linear form of stage k: Y = a X_src + b X_k + c dt L(X_k), with X_0 = U^n
+// the step state and X_{k+1} = Y_k; src is 0 except for the last stage of
+// SSPRK(5,3), whose combine takes the output of its third stage (X_3).
+struct StageCoef { double a, b, c; int src; };
+
+// the stage whose input a later stage combines with (0 if none)
+static int stage_src(const std::vector<StageCoef>& co) {
+  int src = 0;
+  for (const auto& x : co) src = std::max(src, x.src);
+  return src;
+}
 
 std::vector<StageCoef> stage_coefs(const amun_ctx* c) {
-  switch (c->cfg.time_advance) {
-  case AMUN_TA_EULER: return {{0.0, 1.0, 1.0}};
-  case AMUN_TA_RK2: return {{0.0, 1.0, 1.0}, {0.5, 0.5, 0.5}};
-  default: return {{0.0, 1.0, 1.0}, {0.75, 0.25, 0.25}, {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0}};
+  const auto t = stage_table(c);
+  std::vector<StageCoef> co;
+  for (size_t k = 0; k < t.size(); k++) {
+    const StageDef& s = t[k];
+    switch (s.kind) {
+    case 0: case 4: co.push_back({0.0, 1.0, s.fac, 0}); break;
+    case 1: co.push_back({0.75, 0.25, s.fac, 0}); break;
+    case 2: co.push_back({1.0 / 3.0, 2.0 / 3.0, s.fac, 0}); break;
+    case 3: co.push_back({0.5, 0.5, 0.5 * s.fac, 0}); break;
+    case 5: co.push_back({s.ca, s.cb, s.cb * s.fac, 0}); break;
+    case 6: co.push_back({s.ca, s.cb, s.fac, 0}); break;
+    default: co.push_back({s.ca, s.cb, s.fac, (int)k - 1}); break;
+    }
   }
+  return co;
 }
 
 // recompute steps [s0, s1) from the checkpoint of s0, storing the stage
@@ -1188,7 +1261,9 @@ int amun_dev_tape_tangent(amun_ctx* c, const double* du0, double dgamma, double*
   HIP_TRY(B.get(&B.LX, n));      // tangent of the stage input
   HIP_TRY(B.get(&B.LU0, n));     // tangent of U^n
   HIP_TRY(B.get(&B.LT, n));      // tangent of L
+  HIP_TRY(B.get(&B.LS, n));
   for (int k = 0; k < nd; k++) HIP_TRY(B.get(&B.AL[k], n));   // face flux tangents
+  const int ksrc = stage_src(co);
   GradKeep keep;
   if ((r = grad_state(c, keep, false))) return r;
   const long long nn = (long long)n;
@@ -1205,6 +1280,8 @@ int amun_dev_tape_tangent(amun_ctx* c, const double* du0, double dgamma, double*
       hipLaunchKernelGGL(k_axpby, dim3(2048), bl, 0, c->stream, nn, 1.0, B.LU0, 0.0, B.LX);
       for (int k = 0; k < ns; k++) {
         const double* X = X0 + (size_t)k * n;
+        if (ksrc > 0 && k == ksrc)
+          hipLaunchKernelGGL(k_axpby, dim3(2048), bl, 0, c->stream, nn, 1.0, B.LX, 0.0, B.LS);
         hipLaunchKernelGGL(d.adj_prim, gc, bl, 0, c->stream, X, B.Q, rc);
         for (int dd = 0; dd < nd; dd++)
           hipLaunchKernelGGL(d.tlm_face[dd], gc, bl, 0, c->stream, X, B.Q, B.LX, dgamma,
@@ -1212,10 +1289,11 @@ int amun_dev_tape_tangent(amun_ctx* c, const double* du0, double dgamma, double*
         hipLaunchKernelGGL(d.tlm_cell, gc, bl, 0, c->stream, B.AL[0], B.AL[1],
                            nd == 3 ? B.AL[2] : B.AL[1], B.LT, c->nbr, rc, h.dxi[0], h.dxi[1],
                            h.dxi[2]);
-        // dY = a dU^n + b dX + c dt dL
+        // dY = a dX_src + b dX + c dt dL
         hipLaunchKernelGGL(k_axpby, dim3(2048), bl, 0, c->stream, nn, co[k].b, B.LX, 0.0, B.LX);
         if (co[k].a != 0.0)
-          hipLaunchKernelGGL(k_axpby, dim3(2048), bl, 0, c->stream, nn, co[k].a, B.LU0, 1.0, B.LX);
+          hipLaunchKernelGGL(k_axpby, dim3(2048), bl, 0, c->stream, nn, co[k].a,
+                             co[k].src ? B.LS : B.LU0, 1.0, B.LX);
         hipLaunchKernelGGL(k_axpby, dim3(2048), bl, 0, c->stream, nn, co[k].c * T.dt[s], B.LT,
                            1.0, B.LX);
       }
@@ -1249,7 +1327,9 @@ int amun_dev_tape_backward(amun_ctx* c, const double* lam, double* lam0, double*
   HIP_TRY(B.get(&B.LT, n));
   HIP_TRY(B.get(&B.LY, n));
   HIP_TRY(B.get(&B.LU0, n));
+  HIP_TRY(B.get(&B.LS, n));
   HIP_TRY(B.get(&B.GC, ncell));
+  const int ksrc = stage_src(co);
   HIP_TRY(B.get(&B.GP, npart));
   for (int k = 0; k < nd; k++) {
     HIP_TRY(B.get(&B.AL[k], n));
@@ -1275,6 +1355,8 @@ int amun_dev_tape_backward(amun_ctx* c, const double* lam, double* lam0, double*
       if (c->cfg.eqsys == AMUN_EQSYS_MHD)
         hipLaunchKernelGGL(k_scale_var, dim3(2048), bl, 0, c->stream, B.LY, c->nv, 7, rc.decay, rc);
       hipLaunchKernelGGL(k_axpby, dim3(2048), bl, 0, c->stream, nn, 0.0, B.LY, 0.0, B.LU0);
+      if (ksrc > 0)
+        hipLaunchKernelGGL(k_axpby, dim3(2048), bl, 0, c->stream, nn, 0.0, B.LY, 0.0, B.LS);
       for (int k = ns - 1; k >= 0; k--) {
         const double* X = X0 + (size_t)k * n;
         // MU = c dt lambda_Y, LT = (dL/dX)^T MU
@@ -1289,11 +1371,15 @@ int amun_dev_tape_backward(amun_ctx* c, const double* lam, double* lam0, double*
                            B.AR[1], B.AL[z], B.AR[z], B.GF[0], B.GF[1], B.GF[z], B.LT, B.GC,
                            c->nbr, rc);
         hipLaunchKernelGGL(k_sum_acc, dim3(npart), bl, 0, c->stream, B.GC, (long long)ncell, B.GP);
-        // lambda of U^n gets a lambda_Y; lambda of X = b lambda_Y + LT
+        // lambda of X_src gets a lambda_Y; lambda of X = b lambda_Y + LT
+        // (+ what a later stage passed to it as its X_src)
         if (co[k].a != 0.0)
-          hipLaunchKernelGGL(k_axpby, dim3(2048), bl, 0, c->stream, nn, co[k].a, B.LY, 1.0, B.LU0);
+          hipLaunchKernelGGL(k_axpby, dim3(2048), bl, 0, c->stream, nn, co[k].a, B.LY, 1.0,
+                             co[k].src ? B.LS : B.LU0);
         hipLaunchKernelGGL(k_axpby, dim3(2048), bl, 0, c->stream, nn, co[k].b, B.LY, 0.0, B.MU);
         hipLaunchKernelGGL(k_axpby, dim3(2048), bl, 0, c->stream, nn, 1.0, B.LT, 1.0, B.MU);
+        if (ksrc > 0 && k == ksrc)
+          hipLaunchKernelGGL(k_axpby, dim3(2048), bl, 0, c->stream, nn, 1.0, B.LS, 1.0, B.MU);
         if (k == 0)
           hipLaunchKernelGGL(k_axpby, dim3(2048), bl, 0, c->stream, nn, 1.0, B.MU, 1.0, B.LU0);
         else

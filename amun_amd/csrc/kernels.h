@@ -208,20 +208,44 @@ struct SweepArgs {
   int swz;                         // XCD-aware workgroup order (autotuned)
   int kind;                        // stage combine, see combine()
   int last;                        // last stage of the step
+  int comp;                        // the stage writes the step state U0
   int first;                       // first sweep of the step
   double fac;                      // ds = fac * dt
+  double ca, cb;                   // state weights of combine kinds 5-7
   double dtnext;
 };
 
 // stage combines of evolve_euler/rk2/rk3 (evolution.F90:570, 700, 740, 1080,
-// 1130, 1177)
+// 1130, 1177) and of the SSPRK schemes (evolution.F90:917, 956, 1303, 1344,
+// 1395, 1442, 1667, 1714, 1766)
 __device__ __forceinline__ double combine(int kind, double u0, double u1, double ds,
-                                          double du) {
+                                          double du, double ca, double cb) {
   switch (kind) {
   case 0: return u0 + ds * du;
   case 1: return ((3.0 / 4.0) * u0 + (1.0 / 4.0) * u1) + ds * du;
   case 2: return ((1.0 / 3.0) * u0 + (2.0 / 3.0) * u1) + ds * du;
-  default: return 0.5 * ((u0 + u1) + ds * du);
+  case 3: return 0.5 * ((u0 + u1) + ds * du);
+  case 4: return u1 + ds * du;
+  case 5: return ca * u0 + cb * (u1 + ds * du);
+  case 6: return (ca * u0 + cb * u1) + ds * du;
+  default: return (ca * u1 + cb * u0) + ds * du;
+  }
+}
+
+// the same combines as increments of the compensated step state u0 - cp
+// (every scheme's state weights sum to one)
+__device__ __forceinline__ double combine_delta(int kind, double u0, double u1, double cp,
+                                                double dd, double ca, double cb) {
+  const double w = (u1 - u0) + cp;
+  switch (kind) {
+  case 0: return dd;
+  case 1: return 0.25 * w + dd;
+  case 2: return (2.0 / 3.0) * w + dd;
+  case 3: return 0.5 * (w + dd);
+  case 4: return w + dd;
+  case 5: return cb * (w + dd);
+  case 6: return cb * w + dd;
+  default: return ca * w + dd;
   }
 }
 
@@ -507,9 +531,9 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a, RunConst rc) {
           if constexpr (kLast) {
             u0pre[v] = a.U0[bd + off];
             if (a.kind != 0) u1pre[v] = a.U1[bd + off];
-            // the compensation enters the step update; the transient
-            // stages ignore it (it is below the FP64 rounding of U0)
-            if constexpr (sizeof(T) == 4) cpre[v] = a.last ? a.C[bf + off] : 0.0f;
+            // the compensation enters the updates of the step state; the
+            // transient stages ignore it (it is below the FP64 rounding of U0)
+            if constexpr (sizeof(T) == 4) cpre[v] = a.comp ? a.C[bf + off] : 0.0f;
           }
         }
       }
@@ -554,30 +578,24 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a, RunConst rc) {
               if constexpr (kDout) sd[(((c - 1) & 3) * NV + v) * wg + tid] = du;
               else Dp[bt + off] = du;
             } else if constexpr (sizeof(T) == 8) {
-              double un = combine(a.kind, u0pre[v], u1pre[v], ds, du);
+              double un = combine(a.kind, u0pre[v], u1pre[v], ds, du, a.ca, a.cb);
               if (a.last && v == P::ibp) un = rc.decay * un;
               unew[v] = un;
             } else {
               // mixed precision: the FP32 increment is added to the FP64
               // state in increment form with Kahan compensation.  The true
-              // step state is U0 - C; the last stage forms U0_true + delta
-              // and keeps the rounding error of that sum in C (the
-              // intermediate stages use U0 alone).
+              // step state is U0 - C; a stage that writes the step state
+              // forms U0_true + delta and keeps the rounding error of that
+              // sum in C (the stages into U1 use U0 alone).
               const double u0 = u0pre[v];
               const double cp = double(cpre[v]);
               const double dd = ds * double(du);
-              double delta;
-              switch (a.kind) {
-              case 0: delta = dd; break;
-              case 1: delta = 0.25 * ((u1pre[v] - u0) + cp) + dd; break;
-              case 2: delta = (2.0 / 3.0) * ((u1pre[v] - u0) + cp) + dd; break;
-              default: delta = 0.5 * (((u1pre[v] - u0) + cp) + dd); break;
-              }
-              const double y = delta - cp;
+              const double y =
+                  combine_delta(a.kind, u0, u1pre[v], cp, dd, a.ca, a.cb) - cp;
               double un = u0 + y;
-              if (a.last) {
+              if (a.comp) {
                 double cn = (un - u0) - y;
-                if (v == P::ibp) { un = rc.decay * un; cn = rc.decay * cn; }
+                if (a.last && v == P::ibp) { un = rc.decay * un; cn = rc.decay * cn; }
                 a.C[bf + off] = float(cn);
               }
               unew[v] = un;

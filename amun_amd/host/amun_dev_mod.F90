@@ -26,7 +26,8 @@ module amun_dev_mod
   integer(c_int32_t), parameter :: AMUN_RS_HLL = 1, AMUN_RS_HLLC = 2
   integer(c_int32_t), parameter :: AMUN_RS_HLLD = 3
   integer(c_int32_t), parameter :: AMUN_TA_EULER = 0, AMUN_TA_RK2 = 1
-  integer(c_int32_t), parameter :: AMUN_TA_RK3 = 2
+  integer(c_int32_t), parameter :: AMUN_TA_RK3 = 2, AMUN_TA_SSPRK_M2 = 3
+  integer(c_int32_t), parameter :: AMUN_TA_SSPRK43 = 4, AMUN_TA_SSPRK53 = 5
   integer(c_int32_t), parameter :: AMUN_PREC_FP64 = 0, AMUN_PREC_MIXED = 1
 
   type, bind(C) :: amun_cfg
@@ -40,7 +41,7 @@ module amun_dev_mod
     real(c_double)     :: tolerance, pmin
     integer(c_int32_t) :: nr_maxit, nr_extra
     real(c_double)     :: sgp, kappa, kbeta
-    integer(c_int32_t) :: ngp, reserved
+    integer(c_int32_t) :: ngp, stages
   end type amun_cfg
 
   interface

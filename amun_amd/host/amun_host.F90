@@ -111,6 +111,12 @@ program amun_host
     cfg%time_advance = AMUN_TA_RK2
   case("rk3", "RK3")
     cfg%time_advance = AMUN_TA_RK3
+  case("ssprk(m,2)", "SSPRK(m,2)")
+    cfg%time_advance = AMUN_TA_SSPRK_M2
+  case("rk3.4", "RK3.4", "ssprk(4,3)", "SSPRK(4,3)")
+    cfg%time_advance = AMUN_TA_SSPRK43
+  case("rk3.5", "RK3.5", "ssprk(5,3)", "SSPRK(5,3)")
+    cfg%time_advance = AMUN_TA_SSPRK53
   case default
     cfg%time_advance = -1
   end select
@@ -139,6 +145,7 @@ program amun_host
   cfg%kappa     = get_real("kappa", 1.0d0)
   cfg%kbeta     = get_real("kbeta", 1.0d0)
   cfg%ngp       = get_int("ngp", 5)
+  cfg%stages    = get_int("stages", 2)
   nmax    = get_int("nmax", 10)
   tmax    = get_real("tmax", 1.0d30)
   seed    = int(get_int("seed", 20261202), int64)

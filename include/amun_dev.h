@@ -61,6 +61,9 @@ extern "C" {
 #define AMUN_TA_EULER      0
 #define AMUN_TA_RK2        1   /* SSPRK(2,2), evolve_rk2 evolution.F90:628 */
 #define AMUN_TA_RK3        2   /* SSPRK(3,3), evolve_rk3 evolution.F90:1005 */
+#define AMUN_TA_SSPRK_M2   3   /* SSPRK(m,2), evolve_ssprk2 evolution.F90:791 */
+#define AMUN_TA_SSPRK43    4   /* SSPRK(4,3), evolve_ssprk34 evolution.F90:1228 */
+#define AMUN_TA_SSPRK53    5   /* SSPRK(5,3), evolve_ssprk35 evolution.F90:1492 */
 
 /* arithmetic of reconstruction + fluxes; the conserved state is always FP64 */
 #define AMUN_PREC_FP64     0
@@ -100,7 +103,7 @@ typedef struct amun_cfg {
   double  kappa;            /* "kappa" (1), used as min(kappa, (1-cfl)/cfl)   */
   double  kbeta;            /* "kbeta" (1)                                    */
   int32_t ngp;              /* "ngp": stencil points (5)                      */
-  int32_t reserved;
+  int32_t stages;           /* "stages" of SSPRK(m,2), clamped to [2, 9] (2)  */
 } amun_cfg;
 
 typedef struct amun_ctx amun_ctx;

@@ -59,6 +59,7 @@ typedef struct {
   int    mhd, sr;          /* magnetic field present / relativistic system   */
   double gammam1, gammam1i, gammaxi;
   double cflx;             /* cfl as adjusted by initialize_evolution        */
+  int    stages;           /* m of SSPRK(m,2)                                */
   double decay;            /* exp(-alpha*cfl), evolution.F90:211-213         */
   double adx, ady, adz, adxi, adyi, adzi;
   double cmax, cmax2;      /* equations module state                         */
@@ -118,7 +119,13 @@ static int oc_setup(ocfg* o, const amun_cfg* c)
     int r = prepare_gp(o);
     if (r) return r;
   }
-  o->cflx  = c->cfl;       /* euler, rk2 and rk3 keep the CFL unchanged */
+  /* initialize_evolution (evolution.F90:156-200): the m-stage and
+   * four/five-stage SSPRK schemes run at a multiple of the CFL number */
+  o->cflx  = c->cfl;
+  o->stages = c->stages < 2 ? 2 : (c->stages > 9 ? 9 : c->stages);
+  if (c->time_advance == AMUN_TA_SSPRK_M2) o->cflx = (o->stages - 1) * c->cfl;
+  if (c->time_advance == AMUN_TA_SSPRK43) o->cflx = 2.0 * c->cfl;
+  if (c->time_advance == AMUN_TA_SSPRK53) o->cflx = 2.65062919143939 * c->cfl;
   o->decay = exp(-c->alpha * o->cflx);
   {
     int ir = c->xblocks, jr = c->yblocks, kr = (o->nd == 3) ? c->zblocks : 1;
@@ -1713,9 +1720,10 @@ static void new_time_step(ogrid* g, double dtnext)
   if (dtnext > 0.0) g->dt = fmin(g->dtn, dtnext);
 }
 
-/* one stage: fluxes of u (through q), increment, combine into dst */
-static void stage(ogrid* g, int kind, double ds, double* dst, const double* a0,
-                  const double* a1)
+/* one stage: fluxes of u (through q), increment, combine into dst (which
+ * may be a0 or a1: every element is read before it is written) */
+static void stage2(ogrid* g, int kind, double ds, double* dst, const double* a0,
+                   const double* a1, double ca, double cb)
 {
   ocfg* o = &g->o;
   const size_t bs = bsize(o);
@@ -1732,11 +1740,25 @@ static void stage(ogrid* g, int kind, double ds, double* dst, const double* a0,
         v = ((3.0 / 4.0) * a0[b * bs + n] + (1.0 / 4.0) * a1[b * bs + n]) + ds * g->du[n];
       else if (kind == 2)  /* rk3 3rd: 1/3 u0 + 2/3 u1 + ds du */
         v = ((1.0 / 3.0) * a0[b * bs + n] + (2.0 / 3.0) * a1[b * bs + n]) + ds * g->du[n];
-      else                 /* rk2 2nd: 1/2 (u0 + u1 + dt du) */
+      else if (kind == 3)  /* rk2 2nd: 1/2 (u0 + u1 + dt du) */
         v = 0.5 * ((a0[b * bs + n] + a1[b * bs + n]) + ds * g->du[n]);
+      else if (kind == 4)  /* ssprk: u1 + ds du */
+        v = a1[b * bs + n] + ds * g->du[n];
+      else if (kind == 5)  /* ssprk(m,2) last: fl u0 + fr (u1 + ds du) */
+        v = ca * a0[b * bs + n] + cb * (a1[b * bs + n] + ds * g->du[n]);
+      else if (kind == 6)  /* a u0 + b u1 + ds du */
+        v = (ca * a0[b * bs + n] + cb * a1[b * bs + n]) + ds * g->du[n];
+      else                 /* a u1 + b u0 + ds du */
+        v = (ca * a1[b * bs + n] + cb * a0[b * bs + n]) + ds * g->du[n];
       dst[b * bs + n] = v;
     }
   }
+}
+
+static void stage(ogrid* g, int kind, double ds, double* dst, const double* a0,
+                  const double* a1)
+{
+  stage2(g, kind, ds, dst, a0, a1, 0.0, 0.0);
 }
 
 static void psi_decay(ogrid* g, double* u)
@@ -1759,6 +1781,46 @@ static void advance(ogrid* g, double dtnext)
     stage(g, 0, g->dt, g->u1, g->u0, NULL);
     update_variables(g, g->u1);
     stage(g, 3, g->dt, g->u0, g->u0, g->u1);
+    psi_decay(g, g->u0);
+    update_variables(g, g->u0);
+  } else if (o->c.time_advance == AMUN_TA_SSPRK_M2) {   /* evolve_ssprk2 791 */
+    const double ft = 1.0 / (o->stages - 1), fl = 1.0 / o->stages, fr = 1.0 - fl;
+    const double ds = ft * g->dt;
+    int n;
+    memcpy(g->u1, g->u0, bsize(o) * g->nblk * sizeof(double));
+    for (n = 1; n <= o->stages - 1; n++) {
+      stage(g, 4, ds, g->u1, g->u0, g->u1);
+      update_variables(g, g->u1);
+    }
+    stage2(g, 5, ds, g->u0, g->u0, g->u1, fl, fr);
+    psi_decay(g, g->u0);
+    update_variables(g, g->u0);
+  } else if (o->c.time_advance == AMUN_TA_SSPRK43) {    /* evolve_ssprk34 1228 */
+    const double b1 = 1.0 / 2.0, b3 = 1.0 / 6.0, a31 = 2.0 / 3.0, a33 = 1.0 / 3.0;
+    stage(g, 0, b1 * g->dt, g->u1, g->u0, NULL);
+    update_variables(g, g->u1);
+    stage(g, 4, b1 * g->dt, g->u1, g->u0, g->u1);
+    update_variables(g, g->u1);
+    stage2(g, 6, b3 * g->dt, g->u1, g->u0, g->u1, a31, a33);
+    update_variables(g, g->u1);
+    stage(g, 4, b1 * g->dt, g->u0, g->u0, g->u1);
+    psi_decay(g, g->u0);
+    update_variables(g, g->u0);
+  } else if (o->c.time_advance == AMUN_TA_SSPRK53) {    /* evolve_ssprk35 1492 */
+    const double b1 = 3.77268915331368e-01, b3 = 2.42995220537396e-01;
+    const double b4 = 2.38458932846290e-01, b5 = 2.87632146308408e-01;
+    const double a31 = 3.55909775063327e-01, a33 = 6.44090224936673e-01;
+    const double a41 = 3.67933791638137e-01, a44 = 6.32066208361863e-01;
+    const double a53 = 2.37593836598569e-01, a55 = 7.62406163401431e-01;
+    stage(g, 0, b1 * g->dt, g->u1, g->u0, NULL);
+    update_variables(g, g->u1);
+    stage(g, 4, b1 * g->dt, g->u1, g->u0, g->u1);
+    update_variables(g, g->u1);
+    stage2(g, 6, b3 * g->dt, g->u1, g->u0, g->u1, a31, a33);
+    update_variables(g, g->u1);
+    stage2(g, 6, b4 * g->dt, g->u0, g->u0, g->u1, a41, a44);
+    update_variables(g, g->u0);
+    stage2(g, 7, b5 * g->dt, g->u0, g->u0, g->u1, a53, a55);
     psi_decay(g, g->u0);
     update_variables(g, g->u0);
   } else {                                         /* evolve_rk3 1005 */
@@ -1787,8 +1849,7 @@ int orc_create(const amun_cfg* c, void** handle)
   if (!r) r = riemann_check(c);
   if (!r && c->reconstruction != AMUN_REC_TVD && c->reconstruction != AMUN_REC_WENO5Z &&
       c->reconstruction != AMUN_REC_GP) r = 200;
-  if (!r && c->time_advance != AMUN_TA_EULER && c->time_advance != AMUN_TA_RK2 &&
-      c->time_advance != AMUN_TA_RK3) r = 400;
+  if (!r && (c->time_advance < AMUN_TA_EULER || c->time_advance > AMUN_TA_SSPRK53)) r = 400;
   if (r) { free(g); return r; }
   g->ir = c->xblocks; g->jr = c->yblocks; g->kr = (c->ndims == 3) ? c->zblocks : 1;
   g->nblk = g->ir * g->jr * g->kr;

@@ -187,6 +187,35 @@ def main_gp():
     print(name, p3.shape, "dt", tr[-1, 0])
 
 
+SSPRK_RUNS = [("ssprk(m,2)", 4), ("ssprk(4,3)", 2), ("ssprk(5,3)", 2)]
+
+
+def ssprk_params(ta, stages):
+    return Params(ndims=2, equation_system="mhd", reconstruction="tvd", limiter="mc",
+                  riemann_solver="hlld", time_advance=ta, gamma=5.0 / 3.0, cfl=0.3,
+                  ncells=16, nghosts=2, xblocks=2, yblocks=2, stages=stages)
+
+
+def main_ssprk():
+    """Orszag-Tang runs of the three SSPRK families (own generator)."""
+    with open(os.path.join(HERE, "index.json")) as fh:
+        index = json.load(fh)
+    for ta, stages in SSPRK_RUNS:
+        p = ssprk_params(ta, stages)
+        u0 = problems.orszag_tang_2d(p, seed=9)
+        nsteps, stride = 10, 3
+        u, tr = ol.ref_run(p, u0, nsteps)
+        name = "ssprk_" + ta.replace("(", "").replace(")", "").replace(",", "") + "_ot2d_32_10"
+        np.savez_compressed(os.path.join(HERE, name + ".npz"),
+                            params=np.array(p.to_params_in()), ndims=2, nsteps=nsteps,
+                            stride=stride, trace=tr, sub=u[:, :, ::stride, ::stride],
+                            sha_u0=sha(u0), sha_u=sha(u))
+        index[name] = {"steps": nsteps, "sha_u": sha(u), "dt_last": float(tr[-1, 0])}
+        print(name, p.shape, "dt", tr[-1, 0])
+    with open(os.path.join(HERE, "index.json"), "w") as fh:
+        json.dump(index, fh, indent=1)
+
+
 def main():
     if not (ol.ref_available(2) and ol.ref_available(3)):
         sys.exit("reference binaries missing: run oracle/build_ref.sh first")
@@ -195,6 +224,9 @@ def main():
         return
     if sys.argv[1:] == ["gp"]:
         main_gp()
+        return
+    if sys.argv[1:] == ["ssprk"]:
+        main_ssprk()
         return
     index = {}
     for name, p, builder, nsteps, stride in run_cases():
@@ -260,6 +292,7 @@ def main():
         json.dump(index, fh, indent=1)
     main_sr()
     main_gp()
+    main_ssprk()
 
 
 if __name__ == "__main__":

@@ -478,3 +478,71 @@ def test_gp_256_mixed_vs_fp64():
     l1 = _l1(res[1][0], res[0][0])
     print(f"GP 256^3 x 10 mixed vs fp64: density L1 {l1:.3e}")
     assert l1 <= 1e-6
+
+
+# ------------------------------------------------------------ integrators
+SSPRK = [("ssprk(m,2)", 2), ("ssprk(m,2)", 3), ("ssprk(m,2)", 9), ("ssprk(4,3)", 2),
+         ("ssprk(5,3)", 2)]
+
+
+@pytest.mark.parametrize("nd", [2, 3])
+@pytest.mark.parametrize("eq,rs,rec", [("hd", "hllc", "weno5z"), ("mhd", "hlld", "tvd")])
+@pytest.mark.parametrize("ta,stages", SSPRK)
+def test_ssprk_one_step(nd, eq, rs, rec, ta, stages):
+    """SSPRK(m,2), SSPRK(4,3) and SSPRK(5,3): the stage combines (written into
+    U1 or the step state), the scaled CFL number and the psi decay."""
+    p = Params(ndims=nd, equation_system=eq, riemann_solver=rs, reconstruction=rec,
+               limiter="mc" if rec == "tvd" else "mm", time_advance=ta, stages=stages,
+               ncells=8, nghosts=4 if rec == "weno5z" else 2, xblocks=2, yblocks=3,
+               zblocks=2, cfl=0.1, gamma=1.4 if eq == "hd" else 5.0 / 3.0, xmin=-0.5,
+               xmax=0.5, ymin=-0.75, ymax=0.75, zmin=0.0, zmax=1.0)
+    u0 = initial_state(p) if eq == "hd" else _mhd_state(p)
+    dt, cmax, ug, tr, uo = one_step(p, u0)
+    assert abs(dt - tr[0]) <= 1e-13 * tr[0]
+    assert relerr(ug, uo).max() <= 1e-12
+
+
+@pytest.mark.parametrize("name", ["ssprk_ssprkm2_ot2d_32_10", "ssprk_ssprk43_ot2d_32_10",
+                                  "ssprk_ssprk53_ot2d_32_10"])
+def test_ssprk_golden_runs(name):
+    """Ten steps of Orszag-Tang against the reference's own output (golden
+    fixture: dt trace and the final state)."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", name + ".npz"))
+    p = Params.from_text(str(g["params"]), ndims=2)
+    from amun_amd.problems import orszag_tang_2d
+    u0 = orszag_tang_2d(p, seed=9)
+    print(f"[case] {name}", flush=True)
+    with Solver(p) as s:
+        s.upload(u0)
+        dts = s.run(int(g["nsteps"]))
+        u = s.download()
+    assert np.abs(dts - g["trace"][:, 0]).max() <= 1e-10 * g["trace"][:, 0].max()
+    st = int(g["stride"])
+    assert relerr(u[:, :, ::st, ::st], g["sub"]).max() <= 1e-8
+
+
+@pytest.mark.parametrize("ta,stages", SSPRK)
+def test_ssprk_mixed_vs_oracle(ta, stages):
+    """Mixed precision with the compensated step state for every SSPRK
+    scheme (SSPRK(5,3) writes the step state in two stages)."""
+    p = Params(ndims=2, equation_system="mhd", reconstruction="tvd", limiter="mc",
+               riemann_solver="hlld", time_advance=ta, stages=stages, ncells=16, nghosts=2,
+               xblocks=4, yblocks=4, cfl=0.3)
+    from amun_amd.problems import orszag_tang_2d
+    u0 = orszag_tang_2d(p)
+    o = ol.OracleRun(p)
+    o.set_state(u0)
+    o.run(10)
+    uo = o.get_state()
+    p.precision = "mixed"
+    with Solver(p) as s:
+        s.upload(u0)
+        s.run(10)
+        um = s.download()
+    err = np.delete(relerr(um, uo), 7)
+    print(f"{ta} m={stages} mixed 10 steps: density L1 {_l1(um[0], uo[0]):.3e}, "
+          f"max rel {err.max():.3e}")
+    assert np.abs(um[7] - uo[7]).max() <= 1e-5 * np.abs(uo[4:7]).max()
+    assert err.max() <= 1e-4
+    assert _l1(um[0], uo[0]) <= 1e-6

@@ -163,3 +163,21 @@ def test_gp_run_bit_exact():
     tr = o.run(int(g["nsteps"]))
     np.testing.assert_array_equal(tr, g["trace"])
     assert sha(o.get_state()) == str(g["sha_u"])
+
+
+# ------------------------------------------------------------ integrators
+@pytest.mark.parametrize("name", ["ssprk_ssprkm2_ot2d_32_10", "ssprk_ssprk43_ot2d_32_10",
+                                  "ssprk_ssprk53_ot2d_32_10"])
+def test_ssprk_run_bit_exact(name):
+    """SSPRK(4,2), SSPRK(4,3) and SSPRK(5,3) on Orszag-Tang (TVD-MC + HLLD)."""
+    g = load(name)
+    p = params_of(g)
+    u0 = problems.orszag_tang_2d(p, seed=9)
+    assert sha(u0) == str(g["sha_u0"])
+    o = ol.OracleRun(p)
+    o.set_state(u0)
+    tr = o.run(int(g["nsteps"]))
+    np.testing.assert_array_equal(tr, g["trace"])
+    s = int(g["stride"])
+    np.testing.assert_array_equal(o.get_state()[:, :, ::s, ::s], g["sub"])
+    assert sha(o.get_state()) == str(g["sha_u"])

@@ -106,3 +106,29 @@ def test_gp_fresh_inputs(sgp, kappa):
     o.set_state(u0)
     np.testing.assert_array_equal(o.run(3), tr)
     np.testing.assert_array_equal(o.get_state(), ur)
+
+
+@needs_ref
+@pytest.mark.parametrize("ta,stages", [("ssprk(m,2)", 2), ("ssprk(m,2)", 3), ("ssprk(m,2)", 5),
+                                       ("ssprk(m,2)", 12), ("ssprk(4,3)", 2), ("ssprk(5,3)", 2)])
+@pytest.mark.parametrize("eq", ["mhd", "hd"])
+def test_ssprk_integrators(ta, stages, eq):
+    """SSPRK(m,2) (m clamped to 2..9), SSPRK(4,3) and SSPRK(5,3): stage
+    combinations, CFL scaling and the psi decay at the scaled CFL."""
+    if eq == "mhd":
+        p = Params(ndims=2, equation_system="mhd", reconstruction="tvd", limiter="mc",
+                   riemann_solver="hlld", time_advance=ta, gamma=5.0 / 3.0, cfl=0.3,
+                   ncells=8, nghosts=2, xblocks=3, yblocks=2, stages=stages)
+        u0 = problems.orszag_tang_2d(p, seed=4)
+    else:
+        p = Params(ndims=2, equation_system="hd", reconstruction="weno5z", limiter="mm",
+                   riemann_solver="hllc", time_advance=ta, gamma=1.4, cfl=0.05,
+                   ncells=8, nghosts=4, xblocks=2, yblocks=2, stages=stages)
+        u0 = problems.blast_hd_2d(p)
+    ur, tr = ol.ref_run(p, u0, 4)
+    assert np.isfinite(ur).all()
+    o = ol.OracleRun(p)
+    o.set_state(u0)
+    to = o.run(4)
+    np.testing.assert_array_equal(to, tr)
+    np.testing.assert_array_equal(o.get_state(), ur)
