@@ -50,6 +50,7 @@ class AmunCfg(ctypes.Structure):
         ("nr_maxit", ctypes.c_int32), ("nr_extra", ctypes.c_int32),
         ("sgp", ctypes.c_double), ("kappa", ctypes.c_double), ("kbeta", ctypes.c_double),
         ("ngp", ctypes.c_int32), ("stages", ctypes.c_int32),
+        ("prol_limiter", ctypes.c_int32), ("reserved2", ctypes.c_int32),
     ]
 
 
@@ -106,6 +107,7 @@ class Params:
     kbeta: float = 1.0
     ngp: int = 5
     stages: int = 2
+    prolongation_limiter: str = "mm"
     extra: dict = field(default_factory=dict)
 
     # ---------------------------------------------------------------- derived
@@ -160,6 +162,10 @@ class Params:
         c.nr_maxit, c.nr_extra = int(self.nr_maxit), int(self.nr_extra)
         c.sgp, c.kappa, c.kbeta, c.ngp = self.sgp, self.kappa, self.kbeta, int(self.ngp)
         c.stages = int(self.stages)
+        # interpolations.F90:412-428: mm, mc, sb, vl, anything else the zero slope
+        c.prol_limiter = {"mm": 1, "minmod": 1, "mc": 2, "monotonized_central": 2,
+                          "sb": 3, "superbee": 3, "vl": 4, "vanleer": 4}.get(
+                              self.prolongation_limiter.lower(), 0)
         return c
 
     def dx(self):
@@ -191,6 +197,7 @@ class Params:
             f"nr_maxit = {self.nr_maxit}", f"nr_extra = {self.nr_extra}",
             f"sgp = {self.sgp!r}", f"kappa = {self.kappa!r}", f"kbeta = {self.kbeta!r}",
             f"ngp = {self.ngp}", f"stages = {self.stages}",
+            f"prolongation_limiter = {self.prolongation_limiter}",
             "minlev = 1", "maxlev = 1",
             "xlbndry = periodic", "xubndry = periodic",
             "ylbndry = periodic", "yubndry = periodic",
@@ -198,6 +205,8 @@ class Params:
             "problem = blast", "snapshot_interval = 0", "restart_interval = 0",
             "integrals_interval = 1000000", "tmax = 1.0e+30", "nmax = 1000000000",
         ]
+        # keys given in extra (e.g. maxlev, problem) replace the defaults
+        lines = [ln for ln in lines if ln.split(" = ")[0] not in self.extra]
         for k, v in self.extra.items():
             lines.append(f"{k} = {v}")
         return "\n".join(lines) + "\n"

@@ -5,6 +5,9 @@
 // inst_tvd.hip, inst_weno5z.hip and inst_gp.hip.
 #include "kernels.h"
 
+#include <array>
+#include <map>
+
 // ===========================================================================
 // context
 // ===========================================================================
@@ -48,7 +51,24 @@ struct amun_ctx {
     double* ck = nullptr;       // ceil(nsteps / interval) states
     double* dtc = nullptr;      // device copy of (dt, cmax) per step
   } tape;
+  // multi-level mesh (amun_dev_init_amr): real blocks [0, nreal) of the
+  // pools, virtual blocks [nreal, nreal + nvirt) of the primitive pools
+  struct Amr {
+    bool on = false;
+    int nreal = 0, nvirt = 0, nlev = 1, nfix = 0;
+    std::vector<int32_t> meta;          // level, i0, j0, k0 per real block
+    int* blev = nullptr;                // level - 1 per real block
+    int4* rtask = nullptr;              // restrictions
+    int4* ptask = nullptr;              // prolongations
+    int4* ftask[3] = {nullptr, nullptr, nullptr};   // fine fluxes per direction
+    int nr = 0, np = 0, nf[3] = {0, 0, 0};
+    int* ffix = nullptr;                // correction slot per block face
+    double* FB = nullptr;               // averaged fine fluxes per slot
+    DevState* dummy = nullptr;          // events of the fine-flux recomputation
+  } amr;
 };
+
+static int amr_fill(amun_ctx* c, int qcur, int slot);
 
 namespace {
 
@@ -169,6 +189,7 @@ RunConst make_rc(const amun_cfg& cfg) {
   rc.nruns = 1;
   rc.ncell = (nd == 3) ? nc * nc * nc : nc * nc;
   rc.limiter = cfg.limiter;
+  rc.prol_limiter = cfg.prol_limiter;
   rc.gamma = cfg.gamma;
   rc.gammam1 = cfg.gamma - 1.0;
   rc.gammam1i = 1.0 / rc.gammam1;
@@ -246,7 +267,12 @@ void amun_dev_default_cfg(amun_cfg* c) {
   c->kbeta = 1.0;
   c->ngp = 5;
   c->stages = 2;
+  c->prol_limiter = AMUN_LIM_MINMOD;
 }
+
+static int init_device(amun_ctx* c, const std::vector<int32_t>& nbr,
+                       const std::vector<int32_t>& brun, const std::vector<RunState>& hrs,
+                       int nqblk, amun_ctx** out);
 
 int amun_dev_init_batch(const amun_cfg* cfg, int32_t nruns, const amun_grid* grids,
                         amun_ctx** out) {
@@ -319,12 +345,25 @@ int amun_dev_init_batch(const amun_cfg* cfg, int32_t nruns, const amun_grid* gri
               }
         }
   }
+  return init_device(c, nbr, brun, hrs, rc.nblk, out);
+}
+
+// Device buffers, tables and kernel selection of a context whose host-side
+// layout (runs, blocks, neighbour table) is set up; the primitive pools hold
+// nqblk blocks (the real blocks first, then the virtual blocks of a
+// multi-level mesh).
+static int init_device(amun_ctx* c, const std::vector<int32_t>& nbr,
+                       const std::vector<int32_t>& brun, const std::vector<RunState>& hrs,
+                       int nqblk, amun_ctx** out) {
+  const amun_cfg* cfg = &c->cfg;
+  const RunConst& rc = c->rc;
+  const int nruns = rc.nruns, nv = c->nv, nd = cfg->ndims;
   c->tsteps = (nruns == 1) ? kTrace : std::max(64, kTrace / nruns);
 
   c->nstate = (size_t)rc.nblk * nv * rc.ncell;
   const size_t bytes = c->nstate * sizeof(double);
   c->qsize = (cfg->precision == AMUN_PREC_MIXED) ? sizeof(float) : sizeof(double);
-  const size_t qbytes = c->nstate * c->qsize;
+  const size_t qbytes = (size_t)nqblk * nv * rc.ncell * c->qsize;
   hipError_t e;
 #define ALLOC(ptr, sz)                                                       \
   e = hipMalloc((void**)&(ptr), (sz));                                       \
@@ -401,6 +440,274 @@ int amun_dev_init(const amun_cfg* cfg, amun_ctx** out) {
   return amun_dev_init_batch(cfg, 1, &g, out);
 }
 
+// A multi-level mesh of leaf blocks on the periodic base grid of cfg.
+// meta[4 b .. 4 b + 3] = level (1 = base) and the first interior cell
+// (i0, j0, k0) of block b in cells of its level.  The mesh must be 2:1
+// balanced across faces, edges and corners (as AMUN's refinement keeps it).
+int amun_dev_init_amr(const amun_cfg* cfg, int32_t nblk, const int32_t* meta, amun_ctx** out) {
+  if (!cfg || !meta || !out || nblk < 1) return fail(500, "bad argument");
+  *out = nullptr;
+  int nv = 0;
+  int r = validate(*cfg, &nv);
+  if (r) return r;
+  if (cfg->eqsys == AMUN_EQSYS_SRMHD)
+    return fail(600, "multi-level meshes are not supported for the relativistic system");
+  const int nd = cfg->ndims, nc = cfg->ncells, ng = cfg->nghosts;
+  if (ng % 2 || 2 * ng > nc)
+    return fail(600, "multi-level meshes need an even nghosts of at most ncells / 2");
+  amun_grid G;
+  std::memset(&G, 0, sizeof(G));
+  G.xblocks = cfg->xblocks; G.yblocks = cfg->yblocks; G.zblocks = nd == 3 ? cfg->zblocks : 1;
+  G.xmin = cfg->xmin; G.xmax = cfg->xmax; G.ymin = cfg->ymin; G.ymax = cfg->ymax;
+  G.zmin = nd == 3 ? cfg->zmin : 0.0; G.zmax = nd == 3 ? cfg->zmax : 1.0;
+  if ((r = validate_grid(*cfg, G))) return r;
+  const int base[3] = {G.xblocks, G.yblocks, G.zblocks};
+
+  // leaf table per level
+  int nlev = 1;
+  for (int b = 0; b < nblk; b++) {
+    const int L = meta[4 * b];
+    if (L < 1 || L > 16) return fail(500, "block level out of range");
+    nlev = std::max(nlev, L);
+    for (int d = 0; d < 3; d++)
+      if (meta[4 * b + 1 + d] % nc || meta[4 * b + 1 + d] < 0)
+        return fail(500, "block offsets must be non-negative multiples of ncells");
+  }
+  auto nb_at = [&](int L, int d) -> long { return (d < nd) ? (long)base[d] << (L - 1) : 1; };
+  std::vector<std::vector<int>> lut(nlev + 2);
+  for (int L = 1; L <= nlev; L++)
+    lut[L].assign((size_t)nb_at(L, 0) * nb_at(L, 1) * nb_at(L, 2), -1);
+  auto bid = [&](int L, long x, long y, long z) -> long {   // block position id
+    return (z * nb_at(L, 1) + y) * nb_at(L, 0) + x;
+  };
+  for (int b = 0; b < nblk; b++) {
+    const int L = meta[4 * b];
+    const long x = meta[4 * b + 1] / nc, y = meta[4 * b + 2] / nc, z = meta[4 * b + 3] / nc;
+    if (x >= nb_at(L, 0) || y >= nb_at(L, 1) || z >= nb_at(L, 2))
+      return fail(500, "block outside the domain");
+    int& e = lut[L][bid(L, x, y, z)];
+    if (e >= 0) return fail(500, "two blocks at one position");
+    e = b;
+  }
+  auto wrapl = [](long v, long n) { v %= n; return v < 0 ? v + n : v; };
+  // leaf covering cell (x, y, z) of level L (wrapped), -1 if none at L
+  auto leaf = [&](int L, long x, long y, long z) -> int {
+    if (L < 1 || L > nlev) return -1;
+    return lut[L][bid(L, x / nc, y / nc, z / nc)];
+  };
+
+  // neighbour table with virtual blocks for positions without a same-level leaf
+  std::vector<int32_t> nbr((size_t)nblk * 27, -1);
+  std::map<std::array<long, 4>, int> vmap;
+  std::vector<std::array<long, 4>> vpos;          // level, block x, y, z
+  std::vector<std::vector<unsigned char>> vmask;  // cells to fill
+  const int ncell = (nd == 3) ? nc * nc * nc : nc * nc;
+  for (int b = 0; b < nblk; b++) {
+    const int L = meta[4 * b];
+    const long p0[3] = {meta[4 * b + 1] / nc, meta[4 * b + 2] / nc, meta[4 * b + 3] / nc};
+    for (int dk = -1; dk <= 1; dk++)
+      for (int dj = -1; dj <= 1; dj++)
+        for (int di = -1; di <= 1; di++) {
+          if (nd == 2 && dk != 0) continue;
+          const long x = wrapl(p0[0] + di, nb_at(L, 0)), y = wrapl(p0[1] + dj, nb_at(L, 1));
+          const long z = wrapl(p0[2] + dk, nb_at(L, 2));
+          int& slot = nbr[(size_t)b * 27 + (dk + 1) * 9 + (dj + 1) * 3 + (di + 1)];
+          const int same = lut[L][bid(L, x, y, z)];
+          if (same >= 0) { slot = same; continue; }
+          const std::array<long, 4> key = {L, x, y, z};
+          auto it = vmap.find(key);
+          int v;
+          if (it == vmap.end()) {
+            v = (int)vpos.size();
+            vmap[key] = v;
+            vpos.push_back(key);
+            vmask.emplace_back(ncell, 0);
+          } else {
+            v = it->second;
+          }
+          slot = nblk + v;
+          // the cells within nghosts of block b
+          const int d3[3] = {di, dj, dk};
+          int lo[3], hi[3];
+          for (int d = 0; d < 3; d++) {
+            if (d >= nd) { lo[d] = 0; hi[d] = 1; continue; }
+            lo[d] = d3[d] > 0 ? 0 : (d3[d] < 0 ? nc - ng : 0);
+            hi[d] = d3[d] > 0 ? ng : nc;
+          }
+          for (int k = lo[2]; k < hi[2]; k++)
+            for (int j = lo[1]; j < hi[1]; j++)
+              for (int i = lo[0]; i < hi[0]; i++)
+                vmask[v][((size_t)k * nc + j) * nc + i] = 1;
+        }
+  }
+  const int nvirt = (int)vpos.size();
+  auto pack = [](int i, int j, int k) { return i | (j << 8) | (k << 16); };
+
+  // ghost tasks of the virtual blocks
+  std::vector<int4> rtask, ptask;
+  for (int v = 0; v < nvirt; v++) {
+    const int L = (int)vpos[v][0];
+    for (int k = 0; k < (nd == 3 ? nc : 1); k++)
+      for (int j = 0; j < nc; j++)
+        for (int i = 0; i < nc; i++) {
+          if (!vmask[v][((size_t)k * nc + j) * nc + i]) continue;
+          const long x = vpos[v][1] * nc + i, y = vpos[v][2] * nc + j;
+          const long z = nd == 3 ? vpos[v][3] * nc + k : 0;
+          const int F = leaf(L + 1, 2 * x, 2 * y, 2 * z);
+          if (F >= 0) {
+            const int fi = (int)(2 * x - meta[4 * F + 1]), fj = (int)(2 * y - meta[4 * F + 2]);
+            const int fk = nd == 3 ? (int)(2 * z - meta[4 * F + 3]) : 0;
+            rtask.push_back(make_int4(nblk + v, pack(i, j, k), F, pack(fi, fj, fk)));
+            continue;
+          }
+          const int C = leaf(L - 1, x / 2, y / 2, z / 2);
+          if (C < 0) return fail(700, "mesh is not 2:1 balanced (or has a hole)");
+          const int ci = (int)(x / 2 - meta[4 * C + 1]), cj = (int)(y / 2 - meta[4 * C + 2]);
+          const int ck = nd == 3 ? (int)(z / 2 - meta[4 * C + 3]) : 0;
+          // the slope stencil must only meet real blocks or restricted cells
+          for (int d = 0; d < nd; d++)
+            for (int sgn = -1; sgn <= 1; sgn += 2) {
+              int q[3] = {ci, cj, ck};
+              q[d] += sgn;
+              if (q[d] >= 0 && q[d] < nc) continue;
+              const int di = q[0] < 0 ? -1 : (q[0] >= nc ? 1 : 0);
+              const int dj = q[1] < 0 ? -1 : (q[1] >= nc ? 1 : 0);
+              const int dk = q[2] < 0 ? -1 : (q[2] >= nc ? 1 : 0);
+              const int nb = nbr[(size_t)C * 27 + (dk + 1) * 9 + (dj + 1) * 3 + (di + 1)];
+              if (nb < nblk) continue;
+              const int w = nb - nblk;
+              const long cx = vpos[w][1] * nc + (q[0] - di * nc);
+              const long cy = vpos[w][2] * nc + (q[1] - dj * nc);
+              const long cz = nd == 3 ? vpos[w][3] * nc + (q[2] - dk * nc) : 0;
+              const bool restricted = leaf(L, 2 * cx, 2 * cy, 2 * cz) >= 0;
+              const size_t lc = ((size_t)(q[2] - dk * nc) * nc + (q[1] - dj * nc)) * nc + (q[0] - di * nc);
+              if (!restricted || !vmask[w][lc])
+                return fail(701, "prolongation stencil outside the restricted ghost cells");
+            }
+          ptask.push_back(make_int4(nblk + v, pack(i, j, k), C,
+                                    pack(ci, cj, ck) | (int)((x & 1) << 24) |
+                                        (int)((y & 1) << 25) | (int)((z & 1) << 26)));
+        }
+  }
+
+  // flux-correction slots: coarse faces whose neighbours are finer
+  std::vector<int32_t> ffix((size_t)nblk * 6, -1);
+  std::vector<int4> ftask[3];
+  int nfix = 0;
+  for (int b = 0; b < nblk; b++) {
+    const int L = meta[4 * b];
+    const long o[3] = {meta[4 * b + 1], meta[4 * b + 2], meta[4 * b + 3]};
+    for (int n = 0; n < nd; n++)
+      for (int sd = 0; sd < 2; sd++) {
+        long cc[3] = {o[0], o[1], o[2]};
+        cc[n] = sd == 0 ? o[n] - 1 : o[n] + nc;
+        for (int d = 0; d < nd; d++) cc[d] = wrapl(cc[d], nb_at(L, d) * nc);
+        if (leaf(L, cc[0], cc[1], cc[2]) >= 0) continue;
+        if (leaf(L + 1, 2 * cc[0], 2 * cc[1], 2 * cc[2]) < 0) continue;   // coarser
+        const int fs = nfix++;
+        ffix[(size_t)b * 6 + 2 * n + sd] = fs;
+        const int t1 = n == 0 ? 1 : 0, t2 = n == 2 ? 1 : 2;
+        for (int c2 = 0; c2 < (nd == 3 ? nc : 1); c2++)
+          for (int c1 = 0; c1 < nc; c1++) {
+            long e[3] = {0, 0, 0};
+            e[n] = sd == 0 ? 2 * cc[n] + 1 : 2 * cc[n];
+            e[t1] = 2 * (o[t1] + c1);
+            if (nd == 3) e[t2] = 2 * (o[t2] + c2);
+            const int F = leaf(L + 1, e[0], e[1], e[2]);
+            if (F < 0) return fail(700, "mesh is not 2:1 balanced");
+            const int f1 = (int)(e[t1] - meta[4 * F + 1 + t1]);
+            const int f2 = nd == 3 ? (int)(e[t2] - meta[4 * F + 1 + t2]) : 0;
+            ftask[n].push_back(make_int4(fs, c1 + nc * c2, F,
+                                         f1 | (f2 << 8) | ((sd == 0 ? nc : 0) << 16)));
+          }
+      }
+  }
+
+  int ndev = 0;
+  HIP_TRY(hipGetDeviceCount(&ndev));
+  if (ndev < 1) return fail(1000 + (int)hipErrorNoDevice, "no HIP device");
+  amun_ctx* c = new amun_ctx();
+  c->cfg = *cfg;
+  c->cfg.nv = nv;
+  c->nv = nv;
+  c->grids.assign(1, G);
+  RunConst& rc = c->rc;
+  rc = make_rc(*cfg);
+  rc.nruns = 1;
+  rc.nblk = nblk;
+  c->run_b0 = {0, nblk};
+  c->coords.assign((size_t)nblk * 3, 0);
+  for (int b = 0; b < nblk; b++)
+    for (int d = 0; d < 3; d++) c->coords[b * 3 + d] = meta[4 * b + 1 + d] / nc;
+  // cell sizes: level 1 in dxi, 2^(level-1) per block in the sweeps; dt from
+  // the finest level present (coordinates.F90:330-356, evolution.F90:467-476)
+  std::vector<RunState> hrs(1);
+  RunState& R = hrs[0];
+  std::memset(&R, 0, sizeof(R));
+  std::memcpy(&R.cmax_bits[0], &kTiny, 8);
+  std::memcpy(&R.cmax_bits[1], &kTiny, 8);
+  R.dt = 1.0;
+  const int ff = 1 << (nlev - 1);
+  const double adx1 = (G.xmax - G.xmin) / (G.xblocks * nc);
+  const double ady1 = (G.ymax - G.ymin) / (G.yblocks * nc);
+  const double adz1 = nd == 3 ? (G.zmax - G.zmin) / (G.zblocks * nc) : 1.0;
+  R.dxi[0] = 1.0 / adx1;
+  R.dxi[1] = 1.0 / ady1;
+  R.dxi[2] = nd == 3 ? 1.0 / adz1 : 1.0;
+  R.dxmin = std::fmin((G.xmax - G.xmin) / (G.xblocks * (nc * ff)),
+                      (G.ymax - G.ymin) / (G.yblocks * (nc * ff)));
+  if (nd == 3) R.dxmin = std::fmin(R.dxmin, (G.zmax - G.zmin) / (G.zblocks * (nc * ff)));
+  std::vector<int32_t> brun(nblk, 0), blev(nblk);
+  for (int b = 0; b < nblk; b++) blev[b] = meta[4 * b] - 1;
+  auto& A = c->amr;
+  A.on = true;
+  A.nreal = nblk;
+  A.nvirt = nvirt;
+  A.nlev = nlev;
+  A.nfix = nfix;
+  A.meta.assign(meta, meta + 4 * (size_t)nblk);
+  A.nr = (int)rtask.size();
+  A.np = (int)ptask.size();
+  for (int d = 0; d < 3; d++) A.nf[d] = (int)ftask[d].size();
+  if ((r = init_device(c, nbr, brun, hrs, nblk + nvirt, out))) return r;
+  // init_device has freed c on failure; from here c is live
+  hipError_t e = hipSuccess;
+  auto up = [&](void** dst, const void* src, size_t bytes) {
+    if (e != hipSuccess || !bytes) return;
+    e = hipMalloc(dst, bytes);
+    if (e == hipSuccess) e = hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice);
+  };
+  up((void**)&A.blev, blev.data(), blev.size() * sizeof(int32_t));
+  up((void**)&A.rtask, rtask.data(), rtask.size() * sizeof(int4));
+  up((void**)&A.ptask, ptask.data(), ptask.size() * sizeof(int4));
+  for (int d = 0; d < 3; d++)
+    up((void**)&A.ftask[d], ftask[d].data(), ftask[d].size() * sizeof(int4));
+  up((void**)&A.ffix, ffix.data(), ffix.size() * sizeof(int32_t));
+  if (e == hipSuccess)
+    e = hipMalloc((void**)&A.FB, (size_t)std::max(nfix, 1) * nv * (nd == 3 ? nc * nc : nc) *
+                                     sizeof(double));
+  if (e == hipSuccess) e = hipMalloc((void**)&A.dummy, sizeof(DevState));
+  if (e == hipSuccess) e = hipMemset(A.dummy, 0, sizeof(DevState));
+  if (e != hipSuccess) {
+    amun_dev_finalize(c);
+    *out = nullptr;
+    return fail(1000 + (int)e, "multi-level mesh tables");
+  }
+  return 0;
+}
+
+// blocks and virtual blocks of a multi-level mesh, its correction slots
+int amun_dev_amr_info(amun_ctx* c, int32_t* info) {
+  if (!c || !info) return fail(500, "null argument");
+  info[0] = c->amr.on ? c->amr.nreal : c->rc.nblk;
+  info[1] = c->amr.nvirt;
+  info[2] = c->amr.nlev;
+  info[3] = c->amr.nr;
+  info[4] = c->amr.np;
+  info[5] = c->amr.nfix;
+  return 0;
+}
+
 static void drop_graphs(amun_ctx* c) {
   for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.exec);
   c->graphs.clear();
@@ -424,6 +731,13 @@ void amun_dev_finalize(amun_ctx* c) {
   hipFree(c->rs);
   hipFree(c->trace);
   hipFree(c->scratch);
+  hipFree(c->amr.blev);
+  hipFree(c->amr.rtask);
+  hipFree(c->amr.ptask);
+  for (int d = 0; d < 3; d++) hipFree(c->amr.ftask[d]);
+  hipFree(c->amr.ffix);
+  hipFree(c->amr.FB);
+  hipFree(c->amr.dummy);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
 }
@@ -493,8 +807,66 @@ static void gather_run(const amun_ctx* c, int r, const double* h, double* u, int
   }
 }
 
+// Multi-level meshes: interiors of u from U0, q with its ghost layers
+// (virtual blocks filled from the current state), and the ghost layers of u
+// as prim2cons of q's (update_ghost_cells, boundaries.F90:6633-6736).
+static int download_blocks_amr(amun_ctx* c, double* u, double* q) {
+  const RunConst& rc = c->rc;
+  const int ng = c->cfg.nghosts, nc = rc.nc, nv = c->nv, nd = rc.nd;
+  const int im = nc + 2 * ng, km = (nd == 3) ? im : 1, kg = (nd == 3) ? ng : 0;
+  const size_t per = (size_t)im * im * km;
+  int r = amr_fill(c, c->qcur, (int)(c->nsteps & 1));
+  if (r) return r;
+  const size_t n = (size_t)rc.nblk * per * nv;
+  std::vector<double> qh(n);
+  double* buf = nullptr;
+  HIP_TRY(hipMalloc((void**)&buf, n * sizeof(double)));
+  hipLaunchKernelGGL(c->disp.pack_q, dim3(2048), dim3(256), 0, c->stream,
+                     (const void*)c->Q[c->qcur], buf, c->nbr, rc, ng);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e == hipSuccess) e = hipMemcpy(qh.data(), buf, n * sizeof(double), hipMemcpyDeviceToHost);
+  hipFree(buf);
+  if (e != hipSuccess) return fail(1000 + (int)e, "download_blocks");
+  std::vector<double> h(c->nstate);
+  HIP_TRY(hipMemcpy(h.data(), c->U0, c->nstate * sizeof(double), hipMemcpyDeviceToHost));
+  const bool mhd = nv == 9;
+  const double g1 = c->cfg.gamma - 1.0;
+  for (int b = 0; b < rc.nblk; b++)
+    for (int k = 0; k < km; k++)
+      for (int j = 0; j < im; j++)
+        for (int i = 0; i < im; i++) {
+          const size_t cell = (size_t)b * per + ((size_t)k * im + j) * im + i;
+          double* uo = u + cell * nv;
+          const double* qq = qh.data() + cell * nv;
+          const int li = i - ng, lj = j - ng, lk = k - kg;
+          if (li >= 0 && li < nc && lj >= 0 && lj < nc && lk >= 0 && lk < (nd == 3 ? nc : 1)) {
+            for (int v = 0; v < nv; v++)
+              uo[v] = h[((size_t)b * nv + v) * rc.ncell + lidx<8>(nc, nd, li, lj, lk)];
+          } else {
+            // prim2cons_hd_adi / prim2cons_mhd_adi (equations.F90:1762, 2815)
+            uo[0] = qq[0];
+            uo[1] = qq[0] * qq[1];
+            uo[2] = qq[0] * qq[2];
+            uo[3] = qq[0] * qq[3];
+            const double ek = 0.5 * ((uo[1] * qq[1] + uo[2] * qq[2]) + uo[3] * qq[3]);
+            if (mhd) {
+              for (int v = 4; v < 8; v++) uo[v] = qq[v];
+              const double em = 0.5 * ((qq[4] * qq[4] + qq[5] * qq[5]) + qq[6] * qq[6]);
+              uo[8] = (qq[8] / g1 + ek) + em;
+            } else {
+              uo[4] = qq[4] / g1 + ek;
+            }
+          }
+        }
+  if (q) std::memcpy(q, qh.data(), n * sizeof(double));
+  return 0;
+}
+
 int amun_dev_upload_run(amun_ctx* c, int32_t r, const double* u) {
   if (!c || !u) return fail(500, "null argument");
+  if (c->amr.on)
+    return fail(600, "multi-level meshes transfer block arrays (amun_dev_upload_blocks)");
   if (r < 0 || r >= c->rc.nruns) return fail(500, "run index out of range");
   const size_t per = (size_t)c->nv * c->rc.ncell;
   const size_t n = (size_t)(c->run_b0[r + 1] - c->run_b0[r]) * per;
@@ -534,6 +906,8 @@ int amun_dev_upload_blocks(amun_ctx* c, const double* u) {
 
 int amun_dev_download_run(amun_ctx* c, int32_t r, double* u, double* q) {
   if (!c || !u) return fail(500, "null argument");
+  if (c->amr.on)
+    return fail(600, "multi-level meshes transfer block arrays (amun_dev_download_blocks)");
   if (r < 0 || r >= c->rc.nruns) return fail(500, "run index out of range");
   HIP_TRY(hipStreamSynchronize(c->stream));
   const size_t per = (size_t)c->nv * c->rc.ncell;
@@ -566,6 +940,7 @@ int amun_dev_download_global(amun_ctx* c, double* u, double* q) {
 
 int amun_dev_download_blocks(amun_ctx* c, double* u, double* q) {
   if (!c || !u) return fail(500, "null argument");
+  if (c->amr.on) return download_blocks_amr(c, u, q);
   Dispatch& d = dispatch_of(c);
   const int ng = c->cfg.nghosts, nc = c->rc.nc;
   const int im = nc + 2 * ng, km = (c->rc.nd == 3) ? im : 1;
@@ -650,6 +1025,38 @@ static int kernels_per_step(const amun_ctx* c) {
 // Enqueue one whole step for cmax slot `slot` with the primitive state in
 // Q[qcur]; host-side state is left alone.  With ev != nullptr an event is
 // recorded after every kernel.
+// Multi-level meshes, before the sweeps of a stage: the virtual blocks of the
+// stage input Q[qcur] (restrictions, then prolongations), then the averaged
+// fine fluxes of the coarse faces with finer neighbours.
+static int amr_fill(amun_ctx* c, int qcur, int slot) {
+  auto& A = c->amr;
+  const dim3 bl(256);
+  if (c->qsize == sizeof(double)) {
+    double* Q = static_cast<double*>(c->Q[qcur]);
+    if (A.nr)
+      hipLaunchKernelGGL(k_amr_restrict<double>, dim3((A.nr + 255) / 256), bl, 0, c->stream,
+                         A.rtask, A.nr, Q, c->rc, c->nv);
+    if (A.np)
+      hipLaunchKernelGGL(k_amr_prolong<double>, dim3((A.np + 255) / 256), bl, 0, c->stream,
+                         A.ptask, A.np, Q, c->nbr, c->rc, c->nv, c->nv - 1, c->st);
+  } else {
+    float* Q = static_cast<float*>(c->Q[qcur]);
+    if (A.nr)
+      hipLaunchKernelGGL(k_amr_restrict<float>, dim3((A.nr + 255) / 256), bl, 0, c->stream,
+                         A.rtask, A.nr, Q, c->rc, c->nv);
+    if (A.np)
+      hipLaunchKernelGGL(k_amr_prolong<float>, dim3((A.np + 255) / 256), bl, 0, c->stream,
+                         A.ptask, A.np, Q, c->nbr, c->rc, c->nv, c->nv - 1, c->st);
+  }
+  for (int d = 0; d < c->rc.nd; d++)
+    if (A.nf[d])
+      hipLaunchKernelGGL(c->disp.fineflux[d], dim3((A.nf[d] + 255) / 256), bl, 0, c->stream,
+                         A.ftask[d], A.nf[d], (const void*)c->Q[qcur], c->nbr, c->rc, c->rs,
+                         slot, A.FB, A.dummy);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
 static int enqueue_step(amun_ctx* c, double dtnext, int slot, int qcur, hipEvent_t* ev,
                         double* rec = nullptr) {
   Dispatch& d = dispatch_of(c);
@@ -662,7 +1069,14 @@ static int enqueue_step(amun_ctx* c, double dtnext, int slot, int qcur, hipEvent
     HIP_TRY(hipMemcpyAsync(rec, c->U0, c->nstate * sizeof(double), hipMemcpyDeviceToDevice,
                            c->stream));
   for (size_t s = 0; s < stages.size(); s++) {
+    if (c->amr.on) {
+      int r = amr_fill(c, qcur, slot);
+      if (r) return r;
+    }
     SweepArgs a;
+    a.blev = c->amr.on ? c->amr.blev : nullptr;
+    a.ffix = c->amr.on ? c->amr.ffix : nullptr;
+    a.FB = c->amr.on ? c->amr.FB : nullptr;
     a.Q = c->Q[qcur];
     a.U0 = c->U0;
     a.U1 = c->U1;
@@ -959,7 +1373,7 @@ static int integrals_range(amun_ctx* c, int r0, int nr, double out[8]) {
   for (int g = r0; g < r0 + nr; g++) {
     const int b0 = c->run_b0[g], nb = c->run_b0[g + 1] - b0;
     hipLaunchKernelGGL(d.integ, dim3(nwg), dim3(256), 0, c->stream, c->U0, c->rc, c->scratch,
-                       b0, nb);
+                       b0, nb, c->amr.on ? (const int*)c->amr.blev : nullptr);
     HIP_TRY(hipGetLastError());
     std::vector<double> part((size_t)nwg * 8);
     HIP_TRY(hipMemcpyAsync(part.data(), c->scratch, part.size() * sizeof(double),
@@ -1063,6 +1477,7 @@ void* amun_dev_stream(amun_ctx* c) { return c ? (void*)c->stream : nullptr; }
 // ---------------------------------------------------------------------------
 static int tape_check(amun_ctx* c) {
   if (c->rc.nruns != 1) return fail(600, "gradients need a single-run context");
+  if (c->amr.on) return fail(600, "gradients are not supported on multi-level meshes");
   if (!c->disp.has_adj)
     return fail(600, "gradients need FP64, TVD reconstruction and the HD or MHD system");
   return 0;

@@ -90,6 +90,7 @@ struct RunConst {
   int nd, nc, nblk, ncell;
   int nruns;                        // independent grids in the block pool
   int limiter;
+  int prol_limiter;                 // prolongation of multi-level ghost fills
   double gamma, gammam1, gammam1i;
   double cfl, eps, decay;
   double cgp[5];                    // Gaussian-process weights (prepare_gp)
@@ -213,6 +214,13 @@ struct SweepArgs {
   double fac;                      // ds = fac * dt
   double ca, cb;                   // state weights of combine kinds 5-7
   double dtnext;
+  // multi-level meshes (null on single-level grids): level - 1 of every
+  // block (cell sizes scale by 2^-(level-1)), the flux-correction slot of
+  // every block face (block * 6 + 2 dir + side, -1 if none) and the
+  // averaged fine fluxes of the slots
+  const int* __restrict__ blev;
+  const int* __restrict__ ffix;
+  const double* __restrict__ FB;
 };
 
 // stage combines of evolve_euler/rk2/rk3 (evolution.F90:570, 700, 740, 1080,
@@ -555,6 +563,16 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a, RunConst rc) {
         riemann<P, RS, T>(e, rc.sc, cmax, cmax2, ql, qr, fr);
 #pragma unroll
         for (int p = 0; p < NV; p++) f[rot<P, DIR>(p)] = fr[p];
+        if (a.ffix && (c == 0 || c == nc)) {
+          // a face with finer neighbours: their averaged fluxes
+          // (boundary_fluxes, boundaries.F90:494-1060)
+          const int fs = a.ffix[b * 6 + 2 * DIR + (c == nc)];
+          if (fs >= 0) {
+            const int nf = (ND == 3) ? nc * nc : nc;
+#pragma unroll
+            for (int p = 0; p < NV; p++) f[p] = T(a.FB[((size_t)fs * NV + p) * nf + t]);
+          }
+        }
         if (c >= 1) {
           // cell c-1 has both faces
           int i, j, k;
@@ -563,7 +581,7 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a, RunConst rc) {
           const size_t bt = bb + (size_t)lidx<sizeof(T)>(nc, ND, i, j, k);
           const size_t bd = bb + (size_t)lidx<8>(nc, ND, i, j, k);
           const size_t bf = bb + (size_t)lidx<4>(nc, ND, i, j, k);
-          const T dxi = T(R->dxi[DIR]);
+          const T dxi = T(a.blev ? R->dxi[DIR] * double(1 << a.blev[b]) : R->dxi[DIR]);
           double unew[NV];
 #pragma unroll
           for (int v = 0; v < NV; v++) {
@@ -725,7 +743,7 @@ static __global__ void k_peek_dt(RunState* rs, RunConst rc, int slot, double dtn
 template <class P>
 __global__ __launch_bounds__(256) void k_integrals(const double* __restrict__ U,
                                                    RunConst rc, double* part, int b0,
-                                                   int nb) {
+                                                   int nb, const int* __restrict__ blev) {
   constexpr int NV = P::nv;
   __shared__ double sh[8][256];
   U += (size_t)b0 * NV * rc.ncell;
@@ -734,9 +752,11 @@ __global__ __launch_bounds__(256) void k_integrals(const double* __restrict__ U,
   for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < n;
        g += (long long)gridDim.x * blockDim.x) {
     long long b = g / rc.ncell, c = g % rc.ncell;
+    // cells of level l hold 2^(-nd (l - 1)) of a base-level cell's volume
+    const double w = blev ? ldexp(1.0, -rc.nd * blev[b0 + b]) : 1.0;
     double u[NV];
 #pragma unroll
-    for (int v = 0; v < NV; v++) u[v] = U[(b * NV + v) * rc.ncell + c];
+    for (int v = 0; v < NV; v++) u[v] = w * U[(b * NV + v) * rc.ncell + c];
     acc[0] += u[P::idn];
     acc[1] += u[P::imx];
     acc[2] += u[P::imy];
@@ -861,6 +881,7 @@ typedef void (*SweepFn)(SweepArgs, RunConst);
 }  // namespace amun_dev
 
 #include "adjoint.h"
+#include "amr.h"
 
 using namespace amun_dev;
 
@@ -869,7 +890,7 @@ struct Dispatch {
   SweepFn sweep[3] = {nullptr, nullptr, nullptr};
   void (*prim)(const double*, void*, RunConst, DevState*, RunState*, const int*, int, int,
                int) = nullptr;
-  void (*integ)(const double*, RunConst, double*, int, int) = nullptr;
+  void (*integ)(const double*, RunConst, double*, int, int, const int*) = nullptr;
   void (*pack_u)(const void*, double*, const int*, RunConst, int) = nullptr;
   void (*pack_q)(const void*, double*, const int*, RunConst, int) = nullptr;
   void (*unit_riemann)(int, double, const double*, const double*, double*, RunConst) = nullptr;
@@ -887,6 +908,9 @@ struct Dispatch {
                    const double*, double*, double*, const int*, RunConst) = nullptr;
   void (*tlm_cell)(const double*, const double*, const double*, double*, const int*, RunConst,
                    double, double, double) = nullptr;
+  // multi-level meshes: averaged fine fluxes of coarse faces, per direction
+  void (*fineflux[3])(const int4*, int, const void*, const int*, RunConst, const RunState*, int,
+                      double*, DevState*) = {nullptr, nullptr, nullptr};
 };
 
 
@@ -903,6 +927,9 @@ void fill(Dispatch& d) {
   d.pack_q = k_pack_blocks<P::nv, T>;
   d.unit_riemann = k_unit_riemann<P, RS, T>;
   d.unit_eqs = k_unit_eqs<P, T>;
+  d.fineflux[0] = k_amr_fineflux<P, REC, RS, ND, 0, T>;
+  d.fineflux[1] = k_amr_fineflux<P, REC, RS, ND, 1, T>;
+  if constexpr (ND == 3) d.fineflux[2] = k_amr_fineflux<P, REC, RS, ND, 2, T>;
   if constexpr (REC == REC_TVD && !P::sr && std::is_same<T, double>::value) {
     d.has_adj = true;
     d.adj_prim = k_adj_prim<P>;

@@ -42,6 +42,7 @@ module amun_dev_mod
     integer(c_int32_t) :: nr_maxit, nr_extra
     real(c_double)     :: sgp, kappa, kbeta
     integer(c_int32_t) :: ngp, stages
+    integer(c_int32_t) :: prol_limiter, reserved2
   end type amun_cfg
 
   interface
@@ -55,6 +56,17 @@ module amun_dev_mod
       import :: amun_cfg, c_int, c_ptr
       type(amun_cfg), intent(in) :: cfg
       type(c_ptr), intent(out)   :: ctx
+    end function
+
+    ! a frozen multi-level mesh: meta(4, nblk) = level, i0, j0, k0 per leaf
+    ! (the mesh module's leaf list after generate_mesh / update_mesh)
+    integer(c_int) function amun_dev_init_amr(cfg, nblk, meta, ctx) &
+                                               bind(C, name="amun_dev_init_amr")
+      import :: amun_cfg, c_int, c_int32_t, c_ptr
+      type(amun_cfg), intent(in)        :: cfg
+      integer(c_int32_t), value         :: nblk
+      integer(c_int32_t), intent(in)    :: meta(4, *)
+      type(c_ptr), intent(out)          :: ctx
     end function
 
     subroutine amun_dev_finalize(ctx) bind(C, name="amun_dev_finalize")

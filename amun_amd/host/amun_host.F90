@@ -92,6 +92,19 @@ program amun_host
   case default
     cfg%limiter = AMUN_LIM_ZERO
   end select
+  str = get_str("prolongation_limiter", "mm")
+  select case(trim(str))
+  case("mm", "minmod")
+    cfg%prol_limiter = AMUN_LIM_MINMOD
+  case("mc", "monotonized_central")
+    cfg%prol_limiter = AMUN_LIM_MC
+  case("sb", "superbee")
+    cfg%prol_limiter = AMUN_LIM_SUPERBEE
+  case("vl", "vanleer")
+    cfg%prol_limiter = AMUN_LIM_VANLEER
+  case default
+    cfg%prol_limiter = AMUN_LIM_ZERO
+  end select
   str = get_str("riemann_solver", "hlld")
   select case(trim(str))
   case("hll", "HLL")

@@ -33,6 +33,7 @@ EXPORTS = (
     "amun_dev_autotune", "amun_dev_set_launch", "amun_dev_unit_riemann",
     "amun_dev_unit_eqs", "amun_dev_tape_forward", "amun_dev_tape_steps",
     "amun_dev_tape_backward", "amun_dev_tape_tangent", "amun_dev_run_fixed",
+    "amun_dev_init_amr", "amun_dev_amr_info",
 )
 
 
@@ -101,6 +102,9 @@ def load_library(path: str = LIB_PATH):
                                         ctypes.c_double, _P, _P, _P]
     L.amun_dev_unit_eqs.argtypes = [ctypes.POINTER(AmunCfg), ctypes.c_int32, ctypes.c_double,
                                     _P, _P, _P, _P, _P, _P]
+    L.amun_dev_init_amr.argtypes = [ctypes.POINTER(AmunCfg), ctypes.c_int32,
+                                    ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(vp)]
+    L.amun_dev_amr_info.argtypes = [vp, ctypes.POINTER(ctypes.c_int32)]
     L.amun_dev_stream.restype = ctypes.c_void_p
     L.amun_dev_last_error.argtypes = []
     L.amun_dev_last_error.restype = ctypes.c_char_p
@@ -319,6 +323,64 @@ class Solver:
         n = (ctypes.c_uint64 * 3)()
         self._check(self.lib.amun_dev_c2p_events(self.h, n))
         return list(n)
+
+
+class AmrSolver(Solver):
+    """A multi-level block mesh (amun_dev_init_amr), frozen: ``meta`` holds
+    (level, i0, j0, k0) per leaf block, the first interior cell in cells of
+    the block's level.  State moves as per-block interiors in meta order,
+    u[b][k][j][i][v] (the Fortran u(nv, ib:ie, jb:je, kb:ke) of every block,
+    one after another)."""
+
+    def __init__(self, params: Params, meta: np.ndarray):
+        self.lib = load_library()
+        self.params = params
+        self.cfg = params.to_cfg()
+        self.meta = np.ascontiguousarray(meta, dtype=np.int32)
+        h = ctypes.c_void_p()
+        self._check(self.lib.amun_dev_init_amr(
+            ctypes.byref(self.cfg), self.meta.shape[0],
+            self.meta.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), ctypes.byref(h)))
+        self.h = h
+
+    def info(self) -> dict:
+        a = np.zeros(6, dtype=np.int32)
+        self._check(self.lib.amun_dev_amr_info(self.h, a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+        keys = ("blocks", "virtual_blocks", "levels", "restricted", "prolonged", "corrected_faces")
+        return dict(zip(keys, (int(x) for x in a)))
+
+    def _block_shape(self):
+        p = self.params
+        im = p.ncells + 2 * p.nghosts
+        km = im if p.ndims == 3 else 1
+        return (self.meta.shape[0], km, im, im, p.nv)
+
+    def _interior(self, a):
+        p = self.params
+        g = p.nghosts
+        if p.ndims == 2:
+            return a[:, 0, g:g + p.ncells, g:g + p.ncells, :]
+        return a[:, g:g + p.ncells, g:g + p.ncells, g:g + p.ncells, :]
+
+    def upload(self, u: np.ndarray):
+        full = np.zeros(self._block_shape())
+        inner = self._interior(full)
+        inner[...] = np.asarray(u, dtype=np.float64).reshape(inner.shape)
+        self.upload_blocks(full)
+
+    def download(self, with_q: bool = False):
+        """Interiors of u (and q with its ghost layers, as after
+        boundary_variables, if with_q)."""
+        if with_q:
+            u, q = self.download_blocks(with_q=True)
+        else:
+            u, q = self.download_blocks(), None
+        ui = np.ascontiguousarray(self._interior(u))
+        if self.params.ndims == 2:
+            ui = ui.reshape(ui.shape[0], self.params.ncells, self.params.ncells, -1)
+            if with_q:
+                q = q[:, 0]
+        return (ui, q) if with_q else ui
 
 
 class Batch(Solver):

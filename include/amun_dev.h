@@ -104,6 +104,11 @@ typedef struct amun_cfg {
   double  kbeta;            /* "kbeta" (1)                                    */
   int32_t ngp;              /* "ngp": stencil points (5)                      */
   int32_t stages;           /* "stages" of SSPRK(m,2), clamped to [2, 9] (2)  */
+  /* limiter of the coarse-to-fine prolongation in the ghost fill of
+   * multi-level meshes ("prolongation_limiter", interpolations.F90:165, 203,
+   * 412-428): AMUN_LIM_MINMOD, _MC, _SUPERBEE, _VANLEER or _ZERO (minmod) */
+  int32_t prol_limiter;
+  int32_t reserved2;
 } amun_cfg;
 
 typedef struct amun_ctx amun_ctx;
@@ -135,6 +140,27 @@ int  amun_dev_init(const amun_cfg* cfg, amun_ctx** ctx);
  * process (driver.F90:157-657). */
 int  amun_dev_init_batch(const amun_cfg* cfg, int32_t nruns, const amun_grid* grids,
                          amun_ctx** ctx);
+
+/* A multi-level (AMR) mesh of nblk leaf blocks on the periodic base grid of
+ * cfg: meta[4 b] = level (1 = base), meta[4 b + 1..3] = first interior cell
+ * (i0, j0, k0) of block b in cells of its level.  Replaces the mesh state
+ * the reference's block lists and neighbour pointers hold after
+ * generate_mesh / update_mesh (mesh.F90:431-740, blocks.F90) with the mesh
+ * frozen, and makes every step do boundary_variables with restriction and
+ * prolongation (boundaries.F90:372-479, 5637-6614), the flux correction of
+ * boundary_fluxes (494-1060), per-level cell sizes in update_increment
+ * (evolution.F90:1898-1977) and the finest-level dx in new_time_step
+ * (evolution.F90:379-493).  The mesh must be 2:1 balanced (700/701
+ * otherwise); nghosts must be even and at most ncells / 2; HD and MHD only.
+ * State moves with amun_dev_upload_blocks / amun_dev_download_blocks in
+ * meta order.  No gradients, no batching on such a context. */
+int  amun_dev_init_amr(const amun_cfg* cfg, int32_t nblk, const int32_t* meta,
+                       amun_ctx** ctx);
+
+/* info[0..5]: real blocks, virtual blocks (same-level stand-ins of the
+ * ghost regions at other levels), levels, restricted and prolonged ghost
+ * cells per stage, coarse faces with flux correction. */
+int  amun_dev_amr_info(amun_ctx* ctx, int32_t* info);
 
 /* Number of runs of the context, and the first pool block and block count
  * of run r (its blocks in amun_dev_block_coords / *_blocks order). */

@@ -31,7 +31,13 @@
  *   - same-level ghost exchange of q (boundaries.F90:372-479 with the face,
  *     edge and corner copies 1273, 2447, 3867) on the periodic base-level
  *     block grid of setup_domain_default (domains.F90:136-500), followed by
- *     update_ghost_cells (boundaries.F90:6633).
+ *     update_ghost_cells (boundaries.F90:6633);
+ *   - on multi-level meshes: restriction (block_face/edge/corner_restrict,
+ *     boundaries.F90:5637, 5919, 6256) and limited prolongation
+ *     (block_face/edge/corner_prolong, 5762, 6071, 6351) of ghost cells, the
+ *     coarse-face flux correction (boundary_fluxes 494-1060 with
+ *     block_update_flux 6539-6614), per-level cell sizes
+ *     (coordinates.F90:330-356) and the finest-level CFL step.
  *
  * Pins (see tests/): golden vectors written by the reference binaries
  * (tests/golden/), round trips, consistency F(u,u)=F(u), exact discrete
@@ -64,6 +70,7 @@ typedef struct {
   double adx, ady, adz, adxi, adyi, adzi;
   double cmax, cmax2;      /* equations module state                         */
   long   npos_events;      /* "Positive variable is not positive" prints     */
+  int    amr_err;          /* multi-level mesh the ghost fill cannot serve   */
   long   c2p_fail, c2p_pneg, c2p_noconv;   /* SR primitive solver messages */
   double kappa;            /* min(kappa, (1 - cfl)/cfl), interpolations.F90:217 */
   int    mgp;              /* (ngp - 1) / 2                                  */
@@ -1614,12 +1621,21 @@ static void update_flux(ocfg* o, const double* q, double* f, double* qs)
 /* ------------------------------------------------------------------------ */
 /* block grid with ghost exchange                                             */
 /* ------------------------------------------------------------------------ */
+#define OMAXLEV 12
 typedef struct {
   ocfg o;
   int ir, jr, kr, nblk;
   double *u0, *u1, *q, *f, *du, *qs;    /* u0,u1,q: nblk blocks */
   double time, dt, dtn;
   long step;
+  /* multi-level meshes (leaf blocks only) */
+  int amr, nlev;                         /* levels 1..nlev may hold blocks  */
+  int *lev, *bx, *by, *bz;               /* level, position in blocks there */
+  int *lut[OMAXLEV + 1];                 /* position -> block id or -1      */
+  double lvd[OMAXLEV + 1][6];            /* adx, ady, adz, adxi, adyi, adzi */
+  unsigned char* gflag;                  /* ghost cell filled by prolongation */
+  double* fall;                          /* fluxes of every block           */
+  int skip_fix;                          /* tests: leave the coarse fluxes  */
 } ogrid;
 
 static size_t bsize(const ocfg* o) { return (size_t)o->im * o->jm * o->km * o->nv; }
@@ -1663,6 +1679,238 @@ static void boundary_variables(ogrid* g, double* u)
   }
 }
 
+/* ---- multi-level meshes ----------------------------------------------- */
+/* Every ghost cell takes the value the reference's boundary_variables leaves
+ * there (boundaries.F90:372-479).  Under the 2:1 level balance of AMUN's
+ * meshes the region a ghost cell lies in is covered by exactly one leaf, and
+ * each of the reference's copy, restriction and prolongation regions fills
+ * the cells of its neighbour's coverage:
+ *   same level  -> copy of the neighbour's cell (boundaries_*_copy);
+ *   finer leaf  -> average of the 2^d covering cells (block_*_restrict,
+ *                  boundaries.F90:5637, 5919, 6256; the summation order is
+ *                  theirs);
+ *   coarser leaf-> the parent cell plus limited slopes (block_*_prolong,
+ *                  5762, 6071, 6351) from the parent's neighbours, which the
+ *                  copies and restrictions have filled before.
+ * The regions overlap in places (the edge prolongation covers corner cells,
+ * boundaries.F90:3445-3470 via edges_gp, coordinates.F90:428-436); every
+ * overlapping write of a cell carries the same value, so the order of the
+ * reference's passes matters only between restriction and prolongation. */
+static long wrapc(long x, long n) { x %= n; return x < 0 ? x + n : x; }
+
+static long lev_cells(const ogrid* g, int L, int d)
+{
+  const int r = d == 0 ? g->ir : (d == 1 ? g->jr : g->kr);
+  return (long)r * g->o.nc << (L - 1);
+}
+
+/* leaf block covering cell (x, y, z) of level L (already wrapped), or -1 */
+static int owner(const ogrid* g, int L, long x, long y, long z)
+{
+  const int nc = g->o.nc;
+  const long nbx = (long)g->ir << (L - 1), nby = (long)g->jr << (L - 1);
+  if (L < 1 || L > g->nlev) return -1;
+  return g->lut[L][((z / nc) * nby + y / nc) * nbx + x / nc];
+}
+
+/* padded local index of global cell (x, y, z) in block b */
+static size_t local_cell(const ogrid* g, int b, long x, long y, long z)
+{
+  const ocfg* o = &g->o;
+  const int i = (int)(x - (long)g->bx[b] * o->nc) + o->ng;
+  const int j = (int)(y - (long)g->by[b] * o->nc) + o->ng;
+  const int k = (o->nd == 3) ? (int)(z - (long)g->bz[b] * o->nc) + o->ng : 0;
+  return QI(o, 0, i, j, k);
+}
+
+static int boundary_variables_amr(ogrid* g, double* u)
+{
+  ocfg* o = &g->o;
+  const size_t bs = bsize(o), ncp = (size_t)o->im * o->jm * o->km;
+  const int nv = o->nv, nd = o->nd;
+  int b, i, j, k, p, pass;
+  for (pass = 0; pass < 2; pass++)
+    for (b = 0; b < g->nblk; b++) {
+      const int L = g->lev[b];
+      const long nx = lev_cells(g, L, 0), ny = lev_cells(g, L, 1);
+      const long nz = nd == 3 ? lev_cells(g, L, 2) : 1;
+      double* qb = g->q + b * bs;
+      for (k = 0; k < o->km; k++)
+        for (j = 0; j < o->jm; j++)
+          for (i = 0; i < o->im; i++) {
+            const int gi = i - o->ng, gj = j - o->ng, gk = nd == 3 ? k - o->ng : 0;
+            long x, y, z;
+            int nb;
+            const size_t lc = QI(o, 0, i, j, k);
+            if (gi >= 0 && gi < o->nc && gj >= 0 && gj < o->nc && gk >= 0 && gk < o->nc)
+              continue;
+            x = wrapc((long)g->bx[b] * o->nc + gi, nx);
+            y = wrapc((long)g->by[b] * o->nc + gj, ny);
+            z = nd == 3 ? wrapc((long)g->bz[b] * o->nc + gk, nz) : 0;
+            if (pass == 0) {
+              g->gflag[b * ncp + lc / nv] = 0;
+              if ((nb = owner(g, L, x, y, z)) >= 0) {
+                memcpy(&qb[lc], &g->q[nb * bs + local_cell(g, nb, x, y, z)],
+                       nv * sizeof(double));
+              } else if ((nb = owner(g, L + 1, 2 * x, 2 * y, 2 * z)) >= 0) {
+                /* restriction: il/ip, jl/jp, kl/kp = the lower/upper child */
+                const double* F = g->q + nb * bs;
+                const size_t lll = local_cell(g, nb, 2 * x, 2 * y, 2 * z);
+                const size_t di = nv, dj = (size_t)o->im * nv, dk = (size_t)o->im * o->jm * nv;
+                for (p = 0; p < nv; p++) {
+                  const double* a = F + lll + p;
+                  if (nd == 2)
+                    qb[lc + p] = 2.50e-01 * ((a[0] + a[di + dj]) + (a[dj] + a[di]));
+                  else
+                    qb[lc + p] = 1.25e-01 * (((a[0] + a[di + dj + dk]) + (a[dk] + a[di + dj]))
+                                           + ((a[dj + dk] + a[di]) + (a[dj] + a[di + dk])));
+                }
+              } else if (owner(g, L - 1, x / 2, y / 2, z / 2) >= 0) {
+                g->gflag[b * ncp + lc / nv] = 1;
+              } else {
+                return 700;   /* hole in the mesh or level jump above one */
+              }
+            } else if (g->gflag[b * ncp + lc / nv]) {
+              /* prolongation from the parent cell of coarse leaf C */
+              const long cx = x / 2, cy = y / 2, cz = z / 2;
+              const int C = owner(g, L - 1, cx, cy, cz);
+              const double* Q = g->q + C * bs;
+              const size_t c0 = local_cell(g, C, cx, cy, cz);
+              const size_t st[3] = {(size_t)nv, (size_t)o->im * nv, (size_t)o->im * o->jm * nv};
+              const int sx = (int)(x & 1), sy = (int)(y & 1), sz = (int)(z & 1);
+              int d;
+              for (d = 0; d < nd; d++)
+                if (g->gflag[C * ncp + (c0 - st[d]) / nv] || g->gflag[C * ncp + (c0 + st[d]) / nv])
+                  return 701;  /* the slope would read a prolonged cell */
+              for (p = 0; p < nv; p++) {
+                const double qc = Q[c0 + p];
+                double dq[3] = {0.0, 0.0, 0.0}, v;
+                for (d = 0; d < nd; d++) {
+                  const double dql = qc - Q[c0 - st[d] + p], dqr = Q[c0 + st[d] + p] - qc;
+                  dq[d] = lim(o->c.prol_limiter, 0.5, dql, dqr);
+                }
+                if (p == o->idn || p == o->ipr) {
+                  if (qc > 0.0) {
+                    for (;;) {
+                      double sa = fabs(dq[0]) + fabs(dq[1]);
+                      if (nd == 3) sa = sa + fabs(dq[2]);
+                      if (!(qc <= sa)) break;
+                      for (d = 0; d < nd; d++) dq[d] = 0.5 * dq[d];
+                    }
+                  } else {
+                    o->npos_events++;
+                    dq[0] = dq[1] = dq[2] = 0.0;
+                  }
+                }
+                for (d = 0; d < nd; d++) dq[d] = 0.5 * dq[d];
+                if (nd == 2) {
+                  const double dq1 = dq[0] + dq[1], dq2 = dq[0] - dq[1];
+                  const int s = sx + 2 * sy;
+                  v = s == 0 ? qc - dq1 : (s == 1 ? qc + dq2 : (s == 2 ? qc - dq2 : qc + dq1));
+                } else {
+                  const double dq1 = dq[0] + dq[1] + dq[2], dq2 = dq[0] - dq[1] - dq[2];
+                  const double dq3 = dq[0] - dq[1] + dq[2], dq4 = dq[0] + dq[1] - dq[2];
+                  switch (sx + 2 * sy + 4 * sz) {
+                  case 0: v = qc - dq1; break;
+                  case 1: v = qc + dq2; break;
+                  case 2: v = qc - dq3; break;
+                  case 3: v = qc + dq4; break;
+                  case 4: v = qc - dq4; break;
+                  case 5: v = qc + dq3; break;
+                  case 6: v = qc - dq2; break;
+                  default: v = qc + dq1; break;
+                  }
+                }
+                qb[lc + p] = v;
+              }
+            }
+          }
+    }
+  /* update_ghost_cells: u of every ghost cell */
+  for (b = 0; b < g->nblk; b++)
+    for (k = 0; k < o->km; k++)
+      for (j = 0; j < o->jm; j++)
+        for (i = 0; i < o->im; i++) {
+          const int gi = i - o->ng, gj = j - o->ng, gk = nd == 3 ? k - o->ng : 0;
+          if (gi >= 0 && gi < o->nc && gj >= 0 && gj < o->nc && gk >= 0 && gk < o->nc)
+            continue;
+          prim2cons(o, &g->q[b * bs + QI(o, 0, i, j, k)], &u[b * bs + QI(o, 0, i, j, k)]);
+        }
+  return 0;
+}
+
+/* boundary_fluxes (boundaries.F90:494-1060) with block_update_flux
+ * (6539-6614): the flux through a coarse block's face with finer
+ * neighbours becomes the average of the fine fluxes through it, 2 (2D) or 4
+ * (3D) per coarse face, summed in the reference's order */
+static void boundary_fluxes(ogrid* g)
+{
+  ocfg* o = &g->o;
+  const size_t bs = bsize(o);
+  const int nd = o->nd, nv = o->nv, nc = o->nc;
+  int b, n, side, a1, a2, p;
+  for (b = 0; b < g->nblk; b++) {
+    const int L = g->lev[b];
+    long org[3];
+    if (L >= g->nlev) continue;
+    org[0] = (long)g->bx[b] * nc; org[1] = (long)g->by[b] * nc;
+    org[2] = nd == 3 ? (long)g->bz[b] * nc : 0;
+    for (n = 0; n < nd; n++)
+      for (side = 0; side < 2; side++)
+        for (a2 = 0; a2 < (nd == 3 ? nc : 1); a2++)
+          for (a1 = 0; a1 < nc; a1++) {
+            /* coarse cell across the face, and this block's face cell */
+            long c[3], fc[3];
+            int t1 = n == 0 ? 1 : 0, t2 = n == 2 ? 1 : 2, F, d;
+            size_t fo[4];
+            double* fb;
+            c[n] = side == 0 ? org[n] - 1 : org[n] + nc;
+            c[t1] = org[t1] + a1;
+            if (nd == 3) c[t2] = org[t2] + a2; else c[2] = 0;
+            for (d = 0; d < nd; d++) c[d] = wrapc(c[d], lev_cells(g, L, d));
+            if (owner(g, L, c[0], c[1], c[2]) >= 0) continue;
+            F = owner(g, L + 1, 2 * c[0], 2 * c[1], 2 * c[2]);
+            if (F < 0) continue;
+            /* the fine cells next to the face: the upper cell layer of a
+             * lower neighbour (flux at its ie), the lower layer of an upper
+             * neighbour (flux at ibl = ib - 1) */
+            for (d = 0; d < 3; d++) fc[d] = 2 * c[d];
+            if (side == 0) fc[n] = 2 * c[n] + 1;
+            for (d = 0; d < 4; d++) {
+              long e[3] = {fc[0], fc[1], fc[2]};
+              size_t cell;
+              e[t1] += d & 1;
+              if (nd == 3) e[t2] += d >> 1;
+              cell = local_cell(g, F, e[0], e[1], e[2]) / nv;
+              if (side == 1) cell -= (n == 0 ? 1 : (n == 1 ? o->im : (size_t)o->im * o->jm));
+              fo[d] = cell;
+              if (nd == 2 && d == 1) break;
+            }
+            {
+              /* this block's face: f at ibl (lower) or ie (upper) along n */
+              int ii[3] = {o->ng, o->ng, nd == 3 ? o->ng : 0};
+              size_t cb;
+              ii[n] = side == 0 ? o->ng - 1 : o->ng + nc - 1;
+              ii[t1] += a1;
+              if (nd == 3) ii[t2] += a2;
+              cb = QI(o, 0, ii[0], ii[1], ii[2]) / nv;
+              fb = g->fall + b * bs * nd;
+              for (p = 0; p < nv; p++) {
+                const double* ff = g->fall + F * bs * nd;
+                double v;
+#define FF(cell) ff[((cell) * nv + p) * nd + n]
+                if (nd == 2)
+                  v = (FF(fo[0]) + FF(fo[1])) / 2.0;
+                else
+                  v = ((FF(fo[0]) + FF(fo[3])) + (FF(fo[2]) + FF(fo[1]))) / 4.0;
+#undef FF
+                fb[(cb * nv + p) * nd + n] = v;
+              }
+            }
+          }
+  }
+}
+
 /* update_primitive_variables (equations.F90:1088-1148): interior cons2prim,
  * the ghost layers are then replaced by the exchange */
 static void update_variables(ogrid* g, double* u)
@@ -1675,11 +1923,17 @@ static void update_variables(ogrid* g, double* u)
       for (j = o->jb; j <= o->je; j++)
         for (i = o->ib; i <= o->ie; i++)
           cons2prim(o, &u[b * bs + QI(o, 0, i, j, k)], &g->q[b * bs + QI(o, 0, i, j, k)]);
-  boundary_variables(g, u);
+  if (g->amr) {
+    if (!g->o.amr_err) g->o.amr_err = boundary_variables_amr(g, u);
+  } else {
+    boundary_variables(g, u);
+  }
 }
 
-/* update_increment, evolution.F90:1898-1977 */
-static void update_increment(const ocfg* o, const double* f, double* du)
+/* update_increment, evolution.F90:1898-1977 (cell sizes of the block's
+ * level) */
+static void update_increment_dx(const ocfg* o, const double* f, double* du, double adxi,
+                                double adyi, double adzi)
 {
   int p, i, j, k;
   memset(du, 0, bsize(o) * sizeof(double));
@@ -1688,12 +1942,17 @@ static void update_increment(const ocfg* o, const double* f, double* du)
       for (i = o->ibl; i <= o->ieu; i++)
         for (p = 0; p < o->nv; p++) {
           double v;
-          v = -o->adxi * (f[FI(o, 0, p, i, j, k)] - f[FI(o, 0, p, i - 1, j, k)])
-              - o->adyi * (f[FI(o, 1, p, i, j, k)] - f[FI(o, 1, p, i, j - 1, k)]);
+          v = -adxi * (f[FI(o, 0, p, i, j, k)] - f[FI(o, 0, p, i - 1, j, k)])
+              - adyi * (f[FI(o, 1, p, i, j, k)] - f[FI(o, 1, p, i, j - 1, k)]);
           if (o->nd == 3)
-            v = v - o->adzi * (f[FI(o, 2, p, i, j, k)] - f[FI(o, 2, p, i, j, k - 1)]);
+            v = v - adzi * (f[FI(o, 2, p, i, j, k)] - f[FI(o, 2, p, i, j, k - 1)]);
           du[QI(o, p, i, j, k)] = v;
         }
+}
+
+static void update_increment(const ocfg* o, const double* f, double* du)
+{
+  update_increment_dx(o, f, du, o->adxi, o->adyi, o->adzi);
 }
 
 /* new_time_step, evolution.F90:379-493 (single level, no viscosity) */
@@ -1714,8 +1973,15 @@ static void new_time_step(ogrid* g, double dtnext)
     o->cmax = fmax(o->cmax, ms);
   }
   o->cmax2 = o->cmax * o->cmax;
-  dxmin = fmin(o->adx, o->ady);
-  if (o->nd == 3) dxmin = fmin(dxmin, o->adz);
+  if (g->amr) {                 /* dx_min of the highest level in use */
+    int mlev = 1;
+    for (b = 0; b < g->nblk; b++) mlev = g->lev[b] > mlev ? g->lev[b] : mlev;
+    dxmin = fmin(g->lvd[mlev][0], g->lvd[mlev][1]);
+    if (o->nd == 3) dxmin = fmin(dxmin, g->lvd[mlev][2]);
+  } else {
+    dxmin = fmin(o->adx, o->ady);
+    if (o->nd == 3) dxmin = fmin(dxmin, o->adz);
+  }
   g->dtn = o->cflx * dxmin / fmax(o->cmax + 2.0 * 0.0 / dxmin, eps);
   if (dtnext > 0.0) g->dt = fmin(g->dtn, dtnext);
 }
@@ -1729,9 +1995,18 @@ static void stage2(ogrid* g, int kind, double ds, double* dst, const double* a0,
   const size_t bs = bsize(o);
   int b;
   size_t n;
+  if (g->amr) {                 /* update_fluxes: every block, then the correction */
+    for (b = 0; b < g->nblk; b++) update_flux(o, g->q + b * bs, g->fall + b * bs * o->nd, g->qs);
+    if (!g->skip_fix) boundary_fluxes(g);
+  }
   for (b = 0; b < g->nblk; b++) {
-    update_flux(o, g->q + b * bs, g->f, g->qs);
-    update_increment(o, g->f, g->du);
+    if (g->amr) {
+      const double* lv = g->lvd[g->lev[b]];
+      update_increment_dx(o, g->fall + b * bs * o->nd, g->du, lv[3], lv[4], lv[5]);
+    } else {
+      update_flux(o, g->q + b * bs, g->f, g->qs);
+      update_increment(o, g->f, g->du);
+    }
     for (n = 0; n < bs; n++) {
       double v;
       if (kind == 0)       /* u1 = u0 + ds du */
@@ -1869,9 +2144,129 @@ int orc_create(const amun_cfg* c, void** handle)
 void orc_destroy(void* h)
 {
   ogrid* g = (ogrid*)h;
+  int L;
   if (!g) return;
   free(g->u0); free(g->u1); free(g->q); free(g->f); free(g->du); free(g->qs);
+  free(g->lev); free(g->bx); free(g->by); free(g->bz); free(g->gflag); free(g->fall);
+  for (L = 0; L <= OMAXLEV; L++) free(g->lut[L]);
   free(g);
+}
+
+/* A multi-level mesh of nblk leaf blocks: meta[4 b + 0..3] = level and the
+ * block's first interior cell (i0, j0, k0) in cells of its level; the base
+ * level has the cfg's xblocks x yblocks (x zblocks) blocks, periodic.  Cell
+ * sizes per level as coordinates.F90:330-356. */
+int orc_create_amr(const amun_cfg* c, int nblk, const int32_t* meta, void** handle)
+{
+  ogrid* g;
+  int r, b, L;
+  size_t bs, ncell;
+  const double xlen = c->xmax - c->xmin, ylen = c->ymax - c->ymin, zlen = c->zmax - c->zmin;
+  *handle = NULL;
+  if (nblk < 1) return 500;
+  r = orc_create(c, (void**)&g);
+  if (r) return r;
+  if (c->nghosts % 2 || c->ncells < 2 * c->nghosts) { orc_destroy(g); return 600; }
+  free(g->u0); free(g->u1); free(g->q);
+  g->nblk = nblk;
+  g->amr = 1;
+  bs = bsize(&g->o);
+  ncell = (size_t)g->o.im * g->o.jm * g->o.km;
+  g->u0 = (double*)calloc(bs * nblk, sizeof(double));
+  g->u1 = (double*)calloc(bs * nblk, sizeof(double));
+  g->q  = (double*)calloc(bs * nblk, sizeof(double));
+  g->fall = (double*)calloc(bs * g->o.nd * nblk, sizeof(double));
+  g->gflag = (unsigned char*)calloc(ncell * nblk, 1);
+  g->lev = (int*)calloc(nblk, sizeof(int));
+  g->bx = (int*)calloc(nblk, sizeof(int));
+  g->by = (int*)calloc(nblk, sizeof(int));
+  g->bz = (int*)calloc(nblk, sizeof(int));
+  g->nlev = 1;
+  for (b = 0; b < nblk; b++) {
+    g->lev[b] = meta[4 * b];
+    if (g->lev[b] < 1 || g->lev[b] > OMAXLEV) { orc_destroy(g); return 500; }
+    if (meta[4 * b + 1] % c->ncells || meta[4 * b + 2] % c->ncells ||
+        meta[4 * b + 3] % c->ncells) { orc_destroy(g); return 500; }
+    g->bx[b] = meta[4 * b + 1] / c->ncells;
+    g->by[b] = meta[4 * b + 2] / c->ncells;
+    g->bz[b] = meta[4 * b + 3] / c->ncells;
+    if (g->lev[b] > g->nlev) g->nlev = g->lev[b];
+  }
+  for (L = 1; L <= g->nlev; L++) {
+    const long nbx = (long)g->ir << (L - 1), nby = (long)g->jr << (L - 1);
+    const long nbz = (g->o.nd == 3) ? (long)g->kr << (L - 1) : 1;
+    const int ff = 1 << (L - 1), ni = g->o.nc * ff;
+    long n;
+    g->lut[L] = (int*)malloc(nbx * nby * nbz * sizeof(int));
+    for (n = 0; n < nbx * nby * nbz; n++) g->lut[L][n] = -1;
+    g->lvd[L][0] = xlen / (g->ir * ni);
+    g->lvd[L][1] = ylen / (g->jr * ni);
+    g->lvd[L][2] = (g->o.nd == 3) ? zlen / (g->kr * ni) : 1.0;
+    g->lvd[L][3] = 1.0 / g->lvd[L][0];
+    g->lvd[L][4] = 1.0 / g->lvd[L][1];
+    g->lvd[L][5] = (g->o.nd == 3) ? 1.0 / g->lvd[L][2] : 1.0;
+  }
+  for (b = 0; b < nblk; b++) {
+    L = g->lev[b];
+    {
+      const long nbx = (long)g->ir << (L - 1), nby = (long)g->jr << (L - 1);
+      const long nbz = (g->o.nd == 3) ? (long)g->kr << (L - 1) : 1;
+      const long id = ((long)g->bz[b] * nby + g->by[b]) * nbx + g->bx[b];
+      if (g->bx[b] >= nbx || g->by[b] >= nby || g->bz[b] >= nbz || g->lut[L][id] >= 0) {
+        orc_destroy(g);
+        return 500;
+      }
+      g->lut[L][id] = b;
+    }
+  }
+  *handle = g;
+  return 0;
+}
+
+/* interiors of every block, u[b][k][j][i][v] (v fastest), then q, ghost fill
+ * and the first time step estimate as orc_set_state */
+int orc_set_blocks(void* h, const double* u, double dtnext)
+{
+  ogrid* g = (ogrid*)h;
+  ocfg* o = &g->o;
+  const size_t bs = bsize(o);
+  const int kn = o->nd == 3 ? o->nc : 1;
+  int b, i, j, k;
+  for (b = 0; b < g->nblk; b++)
+    for (k = 0; k < kn; k++)
+      for (j = 0; j < o->nc; j++)
+        for (i = 0; i < o->nc; i++)
+          memcpy(&g->u0[b * bs + QI(o, 0, i + o->ib, j + o->jb, k + o->kb)],
+                 &u[((((size_t)b * kn + k) * o->nc + j) * o->nc + i) * o->nv],
+                 o->nv * sizeof(double));
+  o->amr_err = 0;
+  update_variables(g, g->u0);
+  if (o->amr_err) return o->amr_err;
+  new_time_step(g, dtnext);
+  return 0;
+}
+
+/* Tests only: with skip != 0 the coarse faces keep their own fluxes (the
+ * step without boundary_fluxes), to show what the correction changes. */
+void orc_amr_skip_correction(void* h, int skip) { ((ogrid*)h)->skip_fix = skip; }
+
+/* interiors of u0 (and, if q != NULL, every block's q with its ghosts) */
+int orc_get_blocks(void* h, double* u, double* q)
+{
+  ogrid* g = (ogrid*)h;
+  ocfg* o = &g->o;
+  const size_t bs = bsize(o);
+  const int kn = o->nd == 3 ? o->nc : 1;
+  int b, i, j, k;
+  for (b = 0; b < g->nblk; b++)
+    for (k = 0; k < kn; k++)
+      for (j = 0; j < o->nc; j++)
+        for (i = 0; i < o->nc; i++)
+          memcpy(&u[((((size_t)b * kn + k) * o->nc + j) * o->nc + i) * o->nv],
+                 &g->u0[b * bs + QI(o, 0, i + o->ib, j + o->jb, k + o->kb)],
+                 o->nv * sizeof(double));
+  if (q) memcpy(q, g->q, bs * g->nblk * sizeof(double));
+  return o->amr_err;
 }
 
 /* global array U[nv][nz][ny][nx] <-> block interiors */
@@ -1923,6 +2318,7 @@ int orc_run(void* h, int nsteps, double dtnext, double* trace)
   ogrid* g = (ogrid*)h;
   int s;
   for (s = 0; s < nsteps; s++) {
+    if (g->o.amr_err) return g->o.amr_err;
     advance(g, dtnext);
     if (trace) { trace[2 * s] = g->dt; trace[2 * s + 1] = g->o.cmax; }
     g->time += g->dt;

@@ -14,7 +14,7 @@
 ! (parameters.F90:99-195) from the file given with "-i".  Extra keys used only
 ! by this harness (ignored by every reference module):
 !
-!   harness_mode   = run | ghosts | flux | recon | eqs | c2p | algebra
+!   harness_mode   = run | ghosts | amr | flux | recon | eqs | c2p | algebra
 !   harness_input  = input file (raw FP64)
 !   harness_output = output file prefix
 !   harness_steps  = number of steps (run)
@@ -38,6 +38,22 @@
 !   c2p   in : u(nv,n), q(nv,n) (initial content of the output)
 !         out: <prefix>_c2p.bin q(nv,n) after cons2prim
 !   algebra in: a(5,n); out: <prefix>_alg.bin r(19,n) (see mode_algebra)
+!   amr   in : none (the reference's generate_mesh refines the reference
+!              problem named by "problem" up to maxlev)
+!         out: <prefix>_mesh0.bin per data block: level, i0, j0, k0 (cell
+!              offsets at the block's level, as doubles), u(nv,nc,nc,nc)
+!              interior, for the initial state (q recomputed from u)
+!              <prefix>_q0.bin per data block q(nv,im,jm,km) with ghosts
+!              after the initial boundary_variables
+!              <prefix>_mesh1.bin the same as mesh0 after harness_steps steps
+!              <prefix>_dt.bin (dt, cmax) for each step
+!         With harness_amr_input = 1 the block interiors are replaced by
+!         harness_input, written in the layout of <prefix>_mesh0.bin without
+!         the four leading numbers, before q is recomputed.
+!         The mesh is frozen after generation (toplev = 1 skips update_mesh
+!         in advance, evolution.F90:329), so the steps run on a fixed
+!         multi-level mesh with the reference's restriction, prolongation
+!         and flux correction.
 !
 !===============================================================================
 !
@@ -51,6 +67,7 @@ program amun_ref_harness
   use coordinates   , only : initialize_coordinates
   use coordinates   , only : im, jm, km, ib, ie, jb, je, kb, ke, nc
   use coordinates   , only : adx, ady, adz, xmin, ymin, zmin, ir, jr, kr
+  use coordinates   , only : toplev
   use equations     , only : initialize_equations, nv, cmax, cmax2
   use equations     , only : prim2cons, cons2prim, fluxspeed
   use equations     , only : update_primitive_variables
@@ -77,7 +94,7 @@ program amun_ref_harness
   character(len=255) :: mode = "run", fin = "in.bin", fout = "out"
   integer            :: iret = 0, nsteps = 1, ncount = 1, n = 1
   real(kind=8)       :: hcmax = 1.0d+00, dtnext, tmax = 1.0d+30
-  integer            :: nrun = 1
+  integer            :: nrun = 1, amr_input = 0
 
   call initialize_mpitools()
   call read_parameters(iret)
@@ -90,6 +107,7 @@ program amun_ref_harness
   call get_parameter_integer("harness_count" , ncount)
   call get_parameter_integer("harness_n"     , n     )
   call get_parameter_real   ("harness_cmax"  , hcmax )
+  call get_parameter_integer("harness_amr_input", amr_input)
   call get_parameter_real   ("tmax"          , tmax  )
   dtnext = 2.0d+00 * tmax
 
@@ -118,6 +136,8 @@ program amun_ref_harness
   select case(trim(mode))
   case("run", "ghosts")
     call mode_run()
+  case("amr")
+    call mode_amr()
   case("flux")
     call mode_flux()
   case("recon")
@@ -247,6 +267,88 @@ contains
     close(u)
 
   end subroutine mode_run
+
+!-------------------------------------------------------------------------------
+! multi-level runs on the reference's own refined mesh
+!-------------------------------------------------------------------------------
+  subroutine mode_amr()
+
+    type(block_data), pointer :: pdata
+    real(kind=8), dimension(:,:), allocatable :: trace
+    integer :: s, u
+
+    allocate(trace(2,max(nsteps,1)))
+    trace(:,:) = 0.0d+00
+    call initialize_mesh(nrun, .false., iret)
+    call generate_mesh()
+
+! a well-defined start: q = cons2prim(u) in every block (u optionally read)
+    if (amr_input == 1) open(newunit=u, file=trim(fin), access="stream",     &
+                             form="unformatted", status="old")
+    pdata => list_data
+    do while (associated(pdata))
+      if (amr_input == 1) read(u) pdata%u0(1:nv,ib:ie,jb:je,kb:ke)
+      pdata%u => pdata%u0
+      call update_primitive_variables(pdata%u0, pdata%q)
+      pdata => pdata%next
+    end do
+    if (amr_input == 1) close(u)
+    call dump_blocks(trim(fout)//"_mesh0.bin")
+
+    call boundary_variables(0.0d+00, dtnext)
+    open(newunit=u, file=trim(fout)//"_q0.bin", access="stream",              &
+         form="unformatted", status="replace")
+    pdata => list_data
+    do while (associated(pdata))
+      write(u) pdata%q
+      pdata => pdata%next
+    end do
+    close(u)
+    call new_time_step(dtnext)
+
+! freeze the mesh: advance() calls update_mesh only while toplev > 1
+    toplev = 1
+    do s = 1, nsteps
+      call advance(dtnext)
+      trace(1,s) = dt
+      trace(2,s) = cmax
+      time = time + dt
+      step = step + 1
+    end do
+    call dump_blocks(trim(fout)//"_mesh1.bin")
+    open(newunit=u, file=trim(fout)//"_dt.bin", access="stream",             &
+         form="unformatted", status="replace")
+    write(u) trace(:,1:nsteps)
+    close(u)
+
+  end subroutine mode_amr
+
+  subroutine dump_blocks(fname)
+
+    character(len=*), intent(in) :: fname
+    type(block_data), pointer :: pdata
+    type(block_meta), pointer :: pmeta
+    integer :: u
+
+    open(newunit=u, file=fname, access="stream", form="unformatted",         &
+         status="replace")
+    pdata => list_data
+    do while (associated(pdata))
+      pmeta => pdata%meta
+      write(u) real(pmeta%level, kind=8),                                     &
+               (pmeta%xmin - xmin) / adx(pmeta%level),                        &
+               (pmeta%ymin - ymin) / ady(pmeta%level),                        &
+#if NDIMS == 3
+               (pmeta%zmin - zmin) / adz(pmeta%level),                        &
+#else
+               0.0d+00,                                                       &
+#endif
+               pdata%u(1:nv,ib:ie,jb:je,kb:ke)
+      pdata => pdata%next
+    end do
+    close(u)
+
+  end subroutine dump_blocks
 
 !-------------------------------------------------------------------------------
 ! one block: reconstruction + Riemann fluxes (update_flux) on given q

@@ -216,6 +216,47 @@ def main_ssprk():
         json.dump(index, fh, indent=1)
 
 
+def amr_params(nd):
+    """Blast on a three-level mesh that the reference's generate_mesh refines
+    by its second-derivative indicator (crefmin/crefmax on density and
+    pressure), frozen for the run."""
+    p = Params(ndims=nd, equation_system="mhd", reconstruction="tvd", limiter="mc",
+               riemann_solver="hlld", time_advance="rk3", gamma=5.0 / 3.0, cfl=0.4,
+               ncells=8, nghosts=2, xblocks=2, yblocks=2, zblocks=2,
+               xmin=-0.5, xmax=0.5, ymin=-0.5, ymax=0.5, zmin=-0.5, zmax=0.5)
+    p.extra.update({"problem": "blast", "maxlev": 3, "radius": 0.15, "ratio": 100.0,
+                    "buni": 1.0, "refinement_variables": "dens pres", "crefmin": 0.1,
+                    "crefmax": 0.3})
+    return p
+
+
+def main_amr():
+    """Multi-level runs of the reference: mesh from its generate_mesh, the
+    state replaced by this project's seeded blast (amun_amd.mesh), steps on
+    the frozen mesh with restriction, prolongation and flux correction."""
+    from amun_amd import mesh
+    with open(os.path.join(HERE, "index.json")) as fh:
+        index = json.load(fh)
+    for nd, nsteps in ((2, 10), (3, 4)):
+        p = amr_params(nd)
+        meta, _, _, _, _ = ol.ref_amr(p, 1)
+        u0 = mesh.block_state(p, meta, mesh.blast_prim(p))
+        m0, u0r, m1, u1, tr = ol.ref_amr(p, nsteps, u_in=u0)
+        assert np.array_equal(m0, meta) and np.array_equal(u0r, u0)
+        name = f"amr_mhd{nd}d_blast_{nsteps}"
+        sums = u1.reshape(u1.shape[0], -1, u1.shape[-1]).sum(axis=1)
+        extra = {"u1": u1} if nd == 2 else {}
+        np.savez_compressed(os.path.join(HERE, name + ".npz"),
+                            params=np.array(p.to_params_in()), ndims=nd, nsteps=nsteps,
+                            meta=meta, trace=tr, block_sums=sums, sha_u0=sha(u0),
+                            sha_u=sha(u1), **extra)
+        index[name] = {"steps": nsteps, "sha_u": sha(u1), "dt_last": float(tr[-1, 0]),
+                       "blocks": int(meta.shape[0])}
+        print(name, meta.shape[0], "blocks, dt", tr[-1, 0])
+    with open(os.path.join(HERE, "index.json"), "w") as fh:
+        json.dump(index, fh, indent=1)
+
+
 def main():
     if not (ol.ref_available(2) and ol.ref_available(3)):
         sys.exit("reference binaries missing: run oracle/build_ref.sh first")
@@ -227,6 +268,9 @@ def main():
         return
     if sys.argv[1:] == ["ssprk"]:
         main_ssprk()
+        return
+    if sys.argv[1:] == ["amr"]:
+        main_amr()
         return
     index = {}
     for name, p, builder, nsteps, stride in run_cases():
@@ -293,6 +337,7 @@ def main():
     main_sr()
     main_gp()
     main_ssprk()
+    main_amr()
 
 
 if __name__ == "__main__":

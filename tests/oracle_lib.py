@@ -48,6 +48,12 @@ def lib():
         L.orc_gp_weights.argtypes = [ctypes.c_void_p, _P]
         L.orc_cons2prim.argtypes = [ctypes.c_void_p, ctypes.c_int, _P, _P,
                                     ctypes.POINTER(ctypes.c_long)]
+        L.orc_create_amr.argtypes = [ctypes.c_void_p, ctypes.c_int,
+                                     ctypes.POINTER(ctypes.c_int32),
+                                     ctypes.POINTER(ctypes.c_void_p)]
+        L.orc_set_blocks.argtypes = [ctypes.c_void_p, _P, ctypes.c_double]
+        L.orc_get_blocks.argtypes = [ctypes.c_void_p, _P, _P]
+        L.orc_amr_skip_correction.argtypes = [ctypes.c_void_p, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -92,6 +98,46 @@ class OracleRun:
         ev = (ctypes.c_long * 3)()
         lib().orc_c2p_events(self.h, ev)
         return list(ev)
+
+
+class OracleAmr(OracleRun):
+    """The oracle on a multi-level mesh: meta (nblk, 4) = level and the
+    first interior cell (i0, j0, k0) of every block in cells of its level."""
+
+    def __init__(self, p: Params, meta: np.ndarray):
+        self.p = p
+        self.cfg = p.to_cfg()
+        self.meta = np.ascontiguousarray(meta, dtype=np.int32)
+        h = ctypes.c_void_p()
+        r = lib().orc_create_amr(ctypes.byref(self.cfg), self.meta.shape[0],
+                                 self.meta.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                 ctypes.byref(h))
+        if r:
+            raise RuntimeError(f"orc_create_amr failed with code {r}")
+        self.h = h
+
+    def set_blocks(self, u: np.ndarray, dtnext: float = 2.0e30):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        r = lib().orc_set_blocks(self.h, _dp(u), dtnext)
+        if r:
+            raise RuntimeError(f"orc_set_blocks failed with code {r}")
+
+    def get_blocks(self, with_q: bool = False):
+        nb, nc, ng = self.meta.shape[0], self.p.ncells, self.p.nghosts
+        shp = (nb, nc, nc, self.p.nv) if self.p.ndims == 2 else (nb, nc, nc, nc, self.p.nv)
+        u = np.empty(shp)
+        im = nc + 2 * ng
+        qs = (nb, im, im, self.p.nv) if self.p.ndims == 2 else (nb, im, im, im, self.p.nv)
+        q = np.empty(qs) if with_q else None
+        r = lib().orc_get_blocks(self.h, _dp(u), _dp(q) if with_q else None)
+        if r:
+            raise RuntimeError(f"orc_get_blocks failed with code {r}")
+        return (u, q) if with_q else u
+
+
+def amr_meta(m: np.ndarray, nc: int) -> np.ndarray:
+    """Harness records (level, i0, j0, k0) -> the oracle's block meta."""
+    return np.ascontiguousarray(m, dtype=np.int32)
 
 
 def oracle_riemann(p: Params, ql: np.ndarray, qr: np.ndarray, cmax: float) -> np.ndarray:
@@ -264,3 +310,41 @@ def ref_cons2prim(p: Params, u: np.ndarray, q0: np.ndarray):
                           "harness_output": "out", "harness_n": n}, d)
         q = np.fromfile(os.path.join(d, "out_c2p.bin")).reshape(n, p.nv)
     return q, r.stderr
+
+
+def _read_blocks(path, ndims, nv, nc):
+    """Per-block records of the harness's AMR dumps: (level, i0, j0, k0) and
+    the interior u as (nc[,nc],nc, nv) arrays in (k, j, i, var) order."""
+    ncell = nc ** ndims
+    rec = 4 + nv * ncell
+    raw = np.fromfile(path)
+    assert raw.size % rec == 0
+    raw = raw.reshape(-1, rec)
+    meta = raw[:, :4].astype(np.int64)
+    shape = (-1, nc, nc, nv) if ndims == 2 else (-1, nc, nc, nc, nv)
+    u = raw[:, 4:].reshape(shape)
+    return meta, u
+
+
+def ref_amr(p: Params, nsteps: int, qdump: bool = False, u_in=None):
+    """The reference's own refined mesh (generate_mesh on the problem named
+    in p.extra["problem"], up to p.extra["maxlev"]) and nsteps steps on it,
+    frozen.  u_in (per-block interiors in the order of the mesh the same
+    parameters generate) replaces the problem's state.  Returns (meta0, u0,
+    meta1, u1, trace[, q0 with ghosts])."""
+    nv = {"hd": 5, "mhd": 9}[p.equation_system]
+    with tempfile.TemporaryDirectory() as d:
+        extra = {"harness_mode": "amr", "harness_output": "out", "harness_steps": nsteps}
+        if u_in is not None:
+            np.ascontiguousarray(u_in, dtype=np.float64).tofile(os.path.join(d, "in.bin"))
+            extra.update({"harness_amr_input": 1, "harness_input": "in.bin"})
+        _ref_exec(p, extra, d)
+        m0, u0 = _read_blocks(os.path.join(d, "out_mesh0.bin"), p.ndims, nv, p.ncells)
+        m1, u1 = _read_blocks(os.path.join(d, "out_mesh1.bin"), p.ndims, nv, p.ncells)
+        tr = np.fromfile(os.path.join(d, "out_dt.bin")).reshape(nsteps, 2)
+        out = [m0, u0, m1, u1, tr]
+        if qdump:
+            im = p.ncells + 2 * p.nghosts
+            shp = (-1, im, im, nv) if p.ndims == 2 else (-1, im, im, im, nv)
+            out.append(np.fromfile(os.path.join(d, "out_q0.bin")).reshape(shp))
+    return tuple(out)

@@ -90,7 +90,7 @@ def test_batch_members_against_oracle():
     for r in small:
         o = ol.OracleRun(runs[r])
         o.set_state(kelvin_helmholtz_2d(runs[r], seed=r))
-        for n, (fmax, l1max) in ((1, (0.01, 1e-8)), (2, (0.2, 1e-7))):
+        for n, (fmax, l1max) in ((1, (0.01, 1e-6)), (2, (0.2, 1e-5))):
             o.run(n)
             uo = o.get_state()
             scale = np.maximum(np.abs(uo).max(axis=(1, 2, 3)), np.abs(uo[4:7]).max())

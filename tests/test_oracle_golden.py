@@ -181,3 +181,24 @@ def test_ssprk_run_bit_exact(name):
     s = int(g["stride"])
     np.testing.assert_array_equal(o.get_state()[:, :, ::s, ::s], g["sub"])
     assert sha(o.get_state()) == str(g["sha_u"])
+
+
+# ---------------------------------------------------- multi-level meshes
+@pytest.mark.parametrize("name", ["amr_mhd2d_blast_10", "amr_mhd3d_blast_4"])
+def test_amr_run_bit_exact(name):
+    """Blast on the reference's own three-level mesh (frozen), this
+    project's seeded state: dt trace and final state of the reference."""
+    from amun_amd import mesh
+    g = load(name)
+    p = params_of(g)
+    meta = g["meta"]
+    u0 = mesh.block_state(p, meta, mesh.blast_prim(p))
+    assert sha(u0) == str(g["sha_u0"])
+    o = ol.OracleAmr(p, meta)
+    o.set_blocks(u0)
+    tr = o.run(int(g["nsteps"]))
+    np.testing.assert_array_equal(tr, g["trace"])
+    u = o.get_blocks()
+    assert sha(u) == str(g["sha_u"])
+    np.testing.assert_array_equal(u.reshape(u.shape[0], -1, u.shape[-1]).sum(axis=1),
+                                  g["block_sums"])

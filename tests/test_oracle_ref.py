@@ -132,3 +132,47 @@ def test_ssprk_integrators(ta, stages, eq):
     to = o.run(4)
     np.testing.assert_array_equal(to, tr)
     np.testing.assert_array_equal(o.get_state(), ur)
+
+
+# ------------------------------------------------------- multi-level meshes
+def _amr_params(nd, eq, rec, lim, ta, plim="mm", maxlev=3, base=2, problem="blast"):
+    p = Params(ndims=nd, equation_system=eq, reconstruction=rec, limiter=lim,
+               riemann_solver="hlld" if eq == "mhd" else "hllc", time_advance=ta,
+               gamma=5.0 / 3.0 if eq == "mhd" else 1.4, cfl=0.4, ncells=8,
+               nghosts=4 if rec == "weno5z" else 2, xblocks=base, yblocks=base, zblocks=base,
+               xmin=-0.5, xmax=0.5, ymin=-0.5, ymax=0.5, zmin=-0.5, zmax=0.5,
+               prolongation_limiter=plim)
+    if rec == "weno5z":
+        p.ncells = 8
+    p.extra.update({"problem": problem, "maxlev": maxlev, "radius": 0.15, "ratio": 100.0,
+                    "refinement_variables": "dens pres", "crefmin": 0.1, "crefmax": 0.3})
+    if eq == "mhd":
+        p.extra["buni"] = 1.0
+    return p
+
+
+@needs_ref
+@pytest.mark.parametrize("nd,eq,rec,lim,ta,plim,maxlev", [
+    (2, "mhd", "tvd", "mc", "rk3", "mm", 3),
+    (2, "hd", "tvd", "vl", "rk2", "mc", 3),
+    (2, "hd", "weno5z", "mm", "rk3", "sb", 3),
+    (2, "mhd", "tvd", "mm", "euler", "vl", 4),
+    (2, "mhd", "tvd", "mc", "ssprk(4,3)", "zero", 3),
+    (3, "mhd", "tvd", "mc", "rk3", "mm", 3),
+    (3, "hd", "tvd", "mc", "rk2", "mc", 3),
+])
+def test_amr_ghost_fill_and_run(nd, eq, rec, lim, ta, plim, maxlev):
+    """The reference's own refined mesh (generate_mesh on a blast, frozen):
+    every block's ghost layers after the initial boundary_variables, the dt
+    trace and the block states after three steps, bit for bit."""
+    p = _amr_params(nd, eq, rec, lim, ta, plim, maxlev)
+    m0, u0, m1, u1, tr, q0 = ol.ref_amr(p, 3, qdump=True)
+    assert len(np.unique(m0[:, 0])) >= 2, "the mesh should hold two levels at least"
+    assert np.array_equal(m0, m1)
+    o = ol.OracleAmr(p, m0)
+    o.set_blocks(u0)
+    uo, qo = o.get_blocks(with_q=True)
+    np.testing.assert_array_equal(qo, q0)
+    to = o.run(3)
+    np.testing.assert_array_equal(to, tr)
+    np.testing.assert_array_equal(o.get_blocks(), u1)
