@@ -224,13 +224,13 @@ int validate_grid(const amun_cfg& c, const amun_grid& g) {
   return 0;
 }
 
-// LDS bytes of the sweep along dir for a workgroup of wg lanes (chunked
-// pencil I/O of the x and y sweeps: primitive slots, increment slots)
-size_t sweep_lds(const amun_ctx* c, int dir, int wg) {
+// LDS bytes of the sweep along dir (chunked pencil I/O of the x and y
+// sweeps: primitive slots, increment slots, kLdsLanes lanes each)
+size_t sweep_lds(const amun_ctx* c, int dir, int /*wg*/) {
   const bool chunk = dir < 2, last = dir == c->rc.nd - 1;
   const bool din = chunk && dir > 0, dout = chunk && !last;
   const int nbuf = (chunk ? 1 : 0) + ((din || dout) ? 1 : 0);
-  return (size_t)nbuf * 4 * c->nv * wg * c->qsize;
+  return (size_t)nbuf * 4 * c->nv * kLdsLanes * c->qsize;
 }
 
 }  // namespace
@@ -1314,9 +1314,14 @@ int amun_dev_autotune(amun_ctx* c, int32_t nsteps, int32_t* choice) {
   const int nk = kernels_per_step(c);
   std::vector<double> kms(nk);
   int r = 0;
+  // the chunked x and y sweeps stage kLdsLanes lanes of LDS per workgroup
+  // whatever its size, so only full workgroups fill a CU there
   for (int w = 0; w < 3 && !r; w++)
     for (int z = 0; z < 2 && !r; z++) {
-      for (int dir = 0; dir < nd; dir++) { c->wg[dir] = wgs[w]; c->swz[dir] = z; }
+      for (int dir = 0; dir < nd; dir++) {
+        c->wg[dir] = dir < 2 ? kLdsLanes : wgs[w];
+        c->swz[dir] = z;
+      }
       double ms = 0.0;
       r = timed_steps(c, 1, 2.0e30, &ms, nullptr, 0);
       if (!r) r = timed_steps(c, nsteps, 2.0e30, &ms, kms.data(), nk);
@@ -1326,7 +1331,7 @@ int amun_dev_autotune(amun_ctx* c, int32_t nsteps, int32_t* choice) {
       for (int dir = 0; dir < nd; dir++) {
         double t = 0.0;
         for (int s = 0; s < stages_per_step(c); s++) t += kms[(size_t)s * nd + dir];
-        if (t < best[dir]) { best[dir] = t; bwg[dir] = wgs[w]; bsw[dir] = z; }
+        if (t < best[dir]) { best[dir] = t; bwg[dir] = c->wg[dir]; bsw[dir] = z; }
       }
     }
   for (int dir = 0; dir < nd; dir++) { c->wg[dir] = bwg[dir]; c->swz[dir] = bsw[dir]; }
@@ -1347,6 +1352,8 @@ int amun_dev_set_launch(amun_ctx* c, const int32_t* choice) {
   for (int dir = 0; dir < c->rc.nd; dir++) {
     const int wg = choice[2 * dir];
     if (wg != 64 && wg != 128 && wg != 256) return fail(500, "workgroup size must be 64, 128 or 256");
+    if (dir < 2 && wg != kLdsLanes)
+      return fail(500, "the x and y sweeps run workgroups of 256 lanes");
   }
   HIP_TRY(hipStreamSynchronize(c->stream));
   for (int dir = 0; dir < c->rc.nd; dir++) {

@@ -193,6 +193,9 @@ __device__ __forceinline__ void riemann(const EosConst<T>& e, const SrConst& sc,
   else riemann_hll<P, T>(e, cmax, cmax2, ql, qr, f);
 }
 
+// lanes of the per-lane LDS slots of the chunked (x and y) sweeps
+constexpr int kLdsLanes = 256;
+
 struct SweepArgs {
   const void* Q;                   // primitive state of the stage input (T)
   const double* U0;                // state at the start of the step
@@ -419,9 +422,12 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a, RunConst rc) {
   constexpr bool kDout = kChunk && !kLast;       // increment written in chunks
   // per-lane LDS slots of the chunked pencil I/O: sq holds four cells of the
   // primitive window, sd four cells of the increment (4 NV values per lane
-  // each); sized by the launch for its workgroup size
+  // each), slot-major with a fixed stride of kLdsLanes lanes (consecutive
+  // lanes hit consecutive banks; a compile-time stride keeps the slot
+  // addresses out of the register file).  The chunked sweeps launch
+  // workgroups of at most kLdsLanes lanes.
   extern __shared__ __align__(16) unsigned char smem[];
-  const int wg = blockDim.x;
+  constexpr int wg = kLdsLanes;
   T* const sq = reinterpret_cast<T*>(smem);
   T* const sd = sq + (kChunk ? 4 * NV * wg : 0);
   const int nc = rc.nc;

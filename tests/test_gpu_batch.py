@@ -9,7 +9,7 @@ import pytest
 
 from amun_amd.config import Params, c2_params, c3_batch_params
 from amun_amd.problems import kelvin_helmholtz_2d, initial_state, orszag_tang_2d
-from amun_amd.solver import Batch, Solver
+from amun_amd.solver import AmunError, Batch, Solver
 import oracle_lib as ol
 
 pytestmark = pytest.mark.gpu
@@ -173,11 +173,15 @@ def test_autotune_keeps_state_and_results():
         assert b.step == a.step and b.time == a.time
         np.testing.assert_array_equal(b.run(3), a.run(3))
         np.testing.assert_array_equal(b.download(), a.download())
+        assert choice[0][0] == 256 and choice[1][0] == 256   # x, y sweeps
         for wg in (64, 128):
-            b.set_launch([(wg, 1)] * 3)
+            # the x and y sweeps keep their 256-lane workgroups
+            b.set_launch([(256, 0), (256, 1), (wg, 1)])
             a.run(1)
             b.run(1)
             np.testing.assert_array_equal(b.download(), a.download())
+        with pytest.raises(AmunError):
+            b.set_launch([(128, 0), (256, 0), (256, 0)])
 
 
 def test_c3_batch_fp32_vs_fp64():
