@@ -35,7 +35,8 @@ def _run(cmd, cwd=None):
     return r
 
 
-HIP_UNITS = ("amun_dev", "inst_tvd", "inst_weno5z", "inst_gp")
+HIP_UNITS = ("amun_dev", "inst_tvd", "inst_weno5z", "inst_weno5yc", "inst_weno5ns", "inst_mp5",
+             "inst_mp7", "inst_gp")
 
 
 def build_hip():

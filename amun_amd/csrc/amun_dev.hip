@@ -76,6 +76,10 @@ bool pick(Dispatch& d, const amun_cfg& c) {
   switch (c.reconstruction) {
   case AMUN_REC_TVD: return pick_tvd(d, c);
   case AMUN_REC_WENO5Z: return pick_weno5z(d, c);
+  case AMUN_REC_WENO5YC: return pick_weno5yc(d, c);
+  case AMUN_REC_WENO5NS: return pick_weno5ns(d, c);
+  case AMUN_REC_MP5: return pick_mp5(d, c);
+  case AMUN_REC_MP7: return pick_mp7(d, c);
   case AMUN_REC_GP: return pick_gp(d, c);
   default: return false;
   }
@@ -87,9 +91,11 @@ int validate(const amun_cfg& c, int* nv) {
   else if (c.eqsys == AMUN_EQSYS_MHD || c.eqsys == AMUN_EQSYS_SRMHD) *nv = 9;
   else return fail(100, "unsupported equation_system");
   if (c.eos != AMUN_EOS_ADI) return fail(110, "unsupported equation_of_state");
-  if (c.reconstruction != AMUN_REC_TVD && c.reconstruction != AMUN_REC_WENO5Z &&
-      c.reconstruction != AMUN_REC_GP)
-    return fail(200, "unsupported reconstruction");
+  switch (c.reconstruction) {
+  case AMUN_REC_TVD: case AMUN_REC_WENO5Z: case AMUN_REC_WENO5YC: case AMUN_REC_WENO5NS:
+  case AMUN_REC_MP5: case AMUN_REC_MP7: case AMUN_REC_GP: break;
+  default: return fail(200, "unsupported reconstruction");
+  }
   if (c.reconstruction == AMUN_REC_GP && c.ngp != 5)
     return fail(600, "the Gaussian-process reconstruction runs with ngp = 5 on the device path");
   if (c.reconstruction == AMUN_REC_GP && !(c.sgp > 0.0))
@@ -114,9 +120,16 @@ int validate(const amun_cfg& c, int* nv) {
     return fail(600, "fix_positivity / clip_extrema are not supported on the device path");
   if (c.ncells < 4 || c.ncells > 64 || c.ncells % 4)
     return fail(500, "ncells must be a multiple of 4 in [4, 64] (4x4 brick layout)");
-  int need = (c.reconstruction == AMUN_REC_WENO5Z) ? 4 : (c.reconstruction == AMUN_REC_GP ? 3 : 2);
+  // interior formulas at the faces of cells -1 and ncells (the reference's
+  // lower-order end stencils then stay in the outer ghost layers)
+  int need = 2;
+  switch (c.reconstruction) {
+  case AMUN_REC_WENO5Z: case AMUN_REC_WENO5YC: case AMUN_REC_WENO5NS: case AMUN_REC_MP7: need = 4; break;
+  case AMUN_REC_GP: case AMUN_REC_MP5: need = 3; break;
+  }
   if (c.nghosts < need)
-    return fail(600, "nghosts too small for the reconstruction stencil (WENO5-Z needs 4, GP 3, TVD 2)");
+    return fail(600, "nghosts too small for the reconstruction stencil (WENO5 and MP7 need 4, "
+                     "GP and MP5 3, TVD 2)");
   if (c.ncells < 3) return fail(500, "ncells too small for the stencil");
   if (!(c.gamma > 1.0) || !(c.cfl > 0.0)) return fail(500, "gamma must exceed 1 and cfl be positive");
   return 0;

@@ -181,7 +181,11 @@ __device__ __forceinline__ constexpr int rot(int p) {
   }
 }
 
-enum { REC_TVD = AMUN_REC_TVD, REC_WENO5Z = AMUN_REC_WENO5Z, REC_GP = AMUN_REC_GP };
+enum {
+  REC_TVD = AMUN_REC_TVD, REC_WENO5Z = AMUN_REC_WENO5Z, REC_WENO5YC = AMUN_REC_WENO5YC,
+  REC_WENO5NS = AMUN_REC_WENO5NS, REC_MP5 = AMUN_REC_MP5, REC_MP7 = AMUN_REC_MP7,
+  REC_GP = AMUN_REC_GP
+};
 enum { RS_HLL = AMUN_RS_HLL, RS_HLLC = AMUN_RS_HLLC, RS_HLLD = AMUN_RS_HLLD };
 
 template <class P, int RS, typename T>
@@ -286,6 +290,64 @@ struct Recon<P, REC_WENO5Z, ND, DIR, T> {
     const T eps = T(rc.eps);
 #pragma unroll
     for (int v = 0; v < P::nv; v++) weno5z_cell<T>(w[v], eps, flc[v], frc[v]);
+  }
+};
+
+// WENO5-YC and WENO5-NS (reconstruct_weno5yc / _weno5ns): the window of
+// WENO5-Z with their own weights
+template <class P, int ND, int DIR, typename T>
+struct Recon<P, REC_WENO5YC, ND, DIR, T> : Recon<P, REC_WENO5Z, ND, DIR, T> {
+  __device__ __forceinline__ void cell(const RunConst& rc, T* flc, T* frc, const T*,
+                                       const int*, int, int, int, int, DevState*) {
+    const T eps = T(rc.eps);
+#pragma unroll
+    for (int v = 0; v < P::nv; v++) weno5z_cell<T, true>(this->w[v], eps, flc[v], frc[v]);
+  }
+};
+
+template <class P, int ND, int DIR, typename T>
+struct Recon<P, REC_WENO5NS, ND, DIR, T> : Recon<P, REC_WENO5Z, ND, DIR, T> {
+  __device__ __forceinline__ void cell(const RunConst& rc, T* flc, T* frc, const T*,
+                                       const int*, int, int, int, int, DevState*) {
+    const T eps = T(rc.eps);
+#pragma unroll
+    for (int v = 0; v < P::nv; v++) weno5ns_cell<T>(this->w[v], eps, flc[v], frc[v]);
+  }
+};
+
+// MP5 (reconstruct_mp5): the five-cell window, polynomial face values and
+// mp_limiting
+template <class P, int ND, int DIR, typename T>
+struct Recon<P, REC_MP5, ND, DIR, T> : Recon<P, REC_WENO5Z, ND, DIR, T> {
+  __device__ __forceinline__ void cell(const RunConst& rc, T* flc, T* frc, const T*,
+                                       const int*, int, int, int, int, DevState*) {
+    const T kappa = T(rc.kappa), kbeta = T(rc.kbeta), eps = T(rc.eps);
+#pragma unroll
+    for (int v = 0; v < P::nv; v++) mp_cell<T, 2>(this->w[v], kappa, kbeta, eps, flc[v], frc[v]);
+  }
+};
+
+// MP7 (reconstruct_mp7): a seven-cell window
+template <class P, int ND, int DIR, typename T>
+struct Recon<P, REC_MP7, ND, DIR, T> {
+  static constexpr int R = 3;
+  static constexpr int H = 4;
+  T w[P::nv][7];
+  __device__ __forceinline__ void load(int slotm, const T* q) {
+#pragma unroll
+    for (int v = 0; v < P::nv; v++) w[v][slotm] = q[v];
+  }
+  __device__ __forceinline__ void shift() {
+#pragma unroll
+    for (int v = 0; v < P::nv; v++)
+#pragma unroll
+      for (int m = 0; m < 6; m++) w[v][m] = w[v][m + 1];
+  }
+  __device__ __forceinline__ void cell(const RunConst& rc, T* flc, T* frc, const T*,
+                                       const int*, int, int, int, int, DevState*) {
+    const T kappa = T(rc.kappa), kbeta = T(rc.kbeta), eps = T(rc.eps);
+#pragma unroll
+    for (int v = 0; v < P::nv; v++) mp_cell<T, 3>(w[v], kappa, kbeta, eps, flc[v], frc[v]);
   }
 };
 
@@ -983,6 +1045,10 @@ bool pick_rec(Dispatch& d, const amun_cfg& c) {
 
 bool pick_tvd(Dispatch& d, const amun_cfg& c);
 bool pick_weno5z(Dispatch& d, const amun_cfg& c);
+bool pick_weno5yc(Dispatch& d, const amun_cfg& c);
+bool pick_weno5ns(Dispatch& d, const amun_cfg& c);
+bool pick_mp5(Dispatch& d, const amun_cfg& c);
+bool pick_mp7(Dispatch& d, const amun_cfg& c);
 bool pick_gp(Dispatch& d, const amun_cfg& c);
 
 }  // namespace amun_dev

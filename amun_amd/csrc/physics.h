@@ -109,7 +109,10 @@ __device__ __forceinline__ T limiter(int kind, T x, T a, T b) {
 // the state at the left face (fr(i-1), i-1/2).  The smoothness indicators are
 // shared by both faces exactly as in the reference loop body.
 // ---------------------------------------------------------------------------
-template <typename T>
+// YC = true: reconstruct_weno5yc (interpolations.F90:2107-2254), whose tau
+// is the squared fourth difference (6 f(i) - 4 (f(i-1) + f(i+1)) + f(i-2) +
+// f(i+2))^2; everything else is WENO5-Z's
+template <typename T, bool YC = false>
 __device__ __forceinline__ void weno5z_cell(const T f[5], T eps, T& fl, T& fr) {
   const T c1 = T(13.0 / 12.0), c2 = T(0.25);
   const T dl = T(0.1), dc = T(0.6), dr = T(0.3);
@@ -129,7 +132,13 @@ __device__ __forceinline__ void weno5z_cell(const T f[5], T eps, T& fl, T& fr) {
   t = T(3) * dfm_0 - dfm_m1; const T bl = df2_m1 + c2 * (t * t);
   t = dfp_0 + dfm_0;         const T bc = df2_0 + c2 * (t * t);
   t = T(3) * dfp_0 - dfp_p1; const T br = df2_p1 + c2 * (t * t);
-  const T tt = tabs(br - bl);
+  T tt;
+  if constexpr (YC) {
+    const T d4 = (T(6) * f[2] - T(4) * (f[1] + f[3])) + (f[0] + f[4]);
+    tt = d4 * d4;
+  } else {
+    tt = tabs(br - bl);
+  }
   // the three interpolations of each face
   const T ql = a11 * f[0] + a12 * f[1] + a13 * f[2];
   const T qc = a21 * f[1] + a22 * f[2] + a23 * f[3];
@@ -173,13 +182,11 @@ __device__ __forceinline__ T mmod(T a, T b) {   // minmod, interpolations.F90:48
   return (tsign(T(0.5), a) + tsign(T(0.5), b)) * tmin(tabs(a), tabs(b));
 }
 
+// mp_limiting (interpolations.F90:5209-5284) of the unlimited face value u
+// of the cell whose neighbourhood is f[0..4] = f(i-2..i+2)
 template <typename T>
-__device__ __forceinline__ T gp_face(const T f[5], const T cgp[5], T kappa, T kbeta, T eps) {
+__device__ __forceinline__ T mp_limit(const T f[5], T u, T kappa, T kbeta, T eps) {
   const T fi = f[2];
-  T u = cgp[0] * (f[0] - fi);
-#pragma unroll
-  for (int m = 1; m < 5; m++) u = u + cgp[m] * (f[m] - fi);
-  u = u + fi;
   // dm(k) = f(k) - f(k-1) around the cell
   const T dmm = f[1] - f[0], dm0 = f[2] - f[1], dmp = f[3] - f[2], dmpp = f[4] - f[3];
   const T df = (dm0 * dmp >= T(0)) ? kappa * dm0 : kbeta * dm0;
@@ -197,6 +204,90 @@ __device__ __forceinline__ T gp_face(const T f[5], const T cgp[5], T kappa, T kb
     u = u + mmod(fmn - u, fmx - u);               // median(u, fmn, fmx)
   }
   return u;
+}
+
+template <typename T>
+__device__ __forceinline__ T gp_face(const T f[5], const T cgp[5], T kappa, T kbeta, T eps) {
+  const T fi = f[2];
+  T u = cgp[0] * (f[0] - fi);
+#pragma unroll
+  for (int m = 1; m < 5; m++) u = u + cgp[m] * (f[m] - fi);
+  return mp_limit<T>(f, u + fi, kappa, kbeta, eps);
+}
+
+// reconstruct_mp5 / reconstruct_mp7 (interpolations.F90:3675-3770,
+// 3799-3901): the fifth- or seventh-order polynomial face value, summed in
+// stencil order, limited by mp_limiting; f holds 2 M + 1 cells centred on
+// the cell
+template <typename T, int M>
+__device__ __forceinline__ T mp_face(const T* f, T kappa, T kbeta, T eps) {
+  T u;
+  if constexpr (M == 2) {
+    const T ce[5] = {T(2.0 / 6.0e+01), T(-1.3e+01 / 6.0e+01), T(4.7e+01 / 6.0e+01),
+                     T(2.7e+01 / 6.0e+01), T(-3.0 / 6.0e+01)};
+    u = ce[0] * f[0];
+#pragma unroll
+    for (int m = 1; m < 5; m++) u = u + ce[m] * f[m];
+  } else {
+    const T ce[7] = {T(-3.0 / 4.2e+02), T(2.5e+01 / 4.2e+02), T(-1.01e+02 / 4.2e+02),
+                     T(3.19e+02 / 4.2e+02), T(2.14e+02 / 4.2e+02), T(-3.8e+01 / 4.2e+02),
+                     T(4.0 / 4.2e+02)};
+    u = ce[0] * f[0];
+#pragma unroll
+    for (int m = 1; m < 7; m++) u = u + ce[m] * f[m];
+  }
+  return mp_limit<T>(f + M - 2, u, kappa, kbeta, eps);
+}
+
+template <typename T, int M>
+__device__ __forceinline__ void mp_cell(const T* f, T kappa, T kbeta, T eps, T& fl, T& fr) {
+  fl = mp_face<T, M>(f, kappa, kbeta, eps);
+  T g[2 * M + 1];
+#pragma unroll
+  for (int m = 0; m <= 2 * M; m++) g[m] = f[2 * M - m];
+  fr = mp_face<T, M>(g, kappa, kbeta, eps);
+}
+
+// reconstruct_weno5ns (interpolations.F90:2278-2441): WENO5 with the
+// smoothness measures and weights of the "NS" variant (absolute first
+// differences, xi = 0.4, zeta from the cubed slope)
+template <typename T>
+__device__ __forceinline__ void weno5ns_cell(const T f[5], T eps, T& fl, T& fr) {
+  const T dl = T(0.1), dc = T(0.6), dr = T(0.3), xi = T(4.0e-01);
+  const T a11 = T(2.0 / 6.0), a12 = T(-7.0 / 6.0), a13 = T(11.0 / 6.0);
+  const T a21 = T(-1.0 / 6.0), a22 = T(5.0 / 6.0), a23 = T(2.0 / 6.0);
+  const T a31 = T(2.0 / 6.0), a32 = T(5.0 / 6.0), a33 = T(-1.0 / 6.0);
+  const T dfm_m1 = f[1] - f[0], dfm_0 = f[2] - f[1], dfp_0 = f[3] - f[2], dfp_p1 = f[4] - f[3];
+  const T dfp_m1 = dfm_0, dfm_p1 = dfp_0;
+  const T d2m = T(0.5) * tabs(dfp_m1 - dfm_m1);
+  const T d20 = T(0.5) * tabs(dfp_0 - dfm_0);
+  const T d2p = T(0.5) * tabs(dfp_p1 - dfm_p1);
+  auto weights = [&](T df, T b2l, T b2r, T s, T& wl, T& wc, T& wr) {
+    const T lq = xi * df;
+    const T bl = b2l + xi * s, bc = d20 + lq, br = b2r + lq;
+    const T l3 = (df * df) * df;
+    const T t = bl - br, r = l3 / (T(1) + l3);
+    const T zt = T(0.5) * (t * t + r * r);
+    const T el = bl + eps, ec = bc + eps, er = br + eps;
+    const T al = dl * (T(1) + zt / (el * el));
+    const T ac = dc * (T(1) + zt / (ec * ec));
+    const T ar = dr * (T(1) + zt / (er * er));
+    const T aa = (al + ar) + ac;
+    wl = al / aa;
+    wr = ar / aa;
+    wc = T(1) - (wl + wr);
+  };
+  T wl, wc, wr;
+  weights(tabs(dfp_0), d2m, d2p, tabs(T(2) * dfm_0 - dfm_m1), wl, wc, wr);
+  const T ql = a11 * f[0] + a12 * f[1] + a13 * f[2];
+  const T qc = a21 * f[1] + a22 * f[2] + a23 * f[3];
+  const T qr = a31 * f[2] + a32 * f[3] + a33 * f[4];
+  fl = (wl * ql + wr * qr) + wc * qc;
+  weights(tabs(dfm_0), d2p, d2m, tabs(T(2) * dfp_0 - dfp_p1), wl, wc, wr);
+  const T pl = a11 * f[4] + a12 * f[3] + a13 * f[2];
+  const T pc = a21 * f[3] + a22 * f[2] + a23 * f[1];
+  const T pr = a31 * f[2] + a32 * f[1] + a33 * f[0];
+  fr = (wl * pl + wr * pr) + wc * pc;
 }
 
 template <typename T>

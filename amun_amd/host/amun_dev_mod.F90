@@ -19,6 +19,10 @@ module amun_dev_mod
   integer(c_int32_t), parameter :: AMUN_EQSYS_SRMHD = 4
   integer(c_int32_t), parameter :: AMUN_EOS_ADI = 1
   integer(c_int32_t), parameter :: AMUN_REC_TVD = 1, AMUN_REC_WENO5Z = 5
+  integer(c_int32_t), parameter :: AMUN_REC_WENO5YC = 6
+  integer(c_int32_t), parameter :: AMUN_REC_WENO5NS = 7
+  integer(c_int32_t), parameter :: AMUN_REC_MP5 = 11
+  integer(c_int32_t), parameter :: AMUN_REC_MP7 = 12
   integer(c_int32_t), parameter :: AMUN_REC_GP = 20
   integer(c_int32_t), parameter :: AMUN_LIM_ZERO = 0, AMUN_LIM_MINMOD = 1
   integer(c_int32_t), parameter :: AMUN_LIM_MC = 2, AMUN_LIM_SUPERBEE = 3

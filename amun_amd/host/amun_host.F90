@@ -74,6 +74,14 @@ program amun_host
     cfg%reconstruction = AMUN_REC_WENO5Z
   case("gp", "GP")
     cfg%reconstruction = AMUN_REC_GP
+  case("weno5yc", "weno5-yc", "WENO5YC", "WENO5-YC")
+    cfg%reconstruction = AMUN_REC_WENO5YC
+  case("weno5ns", "weno5-ns", "WENO5NS", "WENO5-NS")
+    cfg%reconstruction = AMUN_REC_WENO5NS
+  case("mp5", "MP5")
+    cfg%reconstruction = AMUN_REC_MP5
+  case("mp7", "MP7")
+    cfg%reconstruction = AMUN_REC_MP7
   case default
     cfg%reconstruction = -1
   end select

@@ -42,6 +42,10 @@ extern "C" {
 /* reconstructions: interpolations.F90:237-376 (parameter "reconstruction") */
 #define AMUN_REC_TVD       1
 #define AMUN_REC_WENO5Z    5
+#define AMUN_REC_WENO5YC   6   /* reconstruct_weno5yc, interpolations.F90:2107 */
+#define AMUN_REC_WENO5NS   7   /* reconstruct_weno5ns, interpolations.F90:2278 */
+#define AMUN_REC_MP5       11  /* reconstruct_mp5, interpolations.F90:3675     */
+#define AMUN_REC_MP7       12  /* reconstruct_mp7, interpolations.F90:3799     */
 #define AMUN_REC_GP        20  /* Gaussian process, reconstruct_gp interpolations.F90:4561 */
 
 /* TVD limiters: interpolations.F90:379-398 (parameter "limiter") */

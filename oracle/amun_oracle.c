@@ -179,8 +179,11 @@ static double lim(int kind, double x, double a, double b)
 /* ------------------------------------------------------------------------ */
 /* reconstruct_weno5z, interpolations.F90:1937-2083, one pencil of length n   */
 /* ------------------------------------------------------------------------ */
-static void weno5z(const ocfg* o, int n, const double* f, double* fl,
-                   double* fr, double* dfm, double* dfp, double* df2)
+/* reconstruct_weno5z (interpolations.F90:1937-2083) and, with yc != 0,
+ * reconstruct_weno5yc (2107-2254), whose tau is the squared fourth
+ * difference instead of |beta_r - beta_l| */
+static void weno5z_v(const ocfg* o, int n, const double* f, double* fl,
+                     double* fr, double* dfm, double* dfp, double* df2, int yc)
 {
   const double c1 = 13.0 / 12.0, c2 = 0.25;
   const double dl = 0.1, dc = 0.6, dr = 0.3;
@@ -207,7 +210,12 @@ static void weno5z(const ocfg* o, int n, const double* f, double* fl,
     bc = df2[i] + c2 * (t * t);
     t  = 3.0 * dfp[i] - dfp[i + 1];
     br = df2[i + 1] + c2 * (t * t);
-    tt = fabs(br - bl);
+    if (yc) {
+      t  = ((6.0 * f[i] - 4.0 * (f[i - 1] + f[i + 1])) + (f[i - 2] + f[i + 2]));
+      tt = t * t;
+    } else {
+      tt = fabs(br - bl);
+    }
     al = 1.0 + tt / (bl + eps);
     ac = 1.0 + tt / (bc + eps);
     ar = 1.0 + tt / (br + eps);
@@ -223,6 +231,85 @@ static void weno5z(const ocfg* o, int n, const double* f, double* fl,
     wl = dl * ar; wc = dc * ac; wr = dr * al;
     ww = (wl + wr) + wc;
     wl = wl / ww; wr = wr / ww; wc = 1.0 - (wl + wr);
+    ql = a11 * f[i + 2] + a12 * f[i + 1] + a13 * f[i];
+    qc = a21 * f[i + 1] + a22 * f[i] + a23 * f[i - 1];
+    qr = a31 * f[i] + a32 * f[i - 1] + a33 * f[i - 2];
+    fr[i - 1] = (wl * ql + wr * qr) + wc * qc;
+  }
+  {
+    double df;
+    fl[0] = 0.5 * (f[0] + f[1]);
+    df    = lim(o->c.limiter, 0.5, dfm[1], dfp[1]);
+    fr[0] = f[1] - df;
+    fl[1] = f[1] + df;
+    i     = n - 2;
+    df    = lim(o->c.limiter, 0.5, dfm[i], dfp[i]);
+    fr[i - 1] = f[i] - df;
+    fl[i]     = f[i] + df;
+    fr[i]     = 0.5 * (f[i] + f[n - 1]);
+    fl[n - 1] = f[n - 1];
+    fr[n - 1] = f[n - 1];
+  }
+}
+
+static void weno5z(const ocfg* o, int n, const double* f, double* fl,
+                   double* fr, double* dfm, double* dfp, double* df2)
+{
+  weno5z_v(o, n, f, fl, fr, dfm, dfp, df2, 0);
+}
+
+/* reconstruct_weno5ns, interpolations.F90:2278-2441 */
+static void weno5ns(const ocfg* o, int n, const double* f, double* fl,
+                    double* fr, double* dfm, double* dfp, double* df2)
+{
+  const double dl = 0.1, dc = 0.6, dr = 0.3, xi = 4.0e-01;
+  const double a11 = 2.0 / 6.0, a12 = -7.0 / 6.0, a13 = 11.0 / 6.0;
+  const double a21 = -1.0 / 6.0, a22 = 5.0 / 6.0, a23 = 2.0 / 6.0;
+  const double a31 = 2.0 / 6.0, a32 = 5.0 / 6.0, a33 = -1.0 / 6.0;
+  const double eps = o->c.eps;
+  int i;
+  for (i = 0; i < n - 1; i++) {
+    dfp[i]     = f[i + 1] - f[i];
+    dfm[i + 1] = dfp[i];
+  }
+  dfm[0]     = dfp[0];
+  dfp[n - 1] = dfm[n - 1];
+  for (i = 0; i < n; i++) df2[i] = 0.5 * fabs(dfp[i] - dfm[i]);
+  for (i = 2; i < n - 2; i++) {
+    double df, lq, bl, bc, br, l3, zt, al, ac, ar, aa, wl, wc, wr, ql, qc, qr, t, s;
+    df = fabs(dfp[i]);
+    lq = xi * df;
+    bl = df2[i - 1] + xi * fabs(2.0 * dfm[i] - dfm[i - 1]);
+    bc = df2[i] + lq;
+    br = df2[i + 1] + lq;
+    l3 = (df * df) * df;
+    t  = bl - br;
+    s  = l3 / (1.0 + l3);
+    zt = 0.5 * (t * t + s * s);
+    t = bl + eps; al = dl * (1.0 + zt / (t * t));
+    t = bc + eps; ac = dc * (1.0 + zt / (t * t));
+    t = br + eps; ar = dr * (1.0 + zt / (t * t));
+    aa = (al + ar) + ac;
+    wl = al / aa; wr = ar / aa; wc = 1.0 - (wl + wr);
+    ql = a11 * f[i - 2] + a12 * f[i - 1] + a13 * f[i];
+    qc = a21 * f[i - 1] + a22 * f[i] + a23 * f[i + 1];
+    qr = a31 * f[i] + a32 * f[i + 1] + a33 * f[i + 2];
+    fl[i] = (wl * ql + wr * qr) + wc * qc;
+
+    df = fabs(dfm[i]);
+    lq = xi * df;
+    bl = df2[i + 1] + xi * fabs(2.0 * dfp[i] - dfp[i + 1]);
+    bc = df2[i] + lq;
+    br = df2[i - 1] + lq;
+    l3 = (df * df) * df;
+    t  = bl - br;
+    s  = l3 / (1.0 + l3);
+    zt = 0.5 * (t * t + s * s);
+    t = bl + eps; al = dl * (1.0 + zt / (t * t));
+    t = bc + eps; ac = dc * (1.0 + zt / (t * t));
+    t = br + eps; ar = dr * (1.0 + zt / (t * t));
+    aa = (al + ar) + ac;
+    wl = al / aa; wr = ar / aa; wc = 1.0 - (wl + wr);
     ql = a11 * f[i + 2] + a12 * f[i + 1] + a13 * f[i];
     qc = a21 * f[i + 1] + a22 * f[i] + a23 * f[i - 1];
     qr = a31 * f[i] + a32 * f[i - 1] + a33 * f[i - 2];
@@ -417,6 +504,61 @@ static void gp_pass(const ocfg* o, int n, const double* f, double* u, double* dm
   mp_limiting(o, n, f, u, dm);
 }
 
+/* one upwind pass of reconstruct_mp5 / reconstruct_mp7 (interpolations.F90
+ * 3675-3770, 3799-3901): polynomial face values (fifth or seventh order in
+ * the interior, lower order near the ends), then mp_limiting */
+static void mp_pass(const ocfg* o, int n, const double* f, double* u, double* dm, int order)
+{
+  static const double ce7[7] = {-3.0 / 4.2e+02, 2.5e+01 / 4.2e+02, -1.01e+02 / 4.2e+02,
+                                3.19e+02 / 4.2e+02, 2.14e+02 / 4.2e+02, -3.8e+01 / 4.2e+02,
+                                4.0 / 4.2e+02};
+  static const double ce5[5] = {2.0 / 6.0e+01, -1.3e+01 / 6.0e+01, 4.70e+01 / 6.0e+01,
+                                2.70e+01 / 6.0e+01, -3.0 / 6.0e+01};
+  static const double ce3[3] = {-1.0 / 6.0, 5.0 / 6.0, 2.0 / 6.0};
+  int i, m;
+  if (order == 7) {
+    for (i = 4; i <= n - 3; i++) {
+      double acc = ce7[0] * f[i - 4];
+      for (m = 1; m < 7; m++) acc = acc + ce7[m] * f[i - 4 + m];
+      u[i - 1] = acc;
+    }
+  } else {
+    for (i = 3; i <= n - 2; i++) {
+      double acc = ce5[0] * f[i - 3];
+      for (m = 1; m < 5; m++) acc = acc + ce5[m] * f[i - 3 + m];
+      u[i - 1] = acc;
+    }
+  }
+  u[0] = 5.0e-01 * f[0] + 5.0e-01 * f[1];
+  u[1] = (ce3[0] * f[0] + ce3[1] * f[1]) + ce3[2] * f[2];
+  if (order == 7) {
+    double acc = ce5[0] * f[0];
+    for (m = 1; m < 5; m++) acc = acc + ce5[m] * f[m];
+    u[2] = acc;
+    acc = ce5[0] * f[n - 5];
+    for (m = 1; m < 5; m++) acc = acc + ce5[m] * f[n - 5 + m];
+    u[n - 3] = acc;
+  }
+  u[n - 2] = (ce3[0] * f[n - 3] + ce3[1] * f[n - 2]) + ce3[2] * f[n - 1];
+  u[n - 1] = f[n - 1];
+  mp_limiting(o, n, f, u, dm);
+}
+
+static void reconstruct_mp(const ocfg* o, int n, const double* f, double* fl, double* fr,
+                           double* w1, double* w2, double* w3, int order)
+{
+  int i;
+  mp_pass(o, n, f, fl, w3, order);
+  for (i = 0; i < n; i++) w1[i] = f[n - 1 - i];
+  mp_pass(o, n, w1, w2, w3, order);
+  for (i = 0; i < n - 1; i++) fr[i] = w2[n - 2 - i];
+  i = n - 2;
+  fl[0] = 0.5 * (f[0] + f[1]);
+  fr[i] = 0.5 * (f[i] + f[n - 1]);
+  fl[n - 1] = f[n - 1];
+  fr[n - 1] = f[n - 1];
+}
+
 /* reconstruct_gp, interpolations.F90:4561-4668 */
 static void reconstruct_gp(const ocfg* o, int n, const double* f, double* fl, double* fr,
                            double* w1, double* w2, double* w3)
@@ -441,8 +583,14 @@ static void reconstruct_gp(const ocfg* o, int n, const double* f, double* fl, do
 static void recon_dir(const ocfg* o, int n, const double* f, double* fl, double* fr,
                       double* w1, double* w2, double* w3)
 {
-  if (o->c.reconstruction == AMUN_REC_GP) reconstruct_gp(o, n, f, fl, fr, w1, w2, w3);
-  else weno5z(o, n, f, fl, fr, w1, w2, w3);
+  switch (o->c.reconstruction) {
+  case AMUN_REC_GP: reconstruct_gp(o, n, f, fl, fr, w1, w2, w3); break;
+  case AMUN_REC_WENO5YC: weno5z_v(o, n, f, fl, fr, w1, w2, w3, 1); break;
+  case AMUN_REC_WENO5NS: weno5ns(o, n, f, fl, fr, w1, w2, w3); break;
+  case AMUN_REC_MP5: reconstruct_mp(o, n, f, fl, fr, w1, w2, w3, 5); break;
+  case AMUN_REC_MP7: reconstruct_mp(o, n, f, fl, fr, w1, w2, w3, 7); break;
+  default: weno5z(o, n, f, fl, fr, w1, w2, w3);
+  }
 }
 
 static void reconstruct_interfaces(ocfg* o, const double* q, double* qs)
@@ -2123,6 +2271,8 @@ int orc_create(const amun_cfg* c, void** handle)
   r = oc_setup(&g->o, c);
   if (!r) r = riemann_check(c);
   if (!r && c->reconstruction != AMUN_REC_TVD && c->reconstruction != AMUN_REC_WENO5Z &&
+      c->reconstruction != AMUN_REC_WENO5YC && c->reconstruction != AMUN_REC_WENO5NS &&
+      c->reconstruction != AMUN_REC_MP5 && c->reconstruction != AMUN_REC_MP7 &&
       c->reconstruction != AMUN_REC_GP) r = 200;
   if (!r && (c->time_advance < AMUN_TA_EULER || c->time_advance > AMUN_TA_SSPRK53)) r = 400;
   if (r) { free(g); return r; }

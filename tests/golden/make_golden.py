@@ -257,6 +257,41 @@ def main_amr():
         json.dump(index, fh, indent=1)
 
 
+SCHEMES = ("weno5yc", "weno5ns", "mp5", "mp7")
+
+
+def main_schemes():
+    """WENO5-YC, WENO5-NS, MP5 and MP7: pencils and short Orszag-Tang runs
+    of the reference (own generator)."""
+    rng = np.random.default_rng(20261205)
+    fp = np.concatenate([rng.normal(size=(6, 24)), np.cumsum(rng.normal(size=(4, 24)), axis=1),
+                         np.sin(np.linspace(0, 6, 24))[None, :] * np.ones((2, 1)),
+                         np.where(np.arange(24) < 12, 1.0, 0.1)[None, :] * np.ones((2, 1))])
+    out = {"fp": fp}
+    with open(os.path.join(HERE, "index.json")) as fh:
+        index = json.load(fh)
+    for rec in SCHEMES:
+        pu = Params(ndims=3, equation_system="mhd", reconstruction=rec, limiter="mm",
+                    ncells=16, nghosts=4, cfl=0.3)
+        out["fl_" + rec], out["fr_" + rec] = ol.ref_reconstruct(pu, fp)
+        p = Params(ndims=2, equation_system="mhd", reconstruction=rec, limiter="mm",
+                   riemann_solver="hlld", time_advance="rk3", gamma=5.0 / 3.0, cfl=0.3,
+                   ncells=8, nghosts=4, xblocks=4, yblocks=4)
+        u0 = problems.orszag_tang_2d(p, seed=6)
+        nsteps = 8
+        u, tr = ol.ref_run(p, u0, nsteps)
+        name = f"{rec}_ot2d_32_{nsteps}"
+        np.savez_compressed(os.path.join(HERE, name + ".npz"),
+                            params=np.array(p.to_params_in()), ndims=2, nsteps=nsteps,
+                            stride=3, trace=tr, sub=u[:, :, ::3, ::3], sha_u0=sha(u0),
+                            sha_u=sha(u))
+        index[name] = {"steps": nsteps, "sha_u": sha(u), "dt_last": float(tr[-1, 0])}
+        print(name, "dt", tr[-1, 0])
+    np.savez_compressed(os.path.join(HERE, "unit_schemes.npz"), **out)
+    with open(os.path.join(HERE, "index.json"), "w") as fh:
+        json.dump(index, fh, indent=1)
+
+
 def main():
     if not (ol.ref_available(2) and ol.ref_available(3)):
         sys.exit("reference binaries missing: run oracle/build_ref.sh first")
@@ -271,6 +306,9 @@ def main():
         return
     if sys.argv[1:] == ["amr"]:
         main_amr()
+        return
+    if sys.argv[1:] == ["schemes"]:
+        main_schemes()
         return
     index = {}
     for name, p, builder, nsteps, stride in run_cases():
@@ -338,6 +376,7 @@ def main():
     main_gp()
     main_ssprk()
     main_amr()
+    main_schemes()
 
 
 if __name__ == "__main__":

@@ -550,3 +550,39 @@ def test_ssprk_mixed_vs_oracle(ta, stages):
     assert np.abs(um[7] - uo[7]).max() <= 1e-5 * np.abs(uo[4:7]).max()
     assert err.max() <= 1e-4
     assert _l1(um[0], uo[0]) <= 1e-6
+
+
+# ------------------------------------------------ more 1D reconstructions
+@pytest.mark.parametrize("nd", [2, 3])
+@pytest.mark.parametrize("eq,rs", [("hd", "hllc"), ("mhd", "hlld")])
+@pytest.mark.parametrize("rec", ["weno5yc", "weno5ns", "mp5", "mp7"])
+def test_more_schemes_one_step(nd, eq, rs, rec):
+    """WENO5-YC, WENO5-NS, MP5, MP7 (interior formulas on the device; the
+    reference's end stencils lie in the outer ghost layers)."""
+    p = Params(ndims=nd, equation_system=eq, riemann_solver=rs, reconstruction=rec,
+               limiter="mm", time_advance="rk3", ncells=8, nghosts=4, xblocks=2, yblocks=3,
+               zblocks=2, cfl=0.3, gamma=1.4 if eq == "hd" else 5.0 / 3.0, xmin=-0.5,
+               xmax=0.5, ymin=-0.75, ymax=0.75, zmin=0.0, zmax=1.0)
+    u0 = initial_state(p) if eq == "hd" else _mhd_state(p)
+    dt, cmax, ug, tr, uo = one_step(p, u0)
+    assert abs(dt - tr[0]) <= 1e-13 * tr[0]
+    # the HD blast starts at rest (momentum a small difference of pressure
+    # fluxes), as in the GP case
+    assert relerr(ug, uo).max() <= (1e-11 if eq == "hd" else 1e-12)
+
+
+@pytest.mark.parametrize("rec", ["weno5yc", "weno5ns", "mp5", "mp7"])
+def test_more_schemes_mixed_vs_fp64(rec):
+    """Mixed precision on the 64^3 MHD case, 10 steps: density L1."""
+    res = []
+    for prec in ("fp64", "mixed"):
+        p = c2_params(64, precision=prec)
+        p.reconstruction = rec
+        u0 = initial_state(p)
+        with Solver(p) as s:
+            s.upload(u0)
+            s.run(10)
+            res.append(s.download())
+    l1 = _l1(res[1][0], res[0][0])
+    print(f"{rec} 64^3 x 10 mixed vs fp64: density L1 {l1:.3e}")
+    assert l1 <= 1e-6

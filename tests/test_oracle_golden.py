@@ -202,3 +202,27 @@ def test_amr_run_bit_exact(name):
     assert sha(u) == str(g["sha_u"])
     np.testing.assert_array_equal(u.reshape(u.shape[0], -1, u.shape[-1]).sum(axis=1),
                                   g["block_sums"])
+
+
+# ------------------------------------------------ more 1D reconstructions
+@pytest.mark.parametrize("rec", ["weno5yc", "weno5ns", "mp5", "mp7"])
+def test_more_schemes_pencils(rec):
+    g = load("unit_schemes")
+    p = Params(ndims=3, equation_system="mhd", reconstruction=rec, limiter="mm", ncells=16,
+               nghosts=4, cfl=0.3)
+    fl, fr = ol.oracle_reconstruct(p, g["fp"])
+    np.testing.assert_array_equal(fl, g["fl_" + rec])
+    np.testing.assert_array_equal(fr, g["fr_" + rec])
+
+
+@pytest.mark.parametrize("rec", ["weno5yc", "weno5ns", "mp5", "mp7"])
+def test_more_schemes_runs_bit_exact(rec):
+    g = load(f"{rec}_ot2d_32_8")
+    p = params_of(g)
+    u0 = problems.orszag_tang_2d(p, seed=6)
+    assert sha(u0) == str(g["sha_u0"])
+    o = ol.OracleRun(p)
+    o.set_state(u0)
+    tr = o.run(int(g["nsteps"]))
+    np.testing.assert_array_equal(tr, g["trace"])
+    assert sha(o.get_state()) == str(g["sha_u"])

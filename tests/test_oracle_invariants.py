@@ -500,3 +500,35 @@ def test_amr_mesh_is_balanced():
                     n = 2 << (L - 1)
                     c = cover(L, (x + dx) % n, (y + dy) % n, (z + dz) % n)
                     assert c is None or c >= L - 1
+
+
+# ------------------------------------------------ more 1D reconstructions
+def _cell_averages(coef, n, h=1.0):
+    """Exact cell averages of the polynomial sum coef[k] x^k over [i, i+1] h
+    and its values at the upper faces."""
+    P = np.polynomial.Polynomial(coef)
+    I = P.integ()
+    x = np.arange(n + 1) * h
+    avg = (I(x[1:]) - I(x[:-1])) / h
+    return avg, P(x[1:])
+
+
+@pytest.mark.parametrize("rec,deg", [("mp5", 4), ("mp7", 6), ("weno5yc", 2), ("weno5ns", 2)])
+def test_more_schemes_polynomial_exactness(rec, deg):
+    """On monotone polynomial data the MP limiter stays inactive, so MP5 and
+    MP7 return their fifth- and seventh-order face values, exact for
+    polynomials of degree 4 and 6; the WENO5 variants are convex
+    combinations of three stencils that are each exact for parabolas."""
+    n = 24
+    rng = np.random.default_rng(21)
+    for _ in range(5):
+        # monotone increasing on [0, n]: positive derivative coefficients
+        coef = np.concatenate([[rng.uniform(-1, 1)], rng.uniform(0.1, 1.0, deg) / n ** np.arange(deg)])
+        avg, face = _cell_averages(coef, n)
+        p = Params(ndims=3, equation_system="hd", reconstruction=rec, limiter="mm", ncells=16,
+                   nghosts=4, cfl=0.3)
+        fl, fr = ol.oracle_reconstruct(p, avg[None, :])
+        lo, hi = 4, n - 4            # interior formulas
+        np.testing.assert_allclose(fl[0, lo:hi], face[lo:hi], rtol=0, atol=1e-11 * np.abs(face).max())
+        np.testing.assert_allclose(fr[0, lo - 1:hi - 1], face[lo - 1:hi - 1], rtol=0,
+                                   atol=1e-11 * np.abs(face).max())

@@ -176,3 +176,42 @@ def test_amr_ghost_fill_and_run(nd, eq, rec, lim, ta, plim, maxlev):
     to = o.run(3)
     np.testing.assert_array_equal(to, tr)
     np.testing.assert_array_equal(o.get_blocks(), u1)
+
+
+# ------------------------------------------------------ more 1D schemes
+@needs_ref
+@pytest.mark.parametrize("rec", ["weno5yc", "weno5ns", "mp5", "mp7"])
+def test_more_schemes_fresh_pencils(rec):
+    rng = np.random.default_rng(17)
+    fp = np.concatenate([rng.normal(size=(6, 24)), np.cumsum(rng.normal(size=(4, 24)), axis=1),
+                         np.sin(np.linspace(0, 6, 24))[None, :] * np.ones((2, 1)),
+                         np.where(np.arange(24) < 12, 1.0, 0.1)[None, :] * np.ones((2, 1))])
+    for lim in ("mm", "mc"):
+        p = Params(ndims=3, equation_system="mhd", reconstruction=rec, limiter=lim, ncells=16,
+                   nghosts=4, cfl=0.3)
+        fl, fr = ol.oracle_reconstruct(p, fp)
+        rl, rr = ol.ref_reconstruct(p, fp)
+        np.testing.assert_array_equal(fl, rl)
+        np.testing.assert_array_equal(fr, rr)
+
+
+@needs_ref
+@pytest.mark.parametrize("rec", ["weno5yc", "weno5ns", "mp5", "mp7"])
+@pytest.mark.parametrize("nd,eq", [(2, "mhd"), (2, "hd"), (3, "mhd")])
+def test_more_schemes_runs(rec, nd, eq):
+    p = Params(ndims=nd, equation_system=eq, reconstruction=rec, limiter="mm",
+               riemann_solver="hlld" if eq == "mhd" else "hllc", time_advance="rk3",
+               gamma=5.0 / 3.0 if eq == "mhd" else 1.4, cfl=0.3, ncells=8, nghosts=4,
+               xblocks=3, yblocks=2, zblocks=2)
+    if eq == "hd":
+        u0 = problems.blast_hd_2d(p)
+    elif nd == 2:
+        u0 = problems.orszag_tang_2d(p, seed=5)
+    else:
+        u0 = problems.turbulent_mhd_3d(p, seed=5)
+    ur, tr = ol.ref_run(p, u0, 3)
+    o = ol.OracleRun(p)
+    o.set_state(u0)
+    to = o.run(3)
+    np.testing.assert_array_equal(to, tr)
+    np.testing.assert_array_equal(o.get_state(), ur)
