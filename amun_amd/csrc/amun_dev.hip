@@ -107,8 +107,7 @@ int validate(const amun_cfg& c, int* nv) {
     if (!(c.tolerance > 0.0) || c.nr_maxit < 1 || c.nr_extra < 0 || !(c.pmin >= 0.0))
       return fail(500, "tolerance, nr_maxit, nr_extra or pmin out of range");
   } else {
-    bool rs_ok = c.riemann_solver == AMUN_RS_HLL ||
-                 (c.riemann_solver == AMUN_RS_HLLC && c.eqsys == AMUN_EQSYS_HD) ||
+    bool rs_ok = c.riemann_solver == AMUN_RS_HLL || c.riemann_solver == AMUN_RS_HLLC ||
                  (c.riemann_solver == AMUN_RS_HLLD && c.eqsys == AMUN_EQSYS_MHD);
     if (!rs_ok) return fail(300, "riemann_solver does not match the equation system");
   }

@@ -510,6 +510,108 @@ __device__ __forceinline__ void riemann_hllc(const EosConst<T>& e, T cmax, T cma
   }
 }
 
+// riemann_mhd_adi_hllc, schemes.F90:3305-3571: one intermediate state on
+// each side of the contact (GLM pre-step as HLL; its own Bx = 0 branch)
+template <class P, typename T>
+__device__ __forceinline__ void riemann_hllc_mhd(const EosConst<T>& e, T cmax, T cmax2,
+                                                 T* ql, T* qr, T* f) {
+  constexpr int nv = P::nv;
+  T ul[nv], ur[nv], fl[nv], fr[nv], wl[nv], wr[nv], ui[nv];
+  T clm, clp, crm, crp;
+  glm_upwind<P>(ql, qr, cmax);
+  prim2cons<P>(e, ql, ul);
+  prim2cons<P>(e, qr, ur);
+  fluxspeed<P>(e, cmax2, ql, ul, fl, clm, clp);
+  fluxspeed<P>(e, cmax2, qr, ur, fr, crm, crp);
+  const T sl = tmin(clm, crm), sr = tmax(clp, crp);
+  if (sl >= T(0)) {
+#pragma unroll
+    for (int p = 0; p < nv; p++) f[p] = fl[p];
+    return;
+  }
+  if (sr <= T(0)) {
+#pragma unroll
+    for (int p = 0; p < nv; p++) f[p] = fr[p];
+    return;
+  }
+  const T srml = sr - sl, rsrml = rcp(srml);
+#pragma unroll
+  for (int p = 0; p < nv; p++) {
+    wl[p] = sl * ul[p] - fl[p];
+    wr[p] = sr * ur[p] - fr[p];
+  }
+  const T rdn = rcp(wr[P::idn] - wl[P::idn]);
+  const T sm = (wr[P::imx] - wl[P::imx]) * rdn;
+  const T bx = ql[P::ibx], b2 = ql[P::ibx] * qr[P::ibx];
+  if (b2 > T(0)) {
+    const T pt = T(0.5) * ((wl[P::idn] * sm - wl[P::imx]) + (wr[P::idn] * sm - wr[P::imx])) + b2;
+    const T vy = (wr[P::imy] - wl[P::imy]) * rdn;
+    const T vz = (wr[P::imz] - wl[P::imz]) * rdn;
+    const T by = (wr[P::iby] - wl[P::iby]) * rsrml;
+    const T bz = (wr[P::ibz] - wl[P::ibz]) * rsrml;
+    const T vb = (sm * bx + vy * by) + vz * bz;
+    if (sm > T(0)) {
+      const T rslmm = rcp(sl - sm);
+      ui[P::idn] = wl[P::idn] * rslmm;
+      ui[P::imx] = ui[P::idn] * sm;
+      ui[P::imy] = ui[P::idn] * vy;
+      ui[P::imz] = ui[P::idn] * vz;
+      ui[P::ibx] = bx; ui[P::iby] = by; ui[P::ibz] = bz; ui[P::ibp] = ql[P::ibp];
+      ui[P::ien] = ((wl[P::ien] + sm * pt) - bx * vb) * rslmm;
+#pragma unroll
+      for (int p = 0; p < nv; p++) f[p] = sl * ui[p] - wl[p];
+    } else if (sm < T(0)) {
+      const T rsrmm = rcp(sr - sm);
+      ui[P::idn] = wr[P::idn] * rsrmm;
+      ui[P::imx] = ui[P::idn] * sm;
+      ui[P::imy] = ui[P::idn] * vy;
+      ui[P::imz] = ui[P::idn] * vz;
+      ui[P::ibx] = bx; ui[P::iby] = by; ui[P::ibz] = bz; ui[P::ibp] = qr[P::ibp];
+      ui[P::ien] = ((wr[P::ien] + sm * pt) - bx * vb) * rsrmm;
+#pragma unroll
+      for (int p = 0; p < nv; p++) f[p] = sr * ui[p] - wr[p];
+    } else {
+#pragma unroll
+      for (int p = 0; p < nv; p++) f[p] = (sl * wr[p] - sr * wl[p]) * rsrml;
+    }
+  } else {
+    const T pt = T(0.5) * ((wl[P::idn] * sm - wl[P::imx]) + (wr[P::idn] * sm - wr[P::imx]));
+    if (sm > T(0)) {
+      const T rslmm = rcp(sl - sm);
+      ui[P::idn] = wl[P::idn] * rslmm;
+      ui[P::imx] = ui[P::idn] * sm;
+      ui[P::imy] = ui[P::idn] * ql[P::ivy];
+      ui[P::imz] = ui[P::idn] * ql[P::ivz];
+      ui[P::ibx] = T(0); ui[P::iby] = wl[P::iby] * rslmm; ui[P::ibz] = wl[P::ibz] * rslmm;
+      ui[P::ibp] = ql[P::ibp];
+      ui[P::ien] = (wl[P::ien] + sm * pt) * rslmm;
+#pragma unroll
+      for (int p = 0; p < nv; p++) f[p] = sl * ui[p] - wl[p];
+    } else if (sm < T(0)) {
+      const T rsrmm = rcp(sr - sm);
+      ui[P::idn] = wr[P::idn] * rsrmm;
+      ui[P::imx] = ui[P::idn] * sm;
+      ui[P::imy] = ui[P::idn] * qr[P::ivy];
+      ui[P::imz] = ui[P::idn] * qr[P::ivz];
+      ui[P::ibx] = T(0); ui[P::iby] = wr[P::iby] * rsrmm; ui[P::ibz] = wr[P::ibz] * rsrmm;
+      ui[P::ibp] = qr[P::ibp];
+      ui[P::ien] = (wr[P::ien] + sm * pt) * rsrmm;
+#pragma unroll
+      for (int p = 0; p < nv; p++) f[p] = sr * ui[p] - wr[p];
+    } else {
+      f[P::idn] = T(0);
+      f[P::imx] = T(-0.5) * (wl[P::imx] + wr[P::imx]);
+      f[P::imy] = T(0);
+      f[P::imz] = T(0);
+      f[P::ibx] = fl[P::ibx];
+      f[P::iby] = T(0);
+      f[P::ibz] = T(0);
+      f[P::ibp] = fl[P::ibp];
+      f[P::ien] = T(0);
+    }
+  }
+}
+
 // star state of the HLLD fan from a wave vector w (outer state, speed diff
 // smm = s - sm, divisor dv = smm*w_rho - Bx^2), schemes.F90:3740-3760
 template <class P, typename T>

@@ -1496,6 +1496,100 @@ static void riemann_hd_hllc(const ocfg* o, double* ql, double* qr, double* f)
   }
 }
 
+/* riemann_mhd_adi_hllc, schemes.F90:3305-3571 (Gurski / Li HLLC-MHD with
+ * the GLM pre-step; its own Bx = 0 branch) */
+static void riemann_mhd_hllc(const ocfg* o, double* ql, double* qr, double* f)
+{
+  const int idn = o->idn, imx = o->imx, imy = o->imy, imz = o->imz;
+  const int ivy = o->ivy, ivz = o->ivz;
+  const int ibx = o->ibx, iby = o->iby, ibz = o->ibz, ibp = o->ibp, ien = o->ien;
+  double ul[NVMAX], ur[NVMAX], fl[NVMAX], fr[NVMAX], wl[NVMAX], wr[NVMAX], ui[NVMAX];
+  double clm, clp, crm, crp, sl, sr, sm, srml, slmm, srmm;
+  double dn, bx, bp, b2, pt, vy, vz, by, bz, vb;
+  int p, nv = o->nv;
+
+  bx = 0.5 * ((qr[ibx] + ql[ibx]) - (qr[ibp] - ql[ibp]) / o->cmax);
+  bp = 0.5 * ((qr[ibp] + ql[ibp]) - (qr[ibx] - ql[ibx]) * o->cmax);
+  ql[ibx] = bx; qr[ibx] = bx;
+  ql[ibp] = bp; qr[ibp] = bp;
+
+  prim2cons(o, ql, ul);
+  prim2cons(o, qr, ur);
+  fluxspeed(o, ql, ul, fl, &clm, &clp);
+  fluxspeed(o, qr, ur, fr, &crm, &crp);
+  sl = fmin(clm, crm);
+  sr = fmax(clp, crp);
+  if (sl >= 0.0) {
+    for (p = 0; p < nv; p++) f[p] = fl[p];
+    return;
+  }
+  if (sr <= 0.0) {
+    for (p = 0; p < nv; p++) f[p] = fr[p];
+    return;
+  }
+  srml = sr - sl;
+  for (p = 0; p < nv; p++) {
+    wl[p] = sl * ul[p] - fl[p];
+    wr[p] = sr * ur[p] - fr[p];
+  }
+  dn = wr[idn] - wl[idn];
+  sm = (wr[imx] - wl[imx]) / dn;
+  bx = ql[ibx];
+  b2 = ql[ibx] * qr[ibx];
+  if (b2 > 0.0) {
+    pt = 0.5 * ((wl[idn] * sm - wl[imx]) + (wr[idn] * sm - wr[imx])) + b2;
+    vy = (wr[imy] - wl[imy]) / dn;
+    vz = (wr[imz] - wl[imz]) / dn;
+    by = (wr[iby] - wl[iby]) / srml;
+    bz = (wr[ibz] - wl[ibz]) / srml;
+    vb = (sm * bx + vy * by) + vz * bz;
+    if (sm > 0.0) {
+      slmm = sl - sm;
+      ui[idn] = wl[idn] / slmm;
+      ui[imx] = ui[idn] * sm; ui[imy] = ui[idn] * vy; ui[imz] = ui[idn] * vz;
+      ui[ibx] = bx; ui[iby] = by; ui[ibz] = bz; ui[ibp] = ql[ibp];
+      ui[ien] = ((wl[ien] + sm * pt) - bx * vb) / slmm;
+      for (p = 0; p < nv; p++) f[p] = sl * ui[p] - wl[p];
+    } else if (sm < 0.0) {
+      srmm = sr - sm;
+      ui[idn] = wr[idn] / srmm;
+      ui[imx] = ui[idn] * sm; ui[imy] = ui[idn] * vy; ui[imz] = ui[idn] * vz;
+      ui[ibx] = bx; ui[iby] = by; ui[ibz] = bz; ui[ibp] = qr[ibp];
+      ui[ien] = ((wr[ien] + sm * pt) - bx * vb) / srmm;
+      for (p = 0; p < nv; p++) f[p] = sr * ui[p] - wr[p];
+    } else {
+      for (p = 0; p < nv; p++) f[p] = (sl * wr[p] - sr * wl[p]) / srml;
+    }
+  } else {
+    pt = 0.5 * ((wl[idn] * sm - wl[imx]) + (wr[idn] * sm - wr[imx]));
+    if (sm > 0.0) {
+      slmm = sl - sm;
+      ui[idn] = wl[idn] / slmm;
+      ui[imx] = ui[idn] * sm; ui[imy] = ui[idn] * ql[ivy]; ui[imz] = ui[idn] * ql[ivz];
+      ui[ibx] = 0.0; ui[iby] = wl[iby] / slmm; ui[ibz] = wl[ibz] / slmm; ui[ibp] = ql[ibp];
+      ui[ien] = (wl[ien] + sm * pt) / slmm;
+      for (p = 0; p < nv; p++) f[p] = sl * ui[p] - wl[p];
+    } else if (sm < 0.0) {
+      srmm = sr - sm;
+      ui[idn] = wr[idn] / srmm;
+      ui[imx] = ui[idn] * sm; ui[imy] = ui[idn] * qr[ivy]; ui[imz] = ui[idn] * qr[ivz];
+      ui[ibx] = 0.0; ui[iby] = wr[iby] / srmm; ui[ibz] = wr[ibz] / srmm; ui[ibp] = qr[ibp];
+      ui[ien] = (wr[ien] + sm * pt) / srmm;
+      for (p = 0; p < nv; p++) f[p] = sr * ui[p] - wr[p];
+    } else {
+      f[idn] = 0.0;
+      f[imx] = -0.5 * (wl[imx] + wr[imx]);
+      f[imy] = 0.0;
+      f[imz] = 0.0;
+      f[ibx] = fl[ibx];
+      f[iby] = 0.0;
+      f[ibz] = 0.0;
+      f[ibp] = fl[ibp];
+      f[ien] = 0.0;
+    }
+  }
+}
+
 /* riemann_mhd_adi_hlld, schemes.F90:3596-4203 */
 static void riemann_mhd_hlld(const ocfg* o, double* ql, double* qr, double* f)
 {
@@ -1698,7 +1792,8 @@ static void riemann_mhd_hlld(const ocfg* o, double* ql, double* qr, double* f)
 static int riemann_check(const amun_cfg* c)
 {
   if (c->riemann_solver == AMUN_RS_HLL) return 0;
-  if (c->riemann_solver == AMUN_RS_HLLC && c->eqsys == AMUN_EQSYS_HD) return 0;
+  if (c->riemann_solver == AMUN_RS_HLLC &&
+      (c->eqsys == AMUN_EQSYS_HD || c->eqsys == AMUN_EQSYS_MHD)) return 0;
   if (c->riemann_solver == AMUN_RS_HLLD && c->eqsys == AMUN_EQSYS_MHD) return 0;
   /* SRMHD: any name other than "hllc" selects HLL (schemes.F90:479-498) */
   if (c->eqsys == AMUN_EQSYS_SRMHD && c->riemann_solver != AMUN_RS_HLLC) return 0;
@@ -1710,7 +1805,10 @@ static void riemann(const ocfg* o, double* ql, double* qr, double* f)
 {
   if (o->sr) { riemann_hll(o, ql, qr, f); return; }   /* riemann_srmhd_adi_hll */
   switch (o->c.riemann_solver) {
-  case AMUN_RS_HLLC: riemann_hd_hllc(o, ql, qr, f); break;
+  case AMUN_RS_HLLC:
+    if (o->mhd) riemann_mhd_hllc(o, ql, qr, f);
+    else riemann_hd_hllc(o, ql, qr, f);
+    break;
   case AMUN_RS_HLLD: riemann_mhd_hlld(o, ql, qr, f); break;
   default:           riemann_hll(o, ql, qr, f); break;
   }

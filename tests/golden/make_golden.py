@@ -257,6 +257,43 @@ def main_amr():
         json.dump(index, fh, indent=1)
 
 
+def main_hllc_mhd():
+    """riemann_mhd_adi_hllc through update_flux (zero limiter), random states
+    with weak, zero and strong Bx, and an Orszag-Tang run (own generator)."""
+    rng = np.random.default_rng(20261206)
+    pf = Params(ndims=2, equation_system="mhd", reconstruction="tvd", limiter="zero",
+                riemann_solver="hllc", gamma=5.0 / 3.0, ncells=8, nghosts=2)
+    qb = np.zeros((4, 1, 12, 12, 9))
+    qb[..., 0] = rng.uniform(0.2, 2.0, qb.shape[:-1])
+    qb[..., 1:4] = rng.normal(scale=1.5, size=qb.shape[:-1] + (3,))
+    qb[..., 8] = rng.uniform(0.1, 2.0, qb.shape[:-1])
+    qb[..., 4:7] = rng.normal(size=qb.shape[:-1] + (3,))
+    qb[..., 7] = rng.normal(scale=0.1, size=qb.shape[:-1])
+    qb[1, ..., 4] = 1e-6 * rng.normal(size=qb.shape[1:-1])
+    qb[2, ..., 4] = 0.0
+    qb[2, ..., 7] = 0.0                 # Bx* = 0 exactly: the Bx = 0 branch
+    qb[3, ..., 1] = qb[3, ..., 1] * 5.0
+    f = ol.ref_update_flux(pf, qb, cmax=2.3)
+    np.savez_compressed(os.path.join(HERE, "unit_flux_mhd_hllc.npz"), q=qb, f=f, cmax=2.3,
+                        params=np.array(pf.to_params_in()))
+    p = Params(ndims=2, equation_system="mhd", reconstruction="tvd", limiter="mc",
+               riemann_solver="hllc", time_advance="rk3", gamma=5.0 / 3.0, cfl=0.4,
+               ncells=16, nghosts=2, xblocks=2, yblocks=2)
+    u0 = problems.orszag_tang_2d(p, seed=8)
+    nsteps = 10
+    u, tr = ol.ref_run(p, u0, nsteps)
+    name = f"hllc_mhd_ot2d_32_{nsteps}"
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), params=np.array(p.to_params_in()),
+                        ndims=2, nsteps=nsteps, stride=3, trace=tr, sub=u[:, :, ::3, ::3],
+                        sha_u0=sha(u0), sha_u=sha(u))
+    with open(os.path.join(HERE, "index.json")) as fh:
+        index = json.load(fh)
+    index[name] = {"steps": nsteps, "sha_u": sha(u), "dt_last": float(tr[-1, 0])}
+    with open(os.path.join(HERE, "index.json"), "w") as fh:
+        json.dump(index, fh, indent=1)
+    print(name, "dt", tr[-1, 0])
+
+
 SCHEMES = ("weno5yc", "weno5ns", "mp5", "mp7")
 
 
@@ -309,6 +346,9 @@ def main():
         return
     if sys.argv[1:] == ["schemes"]:
         main_schemes()
+        return
+    if sys.argv[1:] == ["hllc_mhd"]:
+        main_hllc_mhd()
         return
     index = {}
     for name, p, builder, nsteps, stride in run_cases():
@@ -377,6 +417,7 @@ def main():
     main_ssprk()
     main_amr()
     main_schemes()
+    main_hllc_mhd()
 
 
 if __name__ == "__main__":

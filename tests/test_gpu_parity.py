@@ -69,7 +69,8 @@ def test_c2_one_step_32():
 
 
 @pytest.mark.parametrize("nd", [2, 3])
-@pytest.mark.parametrize("eq,rs", [("hd", "hll"), ("hd", "hllc"), ("mhd", "hll"), ("mhd", "hlld")])
+@pytest.mark.parametrize("eq,rs", [("hd", "hll"), ("hd", "hllc"), ("mhd", "hll"), ("mhd", "hlld"),
+                                   ("mhd", "hllc")])
 @pytest.mark.parametrize("rec,lim", [("tvd", "mc"), ("tvd", "mm"), ("weno5z", "mm")])
 @pytest.mark.parametrize("ta", ["euler", "rk2", "rk3"])
 def test_matrix_one_step(nd, eq, rs, rec, lim, ta):

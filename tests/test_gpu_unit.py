@@ -40,7 +40,8 @@ def _states(nv, n, rng, degenerate=True):
     return q
 
 
-CASES = [("hd", "hll"), ("hd", "hllc"), ("mhd", "hll"), ("mhd", "hlld"), ("srmhd", "hll")]
+CASES = [("hd", "hll"), ("hd", "hllc"), ("mhd", "hll"), ("mhd", "hlld"), ("mhd", "hllc"),
+         ("srmhd", "hll")]
 
 
 @pytest.mark.parametrize("eq,rs", CASES)
@@ -85,7 +86,8 @@ def test_equation_of_state_golden(eq):
     np.testing.assert_allclose(cp, g["cp"], rtol=1e-12, atol=1e-13)
 
 
-@pytest.mark.parametrize("eq,rs", [("hd", "hllc"), ("mhd", "hlld"), ("srmhd", "hll")])
+@pytest.mark.parametrize("eq,rs", [("hd", "hllc"), ("mhd", "hlld"), ("mhd", "hllc"),
+                                   ("srmhd", "hll")])
 def test_riemann_golden_via_update_flux_states(eq, rs):
     """The golden update_flux fixtures use the zero limiter, so every face is
     the Riemann problem of its two cell values: the device solver on those

@@ -84,7 +84,8 @@ def test_equation_of_state(eq):
         np.testing.assert_array_equal(a, b)
 
 
-@pytest.mark.parametrize("eq,rs", [("hd", "hll"), ("hd", "hllc"), ("mhd", "hll"), ("mhd", "hlld")])
+@pytest.mark.parametrize("eq,rs", [("hd", "hll"), ("hd", "hllc"), ("mhd", "hll"), ("mhd", "hlld"),
+                                   ("mhd", "hllc")])
 def test_riemann_via_update_flux(eq, rs):
     g = load(f"unit_flux_{eq}_{rs}")
     p = Params.from_text(str(g["params"]), ndims=2)
@@ -220,6 +221,19 @@ def test_more_schemes_runs_bit_exact(rec):
     g = load(f"{rec}_ot2d_32_8")
     p = params_of(g)
     u0 = problems.orszag_tang_2d(p, seed=6)
+    assert sha(u0) == str(g["sha_u0"])
+    o = ol.OracleRun(p)
+    o.set_state(u0)
+    tr = o.run(int(g["nsteps"]))
+    np.testing.assert_array_equal(tr, g["trace"])
+    assert sha(o.get_state()) == str(g["sha_u"])
+
+
+def test_hllc_mhd_run_bit_exact():
+    """riemann_mhd_adi_hllc in an Orszag-Tang run (TVD-MC, SSPRK(3,3))."""
+    g = load("hllc_mhd_ot2d_32_10")
+    p = params_of(g)
+    u0 = problems.orszag_tang_2d(p, seed=8)
     assert sha(u0) == str(g["sha_u0"])
     o = ol.OracleRun(p)
     o.set_state(u0)
