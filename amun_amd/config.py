@@ -24,7 +24,7 @@ RECON = {"tvd": 1, "weno5z": 5, "weno5-z": 5, "weno5yc": 6, "weno5-yc": 6, "weno
 LIMITER = {"zero": 0, "mm": 1, "minmod": 1, "mc": 2, "monotonized_central": 2,
            "sb": 3, "superbee": 3, "vl": 4, "vanleer": 4, "va": 5,
            "vanalbada": 5}
-RIEMANN = {"hll": 1, "hllc": 2, "hlld": 3}
+RIEMANN = {"hll": 1, "hllc": 2, "hlld": 3, "roe": 4}
 INTEGRATOR = {"euler": 0, "rk2": 1, "rk3": 2, "ssprk(m,2)": 3, "rk3.4": 4, "ssprk(4,3)": 4,
               "rk3.5": 5, "ssprk(5,3)": 5}
 PRECISION = {"fp64": 0, "mixed": 1}

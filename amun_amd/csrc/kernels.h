@@ -186,7 +186,7 @@ enum {
   REC_WENO5NS = AMUN_REC_WENO5NS, REC_MP5 = AMUN_REC_MP5, REC_MP7 = AMUN_REC_MP7,
   REC_GP = AMUN_REC_GP
 };
-enum { RS_HLL = AMUN_RS_HLL, RS_HLLC = AMUN_RS_HLLC, RS_HLLD = AMUN_RS_HLLD };
+enum { RS_HLL = AMUN_RS_HLL, RS_HLLC = AMUN_RS_HLLC, RS_HLLD = AMUN_RS_HLLD, RS_ROE = AMUN_RS_ROE };
 
 template <class P, int RS, typename T>
 __device__ __forceinline__ void riemann(const EosConst<T>& e, const SrConst& sc, T cmax,
@@ -194,6 +194,7 @@ __device__ __forceinline__ void riemann(const EosConst<T>& e, const SrConst& sc,
   if constexpr (P::sr) riemann_sr_hll<P, T>(sc, cmax, cmax2, ql, qr, f);
   else if constexpr (RS == RS_HLLD) riemann_hlld<P, T>(e, cmax, cmax2, ql, qr, f);
   else if constexpr (RS == RS_HLLC && P::mhd) riemann_hllc_mhd<P, T>(e, cmax, cmax2, ql, qr, f);
+  else if constexpr (RS == RS_ROE) riemann_roe_hd<P, T>(e, cmax, cmax2, ql, qr, f);
   else if constexpr (RS == RS_HLLC) riemann_hllc<P, T>(e, cmax, cmax2, ql, qr, f);
   else riemann_hll<P, T>(e, cmax, cmax2, ql, qr, f);
 }
@@ -1027,6 +1028,7 @@ bool pick_rec_t(Dispatch& d, const amun_cfg& c) {
   if (c.eqsys == AMUN_EQSYS_HD) {
     if (c.riemann_solver == AMUN_RS_HLL) return pick_nd<HD, RS_HLL, REC, T>(d, c.ndims);
     if (c.riemann_solver == AMUN_RS_HLLC) return pick_nd<HD, RS_HLLC, REC, T>(d, c.ndims);
+    if (c.riemann_solver == AMUN_RS_ROE) return pick_nd<HD, RS_ROE, REC, T>(d, c.ndims);
   } else if (c.eqsys == AMUN_EQSYS_MHD) {
     if (c.riemann_solver == AMUN_RS_HLL) return pick_nd<MHD, RS_HLL, REC, T>(d, c.ndims);
     if (c.riemann_solver == AMUN_RS_HLLD) return pick_nd<MHD, RS_HLLD, REC, T>(d, c.ndims);

@@ -510,6 +510,73 @@ __device__ __forceinline__ void riemann_hllc(const EosConst<T>& e, T cmax, T cma
   }
 }
 
+// riemann_hd_adi_roe (schemes.F90:1541-1661) with the eigenvectors of
+// esystem_roe_hd_adi (equations.F90:2075-2196) at the Roe average; the wave
+// contributions are added from the left wave for a non-negative Roe velocity
+// and from the right one otherwise, as the reference does
+template <class P, typename T>
+__device__ __forceinline__ void riemann_roe_hd(const EosConst<T>& e, T cmax, T cmax2,
+                                               T* ql, T* qr, T* f) {
+  static_assert(!P::mhd, "the Roe solver on the device is the hydrodynamic one");
+  constexpr int nv = P::nv;
+  T ul[nv], ur[nv], fl[nv], fr[nv];
+  T cm, cp;
+  prim2cons<P>(e, ql, ul);
+  prim2cons<P>(e, qr, ur);
+  fluxspeed<P>(e, cmax2, ql, ul, fl, cm, cp);
+  fluxspeed<P>(e, cmax2, qr, ur, fr, cm, cp);
+  const T sdl = tsqrt(ql[P::idn]), sdr = tsqrt(qr[P::idn]), sds = sdl + sdr;
+  const T rsds = rcp(sds);
+  const T vx = (sdl * ql[P::ivx] + sdr * qr[P::ivx]) * rsds;
+  const T vy = (sdl * ql[P::ivy] + sdr * qr[P::ivy]) * rsds;
+  const T vz = (sdl * ql[P::ivz] + sdr * qr[P::ivz]) * rsds;
+  const T hh = (dv(ul[P::ien] + ql[P::ipr], sdl) + dv(ur[P::ien] + qr[P::ipr], sdr)) * rsds;
+  const T vv = (vx * vx + vy * vy) + vz * vz, vh = T(0.5) * vv;
+  const T c2 = e.gammam1 * (hh - vh), na = T(0.5) * rcp(c2), cc = tsqrt(c2);
+  const T vc = vx * cc, ng = na * e.gammam1, nd = T(2) * ng, nw = na * vc;
+  const T nh = (na * e.gammam1) * vh, nc = na * cc;
+  const T c[5] = {vx - cc, vx, vx, vx, vx + cc};
+  if (c[0] >= T(0)) {
+#pragma unroll
+    for (int p = 0; p < nv; p++) f[p] = fl[p];
+    return;
+  }
+  if (c[4] <= T(0)) {
+#pragma unroll
+    for (int p = 0; p < nv; p++) f[p] = fr[p];
+    return;
+  }
+  T du[nv];
+#pragma unroll
+  for (int p = 0; p < nv; p++) du[p] = ur[p] - ul[p];
+  // wave strengths: al_k = sum_p l(p, k) du(p)
+  T al[5];
+  al[0] = (nh + nw) * du[P::idn] + (-ng * vx - nc) * du[P::imx] + (-ng * vy) * du[P::imy] +
+          (-ng * vz) * du[P::imz] + ng * du[P::ien];
+  al[1] = -vy * du[P::idn] + du[P::imy];
+  al[2] = -vz * du[P::idn] + du[P::imz];
+  al[3] = (T(1) - ng * vv) * du[P::idn] + nd * vx * du[P::imx] + nd * vy * du[P::imy] +
+          nd * vz * du[P::imz] - nd * du[P::ien];
+  al[4] = (nh - nw) * du[P::idn] + (-ng * vx + nc) * du[P::imx] + (-ng * vy) * du[P::imy] +
+          (-ng * vz) * du[P::imz] + ng * du[P::ien];
+  // right eigenvectors r(k, :)
+  const T r[5][5] = {{T(1), vx - cc, vy, vz, hh - vc},
+                     {T(0), T(0), T(1), T(0), vy},
+                     {T(0), T(0), T(0), T(1), vz},
+                     {T(1), vx, vy, vz, vh},
+                     {T(1), vx + cc, vy, vz, hh + vc}};
+#pragma unroll
+  for (int p = 0; p < nv; p++) f[p] = T(0.5) * (fl[p] + fr[p]);
+  const bool fwd = vx >= T(0);
+#pragma unroll
+  for (int m = 0; m < 5; m++) {
+    const int k = fwd ? m : 4 - m;
+    const T xx = -(T(0.5) * al[k] * tabs(c[k]));
+#pragma unroll
+    for (int p = 0; p < nv; p++) f[p] = f[p] + xx * r[k][p];
+  }
+}
+
 // riemann_mhd_adi_hllc, schemes.F90:3305-3571: one intermediate state on
 // each side of the contact (GLM pre-step as HLL; its own Bx = 0 branch)
 template <class P, typename T>

@@ -121,6 +121,8 @@ program amun_host
     cfg%riemann_solver = AMUN_RS_HLLC
   case("hlld", "HLLD")
     cfg%riemann_solver = AMUN_RS_HLLD
+  case("roe", "ROE")
+    cfg%riemann_solver = AMUN_RS_ROE
   case default
     cfg%riemann_solver = -1
   end select

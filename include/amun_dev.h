@@ -60,6 +60,7 @@ extern "C" {
 #define AMUN_RS_HLL        1
 #define AMUN_RS_HLLC       2
 #define AMUN_RS_HLLD       3
+#define AMUN_RS_ROE        4   /* Roe, riemann_hd_adi_roe schemes.F90:1541 */
 
 /* integrators: evolution.F90:156-200 (parameter "time_advance") */
 #define AMUN_TA_EULER      0

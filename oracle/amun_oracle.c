@@ -1496,6 +1496,98 @@ static void riemann_hd_hllc(const ocfg* o, double* ql, double* qr, double* f)
   }
 }
 
+/* riemann_hd_adi_roe (schemes.F90:1541-1661) with esystem_roe_hd_adi
+ * (equations.F90:2075-2196): Roe averages, the left (l[conserved][wave]) and
+ * right (r[wave][conserved]) eigenvectors, and the flux 1/2 (fl + fr) minus
+ * the wave contributions, summed from the left wave for a non-negative Roe
+ * velocity and from the right one otherwise */
+static void riemann_hd_roe(const ocfg* o, double* ql, double* qr, double* f)
+{
+  const int idn = o->idn, ivx = o->ivx, ivy = o->ivy, ivz = o->ivz, ien = o->ien;
+  const int nv = o->nv;
+  double ul[NVMAX], ur[NVMAX], fl[NVMAX], fr[NVMAX], qi[NVMAX], ci[NVMAX], al[NVMAX];
+  double li[NVMAX][NVMAX], ri[NVMAX][NVMAX];
+  double sdl, sdr, sds, vv, vh, c2, na, cc, vc, ng, nd, nw, nh, nc, xx, cm_, cp_;
+  int p, k;
+  prim2cons(o, ql, ul);
+  prim2cons(o, qr, ur);
+  fluxspeed(o, ql, ul, fl, &cm_, &cp_);
+  fluxspeed(o, qr, ur, fr, &cm_, &cp_);
+  sdl = sqrt(ql[idn]);
+  sdr = sqrt(qr[idn]);
+  sds = sdl + sdr;
+  qi[idn] = sdl * sdr;
+  qi[ivx] = (sdl * ql[ivx] + sdr * qr[ivx]) / sds;
+  qi[ivy] = (sdl * ql[ivy] + sdr * qr[ivy]) / sds;
+  qi[ivz] = (sdl * ql[ivz] + sdr * qr[ivz]) / sds;
+  qi[o->ipr] = ((ul[ien] + ql[o->ipr]) / sdl + (ur[ien] + qr[o->ipr]) / sdr) / sds;
+  /* esystem_roe_hd_adi: q(ien) is the Roe enthalpy */
+  for (p = 0; p < nv; p++)
+    for (k = 0; k < nv; k++) { li[p][k] = 0.0; ri[p][k] = 0.0; }
+  vv = (qi[ivx] * qi[ivx] + qi[ivy] * qi[ivy]) + qi[ivz] * qi[ivz];
+  vh = 0.5 * vv;
+  c2 = o->gammam1 * (qi[ien] - vh);
+  na = 0.5 / c2;
+  cc = sqrt(c2);
+  vc = qi[ivx] * cc;
+  ng = na * o->gammam1;
+  nd = 2.0 * ng;
+  nw = na * vc;
+  nh = (na * o->gammam1) * vh;
+  nc = na * cc;
+  ci[0] = qi[ivx] - cc;
+  ci[1] = qi[ivx];
+  ci[2] = qi[ivx];
+  ci[3] = qi[ivx];
+  ci[4] = qi[ivx] + cc;
+  li[ivy][1] = 1.0;
+  li[ivz][2] = 1.0;
+  ri[0][idn] = 1.0; ri[1][ivy] = 1.0; ri[2][ivz] = 1.0; ri[3][idn] = 1.0; ri[4][idn] = 1.0;
+  li[idn][0] = nh + nw;
+  li[ivx][0] = -ng * qi[ivx] - nc;
+  li[ivy][0] = -ng * qi[ivy];
+  li[ivz][0] = -ng * qi[ivz];
+  li[ien][0] = ng;
+  li[idn][1] = -qi[ivy];
+  li[idn][2] = -qi[ivz];
+  li[idn][3] = 1.0 - ng * vv;
+  li[ivx][3] = nd * qi[ivx];
+  li[ivy][3] = nd * qi[ivy];
+  li[ivz][3] = nd * qi[ivz];
+  li[ien][3] = -nd;
+  li[idn][4] = nh - nw;
+  li[ivx][4] = -ng * qi[ivx] + nc;
+  li[ivy][4] = -ng * qi[ivy];
+  li[ivz][4] = -ng * qi[ivz];
+  li[ien][4] = ng;
+  ri[0][ivx] = qi[ivx] - cc; ri[0][ivy] = qi[ivy]; ri[0][ivz] = qi[ivz]; ri[0][ien] = qi[ien] - vc;
+  ri[1][ien] = qi[ivy];
+  ri[2][ien] = qi[ivz];
+  ri[3][ivx] = qi[ivx]; ri[3][ivy] = qi[ivy]; ri[3][ivz] = qi[ivz]; ri[3][ien] = vh;
+  ri[4][ivx] = qi[ivx] + cc; ri[4][ivy] = qi[ivy]; ri[4][ivz] = qi[ivz]; ri[4][ien] = qi[ien] + vc;
+  if (ci[0] >= 0.0) {
+    for (p = 0; p < nv; p++) f[p] = fl[p];
+  } else if (ci[nv - 1] <= 0.0) {
+    for (p = 0; p < nv; p++) f[p] = fr[p];
+  } else {
+    for (k = 0; k < nv; k++) al[k] = 0.0;
+    for (p = 0; p < nv; p++)
+      for (k = 0; k < nv; k++) al[k] = al[k] + li[p][k] * (ur[p] - ul[p]);
+    for (k = 0; k < nv; k++) f[k] = 0.5 * (fl[k] + fr[k]);
+    if (qi[ivx] >= 0.0) {
+      for (p = 0; p < nv; p++) {
+        xx = -(0.5 * al[p] * fabs(ci[p]));
+        for (k = 0; k < nv; k++) f[k] = f[k] + xx * ri[p][k];
+      }
+    } else {
+      for (p = nv - 1; p >= 0; p--) {
+        xx = -(0.5 * al[p] * fabs(ci[p]));
+        for (k = 0; k < nv; k++) f[k] = f[k] + xx * ri[p][k];
+      }
+    }
+  }
+}
+
 /* riemann_mhd_adi_hllc, schemes.F90:3305-3571 (Gurski / Li HLLC-MHD with
  * the GLM pre-step; its own Bx = 0 branch) */
 static void riemann_mhd_hllc(const ocfg* o, double* ql, double* qr, double* f)
@@ -1794,6 +1886,7 @@ static int riemann_check(const amun_cfg* c)
   if (c->riemann_solver == AMUN_RS_HLL) return 0;
   if (c->riemann_solver == AMUN_RS_HLLC &&
       (c->eqsys == AMUN_EQSYS_HD || c->eqsys == AMUN_EQSYS_MHD)) return 0;
+  if (c->riemann_solver == AMUN_RS_ROE && c->eqsys == AMUN_EQSYS_HD) return 0;
   if (c->riemann_solver == AMUN_RS_HLLD && c->eqsys == AMUN_EQSYS_MHD) return 0;
   /* SRMHD: any name other than "hllc" selects HLL (schemes.F90:479-498) */
   if (c->eqsys == AMUN_EQSYS_SRMHD && c->riemann_solver != AMUN_RS_HLLC) return 0;
@@ -1810,6 +1903,7 @@ static void riemann(const ocfg* o, double* ql, double* qr, double* f)
     else riemann_hd_hllc(o, ql, qr, f);
     break;
   case AMUN_RS_HLLD: riemann_mhd_hlld(o, ql, qr, f); break;
+  case AMUN_RS_ROE:  riemann_hd_roe(o, ql, qr, f); break;
   default:           riemann_hll(o, ql, qr, f); break;
   }
 }

@@ -294,6 +294,45 @@ def main_hllc_mhd():
     print(name, "dt", tr[-1, 0])
 
 
+def main_roe_hd():
+    """riemann_hd_adi_roe through update_flux (zero limiter): random states,
+    supersonic pairs on both sides (the pure left and right fluxes), Roe
+    velocities of both signs and pairs at rest; and a blast run (own
+    generator)."""
+    rng = np.random.default_rng(20261207)
+    pf = Params(ndims=2, equation_system="hd", reconstruction="tvd", limiter="zero",
+                riemann_solver="roe", gamma=1.4, ncells=8, nghosts=2)
+    qb = np.zeros((4, 1, 12, 12, 5))
+    qb[..., 0] = rng.uniform(0.2, 2.0, qb.shape[:-1])
+    qb[..., 1:4] = rng.normal(scale=1.5, size=qb.shape[:-1] + (3,))
+    qb[..., 4] = rng.uniform(0.1, 2.0, qb.shape[:-1])
+    qb[1, ..., 1] = qb[1, ..., 1] * 6.0           # supersonic both ways
+    qb[2, ..., 1:4] = 0.0                         # gas at rest
+    qb[3, ..., 1] = 1e-3 * rng.normal(size=qb.shape[1:-1])   # Roe velocity ~ 0
+    f = ol.ref_update_flux(pf, qb, cmax=2.3)
+    np.savez_compressed(os.path.join(HERE, "unit_flux_hd_roe.npz"), q=qb, f=f, cmax=2.3,
+                        params=np.array(pf.to_params_in()))
+    with open(os.path.join(HERE, "index.json")) as fh:
+        index = json.load(fh)
+    for nd, rec, ta, nsteps in ((2, "tvd", "rk2", 20), (3, "weno5z", "rk3", 4)):
+        p = Params(ndims=nd, equation_system="hd", reconstruction=rec,
+                   limiter="mc" if rec == "tvd" else "mm", riemann_solver="roe",
+                   time_advance=ta, gamma=1.4, cfl=0.4 if nd == 2 else 0.3,
+                   ncells=16 if nd == 2 else 8, nghosts=2 if rec == "tvd" else 4,
+                   xblocks=2, yblocks=2, zblocks=2)
+        u0 = problems.blast_hd_2d(p, seed=9) if nd == 2 else hd3_blast(p)
+        u, tr = ol.ref_run(p, u0, nsteps)
+        name = f"roe_hd{nd}d_blast_{nsteps}"
+        np.savez_compressed(os.path.join(HERE, name + ".npz"),
+                            params=np.array(p.to_params_in()), ndims=nd, nsteps=nsteps,
+                            stride=3, trace=tr, sha_u0=sha(u0), sha_u=sha(u),
+                            sub=u[:, ::3, ::3, ::3] if nd == 3 else u[:, :, ::3, ::3])
+        index[name] = {"steps": nsteps, "sha_u": sha(u), "dt_last": float(tr[-1, 0])}
+        print(name, "dt", tr[-1, 0])
+    with open(os.path.join(HERE, "index.json"), "w") as fh:
+        json.dump(index, fh, indent=1)
+
+
 SCHEMES = ("weno5yc", "weno5ns", "mp5", "mp7")
 
 
@@ -349,6 +388,9 @@ def main():
         return
     if sys.argv[1:] == ["hllc_mhd"]:
         main_hllc_mhd()
+        return
+    if sys.argv[1:] == ["roe_hd"]:
+        main_roe_hd()
         return
     index = {}
     for name, p, builder, nsteps, stride in run_cases():
@@ -418,6 +460,7 @@ def main():
     main_amr()
     main_schemes()
     main_hllc_mhd()
+    main_roe_hd()
 
 
 if __name__ == "__main__":

@@ -23,7 +23,7 @@ def _ot(n=32, ta="rk3", rs="hlld", lim="mc"):
 
 
 CASES = [("ot", "rk3", "hlld", "mc"), ("ot", "rk2", "hll", "mm"), ("ot", "euler", "hlld", "vl"),
-         ("hd", "rk2", "hllc", "mc"), ("mhd3d", "rk3", "hlld", "mc")]
+         ("hd", "rk2", "hllc", "mc"), ("mhd3d", "rk3", "hlld", "mc"), ("hd", "rk3", "roe", "vl")]
 
 
 def _case(name, ta, rs, lim):

@@ -70,7 +70,7 @@ def test_c2_one_step_32():
 
 @pytest.mark.parametrize("nd", [2, 3])
 @pytest.mark.parametrize("eq,rs", [("hd", "hll"), ("hd", "hllc"), ("mhd", "hll"), ("mhd", "hlld"),
-                                   ("mhd", "hllc")])
+                                   ("mhd", "hllc"), ("hd", "roe")])
 @pytest.mark.parametrize("rec,lim", [("tvd", "mc"), ("tvd", "mm"), ("weno5z", "mm")])
 @pytest.mark.parametrize("ta", ["euler", "rk2", "rk3"])
 def test_matrix_one_step(nd, eq, rs, rec, lim, ta):
@@ -551,6 +551,33 @@ def test_ssprk_mixed_vs_oracle(ta, stages):
     assert np.abs(um[7] - uo[7]).max() <= 1e-5 * np.abs(uo[4:7]).max()
     assert err.max() <= 1e-4
     assert _l1(um[0], uo[0]) <= 1e-6
+
+
+@pytest.mark.parametrize("name", ["roe_hd2d_blast_20", "roe_hd3d_blast_4"])
+def test_roe_golden_runs(name):
+    """The HD Roe solver in blast runs against the reference's own output
+    (dt trace and a subsample of the final state)."""
+    import os
+    import sys
+    gold = os.path.join(os.path.dirname(__file__), "golden")
+    g = np.load(os.path.join(gold, name + ".npz"))
+    p = Params.from_text(str(g["params"]), ndims=int(g["ndims"]))
+    if p.ndims == 2:
+        from amun_amd.problems import blast_hd_2d
+        u0 = blast_hd_2d(p, seed=9)
+    else:
+        sys.path.insert(0, gold)
+        from make_golden import hd3_blast
+        u0 = hd3_blast(p)
+    print(f"[case] {name}", flush=True)
+    with Solver(p) as s:
+        s.upload(u0)
+        dts = s.run(int(g["nsteps"]))
+        u = s.download()
+    assert np.abs(dts - g["trace"][:, 0]).max() <= 1e-12 * g["trace"][:, 0].max()
+    st = int(g["stride"])
+    sub = u[:, ::st, ::st, ::st] if p.ndims == 3 else u[:, :, ::st, ::st]
+    assert np.abs(sub - g["sub"]).max() <= 1e-10 * np.abs(g["sub"]).max()
 
 
 # ------------------------------------------------ more 1D reconstructions

@@ -41,7 +41,7 @@ def _states(nv, n, rng, degenerate=True):
 
 
 CASES = [("hd", "hll"), ("hd", "hllc"), ("mhd", "hll"), ("mhd", "hlld"), ("mhd", "hllc"),
-         ("srmhd", "hll")]
+         ("srmhd", "hll"), ("hd", "roe")]
 
 
 @pytest.mark.parametrize("eq,rs", CASES)
@@ -87,7 +87,7 @@ def test_equation_of_state_golden(eq):
 
 
 @pytest.mark.parametrize("eq,rs", [("hd", "hllc"), ("mhd", "hlld"), ("mhd", "hllc"),
-                                   ("srmhd", "hll")])
+                                   ("srmhd", "hll"), ("hd", "roe")])
 def test_riemann_golden_via_update_flux_states(eq, rs):
     """The golden update_flux fixtures use the zero limiter, so every face is
     the Riemann problem of its two cell values: the device solver on those

@@ -85,7 +85,7 @@ def test_equation_of_state(eq):
 
 
 @pytest.mark.parametrize("eq,rs", [("hd", "hll"), ("hd", "hllc"), ("mhd", "hll"), ("mhd", "hlld"),
-                                   ("mhd", "hllc")])
+                                   ("mhd", "hllc"), ("hd", "roe")])
 def test_riemann_via_update_flux(eq, rs):
     g = load(f"unit_flux_{eq}_{rs}")
     p = Params.from_text(str(g["params"]), ndims=2)
@@ -234,6 +234,24 @@ def test_hllc_mhd_run_bit_exact():
     g = load("hllc_mhd_ot2d_32_10")
     p = params_of(g)
     u0 = problems.orszag_tang_2d(p, seed=8)
+    assert sha(u0) == str(g["sha_u0"])
+    o = ol.OracleRun(p)
+    o.set_state(u0)
+    tr = o.run(int(g["nsteps"]))
+    np.testing.assert_array_equal(tr, g["trace"])
+    assert sha(o.get_state()) == str(g["sha_u"])
+
+
+@pytest.mark.parametrize("name", ["roe_hd2d_blast_20", "roe_hd3d_blast_4"])
+def test_roe_hd_run_bit_exact(name):
+    """riemann_hd_adi_roe in blast runs (2D TVD-MC + RK2, 3D WENO5-Z + RK3)."""
+    g = load(name)
+    p = params_of(g)
+    assert p.riemann_solver == "roe"
+    import sys
+    sys.path.insert(0, GOLD)
+    from make_golden import hd3_blast
+    u0 = problems.blast_hd_2d(p, seed=9) if p.ndims == 2 else hd3_blast(p)
     assert sha(u0) == str(g["sha_u0"])
     o = ol.OracleRun(p)
     o.set_state(u0)

@@ -215,3 +215,20 @@ def test_more_schemes_runs(rec, nd, eq):
     to = o.run(3)
     np.testing.assert_array_equal(to, tr)
     np.testing.assert_array_equal(o.get_state(), ur)
+
+
+# ------------------------------------------------------------ HD Roe
+@needs_ref
+@pytest.mark.parametrize("nd,rec,ta", [(2, "tvd", "rk2"), (2, "weno5z", "ssprk(4,3)"),
+                                       (3, "tvd", "rk3")])
+def test_roe_hd_runs(nd, rec, ta):
+    p = Params(ndims=nd, equation_system="hd", reconstruction=rec, limiter="vl",
+               riemann_solver="roe", time_advance=ta, gamma=1.4, cfl=0.3, ncells=8,
+               nghosts=4 if rec == "weno5z" else 2, xblocks=3, yblocks=2, zblocks=2)
+    u0 = problems.blast_hd_2d(p, seed=13)
+    ur, tr = ol.ref_run(p, u0, 4)
+    o = ol.OracleRun(p)
+    o.set_state(u0)
+    to = o.run(4)
+    np.testing.assert_array_equal(to, tr)
+    np.testing.assert_array_equal(o.get_state(), ur)
