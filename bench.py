@@ -86,7 +86,20 @@ def main():
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the mixed-precision leg reported beside the FP64 value")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run three 1 GiB device-to-device copies (the calibration "
+                         "of the FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
+
+    if args.copy_probe:
+        import torch
+        x = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print("copied 3 x 1 GiB", flush=True)
+        return
 
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not "
