@@ -19,7 +19,7 @@ from dataclasses import dataclass, field, fields
 EQSYS = {"hd": 1, "hydrodynamic": 1, "mhd": 2, "magnetohydrodynamic": 2,
          "srmhd": 4}
 EOS = {"adi": 1, "adiabatic": 1}
-RECON = {"tvd": 1, "weno5z": 5, "weno5-z": 5, "weno5yc": 6, "weno5-yc": 6, "weno5ns": 7,
+RECON = {"tvd": 1, "weno3": 2, "limo3": 3, "ppm": 4, "weno5z": 5, "weno5-z": 5, "weno5yc": 6, "weno5-yc": 6, "weno5ns": 7,
          "weno5-ns": 7, "mp5": 11, "mp7": 12, "gp": 20}
 LIMITER = {"zero": 0, "mm": 1, "minmod": 1, "mc": 2, "monotonized_central": 2,
            "sb": 3, "superbee": 3, "vl": 4, "vanleer": 4, "va": 5,
@@ -52,6 +52,7 @@ class AmunCfg(ctypes.Structure):
         ("sgp", ctypes.c_double), ("kappa", ctypes.c_double), ("kbeta", ctypes.c_double),
         ("ngp", ctypes.c_int32), ("stages", ctypes.c_int32),
         ("prol_limiter", ctypes.c_int32), ("reserved2", ctypes.c_int32),
+        ("limo3_rad", ctypes.c_double), ("ppm_const", ctypes.c_double),
     ]
 
 
@@ -109,6 +110,8 @@ class Params:
     ngp: int = 5
     stages: int = 2
     prolongation_limiter: str = "mm"
+    limo3_rad: float = 0.5
+    ppm_const: float = 1.25
     extra: dict = field(default_factory=dict)
 
     # ---------------------------------------------------------------- derived
@@ -163,6 +166,7 @@ class Params:
         c.nr_maxit, c.nr_extra = int(self.nr_maxit), int(self.nr_extra)
         c.sgp, c.kappa, c.kbeta, c.ngp = self.sgp, self.kappa, self.kbeta, int(self.ngp)
         c.stages = int(self.stages)
+        c.limo3_rad, c.ppm_const = self.limo3_rad, self.ppm_const
         # interpolations.F90:412-428: mm, mc, sb, vl, anything else the zero slope
         c.prol_limiter = {"mm": 1, "minmod": 1, "mc": 2, "monotonized_central": 2,
                           "sb": 3, "superbee": 3, "vl": 4, "vanleer": 4}.get(
@@ -199,6 +203,7 @@ class Params:
             f"sgp = {self.sgp!r}", f"kappa = {self.kappa!r}", f"kbeta = {self.kbeta!r}",
             f"ngp = {self.ngp}", f"stages = {self.stages}",
             f"prolongation_limiter = {self.prolongation_limiter}",
+            f"limo3_rad = {self.limo3_rad!r}", f"ppm_const = {self.ppm_const!r}",
             "minlev = 1", "maxlev = 1",
             "xlbndry = periodic", "xubndry = periodic",
             "ylbndry = periodic", "yubndry = periodic",

@@ -78,7 +78,7 @@ __device__ __forceinline__ double halving(const double* __restrict__ Q, const in
     dq[d] = limiter<double>(rc.limiter, 0.5, qc - qm, qp - qc);
   }
   double f = 1.0;
-  for (;;) {
+  for (int it = 0; it < kMaxHalvings; it++) {
     double sa = fabs(dq[0]) + fabs(dq[1]);
     if (ND == 3) sa += fabs(dq[2]);
     if (!(qc <= sa)) break;

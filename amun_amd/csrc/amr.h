@@ -111,7 +111,7 @@ static __global__ void k_amr_prolong(const int4* __restrict__ task, int ntask,
       dq[d] = limiter<double>(rc.prol_limiter, 0.5, qc - at(pm[d], v), at(pp[d], v) - qc);
     if (v == 0 || v == ipr) {
       if (qc > 0.0) {
-        for (;;) {
+        for (int it = 0; it < kMaxHalvings; it++) {
           double sa = fabs(dq[0]) + fabs(dq[1]);
           if (nd == 3) sa = sa + fabs(dq[2]);
           if (!(qc <= sa)) break;
@@ -152,7 +152,8 @@ static __global__ void k_amr_prolong(const int4* __restrict__ task, int ntask,
 template <class P, int REC, int RS, int ND, int DIR, typename T>
 __device__ __forceinline__ void face_flux(const RunConst& rc, const T* __restrict__ Qp,
                                           const int* __restrict__ nbr, int b, int c, int c1,
-                                          int c2, T cmax, T cmax2, DevState* st, T* f) {
+                                          int c2, T cmax, T cmax2, DevState* st, double h,
+                                          T* f) {
   constexpr int NV = P::nv;
   using RC = Recon<P, REC, ND, DIR, T>;
   const EosConst<T> e{T(rc.gamma), T(rc.gammam1), T(rc.gammam1i)};
@@ -177,12 +178,12 @@ __device__ __forceinline__ void face_flux(const RunConst& rc, const T* __restric
   }
   int i, j, k;
   coords(c - 1, i, j, k);
-  rec.cell(rc, flm, frm, Qp, nbr, b, i, j, k, st);
+  rec.cell(rc, flm, frm, Qp, nbr, b, i, j, k, st, h);
   rec.shift();
   loadq(c + RC::R, qtmp);
   rec.load(2 * RC::R, qtmp);
   coords(c, i, j, k);
-  rec.cell(rc, flc, frc, Qp, nbr, b, i, j, k, st);
+  rec.cell(rc, flc, frc, Qp, nbr, b, i, j, k, st, h);
   T ql[NV], qr[NV], fr[NV];
 #pragma unroll
   for (int p = 0; p < NV; p++) {
@@ -202,7 +203,8 @@ template <class P, int REC, int RS, int ND, int DIR, typename T>
 static __global__ void k_amr_fineflux(const int4* __restrict__ task, int ntask, const void* Q,
                                       const int* __restrict__ nbr, RunConst rc,
                                       const RunState* __restrict__ rs, int slot,
-                                      double* __restrict__ FB, DevState* st) {
+                                      double* __restrict__ FB, DevState* st,
+                                      const int* __restrict__ blev) {
   constexpr int NV = P::nv;
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= ntask) return;
@@ -214,15 +216,16 @@ static __global__ void k_amr_fineflux(const int4* __restrict__ task, int ntask, 
   T fa[NV], fb[NV];
   const int nf = (ND == 3) ? rc.nc * rc.nc : rc.nc;
   double* out = FB + (size_t)t.x * NV * nf + t.y;
-  face_flux<P, REC, RS, ND, DIR, T>(rc, Qp, nbr, t.z, c, f1, f2, cmax, cmax2, st, fa);
-  face_flux<P, REC, RS, ND, DIR, T>(rc, Qp, nbr, t.z, c, f1 + 1, f2, cmax, cmax2, st, fb);
+  const double h = rs->h[DIR] / double(1 << blev[t.z]);   // the fine block's cell size
+  face_flux<P, REC, RS, ND, DIR, T>(rc, Qp, nbr, t.z, c, f1, f2, cmax, cmax2, st, h, fa);
+  face_flux<P, REC, RS, ND, DIR, T>(rc, Qp, nbr, t.z, c, f1 + 1, f2, cmax, cmax2, st, h, fb);
   if constexpr (ND == 2) {
 #pragma unroll
     for (int v = 0; v < NV; v++) out[(size_t)v * nf] = (double(fa[v]) + double(fb[v])) / 2.0;
   } else {
     T fc[NV], fd[NV];
-    face_flux<P, REC, RS, ND, DIR, T>(rc, Qp, nbr, t.z, c, f1, f2 + 1, cmax, cmax2, st, fc);
-    face_flux<P, REC, RS, ND, DIR, T>(rc, Qp, nbr, t.z, c, f1 + 1, f2 + 1, cmax, cmax2, st, fd);
+    face_flux<P, REC, RS, ND, DIR, T>(rc, Qp, nbr, t.z, c, f1, f2 + 1, cmax, cmax2, st, h, fc);
+    face_flux<P, REC, RS, ND, DIR, T>(rc, Qp, nbr, t.z, c, f1 + 1, f2 + 1, cmax, cmax2, st, h, fd);
     // ((l,l) + (p,p)) + ((l,p) + (p,l)) over the two transverse axes
 #pragma unroll
     for (int v = 0; v < NV; v++)

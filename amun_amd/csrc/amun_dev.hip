@@ -81,6 +81,9 @@ bool pick(Dispatch& d, const amun_cfg& c) {
   case AMUN_REC_MP5: return pick_mp5(d, c);
   case AMUN_REC_MP7: return pick_mp7(d, c);
   case AMUN_REC_GP: return pick_gp(d, c);
+  case AMUN_REC_WENO3: return pick_weno3(d, c);
+  case AMUN_REC_LIMO3: return pick_limo3(d, c);
+  case AMUN_REC_PPM: return pick_ppm(d, c);
   default: return false;
   }
 }
@@ -93,7 +96,8 @@ int validate(const amun_cfg& c, int* nv) {
   if (c.eos != AMUN_EOS_ADI) return fail(110, "unsupported equation_of_state");
   switch (c.reconstruction) {
   case AMUN_REC_TVD: case AMUN_REC_WENO5Z: case AMUN_REC_WENO5YC: case AMUN_REC_WENO5NS:
-  case AMUN_REC_MP5: case AMUN_REC_MP7: case AMUN_REC_GP: break;
+  case AMUN_REC_MP5: case AMUN_REC_MP7: case AMUN_REC_GP: case AMUN_REC_WENO3:
+  case AMUN_REC_LIMO3: case AMUN_REC_PPM: break;
   default: return fail(200, "unsupported reconstruction");
   }
   if (c.reconstruction == AMUN_REC_GP && c.ngp != 5)
@@ -124,12 +128,13 @@ int validate(const amun_cfg& c, int* nv) {
   // lower-order end stencils then stay in the outer ghost layers)
   int need = 2;
   switch (c.reconstruction) {
-  case AMUN_REC_WENO5Z: case AMUN_REC_WENO5YC: case AMUN_REC_WENO5NS: case AMUN_REC_MP7: need = 4; break;
+  case AMUN_REC_WENO5Z: case AMUN_REC_WENO5YC: case AMUN_REC_WENO5NS: case AMUN_REC_MP7:
+  case AMUN_REC_PPM: need = 4; break;
   case AMUN_REC_GP: case AMUN_REC_MP5: need = 3; break;
   }
   if (c.nghosts < need)
-    return fail(600, "nghosts too small for the reconstruction stencil (WENO5 and MP7 need 4, "
-                     "GP and MP5 3, TVD 2)");
+    return fail(600, "nghosts too small for the reconstruction stencil (WENO5, MP7 and PPM "
+                     "need 4, GP and MP5 3, TVD, WENO3 and LimO3 2)");
   if (c.ncells < 3) return fail(500, "ncells too small for the stencil");
   if (!(c.gamma > 1.0) || !(c.cfl > 0.0)) return fail(500, "gamma must exceed 1 and cfl be positive");
   return 0;
@@ -214,7 +219,11 @@ RunConst make_rc(const amun_cfg& cfg) {
   if (cfg.time_advance == AMUN_TA_SSPRK43) cfl = 2.0 * cfg.cfl;
   if (cfg.time_advance == AMUN_TA_SSPRK53) cfl = 2.65062919143939 * cfg.cfl;
   rc.cfl = cfl;
-  rc.eps = cfg.eps;
+  // the LimO3 scheme raises the module's eps to at least 1e-12
+  // (interpolations.F90:248); every user of eps then sees the raised value
+  rc.eps = cfg.reconstruction == AMUN_REC_LIMO3 ? std::fmax(1.0e-12, cfg.eps) : cfg.eps;
+  rc.rad = cfg.limo3_rad;
+  rc.ppm_const = cfg.ppm_const;
   rc.decay = std::exp(-cfg.alpha * cfl);
   rc.sc.gamma = cfg.gamma;
   rc.sc.gammaxi = rc.gammam1 / cfg.gamma;       // equations.F90:843
@@ -281,6 +290,8 @@ void amun_dev_default_cfg(amun_cfg* c) {
   c->ngp = 5;
   c->stages = 2;
   c->prol_limiter = AMUN_LIM_MINMOD;
+  c->limo3_rad = 0.5;
+  c->ppm_const = 1.25;
 }
 
 static int init_device(amun_ctx* c, const std::vector<int32_t>& nbr,
@@ -330,6 +341,9 @@ int amun_dev_init_batch(const amun_cfg* cfg, int32_t nruns, const amun_grid* gri
     R.dxi[0] = 1.0 / adx;
     R.dxi[1] = 1.0 / ady;
     R.dxi[2] = (nd == 3) ? 1.0 / adz : 1.0;
+    R.h[0] = adx;
+    R.h[1] = ady;
+    R.h[2] = adz;
     R.dxmin = std::fmin(adx, ady);
     if (nd == 3) R.dxmin = std::fmin(R.dxmin, adz);
   }
@@ -667,6 +681,9 @@ int amun_dev_init_amr(const amun_cfg* cfg, int32_t nblk, const int32_t* meta, am
   R.dxi[0] = 1.0 / adx1;
   R.dxi[1] = 1.0 / ady1;
   R.dxi[2] = nd == 3 ? 1.0 / adz1 : 1.0;
+  R.h[0] = adx1;
+  R.h[1] = ady1;
+  R.h[2] = adz1;
   R.dxmin = std::fmin((G.xmax - G.xmin) / (G.xblocks * (nc * ff)),
                       (G.ymax - G.ymin) / (G.yblocks * (nc * ff)));
   if (nd == 3) R.dxmin = std::fmin(R.dxmin, (G.zmax - G.zmin) / (G.zblocks * (nc * ff)));
@@ -1065,7 +1082,7 @@ static int amr_fill(amun_ctx* c, int qcur, int slot) {
     if (A.nf[d])
       hipLaunchKernelGGL(c->disp.fineflux[d], dim3((A.nf[d] + 255) / 256), bl, 0, c->stream,
                          A.ftask[d], A.nf[d], (const void*)c->Q[qcur], c->nbr, c->rc, c->rs,
-                         slot, A.FB, A.dummy);
+                         slot, A.FB, A.dummy, (const int*)A.blev);
   HIP_TRY(hipGetLastError());
   return 0;
 }
@@ -1887,6 +1904,34 @@ int amun_dev_unit_riemann(const amun_cfg* cfg, int32_t n, double cmax, const dou
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipDeviceSynchronize());
   HIP_TRY(hipMemcpy(f, df, m * sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int amun_dev_unit_reconstruct(const amun_cfg* cfg, int32_t n, int32_t npencil, double h,
+                              const double* fp, double* fl, double* fr) {
+  Dispatch d;
+  RunConst rc;
+  int r = unit_setup(cfg, d, rc);
+  if (r) return r;
+  if (!d.unit_recon) return fail(600, "the TVD reconstruction has no pencil form");
+  if (n < 8 || n > 4096 || npencil < 1 || !fp || !fl || !fr || !(h > 0.0))
+    return fail(500, "bad argument");
+  const size_t m = (size_t)n * npencil;
+  DevBufs B;
+  double *df, *dl, *dr;
+  HIP_TRY(B.alloc(&df, m));
+  HIP_TRY(B.alloc(&dl, m));
+  HIP_TRY(B.alloc(&dr, m));
+  HIP_TRY(hipMemcpy(df, fp, m * sizeof(double), hipMemcpyHostToDevice));
+  // entries outside the window range keep what the caller passed in
+  HIP_TRY(hipMemcpy(dl, fl, m * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(dr, fr, m * sizeof(double), hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(d.unit_recon, dim3((unsigned)((npencil + 63) / 64)), dim3(64), 0, 0, n,
+                     npencil, h, df, dl, dr, rc);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipDeviceSynchronize());
+  HIP_TRY(hipMemcpy(fl, dl, m * sizeof(double), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(fr, dr, m * sizeof(double), hipMemcpyDeviceToHost));
   return 0;
 }
 

@@ -82,6 +82,7 @@ struct RunState {
   long long step;
   double dxmin;                     // smallest cell size (evolution.F90:467-476)
   double dxi[3];                    // inverse cell sizes
+  double h[3];                      // cell sizes (the reconstructions' h)
   double fix_dt, fix_cmax;          // > 0: dt and cmax held at these values
 };
 
@@ -95,6 +96,7 @@ struct RunConst {
   double cfl, eps, decay;
   double cgp[5];                    // Gaussian-process weights (prepare_gp)
   double kappa, kbeta;              // MP limiter of the GP reconstruction
+  double rad, ppm_const;            // LimO3 radius, PPM curvature ratio
   SrConst sc;                       // relativistic primitive solver
 };
 
@@ -182,7 +184,8 @@ __device__ __forceinline__ constexpr int rot(int p) {
 }
 
 enum {
-  REC_TVD = AMUN_REC_TVD, REC_WENO5Z = AMUN_REC_WENO5Z, REC_WENO5YC = AMUN_REC_WENO5YC,
+  REC_TVD = AMUN_REC_TVD, REC_WENO3 = AMUN_REC_WENO3, REC_LIMO3 = AMUN_REC_LIMO3,
+  REC_PPM = AMUN_REC_PPM, REC_WENO5Z = AMUN_REC_WENO5Z, REC_WENO5YC = AMUN_REC_WENO5YC,
   REC_WENO5NS = AMUN_REC_WENO5NS, REC_MP5 = AMUN_REC_MP5, REC_MP7 = AMUN_REC_MP7,
   REC_GP = AMUN_REC_GP
 };
@@ -266,6 +269,10 @@ __device__ __forceinline__ double combine_delta(int kind, double u0, double u1, 
   }
 }
 
+// bound of the positivity halving loops (more than the 2098 halvings that
+// take the largest double below the smallest positive one)
+constexpr int kMaxHalvings = 2200;
+
 // reconstruction of one cell along DIR from its primitive window:
 // flc = state at the upper face, frc = state at the lower face
 template <class P, int REC, int ND, int DIR, typename T>
@@ -288,7 +295,7 @@ struct Recon<P, REC_WENO5Z, ND, DIR, T> {
       for (int m = 0; m < 4; m++) w[v][m] = w[v][m + 1];
   }
   __device__ __forceinline__ void cell(const RunConst& rc, T* flc, T* frc, const T*,
-                                       const int*, int, int, int, int, DevState*) {
+                                       const int*, int, int, int, int, DevState*, double) {
     const T eps = T(rc.eps);
 #pragma unroll
     for (int v = 0; v < P::nv; v++) weno5z_cell<T>(w[v], eps, flc[v], frc[v]);
@@ -300,7 +307,7 @@ struct Recon<P, REC_WENO5Z, ND, DIR, T> {
 template <class P, int ND, int DIR, typename T>
 struct Recon<P, REC_WENO5YC, ND, DIR, T> : Recon<P, REC_WENO5Z, ND, DIR, T> {
   __device__ __forceinline__ void cell(const RunConst& rc, T* flc, T* frc, const T*,
-                                       const int*, int, int, int, int, DevState*) {
+                                       const int*, int, int, int, int, DevState*, double) {
     const T eps = T(rc.eps);
 #pragma unroll
     for (int v = 0; v < P::nv; v++) weno5z_cell<T, true>(this->w[v], eps, flc[v], frc[v]);
@@ -310,7 +317,7 @@ struct Recon<P, REC_WENO5YC, ND, DIR, T> : Recon<P, REC_WENO5Z, ND, DIR, T> {
 template <class P, int ND, int DIR, typename T>
 struct Recon<P, REC_WENO5NS, ND, DIR, T> : Recon<P, REC_WENO5Z, ND, DIR, T> {
   __device__ __forceinline__ void cell(const RunConst& rc, T* flc, T* frc, const T*,
-                                       const int*, int, int, int, int, DevState*) {
+                                       const int*, int, int, int, int, DevState*, double) {
     const T eps = T(rc.eps);
 #pragma unroll
     for (int v = 0; v < P::nv; v++) weno5ns_cell<T>(this->w[v], eps, flc[v], frc[v]);
@@ -322,7 +329,7 @@ struct Recon<P, REC_WENO5NS, ND, DIR, T> : Recon<P, REC_WENO5Z, ND, DIR, T> {
 template <class P, int ND, int DIR, typename T>
 struct Recon<P, REC_MP5, ND, DIR, T> : Recon<P, REC_WENO5Z, ND, DIR, T> {
   __device__ __forceinline__ void cell(const RunConst& rc, T* flc, T* frc, const T*,
-                                       const int*, int, int, int, int, DevState*) {
+                                       const int*, int, int, int, int, DevState*, double) {
     const T kappa = T(rc.kappa), kbeta = T(rc.kbeta), eps = T(rc.eps);
 #pragma unroll
     for (int v = 0; v < P::nv; v++) mp_cell<T, 2>(this->w[v], kappa, kbeta, eps, flc[v], frc[v]);
@@ -346,7 +353,7 @@ struct Recon<P, REC_MP7, ND, DIR, T> {
       for (int m = 0; m < 6; m++) w[v][m] = w[v][m + 1];
   }
   __device__ __forceinline__ void cell(const RunConst& rc, T* flc, T* frc, const T*,
-                                       const int*, int, int, int, int, DevState*) {
+                                       const int*, int, int, int, int, DevState*, double) {
     const T kappa = T(rc.kappa), kbeta = T(rc.kbeta), eps = T(rc.eps);
 #pragma unroll
     for (int v = 0; v < P::nv; v++) mp_cell<T, 3>(w[v], kappa, kbeta, eps, flc[v], frc[v]);
@@ -358,11 +365,59 @@ struct Recon<P, REC_MP7, ND, DIR, T> {
 template <class P, int ND, int DIR, typename T>
 struct Recon<P, REC_GP, ND, DIR, T> : Recon<P, REC_WENO5Z, ND, DIR, T> {
   __device__ __forceinline__ void cell(const RunConst& rc, T* flc, T* frc, const T*,
-                                       const int*, int, int, int, int, DevState*) {
+                                       const int*, int, int, int, int, DevState*, double) {
     const T cgp[5] = {T(rc.cgp[0]), T(rc.cgp[1]), T(rc.cgp[2]), T(rc.cgp[3]), T(rc.cgp[4])};
     const T kappa = T(rc.kappa), kbeta = T(rc.kbeta), eps = T(rc.eps);
 #pragma unroll
     for (int v = 0; v < P::nv; v++) gp_cell<T>(this->w[v], cgp, kappa, kbeta, eps, flc[v], frc[v]);
+  }
+};
+
+// WENO3 and LimO3 (interfaces_dir -> reconstruct_weno3 / _limo3): a
+// three-cell window; both weigh by the cell size h of the sweep direction
+template <class P, int ND, int DIR, typename T>
+struct Recon<P, REC_WENO3, ND, DIR, T> {
+  static constexpr int R = 1;
+  static constexpr int H = 2;
+  T w[P::nv][3];
+  __device__ __forceinline__ void load(int slotm, const T* q) {
+#pragma unroll
+    for (int v = 0; v < P::nv; v++) w[v][slotm] = q[v];
+  }
+  __device__ __forceinline__ void shift() {
+#pragma unroll
+    for (int v = 0; v < P::nv; v++) {
+      w[v][0] = w[v][1];
+      w[v][1] = w[v][2];
+    }
+  }
+  __device__ __forceinline__ void cell(const RunConst&, T* flc, T* frc, const T*,
+                                       const int*, int, int, int, int, DevState*, double h) {
+    const T h2 = T(h * h);
+#pragma unroll
+    for (int v = 0; v < P::nv; v++) weno3_cell<T>(w[v], h2, flc[v], frc[v]);
+  }
+};
+
+template <class P, int ND, int DIR, typename T>
+struct Recon<P, REC_LIMO3, ND, DIR, T> : Recon<P, REC_WENO3, ND, DIR, T> {
+  __device__ __forceinline__ void cell(const RunConst& rc, T* flc, T* frc, const T*,
+                                       const int*, int, int, int, int, DevState*, double h) {
+    const double rdx = rc.rad * h;
+    const T eps = T(rc.eps), rdx2 = T(rdx * rdx);
+#pragma unroll
+    for (int v = 0; v < P::nv; v++) limo3_cell<T>(this->w[v], eps, rdx2, flc[v], frc[v]);
+  }
+};
+
+// PPM (interfaces_dir -> reconstruct_ppm): the seven-cell window of MP7
+template <class P, int ND, int DIR, typename T>
+struct Recon<P, REC_PPM, ND, DIR, T> : Recon<P, REC_MP7, ND, DIR, T> {
+  __device__ __forceinline__ void cell(const RunConst& rc, T* flc, T* frc, const T*,
+                                       const int*, int, int, int, int, DevState*, double) {
+    const T pc = T(rc.ppm_const);
+#pragma unroll
+    for (int v = 0; v < P::nv; v++) ppm_cell<T>(this->w[v], pc, flc[v], frc[v]);
   }
 };
 
@@ -388,7 +443,7 @@ struct Recon<P, REC_TVD, ND, DIR, T> {
   __device__ __forceinline__ void cell(const RunConst& rc, T* flc, T* frc,
                                        const T* __restrict__ Q,
                                        const int* __restrict__ nbr, int b, int i, int j,
-                                       int k, DevState* st) {
+                                       int k, DevState* st, double) {
 #pragma unroll
     for (int v = 0; v < P::nv; v++) {
       const T qc = w[v][1];
@@ -406,7 +461,9 @@ struct Recon<P, REC_TVD, ND, DIR, T> {
           dqt[d] = limiter<T>(rc.limiter, T(0.5), qc - qm, qp - qc);
         }
         if (qc > T(0)) {
-          for (;;) {
+          // a finite slope falls below qc within kMaxHalvings halvings; the
+          // bound only stops an infinite one (the reference loops forever)
+          for (int it = 0; it < kMaxHalvings; it++) {
             T sa = tabs(dqt[0]) + tabs(dqt[1]);
             if (ND == 3) sa += tabs(dqt[2]);
             if (!(qc <= sa)) break;
@@ -568,6 +625,8 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a, RunConst rc) {
       }
     };
 
+    // cell size along DIR at this block's level (coordinates.F90:330-341)
+    const double hdir = a.blev ? R->h[DIR] / double(1 << a.blev[b]) : R->h[DIR];
     RC rec;
     T qtmp[NV];
     // prime the window for cell c = -1: cells -1-R .. -1+R-1
@@ -621,7 +680,7 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a, RunConst rc) {
       T flc[NV], frc[NV];
       int ci, cj, ck;
       coords(c, ci, cj, ck);
-      rec.cell(rc, flc, frc, Qp, a.nbr, b, ci, cj, ck, a.st);
+      rec.cell(rc, flc, frc, Qp, a.nbr, b, ci, cj, ck, a.st, hdir);
       if (c >= 0) {
         // face c-1/2: left state from cell c-1, right state from cell c
         T ql[NV], qr[NV], fr[NV], f[NV];
@@ -900,6 +959,37 @@ __global__ void k_unit_riemann(int n, double cmaxd, const double* __restrict__ q
   for (int v = 0; v < NV; v++) f[(size_t)i * NV + v] = double(g[v]);
 }
 
+// the 1D reconstruction of the sweeps (reconstruct_states of interfaces_dir,
+// interpolations.F90:867-959) on pencils: one thread per pencil slides the
+// window of the sweep along it; cell i writes fl(i) (its upper face) and
+// fr(i-1) (its lower face) for the cells whose window lies in the pencil
+template <class P, int REC, typename T>
+__global__ void k_unit_recon(int n, int npen, double h, const double* __restrict__ fp,
+                             double* __restrict__ fl, double* __restrict__ fr, RunConst rc) {
+  using RC = Recon<P, REC, 3, 0, T>;
+  constexpr int NV = P::nv;
+  const int p = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (p >= npen) return;
+  const double* f = fp + (size_t)p * n;
+  RC rec;
+  T q[NV];
+  for (int m = 0; m < 2 * RC::R; m++) {
+#pragma unroll
+    for (int v = 0; v < NV; v++) q[v] = T(f[m]);
+    rec.load(m, q);
+  }
+  for (int i = RC::R; i < n - RC::R; i++) {
+    if (i > RC::R) rec.shift();
+#pragma unroll
+    for (int v = 0; v < NV; v++) q[v] = T(f[i + RC::R]);
+    rec.load(2 * RC::R, q);
+    T a[NV], b[NV];
+    rec.cell(rc, a, b, nullptr, nullptr, 0, 0, 0, 0, nullptr, h);
+    fl[(size_t)p * n + i] = double(a[0]);
+    fr[(size_t)p * n + i - 1] = double(b[0]);
+  }
+}
+
 // u = prim2cons(q), f/cm/cp = fluxspeed(q, u) in T; q2 = cons2prim(u) in
 // double (the kernels recover primitives in FP64 in every mode; q2 enters
 // holding the previous primitives, kept by a failed relativistic recovery)
@@ -964,6 +1054,7 @@ struct Dispatch {
   void (*pack_u)(const void*, double*, const int*, RunConst, int) = nullptr;
   void (*pack_q)(const void*, double*, const int*, RunConst, int) = nullptr;
   void (*unit_riemann)(int, double, const double*, const double*, double*, RunConst) = nullptr;
+  void (*unit_recon)(int, int, double, const double*, double*, double*, RunConst) = nullptr;
   void (*unit_eqs)(int, double, const double*, double*, double*, double*, double*, double*,
                    RunConst, DevState*) = nullptr;
   // tangent-linear / adjoint kernels (FP64, TVD reconstruction, HD and MHD)
@@ -980,7 +1071,7 @@ struct Dispatch {
                    double, double, double) = nullptr;
   // multi-level meshes: averaged fine fluxes of coarse faces, per direction
   void (*fineflux[3])(const int4*, int, const void*, const int*, RunConst, const RunState*, int,
-                      double*, DevState*) = {nullptr, nullptr, nullptr};
+                      double*, DevState*, const int*) = {nullptr, nullptr, nullptr};
 };
 
 
@@ -996,6 +1087,8 @@ void fill(Dispatch& d) {
   d.pack_u = k_pack_blocks<P::nv, double>;
   d.pack_q = k_pack_blocks<P::nv, T>;
   d.unit_riemann = k_unit_riemann<P, RS, T>;
+  // TVD reads the transverse slopes of the block: no pencil form
+  if constexpr (REC != REC_TVD) d.unit_recon = k_unit_recon<P, REC, T>;
   d.unit_eqs = k_unit_eqs<P, T>;
   d.fineflux[0] = k_amr_fineflux<P, REC, RS, ND, 0, T>;
   d.fineflux[1] = k_amr_fineflux<P, REC, RS, ND, 1, T>;
@@ -1054,5 +1147,8 @@ bool pick_weno5ns(Dispatch& d, const amun_cfg& c);
 bool pick_mp5(Dispatch& d, const amun_cfg& c);
 bool pick_mp7(Dispatch& d, const amun_cfg& c);
 bool pick_gp(Dispatch& d, const amun_cfg& c);
+bool pick_weno3(Dispatch& d, const amun_cfg& c);
+bool pick_limo3(Dispatch& d, const amun_cfg& c);
+bool pick_ppm(Dispatch& d, const amun_cfg& c);
 
 }  // namespace amun_dev

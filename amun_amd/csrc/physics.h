@@ -50,8 +50,12 @@ template <> __device__ __forceinline__ double tsqrt<double>(double x) { return s
 template <> __device__ __forceinline__ float tsqrt<float>(float x) { return sqrtf(x); }
 
 template <typename T> __device__ __forceinline__ T tabs(T x) { return x < T(0) ? -x : x; }
-template <typename T> __device__ __forceinline__ T tmin(T a, T b) { return b < a ? b : a; }
-template <typename T> __device__ __forceinline__ T tmax(T a, T b) { return b > a ? b : a; }
+// MIN and MAX as the reference's compiler evaluates them (a < b ? a : b,
+// a > b ? a : b): a NaN first operand yields the second, a NaN second
+// operand propagates -- the wave speeds of a negative reconstructed pressure
+// take the same path as in the reference
+template <typename T> __device__ __forceinline__ T tmin(T a, T b) { return a < b ? a : b; }
+template <typename T> __device__ __forceinline__ T tmax(T a, T b) { return a > b ? a : b; }
 template <typename T> __device__ __forceinline__ T tsign(T x, T a) {
   return copysign(x, a);
 }
@@ -288,6 +292,126 @@ __device__ __forceinline__ void weno5ns_cell(const T f[5], T eps, T& fl, T& fr) 
   const T pc = a21 * f[3] + a22 * f[2] + a23 * f[1];
   const T pr = a31 * f[2] + a32 * f[1] + a33 * f[0];
   fr = (wl * pl + wr * pr) + wc * pc;
+}
+
+// reconstruct_weno3 (interpolations.F90:1469-1574): three-cell window
+// f[0..2] around the cell f[1]; h2 is the squared cell size of the sweep
+// direction (it enters the smoothness weights)
+template <typename T>
+__device__ __forceinline__ void weno3_cell(const T f[3], T h2, T& fl, T& fr) {
+  const T dp = T(2.0 / 3.0), dm = T(1.0 / 3.0);
+  const T dfl = f[1] - f[0], dfr = f[2] - f[1];
+  T ww = (dfr - dfl) * (dfr - dfl);
+  const T ap = T(1) + ww / (dfr * dfr + h2);
+  const T am = T(1) + ww / (dfl * dfl + h2);
+  T wp = dp * ap, wm = dm * am;
+  ww = T(2) * (wp + wm);
+  fl = (wp * (f[1] + f[2]) + wm * (-f[0] + T(3) * f[1])) / ww;
+  wp = dp * am;
+  wm = dm * ap;
+  ww = T(2) * (wp + wm);
+  fr = (wp * (f[1] + f[0]) + wm * (-f[2] + T(3) * f[1])) / ww;
+}
+
+// reconstruct_limo3 (interpolations.F90:1598-1719): LimO3 (Cada & Torrilhon
+// 2009) blended with the plain third-order slope by the indicator over a
+// radius of limo3_rad cells; rdx2 = (limo3_rad h)^2
+template <typename T>
+__device__ __forceinline__ T limo3_slope(T th, T xl, T xi) {
+  const T f1 = (T(2) + th) / T(6);
+  const T f2 = th >= T(0) ? tmax(T(0), tmin(tmin(f1, th), T(0.8)))
+                          : tmax(T(0), tmin(f1, T(-0.25) * th));
+  return xl * f1 + xi * f2;
+}
+
+template <typename T>
+__device__ __forceinline__ void limo3_cell(const T f[3], T eps, T rdx2, T& fl, T& fr) {
+  const T dfl = f[1] - f[0], dfr = f[2] - f[1];
+  const T et = (dfl * dfl + dfr * dfr) / rdx2;
+  const T xi = tmax(T(0), T(0.5) * tmin(T(2), T(1) + (et - T(1)) / eps));
+  const T xl = T(1) - xi;
+  fl = tabs(dfr) > eps ? f[1] + dfr * limo3_slope(dfl / dfr, xl, xi) : f[1];
+  fr = tabs(dfl) > eps ? f[1] - dfl * limo3_slope(dfr / dfl, xl, xi) : f[1];
+}
+
+// reconstruct_ppm (interpolations.F90:1746-1913): PPM face values with the
+// Colella-Sekora limiters, on the window f[0..6] of the cell f[3] (cells
+// -3..3; the cell states use -2..3).  fi(j) is the limited value at the upper
+// face of window cell j, d2(j) the second difference at window cell j.
+template <typename T>
+__device__ __forceinline__ T ppm_d2(const T* f, int j) {
+  return (f[j + 1] + f[j - 1]) - T(2) * f[j];
+}
+
+template <typename T>
+__device__ __forceinline__ T ppm_face(const T* f, int j, T pc) {
+  T fi = (T(7) * (f[j] + f[j + 1]) - (f[j - 1] + f[j + 2])) / T(12);
+  if ((f[j + 1] - fi) * (fi - f[j]) < T(0)) {
+    const T c = T(3) * ((f[j + 1] + f[j]) - T(2) * fi);
+    const T m = ppm_d2(f, j), p = ppm_d2(f, j + 1);
+    T l = T(0);
+    if (tmin(tmin(c, m), p) * tmax(tmax(c, m), p) > T(0))
+      l = tsign(tmin(pc * tmin(tabs(m), tabs(p)), tabs(c)), c);
+    fi = T(0.5) * (f[j] + f[j + 1]) - l / T(6);
+  }
+  return fi;
+}
+
+template <typename T>
+__device__ __forceinline__ void ppm_cell(const T f[7], T pc, T& fl, T& fr) {
+  const T fim = ppm_face(f, 2, pc), fic = ppm_face(f, 3, pc), fip = ppm_face(f, 4, pc);
+  const T fc = f[3];
+  T alm = fim - fc, alp = fic - fc;
+  const bool cm = tabs(alm) >= T(2) * tabs(alp);
+  const bool cp = tabs(alp) >= T(2) * tabs(alm);
+  bool ext = false;
+  if (alm * alp > T(0)) {
+    ext = true;
+  } else if (cm || cp) {
+    // the reference differences the lower face with itself (im2 = im1)
+    const T dfm = fim - fim, dfp = fip - fic;
+    const T dcm = fc - f[2], dcp = f[4] - fc;
+    if (tmin(tabs(dfm), tabs(dfp)) >= tmin(tabs(dcm), tabs(dcp))) ext = dfm * dfp < T(0);
+    else ext = dcm * dcp < T(0);
+  }
+  if (ext) {
+    const T s = T(6) * (alm + alp);
+    const T m = ppm_d2(f, 2), c = ppm_d2(f, 3), p = ppm_d2(f, 4);
+    if (tmin(tmin(tmin(s, c), m), p) * tmax(tmax(tmax(s, c), m), p) > T(0)) {
+      const T l = tsign(tmin(pc * tmin(tmin(tabs(m), tabs(c)), tabs(p)), tabs(s)), s);
+      if (tabs(l) > T(0)) {
+        alm = alm * l / s;
+        alp = alp * l / s;
+      } else {
+        alm = T(0);
+        alp = T(0);
+      }
+    } else {
+      alm = T(0);
+      alp = T(0);
+    }
+  } else {
+    if (cp) {
+      const T sgn = tsign(T(1), alp);
+      const T amx = -(T(0.25) * (alp * alp) / (alp + alm));
+      const T dfp = f[4] - fc;
+      if (sgn * amx >= sgn * dfp) {
+        const T del = dfp * (dfp - alm);
+        alp = del >= T(0) ? T(-2) * (dfp + sgn * tsqrt(del)) : T(-2) * alm;
+      }
+    }
+    if (cm) {
+      const T sgn = tsign(T(1), alm);
+      const T amx = -(T(0.25) * (alm * alm) / (alp + alm));
+      const T dfm = f[2] - fc;
+      if (sgn * amx >= sgn * dfm) {
+        const T del = dfm * (dfm - alp);
+        alm = del >= T(0) ? T(-2) * (dfm + sgn * tsqrt(del)) : T(-2) * alp;
+      }
+    }
+  }
+  fl = fc + alp;
+  fr = fc + alm;
 }
 
 template <typename T>

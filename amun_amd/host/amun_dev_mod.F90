@@ -19,6 +19,8 @@ module amun_dev_mod
   integer(c_int32_t), parameter :: AMUN_EQSYS_SRMHD = 4
   integer(c_int32_t), parameter :: AMUN_EOS_ADI = 1
   integer(c_int32_t), parameter :: AMUN_REC_TVD = 1, AMUN_REC_WENO5Z = 5
+  integer(c_int32_t), parameter :: AMUN_REC_WENO3 = 2, AMUN_REC_LIMO3 = 3
+  integer(c_int32_t), parameter :: AMUN_REC_PPM = 4
   integer(c_int32_t), parameter :: AMUN_REC_WENO5YC = 6
   integer(c_int32_t), parameter :: AMUN_REC_WENO5NS = 7
   integer(c_int32_t), parameter :: AMUN_REC_MP5 = 11
@@ -47,6 +49,7 @@ module amun_dev_mod
     real(c_double)     :: sgp, kappa, kbeta
     integer(c_int32_t) :: ngp, stages
     integer(c_int32_t) :: prol_limiter, reserved2
+    real(c_double)     :: limo3_rad, ppm_const
   end type amun_cfg
 
   interface

@@ -82,6 +82,12 @@ program amun_host
     cfg%reconstruction = AMUN_REC_MP5
   case("mp7", "MP7")
     cfg%reconstruction = AMUN_REC_MP7
+  case("weno3", "WENO3")
+    cfg%reconstruction = AMUN_REC_WENO3
+  case("limo3", "LIMO3", "LimO3")
+    cfg%reconstruction = AMUN_REC_LIMO3
+  case("ppm", "PPM")
+    cfg%reconstruction = AMUN_REC_PPM
   case default
     cfg%reconstruction = -1
   end select
@@ -169,6 +175,8 @@ program amun_host
   cfg%kbeta     = get_real("kbeta", 1.0d0)
   cfg%ngp       = get_int("ngp", 5)
   cfg%stages    = get_int("stages", 2)
+  cfg%limo3_rad = get_real("limo3_rad", 0.5d0)
+  cfg%ppm_const = get_real("ppm_const", 1.25d0)
   nmax    = get_int("nmax", 10)
   tmax    = get_real("tmax", 1.0d30)
   seed    = int(get_int("seed", 20261202), int64)

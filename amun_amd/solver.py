@@ -31,6 +31,7 @@ EXPORTS = (
     "amun_dev_run_blocks", "amun_dev_upload_run", "amun_dev_download_run",
     "amun_dev_run_state", "amun_dev_integrals_run", "amun_dev_set_graph",
     "amun_dev_autotune", "amun_dev_set_launch", "amun_dev_unit_riemann",
+    "amun_dev_unit_reconstruct",
     "amun_dev_unit_eqs", "amun_dev_tape_forward", "amun_dev_tape_steps",
     "amun_dev_tape_backward", "amun_dev_tape_tangent", "amun_dev_run_fixed",
     "amun_dev_init_amr", "amun_dev_amr_info",
@@ -102,6 +103,8 @@ def load_library(path: str = LIB_PATH):
                                         ctypes.c_double, _P, _P, _P]
     L.amun_dev_unit_eqs.argtypes = [ctypes.POINTER(AmunCfg), ctypes.c_int32, ctypes.c_double,
                                     _P, _P, _P, _P, _P, _P]
+    L.amun_dev_unit_reconstruct.argtypes = [ctypes.POINTER(AmunCfg), ctypes.c_int32,
+                                            ctypes.c_int32, ctypes.c_double, _P, _P, _P]
     L.amun_dev_init_amr.argtypes = [ctypes.POINTER(AmunCfg), ctypes.c_int32,
                                     ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(vp)]
     L.amun_dev_amr_info.argtypes = [vp, ctypes.POINTER(ctypes.c_int32)]
@@ -457,6 +460,19 @@ def unit_riemann(params: Params, ql: np.ndarray, qr: np.ndarray, cmax: float) ->
     _unit_check(lib, lib.amun_dev_unit_riemann(ctypes.byref(cfg), ql.shape[0], cmax, _dp(ql),
                                                _dp(qr), _dp(f)))
     return f
+
+
+def unit_reconstruct(params: Params, fp: np.ndarray, h: float = 1.0):
+    """Device 1D reconstruction of the configuration on pencils fp (npen, n):
+    returns fl, fr (NaN where the window leaves the pencil)."""
+    lib = load_library()
+    fp = np.ascontiguousarray(fp, dtype=np.float64)
+    fl = np.full_like(fp, np.nan)
+    fr = np.full_like(fp, np.nan)
+    cfg = params.to_cfg()
+    _unit_check(lib, lib.amun_dev_unit_reconstruct(ctypes.byref(cfg), fp.shape[1], fp.shape[0],
+                                                   h, _dp(fp), _dp(fl), _dp(fr)))
+    return fl, fr
 
 
 def unit_eqs(params: Params, q: np.ndarray, cmax: float = 1.0, q2_init=None):

@@ -41,6 +41,9 @@ extern "C" {
 
 /* reconstructions: interpolations.F90:237-376 (parameter "reconstruction") */
 #define AMUN_REC_TVD       1
+#define AMUN_REC_WENO3     2   /* reconstruct_weno3, interpolations.F90:1469   */
+#define AMUN_REC_LIMO3     3   /* reconstruct_limo3, interpolations.F90:1598   */
+#define AMUN_REC_PPM       4   /* reconstruct_ppm, interpolations.F90:1746     */
 #define AMUN_REC_WENO5Z    5
 #define AMUN_REC_WENO5YC   6   /* reconstruct_weno5yc, interpolations.F90:2107 */
 #define AMUN_REC_WENO5NS   7   /* reconstruct_weno5ns, interpolations.F90:2278 */
@@ -114,6 +117,11 @@ typedef struct amun_cfg {
    * 412-428): AMUN_LIM_MINMOD, _MC, _SUPERBEE, _VANLEER or _ZERO (minmod) */
   int32_t prol_limiter;
   int32_t reserved2;
+  /* "limo3_rad" (0.5): the radius of the LimO3 smoothness indicator in cell
+   * sizes, and "ppm_const" (1.25): the curvature ratio of the PPM extremum
+   * limiter (interpolations.F90:83, 106, 209, 212) */
+  double  limo3_rad;
+  double  ppm_const;
 } amun_cfg;
 
 typedef struct amun_ctx amun_ctx;
@@ -311,6 +319,15 @@ int  amun_dev_unit_riemann(const amun_cfg* cfg, int32_t n, double cmax, const do
                            const double* qr, double* f);
 int  amun_dev_unit_eqs(const amun_cfg* cfg, int32_t n, double cmax, const double* q,
                        double* u, double* q2, double* f, double* cm, double* cp);
+/* _reconstruct: the 1D scheme of cfg->reconstruction (reconstruct_states of
+ * interfaces_dir, interpolations.F90:867-959, e.g. reconstruct_ppm at
+ * 1746-1913) on npencil pencils fp of n cells (8 <= n <= 4096) with cell
+ * size h: fl[i] / fr[i-1] are the states at the upper / lower face of cell
+ * i, written for the cells whose window (the sweeps' stencil) lies in the
+ * pencil; other entries keep their input values.  Not for TVD, whose slopes
+ * couple the directions (error 600). */
+int  amun_dev_unit_reconstruct(const amun_cfg* cfg, int32_t n, int32_t npencil, double h,
+                               const double* fp, double* fl, double* fr);
 
 /* Message of the last failing call of this thread. */
 const char* amun_dev_last_error(void);

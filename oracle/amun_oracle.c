@@ -68,6 +68,7 @@ typedef struct {
   int    stages;           /* m of SSPRK(m,2)                                */
   double decay;            /* exp(-alpha*cfl), evolution.F90:211-213         */
   double adx, ady, adz, adxi, adyi, adzi;
+  double h[3];             /* cell sizes of the block being reconstructed   */
   double cmax, cmax2;      /* equations module state                         */
   long   npos_events;      /* "Positive variable is not positive" prints     */
   int    amr_err;          /* multi-level mesh the ghost fill cannot serve   */
@@ -78,6 +79,13 @@ typedef struct {
 } ocfg;
 
 static int prepare_gp(ocfg* o);
+
+/* MIN and MAX as the reference's compiler evaluates them: a < b ? a : b and
+ * a > b ? a : b, so a NaN first operand yields the second and a NaN second
+ * operand propagates (wave speeds of a negative reconstructed pressure) */
+static double fmin_r(double a, double b) { return a < b ? a : b; }
+static double fmax_r(double a, double b) { return a > b ? a : b; }
+
 
 static int oc_setup(ocfg* o, const amun_cfg* c)
 {
@@ -121,7 +129,7 @@ static int oc_setup(ocfg* o, const amun_cfg* c)
   o->gammam1  = c->gamma - 1.0;
   o->gammam1i = 1.0 / o->gammam1;
   o->gammaxi  = o->gammam1 / c->gamma;       /* equations.F90:843 */
-  o->kappa    = fmin(c->kappa, (1.0 - c->cfl) / c->cfl);   /* interpolations.F90:217 */
+  o->kappa    = fmin_r(c->kappa, (1.0 - c->cfl) / c->cfl);   /* interpolations.F90:217 */
   if (c->reconstruction == AMUN_REC_GP) {
     int r = prepare_gp(o);
     if (r) return r;
@@ -144,9 +152,18 @@ static int oc_setup(ocfg* o, const amun_cfg* c)
     o->adxi = 1.0 / o->adx;
     o->adyi = 1.0 / o->ady;
     o->adzi = (o->nd == 3) ? 1.0 / o->adz : 1.0;
+    o->h[0] = o->adx; o->h[1] = o->ady; o->h[2] = o->adz;
   }
+  /* the LimO3 branch of initialize_interpolations raises eps
+   * (interpolations.F90:248) */
+  if (c->reconstruction == AMUN_REC_LIMO3) o->c.eps = fmax_r(1.0e-12, c->eps);
   return 0;
 }
+
+/* bound of the positivity halving loops of interfaces_tvd and the
+ * prolongation: a finite slope falls below qc within 2098 halvings, the
+ * bound only ends an infinite one (on which the reference loops forever) */
+#define OMAXHALV 2200
 
 #define QI(o, p, i, j, k) ((((size_t)(k) * (o)->jm + (j)) * (o)->im + (i)) * (o)->nv + (p))
 
@@ -158,19 +175,19 @@ static double lim(int kind, double x, double a, double b)
   double c;
   switch (kind) {
   case AMUN_LIM_MINMOD:     /* 4731 */
-    return 0.5 * (copysign(x, a) + copysign(x, b)) * fmin(fabs(a), fabs(b));
+    return 0.5 * (copysign(x, a) + copysign(x, b)) * fmin_r(fabs(a), fabs(b));
   case AMUN_LIM_MC:         /* 4765 */
     return (copysign(x, a) + copysign(x, b))
-         * fmin(fmin(fabs(a), fabs(b)), 0.25 * fabs(a + b));
+         * fmin_r(fmin_r(fabs(a), fabs(b)), 0.25 * fabs(a + b));
   case AMUN_LIM_SUPERBEE:   /* 4799 */
     return 0.5 * (copysign(x, a) + copysign(x, b))
-         * fmax(fmin(2.0 * fabs(a), fabs(b)), fmin(fabs(a), 2.0 * fabs(b)));
+         * fmax_r(fmin_r(2.0 * fabs(a), fabs(b)), fmin_r(fabs(a), 2.0 * fabs(b)));
   case AMUN_LIM_VANLEER:    /* 4834 */
     c = a * b;
     if (c > 0.0) return 2.0 * x * c / (a + b);
     return 0.0;
   case AMUN_LIM_VANALBADA:  /* 4873: eps = epsilon(1d0) default */
-    return x * a * b * (a + b) / fmax(2.220446049250313e-16, a * a + b * b);
+    return x * a * b * (a + b) / fmax_r(2.220446049250313e-16, a * a + b * b);
   default:                  /* limiter_zero 4700 */
     return 0.0;
   }
@@ -436,7 +453,7 @@ static int prepare_gp(ocfg* o)
 /* minmod, minmod4, median (interpolations.F90:4892-4975) */
 static double mmod(double a, double b)
 {
-  return (copysign(0.5, a) + copysign(0.5, b)) * fmin(fabs(a), fabs(b));
+  return (copysign(0.5, a) + copysign(0.5, b)) * fmin_r(fabs(a), fabs(b));
 }
 static double mmod4(double a, double b, double c, double d) { return mmod(mmod(a, b), mmod(c, d)); }
 static double median3(double a, double b, double c) { return a + mmod(b - a, c - a); }
@@ -470,8 +487,8 @@ static void mp_limiting(const ocfg* o, int n, const double* f, double* v, double
       fmd = (fi + 0.5 * dm[i + 1]) - dmr;
       ful = fi + df;
       flc = (fi + 0.5 * df) + dml;
-      fmx = fmax(fmin(fmin(fi, f[i]), fmd), fmin(fmin(fi, ful), flc));
-      fmn = fmin(fmax(fmax(fi, f[i]), fmd), fmax(fmax(fi, ful), flc));
+      fmx = fmax_r(fmin_r(fmin_r(fi, f[i]), fmd), fmin_r(fmin_r(fi, ful), flc));
+      fmn = fmin_r(fmax_r(fmax_r(fi, f[i]), fmd), fmax_r(fmax_r(fi, ful), flc));
       v[i - 1] = median3(v[i - 1], fmn, fmx);
     }
   }
@@ -559,6 +576,175 @@ static void reconstruct_mp(const ocfg* o, int n, const double* f, double* fl, do
   fr[n - 1] = f[n - 1];
 }
 
+/* reconstruct_weno3, interpolations.F90:1469-1574: third-order WENO with
+ * the cell size squared in the smoothness weights */
+static void reconstruct_weno3(int n, double h, const double* f, double* fl, double* fr)
+{
+  const double dp = 2.0 / 3.0, dm = 1.0 / 3.0, h2 = h * h;
+  double dfl, dfr, ww, bp, bm, ap, am, wp, wm, fp, fm;
+  int i;
+  for (i = 1; i < n - 1; i++) {
+    dfl = f[i] - f[i - 1];
+    dfr = f[i + 1] - f[i];
+    ww = (dfr - dfl) * (dfr - dfl);
+    bp = dfr * dfr;
+    bm = dfl * dfl;
+    ap = 1.0 + ww / (bp + h2);
+    am = 1.0 + ww / (bm + h2);
+    wp = dp * ap;
+    wm = dm * am;
+    ww = 2.0 * (wp + wm);
+    fp = f[i] + f[i + 1];
+    fm = -f[i - 1] + 3.0 * f[i];
+    fl[i] = (wp * fp + wm * fm) / ww;
+    wp = dp * am;
+    wm = dm * ap;
+    ww = 2.0 * (wp + wm);
+    fp = f[i] + f[i - 1];
+    fm = -f[i + 1] + 3.0 * f[i];
+    fr[i - 1] = (wp * fp + wm * fm) / ww;
+  }
+  i = n - 2;
+  fl[0] = 0.5 * (f[0] + f[1]);
+  fr[i] = 0.5 * (f[i] + f[n - 1]);
+  fl[n - 1] = f[n - 1];
+  fr[n - 1] = f[n - 1];
+}
+
+/* the third-order limited slope of reconstruct_limo3 (eq. 3.13 of Cada &
+ * Torrilhon 2009, halved): th is the slope ratio */
+static double limo3_phi(double th, double xl, double xi)
+{
+  double f1 = (2.0 + th) / 6.0, f2;
+  if (th >= 0.0) f2 = fmax_r(0.0, fmin_r(fmin_r(f1, th), 0.8));
+  else           f2 = fmax_r(0.0, fmin_r(f1, -0.25 * th));
+  return xl * f1 + xi * f2;
+}
+
+/* reconstruct_limo3, interpolations.F90:1598-1719: LimO3 with the switch
+ * between the limited and the plain third-order slope over a radius of
+ * limo3_rad cells */
+static void reconstruct_limo3(const ocfg* o, int n, double h, const double* f, double* fl,
+                              double* fr)
+{
+  const double eps = o->c.eps, rdx = o->c.limo3_rad * h, rdx2 = rdx * rdx;
+  double dfl, dfr, et, xi, xl;
+  int i;
+  for (i = 1; i < n - 1; i++) {
+    dfl = f[i] - f[i - 1];
+    dfr = f[i + 1] - f[i];
+    et = (dfl * dfl + dfr * dfr) / rdx2;
+    xi = fmax_r(0.0, 0.5 * fmin_r(2.0, 1.0 + (et - 1.0) / eps));
+    xl = 1.0 - xi;
+    if (fabs(dfr) > eps) fl[i] = f[i] + dfr * limo3_phi(dfl / dfr, xl, xi);
+    else                 fl[i] = f[i];
+    if (fabs(dfl) > eps) fr[i - 1] = f[i] - dfl * limo3_phi(dfr / dfl, xl, xi);
+    else                 fr[i - 1] = f[i];
+  }
+  i = n - 2;
+  fl[0] = 0.5 * (f[0] + f[1]);
+  fr[i] = 0.5 * (f[i] + f[n - 1]);
+  fl[n - 1] = f[n - 1];
+  fr[n - 1] = f[n - 1];
+}
+
+/* reconstruct_ppm, interpolations.F90:1746-1913: fourth-order face values
+ * limited as in Colella & Sekora 2008, then the extremum and monotonicity
+ * limiters of the cell states.  The neighbour face difference dfm of the
+ * monotonicity test is taken between the same face (im2 = max(1, i - 1) in
+ * the reference), so it is zero; kept as written. */
+static void reconstruct_ppm(const ocfg* o, int n, const double* f, double* fl, double* fr,
+                            double* fi, double* df2)
+{
+  const double pc = o->c.ppm_const;
+  double df2c, df2m, df2p, df2s, df2l, dfm, dfp, dcm, dcp, alm, alp, amx, sgn, del;
+  int i, cm, cp, ext, im2;
+  fi[0] = 0.5 * (f[0] + f[1]);
+  for (i = 1; i < n - 2; i++)
+    fi[i] = (7.0 * (f[i] + f[i + 1]) - (f[i - 1] + f[i + 2])) / 12.0;
+  fi[n - 2] = 0.5 * (f[n - 2] + f[n - 1]);
+  fi[n - 1] = f[n - 1];
+  df2[0] = 0.0;
+  for (i = 1; i < n - 1; i++) df2[i] = (f[i + 1] + f[i - 1]) - 2.0 * f[i];
+  df2[n - 1] = 0.0;
+  for (i = 1; i < n - 2; i++) {
+    if ((f[i + 1] - fi[i]) * (fi[i] - f[i]) < 0.0) {
+      df2c = 3.0 * ((f[i + 1] + f[i]) - 2.0 * fi[i]);
+      df2m = df2[i];
+      df2p = df2[i + 1];
+      if (fmin_r(fmin_r(df2c, df2m), df2p) * fmax_r(fmax_r(df2c, df2m), df2p) > 0.0)
+        df2l = copysign(fmin_r(pc * fmin_r(fabs(df2m), fabs(df2p)), fabs(df2c)), df2c);
+      else
+        df2l = 0.0;
+      fi[i] = 0.5 * (f[i] + f[i + 1]) - df2l / 6.0;
+    }
+  }
+  for (i = 1; i < n - 1; i++) {
+    alm = fi[i - 1] - f[i];
+    alp = fi[i] - f[i];
+    cm = fabs(alm) >= 2.0 * fabs(alp);
+    cp = fabs(alp) >= 2.0 * fabs(alm);
+    ext = 0;
+    if (alm * alp > 0.0) {
+      ext = 1;
+    } else if (cm || cp) {
+      im2 = i - 1 > 0 ? i - 1 : 0;
+      dfm = fi[i - 1] - fi[im2];
+      dfp = fi[i + 1] - fi[i];
+      dcm = f[i] - f[i - 1];
+      dcp = f[i + 1] - f[i];
+      if (fmin_r(fabs(dfm), fabs(dfp)) >= fmin_r(fabs(dcm), fabs(dcp))) ext = dfm * dfp < 0.0;
+      else                                                         ext = dcm * dcp < 0.0;
+    }
+    if (ext) {
+      df2s = 6.0 * (alm + alp);
+      df2m = df2[i - 1];
+      df2c = df2[i];
+      df2p = df2[i + 1];
+      if (fmin_r(fmin_r(fmin_r(df2s, df2c), df2m), df2p) * fmax_r(fmax_r(fmax_r(df2s, df2c), df2m), df2p) > 0.0) {
+        df2l = copysign(fmin_r(pc * fmin_r(fmin_r(fabs(df2m), fabs(df2c)), fabs(df2p)), fabs(df2s)), df2s);
+        if (fabs(df2l) > 0.0) {
+          alm = alm * df2l / df2s;
+          alp = alp * df2l / df2s;
+        } else {
+          alm = 0.0;
+          alp = 0.0;
+        }
+      } else {
+        alm = 0.0;
+        alp = 0.0;
+      }
+    } else {
+      if (cp) {
+        sgn = copysign(1.0, alp);
+        amx = -(0.25 * (alp * alp) / (alp + alm));
+        dfp = f[i + 1] - f[i];
+        if (sgn * amx >= sgn * dfp) {
+          del = dfp * (dfp - alm);
+          if (del >= 0.0) alp = -2.0 * (dfp + sgn * sqrt(del));
+          else            alp = -2.0 * alm;
+        }
+      }
+      if (cm) {
+        sgn = copysign(1.0, alm);
+        amx = -(0.25 * (alm * alm) / (alp + alm));
+        dfm = f[i - 1] - f[i];
+        if (sgn * amx >= sgn * dfm) {
+          del = dfm * (dfm - alp);
+          if (del >= 0.0) alm = -2.0 * (dfm + sgn * sqrt(del));
+          else            alm = -2.0 * alp;
+        }
+      }
+    }
+    fr[i - 1] = f[i] + alm;
+    fl[i] = f[i] + alp;
+  }
+  fl[0] = fi[0];
+  fr[n - 2] = fi[n - 1];
+  fl[n - 1] = fi[n - 1];
+  fr[n - 1] = f[n - 1];
+}
+
 /* reconstruct_gp, interpolations.F90:4561-4668 */
 static void reconstruct_gp(const ocfg* o, int n, const double* f, double* fl, double* fr,
                            double* w1, double* w2, double* w3)
@@ -580,10 +766,13 @@ static void reconstruct_gp(const ocfg* o, int n, const double* f, double* fl, do
 /* qs layout: [(d*2 + s)][k][j][i][p]                                         */
 /* ------------------------------------------------------------------------ */
 /* reconstruct_states of interfaces_dir: the selected 1D scheme */
-static void recon_dir(const ocfg* o, int n, const double* f, double* fl, double* fr,
+static void recon_dir(const ocfg* o, int n, double h, const double* f, double* fl, double* fr,
                       double* w1, double* w2, double* w3)
 {
   switch (o->c.reconstruction) {
+  case AMUN_REC_WENO3: reconstruct_weno3(n, h, f, fl, fr); break;
+  case AMUN_REC_LIMO3: reconstruct_limo3(o, n, h, f, fl, fr); break;
+  case AMUN_REC_PPM: reconstruct_ppm(o, n, f, fl, fr, w1, w2); break;
   case AMUN_REC_GP: reconstruct_gp(o, n, f, fl, fr, w1, w2, w3); break;
   case AMUN_REC_WENO5YC: weno5z_v(o, n, f, fl, fr, w1, w2, w3, 1); break;
   case AMUN_REC_WENO5NS: weno5ns(o, n, f, fl, fr, w1, w2, w3); break;
@@ -627,7 +816,8 @@ static void reconstruct_interfaces(ocfg* o, const double* q, double* qs)
             }
             if (positive) {
               if (qc > 0.0) {
-                for (;;) {
+                int it;
+                for (it = 0; it < OMAXHALV; it++) {
                   double sa = fabs(dq[0]) + fabs(dq[1]);
                   if (nd == 3) sa += fabs(dq[2]);
                   if (!(qc <= sa)) break;
@@ -657,13 +847,13 @@ static void reconstruct_interfaces(ocfg* o, const double* q, double* qs)
       for (k = o->kbl; k <= o->keu; k++) {
         for (j = o->jbl; j <= o->jeu; j++) {
           for (i = 0; i < im; i++) fp[i] = q[QI(o, p, i, j, k)];
-          recon_dir(o, im, fp, fl, fr, w1, w2, w3);
+          recon_dir(o, im, o->h[0], fp, fl, fr, w1, w2, w3);
           if (positive && o->c.fix_positivity) fix_positivity(im, fp, fl, fr);
           for (i = 0; i < im; i++) { QS(0, 0, i, j, k, p) = fl[i]; QS(0, 1, i, j, k, p) = fr[i]; }
         }
         for (i = o->ibl; i <= o->ieu; i++) {
           for (j = 0; j < jm; j++) fp[j] = q[QI(o, p, i, j, k)];
-          recon_dir(o, jm, fp, fl, fr, w1, w2, w3);
+          recon_dir(o, jm, o->h[1], fp, fl, fr, w1, w2, w3);
           if (positive && o->c.fix_positivity) fix_positivity(jm, fp, fl, fr);
           for (j = 0; j < jm; j++) { QS(1, 0, i, j, k, p) = fl[j]; QS(1, 1, i, j, k, p) = fr[j]; }
         }
@@ -672,7 +862,7 @@ static void reconstruct_interfaces(ocfg* o, const double* q, double* qs)
         for (j = o->jbl; j <= o->jeu; j++)
           for (i = o->ibl; i <= o->ieu; i++) {
             for (k = 0; k < km; k++) fp[k] = q[QI(o, p, i, j, k)];
-            recon_dir(o, km, fp, fl, fr, w1, w2, w3);
+            recon_dir(o, km, o->h[2], fp, fl, fr, w1, w2, w3);
             if (positive && o->c.fix_positivity) fix_positivity(km, fp, fl, fr);
             for (k = 0; k < km; k++) { QS(2, 0, i, j, k, p) = fl[k]; QS(2, 1, i, j, k, p) = fr[k]; }
           }
@@ -714,7 +904,7 @@ static int alg_quadratic(const double* a, double* x)
 {
   double b[3], bh, dl, dr, tm, mx;
   int nr;
-  mx = fmax(fmax(fabs(a[0]), fabs(a[1])), fabs(a[2]));
+  mx = fmax_r(fmax_r(fabs(a[0]), fabs(a[1])), fabs(a[2]));
   b[0] = a[0] / mx; b[1] = a[1] / mx; b[2] = a[2] / mx;
   if (b[2] != 0.0) {
     if (b[0] != 0.0) {
@@ -1082,7 +1272,7 @@ static int nr_initial_brackets(const ocfg* o, double mm, double bb, double mb,
     a[1] = -(2.0 * (ss + dd * bb));
     a[0] = -(ss + dd * bb) * bb;
     nr = alg_quartic(a, x);
-    if (nr > 0) *wl = fmax(*wl, x[nr - 1]);
+    if (nr > 0) *wl = fmax_r(*wl, x[nr - 1]);
   } else {
     keep = 1;
     it = nrmax;
@@ -1268,8 +1458,8 @@ static void fluxspeed_sr(const ocfg* o, const double* q, const double* u,
     a[0] = c2 * ca / rt;
     nr = alg_quadratic(a, x);
     if (nr > 0) {
-      zm = fmin(x[0], x[1]);
-      zp = fmax(x[0], x[1]);
+      zm = fmin_r(x[0], x[1]);
+      zp = fmax_r(x[0], x[1]);
       if (zm >= 0.0) {
         zm = sqrt(zm);
         zp = sqrt(zp);
@@ -1287,7 +1477,7 @@ static void fluxspeed_sr(const ocfg* o, const double* q, const double* u,
   }
   if (nr > 1) {
     double mn = x[0], mx = x[0];
-    for (n = 1; n < nr; n++) { mn = fmin(mn, x[n]); mx = fmax(mx, x[n]); }
+    for (n = 1; n < nr; n++) { mn = fmin_r(mn, x[n]); mx = fmax_r(mx, x[n]); }
     *cm = mn;
     *cp = mx;
   } else {
@@ -1430,8 +1620,8 @@ static void riemann_hll(const ocfg* o, double* ql, double* qr, double* f)
   prim2cons(o, qr, ur);
   fluxspeed(o, ql, ul, fl, &clm, &clp);
   fluxspeed(o, qr, ur, fr, &crm, &crp);
-  sl = fmin(clm, crm);
-  sr = fmax(clp, crp);
+  sl = fmin_r(clm, crm);
+  sr = fmax_r(clp, crp);
   if (sl >= 0.0) {
     for (p = 0; p < nv; p++) f[p] = fl[p];
   } else if (sr <= 0.0) {
@@ -1456,8 +1646,8 @@ static void riemann_hd_hllc(const ocfg* o, double* ql, double* qr, double* f)
   prim2cons(o, qr, ur);
   fluxspeed(o, ql, ul, fl, &clm, &clp);
   fluxspeed(o, qr, ur, fr, &crm, &crp);
-  sl = fmin(clm, crm);
-  sr = fmax(clp, crp);
+  sl = fmin_r(clm, crm);
+  sr = fmax_r(clp, crp);
   if (sl >= 0.0) {
     for (p = 0; p < nv; p++) f[p] = fl[p];
   } else if (sr <= 0.0) {
@@ -1609,8 +1799,8 @@ static void riemann_mhd_hllc(const ocfg* o, double* ql, double* qr, double* f)
   prim2cons(o, qr, ur);
   fluxspeed(o, ql, ul, fl, &clm, &clp);
   fluxspeed(o, qr, ur, fr, &crm, &crp);
-  sl = fmin(clm, crm);
-  sr = fmax(clp, crp);
+  sl = fmin_r(clm, crm);
+  sr = fmax_r(clp, crp);
   if (sl >= 0.0) {
     for (p = 0; p < nv; p++) f[p] = fl[p];
     return;
@@ -1703,8 +1893,8 @@ static void riemann_mhd_hlld(const ocfg* o, double* ql, double* qr, double* f)
   prim2cons(o, qr, ur);
   fluxspeed(o, ql, ul, fl, &clm, &clp);
   fluxspeed(o, qr, ur, fr, &crm, &crp);
-  sl = fmin(clm, crm);
-  sr = fmax(clp, crp);
+  sl = fmin_r(clm, crm);
+  sr = fmax_r(clp, crp);
 
   if (sl >= 0.0) {
     for (p = 0; p < nv; p++) f[p] = fl[p];
@@ -1748,7 +1938,7 @@ static void riemann_mhd_hlld(const ocfg* o, double* ql, double* qr, double* f)
   ui[ibx] = bx; ui[iby] = by; ui[ibz] = bz; ui[ibp] = ur[ibp];                 \
   ui[ien] = ((wr[ien] + sm * pt) - bx * vb) / srmm;
 
-  if (b2 > 1.0e-08 * fmax(dnl * (sl * sl), dnr * (sr * sr))) {
+  if (b2 > 1.0e-08 * fmax_r(dnl * (sl * sl), dnr * (sr * sr))) {
     cal = sqrt(b2 / dnl);
     car = sqrt(b2 / dnr);
     sml = sm - cal;
@@ -2131,7 +2321,8 @@ static int boundary_variables_amr(ogrid* g, double* u)
                 }
                 if (p == o->idn || p == o->ipr) {
                   if (qc > 0.0) {
-                    for (;;) {
+                    int it;
+                    for (it = 0; it < OMAXHALV; it++) {
                       double sa = fabs(dq[0]) + fabs(dq[1]);
                       if (nd == 3) sa = sa + fabs(dq[2]);
                       if (!(qc <= sa)) break;
@@ -2309,21 +2500,21 @@ static void new_time_step(ogrid* g, double dtnext)
     for (k = o->kb; k <= o->ke; k++)
       for (j = o->jb; j <= o->je; j++)
         for (i = o->ib; i <= o->ie; i++)
-          ms = fmax(ms, maxspeed_cell(o, &g->q[b * bs + QI(o, 0, i, j, k)]));
-    o->cmax = fmax(o->cmax, ms);
+          ms = fmax_r(ms, maxspeed_cell(o, &g->q[b * bs + QI(o, 0, i, j, k)]));
+    o->cmax = fmax_r(o->cmax, ms);
   }
   o->cmax2 = o->cmax * o->cmax;
   if (g->amr) {                 /* dx_min of the highest level in use */
     int mlev = 1;
     for (b = 0; b < g->nblk; b++) mlev = g->lev[b] > mlev ? g->lev[b] : mlev;
-    dxmin = fmin(g->lvd[mlev][0], g->lvd[mlev][1]);
-    if (o->nd == 3) dxmin = fmin(dxmin, g->lvd[mlev][2]);
+    dxmin = fmin_r(g->lvd[mlev][0], g->lvd[mlev][1]);
+    if (o->nd == 3) dxmin = fmin_r(dxmin, g->lvd[mlev][2]);
   } else {
-    dxmin = fmin(o->adx, o->ady);
-    if (o->nd == 3) dxmin = fmin(dxmin, o->adz);
+    dxmin = fmin_r(o->adx, o->ady);
+    if (o->nd == 3) dxmin = fmin_r(dxmin, o->adz);
   }
-  g->dtn = o->cflx * dxmin / fmax(o->cmax + 2.0 * 0.0 / dxmin, eps);
-  if (dtnext > 0.0) g->dt = fmin(g->dtn, dtnext);
+  g->dtn = o->cflx * dxmin / fmax_r(o->cmax + 2.0 * 0.0 / dxmin, eps);
+  if (dtnext > 0.0) g->dt = fmin_r(g->dtn, dtnext);
 }
 
 /* one stage: fluxes of u (through q), increment, combine into dst (which
@@ -2336,7 +2527,11 @@ static void stage2(ogrid* g, int kind, double ds, double* dst, const double* a0,
   int b;
   size_t n;
   if (g->amr) {                 /* update_fluxes: every block, then the correction */
-    for (b = 0; b < g->nblk; b++) update_flux(o, g->q + b * bs, g->fall + b * bs * o->nd, g->qs);
+    for (b = 0; b < g->nblk; b++) {
+      const double* lv = g->lvd[g->lev[b]];
+      o->h[0] = lv[0]; o->h[1] = lv[1]; o->h[2] = lv[2];
+      update_flux(o, g->q + b * bs, g->fall + b * bs * o->nd, g->qs);
+    }
     if (!g->skip_fix) boundary_fluxes(g);
   }
   for (b = 0; b < g->nblk; b++) {
@@ -2465,7 +2660,8 @@ int orc_create(const amun_cfg* c, void** handle)
   if (!r && c->reconstruction != AMUN_REC_TVD && c->reconstruction != AMUN_REC_WENO5Z &&
       c->reconstruction != AMUN_REC_WENO5YC && c->reconstruction != AMUN_REC_WENO5NS &&
       c->reconstruction != AMUN_REC_MP5 && c->reconstruction != AMUN_REC_MP7 &&
-      c->reconstruction != AMUN_REC_GP) r = 200;
+      c->reconstruction != AMUN_REC_GP && c->reconstruction != AMUN_REC_WENO3 &&
+      c->reconstruction != AMUN_REC_LIMO3 && c->reconstruction != AMUN_REC_PPM) r = 200;
   if (!r && (c->time_advance < AMUN_TA_EULER || c->time_advance > AMUN_TA_SSPRK53)) r = 400;
   if (r) { free(g); return r; }
   g->ir = c->xblocks; g->jr = c->yblocks; g->kr = (c->ndims == 3) ? c->zblocks : 1;
@@ -2744,9 +2940,9 @@ int orc_update_flux(const amun_cfg* c, const double* q, const double* dx,
   ocfg o;
   double* qs;
   int r = oc_setup(&o, c);
-  (void)dx;
   if (!r) r = riemann_check(c);
   if (r) return r;
+  o.h[0] = dx[0]; o.h[1] = dx[1]; o.h[2] = o.nd == 3 ? dx[2] : 1.0;
   o.cmax = cmax;
   o.cmax2 = cmax * cmax;
   qs = (double*)malloc((size_t)o.im * o.jm * o.km * o.nv * 2 * o.nd * sizeof(double));
@@ -2756,7 +2952,7 @@ int orc_update_flux(const amun_cfg* c, const double* q, const double* dx,
 }
 
 /* unit level: reconstruct() on npencil pencils of length n (WENO5-Z, GP) */
-int orc_reconstruct(const amun_cfg* c, int n, int npencil, const double* fp,
+int orc_reconstruct(const amun_cfg* c, int n, int npencil, double h, const double* fp,
                     double* fl, double* fr)
 {
   ocfg o;
@@ -2765,7 +2961,7 @@ int orc_reconstruct(const amun_cfg* c, int n, int npencil, const double* fp,
   if (r) return r;
   w = (double*)malloc(3 * (size_t)(n + 4) * sizeof(double));
   for (p = 0; p < npencil; p++)
-    recon_dir(&o, n, fp + (size_t)p * n, fl + (size_t)p * n, fr + (size_t)p * n, w, w + n + 4,
+    recon_dir(&o, n, h, fp + (size_t)p * n, fl + (size_t)p * n, fr + (size_t)p * n, w, w + n + 4,
               w + 2 * (n + 4));
   free(w);
   return 0;

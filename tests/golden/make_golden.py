@@ -368,6 +368,107 @@ def main_schemes():
         json.dump(index, fh, indent=1)
 
 
+O3_SCHEMES = ("weno3", "limo3", "ppm")
+
+
+def main_negp():
+    """Riemann solvers through update_flux (zero limiter) with a few cells of
+    negative pressure: the reference's NaN wave speeds, its MIN/MAX order
+    (a NaN first operand yields the second, a NaN second operand
+    propagates) and the NaN fluxes that follow (own generator)."""
+    rng = np.random.default_rng(20261209)
+    for eq, rs in (("hd", "hll"), ("hd", "hllc"), ("hd", "roe"), ("mhd", "hll"),
+                   ("mhd", "hlld"), ("mhd", "hllc")):
+        nv = 5 if eq == "hd" else 9
+        pf = Params(ndims=2, equation_system=eq, reconstruction="tvd", limiter="zero",
+                    riemann_solver=rs, gamma=1.4 if eq == "hd" else 5.0 / 3.0, ncells=8,
+                    nghosts=2)
+        qb = np.zeros((2, 1, 12, 12, nv))
+        qb[..., 0] = rng.uniform(0.5, 2.0, qb.shape[:-1])
+        qb[..., 1:4] = rng.normal(scale=0.5, size=qb.shape[:-1] + (3,))
+        qb[..., -1] = rng.uniform(0.1, 2.0, qb.shape[:-1])
+        if nv == 9:
+            qb[..., 4:7] = 0.3 * rng.normal(size=qb.shape[:-1] + (3,))
+        neg = rng.random(qb.shape[:-1]) < 0.08
+        qb[..., -1] = np.where(neg, -rng.uniform(0.01, 0.3, qb.shape[:-1]), qb[..., -1])
+        f = ol.ref_update_flux(pf, qb, cmax=2.3)
+        np.savez_compressed(os.path.join(HERE, f"unit_flux_{eq}_{rs}_negp.npz"), q=qb, f=f,
+                            cmax=2.3, params=np.array(pf.to_params_in()))
+        print(eq, rs, "non-finite fluxes", int((~np.isfinite(f)).sum()))
+
+
+def ot_noisy(p, seed=6):
+    """Orszag-Tang with 1 % seeded noise on density, velocity, field and
+    pressure.  PPM's extremum test classifies round-off noise on a flat
+    profile as extrema, so its result on a uniform pressure depends on the
+    last bit of the primitive recovery; the noise leaves no flat profile."""
+    nz, ny, nx = p.shape
+    z, y, x = problems.cell_centres(p)
+    Y, X = np.meshgrid(y, x, indexing="ij")
+    xi = [problems.uniform(seed, nz * ny * nx, stream=s).reshape(nz, ny, nx) - 0.5
+          for s in range(20, 25)]
+    q = np.zeros((p.nv, nz, ny, nx))
+    b0 = 1.0 / np.sqrt(4.0 * np.pi)
+    q[0] = 25.0 / (36.0 * np.pi) * (1.0 + 0.02 * xi[0])
+    q[1] = -np.sin(2.0 * np.pi * Y) + 0.02 * xi[1]
+    q[2] = np.sin(2.0 * np.pi * X) + 0.02 * xi[2]
+    q[4] = -b0 * np.sin(2.0 * np.pi * Y) + 0.02 * b0 * xi[3]
+    q[5] = b0 * np.sin(4.0 * np.pi * X)
+    q[8] = 5.0 / (12.0 * np.pi) * (1.0 + 0.02 * xi[4])
+    return problems.prim_to_cons(p, q)
+
+
+def main_o3():
+    """WENO3, LimO3 and PPM: pencils at two cell sizes (WENO3 and LimO3 weigh
+    by it), short Orszag-Tang runs, and a LimO3 run on the reference's
+    three-level blast mesh (per-level cell sizes) (own generator)."""
+    from amun_amd import mesh
+    rng = np.random.default_rng(20261208)
+    fp = np.concatenate([rng.normal(size=(6, 24)), np.cumsum(rng.normal(size=(4, 24)), axis=1),
+                         np.sin(np.linspace(0, 6, 24))[None, :] * np.ones((2, 1)),
+                         np.where(np.arange(24) < 12, 1.0, 0.1)[None, :] * np.ones((2, 1)),
+                         1e-3 * np.cumsum(rng.normal(size=(2, 24)), axis=1)])
+    out = {"fp": fp, "h": np.array([1.0, 0.0078125])}
+    with open(os.path.join(HERE, "index.json")) as fh:
+        index = json.load(fh)
+    for rec in O3_SCHEMES:
+        pu = Params(ndims=3, equation_system="mhd", reconstruction=rec, limiter="mm",
+                    ncells=16, nghosts=4, cfl=0.3)
+        for ih, h in enumerate(out["h"]):
+            out[f"fl_{rec}_{ih}"], out[f"fr_{rec}_{ih}"] = ol.ref_reconstruct(pu, fp, float(h))
+        p = Params(ndims=2, equation_system="mhd", reconstruction=rec, limiter="mm",
+                   riemann_solver="hlld", time_advance="rk3", gamma=5.0 / 3.0, cfl=0.3,
+                   ncells=8, nghosts=4 if rec == "ppm" else 2, xblocks=4, yblocks=4)
+        u0 = ot_noisy(p) if rec == "ppm" else problems.orszag_tang_2d(p, seed=6)
+        nsteps = 8
+        u, tr = ol.ref_run(p, u0, nsteps)
+        name = f"{rec}_ot{'noisy' if rec == 'ppm' else ''}2d_32_{nsteps}"
+        np.savez_compressed(os.path.join(HERE, name + ".npz"),
+                            params=np.array(p.to_params_in()), ndims=2, nsteps=nsteps,
+                            stride=3, trace=tr, sub=u[:, :, ::3, ::3], sha_u0=sha(u0),
+                            sha_u=sha(u))
+        index[name] = {"steps": nsteps, "sha_u": sha(u), "dt_last": float(tr[-1, 0])}
+        print(name, "dt", tr[-1, 0])
+    np.savez_compressed(os.path.join(HERE, "unit_schemes_o3.npz"), **out)
+    p = amr_params(2)
+    p.reconstruction = "limo3"
+    meta, _, _, _, _ = ol.ref_amr(p, 1)
+    u0 = mesh.block_state(p, meta, mesh.blast_prim(p))
+    nsteps = 6
+    m0, u0r, m1, u1, tr = ol.ref_amr(p, nsteps, u_in=u0)
+    assert np.array_equal(m0, meta) and np.array_equal(u0r, u0)
+    name = f"amr_limo3_mhd2d_blast_{nsteps}"
+    sums = u1.reshape(u1.shape[0], -1, u1.shape[-1]).sum(axis=1)
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), params=np.array(p.to_params_in()),
+                        ndims=2, nsteps=nsteps, meta=meta, trace=tr, block_sums=sums,
+                        sha_u0=sha(u0), sha_u=sha(u1), u1=u1)
+    index[name] = {"steps": nsteps, "sha_u": sha(u1), "dt_last": float(tr[-1, 0]),
+                   "blocks": int(meta.shape[0])}
+    print(name, meta.shape[0], "blocks, dt", tr[-1, 0])
+    with open(os.path.join(HERE, "index.json"), "w") as fh:
+        json.dump(index, fh, indent=1)
+
+
 def main():
     if not (ol.ref_available(2) and ol.ref_available(3)):
         sys.exit("reference binaries missing: run oracle/build_ref.sh first")
@@ -391,6 +492,12 @@ def main():
         return
     if sys.argv[1:] == ["roe_hd"]:
         main_roe_hd()
+        return
+    if sys.argv[1:] == ["o3"]:
+        main_o3()
+        return
+    if sys.argv[1:] == ["negp"]:
+        main_negp()
         return
     index = {}
     for name, p, builder, nsteps, stride in run_cases():
@@ -461,6 +568,8 @@ def main():
     main_schemes()
     main_hllc_mhd()
     main_roe_hd()
+    main_o3()
+    main_negp()
 
 
 if __name__ == "__main__":

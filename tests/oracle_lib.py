@@ -40,7 +40,8 @@ def lib():
         L.orc_positivity_events.argtypes = [ctypes.c_void_p]
         L.orc_positivity_events.restype = ctypes.c_long
         L.orc_update_flux.argtypes = [ctypes.c_void_p, _P, _P, ctypes.c_double, _P]
-        L.orc_reconstruct.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _P, _P, _P]
+        L.orc_reconstruct.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_double, _P, _P, _P]
         L.orc_eqs.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, _P, _P, _P, _P, _P, _P]
         L.orc_riemann.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, _P, _P, _P]
         L.orc_c2p_events.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_long)]
@@ -162,13 +163,13 @@ def oracle_update_flux(p: Params, q: np.ndarray, cmax: float) -> np.ndarray:
     return f
 
 
-def oracle_reconstruct(p: Params, fp: np.ndarray):
+def oracle_reconstruct(p: Params, fp: np.ndarray, h: float = 1.0):
     fp = np.ascontiguousarray(fp, dtype=np.float64)
     npen, n = fp.shape
     fl = np.empty_like(fp)
     fr = np.empty_like(fp)
     cfg = p.to_cfg()
-    lib().orc_reconstruct(ctypes.byref(cfg), n, npen, _dp(fp), _dp(fl), _dp(fr))
+    lib().orc_reconstruct(ctypes.byref(cfg), n, npen, h, _dp(fp), _dp(fl), _dp(fr))
     return fl, fr
 
 

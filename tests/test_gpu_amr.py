@@ -25,7 +25,7 @@ def _case(nd, eq="mhd", rec="tvd", ta="rk3", lim="mc", plim="mm", precision="fp6
     p = Params(ndims=nd, equation_system=eq, reconstruction=rec, limiter=lim,
                riemann_solver="hlld" if eq == "mhd" else "hllc", time_advance=ta,
                gamma=5.0 / 3.0 if eq == "mhd" else 1.4, cfl=0.4, ncells=8,
-               nghosts=4 if rec == "weno5z" else 2, xblocks=2, yblocks=2, zblocks=2,
+               nghosts=4 if rec in ("weno5z", "ppm") else 2, xblocks=2, yblocks=2, zblocks=2,
                xmin=-0.5, xmax=0.5, ymin=-0.5, ymax=0.5, zmin=-0.5, zmax=0.5,
                prolongation_limiter=plim, precision=precision)
     if problem == "blast":
@@ -72,7 +72,9 @@ def test_ghost_fill_matches_oracle(nd):
 
 CASES = [(2, "mhd", "tvd", "rk3", "mc", "mm"), (2, "hd", "weno5z", "rk2", "mm", "mc"),
          (2, "mhd", "tvd", "ssprk(4,3)", "vl", "sb"), (2, "hd", "tvd", "euler", "sb", "vl"),
-         (3, "mhd", "tvd", "rk3", "mc", "mm"), (3, "hd", "weno5z", "rk3", "mm", "mm")]
+         (3, "mhd", "tvd", "rk3", "mc", "mm"), (3, "hd", "weno5z", "rk3", "mm", "mm"),
+         (2, "mhd", "limo3", "rk3", "mm", "mm"), (2, "hd", "weno3", "rk2", "mm", "mc"),
+         (3, "mhd", "limo3", "rk2", "mm", "vl")]
 
 
 @pytest.mark.parametrize("case", CASES)
@@ -94,7 +96,8 @@ def test_one_step_vs_oracle(case):
     assert err.max() <= 1e-12
 
 
-@pytest.mark.parametrize("name", ["amr_mhd2d_blast_10", "amr_mhd3d_blast_4"])
+@pytest.mark.parametrize("name", ["amr_mhd2d_blast_10", "amr_mhd3d_blast_4",
+                                  "amr_limo3_mhd2d_blast_6"])
 def test_reference_runs(name):
     """The reference's own three-level blast runs (golden fixtures)."""
     g = np.load(os.path.join(GOLD, name + ".npz"), allow_pickle=False)

@@ -1,5 +1,6 @@
 """Unit-level parity of the device routines (amun_dev_unit_riemann,
-amun_dev_unit_eqs) on vectors of states: against the golden vectors written
+amun_dev_unit_eqs, amun_dev_unit_reconstruct) on vectors of states and
+pencils: against the golden vectors written
 by the reference binaries and against the oracle on random and degenerate
 states.  FP64: <= 1e-12 of the magnitude of each vector (FMA contraction and
 refined reciprocals are the only differences); mixed: FP32 round-off."""
@@ -9,7 +10,7 @@ import numpy as np
 import pytest
 
 from amun_amd.config import Params
-from amun_amd.solver import unit_eqs, unit_riemann
+from amun_amd.solver import unit_eqs, unit_reconstruct, unit_riemann
 import oracle_lib as ol
 
 pytestmark = pytest.mark.gpu
@@ -105,3 +106,74 @@ def test_riemann_golden_via_update_flux_states(eq, rs):
     ok = close(fg, fx, 1e-12)
     print(f"[case] golden {eq}/{rs}: max abs {np.abs(fg - fx).max():.3e}")
     assert ok.all(), np.argwhere(~ok)[:5]
+
+
+@pytest.mark.parametrize("eq,rs", [("hd", "hll"), ("hd", "hllc"), ("hd", "roe"), ("mhd", "hll"),
+                                   ("mhd", "hlld"), ("mhd", "hllc")])
+def test_riemann_negative_pressure_golden(eq, rs):
+    """Face pairs with negative pressure on either side: the device solvers
+    take the reference's MIN/MAX order with NaN wave speeds, so the NaN
+    fluxes sit where the reference's do and the rest agree to 1e-12."""
+    g = np.load(os.path.join(GOLD, f"unit_flux_{eq}_{rs}_negp.npz"), allow_pickle=False)
+    p = Params.from_text(str(g["params"]), ndims=2)
+    q, fref = g["q"], g["f"]
+    lo, hi = p.nghosts - 1, p.nghosts + p.ncells
+    ql = q[:, 0, lo:hi + 1, lo:hi, :].reshape(-1, p.nv)
+    qr = q[:, 0, lo:hi + 1, lo + 1:hi + 1, :].reshape(-1, p.nv)
+    fx = fref[:, 0, lo:hi + 1, lo:hi, :, 0].reshape(-1, p.nv)
+    fg = unit_riemann(p, ql, qr, float(g["cmax"]))
+    nan_ref, nan_dev = ~np.isfinite(fx), ~np.isfinite(fg)
+    print(f"[case] {eq}/{rs}: non-finite ref {nan_ref.sum()} device {nan_dev.sum()}")
+    np.testing.assert_array_equal(nan_dev, nan_ref)
+    ok = close(np.where(nan_ref, 0.0, fg), np.where(nan_ref, 0.0, fx), 1e-12)
+    assert ok.all(), np.argwhere(~ok)[:5]
+
+
+def _pencil_cases():
+    """(name, params, fp, h, fl, fr) of the reference's golden pencils."""
+    out = []
+    g = np.load(os.path.join(GOLD, "unit_weno5z.npz"), allow_pickle=False)
+    out.append(("weno5z", Params(ndims=3, equation_system="mhd", reconstruction="weno5z",
+                                 limiter="mm", ncells=16, nghosts=4, cfl=0.3),
+                g["fp"], 1.0, g["fl"], g["fr"]))
+    g = np.load(os.path.join(GOLD, "unit_gp.npz"), allow_pickle=False)
+    out.append(("gp", Params.from_text(str(g["params"]), ndims=3), g["fp"], 1.0, g["fl"], g["fr"]))
+    g = np.load(os.path.join(GOLD, "unit_schemes.npz"), allow_pickle=False)
+    for rec in ("weno5yc", "weno5ns", "mp5", "mp7"):
+        out.append((rec, Params(ndims=3, equation_system="mhd", reconstruction=rec, limiter="mm",
+                                ncells=16, nghosts=4, cfl=0.3),
+                    g["fp"], 1.0, g["fl_" + rec], g["fr_" + rec]))
+    g = np.load(os.path.join(GOLD, "unit_schemes_o3.npz"), allow_pickle=False)
+    for rec in ("weno3", "limo3", "ppm"):
+        for ih, h in enumerate(g["h"]):
+            out.append((f"{rec}_h{ih}", Params(ndims=3, equation_system="mhd", reconstruction=rec,
+                                               limiter="mm", ncells=16, nghosts=4, cfl=0.3),
+                        g["fp"], float(h), g[f"fl_{rec}_{ih}"], g[f"fr_{rec}_{ih}"]))
+    return out
+
+
+@pytest.mark.parametrize("case", _pencil_cases(), ids=lambda c: c[0])
+def test_reconstruction_pencils_golden(case):
+    """The device reconstruction of every 1D scheme on the reference's own
+    pencils (interior cells, four from either end): FP64 to 1e-12 of the
+    pencil's range; FP32 to 1e-5."""
+    name, p, fp, h, rl, rr = case
+    fl, fr = unit_reconstruct(p, fp, h)
+    sl = slice(4, fp.shape[1] - 4)
+    scale = np.abs(fp).max(axis=1, keepdims=True)
+    el = np.abs(fl[:, sl] - rl[:, sl]) / scale
+    er = np.abs(fr[:, 3:fp.shape[1] - 5] - rr[:, 3:fp.shape[1] - 5]) / scale
+    print(f"[case] {name}: max rel fl {el.max():.2e} fr {er.max():.2e}")
+    # the GP weights come from the quad-precision solve of prepare_gp on the
+    # host (rounded once to FP64), the reference rounds its own way
+    tol = 1e-11 if name == "gp" else 1e-12
+    assert el.max() <= tol, np.argwhere(el > tol)[:5]
+    assert er.max() <= tol, np.argwhere(er > tol)[:5]
+    pm = Params(**{k: v for k, v in p.__dict__.items() if k != "extra"})
+    pm.precision = "mixed"
+    ml, mr = unit_reconstruct(pm, fp, h)
+    # FP32 round-off can move a value across a limiter switch (the PPM and
+    # LimO3 branches, the MP bounds): judged by the share of close values
+    em = np.abs(ml[:, sl] - rl[:, sl]) / scale
+    print(f"  mixed: share within 1e-5 {np.mean(em <= 1e-5):.4f}")
+    assert np.mean(em <= 1e-5) >= 0.97

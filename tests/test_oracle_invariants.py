@@ -513,12 +513,16 @@ def _cell_averages(coef, n, h=1.0):
     return avg, P(x[1:])
 
 
-@pytest.mark.parametrize("rec,deg", [("mp5", 4), ("mp7", 6), ("weno5yc", 2), ("weno5ns", 2)])
+@pytest.mark.parametrize("rec,deg", [("mp5", 4), ("mp7", 6), ("weno5yc", 2), ("weno5ns", 2),
+                                     ("weno3", 1), ("limo3", 1), ("ppm", 3)])
 def test_more_schemes_polynomial_exactness(rec, deg):
     """On monotone polynomial data the MP limiter stays inactive, so MP5 and
     MP7 return their fifth- and seventh-order face values, exact for
     polynomials of degree 4 and 6; the WENO5 variants are convex
-    combinations of three stencils that are each exact for parabolas."""
+    combinations of three stencils that are each exact for parabolas.  WENO3
+    and LimO3 reproduce lines (equal one-sided slopes: equal weights, slope
+    ratio one); PPM's fourth-order face value (7 (f0 + f1) - (f-1 + f2)) / 12
+    is exact for cubics and stays unlimited on monotone smooth data."""
     n = 24
     rng = np.random.default_rng(21)
     for _ in range(5):

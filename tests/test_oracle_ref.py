@@ -160,6 +160,8 @@ def _amr_params(nd, eq, rec, lim, ta, plim="mm", maxlev=3, base=2, problem="blas
     (2, "mhd", "tvd", "mc", "ssprk(4,3)", "zero", 3),
     (3, "mhd", "tvd", "mc", "rk3", "mm", 3),
     (3, "hd", "tvd", "mc", "rk2", "mc", 3),
+    (2, "mhd", "limo3", "mm", "rk3", "mm", 3),
+    (2, "hd", "weno3", "mm", "rk2", "mc", 3),
 ])
 def test_amr_ghost_fill_and_run(nd, eq, rec, lim, ta, plim, maxlev):
     """The reference's own refined mesh (generate_mesh on a blast, frozen):
@@ -180,7 +182,7 @@ def test_amr_ghost_fill_and_run(nd, eq, rec, lim, ta, plim, maxlev):
 
 # ------------------------------------------------------ more 1D schemes
 @needs_ref
-@pytest.mark.parametrize("rec", ["weno5yc", "weno5ns", "mp5", "mp7"])
+@pytest.mark.parametrize("rec", ["weno5yc", "weno5ns", "mp5", "mp7", "weno3", "limo3", "ppm"])
 def test_more_schemes_fresh_pencils(rec):
     rng = np.random.default_rng(17)
     fp = np.concatenate([rng.normal(size=(6, 24)), np.cumsum(rng.normal(size=(4, 24)), axis=1),
@@ -189,19 +191,21 @@ def test_more_schemes_fresh_pencils(rec):
     for lim in ("mm", "mc"):
         p = Params(ndims=3, equation_system="mhd", reconstruction=rec, limiter=lim, ncells=16,
                    nghosts=4, cfl=0.3)
-        fl, fr = ol.oracle_reconstruct(p, fp)
-        rl, rr = ol.ref_reconstruct(p, fp)
-        np.testing.assert_array_equal(fl, rl)
-        np.testing.assert_array_equal(fr, rr)
+        for h in (1.0, 0.03125, 0.7):      # WENO3 and LimO3 weigh by the cell size
+            fl, fr = ol.oracle_reconstruct(p, fp, h)
+            rl, rr = ol.ref_reconstruct(p, fp, h)
+            np.testing.assert_array_equal(fl, rl)
+            np.testing.assert_array_equal(fr, rr)
 
 
 @needs_ref
-@pytest.mark.parametrize("rec", ["weno5yc", "weno5ns", "mp5", "mp7"])
+@pytest.mark.parametrize("rec", ["weno5yc", "weno5ns", "mp5", "mp7", "weno3", "limo3", "ppm"])
 @pytest.mark.parametrize("nd,eq", [(2, "mhd"), (2, "hd"), (3, "mhd")])
 def test_more_schemes_runs(rec, nd, eq):
     p = Params(ndims=nd, equation_system=eq, reconstruction=rec, limiter="mm",
                riemann_solver="hlld" if eq == "mhd" else "hllc", time_advance="rk3",
-               gamma=5.0 / 3.0 if eq == "mhd" else 1.4, cfl=0.3, ncells=8, nghosts=4,
+               gamma=5.0 / 3.0 if eq == "mhd" else 1.4, cfl=0.3, ncells=8,
+               nghosts=2 if rec in ("weno3", "limo3") else 4,
                xblocks=3, yblocks=2, zblocks=2)
     if eq == "hd":
         u0 = problems.blast_hd_2d(p)
