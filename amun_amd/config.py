@@ -28,6 +28,8 @@ RIEMANN = {"hll": 1, "hllc": 2, "hlld": 3, "roe": 4}
 INTEGRATOR = {"euler": 0, "rk2": 1, "rk3": 2, "ssprk(m,2)": 3, "rk3.4": 4, "ssprk(4,3)": 4,
               "rk3.5": 5, "ssprk(5,3)": 5}
 PRECISION = {"fp64": 0, "mixed": 1}
+# equations.F90:747-789
+PRIMSOLVER = {"1dw": 0, "1d(w)": 0, "2dwv": 1, "2d(w,v)": 1, "2dwu": 2, "2d(w,u)": 2}
 
 
 class AmunCfg(ctypes.Structure):
@@ -51,7 +53,7 @@ class AmunCfg(ctypes.Structure):
         ("nr_maxit", ctypes.c_int32), ("nr_extra", ctypes.c_int32),
         ("sgp", ctypes.c_double), ("kappa", ctypes.c_double), ("kbeta", ctypes.c_double),
         ("ngp", ctypes.c_int32), ("stages", ctypes.c_int32),
-        ("prol_limiter", ctypes.c_int32), ("reserved2", ctypes.c_int32),
+        ("prol_limiter", ctypes.c_int32), ("primitive_solver", ctypes.c_int32),
         ("limo3_rad", ctypes.c_double), ("ppm_const", ctypes.c_double),
     ]
 
@@ -110,6 +112,7 @@ class Params:
     ngp: int = 5
     stages: int = 2
     prolongation_limiter: str = "mm"
+    primitive_solver: str = "1Dw"
     limo3_rad: float = 0.5
     ppm_const: float = 1.25
     extra: dict = field(default_factory=dict)
@@ -167,6 +170,7 @@ class Params:
         c.sgp, c.kappa, c.kbeta, c.ngp = self.sgp, self.kappa, self.kbeta, int(self.ngp)
         c.stages = int(self.stages)
         c.limo3_rad, c.ppm_const = self.limo3_rad, self.ppm_const
+        c.primitive_solver = pick(PRIMSOLVER, self.primitive_solver, "primitive_solver", 110)
         # interpolations.F90:412-428: mm, mc, sb, vl, anything else the zero slope
         c.prol_limiter = {"mm": 1, "minmod": 1, "mc": 2, "monotonized_central": 2,
                           "sb": 3, "superbee": 3, "vl": 4, "vanleer": 4}.get(
@@ -204,6 +208,7 @@ class Params:
             f"ngp = {self.ngp}", f"stages = {self.stages}",
             f"prolongation_limiter = {self.prolongation_limiter}",
             f"limo3_rad = {self.limo3_rad!r}", f"ppm_const = {self.ppm_const!r}",
+            f"primitive_solver = {self.primitive_solver}",
             "minlev = 1", "maxlev = 1",
             "xlbndry = periodic", "xubndry = periodic",
             "ylbndry = periodic", "yubndry = periodic",

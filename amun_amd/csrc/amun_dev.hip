@@ -110,6 +110,8 @@ int validate(const amun_cfg& c, int* nv) {
       return fail(600, "the relativistic HLLC solver is not supported on the device path");
     if (!(c.tolerance > 0.0) || c.nr_maxit < 1 || c.nr_extra < 0 || !(c.pmin >= 0.0))
       return fail(500, "tolerance, nr_maxit, nr_extra or pmin out of range");
+    if (c.primitive_solver < AMUN_C2P_1DW || c.primitive_solver > AMUN_C2P_2DWU)
+      return fail(110, "unsupported primitive_solver");
   } else {
     bool rs_ok = c.riemann_solver == AMUN_RS_HLL || c.riemann_solver == AMUN_RS_HLLC ||
                  (c.riemann_solver == AMUN_RS_HLLD && c.eqsys == AMUN_EQSYS_MHD) ||
@@ -232,6 +234,7 @@ RunConst make_rc(const amun_cfg& cfg) {
   rc.sc.pmin = cfg.pmin;
   rc.sc.nrmax = cfg.nr_maxit;
   rc.sc.nrext = cfg.nr_extra;
+  rc.sc.solver = cfg.primitive_solver;
   rc.kappa = std::fmin(cfg.kappa, (1.0 - cfg.cfl) / cfg.cfl);   // interpolations.F90:217
   rc.kbeta = cfg.kbeta;
   if (cfg.reconstruction == AMUN_REC_GP) gp_weights(cfg.sgp, rc.cgp);
@@ -1438,6 +1441,65 @@ int amun_dev_integrals_run(amun_ctx* c, int32_t r, double out[8]) {
   return integrals_range(c, r, 1, out);
 }
 
+// statistics of runs r0 .. r0+nr-1 of the state U0 (its primitives Q[qcur]):
+// mean, minimum and maximum of the seven quantities of store_integrals; the
+// means weigh every cell with its volume and divide by the runs' volume
+static int statistics_range(amun_ctx* c, int r0, int nr, double out[21]) {
+  Dispatch& d = dispatch_of(c);
+  const int nwg = 1024;
+  const double big = 1.7976931348623157e308;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  double sum[7] = {0, 0, 0, 0, 0, 0, 0}, lo[7], hi[7], wsum = 0.0;
+  for (int m = 0; m < 7; m++) { lo[m] = big; hi[m] = -big; }
+  std::vector<int32_t> blev;
+  if (c->amr.on) {
+    blev.resize(c->rc.nblk);
+    HIP_TRY(hipMemcpy(blev.data(), c->amr.blev, blev.size() * sizeof(int32_t),
+                      hipMemcpyDeviceToHost));
+  }
+  for (int g = r0; g < r0 + nr; g++) {
+    const int b0 = c->run_b0[g], nb = c->run_b0[g + 1] - b0;
+    hipLaunchKernelGGL(d.stats, dim3(nwg), dim3(256), 0, c->stream, (const void*)c->Q[c->qcur],
+                       c->rc, c->scratch, b0, nb,
+                       c->amr.on ? (const int*)c->amr.blev : nullptr);
+    HIP_TRY(hipGetLastError());
+    std::vector<double> part((size_t)nwg * 21);
+    HIP_TRY(hipMemcpyAsync(part.data(), c->scratch, part.size() * sizeof(double),
+                           hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (int w = 0; w < nwg; w++)
+      for (int m = 0; m < 7; m++) {
+        const double* p = &part[(size_t)w * 21 + 3 * m];
+        sum[m] += p[0];
+        lo[m] = p[1] < lo[m] ? p[1] : lo[m];
+        hi[m] = p[2] > hi[m] ? p[2] : hi[m];
+      }
+    // the leaves tile the domain: their weights add up to the base cells
+    for (int b = b0; b < b0 + nb; b++)
+      wsum += (c->amr.on ? std::ldexp(1.0, -c->rc.nd * blev[b]) : 1.0) * c->rc.ncell;
+  }
+  // without a magnetic field the reference reports zeros for |B|, psi, M_A
+  const bool mhd = c->nv == 9;
+  for (int m = 0; m < 7; m++) {
+    const bool zero = !mhd && (m == 3 || m == 4 || m == 6);
+    out[3 * m] = zero ? 0.0 : sum[m] / wsum;
+    out[3 * m + 1] = zero ? 0.0 : lo[m];
+    out[3 * m + 2] = zero ? 0.0 : hi[m];
+  }
+  return 0;
+}
+
+int amun_dev_statistics(amun_ctx* c, double out[21]) {
+  if (!c || !out) return fail(500, "null argument");
+  return statistics_range(c, 0, c->rc.nruns, out);
+}
+
+int amun_dev_statistics_run(amun_ctx* c, int32_t r, double out[21]) {
+  if (!c || !out) return fail(500, "null argument");
+  if (r < 0 || r >= c->rc.nruns) return fail(500, "run index out of range");
+  return statistics_range(c, r, 1, out);
+}
+
 int amun_dev_run_state(amun_ctx* c, double* time, double* dt, int64_t* step) {
   if (!c) return fail(500, "null argument");
   std::vector<RunState> h(c->rc.nruns);
@@ -1904,6 +1966,37 @@ int amun_dev_unit_riemann(const amun_cfg* cfg, int32_t n, double cmax, const dou
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipDeviceSynchronize());
   HIP_TRY(hipMemcpy(f, df, m * sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int amun_dev_unit_cons2prim(const amun_cfg* cfg, int32_t n, const double* u, double* q,
+                            int64_t* events) {
+  Dispatch d;
+  RunConst rc;
+  int r = unit_setup(cfg, d, rc);
+  if (r) return r;
+  if (n < 1 || !u || !q) return fail(500, "bad argument");
+  const int nv = (cfg->eqsys == AMUN_EQSYS_HD) ? 5 : 9;
+  const size_t m = (size_t)n * nv;
+  DevBufs B;
+  double *du, *dq;
+  DevState* st;
+  HIP_TRY(B.alloc(&du, m));
+  HIP_TRY(B.alloc(&dq, m));
+  HIP_TRY(B.alloc(&st, 1));
+  HIP_TRY(hipMemset(st, 0, sizeof(DevState)));
+  HIP_TRY(hipMemcpy(du, u, m * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(dq, q, m * sizeof(double), hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(d.unit_c2p, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, 0, n, du, dq,
+                     rc, st);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipDeviceSynchronize());
+  HIP_TRY(hipMemcpy(q, dq, m * sizeof(double), hipMemcpyDeviceToHost));
+  if (events) {
+    DevState h;
+    HIP_TRY(hipMemcpy(&h, st, sizeof(DevState), hipMemcpyDeviceToHost));
+    for (int k = 0; k < 3; k++) events[k] = (int64_t)h.c2p[k];
+  }
   return 0;
 }
 

@@ -484,7 +484,7 @@ struct Recon<P, REC_TVD, ND, DIR, T> {
 // Counts relativistic primitive-solver events (the reference prints each to
 // stderr and goes on).
 __device__ __forceinline__ void c2p_count(DevState* st, int cs) {
-  if (cs & C2P_FAIL) atomicAdd(&st->c2p[0], 1ull);
+  if (cs & (C2P_FAIL | C2P_LOWER)) atomicAdd(&st->c2p[0], 1ull);
   if (cs & C2P_PNEG) atomicAdd(&st->c2p[1], 1ull);
   if (cs & C2P_NOCONV) atomicAdd(&st->c2p[2], 1ull);
 }
@@ -912,6 +912,66 @@ __global__ __launch_bounds__(256) void k_integrals(const double* __restrict__ U,
     for (int m = 0; m < 8; m++) part[blockIdx.x * 8 + m] = sh[m][0];
 }
 
+// statistics of store_integrals (integrals.F90:331-431) from the primitive
+// state: per workgroup, the volume-weighted sums, minima and maxima of
+// density, pressure, |v|, |B|, psi, sqrt(rho)|v|/sqrt(gamma p) and
+// sqrt(rho)|v|/max(eps, |B|); part[wg][3 m + {0, 1, 2}]
+template <class P, typename TQ>
+__global__ __launch_bounds__(256) void k_stats(const void* Qv, RunConst rc, double* part,
+                                               int b0, int nb, const int* __restrict__ blev) {
+  constexpr int NV = P::nv;
+  constexpr double big = 1.7976931348623157e308, eps = 2.220446049250313e-16;
+  __shared__ double sh[21][256];
+  const TQ* __restrict__ Q = static_cast<const TQ*>(Qv) + (size_t)b0 * NV * rc.ncell;
+  const long long n = (long long)nb * rc.ncell;
+  double s[7], lo[7], hi[7];
+#pragma unroll
+  for (int m = 0; m < 7; m++) { s[m] = 0.0; lo[m] = big; hi[m] = -big; }
+  for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < n;
+       g += (long long)gridDim.x * blockDim.x) {
+    const long long b = g / rc.ncell, c = g % rc.ncell;
+    const double w = blev ? ldexp(1.0, -rc.nd * blev[b0 + b]) : 1.0;
+    double q[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) q[v] = double(Q[(b * NV + v) * rc.ncell + c]);
+    const double vel = sqrt((q[P::ivx] * q[P::ivx] + q[P::ivy] * q[P::ivy]) + q[P::ivz] * q[P::ivz]);
+    double mag = 0.0, psi = 0.0;
+    if constexpr (P::mhd) {
+      mag = sqrt((q[P::ibx] * q[P::ibx] + q[P::iby] * q[P::iby]) + q[P::ibz] * q[P::ibz]);
+      psi = q[P::ibp];
+    }
+    const double sv = sqrt(q[P::idn]) * vel;
+    const double t[7] = {q[P::idn], q[P::ipr], vel, mag, psi, sv / sqrt(rc.gamma * q[P::ipr]),
+                         sv / tmax(eps, mag)};
+#pragma unroll
+    for (int m = 0; m < 7; m++) {
+      s[m] += w * t[m];
+      lo[m] = tmin(lo[m], t[m]);
+      hi[m] = tmax(hi[m], t[m]);
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < 7; m++) {
+    sh[3 * m][threadIdx.x] = s[m];
+    sh[3 * m + 1][threadIdx.x] = lo[m];
+    sh[3 * m + 2][threadIdx.x] = hi[m];
+  }
+  __syncthreads();
+  for (int r = 128; r > 0; r >>= 1) {
+    if ((int)threadIdx.x < r)
+#pragma unroll
+      for (int m = 0; m < 7; m++) {
+        sh[3 * m][threadIdx.x] += sh[3 * m][threadIdx.x + r];
+        sh[3 * m + 1][threadIdx.x] = tmin(sh[3 * m + 1][threadIdx.x], sh[3 * m + 1][threadIdx.x + r]);
+        sh[3 * m + 2][threadIdx.x] = tmax(sh[3 * m + 2][threadIdx.x], sh[3 * m + 2][threadIdx.x + r]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int m = 0; m < 21; m++) part[blockIdx.x * 21 + m] = sh[m][0];
+}
+
 // pack interior + ghost layers of every block into the reference host layout
 // u(nv,im,jm,km) (boundary_variables: ghost cells are neighbour copies)
 template <int NV, typename TX>
@@ -990,6 +1050,31 @@ __global__ void k_unit_recon(int n, int npen, double h, const double* __restrict
   }
 }
 
+// q = cons2prim(u) in FP64 on conserved vectors (cons2prim_iface); q holds
+// the previous primitives on entry (kept by a failed relativistic recovery)
+template <class P>
+__global__ void k_unit_c2p(int n, const double* __restrict__ u, double* __restrict__ q,
+                           RunConst rc, DevState* st) {
+  constexpr int NV = P::nv;
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i >= n) return;
+  double ud[NV], qd[NV];
+#pragma unroll
+  for (int v = 0; v < NV; v++) {
+    ud[v] = u[(size_t)i * NV + v];
+    qd[v] = q[(size_t)i * NV + v];
+  }
+  if constexpr (P::sr) {
+    const int cs = cons2prim_sr<P>(rc.sc, ud, qd);
+    if (cs) c2p_count(st, cs);
+  } else {
+    const EosConst<double> ed{rc.gamma, rc.gammam1, rc.gammam1i};
+    cons2prim<P, double>(ed, ud, qd);
+  }
+#pragma unroll
+  for (int v = 0; v < NV; v++) q[(size_t)i * NV + v] = qd[v];
+}
+
 // u = prim2cons(q), f/cm/cp = fluxspeed(q, u) in T; q2 = cons2prim(u) in
 // double (the kernels recover primitives in FP64 in every mode; q2 enters
 // holding the previous primitives, kept by a failed relativistic recovery)
@@ -1051,10 +1136,12 @@ struct Dispatch {
   void (*prim)(const double*, void*, RunConst, DevState*, RunState*, const int*, int, int,
                int) = nullptr;
   void (*integ)(const double*, RunConst, double*, int, int, const int*) = nullptr;
+  void (*stats)(const void*, RunConst, double*, int, int, const int*) = nullptr;
   void (*pack_u)(const void*, double*, const int*, RunConst, int) = nullptr;
   void (*pack_q)(const void*, double*, const int*, RunConst, int) = nullptr;
   void (*unit_riemann)(int, double, const double*, const double*, double*, RunConst) = nullptr;
   void (*unit_recon)(int, int, double, const double*, double*, double*, RunConst) = nullptr;
+  void (*unit_c2p)(int, const double*, double*, RunConst, DevState*) = nullptr;
   void (*unit_eqs)(int, double, const double*, double*, double*, double*, double*, double*,
                    RunConst, DevState*) = nullptr;
   // tangent-linear / adjoint kernels (FP64, TVD reconstruction, HD and MHD)
@@ -1084,9 +1171,11 @@ void fill(Dispatch& d) {
   if constexpr (ND == 3) d.sweep[2] = k_sweep<P, REC, RS, ND, 2, T>;
   d.prim = k_prim<P, T>;
   d.integ = k_integrals<P>;
+  d.stats = k_stats<P, T>;
   d.pack_u = k_pack_blocks<P::nv, double>;
   d.pack_q = k_pack_blocks<P::nv, T>;
   d.unit_riemann = k_unit_riemann<P, RS, T>;
+  d.unit_c2p = k_unit_c2p<P>;
   // TVD reads the transverse slopes of the block: no pencil form
   if constexpr (REC != REC_TVD) d.unit_recon = k_unit_recon<P, REC, T>;
   d.unit_eqs = k_unit_eqs<P, T>;

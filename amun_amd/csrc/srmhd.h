@@ -38,10 +38,12 @@ struct SrConst {
   double gamma, gammaxi, rgammaxi;   // gammaxi = (gamma-1)/gamma
   double tol, pmin;
   int nrmax, nrext;
+  int solver;                        // AMUN_C2P_1DW, _2DWV or _2DWU
 };
 
 // status bits of one primitive recovery
-enum { C2P_FAIL = 1, C2P_PNEG = 2, C2P_NOCONV = 4 };
+// (C2P_LOWER: no bracket, the two-variable solvers go on from the lower one)
+enum { C2P_FAIL = 1, C2P_PNEG = 2, C2P_NOCONV = 4, C2P_LOWER = 8 };
 
 // ---------------------------------------------------------------------------
 // polynomial roots, algebra.F90:86-1085 (coefficients in increasing powers)
@@ -517,15 +519,22 @@ __device__ __forceinline__ void nr_function(const SrConst& s, double mm, double 
   }
 }
 
-// nr_initial_brackets_srmhd_adi (5459-5635) + nr_iterate_srmhd_adi_1dw
-// (5665-5788).  Returns the status bits; on success w and vv hold W and v^2.
-__device__ __forceinline__ int nr_iterate_1dw(const SrConst& s, double mm, double bb,
-                                              double mb, double en, double dn, double& w,
-                                              double& vv) {
+// nr_velocity_srmhd_adi_1d (5249-5290): v^2 of W
+__device__ __forceinline__ double nr_velocity(double mm, double bb, double mb, double w) {
+  const double ss = mb * mb, ww = w * w, wt = w + bb;
+  return (mm * ww + (w + wt) * ss) / (ww * (wt * wt));
+}
+
+// nr_initial_brackets_srmhd_adi (5459-5635): the bracket [wl, wu] of the
+// root of F(W), F at its ends and the secant start w; false without one
+// (wl then holds the last lower bracket tried)
+__device__ __forceinline__ bool nr_brackets(const SrConst& s, double mm, double bb, double mb,
+                                            double en, double dn, double& wl, double& wu,
+                                            double& w, double& fl, double& fu) {
   const double dd = dn * dn;
   const double ss = mb * mb;
   const double ec = en + s.pmin;
-  double wl, wu = en + s.pmin, fl, fu;
+  wu = en + s.pmin;
   // lower bracket: the largest roots of the cubic (pressure) and quartic
   // (1 - v^2) positivity conditions
   {
@@ -542,7 +551,7 @@ __device__ __forceinline__ int nr_iterate_1dw(const SrConst& s, double mm, doubl
       a[1] = -(2.0 * (ss + dd * bb));
       a[0] = -(ss + dd * bb) * bb;
       nr = alg_quartic(a, x);
-      if (nr > 0) wl = fmax(wl, pick<4>(x, nr - 1));
+      if (nr > 0) wl = tmax(wl, pick<4>(x, nr - 1));
     } else {
       // Newton on the pressure from the upper bound (never reached: the
       // normalised cubic always returns a root)
@@ -561,7 +570,7 @@ __device__ __forceinline__ int nr_iterate_1dw(const SrConst& s, double mm, doubl
     }
   }
   nr_function(s, mm, bb, mb, en, dn, wl, fl, nullptr);
-  if (!(fl <= 0.0)) return C2P_FAIL;
+  if (!(fl <= 0.0)) return false;
   if (wu <= wl) wu = 2.0 * wl;
   nr_function(s, mm, bb, mb, en, dn, wu, fu, nullptr);
   {
@@ -575,9 +584,19 @@ __device__ __forceinline__ int nr_iterate_1dw(const SrConst& s, double mm, doubl
       nr_function(s, mm, bb, mb, en, dn, wu, fu, nullptr);
       keep = (fl * fu > 0.0) && it > 0;
     }
-    if (!(it > 0 && fu >= 0.0)) return C2P_FAIL;
+    if (!(it > 0 && fu >= 0.0)) return false;
   }
   w = wl - fl * (wu - wl) / (fu - fl);
+  return true;
+}
+
+// nr_iterate_srmhd_adi_1dw (5665-5788).  Returns the status bits; on
+// success w and vv hold W and v^2.
+__device__ __forceinline__ int nr_iterate_1dw(const SrConst& s, double mm, double bb,
+                                              double mb, double en, double dn, double& w,
+                                              double& vv) {
+  double wl, wu, fl, fu;
+  if (!nr_brackets(s, mm, bb, mb, en, dn, wl, wu, w, fl, fu)) return C2P_FAIL;
   // Newton-Raphson with bracket updates and bisection when outside
   int it = s.nrmax, cn = s.nrext;
   bool keep = true;
@@ -599,11 +618,67 @@ __device__ __forceinline__ int nr_iterate_1dw(const SrConst& s, double mm, doubl
     if (w < wl || w > wu) w = 0.5 * (wl + wu);
     it = it - 1;
   }
-  // nr_velocity_srmhd_adi_1d
-  {
-    const double ww = w * w, wt = w + bb;
-    vv = (mm * ww + (w + wt) * ss) / (ww * (wt * wt));
+  vv = nr_velocity(mm, bb, mb, w);
+  return err >= s.tol ? C2P_NOCONV : 0;
+}
+
+// nr_iterate_srmhd_adi_2dwv (5819-6000) and _2dwu (6020-6210): Newton on
+// (W, v^2), or on (W, u^2 = v^2 / (1 - v^2)), from the bracketed start of
+// 1D(W), with the 2x2 system solved by its inverse.  Without a bracket the
+// lower one is the solution (C2P_LOWER); an iterate below it or with an
+// unphysical speed fails.
+template <bool WU>
+__device__ __forceinline__ int nr_iterate_2d(const SrConst& s, double mm, double bb, double mb,
+                                             double en, double dn, double& w, double& vv) {
+  const double gx = s.gammaxi;
+  double wl, wu, fl, fu;
+  if (!nr_brackets(s, mm, bb, mb, en, dn, wl, wu, w, fl, fu)) {
+    w = wl;
+    vv = nr_velocity(mm, bb, mb, w);
+    return C2P_LOWER;
   }
+  vv = nr_velocity(mm, bb, mb, w);
+  double x = WU ? vv / (1.0 - vv) : vv, err = 0.0;
+  int it = s.nrmax, cn = s.nrext;
+  bool keep = true;
+  while (keep) {
+    double f, g, dfw, dfx, dgw, dgx;
+    if constexpr (!WU) {
+      const double mw = (mb / w) * (mb / w), wt = w + bb, wt2 = wt * wt;
+      const double vm = 1.0 - x, vs = sqrt(vm);
+      f = ((w - en) - gx * (w * vm - dn * vs)) + 0.5 * ((1.0 + x) * bb - mw);
+      g = (x * wt2 - (wt + w) * mw) - mm;
+      dfw = (1.0 - gx * vm) + mw / w;
+      dfx = -(gx * (0.5 * dn / vs - w)) + 0.5 * bb;
+      dgw = 2.0 * wt * (x + mw / w);
+      dgx = wt2;
+    } else {
+      const double up = 1.0 + x, gm = sqrt(up), ss = mb * mb, ww = w * w;
+      const double wt = w + bb, wd = w - gm * dn, wp = wt / up;
+      f = ((w - en) - gx * wd / up) + 0.5 * (bb * (1.0 + x / up) - ss / ww);
+      g = (wp * wt * x - (w + wt) * ss / ww) - mm;
+      dfw = (1.0 - gx / up) + ss / ww / w;
+      dfx = 0.5 * (gx * (w + wd) + bb) / (up * up);
+      dgw = 2.0 * wt * (x / up + ss / ww / w);
+      dgx = wp * wp;
+    }
+    const double det = dfw * dgx - dfx * dgw;
+    const double jfw = dgx / det, jgw = -dfx / det, jfx = -dgw / det, jgx = dfw / det;
+    const double dw = f * jfw + g * jgw, dx = f * jfx + g * jgx;
+    w = w - dw;
+    x = x - dx;
+    if (w < wl) return C2P_FAIL;
+    if (WU ? x < 0.0 : (x < 0.0 || x >= 1.0)) return C2P_FAIL;
+    err = tmax(fabs(dw / w), fabs(dx));
+    if (err < s.tol) {
+      keep = cn > 0;
+      cn = cn - 1;
+    } else {
+      keep = it > 0;
+    }
+    it = it - 1;
+  }
+  vv = WU ? x / (1.0 + x) : x;
   return err >= s.tol ? C2P_NOCONV : 0;
 }
 
@@ -618,7 +693,9 @@ __device__ __forceinline__ int cons2prim_sr(const SrConst& s, const double* u, d
   const double en = u[P::ien] + u[P::idn];
   const double dn = u[P::idn];
   double w = 0.0, vv = 0.0;
-  int st = nr_iterate_1dw(s, mm, bb, mb, en, dn, w, vv);
+  int st = s.solver == AMUN_C2P_2DWV   ? nr_iterate_2d<false>(s, mm, bb, mb, en, dn, w, vv)
+           : s.solver == AMUN_C2P_2DWU ? nr_iterate_2d<true>(s, mm, bb, mb, en, dn, w, vv)
+                                       : nr_iterate_1dw(s, mm, bb, mb, en, dn, w, vv);
   if (st & C2P_FAIL) {
     q[P::ipr] = 0.0;
     return st;

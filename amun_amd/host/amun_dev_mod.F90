@@ -35,6 +35,8 @@ module amun_dev_mod
   integer(c_int32_t), parameter :: AMUN_TA_RK3 = 2, AMUN_TA_SSPRK_M2 = 3
   integer(c_int32_t), parameter :: AMUN_TA_SSPRK43 = 4, AMUN_TA_SSPRK53 = 5
   integer(c_int32_t), parameter :: AMUN_PREC_FP64 = 0, AMUN_PREC_MIXED = 1
+  integer(c_int32_t), parameter :: AMUN_C2P_1DW = 0, AMUN_C2P_2DWV = 1
+  integer(c_int32_t), parameter :: AMUN_C2P_2DWU = 2
 
   type, bind(C) :: amun_cfg
     integer(c_int32_t) :: ndims, eqsys, eos, nv, ncells, nghosts
@@ -48,7 +50,7 @@ module amun_dev_mod
     integer(c_int32_t) :: nr_maxit, nr_extra
     real(c_double)     :: sgp, kappa, kbeta
     integer(c_int32_t) :: ngp, stages
-    integer(c_int32_t) :: prol_limiter, reserved2
+    integer(c_int32_t) :: prol_limiter, primitive_solver
     real(c_double)     :: limo3_rad, ppm_const
   end type amun_cfg
 
@@ -126,6 +128,13 @@ module amun_dev_mod
       import :: c_int, c_ptr, c_double
       type(c_ptr), value                        :: ctx
       real(c_double), dimension(8), intent(out) :: out
+    end function
+
+    integer(c_int) function amun_dev_statistics(ctx, out) &
+                                             bind(C, name="amun_dev_statistics")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value                         :: ctx
+      real(c_double), dimension(21), intent(out) :: out
     end function
 
     integer(c_int) function amun_dev_get_time(ctx, time, step) &

@@ -42,7 +42,7 @@ program amun_host
   integer        :: iret, nv, nc, ng, im, km, nblk, nx, ny, nz, kn
   integer        :: nmax, s, b, i, j, k, p
   integer(int64) :: seed, c0, c1, crate, c2p(3)
-  real(real64)   :: dt, cmax, time, tmax, dtnext, elapsed, ints(8)
+  real(real64)   :: dt, cmax, time, tmax, dtnext, elapsed, ints(8), stats(21)
   character(len=256) :: fin, fout, problem, str
   real(c_double), allocatable, target :: u(:,:,:,:,:)
   integer(c_int32_t), allocatable :: coords(:,:)
@@ -118,6 +118,17 @@ program amun_host
     cfg%prol_limiter = AMUN_LIM_VANLEER
   case default
     cfg%prol_limiter = AMUN_LIM_ZERO
+  end select
+  str = get_str("primitive_solver", "1Dw")
+  select case(trim(str))
+  case("1Dw", "1dw", "1DW", "1D(w)", "1D(W)")
+    cfg%primitive_solver = AMUN_C2P_1DW
+  case("2dwv", "2Dwv", "2D(w,v)", "2D(W,v)")
+    cfg%primitive_solver = AMUN_C2P_2DWV
+  case("2dwu", "2Dwu", "2D(w,u)", "2D(W,u)")
+    cfg%primitive_solver = AMUN_C2P_2DWU
+  case default
+    cfg%primitive_solver = -1
   end select
   str = get_str("riemann_solver", "hlld")
   select case(trim(str))
@@ -240,6 +251,7 @@ program amun_host
   elapsed = real(c1 - c0, real64) / real(crate, real64)
 
   iret = amun_dev_integrals(ctx, ints)
+  iret = amun_dev_statistics(ctx, stats)
   iret = amun_dev_download_blocks(ctx, u, c_null_ptr)
   if (iret /= 0) call die("download")
   call gather()
@@ -248,6 +260,7 @@ program amun_host
   write(*,"(a,i0,a,es22.15)") "steps = ", s, "  time = ", time
   write(*,"(a,es22.15)") "last dt = ", dt
   write(*,"(a,8es22.14)") "integrals = ", ints
+  write(*,"(a,21es16.8)") "statistics = ", stats
   if (cfg%eqsys == AMUN_EQSYS_SRMHD) then
     iret = amun_dev_c2p_events(ctx, c2p)
     write(*,"(a,3(1x,i0))") "primitive recovery failed / p<=0 / not converged =", c2p

@@ -31,10 +31,11 @@ EXPORTS = (
     "amun_dev_run_blocks", "amun_dev_upload_run", "amun_dev_download_run",
     "amun_dev_run_state", "amun_dev_integrals_run", "amun_dev_set_graph",
     "amun_dev_autotune", "amun_dev_set_launch", "amun_dev_unit_riemann",
-    "amun_dev_unit_reconstruct",
+    "amun_dev_unit_reconstruct", "amun_dev_unit_cons2prim",
     "amun_dev_unit_eqs", "amun_dev_tape_forward", "amun_dev_tape_steps",
     "amun_dev_tape_backward", "amun_dev_tape_tangent", "amun_dev_run_fixed",
-    "amun_dev_init_amr", "amun_dev_amr_info",
+    "amun_dev_init_amr", "amun_dev_amr_info", "amun_dev_statistics",
+    "amun_dev_statistics_run",
 )
 
 
@@ -75,6 +76,8 @@ def load_library(path: str = LIB_PATH):
     L.amun_dev_step.argtypes = [vp, ctypes.c_double, _P, _P]
     L.amun_dev_run.argtypes = [vp, ctypes.c_int32, ctypes.c_double, _P]
     L.amun_dev_integrals.argtypes = [vp, _P]
+    L.amun_dev_statistics.argtypes = [vp, _P]
+    L.amun_dev_statistics_run.argtypes = [vp, ctypes.c_int32, _P]
     L.amun_dev_get_time.argtypes = [vp, _P, ctypes.POINTER(ctypes.c_int64)]
     L.amun_dev_nblocks.argtypes = [vp, ctypes.POINTER(ctypes.c_int32)]
     L.amun_dev_block_coords.argtypes = [vp, ctypes.POINTER(ctypes.c_int32)]
@@ -105,6 +108,8 @@ def load_library(path: str = LIB_PATH):
                                     _P, _P, _P, _P, _P, _P]
     L.amun_dev_unit_reconstruct.argtypes = [ctypes.POINTER(AmunCfg), ctypes.c_int32,
                                             ctypes.c_int32, ctypes.c_double, _P, _P, _P]
+    L.amun_dev_unit_cons2prim.argtypes = [ctypes.POINTER(AmunCfg), ctypes.c_int32, _P, _P,
+                                          ctypes.POINTER(ctypes.c_int64)]
     L.amun_dev_init_amr.argtypes = [ctypes.POINTER(AmunCfg), ctypes.c_int32,
                                     ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(vp)]
     L.amun_dev_amr_info.argtypes = [vp, ctypes.POINTER(ctypes.c_int32)]
@@ -315,6 +320,13 @@ class Solver:
         keys = ("mass", "momx", "momy", "momz", "ener", "ekin", "emag", "eint")
         return dict(zip(keys, out.tolist()))
 
+    def statistics(self) -> np.ndarray:
+        """store_integrals' statistics: (mean, min, max) of density, pressure,
+        |v|, |B|, psi, sonic and Alfvenic Mach number, shape (21,)."""
+        out = np.zeros(21)
+        self._check(self.lib.amun_dev_statistics(self.h, _dp(out)))
+        return out
+
     def positivity_events(self) -> int:
         n = ctypes.c_uint64()
         self._check(self.lib.amun_dev_positivity_events(self.h, ctypes.byref(n)))
@@ -439,6 +451,11 @@ class Batch(Solver):
         keys = ("mass", "momx", "momy", "momz", "ener", "ekin", "emag", "eint")
         return dict(zip(keys, out.tolist()))
 
+    def statistics_run(self, r: int) -> np.ndarray:
+        out = np.zeros(21)
+        self._check(self.lib.amun_dev_statistics_run(self.h, r, _dp(out)))
+        return out
+
     def run_blocks(self, r: int):
         f, n = ctypes.c_int32(), ctypes.c_int32()
         self._check(self.lib.amun_dev_run_blocks(self.h, r, ctypes.byref(f), ctypes.byref(n)))
@@ -473,6 +490,19 @@ def unit_reconstruct(params: Params, fp: np.ndarray, h: float = 1.0):
     _unit_check(lib, lib.amun_dev_unit_reconstruct(ctypes.byref(cfg), fp.shape[1], fp.shape[0],
                                                    h, _dp(fp), _dp(fl), _dp(fr)))
     return fl, fr
+
+
+def unit_cons2prim(params: Params, u: np.ndarray, q0: np.ndarray, events: bool = False):
+    """Device FP64 cons2prim on conserved vectors u (n, nv); q0 holds the
+    previous primitives.  Returns q (and the event counts)."""
+    lib = load_library()
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    q = np.array(q0, dtype=np.float64, copy=True, order="C")
+    ev = (ctypes.c_int64 * 3)()
+    cfg = params.to_cfg()
+    _unit_check(lib, lib.amun_dev_unit_cons2prim(ctypes.byref(cfg), u.shape[0], _dp(u), _dp(q),
+                                                 ev))
+    return (q, list(ev)) if events else q
 
 
 def unit_eqs(params: Params, q: np.ndarray, cmax: float = 1.0, q2_init=None):

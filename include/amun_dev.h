@@ -40,6 +40,10 @@ extern "C" {
 #define AMUN_EOS_ADI       1
 
 /* reconstructions: interpolations.F90:237-376 (parameter "reconstruction") */
+#define AMUN_C2P_1DW       0   /* nr_iterate_srmhd_adi_1dw, equations.F90:5665  */
+#define AMUN_C2P_2DWV      1   /* nr_iterate_srmhd_adi_2dwv, equations.F90:5819 */
+#define AMUN_C2P_2DWU      2   /* nr_iterate_srmhd_adi_2dwu, equations.F90:6020 */
+
 #define AMUN_REC_TVD       1
 #define AMUN_REC_WENO3     2   /* reconstruct_weno3, interpolations.F90:1469   */
 #define AMUN_REC_LIMO3     3   /* reconstruct_limo3, interpolations.F90:1598   */
@@ -116,7 +120,9 @@ typedef struct amun_cfg {
    * multi-level meshes ("prolongation_limiter", interpolations.F90:165, 203,
    * 412-428): AMUN_LIM_MINMOD, _MC, _SUPERBEE, _VANLEER or _ZERO (minmod) */
   int32_t prol_limiter;
-  int32_t reserved2;
+  /* "primitive_solver" of the relativistic system (equations.F90:309,
+   * 747-789): AMUN_C2P_1DW (default), AMUN_C2P_2DWV, AMUN_C2P_2DWU */
+  int32_t primitive_solver;
   /* "limo3_rad" (0.5): the radius of the LimO3 smoothness indicator in cell
    * sizes, and "ppm_const" (1.25): the curvature ratio of the PPM extremum
    * limiter (interpolations.F90:83, 106, 209, 212) */
@@ -190,6 +196,8 @@ int  amun_dev_run_state(amun_ctx* ctx, double* time, double* dt, int64_t* step);
 
 /* amun_dev_integrals for run r alone. */
 int  amun_dev_integrals_run(amun_ctx* ctx, int32_t r, double out[8]);
+/* amun_dev_statistics for run r alone. */
+int  amun_dev_statistics_run(amun_ctx* ctx, int32_t r, double out[21]);
 
 /* With enable != 0, steps are captured once into HIP graphs (one per cmax
  * slot, primitive buffer and dtnext) and replayed with hipGraphLaunch.  The
@@ -252,6 +260,14 @@ int  amun_dev_timed_run(amun_ctx* ctx, int32_t nsteps, double dtnext,
  * [6]=magnetic energy, [7]=internal energy (summed over the runs of a
  * batch). */
 int  amun_dev_integrals(amun_ctx* ctx, double out[8]);
+/* The statistics half of store_integrals (integrals.F90:331-431, the
+ * "statistics" file): out = mean, min, max of density, pressure, |v|, |B|,
+ * psi, the sonic Mach number sqrt(rho)|v|/sqrt(gamma p) and the Alfvenic
+ * sqrt(rho)|v|/max(eps, |B|), from the primitive state; the means weigh
+ * cells by their volume (levels included) over the domain volume.  |B|,
+ * psi and M_A are 0 without a magnetic field.  In the mixed mode the
+ * primitives are the stored FP32 ones. */
+int  amun_dev_statistics(amun_ctx* ctx, double out[21]);
 
 /* Simulation time and step counter kept by the context (evolution.F90:75);
  * run 0 of a batch. */
@@ -328,6 +344,13 @@ int  amun_dev_unit_eqs(const amun_cfg* cfg, int32_t n, double cmax, const double
  * couple the directions (error 600). */
 int  amun_dev_unit_reconstruct(const amun_cfg* cfg, int32_t n, int32_t npencil, double h,
                                const double* fp, double* fl, double* fr);
+/* _cons2prim: q = cons2prim(u) in FP64 on n conserved vectors
+ * (cons2prim_iface, equations.F90:70-108; for the relativistic system the
+ * primitive solver of cfg->primitive_solver); q enters holding the previous
+ * primitives (kept but p = 0 by a failed recovery); events (may be NULL)
+ * gets the counts of amun_dev_c2p_events. */
+int  amun_dev_unit_cons2prim(const amun_cfg* cfg, int32_t n, const double* u, double* q,
+                             int64_t* events);
 
 /* Message of the last failing call of this thread. */
 const char* amun_dev_last_error(void);

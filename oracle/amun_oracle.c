@@ -1346,6 +1346,80 @@ static int nr_iterate_1dw(const ocfg* o, double mm, double bb, double mb,
   return info;
 }
 
+/* the Newton step of the two-variable solvers: the 2x2 system
+ * (dfw dfx; dgw dgx) (dw, dx) = (f, g) solved by its inverse, as
+ * nr_iterate_srmhd_adi_2dwv / _2dwu write it */
+static void nr_solve2(double f, double g, double dfw, double dfx, double dgw, double dgx,
+                      double* dw, double* dx)
+{
+  const double det = dfw * dgx - dfx * dgw;
+  const double jfw = dgx / det, jgw = -dfx / det, jfx = -dgw / det, jgx = dfw / det;
+  *dw = f * jfw + g * jgw;
+  *dx = f * jfx + g * jgx;
+}
+
+/* nr_iterate_srmhd_adi_2dwv (equations.F90:5819-6000) and _2dwu (6020-6210):
+ * Newton on (W, v^2) or (W, u^2 = v^2 / (1 - v^2)) from the bracketed
+ * start of 1D(W).  Without a bracket the lower one is the solution (and
+ * counts as a bracketing event); an iterate below it or with an unphysical
+ * speed fails (info 0). */
+static int nr_iterate_2d(ocfg* o, int wu_form, double mm, double bb, double mb, double en,
+                         double dn, double* w, double* vv, int* noconv)
+{
+  const double tol = o->c.tolerance, gx = o->gammaxi;
+  double wl, wu, fl, fu, f, g, dfw, dfx, dgw, dgx, dw, dx, err = 0.0, x;
+  int it, cn, keep;
+  *noconv = 0;
+  if (!nr_initial_brackets(o, mm, bb, mb, en, dn, &wl, &wu, w, &fl, &fu)) {
+    o->c2p_fail++;
+    *w = wl;
+    *vv = nr_velocity(mm, bb, mb, *w);
+    return 1;
+  }
+  *vv = nr_velocity(mm, bb, mb, *w);
+  x = wu_form ? *vv / (1.0 - *vv) : *vv;
+  keep = 1;
+  it = o->c.nr_maxit;
+  cn = o->c.nr_extra;
+  while (keep) {
+    if (!wu_form) {
+      const double mw = (mb / *w) * (mb / *w), wt = *w + bb, wt2 = wt * wt;
+      const double vm = 1.0 - x, vs = sqrt(vm);
+      f = ((*w - en) - gx * (*w * vm - dn * vs)) + 0.5 * ((1.0 + x) * bb - mw);
+      g = (x * wt2 - (wt + *w) * mw) - mm;
+      dfw = (1.0 - gx * vm) + mw / *w;
+      dfx = -(gx * (0.5 * dn / vs - *w)) + 0.5 * bb;
+      dgw = 2.0 * wt * (x + mw / *w);
+      dgx = wt2;
+    } else {
+      const double up = 1.0 + x, gm = sqrt(up), ss = mb * mb, ww = *w * *w;
+      const double wt = *w + bb, wd = *w - gm * dn, wp = wt / up;
+      f = ((*w - en) - gx * wd / up) + 0.5 * (bb * (1.0 + x / up) - ss / ww);
+      g = (wp * wt * x - (*w + wt) * ss / ww) - mm;
+      dfw = (1.0 - gx / up) + ss / ww / *w;
+      dfx = 0.5 * (gx * (*w + wd) + bb) / (up * up);
+      dgw = 2.0 * wt * (x / up + ss / ww / *w);
+      dgx = wp * wp;
+    }
+    nr_solve2(f, g, dfw, dfx, dgw, dgx, &dw, &dx);
+    *w = *w - dw;
+    x = x - dx;
+    if (*w < wl) return 0;
+    if (wu_form ? x < 0.0 : (x < 0.0 || x >= 1.0)) return 0;
+    err = fmax_r(fabs(dw / *w), fabs(dx));
+    if (err < tol) {
+      keep = cn > 0;
+      cn = cn - 1;
+    } else {
+      keep = it > 0;
+    }
+    it = it - 1;
+  }
+  *vv = wu_form ? x / (1.0 + x) : x;
+  if (err >= tol) *noconv = 1;
+  return 1;
+}
+
 /* cons2prim_srmhd_adi, equations.F90:4671-4790.  On failure only the
  * pressure is written (0), the other entries of q keep what they held. */
 static void cons2prim_sr(ocfg* o, const double* u, double* q)
@@ -1357,7 +1431,11 @@ static void cons2prim_sr(ocfg* o, const double* u, double* q)
   bb = (u[o->ibx] * u[o->ibx] + u[o->iby] * u[o->iby]) + u[o->ibz] * u[o->ibz];
   en = u[o->ien] + u[o->idn];
   dn = u[o->idn];
-  info = nr_iterate_1dw(o, mm, bb, mb, en, dn, &w, &vv, &noconv);
+  if (o->c.primitive_solver == AMUN_C2P_2DWV || o->c.primitive_solver == AMUN_C2P_2DWU)
+    info = nr_iterate_2d(o, o->c.primitive_solver == AMUN_C2P_2DWU, mm, bb, mb, en, dn, &w, &vv,
+                         &noconv);
+  else
+    info = nr_iterate_1dw(o, mm, bb, mb, en, dn, &w, &vv, &noconv);
   if (noconv) o->c2p_noconv++;
   if (info) {
     vm = 1.0 - vv;
@@ -2847,6 +2925,74 @@ int orc_get_state(void* h, double* uglob, double* qglob)
   ogrid* g = (ogrid*)h;
   xfer_global(g, g->u0, uglob, 0);
   if (qglob) xfer_global(g, g->q, qglob, 0);
+  return 0;
+}
+
+/* the statistics of store_integrals (integrals.F90:254-485) of the current
+ * primitive state: volume-weighted mean, minimum and maximum of density,
+ * pressure, |v|, |B|, psi, the sonic Mach number sqrt(rho)|v|/sqrt(gamma p)
+ * and the Alfvenic one sqrt(rho)|v|/max(eps, |B|), in that order, out[21];
+ * per-block sums times the block's cell volume, added in block order */
+int orc_statistics(void* h, double* out)
+{
+  ogrid* g = (ogrid*)h;
+  const ocfg* o = &g->o;
+  const size_t bs = bsize(o);
+  const double eps = 2.220446049250313e-16, big = 1.7976931348623157e308;
+  double av[7], mn[7], mx[7], vol;
+  int b, i, j, k, m;
+  for (m = 0; m < 7; m++) { av[m] = 0.0; mn[m] = big; mx[m] = -big; }
+  if (!o->mhd) { mn[3] = mx[3] = mn[4] = mx[4] = mn[6] = mx[6] = 0.0; }
+  for (b = 0; b < g->nblk; b++) {
+    const double* qb = g->q + b * bs;
+    double dvol, s[7] = {0, 0, 0, 0, 0, 0, 0}, lo[7], hi[7];
+    if (g->amr) {
+      const double* lv = g->lvd[g->lev[b]];
+      dvol = (lv[0] * lv[1]) * lv[2];
+    } else {
+      dvol = (o->adx * o->ady) * o->adz;
+    }
+    for (m = 0; m < 7; m++) { lo[m] = big; hi[m] = -big; }
+    for (k = o->kb; k <= o->ke; k++)
+      for (j = o->jb; j <= o->je; j++)
+        for (i = o->ib; i <= o->ie; i++) {
+          const double* q = qb + QI(o, 0, i, j, k);
+          double t[7], vel, mag = 0.0, sqd;
+          vel = sqrt((q[o->ivx] * q[o->ivx] + q[o->ivy] * q[o->ivy]) + q[o->ivz] * q[o->ivz]);
+          if (o->mhd)
+            mag = sqrt((q[o->ibx] * q[o->ibx] + q[o->iby] * q[o->iby]) + q[o->ibz] * q[o->ibz]);
+          sqd = sqrt(q[o->idn]);
+          t[0] = q[o->idn];
+          t[1] = q[o->ipr];
+          t[2] = vel;
+          t[3] = mag;
+          t[4] = o->mhd ? q[o->ibp] : 0.0;
+          t[5] = sqd * vel / sqrt(o->c.gamma * q[o->ipr]);
+          t[6] = sqd * vel / fmax_r(eps, mag);
+          for (m = 0; m < 7; m++) {
+            if (!o->mhd && (m == 3 || m == 4 || m == 6)) continue;
+            s[m] += t[m];
+            lo[m] = fmin_r(lo[m], t[m]);
+            hi[m] = fmax_r(hi[m], t[m]);
+          }
+        }
+    for (m = 0; m < 7; m++) {
+      if (!o->mhd && (m == 3 || m == 4 || m == 6)) continue;
+      av[m] = av[m] + s[m] * dvol;
+      mn[m] = fmin_r(mn[m], lo[m]);
+      mx[m] = fmax_r(mx[m], hi[m]);
+    }
+  }
+  {
+    const double xlen = o->c.xmax - o->c.xmin, ylen = o->c.ymax - o->c.ymin;
+    const double zlen = o->nd == 3 ? o->c.zmax - o->c.zmin : 1.0;
+    vol = 1.0 / ((xlen * ylen) * zlen);
+  }
+  for (m = 0; m < 7; m++) {
+    out[3 * m] = av[m] * vol;
+    out[3 * m + 1] = mn[m];
+    out[3 * m + 2] = mx[m];
+  }
   return 0;
 }
 

@@ -74,6 +74,8 @@ program amun_ref_harness
   use evolution     , only : initialize_evolution, advance, new_time_step
   use evolution     , only : dt, time, step
   use gravity       , only : initialize_gravity
+  use integrals     , only : initialize_integrals, store_integrals
+  use integrals     , only : finalize_integrals
   use interpolations, only : initialize_interpolations, reconstruct
   use io            , only : initialize_io
   use mesh          , only : initialize_mesh, generate_mesh
@@ -164,7 +166,7 @@ contains
     type(block_meta), pointer :: pmeta
     real(kind=8), dimension(:,:,:,:), allocatable :: ug
     real(kind=8), dimension(:,:)    , allocatable :: trace
-    integer :: nx, ny, nz, i0, j0, k0, p, s, u
+    integer :: nx, ny, nz, i0, j0, k0, p, s, u, hint
     integer(kind=8) :: c0, c1, crate
 
     nx = ir * nc
@@ -234,6 +236,15 @@ contains
       step = step + 1
     end do
     call system_clock(c1)
+
+! the reference's integrals and statistics files of the final state
+! (integrals_interval = 1 makes store_integrals write at any step)
+    call get_parameter_integer("harness_integrals", hint)
+    if (hint > 0) then
+      call initialize_integrals(.false., nrun, iret)
+      call store_integrals()
+      call finalize_integrals()
+    end if
 
 ! wall-clock seconds of the step loop only (CPU baseline timing)
     open(newunit=u, file=trim(fout)//"_time.txt", status="replace")

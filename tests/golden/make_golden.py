@@ -469,6 +469,82 @@ def main_o3():
         json.dump(index, fh, indent=1)
 
 
+def main_c2p2d():
+    """The 2D(W,v^2) and 2D(W,u^2) primitive solvers of the relativistic
+    system: the reference's cons2prim on random physical states and on the
+    failing and hard states of unit_eqs_srmhd, and short blast runs (own
+    generator)."""
+    rng = np.random.default_rng(20261210)
+    g = np.load(os.path.join(HERE, "unit_eqs_srmhd.npz"), allow_pickle=False)
+    with open(os.path.join(HERE, "index.json")) as fh:
+        index = json.load(fh)
+    for ps in ("2dwv", "2dwu"):
+        pe = Params(ndims=2, equation_system="srmhd", riemann_solver="hll", gamma=5.0 / 3.0,
+                    tolerance=1.0e-12, nr_maxit=50, primitive_solver=ps)
+        q = sr_states(rng, 400)
+        u, _, _, _, _ = ol.ref_eqs(pe, q, cmax=0.8)
+        ub = np.concatenate([u, g["ub"]])
+        q0 = np.full_like(ub, -7.0)
+        qb, _ = ol.ref_cons2prim(pe, ub, q0)
+        p = Params(ndims=2, equation_system="srmhd", reconstruction="tvd", limiter="mm",
+                   riemann_solver="hll", time_advance="rk2", gamma=5.0 / 3.0, cfl=0.25,
+                   ncells=8, nghosts=2, xblocks=4, yblocks=4, xmin=-0.5, xmax=0.5, ymin=-0.5,
+                   ymax=0.5, tolerance=1.0e-12, nr_maxit=50, primitive_solver=ps)
+        u0 = problems.srmhd_blast_2d(p, noise=0.01)
+        nsteps = 8
+        uf, tr = ol.ref_run(p, u0, nsteps)
+        name = f"srmhd_{ps}_blast2d_32_{nsteps}"
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), params=np.array(p.to_params_in()),
+                            ndims=2, nsteps=nsteps, stride=3, trace=tr, sub=uf[:, :, ::3, ::3],
+                            sha_u0=sha(u0), sha_u=sha(uf), ub=ub, q0=q0, qb=qb,
+                            unit_params=np.array(pe.to_params_in()))
+        index[name] = {"steps": nsteps, "sha_u": sha(uf), "dt_last": float(tr[-1, 0])}
+        print(name, "dt", tr[-1, 0], "failed cells", int((qb[:, 8] == 0.0).sum()))
+    with open(os.path.join(HERE, "index.json"), "w") as fh:
+        json.dump(index, fh, indent=1)
+
+
+STAT_CASES = ("hd2d", "mhd2d", "mhd3d", "srmhd2d")
+
+
+def stat_case(name):
+    """Parameters and initial state of the statistics fixtures."""
+    if name == "hd2d":
+        p = Params(ndims=2, equation_system="hd", reconstruction="tvd", limiter="mc",
+                   riemann_solver="hllc", time_advance="rk2", gamma=1.4, cfl=0.4, ncells=16,
+                   nghosts=2, xblocks=3, yblocks=3, xmin=-0.5, xmax=0.5, ymin=-0.5, ymax=0.5)
+        return p, problems.blast_hd_2d(p, seed=5)
+    if name == "mhd2d":
+        p = Params(ndims=2, equation_system="mhd", reconstruction="tvd", limiter="mc",
+                   riemann_solver="hlld", time_advance="rk3", gamma=5.0 / 3.0, cfl=0.4,
+                   ncells=8, nghosts=2, xblocks=3, yblocks=2)
+        return p, problems.orszag_tang_2d(p, seed=3)
+    if name == "mhd3d":
+        p = c2_params(32)
+        p.xblocks, p.yblocks, p.zblocks, p.ncells = 3, 2, 2, 8
+        return p, problems.turbulent_mhd_3d(p, seed=11)
+    p = Params(ndims=2, equation_system="srmhd", reconstruction="tvd", limiter="mm",
+               riemann_solver="hll", time_advance="rk2", gamma=5.0 / 3.0, cfl=0.25, ncells=8,
+               nghosts=2, xblocks=4, yblocks=4, xmin=-0.5, xmax=0.5, ymin=-0.5, ymax=0.5,
+               tolerance=1.0e-12, nr_maxit=50)
+    return p, problems.srmhd_blast_2d(p, noise=0.01)
+
+
+def main_stats():
+    """The reference's integrals and statistics files (store_integrals, 9
+    significant digits) after three steps of four configurations (own
+    generator)."""
+    out = {}
+    for name in STAT_CASES:
+        p, u0 = stat_case(name)
+        ints, st = ol.ref_statistics(p, u0, 3)
+        out["ints_" + name], out["stats_" + name] = ints, st
+        out["params_" + name] = np.array(p.to_params_in())
+        out["sha_u0_" + name] = np.array(sha(u0))
+        print(name, st[:3])
+    np.savez_compressed(os.path.join(HERE, "stats_ref.npz"), **out)
+
+
 def main():
     if not (ol.ref_available(2) and ol.ref_available(3)):
         sys.exit("reference binaries missing: run oracle/build_ref.sh first")
@@ -498,6 +574,12 @@ def main():
         return
     if sys.argv[1:] == ["negp"]:
         main_negp()
+        return
+    if sys.argv[1:] == ["stats"]:
+        main_stats()
+        return
+    if sys.argv[1:] == ["c2p2d"]:
+        main_c2p2d()
         return
     index = {}
     for name, p, builder, nsteps, stride in run_cases():
@@ -570,6 +652,8 @@ def main():
     main_roe_hd()
     main_o3()
     main_negp()
+    main_stats()
+    main_c2p2d()
 
 
 if __name__ == "__main__":

@@ -37,6 +37,7 @@ def lib():
         L.orc_set_state.argtypes = [ctypes.c_void_p, _P, ctypes.c_double]
         L.orc_get_state.argtypes = [ctypes.c_void_p, _P, _P]
         L.orc_run.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, _P]
+        L.orc_statistics.argtypes = [ctypes.c_void_p, _P]
         L.orc_positivity_events.argtypes = [ctypes.c_void_p]
         L.orc_positivity_events.restype = ctypes.c_long
         L.orc_update_flux.argtypes = [ctypes.c_void_p, _P, _P, ctypes.c_double, _P]
@@ -91,6 +92,13 @@ class OracleRun:
         tr = np.zeros((nsteps, 2))
         lib().orc_run(self.h, nsteps, dtnext, _dp(tr))
         return tr
+
+    def statistics(self) -> np.ndarray:
+        """store_integrals' statistics: (mean, min, max) of density,
+        pressure, |v|, |B|, psi, sonic and Alfvenic Mach number."""
+        out = np.empty(21)
+        lib().orc_statistics(self.h, _dp(out))
+        return out
 
     def positivity_events(self) -> int:
         return lib().orc_positivity_events(self.h)
@@ -238,6 +246,24 @@ def ref_run(p: Params, u0: np.ndarray, nsteps: int):
         u = np.fromfile(os.path.join(d, "out_u.bin")).reshape(u0.shape)
         tr = np.fromfile(os.path.join(d, "out_dt.bin")).reshape(nsteps, 2)
     return u, tr
+
+
+def ref_statistics(p: Params, u0: np.ndarray, nsteps: int):
+    """Run the reference binary nsteps from u0 and let its store_integrals
+    write the integrals and statistics files of the final state; returns
+    (integrals[8], statistics[21]) as printed (9 significant digits)."""
+    with tempfile.TemporaryDirectory() as d:
+        np.ascontiguousarray(u0, dtype=np.float64).tofile(os.path.join(d, "in.bin"))
+        _ref_exec(p, {"harness_mode": "run", "harness_input": "in.bin",
+                      "harness_output": "out", "harness_steps": nsteps,
+                      "harness_integrals": 1, "integrals_interval": 1}, d)
+
+        def last_row(name):
+            rows = [ln.split() for ln in open(os.path.join(d, name)) if not ln.startswith("#")]
+            return np.array([float(x) for x in rows[-1]])
+        ints = last_row("integrals_01.dat")
+        stats = last_row("statistics_01.dat")
+    return ints[3:], stats[2:]
 
 
 def ref_update_flux(p: Params, qblocks: np.ndarray, cmax: float) -> np.ndarray:

@@ -144,6 +144,25 @@ def test_conservation(nd):
     assert ints["mass"] == pytest.approx(totals(u)[0], rel=1e-12)
 
 
+@pytest.mark.parametrize("nd", [2, 3])
+def test_statistics_vs_oracle(nd):
+    """amun_dev_statistics on a multi-level mesh: level-weighted means,
+    minima and maxima of the leaves against the oracle after 3 steps."""
+    p, meta, u0 = _case(nd)
+    print(f"[case] statistics {nd}D blocks={len(meta)}", flush=True)
+    o = ol.OracleAmr(p, meta)
+    o.set_blocks(u0)
+    o.run(3)
+    so = o.statistics()
+    with AmrSolver(p, meta) as s:
+        s.upload(u0)
+        s.run(3)
+        st = s.statistics()
+    # each (mean, min, max) triple on the scale of its largest magnitude
+    scale = np.repeat(np.abs(so).reshape(7, 3).max(axis=1), 3) + 1e-300
+    assert (np.abs(st - so) <= 1e-12 * scale).all(), (st, so)
+
+
 def test_mixed_precision_vs_fp64():
     """FP32 reconstruction, fluxes and ghost fill with the FP64 state: 3D
     blast on three levels, 10 steps, density L1 against the FP64 run."""

@@ -50,6 +50,7 @@ def test_batch_equals_single_runs(precision):
         ub = [b.download(r) for r in range(len(runs))]
         t, dt, st = b.state()
         ints = [b.integrals_run(r) for r in range(len(runs))]
+        stats = [b.statistics_run(r) for r in range(len(runs))]
     for r, (p, u0) in enumerate(zip(runs, u0s)):
         print(f"[case] run {r} shape {p.shape} {precision}", flush=True)
         us, ds = _single(p, u0, 7)
@@ -59,6 +60,11 @@ def test_batch_equals_single_runs(precision):
         with Solver(p) as s:
             s.upload(ub[r])
             ir = s.integrals()
+            sr = s.statistics()
+        # min and max exact; means to the summation order
+        np.testing.assert_array_equal(stats[r][1::3], sr[1::3])
+        np.testing.assert_array_equal(stats[r][2::3], sr[2::3])
+        np.testing.assert_allclose(stats[r][0::3], sr[0::3], rtol=1e-13, atol=1e-13 * np.abs(sr).max())
         for k in ints[r]:
             assert ints[r][k] == pytest.approx(ir[k], rel=1e-14, abs=1e-300)
 

@@ -709,3 +709,84 @@ def test_more_schemes_mixed_vs_fp64(rec):
     l1 = _l1(res[1][0], res[0][0])
     print(f"{rec} 64^3 x 10 mixed vs fp64: density L1 {l1:.3e}")
     assert l1 <= 1e-6
+
+
+# ------------------------------------------------------ store_integrals
+def _stats_tol_ok(a, ref, rel):
+    """Each (mean, min, max) triple judged on the scale of its largest
+    magnitude: minima of |v| or the Mach numbers and the mean of psi sit at
+    round-off level above zero, where a relative bound means nothing."""
+    a, ref = np.asarray(a), np.asarray(ref)
+    scale = np.repeat(np.abs(ref).reshape(7, 3).max(axis=1), 3) + 1e-300
+    return np.abs(a - ref) <= rel * scale
+
+
+@pytest.mark.parametrize("name", ["hd2d", "mhd2d", "mhd3d", "srmhd2d"])
+def test_statistics_and_integrals(name):
+    """amun_dev_statistics / amun_dev_integrals after three steps: against
+    the oracle (1e-12) and the reference's own integrals and statistics files
+    (9 printed digits)."""
+    import os
+    import sys
+    gold = os.path.join(os.path.dirname(__file__), "golden")
+    sys.path.insert(0, gold)
+    from make_golden import stat_case
+    g = np.load(os.path.join(gold, "stats_ref.npz"))
+    p, u0 = stat_case(name)
+    print(f"[case] statistics {name}", flush=True)
+    with Solver(p) as s:
+        s.upload(u0)
+        s.run(3)
+        st = s.statistics()
+        ints = np.array(list(s.integrals().values()))
+    o = ol.OracleRun(p)
+    o.set_state(u0)
+    o.run(3)
+    so = o.statistics()
+    ok = _stats_tol_ok(st, so, 1e-12)
+    assert ok.all(), (np.argwhere(~ok).ravel(), st, so)
+    ok = _stats_tol_ok(st, g["stats_" + name], 2e-8)
+    assert ok.all(), (np.argwhere(~ok).ravel(), st, g["stats_" + name])
+    ri = g["ints_" + name]
+    scale = np.abs(ri).max()
+    assert (np.abs(ints - ri) <= 2e-8 * np.abs(ri) + 1e-14 * scale).all(), (ints, ri)
+
+
+def test_statistics_mixed():
+    p = c2_params(32, precision="mixed")
+    u0 = initial_state(p)
+    with Solver(p) as s:
+        s.upload(u0)
+        s.run(2)
+        sm = s.statistics()
+    p.precision = "fp64"
+    with Solver(p) as s:
+        s.upload(u0)
+        s.run(2)
+        sd = s.statistics()
+    # FP32 primitives (psi, made by cancellation in div B, to 1e-5 of its range)
+    assert _stats_tol_ok(sm, sd, 1e-5).all(), (sm, sd)
+
+
+# ------------------------------------------- 2D relativistic primitive solvers
+@pytest.mark.parametrize("ps", ["2dwv", "2dwu"])
+def test_srmhd_2d_primitive_solver_runs(ps):
+    """The SR blast with the 2D(W,v^2) / 2D(W,u^2) primitive solvers: one step
+    against the oracle (1e-12) and eight steps against the reference's own
+    run (golden fixture)."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", f"srmhd_{ps}_blast2d_32_8.npz"))
+    p = Params.from_text(str(g["params"]), ndims=2)
+    assert p.primitive_solver == ps
+    u0 = srmhd_blast_2d(p, noise=0.01)
+    dt, cmax, ug, tr, uo = one_step(p, u0)
+    assert abs(dt - tr[0]) <= 1e-13 * tr[0]
+    assert relerr(ug, uo).max() <= 1e-12
+    with Solver(p) as s:
+        s.upload(u0)
+        dts = s.run(int(g["nsteps"]))
+        u = s.download()
+        ev = s.c2p_events()
+    assert np.abs(dts - g["trace"][:, 0]).max() <= 1e-12 * g["trace"][:, 0].max()
+    assert relerr(u[:, :, ::3, ::3], g["sub"]).max() <= 1e-10
+    assert ev == [0, 0, 0]

@@ -177,3 +177,25 @@ def test_reconstruction_pencils_golden(case):
     em = np.abs(ml[:, sl] - rl[:, sl]) / scale
     print(f"  mixed: share within 1e-5 {np.mean(em <= 1e-5):.4f}")
     assert np.mean(em <= 1e-5) >= 0.97
+
+
+@pytest.mark.parametrize("ps", ["2dwv", "2dwu"])
+def test_srmhd_2d_primitive_solvers_golden(ps):
+    """The device's 2D(W,v^2) and 2D(W,u^2) recoveries (FP64) on the
+    reference's own vectors: random physical states and states without a
+    bracket (the lower bracket becomes the solution)."""
+    g = np.load(os.path.join(GOLD, f"srmhd_{ps}_blast2d_32_8.npz"), allow_pickle=False)
+    p = Params.from_text(str(g["unit_params"]), ndims=2)
+    ub, qb = g["ub"], g["qb"]
+    from amun_amd.solver import unit_cons2prim
+    q2, ev = unit_cons2prim(p, ub, g["q0"], events=True)
+    qo, evo = ol.oracle_cons2prim(p, ub, g["q0"])
+    print(f"[case] {ps}: max abs {np.abs(q2 - qb).max():.3e}, events {ev} (oracle {list(evo)})")
+    # 400 random physical states: converged Newton solutions
+    ok = close(q2[:400], qb[:400], 1e-12)
+    assert ok.all(), np.argwhere(~ok)[:5]
+    # the hard states: many take the lower bracket (roots of the positivity
+    # polynomials) as W, and p = W - E + ... cancels there
+    ok = close(q2[400:], qb[400:], 1e-9)
+    assert ok.all(), np.argwhere(~ok)[:5]
+    assert ev == list(evo)

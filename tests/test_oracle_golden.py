@@ -317,3 +317,53 @@ def test_roe_hd_run_bit_exact(name):
     tr = o.run(int(g["nsteps"]))
     np.testing.assert_array_equal(tr, g["trace"])
     assert sha(o.get_state()) == str(g["sha_u"])
+
+
+# ------------------------------------------------------ store_integrals
+def _stats_close(a, ref, ndig_rel=2e-8):
+    """Statistics against the reference's printed values (9 significant
+    digits).  The mean of psi is a sum of round-off-sized terms of both
+    signs: judged on the scale of psi's extremes."""
+    a, ref = np.asarray(a), np.asarray(ref)
+    tol = ndig_rel * np.abs(ref) + 1e-300
+    tol[12] = 1e-12 * max(abs(ref[13]), abs(ref[14]), 1e-300)
+    return np.abs(a - ref) <= tol
+
+
+@pytest.mark.parametrize("name", ["hd2d", "mhd2d", "mhd3d", "srmhd2d"])
+def test_statistics_vs_reference_files(name):
+    """The oracle's statistics after three steps against the reference's
+    statistics file of the same run."""
+    import sys
+    sys.path.insert(0, GOLD)
+    from make_golden import stat_case
+    g = load("stats_ref")
+    p, u0 = stat_case(name)
+    assert sha(u0) == str(g["sha_u0_" + name])
+    o = ol.OracleRun(p)
+    o.set_state(u0)
+    o.run(3)
+    ok = _stats_close(o.statistics(), g["stats_" + name])
+    assert ok.all(), (np.argwhere(~ok).ravel(), o.statistics(), g["stats_" + name])
+
+
+# ------------------------------------------- 2D relativistic primitive solvers
+@pytest.mark.parametrize("ps", ["2dwv", "2dwu"])
+def test_srmhd_2d_primitive_solvers(ps):
+    """nr_iterate_srmhd_adi_2dwv / _2dwu: the reference's cons2prim on random
+    physical and on failing states (where the lower bracket becomes the
+    solution), and an 8-step blast run, bit for bit."""
+    g = load(f"srmhd_{ps}_blast2d_32_8")
+    pe = Params.from_text(str(g["unit_params"]), ndims=2)
+    assert pe.primitive_solver == ps
+    qb, ev = ol.oracle_cons2prim(pe, g["ub"], g["q0"])
+    np.testing.assert_array_equal(qb, g["qb"])
+    assert ev[0] > 0          # the bracket fallback was taken
+    p = params_of(g)
+    u0 = problems.srmhd_blast_2d(p, noise=0.01)
+    assert sha(u0) == str(g["sha_u0"])
+    o = ol.OracleRun(p)
+    o.set_state(u0)
+    tr = o.run(int(g["nsteps"]))
+    np.testing.assert_array_equal(tr, g["trace"])
+    assert sha(o.get_state()) == str(g["sha_u"])

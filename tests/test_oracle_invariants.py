@@ -536,3 +536,79 @@ def test_more_schemes_polynomial_exactness(rec, deg):
         np.testing.assert_allclose(fl[0, lo:hi], face[lo:hi], rtol=0, atol=1e-11 * np.abs(face).max())
         np.testing.assert_allclose(fr[0, lo - 1:hi - 1], face[lo - 1:hi - 1], rtol=0,
                                    atol=1e-11 * np.abs(face).max())
+
+
+# ------------------------------------------------------ store_integrals
+def _numpy_stats(p, q, vol_w):
+    """Volume-weighted means, minima and maxima from the definitions
+    (integrals.F90:331-431); q (nv, ncell), vol_w (ncell) cell volumes."""
+    mhd = q.shape[0] == 9
+    rho, pr = q[0], q[-1]
+    vel = np.sqrt(q[1] ** 2 + q[2] ** 2 + q[3] ** 2)
+    mag = np.sqrt(q[4] ** 2 + q[5] ** 2 + q[6] ** 2) if mhd else np.zeros_like(rho)
+    psi = q[7] if mhd else np.zeros_like(rho)
+    ms = np.sqrt(rho) * vel / np.sqrt(p.gamma * pr)
+    ma = np.sqrt(rho) * vel / np.maximum(np.finfo(float).eps, mag) if mhd else np.zeros_like(rho)
+    out = []
+    for t, on in ((rho, True), (pr, True), (vel, True), (mag, mhd), (psi, mhd), (ms, True),
+                  (ma, mhd)):
+        if on:
+            out += [(t * vol_w).sum() / vol_w.sum(), t.min(), t.max()]
+        else:
+            out += [0.0, 0.0, 0.0]
+    return np.array(out)
+
+
+@pytest.mark.parametrize("eq", ["hd", "mhd"])
+def test_statistics_definitions_uniform(eq):
+    p = Params(ndims=2, equation_system=eq, reconstruction="tvd", limiter="mc",
+               riemann_solver="hllc" if eq == "hd" else "hlld", time_advance="rk2",
+               gamma=1.4 if eq == "hd" else 5.0 / 3.0, cfl=0.4, ncells=8, nghosts=2,
+               xblocks=3, yblocks=2, xmin=-0.5, xmax=0.5, ymin=-0.5, ymax=0.5)
+    u0 = problems.blast_hd_2d(p, seed=2) if eq == "hd" else problems.orszag_tang_2d(p, seed=2)
+    o = ol.OracleRun(p)
+    o.set_state(u0)
+    o.run(2)
+    _, q = o.get_state(with_q=True)
+    ref = _numpy_stats(p, q.reshape(p.nv, -1), np.ones(q[0].size))
+    np.testing.assert_allclose(o.statistics(), ref, rtol=1e-13, atol=1e-15)
+
+
+def test_statistics_definitions_multilevel():
+    """Level volumes enter the means: a cell of level l weighs 2^(-nd (l-1))."""
+    from amun_amd import mesh
+    p = Params(ndims=2, equation_system="mhd", reconstruction="tvd", limiter="mc",
+               riemann_solver="hlld", time_advance="rk3", gamma=5.0 / 3.0, cfl=0.4, ncells=8,
+               nghosts=2, xblocks=2, yblocks=2, xmin=-0.5, xmax=0.5, ymin=-0.5, ymax=0.5)
+    meta = mesh.refine(p, mesh.near_sphere(p, 0.15, 0.05), 3)
+    u0 = mesh.block_state(p, meta, mesh.blast_prim(p))
+    o = ol.OracleAmr(p, meta)
+    o.set_blocks(u0)
+    o.run(2)
+    _, q = o.get_blocks(with_q=True)
+    ng, nc = p.nghosts, p.ncells
+    qi = q[:, ng:ng + nc, ng:ng + nc, :]                     # (nb, nc, nc, nv)
+    w = np.repeat(2.0 ** (-2 * (meta[:, 0] - 1)), nc * nc)
+    ref = _numpy_stats(p, np.moveaxis(qi, -1, 0).reshape(p.nv, -1), w)
+    assert len(np.unique(meta[:, 0])) >= 2
+    np.testing.assert_allclose(o.statistics(), ref, rtol=1e-12, atol=1e-15)
+
+
+@pytest.mark.parametrize("ps", ["1Dw", "2dwv", "2dwu"])
+def test_srmhd_primitive_solvers_round_trip(ps):
+    """prim -> cons -> prim through each primitive solver recovers random
+    physical states (|v| < 0.99, pressures and fields over decades)."""
+    import os
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
+    from make_golden import sr_states
+    rng = np.random.default_rng(77)
+    p = Params(ndims=2, equation_system="srmhd", riemann_solver="hll", gamma=5.0 / 3.0,
+               tolerance=1.0e-12, nr_maxit=50, primitive_solver=ps)
+    q = sr_states(rng, 300)
+    u, _, _, _, _ = ol.oracle_eqs(p, q, 0.8)
+    q2, ev = ol.oracle_cons2prim(p, u, np.zeros_like(u))
+    assert ev[0] == 0 and ev[1] == 0
+    scale = np.abs(q).max(axis=1, keepdims=True)
+    err = np.abs(q2 - q) / scale
+    assert np.median(err) <= 1e-13 and np.quantile(err, 0.99) <= 1e-8, err.max()
