@@ -106,8 +106,6 @@ int validate(const amun_cfg& c, int* nv) {
     return fail(202, "sgp must be positive");
   if (c.limiter < 0 || c.limiter > 5) return fail(200, "unsupported limiter");
   if (c.eqsys == AMUN_EQSYS_SRMHD) {
-    if (c.riemann_solver == AMUN_RS_HLLC)
-      return fail(600, "the relativistic HLLC solver is not supported on the device path");
     if (!(c.tolerance > 0.0) || c.nr_maxit < 1 || c.nr_extra < 0 || !(c.pmin >= 0.0))
       return fail(500, "tolerance, nr_maxit, nr_extra or pmin out of range");
     if (c.primitive_solver < AMUN_C2P_1DW || c.primitive_solver > AMUN_C2P_2DWU)

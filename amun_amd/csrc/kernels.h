@@ -194,7 +194,8 @@ enum { RS_HLL = AMUN_RS_HLL, RS_HLLC = AMUN_RS_HLLC, RS_HLLD = AMUN_RS_HLLD, RS_
 template <class P, int RS, typename T>
 __device__ __forceinline__ void riemann(const EosConst<T>& e, const SrConst& sc, T cmax,
                                         T cmax2, T* ql, T* qr, T* f) {
-  if constexpr (P::sr) riemann_sr_hll<P, T>(sc, cmax, cmax2, ql, qr, f);
+  if constexpr (P::sr && RS == RS_HLLC) riemann_sr_hllc<P, T>(sc, cmax, cmax2, ql, qr, f);
+  else if constexpr (P::sr) riemann_sr_hll<P, T>(sc, cmax, cmax2, ql, qr, f);
   else if constexpr (RS == RS_HLLD) riemann_hlld<P, T>(e, cmax, cmax2, ql, qr, f);
   else if constexpr (RS == RS_HLLC && P::mhd) riemann_hllc_mhd<P, T>(e, cmax, cmax2, ql, qr, f);
   else if constexpr (RS == RS_ROE) riemann_roe_hd<P, T>(e, cmax, cmax2, ql, qr, f);
@@ -1216,7 +1217,9 @@ bool pick_rec_t(Dispatch& d, const amun_cfg& c) {
     if (c.riemann_solver == AMUN_RS_HLLD) return pick_nd<MHD, RS_HLLD, REC, T>(d, c.ndims);
     if (c.riemann_solver == AMUN_RS_HLLC) return pick_nd<MHD, RS_HLLC, REC, T>(d, c.ndims);
   } else if (c.eqsys == AMUN_EQSYS_SRMHD) {
-    // every solver name but "hllc" selects HLL (schemes.F90:479-498)
+    // "hllc" selects riemann_srmhd_adi_hllc, every other name HLL
+    // (schemes.F90:479-498)
+    if (c.riemann_solver == AMUN_RS_HLLC) return pick_nd<SRMHD, RS_HLLC, REC, T>(d, c.ndims);
     return pick_nd<SRMHD, RS_HLL, REC, T>(d, c.ndims);
   }
   return false;

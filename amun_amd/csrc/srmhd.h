@@ -753,4 +753,113 @@ __device__ __forceinline__ void riemann_sr_hll(const SrConst& s, T cmax, T cmax2
   }
 }
 
+// riemann_srmhd_adi_hllc (schemes.F90:5382-5700, Mignone & Bodo 2006): the
+// HLL average state and flux, the contact speed as the smaller root of the
+// quadratic of the average state (solved in FP64), the star state of the
+// Bx != 0 or the Bx = 0 case; every degenerate case keeps the HLL flux
+template <class P, typename T>
+__device__ __forceinline__ void riemann_sr_hllc(const SrConst& s, T cmax, T cmax2, T* ql,
+                                                T* qr, T* f) {
+  constexpr int nv = P::nv;
+  constexpr int idn = P::idn, imx = P::imx, imy = P::imy, imz = P::imz, ien = P::ien;
+  constexpr int ibx = P::ibx, iby = P::iby, ibz = P::ibz, ibp = P::ibp, ivx = P::ivx;
+  T ul[nv], ur[nv], fl[nv], fr[nv], wl[nv], wr[nv], uh[nv], fh[nv];
+  T clm, clp, crm, crp;
+  glm_upwind<P>(ql, qr, cmax);
+  const T rgx = T(s.rgammaxi);
+  prim2cons_sr<P>(rgx, ql, ul);
+  prim2cons_sr<P>(rgx, qr, ur);
+  fluxspeed_sr<P>(s, cmax2, ql, ul, fl, clm, clp);
+  fluxspeed_sr<P>(s, cmax2, qr, ur, fr, crm, crp);
+  const T sl = tmin(clm, crm), sr = tmax(clp, crp);
+  if (sl >= T(0)) {
+#pragma unroll
+    for (int p = 0; p < nv; p++) f[p] = fl[p];
+    return;
+  }
+  if (sr <= T(0)) {
+#pragma unroll
+    for (int p = 0; p < nv; p++) f[p] = fr[p];
+    return;
+  }
+  const T srml = sr - sl;
+#pragma unroll
+  for (int p = 0; p < nv; p++) {
+    wl[p] = sl * ul[p] - fl[p];
+    wr[p] = sr * ur[p] - fr[p];
+    uh[p] = (wr[p] - wl[p]) / srml;
+    fh[p] = (sl * wr[p] - sr * wl[p]) / srml;
+    f[p] = fh[p];                           // the flux of every fallback
+  }
+  wl[ien] = wl[ien] + wl[idn];
+  wr[ien] = wr[ien] + wr[idn];
+  const T bx2 = ql[ibx] * qr[ibx];
+  double a[3], x[2];
+  T us[nv];
+  if (bx2 >= T(1.0e-16)) {
+    const T uu = uh[iby] * uh[iby] + uh[ibz] * uh[ibz];
+    const T ff = fh[iby] * fh[iby] + fh[ibz] * fh[ibz];
+    const T uf = uh[iby] * fh[iby] + uh[ibz] * fh[ibz];
+    a[0] = double(uh[imx] - uf);
+    a[1] = double((uu + ff) - ((fh[imx] + uh[ien]) + uh[idn]));
+    a[2] = double((fh[ien] + fh[idn]) - uf);
+    if (alg_quadratic(a, x) < 1) return;
+    const T sm = T(x[0]);
+    if (sm <= sl || sm >= sr) return;
+    us[ibx] = ql[ibx];
+    us[iby] = uh[iby];
+    us[ibz] = uh[ibz];
+    us[ibp] = ql[ibp];
+    const T vs1 = (us[iby] * sm - fh[iby]) / us[ibx];
+    const T vs2 = (us[ibz] * sm - fh[ibz]) / us[ibx];
+    const T vv = (sm * sm + vs1 * vs1) + vs2 * vs2;
+    const T vb = (sm * us[ibx] + vs1 * us[iby]) + vs2 * us[ibz];
+    const T gi = T(1) - vv;
+    const T pt = (fh[imx] + gi * bx2) - ((fh[ien] + fh[idn]) - us[ibx] * vb) * sm;
+    if (pt <= T(0) || sm == T(0)) return;
+    const bool left = sm > T(0);
+    const T sx = left ? sl : sr;
+    const T* qq = left ? ql : qr;
+    const T* uq = left ? ul : ur;
+    const T* fq = left ? fl : fr;
+    const T dv = sx - sm;
+    us[idn] = (sx - qq[ivx]) / dv * uq[idn];
+    us[imy] = ((sx * uq[imy] - fq[imy]) - us[ibx] * (gi * us[iby] + vs1 * vb)) / dv;
+    us[imz] = ((sx * uq[imz] - fq[imz]) - us[ibx] * (gi * us[ibz] + vs2 * vb)) / dv;
+    us[ien] = ((((sx * (uq[ien] + uq[idn])) - (fq[ien] + fq[idn])) + pt * sm) - us[ibx] * vb) / dv -
+              us[idn];
+    us[imx] = ((us[ien] + us[idn]) + pt) * sm - us[ibx] * vb;
+#pragma unroll
+    for (int p = 0; p < nv; p++) f[p] = fq[p] + sx * (us[p] - uq[p]);
+  } else {
+    a[0] = double(uh[imx]);
+    a[1] = double(-((fh[imx] + uh[ien]) + uh[idn]));
+    a[2] = double(fh[ien] + fh[idn]);
+    if (alg_quadratic(a, x) < 1) return;
+    const T sm = T(x[0]);
+    if (sm <= sl || sm >= sr) return;
+    const T pt = fh[imx] - (fh[ien] + fh[idn]) * sm;
+    if (pt <= T(0) || sm == T(0)) return;
+    const bool left = sm > T(0);
+    const T sx = left ? sl : sr;
+    const T* w = left ? wl : wr;
+    const T* qq = left ? ql : qr;
+    const T* uq = left ? ul : ur;
+    const T* fq = left ? fl : fr;
+    const T dv = sx - sm;
+    us[idn] = w[idn] / dv;
+    us[imy] = w[imy] / dv;
+    us[imz] = w[imz] / dv;
+    us[ien] = (w[ien] + pt * sm) / dv;
+    us[imx] = (us[ien] + pt) * sm;
+    us[ien] = us[ien] - us[idn];
+    us[ibx] = T(0);
+    us[iby] = w[iby] / dv;
+    us[ibz] = w[ibz] / dv;
+    us[ibp] = qq[ibp];
+#pragma unroll
+    for (int p = 0; p < nv; p++) f[p] = fq[p] + sx * (us[p] - uq[p]);
+  }
+}
+
 }  // namespace amun

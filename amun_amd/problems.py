@@ -162,10 +162,13 @@ def turbulent_mhd_3d(p: Params, seed: int = DEFAULT_SEED, amplitude: float = 0.1
     return prim_to_cons(p, q)
 
 
-def srmhd_blast_2d(p: Params, seed: int = DEFAULT_SEED, noise: float = 0.0) -> np.ndarray:
+def srmhd_blast_2d(p: Params, seed: int = DEFAULT_SEED, noise: float = 0.0,
+                   vnoise: float = 0.0) -> np.ndarray:
     """Relativistic magnetised cylindrical blast: rho = 1, v = 0, p = 1 where
     the cell centre has r < 0.1 and 0.01 elsewhere, uniform Bx = 0.5, psi = 0;
-    optionally a seeded uniform density perturbation rho *= 1 + noise (2 xi - 1)."""
+    optionally a seeded uniform density perturbation rho *= 1 + noise (2 xi - 1)
+    and seeded velocities vnoise (2 xi - 1) with transverse fields
+    0.5 vnoise (2 xi - 1) (a state off the exact zeros of the gas at rest)."""
     nz, ny, nx = p.shape
     z, y, x = cell_centres(p)
     r = np.sqrt(x[None, None, :] ** 2 + y[None, :, None] ** 2) * np.ones((nz, 1, 1))
@@ -175,6 +178,10 @@ def srmhd_blast_2d(p: Params, seed: int = DEFAULT_SEED, noise: float = 0.0) -> n
         xi = uniform(seed, nz * ny * nx, stream=7).reshape(nz, ny, nx)
         q[0] = 1.0 + noise * (2.0 * xi - 1.0)
     q[4] = 0.5
+    if vnoise:
+        for k, v in enumerate((1, 2, 3, 5, 6)):
+            xi = uniform(seed, nz * ny * nx, stream=30 + k).reshape(nz, ny, nx)
+            q[v] = (0.5 if v > 4 else 1.0) * vnoise * (2.0 * xi - 1.0)
     q[8] = np.where(r < 0.1, 1.0, 0.01)
     return prim_to_cons(p, q)
 

@@ -2149,6 +2149,119 @@ static void riemann_mhd_hlld(const ocfg* o, double* ql, double* qr, double* f)
 #undef OUTER_RIGHT
 }
 
+/* riemann_srmhd_adi_hllc, schemes.F90:5382-5700 (Mignone & Bodo 2006): the
+ * GLM pre-step, the HLL average state and flux, the contact speed as the
+ * smaller root of the quadratic of the average state, and the star states
+ * of the Bx != 0 and Bx = 0 cases; every degenerate case returns the HLL
+ * flux */
+static void riemann_srmhd_hllc(const ocfg* o, double* ql, double* qr, double* f)
+{
+  const int idn = o->idn, imx = o->imx, imy = o->imy, imz = o->imz, ien = o->ien;
+  const int ibx = o->ibx, iby = o->iby, ibz = o->ibz, ibp = o->ibp, ivx = o->ivx;
+  const int nv = o->nv;
+  double ul[NVMAX], ur[NVMAX], fl[NVMAX], fr[NVMAX], wl[NVMAX], wr[NVMAX];
+  double uh[NVMAX], fh[NVMAX], us[NVMAX], a[3], x[2], vs[3];
+  double clm, clp, crm, crp, sl, sr, srml, sm, bx, bp, bx2, uu, ff, uf, vv, vb, gi, pt, dv, fc;
+  int p, nr;
+  bx = 0.5 * ((qr[ibx] + ql[ibx]) - (qr[ibp] - ql[ibp]) / o->cmax);
+  bp = 0.5 * ((qr[ibp] + ql[ibp]) - (qr[ibx] - ql[ibx]) * o->cmax);
+  ql[ibx] = bx; qr[ibx] = bx;
+  ql[ibp] = bp; qr[ibp] = bp;
+  prim2cons(o, ql, ul);
+  prim2cons(o, qr, ur);
+  fluxspeed(o, ql, ul, fl, &clm, &clp);
+  fluxspeed(o, qr, ur, fr, &crm, &crp);
+  sl = fmin_r(clm, crm);
+  sr = fmax_r(clp, crp);
+  if (sl >= 0.0) {
+    for (p = 0; p < nv; p++) f[p] = fl[p];
+    return;
+  }
+  if (sr <= 0.0) {
+    for (p = 0; p < nv; p++) f[p] = fr[p];
+    return;
+  }
+  srml = sr - sl;
+  for (p = 0; p < nv; p++) {
+    wl[p] = sl * ul[p] - fl[p];
+    wr[p] = sr * ur[p] - fr[p];
+    uh[p] = (wr[p] - wl[p]) / srml;
+    fh[p] = (sl * wr[p] - sr * wl[p]) / srml;
+  }
+  wl[ien] = wl[ien] + wl[idn];
+  wr[ien] = wr[ien] + wr[idn];
+  for (p = 0; p < nv; p++) f[p] = fh[p];     /* the HLL flux of every fallback */
+  bx2 = ql[ibx] * qr[ibx];
+  if (bx2 >= 1.0e-16) {
+    uu = uh[iby] * uh[iby] + uh[ibz] * uh[ibz];
+    ff = fh[iby] * fh[iby] + fh[ibz] * fh[ibz];
+    uf = uh[iby] * fh[iby] + uh[ibz] * fh[ibz];
+    a[0] = uh[imx] - uf;
+    a[1] = (uu + ff) - ((fh[imx] + uh[ien]) + uh[idn]);
+    a[2] = (fh[ien] + fh[idn]) - uf;
+    nr = alg_quadratic(a, x);
+    if (nr < 1) return;
+    sm = x[0];
+    if (sm <= sl || sm >= sr) return;
+    us[ibx] = ql[ibx];
+    us[iby] = uh[iby];
+    us[ibz] = uh[ibz];
+    us[ibp] = ql[ibp];
+    vs[0] = sm;
+    vs[1] = (us[iby] * sm - fh[iby]) / us[ibx];
+    vs[2] = (us[ibz] * sm - fh[ibz]) / us[ibx];
+    vv = (vs[0] * vs[0] + vs[1] * vs[1]) + vs[2] * vs[2];
+    vb = (vs[0] * us[ibx] + vs[1] * us[iby]) + vs[2] * us[ibz];
+    gi = 1.0 - vv;
+    pt = (fh[imx] + gi * bx2) - ((fh[ien] + fh[idn]) - us[ibx] * vb) * sm;
+    if (pt <= 0.0) return;
+    if (sm > 0.0 || sm < 0.0) {
+      const double s = sm > 0.0 ? sl : sr;
+      const double* qq = sm > 0.0 ? ql : qr;
+      const double* uq = sm > 0.0 ? ul : ur;
+      const double* fq = sm > 0.0 ? fl : fr;
+      dv = s - sm;
+      fc = (s - qq[ivx]) / dv;
+      us[idn] = fc * uq[idn];
+      us[imy] = ((s * uq[imy] - fq[imy]) - us[ibx] * (gi * us[iby] + vs[1] * vb)) / dv;
+      us[imz] = ((s * uq[imz] - fq[imz]) - us[ibx] * (gi * us[ibz] + vs[2] * vb)) / dv;
+      us[ien] = ((((s * (uq[ien] + uq[idn])) - (fq[ien] + fq[idn])) + pt * sm) - us[ibx] * vb) / dv
+              - us[idn];
+      us[imx] = ((us[ien] + us[idn]) + pt) * sm - us[ibx] * vb;
+      for (p = 0; p < nv; p++) f[p] = fq[p] + s * (us[p] - uq[p]);
+    }
+  } else {
+    a[0] = uh[imx];
+    a[1] = -((fh[imx] + uh[ien]) + uh[idn]);
+    a[2] = fh[ien] + fh[idn];
+    nr = alg_quadratic(a, x);
+    if (nr < 1) return;
+    sm = x[0];
+    if (sm <= sl || sm >= sr) return;
+    pt = fh[imx] - (fh[ien] + fh[idn]) * sm;
+    if (pt <= 0.0) return;
+    if (sm > 0.0 || sm < 0.0) {
+      const double s = sm > 0.0 ? sl : sr;
+      const double* w = sm > 0.0 ? wl : wr;
+      const double* qq = sm > 0.0 ? ql : qr;
+      const double* uq = sm > 0.0 ? ul : ur;
+      const double* fq = sm > 0.0 ? fl : fr;
+      dv = s - sm;
+      us[idn] = w[idn] / dv;
+      us[imy] = w[imy] / dv;
+      us[imz] = w[imz] / dv;
+      us[ien] = (w[ien] + pt * sm) / dv;
+      us[imx] = (us[ien] + pt) * sm;
+      us[ien] = us[ien] - us[idn];
+      us[ibx] = 0.0;
+      us[iby] = w[iby] / dv;
+      us[ibz] = w[ibz] / dv;
+      us[ibp] = qq[ibp];
+      for (p = 0; p < nv; p++) f[p] = fq[p] + s * (us[p] - uq[p]);
+    }
+  }
+}
+
 static int riemann_check(const amun_cfg* c)
 {
   if (c->riemann_solver == AMUN_RS_HLL) return 0;
@@ -2156,15 +2269,19 @@ static int riemann_check(const amun_cfg* c)
       (c->eqsys == AMUN_EQSYS_HD || c->eqsys == AMUN_EQSYS_MHD)) return 0;
   if (c->riemann_solver == AMUN_RS_ROE && c->eqsys == AMUN_EQSYS_HD) return 0;
   if (c->riemann_solver == AMUN_RS_HLLD && c->eqsys == AMUN_EQSYS_MHD) return 0;
-  /* SRMHD: any name other than "hllc" selects HLL (schemes.F90:479-498) */
-  if (c->eqsys == AMUN_EQSYS_SRMHD && c->riemann_solver != AMUN_RS_HLLC) return 0;
-  /* riemann_srmhd_adi_hllc (schemes.F90:5382) is not restated */
+  /* SRMHD: "hllc" selects riemann_srmhd_adi_hllc, any other name HLL
+   * (schemes.F90:479-498) */
+  if (c->eqsys == AMUN_EQSYS_SRMHD) return 0;
   return 300;
 }
 
 static void riemann(const ocfg* o, double* ql, double* qr, double* f)
 {
-  if (o->sr) { riemann_hll(o, ql, qr, f); return; }   /* riemann_srmhd_adi_hll */
+  if (o->sr) {
+    if (o->c.riemann_solver == AMUN_RS_HLLC) riemann_srmhd_hllc(o, ql, qr, f);
+    else riemann_hll(o, ql, qr, f);               /* riemann_srmhd_adi_hll */
+    return;
+  }
   switch (o->c.riemann_solver) {
   case AMUN_RS_HLLC:
     if (o->mhd) riemann_mhd_hllc(o, ql, qr, f);

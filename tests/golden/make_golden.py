@@ -504,6 +504,40 @@ def main_c2p2d():
         json.dump(index, fh, indent=1)
 
 
+def main_sr_hllc():
+    """riemann_srmhd_adi_hllc: update_flux with the zero limiter on random
+    relativistic states (v = 0, Bx = 0 and B = 0 rows included), and a blast
+    run (own generator)."""
+    rng = np.random.default_rng(20261211)
+    pf = Params(ndims=2, equation_system="srmhd", reconstruction="tvd", limiter="zero",
+                riemann_solver="hllc", gamma=5.0 / 3.0, ncells=8, nghosts=2,
+                tolerance=1.0e-12, nr_maxit=50)
+    qb = sr_states(rng, 4 * 144).reshape(4, 1, 12, 12, 9)
+    f = ol.ref_update_flux(pf, qb, cmax=0.9)
+    np.savez_compressed(os.path.join(HERE, "unit_flux_srmhd_hllc.npz"), q=qb, f=f, cmax=0.9,
+                        params=np.array(pf.to_params_in()))
+    with open(os.path.join(HERE, "index.json")) as fh:
+        index = json.load(fh)
+    p = Params(ndims=2, equation_system="srmhd", reconstruction="tvd", limiter="mm",
+               riemann_solver="hllc", time_advance="rk2", gamma=5.0 / 3.0, cfl=0.25, ncells=8,
+               nghosts=2, xblocks=4, yblocks=4, xmin=-0.5, xmax=0.5, ymin=-0.5, ymax=0.5,
+               tolerance=1.0e-12, nr_maxit=50)
+    # velocities and transverse fields off zero: at a gas at rest the
+    # contact speed is round-off and its sign picks the star state or, at an
+    # exact zero, the HLL flux (schemes.F90:5560-5600)
+    u0 = problems.srmhd_blast_2d(p, noise=0.01, vnoise=0.01)
+    nsteps = 8
+    u, tr = ol.ref_run(p, u0, nsteps)
+    name = f"srmhd_hllc_blast2d_32_{nsteps}"
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), params=np.array(p.to_params_in()),
+                        ndims=2, nsteps=nsteps, stride=3, trace=tr, sub=u[:, :, ::3, ::3],
+                        sha_u0=sha(u0), sha_u=sha(u))
+    index[name] = {"steps": nsteps, "sha_u": sha(u), "dt_last": float(tr[-1, 0])}
+    with open(os.path.join(HERE, "index.json"), "w") as fh:
+        json.dump(index, fh, indent=1)
+    print(name, "dt", tr[-1, 0])
+
+
 STAT_CASES = ("hd2d", "mhd2d", "mhd3d", "srmhd2d")
 
 
@@ -581,6 +615,9 @@ def main():
     if sys.argv[1:] == ["c2p2d"]:
         main_c2p2d()
         return
+    if sys.argv[1:] == ["sr_hllc"]:
+        main_sr_hllc()
+        return
     index = {}
     for name, p, builder, nsteps, stride in run_cases():
         u0 = builder(p) if builder else hd3_blast(p)
@@ -654,6 +691,7 @@ def main():
     main_negp()
     main_stats()
     main_c2p2d()
+    main_sr_hllc()
 
 
 if __name__ == "__main__":

@@ -790,3 +790,23 @@ def test_srmhd_2d_primitive_solver_runs(ps):
     assert np.abs(dts - g["trace"][:, 0]).max() <= 1e-12 * g["trace"][:, 0].max()
     assert relerr(u[:, :, ::3, ::3], g["sub"]).max() <= 1e-10
     assert ev == [0, 0, 0]
+
+
+def test_srmhd_hllc_runs():
+    """riemann_srmhd_adi_hllc: one step of the SR blast against the oracle
+    (1e-12) and eight steps against the reference's own run."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "srmhd_hllc_blast2d_32_8.npz"))
+    p = Params.from_text(str(g["params"]), ndims=2)
+    # off the gas at rest: there the contact speed is round-off, and its sign
+    # (or an exact zero, which takes the HLL flux) decides the flux
+    u0 = srmhd_blast_2d(p, noise=0.01, vnoise=0.01)
+    dt, cmax, ug, tr, uo = one_step(p, u0)
+    assert abs(dt - tr[0]) <= 1e-13 * tr[0]
+    assert relerr(ug, uo).max() <= 1e-12
+    with Solver(p) as s:
+        s.upload(u0)
+        dts = s.run(int(g["nsteps"]))
+        u = s.download()
+    assert np.abs(dts - g["trace"][:, 0]).max() <= 1e-12 * g["trace"][:, 0].max()
+    assert relerr(u[:, :, ::3, ::3], g["sub"]).max() <= 1e-10

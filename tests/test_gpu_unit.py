@@ -42,7 +42,7 @@ def _states(nv, n, rng, degenerate=True):
 
 
 CASES = [("hd", "hll"), ("hd", "hllc"), ("mhd", "hll"), ("mhd", "hlld"), ("mhd", "hllc"),
-         ("srmhd", "hll"), ("hd", "roe")]
+         ("srmhd", "hll"), ("hd", "roe"), ("srmhd", "hllc")]
 
 
 @pytest.mark.parametrize("eq,rs", CASES)
@@ -88,7 +88,7 @@ def test_equation_of_state_golden(eq):
 
 
 @pytest.mark.parametrize("eq,rs", [("hd", "hllc"), ("mhd", "hlld"), ("mhd", "hllc"),
-                                   ("srmhd", "hll"), ("hd", "roe")])
+                                   ("srmhd", "hll"), ("hd", "roe"), ("srmhd", "hllc")])
 def test_riemann_golden_via_update_flux_states(eq, rs):
     """The golden update_flux fixtures use the zero limiter, so every face is
     the Riemann problem of its two cell values: the device solver on those

@@ -85,7 +85,7 @@ def test_equation_of_state(eq):
 
 
 @pytest.mark.parametrize("eq,rs", [("hd", "hll"), ("hd", "hllc"), ("mhd", "hll"), ("mhd", "hlld"),
-                                   ("mhd", "hllc"), ("hd", "roe")])
+                                   ("mhd", "hllc"), ("hd", "roe"), ("srmhd", "hllc")])
 def test_riemann_via_update_flux(eq, rs):
     g = load(f"unit_flux_{eq}_{rs}")
     p = Params.from_text(str(g["params"]), ndims=2)
@@ -361,6 +361,20 @@ def test_srmhd_2d_primitive_solvers(ps):
     assert ev[0] > 0          # the bracket fallback was taken
     p = params_of(g)
     u0 = problems.srmhd_blast_2d(p, noise=0.01)
+    assert sha(u0) == str(g["sha_u0"])
+    o = ol.OracleRun(p)
+    o.set_state(u0)
+    tr = o.run(int(g["nsteps"]))
+    np.testing.assert_array_equal(tr, g["trace"])
+    assert sha(o.get_state()) == str(g["sha_u"])
+
+
+def test_srmhd_hllc_run_bit_exact():
+    """riemann_srmhd_adi_hllc in an 8-step relativistic blast run."""
+    g = load("srmhd_hllc_blast2d_32_8")
+    p = params_of(g)
+    assert p.riemann_solver == "hllc"
+    u0 = problems.srmhd_blast_2d(p, noise=0.01, vnoise=0.01)
     assert sha(u0) == str(g["sha_u0"])
     o = ol.OracleRun(p)
     o.set_state(u0)
