@@ -113,7 +113,7 @@ int validate(const amun_cfg& c, int* nv) {
   } else {
     bool rs_ok = c.riemann_solver == AMUN_RS_HLL || c.riemann_solver == AMUN_RS_HLLC ||
                  (c.riemann_solver == AMUN_RS_HLLD && c.eqsys == AMUN_EQSYS_MHD) ||
-                 (c.riemann_solver == AMUN_RS_ROE && c.eqsys == AMUN_EQSYS_HD);
+                 c.riemann_solver == AMUN_RS_ROE;
     if (!rs_ok) return fail(300, "riemann_solver does not match the equation system");
   }
   if (c.time_advance < AMUN_TA_EULER || c.time_advance > AMUN_TA_SSPRK53)

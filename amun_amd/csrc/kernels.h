@@ -198,6 +198,7 @@ __device__ __forceinline__ void riemann(const EosConst<T>& e, const SrConst& sc,
   else if constexpr (P::sr) riemann_sr_hll<P, T>(sc, cmax, cmax2, ql, qr, f);
   else if constexpr (RS == RS_HLLD) riemann_hlld<P, T>(e, cmax, cmax2, ql, qr, f);
   else if constexpr (RS == RS_HLLC && P::mhd) riemann_hllc_mhd<P, T>(e, cmax, cmax2, ql, qr, f);
+  else if constexpr (RS == RS_ROE && P::mhd) riemann_roe_mhd<P, T>(e, cmax, cmax2, ql, qr, f);
   else if constexpr (RS == RS_ROE) riemann_roe_hd<P, T>(e, cmax, cmax2, ql, qr, f);
   else if constexpr (RS == RS_HLLC) riemann_hllc<P, T>(e, cmax, cmax2, ql, qr, f);
   else riemann_hll<P, T>(e, cmax, cmax2, ql, qr, f);
@@ -1216,6 +1217,7 @@ bool pick_rec_t(Dispatch& d, const amun_cfg& c) {
     if (c.riemann_solver == AMUN_RS_HLL) return pick_nd<MHD, RS_HLL, REC, T>(d, c.ndims);
     if (c.riemann_solver == AMUN_RS_HLLD) return pick_nd<MHD, RS_HLLD, REC, T>(d, c.ndims);
     if (c.riemann_solver == AMUN_RS_HLLC) return pick_nd<MHD, RS_HLLC, REC, T>(d, c.ndims);
+    if (c.riemann_solver == AMUN_RS_ROE) return pick_nd<MHD, RS_ROE, REC, T>(d, c.ndims);
   } else if (c.eqsys == AMUN_EQSYS_SRMHD) {
     // "hllc" selects riemann_srmhd_adi_hllc, every other name HLL
     // (schemes.F90:479-498)

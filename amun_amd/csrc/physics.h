@@ -701,6 +701,176 @@ __device__ __forceinline__ void riemann_roe_hd(const EosConst<T>& e, T cmax, T c
   }
 }
 
+// riemann_mhd_adi_roe (schemes.F90:4230-4410) with the eigensystem of
+// esystem_roe_mhd_adi (equations.F90:3162-3454) at the Roe average with the
+// magnetic corrections x, y.  The wave strengths are the left eigenvectors
+// applied to the conserved jump, the wave sum runs from the left wave for a
+// non-negative Roe velocity and from the right one otherwise; the Bx and psi
+// fluxes are the left state's.  Waves 5 and 9 carry nothing.
+template <class P, typename T>
+__device__ __forceinline__ void riemann_roe_mhd(const EosConst<T>& e, T cmax, T cmax2, T* ql,
+                                                T* qr, T* f) {
+  static_assert(P::mhd, "the MHD Roe solver");
+  constexpr int nv = P::nv;
+  constexpr int idn = P::idn, ivx = P::ivx, ivy = P::ivy, ivz = P::ivz, ipr = P::ipr;
+  constexpr int ibx = P::ibx, iby = P::iby, ibz = P::ibz, ien = P::ien;
+  constexpr int imx = P::imx, imy = P::imy, imz = P::imz;
+  T ul[nv], ur[nv], fl[nv], fr[nv];
+  T cm, cp;
+  glm_upwind<P>(ql, qr, cmax);
+  prim2cons<P>(e, ql, ul);
+  prim2cons<P>(e, qr, ur);
+  fluxspeed<P>(e, cmax2, ql, ul, fl, cm, cp);
+  fluxspeed<P>(e, cmax2, qr, ur, fr, cm, cp);
+  const T sdl = tsqrt(ql[idn]), sdr = tsqrt(qr[idn]), sds = sdl + sdr;
+  const T pml = T(0.5) * ((ql[ibx] * ql[ibx] + ql[iby] * ql[iby]) + ql[ibz] * ql[ibz]);
+  const T pmr = T(0.5) * ((qr[ibx] * qr[ibx] + qr[iby] * qr[iby]) + qr[ibz] * qr[ibz]);
+  const T dn = sdl * sdr;
+  const T vx = (sdl * ql[ivx] + sdr * qr[ivx]) / sds;
+  const T vy = (sdl * ql[ivy] + sdr * qr[ivy]) / sds;
+  const T vz = (sdl * ql[ivz] + sdr * qr[ivz]) / sds;
+  const T hh = (((ul[ien] + ql[ipr]) + pml) / sdl + ((ur[ien] + qr[ipr]) + pmr) / sdr) / sds;
+  const T bx = ql[ibx];
+  const T by = (sdr * ql[iby] + sdl * qr[iby]) / sds;
+  const T bz = (sdr * ql[ibz] + sdl * qr[ibz]) / sds;
+  const T dby = ql[iby] - qr[iby], dbz = ql[ibz] - qr[ibz];
+  const T x = T(0.5) * (dby * dby + dbz * dbz) / (sds * sds);
+  const T y = T(0.5) * (ql[idn] + qr[idn]) / dn;
+  // esystem_roe_mhd_adi
+  const T gm1 = e.gammam1, gm2 = e.gamma - T(2);
+  const T di = T(1) / dn;
+  const T casq = bx * bx * di, ca = tsqrt(casq);
+  const T vsq = (vx * vx + vy * vy) + vz * vz;
+  const T btsq = by * by + bz * bz;
+  const T bt_starsq = (gm1 - gm2 * y) * btsq;
+  const T hp = hh - (casq + btsq * di);
+  const T twid_asq = tmax(T(2.220446049250313e-16), gm1 * (hp - T(0.5) * vsq) - gm2 * x);
+  const T ct2 = bt_starsq * di;
+  const T tsum = (casq + ct2) + twid_asq, tdif = (casq + ct2) - twid_asq;
+  const T cfsq = T(0.5) * (tsum + tsqrt(tdif * tdif + T(4) * twid_asq * ct2));
+  const T cf = tsqrt(cfsq);
+  const T cssq = twid_asq * casq / cfsq, cs = tsqrt(cssq);
+  const T c[9] = {vx - cf, vx - ca, vx - cs, vx, vx, vx + cs, vx + ca, vx + cf, vx + cf};
+  if (c[0] >= T(0)) {
+#pragma unroll
+    for (int p = 0; p < nv; p++) f[p] = fl[p];
+    return;
+  }
+  if (c[7] <= T(0)) {
+#pragma unroll
+    for (int p = 0; p < nv; p++) f[p] = fr[p];
+    return;
+  }
+  const T bt = tsqrt(btsq), bt_star = tsqrt(bt_starsq);
+  T bet2 = T(1), bet3 = T(0);
+  if (!(bt == T(0))) {
+    bet2 = by / bt;
+    bet3 = bz / bt;
+  }
+  const T bet2_star = bet2 / tsqrt(gm1 - gm2 * y), bet3_star = bet3 / tsqrt(gm1 - gm2 * y);
+  const T bet_starsq = bet2_star * bet2_star + bet3_star * bet3_star;
+  const T vbet = vy * bet2_star + vz * bet3_star;
+  T alpha_f, alpha_s;
+  if ((cfsq - cssq) == T(0)) { alpha_f = T(1); alpha_s = T(0); }
+  else if ((twid_asq - cssq) <= T(0)) { alpha_f = T(0); alpha_s = T(1); }
+  else if ((cfsq - twid_asq) <= T(0)) { alpha_f = T(1); alpha_s = T(0); }
+  else {
+    alpha_f = tsqrt((twid_asq - cssq) / (cfsq - cssq));
+    alpha_s = tsqrt((cfsq - twid_asq) / (cfsq - cssq));
+  }
+  const T sqrtd = tsqrt(dn), isqrtd = T(1) / sqrtd;
+  const T s = tsign(T(1), bx);
+  const T twid_a = tsqrt(twid_asq);
+  T qf = cf * alpha_f * s, qs = cs * alpha_s * s;
+  const T af_prime = twid_a * alpha_f * isqrtd, as_prime = twid_a * alpha_s * isqrtd;
+  const T afpbb = af_prime * bt_star * bet_starsq, aspbb = as_prime * bt_star * bet_starsq;
+  // right eigenvectors R[wave][m] over the flux components m: idn, imx, imy,
+  // imz, ien, iby, ibz
+  T R[9][7];
+#pragma unroll
+  for (int k = 0; k < 9; k++)
+#pragma unroll
+    for (int m = 0; m < 7; m++) R[k][m] = T(0);
+  R[3][0] = T(1);
+  R[0][0] = alpha_f; R[2][0] = alpha_s; R[5][0] = alpha_s; R[7][0] = alpha_f;
+  R[0][1] = alpha_f * c[0]; R[2][1] = alpha_s * c[2]; R[3][1] = vx;
+  R[5][1] = alpha_s * c[5]; R[7][1] = alpha_f * c[7];
+  {
+    const T qa = alpha_f * vy, qb = alpha_s * vy, qc = qs * bet2_star, qd = qf * bet2_star;
+    R[0][2] = qa + qc; R[1][2] = -bet3; R[2][2] = qb - qd; R[3][2] = vy;
+    R[5][2] = qb + qd; R[6][2] = bet3; R[7][2] = qa - qc;
+  }
+  {
+    const T qa = alpha_f * vz, qb = alpha_s * vz, qc = qs * bet3_star, qd = qf * bet3_star;
+    R[0][3] = qa + qc; R[1][3] = bet2; R[2][3] = qb - qd; R[3][3] = vz;
+    R[5][3] = qb + qd; R[6][3] = -bet2; R[7][3] = qa - qc;
+  }
+  R[0][4] = (alpha_f * (hp - vx * cf) + qs * vbet) + aspbb;
+  R[1][4] = -(vy * bet3 - vz * bet2);
+  R[2][4] = (alpha_s * (hp - vx * cs) - qf * vbet) - afpbb;
+  R[3][4] = T(0.5) * vsq + gm2 * x / gm1;
+  R[5][4] = (alpha_s * (hp + vx * cs) + qf * vbet) - afpbb;
+  R[6][4] = -R[1][4];
+  R[7][4] = (alpha_f * (hp + vx * cf) - qs * vbet) + aspbb;
+  R[0][5] = as_prime * bet2_star; R[1][5] = -(bet3 * s * isqrtd);
+  R[2][5] = -(af_prime * bet2_star);
+  R[5][5] = R[2][5]; R[6][5] = R[1][5]; R[7][5] = R[0][5];
+  R[0][6] = as_prime * bet3_star; R[1][6] = bet2 * s * isqrtd;
+  R[2][6] = -(af_prime * bet3_star);
+  R[5][6] = R[2][6]; R[6][6] = R[1][6]; R[7][6] = R[0][6];
+  // left eigenvectors applied to the conserved jump: al[wave]
+  T norm = T(0.5) / twid_asq;
+  const T cff = norm * alpha_f * cf, css = norm * alpha_s * cs;
+  qf = qf * norm;
+  qs = qs * norm;
+  const T af = norm * af_prime * dn, as = norm * as_prime * dn;
+  const T afpb = norm * af_prime * bt_star, aspb = norm * as_prime * bt_star;
+  norm = norm * gm1;
+  const T af_n = alpha_f * norm, as_n = alpha_s * norm;
+  const T q2_star = bet2_star / bet_starsq, q3_star = bet3_star / bet_starsq;
+  const T vqstr = vy * q2_star + vz * q3_star;
+  norm = T(2) * norm;
+  const T d0 = ur[idn] - ul[idn], d1 = ur[imx] - ul[imx], d2 = ur[imy] - ul[imy];
+  const T d3 = ur[imz] - ul[imz], d5 = ur[iby] - ul[iby], d6 = ur[ibz] - ul[ibz];
+  const T d8 = ur[ien] - ul[ien];
+  auto lsum = [&](T l0, T l1, T l2, T l3, T l5, T l6, T l8) {
+    return (((((l0 * d0 + l1 * d1) + l2 * d2) + l3 * d3) + l5 * d5) + l6 * d6) + l8 * d8;
+  };
+  T al[9];
+  al[0] = lsum(((af_n * (vsq - hp) + cff * (cf + vx)) - qs * vqstr) - aspb, -(af_n * vx) - cff,
+               -(af_n * vy) + qs * q2_star, -(af_n * vz) + qs * q3_star,
+               as * q2_star - af_n * by, as * q3_star - af_n * bz, af_n);
+  const T l20 = T(0.5) * (vy * bet3 - vz * bet2), l22 = -(T(0.5) * bet3), l23 = T(0.5) * bet2;
+  const T l25 = -(T(0.5) * sqrtd * bet3 * s), l26 = T(0.5) * sqrtd * bet2 * s;
+  al[1] = lsum(l20, T(0), l22, l23, l25, l26, T(0));
+  const T l35 = -(af * q2_star) - as_n * by, l36 = -(af * q3_star) - as_n * bz;
+  al[2] = lsum(((as_n * (vsq - hp) + css * (cs + vx)) + qf * vqstr) + afpb, -(as_n * vx) - css,
+               -(as_n * vy) - qf * q2_star, -(as_n * vz) - qf * q3_star, l35, l36, as_n);
+  al[3] = lsum(T(1) - norm * (T(0.5) * vsq - gm2 * x / gm1), norm * vx, norm * vy, norm * vz,
+               norm * by, norm * bz, -norm);
+  al[4] = T(0);
+  al[5] = lsum(((as_n * (vsq - hp) + css * (cs - vx)) - qf * vqstr) + afpb, -(as_n * vx) + css,
+               -(as_n * vy) + qf * q2_star, -(as_n * vz) + qf * q3_star, l35, l36, as_n);
+  al[6] = lsum(-l20, T(0), -l22, -l23, l25, l26, T(0));
+  const T l15 = as * q2_star - af_n * by, l16 = as * q3_star - af_n * bz;
+  al[7] = lsum(((af_n * (vsq - hp) + cff * (cf - vx)) + qs * vqstr) - aspb, -(af_n * vx) + cff,
+               -(af_n * vy) - qs * q2_star, -(af_n * vz) - qs * q3_star, l15, l16, af_n);
+  al[8] = T(0);
+  constexpr int fv[7] = {idn, imx, imy, imz, ien, iby, ibz};
+#pragma unroll
+  for (int m = 0; m < 7; m++) f[fv[m]] = T(0.5) * (fl[fv[m]] + fr[fv[m]]);
+  f[ibx] = fl[ibx];
+  f[P::ibp] = fl[P::ibp];
+  const bool fwd = vx >= T(0);
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    const int w = fwd ? k : 8 - k;
+    const T xx = -(T(0.5) * al[w] * tabs(c[w]));
+#pragma unroll
+    for (int m = 0; m < 7; m++) f[fv[m]] = f[fv[m]] + xx * R[w][m];
+  }
+}
+
 // riemann_mhd_adi_hllc, schemes.F90:3305-3571: one intermediate state on
 // each side of the contact (GLM pre-step as HLL; its own Bx = 0 branch)
 template <class P, typename T>

@@ -2149,6 +2149,217 @@ static void riemann_mhd_hlld(const ocfg* o, double* ql, double* qr, double* f)
 #undef OUTER_RIGHT
 }
 
+/* esystem_roe_mhd_adi, equations.F90:3162-3454: the eigenvalues c[9] and
+ * the left (l[var][wave]) and right (r[wave][var]) eigenvectors of the
+ * adiabatic MHD Roe matrix at the averaged state q (q[ien] is the Roe
+ * enthalpy; x and y the magnetic corrections).  Waves 5 and 9 and the
+ * components ibx and ibp stay zero; nothing but c is set when c[0] >= 0 or
+ * c[7] <= 0 (returns 0 then). */
+static int esystem_roe_mhd(const ocfg* o, double x, double y, const double* q, double* c,
+                           double (*l)[NVMAX], double (*r)[NVMAX])
+{
+  const int idn = o->idn, ivx = o->ivx, ivy = o->ivy, ivz = o->ivz, ipr = o->ipr;
+  const int iby = o->iby, ibz = o->ibz, ibx = o->ibx, nv = o->nv;
+  const double gammam1 = o->gammam1, gammam2 = o->c.gamma - 2.0;
+  const double eps = 2.220446049250313e-16;
+  double di, vsq, btsq, bt_starsq, casq, hp, twid_asq, ct2, tsum, tdif, cf2_cs2, cfsq, cf;
+  double cssq, cs, ca, bt, bt_star, bet2, bet3, bet2_star, bet3_star, bet_starsq, vbet;
+  double alpha_f, alpha_s, af_prime, as_prime, sqrtd, isqrtd, s, twid_a, qf, qs, afpbb, aspbb;
+  double qa, qb, qc, qd, norm, cff, css, af, as, afpb, aspb, q2_star, q3_star, vqstr;
+  int p, k;
+  di = 1.0 / q[idn];
+  casq = q[ibx] * q[ibx] * di;
+  ca = sqrt(casq);
+  vsq = (q[ivx] * q[ivx] + q[ivy] * q[ivy]) + q[ivz] * q[ivz];
+  btsq = q[iby] * q[iby] + q[ibz] * q[ibz];
+  bt_starsq = (gammam1 - gammam2 * y) * btsq;
+  hp = q[ipr] - (casq + btsq * di);
+  twid_asq = fmax_r(eps, gammam1 * (hp - 0.5 * vsq) - gammam2 * x);
+  ct2 = bt_starsq * di;
+  tsum = (casq + ct2) + twid_asq;
+  tdif = (casq + ct2) - twid_asq;
+  cf2_cs2 = sqrt(tdif * tdif + 4.0 * twid_asq * ct2);
+  cfsq = 0.5 * (tsum + cf2_cs2);
+  cf = sqrt(cfsq);
+  cssq = twid_asq * casq / cfsq;
+  cs = sqrt(cssq);
+  c[0] = q[ivx] - cf; c[1] = q[ivx] - ca; c[2] = q[ivx] - cs; c[3] = q[ivx];
+  c[4] = q[ivx]; c[5] = q[ivx] + cs; c[6] = q[ivx] + ca; c[7] = q[ivx] + cf; c[8] = c[7];
+  if (c[0] >= 0.0 || c[7] <= 0.0) return 0;
+  for (p = 0; p < nv; p++)
+    for (k = 0; k < nv; k++) { l[p][k] = 0.0; r[p][k] = 0.0; }
+  bt = sqrt(btsq);
+  bt_star = sqrt(bt_starsq);
+  if (bt == 0.0) { bet2 = 1.0; bet3 = 0.0; }
+  else { bet2 = q[iby] / bt; bet3 = q[ibz] / bt; }
+  bet2_star = bet2 / sqrt(gammam1 - gammam2 * y);
+  bet3_star = bet3 / sqrt(gammam1 - gammam2 * y);
+  bet_starsq = bet2_star * bet2_star + bet3_star * bet3_star;
+  vbet = q[ivy] * bet2_star + q[ivz] * bet3_star;
+  if ((cfsq - cssq) == 0.0) { alpha_f = 1.0; alpha_s = 0.0; }
+  else if ((twid_asq - cssq) <= 0.0) { alpha_f = 0.0; alpha_s = 1.0; }
+  else if ((cfsq - twid_asq) <= 0.0) { alpha_f = 1.0; alpha_s = 0.0; }
+  else {
+    alpha_f = sqrt((twid_asq - cssq) / (cfsq - cssq));
+    alpha_s = sqrt((cfsq - twid_asq) / (cfsq - cssq));
+  }
+  sqrtd = sqrt(q[idn]);
+  isqrtd = 1.0 / sqrtd;
+  s = copysign(1.0, q[ibx]);
+  twid_a = sqrt(twid_asq);
+  qf = cf * alpha_f * s;
+  qs = cs * alpha_s * s;
+  af_prime = twid_a * alpha_f * isqrtd;
+  as_prime = twid_a * alpha_s * isqrtd;
+  afpbb = af_prime * bt_star * bet_starsq;
+  aspbb = as_prime * bt_star * bet_starsq;
+  /* right eigenvectors r[wave][var] (evroe(2, wave, var)) */
+  r[3][idn] = 1.0;
+  r[0][idn] = alpha_f; r[2][idn] = alpha_s; r[5][idn] = alpha_s; r[7][idn] = alpha_f;
+  r[0][ivx] = alpha_f * c[0]; r[2][ivx] = alpha_s * c[2]; r[3][ivx] = q[ivx];
+  r[5][ivx] = alpha_s * c[5]; r[7][ivx] = alpha_f * c[7];
+  qa = alpha_f * q[ivy]; qb = alpha_s * q[ivy]; qc = qs * bet2_star; qd = qf * bet2_star;
+  r[0][ivy] = qa + qc; r[1][ivy] = -bet3; r[2][ivy] = qb - qd; r[3][ivy] = q[ivy];
+  r[5][ivy] = qb + qd; r[6][ivy] = bet3; r[7][ivy] = qa - qc;
+  qa = alpha_f * q[ivz]; qb = alpha_s * q[ivz]; qc = qs * bet3_star; qd = qf * bet3_star;
+  r[0][ivz] = qa + qc; r[1][ivz] = bet2; r[2][ivz] = qb - qd; r[3][ivz] = q[ivz];
+  r[5][ivz] = qb + qd; r[6][ivz] = -bet2; r[7][ivz] = qa - qc;
+  r[0][ipr] = (alpha_f * (hp - q[ivx] * cf) + qs * vbet) + aspbb;
+  r[1][ipr] = -(q[ivy] * bet3 - q[ivz] * bet2);
+  r[2][ipr] = (alpha_s * (hp - q[ivx] * cs) - qf * vbet) - afpbb;
+  r[3][ipr] = 0.5 * vsq + gammam2 * x / gammam1;
+  r[5][ipr] = (alpha_s * (hp + q[ivx] * cs) + qf * vbet) - afpbb;
+  r[6][ipr] = -r[1][ipr];
+  r[7][ipr] = (alpha_f * (hp + q[ivx] * cf) - qs * vbet) + aspbb;
+  r[0][iby] = as_prime * bet2_star; r[1][iby] = -(bet3 * s * isqrtd);
+  r[2][iby] = -(af_prime * bet2_star);
+  r[5][iby] = r[2][iby]; r[6][iby] = r[1][iby]; r[7][iby] = r[0][iby];
+  r[0][ibz] = as_prime * bet3_star; r[1][ibz] = bet2 * s * isqrtd;
+  r[2][ibz] = -(af_prime * bet3_star);
+  r[5][ibz] = r[2][ibz]; r[6][ibz] = r[1][ibz]; r[7][ibz] = r[0][ibz];
+  /* left eigenvectors l[var][wave] (evroe(1, var, wave)) */
+  norm = 0.5 / twid_asq;
+  cff = norm * alpha_f * cf;
+  css = norm * alpha_s * cs;
+  qf = qf * norm;
+  qs = qs * norm;
+  af = norm * af_prime * q[idn];
+  as = norm * as_prime * q[idn];
+  afpb = norm * af_prime * bt_star;
+  aspb = norm * as_prime * bt_star;
+  norm = norm * gammam1;
+  alpha_f = alpha_f * norm;
+  alpha_s = alpha_s * norm;
+  q2_star = bet2_star / bet_starsq;
+  q3_star = bet3_star / bet_starsq;
+  vqstr = q[ivy] * q2_star + q[ivz] * q3_star;
+  norm = 2.0 * norm;
+  l[idn][0] = ((alpha_f * (vsq - hp) + cff * (cf + q[ivx])) - qs * vqstr) - aspb;
+  l[ivx][0] = -(alpha_f * q[ivx]) - cff;
+  l[ivy][0] = -(alpha_f * q[ivy]) + qs * q2_star;
+  l[ivz][0] = -(alpha_f * q[ivz]) + qs * q3_star;
+  l[ipr][0] = alpha_f;
+  l[iby][0] = as * q2_star - alpha_f * q[iby];
+  l[ibz][0] = as * q3_star - alpha_f * q[ibz];
+  l[idn][1] = 0.5 * (q[ivy] * bet3 - q[ivz] * bet2);
+  l[ivy][1] = -(0.5 * bet3);
+  l[ivz][1] = 0.5 * bet2;
+  l[iby][1] = -(0.5 * sqrtd * bet3 * s);
+  l[ibz][1] = 0.5 * sqrtd * bet2 * s;
+  l[idn][2] = ((alpha_s * (vsq - hp) + css * (cs + q[ivx])) + qf * vqstr) + afpb;
+  l[ivx][2] = -(alpha_s * q[ivx]) - css;
+  l[ivy][2] = -(alpha_s * q[ivy]) - qf * q2_star;
+  l[ivz][2] = -(alpha_s * q[ivz]) - qf * q3_star;
+  l[ipr][2] = alpha_s;
+  l[iby][2] = -(af * q2_star) - alpha_s * q[iby];
+  l[ibz][2] = -(af * q3_star) - alpha_s * q[ibz];
+  l[idn][3] = 1.0 - norm * (0.5 * vsq - gammam2 * x / gammam1);
+  l[ivx][3] = norm * q[ivx];
+  l[ivy][3] = norm * q[ivy];
+  l[ivz][3] = norm * q[ivz];
+  l[ipr][3] = -norm;
+  l[iby][3] = norm * q[iby];
+  l[ibz][3] = norm * q[ibz];
+  l[idn][5] = ((alpha_s * (vsq - hp) + css * (cs - q[ivx])) - qf * vqstr) + afpb;
+  l[ivx][5] = -(alpha_s * q[ivx]) + css;
+  l[ivy][5] = -(alpha_s * q[ivy]) + qf * q2_star;
+  l[ivz][5] = -(alpha_s * q[ivz]) + qf * q3_star;
+  l[ipr][5] = alpha_s;
+  l[iby][5] = l[iby][2];
+  l[ibz][5] = l[ibz][2];
+  l[idn][6] = -l[idn][1];
+  l[ivy][6] = -l[ivy][1];
+  l[ivz][6] = -l[ivz][1];
+  l[iby][6] = l[iby][1];
+  l[ibz][6] = l[ibz][1];
+  l[idn][7] = ((alpha_f * (vsq - hp) + cff * (cf - q[ivx])) + qs * vqstr) - aspb;
+  l[ivx][7] = -(alpha_f * q[ivx]) + cff;
+  l[ivy][7] = -(alpha_f * q[ivy]) - qs * q2_star;
+  l[ivz][7] = -(alpha_f * q[ivz]) - qs * q3_star;
+  l[ipr][7] = alpha_f;
+  l[iby][7] = l[iby][0];
+  l[ibz][7] = l[ibz][0];
+  return 1;
+}
+
+/* riemann_mhd_adi_roe, schemes.F90:4230-4410: the GLM pre-step, Roe
+ * averages (with the magnetic corrections x, y), the eigensystem above and
+ * the flux 1/2 (fl + fr) minus the wave contributions, added from the left
+ * wave for a non-negative Roe velocity and from the right one otherwise;
+ * the Bx and psi fluxes are the left ones */
+static void riemann_mhd_roe(const ocfg* o, double* ql, double* qr, double* f)
+{
+  const int idn = o->idn, ivx = o->ivx, ivy = o->ivy, ivz = o->ivz, ipr = o->ipr;
+  const int ibx = o->ibx, iby = o->iby, ibz = o->ibz, ibp = o->ibp, ien = o->ien;
+  const int nv = o->nv;
+  const int fv[7] = {o->idn, o->imx, o->imy, o->imz, o->ien, o->iby, o->ibz};
+  double ul[NVMAX], ur[NVMAX], fl[NVMAX], fr[NVMAX], qi[NVMAX], ci[NVMAX], al[NVMAX];
+  double li[NVMAX][NVMAX], ri[NVMAX][NVMAX], sdl, sdr, sds, pml, pmr, xx, yy, bx, bp, cm_, cp_;
+  int p, k, m;
+  bx = 0.5 * ((qr[ibx] + ql[ibx]) - (qr[ibp] - ql[ibp]) / o->cmax);
+  bp = 0.5 * ((qr[ibp] + ql[ibp]) - (qr[ibx] - ql[ibx]) * o->cmax);
+  ql[ibx] = bx; qr[ibx] = bx;
+  ql[ibp] = bp; qr[ibp] = bp;
+  prim2cons(o, ql, ul);
+  prim2cons(o, qr, ur);
+  fluxspeed(o, ql, ul, fl, &cm_, &cp_);
+  fluxspeed(o, qr, ur, fr, &cm_, &cp_);
+  sdl = sqrt(ql[idn]);
+  sdr = sqrt(qr[idn]);
+  sds = sdl + sdr;
+  pml = 0.5 * ((ql[ibx] * ql[ibx] + ql[iby] * ql[iby]) + ql[ibz] * ql[ibz]);
+  pmr = 0.5 * ((qr[ibx] * qr[ibx] + qr[iby] * qr[iby]) + qr[ibz] * qr[ibz]);
+  for (p = 0; p < nv; p++) qi[p] = 0.0;
+  qi[idn] = sdl * sdr;
+  qi[ivx] = (sdl * ql[ivx] + sdr * qr[ivx]) / sds;
+  qi[ivy] = (sdl * ql[ivy] + sdr * qr[ivy]) / sds;
+  qi[ivz] = (sdl * ql[ivz] + sdr * qr[ivz]) / sds;
+  qi[ipr] = (((ul[ien] + ql[ipr]) + pml) / sdl + ((ur[ien] + qr[ipr]) + pmr) / sdr) / sds;
+  qi[ibx] = ql[ibx];
+  qi[iby] = (sdr * ql[iby] + sdl * qr[iby]) / sds;
+  qi[ibz] = (sdr * ql[ibz] + sdl * qr[ibz]) / sds;
+  qi[ibp] = ql[ibp];
+  xx = 0.5 * ((ql[iby] - qr[iby]) * (ql[iby] - qr[iby]) + (ql[ibz] - qr[ibz]) * (ql[ibz] - qr[ibz]))
+     / (sds * sds);
+  yy = 0.5 * (ql[idn] + qr[idn]) / qi[idn];
+  if (!esystem_roe_mhd(o, xx, yy, qi, ci, li, ri)) {
+    const double* fs = ci[0] >= 0.0 ? fl : fr;
+    for (p = 0; p < nv; p++) f[p] = fs[p];
+    return;
+  }
+  f[ibx] = fl[ibx];
+  f[ibp] = fl[ibp];
+  for (k = 0; k < nv; k++) al[k] = 0.0;
+  for (p = 0; p < nv; p++)
+    for (k = 0; k < nv; k++) al[k] = al[k] + li[p][k] * (ur[p] - ul[p]);
+  for (m = 0; m < 7; m++) f[fv[m]] = 0.5 * (fl[fv[m]] + fr[fv[m]]);
+  for (k = 0; k < nv; k++) {
+    const int w = qi[ivx] >= 0.0 ? k : nv - 1 - k;
+    xx = -(0.5 * al[w] * fabs(ci[w]));
+    for (m = 0; m < 7; m++) f[fv[m]] = f[fv[m]] + xx * ri[w][fv[m]];
+  }
+}
+
 /* riemann_srmhd_adi_hllc, schemes.F90:5382-5700 (Mignone & Bodo 2006): the
  * GLM pre-step, the HLL average state and flux, the contact speed as the
  * smaller root of the quadratic of the average state, and the star states
@@ -2267,7 +2478,8 @@ static int riemann_check(const amun_cfg* c)
   if (c->riemann_solver == AMUN_RS_HLL) return 0;
   if (c->riemann_solver == AMUN_RS_HLLC &&
       (c->eqsys == AMUN_EQSYS_HD || c->eqsys == AMUN_EQSYS_MHD)) return 0;
-  if (c->riemann_solver == AMUN_RS_ROE && c->eqsys == AMUN_EQSYS_HD) return 0;
+  if (c->riemann_solver == AMUN_RS_ROE &&
+      (c->eqsys == AMUN_EQSYS_HD || c->eqsys == AMUN_EQSYS_MHD)) return 0;
   if (c->riemann_solver == AMUN_RS_HLLD && c->eqsys == AMUN_EQSYS_MHD) return 0;
   /* SRMHD: "hllc" selects riemann_srmhd_adi_hllc, any other name HLL
    * (schemes.F90:479-498) */
@@ -2288,7 +2500,10 @@ static void riemann(const ocfg* o, double* ql, double* qr, double* f)
     else riemann_hd_hllc(o, ql, qr, f);
     break;
   case AMUN_RS_HLLD: riemann_mhd_hlld(o, ql, qr, f); break;
-  case AMUN_RS_ROE:  riemann_hd_roe(o, ql, qr, f); break;
+  case AMUN_RS_ROE:
+    if (o->mhd) riemann_mhd_roe(o, ql, qr, f);
+    else riemann_hd_roe(o, ql, qr, f);
+    break;
   default:           riemann_hll(o, ql, qr, f); break;
   }
 }

@@ -538,6 +538,43 @@ def main_sr_hllc():
     print(name, "dt", tr[-1, 0])
 
 
+def main_roe_mhd():
+    """riemann_mhd_adi_roe through update_flux (zero limiter): random states
+    with zero transverse field, Bx = 0 and fast flows; and an Orszag-Tang run
+    with noise on every field (own generator)."""
+    rng = np.random.default_rng(20261212)
+    pf = Params(ndims=2, equation_system="mhd", reconstruction="tvd", limiter="zero",
+                riemann_solver="roe", gamma=5.0 / 3.0, ncells=8, nghosts=2)
+    qb = np.zeros((4, 1, 12, 12, 9))
+    qb[..., 0] = rng.uniform(0.2, 2.0, qb.shape[:-1])
+    qb[..., 1:4] = rng.normal(scale=1.0, size=qb.shape[:-1] + (3,))
+    qb[..., 8] = rng.uniform(0.1, 2.0, qb.shape[:-1])
+    qb[..., 4:7] = rng.normal(size=qb.shape[:-1] + (3,))
+    qb[..., 7] = rng.normal(scale=0.1, size=qb.shape[:-1])
+    qb[1, ..., 5:7] = 0.0                 # no transverse field
+    qb[2, ..., 4] = 0.0                   # Bx = 0
+    qb[3, ..., 1] = qb[3, ..., 1] * 5.0   # fast flows
+    f = ol.ref_update_flux(pf, qb, cmax=2.3)
+    np.savez_compressed(os.path.join(HERE, "unit_flux_mhd_roe.npz"), q=qb, f=f, cmax=2.3,
+                        params=np.array(pf.to_params_in()))
+    with open(os.path.join(HERE, "index.json")) as fh:
+        index = json.load(fh)
+    p = Params(ndims=2, equation_system="mhd", reconstruction="tvd", limiter="mc",
+               riemann_solver="roe", time_advance="rk3", gamma=5.0 / 3.0, cfl=0.4, ncells=16,
+               nghosts=2, xblocks=2, yblocks=2)
+    u0 = ot_noisy(p)
+    nsteps = 10
+    u, tr = ol.ref_run(p, u0, nsteps)
+    name = f"roe_mhd_otnoisy2d_32_{nsteps}"
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), params=np.array(p.to_params_in()),
+                        ndims=2, nsteps=nsteps, stride=3, trace=tr, sub=u[:, :, ::3, ::3],
+                        sha_u0=sha(u0), sha_u=sha(u))
+    index[name] = {"steps": nsteps, "sha_u": sha(u), "dt_last": float(tr[-1, 0])}
+    with open(os.path.join(HERE, "index.json"), "w") as fh:
+        json.dump(index, fh, indent=1)
+    print(name, "dt", tr[-1, 0])
+
+
 STAT_CASES = ("hd2d", "mhd2d", "mhd3d", "srmhd2d")
 
 
@@ -618,6 +655,9 @@ def main():
     if sys.argv[1:] == ["sr_hllc"]:
         main_sr_hllc()
         return
+    if sys.argv[1:] == ["roe_mhd"]:
+        main_roe_mhd()
+        return
     index = {}
     for name, p, builder, nsteps, stride in run_cases():
         u0 = builder(p) if builder else hd3_blast(p)
@@ -692,6 +732,7 @@ def main():
     main_stats()
     main_c2p2d()
     main_sr_hllc()
+    main_roe_mhd()
 
 
 if __name__ == "__main__":

@@ -70,7 +70,7 @@ def test_c2_one_step_32():
 
 @pytest.mark.parametrize("nd", [2, 3])
 @pytest.mark.parametrize("eq,rs", [("hd", "hll"), ("hd", "hllc"), ("mhd", "hll"), ("mhd", "hlld"),
-                                   ("mhd", "hllc"), ("hd", "roe")])
+                                   ("mhd", "hllc"), ("hd", "roe"), ("mhd", "roe")])
 @pytest.mark.parametrize("rec,lim", [("tvd", "mc"), ("tvd", "mm"), ("weno5z", "mm")])
 @pytest.mark.parametrize("ta", ["euler", "rk2", "rk3"])
 def test_matrix_one_step(nd, eq, rs, rec, lim, ta):
@@ -804,6 +804,25 @@ def test_srmhd_hllc_runs():
     dt, cmax, ug, tr, uo = one_step(p, u0)
     assert abs(dt - tr[0]) <= 1e-13 * tr[0]
     assert relerr(ug, uo).max() <= 1e-12
+    with Solver(p) as s:
+        s.upload(u0)
+        dts = s.run(int(g["nsteps"]))
+        u = s.download()
+    assert np.abs(dts - g["trace"][:, 0]).max() <= 1e-12 * g["trace"][:, 0].max()
+    assert relerr(u[:, :, ::3, ::3], g["sub"]).max() <= 1e-10
+
+
+def test_roe_mhd_golden_run():
+    """The MHD Roe solver in ten Orszag-Tang steps (noise on every field)
+    against the reference's own run."""
+    import os
+    import sys
+    gold = os.path.join(os.path.dirname(__file__), "golden")
+    g = np.load(os.path.join(gold, "roe_mhd_otnoisy2d_32_10.npz"))
+    p = Params.from_text(str(g["params"]), ndims=2)
+    sys.path.insert(0, gold)
+    from make_golden import ot_noisy
+    u0 = ot_noisy(p)
     with Solver(p) as s:
         s.upload(u0)
         dts = s.run(int(g["nsteps"]))

@@ -42,7 +42,7 @@ def _states(nv, n, rng, degenerate=True):
 
 
 CASES = [("hd", "hll"), ("hd", "hllc"), ("mhd", "hll"), ("mhd", "hlld"), ("mhd", "hllc"),
-         ("srmhd", "hll"), ("hd", "roe"), ("srmhd", "hllc")]
+         ("srmhd", "hll"), ("hd", "roe"), ("srmhd", "hllc"), ("mhd", "roe")]
 
 
 @pytest.mark.parametrize("eq,rs", CASES)
@@ -60,6 +60,16 @@ def test_riemann_vs_oracle(eq, rs):
     fo = ol.oracle_riemann(p, ql, qr, cmax=1.7)
     fg = unit_riemann(p, ql, qr, cmax=1.7)
     ok = close(fg, fo, 1e-12)
+    if eq == "mhd" and rs == "roe":
+        # without transverse field esystem_roe_mhd_adi's cfsq equals a~^2 or
+        # c_A^2 up to round-off, and alpha_s = sqrt((cfsq - a~^2) / ...) is the
+        # square root of that round-off (equations.F90:3276-3289): those rows
+        # (250:500, also the identical and near-identical pairs among the
+        # first 400) agree to sqrt(eps)
+        rows = np.zeros(len(fo), bool)
+        rows[250:500] = True
+        rows[:400] |= (np.abs(ql[:400, 5:7]).sum(axis=1) == 0)
+        ok[rows] = close(fg[rows], fo[rows], 1e-7)
     print(f"[case] {eq}/{rs}: max rel {np.abs(fg - fo).max():.3e}")
     assert ok.all(), np.argwhere(~ok)[:5]
     pm = Params(**{k: v for k, v in p.__dict__.items() if k != "extra"})
@@ -88,7 +98,7 @@ def test_equation_of_state_golden(eq):
 
 
 @pytest.mark.parametrize("eq,rs", [("hd", "hllc"), ("mhd", "hlld"), ("mhd", "hllc"),
-                                   ("srmhd", "hll"), ("hd", "roe"), ("srmhd", "hllc")])
+                                   ("srmhd", "hll"), ("hd", "roe"), ("srmhd", "hllc"), ("mhd", "roe")])
 def test_riemann_golden_via_update_flux_states(eq, rs):
     """The golden update_flux fixtures use the zero limiter, so every face is
     the Riemann problem of its two cell values: the device solver on those

@@ -85,7 +85,8 @@ def test_equation_of_state(eq):
 
 
 @pytest.mark.parametrize("eq,rs", [("hd", "hll"), ("hd", "hllc"), ("mhd", "hll"), ("mhd", "hlld"),
-                                   ("mhd", "hllc"), ("hd", "roe"), ("srmhd", "hllc")])
+                                   ("mhd", "hllc"), ("hd", "roe"), ("srmhd", "hllc"),
+                                   ("mhd", "roe")])
 def test_riemann_via_update_flux(eq, rs):
     g = load(f"unit_flux_{eq}_{rs}")
     p = Params.from_text(str(g["params"]), ndims=2)
@@ -375,6 +376,23 @@ def test_srmhd_hllc_run_bit_exact():
     p = params_of(g)
     assert p.riemann_solver == "hllc"
     u0 = problems.srmhd_blast_2d(p, noise=0.01, vnoise=0.01)
+    assert sha(u0) == str(g["sha_u0"])
+    o = ol.OracleRun(p)
+    o.set_state(u0)
+    tr = o.run(int(g["nsteps"]))
+    np.testing.assert_array_equal(tr, g["trace"])
+    assert sha(o.get_state()) == str(g["sha_u"])
+
+
+def test_roe_mhd_run_bit_exact():
+    """riemann_mhd_adi_roe in a 10-step Orszag-Tang run (noise on every field)."""
+    import sys
+    sys.path.insert(0, GOLD)
+    from make_golden import ot_noisy
+    g = load("roe_mhd_otnoisy2d_32_10")
+    p = params_of(g)
+    assert p.riemann_solver == "roe"
+    u0 = ot_noisy(p)
     assert sha(u0) == str(g["sha_u0"])
     o = ol.OracleRun(p)
     o.set_state(u0)
