@@ -29,6 +29,9 @@ INTEGRATOR = {"euler": 0, "rk2": 1, "rk3": 2, "ssprk(m,2)": 3, "rk3.4": 4, "sspr
               "rk3.5": 5, "ssprk(5,3)": 5}
 PRECISION = {"fp64": 0, "mixed": 1}
 # equations.F90:747-789
+# boundaries.F90:5024-5637 (hydrostatic and user types are not built)
+BNDTYPE = {"periodic": 0, "open": 1, "outflow": 2, "out": 2, "reflective": 3, "reflecting": 3,
+           "reflect": 3}
 PRIMSOLVER = {"1dw": 0, "1d(w)": 0, "2dwv": 1, "2d(w,v)": 1, "2dwu": 2, "2d(w,u)": 2}
 
 
@@ -55,6 +58,7 @@ class AmunCfg(ctypes.Structure):
         ("ngp", ctypes.c_int32), ("stages", ctypes.c_int32),
         ("prol_limiter", ctypes.c_int32), ("primitive_solver", ctypes.c_int32),
         ("limo3_rad", ctypes.c_double), ("ppm_const", ctypes.c_double),
+        ("bnd_type", ctypes.c_int32 * 6),
     ]
 
 
@@ -113,6 +117,12 @@ class Params:
     stages: int = 2
     prolongation_limiter: str = "mm"
     primitive_solver: str = "1Dw"
+    xlbndry: str = "periodic"
+    xubndry: str = "periodic"
+    ylbndry: str = "periodic"
+    yubndry: str = "periodic"
+    zlbndry: str = "periodic"
+    zubndry: str = "periodic"
     limo3_rad: float = 0.5
     ppm_const: float = 1.25
     extra: dict = field(default_factory=dict)
@@ -171,6 +181,9 @@ class Params:
         c.stages = int(self.stages)
         c.limo3_rad, c.ppm_const = self.limo3_rad, self.ppm_const
         c.primitive_solver = pick(PRIMSOLVER, self.primitive_solver, "primitive_solver", 110)
+        for n, side in enumerate((self.xlbndry, self.xubndry, self.ylbndry, self.yubndry,
+                                  self.zlbndry, self.zubndry)):
+            c.bnd_type[n] = pick(BNDTYPE, side, "boundary type", 600)
         # interpolations.F90:412-428: mm, mc, sb, vl, anything else the zero slope
         c.prol_limiter = {"mm": 1, "minmod": 1, "mc": 2, "monotonized_central": 2,
                           "sb": 3, "superbee": 3, "vl": 4, "vanleer": 4}.get(
@@ -210,9 +223,9 @@ class Params:
             f"limo3_rad = {self.limo3_rad!r}", f"ppm_const = {self.ppm_const!r}",
             f"primitive_solver = {self.primitive_solver}",
             "minlev = 1", "maxlev = 1",
-            "xlbndry = periodic", "xubndry = periodic",
-            "ylbndry = periodic", "yubndry = periodic",
-            "zlbndry = periodic", "zubndry = periodic",
+            f"xlbndry = {self.xlbndry}", f"xubndry = {self.xubndry}",
+            f"ylbndry = {self.ylbndry}", f"yubndry = {self.yubndry}",
+            f"zlbndry = {self.zlbndry}", f"zubndry = {self.zubndry}",
             "problem = blast", "snapshot_interval = 0", "restart_interval = 0",
             "integrals_interval = 1000000", "tmax = 1.0e+30", "nmax = 1000000000",
         ]

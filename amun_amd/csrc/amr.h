@@ -146,6 +146,53 @@ static __global__ void k_amr_prolong(const int4* __restrict__ task, int ntask,
   }
 }
 
+// Ghost blocks of the non-periodic domain sides (boundaries_specific,
+// boundaries.F90:1089-1251, block_boundary_specific 5024-5637).  A virtual
+// block stands next to a boundary block across one or more non-periodic
+// sides; task = {virtual block, source block, side per axis (2 bits each:
+// 0 none, 1 low, 2 high)}.  Its cells are the source block's, mirrored
+// (reflective) or held at the edge cell (open, outflow) along every such
+// axis, with the side types applied in x, y, z order: outflow clamps the
+// normal velocity to point outwards, reflective negates the normal velocity
+// and field.  One thread per cell.
+template <int NV, typename T>
+static __global__ void k_bnd_fill(const int4* __restrict__ task, int ntask, T* Q, RunConst rc,
+                                  int ivx, int ibx) {
+  const int nc = rc.nc, nd = rc.nd;
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= (long long)ntask * rc.ncell) return;
+  const int4 t = task[n / rc.ncell];
+  const int c = (int)(n % rc.ncell);
+  const int ci[3] = {c % nc, (c / nc) % nc, nd == 3 ? c / (nc * nc) : 0};
+  int si[3], side[3], type[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    side[a] = (t.z >> (2 * a)) & 3;
+    type[a] = side[a] ? rc.bnd[2 * a + side[a] - 1] : 0;
+    if (!side[a]) si[a] = ci[a];
+    else if (type[a] == AMUN_BND_REFLECTIVE) si[a] = nc - 1 - ci[a];
+    else si[a] = side[a] == 1 ? 0 : nc - 1;
+  }
+  const size_t dst = (size_t)t.x * NV * rc.ncell + lidx<sizeof(T)>(nc, nd, ci[0], ci[1], ci[2]);
+  const size_t src = (size_t)t.y * NV * rc.ncell + lidx<sizeof(T)>(nc, nd, si[0], si[1], si[2]);
+  T q[NV];
+#pragma unroll
+  for (int v = 0; v < NV; v++) q[v] = Q[src + (size_t)v * rc.ncell];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    if (!side[a]) continue;
+    if (type[a] == AMUN_BND_OUTFLOW) {
+      const T x = q[ivx + a];
+      q[ivx + a] = side[a] == 1 ? (T(0) < x ? T(0) : x) : (T(0) > x ? T(0) : x);
+    } else if (type[a] == AMUN_BND_REFLECTIVE) {
+      q[ivx + a] = -q[ivx + a];
+      if (ibx >= 0) q[ibx + a] = -q[ibx + a];
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < NV; v++) Q[dst + (size_t)v * rc.ncell] = q[v];
+}
+
 // Flux through face c (0: the lower face of the pencil's first cell, nc:
 // the upper face of its last) of pencil (c1, c2) of block b along DIR, as
 // the sweep computes it (original component order).

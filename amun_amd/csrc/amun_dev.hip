@@ -66,9 +66,40 @@ struct amun_ctx {
     double* FB = nullptr;               // averaged fine fluxes per slot
     DevState* dummy = nullptr;          // events of the fine-flux recomputation
   } amr;
+  // ghost blocks of the non-periodic domain sides
+  struct Bnd {
+    bool on = false;
+    int n = 0;                          // virtual blocks
+    int4* task = nullptr;               // {virtual, source, sides}
+  } bnd;
 };
 
 static int amr_fill(amun_ctx* c, int qcur, int slot);
+
+// the ghost blocks of the non-periodic sides from the primitive state Q[qcur]
+static int bnd_fill(amun_ctx* c, int qcur) {
+  if (!c->bnd.on) return 0;
+  const long long n = (long long)c->bnd.n * c->rc.ncell;
+  const dim3 grid((unsigned)((n + 255) / 256)), bl(256);
+  const int ibx = c->nv == 9 ? 4 : -1;
+  if (c->qsize == sizeof(double)) {
+    if (c->nv == 9)
+      hipLaunchKernelGGL((k_bnd_fill<9, double>), grid, bl, 0, c->stream, c->bnd.task, c->bnd.n,
+                         static_cast<double*>(c->Q[qcur]), c->rc, 1, ibx);
+    else
+      hipLaunchKernelGGL((k_bnd_fill<5, double>), grid, bl, 0, c->stream, c->bnd.task, c->bnd.n,
+                         static_cast<double*>(c->Q[qcur]), c->rc, 1, ibx);
+  } else {
+    if (c->nv == 9)
+      hipLaunchKernelGGL((k_bnd_fill<9, float>), grid, bl, 0, c->stream, c->bnd.task, c->bnd.n,
+                         static_cast<float*>(c->Q[qcur]), c->rc, 1, ibx);
+    else
+      hipLaunchKernelGGL((k_bnd_fill<5, float>), grid, bl, 0, c->stream, c->bnd.task, c->bnd.n,
+                         static_cast<float*>(c->Q[qcur]), c->rc, 1, ibx);
+  }
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
 
 namespace {
 
@@ -136,6 +167,18 @@ int validate(const amun_cfg& c, int* nv) {
     return fail(600, "nghosts too small for the reconstruction stencil (WENO5, MP7 and PPM "
                      "need 4, GP and MP5 3, TVD, WENO3 and LimO3 2)");
   if (c.ncells < 3) return fail(500, "ncells too small for the stencil");
+  for (int n = 0; n < 3; n++) {
+    const int a = c.bnd_type[2 * n], b = c.bnd_type[2 * n + 1];
+    if (a < AMUN_BND_PERIODIC || a > AMUN_BND_REFLECTIVE || b < AMUN_BND_PERIODIC ||
+        b > AMUN_BND_REFLECTIVE)
+      return fail(600, "boundary type not supported on the device path (periodic, open, "
+                       "outflow, reflective)");
+    if ((a == AMUN_BND_PERIODIC) != (b == AMUN_BND_PERIODIC))
+      return fail(600, "an axis is periodic on both sides or on neither");
+  }
+  if ((c.bnd_type[0] || c.bnd_type[2] || (c.ndims == 3 && c.bnd_type[4])) &&
+      c.eqsys == AMUN_EQSYS_SRMHD)
+    return fail(600, "non-periodic sides are built for the HD and MHD systems");
   if (!(c.gamma > 1.0) || !(c.cfl > 0.0)) return fail(500, "gamma must exceed 1 and cfl be positive");
   return 0;
 }
@@ -222,6 +265,7 @@ RunConst make_rc(const amun_cfg& cfg) {
   // the LimO3 scheme raises the module's eps to at least 1e-12
   // (interpolations.F90:248); every user of eps then sees the raised value
   rc.eps = cfg.reconstruction == AMUN_REC_LIMO3 ? std::fmax(1.0e-12, cfg.eps) : cfg.eps;
+  for (int n = 0; n < 6; n++) rc.bnd[n] = cfg.bnd_type[n];
   rc.rad = cfg.limo3_rad;
   rc.ppm_const = cfg.ppm_const;
   rc.decay = std::exp(-cfg.alpha * cfl);
@@ -349,6 +393,14 @@ int amun_dev_init_batch(const amun_cfg* cfg, int32_t nruns, const amun_grid* gri
     if (nd == 3) R.dxmin = std::fmin(R.dxmin, adz);
   }
   rc.nblk = c->run_b0[nruns];
+  const bool bspec = cfg->bnd_type[0] || cfg->bnd_type[2] || (nd == 3 && cfg->bnd_type[4]);
+  if (bspec && nruns != 1) {
+    delete c;
+    return fail(600, "non-periodic sides need a single-run context");
+  }
+  // virtual ghost blocks across non-periodic sides: key (source, sides)
+  std::map<long long, int> vmap;
+  std::vector<int4> btask;
   std::vector<int32_t> nbr((size_t)rc.nblk * 27, -1), brun(rc.nblk);
   c->coords.resize((size_t)rc.nblk * 3);
   for (int g = 0; g < nruns; g++) {
@@ -366,14 +418,44 @@ int amun_dev_init_batch(const amun_cfg* cfg, int32_t nruns, const amun_grid* gri
             for (int dj = -1; dj <= 1; dj++)
               for (int di = -1; di <= 1; di++) {
                 if (nd == 2 && dk != 0) continue;
-                const int nx = (bx + di + ir) % ir, ny = (by + dj + jr) % jr;
-                const int nz = (bz + dk + kr) % kr;
-                nbr[(size_t)b * 27 + (dk + 1) * 9 + (dj + 1) * 3 + (di + 1)] =
-                    b0 + (nz * jr + ny) * ir + nx;
+                const int pos[3] = {bx + di, by + dj, bz + dk}, cnt[3] = {ir, jr, kr};
+                const int off[3] = {di, dj, dk};
+                int sides = 0, src[3];
+                for (int a = 0; a < 3; a++) {
+                  src[a] = pos[a];
+                  if (bspec && cfg->bnd_type[2 * a] && (pos[a] < 0 || pos[a] >= cnt[a])) {
+                    sides |= (off[a] < 0 ? 1 : 2) << (2 * a);
+                    src[a] = pos[a] - off[a];
+                  }
+                  src[a] = (src[a] + cnt[a]) % cnt[a];
+                }
+                int id = b0 + (src[2] * jr + src[1]) * ir + src[0];
+                if (sides) {
+                  const long long key = (long long)id * 64 + sides;
+                  auto it = vmap.find(key);
+                  if (it == vmap.end()) {
+                    const int v = rc.nblk + (int)btask.size();
+                    vmap[key] = v;
+                    btask.push_back(make_int4(v, id, sides, 0));
+                    id = v;
+                  } else {
+                    id = it->second;
+                  }
+                }
+                nbr[(size_t)b * 27 + (dk + 1) * 9 + (dj + 1) * 3 + (di + 1)] = id;
               }
         }
   }
-  return init_device(c, nbr, brun, hrs, rc.nblk, out);
+  const int nqblk = rc.nblk + (int)btask.size();
+  int r2 = init_device(c, nbr, brun, hrs, nqblk, out);
+  if (r2 || btask.empty()) return r2;
+  amun_ctx* cc = *out;
+  cc->bnd.on = true;
+  cc->bnd.n = (int)btask.size();
+  HIP_TRY(hipMalloc((void**)&cc->bnd.task, btask.size() * sizeof(int4)));
+  HIP_TRY(hipMemcpy(cc->bnd.task, btask.data(), btask.size() * sizeof(int4),
+                    hipMemcpyHostToDevice));
+  return 0;
 }
 
 // Device buffers, tables and kernel selection of a context whose host-side
@@ -480,6 +562,8 @@ int amun_dev_init_amr(const amun_cfg* cfg, int32_t nblk, const int32_t* meta, am
   if (r) return r;
   if (cfg->eqsys == AMUN_EQSYS_SRMHD)
     return fail(600, "multi-level meshes are not supported for the relativistic system");
+  if (cfg->bnd_type[0] || cfg->bnd_type[2] || (cfg->ndims == 3 && cfg->bnd_type[4]))
+    return fail(600, "multi-level meshes are built for periodic domains");
   const int nd = cfg->ndims, nc = cfg->ncells, ng = cfg->nghosts;
   if (ng % 2 || 2 * ng > nc)
     return fail(600, "multi-level meshes need an even nghosts of at most ncells / 2");
@@ -763,6 +847,7 @@ void amun_dev_finalize(amun_ctx* c) {
   hipFree(c->trace);
   hipFree(c->scratch);
   hipFree(c->amr.blev);
+  hipFree(c->bnd.task);
   hipFree(c->amr.rtask);
   hipFree(c->amr.ptask);
   for (int d = 0; d < 3; d++) hipFree(c->amr.ftask[d]);
@@ -846,7 +931,7 @@ static int download_blocks_amr(amun_ctx* c, double* u, double* q) {
   const int ng = c->cfg.nghosts, nc = rc.nc, nv = c->nv, nd = rc.nd;
   const int im = nc + 2 * ng, km = (nd == 3) ? im : 1, kg = (nd == 3) ? ng : 0;
   const size_t per = (size_t)im * im * km;
-  int r = amr_fill(c, c->qcur, (int)(c->nsteps & 1));
+  int r = c->amr.on ? amr_fill(c, c->qcur, (int)(c->nsteps & 1)) : bnd_fill(c, c->qcur);
   if (r) return r;
   const size_t n = (size_t)rc.nblk * per * nv;
   std::vector<double> qh(n);
@@ -971,7 +1056,8 @@ int amun_dev_download_global(amun_ctx* c, double* u, double* q) {
 
 int amun_dev_download_blocks(amun_ctx* c, double* u, double* q) {
   if (!c || !u) return fail(500, "null argument");
-  if (c->amr.on) return download_blocks_amr(c, u, q);
+  // ghost layers from virtual blocks: u ghosts as prim2cons of q's
+  if (c->amr.on || c->bnd.on) return download_blocks_amr(c, u, q);
   Dispatch& d = dispatch_of(c);
   const int ng = c->cfg.nghosts, nc = c->rc.nc;
   const int im = nc + 2 * ng, km = (c->rc.nd == 3) ? im : 1;
@@ -1102,6 +1188,9 @@ static int enqueue_step(amun_ctx* c, double dtnext, int slot, int qcur, hipEvent
   for (size_t s = 0; s < stages.size(); s++) {
     if (c->amr.on) {
       int r = amr_fill(c, qcur, slot);
+      if (r) return r;
+    } else if (c->bnd.on) {
+      int r = bnd_fill(c, qcur);
       if (r) return r;
     }
     SweepArgs a;
@@ -1575,6 +1664,7 @@ void* amun_dev_stream(amun_ctx* c) { return c ? (void*)c->stream : nullptr; }
 static int tape_check(amun_ctx* c) {
   if (c->rc.nruns != 1) return fail(600, "gradients need a single-run context");
   if (c->amr.on) return fail(600, "gradients are not supported on multi-level meshes");
+  if (c->bnd.on) return fail(600, "gradients are built for periodic domains");
   if (!c->disp.has_adj)
     return fail(600, "gradients need FP64, TVD reconstruction and the HD or MHD system");
   return 0;

@@ -97,6 +97,7 @@ struct RunConst {
   double cgp[5];                    // Gaussian-process weights (prepare_gp)
   double kappa, kbeta;              // MP limiter of the GP reconstruction
   double rad, ppm_const;            // LimO3 radius, PPM curvature ratio
+  int bnd[6];                       // AMUN_BND_* per domain side (xl, xu, ...)
   SrConst sc;                       // relativistic primitive solver
 };
 

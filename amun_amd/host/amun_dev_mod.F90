@@ -37,6 +37,8 @@ module amun_dev_mod
   integer(c_int32_t), parameter :: AMUN_PREC_FP64 = 0, AMUN_PREC_MIXED = 1
   integer(c_int32_t), parameter :: AMUN_C2P_1DW = 0, AMUN_C2P_2DWV = 1
   integer(c_int32_t), parameter :: AMUN_C2P_2DWU = 2
+  integer(c_int32_t), parameter :: AMUN_BND_PERIODIC = 0, AMUN_BND_OPEN = 1
+  integer(c_int32_t), parameter :: AMUN_BND_OUTFLOW = 2, AMUN_BND_REFLECTIVE = 3
 
   type, bind(C) :: amun_cfg
     integer(c_int32_t) :: ndims, eqsys, eos, nv, ncells, nghosts
@@ -52,6 +54,7 @@ module amun_dev_mod
     integer(c_int32_t) :: ngp, stages
     integer(c_int32_t) :: prol_limiter, primitive_solver
     real(c_double)     :: limo3_rad, ppm_const
+    integer(c_int32_t) :: bnd_type(6)
   end type amun_cfg
 
   interface

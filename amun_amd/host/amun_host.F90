@@ -119,6 +119,12 @@ program amun_host
   case default
     cfg%prol_limiter = AMUN_LIM_ZERO
   end select
+  cfg%bnd_type(1) = bnd_of(get_str("xlbndry", "periodic"))
+  cfg%bnd_type(2) = bnd_of(get_str("xubndry", "periodic"))
+  cfg%bnd_type(3) = bnd_of(get_str("ylbndry", "periodic"))
+  cfg%bnd_type(4) = bnd_of(get_str("yubndry", "periodic"))
+  cfg%bnd_type(5) = bnd_of(get_str("zlbndry", "periodic"))
+  cfg%bnd_type(6) = bnd_of(get_str("zubndry", "periodic"))
   str = get_str("primitive_solver", "1Dw")
   select case(trim(str))
   case("1Dw", "1dw", "1DW", "1D(w)", "1D(W)")
@@ -341,6 +347,24 @@ contains
       if (trim(pnames(n)) == name) r = pvalues(n)
     end do
   end function get_str
+
+! the boundary type names of boundaries.F90:170-260
+  function bnd_of(name) result(t)
+    character(len=*), intent(in) :: name
+    integer(c_int32_t)           :: t
+    select case(trim(name))
+    case("periodic")
+      t = AMUN_BND_PERIODIC
+    case("open")
+      t = AMUN_BND_OPEN
+    case("outflow", "out")
+      t = AMUN_BND_OUTFLOW
+    case("reflective", "reflecting", "reflect")
+      t = AMUN_BND_REFLECTIVE
+    case default
+      t = -1
+    end select
+  end function bnd_of
 
   integer function get_int(name, def)
     character(len=*), intent(in) :: name

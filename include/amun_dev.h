@@ -40,6 +40,11 @@ extern "C" {
 #define AMUN_EOS_ADI       1
 
 /* reconstructions: interpolations.F90:237-376 (parameter "reconstruction") */
+#define AMUN_BND_PERIODIC  0
+#define AMUN_BND_OPEN      1   /* zero gradient                                */
+#define AMUN_BND_OUTFLOW   2   /* zero gradient, normal velocity only outwards */
+#define AMUN_BND_REFLECTIVE 3  /* mirror, normal velocity and field negated    */
+
 #define AMUN_C2P_1DW       0   /* nr_iterate_srmhd_adi_1dw, equations.F90:5665  */
 #define AMUN_C2P_2DWV      1   /* nr_iterate_srmhd_adi_2dwv, equations.F90:5819 */
 #define AMUN_C2P_2DWU      2   /* nr_iterate_srmhd_adi_2dwu, equations.F90:6020 */
@@ -128,6 +133,10 @@ typedef struct amun_cfg {
    * limiter (interpolations.F90:83, 106, 209, 212) */
   double  limo3_rad;
   double  ppm_const;
+  /* "xlbndry", "xubndry", "ylbndry", "yubndry", "zlbndry", "zubndry"
+   * (coordinates.F90, boundaries.F90:1089-1251, 5024-5637): AMUN_BND_*;
+   * an axis is periodic on both sides or on neither */
+  int32_t bnd_type[6];
 } amun_cfg;
 
 typedef struct amun_ctx amun_ctx;

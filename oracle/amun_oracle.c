@@ -2589,13 +2589,50 @@ static int nbr_block(const ogrid* g, int b, int di, int dj, int dk)
   return (bz * g->jr + by) * g->ir + bx;
 }
 
-/* boundary_variables for a single-level periodic mesh: every ghost cell of q
- * is the neighbour's interior value (face, edge and corner copies), then
- * update_ghost_cells converts the ghost q to u */
+/* block_boundary_specific (boundaries.F90:5024-5637) for the open, outflow
+ * and reflective types: the ghost layers of axis n on side s (0 low, 1 high)
+ * over the block's whole transverse range, ghosts included */
+static void block_boundary_specific(const ocfg* o, double* qb, int n, int s, int type)
+{
+  const int nv = o->nv, ng = o->ng;
+  const int vn = n == 0 ? o->ivx : (n == 1 ? o->ivy : o->ivz);
+  const int bn = o->mhd ? (n == 0 ? o->ibx : (n == 1 ? o->iby : o->ibz)) : -1;
+  const int lo = ng, hi = ng + o->nc - 1;            /* ib, ie (0-based) */
+  const int m1 = n == 0 ? o->jm : o->im, m2 = n == 2 ? o->jm : o->km;
+  int a, b, c, p;
+  for (b = 0; b < m2; b++)
+    for (a = 0; a < m1; a++)
+      for (c = 1; c <= ng; c++) {
+        /* target ghost t and source cell r along axis n */
+        int t = s == 0 ? lo - c : hi + c, r;
+        double* qt;
+        const double* qr;
+        if (type == AMUN_BND_REFLECTIVE) r = s == 0 ? lo - 1 + c : hi + 1 - c;
+        else r = s == 0 ? lo : hi;
+#define AT(x) (n == 0 ? &qb[QI(o, 0, x, a, b)] : (n == 1 ? &qb[QI(o, 0, a, x, b)] : &qb[QI(o, 0, a, b, x)]))
+        qt = AT(t);
+        qr = AT(r);
+#undef AT
+        for (p = 0; p < nv; p++) qt[p] = qr[p];
+        if (type == AMUN_BND_OUTFLOW)
+          qt[vn] = s == 0 ? fmin_r(0.0, qr[vn]) : fmax_r(0.0, qr[vn]);
+        if (type == AMUN_BND_REFLECTIVE) {
+          qt[vn] = -qr[vn];
+          if (bn >= 0) qt[bn] = -qr[bn];
+        }
+      }
+}
+
+/* boundary_variables for a single-level mesh: every ghost cell of q is the
+ * neighbour's interior value (face, edge and corner copies) where the
+ * neighbour exists (periodic axes wrap), then boundaries_specific fills the
+ * ghost layers at the non-periodic domain sides (per block, x, y, z in
+ * turn), then update_ghost_cells converts the ghost q to u */
 static void boundary_variables(ogrid* g, double* u)
 {
   ocfg* o = &g->o;
   const size_t bs = bsize(o);
+  const int* bt = o->c.bnd_type;
   int b, i, j, k;
   for (b = 0; b < g->nblk; b++) {
     double* qb = g->q + b * bs;
@@ -2608,12 +2645,40 @@ static void boundary_variables(ogrid* g, double* u)
           int dk = (o->nd == 3) ? (gk < 0 ? -1 : (gk >= o->nc ? 1 : 0)) : 0;
           int nb, li, lj, lk;
           if (di == 0 && dj == 0 && dk == 0) continue;
+          {
+            /* no neighbour across a non-periodic domain side */
+            const int px = b % g->ir + di, py = (b / g->ir) % g->jr + dj;
+            const int pz = b / (g->ir * g->jr) + dk;
+            if ((bt[0] && (px < 0 || px >= g->ir)) || (bt[2] && (py < 0 || py >= g->jr)) ||
+                (o->nd == 3 && bt[4] && (pz < 0 || pz >= g->kr)))
+              continue;
+          }
           nb = nbr_block(g, b, di, dj, dk);
           li = gi - di * o->nc + o->ng;
           lj = gj - dj * o->nc + o->ng;
           lk = (o->nd == 3) ? gk - dk * o->nc + o->ng : 0;
           memcpy(&qb[QI(o, 0, i, j, k)], &g->q[nb * bs + QI(o, 0, li, lj, lk)],
                  o->nv * sizeof(double));
+          prim2cons(o, &qb[QI(o, 0, i, j, k)], &u[b * bs + QI(o, 0, i, j, k)]);
+        }
+  }
+  if (!(bt[0] || bt[2] || bt[4])) return;
+  for (b = 0; b < g->nblk; b++) {
+    const int pos[3] = {b % g->ir, (b / g->ir) % g->jr, b / (g->ir * g->jr)};
+    const int nbk[3] = {g->ir, g->jr, g->kr};
+    double* qb = g->q + b * bs;
+    int n;
+    for (n = 0; n < o->nd; n++) {
+      if (!bt[2 * n]) continue;
+      if (pos[n] == 0) block_boundary_specific(o, qb, n, 0, bt[2 * n]);
+      if (pos[n] == nbk[n] - 1) block_boundary_specific(o, qb, n, 1, bt[2 * n + 1]);
+    }
+    for (k = 0; k < o->km; k++)
+      for (j = 0; j < o->jm; j++)
+        for (i = 0; i < o->im; i++) {
+          int gi = i - o->ng, gj = j - o->ng, gk = (o->nd == 3) ? k - o->ng : 0;
+          if (gi >= 0 && gi < o->nc && gj >= 0 && gj < o->nc && gk >= 0 && (o->nd == 2 || gk < o->nc))
+            continue;
           prim2cons(o, &qb[QI(o, 0, i, j, k)], &u[b * bs + QI(o, 0, i, j, k)]);
         }
   }
@@ -3073,6 +3138,14 @@ int orc_create(const amun_cfg* c, void** handle)
       c->reconstruction != AMUN_REC_GP && c->reconstruction != AMUN_REC_WENO3 &&
       c->reconstruction != AMUN_REC_LIMO3 && c->reconstruction != AMUN_REC_PPM) r = 200;
   if (!r && (c->time_advance < AMUN_TA_EULER || c->time_advance > AMUN_TA_SSPRK53)) r = 400;
+  if (!r) {
+    int n;
+    for (n = 0; n < 3; n++) {
+      const int a = c->bnd_type[2 * n], b = c->bnd_type[2 * n + 1];
+      if (a < 0 || a > AMUN_BND_REFLECTIVE || b < 0 || b > AMUN_BND_REFLECTIVE) r = 600;
+      if ((a == AMUN_BND_PERIODIC) != (b == AMUN_BND_PERIODIC)) r = 600;
+    }
+  }
   if (r) { free(g); return r; }
   g->ir = c->xblocks; g->jr = c->yblocks; g->kr = (c->ndims == 3) ? c->zblocks : 1;
   g->nblk = g->ir * g->jr * g->kr;
@@ -3115,6 +3188,7 @@ int orc_create_amr(const amun_cfg* c, int nblk, const int32_t* meta, void** hand
   r = orc_create(c, (void**)&g);
   if (r) return r;
   if (c->nghosts % 2 || c->ncells < 2 * c->nghosts) { orc_destroy(g); return 600; }
+  if (c->bnd_type[0] || c->bnd_type[2] || c->bnd_type[4]) { orc_destroy(g); return 600; }
   free(g->u0); free(g->u1); free(g->q);
   g->nblk = nblk;
   g->amr = 1;

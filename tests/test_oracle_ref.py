@@ -236,3 +236,58 @@ def test_roe_hd_runs(nd, rec, ta):
     to = o.run(4)
     np.testing.assert_array_equal(to, tr)
     np.testing.assert_array_equal(o.get_state(), ur)
+
+
+# ------------------------------------------------- non-periodic boundaries
+def _bnd_params(nd, eq, bx, by, bz, rec="tvd", ng=2):
+    return Params(ndims=nd, equation_system=eq, reconstruction=rec, limiter="mc",
+                  riemann_solver="hlld" if eq == "mhd" else "hllc", time_advance="rk2",
+                  gamma=5.0 / 3.0 if eq == "mhd" else 1.4, cfl=0.4, ncells=8, nghosts=ng,
+                  xblocks=3, yblocks=2, zblocks=2, xmin=-0.5, xmax=0.5, ymin=-0.5, ymax=0.5,
+                  zmin=-0.5, zmax=0.5, xlbndry=bx, xubndry=bx, ylbndry=by, yubndry=by,
+                  zlbndry=bz, zubndry=bz)
+
+
+BND_CASES = [(2, "mhd", "reflective", "open", "periodic"), (2, "hd", "outflow", "reflective", "periodic"),
+             (3, "mhd", "open", "periodic", "reflective"), (2, "mhd", "outflow", "outflow", "periodic"),
+             (3, "hd", "reflective", "outflow", "open")]
+
+
+def _bnd_state(p, eq, nd):
+    if eq == "hd":
+        return problems.blast_hd_2d(p, seed=3)
+    return problems.orszag_tang_2d(p, seed=3) if nd == 2 else problems.turbulent_mhd_3d(p, seed=3)
+
+
+@needs_ref
+@pytest.mark.parametrize("nd,eq,bx,by,bz", BND_CASES)
+def test_boundary_types_ghosts_and_runs(nd, eq, bx, by, bz):
+    """boundaries_specific (open, outflow, reflective) on every side type and
+    axis mix: every block's ghost layers, edges and corners included, after
+    the initial boundary_variables, and 4 steps, bit for bit."""
+    import os
+    import tempfile
+    p = _bnd_params(nd, eq, bx, by, bz)
+    u0 = _bnd_state(p, eq, nd)
+    im = p.ncells + 2 * p.nghosts
+    km = im if nd == 3 else 1
+    with tempfile.TemporaryDirectory() as d:
+        u0.tofile(os.path.join(d, "in.bin"))
+        ol._ref_exec(p, {"harness_mode": "ghosts", "harness_input": "in.bin",
+                         "harness_output": "out", "harness_steps": 0}, d)
+        raw = np.fromfile(os.path.join(d, "out_q.bin"))
+    per = 3 + im * im * km * p.nv
+    raw = raw.reshape(-1, per)
+    o = ol.OracleRun(p)
+    o.set_state(u0)
+    qb = np.empty((raw.shape[0], km, im, im, p.nv))
+    ol.lib().orc_get_blocks(o.h, ol._dp(np.empty(raw.shape[0] * p.ncells ** nd * p.nv)), ol._dp(qb))
+    for row in raw:
+        x0, y0, z0 = (int(round(v)) for v in row[:3])
+        b = ((z0 // p.ncells if nd == 3 else 0) * p.yblocks + y0 // p.ncells) * p.xblocks + x0 // p.ncells
+        np.testing.assert_array_equal(qb[b].ravel(), row[3:])
+    ur, tr = ol.ref_run(p, u0, 4)
+    to = o.run(4)
+    np.testing.assert_array_equal(to, tr)
+    np.testing.assert_array_equal(o.get_state(), ur)
+
