@@ -36,7 +36,7 @@ def _run(cmd, cwd=None):
 
 
 HIP_UNITS = ("amun_dev", "inst_tvd", "inst_weno5z", "inst_weno5yc", "inst_weno5ns", "inst_mp5",
-             "inst_mp7", "inst_gp", "inst_weno3", "inst_limo3", "inst_ppm")
+             "inst_mp7", "inst_gp", "inst_weno3", "inst_limo3", "inst_ppm", "inst_crweno")
 
 
 def build_hip():

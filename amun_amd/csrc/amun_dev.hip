@@ -115,6 +115,8 @@ bool pick(Dispatch& d, const amun_cfg& c) {
   case AMUN_REC_WENO3: return pick_weno3(d, c);
   case AMUN_REC_LIMO3: return pick_limo3(d, c);
   case AMUN_REC_PPM: return pick_ppm(d, c);
+  case AMUN_REC_CRWENO5Z: case AMUN_REC_CRWENO5YC: case AMUN_REC_CRWENO5NS:
+    return pick_crweno(d, c);
   default: return false;
   }
 }
@@ -128,7 +130,8 @@ int validate(const amun_cfg& c, int* nv) {
   switch (c.reconstruction) {
   case AMUN_REC_TVD: case AMUN_REC_WENO5Z: case AMUN_REC_WENO5YC: case AMUN_REC_WENO5NS:
   case AMUN_REC_MP5: case AMUN_REC_MP7: case AMUN_REC_GP: case AMUN_REC_WENO3:
-  case AMUN_REC_LIMO3: case AMUN_REC_PPM: break;
+  case AMUN_REC_LIMO3: case AMUN_REC_PPM: case AMUN_REC_CRWENO5Z: case AMUN_REC_CRWENO5YC:
+  case AMUN_REC_CRWENO5NS: break;
   default: return fail(200, "unsupported reconstruction");
   }
   if (c.reconstruction == AMUN_REC_GP && c.ngp != 5)
@@ -163,6 +166,10 @@ int validate(const amun_cfg& c, int* nv) {
   case AMUN_REC_PPM: need = 4; break;
   case AMUN_REC_GP: case AMUN_REC_MP5: need = 3; break;
   }
+  if (c.ncells + 2 * c.nghosts > 96)
+    return fail(600, "ncells + 2 nghosts above 96");
+  if (c.nghosts > c.ncells)
+    return fail(600, "nghosts above ncells (ghost reads reach one neighbour block only)");
   if (c.nghosts < need)
     return fail(600, "nghosts too small for the reconstruction stencil (WENO5, MP7 and PPM "
                      "need 4, GP and MP5 3, TVD, WENO3 and LimO3 2)");
@@ -266,6 +273,7 @@ RunConst make_rc(const amun_cfg& cfg) {
   // (interpolations.F90:248); every user of eps then sees the raised value
   rc.eps = cfg.reconstruction == AMUN_REC_LIMO3 ? std::fmax(1.0e-12, cfg.eps) : cfg.eps;
   for (int n = 0; n < 6; n++) rc.bnd[n] = cfg.bnd_type[n];
+  rc.ng = cfg.nghosts;
   rc.rad = cfg.limo3_rad;
   rc.ppm_const = cfg.ppm_const;
   rc.decay = std::exp(-cfg.alpha * cfl);
@@ -495,6 +503,14 @@ static int init_device(amun_ctx* c, const std::vector<int32_t>& nbr,
   ALLOC(c->rs, (size_t)nruns * sizeof(RunState));
   ALLOC(c->trace, (size_t)nruns * c->tsteps * sizeof(double));
   ALLOC(c->scratch, (size_t)std::max(4096 * 8, 2 * nruns) * sizeof(double));
+  if (cfg->reconstruction == AMUN_REC_CRWENO5Z || cfg->reconstruction == AMUN_REC_CRWENO5YC ||
+      cfg->reconstruction == AMUN_REC_CRWENO5NS) {
+    // face states of one direction: [block][var][cell -1 .. nc][pencil]
+    const size_t npen = rc.nd == 3 ? (size_t)rc.nc * rc.nc : (size_t)rc.nc;
+    const size_t fsz = (size_t)rc.nblk * nv * (rc.nc + 2) * npen * c->qsize;
+    ALLOC(c->rc.fsl, fsz);
+    ALLOC(c->rc.fsr, fsz);
+  }
 #undef ALLOC
   e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
@@ -564,6 +580,9 @@ int amun_dev_init_amr(const amun_cfg* cfg, int32_t nblk, const int32_t* meta, am
     return fail(600, "multi-level meshes are not supported for the relativistic system");
   if (cfg->bnd_type[0] || cfg->bnd_type[2] || (cfg->ndims == 3 && cfg->bnd_type[4]))
     return fail(600, "multi-level meshes are built for periodic domains");
+  if (cfg->reconstruction == AMUN_REC_CRWENO5Z || cfg->reconstruction == AMUN_REC_CRWENO5YC ||
+      cfg->reconstruction == AMUN_REC_CRWENO5NS)
+    return fail(600, "the compact schemes are built for single-level meshes");
   const int nd = cfg->ndims, nc = cfg->ncells, ng = cfg->nghosts;
   if (ng % 2 || 2 * ng > nc)
     return fail(600, "multi-level meshes need an even nghosts of at most ncells / 2");
@@ -846,6 +865,8 @@ void amun_dev_finalize(amun_ctx* c) {
   hipFree(c->rs);
   hipFree(c->trace);
   hipFree(c->scratch);
+  hipFree(c->rc.fsl);
+  hipFree(c->rc.fsr);
   hipFree(c->amr.blev);
   hipFree(c->bnd.task);
   hipFree(c->amr.rtask);
@@ -1219,6 +1240,11 @@ static int enqueue_step(amun_ctx* c, double dtnext, int slot, int qcur, hipEvent
     for (int dir = 0; dir < rc.nd; dir++) {
       const int wg = c->wg[dir];
       a.first = (s == 0 && dir == 0);
+      if (d.compact[dir]) {
+        const long long nt = (long long)rc.nblk * (rc.nd == 3 ? rc.nc * rc.nc : rc.nc) * c->nv;
+        hipLaunchKernelGGL(d.compact[dir], dim3((unsigned)((nt + 63) / 64)), dim3(64), 0,
+                           c->stream, (const void*)c->Q[qcur], c->nbr, rc);
+      }
       a.swz = c->swz[dir];
       const dim3 grid((unsigned)((nthreads + wg - 1) / wg)), block(wg);
       hipLaunchKernelGGL(d.sweep[dir], grid, block, sweep_lds(c, dir, wg), c->stream, a, rc);
@@ -2094,7 +2120,9 @@ int amun_dev_unit_reconstruct(const amun_cfg* cfg, int32_t n, int32_t npencil, d
   RunConst rc;
   int r = unit_setup(cfg, d, rc);
   if (r) return r;
-  if (!d.unit_recon) return fail(600, "the TVD reconstruction has no pencil form");
+  if (!d.unit_recon)
+    return fail(600, "TVD (slopes coupled across directions) and the compact schemes (whole "
+                     "block pencils) have no pencil form here");
   if (n < 8 || n > 4096 || npencil < 1 || !fp || !fl || !fr || !(h > 0.0))
     return fail(500, "bad argument");
   const size_t m = (size_t)n * npencil;

@@ -98,6 +98,9 @@ struct RunConst {
   double kappa, kbeta;              // MP limiter of the GP reconstruction
   double rad, ppm_const;            // LimO3 radius, PPM curvature ratio
   int bnd[6];                       // AMUN_BND_* per domain side (xl, xu, ...)
+  int ng;                           // ghost layers of the reference's pencils
+  void* fsl;                        // compact schemes: face states of the
+  void* fsr;                        //   current sweep direction
   SrConst sc;                       // relativistic primitive solver
 };
 
@@ -185,6 +188,8 @@ __device__ __forceinline__ constexpr int rot(int p) {
 }
 
 enum {
+  REC_CRWENO5Z = AMUN_REC_CRWENO5Z, REC_CRWENO5YC = AMUN_REC_CRWENO5YC,
+  REC_CRWENO5NS = AMUN_REC_CRWENO5NS,
   REC_TVD = AMUN_REC_TVD, REC_WENO3 = AMUN_REC_WENO3, REC_LIMO3 = AMUN_REC_LIMO3,
   REC_PPM = AMUN_REC_PPM, REC_WENO5Z = AMUN_REC_WENO5Z, REC_WENO5YC = AMUN_REC_WENO5YC,
   REC_WENO5NS = AMUN_REC_WENO5NS, REC_MP5 = AMUN_REC_MP5, REC_MP7 = AMUN_REC_MP7,
@@ -423,6 +428,193 @@ struct Recon<P, REC_PPM, ND, DIR, T> : Recon<P, REC_MP7, ND, DIR, T> {
     for (int v = 0; v < P::nv; v++) ppm_cell<T>(this->w[v], pc, flc[v], frc[v]);
   }
 };
+
+// compact WENO (CRWENO5-Z/-YC/-NS): the face states of the whole pencil come
+// from a tridiagonal solve (k_compact, run before the sweep of each
+// direction into rc.fsl / rc.fsr, [block][var][cell -1 .. nc][pencil])
+constexpr bool is_compact(int rec) {
+  return rec == REC_CRWENO5Z || rec == REC_CRWENO5YC || rec == REC_CRWENO5NS;
+}
+
+template <class P, int REC, int ND, int DIR, typename T>
+struct ReconCompact {
+  static constexpr int R = 0;
+  static constexpr int H = 2;
+  __device__ __forceinline__ void load(int, const T*) {}
+  __device__ __forceinline__ void shift() {}
+  __device__ __forceinline__ void cell(const RunConst& rc, T* flc, T* frc, const T*,
+                                       const int*, int b, int i, int j, int k, DevState*,
+                                       double) {
+    const int nc = rc.nc, npen = ND == 3 ? nc * nc : nc;
+    const int c = DIR == 0 ? i : (DIR == 1 ? j : k);
+    const int t1 = DIR == 0 ? j : i, t2 = DIR == 2 ? j : k;
+    const int p = t1 + (ND == 3 ? nc * t2 : 0);
+    const T* L = static_cast<const T*>(rc.fsl);
+    const T* Rr = static_cast<const T*>(rc.fsr);
+#pragma unroll
+    for (int v = 0; v < P::nv; v++) {
+      const size_t o = ((size_t)(b * P::nv + v) * (nc + 2) + (c + 1)) * npen + p;
+      flc[v] = L[o];
+      frc[v] = Rr[o];
+    }
+  }
+};
+template <class P, int ND, int DIR, typename T>
+struct Recon<P, REC_CRWENO5Z, ND, DIR, T> : ReconCompact<P, REC_CRWENO5Z, ND, DIR, T> {};
+template <class P, int ND, int DIR, typename T>
+struct Recon<P, REC_CRWENO5YC, ND, DIR, T> : ReconCompact<P, REC_CRWENO5YC, ND, DIR, T> {};
+template <class P, int ND, int DIR, typename T>
+struct Recon<P, REC_CRWENO5NS, ND, DIR, T> : ReconCompact<P, REC_CRWENO5NS, ND, DIR, T> {};
+
+// the longest pencil (ncells + 2 nghosts) of the compact schemes
+constexpr int kMaxPencil = 96;
+
+// reconstruct_crweno5z / _yc / _ns (interpolations.F90:2477-3646) and
+// tridiag (algebra.F90:1115-1174) on one pencil f[0..n) of the reference's
+// layout (ng ghosts each side): the nonlinear weights, the explicit WENO5
+// rows within ng cells of the ends, the compact rows between them, the
+// Thomas algorithm with the rows built on the fly.  Writes the states at the
+// faces of the cells -1 .. nc: ol[c + 1] upper, or[c + 1] lower.
+template <int REC, typename T>
+__device__ void crweno_pencil(int n, int ng, const T* f, T eps, T* ol, T* orr) {
+  const T c1 = T(13.0 / 12.0), c2 = T(0.25), cl = T(1.0 / 9.0), cc = T(5.0 / 9.0),
+          cr = T(1.0 / 3.0), dl = T(0.1), dc = T(0.6), dr = T(0.3), dq = T(0.5), xi = T(0.4);
+  const T a11 = T(2.0 / 6.0), a12 = T(-7.0 / 6.0), a13 = T(11.0 / 6.0);
+  const T a21 = T(-1.0 / 6.0), a22 = T(5.0 / 6.0), a23 = T(2.0 / 6.0);
+  const T a31 = T(2.0 / 6.0), a32 = T(5.0 / 6.0), a33 = T(-1.0 / 6.0);
+  auto dfp = [&](int i) { return i < n - 1 ? f[i + 1] - f[i] : f[n - 1] - f[n - 2]; };
+  auto dfm = [&](int i) { return i > 0 ? f[i] - f[i - 1] : f[1] - f[0]; };
+  auto df2 = [&](int i) {
+    const T d = dfp(i) - dfm(i);
+    return REC == REC_CRWENO5NS ? T(0.5) * tabs(d) : c1 * (d * d);
+  };
+  // weights (al, ac, ar) of cell i for the left (s = 0) or right state
+  auto weights = [&](int i, int s, T& al, T& ac, T& ar) {
+    if constexpr (REC == REC_CRWENO5NS) {
+      const T df = s == 0 ? tabs(dfp(i)) : tabs(dfm(i)), lq = xi * df;
+      const T bl = s == 0 ? df2(i - 1) + xi * tabs(T(2) * dfm(i) - dfm(i - 1)) : df2(i - 1) + lq;
+      const T bc = df2(i) + lq;
+      const T br = s == 0 ? df2(i + 1) + lq : df2(i + 1) + xi * tabs(T(2) * dfp(i) - dfp(i + 1));
+      const T l3 = (df * df) * df, q3 = l3 / (T(1) + l3);
+      const T zt = T(0.5) * ((bl - br) * (bl - br) + q3 * q3);
+      al = T(1) + zt / ((bl + eps) * (bl + eps));
+      ac = T(1) + zt / ((bc + eps) * (bc + eps));
+      ar = T(1) + zt / ((br + eps) * (br + eps));
+    } else {
+      const T t1 = T(3) * dfm(i) - dfm(i - 1), t2 = dfp(i) + dfm(i), t3 = T(3) * dfp(i) - dfp(i + 1);
+      const T bl = df2(i - 1) + c2 * (t1 * t1), bc = df2(i) + c2 * (t2 * t2);
+      const T br = df2(i + 1) + c2 * (t3 * t3);
+      T tt;
+      if constexpr (REC == REC_CRWENO5YC) {
+        const int im2 = i - 2 > 0 ? i - 2 : 0, ip2 = i + 2 < n - 1 ? i + 2 : n - 1;
+        const T d4 = (T(6) * f[i] + (f[im2] + f[ip2])) - T(4) * (f[i - 1] + f[i + 1]);
+        tt = d4 * d4;
+      } else {
+        tt = tabs(br - bl);
+      }
+      al = T(1) + tt / (bl + eps);
+      ac = T(1) + tt / (bc + eps);
+      ar = T(1) + tt / (br + eps);
+    }
+  };
+  // row i of system s: sub-diagonal a, diagonal b, super-diagonal c, rhs r
+  auto row = [&](int i, int s, T& a, T& b, T& c, T& r) {
+    if (i == 0) { a = T(0); b = T(1); c = T(0); r = s == 0 ? T(0.5) * (f[0] + f[1]) : f[0]; return; }
+    if (i == n - 1) {
+      a = T(0); b = T(1); c = T(0); r = s == 0 ? f[n - 1] : T(0.5) * (f[n - 2] + f[n - 1]);
+      return;
+    }
+    T al, ac, ar;
+    weights(i, s, al, ac, ar);
+    const bool expl = s == 0 ? (i <= ng - 1 || i >= n - ng - 1) : (i <= ng || i >= n - ng);
+    if (expl) {
+      const int im2 = i - 2 > 0 ? i - 2 : 0, ip2 = i + 2 < n - 1 ? i + 2 : n - 1;
+      T wl = s == 0 ? dl * al : dl * ar, wc = dc * ac, wr = s == 0 ? dr * ar : dr * al;
+      const T ww = (wl + wr) + wc;
+      wl = wl / ww; wr = wr / ww; wc = T(1) - (wl + wr);
+      T ql, qc, qr;
+      if (s == 0) {
+        ql = (a11 * f[im2] + a12 * f[i - 1]) + a13 * f[i];
+        qc = (a21 * f[i - 1] + a22 * f[i]) + a23 * f[i + 1];
+        qr = (a31 * f[i] + a32 * f[i + 1]) + a33 * f[ip2];
+      } else {
+        ql = (a11 * f[ip2] + a12 * f[i + 1]) + a13 * f[i];
+        qc = (a21 * f[i + 1] + a22 * f[i]) + a23 * f[i - 1];
+        qr = (a31 * f[i] + a32 * f[i - 1]) + a33 * f[im2];
+      }
+      a = T(0); b = T(1); c = T(0);
+      r = (wl * ql + wr * qr) + wc * qc;
+      return;
+    }
+    T wl = s == 0 ? cl * al : cl * ar, wc = cc * ac, wr = s == 0 ? cr * ar : cr * al;
+    const T ww = (wl + wr) + wc;
+    wl = wl / ww; wr = wr / ww; wc = T(1) - (wl + wr);
+    b = wl + T(2) * (wc + wr);
+    if (s == 0) {
+      a = T(2) * wl + wc; c = wr;
+      r = ((wl * f[i - 1] + (T(5) * (wl + wc) + wr) * f[i]) + (wc + T(5) * wr) * f[i + 1]) * dq;
+    } else {
+      a = wr; c = T(2) * wl + wc;
+      r = ((wl * f[i + 1] + (T(5) * (wl + wc) + wr) * f[i]) + (wc + T(5) * wr) * f[i - 1]) * dq;
+    }
+  };
+  T g[kMaxPencil], x[kMaxPencil];
+  for (int s = 0; s < 2; s++) {
+    T a, b, c, r, cprev;
+    row(0, s, a, b, c, r);
+    T t = b;
+    x[0] = r / t;
+    cprev = c;
+    int i = 1;
+    for (; i < n; i++) {
+      row(i, s, a, b, c, r);
+      g[i] = cprev / t;
+      t = b - a * g[i];
+      if (t == T(0)) break;
+      x[i] = (r - a * x[i - 1]) / t;
+      cprev = c;
+    }
+    if (i == n)
+      for (int m = n - 2; m >= 0; m--) x[m] = x[m] - g[m + 1] * x[m + 1];
+    // cells -1 .. nc of the block: pencil cells l = ng - 1 .. n - ng; the
+    // left system solves the upper-face state of cell l, the right one (as
+    // fr(1:n-1) = u(2:n)) the lower-face state of cell l at row l as well
+    for (int cidx = 0; cidx < n - 2 * ng + 2; cidx++) {
+      if (s == 0) ol[cidx] = x[ng - 1 + cidx];
+      else orr[cidx] = x[ng - 1 + cidx];
+    }
+  }
+}
+
+// the face states of one direction for the compact schemes: one thread per
+// (block, pencil, variable) gathers the pencil with ng ghosts each side
+template <class P, int REC, int ND, int DIR, typename T>
+__global__ __launch_bounds__(64) void k_compact(const void* Qv, const int* __restrict__ nbr,
+                                                RunConst rc) {
+  constexpr int NV = P::nv;
+  const int nc = rc.nc, ng = rc.ng, n = nc + 2 * ng, npen = ND == 3 ? nc * nc : nc;
+  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (long long)rc.nblk * npen * NV) return;
+  const int v = (int)(g % NV), p = (int)((g / NV) % npen), b = (int)(g / ((long long)NV * npen));
+  const int t1 = p % nc, t2 = ND == 3 ? p / nc : 0;
+  const T* Q = static_cast<const T*>(Qv);
+  T f[kMaxPencil], ol[kMaxPencil], orr[kMaxPencil];
+  for (int m = 0; m < n; m++) {
+    const int c = m - ng;       // the sweep's coordinates of pencil (t1, t2)
+    const int i = DIR == 0 ? c : t1;
+    const int j = DIR == 0 ? t1 : (DIR == 1 ? c : t2);
+    const int k = DIR == 2 ? c : t2;
+    f[m] = Q[var_index<NV>(rc, cell_index<sizeof(T)>(nbr, rc, b, i, j, k), v)];
+  }
+  crweno_pencil<REC, T>(n, ng, f, T(rc.eps), ol, orr);
+  T* L = static_cast<T*>(rc.fsl);
+  T* Rr = static_cast<T*>(rc.fsr);
+  for (int c = 0; c < nc + 2; c++) {
+    const size_t o = ((size_t)(b * NV + v) * (nc + 2) + c) * npen + p;
+    L[o] = ol[c];
+    Rr[o] = orr[c];
+  }
+}
 
 // TVD (interfaces_tvd, interpolations.F90:787-833): three-cell window along
 // DIR; density and pressure also need the transverse slopes for the joint
@@ -1136,6 +1328,7 @@ using namespace amun_dev;
 // kernel instantiations selected for one configuration
 struct Dispatch {
   SweepFn sweep[3] = {nullptr, nullptr, nullptr};
+  void (*compact[3])(const void*, const int*, RunConst) = {nullptr, nullptr, nullptr};
   void (*prim)(const double*, void*, RunConst, DevState*, RunState*, const int*, int, int,
                int) = nullptr;
   void (*integ)(const double*, RunConst, double*, int, int, const int*) = nullptr;
@@ -1169,6 +1362,11 @@ namespace amun_dev {
 
 template <class P, int REC, int RS, int ND, typename T>
 void fill(Dispatch& d) {
+  if constexpr (is_compact(REC)) {
+    d.compact[0] = k_compact<P, REC, ND, 0, T>;
+    d.compact[1] = k_compact<P, REC, ND, 1, T>;
+    if constexpr (ND == 3) d.compact[2] = k_compact<P, REC, ND, 2, T>;
+  }
   d.sweep[0] = k_sweep<P, REC, RS, ND, 0, T>;
   d.sweep[1] = k_sweep<P, REC, RS, ND, 1, T>;
   if constexpr (ND == 3) d.sweep[2] = k_sweep<P, REC, RS, ND, 2, T>;
@@ -1180,7 +1378,7 @@ void fill(Dispatch& d) {
   d.unit_riemann = k_unit_riemann<P, RS, T>;
   d.unit_c2p = k_unit_c2p<P>;
   // TVD reads the transverse slopes of the block: no pencil form
-  if constexpr (REC != REC_TVD) d.unit_recon = k_unit_recon<P, REC, T>;
+  if constexpr (REC != REC_TVD && !is_compact(REC)) d.unit_recon = k_unit_recon<P, REC, T>;
   d.unit_eqs = k_unit_eqs<P, T>;
   d.fineflux[0] = k_amr_fineflux<P, REC, RS, ND, 0, T>;
   d.fineflux[1] = k_amr_fineflux<P, REC, RS, ND, 1, T>;
@@ -1245,5 +1443,6 @@ bool pick_gp(Dispatch& d, const amun_cfg& c);
 bool pick_weno3(Dispatch& d, const amun_cfg& c);
 bool pick_limo3(Dispatch& d, const amun_cfg& c);
 bool pick_ppm(Dispatch& d, const amun_cfg& c);
+bool pick_crweno(Dispatch& d, const amun_cfg& c);
 
 }  // namespace amun_dev

@@ -56,6 +56,9 @@ extern "C" {
 #define AMUN_REC_WENO5Z    5
 #define AMUN_REC_WENO5YC   6   /* reconstruct_weno5yc, interpolations.F90:2107 */
 #define AMUN_REC_WENO5NS   7   /* reconstruct_weno5ns, interpolations.F90:2278 */
+#define AMUN_REC_CRWENO5Z  8   /* reconstruct_crweno5z, interpolations.F90:2477 */
+#define AMUN_REC_CRWENO5YC 9   /* reconstruct_crweno5yc, interpolations.F90:2870 */
+#define AMUN_REC_CRWENO5NS 10  /* reconstruct_crweno5ns, interpolations.F90:3266 */
 #define AMUN_REC_MP5       11  /* reconstruct_mp5, interpolations.F90:3675     */
 #define AMUN_REC_MP7       12  /* reconstruct_mp7, interpolations.F90:3799     */
 #define AMUN_REC_GP        20  /* Gaussian process, reconstruct_gp interpolations.F90:4561 */

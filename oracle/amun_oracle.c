@@ -745,6 +745,147 @@ static void reconstruct_ppm(const ocfg* o, int n, const double* f, double* fl, d
   fr[n - 1] = f[n - 1];
 }
 
+/* tridiag, algebra.F90:1115-1174: the Thomas algorithm for the system with
+ * sub-diagonal l, diagonal d, super-diagonal u; g is workspace.  A zero
+ * pivot stops it as the reference does (x left partly solved). */
+static void tridiag(int n, const double* l, const double* d, const double* u, const double* r,
+                    double* x, double* g)
+{
+  double t = d[0];
+  int i;
+  x[0] = r[0] / t;
+  for (i = 1; i < n; i++) {
+    g[i] = u[i - 1] / t;
+    t = d[i] - l[i] * g[i];
+    if (t == 0.0) return;
+    x[i] = (r[i] - l[i] * x[i - 1]) / t;
+  }
+  for (i = n - 2; i >= 0; i--) x[i] = x[i] - g[i + 1] * x[i + 1];
+}
+
+/* reconstruct_crweno5z / _crweno5yc / _crweno5ns (interpolations.F90:
+ * 2477-2834, 2870-3230, 3266-3646): compact WENO (Ghosh & Baeder 2012).
+ * The nonlinear weights of each cell make a tridiagonal system for the face
+ * values; the rows within ng cells of the pencil ends hold the explicit
+ * WENO5 values (the weights of the variant) and the end rows the averages.
+ * variant 0: Z, 1: YC, 2: NS (separate weights for the two states). */
+static void reconstruct_crweno(const ocfg* o, int n, const double* f, double* fl, double* fr,
+                               int variant)
+{
+  const double c1 = 13.0 / 12.0, c2 = 0.25, cl = 1.0 / 9.0, cc = 5.0 / 9.0, cr = 1.0 / 3.0;
+  const double dl = 0.1, dc = 0.6, dr = 0.3, dq = 0.5, xi = 0.4, eps = o->c.eps;
+  const double a11 = 2.0 / 6.0, a12 = -7.0 / 6.0, a13 = 11.0 / 6.0;
+  const double a21 = -1.0 / 6.0, a22 = 5.0 / 6.0, a23 = 2.0 / 6.0;
+  const double a31 = 2.0 / 6.0, a32 = 5.0 / 6.0, a33 = -1.0 / 6.0;
+  const int ng = o->ng;
+  double *dfm, *dfp, *df2, *al, *ac, *ar, *a, *b, *c, *r, *u, *g;
+  double bl, bc, br, tt, wl, wc, wr, ww, ql, qc, qr;
+  int i, s;
+  dfm = (double*)calloc(19 * (size_t)n, sizeof(double));
+  dfp = dfm + n; df2 = dfp + n;
+  al = df2 + n; ac = al + 2 * n; ar = ac + 2 * n;        /* [s * n + i] */
+  a = ar + 2 * n; b = a + 2 * n; c = b + 2 * n; r = c + 2 * n;
+  u = r + 2 * n; g = u + n;
+  for (i = 0; i < n - 1; i++) { dfp[i] = f[i + 1] - f[i]; dfm[i + 1] = dfp[i]; }
+  dfm[0] = dfp[0];
+  dfp[n - 1] = dfm[n - 1];
+  for (i = 0; i < n; i++)
+    df2[i] = variant == 2 ? 0.5 * fabs(dfp[i] - dfm[i])
+                          : c1 * ((dfp[i] - dfm[i]) * (dfp[i] - dfm[i]));
+  for (i = 1; i < n - 1; i++) {
+    if (variant == 2) {
+      double df, lq, l3, zt;
+      df = fabs(dfp[i]); lq = xi * df;
+      bl = df2[i - 1] + xi * fabs(2.0 * dfm[i] - dfm[i - 1]);
+      bc = df2[i] + lq; br = df2[i + 1] + lq;
+      l3 = (df * df) * df;
+      zt = 0.5 * ((bl - br) * (bl - br) + (l3 / (1.0 + l3)) * (l3 / (1.0 + l3)));
+      al[i] = 1.0 + zt / ((bl + eps) * (bl + eps));
+      ac[i] = 1.0 + zt / ((bc + eps) * (bc + eps));
+      ar[i] = 1.0 + zt / ((br + eps) * (br + eps));
+      df = fabs(dfm[i]); lq = xi * df;
+      bl = df2[i - 1] + lq; bc = df2[i] + lq;
+      br = df2[i + 1] + xi * fabs(2.0 * dfp[i] - dfp[i + 1]);
+      l3 = (df * df) * df;
+      zt = 0.5 * ((bl - br) * (bl - br) + (l3 / (1.0 + l3)) * (l3 / (1.0 + l3)));
+      al[n + i] = 1.0 + zt / ((bl + eps) * (bl + eps));
+      ac[n + i] = 1.0 + zt / ((bc + eps) * (bc + eps));
+      ar[n + i] = 1.0 + zt / ((br + eps) * (br + eps));
+    } else {
+      const double t1 = 3.0 * dfm[i] - dfm[i - 1], t2 = dfp[i] + dfm[i];
+      const double t3 = 3.0 * dfp[i] - dfp[i + 1];
+      bl = df2[i - 1] + c2 * (t1 * t1);
+      bc = df2[i] + c2 * (t2 * t2);
+      br = df2[i + 1] + c2 * (t3 * t3);
+      if (variant == 1) {
+        const int im2 = i - 2 > 0 ? i - 2 : 0, ip2 = i + 2 < n - 1 ? i + 2 : n - 1;
+        const double d4 = (6.0 * f[i] + (f[im2] + f[ip2])) - 4.0 * (f[i - 1] + f[i + 1]);
+        tt = d4 * d4;
+      } else {
+        tt = fabs(br - bl);
+      }
+      al[i] = 1.0 + tt / (bl + eps);
+      ac[i] = 1.0 + tt / (bc + eps);
+      ar[i] = 1.0 + tt / (br + eps);
+      al[n + i] = al[i]; ac[n + i] = ac[i]; ar[n + i] = ar[i];
+    }
+  }
+  /* compact rows i = ng .. n-ng+1 (1-based) */
+  for (i = ng - 1; i <= n - ng; i++) {
+    wl = cl * al[i]; wc = cc * ac[i]; wr = cr * ar[i];
+    ww = (wl + wr) + wc; wl = wl / ww; wr = wr / ww; wc = 1.0 - (wl + wr);
+    a[i] = 2.0 * wl + wc; b[i] = wl + 2.0 * (wc + wr); c[i] = wr;
+    r[i] = ((wl * f[i - 1] + (5.0 * (wl + wc) + wr) * f[i]) + (wc + 5.0 * wr) * f[i + 1]) * dq;
+    wl = cl * ar[n + i]; wc = cc * ac[n + i]; wr = cr * al[n + i];
+    ww = (wl + wr) + wc; wl = wl / ww; wr = wr / ww; wc = 1.0 - (wl + wr);
+    a[n + i] = wr; b[n + i] = wl + 2.0 * (wc + wr); c[n + i] = 2.0 * wl + wc;
+    r[n + i] = ((wl * f[i + 1] + (5.0 * (wl + wc) + wr) * f[i]) + (wc + 5.0 * wr) * f[i - 1]) * dq;
+  }
+  /* explicit WENO5 rows of the left states: 2..ng and n-ng..n-1 (1-based) */
+  for (s = 0; s < 2; s++) {
+    const int lo = s == 0 ? 1 : n - ng - 1, hi = s == 0 ? ng - 1 : n - 2;
+    for (i = lo; i <= hi; i++) {
+      const int im2 = i - 2 > 0 ? i - 2 : 0, ip2 = i + 2 < n - 1 ? i + 2 : n - 1;
+      wl = dl * al[i]; wc = dc * ac[i]; wr = dr * ar[i];
+      ww = (wl + wr) + wc; wl = wl / ww; wr = wr / ww; wc = 1.0 - (wl + wr);
+      ql = (a11 * f[im2] + a12 * f[i - 1]) + a13 * f[i];
+      qc = (a21 * f[i - 1] + a22 * f[i]) + a23 * f[i + 1];
+      qr = (a31 * f[i] + a32 * f[i + 1]) + a33 * f[ip2];
+      a[i] = 0.0; b[i] = 1.0; c[i] = 0.0;
+      r[i] = (wl * ql + wr * qr) + wc * qc;
+    }
+  }
+  a[0] = 0.0; b[0] = 1.0; c[0] = 0.0; r[0] = 0.5 * (f[0] + f[1]);
+  a[n - 1] = 0.0; b[n - 1] = 1.0; c[n - 1] = 0.0; r[n - 1] = f[n - 1];
+  /* explicit rows of the right states: 2..ng+1 and n-ng+1..n-1 (1-based) */
+  for (s = 0; s < 2; s++) {
+    const int lo = s == 0 ? 1 : n - ng, hi = s == 0 ? ng : n - 2;
+    for (i = lo; i <= hi; i++) {
+      const int im2 = i - 2 > 0 ? i - 2 : 0, ip2 = i + 2 < n - 1 ? i + 2 : n - 1;
+      wl = dl * ar[n + i]; wc = dc * ac[n + i]; wr = dr * al[n + i];
+      ww = (wl + wr) + wc; wl = wl / ww; wr = wr / ww; wc = 1.0 - (wl + wr);
+      ql = (a11 * f[ip2] + a12 * f[i + 1]) + a13 * f[i];
+      qc = (a21 * f[i + 1] + a22 * f[i]) + a23 * f[i - 1];
+      qr = (a31 * f[i] + a32 * f[i - 1]) + a33 * f[im2];
+      a[n + i] = 0.0; b[n + i] = 1.0; c[n + i] = 0.0;
+      r[n + i] = (wl * ql + wr * qr) + wc * qc;
+    }
+  }
+  a[n] = 0.0; b[n] = 1.0; c[n] = 0.0; r[n] = f[0];
+  a[2 * n - 1] = 0.0; b[2 * n - 1] = 1.0; c[2 * n - 1] = 0.0;
+  r[2 * n - 1] = 0.5 * (f[n - 2] + f[n - 1]);
+  tridiag(n, a, b, c, r, u, g);
+  for (i = 0; i < n; i++) fl[i] = u[i];
+  tridiag(n, a + n, b + n, c + n, r + n, u, g);
+  for (i = 0; i < n - 1; i++) fr[i] = u[i + 1];
+  i = n - 2;
+  fl[0] = 0.5 * (f[0] + f[1]);
+  fr[i] = 0.5 * (f[i] + f[n - 1]);
+  fl[n - 1] = f[n - 1];
+  fr[n - 1] = f[n - 1];
+  free(dfm);
+}
+
 /* reconstruct_gp, interpolations.F90:4561-4668 */
 static void reconstruct_gp(const ocfg* o, int n, const double* f, double* fl, double* fr,
                            double* w1, double* w2, double* w3)
@@ -771,6 +912,9 @@ static void recon_dir(const ocfg* o, int n, double h, const double* f, double* f
 {
   switch (o->c.reconstruction) {
   case AMUN_REC_WENO3: reconstruct_weno3(n, h, f, fl, fr); break;
+  case AMUN_REC_CRWENO5Z: reconstruct_crweno(o, n, f, fl, fr, 0); break;
+  case AMUN_REC_CRWENO5YC: reconstruct_crweno(o, n, f, fl, fr, 1); break;
+  case AMUN_REC_CRWENO5NS: reconstruct_crweno(o, n, f, fl, fr, 2); break;
   case AMUN_REC_LIMO3: reconstruct_limo3(o, n, h, f, fl, fr); break;
   case AMUN_REC_PPM: reconstruct_ppm(o, n, f, fl, fr, w1, w2); break;
   case AMUN_REC_GP: reconstruct_gp(o, n, f, fl, fr, w1, w2, w3); break;
@@ -3136,7 +3280,9 @@ int orc_create(const amun_cfg* c, void** handle)
       c->reconstruction != AMUN_REC_WENO5YC && c->reconstruction != AMUN_REC_WENO5NS &&
       c->reconstruction != AMUN_REC_MP5 && c->reconstruction != AMUN_REC_MP7 &&
       c->reconstruction != AMUN_REC_GP && c->reconstruction != AMUN_REC_WENO3 &&
-      c->reconstruction != AMUN_REC_LIMO3 && c->reconstruction != AMUN_REC_PPM) r = 200;
+      c->reconstruction != AMUN_REC_LIMO3 && c->reconstruction != AMUN_REC_PPM &&
+      c->reconstruction != AMUN_REC_CRWENO5Z && c->reconstruction != AMUN_REC_CRWENO5YC &&
+      c->reconstruction != AMUN_REC_CRWENO5NS) r = 200;
   if (!r && (c->time_advance < AMUN_TA_EULER || c->time_advance > AMUN_TA_SSPRK53)) r = 400;
   if (!r) {
     int n;

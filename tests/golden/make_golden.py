@@ -575,6 +575,42 @@ def main_roe_mhd():
     print(name, "dt", tr[-1, 0])
 
 
+CR_SCHEMES = ("crweno5z", "crweno5yc", "crweno5ns")
+
+
+def main_crweno():
+    """The compact WENO schemes: the reference's pencils at ng = 2, 3, 4 and
+    Orszag-Tang runs (noise on every field) (own generator)."""
+    rng = np.random.default_rng(20261213)
+    fp = np.concatenate([rng.normal(size=(6, 24)), np.cumsum(rng.normal(size=(4, 24)), axis=1),
+                         np.sin(np.linspace(0, 6, 24))[None, :] * np.ones((2, 1)),
+                         np.where(np.arange(24) < 12, 1.0, 0.1)[None, :] * np.ones((2, 1))])
+    out = {"fp": fp}
+    with open(os.path.join(HERE, "index.json")) as fh:
+        index = json.load(fh)
+    for rec in CR_SCHEMES:
+        for ng in (2, 3, 4):
+            pu = Params(ndims=3, equation_system="mhd", reconstruction=rec, limiter="mm",
+                        ncells=16, nghosts=ng, cfl=0.3)
+            out[f"fl_{rec}_{ng}"], out[f"fr_{rec}_{ng}"] = ol.ref_reconstruct(pu, fp)
+        p = Params(ndims=2, equation_system="mhd", reconstruction=rec, limiter="mm",
+                   riemann_solver="hlld", time_advance="rk3", gamma=5.0 / 3.0, cfl=0.3,
+                   ncells=8, nghosts=4, xblocks=4, yblocks=4)
+        u0 = ot_noisy(p)
+        nsteps = 8
+        u, tr = ol.ref_run(p, u0, nsteps)
+        name = f"{rec}_otnoisy2d_32_{nsteps}"
+        np.savez_compressed(os.path.join(HERE, name + ".npz"),
+                            params=np.array(p.to_params_in()), ndims=2, nsteps=nsteps,
+                            stride=3, trace=tr, sub=u[:, :, ::3, ::3], sha_u0=sha(u0),
+                            sha_u=sha(u))
+        index[name] = {"steps": nsteps, "sha_u": sha(u), "dt_last": float(tr[-1, 0])}
+        print(name, "dt", tr[-1, 0])
+    np.savez_compressed(os.path.join(HERE, "unit_schemes_cr.npz"), **out)
+    with open(os.path.join(HERE, "index.json"), "w") as fh:
+        json.dump(index, fh, indent=1)
+
+
 STAT_CASES = ("hd2d", "mhd2d", "mhd3d", "srmhd2d")
 
 
@@ -658,6 +694,9 @@ def main():
     if sys.argv[1:] == ["roe_mhd"]:
         main_roe_mhd()
         return
+    if sys.argv[1:] == ["crweno"]:
+        main_crweno()
+        return
     index = {}
     for name, p, builder, nsteps, stride in run_cases():
         u0 = builder(p) if builder else hd3_blast(p)
@@ -733,6 +772,7 @@ def main():
     main_c2p2d()
     main_sr_hllc()
     main_roe_mhd()
+    main_crweno()
 
 
 if __name__ == "__main__":

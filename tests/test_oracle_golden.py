@@ -399,3 +399,30 @@ def test_roe_mhd_run_bit_exact():
     tr = o.run(int(g["nsteps"]))
     np.testing.assert_array_equal(tr, g["trace"])
     assert sha(o.get_state()) == str(g["sha_u"])
+
+
+# ------------------------------------------------------- compact WENO
+@pytest.mark.parametrize("rec", ["crweno5z", "crweno5yc", "crweno5ns"])
+def test_crweno_pencils_and_runs(rec):
+    """reconstruct_crweno5z / _yc / _ns with tridiag on the reference's
+    pencils (ng = 2, 3, 4: the explicit end rows move with ng) and an 8-step
+    Orszag-Tang run (noise on every field), bit for bit."""
+    import sys
+    sys.path.insert(0, GOLD)
+    from make_golden import ot_noisy
+    g = load("unit_schemes_cr")
+    for ng in (2, 3, 4):
+        p = Params(ndims=3, equation_system="mhd", reconstruction=rec, limiter="mm", ncells=16,
+                   nghosts=ng, cfl=0.3)
+        fl, fr = ol.oracle_reconstruct(p, g["fp"])
+        np.testing.assert_array_equal(fl, g[f"fl_{rec}_{ng}"])
+        np.testing.assert_array_equal(fr, g[f"fr_{rec}_{ng}"])
+    r = load(f"{rec}_otnoisy2d_32_8")
+    p = params_of(r)
+    u0 = ot_noisy(p)
+    assert sha(u0) == str(r["sha_u0"])
+    o = ol.OracleRun(p)
+    o.set_state(u0)
+    tr = o.run(int(r["nsteps"]))
+    np.testing.assert_array_equal(tr, r["trace"])
+    assert sha(o.get_state()) == str(r["sha_u"])

@@ -612,3 +612,22 @@ def test_srmhd_primitive_solvers_round_trip(ps):
     scale = np.abs(q).max(axis=1, keepdims=True)
     err = np.abs(q2 - q) / scale
     assert np.median(err) <= 1e-13 and np.quantile(err, 0.99) <= 1e-8, err.max()
+
+
+@pytest.mark.parametrize("rec", ["crweno5z", "crweno5yc", "crweno5ns"])
+def test_crweno_exact_for_lines_and_tridiagonal(rec):
+    """Compact WENO on linear data: every stencil and the compact relation
+    (2 wl + wc) f_{i-1/2} + (wl + 2 (wc + wr)) f_{i+1/2} + wr f_{i+3/2} = ...
+    are exact for lines, so the face values are the exact ones (the end rows
+    use averages, also exact for lines)."""
+    n = 24
+    f = 0.3 + 0.7 * np.arange(n)[None, :] * np.ones((1, 1))
+    p = Params(ndims=3, equation_system="hd", reconstruction=rec, limiter="mm", ncells=16,
+               nghosts=4, cfl=0.3)
+    fl, fr = ol.oracle_reconstruct(p, f)
+    face = 0.3 + 0.7 * (np.arange(n) + 0.5)
+    # the explicit rows next to the pencil ends clamp their stencils
+    # (im2 = max(1, i - 2), ip2 = min(n, i + 2)): those two are not exact
+    sl = slice(2, n - 3)
+    np.testing.assert_allclose(fl[0, sl], face[sl], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(fr[0, sl], face[sl], rtol=0, atol=1e-12)
