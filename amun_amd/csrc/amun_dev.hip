@@ -116,6 +116,7 @@ bool pick(Dispatch& d, const amun_cfg& c) {
   case AMUN_REC_LIMO3: return pick_limo3(d, c);
   case AMUN_REC_PPM: return pick_ppm(d, c);
   case AMUN_REC_CRWENO5Z: case AMUN_REC_CRWENO5YC: case AMUN_REC_CRWENO5NS:
+  case AMUN_REC_CRMP5: case AMUN_REC_CRMP5LD: case AMUN_REC_CRMP7:
     return pick_crweno(d, c);
   default: return false;
   }
@@ -131,7 +132,8 @@ int validate(const amun_cfg& c, int* nv) {
   case AMUN_REC_TVD: case AMUN_REC_WENO5Z: case AMUN_REC_WENO5YC: case AMUN_REC_WENO5NS:
   case AMUN_REC_MP5: case AMUN_REC_MP7: case AMUN_REC_GP: case AMUN_REC_WENO3:
   case AMUN_REC_LIMO3: case AMUN_REC_PPM: case AMUN_REC_CRWENO5Z: case AMUN_REC_CRWENO5YC:
-  case AMUN_REC_CRWENO5NS: break;
+  case AMUN_REC_CRWENO5NS: case AMUN_REC_CRMP5: case AMUN_REC_CRMP5LD: case AMUN_REC_CRMP7:
+    break;
   default: return fail(200, "unsupported reconstruction");
   }
   if (c.reconstruction == AMUN_REC_GP && c.ngp != 5)
@@ -503,8 +505,7 @@ static int init_device(amun_ctx* c, const std::vector<int32_t>& nbr,
   ALLOC(c->rs, (size_t)nruns * sizeof(RunState));
   ALLOC(c->trace, (size_t)nruns * c->tsteps * sizeof(double));
   ALLOC(c->scratch, (size_t)std::max(4096 * 8, 2 * nruns) * sizeof(double));
-  if (cfg->reconstruction == AMUN_REC_CRWENO5Z || cfg->reconstruction == AMUN_REC_CRWENO5YC ||
-      cfg->reconstruction == AMUN_REC_CRWENO5NS) {
+  if (is_compact(cfg->reconstruction)) {
     // face states of one direction: [block][var][cell -1 .. nc][pencil]
     const size_t npen = rc.nd == 3 ? (size_t)rc.nc * rc.nc : (size_t)rc.nc;
     const size_t fsz = (size_t)rc.nblk * nv * (rc.nc + 2) * npen * c->qsize;
@@ -580,8 +581,7 @@ int amun_dev_init_amr(const amun_cfg* cfg, int32_t nblk, const int32_t* meta, am
     return fail(600, "multi-level meshes are not supported for the relativistic system");
   if (cfg->bnd_type[0] || cfg->bnd_type[2] || (cfg->ndims == 3 && cfg->bnd_type[4]))
     return fail(600, "multi-level meshes are built for periodic domains");
-  if (cfg->reconstruction == AMUN_REC_CRWENO5Z || cfg->reconstruction == AMUN_REC_CRWENO5YC ||
-      cfg->reconstruction == AMUN_REC_CRWENO5NS)
+  if (is_compact(cfg->reconstruction))
     return fail(600, "the compact schemes are built for single-level meshes");
   const int nd = cfg->ndims, nc = cfg->ncells, ng = cfg->nghosts;
   if (ng % 2 || 2 * ng > nc)

@@ -189,7 +189,8 @@ __device__ __forceinline__ constexpr int rot(int p) {
 
 enum {
   REC_CRWENO5Z = AMUN_REC_CRWENO5Z, REC_CRWENO5YC = AMUN_REC_CRWENO5YC,
-  REC_CRWENO5NS = AMUN_REC_CRWENO5NS,
+  REC_CRWENO5NS = AMUN_REC_CRWENO5NS, REC_CRMP5 = AMUN_REC_CRMP5,
+  REC_CRMP5LD = AMUN_REC_CRMP5LD, REC_CRMP7 = AMUN_REC_CRMP7,
   REC_TVD = AMUN_REC_TVD, REC_WENO3 = AMUN_REC_WENO3, REC_LIMO3 = AMUN_REC_LIMO3,
   REC_PPM = AMUN_REC_PPM, REC_WENO5Z = AMUN_REC_WENO5Z, REC_WENO5YC = AMUN_REC_WENO5YC,
   REC_WENO5NS = AMUN_REC_WENO5NS, REC_MP5 = AMUN_REC_MP5, REC_MP7 = AMUN_REC_MP7,
@@ -429,11 +430,13 @@ struct Recon<P, REC_PPM, ND, DIR, T> : Recon<P, REC_MP7, ND, DIR, T> {
   }
 };
 
-// compact WENO (CRWENO5-Z/-YC/-NS): the face states of the whole pencil come
+// compact WENO (CRWENO5-Z/-YC/-NS) and compact MP (CRMP5, CRMP5LD, CRMP7):
+// the face states of the whole pencil come
 // from a tridiagonal solve (k_compact, run before the sweep of each
 // direction into rc.fsl / rc.fsr, [block][var][cell -1 .. nc][pencil])
 constexpr bool is_compact(int rec) {
-  return rec == REC_CRWENO5Z || rec == REC_CRWENO5YC || rec == REC_CRWENO5NS;
+  return rec == REC_CRWENO5Z || rec == REC_CRWENO5YC || rec == REC_CRWENO5NS ||
+         rec == REC_CRMP5 || rec == REC_CRMP5LD || rec == REC_CRMP7;
 }
 
 template <class P, int REC, int ND, int DIR, typename T>
@@ -465,6 +468,12 @@ template <class P, int ND, int DIR, typename T>
 struct Recon<P, REC_CRWENO5YC, ND, DIR, T> : ReconCompact<P, REC_CRWENO5YC, ND, DIR, T> {};
 template <class P, int ND, int DIR, typename T>
 struct Recon<P, REC_CRWENO5NS, ND, DIR, T> : ReconCompact<P, REC_CRWENO5NS, ND, DIR, T> {};
+template <class P, int ND, int DIR, typename T>
+struct Recon<P, REC_CRMP5, ND, DIR, T> : ReconCompact<P, REC_CRMP5, ND, DIR, T> {};
+template <class P, int ND, int DIR, typename T>
+struct Recon<P, REC_CRMP5LD, ND, DIR, T> : ReconCompact<P, REC_CRMP5LD, ND, DIR, T> {};
+template <class P, int ND, int DIR, typename T>
+struct Recon<P, REC_CRMP7, ND, DIR, T> : ReconCompact<P, REC_CRMP7, ND, DIR, T> {};
 
 // the longest pencil (ncells + 2 nghosts) of the compact schemes
 constexpr int kMaxPencil = 96;
@@ -586,6 +595,82 @@ __device__ void crweno_pencil(int n, int ng, const T* f, T eps, T* ol, T* orr) {
   }
 }
 
+// reconstruct_crmp5 / _crmp5ld / _crmp7 (interpolations.F90:3930-4427) on
+// one pencil f[0..n), n ghosts ng >= 2 each side: a tridiagonal system of
+// constant rows (identity within ng cells of the ends, where the right-hand
+// side is the explicit 2-, 3-, 5- or 7-point interpolation) solved by the
+// Thomas algorithm, then the Suresh-Huynh limiter of each face value; the
+// right states solve the same system on the mirrored pencil.  Writes ol / or
+// as crweno_pencil does.
+template <int REC, typename T>
+__device__ void crmp_pencil(int n, int ng, const T* f, T kappa, T kbeta, T eps, T* ol,
+                            T* orr) {
+  constexpr int V = REC == REC_CRMP5 ? 0 : (REC == REC_CRMP5LD ? 1 : 2);
+  const T d0 = V == 0 ? T(0.3) : (V == 1 ? T(0.25) : T(2.0 / 7.0));
+  const T d1 = V == 2 ? T(4.0 / 7.0) : T(0.6);
+  const T d2 = V == 0 ? T(0.1) : (V == 1 ? T(0.15) : T(1.0 / 7.0));
+  const T e5[5] = {T(2.0 / 60.0), T(-13.0 / 60.0), T(47.0 / 60.0), T(27.0 / 60.0), T(-3.0 / 60.0)};
+  const T e3[3] = {T(-1.0 / 6.0), T(5.0 / 6.0), T(2.0 / 6.0)};
+  const T e7[7] = {T(-3.0 / 420.0), T(25.0 / 420.0), T(-101.0 / 420.0), T(319.0 / 420.0),
+                   T(214.0 / 420.0), T(-38.0 / 420.0), T(4.0 / 420.0)};
+  T g[kMaxPencil], x[kMaxPencil];
+  for (int s = 0; s < 2; s++) {
+    // pencil of system s: f itself (s = 0) or mirrored (s = 1)
+    auto F = [&](int m) { return s == 0 ? f[m] : f[n - 1 - m]; };
+    auto sum = [&](const T* c, int k, int i0) {
+      T a = c[0] * F(i0);
+      for (int m = 1; m < k; m++) a = a + c[m] * F(i0 + m);
+      return a;
+    };
+    auto rhs = [&](int i) -> T {
+      if (i == 0) return T(0.5) * F(0) + T(0.5) * F(1);
+      if (i == 1) return sum(e3, 3, 0);
+      if (i == n - 2) return sum(e3, 3, n - 3);
+      if (i == n - 1) return F(n - 1);
+      if (V < 2) {
+        if (i <= ng - 1 || i >= n - ng - 1) return sum(e5, 5, i - 2);
+        if (V == 0) return (T(1.0 / 30.0) * F(i - 1) + T(19.0 / 30.0) * F(i)) + T(10.0 / 30.0) * F(i + 1);
+        return ((T(3.0 / 120.0) * F(i - 1) + T(67.0 / 120.0) * F(i)) + T(49.0 / 120.0) * F(i + 1)) +
+               T(1.0 / 120.0) * F(i + 2);
+      }
+      if (i == 2 || i == n - 3) return sum(e5, 5, i - 2);
+      if (i <= ng - 1 || i >= n - ng - 1) return sum(e7, 7, i - 3);
+      return (((T(-1.0 / 420.0) * F(i - 2) + T(19.0 / 420.0) * F(i - 1)) + T(239.0 / 420.0) * F(i)) +
+              T(159.0 / 420.0) * F(i + 1)) + T(4.0 / 420.0) * F(i + 2);
+    };
+    // rows ng .. n - ng - 2 carry (d0, d1, d2), the others are identity
+    auto inner = [&](int i) { return i >= ng && i <= n - ng - 2; };
+    T t = T(1), cprev = T(0);
+    x[0] = rhs(0);
+    int i = 1;
+    for (; i < n; i++) {
+      const T a = inner(i) ? d0 : T(0), b = inner(i) ? d1 : T(1);
+      g[i] = cprev / t;
+      t = b - a * g[i];
+      if (t == T(0)) break;
+      x[i] = (rhs(i) - a * x[i - 1]) / t;
+      cprev = inner(i) ? d2 : T(0);
+    }
+    if (i == n)
+      for (int m = n - 2; m >= 0; m--) x[m] = x[m] - g[m + 1] * x[m + 1];
+    // the limited face values the block needs: system 0 at pencil cells
+    // l = ng - 1 .. n - ng (upper faces), system 1 at the mirrored rows
+    // n - 1 - l (lower faces, fr(1:n-1) = u(n-1:1:-1))
+    for (int cidx = 0; cidx < n - 2 * ng + 2; cidx++) {
+      const int q = s == 0 ? ng - 1 + cidx : n - ng - cidx;
+      T w[5];
+#pragma unroll
+      for (int m = 0; m < 5; m++) {
+        const int r = q - 2 + m;
+        w[m] = F(r < 0 ? 0 : (r > n - 1 ? n - 1 : r));
+      }
+      const T u = mp_limit<T>(w, x[q], kappa, kbeta, eps);
+      if (s == 0) ol[cidx] = u;
+      else orr[cidx] = u;
+    }
+  }
+}
+
 // the face states of one direction for the compact schemes: one thread per
 // (block, pencil, variable) gathers the pencil with ng ghosts each side
 template <class P, int REC, int ND, int DIR, typename T>
@@ -606,7 +691,10 @@ __global__ __launch_bounds__(64) void k_compact(const void* Qv, const int* __res
     const int k = DIR == 2 ? c : t2;
     f[m] = Q[var_index<NV>(rc, cell_index<sizeof(T)>(nbr, rc, b, i, j, k), v)];
   }
-  crweno_pencil<REC, T>(n, ng, f, T(rc.eps), ol, orr);
+  if constexpr (REC == REC_CRMP5 || REC == REC_CRMP5LD || REC == REC_CRMP7)
+    crmp_pencil<REC, T>(n, ng, f, T(rc.kappa), T(rc.kbeta), T(rc.eps), ol, orr);
+  else
+    crweno_pencil<REC, T>(n, ng, f, T(rc.eps), ol, orr);
   T* L = static_cast<T*>(rc.fsl);
   T* Rr = static_cast<T*>(rc.fsr);
   for (int c = 0; c < nc + 2; c++) {

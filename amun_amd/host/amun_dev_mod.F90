@@ -26,6 +26,10 @@ module amun_dev_mod
   integer(c_int32_t), parameter :: AMUN_REC_MP5 = 11
   integer(c_int32_t), parameter :: AMUN_REC_MP7 = 12
   integer(c_int32_t), parameter :: AMUN_REC_GP = 20
+  integer(c_int32_t), parameter :: AMUN_REC_CRWENO5Z = 8, AMUN_REC_CRWENO5YC = 9
+  integer(c_int32_t), parameter :: AMUN_REC_CRWENO5NS = 10
+  integer(c_int32_t), parameter :: AMUN_REC_CRMP5 = 13, AMUN_REC_CRMP5LD = 14
+  integer(c_int32_t), parameter :: AMUN_REC_CRMP7 = 15
   integer(c_int32_t), parameter :: AMUN_LIM_ZERO = 0, AMUN_LIM_MINMOD = 1
   integer(c_int32_t), parameter :: AMUN_LIM_MC = 2, AMUN_LIM_SUPERBEE = 3
   integer(c_int32_t), parameter :: AMUN_LIM_VANLEER = 4, AMUN_LIM_VANALBADA = 5

@@ -88,6 +88,18 @@ program amun_host
     cfg%reconstruction = AMUN_REC_LIMO3
   case("ppm", "PPM")
     cfg%reconstruction = AMUN_REC_PPM
+  case("crweno5z", "crweno5-z", "CRWENO5Z", "CRWENO5-Z")
+    cfg%reconstruction = AMUN_REC_CRWENO5Z
+  case("crweno5yc", "crweno5-yc", "CRWENO5YC", "CRWENO5-YC")
+    cfg%reconstruction = AMUN_REC_CRWENO5YC
+  case("crweno5ns", "crweno5-ns", "CRWENO5NS", "CRWENO5-NS")
+    cfg%reconstruction = AMUN_REC_CRWENO5NS
+  case("crmp5", "CRMP5")
+    cfg%reconstruction = AMUN_REC_CRMP5
+  case("crmp5l", "crmp5ld", "CRMP5L", "CRMP5LD")
+    cfg%reconstruction = AMUN_REC_CRMP5LD
+  case("crmp7", "CRMP7")
+    cfg%reconstruction = AMUN_REC_CRMP7
   case default
     cfg%reconstruction = -1
   end select

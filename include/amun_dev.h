@@ -61,6 +61,9 @@ extern "C" {
 #define AMUN_REC_CRWENO5NS 10  /* reconstruct_crweno5ns, interpolations.F90:3266 */
 #define AMUN_REC_MP5       11  /* reconstruct_mp5, interpolations.F90:3675     */
 #define AMUN_REC_MP7       12  /* reconstruct_mp7, interpolations.F90:3799     */
+#define AMUN_REC_CRMP5     13  /* reconstruct_crmp5, interpolations.F90:3930   */
+#define AMUN_REC_CRMP5LD   14  /* reconstruct_crmp5ld, interpolations.F90:4107 */
+#define AMUN_REC_CRMP7     15  /* reconstruct_crmp7, interpolations.F90:4278   */
 #define AMUN_REC_GP        20  /* Gaussian process, reconstruct_gp interpolations.F90:4561 */
 
 /* TVD limiters: interpolations.F90:379-398 (parameter "limiter") */

@@ -886,6 +886,85 @@ static void reconstruct_crweno(const ocfg* o, int n, const double* f, double* fl
   free(dfm);
 }
 
+/* right-hand side of the compact MP systems on the pencil x: the compact
+ * rows ng .. n-ng+1 (1-based), then the explicit rows of the ends */
+static void crmp_rhs(int n, int ng, int variant, const double* x, double* r)
+{
+  static const double ci5[3] = {1.0 / 30.0, 19.0 / 30.0, 10.0 / 30.0};
+  static const double ci5ld[4] = {3.0 / 120.0, 67.0 / 120.0, 49.0 / 120.0, 1.0 / 120.0};
+  static const double ci7[5] = {-1.0 / 420.0, 19.0 / 420.0, 239.0 / 420.0, 159.0 / 420.0,
+                                4.0 / 420.0};
+  static const double ce7[7] = {-3.0 / 420.0, 25.0 / 420.0, -101.0 / 420.0, 319.0 / 420.0,
+                                214.0 / 420.0, -38.0 / 420.0, 4.0 / 420.0};
+  static const double ce5[5] = {2.0 / 60.0, -13.0 / 60.0, 47.0 / 60.0, 27.0 / 60.0, -3.0 / 60.0};
+  static const double ce3[3] = {-1.0 / 6.0, 5.0 / 6.0, 2.0 / 6.0};
+  /* stencil reach of the compact rows; rows whose stencil leaves the pencil
+   * (ng < 3) are overwritten by the explicit rows below */
+  const int lo = variant == 2 ? 2 : 1, hi = variant == 0 ? 1 : 2;
+  int i, m;
+  for (i = ng - 1; i <= n - ng; i++) {
+    double s = 0.0;
+    if (i < lo || i + hi > n - 1) continue;
+    if (variant == 0)      for (m = 0; m < 3; m++) s = m ? s + ci5[m] * x[i - 1 + m] : ci5[0] * x[i - 1];
+    else if (variant == 1) for (m = 0; m < 4; m++) s = m ? s + ci5ld[m] * x[i - 1 + m] : ci5ld[0] * x[i - 1];
+    else                   for (m = 0; m < 5; m++) s = m ? s + ci7[m] * x[i - 2 + m] : ci7[0] * x[i - 2];
+    r[i] = s;
+  }
+  r[0] = 0.5 * x[0] + 0.5 * x[1];
+  r[1] = (ce3[0] * x[0] + ce3[1] * x[1]) + ce3[2] * x[2];
+#define SUMC(c, k, i0) { double s = c[0] * x[i0]; for (m = 1; m < k; m++) s = s + c[m] * x[i0 + m]; r[i] = s; }
+  if (variant < 2) {
+    for (i = 2; i <= ng - 1; i++) SUMC(ce5, 5, i - 2)
+    for (i = n - ng - 1; i <= n - 3; i++) SUMC(ce5, 5, i - 2)
+  } else {
+    i = 2; SUMC(ce5, 5, 0)
+    for (i = 3; i <= ng - 1; i++) SUMC(ce7, 7, i - 3)
+    for (i = n - ng - 1; i <= n - 4; i++) SUMC(ce7, 7, i - 3)
+    i = n - 3; SUMC(ce5, 5, n - 5)
+  }
+#undef SUMC
+  r[n - 2] = (ce3[0] * x[n - 3] + ce3[1] * x[n - 2]) + ce3[2] * x[n - 1];
+  r[n - 1] = x[n - 1];
+}
+
+/* reconstruct_crmp5 / _crmp5ld / _crmp7 (interpolations.F90:3930-4427):
+ * compact fifth/seventh-order face values from a tridiagonal system of
+ * constant coefficients (identity rows within ng cells of the pencil ends),
+ * then mp_limiting; the right states from the reversed pencil.  variant 0:
+ * crmp5, 1: crmp5ld, 2: crmp7. */
+static void reconstruct_crmp(const ocfg* o, int n, const double* f, double* fl, double* fr,
+                             int variant)
+{
+  static const double di[3][3] = {{0.3, 0.6, 0.1}, {0.25, 0.6, 0.15},
+                                  {2.0 / 7.0, 4.0 / 7.0, 1.0 / 7.0}};
+  const int ng = o->ng;
+  double *a, *b, *c, *r, *u, *g, *fi, *dm;
+  int i;
+  a = (double*)calloc(9 * (size_t)n + 8, sizeof(double));
+  b = a + n; c = b + n; r = c + n; u = r + n; g = u + n; fi = g + n; dm = fi + n;
+  for (i = 0; i < n; i++) {
+    const int id = i >= ng && i <= n - ng - 2;
+    a[i] = id ? di[variant][0] : 0.0;
+    b[i] = id ? di[variant][1] : 1.0;
+    c[i] = id ? di[variant][2] : 0.0;
+  }
+  crmp_rhs(n, ng, variant, f, r);
+  tridiag(n, a, b, c, r, u, g);
+  mp_limiting(o, n, f, u, dm);
+  for (i = 0; i < n; i++) fl[i] = u[i];
+  for (i = 0; i < n; i++) fi[i] = f[n - 1 - i];
+  crmp_rhs(n, ng, variant, fi, r);
+  tridiag(n, a, b, c, r, u, g);
+  mp_limiting(o, n, fi, u, dm);
+  for (i = 0; i < n - 1; i++) fr[i] = u[n - 2 - i];
+  i = n - 2;
+  fl[0] = 0.5 * (f[0] + f[1]);
+  fr[i] = 0.5 * (f[i] + f[n - 1]);
+  fl[n - 1] = f[n - 1];
+  fr[n - 1] = f[n - 1];
+  free(a);
+}
+
 /* reconstruct_gp, interpolations.F90:4561-4668 */
 static void reconstruct_gp(const ocfg* o, int n, const double* f, double* fl, double* fr,
                            double* w1, double* w2, double* w3)
@@ -915,6 +994,9 @@ static void recon_dir(const ocfg* o, int n, double h, const double* f, double* f
   case AMUN_REC_CRWENO5Z: reconstruct_crweno(o, n, f, fl, fr, 0); break;
   case AMUN_REC_CRWENO5YC: reconstruct_crweno(o, n, f, fl, fr, 1); break;
   case AMUN_REC_CRWENO5NS: reconstruct_crweno(o, n, f, fl, fr, 2); break;
+  case AMUN_REC_CRMP5: reconstruct_crmp(o, n, f, fl, fr, 0); break;
+  case AMUN_REC_CRMP5LD: reconstruct_crmp(o, n, f, fl, fr, 1); break;
+  case AMUN_REC_CRMP7: reconstruct_crmp(o, n, f, fl, fr, 2); break;
   case AMUN_REC_LIMO3: reconstruct_limo3(o, n, h, f, fl, fr); break;
   case AMUN_REC_PPM: reconstruct_ppm(o, n, f, fl, fr, w1, w2); break;
   case AMUN_REC_GP: reconstruct_gp(o, n, f, fl, fr, w1, w2, w3); break;
@@ -3282,7 +3364,8 @@ int orc_create(const amun_cfg* c, void** handle)
       c->reconstruction != AMUN_REC_GP && c->reconstruction != AMUN_REC_WENO3 &&
       c->reconstruction != AMUN_REC_LIMO3 && c->reconstruction != AMUN_REC_PPM &&
       c->reconstruction != AMUN_REC_CRWENO5Z && c->reconstruction != AMUN_REC_CRWENO5YC &&
-      c->reconstruction != AMUN_REC_CRWENO5NS) r = 200;
+      c->reconstruction != AMUN_REC_CRWENO5NS && c->reconstruction != AMUN_REC_CRMP5 &&
+      c->reconstruction != AMUN_REC_CRMP5LD && c->reconstruction != AMUN_REC_CRMP7) r = 200;
   if (!r && (c->time_advance < AMUN_TA_EULER || c->time_advance > AMUN_TA_SSPRK53)) r = 400;
   if (!r) {
     int n;

@@ -576,11 +576,13 @@ def main_roe_mhd():
 
 
 CR_SCHEMES = ("crweno5z", "crweno5yc", "crweno5ns")
+CRMP_SCHEMES = ("crmp5", "crmp5ld", "crmp7")
 
 
-def main_crweno():
-    """The compact WENO schemes: the reference's pencils at ng = 2, 3, 4 and
-    Orszag-Tang runs (noise on every field) (own generator)."""
+def main_crweno(schemes=CR_SCHEMES, outname="unit_schemes_cr"):
+    """The compact WENO (or compact MP) schemes: the reference's pencils at
+    ng = 2, 3, 4 and Orszag-Tang runs (noise on every field) (own
+    generator)."""
     rng = np.random.default_rng(20261213)
     fp = np.concatenate([rng.normal(size=(6, 24)), np.cumsum(rng.normal(size=(4, 24)), axis=1),
                          np.sin(np.linspace(0, 6, 24))[None, :] * np.ones((2, 1)),
@@ -588,7 +590,7 @@ def main_crweno():
     out = {"fp": fp}
     with open(os.path.join(HERE, "index.json")) as fh:
         index = json.load(fh)
-    for rec in CR_SCHEMES:
+    for rec in schemes:
         for ng in (2, 3, 4):
             pu = Params(ndims=3, equation_system="mhd", reconstruction=rec, limiter="mm",
                         ncells=16, nghosts=ng, cfl=0.3)
@@ -606,7 +608,7 @@ def main_crweno():
                             sha_u=sha(u))
         index[name] = {"steps": nsteps, "sha_u": sha(u), "dt_last": float(tr[-1, 0])}
         print(name, "dt", tr[-1, 0])
-    np.savez_compressed(os.path.join(HERE, "unit_schemes_cr.npz"), **out)
+    np.savez_compressed(os.path.join(HERE, outname + ".npz"), **out)
     with open(os.path.join(HERE, "index.json"), "w") as fh:
         json.dump(index, fh, indent=1)
 
@@ -696,6 +698,8 @@ def main():
         return
     if sys.argv[1:] == ["crweno"]:
         main_crweno()
+    if sys.argv[1:] == ["crmp"]:
+        main_crweno(CRMP_SCHEMES, "unit_schemes_crmp")
         return
     index = {}
     for name, p, builder, nsteps, stride in run_cases():
@@ -773,6 +777,7 @@ def main():
     main_sr_hllc()
     main_roe_mhd()
     main_crweno()
+    main_crweno(CRMP_SCHEMES, "unit_schemes_crmp")
 
 
 if __name__ == "__main__":

@@ -51,7 +51,7 @@ def _init_code(p):
 
 
 @pytest.mark.parametrize("field,value,code", [
-    ("eqsys", 7, 100), ("eos", 3, 110), ("reconstruction", 13, 200),
+    ("eqsys", 7, 100), ("eos", 3, 110), ("reconstruction", 16, 200),
     ("riemann_solver", 3, 300), ("time_advance", 9, 400), ("nghosts", 1, 600),
     ("ncells", 2, 500),
 ])

@@ -602,7 +602,8 @@ def test_roe_golden_runs(name):
 @pytest.mark.parametrize("nd", [2, 3])
 @pytest.mark.parametrize("eq,rs", [("hd", "hllc"), ("mhd", "hlld")])
 @pytest.mark.parametrize("rec", ["weno5yc", "weno5ns", "mp5", "mp7", "weno3", "limo3", "ppm",
-                                 "crweno5z", "crweno5yc", "crweno5ns"])
+                                 "crweno5z", "crweno5yc", "crweno5ns", "crmp5", "crmp5ld",
+                                 "crmp7"])
 def test_more_schemes_one_step(nd, eq, rs, rec):
     """WENO5-YC, WENO5-NS, MP5, MP7, WENO3, LimO3, PPM (interior formulas on
     the device; the reference's end stencils lie in the outer ghost layers)."""
@@ -628,7 +629,8 @@ def test_more_schemes_one_step(nd, eq, rs, rec):
 @pytest.mark.parametrize("name", ["weno3_ot2d_32_8", "limo3_ot2d_32_8", "ppm_otnoisy2d_32_8",
                                   "weno5yc_ot2d_32_8", "mp7_ot2d_32_8",
                                   "crweno5z_otnoisy2d_32_8", "crweno5yc_otnoisy2d_32_8",
-                                  "crweno5ns_otnoisy2d_32_8"])
+                                  "crweno5ns_otnoisy2d_32_8", "crmp5_otnoisy2d_32_8",
+                                  "crmp5ld_otnoisy2d_32_8", "crmp7_otnoisy2d_32_8"])
 def test_schemes_golden_runs(name):
     """Eight Orszag-Tang steps against the reference's own output (dt trace
     and a subsample of the final state).  PPM runs on the noisy variant: its
@@ -697,7 +699,8 @@ def test_ppm_mixed_vs_fp64():
     assert l1 <= 1e-5
 
 
-@pytest.mark.parametrize("rec", ["weno5yc", "weno5ns", "mp5", "mp7", "weno3", "limo3", "crweno5z"])
+@pytest.mark.parametrize("rec", ["weno5yc", "weno5ns", "mp5", "mp7", "weno3", "limo3", "crweno5z",
+                                 "crmp5"])
 def test_more_schemes_mixed_vs_fp64(rec):
     """Mixed precision on the 64^3 MHD case, 10 steps: density L1."""
     res = []

@@ -402,15 +402,17 @@ def test_roe_mhd_run_bit_exact():
 
 
 # ------------------------------------------------------- compact WENO
-@pytest.mark.parametrize("rec", ["crweno5z", "crweno5yc", "crweno5ns"])
+@pytest.mark.parametrize("rec", ["crweno5z", "crweno5yc", "crweno5ns", "crmp5", "crmp5ld",
+                                 "crmp7"])
 def test_crweno_pencils_and_runs(rec):
-    """reconstruct_crweno5z / _yc / _ns with tridiag on the reference's
-    pencils (ng = 2, 3, 4: the explicit end rows move with ng) and an 8-step
-    Orszag-Tang run (noise on every field), bit for bit."""
+    """reconstruct_crweno5z / _yc / _ns and reconstruct_crmp5 / _crmp5ld /
+    _crmp7 with tridiag on the reference's pencils (ng = 2, 3, 4: the
+    explicit end rows move with ng) and an 8-step Orszag-Tang run (noise on
+    every field), bit for bit."""
     import sys
     sys.path.insert(0, GOLD)
     from make_golden import ot_noisy
-    g = load("unit_schemes_cr")
+    g = load("unit_schemes_crmp" if rec.startswith("crmp") else "unit_schemes_cr")
     for ng in (2, 3, 4):
         p = Params(ndims=3, equation_system="mhd", reconstruction=rec, limiter="mm", ncells=16,
                    nghosts=ng, cfl=0.3)
