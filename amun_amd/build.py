@@ -36,7 +36,8 @@ def _run(cmd, cwd=None):
 
 
 HIP_UNITS = ("amun_dev", "inst_tvd", "inst_weno5z", "inst_weno5yc", "inst_weno5ns", "inst_mp5",
-             "inst_mp7", "inst_gp", "inst_weno3", "inst_limo3", "inst_ppm", "inst_crweno")
+             "inst_mp7", "inst_gp", "inst_weno3", "inst_limo3", "inst_ppm", "inst_crweno",
+             "inst_mgp")
 
 
 def build_hip():
@@ -50,11 +51,15 @@ def build_hip():
     for u in HIP_UNITS:
         obj = os.path.join(bdir, u + ".o")
         jobs.append((obj, [HIPCC, *HIP_FLAGS, "-c", "-o", obj, os.path.join(CSRC, u + ".hip")]))
+    # host-only quad-precision weights (libquadmath): g++, not hipcc
+    qobj = os.path.join(bdir, "gp_weights.o")
+    jobs.append((qobj, ["g++", "-O2", "-ffp-contract=off", "-fPIC", "-c", "-o", qobj,
+                        os.path.join(CSRC, "gp_weights.cpp")]))
     with ThreadPoolExecutor(max_workers=len(jobs)) as ex:
         for f in [ex.submit(_run, cmd) for _, cmd in jobs]:
             f.result()
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", out + ".tmp",
-          *[obj for obj, _ in jobs]])
+          *[obj for obj, _ in jobs], "-lquadmath"])
     os.replace(out + ".tmp", out)
     return out
 

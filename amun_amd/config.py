@@ -20,7 +20,7 @@ EQSYS = {"hd": 1, "hydrodynamic": 1, "mhd": 2, "magnetohydrodynamic": 2,
          "srmhd": 4}
 EOS = {"adi": 1, "adiabatic": 1}
 RECON = {"tvd": 1, "weno3": 2, "limo3": 3, "ppm": 4, "crweno5z": 8, "crweno5-z": 8,
-         "crweno5yc": 9, "crweno5-yc": 9, "crweno5ns": 10, "crweno5-ns": 10, "crmp5": 13,
+         "crweno5yc": 9, "crweno5-yc": 9, "crweno5ns": 10, "crweno5-ns": 10, "crmp5": 13, "mgp": 21,
          "crmp5l": 14, "crmp5ld": 14, "crmp7": 15, "weno5z": 5, "weno5-z": 5, "weno5yc": 6, "weno5-yc": 6, "weno5ns": 7,
          "weno5-ns": 7, "mp5": 11, "mp7": 12, "gp": 20}
 LIMITER = {"zero": 0, "mm": 1, "minmod": 1, "mc": 2, "monotonized_central": 2,

@@ -27,6 +27,7 @@ struct amun_ctx {
   double* trace = nullptr;  // dt ring, [tsteps][nruns]
   int tsteps = 0;
   double* scratch = nullptr;   // reductions / small transfers
+  double* ugp = nullptr;       // MGP weights (RunConst::ugp points here)
   int qcur = 0;                // Q[qcur] holds the primitive state of U0
   long long nsteps = 0;        // steps launched (slot = nsteps & 1)
   hipStream_t stream = nullptr;
@@ -118,6 +119,7 @@ bool pick(Dispatch& d, const amun_cfg& c) {
   case AMUN_REC_CRWENO5Z: case AMUN_REC_CRWENO5YC: case AMUN_REC_CRWENO5NS:
   case AMUN_REC_CRMP5: case AMUN_REC_CRMP5LD: case AMUN_REC_CRMP7:
     return pick_crweno(d, c);
+  case AMUN_REC_MGP: return pick_mgp(d, c);
   default: return false;
   }
 }
@@ -133,12 +135,15 @@ int validate(const amun_cfg& c, int* nv) {
   case AMUN_REC_MP5: case AMUN_REC_MP7: case AMUN_REC_GP: case AMUN_REC_WENO3:
   case AMUN_REC_LIMO3: case AMUN_REC_PPM: case AMUN_REC_CRWENO5Z: case AMUN_REC_CRWENO5YC:
   case AMUN_REC_CRWENO5NS: case AMUN_REC_CRMP5: case AMUN_REC_CRMP5LD: case AMUN_REC_CRMP7:
+  case AMUN_REC_MGP:
     break;
   default: return fail(200, "unsupported reconstruction");
   }
   if (c.reconstruction == AMUN_REC_GP && c.ngp != 5)
     return fail(600, "the Gaussian-process reconstruction runs with ngp = 5 on the device path");
-  if (c.reconstruction == AMUN_REC_GP && !(c.sgp > 0.0))
+  if (c.reconstruction == AMUN_REC_MGP && c.ngp != 3 && c.ngp != 5)
+    return fail(600, "the multi-dimensional GP runs with ngp = 3 or 5 on the device path");
+  if ((c.reconstruction == AMUN_REC_GP || c.reconstruction == AMUN_REC_MGP) && !(c.sgp > 0.0))
     return fail(202, "sgp must be positive");
   if (c.limiter < 0 || c.limiter > 5) return fail(200, "unsupported limiter");
   if (c.eqsys == AMUN_EQSYS_SRMHD) {
@@ -167,6 +172,7 @@ int validate(const amun_cfg& c, int* nv) {
   case AMUN_REC_WENO5Z: case AMUN_REC_WENO5YC: case AMUN_REC_WENO5NS: case AMUN_REC_MP7:
   case AMUN_REC_PPM: need = 4; break;
   case AMUN_REC_GP: case AMUN_REC_MP5: need = 3; break;
+  case AMUN_REC_MGP: need = (c.ngp - 1) / 2 + 1; break;
   }
   if (c.ncells + 2 * c.nghosts > 96)
     return fail(600, "ncells + 2 nghosts above 96");
@@ -245,6 +251,14 @@ void gp_weights(double sgp, double* cgp) {
     for (int i = 0; i < n; i++) acc += x[i] * g[i][j + n];
     cgp[j] = (double)acc;
   }
+}
+
+// prepare_mgp in quad precision (gp_weights.cpp, built by the host compiler)
+extern "C" bool amun_mgp_weights_quad(int nd, int ngp, double sgp, double* out);
+bool mgp_weights(const amun_cfg& cfg, std::vector<double>& out) {
+  const int n = cfg.ndims == 3 ? cfg.ngp * cfg.ngp * cfg.ngp : cfg.ngp * cfg.ngp;
+  out.assign((size_t)2 * cfg.ndims * n, 0.0);
+  return amun_mgp_weights_quad(cfg.ndims, cfg.ngp, cfg.sgp, out.data());
 }
 
 int ssprk_stages(const amun_cfg& cfg) { return std::max(2, std::min(9, (int)cfg.stages)); }
@@ -511,6 +525,22 @@ static int init_device(amun_ctx* c, const std::vector<int32_t>& nbr,
     const size_t fsz = (size_t)rc.nblk * nv * (rc.nc + 2) * npen * c->qsize;
     ALLOC(c->rc.fsl, fsz);
     ALLOC(c->rc.fsr, fsz);
+  }
+  if (cfg->reconstruction == AMUN_REC_MGP) {
+    std::vector<double> w;
+    if (!mgp_weights(*cfg, w)) {
+      amun_dev_finalize(c);
+      return fail(201, "could not invert the MGP covariance matrix");
+    }
+    ALLOC(c->ugp, w.size() * sizeof(double));
+    e = hipMemcpy(c->ugp, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      amun_dev_finalize(c);
+      return fail(1000 + (int)e, std::string("hipMemcpy: ") + hipGetErrorString(e));
+    }
+    c->rc.ugp = c->ugp;
+    c->rc.mgp = (cfg->ngp - 1) / 2;
+    c->rc.dgp = (int)(w.size() / (2 * rc.nd));
   }
 #undef ALLOC
   e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
@@ -866,6 +896,7 @@ void amun_dev_finalize(amun_ctx* c) {
   hipFree(c->trace);
   hipFree(c->scratch);
   hipFree(c->rc.fsl);
+  hipFree(c->ugp);
   hipFree(c->rc.fsr);
   hipFree(c->amr.blev);
   hipFree(c->bnd.task);

@@ -101,6 +101,8 @@ struct RunConst {
   int ng;                           // ghost layers of the reference's pencils
   void* fsl;                        // compact schemes: face states of the
   void* fsr;                        //   current sweep direction
+  const double* ugp;                // MGP weights [dir][side][stencil]
+  int mgp, dgp;                     // MGP half width and stencil size ngp^nd
   SrConst sc;                       // relativistic primitive solver
 };
 
@@ -190,7 +192,7 @@ __device__ __forceinline__ constexpr int rot(int p) {
 enum {
   REC_CRWENO5Z = AMUN_REC_CRWENO5Z, REC_CRWENO5YC = AMUN_REC_CRWENO5YC,
   REC_CRWENO5NS = AMUN_REC_CRWENO5NS, REC_CRMP5 = AMUN_REC_CRMP5,
-  REC_CRMP5LD = AMUN_REC_CRMP5LD, REC_CRMP7 = AMUN_REC_CRMP7,
+  REC_CRMP5LD = AMUN_REC_CRMP5LD, REC_CRMP7 = AMUN_REC_CRMP7, REC_MGP = AMUN_REC_MGP,
   REC_TVD = AMUN_REC_TVD, REC_WENO3 = AMUN_REC_WENO3, REC_LIMO3 = AMUN_REC_LIMO3,
   REC_PPM = AMUN_REC_PPM, REC_WENO5Z = AMUN_REC_WENO5Z, REC_WENO5YC = AMUN_REC_WENO5YC,
   REC_WENO5NS = AMUN_REC_WENO5NS, REC_MP5 = AMUN_REC_MP5, REC_MP7 = AMUN_REC_MP7,
@@ -760,6 +762,81 @@ struct Recon<P, REC_TVD, ND, DIR, T> {
       }
       flc[v] = qc + dq;
       frc[v] = qc - dq;
+    }
+  }
+};
+
+// Multi-dimensional Gaussian process (interfaces_mgp, interpolations.F90:
+// 978-1144, mlp_limiting off): the 2 ND face values of the cell from the
+// weights ugp over its ngp^ND neighbourhood (read from the pool, i fastest,
+// as the reference's reshape); where any face value leaves the range of its
+// two cells by more than eps, all of them fall back to the limited TVD slopes
+// with the positivity halving.  The sweep along DIR keeps that direction's
+// pair.
+template <class P, int ND, int DIR, typename T>
+struct Recon<P, REC_MGP, ND, DIR, T> {
+  static constexpr int R = 0;
+  static constexpr int H = 2;
+  __device__ __forceinline__ void load(int, const T*) {}
+  __device__ __forceinline__ void shift() {}
+  __device__ __forceinline__ void cell(const RunConst& rc, T* flc, T* frc,
+                                       const T* __restrict__ Q,
+                                       const int* __restrict__ nbr, int b, int i, int j,
+                                       int k, DevState* st, double) {
+    const int mg = rc.mgp, dgp = rc.dgp;
+    const double* __restrict__ ug = rc.ugp;
+    const T eps = T(rc.eps);
+    for (int v = 0; v < P::nv; v++) {
+      auto at = [&](int a, int bb, int c) {
+        return Q[var_index<P::nv>(rc, cell_index<sizeof(T)>(nbr, rc, b, i + a, j + bb, k + c), v)];
+      };
+      const T qc = at(0, 0, 0);
+      T fv[2 * ND];
+#pragma unroll
+      for (int s = 0; s < 2 * ND; s++) fv[s] = T(0);
+      int m = 0;
+      for (int c = (ND == 3 ? -mg : 0); c <= (ND == 3 ? mg : 0); c++)
+        for (int bb = -mg; bb <= mg; bb++)
+          for (int a = -mg; a <= mg; a++, m++) {
+            const T u = at(a, bb, c) - qc;
+#pragma unroll
+            for (int s = 0; s < 2 * ND; s++) fv[s] = fv[s] + T(ug[s * dgp + m]) * u;
+          }
+      T qn[2 * ND];   // the upper and lower neighbours along each direction
+#pragma unroll
+      for (int d = 0; d < ND; d++) {
+        qn[2 * d] = at(d == 0, d == 1, d == 2);
+        qn[2 * d + 1] = at(-(d == 0), -(d == 1), -(d == 2));
+      }
+      bool flag = false;
+#pragma unroll
+      for (int s = 0; s < 2 * ND; s++) {
+        fv[s] = fv[s] + qc;
+        flag = flag || ((fv[s] - qn[s]) * (fv[s] - qc) > eps);
+      }
+      if (flag) {
+        T dq[3] = {T(0), T(0), T(0)};
+#pragma unroll
+        for (int d = 0; d < ND; d++)
+          dq[d] = limiter<T>(rc.limiter, T(0.5), qc - qn[2 * d + 1], qn[2 * d] - qc);
+        if (v == P::idn || v == P::ipr) {
+          if (qc > T(0)) {
+            for (int it = 0; it < kMaxHalvings; it++) {
+              T sa = tabs(dq[0]) + tabs(dq[1]);
+              if (ND == 3) sa += tabs(dq[2]);
+              if (!(qc <= sa)) break;
+              dq[0] *= T(0.5); dq[1] *= T(0.5); dq[2] *= T(0.5);
+            }
+          } else {
+            atomicAdd(&st->npos, 1ull);
+            dq[0] = dq[1] = dq[2] = T(0);
+          }
+        }
+        fv[2 * DIR] = qc + dq[DIR];
+        fv[2 * DIR + 1] = qc - dq[DIR];
+      }
+      flc[v] = fv[2 * DIR];
+      frc[v] = fv[2 * DIR + 1];
     }
   }
 };
@@ -1466,7 +1543,7 @@ void fill(Dispatch& d) {
   d.unit_riemann = k_unit_riemann<P, RS, T>;
   d.unit_c2p = k_unit_c2p<P>;
   // TVD reads the transverse slopes of the block: no pencil form
-  if constexpr (REC != REC_TVD && !is_compact(REC)) d.unit_recon = k_unit_recon<P, REC, T>;
+  if constexpr (REC != REC_TVD && REC != REC_MGP && !is_compact(REC)) d.unit_recon = k_unit_recon<P, REC, T>;
   d.unit_eqs = k_unit_eqs<P, T>;
   d.fineflux[0] = k_amr_fineflux<P, REC, RS, ND, 0, T>;
   d.fineflux[1] = k_amr_fineflux<P, REC, RS, ND, 1, T>;
@@ -1532,5 +1609,6 @@ bool pick_weno3(Dispatch& d, const amun_cfg& c);
 bool pick_limo3(Dispatch& d, const amun_cfg& c);
 bool pick_ppm(Dispatch& d, const amun_cfg& c);
 bool pick_crweno(Dispatch& d, const amun_cfg& c);
+bool pick_mgp(Dispatch& d, const amun_cfg& c);
 
 }  // namespace amun_dev

@@ -29,7 +29,7 @@ module amun_dev_mod
   integer(c_int32_t), parameter :: AMUN_REC_CRWENO5Z = 8, AMUN_REC_CRWENO5YC = 9
   integer(c_int32_t), parameter :: AMUN_REC_CRWENO5NS = 10
   integer(c_int32_t), parameter :: AMUN_REC_CRMP5 = 13, AMUN_REC_CRMP5LD = 14
-  integer(c_int32_t), parameter :: AMUN_REC_CRMP7 = 15
+  integer(c_int32_t), parameter :: AMUN_REC_CRMP7 = 15, AMUN_REC_MGP = 21
   integer(c_int32_t), parameter :: AMUN_LIM_ZERO = 0, AMUN_LIM_MINMOD = 1
   integer(c_int32_t), parameter :: AMUN_LIM_MC = 2, AMUN_LIM_SUPERBEE = 3
   integer(c_int32_t), parameter :: AMUN_LIM_VANLEER = 4, AMUN_LIM_VANALBADA = 5

@@ -100,6 +100,8 @@ program amun_host
     cfg%reconstruction = AMUN_REC_CRMP5LD
   case("crmp7", "CRMP7")
     cfg%reconstruction = AMUN_REC_CRMP7
+  case("mgp", "MGP")
+    cfg%reconstruction = AMUN_REC_MGP
   case default
     cfg%reconstruction = -1
   end select

@@ -64,6 +64,7 @@ extern "C" {
 #define AMUN_REC_CRMP5     13  /* reconstruct_crmp5, interpolations.F90:3930   */
 #define AMUN_REC_CRMP5LD   14  /* reconstruct_crmp5ld, interpolations.F90:4107 */
 #define AMUN_REC_CRMP7     15  /* reconstruct_crmp7, interpolations.F90:4278   */
+#define AMUN_REC_MGP       21  /* interfaces_mgp, interpolations.F90:978       */
 #define AMUN_REC_GP        20  /* Gaussian process, reconstruct_gp interpolations.F90:4561 */
 
 /* TVD limiters: interpolations.F90:379-398 (parameter "limiter") */

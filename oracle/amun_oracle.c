@@ -76,9 +76,12 @@ typedef struct {
   double kappa;            /* min(kappa, (1 - cfl)/cfl), interpolations.F90:217 */
   int    mgp;              /* (ngp - 1) / 2                                  */
   double cgp[16];          /* Gaussian-process weights                      */
+  int    dgp;              /* ngp^nd, the MGP stencil                        */
+  double ugp[750];         /* MGP weights [dir][side][stencil], ngp <= 5     */
 } ocfg;
 
 static int prepare_gp(ocfg* o);
+static int prepare_mgp(ocfg* o);
 
 /* MIN and MAX as the reference's compiler evaluates them: a < b ? a : b and
  * a > b ? a : b, so a NaN first operand yields the second and a NaN second
@@ -133,6 +136,11 @@ static int oc_setup(ocfg* o, const amun_cfg* c)
   if (c->reconstruction == AMUN_REC_GP) {
     int r = prepare_gp(o);
     if (r) return r;
+  }
+  if (c->reconstruction == AMUN_REC_MGP) {
+    int r = prepare_mgp(o);
+    if (r) return r;
+    if (o->ng < o->mgp + 1) return 600;   /* the stencil of the cells next to the block */
   }
   /* initialize_evolution (evolution.F90:156-200): the m-stage and
    * four/five-stage SSPRK schemes run at a multiple of the CFL number */
@@ -373,42 +381,118 @@ static void fix_positivity(int n, const double* f, double* fl, double* fr)
  * (max_real_kind, algebra.F90:42); returns 0 when the matrix is singular */
 static int invert_q(int n, const __float128* m, __float128* r)
 {
-  __float128 g[16][32], t;
-  int i, j, k, flag;
+  __float128 *gm, t;
+  int i, j, k, flag, ok = 0;
+  gm = (__float128*)malloc((size_t)n * 2 * n * sizeof(__float128));
+  if (!gm) return 0;
+#define g(i, j) gm[(size_t)(i) * 2 * n + (j)]
   for (i = 0; i < n; i++)
     for (j = 0; j < 2 * n; j++)
-      g[i][j] = (j < n) ? m[i * n + j] : ((i + n == j) ? 1.0Q : 0.0Q);
+      g(i, j) = (j < n) ? m[i * n + j] : ((i + n == j) ? 1.0Q : 0.0Q);
   for (k = 0; k < n - 1; k++) {
-    if (g[k][k] == 0.0Q) {
+    if (g(k, k) == 0.0Q) {
       flag = 0;
       for (i = k + 1; i < n; i++) {
-        if (g[i][k] != 0.0Q) {
-          for (j = 0; j < 2 * n; j++) g[k][j] = g[k][j] + g[i][j];
+        if (g(i, k) != 0.0Q) {
+          for (j = 0; j < 2 * n; j++) g(k, j) = g(k, j) + g(i, j);
           flag = 1;
           break;
         }
       }
-      if (!flag) return 0;
+      if (!flag) goto done;
     }
     for (j = k + 1; j < n; j++) {
-      t = g[j][k] / g[k][k];
-      for (i = k; i < 2 * n; i++) g[j][i] = g[j][i] - t * g[k][i];
+      t = g(j, k) / g(k, k);
+      for (i = k; i < 2 * n; i++) g(j, i) = g(j, i) - t * g(k, i);
     }
   }
   for (i = 0; i < n; i++)
-    if (g[i][i] == 0.0Q) return 0;
+    if (g(i, i) == 0.0Q) goto done;
   for (i = 0; i < n; i++) {
-    t = g[i][i];
-    for (j = i; j < 2 * n; j++) g[i][j] = g[i][j] / t;
+    t = g(i, i);
+    for (j = i; j < 2 * n; j++) g(i, j) = g(i, j) / t;
   }
   for (k = n - 2; k >= 0; k--)
     for (i = 0; i <= k; i++) {
-      t = g[i][k + 1];
-      for (j = k; j < 2 * n; j++) g[i][j] = g[i][j] - g[k + 1][j] * t;
+      t = g(i, k + 1);
+      for (j = k; j < 2 * n; j++) g(i, j) = g(i, j) - g(k + 1, j) * t;
     }
   for (i = 0; i < n; i++)
-    for (j = 0; j < n; j++) r[i * n + j] = g[i][j + n];
-  return 1;
+    for (j = 0; j < n; j++) r[i * n + j] = g(i, j + n);
+  ok = 1;
+done:
+#undef g
+  free(gm);
+  return ok;
+}
+
+/* prepare_mgp, interpolations.F90:558-711: the multi-dimensional Gaussian
+ * process over the ngp^nd stencil; covariance of the cell averages and the
+ * face vectors (xgp) in quad precision, ugp(:, s, d) = xgp(:, s, d) A^-1
+ * (no symmetrisation here, unlike prepare_gp).  Stencil index runs i fastest,
+ * then j, then k. */
+static int prepare_mgp(ocfg* o)
+{
+  const int ngp = o->c.ngp, mgp = (ngp - 1) / 2, nd = o->nd;
+  const int dgp = nd == 3 ? ngp * ngp * ngp : ngp * ngp;
+  __float128 *cov, *inv, *xgp, sig, fc;
+  int i, j, a1, b1, c1, a2, b2, c2, s, d, m;
+  if (ngp < 1 || ngp > 5 || ngp % 2 == 0) return 202;
+  cov = (__float128*)malloc(((size_t)2 * dgp * dgp + 6 * dgp) * sizeof(__float128));
+  if (!cov) return 500;
+  inv = cov + (size_t)dgp * dgp; xgp = inv + (size_t)dgp * dgp;
+  sig = (__float128)(sqrt(2.0) * o->c.sgp);
+  fc  = (__float128)(0.5 * sqrt(3.14159265358979323846264338327950)) * sig;
+#define ERFD(t) (erfq((((__float128)(1.0 * (t))) + 0.5Q) / sig) - erfq((((__float128)(1.0 * (t))) - 0.5Q) / sig))
+  i = 0;
+  for (c1 = (nd == 3 ? -mgp : 0); c1 <= (nd == 3 ? mgp : 0); c1++)
+    for (b1 = -mgp; b1 <= mgp; b1++)
+      for (a1 = -mgp; a1 <= mgp; a1++, i++) {
+        j = 0;
+        for (c2 = (nd == 3 ? -mgp : 0); c2 <= (nd == 3 ? mgp : 0); c2++)
+          for (b2 = -mgp; b2 <= mgp; b2++)
+            for (a2 = -mgp; a2 <= mgp; a2++, j++) {
+              __float128 v = fc * ERFD(a1 - a2) * ERFD(b1 - b2);
+              if (nd == 3) v = v * ERFD(c1 - c2);
+              cov[(size_t)i * dgp + j] = v;
+            }
+      }
+  i = 0;
+  for (c1 = (nd == 3 ? -mgp : 0); c1 <= (nd == 3 ? mgp : 0); c1++)
+    for (b1 = -mgp; b1 <= mgp; b1++)
+      for (a1 = -mgp; a1 <= mgp; a1++, i++) {
+        const __float128 xl = ((__float128)(1.0 * a1) - 0.5Q) / sig, xr = ((__float128)(1.0 * a1) + 0.5Q) / sig;
+        const __float128 yl = ((__float128)(1.0 * b1) - 0.5Q) / sig, yr = ((__float128)(1.0 * b1) + 0.5Q) / sig;
+        const __float128 zl = ((__float128)(1.0 * c1) - 0.5Q) / sig, zr = ((__float128)(1.0 * c1) + 0.5Q) / sig;
+        const __float128 fx = erfq(xr) - erfq(xl), fy = erfq(yr) - erfq(yl);
+        const __float128 fz = nd == 3 ? erfq(zr) - erfq(zl) : 1.0Q;
+        if (nd == 3) {
+          xgp[0 * dgp + i] = expq(-(xl * xl)) * fy * fz;
+          xgp[1 * dgp + i] = expq(-(xr * xr)) * fy * fz;
+          xgp[2 * dgp + i] = expq(-(yl * yl)) * fx * fz;
+          xgp[3 * dgp + i] = expq(-(yr * yr)) * fx * fz;
+          xgp[4 * dgp + i] = expq(-(zl * zl)) * fx * fy;
+          xgp[5 * dgp + i] = expq(-(zr * zr)) * fx * fy;
+        } else {
+          xgp[0 * dgp + i] = expq(-(xl * xl)) * fy;
+          xgp[1 * dgp + i] = expq(-(xr * xr)) * fy;
+          xgp[2 * dgp + i] = expq(-(yl * yl)) * fx;
+          xgp[3 * dgp + i] = expq(-(yr * yr)) * fx;
+        }
+      }
+#undef ERFD
+  if (!invert_q(dgp, cov, inv)) { free(cov); return 201; }
+  for (d = 0; d < nd; d++)
+    for (s = 0; s < 2; s++)
+      for (m = 0; m < dgp; m++) {
+        __float128 acc = 0.0Q;
+        for (i = 0; i < dgp; i++) acc = acc + xgp[(2 * d + s) * dgp + i] * inv[(size_t)i * dgp + m];
+        o->ugp[((size_t)(d * 2 + s)) * dgp + m] = (double)acc;
+      }
+  o->dgp = dgp;
+  o->mgp = mgp;
+  free(cov);
+  return 0;
 }
 
 /* prepare_gp, interpolations.F90:4438-4547: the weights cgp = x^T A^-1 of
@@ -1022,7 +1106,73 @@ static void reconstruct_interfaces(ocfg* o, const double* q, double* qs)
 
   for (p = 0; p < nv; p++) {
     int positive = (p == o->idn || p == o->ipr);
-    if (o->c.reconstruction == AMUN_REC_TVD) {
+    if (o->c.reconstruction == AMUN_REC_MGP) {
+      /* interfaces_mgp, interpolations.F90:978-1144 (mlp_limiting off): the
+       * 2 nd face values of each cell from the GP weights; where any of them
+       * leaves the range of its two cells by more than eps, all of them fall
+       * back to the limited TVD slopes with the positivity halving */
+      const int mg = o->mgp, dgp = o->dgp;
+      double u[125];
+      for (k = o->kbl; k <= o->keu; k++)
+        for (j = o->jbl; j <= o->jeu; j++)
+          for (i = o->ibl; i <= o->ieu; i++) {
+            const double qc = q[QI(o, p, i, j, k)];
+            double fv[6], dq[3] = {0.0, 0.0, 0.0}, dql, dqr;
+            int m = 0, a, bb, cc, fl = 0, ds;
+            for (cc = (nd == 3 ? -mg : 0); cc <= (nd == 3 ? mg : 0); cc++)
+              for (bb = -mg; bb <= mg; bb++)
+                for (a = -mg; a <= mg; a++) u[m++] = q[QI(o, p, i + a, j + bb, k + cc)] - qc;
+            for (ds = 0; ds < 2 * nd; ds++) {
+              const double* w = &o->ugp[(size_t)ds * dgp];
+              double acc = 0.0;
+              for (m = 0; m < dgp; m++) acc = acc + w[m] * u[m];
+              fv[ds] = acc + qc;
+            }
+            fl = ((fv[0] - q[QI(o, p, i + 1, j, k)]) * (fv[0] - qc) > o->c.eps) ||
+                 ((fv[1] - q[QI(o, p, i - 1, j, k)]) * (fv[1] - qc) > o->c.eps) ||
+                 ((fv[2] - q[QI(o, p, i, j + 1, k)]) * (fv[2] - qc) > o->c.eps) ||
+                 ((fv[3] - q[QI(o, p, i, j - 1, k)]) * (fv[3] - qc) > o->c.eps);
+            if (nd == 3)
+              fl = fl || ((fv[4] - q[QI(o, p, i, j, k + 1)]) * (fv[4] - qc) > o->c.eps) ||
+                   ((fv[5] - q[QI(o, p, i, j, k - 1)]) * (fv[5] - qc) > o->c.eps);
+            if (fl) {
+              dql = qc - q[QI(o, p, i - 1, j, k)];
+              dqr = q[QI(o, p, i + 1, j, k)] - qc;
+              dq[0] = lim(o->c.limiter, 0.5, dql, dqr);
+              dql = qc - q[QI(o, p, i, j - 1, k)];
+              dqr = q[QI(o, p, i, j + 1, k)] - qc;
+              dq[1] = lim(o->c.limiter, 0.5, dql, dqr);
+              if (nd == 3) {
+                dql = qc - q[QI(o, p, i, j, k - 1)];
+                dqr = q[QI(o, p, i, j, k + 1)] - qc;
+                dq[2] = lim(o->c.limiter, 0.5, dql, dqr);
+              }
+              if (positive) {
+                if (qc > 0.0) {
+                  int it;
+                  for (it = 0; it < OMAXHALV; it++) {
+                    double sa = fabs(dq[0]) + fabs(dq[1]);
+                    if (nd == 3) sa += fabs(dq[2]);
+                    if (!(qc <= sa)) break;
+                    dq[0] *= 0.5; dq[1] *= 0.5; dq[2] *= 0.5;
+                  }
+                } else {
+                  o->npos_events++;
+                  dq[0] = dq[1] = dq[2] = 0.0;
+                }
+              }
+              for (ds = 0; ds < nd; ds++) { fv[2 * ds] = qc + dq[ds]; fv[2 * ds + 1] = qc - dq[ds]; }
+            }
+            QS(0, 0, i, j, k, p)     = fv[0];
+            QS(0, 1, i - 1, j, k, p) = fv[1];
+            QS(1, 0, i, j, k, p)     = fv[2];
+            QS(1, 1, i, j - 1, k, p) = fv[3];
+            if (nd == 3) {
+              QS(2, 0, i, j, k, p)     = fv[4];
+              QS(2, 1, i, j, k - 1, p) = fv[5];
+            }
+          }
+    } else if (o->c.reconstruction == AMUN_REC_TVD) {
       /* interfaces_tvd, interpolations.F90:729-848 */
       for (k = o->kbl; k <= o->keu; k++)
         for (j = o->jbl; j <= o->jeu; j++)
@@ -3365,7 +3515,8 @@ int orc_create(const amun_cfg* c, void** handle)
       c->reconstruction != AMUN_REC_LIMO3 && c->reconstruction != AMUN_REC_PPM &&
       c->reconstruction != AMUN_REC_CRWENO5Z && c->reconstruction != AMUN_REC_CRWENO5YC &&
       c->reconstruction != AMUN_REC_CRWENO5NS && c->reconstruction != AMUN_REC_CRMP5 &&
-      c->reconstruction != AMUN_REC_CRMP5LD && c->reconstruction != AMUN_REC_CRMP7) r = 200;
+      c->reconstruction != AMUN_REC_CRMP5LD && c->reconstruction != AMUN_REC_CRMP7 &&
+      c->reconstruction != AMUN_REC_MGP) r = 200;
   if (!r && (c->time_advance < AMUN_TA_EULER || c->time_advance > AMUN_TA_SSPRK53)) r = 400;
   if (!r) {
     int n;

@@ -613,6 +613,39 @@ def main_crweno(schemes=CR_SCHEMES, outname="unit_schemes_cr"):
         json.dump(index, fh, indent=1)
 
 
+MGP_CASES = (("mgp_otnoisy2d_32_4", 2, 5, 4), ("mgp3_otnoisy2d_32_4", 2, 3, 2),
+             ("mgp_otnoisy3d_16_4", 3, 5, 3))
+
+
+def mgp_params(nd, ngp, ng):
+    p = Params(ndims=nd, equation_system="mhd", reconstruction="mgp", limiter="mm",
+               riemann_solver="hlld", time_advance="rk3", gamma=5.0 / 3.0, cfl=0.3, ncells=8,
+               nghosts=ng, xblocks=4 if nd == 2 else 2, yblocks=4 if nd == 2 else 2,
+               zblocks=2)
+    p.ngp = ngp
+    return p
+
+
+def main_mgp():
+    """Multi-dimensional GP: Orszag-Tang runs (noise on every field), 2D at
+    ngp = 5 and 3, 3D at ngp = 5 (own generator)."""
+    with open(os.path.join(HERE, "index.json")) as fh:
+        index = json.load(fh)
+    for name, nd, ngp, ng in MGP_CASES:
+        p = mgp_params(nd, ngp, ng)
+        u0 = ot_noisy(p)
+        nsteps = 4
+        u, tr = ol.ref_run(p, u0, nsteps)
+        sub = u[:, ::3, ::3, ::3] if nd == 3 else u[:, :, ::3, ::3]
+        np.savez_compressed(os.path.join(HERE, name + ".npz"),
+                            params=np.array(p.to_params_in()), ndims=nd, nsteps=nsteps,
+                            stride=3, trace=tr, sub=sub, sha_u0=sha(u0), sha_u=sha(u))
+        index[name] = {"steps": nsteps, "sha_u": sha(u), "dt_last": float(tr[-1, 0])}
+        print(name, "dt", tr[-1, 0])
+    with open(os.path.join(HERE, "index.json"), "w") as fh:
+        json.dump(index, fh, indent=1)
+
+
 STAT_CASES = ("hd2d", "mhd2d", "mhd3d", "srmhd2d")
 
 
@@ -698,6 +731,9 @@ def main():
         return
     if sys.argv[1:] == ["crweno"]:
         main_crweno()
+    if sys.argv[1:] == ["mgp"]:
+        main_mgp()
+        return
     if sys.argv[1:] == ["crmp"]:
         main_crweno(CRMP_SCHEMES, "unit_schemes_crmp")
         return
@@ -778,6 +814,7 @@ def main():
     main_roe_mhd()
     main_crweno()
     main_crweno(CRMP_SCHEMES, "unit_schemes_crmp")
+    main_mgp()
 
 
 if __name__ == "__main__":
