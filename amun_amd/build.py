@@ -35,33 +35,54 @@ def _run(cmd, cwd=None):
     return r
 
 
-HIP_UNITS = ("amun_dev", "inst_tvd", "inst_weno5z", "inst_weno5yc", "inst_weno5ns", "inst_mp5",
-             "inst_mp7", "inst_gp", "inst_weno3", "inst_limo3", "inst_ppm", "inst_crweno",
-             "inst_mgp")
+def inst_entries():
+    """(name, rec, eqsys, precision) of every kernel-table unit, read from
+    csrc/inst_list.h."""
+    import re
+    text = open(os.path.join(CSRC, "inst_list.h")).read()
+    return re.findall(r"X\((pick_\w+), (\w+), (\w+), (\w+)\)", text)
 
 
-def build_hip():
-    """Compile the host unit and the kernel instantiation units in parallel,
-    then link libamun_hip.so."""
-    from concurrent.futures import ThreadPoolExecutor
-    out = os.path.join(PKG, "libamun_hip.so")
+def hip_jobs():
+    """(object, command) of every HIP compilation: the host unit of the C-ABI
+    and one kernel-table unit per entry of inst_list.h."""
     bdir = os.path.join(PKG, "_build")
     os.makedirs(bdir, exist_ok=True)
-    jobs = []
-    for u in HIP_UNITS:
-        obj = os.path.join(bdir, u + ".o")
-        jobs.append((obj, [HIPCC, *HIP_FLAGS, "-c", "-o", obj, os.path.join(CSRC, u + ".hip")]))
+    obj = os.path.join(bdir, "amun_dev.o")
+    jobs = [(obj, [HIPCC, *HIP_FLAGS, "-c", "-o", obj, os.path.join(CSRC, "amun_dev.hip")])]
+    for name, rec, eq, prec in inst_entries():
+        obj = os.path.join(bdir, name + ".o")
+        jobs.append((obj, [HIPCC, *HIP_FLAGS, f"-DAMUN_INST_ENTRY={name}",
+                           f"-DAMUN_INST_REC={rec}", f"-DAMUN_INST_EQ={eq}",
+                           f"-DAMUN_INST_PREC={prec}", "-c", "-o", obj,
+                           os.path.join(CSRC, "inst_unit.hip")]))
     # host-only quad-precision weights (libquadmath): g++, not hipcc
     qobj = os.path.join(bdir, "gp_weights.o")
     jobs.append((qobj, ["g++", "-O2", "-ffp-contract=off", "-fPIC", "-c", "-o", qobj,
                         os.path.join(CSRC, "gp_weights.cpp")]))
-    with ThreadPoolExecutor(max_workers=len(jobs)) as ex:
-        for f in [ex.submit(_run, cmd) for _, cmd in jobs]:
-            f.result()
+    return jobs
+
+
+def link_hip(objs):
+    out = os.path.join(PKG, "libamun_hip.so")
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", out + ".tmp",
-          *[obj for obj, _ in jobs], "-lquadmath"])
+          *objs, "-lquadmath"])
     os.replace(out + ".tmp", out)
     return out
+
+
+def build_hip(jobs_max=None):
+    """Compile the host unit and the kernel-table units in parallel (largest
+    units first), then link libamun_hip.so."""
+    from concurrent.futures import ThreadPoolExecutor
+    jobs = hip_jobs()
+    # the MHD and SRMHD tables hold the most kernels: start them first
+    order = sorted(jobs, key=lambda j: (("_srmhd_" not in j[0]), ("_mhd_" not in j[0])))
+    workers = jobs_max or max(1, os.cpu_count() or 1)
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        for f in [ex.submit(_run, cmd) for _, cmd in order]:
+            f.result()
+    return link_hip([obj for obj, _ in jobs])
 
 
 def build_oracle():

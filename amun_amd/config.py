@@ -61,6 +61,9 @@ class AmunCfg(ctypes.Structure):
         ("prol_limiter", ctypes.c_int32), ("primitive_solver", ctypes.c_int32),
         ("limo3_rad", ctypes.c_double), ("ppm_const", ctypes.c_double),
         ("bnd_type", ctypes.c_int32 * 6),
+        ("mlp_limiting", ctypes.c_int32), ("fix_unphysical_cells", ctypes.c_int32),
+        ("glm_source_terms", ctypes.c_int32), ("reserved_cfg", ctypes.c_int32),
+        ("viscosity", ctypes.c_double), ("resistivity", ctypes.c_double),
     ]
 
 
@@ -127,6 +130,11 @@ class Params:
     zubndry: str = "periodic"
     limo3_rad: float = 0.5
     ppm_const: float = 1.25
+    mlp_limiting: str = "off"
+    fix_unphysical_cells: str = "off"
+    glm_source_terms: str = "none"
+    viscosity: float = 0.0
+    resistivity: float = 0.0
     extra: dict = field(default_factory=dict)
 
     # ---------------------------------------------------------------- derived
@@ -187,6 +195,12 @@ class Params:
                                   self.zlbndry, self.zubndry)):
             c.bnd_type[n] = pick(BNDTYPE, side, "boundary type", 600)
         # interpolations.F90:412-428: mm, mc, sb, vl, anything else the zero slope
+        c.mlp_limiting = _onoff(self.mlp_limiting)
+        c.fix_unphysical_cells = _onoff(self.fix_unphysical_cells)
+        # sources.F90:128-141: eglm, heglm, anything else none
+        c.glm_source_terms = {"eglm": 1, "heglm": 2}.get(
+            str(self.glm_source_terms).strip().lower(), 0)
+        c.viscosity, c.resistivity = float(self.viscosity), float(self.resistivity)
         c.prol_limiter = {"mm": 1, "minmod": 1, "mc": 2, "monotonized_central": 2,
                           "sb": 3, "superbee": 3, "vl": 4, "vanleer": 4}.get(
                               self.prolongation_limiter.lower(), 0)
@@ -224,6 +238,10 @@ class Params:
             f"prolongation_limiter = {self.prolongation_limiter}",
             f"limo3_rad = {self.limo3_rad!r}", f"ppm_const = {self.ppm_const!r}",
             f"primitive_solver = {self.primitive_solver}",
+            f"mlp_limiting = {self.mlp_limiting}",
+            f"fix_unphysical_cells = {self.fix_unphysical_cells}",
+            f"glm_source_terms = {self.glm_source_terms}",
+            f"viscosity = {self.viscosity!r}", f"resistivity = {self.resistivity!r}",
             "minlev = 1", "maxlev = 1",
             f"xlbndry = {self.xlbndry}", f"xubndry = {self.xubndry}",
             f"ylbndry = {self.ylbndry}", f"yubndry = {self.yubndry}",

@@ -105,23 +105,13 @@ static int bnd_fill(amun_ctx* c, int qcur) {
 namespace {
 
 bool pick(Dispatch& d, const amun_cfg& c) {
-  switch (c.reconstruction) {
-  case AMUN_REC_TVD: return pick_tvd(d, c);
-  case AMUN_REC_WENO5Z: return pick_weno5z(d, c);
-  case AMUN_REC_WENO5YC: return pick_weno5yc(d, c);
-  case AMUN_REC_WENO5NS: return pick_weno5ns(d, c);
-  case AMUN_REC_MP5: return pick_mp5(d, c);
-  case AMUN_REC_MP7: return pick_mp7(d, c);
-  case AMUN_REC_GP: return pick_gp(d, c);
-  case AMUN_REC_WENO3: return pick_weno3(d, c);
-  case AMUN_REC_LIMO3: return pick_limo3(d, c);
-  case AMUN_REC_PPM: return pick_ppm(d, c);
-  case AMUN_REC_CRWENO5Z: case AMUN_REC_CRWENO5YC: case AMUN_REC_CRWENO5NS:
-  case AMUN_REC_CRMP5: case AMUN_REC_CRMP5LD: case AMUN_REC_CRMP7:
-    return pick_crweno(d, c);
-  case AMUN_REC_MGP: return pick_mgp(d, c);
-  default: return false;
-  }
+  // one kernel-table unit per reconstruction, equation system and precision
+  const int prec = c.precision == AMUN_PREC_MIXED ? AMUN_PREC_MIXED : AMUN_PREC_FP64;
+#define AMUN_PICK_CASE(name, rec, eq, pr) \
+  if (c.reconstruction == (rec) && c.eqsys == (eq) && prec == (pr)) return name(d, c);
+  AMUN_INST_LIST(AMUN_PICK_CASE)
+#undef AMUN_PICK_CASE
+  return false;
 }
 
 int validate(const amun_cfg& c, int* nv) {
@@ -163,6 +153,13 @@ int validate(const amun_cfg& c, int* nv) {
     return fail(500, "unsupported precision");
   if (c.fix_positivity || c.clip_extrema)
     return fail(600, "fix_positivity / clip_extrema are not supported on the device path");
+  if (c.mlp_limiting) return fail(600, "mlp_limiting is not supported on the device path");
+  if (c.fix_unphysical_cells)
+    return fail(600, "fix_unphysical_cells is not supported on the device path");
+  if (c.glm_source_terms)
+    return fail(600, "glm_source_terms eglm/heglm are not supported on the device path");
+  if (c.viscosity != 0.0 || c.resistivity != 0.0)
+    return fail(600, "viscosity / resistivity sources are not supported on the device path");
   if (c.ncells < 4 || c.ncells > 64 || c.ncells % 4)
     return fail(500, "ncells must be a multiple of 4 in [4, 64] (4x4 brick layout)");
   // interior formulas at the faces of cells -1 and ncells (the reference's

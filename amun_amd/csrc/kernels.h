@@ -1571,18 +1571,21 @@ bool pick_nd(Dispatch& d, int nd) {
   return false;
 }
 
-template <int REC, typename T>
-bool pick_rec_t(Dispatch& d, const amun_cfg& c) {
-  if (c.eqsys == AMUN_EQSYS_HD) {
+// kernel table of one reconstruction, equation system and precision; each
+// combination is its own instantiation unit (inst_unit.hip compiled once per
+// entry of inst_list.h), so the units build in parallel
+template <int REC, int EQ, typename T>
+bool pick_eq(Dispatch& d, const amun_cfg& c) {
+  if constexpr (EQ == AMUN_EQSYS_HD) {
     if (c.riemann_solver == AMUN_RS_HLL) return pick_nd<HD, RS_HLL, REC, T>(d, c.ndims);
     if (c.riemann_solver == AMUN_RS_HLLC) return pick_nd<HD, RS_HLLC, REC, T>(d, c.ndims);
     if (c.riemann_solver == AMUN_RS_ROE) return pick_nd<HD, RS_ROE, REC, T>(d, c.ndims);
-  } else if (c.eqsys == AMUN_EQSYS_MHD) {
+  } else if constexpr (EQ == AMUN_EQSYS_MHD) {
     if (c.riemann_solver == AMUN_RS_HLL) return pick_nd<MHD, RS_HLL, REC, T>(d, c.ndims);
     if (c.riemann_solver == AMUN_RS_HLLD) return pick_nd<MHD, RS_HLLD, REC, T>(d, c.ndims);
     if (c.riemann_solver == AMUN_RS_HLLC) return pick_nd<MHD, RS_HLLC, REC, T>(d, c.ndims);
     if (c.riemann_solver == AMUN_RS_ROE) return pick_nd<MHD, RS_ROE, REC, T>(d, c.ndims);
-  } else if (c.eqsys == AMUN_EQSYS_SRMHD) {
+  } else if constexpr (EQ == AMUN_EQSYS_SRMHD) {
     // "hllc" selects riemann_srmhd_adi_hllc, every other name HLL
     // (schemes.F90:479-498)
     if (c.riemann_solver == AMUN_RS_HLLC) return pick_nd<SRMHD, RS_HLLC, REC, T>(d, c.ndims);
@@ -1591,24 +1594,9 @@ bool pick_rec_t(Dispatch& d, const amun_cfg& c) {
   return false;
 }
 
-// kernel table of one reconstruction (instantiated by inst_<rec>.hip)
-template <int REC>
-bool pick_rec(Dispatch& d, const amun_cfg& c) {
-  return c.precision == AMUN_PREC_MIXED ? pick_rec_t<REC, float>(d, c)
-                                        : pick_rec_t<REC, double>(d, c);
-}
-
-bool pick_tvd(Dispatch& d, const amun_cfg& c);
-bool pick_weno5z(Dispatch& d, const amun_cfg& c);
-bool pick_weno5yc(Dispatch& d, const amun_cfg& c);
-bool pick_weno5ns(Dispatch& d, const amun_cfg& c);
-bool pick_mp5(Dispatch& d, const amun_cfg& c);
-bool pick_mp7(Dispatch& d, const amun_cfg& c);
-bool pick_gp(Dispatch& d, const amun_cfg& c);
-bool pick_weno3(Dispatch& d, const amun_cfg& c);
-bool pick_limo3(Dispatch& d, const amun_cfg& c);
-bool pick_ppm(Dispatch& d, const amun_cfg& c);
-bool pick_crweno(Dispatch& d, const amun_cfg& c);
-bool pick_mgp(Dispatch& d, const amun_cfg& c);
+#define AMUN_PICK_DECL(name, rec, eq, prec) bool name(Dispatch& d, const amun_cfg& c);
+#include "inst_list.h"
+AMUN_INST_LIST(AMUN_PICK_DECL)
+#undef AMUN_PICK_DECL
 
 }  // namespace amun_dev

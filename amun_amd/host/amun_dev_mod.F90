@@ -59,6 +59,9 @@ module amun_dev_mod
     integer(c_int32_t) :: prol_limiter, primitive_solver
     real(c_double)     :: limo3_rad, ppm_const
     integer(c_int32_t) :: bnd_type(6)
+    integer(c_int32_t) :: mlp_limiting, fix_unphysical_cells
+    integer(c_int32_t) :: glm_source_terms, reserved_cfg
+    real(c_double)     :: viscosity, resistivity
   end type amun_cfg
 
   interface

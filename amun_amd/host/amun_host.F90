@@ -208,6 +208,21 @@ program amun_host
   cfg%stages    = get_int("stages", 2)
   cfg%limo3_rad = get_real("limo3_rad", 0.5d0)
   cfg%ppm_const = get_real("ppm_const", 1.25d0)
+  cfg%fix_positivity       = on_of(get_str("fix_positivity", "off"))
+  cfg%clip_extrema         = on_of(get_str("clip_extrema", "off"))
+  cfg%mlp_limiting         = on_of(get_str("mlp_limiting", "off"))
+  cfg%fix_unphysical_cells = on_of(get_str("fix_unphysical_cells", "off"))
+  str = get_str("glm_source_terms", "none")
+  select case(trim(str))
+  case("eglm", "EGLM")
+    cfg%glm_source_terms = 1
+  case("heglm", "HEGLM")
+    cfg%glm_source_terms = 2
+  case default
+    cfg%glm_source_terms = 0
+  end select
+  cfg%viscosity   = get_real("viscosity", 0.0d0)
+  cfg%resistivity = get_real("resistivity", 0.0d0)
   nmax    = get_int("nmax", 10)
   tmax    = get_real("tmax", 1.0d30)
   seed    = int(get_int("seed", 20261202), int64)
@@ -361,6 +376,18 @@ contains
       if (trim(pnames(n)) == name) r = pvalues(n)
     end do
   end function get_str
+
+! an on/off switch of the parameter file
+  function on_of(name) result(t)
+    character(len=*), intent(in) :: name
+    integer(c_int32_t)           :: t
+    select case(trim(name))
+    case("on", "ON", "On", "yes", "YES", "y", "Y", "t", "T", "true", ".true.", "1")
+      t = 1
+    case default
+      t = 0
+    end select
+  end function on_of
 
 ! the boundary type names of boundaries.F90:170-260
   function bnd_of(name) result(t)

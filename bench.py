@@ -16,9 +16,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
-import subprocess
 import sys
-import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -43,37 +41,55 @@ def ensure_built():
         __graft_entry__.build()
 
 
-def cpu_baseline(p_full, seconds_hint: float = 15.0):
-    """Time the reference binary (oracle/_ref, one core) -- or the C oracle
-    port if the reference binary is absent -- on a 64^3 sample of the same
-    configuration."""
-    from amun_amd.config import c2_params
+def _cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _time_oracle(p, nsteps):
+    """Seconds of nsteps steps of the C oracle (one host thread) from the
+    seeded initial state of p."""
     from amun_amd.problems import initial_state
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as ol
 
+    o = ol.OracleRun(p)
+    o.set_state(initial_state(p))
+    t0 = time.perf_counter()
+    o.run(nsteps)
+    return time.perf_counter() - t0
+
+
+def cpu_baseline(p_full):
+    """The C oracle port of the reference update (one host thread) on a 64^3
+    sample of the same 3D MHD configuration, 2 steps, and on the whole 2D HD
+    128^2 x 200-step configuration beside it."""
+    from amun_amd.config import c1_params, c2_params
+
     p = c2_params(64, precision="fp64")
-    u0 = initial_state(p)
-    cells = p.ncells_total
     nsteps = 2
-    if ol.ref_available(3):
-        with tempfile.TemporaryDirectory() as d:
-            u0.tofile(os.path.join(d, "in.bin"))
-            ol._ref_exec(p, {"harness_mode": "run", "harness_input": "in.bin",
-                             "harness_output": "out", "harness_steps": nsteps}, d)
-            secs = float(open(os.path.join(d, "out_time.txt")).read().split()[0])
-        kind = "reference"
-    else:
-        o = ol.OracleRun(p)
-        o.set_state(u0)
-        t0 = time.perf_counter()
-        o.run(nsteps)
-        secs = time.perf_counter() - t0
-        kind = "port"
-    return {"value": cells * nsteps / secs, "unit": "zone-updates/s", "cores": 1,
-            "kind": kind,
+    secs = _time_oracle(p, nsteps)
+    p1 = c1_params()
+    n1 = 200
+    secs1 = _time_oracle(p1, n1)
+    try:
+        nproc = len(os.sched_getaffinity(0))
+    except AttributeError:
+        nproc = os.cpu_count()
+    return {"value": p.ncells_total * nsteps / secs, "unit": "zone-updates/s", "cores": 1,
+            "kind": "port",
             "sample": f"64^3 (64 blocks of 16^3), {nsteps} SSPRK(3,3) steps, FP64, "
-                      f"{secs:.2f} s single-threaded on the host CPU"}
+                      f"{secs:.2f} s on one host thread",
+            "cpu_model": _cpu_model(), "nproc": nproc,
+            "hd2d_128": {"value": p1.ncells_total * n1 / secs1, "unit": "zone-updates/s",
+                         "cores": 1,
+                         "sample": f"2D HD TVD-MC+HLLC SSPRK(2,2) 128^2 (64 blocks of 16^2), "
+                                   f"{n1} steps, FP64, {secs1:.2f} s on one host thread"}}
 
 
 def main():
@@ -171,6 +187,20 @@ def main():
                     "frac": ach / FP64_VEC_TFLOPS, "dtype": "f64",
                     "flop_per_zone_update": prof["fp64_flop_per_step"] / cells}
     value = cells * args.steps / wall
+    # the bound of the step: the larger of its HBM time (algorithmic bytes at
+    # the HBM peak) and its VALU time (counted FP64 flops at the FP64 vector
+    # peak); both fractions are reported
+    hbm = {"achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": achieved / HBM_PEAK_GBS}
+    roofline = {"bound": "hbm", **hbm}
+    if valu is not None and valu["achieved"] / valu["peak"] > hbm["frac"]:
+        roofline = {"bound": "valu", "achieved": valu["achieved"], "peak": valu["peak"],
+                    "unit": valu["unit"], "frac": valu["frac"], "hbm": hbm}
+    roofline.update({"traffic": traffic,
+                     "kernel": "one SSPRK(3,3) step = 9 k_sweep launches + k_end_step",
+                     "bytes_per_zone_update": bpz,
+                     "device_ms_per_step": step_ms,
+                     "kernel_ms_per_step": [round(float(x), 4) for x in kms]})
     out = {
         "metric": "zone-updates/s (3D MHD WENO5+HLLD 256^3) FP64 vs mixed, % HBM roofline",
         "value": value,
@@ -187,13 +217,7 @@ def main():
         "config": {"workload": f"3D MHD-GLM WENO5-Z+HLLD SSPRK(3,3) {args.size}^3 "
                                f"as {p.nblocks} blocks of 16^3, CFL 0.3, periodic",
                    "precision": args.precision, "cells": cells},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic,
-                     "kernel": "one SSPRK(3,3) step = 9 k_sweep launches + k_end_step",
-                     "bytes_per_zone_update": bpz,
-                     "device_ms_per_step": step_ms,
-                     "kernel_ms_per_step": [round(float(x), 4) for x in kms]},
+        "roofline": roofline,
     }
     if valu is not None:
         out["valu"] = valu

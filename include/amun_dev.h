@@ -144,6 +144,19 @@ typedef struct amun_cfg {
    * (coordinates.F90, boundaries.F90:1089-1251, 5024-5637): AMUN_BND_*;
    * an axis is periodic on both sides or on neither */
   int32_t bnd_type[6];
+  /* terms outside the device path; amun_dev_init refuses any value other
+   * than the default (off / none / 0) with code 600:
+   * "mlp_limiting" of TVD and MGP (interpolations.F90:204, 844, 1140),
+   * "fix_unphysical_cells" of update_primitive_variables (equations.F90:905,
+   * 1168), "glm_source_terms" eglm/heglm = 1/2 (sources.F90:128-141), and the
+   * "viscosity" and "resistivity" sources with their dt term
+   * (sources.F90:145-149, evolution.F90:478-483) */
+  int32_t mlp_limiting;
+  int32_t fix_unphysical_cells;
+  int32_t glm_source_terms;
+  int32_t reserved_cfg;
+  double  viscosity;
+  double  resistivity;
 } amun_cfg;
 
 typedef struct amun_ctx amun_ctx;

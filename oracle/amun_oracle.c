@@ -78,6 +78,10 @@ typedef struct {
   double cgp[16];          /* Gaussian-process weights                      */
   int    dgp;              /* ngp^nd, the MGP stencil                        */
   double ugp[750];         /* MGP weights [dir][side][stencil], ngp <= 5     */
+  /* cells where interfaces_mgp fell back to the TVD slopes, and a hash of
+   * their positions in visiting order (which fallbacks happened, not only how
+   * many): a record for tests, no effect on the update */
+  unsigned long long mgp_fallbacks, mgp_hash;
 } ocfg;
 
 static int prepare_gp(ocfg* o);
@@ -1136,6 +1140,9 @@ static void reconstruct_interfaces(ocfg* o, const double* q, double* qs)
               fl = fl || ((fv[4] - q[QI(o, p, i, j, k + 1)]) * (fv[4] - qc) > o->c.eps) ||
                    ((fv[5] - q[QI(o, p, i, j, k - 1)]) * (fv[5] - qc) > o->c.eps);
             if (fl) {
+              o->mgp_fallbacks++;
+              o->mgp_hash = o->mgp_hash * 1000003ull +
+                            (unsigned long long)(1 + i + 64 * (j + 64 * (k + 64 * p)));
               dql = qc - q[QI(o, p, i - 1, j, k)];
               dqr = q[QI(o, p, i + 1, j, k)] - qc;
               dq[0] = lim(o->c.limiter, 0.5, dql, dqr);
@@ -3798,6 +3805,13 @@ int orc_run(void* h, int nsteps, double dtnext, double* trace)
 }
 
 long orc_positivity_events(void* h) { return ((ogrid*)h)->o.npos_events; }
+
+/* the MGP fallback record so far: count and position hash */
+void orc_mgp_fallbacks(void* h, unsigned long long* out)
+{
+  out[0] = ((ogrid*)h)->o.mgp_fallbacks;
+  out[1] = ((ogrid*)h)->o.mgp_hash;
+}
 
 /* messages of the relativistic primitive solver so far: bracketing failed,
  * non-positive pressure, no convergence */

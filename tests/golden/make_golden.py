@@ -614,7 +614,39 @@ def main_crweno(schemes=CR_SCHEMES, outname="unit_schemes_cr"):
 
 
 MGP_CASES = (("mgp_otnoisy2d_32_4", 2, 5, 4), ("mgp3_otnoisy2d_32_4", 2, 3, 2),
-             ("mgp_otnoisy3d_16_4", 3, 5, 3))
+             ("mgp_otnoisy3d_16_4", 3, 5, 3), ("mgp_noisy3d_16_4", 3, 5, 3))
+
+
+def noisy3d(p, seed=6):
+    """A fully three-dimensional MHD state: smooth O(0.1-1) fields varying
+    along x, y and z in every primitive variable, psi included, plus 2 %
+    seeded noise on each.  The extruded Orszag-Tang state leaves vz, Bz and
+    psi at round-off size in 3D, where interfaces_mgp's monotonicity products
+    sit at its eps threshold and the TVD fallback flips with the last bit of
+    the face sums (an FMA-contracted build of the same code takes other
+    branches); here every product is far from it."""
+    nz, ny, nx = p.shape
+    z, y, x = problems.cell_centres(p)
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    xi = [problems.uniform(seed, nz * ny * nx, stream=s).reshape(nz, ny, nx) - 0.5
+          for s in range(30, 39)]
+    tp = 2.0 * np.pi
+    a = 0.02
+    q = np.zeros((p.nv, nz, ny, nx))
+    q[0] = 1.0 + 0.2 * np.sin(tp * (X + Y + Z)) + a * xi[0]
+    q[1] = 0.5 * np.sin(tp * Y) + 0.3 * np.cos(tp * Z) + a * xi[1]
+    q[2] = 0.5 * np.sin(tp * Z) + 0.3 * np.cos(tp * X) + a * xi[2]
+    q[3] = 0.5 * np.sin(tp * X) + 0.3 * np.cos(tp * Y) + a * xi[3]
+    q[4] = 0.3 * np.sin(tp * (Y + Z)) + 0.2 + a * xi[4]
+    q[5] = 0.3 * np.sin(tp * (Z + X)) - 0.1 + a * xi[5]
+    q[6] = 0.3 * np.sin(tp * (X + Y)) + 0.15 + a * xi[6]
+    q[7] = a * xi[7]
+    q[8] = 1.0 + 0.1 * np.cos(tp * (X - Y + Z)) + a * xi[8]
+    return problems.prim_to_cons(p, q)
+
+
+def mgp_state(name, p):
+    return noisy3d(p) if name.startswith("mgp_noisy3d") else ot_noisy(p)
 
 
 def mgp_params(nd, ngp, ng):
@@ -633,7 +665,7 @@ def main_mgp():
         index = json.load(fh)
     for name, nd, ngp, ng in MGP_CASES:
         p = mgp_params(nd, ngp, ng)
-        u0 = ot_noisy(p)
+        u0 = mgp_state(name, p)
         nsteps = 4
         u, tr = ol.ref_run(p, u0, nsteps)
         sub = u[:, ::3, ::3, ::3] if nd == 3 else u[:, :, ::3, ::3]

@@ -13,7 +13,7 @@ import numpy as np
 from amun_amd.config import Params
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+ORACLE_SO = os.environ.get("AMUN_ORACLE_SO", os.path.join(ROOT, "oracle", "liboracle.so"))
 REF_DIR = os.path.join(ROOT, "oracle", "_ref")
 
 _P = ctypes.POINTER(ctypes.c_double)
@@ -40,6 +40,7 @@ def lib():
         L.orc_statistics.argtypes = [ctypes.c_void_p, _P]
         L.orc_positivity_events.argtypes = [ctypes.c_void_p]
         L.orc_positivity_events.restype = ctypes.c_long
+        L.orc_mgp_fallbacks.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_ulonglong)]
         L.orc_update_flux.argtypes = [ctypes.c_void_p, _P, _P, ctypes.c_double, _P]
         L.orc_reconstruct.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                       ctypes.c_double, _P, _P, _P]
@@ -102,6 +103,13 @@ class OracleRun:
 
     def positivity_events(self) -> int:
         return lib().orc_positivity_events(self.h)
+
+    def mgp_fallbacks(self):
+        """(count, position hash) of the cells where interfaces_mgp took the
+        TVD slopes so far."""
+        out = (ctypes.c_ulonglong * 2)()
+        lib().orc_mgp_fallbacks(self.h, out)
+        return int(out[0]), int(out[1])
 
     def c2p_events(self):
         ev = (ctypes.c_long * 3)()

@@ -65,6 +65,24 @@ def test_config_errors(field, value, code):
     assert lib.amun_dev_last_error()
 
 
+@pytest.mark.parametrize("key,value", [
+    ("mlp_limiting", "on"), ("fix_unphysical_cells", "on"), ("glm_source_terms", "eglm"),
+    ("glm_source_terms", "heglm"), ("viscosity", "1.0e-3"), ("resistivity", "1.0e-3"),
+])
+def test_unsupported_terms_are_refused(key, value):
+    """Parameters of terms outside the device path (MLP limiting, the
+    unphysical-cell fix, the EGLM/HEGLM and viscous/resistive sources) are
+    carried into amun_cfg and refused with 600 instead of being ignored."""
+    p = Params.from_text(f"reconstruction = tvd\nriemann_solver = hllc\n{key} = {value}\n",
+                         ndims=2)
+    assert key not in p.extra
+    cfg = p.to_cfg()
+    lib = solver.load_library()
+    h = ctypes.c_void_p()
+    assert lib.amun_dev_init(ctypes.byref(cfg), ctypes.byref(h)) == 600
+    assert key.split("_")[0].encode() in lib.amun_dev_last_error()
+
+
 def test_weno_needs_four_ghosts():
     p = c2_params(32)
     p.nghosts = 2

@@ -632,13 +632,7 @@ def test_more_schemes_one_step(nd, eq, rs, rec):
                                   "crweno5ns_otnoisy2d_32_8", "crmp5_otnoisy2d_32_8",
                                   "crmp5ld_otnoisy2d_32_8", "crmp7_otnoisy2d_32_8",
                                   "mgp_otnoisy2d_32_4", "mgp3_otnoisy2d_32_4",
-                                  pytest.param("mgp_otnoisy3d_16_4", marks=pytest.mark.xfail(
-                                      strict=False,
-                                      reason="3D MGP: one step agrees to 1e-12 and the 4-step "
-                                             "dt trace to 1e-12, but the final state differs "
-                                             "from the reference by 4.7e-6 relative; cause not "
-                                             "yet shown (a GP/TVD switch at the eps threshold "
-                                             "is suspected)"))])
+                                  "mgp_noisy3d_16_4"])
 def test_schemes_golden_runs(name):
     """Eight Orszag-Tang steps against the reference's own output (dt trace
     and a subsample of the final state).  PPM runs on the noisy variant: its
@@ -653,8 +647,8 @@ def test_schemes_golden_runs(name):
     from amun_amd.problems import orszag_tang_2d
     if "noisy" in name:
         sys.path.insert(0, gold)
-        from make_golden import ot_noisy
-        u0 = ot_noisy(p)
+        from make_golden import mgp_state, ot_noisy
+        u0 = mgp_state(name, p) if name.startswith("mgp") else ot_noisy(p)
     else:
         u0 = orszag_tang_2d(p, seed=6)
     print(f"[case] {name}", flush=True)

@@ -403,18 +403,19 @@ def test_roe_mhd_run_bit_exact():
 
 # ------------------------------------------- multi-dimensional GP (MGP)
 @pytest.mark.parametrize("name", ["mgp_otnoisy2d_32_4", "mgp3_otnoisy2d_32_4",
-                                  "mgp_otnoisy3d_16_4"])
+                                  "mgp_otnoisy3d_16_4", "mgp_noisy3d_16_4"])
 def test_mgp_runs(name):
     """prepare_mgp (quad-precision covariance inverse over the ngp^nd
     stencil) and interfaces_mgp (GP face values, TVD fallback where one
     leaves its cells' range): 4-step Orszag-Tang runs (noise on every field),
-    2D at ngp = 5 and 3, 3D at ngp = 5, bit for bit with the reference."""
+    2D at ngp = 5 and 3, 3D at ngp = 5, and a fully three-dimensional noisy
+    state in 3D, bit for bit with the reference."""
     import sys
     sys.path.insert(0, GOLD)
-    from make_golden import ot_noisy
+    from make_golden import mgp_state
     r = load(name)
     p = params_of(r)
-    u0 = ot_noisy(p)
+    u0 = mgp_state(name, p)
     assert sha(u0) == str(r["sha_u0"])
     o = ol.OracleRun(p)
     o.set_state(u0)
