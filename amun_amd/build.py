@@ -117,7 +117,12 @@ def build_reference():
     if not os.path.exists(os.path.join(ref, "src", "evolution.F90")):
         return None
     _run(["sh", os.path.join(ORACLE, "build_ref.sh"), ref])
-    return os.path.join(ORACLE, "_ref")
+    out = os.path.join(ORACLE, "_ref")
+    missing = [x for x in ("amun_ref_2d.x", "amun_ref_3d.x")
+               if not os.path.exists(os.path.join(out, x))]
+    if missing:
+        raise RuntimeError(f"reference build finished without {', '.join(missing)}")
+    return out
 
 
 def build_all():

@@ -27,6 +27,8 @@ struct amun_ctx {
   double* trace = nullptr;  // dt ring, [tsteps][nruns]
   int tsteps = 0;
   double* scratch = nullptr;   // reductions / small transfers
+  SpeedPart* spart = nullptr;  // per-workgroup maximum speeds of the last sweep
+  int nspart = 0;              // entries of spart (workgroups of >= 64 lanes)
   double* ugp = nullptr;       // MGP weights (RunConst::ugp points here)
   int qcur = 0;                // Q[qcur] holds the primitive state of U0
   long long nsteps = 0;        // steps launched (slot = nsteps & 1)
@@ -316,7 +318,8 @@ int validate_grid(const amun_cfg& c, const amun_grid& g) {
 // sweeps: primitive slots, increment slots, kLdsLanes lanes each)
 size_t sweep_lds(const amun_ctx* c, int dir, int /*wg*/) {
   const bool chunk = dir < 2, last = dir == c->rc.nd - 1;
-  const bool din = chunk && dir > 0, dout = chunk && !last;
+  const bool chunk_d = c->qsize == 8;   // sweep_chunk_d<T>()
+  const bool din = chunk && dir > 0 && chunk_d, dout = chunk && !last && chunk_d;
   const int nbuf = (chunk ? 1 : 0) + ((din || dout) ? 1 : 0);
   return (size_t)nbuf * 4 * c->nv * kLdsLanes * c->qsize;
 }
@@ -516,6 +519,11 @@ static int init_device(amun_ctx* c, const std::vector<int32_t>& nbr,
   ALLOC(c->rs, (size_t)nruns * sizeof(RunState));
   ALLOC(c->trace, (size_t)nruns * c->tsteps * sizeof(double));
   ALLOC(c->scratch, (size_t)std::max(4096 * 8, 2 * nruns) * sizeof(double));
+  {
+    const long long npen = nd == 3 ? (long long)rc.nc * rc.nc : rc.nc;
+    c->nspart = (int)((rc.nblk * npen + 63) / 64);
+    ALLOC(c->spart, (size_t)c->nspart * sizeof(SpeedPart));
+  }
   if (is_compact(cfg->reconstruction)) {
     // face states of one direction: [block][var][cell -1 .. nc][pencil]
     const size_t npen = rc.nd == 3 ? (size_t)rc.nc * rc.nc : (size_t)rc.nc;
@@ -892,6 +900,7 @@ void amun_dev_finalize(amun_ctx* c) {
   hipFree(c->rs);
   hipFree(c->trace);
   hipFree(c->scratch);
+  hipFree(c->spart);
   hipFree(c->rc.fsl);
   hipFree(c->ugp);
   hipFree(c->rc.fsr);
@@ -1257,6 +1266,7 @@ static int enqueue_step(amun_ctx* c, double dtnext, int slot, int qcur, hipEvent
     a.st = c->st;
     a.rs = c->rs;
     a.brun = c->brun;
+    a.spart = c->spart;
     a.slot = slot;
     a.kind = stages[s].kind;
     a.last = (s + 1 == stages.size());
@@ -1271,11 +1281,14 @@ static int enqueue_step(amun_ctx* c, double dtnext, int slot, int qcur, hipEvent
       if (d.compact[dir]) {
         const long long nt = (long long)rc.nblk * (rc.nd == 3 ? rc.nc * rc.nc : rc.nc) * c->nv;
         hipLaunchKernelGGL(d.compact[dir], dim3((unsigned)((nt + 63) / 64)), dim3(64), 0,
-                           c->stream, (const void*)c->Q[qcur], c->nbr, rc);
+                           c->stream, (const void*)c->Q[qcur], c->nbr, rc, c->st);
       }
       a.swz = c->swz[dir];
       const dim3 grid((unsigned)((nthreads + wg - 1) / wg)), block(wg);
       hipLaunchKernelGGL(d.sweep[dir], grid, block, sweep_lds(c, dir, wg), c->stream, a, rc);
+      if (a.last && dir == rc.nd - 1)
+        hipLaunchKernelGGL(k_reduce_speed, dim3(1), dim3(1024), 0, c->stream, c->spart,
+                           (int)grid.x, c->rs, slot ^ 1, rc.nruns);
       if (ev) HIP_TRY(hipEventRecord(*ev++, c->stream));
     }
     qcur ^= 1;
@@ -1696,6 +1709,15 @@ int amun_dev_positivity_events(amun_ctx* c, uint64_t* n) {
   DevState h;
   HIP_TRY(hipMemcpy(&h, c->st, sizeof(DevState), hipMemcpyDeviceToHost));
   *n = h.npos;
+  return 0;
+}
+
+int amun_dev_tridiag_events(amun_ctx* c, uint64_t* n) {
+  if (!c || !n) return fail(500, "null argument");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  DevState h;
+  HIP_TRY(hipMemcpy(&h, c->st, sizeof(DevState), hipMemcpyDeviceToHost));
+  *n = h.tridiag;
   return 0;
 }
 

@@ -71,6 +71,7 @@ constexpr int kTrace = 1 << 16;                      // dt trace entries (all ru
 struct DevState {
   unsigned long long npos;          // non-positive density/pressure events
   unsigned long long c2p[3];        // SR primitive solver: failed, p <= 0, no convergence
+  unsigned long long tridiag;       // compact schemes: pencils with a zero Thomas pivot
 };
 
 // step state of one run (one independent grid of a batch; a single run is a
@@ -216,6 +217,13 @@ __device__ __forceinline__ void riemann(const EosConst<T>& e, const SrConst& sc,
 // lanes of the per-lane LDS slots of the chunked (x and y) sweeps
 constexpr int kLdsLanes = 256;
 
+// maximum speed of one sweep workgroup for the next CFL step, and its run
+// (run < 0: the workgroup spanned several runs and used atomics instead)
+struct SpeedPart {
+  double smax;
+  int run, pad;
+};
+
 struct SweepArgs {
   const void* Q;                   // primitive state of the stage input (T)
   const double* U0;                // state at the start of the step
@@ -228,6 +236,7 @@ struct SweepArgs {
   DevState* st;                    // event counters
   RunState* rs;                    // per-run step state
   const int* __restrict__ brun;    // run of every block
+  SpeedPart* spart;                // per-workgroup maximum speeds (last stage)
   int slot;                        // cmax slot of this step
   int swz;                         // XCD-aware workgroup order (autotuned)
   int kind;                        // stage combine, see combine()
@@ -487,7 +496,7 @@ constexpr int kMaxPencil = 96;
 // Thomas algorithm with the rows built on the fly.  Writes the states at the
 // faces of the cells -1 .. nc: ol[c + 1] upper, or[c + 1] lower.
 template <int REC, typename T>
-__device__ void crweno_pencil(int n, int ng, const T* f, T eps, T* ol, T* orr) {
+__device__ bool crweno_pencil(int n, int ng, const T* f, T eps, T* ol, T* orr) {
   const T c1 = T(13.0 / 12.0), c2 = T(0.25), cl = T(1.0 / 9.0), cc = T(5.0 / 9.0),
           cr = T(1.0 / 3.0), dl = T(0.1), dc = T(0.6), dr = T(0.3), dq = T(0.5), xi = T(0.4);
   const T a11 = T(2.0 / 6.0), a12 = T(-7.0 / 6.0), a13 = T(11.0 / 6.0);
@@ -576,17 +585,17 @@ __device__ void crweno_pencil(int n, int ng, const T* f, T eps, T* ol, T* orr) {
     T t = b;
     x[0] = r / t;
     cprev = c;
-    int i = 1;
-    for (; i < n; i++) {
+    for (int i = 1; i < n; i++) {
       row(i, s, a, b, c, r);
       g[i] = cprev / t;
       t = b - a * g[i];
-      if (t == T(0)) break;
+      // a zero pivot: tridiag prints "Could not find solution!" and leaves
+      // the rest of the solution undefined (algebra.F90:1145-1152)
+      if (t == T(0)) return false;
       x[i] = (r - a * x[i - 1]) / t;
       cprev = c;
     }
-    if (i == n)
-      for (int m = n - 2; m >= 0; m--) x[m] = x[m] - g[m + 1] * x[m + 1];
+    for (int m = n - 2; m >= 0; m--) x[m] = x[m] - g[m + 1] * x[m + 1];
     // cells -1 .. nc of the block: pencil cells l = ng - 1 .. n - ng; the
     // left system solves the upper-face state of cell l, the right one (as
     // fr(1:n-1) = u(2:n)) the lower-face state of cell l at row l as well
@@ -595,6 +604,7 @@ __device__ void crweno_pencil(int n, int ng, const T* f, T eps, T* ol, T* orr) {
       else orr[cidx] = x[ng - 1 + cidx];
     }
   }
+  return true;
 }
 
 // reconstruct_crmp5 / _crmp5ld / _crmp7 (interpolations.F90:3930-4427) on
@@ -605,7 +615,7 @@ __device__ void crweno_pencil(int n, int ng, const T* f, T eps, T* ol, T* orr) {
 // right states solve the same system on the mirrored pencil.  Writes ol / or
 // as crweno_pencil does.
 template <int REC, typename T>
-__device__ void crmp_pencil(int n, int ng, const T* f, T kappa, T kbeta, T eps, T* ol,
+__device__ bool crmp_pencil(int n, int ng, const T* f, T kappa, T kbeta, T eps, T* ol,
                             T* orr) {
   constexpr int V = REC == REC_CRMP5 ? 0 : (REC == REC_CRMP5LD ? 1 : 2);
   const T d0 = V == 0 ? T(0.3) : (V == 1 ? T(0.25) : T(2.0 / 7.0));
@@ -644,17 +654,15 @@ __device__ void crmp_pencil(int n, int ng, const T* f, T kappa, T kbeta, T eps, 
     auto inner = [&](int i) { return i >= ng && i <= n - ng - 2; };
     T t = T(1), cprev = T(0);
     x[0] = rhs(0);
-    int i = 1;
-    for (; i < n; i++) {
+    for (int i = 1; i < n; i++) {
       const T a = inner(i) ? d0 : T(0), b = inner(i) ? d1 : T(1);
       g[i] = cprev / t;
       t = b - a * g[i];
-      if (t == T(0)) break;
+      if (t == T(0)) return false;   // as in crweno_pencil
       x[i] = (rhs(i) - a * x[i - 1]) / t;
       cprev = inner(i) ? d2 : T(0);
     }
-    if (i == n)
-      for (int m = n - 2; m >= 0; m--) x[m] = x[m] - g[m + 1] * x[m + 1];
+    for (int m = n - 2; m >= 0; m--) x[m] = x[m] - g[m + 1] * x[m + 1];
     // the limited face values the block needs: system 0 at pencil cells
     // l = ng - 1 .. n - ng (upper faces), system 1 at the mirrored rows
     // n - 1 - l (lower faces, fr(1:n-1) = u(n-1:1:-1))
@@ -671,13 +679,14 @@ __device__ void crmp_pencil(int n, int ng, const T* f, T kappa, T kbeta, T eps, 
       else orr[cidx] = u;
     }
   }
+  return true;
 }
 
 // the face states of one direction for the compact schemes: one thread per
 // (block, pencil, variable) gathers the pencil with ng ghosts each side
 template <class P, int REC, int ND, int DIR, typename T>
 __global__ __launch_bounds__(64) void k_compact(const void* Qv, const int* __restrict__ nbr,
-                                                RunConst rc) {
+                                                RunConst rc, DevState* st) {
   constexpr int NV = P::nv;
   const int nc = rc.nc, ng = rc.ng, n = nc + 2 * ng, npen = ND == 3 ? nc * nc : nc;
   const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -693,10 +702,18 @@ __global__ __launch_bounds__(64) void k_compact(const void* Qv, const int* __res
     const int k = DIR == 2 ? c : t2;
     f[m] = Q[var_index<NV>(rc, cell_index<sizeof(T)>(nbr, rc, b, i, j, k), v)];
   }
+  bool ok;
   if constexpr (REC == REC_CRMP5 || REC == REC_CRMP5LD || REC == REC_CRMP7)
-    crmp_pencil<REC, T>(n, ng, f, T(rc.kappa), T(rc.kbeta), T(rc.eps), ol, orr);
+    ok = crmp_pencil<REC, T>(n, ng, f, T(rc.kappa), T(rc.kbeta), T(rc.eps), ol, orr);
   else
-    crweno_pencil<REC, T>(n, ng, f, T(rc.eps), ol, orr);
+    ok = crweno_pencil<REC, T>(n, ng, f, T(rc.eps), ol, orr);
+  if (!ok) {
+    // a zero pivot in the pencil's solve: counted (amun_dev_tridiag_events)
+    // and the pencil's face states set to its cell values (first order),
+    // where the reference goes on with an undefined solution
+    atomicAdd(&st->tridiag, 1ull);
+    for (int c = 0; c < nc + 2; c++) ol[c] = orr[c] = f[ng - 1 + c];
+  }
   T* L = static_cast<T*>(rc.fsl);
   T* Rr = static_cast<T*>(rc.fsr);
   for (int c = 0; c < nc + 2; c++) {
@@ -889,18 +906,110 @@ __device__ __forceinline__ void wave_max_speed(RunState* rs, int run, int slot, 
   }
 }
 
+// Maximum speed of a sweep workgroup (at most kMaxWaves waves) into
+// part[blockIdx.x], reduced over the workgroups by k_reduce_speed after the
+// sweep.  Device-scope atomics on one address from every wave of a 256^3
+// sweep (16K of them) serialise at the memory side; a workgroup whose lanes
+// belong to more than one run (a batch) still uses them.  Every thread of
+// the workgroup must call this.
+constexpr int kMaxWaves = 16;
+__device__ __forceinline__ void wg_max_speed(RunState* rs, SpeedPart* part, int run, int slot,
+                                             double smax) {
+  __shared__ double wm[kMaxWaves];
+  __shared__ int wr[kMaxWaves];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int nw = (blockDim.x + 63) >> 6;
+  const int r0 = __shfl(run, 0);
+  const bool uni = __all(run == r0);
+  if (uni) {
+    for (int off = 32; off > 0; off >>= 1) smax = fmax(smax, __shfl_xor(smax, off));
+  } else if (smax > 0.0) {
+    atomicMax(&rs[run].cmax_bits[slot], (unsigned long long)__double_as_longlong(smax));
+  }
+  if (lane == 0) {
+    wm[w] = uni ? smax : 0.0;
+    wr[w] = uni ? r0 : -1;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = wm[0];
+    bool same = wr[0] >= 0;
+    for (int i = 1; i < nw; i++) {
+      same = same && wr[i] == wr[0];
+      m = fmax(m, wm[i]);
+    }
+    if (same) {
+      part[blockIdx.x].smax = m;
+      part[blockIdx.x].run = wr[0];
+    } else {
+      part[blockIdx.x].smax = 0.0;
+      part[blockIdx.x].run = -1;
+      for (int i = 0; i < nw; i++)
+        if (wr[i] >= 0 && wm[i] > 0.0)
+          atomicMax(&rs[wr[i]].cmax_bits[slot], (unsigned long long)__double_as_longlong(wm[i]));
+    }
+  }
+}
+
+// The per-workgroup maxima of the last sweep into the runs' cmax slots (one
+// workgroup of 1024 threads; per-run maxima in LDS when the runs fit)
+constexpr int kReduceRuns = 2048;
+static __global__ __launch_bounds__(1024) void k_reduce_speed(const SpeedPart* __restrict__ part,
+                                                              int n, RunState* rs, int slot,
+                                                              int nruns) {
+  __shared__ unsigned long long lm[kReduceRuns];
+  if (nruns <= kReduceRuns) {
+    for (int r = threadIdx.x; r < nruns; r += blockDim.x) lm[r] = 0ull;
+    __syncthreads();
+    if (nruns == 1) {
+      double m = 0.0;
+      for (int i = threadIdx.x; i < n; i += blockDim.x)
+        if (part[i].run == 0) m = fmax(m, part[i].smax);
+      for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+      if ((threadIdx.x & 63) == 0 && m > 0.0)
+        atomicMax(&lm[0], (unsigned long long)__double_as_longlong(m));
+    } else {
+      for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const SpeedPart p = part[i];
+        if (p.run >= 0 && p.smax > 0.0)
+          atomicMax(&lm[p.run], (unsigned long long)__double_as_longlong(p.smax));
+      }
+    }
+    __syncthreads();
+    for (int r = threadIdx.x; r < nruns; r += blockDim.x)
+      if (lm[r]) atomicMax(&rs[r].cmax_bits[slot], lm[r]);
+  } else {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+      const SpeedPart p = part[i];
+      if (p.run >= 0 && p.smax > 0.0)
+        atomicMax(&rs[p.run].cmax_bits[slot], (unsigned long long)__double_as_longlong(p.smax));
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // the sweep kernel
 // ---------------------------------------------------------------------------
+// FP64 sweeps hold more than 256 registers per lane and run one wave per
+// SIMD (spilling to reach two is several times slower); FP32 sweeps fit two.
+template <typename T>
+constexpr int sweep_min_waves() { return sizeof(T) == 4 ? 2 : 1; }
+
+// The FP64 x and y sweeps also move the increment through LDS in chunks; the
+// FP32 ones read and write it per cell, which leaves LDS for two workgroups
+// per CU.
+template <typename T>
+constexpr bool sweep_chunk_d() { return sizeof(T) == 8; }
+
 template <class P, int REC, int RS, int ND, int DIR, typename T>
-__global__ __launch_bounds__(256) void k_sweep(SweepArgs a, RunConst rc) {
+__global__ __launch_bounds__(256, sweep_min_waves<T>()) void k_sweep(SweepArgs a, RunConst rc) {
   constexpr int NV = P::nv;
   constexpr bool kLast = (DIR == ND - 1);
   using RC = Recon<P, REC, ND, DIR, T>;
   // chunked pencil I/O along x and y (see loadq below)
   constexpr bool kChunk = DIR < 2;
-  constexpr bool kDin = kChunk && DIR > 0;       // increment read in chunks
-  constexpr bool kDout = kChunk && !kLast;       // increment written in chunks
+  constexpr bool kDin = kChunk && DIR > 0 && sweep_chunk_d<T>();    // increment read in chunks
+  constexpr bool kDout = kChunk && !kLast && sweep_chunk_d<T>();    // increment written in chunks
   // per-lane LDS slots of the chunked pencil I/O: sq holds four cells of the
   // primitive window, sd four cells of the increment (4 NV values per lane
   // each), slot-major with a fixed stride of kLdsLanes lanes (consecutive
@@ -1148,7 +1257,7 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a, RunConst rc) {
   }
 
   if constexpr (kLast) {
-    if (a.last) wave_max_speed(a.rs, run, a.slot ^ 1, smax);
+    if (a.last) wg_max_speed(a.rs, a.spart, run, a.slot ^ 1, smax);
   }
 }
 
@@ -1493,7 +1602,7 @@ using namespace amun_dev;
 // kernel instantiations selected for one configuration
 struct Dispatch {
   SweepFn sweep[3] = {nullptr, nullptr, nullptr};
-  void (*compact[3])(const void*, const int*, RunConst) = {nullptr, nullptr, nullptr};
+  void (*compact[3])(const void*, const int*, RunConst, DevState*) = {nullptr, nullptr, nullptr};
   void (*prim)(const double*, void*, RunConst, DevState*, RunState*, const int*, int, int,
                int) = nullptr;
   void (*integ)(const double*, RunConst, double*, int, int, const int*) = nullptr;

@@ -116,7 +116,7 @@ def load_library(path: str = LIB_PATH):
     L.amun_dev_stream.restype = ctypes.c_void_p
     L.amun_dev_last_error.argtypes = []
     L.amun_dev_last_error.restype = ctypes.c_char_p
-    for name in ("amun_dev_sync", "amun_dev_positivity_events"):
+    for name in ("amun_dev_sync", "amun_dev_positivity_events", "amun_dev_tridiag_events"):
         getattr(L, name).argtypes = [vp] + ([ctypes.POINTER(ctypes.c_uint64)]
                                             if name.endswith("events") else [])
     _lib = L
@@ -330,6 +330,12 @@ class Solver:
     def positivity_events(self) -> int:
         n = ctypes.c_uint64()
         self._check(self.lib.amun_dev_positivity_events(self.h, ctypes.byref(n)))
+        return n.value
+
+    def tridiag_events(self) -> int:
+        """Compact-scheme pencils whose tridiagonal solve met a zero pivot."""
+        n = ctypes.c_uint64()
+        self._check(self.lib.amun_dev_tridiag_events(self.h, ctypes.byref(n)))
         return n.value
 
     def c2p_events(self) -> list:

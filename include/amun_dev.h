@@ -324,6 +324,13 @@ int  amun_dev_positivity_events(amun_ctx* ctx, uint64_t* n);
  * ended without reaching the tolerance.  All zero for HD/MHD. */
 int  amun_dev_c2p_events(amun_ctx* ctx, uint64_t n[3]);
 
+/* Pencils of the compact reconstructions (CRWENO5, CRMP) whose Thomas solve
+ * met a zero pivot so far.  The reference's tridiag prints "Could not find
+ * solution!" there and leaves the solution undefined (algebra.F90:1145-1152);
+ * the device counts the pencil and gives it its cell values as face states
+ * (first order). */
+int  amun_dev_tridiag_events(amun_ctx* ctx, uint64_t* n);
+
 /* The context's HIP stream (a hipStream_t), so that a caller can order its
  * own device work after the solver's. */
 void* amun_dev_stream(amun_ctx* ctx);

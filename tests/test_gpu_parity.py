@@ -85,6 +85,23 @@ def test_matrix_one_step(nd, eq, rs, rec, lim, ta):
     assert relerr(ug, uo).max() <= 1e-12
 
 
+@pytest.mark.parametrize("nd", [2, 3])
+@pytest.mark.parametrize("eq,rs", [("hd", "hllc"), ("mhd", "hlld")])
+@pytest.mark.parametrize("lim", ["va", "sb", "vl", "zero"])
+def test_tvd_limiters_one_step(nd, eq, rs, lim):
+    """The TVD limiters beyond mm and mc -- vanAlbada, superbee, vanLeer and
+    the zero slope (interpolations.F90:4731-4877) -- one RK3 step against the
+    oracle, which is bit for bit with the reference for every limiter."""
+    p = Params(ndims=nd, equation_system=eq, riemann_solver=rs, reconstruction="tvd",
+               limiter=lim, time_advance="rk3", ncells=8, nghosts=2, xblocks=2, yblocks=3,
+               zblocks=2, cfl=0.3, gamma=1.4 if eq == "hd" else 5.0 / 3.0,
+               xmin=-0.5, xmax=0.5, ymin=-0.75, ymax=0.75, zmin=0.0, zmax=1.0)
+    u0 = initial_state(p) if eq == "hd" else _mhd_state(p, noise=0.01)
+    dt, cmax, ug, tr, uo = one_step(p, u0)
+    assert abs(dt - tr[0]) <= 1e-13 * tr[0]
+    assert relerr(ug, uo).max() <= (1e-11 if eq == "hd" else 1e-12)
+
+
 def _mhd_state(p, noise=0.0):
     nz, ny, nx = p.shape
     z, y, x = cell_centres(p)

@@ -8,7 +8,9 @@ from amun_amd.config import Params, c1_params, c2_params, c4_params
 from amun_amd import problems
 import oracle_lib as ol
 
-needs_ref = pytest.mark.skipif(not (ol.ref_available(2) and ol.ref_available(3)),
+# each test skips on its own when the binary of its dimension is missing
+# (oracle_lib._ref_exec); this mark only skips the module when neither is
+needs_ref = pytest.mark.skipif(not (ol.ref_available(2) or ol.ref_available(3)),
                                reason="reference binaries (oracle/_ref) not built")
 
 
