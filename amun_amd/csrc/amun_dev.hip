@@ -2081,6 +2081,71 @@ int amun_dev_tape_backward(amun_ctx* c, const double* lam, double* lam0, double*
 }  // extern "C"
 
 // ---------------------------------------------------------------------------
+// the allocation-free cons2prim: its event counters live in a static device
+// array instead of a hipMalloc'd DevState
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int kScratchWords = 32;   // 256 bytes: one DevState and room to spare
+static_assert(sizeof(DevState) <= kScratchWords * sizeof(unsigned long long),
+              "DevState must fit the static scratch array");
+static __device__ __align__(256) unsigned long long scratch[kScratchWords];
+
+// device address of scratch on every device the library has run on (the
+// array's host-side name is not a device address)
+constexpr int kMaxDevices = 64;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_of_current_device(void** out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(500, "device index beyond the scratch table");
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(1000, "hipGetSymbolAddress returned a null address for scratch");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
+__global__ void k_scratch_clear(unsigned long long* p, int n) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i < n) p[i] = 0ull;
+}
+
+__global__ void k_scratch_events(const DevState* st, int64_t* out) {
+  const int i = (int)threadIdx.x;
+  if (i < 3) out[i] = (int64_t)st->c2p[i];
+}
+}  // namespace
+
+extern "C" int amun_dev_cons2prim_noalloc(const amun_cfg* cfg, int32_t n, const double* d_u,
+                                          double* d_q, int64_t* d_events, void* stream) {
+  if (!cfg) return fail(500, "null argument");
+  int nv = 0;
+  int r = validate(*cfg, &nv);
+  if (r) return r;
+  if (n < 1 || !d_u || !d_q) return fail(500, "bad argument: n >= 1 and device arrays d_u, d_q");
+  Dispatch d;
+  if (!pick(d, *cfg)) return fail(600, "no kernel instantiation for this configuration");
+  const RunConst rc = make_rc(*cfg);
+  void* sp = nullptr;
+  r = scratch_of_current_device(&sp);
+  if (r) return r;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  unsigned long long* words = static_cast<unsigned long long*>(sp);
+  hipLaunchKernelGGL(k_scratch_clear, dim3(1), dim3(kScratchWords), 0, s, words, kScratchWords);
+  hipLaunchKernelGGL(d.unit_c2p, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, s, n, d_u, d_q,
+                     rc, static_cast<DevState*>(sp));
+  if (d_events)
+    hipLaunchKernelGGL(k_scratch_events, dim3(1), dim3(64), 0, s,
+                       static_cast<const DevState*>(sp), d_events);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
 // unit level (tests): device routines on host vectors
 // ---------------------------------------------------------------------------
 namespace {

@@ -110,6 +110,9 @@ def load_library(path: str = LIB_PATH):
                                             ctypes.c_int32, ctypes.c_double, _P, _P, _P]
     L.amun_dev_unit_cons2prim.argtypes = [ctypes.POINTER(AmunCfg), ctypes.c_int32, _P, _P,
                                           ctypes.POINTER(ctypes.c_int64)]
+    L.amun_dev_cons2prim_noalloc.argtypes = [ctypes.POINTER(AmunCfg), ctypes.c_int32,
+                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                             ctypes.c_void_p]
     L.amun_dev_init_amr.argtypes = [ctypes.POINTER(AmunCfg), ctypes.c_int32,
                                     ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(vp)]
     L.amun_dev_amr_info.argtypes = [vp, ctypes.POINTER(ctypes.c_int32)]

@@ -388,6 +388,22 @@ int  amun_dev_unit_reconstruct(const amun_cfg* cfg, int32_t n, int32_t npencil, 
 int  amun_dev_unit_cons2prim(const amun_cfg* cfg, int32_t n, const double* u, double* q,
                              int64_t* events);
 
+/* No counterpart in the reference: amun_dev_unit_cons2prim for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  d_u (n conserved vectors) and d_q (n
+ * primitive vectors, entering with the previous primitives) are DEVICE
+ * arrays in the layout of amun_dev_unit_cons2prim; d_events (may be NULL)
+ * is a device array of 3 int64 that receives the counts of
+ * amun_dev_c2p_events for this call.  The event counters live in a static
+ * device array of the library (256 bytes per GPU, held until the process
+ * ends) instead of a hipMalloc'd buffer; the call allocates nothing, clears
+ * the counters with a kernel, and only enqueues kernels on stream (a
+ * hipStream_t; NULL = the null stream) without synchronising.  Calls that
+ * overlap on one GPU share those counters and race.  n is limited only by
+ * int32 (the counters do not grow with n). */
+int  amun_dev_cons2prim_noalloc(const amun_cfg* cfg, int32_t n, const double* d_u, double* d_q,
+                                int64_t* d_events, void* stream);
+
 /* Message of the last failing call of this thread. */
 const char* amun_dev_last_error(void);
 
