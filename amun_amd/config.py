@@ -67,6 +67,53 @@ class AmunCfg(ctypes.Structure):
     ]
 
 
+class AmunRefineCfg(ctypes.Structure):
+    """ctypes mirror of ``amun_refine_cfg`` (include/amun_dev.h)."""
+    _fields_ = [
+        ("crefmin", ctypes.c_double), ("crefmax", ctypes.c_double),
+        ("epsref", ctypes.c_double), ("vortmin", ctypes.c_double),
+        ("vortmax", ctypes.c_double), ("jabsmin", ctypes.c_double),
+        ("jabsmax", ctypes.c_double), ("varmask", ctypes.c_int32),
+        ("vort", ctypes.c_int32), ("jabs", ctypes.c_int32),
+        ("minlev", ctypes.c_int32), ("maxlev", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
+# primitive variable names of equations.F90:410-414, 513-521
+PVARS = {1: ["dens", "velx", "vely", "velz", "pres"],
+         2: ["dens", "velx", "vely", "velz", "magx", "magy", "magz", "bpsi", "pres"],
+         4: ["dens", "velx", "vely", "velz", "magx", "magy", "magz", "bpsi", "pres"]}
+
+
+def refine_cfg(p: "Params", minlev: int = 0, maxlev: int = 0) -> AmunRefineCfg:
+    """The refinement criterion's parameters from the run parameters
+    (p.extra), with the reference's defaults and its selection of variables
+    by name: variable p counts when its name occurs in
+    "refinement_variables" (refinement.F90:136-178, index(variables,
+    pvars(p)) > 0); "vort" and (magnetic systems) "jabs" switch on the
+    vorticity and current-density criteria."""
+    x = p.extra
+
+    def num(key, default):
+        return float(str(x.get(key, default)).replace("d", "e").replace("D", "e"))
+    names = str(x.get("refinement_variables", "dens pres"))
+    eq = EQSYS[p.equation_system.lower()]
+    c = AmunRefineCfg()
+    c.crefmin = num("crefmin", 0.2)
+    c.crefmax = num("crefmax", 0.8)
+    c.epsref = num("epsref", 0.01)
+    c.vortmin = num("vortmin", 1.0e-3)
+    c.vortmax = num("vortmax", 0.1)
+    c.jabsmin = num("jabsmin", 1.0e-3)
+    c.jabsmax = num("jabsmax", 0.1)
+    c.varmask = sum(1 << i for i, n in enumerate(PVARS[eq]) if n in names)
+    c.vort = int("vort" in names)
+    c.jabs = int("jabs" in names and eq != 1)
+    c.minlev, c.maxlev = int(minlev), int(maxlev)
+    return c
+
+
 class ConfigError(ValueError):
     """Invalid parameter, carrying the reference's integer error code."""
 

@@ -199,7 +199,10 @@ __device__ __forceinline__ void face_states(const double* __restrict__ Q, const 
 }
 
 // tangent of the face flux: DF(i) = dF(i+1/2) for tangent dX, dgamma
-template <class P, int RS, int ND, int DIR>
+// TF: arithmetic of the face flux and its Jacobian (double, or float in the
+// FP32-flux mode, whose forward evaluates the fluxes in float); everything
+// else -- primitives, slopes, the stage combines -- stays FP64
+template <class P, int RS, int ND, int DIR, typename TF>
 __global__ void k_tlm_face(const double* __restrict__ X, const double* __restrict__ Q,
                            const double* __restrict__ dX, double dgam, double* __restrict__ DF,
                            const int* __restrict__ nbr, RunConst rc, double cmax) {
@@ -218,7 +221,8 @@ __global__ void k_tlm_face(const double* __restrict__ X, const double* __restric
     load_vec<NV>(dX, rc, pk, du);
     c2p_tangent<P>(rc, u, q, du, dgam, dq[m]);
   }
-  Dual<1> ql[NV], qr[NV];
+  using D1 = Dual<1, TF>;
+  D1 ql[NV], qr[NV];
 #pragma unroll
   for (int v = 0; v < NV; v++) {
     double dm, d0, dp;
@@ -228,30 +232,30 @@ __global__ void k_tlm_face(const double* __restrict__ X, const double* __restric
     const double dsr = dm * dq[1][v] + d0 * dq[2][v] + dp * dq[3][v];
     const double q0 = Q[var_index<NV>(rc, cell_index<8>(nbr, rc, b, i, j, k), v)];
     const double q1 = Q[var_index<NV>(rc, cell_index<8>(nbr, rc, b, i + (DIR == 0), j + (DIR == 1), k + (DIR == 2)), v)];
-    ql[v].v = q0 + sl;
-    ql[v].d[0] = dq[1][v] + dsl;
-    qr[v].v = q1 - sr;
-    qr[v].d[0] = dq[2][v] - dsr;
+    ql[v].v = TF(q0 + sl);
+    ql[v].d[0] = TF(dq[1][v] + dsl);
+    qr[v].v = TF(q1 - sr);
+    qr[v].d[0] = TF(dq[2][v] - dsr);
   }
-  Dual<1> g(rc.gamma);
-  g.d[0] = dgam;
-  const EosConst<Dual<1>> e{g, g - 1.0, rcp(g - 1.0)};
-  Dual<1> a[NV], c[NV], f[NV];
+  D1 g(rc.gamma);
+  g.d[0] = TF(dgam);
+  const EosConst<D1> e{g, g - 1.0, rcp(g - 1.0)};
+  D1 a[NV], c[NV], f[NV];
 #pragma unroll
   for (int p = 0; p < NV; p++) {
     a[p] = ql[rot<P, DIR>(p)];
     c[p] = qr[rot<P, DIR>(p)];
   }
-  riemann<P, RS, Dual<1>>(e, rc.sc, Dual<1>(cmax), Dual<1>(cmax * cmax), a, c, f);
+  riemann<P, RS, D1>(e, rc.sc, D1(cmax), D1(cmax * cmax), a, c, f);
   const size_t base = (size_t)b * NV * rc.ncell + lidx<8>(rc.nc, rc.nd, i, j, k);
 #pragma unroll
-  for (int p = 0; p < NV; p++) DF[base + (size_t)rot<P, DIR>(p) * rc.ncell] = f[p].d[0];
+  for (int p = 0; p < NV; p++) DF[base + (size_t)rot<P, DIR>(p) * rc.ncell] = double(f[p].d[0]);
 }
 
 // adjoint of the face flux: from MU (= c dt lambda_Y) the face flux adjoint
 // lF = dxi (MU(i+1) - MU(i)), then AL(i) = (dF/dqL)^T lF, AR(i) = (dF/dqR)^T
 // lF, GF(i) = (dF/dgamma) . lF
-template <class P, int RS, int ND, int DIR>
+template <class P, int RS, int ND, int DIR, typename TF>
 __global__ void k_adj_face(const double* __restrict__ Q, const double* __restrict__ MU,
                            double* __restrict__ AL, double* __restrict__ AR,
                            double* __restrict__ GF, const int* __restrict__ nbr, RunConst rc,
@@ -273,29 +277,30 @@ __global__ void k_adj_face(const double* __restrict__ Q, const double* __restric
   constexpr int NIN = 2 * NV + 1;
   double gin[NIN];
   for (int c0 = 0; c0 < NIN; c0 += kAdjNch) {
-    Dual<kAdjNch> a[NV], c[NV], f[NV];
+    using DN = Dual<kAdjNch, TF>;
+    DN a[NV], c[NV], f[NV];
 #pragma unroll
     for (int p = 0; p < NV; p++) {
-      a[p] = Dual<kAdjNch>(ql[rot<P, DIR>(p)]);
-      c[p] = Dual<kAdjNch>(qr[rot<P, DIR>(p)]);
+      a[p] = DN(ql[rot<P, DIR>(p)]);
+      c[p] = DN(qr[rot<P, DIR>(p)]);
     }
-    Dual<kAdjNch> g(rc.gamma);
+    DN g(rc.gamma);
 #pragma unroll
     for (int s = 0; s < kAdjNch; s++) {
       const int n = c0 + s;
-      if (n < NV) a[n].d[s] = 1.0;
-      else if (n < 2 * NV) c[n - NV].d[s] = 1.0;
-      else if (n == 2 * NV) g.d[s] = 1.0;
+      if (n < NV) a[n].d[s] = TF(1);
+      else if (n < 2 * NV) c[n - NV].d[s] = TF(1);
+      else if (n == 2 * NV) g.d[s] = TF(1);
     }
-    const EosConst<Dual<kAdjNch>> e{g, g - 1.0, rcp(g - 1.0)};
-    riemann<P, RS, Dual<kAdjNch>>(e, rc.sc, Dual<kAdjNch>(cmax), Dual<kAdjNch>(cmax * cmax), a, c, f);
+    const EosConst<DN> e{g, g - 1.0, rcp(g - 1.0)};
+    riemann<P, RS, DN>(e, rc.sc, DN(cmax), DN(cmax * cmax), a, c, f);
 #pragma unroll
     for (int s = 0; s < kAdjNch; s++) {
       const int n = c0 + s;
       if (n >= NIN) break;
       double acc = 0.0;
 #pragma unroll
-      for (int p = 0; p < NV; p++) acc += lf[rot<P, DIR>(p)] * f[p].d[s];
+      for (int p = 0; p < NV; p++) acc += lf[rot<P, DIR>(p)] * double(f[p].d[s]);
       gin[n] = acc;
     }
   }

@@ -4,6 +4,7 @@
 // the sweep design); the kernel tables of the reconstructions are built in
 // inst_tvd.hip, inst_weno5z.hip and inst_gp.hip.
 #include "kernels.h"
+#include "remesh.h"
 
 #include <array>
 #include <map>
@@ -52,6 +53,7 @@ struct amun_ctx {
     double dtnext = 0.0;
     std::vector<double> dt, cmax;
     double* ck = nullptr;       // ceil(nsteps / interval) states
+    float* ckc = nullptr;       // their Kahan compensations (mixed mode)
     double* dtc = nullptr;      // device copy of (dt, cmax) per step
   } tape;
   // multi-level mesh (amun_dev_init_amr): real blocks [0, nreal) of the
@@ -887,6 +889,7 @@ void amun_dev_finalize(amun_ctx* c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   drop_graphs(c);
   hipFree(c->tape.ck);
+  hipFree(c->tape.ckc);
   hipFree(c->tape.dtc);
   hipFree(c->U0);
   hipFree(c->U1);
@@ -917,8 +920,9 @@ void amun_dev_finalize(amun_ctx* c) {
 }
 
 // primitive variables and maximum speed of runs [r0, r0 + nr) after their
-// conserved state changed on the host side
-static int after_upload(amun_ctx* c, int r0, int nr) {
+// conserved state changed on the host side (zero_c: the FP32 compensation
+// of U0 restarts from zero)
+static int after_upload(amun_ctx* c, int r0, int nr, bool zero_c = true) {
   Dispatch& d = dispatch_of(c);
   const int slot = (int)(c->nsteps & 1);
   const int b0 = c->run_b0[r0], nb = c->run_b0[r0 + nr] - b0;
@@ -929,7 +933,7 @@ static int after_upload(amun_ctx* c, int r0, int nr) {
   // overwrites pdata%q)
   hipLaunchKernelGGL(d.prim, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
                      c->U0, c->Q[c->qcur], c->rc, c->st, c->rs, c->brun, slot, b0, nb);
-  if (c->C)
+  if (c->C && zero_c)
     hipLaunchKernelGGL(k_zero_f32, dim3(2048), dim3(256), 0, c->stream,
                        c->C + (size_t)b0 * c->nv * c->rc.ncell,
                        (long long)nb * c->nv * c->rc.ncell);
@@ -1712,6 +1716,207 @@ int amun_dev_positivity_events(amun_ctx* c, uint64_t* n) {
   return 0;
 }
 
+// Refill the stand-ins of the ghost cells (virtual blocks of level changes
+// and of non-periodic sides) from the current primitive state Q[qcur], as
+// boundary_variables leaves them after a stage; the positivity events of
+// this fill go to a scratch counter (the steps count their own fills).
+static int refill_ghosts(amun_ctx* c) {
+  const RunConst& rc = c->rc;
+  if (c->amr.on) {
+    auto& A = c->amr;
+    const dim3 bl(256);
+    if (c->qsize == sizeof(double)) {
+      double* Q = static_cast<double*>(c->Q[c->qcur]);
+      if (A.nr)
+        hipLaunchKernelGGL(k_amr_restrict<double>, dim3((A.nr + 255) / 256), bl, 0, c->stream,
+                           A.rtask, A.nr, Q, rc, c->nv);
+      if (A.np)
+        hipLaunchKernelGGL(k_amr_prolong<double>, dim3((A.np + 255) / 256), bl, 0, c->stream,
+                           A.ptask, A.np, Q, c->nbr, rc, c->nv, c->nv - 1, A.dummy);
+    } else {
+      float* Q = static_cast<float*>(c->Q[c->qcur]);
+      if (A.nr)
+        hipLaunchKernelGGL(k_amr_restrict<float>, dim3((A.nr + 255) / 256), bl, 0, c->stream,
+                           A.rtask, A.nr, Q, rc, c->nv);
+      if (A.np)
+        hipLaunchKernelGGL(k_amr_prolong<float>, dim3((A.np + 255) / 256), bl, 0, c->stream,
+                           A.ptask, A.np, Q, c->nbr, rc, c->nv, c->nv - 1, A.dummy);
+    }
+    HIP_TRY(hipGetLastError());
+  } else if (c->bnd.on) {
+    int r = bnd_fill(c, c->qcur);
+    if (r) return r;
+  }
+  return 0;
+}
+
+int amun_dev_refinement(amun_ctx* c, const amun_refine_cfg* cfg, int32_t* crit, float* ind) {
+  if (!c || !cfg || !crit) return fail(500, "null argument");
+  const RunConst& rc = c->rc;
+  const int nb = c->amr.on ? c->amr.nreal : rc.nblk;
+  // the ghost cells the criterion reads (one layer beyond the interior, two
+  // for the central differences) must hold the current state
+  if (int r = refill_ghosts(c)) return r;
+  RefineArgs ra;
+  ra.crefmin = cfg->crefmin;
+  ra.crefmax = cfg->crefmax;
+  ra.epsref = cfg->epsref;
+  ra.vortmin = cfg->vortmin;
+  ra.vortmax = cfg->vortmax;
+  ra.jabsmin = cfg->jabsmin;
+  ra.jabsmax = cfg->jabsmax;
+  ra.varmask = cfg->varmask;
+  ra.vort = cfg->vort;
+  ra.jabs = cfg->jabs;
+  int* dcrit = nullptr;
+  float* dind = nullptr;
+  HIP_TRY(hipMalloc((void**)&dcrit, (size_t)nb * sizeof(int)));
+  hipError_t e = hipMalloc((void**)&dind, (size_t)nb * 3 * sizeof(float));
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(c->disp.refine, dim3((unsigned)nb), dim3(256), 0, c->stream,
+                       (const void*)c->Q[c->qcur], c->nbr, rc, ra, dcrit, dind);
+    e = hipGetLastError();
+  }
+  std::vector<int> hc(nb);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e == hipSuccess) e = hipMemcpy(hc.data(), dcrit, (size_t)nb * sizeof(int), hipMemcpyDeviceToHost);
+  if (e == hipSuccess && ind)
+    e = hipMemcpy(ind, dind, (size_t)nb * 3 * sizeof(float), hipMemcpyDeviceToHost);
+  (void)hipFree(dcrit);
+  (void)hipFree(dind);
+  if (e != hipSuccess) return fail(1000 + (int)e, std::string("refinement: ") + hipGetErrorString(e));
+  // the level limits of update_mesh (mesh.F90:1547-1550)
+  for (int b = 0; b < nb; b++) {
+    int r = hc[b];
+    if (cfg->minlev > 0 || cfg->maxlev > 0) {
+      const int lev = c->amr.on ? c->amr.meta[4 * b] : 1;
+      if (cfg->minlev > 0 && lev < cfg->minlev) r = 1;
+      if (cfg->minlev > 0 && lev == cfg->minlev) r = std::max(0, r);
+      if (cfg->maxlev > 0 && lev == cfg->maxlev) r = std::min(0, r);
+      if (cfg->maxlev > 0 && lev > cfg->maxlev) r = -1;
+    }
+    crit[b] = r;
+  }
+  return 0;
+}
+
+// The data movement of update_mesh (mesh.F90:756-833) onto the leaf set
+// meta: a block kept from the old mesh is copied, a child of an old leaf is
+// prolonged from it (prolong_block, mesh.F90:1031-1257), the parent of 2^d
+// old leaves is restricted from them (restrict_block, 1258-1454), all in one
+// launch from the old pools to a new context's; the new context then takes
+// the old one's place (stream, time, step, dt, event counters, launch
+// shapes and graph setting kept) and recomputes the primitive state and the
+// maximum speed as update_variables does after update_mesh
+// (evolution.F90:321-340).
+int amun_dev_amr_remesh(amun_ctx* c, int32_t nblk, const int32_t* meta) {
+  if (!c || !meta || nblk < 1) return fail(500, "bad argument");
+  if (!c->amr.on) return fail(600, "mesh updates need a multi-level context (amun_dev_init_amr)");
+  const int nc = c->rc.nc, nd = c->rc.nd;
+  // the old leaves by level and position (in blocks of their level)
+  std::map<std::array<long, 4>, int> old;
+  const auto& M = c->amr.meta;
+  for (int b = 0; b < c->amr.nreal; b++)
+    old[{M[4 * b], M[4 * b + 1] / nc, M[4 * b + 2] / nc, M[4 * b + 3] / nc}] = b;
+  auto find = [&](long L, long x, long y, long z) {
+    auto it = old.find({L, x, y, z});
+    return it == old.end() ? -1 : it->second;
+  };
+  std::vector<int4> task(nblk);
+  std::vector<int32_t> kids;
+  for (int b = 0; b < nblk; b++) {
+    const long L = meta[4 * b];
+    for (int d = 0; d < 3; d++)
+      if (meta[4 * b + 1 + d] < 0 || meta[4 * b + 1 + d] % nc)
+        return fail(500, "block offsets must be non-negative multiples of ncells");
+    const long x = meta[4 * b + 1] / nc, y = meta[4 * b + 2] / nc;
+    const long z = nd == 3 ? meta[4 * b + 3] / nc : 0;
+    int src = find(L, x, y, z);
+    if (src >= 0) {
+      task[b] = make_int4(REMESH_COPY, src, 0, 0);
+      continue;
+    }
+    src = find(L - 1, x / 2, y / 2, z / 2);
+    if (src >= 0) {
+      task[b] = make_int4(REMESH_PROLONG, src, (int)((x & 1) | (y & 1) << 1 | (z & 1) << 2), 0);
+      continue;
+    }
+    task[b] = make_int4(REMESH_RESTRICT, (int)kids.size(), 0, 0);
+    for (int q = 0; q < 8; q++) {
+      if (nd == 2 && q >= 4) {
+        const int32_t same_k = kids[kids.size() - 4];   // never read in 2D
+        kids.push_back(same_k);
+        continue;
+      }
+      const int k = find(L + 1, 2 * x + (q & 1), 2 * y + (q >> 1 & 1), 2 * z + (q >> 2 & 1));
+      if (k < 0)
+        return fail(702, "a block of the new mesh is neither an old leaf, a child of one, "
+                         "nor the parent of 2^d old leaves");
+      kids.push_back(k);
+    }
+  }
+  // the prolongations read one cell beyond the old parents' interiors
+  if (int r = refill_ghosts(c)) return r;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  amun_ctx* n = nullptr;
+  if (int r = amun_dev_init_amr(&c->cfg, nblk, meta, &n)) return r;
+  int4* dtask = nullptr;
+  int32_t* dkids = nullptr;
+  RunState R;
+  hipError_t e = hipMalloc((void**)&dtask, task.size() * sizeof(int4));
+  if (e == hipSuccess) e = hipMalloc((void**)&dkids, std::max<size_t>(kids.size(), 1) * sizeof(int32_t));
+  if (e == hipSuccess)
+    e = hipMemcpy(dtask, task.data(), task.size() * sizeof(int4), hipMemcpyHostToDevice);
+  if (e == hipSuccess && !kids.empty())
+    e = hipMemcpy(dkids, kids.data(), kids.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+  // events so far carry over; the prolongations add theirs
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(n->st, c->st, sizeof(DevState), hipMemcpyDeviceToDevice, n->stream);
+  if (e == hipSuccess) {
+    const long long cells = (long long)nblk * n->rc.ncell;
+    const dim3 grid((unsigned)((cells + 255) / 256)), bl(256);
+    const int ien = c->nv - 1;
+    if (c->qsize == sizeof(double))
+      hipLaunchKernelGGL(k_remesh<double>, grid, bl, 0, n->stream, dtask, dkids, nblk, c->U0,
+                         c->C, static_cast<const double*>(c->Q[c->qcur]), c->nbr, c->rc, n->U0,
+                         n->C, c->nv, ien, n->st);
+    else
+      hipLaunchKernelGGL(k_remesh<float>, grid, bl, 0, n->stream, dtask, dkids, nblk, c->U0,
+                         c->C, static_cast<const float*>(c->Q[c->qcur]), c->nbr, c->rc, n->U0,
+                         n->C, c->nv, ien, n->st);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(n->stream);
+  // time, step and dt of the run; cell sizes and dxmin are the new mesh's
+  if (e == hipSuccess) e = hipMemcpy(&R, c->rs, sizeof(RunState), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) {
+    RunState Rn;
+    e = hipMemcpy(&Rn, n->rs, sizeof(RunState), hipMemcpyDeviceToHost);
+    Rn.dt = R.dt;
+    Rn.time = R.time;
+    Rn.step = R.step;
+    Rn.fix_dt = R.fix_dt;
+    Rn.fix_cmax = R.fix_cmax;
+    if (e == hipSuccess) e = hipMemcpy(n->rs, &Rn, sizeof(RunState), hipMemcpyHostToDevice);
+  }
+  (void)hipFree(dtask);
+  (void)hipFree(dkids);
+  if (e != hipSuccess) {
+    amun_dev_finalize(n);
+    return fail(1000 + (int)e, std::string("mesh update: ") + hipGetErrorString(e));
+  }
+  for (int d = 0; d < 3; d++) {
+    n->wg[d] = c->wg[d];
+    n->swz[d] = c->swz[d];
+  }
+  n->use_graph = c->use_graph;
+  // the new context takes the caller's handle and stream
+  std::swap(*c, *n);
+  std::swap(c->stream, n->stream);
+  amun_dev_finalize(n);
+  return after_upload(c, 0, 1, false);
+}
+
 int amun_dev_tridiag_events(amun_ctx* c, uint64_t* n) {
   if (!c || !n) return fail(500, "null argument");
   HIP_TRY(hipStreamSynchronize(c->stream));
@@ -1742,7 +1947,7 @@ static int tape_check(amun_ctx* c) {
   if (c->amr.on) return fail(600, "gradients are not supported on multi-level meshes");
   if (c->bnd.on) return fail(600, "gradients are built for periodic domains");
   if (!c->disp.has_adj)
-    return fail(600, "gradients need FP64, TVD reconstruction and the HD or MHD system");
+    return fail(600, "gradients need TVD reconstruction and the HD or MHD system");
   return 0;
 }
 
@@ -1752,19 +1957,26 @@ int amun_dev_tape_forward(amun_ctx* c, int32_t nsteps, int32_t interval, double 
   if (r) return r;
   auto& T = c->tape;
   hipFree(T.ck);
+  hipFree(T.ckc);
   hipFree(T.dtc);
   T.ck = nullptr;
+  T.ckc = nullptr;
   T.dtc = nullptr;
   const int nck = (nsteps + interval - 1) / interval;
   HIP_TRY(hipMalloc((void**)&T.ck, (size_t)nck * c->nstate * sizeof(double)));
+  if (c->C) HIP_TRY(hipMalloc((void**)&T.ckc, (size_t)nck * c->nstate * sizeof(float)));
   HIP_TRY(hipMalloc((void**)&T.dtc, (size_t)2 * nsteps * sizeof(double)));
   T.nsteps = nsteps;
   T.interval = interval;
   T.dtnext = dtnext;
   for (int s = 0; s < nsteps; s++) {
-    if (s % interval == 0)
+    if (s % interval == 0) {
       HIP_TRY(hipMemcpyAsync(T.ck + (size_t)(s / interval) * c->nstate, c->U0,
                              c->nstate * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+      if (c->C)
+        HIP_TRY(hipMemcpyAsync(T.ckc + (size_t)(s / interval) * c->nstate, c->C,
+                               c->nstate * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+    }
     hipLaunchKernelGGL(k_tape, dim3(1), dim3(1), 0, c->stream, c->rs, c->rc,
                        (int)(c->nsteps & 1), dtnext, T.dtc + 2 * s);
     r = launch_step(c, dtnext);
@@ -1861,6 +2073,9 @@ int recompute_segment(amun_ctx* c, int s0, int s1, double* seg) {
                          c->nstate * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
   int r = after_upload(c, 0, 1);
   if (r) return r;
+  if (c->C)   // the step state of the mixed mode is U0 - C
+    HIP_TRY(hipMemcpyAsync(c->C, T.ckc + (size_t)(s0 / T.interval) * c->nstate,
+                           c->nstate * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
   for (int s = s0; s < s1; s++) {
     hipLaunchKernelGGL(k_set_fixed, dim3(1), dim3(1), 0, c->stream, c->rs, T.dt[s], T.cmax[s]);
     r = enqueue_step(c, 2.0e30, (int)(c->nsteps & 1), c->qcur, nullptr,
@@ -1897,6 +2112,7 @@ int pool_to_global(amun_ctx* c, const double* dpool, double* g) {
 // state (time, step, dt) and the host step counters.
 struct GradKeep {
   double* u = nullptr;
+  float* cp = nullptr;        // Kahan compensation (mixed mode)
   RunState rs;
   long long nsteps = 0;
   int qcur = 0;
@@ -1907,6 +2123,10 @@ static int grad_state(amun_ctx* c, GradKeep& k, bool back) {
     HIP_TRY(hipStreamSynchronize(c->stream));
     HIP_TRY(hipMalloc((void**)&k.u, c->nstate * sizeof(double)));
     HIP_TRY(hipMemcpy(k.u, c->U0, c->nstate * sizeof(double), hipMemcpyDeviceToDevice));
+    if (c->C) {
+      HIP_TRY(hipMalloc((void**)&k.cp, c->nstate * sizeof(float)));
+      HIP_TRY(hipMemcpy(k.cp, c->C, c->nstate * sizeof(float), hipMemcpyDeviceToDevice));
+    }
     HIP_TRY(hipMemcpy(&k.rs, c->rs, sizeof(RunState), hipMemcpyDeviceToHost));
     k.nsteps = c->nsteps;
     k.qcur = c->qcur;
@@ -1919,7 +2139,13 @@ static int grad_state(amun_ctx* c, GradKeep& k, bool back) {
   k.u = nullptr;
   c->nsteps = k.nsteps;
   c->qcur = k.qcur;
-  return after_upload(c, 0, 1);
+  int r = after_upload(c, 0, 1);
+  if (!r && k.cp) {
+    HIP_TRY(hipMemcpy(c->C, k.cp, c->nstate * sizeof(float), hipMemcpyDeviceToDevice));
+    HIP_TRY(hipFree(k.cp));
+    k.cp = nullptr;
+  }
+  return r;
 }
 
 int amun_dev_tape_tangent(amun_ctx* c, const double* du0, double dgamma, double* du) {

@@ -1105,6 +1105,10 @@ __global__ __launch_bounds__(256, sweep_min_waves<T>()) void k_sweep(SweepArgs a
       rec.load(m, qtmp);
     }
     T flp[NV], fprev[NV];
+    // unchunked sweeps: the window's next cell is requested one iteration
+    // ahead of its use
+    T qnext[NV];
+    if constexpr (!kChunk) loadq(RC::R - 1, qnext);
     for (int c = -1; c <= nc; c++) {
       // operands of the cell finished in this iteration (c-1), loaded before
       // the reconstruction and Riemann solve so that their latency overlaps
@@ -1144,8 +1148,13 @@ __global__ __launch_bounds__(256, sweep_min_waves<T>()) void k_sweep(SweepArgs a
         }
       }
       if (c > -1) rec.shift();
-      loadq(c + RC::R, qtmp);
-      rec.load(2 * RC::R, qtmp);
+      if constexpr (kChunk) {
+        loadq(c + RC::R, qtmp);
+        rec.load(2 * RC::R, qtmp);
+      } else {
+        rec.load(2 * RC::R, qnext);
+        if (c < nc) loadq(c + RC::R + 1, qnext);
+      }
       T flc[NV], frc[NV];
       int ci, cj, ck;
       coords(c, ci, cj, ck);
@@ -1258,6 +1267,138 @@ __global__ __launch_bounds__(256, sweep_min_waves<T>()) void k_sweep(SweepArgs a
 
   if constexpr (kLast) {
     if (a.last) wg_max_speed(a.rs, a.spart, run, a.slot ^ 1, smax);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// refinement criterion of the leaf blocks (check_refinement_criterion,
+// refinement.F90:257-640), one workgroup per block
+// ---------------------------------------------------------------------------
+struct RefineArgs {
+  double crefmin, crefmax, epsref, vortmin, vortmax, jabsmin, jabsmax;
+  int varmask, vort, jabs;
+};
+
+// one cell's normalised second difference along each direction (the
+// reference's real(4) differences and magnitudes, real(8) quotient,
+// real(4) result); contraction off so that the real(8) sums round as there
+#pragma clang fp contract(off)
+__device__ __forceinline__ float sde_term(double qp, double qm, double qc, double epsref) {
+  const float fr = float(qp - qc);
+  const float fl = float(qm - qc);
+  const float fc = float((fabs(qp) + fabs(qm)) + 2.0 * fabs(qc));
+  const float fsum = fabsf(fr) + fabsf(fl);
+  return float(double(fabsf(fr + fl)) / ((double(fsum) + epsref * double(fc)) + 2.220446049250313e-16));
+}
+
+// |curl|^2 of the vector at variables i0..i0+2 with unit spacing (curl,
+// operators.F90:328-434), summed in real(8)
+template <int NV, int ND, typename T>
+__device__ __forceinline__ double curl2(const T* __restrict__ Q, const int* __restrict__ nbr,
+                                        const RunConst& rc, int b, int i, int j, int k, int i0) {
+  auto at = [&](int c, int di, int dj, int dk) {
+    return double(Q[var_index<NV>(rc, cell_index<sizeof(T)>(nbr, rc, b, i + di, j + dj, k + dk),
+                                  i0 + c)]);
+  };
+  auto d1 = [&](int c, int di, int dj, int dk) {
+    return 0.5 * (at(c, di, dj, dk) - at(c, -di, -dj, -dk)) / 1.0;
+  };
+  double wx, wy, wz;
+  if constexpr (ND == 3) {
+    wx = d1(2, 0, 1, 0) - d1(1, 0, 0, 1);
+    wy = d1(0, 0, 0, 1) - d1(2, 1, 0, 0);
+    wz = d1(1, 1, 0, 0) - d1(0, 0, 1, 0);
+  } else {
+    wx = d1(2, 0, 1, 0);
+    wy = -d1(2, 1, 0, 0);
+    wz = d1(1, 1, 0, 0) - d1(0, 0, 1, 0);
+  }
+  return (wx * wx + wy * wy) + wz * wz;
+}
+#pragma clang fp contract(on)
+
+// per block: crit[b] = -1 / 0 / 1 and ind[3 b ..] = the largest
+// second-derivative error over the selected variables, the vorticity and
+// the current-density magnitude, over the cells one layer beyond the
+// interior (ibl .. ieu), from the primitive state Q (ghosts filled)
+template <class P, typename T>
+__global__ __launch_bounds__(256) void k_refine(const void* Qv, const int* __restrict__ nbr,
+                                                RunConst rc, RefineArgs ra, int* crit,
+                                                float* ind) {
+  constexpr int NV = P::nv;
+  const T* __restrict__ Q = static_cast<const T*>(Qv);
+  const int b = blockIdx.x;
+  const int nc = rc.nc, m = nc + 2, nd = rc.nd;
+  const int ncell = nd == 3 ? m * m * m : m * m;
+  float err[NV], w2 = 0.0f, j2 = 0.0f;
+#pragma unroll
+  for (int v = 0; v < NV; v++) err[v] = 0.0f;
+  for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
+    const int i = c % m - 1, j = (c / m) % m - 1, k = nd == 3 ? c / (m * m) - 1 : 0;
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+      if (!(ra.varmask & (1 << v))) continue;
+      auto at = [&](int di, int dj, int dk) {
+        return double(Q[var_index<NV>(rc, cell_index<sizeof(T)>(nbr, rc, b, i + di, j + dj, k + dk), v)]);
+      };
+      const double qc = at(0, 0, 0);
+      float e = err[v];
+      e = fmaxf(e, sde_term(at(1, 0, 0), at(-1, 0, 0), qc, ra.epsref));
+      e = fmaxf(e, sde_term(at(0, 1, 0), at(0, -1, 0), qc, ra.epsref));
+      if (nd == 3) e = fmaxf(e, sde_term(at(0, 0, 1), at(0, 0, -1), qc, ra.epsref));
+      err[v] = e;
+    }
+    if (ra.vort) {
+      const double s = nd == 3 ? curl2<NV, 3, T>(Q, nbr, rc, b, i, j, k, P::ivx)
+                               : curl2<NV, 2, T>(Q, nbr, rc, b, i, j, k, P::ivx);
+      w2 = fmaxf(w2, float(s));
+    }
+    if constexpr (P::mhd) {
+      if (ra.jabs) {
+        const double s = nd == 3 ? curl2<NV, 3, T>(Q, nbr, rc, b, i, j, k, P::ibx)
+                                 : curl2<NV, 2, T>(Q, nbr, rc, b, i, j, k, P::ibx);
+        j2 = fmaxf(j2, float(s));
+      }
+    }
+  }
+  // workgroup maxima (order-free: maxima of floats)
+  __shared__ float red[NV + 2][256];
+#pragma unroll
+  for (int v = 0; v < NV; v++) red[v][threadIdx.x] = err[v];
+  red[NV][threadIdx.x] = w2;
+  red[NV + 1][threadIdx.x] = j2;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+#pragma unroll
+      for (int v = 0; v < NV + 2; v++)
+        red[v][threadIdx.x] = fmaxf(red[v][threadIdx.x], red[v][threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    int cr = -1;
+    float emax = 0.0f;
+    for (int v = 0; v < NV; v++) {
+      if (!(ra.varmask & (1 << v))) continue;
+      const float cref = red[v][0];
+      if (double(cref) > ra.crefmin) cr = cr > 0 ? cr : 0;
+      if (double(cref) > ra.crefmax) cr = 1;
+      emax = fmaxf(emax, cref);
+    }
+    const float wv = ra.vort ? sqrtf(red[NV][0]) : 0.0f;
+    if (ra.vort) {
+      if (double(wv) > ra.vortmin) cr = cr > 0 ? cr : 0;
+      if (double(wv) > ra.vortmax) cr = 1;
+    }
+    const float jv = (P::mhd && ra.jabs) ? sqrtf(red[NV + 1][0]) : 0.0f;
+    if (P::mhd && ra.jabs) {
+      if (double(jv) > ra.jabsmin) cr = cr > 0 ? cr : 0;
+      if (double(jv) > ra.jabsmax) cr = 1;
+    }
+    crit[b] = cr;
+    ind[3 * b] = emax;
+    ind[3 * b + 1] = wv;
+    ind[3 * b + 2] = jv;
   }
 }
 
@@ -1609,6 +1750,7 @@ struct Dispatch {
   void (*stats)(const void*, RunConst, double*, int, int, const int*) = nullptr;
   void (*pack_u)(const void*, double*, const int*, RunConst, int) = nullptr;
   void (*pack_q)(const void*, double*, const int*, RunConst, int) = nullptr;
+  void (*refine)(const void*, const int*, RunConst, RefineArgs, int*, float*) = nullptr;
   void (*unit_riemann)(int, double, const double*, const double*, double*, RunConst) = nullptr;
   void (*unit_recon)(int, int, double, const double*, double*, double*, RunConst) = nullptr;
   void (*unit_c2p)(int, const double*, double*, RunConst, DevState*) = nullptr;
@@ -1649,6 +1791,7 @@ void fill(Dispatch& d) {
   d.stats = k_stats<P, T>;
   d.pack_u = k_pack_blocks<P::nv, double>;
   d.pack_q = k_pack_blocks<P::nv, T>;
+  d.refine = k_refine<P, T>;
   d.unit_riemann = k_unit_riemann<P, RS, T>;
   d.unit_c2p = k_unit_c2p<P>;
   // TVD reads the transverse slopes of the block: no pencil form
@@ -1657,16 +1800,17 @@ void fill(Dispatch& d) {
   d.fineflux[0] = k_amr_fineflux<P, REC, RS, ND, 0, T>;
   d.fineflux[1] = k_amr_fineflux<P, REC, RS, ND, 1, T>;
   if constexpr (ND == 3) d.fineflux[2] = k_amr_fineflux<P, REC, RS, ND, 2, T>;
-  if constexpr (REC == REC_TVD && !P::sr && std::is_same<T, double>::value) {
+  if constexpr (REC == REC_TVD && !P::sr) {
+    // the face Jacobians in the arithmetic of the forward fluxes (T)
     d.has_adj = true;
     d.adj_prim = k_adj_prim<P>;
-    d.adj_face[0] = k_adj_face<P, RS, ND, 0>;
-    d.adj_face[1] = k_adj_face<P, RS, ND, 1>;
-    d.tlm_face[0] = k_tlm_face<P, RS, ND, 0>;
-    d.tlm_face[1] = k_tlm_face<P, RS, ND, 1>;
+    d.adj_face[0] = k_adj_face<P, RS, ND, 0, T>;
+    d.adj_face[1] = k_adj_face<P, RS, ND, 1, T>;
+    d.tlm_face[0] = k_tlm_face<P, RS, ND, 0, T>;
+    d.tlm_face[1] = k_tlm_face<P, RS, ND, 1, T>;
     if constexpr (ND == 3) {
-      d.adj_face[2] = k_adj_face<P, RS, ND, 2>;
-      d.tlm_face[2] = k_tlm_face<P, RS, ND, 2>;
+      d.adj_face[2] = k_adj_face<P, RS, ND, 2, T>;
+      d.tlm_face[2] = k_tlm_face<P, RS, ND, 2, T>;
     }
     d.adj_cell = k_adj_cell<P, ND>;
     d.tlm_cell = k_tlm_cell<P, ND>;

@@ -7,8 +7,10 @@ cells).  ``refine`` builds one the way AMUN's generate_mesh does
 (mesh.F90:431-740): level by level, leaves whose indicator asks for it are
 split into 2^d children, and neighbours are refined further until every
 leaf's face, edge and corner neighbours differ from it by one level at most
-(the 2:1 balance the ghost-cell exchange relies on).  The mesh stays on the
-host; the device path takes it frozen.
+(the 2:1 balance the ghost-cell exchange relies on).  ``update_leaves`` is
+update_mesh's selection (mesh.F90:756-833): from the per-block refinement
+criterion (amun_dev_refinement) it returns the next leaf set, which
+amun_dev_amr_remesh moves the device state onto.
 """
 from __future__ import annotations
 
@@ -71,6 +73,128 @@ def refine(p: Params, wants, maxlev: int) -> np.ndarray:
     return meta
 
 
+def _touching(p: Params, leaves: dict, blk):
+    """The leaves that share a face, edge or corner with leaf blk = (level,
+    bx, by, bz) on the periodic base grid: same-level ones, the coarser one
+    covering a neighbour position, or the finer ones of that position that
+    touch blk (a 2:1 balanced mesh has no other).  These are the blocks the
+    reference's neighbour pointers reach (faces, edges, corners;
+    blocks.F90:3825-3851 iterate_over_neighbors)."""
+    L, x, y, z = blk
+    nd = p.ndims
+    n = _nb(p, L)
+    out = []
+    for dz in ((-1, 0, 1) if nd == 3 else (0,)):
+        for dy in (-1, 0, 1):
+            for dx in (-1, 0, 1):
+                if dx == dy == dz == 0:
+                    continue
+                q = ((x + dx) % n[0], (y + dy) % n[1], (z + dz) % n[2])
+                if (L,) + q in leaves:
+                    out.append((L,) + q)
+                    continue
+                c = (L - 1, q[0] // 2, q[1] // 2, q[2] // 2)
+                if L > 1 and c in leaves:
+                    out.append(c)
+                    continue
+                # finer: the children of q on the side facing blk
+                for k in range(1 << nd):
+                    s = (k & 1, k >> 1 & 1, k >> 2 & 1)
+                    if any(d != 0 and s[a] != (1 if d < 0 else 0)
+                           for a, d in enumerate((dx, dy, dz))):
+                        continue
+                    f = (L + 1, 2 * q[0] + s[0], 2 * q[1] + s[1], 2 * q[2] + s[2])
+                    if f in leaves:
+                        out.append(f)
+    return out
+
+
+def update_leaves(p: Params, meta: np.ndarray, crit, minlev: int, maxlev: int) -> np.ndarray:
+    """update_mesh's choice of the next leaf set (mesh.F90:756-833) from the
+    refinement criterion crit[b] in {-1, 0, 1} of every leaf of meta:
+
+    - the level limits (check_data_block_refinement, mesh.F90:1545-1550);
+    - update_neighbor_refinement (1694-1748), finest level first, with the
+      rules of reset_neighbors_refinement (blocks.F90:4300-4350): a leaf to
+      be refined keeps its neighbours from derefining and refines its
+      coarser neighbours, a kept leaf keeps its coarser neighbours, a leaf
+      to be derefined is kept when a neighbour will be refined;
+    - prepare_sibling_derefinement (1768-2055): a parent is derefined only
+      when all its children are leaves that ask for it, else they are kept;
+    - derefine_selected_blocks then refine_selected_blocks (2069-2257).
+
+    The result of these rules does not depend on the order in which the
+    leaves of one level are visited (a flag raised to 1 during a level's
+    pass belongs to a coarser leaf, whose own pass then resets its
+    neighbours), so the blocks are visited in meta order.  Returns the new
+    meta: kept leaves in their old order, then the restored parents, then
+    the children of refined leaves."""
+    nd, nc = p.ndims, p.ncells
+    order = [(int(m[0]), int(m[1]) // nc, int(m[2]) // nc, int(m[3]) // nc if nd == 3 else 0)
+             for m in np.asarray(meta)]
+    leaves = {b: i for i, b in enumerate(order)}
+    flag = {}
+    for b, c in zip(order, crit):
+        r = int(c)
+        if b[0] < minlev:
+            r = 1
+        if b[0] == minlev:
+            r = max(0, r)
+        if b[0] == maxlev:
+            r = min(0, r)
+        if b[0] > maxlev:
+            r = -1
+        flag[b] = r
+    top = max(maxlev, max(b[0] for b in order))
+    nbrs = {b: _touching(p, leaves, b) for b in order}
+    for lev in range(top, 0, -1):
+        for b in order:
+            if b[0] != lev:
+                continue
+            for n in nbrs[b]:
+                if flag[b] == 1:
+                    flag[n] = max(0, flag[n])
+                    if n[0] < b[0]:
+                        flag[n] = 1
+                elif flag[b] == 0:
+                    if n[0] < b[0]:
+                        flag[n] = max(0, flag[n])
+                else:
+                    if n[0] < b[0]:
+                        flag[n] = max(0, flag[n])
+                    if flag[n] == 1:
+                        flag[b] = 0
+    parents = []
+    for lev in range(2, top + 1):
+        for b in order:
+            if b[0] != lev or flag[b] != -1:
+                continue
+            par = (lev - 1, b[1] // 2, b[2] // 2, b[3] // 2)
+            kids = [(lev, 2 * par[1] + (k & 1), 2 * par[2] + (k >> 1 & 1),
+                     2 * par[3] + (k >> 2 & 1)) for k in range(1 << nd)]
+            if all(k in flag and flag[k] == -1 for k in kids):
+                if par not in parents:
+                    parents.append(par)
+            else:
+                for k in kids:
+                    if k in flag:
+                        flag[k] = max(0, flag[k])
+    gone = set()
+    for par in parents:
+        gone.update((par[0] + 1, 2 * par[1] + (k & 1), 2 * par[2] + (k >> 1 & 1),
+                     2 * par[3] + (k >> 2 & 1)) for k in range(1 << nd))
+    kept = [b for b in order if b not in gone and flag[b] != 1]
+    born = []
+    for lev in range(1, top):
+        for b in order:
+            if b[0] == lev and flag[b] == 1 and b not in gone:
+                born += [(lev + 1, 2 * b[1] + (k & 1), 2 * b[2] + (k >> 1 & 1),
+                          2 * b[3] + (k >> 2 & 1)) for k in range(1 << nd)]
+    out = kept + parents + born
+    return np.array([(L, x * nc, y * nc, z * nc if nd == 3 else 0) for L, x, y, z in out],
+                    dtype=np.int32)
+
+
 def cell_centres(p: Params, meta: np.ndarray):
     """x, y, z cell centres of every block: arrays (nblk, [nc,] nc, nc)."""
     n = p.ncells
@@ -101,9 +225,10 @@ def block_state(p: Params, meta: np.ndarray, prim) -> np.ndarray:
 
 
 def blast_prim(p: Params, seed: int = 20261202, r0: float = 0.15, pin: float = 10.0,
-               pout: float = 0.1, b0=(0.5, 0.3, 0.2)):
+               pout: float = 0.1, b0=(0.5, 0.3, 0.2), shift=None):
     """Blast with seeded density and velocity noise: p = pin inside r0 of
-    the domain centre, pout outside; uniform B = b0 for MHD.
+    the domain centre (moved by shift, if given), pout outside; uniform B =
+    b0 for MHD.
 
     The state avoids the exact zeros at which the reference's HLLD switches
     branches discontinuously: all three field components are non-zero (at
@@ -119,6 +244,8 @@ def blast_prim(p: Params, seed: int = 20261202, r0: float = 0.15, pin: float = 1
         cx = 0.5 * (p.xmin + p.xmax)
         cy = 0.5 * (p.ymin + p.ymax)
         cz = 0.5 * (p.zmin + p.zmax) if p.ndims == 3 else 0.0
+        if shift is not None:
+            cx, cy, cz = cx + shift[0], cy + shift[1], cz + shift[2]
         r = np.sqrt((x - cx) ** 2 + (y - cy) ** 2 + (z - cz) ** 2)
         q = np.zeros((p.nv,) + x.shape)
         xi = uniform(seed, 4 * x.size, stream=21).reshape((4,) + x.shape)

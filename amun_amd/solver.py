@@ -13,7 +13,7 @@ import os
 
 import numpy as np
 
-from .config import AmunCfg, Params
+from .config import AmunCfg, AmunRefineCfg, Params
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libamun_hip.so")
@@ -110,12 +110,16 @@ def load_library(path: str = LIB_PATH):
                                             ctypes.c_int32, ctypes.c_double, _P, _P, _P]
     L.amun_dev_unit_cons2prim.argtypes = [ctypes.POINTER(AmunCfg), ctypes.c_int32, _P, _P,
                                           ctypes.POINTER(ctypes.c_int64)]
+    L.amun_dev_refinement.argtypes = [vp, ctypes.POINTER(AmunRefineCfg),
+                                      ctypes.POINTER(ctypes.c_int32),
+                                      ctypes.POINTER(ctypes.c_float)]
     L.amun_dev_cons2prim_noalloc.argtypes = [ctypes.POINTER(AmunCfg), ctypes.c_int32,
                                              ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                              ctypes.c_void_p]
     L.amun_dev_init_amr.argtypes = [ctypes.POINTER(AmunCfg), ctypes.c_int32,
                                     ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(vp)]
     L.amun_dev_amr_info.argtypes = [vp, ctypes.POINTER(ctypes.c_int32)]
+    L.amun_dev_amr_remesh.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]
     L.amun_dev_stream.restype = ctypes.c_void_p
     L.amun_dev_last_error.argtypes = []
     L.amun_dev_last_error.restype = ctypes.c_char_p
@@ -335,6 +339,24 @@ class Solver:
         self._check(self.lib.amun_dev_positivity_events(self.h, ctypes.byref(n)))
         return n.value
 
+    def refinement(self, rc=None, minlev: int = 0, maxlev: int = 0):
+        """check_refinement_criterion of every block of the current state
+        (refinement.F90:257): (criterion per block in {-1, 0, 1}, indicators
+        (block, 3): largest second-derivative error, vorticity, current
+        density).  rc: an AmunRefineCfg, by default from the run parameters
+        (config.refine_cfg)."""
+        from .config import refine_cfg
+        if rc is None:
+            rc = refine_cfg(self.params, minlev, maxlev)
+        n = ctypes.c_int32()
+        self._check(self.lib.amun_dev_nblocks(self.h, ctypes.byref(n)))
+        crit = np.zeros(n.value, dtype=np.int32)
+        ind = np.zeros((n.value, 3), dtype=np.float32)
+        self._check(self.lib.amun_dev_refinement(
+            self.h, ctypes.byref(rc), crit.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+            ind.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return crit, ind
+
     def tridiag_events(self) -> int:
         """Compact-scheme pencils whose tridiagonal solve met a zero pivot."""
         n = ctypes.c_uint64()
@@ -366,6 +388,15 @@ class AmrSolver(Solver):
             ctypes.byref(self.cfg), self.meta.shape[0],
             self.meta.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), ctypes.byref(h)))
         self.h = h
+
+    def remesh(self, meta: np.ndarray):
+        """Move the state onto the leaf set meta on the device (update_mesh's
+        copies, prolong_block and restrict_block); the solver then runs on
+        the new mesh from the same time and step."""
+        m = np.ascontiguousarray(meta, dtype=np.int32)
+        self._check(self.lib.amun_dev_amr_remesh(
+            self.h, m.shape[0], m.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+        self.meta = m
 
     def info(self) -> dict:
         a = np.zeros(6, dtype=np.int32)

@@ -159,6 +159,23 @@ typedef struct amun_cfg {
   double  resistivity;
 } amun_cfg;
 
+/* Refinement criterion of the leaf blocks (initialize_refinement,
+ * refinement.F90:99-190): the second-derivative error of the primitive
+ * variables in varmask (bit p: primitive variable p; the reference selects
+ * them by name from "refinement_variables"), the vorticity (vort != 0) and
+ * the current density (jabs != 0, magnetic systems) against their
+ * thresholds. */
+typedef struct amun_refine_cfg {
+  double  crefmin, crefmax;   /* "crefmin" (0.2), "crefmax" (0.8)             */
+  double  epsref;             /* "epsref" (0.01)                              */
+  double  vortmin, vortmax;   /* "vortmin" (1e-3), "vortmax" (0.1)            */
+  double  jabsmin, jabsmax;   /* "jabsmin" (1e-3), "jabsmax" (0.1)            */
+  int32_t varmask;            /* default: density and pressure                */
+  int32_t vort, jabs;
+  int32_t minlev, maxlev;     /* level limits of update_mesh (0: none)        */
+  int32_t reserved;
+} amun_refine_cfg;
+
 typedef struct amun_ctx amun_ctx;
 
 /* One independent uniform grid of a batch: base-level block counts and
@@ -323,6 +340,33 @@ int  amun_dev_positivity_events(amun_ctx* ctx, uint64_t* n);
  * non-positive pressure (set to 0), n[2] = cells whose Newton iteration
  * ended without reaching the tolerance.  All zero for HD/MHD. */
 int  amun_dev_c2p_events(amun_ctx* ctx, uint64_t n[3]);
+
+/* The refinement criterion of every leaf block of the current state
+ * (check_refinement_criterion, refinement.F90:257-640, as update_mesh calls
+ * it, mesh.F90:1545-1550): crit[b] = -1 (may derefine), 0 (keep) or 1
+ * (refine) in the block order of the context; with cfg->minlev / maxlev
+ * above 0 the level limits of update_mesh are applied.  ind (may be NULL)
+ * receives three floats per block: the largest second-derivative error of
+ * the selected variables, the vorticity and the current-density magnitude.
+ * The ghost cells it reads are refilled from the current state first. */
+int  amun_dev_refinement(amun_ctx* ctx, const amun_refine_cfg* cfg, int32_t* crit, float* ind);
+
+/* Moves a multi-level context onto the leaf set meta (nblk blocks, layout as
+ * amun_dev_init_amr) without a state transfer through the host: the device
+ * side of update_mesh's derefine_selected_blocks / refine_selected_blocks
+ * (mesh.F90:756-833, 2069-2257).  Every new block is an old leaf (its data
+ * kept), a child of an old leaf (prolong_block, mesh.F90:1031-1257: the
+ * parent's conserved cells with limited slopes, the prolongation limiter,
+ * density and energy slopes halved until positive) or the parent of 2^d old
+ * leaves (restrict_block, 1258-1454: the mean of the covering cells); 702
+ * otherwise, 700/701 when meta is not 2:1 balanced.  The handle, its stream,
+ * time, step, dt, event counters and launch settings stay; q and the maximum
+ * speed are recomputed as update_variables does after update_mesh
+ * (evolution.F90:321-340).  A forward tape is dropped.  Replaces the
+ * reference's mesh.F90 update_mesh data path (prolong_block /
+ * restrict_block called from refine/derefine_selected_blocks); which blocks
+ * change is the caller's choice (amun_dev_refinement gives the criterion). */
+int  amun_dev_amr_remesh(amun_ctx* ctx, int32_t nblk, const int32_t* meta);
 
 /* Pencils of the compact reconstructions (CRWENO5, CRMP) whose Thomas solve
  * met a zero pivot so far.  The reference's tridiag prints "Could not find

@@ -3806,6 +3806,268 @@ int orc_run(void* h, int nsteps, double dtnext, double* trace)
 
 long orc_positivity_events(void* h) { return ((ogrid*)h)->o.npos_events; }
 
+/* second_derivative_error (refinement.F90:359-462) of primitive variable p
+ * of one block with filled ghosts: the largest normalised second difference
+ * over the cells ibl..ieu, in the reference's mix of real(4) and real(8)
+ * (differences and magnitudes rounded to real(4), the quotient in real(8)
+ * with the real(4) sums promoted, the result and the running maximum in
+ * real(4)) */
+static float ref_second_derivative_error(const ocfg* o, const double* q, int p, double epsref)
+{
+  const double eps = 2.220446049250313e-16;
+  float error = 0.0f;
+  int i, j, k, d;
+  for (k = o->kbl; k <= o->keu; k++)
+    for (j = o->jbl; j <= o->jeu; j++)
+      for (i = o->ibl; i <= o->ieu; i++) {
+        const double qc = q[QI(o, p, i, j, k)];
+        for (d = 0; d < o->nd; d++) {
+          const int di = d == 0, dj = d == 1, dk = d == 2;
+          const double qp = q[QI(o, p, i + di, j + dj, k + dk)];
+          const double qm = q[QI(o, p, i - di, j - dj, k - dk)];
+          const float fr = (float)(qp - qc);
+          const float fl = (float)(qm - qc);
+          const float fc = (float)((fabs(qp) + fabs(qm)) + 2.0 * fabs(qc));
+          const float fsum = fabsf(fr) + fabsf(fl);
+          const float f = (float)((double)fabsf(fr + fl) /
+                                  (((double)fsum + epsref * (double)fc) + eps));
+          error = error > f ? error : f;
+        }
+      }
+  return error;
+}
+
+/* the largest |curl u| of the vector at variables i0..i0+2 (velocity or
+ * magnetic field) over the cells ibl..ieu (vorticity_magnitude /
+ * current_density_magnitude, refinement.F90:481-640, with curl of
+ * operators.F90:328-434: central differences with unit spacing, the 2D form
+ * without z derivatives); the squared magnitude in real(8), its running
+ * maximum in real(4) */
+static float ref_curl_max(const ocfg* o, const double* q, int i0)
+{
+  float wmax = 0.0f;
+  int i, j, k;
+#define D1(c, di, dj, dk) (0.5 * (q[QI(o, i0 + (c), i + (di), j + (dj), k + (dk))] - \
+                               q[QI(o, i0 + (c), i - (di), j - (dj), k - (dk))]) / 1.0)
+  for (k = o->kbl; k <= o->keu; k++)
+    for (j = o->jbl; j <= o->jeu; j++)
+      for (i = o->ibl; i <= o->ieu; i++) {
+        double wx, wy, wz;
+        if (o->nd == 3) {
+          wx = D1(2, 0, 1, 0) - D1(1, 0, 0, 1);
+          wy = D1(0, 0, 0, 1) - D1(2, 1, 0, 0);
+          wz = D1(1, 1, 0, 0) - D1(0, 0, 1, 0);
+        } else {
+          wx = D1(2, 0, 1, 0);
+          wy = -D1(2, 1, 0, 0);
+          wz = D1(1, 1, 0, 0) - D1(0, 0, 1, 0);
+        }
+        {
+          const float v = (float)((wx * wx + wy * wy) + wz * wz);
+          wmax = wmax > v ? wmax : v;
+        }
+      }
+#undef D1
+  return sqrtf(wmax);
+}
+
+/* check_refinement_criterion (refinement.F90:257-317) of every block:
+ * crit[b] = -1 (may derefine), 0 (keep) or 1 (refine) from the largest
+ * second-derivative error of the primitive variables in varmask, and the
+ * vorticity / current-density magnitudes when asked for; ind[3 b ..] (may be
+ * NULL) the three indicator values (the error is the largest over the
+ * variables).  par = crefmin, crefmax, epsref, vortmin, vortmax, jabsmin,
+ * jabsmax; flags = varmask (bit p: primitive variable p), vort, jabs. */
+int orc_refinement(void* h, const double* par, const int* flags, int* crit, float* ind)
+{
+  ogrid* g = (ogrid*)h;
+  const ocfg* o = &g->o;
+  const size_t bs = bsize(o);
+  int b, p;
+  for (b = 0; b < g->nblk; b++) {
+    const double* qb = g->q + (size_t)b * bs;
+    int c = -1;
+    float emax = 0.0f, w = 0.0f, jc = 0.0f;
+    for (p = 0; p < o->nv; p++)
+      if (flags[0] & (1 << p)) {
+        const float cref = ref_second_derivative_error(o, qb, p, par[2]);
+        if ((double)cref > par[0]) c = c > 0 ? c : 0;
+        if ((double)cref > par[1]) c = 1;
+        emax = emax > cref ? emax : cref;
+      }
+    if (flags[1]) {
+      w = ref_curl_max(o, qb, o->ivx);
+      if ((double)w > par[3]) c = c > 0 ? c : 0;
+      if ((double)w > par[4]) c = 1;
+    }
+    if (flags[2] && o->mhd) {
+      jc = ref_curl_max(o, qb, o->ibx);
+      if ((double)jc > par[5]) c = c > 0 ? c : 0;
+      if ((double)jc > par[6]) c = 1;
+    }
+    crit[b] = c;
+    if (ind) {
+      ind[3 * b] = emax;
+      ind[3 * b + 1] = w;
+      ind[3 * b + 2] = jc;
+    }
+  }
+  return 0;
+}
+
+/* prolong_block (mesh.F90:1031-1257) for one child cell and variable: the
+ * parent's cell value U[c] plus its limited slopes over the face neighbours
+ * U[c -/+ st[d]] (prolongation limiter at 1/2), halved until the parent's
+ * density or energy exceeds their absolute sum (set to zero with an event
+ * when that value is not positive), halved once more and combined for
+ * child position s = sx + 2 sy + 4 sz (sub-cell bit 0 = lower half). */
+static double prolong_block_cell(ocfg* o, const double* U, size_t c, const size_t st[3], int p,
+                                 int s)
+{
+  const int nd = o->nd;
+  const double uc = U[c + p];
+  double du[3] = {0.0, 0.0, 0.0};
+  int d;
+  for (d = 0; d < nd; d++)
+    du[d] = lim(o->c.prol_limiter, 0.5, uc - U[c - st[d] + p], U[c + st[d] + p] - uc);
+  if (p == o->idn || p == o->ien) {
+    if (uc > 0.0) {
+      int it;
+      for (it = 0; it < OMAXHALV; it++) {
+        double sa = fabs(du[0]) + fabs(du[1]);
+        if (nd == 3) sa = sa + fabs(du[2]);
+        if (!(uc <= sa)) break;
+        for (d = 0; d < nd; d++) du[d] = 0.5 * du[d];
+      }
+    } else {
+      o->npos_events++;
+      du[0] = du[1] = du[2] = 0.0;
+    }
+  }
+  for (d = 0; d < nd; d++) du[d] = 0.5 * du[d];
+  if (nd == 2) {
+    const double du1 = du[0] + du[1], du2 = du[0] - du[1];
+    return s == 0 ? uc - du1 : (s == 1 ? uc + du2 : (s == 2 ? uc - du2 : uc + du1));
+  } else {
+    const double du1 = du[0] + du[1] + du[2], du2 = du[0] - du[1] - du[2];
+    const double du3 = du[0] - du[1] + du[2], du4 = du[0] + du[1] - du[2];
+    switch (s) {
+    case 0: return uc - du1;
+    case 1: return uc + du2;
+    case 2: return uc - du3;
+    case 3: return uc + du4;
+    case 4: return uc - du4;
+    case 5: return uc + du3;
+    case 6: return uc - du2;
+    default: return uc + du1;
+    }
+  }
+}
+
+/* The data movement of update_mesh (mesh.F90:756-833) onto a new leaf set:
+ * every block of meta either exists in the old mesh (copied), is a child of
+ * an old leaf (prolong_block, mesh.F90:1031-1257, from the parent's
+ * conserved state including one layer of its ghost cells, which hold
+ * prim2cons of the filled primitive ghosts), or is the parent of 2^d old
+ * leaves (restrict_block, mesh.F90:1258-1454: the mean of the 2^d covering
+ * cells in the reference's summation order).  Only the interiors are moved;
+ * the new grid then fills its ghosts (update_variables after update_mesh,
+ * evolution.F90:321-340) and keeps the old grid's time, step and dt.
+ * Returns 702 when a block of meta has no such source. */
+int orc_amr_remesh(const amun_cfg* c, void* hold, int nblk, const int32_t* meta, void** hnew)
+{
+  ogrid* g = (ogrid*)hold;
+  ogrid* n;
+  ocfg* o;
+  const int nc = g->o.nc, nd = g->o.nd, half = nc / 2;
+  size_t bs, st[3];
+  int r, b, i, j, k, p;
+  *hnew = NULL;
+  if (!g->amr) return 600;
+  r = orc_create_amr(c, nblk, meta, (void**)&n);
+  if (r) return r;
+  o = &n->o;
+  bs = bsize(o);
+  st[0] = (size_t)o->nv;
+  st[1] = (size_t)o->im * o->nv;
+  st[2] = (size_t)o->im * o->jm * o->nv;
+  for (b = 0; b < nblk; b++) {
+    const int L = n->lev[b];
+    const long x0 = (long)n->bx[b] * nc, y0 = (long)n->by[b] * nc;
+    const long z0 = nd == 3 ? (long)n->bz[b] * nc : 0;
+    const int same = owner(g, L, x0, y0, z0);
+    double* dst = n->u0 + (size_t)b * bs;
+    if (same >= 0) {
+      memcpy(dst, g->u0 + (size_t)same * bs, bs * sizeof(double));
+      continue;
+    }
+    {
+      /* the old leaf covering this block one level down */
+      const int P = owner(g, L - 1, x0 / 2, y0 / 2, z0 / 2);
+      if (P >= 0) {
+        const double* U = g->u0 + (size_t)P * bs;
+        const int px = n->bx[b] & 1, py = n->by[b] & 1, pz = nd == 3 ? (n->bz[b] & 1) : 0;
+        for (k = 0; k < (nd == 3 ? nc : 1); k++)
+          for (j = 0; j < nc; j++)
+            for (i = 0; i < nc; i++) {
+              const int ci = (i + px * nc) / 2 + o->ib, cj = (j + py * nc) / 2 + o->jb;
+              const int ck = nd == 3 ? (k + pz * nc) / 2 + o->kb : 0;
+              const int s = (i & 1) + 2 * (j & 1) + (nd == 3 ? 4 * (k & 1) : 0);
+              const size_t cc = QI(o, 0, ci, cj, ck);
+              const size_t dc = QI(o, 0, i + o->ib, j + o->jb, k + o->kb);
+              for (p = 0; p < o->nv; p++)
+                dst[dc + p] = prolong_block_cell(&g->o, U, cc, st, p, s);
+            }
+        continue;
+      }
+    }
+    {
+      /* the 2^d old leaves one level up */
+      int kid[8], q;
+      for (q = 0; q < (1 << nd); q++) {
+        kid[q] = owner(g, L + 1, 2 * x0 + (q & 1) * nc, 2 * y0 + ((q >> 1) & 1) * nc,
+                       2 * z0 + ((q >> 2) & 1) * nc);
+        if (kid[q] < 0) { orc_destroy(n); return 702; }
+      }
+      for (k = 0; k < (nd == 3 ? nc : 1); k++)
+        for (j = 0; j < nc; j++)
+          for (i = 0; i < nc; i++) {
+            const int px = i >= half, py = j >= half, pz = nd == 3 && k >= half;
+            const double* U = g->u0 + (size_t)kid[px + 2 * py + 4 * pz] * bs;
+            const int fi = 2 * (i - px * half) + o->ib, fj = 2 * (j - py * half) + o->jb;
+            const int fk = nd == 3 ? 2 * (k - pz * half) + o->kb : 0;
+            const size_t l = QI(o, 0, fi, fj, fk);
+            const size_t dc = QI(o, 0, i + o->ib, j + o->jb, k + o->kb);
+            const size_t a = st[0], bb = st[1], e = st[2];
+            for (p = 0; p < o->nv; p++) {
+              if (nd == 2)
+                dst[dc + p] = 2.50e-01 * ((U[l + p] + U[l + a + bb + p]) +
+                                          (U[l + bb + p] + U[l + a + p]));
+              else
+                dst[dc + p] = 1.25e-01 * (((U[l + p] + U[l + a + bb + e + p]) +
+                                           (U[l + e + p] + U[l + a + bb + p])) +
+                                          ((U[l + bb + e + p] + U[l + a + p]) +
+                                           (U[l + bb + p] + U[l + a + e + p])));
+            }
+          }
+    }
+  }
+  n->time = g->time;
+  n->step = g->step;
+  n->dt = g->dt;
+  n->dtn = g->dtn;
+  o->npos_events = g->o.npos_events;
+  o->amr_err = 0;
+  update_variables(n, n->u0);
+  if (o->amr_err) {
+    r = o->amr_err;
+    orc_destroy(n);
+    return r;
+  }
+  *hnew = n;
+  return 0;
+}
+
 /* the MGP fallback record so far: count and position hash */
 void orc_mgp_fallbacks(void* h, unsigned long long* out)
 {

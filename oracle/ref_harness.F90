@@ -21,6 +21,8 @@
 !   harness_count  = number of blocks / pencils (flux, recon)
 !   harness_n      = pencil length (recon) or vector length (eqs)
 !   harness_cmax   = value of cmax used by the GLM fluxes (flux, eqs)
+!   harness_criterion = 1: also write check_refinement_criterion of every
+!                    block after the steps (amr)
 !
 ! File layouts (all raw FP64, C order written as Fortran order):
 !   run   in : U(nx,ny,nz,nv)            conserved variables of the whole grid
@@ -47,6 +49,12 @@
 !              after the initial boundary_variables
 !              <prefix>_mesh1.bin the same as mesh0 after harness_steps steps
 !              <prefix>_dt.bin (dt, cmax) for each step
+!              <prefix>_crit.bin with harness_criterion = 1: per data block
+!              check_refinement_criterion (refinement.F90:257) after the
+!              steps, as doubles, in the order of mesh1
+!              <prefix>_mesh2.bin with harness_update = 1: the blocks after
+!              one update_mesh (mesh.F90:756-833) that follows the steps,
+!              in the layout of mesh0 and the order of the new data list
 !         With harness_amr_input = 1 the block interiors are replaced by
 !         harness_input, written in the layout of <prefix>_mesh0.bin without
 !         the four leading numbers, before q is recomputed.
@@ -67,7 +75,7 @@ program amun_ref_harness
   use coordinates   , only : initialize_coordinates
   use coordinates   , only : im, jm, km, ib, ie, jb, je, kb, ke, nc
   use coordinates   , only : adx, ady, adz, xmin, ymin, zmin, ir, jr, kr
-  use coordinates   , only : toplev
+  use coordinates   , only : toplev, maxlev
   use equations     , only : initialize_equations, nv, cmax, cmax2
   use equations     , only : prim2cons, cons2prim, fluxspeed
   use equations     , only : update_primitive_variables
@@ -78,14 +86,14 @@ program amun_ref_harness
   use integrals     , only : finalize_integrals
   use interpolations, only : initialize_interpolations, reconstruct
   use io            , only : initialize_io
-  use mesh          , only : initialize_mesh, generate_mesh
+  use mesh          , only : initialize_mesh, generate_mesh, update_mesh
   use mpitools      , only : initialize_mpitools
   use operators     , only : initialize_operators
   use parameters    , only : read_parameters, get_parameter_integer
   use parameters    , only : get_parameter_real, get_parameter_string
   use problems      , only : initialize_problems
   use random        , only : initialize_random
-  use refinement    , only : initialize_refinement
+  use refinement    , only : initialize_refinement, check_refinement_criterion
   use schemes       , only : initialize_schemes, update_flux
   use shapes        , only : initialize_shapes
   use sources       , only : initialize_sources
@@ -96,7 +104,7 @@ program amun_ref_harness
   character(len=255) :: mode = "run", fin = "in.bin", fout = "out"
   integer            :: iret = 0, nsteps = 1, ncount = 1, n = 1
   real(kind=8)       :: hcmax = 1.0d+00, dtnext, tmax = 1.0d+30
-  integer            :: nrun = 1, amr_input = 0
+  integer            :: nrun = 1, amr_input = 0, crit_out = 0, mesh_update = 0
 
   call initialize_mpitools()
   call read_parameters(iret)
@@ -110,6 +118,8 @@ program amun_ref_harness
   call get_parameter_integer("harness_n"     , n     )
   call get_parameter_real   ("harness_cmax"  , hcmax )
   call get_parameter_integer("harness_amr_input", amr_input)
+  call get_parameter_integer("harness_criterion", crit_out)
+  call get_parameter_integer("harness_update", mesh_update)
   call get_parameter_real   ("tmax"          , tmax  )
   dtnext = 2.0d+00 * tmax
 
@@ -331,6 +341,23 @@ contains
          form="unformatted", status="replace")
     write(u) trace(:,1:nsteps)
     close(u)
+    if (crit_out == 1) then
+      open(newunit=u, file=trim(fout)//"_crit.bin", access="stream",         &
+           form="unformatted", status="replace")
+      pdata => list_data
+      do while (associated(pdata))
+        write(u) real(check_refinement_criterion(pdata), kind=8)
+        pdata => pdata%next
+      end do
+      close(u)
+    end if
+    if (mesh_update == 1) then
+! the ghost cells hold the state after the last stage's boundary_variables,
+! as in advance (evolution.F90:321-340)
+      toplev = maxlev
+      call update_mesh()
+      call dump_blocks(trim(fout)//"_mesh2.bin")
+    end if
 
   end subroutine mode_amr
 

@@ -257,6 +257,100 @@ def main_amr():
         json.dump(index, fh, indent=1)
 
 
+REFINE_CASES = (
+    ("refine_mhd2d_dens_pres", 2, 10, {"refinement_variables": "dens pres", "crefmin": 0.05,
+                                       "crefmax": 0.3, "epsref": 0.5}),
+    ("refine_mhd2d_pres_vort", 2, 10, {"refinement_variables": "pres vort", "crefmin": 0.05,
+                                       "crefmax": 0.3, "epsref": 0.5, "vortmin": 0.01,
+                                       "vortmax": 0.03}),
+    ("refine_mhd2d_velx_magy", 2, 10, {"refinement_variables": "dens velx magy vort jabs",
+                                       "crefmin": 0.05, "crefmax": 0.3, "epsref": 0.5,
+                                       "vortmin": 0.005, "vortmax": 0.02, "jabsmin": 0.003,
+                                       "jabsmax": 0.01}),
+    ("refine_mhd3d_all", 3, 4, {"refinement_variables": "dens pres vort jabs", "crefmin": 0.05,
+                                "crefmax": 0.3, "epsref": 0.5, "vortmin": 0.02, "vortmax": 0.1,
+                                "jabsmin": 0.02, "jabsmax": 0.1}),
+)
+
+
+def main_refine():
+    """check_refinement_criterion (refinement.F90:257-462) of every block of
+    the reference's three-level blast meshes after a few steps, with several
+    sets of refinement variables and thresholds (own generator)."""
+    from amun_amd import mesh
+    with open(os.path.join(HERE, "index.json")) as fh:
+        index = json.load(fh)
+    for name, nd, nsteps, rp in REFINE_CASES:
+        p = amr_params(nd)
+        p.extra.update(rp)
+        meta, _, _, _, _ = ol.ref_amr(p, 1)
+        u0 = mesh.block_state(p, meta, mesh.blast_prim(p))
+        m0, _, m1, u1, tr, crit = ol.ref_amr(p, nsteps, u_in=u0, criterion=True)
+        assert np.array_equal(m1, meta)
+        # the state after the steps is the oracle's (bit for bit, pinned by
+        # sha_u1); the fixture keeps the mesh and the criterion
+        np.savez_compressed(os.path.join(HERE, name + ".npz"),
+                            params=np.array(p.to_params_in()), ndims=nd, nsteps=nsteps,
+                            meta=meta, crit=crit, sha_u1=sha(u1))
+        index[name] = {"steps": nsteps, "blocks": int(meta.shape[0]),
+                       "crit": [int((crit == c).sum()) for c in (-1, 0, 1)]}
+        print(name, meta.shape[0], "blocks, criterion -1/0/1:", index[name]["crit"])
+    with open(os.path.join(HERE, "index.json"), "w") as fh:
+        json.dump(index, fh, indent=1)
+
+
+# (name, ndims, maxlev, steps, blast centre shift, refinement parameters):
+# the reference's mesh is generated for the blast at the domain centre, the
+# state is replaced by the seeded blast moved by the shift, and after the
+# steps one update_mesh refines around the new blast and derefines around
+# the old position
+REMESH_CASES = (
+    ("remesh_mhd2d_l3", 2, 3, 2, (0.25, 0.15, 0.0), {"crefmin": 0.05, "crefmax": 0.3,
+                                                     "epsref": 0.5}),
+    ("remesh_mhd2d_l4", 2, 4, 2, (0.3, 0.2, 0.0), {"crefmin": 0.1, "crefmax": 0.4,
+                                                   "epsref": 0.5}),
+    ("remesh_mhd3d_l3", 3, 3, 1, (0.25, 0.15, 0.1), {"crefmin": 0.05, "crefmax": 0.3,
+                                                     "epsref": 0.5}),
+)
+
+
+def remesh_params(nd, maxlev, rp):
+    p = amr_params(nd)
+    p.extra["maxlev"] = maxlev
+    p.extra.update(rp)
+    return p
+
+
+def main_remesh():
+    """One update_mesh of the reference (mesh.F90:756-833: the criterion,
+    the neighbour and sibling rules, derefine_selected_blocks with
+    restrict_block and refine_selected_blocks with prolong_block) after a
+    few steps on its generated mesh, the blast moved away (own generator)."""
+    from amun_amd import mesh
+    with open(os.path.join(HERE, "index.json")) as fh:
+        index = json.load(fh)
+    for name, nd, maxlev, nsteps, shift, rp in REMESH_CASES:
+        p = remesh_params(nd, maxlev, rp)
+        meta, _, _, _, _ = ol.ref_amr(p, 0)
+        u0 = mesh.block_state(p, meta, mesh.blast_prim(p, shift=shift))
+        _, _, m1, u1, tr, crit, (m2, u2) = ol.ref_amr(p, nsteps, u_in=u0, criterion=True,
+                                                      update=True)
+        assert np.array_equal(m1, meta)
+        sums = u2.reshape(u2.shape[0], -1, u2.shape[-1]).sum(axis=1)
+        np.savez_compressed(os.path.join(HERE, name + ".npz"),
+                            params=np.array(p.to_params_in()), ndims=nd, nsteps=nsteps,
+                            shift=np.array(shift), meta=meta, crit=crit, meta2=m2,
+                            block_sums2=sums, sha_u0=sha(u0), sha_u1=sha(u1), sha_u2=sha(u2))
+        s0 = {tuple(r) for r in meta.tolist()}
+        kept = sum(tuple(r) in s0 for r in m2.tolist())
+        index[name] = {"steps": nsteps, "blocks": int(meta.shape[0]),
+                       "blocks_after": int(m2.shape[0]), "kept": int(kept),
+                       "crit": [int((crit == c).sum()) for c in (-1, 0, 1)]}
+        print(name, index[name])
+    with open(os.path.join(HERE, "index.json"), "w") as fh:
+        json.dump(index, fh, indent=1)
+
+
 def main_hllc_mhd():
     """riemann_mhd_adi_hllc through update_flux (zero limiter), random states
     with weak, zero and strong Bx, and an Orszag-Tang run (own generator)."""
@@ -730,6 +824,12 @@ def main():
         return
     if sys.argv[1:] == ["ssprk"]:
         main_ssprk()
+        return
+    if sys.argv[1:] == ["refine"]:
+        main_refine()
+        sys.exit(0)
+    if sys.argv[1:] == ["remesh"]:
+        main_remesh()
         return
     if sys.argv[1:] == ["amr"]:
         main_amr()

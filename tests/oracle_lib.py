@@ -41,6 +41,8 @@ def lib():
         L.orc_positivity_events.argtypes = [ctypes.c_void_p]
         L.orc_positivity_events.restype = ctypes.c_long
         L.orc_mgp_fallbacks.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_ulonglong)]
+        L.orc_refinement.argtypes = [ctypes.c_void_p, _P, ctypes.POINTER(ctypes.c_int),
+                                     ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_float)]
         L.orc_update_flux.argtypes = [ctypes.c_void_p, _P, _P, ctypes.c_double, _P]
         L.orc_reconstruct.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                       ctypes.c_double, _P, _P, _P]
@@ -57,6 +59,9 @@ def lib():
         L.orc_set_blocks.argtypes = [ctypes.c_void_p, _P, ctypes.c_double]
         L.orc_get_blocks.argtypes = [ctypes.c_void_p, _P, _P]
         L.orc_amr_skip_correction.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_amr_remesh.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                     ctypes.POINTER(ctypes.c_int32),
+                                     ctypes.POINTER(ctypes.c_void_p)]
         _lib = L
     return _lib
 
@@ -111,6 +116,23 @@ class OracleRun:
         lib().orc_mgp_fallbacks(self.h, out)
         return int(out[0]), int(out[1])
 
+    def refinement(self, rc):
+        """check_refinement_criterion of every block (refinement.F90:257)
+        with the parameters of rc (config.AmunRefineCfg): (criterion per
+        block, indicators (block, 3): largest second-derivative error,
+        vorticity, current density)."""
+        nb = getattr(self, "meta", None)
+        nb = nb.shape[0] if nb is not None else self.p.nblocks
+        par = np.array([rc.crefmin, rc.crefmax, rc.epsref, rc.vortmin, rc.vortmax, rc.jabsmin,
+                        rc.jabsmax])
+        flags = (ctypes.c_int * 3)(rc.varmask, rc.vort, rc.jabs)
+        crit = np.zeros(nb, dtype=np.int32)
+        ind = np.zeros((nb, 3), dtype=np.float32)
+        lib().orc_refinement(self.h, _dp(par), flags,
+                             crit.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                             ind.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+        return crit, ind
+
     def c2p_events(self):
         ev = (ctypes.c_long * 3)()
         lib().orc_c2p_events(self.h, ev)
@@ -150,6 +172,23 @@ class OracleAmr(OracleRun):
         if r:
             raise RuntimeError(f"orc_get_blocks failed with code {r}")
         return (u, q) if with_q else u
+
+    def remesh(self, meta: np.ndarray) -> "OracleAmr":
+        """A new oracle grid on the leaf set meta with the state moved as
+        update_mesh moves it (copies, prolong_block, restrict_block); time,
+        step and dt carry over."""
+        new = OracleAmr.__new__(OracleAmr)
+        new.p = self.p
+        new.cfg = self.cfg
+        new.meta = np.ascontiguousarray(meta, dtype=np.int32)
+        h = ctypes.c_void_p()
+        r = lib().orc_amr_remesh(ctypes.byref(self.cfg), self.h, new.meta.shape[0],
+                                 new.meta.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                 ctypes.byref(h))
+        if r:
+            raise RuntimeError(f"orc_amr_remesh failed with code {r}")
+        new.h = h
+        return new
 
 
 def amr_meta(m: np.ndarray, nc: int) -> np.ndarray:
@@ -366,18 +405,25 @@ def _read_blocks(path, ndims, nv, nc):
     return meta, u
 
 
-def ref_amr(p: Params, nsteps: int, qdump: bool = False, u_in=None):
+def ref_amr(p: Params, nsteps: int, qdump: bool = False, u_in=None, criterion: bool = False,
+            update: bool = False):
     """The reference's own refined mesh (generate_mesh on the problem named
     in p.extra["problem"], up to p.extra["maxlev"]) and nsteps steps on it,
     frozen.  u_in (per-block interiors in the order of the mesh the same
     parameters generate) replaces the problem's state.  Returns (meta0, u0,
-    meta1, u1, trace[, q0 with ghosts])."""
+    meta1, u1, trace[, q0 with ghosts][, check_refinement_criterion of every
+    block after the steps][, (meta2, u2) after one update_mesh that follows
+    the steps])."""
     nv = {"hd": 5, "mhd": 9}[p.equation_system]
     with tempfile.TemporaryDirectory() as d:
         extra = {"harness_mode": "amr", "harness_output": "out", "harness_steps": nsteps}
         if u_in is not None:
             np.ascontiguousarray(u_in, dtype=np.float64).tofile(os.path.join(d, "in.bin"))
             extra.update({"harness_amr_input": 1, "harness_input": "in.bin"})
+        if criterion:
+            extra["harness_criterion"] = 1
+        if update:
+            extra["harness_update"] = 1
         _ref_exec(p, extra, d)
         m0, u0 = _read_blocks(os.path.join(d, "out_mesh0.bin"), p.ndims, nv, p.ncells)
         m1, u1 = _read_blocks(os.path.join(d, "out_mesh1.bin"), p.ndims, nv, p.ncells)
@@ -387,4 +433,8 @@ def ref_amr(p: Params, nsteps: int, qdump: bool = False, u_in=None):
             im = p.ncells + 2 * p.nghosts
             shp = (-1, im, im, nv) if p.ndims == 2 else (-1, im, im, im, nv)
             out.append(np.fromfile(os.path.join(d, "out_q0.bin")).reshape(shp))
+        if criterion:
+            out.append(np.fromfile(os.path.join(d, "out_crit.bin")).astype(np.int64))
+        if update:
+            out.append(_read_blocks(os.path.join(d, "out_mesh2.bin"), p.ndims, nv, p.ncells))
     return tuple(out)

@@ -281,3 +281,253 @@ def test_c3_amr_blocks_one_step_vs_oracle():
     print(f"  dt {dt:.15e} vs {tr[0, 0]:.15e}; max rel {err.max():.2e}")
     assert abs(dt - tr[0, 0]) <= 1e-13 * tr[0, 0]
     assert err.max() <= 1e-12
+
+
+# ----------------------------------------------------------- refinement
+REFINE = ["refine_mhd2d_dens_pres", "refine_mhd2d_pres_vort", "refine_mhd2d_velx_magy",
+          "refine_mhd3d_all"]
+
+
+@pytest.mark.parametrize("name", REFINE)
+def test_refinement_criterion_vs_reference(name):
+    """amun_dev_refinement on the reference's three-level blast meshes: on the
+    oracle's final state (bit for bit the reference's) the device criterion
+    equals the reference's block for block and its indicators the oracle's
+    bit for bit; after the same steps run on the device (a state within
+    round-off of the reference's) the criterion is the reference's again and
+    the indicators agree to float round-off."""
+    from amun_amd.config import refine_cfg
+    g = np.load(os.path.join(GOLD, name + ".npz"), allow_pickle=False)
+    p = Params.from_text(str(g["params"]), ndims=int(g["ndims"]))
+    meta = g["meta"]
+    nsteps = int(g["nsteps"])
+    u0 = mesh.block_state(p, meta, mesh.blast_prim(p))
+    o = ol.OracleAmr(p, meta)
+    o.set_blocks(u0)
+    o.run(nsteps)
+    rc = refine_cfg(p)
+    co, io = o.refinement(rc)
+    np.testing.assert_array_equal(co, g["crit"])
+    print(f"[case] refinement {name} blocks={len(meta)}", flush=True)
+    with AmrSolver(p, meta) as s:
+        s.upload(o.get_blocks())
+        cg, ig = s.refinement(rc)
+        np.testing.assert_array_equal(cg, g["crit"])
+        np.testing.assert_array_equal(ig, io)
+        s.upload(u0)
+        s.run(nsteps)
+        cr, ir = s.refinement(rc)
+    np.testing.assert_array_equal(cr, g["crit"])
+    np.testing.assert_allclose(ir, io, rtol=1e-5, atol=1e-7)
+
+
+def test_refinement_level_limits():
+    """minlev / maxlev as update_mesh applies them (mesh.F90:1547-1550): a
+    block below minlev refines, at minlev it does not derefine, at maxlev it
+    does not refine, above maxlev it derefines."""
+    g = np.load(os.path.join(GOLD, "refine_mhd2d_dens_pres.npz"), allow_pickle=False)
+    p = Params.from_text(str(g["params"]), ndims=2)
+    meta = g["meta"]
+    u0 = mesh.block_state(p, meta, mesh.blast_prim(p))
+    with AmrSolver(p, meta) as s:
+        s.upload(u0)
+        c0, _ = s.refinement()
+        c1, _ = s.refinement(minlev=2, maxlev=2)
+    lev = meta[:, 0]
+    assert (c1[lev < 2] == 1).all()
+    np.testing.assert_array_equal(c1[lev == 2], np.minimum(0, np.maximum(0, c0[lev == 2])))
+    assert (c1[lev > 2] == -1).all()
+
+
+# ----------------------------------------------------------- mesh update
+REMESH = ["remesh_mhd2d_l3", "remesh_mhd2d_l4", "remesh_mhd3d_l3"]
+
+
+def _sha(a):
+    import hashlib
+    return hashlib.sha256(np.ascontiguousarray(a, dtype=np.float64).tobytes()).hexdigest()
+
+
+def _edge_mask(p, meta, meta2):
+    """Per new block and cell: True where the block is a child of an old
+    leaf and the parent cell's slope stencil reaches outside the parent (it
+    then reads prim2cons of the primitive ghost values)."""
+    nc, nd = p.ncells, p.ndims
+    old = {tuple(m) for m in meta.tolist()}
+    shp = (nc, nc) if nd == 2 else (nc, nc, nc)
+    mask = np.zeros((len(meta2),) + shp, dtype=bool)
+    idx = np.indices(shp)                       # (k,) j, i order of the blocks' arrays
+    for b, (L, x, y, z) in enumerate(meta2.tolist()):
+        if (L, x, y, z) in old:
+            continue
+        par = (L - 1, (x // nc // 2) * nc, (y // nc // 2) * nc, (z // nc // 2) * nc if nd == 3 else 0)
+        if par not in old:
+            continue
+        off = [(x // nc) & 1, (y // nc) & 1, (z // nc) & 1]
+        m = np.zeros(shp, dtype=bool)
+        for a in range(nd):                   # axis a of (x, y, z) is array axis nd - 1 - a
+            c = (idx[nd - 1 - a] + off[a] * nc) // 2
+            m |= (c == 0) | (c == nc - 1)
+        mask[b] = m
+    return mask
+
+
+@pytest.mark.parametrize("name", REMESH)
+def test_remesh_reference_update(name):
+    """amun_dev_amr_remesh onto the leaf set of one reference update_mesh
+    (copies, prolong_block and restrict_block all occur).  From the oracle's
+    state before the update (bit for bit the reference's): kept and restricted
+    blocks, and prolonged cells whose parent stencil stays inside the parent,
+    are the reference's bit for bit; prolonged cells whose parent stencil
+    reads a ghost cell (prim2cons of the device's primitive ghosts, whose
+    cons2prim rounds differently) within 1e-14.  From the device's own run,
+    the host selection (mesh.update_leaves) from the device criterion gives
+    the reference's leaves, and the run continues on the new mesh as the
+    oracle's does."""
+    from amun_amd.config import refine_cfg
+    g = np.load(os.path.join(GOLD, name + ".npz"), allow_pickle=False)
+    p = Params.from_text(str(g["params"]), ndims=int(g["ndims"]))
+    meta, meta2 = g["meta"], g["meta2"]
+    nsteps = int(g["nsteps"])
+    u0 = mesh.block_state(p, meta, mesh.blast_prim(p, shift=tuple(g["shift"])))
+    o = ol.OracleAmr(p, meta)
+    o.set_blocks(u0)
+    o.run(nsteps)
+    uo = o.get_blocks()
+    assert _sha(uo) == str(g["sha_u1"])
+    ref = o.remesh(meta2).get_blocks()
+    assert _sha(ref) == str(g["sha_u2"])
+    edge = _edge_mask(p, meta, meta2)
+    print(f"[case] remesh {name} blocks {len(meta)} -> {len(meta2)}", flush=True)
+    with AmrSolver(p, meta) as s:
+        s.upload(uo)
+        s.remesh(meta2)
+        ud = s.download()
+    same = ud == ref
+    print(f"  from the reference's state: {int((~same).any(axis=-1).sum())} of "
+          f"{same[..., 0].size} cells differ, {int(edge.sum())} cells read ghosts")
+    assert same[~edge].all()
+    e = relerr(ud[edge], ref[edge]) if edge.any() else np.zeros(1)
+    print(f"  ghost-reading cells max rel {e.max():.2e}")
+    assert e.max() <= 1e-14
+    with AmrSolver(p, meta) as s:
+        s.upload(u0)
+        s.run(nsteps)
+        crit, _ = s.refinement(refine_cfg(p))
+        sel = mesh.update_leaves(p, meta, crit, 1, int(p.extra["maxlev"]))
+        assert {tuple(m) for m in sel.tolist()} == {tuple(m) for m in meta2.tolist()}
+        ud = s.download()
+        t0, k0 = s.time, s.step
+        s.remesh(meta2)
+        assert s.time == t0 and s.step == k0 == nsteps
+        un = s.download()
+        on = ol.OracleAmr(p, meta)
+        on.set_blocks(ud)
+        on = on.remesh(meta2)
+        e = relerr(un.reshape(-1, p.nv), on.get_blocks().reshape(-1, p.nv))
+        print(f"  from the device's run: vs the oracle's update of that state max rel {e.max():.2e}")
+        assert e.max() <= 1e-14
+        dts = s.run(2)
+        on.run(2)
+        e2 = relerr(s.download().reshape(-1, p.nv), on.get_blocks().reshape(-1, p.nv))
+        print(f"  two more steps: dt {dts}, vs oracle max rel {e2.max():.2e}")
+        assert np.delete(e2, 7).max() <= 1e-9
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_remesh_identity_continues_bitwise(precision):
+    """A mesh update onto the same leaves (every block copied, in a new
+    order) changes nothing: the run continues bit for bit as the one that
+    was not updated (state, dt, time; the FP32 compensation of the mixed
+    mode is carried with its blocks)."""
+    p, meta, u0 = _case(2, precision=precision)
+    perm = np.random.default_rng(4).permutation(len(meta))
+    with AmrSolver(p, meta) as a, AmrSolver(p, meta) as b:
+        a.upload(u0)
+        b.upload(u0)
+        a.run(3)
+        b.run(3)
+        b.remesh(meta[perm])
+        da, db = a.run(3), b.run(3)
+        np.testing.assert_array_equal(db, da)
+        np.testing.assert_array_equal(b.download(), a.download()[perm])
+        assert a.time == b.time and a.step == b.step
+
+
+def test_remesh_rejects_level_jumps_and_keeps_state():
+    """A new leaf set that moves a block by two levels, or is not 2:1
+    balanced, is refused (702 / 700) and the context runs on unchanged."""
+    from amun_amd.solver import AmunError
+    p, meta, u0 = _case(2)
+    lv = meta[:, 0]
+    b3 = int(np.flatnonzero(lv == 3)[0])
+    L, x, y, z = (int(v) for v in meta[b3])
+    bad = meta.copy()
+    bad[b3] = (1, (x // 4) // p.ncells * p.ncells, (y // 4) // p.ncells * p.ncells, 0)
+    with AmrSolver(p, meta) as a, AmrSolver(p, meta) as b:
+        a.upload(u0)
+        b.upload(u0)
+        with pytest.raises(AmunError) as ei:
+            b.remesh(bad)
+        assert ei.value.code in (500, 700, 702)
+        np.testing.assert_array_equal(b.run(2), a.run(2))
+        np.testing.assert_array_equal(b.download(), a.download())
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_c3_amr_adapts_on_device(precision):
+    """The C3 AMR mesh (6224 leaves of 16^3) adapting without a state
+    transfer through the host: two steps, the device criterion, the host
+    selection, the device mesh update, two more steps.  The volume integrals
+    of mass, momentum, energy and B are conserved through the update to
+    round-off, refining every leaf of a region and derefining it again
+    returns the state to round-off, and the update's duration is printed."""
+    import time
+    from amun_amd.config import refine_cfg
+    p, meta = c3_amr_case(precision)
+    p.extra.update({"refinement_variables": "dens pres", "crefmin": 0.2, "crefmax": 0.5,
+                    "epsref": 0.5})
+    u0 = mesh.block_state(p, meta, mesh.blast_prim(p, r0=0.2, shift=(0.05, 0.03, 0.0)))
+    print(f"[case] c3 amr adapt {precision} leaves={len(meta)}", flush=True)
+
+    def totals(u, m):
+        vol = _volumes(p, m)
+        return (u.reshape(u.shape[0], -1, u.shape[-1]).sum(axis=1) * vol[:, None]).sum(axis=0)
+    with AmrSolver(p, meta) as s:
+        s.upload(u0)
+        s.run(2)
+        crit, _ = s.refinement(refine_cfg(p))
+        new = mesh.update_leaves(p, meta, crit, 1, 3)
+        ub = s.download()
+        s.sync()
+        t = time.perf_counter()
+        s.remesh(new)
+        s.sync()
+        ms = (time.perf_counter() - t) * 1e3
+        ua = s.download()
+        lv0, lv1 = np.bincount(meta[:, 0]), np.bincount(new[:, 0])
+        d = np.abs(totals(ua, new) - totals(ub, meta))[[0, 1, 2, 3, 4, 5, 6, 8]]
+        print(f"  leaves per level {lv0[1:].tolist()} -> {lv1[1:].tolist()}, update "
+              f"{ms:.1f} ms (host tables + one launch), conservation {d.max() / np.abs(ub).max():.2e}")
+        assert (lv1[1:4] != lv0[1:4]).any()
+        assert d.max() <= 1e-12 * np.abs(ub).max()
+        dts = s.run(2)
+        assert np.isfinite(s.download()).all() and (dts > 0).all()
+        # refine eight level-1 leaves (their neighbours are on levels 1 and 2,
+        # so the children keep the 2:1 balance) and derefine them again
+        lev1 = [b for b in range(len(new)) if new[b, 0] == 1][:8]
+        kids = []
+        nc = p.ncells
+        for b in lev1:
+            _, x, y, z = (int(v) for v in new[b])
+            kids += [(2, 2 * x + (q & 1) * nc, 2 * y + (q >> 1 & 1) * nc, 2 * z + (q >> 2 & 1) * nc)
+                     for q in range(8)]
+        fine = np.array([tuple(m) for i, m in enumerate(new.tolist()) if i not in lev1] + kids,
+                        dtype=np.int32)
+        uc = s.download()
+        s.remesh(fine)
+        s.remesh(new)
+        ur = s.download()
+        e = relerr(ur.reshape(-1, p.nv), uc.reshape(-1, p.nv))
+        print(f"  refine + derefine of {len(lev1)} leaves: max rel {e.max():.2e}")
+        assert e.max() <= 1e-14

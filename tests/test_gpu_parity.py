@@ -921,3 +921,40 @@ def test_boundary_types_mixed_and_rejections():
     with pytest.raises(AmunError) as e:
         Solver(p)
     assert e.value.code == 600
+
+
+def test_refinement_indicator_closed_forms_device():
+    """amun_dev_refinement on single-level periodic grids against the oracle
+    on a spike, a solid-body rotation (vorticity) and a current sheet, FP64
+    bit for bit, and the mixed mode (FP32 primitives) to float round-off."""
+    from amun_amd.config import refine_cfg
+    p = Params(ndims=2, equation_system="mhd", reconstruction="tvd", limiter="mc",
+               riemann_solver="hlld", ncells=8, nghosts=2, xblocks=3, yblocks=3)
+    nz, ny, nx = p.shape
+    J, I = np.meshgrid(np.arange(ny, dtype=float), np.arange(nx, dtype=float), indexing="ij")
+    q = np.zeros((9, 1, ny, nx))
+    q[0] = 1.0 + 0.5 * ((I == 12) & (J == 11))
+    q[1] = -0.25 * J
+    q[2] = 0.25 * I
+    q[5] = 0.1 * np.sin(2.0 * np.pi * I / nx)
+    q[8] = 1.0 + 0.3 * np.cos(2.0 * np.pi * J / ny)
+    u0 = prim_to_cons(p, q)
+    p.extra.update({"refinement_variables": "dens pres vort jabs", "vortmin": 0.1,
+                    "vortmax": 1.0, "jabsmin": 0.01, "jabsmax": 0.05})
+    rc = refine_cfg(p)
+    o = ol.OracleRun(p)
+    o.set_state(u0)
+    co, io = o.refinement(rc)
+    with Solver(p) as s:
+        s.upload(u0)
+        cg, ig = s.refinement(rc)
+    print(f"criterion {cg.tolist()}")
+    np.testing.assert_array_equal(cg, co)
+    np.testing.assert_array_equal(ig, io)
+    pm = Params(**{k: v for k, v in p.__dict__.items() if k != "extra"})
+    pm.precision = "mixed"
+    with Solver(pm) as s:
+        s.upload(u0)
+        cm, im = s.refinement(rc)
+    np.testing.assert_array_equal(cm, co)
+    np.testing.assert_allclose(im, io, rtol=1e-5, atol=1e-6)

@@ -451,3 +451,61 @@ def test_crweno_pencils_and_runs(rec):
     tr = o.run(int(r["nsteps"]))
     np.testing.assert_array_equal(tr, r["trace"])
     assert sha(o.get_state()) == str(r["sha_u"])
+
+
+# ----------------------------------------------- refinement criterion
+@pytest.mark.parametrize("name", ["refine_mhd2d_dens_pres", "refine_mhd2d_pres_vort",
+                                  "refine_mhd2d_velx_magy", "refine_mhd3d_all"])
+def test_refinement_criterion(name):
+    """check_refinement_criterion (second_derivative_error, vorticity and
+    current-density magnitudes, refinement.F90:257-640) of every block of the
+    reference's three-level blast meshes after a few steps, four sets of
+    variables and thresholds: the oracle's run reproduces the reference's
+    state (sha) and its criterion block for block."""
+    from amun_amd import mesh
+    from amun_amd.config import refine_cfg
+    r = load(name)
+    p = params_of(r)
+    meta = r["meta"]
+    o = ol.OracleAmr(p, meta)
+    o.set_blocks(mesh.block_state(p, meta, mesh.blast_prim(p)))
+    o.run(int(r["nsteps"]))
+    assert sha(o.get_blocks()) == str(r["sha_u1"])
+    crit, ind = o.refinement(refine_cfg(p))
+    np.testing.assert_array_equal(crit, r["crit"])
+
+
+# ----------------------------------------------- mesh update (remesh)
+@pytest.mark.parametrize("name", ["remesh_mhd2d_l3", "remesh_mhd2d_l4", "remesh_mhd3d_l3"])
+def test_amr_remesh_matches_update_mesh(name):
+    """One update_mesh of the reference (mesh.F90:756-833) after a few steps
+    on its generated blast mesh with the blast moved away: blocks kept,
+    prolonged from a refined parent (prolong_block, 1031-1257) and restricted
+    from derefined children (restrict_block, 1258-1454) all occur; the
+    oracle's data movement onto the reference's new leaf set reproduces the
+    reference's state after the update bit for bit (sha), and the host's
+    selection (amun_amd.mesh.update_leaves) picks the reference's new leaf
+    set from the reference's criterion."""
+    from amun_amd import mesh
+    r = load(name)
+    p = params_of(r)
+    meta, meta2 = r["meta"], r["meta2"]
+    u0 = mesh.block_state(p, meta, mesh.blast_prim(p, shift=tuple(r["shift"])))
+    assert sha(u0) == str(r["sha_u0"])
+    o = ol.OracleAmr(p, meta)
+    o.set_blocks(u0)
+    o.run(int(r["nsteps"]))
+    assert sha(o.get_blocks()) == str(r["sha_u1"])
+    old = {tuple(m) for m in meta.tolist()}
+    new = {tuple(m) for m in meta2.tolist()}
+    nc = p.ncells
+    refined = [m for m in new - old if (m[0] - 1, *((x // nc // 2) * nc for x in m[1:])) in old]
+    assert refined and len(new & old) and len(new - old) > len(refined)   # all three kinds
+    # the host's selection from the reference's criterion gives the same leaves
+    sel = mesh.update_leaves(p, meta, r["crit"], 1, int(p.extra["maxlev"]))
+    assert {tuple(m) for m in sel.tolist()} == new and len(sel) == len(meta2)
+    n = o.remesh(meta2)
+    u2 = n.get_blocks()
+    np.testing.assert_array_equal(u2.reshape(u2.shape[0], -1, p.nv).sum(axis=1),
+                                  r["block_sums2"])
+    assert sha(u2) == str(r["sha_u2"])

@@ -686,3 +686,161 @@ def test_crweno_exact_for_lines_and_tridiagonal(rec):
     sl = slice(2, n - 3)
     np.testing.assert_allclose(fl[0, sl], face[sl], rtol=0, atol=1e-12)
     np.testing.assert_allclose(fr[0, sl], face[sl], rtol=0, atol=1e-12)
+
+
+# ----------------------------------------------- refinement indicators
+def _refine_grid(q_of, eq="hd"):
+    """A 3x3-block 2D grid of 8^2 cells from primitive fields q_of(I, J) of
+    the global cell indices, the oracle's blocks with filled ghosts."""
+    p = Params(ndims=2, equation_system=eq, reconstruction="tvd", limiter="mc",
+               riemann_solver="hllc" if eq == "hd" else "hlld", ncells=8, nghosts=2,
+               xblocks=3, yblocks=3)
+    nz, ny, nx = p.shape
+    J, I = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    q = q_of(I.astype(float), J.astype(float))[:, None]
+    o = ol.OracleRun(p)
+    o.set_state(problems.prim_to_cons(p, q))
+    return p, o
+
+
+def test_refinement_indicator_closed_forms():
+    """second_derivative_error on a single spike and the vorticity of a
+    solid-body rotation against their closed forms, in the reference's
+    real(4)/real(8) mix: a spike of height d in a uniform field gives, at its
+    cell, |fr + fl| / (|fr| + |fl| + epsref fc + eps) with fr = fl = -d and
+    fc = 4 + 2 d; the rotation v = a (-J, I) has curl 2a at every interior
+    cell (central differences with unit spacing are exact on it); a uniform
+    state has no indicator at all."""
+    from amun_amd.config import refine_cfg
+    d, a = 0.5, 0.25
+
+    def spike(I, J):
+        q = np.zeros((5,) + I.shape)
+        q[0] = 1.0 + d * ((I == 12) & (J == 11))
+        q[4] = 1.0
+        return q
+    p, o = _refine_grid(spike)
+    p.extra.update({"refinement_variables": "dens", "epsref": 0.01})
+    crit, ind = o.refinement(refine_cfg(p))
+    f32 = np.float32
+    fr = fl = f32(-d)
+    fc = f32((abs(1.0) + abs(1.0)) + 2.0 * abs(1.0 + d))
+    expect = f32(float(abs(f32(fr + fl))) /
+                 ((float(f32(abs(fr) + abs(fl))) + 0.01 * float(fc)) + 2.220446049250313e-16))
+    assert ind[4, 0] == expect            # the spike sits in the centre block
+    assert crit[4] == 1
+    # blocks whose cells ibl..ieu do not reach the spike see a uniform field
+    assert ind[0, 0] == 0.0 and crit[0] == -1
+
+    def rotation(I, J):
+        q = np.zeros((5,) + I.shape)
+        q[0] = 1.0
+        q[1] = -a * J
+        q[2] = a * I
+        q[4] = 1.0
+        return q
+    p, o = _refine_grid(rotation)
+    p.extra.update({"refinement_variables": "vort", "vortmin": 0.1, "vortmax": 1.0})
+    crit, ind = o.refinement(refine_cfg(p))
+    assert ind[4, 1] == np.float32(2 * a)
+    assert crit[4] == 0                   # 0.1 < 0.5 <= 1.0
+
+    def uniform(I, J):
+        q = np.zeros((5,) + I.shape)
+        q[0], q[1], q[4] = 1.3, 0.2, 0.7
+        return q
+    p, o = _refine_grid(uniform)
+    p.extra.update({"refinement_variables": "dens pres vort"})
+    crit, ind = o.refinement(refine_cfg(p))
+    assert (ind == 0).all() and (crit == -1).all()
+
+
+# ----------------------------------------------- mesh update (remesh)
+def _remesh_params(nd, prol="mc"):
+    return Params(ndims=nd, equation_system="hd", reconstruction="tvd", limiter="mc",
+                  riemann_solver="hll", time_advance="rk3", gamma=1.4, cfl=0.4, ncells=8,
+                  nghosts=2, xblocks=4, yblocks=4, zblocks=4 if nd == 3 else 1,
+                  xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0, zmin=0.0, zmax=1.0,
+                  prolongation_limiter=prol)
+
+
+def _base_meta(p):
+    nc = p.ncells
+    kb = p.zblocks if p.ndims == 3 else 1
+    return np.array([(1, i * nc, j * nc, k * nc) for k in range(kb) for j in range(p.yblocks)
+                     for i in range(p.xblocks)], dtype=np.int32)
+
+
+def _refine(meta, which, nd, nc):
+    """meta with block index `which` replaced by its 2^nd children."""
+    L, x, y, z = (int(v) for v in meta[which])
+    kids = [(L + 1, 2 * x + (q & 1) * nc, 2 * y + (q >> 1 & 1) * nc,
+             2 * z + (q >> 2 & 1) * nc if nd == 3 else 0) for q in range(1 << nd)]
+    rest = [tuple(m) for i, m in enumerate(meta.tolist()) if i != which]
+    return np.array(rest + kids, dtype=np.int32)
+
+
+def _linear_state(p, meta, coef):
+    """Conserved HD state linear in the cell centres: u_v = c0 + c1 x + c2 y
+    (+ c3 z) per variable, gas at rest."""
+    from amun_amd import mesh
+    x, y, z = mesh.cell_centres(p, meta)
+    u = np.zeros(x.shape + (p.nv,))
+    for v, c in coef.items():
+        u[..., v] = c[0] + c[1] * x + c[2] * y + (c[3] * z if p.ndims == 3 else 0.0)
+    return u
+
+
+@pytest.mark.parametrize("nd", [2, 3])
+@pytest.mark.parametrize("prol", ["mm", "mc", "vl", "sb"])
+def test_remesh_linear_field_closed_forms(nd, prol):
+    """On a linear field every prolongation limiter returns the exact slope,
+    so the children prolong_block makes hold the field at their own centres
+    (a swapped sub-cell combination or slope sign moves them by h/2), and
+    restrict_block's mean of the children is the field at the parent's
+    centre.  The parent's slope stencil meets its neighbours' cells through
+    prim2cons of the ghost primitives, hence the 1e-13 tolerance."""
+    p = _remesh_params(nd, prol)
+    coef = {0: (2.0, 0.5, -0.25, 0.125), 4: (10.0, -1.0, 0.75, 0.5)}
+    meta = _base_meta(p)
+    centre = 5 if nd == 2 else 21                 # block (1, 1[, 1]), away from the wrap
+    o = ol.OracleAmr(p, meta)
+    o.set_blocks(_linear_state(p, meta, coef))
+    fine = _refine(meta, centre, nd, p.ncells)
+    n = o.remesh(fine)
+    got = n.get_blocks()
+    want = _linear_state(p, fine, coef)
+    np.testing.assert_allclose(got, want, rtol=1e-13, atol=0)
+    # and back: the children restricted onto the parent
+    back = n.remesh(meta)
+    np.testing.assert_allclose(back.get_blocks(), _linear_state(p, meta, coef), rtol=1e-13,
+                               atol=0)
+
+
+@pytest.mark.parametrize("nd", [2, 3])
+def test_remesh_conserves_and_round_trips(nd):
+    """prolong_block's children sum to 2^d times the parent cell and
+    restrict_block averages them back: refining every block of a blast state
+    and derefining again conserves every variable's volume integral and
+    returns the state to round-off (a dropped slope term or a wrong sub-cell
+    weight breaks one of the two)."""
+    from amun_amd import mesh
+    p = _remesh_params(nd)
+    p.xblocks = p.yblocks = 2
+    p.zblocks = 2 if nd == 3 else 1
+    meta = _base_meta(p)
+    u0 = mesh.block_state(p, meta, mesh.blast_prim(p))
+    o = ol.OracleAmr(p, meta)
+    o.set_blocks(u0)
+    fine = meta
+    for b in range(meta.shape[0]):
+        fine = _refine(fine, 0, nd, p.ncells)       # block 0 is always an original
+    n = o.remesh(fine)
+    uf = n.get_blocks()
+    vol = 2.0 ** -nd
+    tot0 = u0.reshape(-1, p.nv).sum(axis=0)
+    tot1 = uf.reshape(-1, p.nv).sum(axis=0) * vol
+    np.testing.assert_allclose(tot1, tot0, rtol=1e-13)   # summation over 2^15 cells
+    back = n.remesh(meta).get_blocks()
+    np.testing.assert_allclose(back, u0, rtol=1e-14, atol=1e-14 * np.abs(u0).max())
+    assert not np.array_equal(uf[:4], np.repeat(u0[:1], 4, axis=0))   # slopes were used
