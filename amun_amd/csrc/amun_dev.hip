@@ -5,6 +5,7 @@
 // inst_tvd.hip, inst_weno5z.hip and inst_gp.hip.
 #include "kernels.h"
 #include "remesh.h"
+#include "sources.h"
 
 #include <array>
 #include <map>
@@ -31,6 +32,7 @@ struct amun_ctx {
   SpeedPart* spart = nullptr;  // per-workgroup maximum speeds of the last sweep
   int nspart = 0;              // entries of spart (workgroups of >= 64 lanes)
   double* ugp = nullptr;       // MGP weights (RunConst::ugp points here)
+  void* src = nullptr;         // GLM source products per stage (element size qsize)
   int qcur = 0;                // Q[qcur] holds the primitive state of U0
   long long nsteps = 0;        // steps launched (slot = nsteps & 1)
   hipStream_t stream = nullptr;
@@ -160,8 +162,8 @@ int validate(const amun_cfg& c, int* nv) {
   if (c.mlp_limiting) return fail(600, "mlp_limiting is not supported on the device path");
   if (c.fix_unphysical_cells)
     return fail(600, "fix_unphysical_cells is not supported on the device path");
-  if (c.glm_source_terms)
-    return fail(600, "glm_source_terms eglm/heglm are not supported on the device path");
+  if (c.glm_source_terms < 0 || c.glm_source_terms > 2)
+    return fail(500, "glm_source_terms must be 0 (none), 1 (eglm) or 2 (heglm)");
   if (c.viscosity != 0.0 || c.resistivity != 0.0)
     return fail(600, "viscosity / resistivity sources are not supported on the device path");
   if (c.ncells < 4 || c.ncells > 64 || c.ncells % 4)
@@ -525,6 +527,10 @@ static int init_device(amun_ctx* c, const std::vector<int32_t>& nbr,
     const long long npen = nd == 3 ? (long long)rc.nc * rc.nc : rc.nc;
     c->nspart = (int)((rc.nblk * npen + 63) / 64);
     ALLOC(c->spart, (size_t)c->nspart * sizeof(SpeedPart));
+  }
+  if (cfg->glm_source_terms > 0 && nv == 9) {
+    // the eight GLM source products of every cell (k_glm_src)
+    ALLOC(c->src, (size_t)rc.nblk * 8 * rc.ncell * c->qsize);
   }
   if (is_compact(cfg->reconstruction)) {
     // face states of one direction: [block][var][cell -1 .. nc][pencil]
@@ -906,6 +912,7 @@ void amun_dev_finalize(amun_ctx* c) {
   hipFree(c->spart);
   hipFree(c->rc.fsl);
   hipFree(c->ugp);
+  hipFree(c->src);
   hipFree(c->rc.fsr);
   hipFree(c->amr.blev);
   hipFree(c->bnd.task);
@@ -1236,6 +1243,36 @@ static int amr_fill(amun_ctx* c, int qcur, int slot) {
   return 0;
 }
 
+// the GLM source products of the stage input Q[qcur] (k_glm_src)
+static int glm_sources(amun_ctx* c, int qcur) {
+  const RunConst& rc = c->rc;
+  const long long n = (long long)rc.nblk * rc.ncell;
+  const dim3 grid((unsigned)((n + 255) / 256)), bl(256);
+  const int* blev = c->amr.on ? c->amr.blev : nullptr;
+  const bool sr = c->cfg.eqsys == AMUN_EQSYS_SRMHD;
+  if (c->qsize == sizeof(double)) {
+    const double* Q = static_cast<const double*>(c->Q[qcur]);
+    double* S = static_cast<double*>(c->src);
+    if (sr)
+      hipLaunchKernelGGL((k_glm_src<amun::SRMHD, double>), grid, bl, 0, c->stream, Q, c->nbr, rc,
+                         c->rs, c->brun, blev, S);
+    else
+      hipLaunchKernelGGL((k_glm_src<amun::MHD, double>), grid, bl, 0, c->stream, Q, c->nbr, rc,
+                         c->rs, c->brun, blev, S);
+  } else {
+    const float* Q = static_cast<const float*>(c->Q[qcur]);
+    float* S = static_cast<float*>(c->src);
+    if (sr)
+      hipLaunchKernelGGL((k_glm_src<amun::SRMHD, float>), grid, bl, 0, c->stream, Q, c->nbr, rc,
+                         c->rs, c->brun, blev, S);
+    else
+      hipLaunchKernelGGL((k_glm_src<amun::MHD, float>), grid, bl, 0, c->stream, Q, c->nbr, rc,
+                         c->rs, c->brun, blev, S);
+  }
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
 static int enqueue_step(amun_ctx* c, double dtnext, int slot, int qcur, hipEvent_t* ev,
                         double* rec = nullptr) {
   Dispatch& d = dispatch_of(c);
@@ -1255,7 +1292,13 @@ static int enqueue_step(amun_ctx* c, double dtnext, int slot, int qcur, hipEvent
       int r = bnd_fill(c, qcur);
       if (r) return r;
     }
+    if (c->src) {
+      int r = glm_sources(c, qcur);
+      if (r) return r;
+    }
     SweepArgs a;
+    a.src = c->src;
+    a.glm = c->cfg.glm_source_terms;
     a.blev = c->amr.on ? c->amr.blev : nullptr;
     a.ffix = c->amr.on ? c->amr.ffix : nullptr;
     a.FB = c->amr.on ? c->amr.FB : nullptr;
@@ -1948,6 +1991,7 @@ static int tape_check(amun_ctx* c) {
   if (c->bnd.on) return fail(600, "gradients are built for periodic domains");
   if (!c->disp.has_adj)
     return fail(600, "gradients need TVD reconstruction and the HD or MHD system");
+  if (c->src) return fail(600, "gradients do not include the GLM source terms");
   return 0;
 }
 

@@ -253,6 +253,10 @@ struct SweepArgs {
   const int* __restrict__ blev;
   const int* __restrict__ ffix;
   const double* __restrict__ FB;
+  // GLM source terms (null when off): the eight products per cell of
+  // k_glm_src, and the kind (1 EGLM, 2 HEGLM)
+  const void* src;
+  int glm;
 };
 
 // stage combines of evolve_euler/rk2/rk3 (evolution.F90:570, 700, 740, 1080,
@@ -1199,6 +1203,25 @@ __global__ __launch_bounds__(256, sweep_min_waves<T>()) void k_sweep(SweepArgs a
             if constexpr (DIR == 0) du = -dxi * dfx;
             else if constexpr (kDin) du = sd[(((c - 1) & 3) * NV + v) * wg + tid] - dxi * dfx;
             else du = dpre[v] - dxi * dfx;
+            if constexpr (kLast && P::mhd) {
+              // GLM source terms after the completed increment
+              // (update_sources, sources.F90:479-539; products from k_glm_src)
+              if (a.src) {
+#pragma clang fp contract(off)
+                const T* Sp = static_cast<const T*>(a.src) + (size_t)b * 8 * rc.ncell + (bt - bb);
+                const size_t n1 = (size_t)rc.ncell;
+                if (v == P::imx) du = du - Sp[0];
+                if (v == P::imy) du = du - Sp[n1];
+                if (v == P::imz) du = du - Sp[2 * n1];
+                if (v == P::ien) du = du - Sp[3 * n1];
+                if (a.glm > 1) {
+                  if (v == P::ibx) du = du - Sp[4 * n1];
+                  if (v == P::iby) du = du - Sp[5 * n1];
+                  if (v == P::ibz) du = du - Sp[6 * n1];
+                  if (v == P::ien) du = du - Sp[7 * n1];
+                }
+              }
+            }
             if constexpr (!kLast) {
               if constexpr (kDout) sd[(((c - 1) & 3) * NV + v) * wg + tid] = du;
               else Dp[bt + off] = du;

@@ -313,3 +313,17 @@ def near_plane(p: Params, y0: float, width: float):
         lo = p.ymin + by * h
         return lo - width < y0 < lo + h + width
     return wants
+
+
+def c3_amr_params(precision: str = "fp64", nb: int = 8, r: float = 0.3):
+    """The 3D block-AMR run of the batched/AMR configuration: three levels of
+    16^3 blocks (+2 ghosts) on a periodic nb^3 base, refined towards a sphere
+    of radius r (6224 leaves: 280 / 1272 / 4672 on levels 1 / 2 / 3 for
+    nb = 8, r = 0.3), MHD, TVD-MC + HLLD + SSPRK(3,3), CFL 0.4.  Returns
+    (Params, meta); the state is the blast of blast_prim(p, r0=0.2)."""
+    p = Params(ndims=3, equation_system="mhd", reconstruction="tvd", limiter="mc",
+               riemann_solver="hlld", time_advance="rk3", gamma=5.0 / 3.0, cfl=0.4,
+               ncells=16, nghosts=2, xblocks=nb, yblocks=nb, zblocks=nb, xmin=-0.5, xmax=0.5,
+               ymin=-0.5, ymax=0.5, zmin=-0.5, zmax=0.5, precision=precision)
+    meta = refine(p, near_sphere(p, r, 0.015), 3)
+    return p, meta

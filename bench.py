@@ -92,6 +92,53 @@ def cpu_baseline(p_full):
                                    f"{n1} steps, FP64, {secs1:.2f} s on one host thread"}}
 
 
+def c3_legs(steps: int, warmup: int):
+    """The batched/AMR configuration beside the headline: the 64-grid KH
+    batch (one block pool, per-run dt) and the 3-level 3D AMR mesh of 6224
+    leaves of 16^3, FP64 and mixed, steps replayed from captured graphs;
+    zone-updates/s over all interior cells of the batch or leaves."""
+    from amun_amd.config import c3_batch_params
+    from amun_amd import mesh
+    from amun_amd.problems import kelvin_helmholtz_2d
+    from amun_amd.solver import AmrSolver, Batch
+
+    def timed(s):
+        s.set_graph(True)
+        s.run(warmup)
+        s.sync()
+        t0 = time.perf_counter()
+        s.run(steps)
+        s.sync()
+        return time.perf_counter() - t0
+
+    out = {}
+    for prec in ("fp64", "mixed"):
+        common, runs = c3_batch_params(precision=prec)
+        cells = sum(p.ncells_total for p in runs)
+        print(f"[bench] c3 batch {prec}: {len(runs)} runs, {cells} cells", file=sys.stderr,
+              flush=True)
+        with Batch(common, runs) as b:
+            for r, p in enumerate(runs):
+                b.upload(r, kelvin_helmholtz_2d(p, seed=r))
+            wall = timed(b)
+        out[f"batch_{prec}"] = {"value": cells * steps / wall, "unit": "zone-updates/s",
+                                "ms_per_step": wall / steps * 1e3, "cells": cells,
+                                "runs": len(runs)}
+        p, meta = mesh.c3_amr_params(prec)
+        leaves = len(meta)
+        cells = leaves * p.ncells ** 3
+        print(f"[bench] c3 amr {prec}: {leaves} leaves, {cells} cells", file=sys.stderr,
+              flush=True)
+        with AmrSolver(p, meta) as s:
+            s.upload(mesh.block_state(p, meta, mesh.blast_prim(p, r0=0.2)))
+            info = s.info()
+            wall = timed(s)
+        out[f"amr_{prec}"] = {"value": cells * steps / wall, "unit": "zone-updates/s",
+                              "ms_per_step": wall / steps * 1e3, "cells": cells,
+                              "leaves": leaves, "virtual_blocks": info["virtual_blocks"]}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,6 +149,8 @@ def main():
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the mixed-precision leg reported beside the FP64 value")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-c3", action="store_true",
+                    help="skip the batched/AMR legs (64 KH grids, 6224-leaf AMR mesh)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run three 1 GiB device-to-device copies (the calibration "
                          "of the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -223,6 +272,8 @@ def main():
         out["valu"] = valu
     if mixed is not None:
         out["mixed"] = mixed
+    if not args.no_c3:
+        out["c3"] = c3_legs(args.steps, args.warmup)
     if not args.no_cpu_baseline:
         print("[bench] CPU baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(p)

@@ -3375,6 +3375,48 @@ static void new_time_step(ogrid* g, double dtnext)
   if (dtnext > 0.0) g->dt = fmin_r(g->dtn, dtnext);
 }
 
+/* The GLM source terms of update_sources (sources.F90:479-539) on the
+ * interior cells of one block, after update_increment: with db = div B and
+ * grad psi from derivative_1st's central differences (operators.F90:181-308,
+ * 0.5 (u(i+1) - u(i-1)) / dh, summed over the directions in order),
+ *   EGLM (1): du(m) -= db B,  du(E) -= B . grad psi;
+ *   HEGLM (2), in addition: du(B) -= db v,  du(E) -= db (v . B).
+ * Only for systems with a magnetic field (ibx > 0 in the reference). */
+static void glm_sources(const ocfg* o, const double* q, double* du, const double h[3])
+{
+  const int nd = o->nd, nv = o->nv;
+  const size_t st[3] = {(size_t)nv, (size_t)o->im * nv, (size_t)o->im * o->jm * nv};
+  int i, j, k, d;
+  if (o->c.glm_source_terms <= 0 || o->ibx < 0) return;
+  for (k = o->kb; k <= o->ke; k++)
+    for (j = o->jb; j <= o->je; j++)
+      for (i = o->ib; i <= o->ie; i++) {
+        const size_t c = QI(o, 0, i, j, k);
+        const double* qc = q + c;
+        double* dc = du + c;
+        double db = 0.0, gp[3] = {0.0, 0.0, 0.0};
+        for (d = 0; d < nd; d++)
+          db = db + 0.5 * (q[c + st[d] + o->ibx + d] - q[c - st[d] + o->ibx + d]) / h[d];
+        dc[o->imx] = dc[o->imx] - db * qc[o->ibx];
+        dc[o->imy] = dc[o->imy] - db * qc[o->iby];
+        dc[o->imz] = dc[o->imz] - db * qc[o->ibz];
+        if (o->ien >= 0 && o->ibp >= 0) {
+          for (d = 0; d < nd; d++)
+            gp[d] = 0.5 * (q[c + st[d] + o->ibp] - q[c - st[d] + o->ibp]) / h[d];
+          dc[o->ien] = dc[o->ien] -
+                       ((qc[o->ibx] * gp[0] + qc[o->iby] * gp[1]) + qc[o->ibz] * gp[2]);
+        }
+        if (o->c.glm_source_terms > 1) {
+          dc[o->ibx] = dc[o->ibx] - db * qc[o->ivx];
+          dc[o->iby] = dc[o->iby] - db * qc[o->ivy];
+          dc[o->ibz] = dc[o->ibz] - db * qc[o->ivz];
+          if (o->ien >= 0)
+            dc[o->ien] = dc[o->ien] - db * ((qc[o->ivx] * qc[o->ibx] + qc[o->ivy] * qc[o->iby]) +
+                                            qc[o->ivz] * qc[o->ibz]);
+        }
+      }
+}
+
 /* one stage: fluxes of u (through q), increment, combine into dst (which
  * may be a0 or a1: every element is read before it is written) */
 static void stage2(ogrid* g, int kind, double ds, double* dst, const double* a0,
@@ -3396,9 +3438,12 @@ static void stage2(ogrid* g, int kind, double ds, double* dst, const double* a0,
     if (g->amr) {
       const double* lv = g->lvd[g->lev[b]];
       update_increment_dx(o, g->fall + b * bs * o->nd, g->du, lv[3], lv[4], lv[5]);
+      glm_sources(o, g->q + b * bs, g->du, lv);
     } else {
+      const double h[3] = {o->adx, o->ady, o->adz};
       update_flux(o, g->q + b * bs, g->f, g->qs);
       update_increment(o, g->f, g->du);
+      glm_sources(o, g->q + b * bs, g->du, h);
     }
     for (n = 0; n < bs; n++) {
       double v;

@@ -351,6 +351,62 @@ def main_remesh():
         json.dump(index, fh, indent=1)
 
 
+# (name, ndims, glm source terms, reconstruction, steps): Orszag-Tang with
+# noise on the field (2D) and the fully three-dimensional noisy state (3D),
+# so that div B is far from zero
+GLM_CASES = (("eglm_otnoisy2d_32_8", 2, "eglm", "tvd", 8),
+             ("heglm_otnoisy2d_32_8", 2, "heglm", "tvd", 8),
+             ("heglm_noisy3d_16_3", 3, "heglm", "weno5z", 3))
+
+
+def glm_params(nd, glm, rec):
+    p = Params(ndims=nd, equation_system="mhd", reconstruction=rec, limiter="mc",
+               riemann_solver="hlld", time_advance="rk3", gamma=5.0 / 3.0, cfl=0.3, ncells=8,
+               nghosts=4 if rec == "weno5z" else 2, xblocks=4 if nd == 2 else 2,
+               yblocks=4 if nd == 2 else 2, zblocks=2)
+    p.glm_source_terms = glm
+    return p
+
+
+def glm_state(name, p):
+    return noisy3d(p) if "noisy3d" in name else ot_noisy(p)
+
+
+def main_glm():
+    """The EGLM / HEGLM source terms (sources.F90:479-539): short runs of
+    the reference (own generator), and one on its three-level blast mesh."""
+    from amun_amd import mesh
+    with open(os.path.join(HERE, "index.json")) as fh:
+        index = json.load(fh)
+    for name, nd, glm, rec, nsteps in GLM_CASES:
+        p = glm_params(nd, glm, rec)
+        u0 = glm_state(name, p)
+        u, tr = ol.ref_run(p, u0, nsteps)
+        st = 3 if nd == 2 else 2
+        sub = u[:, :, ::st, ::st] if nd == 2 else u[:, ::st, ::st, ::st]
+        np.savez_compressed(os.path.join(HERE, name + ".npz"),
+                            params=np.array(p.to_params_in()), ndims=nd, nsteps=nsteps,
+                            stride=st, trace=tr, sub=sub, sha_u0=sha(u0), sha_u=sha(u))
+        index[name] = {"steps": nsteps, "sha_u": sha(u), "dt_last": float(tr[-1, 0])}
+        print(name, "dt", tr[-1, 0])
+    p = amr_params(2)
+    p.glm_source_terms = "heglm"
+    meta, _, _, _, _ = ol.ref_amr(p, 1)
+    u0 = mesh.block_state(p, meta, mesh.blast_prim(p))
+    nsteps = 6
+    m0, u0r, m1, u1, tr = ol.ref_amr(p, nsteps, u_in=u0)
+    name = f"amr_heglm_mhd2d_blast_{nsteps}"
+    sums = u1.reshape(u1.shape[0], -1, u1.shape[-1]).sum(axis=1)
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), params=np.array(p.to_params_in()),
+                        ndims=2, nsteps=nsteps, meta=meta, trace=tr, block_sums=sums,
+                        sha_u0=sha(u0), sha_u=sha(u1))
+    index[name] = {"steps": nsteps, "sha_u": sha(u1), "dt_last": float(tr[-1, 0]),
+                   "blocks": int(meta.shape[0])}
+    print(name, meta.shape[0], "blocks, dt", tr[-1, 0])
+    with open(os.path.join(HERE, "index.json"), "w") as fh:
+        json.dump(index, fh, indent=1)
+
+
 def main_hllc_mhd():
     """riemann_mhd_adi_hllc through update_flux (zero limiter), random states
     with weak, zero and strong Bx, and an Orszag-Tang run (own generator)."""
@@ -828,6 +884,9 @@ def main():
     if sys.argv[1:] == ["refine"]:
         main_refine()
         sys.exit(0)
+    if sys.argv[1:] == ["glm"]:
+        main_glm()
+        return
     if sys.argv[1:] == ["remesh"]:
         main_remesh()
         return

@@ -66,13 +66,13 @@ def test_config_errors(field, value, code):
 
 
 @pytest.mark.parametrize("key,value", [
-    ("mlp_limiting", "on"), ("fix_unphysical_cells", "on"), ("glm_source_terms", "eglm"),
-    ("glm_source_terms", "heglm"), ("viscosity", "1.0e-3"), ("resistivity", "1.0e-3"),
+    ("mlp_limiting", "on"), ("fix_unphysical_cells", "on"), ("viscosity", "1.0e-3"),
+    ("resistivity", "1.0e-3"),
 ])
 def test_unsupported_terms_are_refused(key, value):
     """Parameters of terms outside the device path (MLP limiting, the
-    unphysical-cell fix, the EGLM/HEGLM and viscous/resistive sources) are
-    carried into amun_cfg and refused with 600 instead of being ignored."""
+    unphysical-cell fix, the viscous/resistive sources) are carried into
+    amun_cfg and refused with 600 instead of being ignored."""
     p = Params.from_text(f"reconstruction = tvd\nriemann_solver = hllc\n{key} = {value}\n",
                          ndims=2)
     assert key not in p.extra
@@ -81,6 +81,20 @@ def test_unsupported_terms_are_refused(key, value):
     h = ctypes.c_void_p()
     assert lib.amun_dev_init(ctypes.byref(cfg), ctypes.byref(h)) == 600
     assert key.split("_")[0].encode() in lib.amun_dev_last_error()
+
+
+@pytest.mark.parametrize("value,code", [("eglm", 1), ("heglm", 2), ("none", 0)])
+def test_glm_source_terms_carried(value, code):
+    """glm_source_terms reaches amun_cfg (eglm 1, heglm 2); values outside
+    0..2 are refused with 500."""
+    p = Params.from_text(f"equation_system = mhd\nriemann_solver = hlld\n"
+                         f"glm_source_terms = {value}\n", ndims=2)
+    cfg = p.to_cfg()
+    assert cfg.glm_source_terms == code
+    cfg.glm_source_terms = 3
+    lib = solver.load_library()
+    h = ctypes.c_void_p()
+    assert lib.amun_dev_init(ctypes.byref(cfg), ctypes.byref(h)) == 500
 
 
 def test_weno_needs_four_ghosts():

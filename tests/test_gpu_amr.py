@@ -195,17 +195,9 @@ def test_graph_replay_bitwise():
 
 # ----------------------------------------------- the batched/AMR configuration
 def c3_amr_case(precision="fp64", nb=8, r=0.3):
-    """The 3D block-AMR run of the batched/AMR configuration: three levels of
-    16^3 blocks (+2 ghosts) on a periodic nb^3 base, refined towards a sphere
-    of radius r (6224 leaves: 280 / 1272 / 4672 on levels 1 / 2 / 3 for
-    nb = 8, r = 0.3), MHD blast with noise, TVD-MC + HLLD + SSPRK(3,3),
-    CFL 0.4."""
-    p = Params(ndims=3, equation_system="mhd", reconstruction="tvd", limiter="mc",
-               riemann_solver="hlld", time_advance="rk3", gamma=5.0 / 3.0, cfl=0.4,
-               ncells=16, nghosts=2, xblocks=nb, yblocks=nb, zblocks=nb, xmin=-0.5, xmax=0.5,
-               ymin=-0.5, ymax=0.5, zmin=-0.5, zmax=0.5, precision=precision)
-    meta = mesh.refine(p, mesh.near_sphere(p, r, 0.015), 3)
-    return p, meta
+    """The 3D block-AMR run of the batched/AMR configuration (6224 leaves of
+    16^3 on three levels for nb = 8, r = 0.3): mesh.c3_amr_params."""
+    return mesh.c3_amr_params(precision, nb, r)
 
 
 def _volumes(p, meta):

@@ -348,9 +348,9 @@ def test_mixed_256_50_steps_density_l1():
 
 
 # ------------------------------------------------------------------- SRMHD
-def _srmhd_state(p, vmax=0.6):
+def _srmhd_state(p, vmax=0.6, bnoise=0.0):
     """Smooth relativistic state with |v| up to vmax, a tilted field and a
-    pressure bump; psi small random."""
+    pressure bump; psi small random (bnoise: relative noise on the field)."""
     nz, ny, nx = p.shape
     z, y, x = cell_centres(p)
     Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
@@ -365,6 +365,8 @@ def _srmhd_state(p, vmax=0.6):
     q[6] = 0.2 * np.cos(2 * np.pi * Z)
     q[7] = 0.01 * rng.standard_normal(X.shape)
     q[8] = 0.5 + 0.5 * (X * X + Y * Y < 0.05)
+    if bnoise:
+        q[4:7] *= 1.0 + bnoise * np.random.default_rng(18).uniform(-1.0, 1.0, q[4:7].shape)
     return prim_to_cons(p, q)
 
 
@@ -958,3 +960,78 @@ def test_refinement_indicator_closed_forms_device():
         cm, im = s.refinement(rc)
     np.testing.assert_array_equal(cm, co)
     np.testing.assert_allclose(im, io, rtol=1e-5, atol=1e-6)
+
+
+# ------------------------------------------------ GLM source terms
+@pytest.mark.parametrize("nd", [2, 3])
+@pytest.mark.parametrize("glm", ["eglm", "heglm"])
+@pytest.mark.parametrize("eq", ["mhd", "srmhd"])
+def test_glm_source_terms_one_step(nd, glm, eq):
+    """EGLM / HEGLM source terms (sources.F90:479-539) in one SSPRK(3,3)
+    step against the oracle, on a state with div B far from zero; the terms
+    change the step by far more than the tolerance."""
+    if eq == "mhd":
+        p = Params(ndims=nd, equation_system="mhd", reconstruction="tvd", limiter="mc",
+                   riemann_solver="hlld", time_advance="rk3", ncells=8, nghosts=2, xblocks=2,
+                   yblocks=3, zblocks=2, cfl=0.3, xmin=-0.5, xmax=0.5, ymin=-0.75, ymax=0.75,
+                   zmin=0.0, zmax=1.0)
+        u0 = _mhd_state(p, noise=0.05)
+    else:
+        p = Params(ndims=nd, equation_system="srmhd", riemann_solver="hll",
+                   reconstruction="tvd", limiter="mc", time_advance="rk3", ncells=8, nghosts=2,
+                   xblocks=2, yblocks=3, zblocks=2, cfl=0.25, gamma=5.0 / 3.0,
+                   tolerance=1e-12, nr_maxit=50, xmin=-0.5, xmax=0.5, ymin=-0.75, ymax=0.75,
+                   zmin=0.0, zmax=1.0)
+        u0 = _srmhd_state(p, bnoise=0.05)
+    p.glm_source_terms = glm
+    dt, cmax, ug, tr, uo = one_step(p, u0)
+    assert abs(dt - tr[0]) <= 1e-13 * tr[0]
+    e = relerr(ug, uo)
+    p.glm_source_terms = "none"
+    o = ol.OracleRun(p)
+    o.set_state(u0)
+    o.run(1)
+    d = relerr(o.get_state(), uo)
+    print(f"  vs oracle max {e.max():.2e}; terms change the step by {d.max():.2e}")
+    assert e.max() <= 1e-12 and d.max() >= 1e-6
+
+
+@pytest.mark.parametrize("name", ["eglm_otnoisy2d_32_8", "heglm_otnoisy2d_32_8",
+                                  "heglm_noisy3d_16_3"])
+def test_glm_source_terms_golden_runs(name):
+    """The reference's own EGLM / HEGLM runs (dt trace and a subsample of the
+    final state), FP64; the mixed mode within 1e-5 of them."""
+    import os
+    import sys
+    gold = os.path.join(os.path.dirname(__file__), "golden")
+    sys.path.insert(0, gold)
+    from make_golden import glm_state
+    g = np.load(os.path.join(gold, name + ".npz"))
+    p = Params.from_text(str(g["params"]), ndims=int(g["ndims"]))
+    u0 = glm_state(name, p)
+    st = int(g["stride"])
+    for prec, bound in (("fp64", 1e-10), ("mixed", 1e-5)):
+        p.precision = prec
+        print(f"[case] {name} {prec}", flush=True)
+        with Solver(p) as s:
+            s.upload(u0)
+            dts = s.run(int(g["nsteps"]))
+            u = s.download()
+        sub = u[:, ::st, ::st, ::st] if p.ndims == 3 else u[:, :, ::st, ::st]
+        e = relerr(sub, g["sub"]).max()
+        print(f"  dt rel {np.abs(dts / g['trace'][:, 0] - 1).max():.2e}, state max rel {e:.2e}")
+        assert e <= bound
+        if prec == "fp64":
+            assert np.abs(dts - g["trace"][:, 0]).max() <= 1e-12 * g["trace"][:, 0].max()
+
+
+def test_glm_source_terms_refused_for_gradients():
+    """The adjoint does not carry the source terms: a tape is refused."""
+    from amun_amd.solver import AmunError
+    p, u0 = Params(ndims=2, equation_system="mhd", reconstruction="tvd", limiter="mc",
+                   riemann_solver="hlld", time_advance="rk3", ncells=8, nghosts=2,
+                   xblocks=2, yblocks=2, glm_source_terms="eglm"), None
+    with Solver(p) as s:
+        s.upload(_mhd_state(p))
+        with pytest.raises(AmunError):
+            s.tape_forward(2, 1)
