@@ -18,7 +18,7 @@ from dataclasses import dataclass, field, fields
 # constants mirrored from include/amun_dev.h
 EQSYS = {"hd": 1, "hydrodynamic": 1, "mhd": 2, "magnetohydrodynamic": 2,
          "srmhd": 4}
-EOS = {"adi": 1, "adiabatic": 1}
+EOS = {"adi": 1, "adiabatic": 1, "iso": 2, "isothermal": 2}
 RECON = {"tvd": 1, "weno3": 2, "limo3": 3, "ppm": 4, "crweno5z": 8, "crweno5-z": 8,
          "crweno5yc": 9, "crweno5-yc": 9, "crweno5ns": 10, "crweno5-ns": 10, "crmp5": 13, "mgp": 21,
          "crmp5l": 14, "crmp5ld": 14, "crmp7": 15, "weno5z": 5, "weno5-z": 5, "weno5yc": 6, "weno5-yc": 6, "weno5ns": 7,
@@ -64,6 +64,7 @@ class AmunCfg(ctypes.Structure):
         ("mlp_limiting", ctypes.c_int32), ("fix_unphysical_cells", ctypes.c_int32),
         ("glm_source_terms", ctypes.c_int32), ("reserved_cfg", ctypes.c_int32),
         ("viscosity", ctypes.c_double), ("resistivity", ctypes.c_double),
+        ("csnd", ctypes.c_double),
     ]
 
 
@@ -136,6 +137,7 @@ class Params:
     ndims: int = 3
     equation_system: str = "hd"
     equation_of_state: str = "adi"
+    csnd: float = 1.0
     reconstruction: str = "tvd"
     limiter: str = "mm"
     riemann_solver: str = "hlld"
@@ -187,7 +189,8 @@ class Params:
     # ---------------------------------------------------------------- derived
     @property
     def nv(self) -> int:
-        return 5 if EQSYS[self.equation_system.lower()] == 1 else 9
+        iso = EOS.get(self.equation_of_state.lower(), 1) == 2
+        return (5 if EQSYS[self.equation_system.lower()] == 1 else 9) - (1 if iso else 0)
 
     @property
     def shape(self):
@@ -248,6 +251,7 @@ class Params:
         c.glm_source_terms = {"eglm": 1, "heglm": 2}.get(
             str(self.glm_source_terms).strip().lower(), 0)
         c.viscosity, c.resistivity = float(self.viscosity), float(self.resistivity)
+        c.csnd = float(self.csnd)
         c.prol_limiter = {"mm": 1, "minmod": 1, "mc": 2, "monotonized_central": 2,
                           "sb": 3, "superbee": 3, "vl": 4, "vanleer": 4}.get(
                               self.prolongation_limiter.lower(), 0)
@@ -264,7 +268,8 @@ class Params:
         """Text of an AMUN params.in with these settings (periodic, one level)."""
         eq = {1: "hd", 2: "mhd", 4: "srmhd"}[EQSYS[self.equation_system.lower()]]
         lines = [
-            f"equation_system = {eq}", "equation_of_state = adi",
+            f"equation_system = {eq}", f"equation_of_state = {self.equation_of_state}",
+            f"csnd = {self.csnd!r}",
             f"reconstruction = {self.reconstruction}", f"limiter = {self.limiter}",
             f"riemann_solver = {self.riemann_solver}",
             f"time_advance = {self.time_advance}",

@@ -200,7 +200,7 @@ template <class P, int REC, int RS, int ND, int DIR, typename T>
 __device__ __forceinline__ void face_flux(const RunConst& rc, const T* __restrict__ Qp,
                                           const int* __restrict__ nbr, int b, int c, int c1,
                                           int c2, T cmax, T cmax2, DevState* st, double h,
-                                          T* f) {
+                                          const T* mlps, const T* mlpg, T* f) {
   constexpr int NV = P::nv;
   using RC = Recon<P, REC, ND, DIR, T>;
   const EosConst<T> e{T(rc.gamma), T(rc.gammam1), T(rc.gammam1i)};
@@ -224,13 +224,19 @@ __device__ __forceinline__ void face_flux(const RunConst& rc, const T* __restric
     rec.load(m, qtmp);
   }
   int i, j, k;
+  const int pen = c1 + rc.nc * c2;
   coords(c - 1, i, j, k);
   rec.cell(rc, flm, frm, Qp, nbr, b, i, j, k, st, h);
+  if (mlps)
+    mlp_apply<P, ND, DIR, T>(rc, Qp, mlps, mlpg, nbr, b, c - 1, pen, i, j, k,
+                             c - 1 == -1 && rc.ng == 2, flm, frm);
   rec.shift();
   loadq(c + RC::R, qtmp);
   rec.load(2 * RC::R, qtmp);
   coords(c, i, j, k);
   rec.cell(rc, flc, frc, Qp, nbr, b, i, j, k, st, h);
+  if (mlps)
+    mlp_apply<P, ND, DIR, T>(rc, Qp, mlps, mlpg, nbr, b, c, pen, i, j, k, false, flc, frc);
   T ql[NV], qr[NV], fr[NV];
 #pragma unroll
   for (int p = 0; p < NV; p++) {
@@ -251,7 +257,8 @@ static __global__ void k_amr_fineflux(const int4* __restrict__ task, int ntask, 
                                       const int* __restrict__ nbr, RunConst rc,
                                       const RunState* __restrict__ rs, int slot,
                                       double* __restrict__ FB, DevState* st,
-                                      const int* __restrict__ blev) {
+                                      const int* __restrict__ blev, const void* mlps_,
+                                      const void* mlpg_) {
   constexpr int NV = P::nv;
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= ntask) return;
@@ -260,19 +267,21 @@ static __global__ void k_amr_fineflux(const int4* __restrict__ task, int ntask, 
   const double cmaxd = cmax_of(rs, slot);
   const T cmax = T(cmaxd), cmax2 = T(cmaxd * cmaxd);
   const T* Qp = static_cast<const T*>(Q);
+  const T* mlps = static_cast<const T*>(mlps_);
+  const T* mlpg = static_cast<const T*>(mlpg_);
   T fa[NV], fb[NV];
   const int nf = (ND == 3) ? rc.nc * rc.nc : rc.nc;
   double* out = FB + (size_t)t.x * NV * nf + t.y;
   const double h = rs->h[DIR] / double(1 << blev[t.z]);   // the fine block's cell size
-  face_flux<P, REC, RS, ND, DIR, T>(rc, Qp, nbr, t.z, c, f1, f2, cmax, cmax2, st, h, fa);
-  face_flux<P, REC, RS, ND, DIR, T>(rc, Qp, nbr, t.z, c, f1 + 1, f2, cmax, cmax2, st, h, fb);
+  face_flux<P, REC, RS, ND, DIR, T>(rc, Qp, nbr, t.z, c, f1, f2, cmax, cmax2, st, h, mlps, mlpg, fa);
+  face_flux<P, REC, RS, ND, DIR, T>(rc, Qp, nbr, t.z, c, f1 + 1, f2, cmax, cmax2, st, h, mlps, mlpg, fb);
   if constexpr (ND == 2) {
 #pragma unroll
     for (int v = 0; v < NV; v++) out[(size_t)v * nf] = (double(fa[v]) + double(fb[v])) / 2.0;
   } else {
     T fc[NV], fd[NV];
-    face_flux<P, REC, RS, ND, DIR, T>(rc, Qp, nbr, t.z, c, f1, f2 + 1, cmax, cmax2, st, h, fc);
-    face_flux<P, REC, RS, ND, DIR, T>(rc, Qp, nbr, t.z, c, f1 + 1, f2 + 1, cmax, cmax2, st, h, fd);
+    face_flux<P, REC, RS, ND, DIR, T>(rc, Qp, nbr, t.z, c, f1, f2 + 1, cmax, cmax2, st, h, mlps, mlpg, fc);
+    face_flux<P, REC, RS, ND, DIR, T>(rc, Qp, nbr, t.z, c, f1 + 1, f2 + 1, cmax, cmax2, st, h, mlps, mlpg, fd);
     // ((l,l) + (p,p)) + ((l,p) + (p,l)) over the two transverse axes
 #pragma unroll
     for (int v = 0; v < NV; v++)

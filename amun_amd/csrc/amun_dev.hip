@@ -33,6 +33,8 @@ struct amun_ctx {
   int nspart = 0;              // entries of spart (workgroups of >= 64 lanes)
   double* ugp = nullptr;       // MGP weights (RunConst::ugp points here)
   void* src = nullptr;         // GLM source products per stage (element size qsize)
+  void* mlps = nullptr;        // multi-dimensional limiting: slopes of the interior cells
+  void* mlpg = nullptr;        // and of the cells beyond the pencil ends (k_mlp)
   int qcur = 0;                // Q[qcur] holds the primitive state of U0
   long long nsteps = 0;        // steps launched (slot = nsteps & 1)
   hipStream_t stream = nullptr;
@@ -159,7 +161,6 @@ int validate(const amun_cfg& c, int* nv) {
     return fail(500, "unsupported precision");
   if (c.fix_positivity || c.clip_extrema)
     return fail(600, "fix_positivity / clip_extrema are not supported on the device path");
-  if (c.mlp_limiting) return fail(600, "mlp_limiting is not supported on the device path");
   if (c.fix_unphysical_cells)
     return fail(600, "fix_unphysical_cells is not supported on the device path");
   if (c.glm_source_terms < 0 || c.glm_source_terms > 2)
@@ -179,6 +180,13 @@ int validate(const amun_cfg& c, int* nv) {
   }
   if (c.ncells + 2 * c.nghosts > 96)
     return fail(600, "ncells + 2 nghosts above 96");
+  // with two ghost layers the reference's multi-dimensional limiting reads
+  // the face state of each block's outermost cell, which is q itself only
+  // after interfaces_tvd and interfaces_mgp
+  if (c.mlp_limiting && c.nghosts == 2 && c.reconstruction != AMUN_REC_TVD &&
+      c.reconstruction != AMUN_REC_MGP)
+    return fail(600, "mlp_limiting with nghosts = 2 is built for the TVD and MGP "
+                     "reconstructions (use nghosts >= 3)");
   if (c.nghosts > c.ncells)
     return fail(600, "nghosts above ncells (ghost reads reach one neighbour block only)");
   if (c.nghosts < need)
@@ -326,6 +334,40 @@ size_t sweep_lds(const amun_ctx* c, int dir, int /*wg*/) {
   const bool din = chunk && dir > 0 && chunk_d, dout = chunk && !last && chunk_d;
   const int nbuf = (chunk ? 1 : 0) + ((din || dout) ? 1 : 0);
   return (size_t)nbuf * 4 * c->nv * kLdsLanes * c->qsize;
+}
+
+// the slopes of the multi-dimensional limiting of the stage input Q[qcur]
+// (k_mlp)
+template <class P, typename T>
+static void mlp_launch_t(amun_ctx* c, int qcur) {
+  const RunConst& rc = c->rc;
+  const int nd = rc.nd, npen = nd == 3 ? rc.nc * rc.nc : rc.nc;
+  const long long n = (long long)rc.nblk * (rc.ncell + nd * 3 * npen);
+  const dim3 grid((unsigned)((n + 255) / 256)), bl(256);
+  const T* Q = static_cast<const T*>(c->Q[qcur]);
+  T* S = static_cast<T*>(c->mlps);
+  T* G = static_cast<T*>(c->mlpg);
+  if (nd == 3)
+    hipLaunchKernelGGL((k_mlp<P, 3, T>), grid, bl, 0, c->stream, Q, c->nbr, rc, S, G);
+  else
+    hipLaunchKernelGGL((k_mlp<P, 2, T>), grid, bl, 0, c->stream, Q, c->nbr, rc, S, G);
+}
+
+static int mlp_launch(amun_ctx* c, int qcur) {
+  const bool f64 = c->qsize == sizeof(double);
+  switch (c->cfg.eqsys) {
+  case AMUN_EQSYS_HD:
+    f64 ? mlp_launch_t<amun::HD, double>(c, qcur) : mlp_launch_t<amun::HD, float>(c, qcur);
+    break;
+  case AMUN_EQSYS_MHD:
+    f64 ? mlp_launch_t<amun::MHD, double>(c, qcur) : mlp_launch_t<amun::MHD, float>(c, qcur);
+    break;
+  default:
+    f64 ? mlp_launch_t<amun::SRMHD, double>(c, qcur) : mlp_launch_t<amun::SRMHD, float>(c, qcur);
+    break;
+  }
+  HIP_TRY(hipGetLastError());
+  return 0;
 }
 
 }  // namespace
@@ -527,6 +569,11 @@ static int init_device(amun_ctx* c, const std::vector<int32_t>& nbr,
     const long long npen = nd == 3 ? (long long)rc.nc * rc.nc : rc.nc;
     c->nspart = (int)((rc.nblk * npen + 63) / 64);
     ALLOC(c->spart, (size_t)c->nspart * sizeof(SpeedPart));
+  }
+  if (cfg->mlp_limiting) {
+    const int gs = 3 * (nd == 3 ? rc.nc * rc.nc : rc.nc);
+    ALLOC(c->mlps, (size_t)rc.nblk * nv * nd * rc.ncell * c->qsize);
+    ALLOC(c->mlpg, (size_t)rc.nblk * nv * nd * gs * c->qsize);
   }
   if (cfg->glm_source_terms > 0 && nv == 9) {
     // the eight GLM source products of every cell (k_glm_src)
@@ -913,6 +960,8 @@ void amun_dev_finalize(amun_ctx* c) {
   hipFree(c->rc.fsl);
   hipFree(c->ugp);
   hipFree(c->src);
+  hipFree(c->mlps);
+  hipFree(c->mlpg);
   hipFree(c->rc.fsr);
   hipFree(c->amr.blev);
   hipFree(c->bnd.task);
@@ -1234,11 +1283,18 @@ static int amr_fill(amun_ctx* c, int qcur, int slot) {
       hipLaunchKernelGGL(k_amr_prolong<float>, dim3((A.np + 255) / 256), bl, 0, c->stream,
                          A.ptask, A.np, Q, c->nbr, c->rc, c->nv, c->nv - 1, c->st);
   }
+  // the slopes of the multi-dimensional limiting read the filled virtual
+  // blocks and feed the fine-face fluxes
+  if (c->mlps) {
+    int r = mlp_launch(c, qcur);
+    if (r) return r;
+  }
   for (int d = 0; d < c->rc.nd; d++)
     if (A.nf[d])
       hipLaunchKernelGGL(c->disp.fineflux[d], dim3((A.nf[d] + 255) / 256), bl, 0, c->stream,
                          A.ftask[d], A.nf[d], (const void*)c->Q[qcur], c->nbr, c->rc, c->rs,
-                         slot, A.FB, A.dummy, (const int*)A.blev);
+                         slot, A.FB, A.dummy, (const int*)A.blev, (const void*)c->mlps,
+                         (const void*)c->mlpg);
   HIP_TRY(hipGetLastError());
   return 0;
 }
@@ -1288,15 +1344,23 @@ static int enqueue_step(amun_ctx* c, double dtnext, int slot, int qcur, hipEvent
     if (c->amr.on) {
       int r = amr_fill(c, qcur, slot);
       if (r) return r;
-    } else if (c->bnd.on) {
-      int r = bnd_fill(c, qcur);
-      if (r) return r;
+    } else {
+      if (c->bnd.on) {
+        int r = bnd_fill(c, qcur);
+        if (r) return r;
+      }
+      if (c->mlps) {
+        int r = mlp_launch(c, qcur);
+        if (r) return r;
+      }
     }
     if (c->src) {
       int r = glm_sources(c, qcur);
       if (r) return r;
     }
     SweepArgs a;
+    a.mlps = c->mlps;
+    a.mlpg = c->mlpg;
     a.src = c->src;
     a.glm = c->cfg.glm_source_terms;
     a.blev = c->amr.on ? c->amr.blev : nullptr;
@@ -1992,6 +2056,8 @@ static int tape_check(amun_ctx* c) {
   if (!c->disp.has_adj)
     return fail(600, "gradients need TVD reconstruction and the HD or MHD system");
   if (c->src) return fail(600, "gradients do not include the GLM source terms");
+  if (c->cfg.mlp_limiting)
+    return fail(600, "gradients do not include the multi-dimensional limiting");
   return 0;
 }
 

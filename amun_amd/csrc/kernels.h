@@ -257,6 +257,10 @@ struct SweepArgs {
   // k_glm_src, and the kind (1 EGLM, 2 HEGLM)
   const void* src;
   int glm;
+  // multi-dimensional limiting (null when off): the slopes of k_mlp for the
+  // interior cells (S) and the cells beyond the pencil ends (G)
+  const void* mlps;
+  const void* mlpg;
 };
 
 // stage combines of evolve_euler/rk2/rk3 (evolution.F90:570, 700, 740, 1080,
@@ -1002,6 +1006,8 @@ constexpr int sweep_min_waves() { return sizeof(T) == 4 ? 2 : 1; }
 // The FP64 x and y sweeps also move the increment through LDS in chunks; the
 // FP32 ones read and write it per cell, which leaves LDS for two workgroups
 // per CU.
+#include "mlp.h"
+
 template <typename T>
 constexpr bool sweep_chunk_d() { return sizeof(T) == 8; }
 
@@ -1163,6 +1169,10 @@ __global__ __launch_bounds__(256, sweep_min_waves<T>()) void k_sweep(SweepArgs a
       int ci, cj, ck;
       coords(c, ci, cj, ck);
       rec.cell(rc, flc, frc, Qp, a.nbr, b, ci, cj, ck, a.st, hdir);
+      if (a.mlps)
+        mlp_apply<P, ND, DIR, T>(rc, Qp, static_cast<const T*>(a.mlps),
+                                 static_cast<const T*>(a.mlpg), a.nbr, b, c, t, ci, cj, ck,
+                                 c == -1 && rc.ng == 2, flc, frc);
       if (c >= 0) {
         // face c-1/2: left state from cell c-1, right state from cell c
         T ql[NV], qr[NV], fr[NV], f[NV];
@@ -1793,7 +1803,8 @@ struct Dispatch {
                    double, double, double) = nullptr;
   // multi-level meshes: averaged fine fluxes of coarse faces, per direction
   void (*fineflux[3])(const int4*, int, const void*, const int*, RunConst, const RunState*, int,
-                      double*, DevState*, const int*) = {nullptr, nullptr, nullptr};
+                      double*, DevState*, const int*, const void*,
+                      const void*) = {nullptr, nullptr, nullptr};
 };
 
 

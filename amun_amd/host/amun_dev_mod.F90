@@ -18,6 +18,7 @@ module amun_dev_mod
   integer(c_int32_t), parameter :: AMUN_EQSYS_HD = 1, AMUN_EQSYS_MHD = 2
   integer(c_int32_t), parameter :: AMUN_EQSYS_SRMHD = 4
   integer(c_int32_t), parameter :: AMUN_EOS_ADI = 1
+  integer(c_int32_t), parameter :: AMUN_EOS_ISO = 2
   integer(c_int32_t), parameter :: AMUN_REC_TVD = 1, AMUN_REC_WENO5Z = 5
   integer(c_int32_t), parameter :: AMUN_REC_WENO3 = 2, AMUN_REC_LIMO3 = 3
   integer(c_int32_t), parameter :: AMUN_REC_PPM = 4
@@ -62,6 +63,7 @@ module amun_dev_mod
     integer(c_int32_t) :: mlp_limiting, fix_unphysical_cells
     integer(c_int32_t) :: glm_source_terms, reserved_cfg
     real(c_double)     :: viscosity, resistivity
+    real(c_double)     :: csnd
   end type amun_cfg
 
   interface

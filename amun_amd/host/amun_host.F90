@@ -64,8 +64,15 @@ program amun_host
     cfg%eqsys = -1
   end select
   str = get_str("equation_of_state", "adi")
-  cfg%eos = merge(AMUN_EOS_ADI, -1_c_int32_t, any(trim(str) == &
-                  [character(len=10) :: "adi", "ADI", "adiabatic", "ADIABATIC"]))
+  select case(trim(str))
+  case("adi", "ADI", "adiabatic", "ADIABATIC")
+    cfg%eos = AMUN_EOS_ADI
+  case("iso", "ISO", "isothermal", "ISOTHERMAL")
+    cfg%eos = AMUN_EOS_ISO
+  case default
+    cfg%eos = -1
+  end select
+  cfg%csnd = get_real("csnd", 1.0d0)
   str = get_str("reconstruction", "tvd")
   select case(trim(str))
   case("tvd", "TVD")
@@ -231,6 +238,7 @@ program amun_host
   fout    = get_str("output", "")
 
   nv   = merge(5, 9, cfg%eqsys == AMUN_EQSYS_HD)
+  if (cfg%eos == AMUN_EOS_ISO) nv = nv - 1
   nc   = cfg%ncells
   ng   = cfg%nghosts
   im   = nc + 2 * ng
@@ -464,6 +472,11 @@ contains
     do p = 2, 4
       ug(:,:,:,p) = q(:,:,:,1) * q(:,:,:,p)
     end do
+    if (cfg%eos == AMUN_EOS_ISO) then
+      ! prim2cons_hd_iso / prim2cons_mhd_iso (equations.F90:1360, 2220)
+      if (nv == 8) ug(:,:,:,5:8) = q(:,:,:,5:8)
+      return
+    end if
     ek = 0.5d0 * (ug(:,:,:,2) * q(:,:,:,2) + ug(:,:,:,3) * q(:,:,:,3) &
                 + ug(:,:,:,4) * q(:,:,:,4))
     if (nv == 5) then

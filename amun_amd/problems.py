@@ -48,8 +48,8 @@ def cell_centres(p: Params):
 
 
 def prim_to_cons(p: Params, q: np.ndarray) -> np.ndarray:
-    """prim2cons of equations.F90:1762 (HD) / 2815 (MHD) / 4590 (SRMHD),
-    array form."""
+    """prim2cons of equations.F90:1762 (HD) / 2815 (MHD) / 4590 (SRMHD) /
+    1360, 2220 (isothermal HD / MHD, 4 or 8 variables), array form."""
     g = p.gamma
     u = np.empty_like(q)
     if EQSYS[p.equation_system.lower()] == 4:
@@ -67,6 +67,11 @@ def prim_to_cons(p: Params, q: np.ndarray) -> np.ndarray:
         return u
     u[0] = q[0]
     u[1:4] = q[0] * q[1:4]
+    if q.shape[0] in (4, 8):
+        # the isothermal systems (prim2cons_hd_iso / _mhd_iso, equations.F90:
+        # 1360, 2220): no energy
+        u[4:] = q[4:]
+        return u
     ek = 0.5 * (u[1] * q[1] + u[2] * q[2] + u[3] * q[3])
     if EQSYS[p.equation_system.lower()] == 1:
         u[4] = q[4] / (g - 1.0) + ek

@@ -38,6 +38,7 @@ extern "C" {
 #define AMUN_EQSYS_SRMHD   4   /* special-relativistic MHD, equations.F90:673-760 */
 /* equations of state: equations.F90:345-395 (parameter "equation_of_state") */
 #define AMUN_EOS_ADI       1
+#define AMUN_EOS_ISO       2   /* isothermal (HD and MHD), equations.F90:399-830 */
 
 /* reconstructions: interpolations.F90:237-376 (parameter "reconstruction") */
 #define AMUN_BND_PERIODIC  0
@@ -80,6 +81,8 @@ extern "C" {
 #define AMUN_RS_HLLC       2
 #define AMUN_RS_HLLD       3
 #define AMUN_RS_ROE        4   /* Roe, riemann_hd_adi_roe schemes.F90:1541 */
+#define AMUN_RS_HLLDM      5   /* HLLD-M of the isothermal MHD system,
+                                  riemann_mhd_iso_hlldm schemes.F90:2388 */
 
 /* integrators: evolution.F90:156-200 (parameter "time_advance") */
 #define AMUN_TA_EULER      0
@@ -144,19 +147,24 @@ typedef struct amun_cfg {
    * (coordinates.F90, boundaries.F90:1089-1251, 5024-5637): AMUN_BND_*;
    * an axis is periodic on both sides or on neither */
   int32_t bnd_type[6];
-  /* terms outside the device path; amun_dev_init refuses any value other
-   * than the default (off / none / 0) with code 600:
-   * "mlp_limiting" of TVD and MGP (interpolations.F90:204, 844, 1140),
+  /* "mlp_limiting" (1 = on): the multi-dimensional limiting process after
+   * every reconstruction (interpolations.F90:204, 844, 955, 1140,
+   * 1171-1321); "glm_source_terms" eglm/heglm = 1/2 (sources.F90:128-141,
+   * 479-539) for the magnetic systems.  Terms outside the device path, which
+   * amun_dev_init refuses with 600 when not at their default (off / 0):
    * "fix_unphysical_cells" of update_primitive_variables (equations.F90:905,
-   * 1168), "glm_source_terms" eglm/heglm = 1/2 (sources.F90:128-141), and the
-   * "viscosity" and "resistivity" sources with their dt term
-   * (sources.F90:145-149, evolution.F90:478-483) */
+   * 1168), and the "viscosity" and "resistivity" sources with their dt term
+   * (sources.F90:145-149, evolution.F90:478-483).  Gradients
+   * (amun_dev_tape_*) refuse mlp_limiting and the GLM source terms. */
   int32_t mlp_limiting;
   int32_t fix_unphysical_cells;
   int32_t glm_source_terms;
   int32_t reserved_cfg;
   double  viscosity;
   double  resistivity;
+  /* "csnd": the sound speed of the isothermal systems (equations.F90:176,
+   * 847-851; 1 by default) */
+  double  csnd;
 } amun_cfg;
 
 /* Refinement criterion of the leaf blocks (initialize_refinement,

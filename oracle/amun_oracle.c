@@ -63,6 +63,8 @@ typedef struct {
   int ib, ie, jb, je, kb, ke, ibl, ieu, jbl, jeu, kbl, keu;  /* 0-based      */
   int idn, ivx, ivy, ivz, imx, imy, imz, ibx, iby, ibz, ibp, ipr, ien;
   int    mhd, sr;          /* magnetic field present / relativistic system   */
+  int    iso;              /* isothermal equation of state (no energy)       */
+  double csnd, csnd2;      /* isothermal sound speed and its square          */
   double gammam1, gammam1i, gammaxi;
   double cflx;             /* cfl as adjusted by initialize_evolution        */
   int    stages;           /* m of SSPRK(m,2)                                */
@@ -102,8 +104,22 @@ static int oc_setup(ocfg* o, const amun_cfg* c)
   o->nc = c->ncells;
   o->ng = c->nghosts;
   if (o->nd != 2 && o->nd != 3) return 500;
-  if (c->eos != AMUN_EOS_ADI) return 110;
-  if (c->eqsys == AMUN_EQSYS_HD) {
+  if (c->eos != AMUN_EOS_ADI && c->eos != AMUN_EOS_ISO) return 110;
+  if (c->eos == AMUN_EOS_ISO) {
+    /* equations.F90:399-830: the isothermal systems carry no energy; the
+     * sound speed csnd is a parameter (equations.F90:176, 847-851) */
+    if (c->eqsys != AMUN_EQSYS_HD && c->eqsys != AMUN_EQSYS_MHD) return 110;
+    o->iso = 1;
+    o->csnd = c->csnd;
+    o->csnd2 = c->csnd * c->csnd;
+    o->mhd = (c->eqsys == AMUN_EQSYS_MHD);
+    o->nv = o->mhd ? 8 : 4;
+    o->idn = 0; o->ivx = 1; o->ivy = 2; o->ivz = 3;
+    o->imx = 1; o->imy = 2; o->imz = 3;
+    if (o->mhd) { o->ibx = 4; o->iby = 5; o->ibz = 6; o->ibp = 7; }
+    else o->ibx = o->iby = o->ibz = o->ibp = -1;
+    o->ipr = o->ien = -1;
+  } else if (c->eqsys == AMUN_EQSYS_HD) {
     o->nv = 5;
     o->idn = 0; o->ivx = 1; o->ivy = 2; o->ivz = 3;
     o->imx = 1; o->imy = 2; o->imz = 3;
@@ -1096,6 +1112,103 @@ static void recon_dir(const ocfg* o, int n, double h, const double* f, double* f
   }
 }
 
+/* mlp_limiting, interpolations.F90:1171-1321: the multi-dimensional
+ * limiting process over the face states of variable p of one block, every
+ * cell (neighbour indices clamped to the block array): minmod slopes in
+ * every direction; where their absolute sum exceeds the distance dqc of the
+ * cell value to the extremes of its 3^d box, slopes from the central
+ * differences scaled to dqc and redistributed between the directions (the
+ * 2D and the cyclic 3D formulas) with the minmod slopes' signs; then the
+ * cell's two face states of each direction are reset to q +- dq when
+ * either lies further from q than |dq|. */
+static void mlp_limiting(const ocfg* o, const double* q, int p, double* qs)
+{
+  const int im = o->im, jm = o->jm, km = o->km, nv = o->nv, nd = o->nd;
+  const size_t ncell = (size_t)im * jm * km;
+  int i, j, k, m, n, a, b, c;
+#define QS(d, s, i, j, k) qs[((size_t)((d) * 2 + (s)) * ncell + ((size_t)(k) * jm + (j)) * im + (i)) * nv + p]
+#define Q(i, j, k) q[QI(o, p, i, j, k)]
+  for (k = 0; k < km; k++) {
+    const int km1 = nd == 3 ? (k - 1 > 0 ? k - 1 : 0) : k;
+    const int kp1 = nd == 3 ? (k + 1 < km - 1 ? k + 1 : km - 1) : k;
+    for (j = 0; j < jm; j++) {
+      const int jm1 = j - 1 > 0 ? j - 1 : 0, jp1 = j + 1 < jm - 1 ? j + 1 : jm - 1;
+      for (i = 0; i < im; i++) {
+        const int im1 = i - 1 > 0 ? i - 1 : 0, ip1 = i + 1 < im - 1 ? i + 1 : im - 1;
+        const double qc = Q(i, j, k);
+        double dq[3] = {0.0, 0.0, 0.0}, dqm[3] = {0.0, 0.0, 0.0}, ap[3] = {0.0, 0.0, 0.0};
+        double qmn, qmx, dqc, sa, fc, fl, dql, dqr;
+        dq[0] = lim(AMUN_LIM_MINMOD, 0.5, qc - Q(im1, j, k), Q(ip1, j, k) - qc);
+        dq[1] = lim(AMUN_LIM_MINMOD, 0.5, qc - Q(i, jm1, k), Q(i, jp1, k) - qc);
+        if (nd == 3) dq[2] = lim(AMUN_LIM_MINMOD, 0.5, qc - Q(i, j, km1), Q(i, j, kp1) - qc);
+        qmn = qmx = qc;
+        for (c = km1; c <= kp1; c++)
+          for (b = jm1; b <= jp1; b++)
+            for (a = im1; a <= ip1; a++) {
+              const double v = Q(a, b, c);
+              qmn = v < qmn ? v : qmn;
+              qmx = v > qmx ? v : qmx;
+            }
+        dqc = fmin_r(qmx - qc, qc - qmn);
+        sa = fabs(dq[0]) + fabs(dq[1]);
+        if (nd == 3) sa = sa + fabs(dq[2]);
+        if (sa > dqc) {
+          double sm;
+          dqm[0] = fabs(Q(ip1, j, k) - Q(im1, j, k));
+          dqm[1] = fabs(Q(i, jp1, k) - Q(i, jm1, k));
+          if (nd == 3) dqm[2] = fabs(Q(i, j, kp1) - Q(i, j, km1));
+          sm = fabs(dqm[0]) + fabs(dqm[1]);
+          if (nd == 3) sm = sm + fabs(dqm[2]);
+          fc = dqc / fmax_r(1.0e-16, sm);
+          for (m = 0; m < nd; m++) ap[m] = fc * fabs(dqm[m]);
+          if (nd == 3) {
+            double hh[3], uu[3];
+            for (n = 0; n < 3; n++) {
+              const int np1 = (n + 1) % 3, np2 = (n + 2) % 3;
+              hh[n] = fmax_r(ap[n] - fabs(dq[n]), 0.0);
+              uu[n] = ap[n] - hh[n];
+              uu[np1] = ap[np1] + 0.5 * hh[n];
+              uu[np2] = ap[np2] + 0.5 * hh[n];
+              fc = hh[n] / (hh[n] + 1.0e-16);
+              fl = fc * (fmax_r(ap[np1] - fabs(dq[np1]), 0.0) - fmax_r(ap[np2] - fabs(dq[np2]), 0.0));
+              ap[n] = uu[n];
+              ap[np1] = uu[np1] - fl;
+              ap[np2] = uu[np2] + fl;
+            }
+          } else {
+            fl = fmax_r(ap[0] - fabs(dq[0]), 0.0) - fmax_r(ap[1] - fabs(dq[1]), 0.0);
+            ap[0] = ap[0] - fl;
+            ap[1] = ap[1] + fl;
+          }
+          for (m = 0; m < nd; m++) dq[m] = copysign(fabs(ap[m]), dq[m]);
+        }
+        dql = QS(0, 0, i, j, k) - qc;
+        dqr = QS(0, 1, im1, j, k) - qc;
+        if (fmax_r(fabs(dql), fabs(dqr)) > fabs(dq[0])) {
+          QS(0, 0, i, j, k) = qc + dq[0];
+          QS(0, 1, im1, j, k) = qc - dq[0];
+        }
+        dql = QS(1, 0, i, j, k) - qc;
+        dqr = QS(1, 1, i, jm1, k) - qc;
+        if (fmax_r(fabs(dql), fabs(dqr)) > fabs(dq[1])) {
+          QS(1, 0, i, j, k) = qc + dq[1];
+          QS(1, 1, i, jm1, k) = qc - dq[1];
+        }
+        if (nd == 3) {
+          dql = QS(2, 0, i, j, k) - qc;
+          dqr = QS(2, 1, i, j, km1) - qc;
+          if (fmax_r(fabs(dql), fabs(dqr)) > fabs(dq[2])) {
+            QS(2, 0, i, j, k) = qc + dq[2];
+            QS(2, 1, i, j, km1) = qc - dq[2];
+          }
+        }
+      }
+    }
+  }
+#undef QS
+#undef Q
+}
+
 static void reconstruct_interfaces(ocfg* o, const double* q, double* qs)
 {
   const int im = o->im, jm = o->jm, km = o->km, nv = o->nv, nd = o->nd;
@@ -1111,7 +1224,7 @@ static void reconstruct_interfaces(ocfg* o, const double* q, double* qs)
   for (p = 0; p < nv; p++) {
     int positive = (p == o->idn || p == o->ipr);
     if (o->c.reconstruction == AMUN_REC_MGP) {
-      /* interfaces_mgp, interpolations.F90:978-1144 (mlp_limiting off): the
+      /* interfaces_mgp, interpolations.F90:978-1144: the
        * 2 nd face values of each cell from the GP weights; where any of them
        * leaves the range of its two cells by more than eps, all of them fall
        * back to the limited TVD slopes with the positivity halving */
@@ -1252,6 +1365,7 @@ static void reconstruct_interfaces(ocfg* o, const double* q, double* qs)
       }
       free(fp);
     }
+    if (o->c.mlp_limiting) mlp_limiting(o, q, p, qs);
   }
 #undef QS
 }
@@ -1958,7 +2072,14 @@ static void prim2cons(const ocfg* o, const double* q, double* u)
   u[o->imx] = q[o->idn] * q[o->ivx];
   u[o->imy] = q[o->idn] * q[o->ivy];
   u[o->imz] = q[o->idn] * q[o->ivz];
-  if (o->c.eqsys == AMUN_EQSYS_HD) {
+  if (o->iso) {      /* prim2cons_hd_iso 1360, prim2cons_mhd_iso 2220 */
+    if (o->mhd) {
+      u[o->ibx] = q[o->ibx];
+      u[o->iby] = q[o->iby];
+      u[o->ibz] = q[o->ibz];
+      u[o->ibp] = q[o->ibp];
+    }
+  } else if (o->c.eqsys == AMUN_EQSYS_HD) {
     ek = 0.5 * (u[o->imx] * q[o->ivx] + u[o->imy] * q[o->ivy] + u[o->imz] * q[o->ivz]);
     ei = o->gammam1i * q[o->ipr];
     u[o->ien] = ei + ek;
@@ -1982,7 +2103,14 @@ static void cons2prim(ocfg* o, const double* u, double* q)
   q[o->ivx] = u[o->imx] / u[o->idn];
   q[o->ivy] = u[o->imy] / u[o->idn];
   q[o->ivz] = u[o->imz] / u[o->idn];
-  if (o->c.eqsys == AMUN_EQSYS_HD) {
+  if (o->iso) {      /* cons2prim_hd_iso 1421, cons2prim_mhd_iso 2285 */
+    if (o->mhd) {
+      q[o->ibx] = u[o->ibx];
+      q[o->iby] = u[o->iby];
+      q[o->ibz] = u[o->ibz];
+      q[o->ibp] = u[o->ibp];
+    }
+  } else if (o->c.eqsys == AMUN_EQSYS_HD) {
     ek = 0.5 * (u[o->imx] * q[o->ivx] + u[o->imy] * q[o->ivy] + u[o->imz] * q[o->ivz]);
     ei = u[o->ien] - ek;
     q[o->ipr] = o->gammam1 * ei;
@@ -2003,7 +2131,38 @@ static void fluxspeed(const ocfg* o, const double* q, const double* u,
 {
   const double g = o->c.gamma;
   if (o->sr) { fluxspeed_sr(o, q, u, f, cm, cp); return; }
-  if (o->c.eqsys == AMUN_EQSYS_HD) {
+  if (o->iso && !o->mhd) {              /* fluxspeed_hd_iso 1484-1542 */
+    f[o->idn] = u[o->imx];
+    f[o->imx] = q[o->ivx] * u[o->imx];
+    f[o->imy] = q[o->ivx] * u[o->imy];
+    f[o->imz] = q[o->ivx] * u[o->imz];
+    f[o->imx] = f[o->imx] + o->csnd2 * q[o->idn];
+    *cm = q[o->ivx] - o->csnd;
+    *cp = q[o->ivx] + o->csnd;
+  } else if (o->iso) {                  /* fluxspeed_mhd_iso 2352-2435 */
+    double bx2, by2, bz2, bb, pt, fa, fb, fc, cf;
+    bx2 = q[o->ibx] * q[o->ibx];
+    by2 = q[o->iby] * q[o->iby];
+    bz2 = q[o->ibz] * q[o->ibz];
+    bb  = bx2 + by2 + bz2;
+    pt  = o->csnd2 * q[o->idn] + 0.5 * bb;
+    f[o->idn] = u[o->imx];
+    f[o->imx] = q[o->ivx] * u[o->imx] - bx2;
+    f[o->imy] = q[o->ivx] * u[o->imy] - q[o->ibx] * q[o->iby];
+    f[o->imz] = q[o->ivx] * u[o->imz] - q[o->ibx] * q[o->ibz];
+    f[o->imx] = f[o->imx] + pt;
+    f[o->ibx] = q[o->ibp];
+    f[o->iby] = q[o->ivx] * q[o->iby] - q[o->ibx] * q[o->ivy];
+    f[o->ibz] = q[o->ivx] * q[o->ibz] - q[o->ibx] * q[o->ivz];
+    f[o->ibp] = o->cmax2 * q[o->ibx];
+    fa = o->csnd2 * q[o->idn];
+    fb = fa + bb;
+    fc = fb * fb - 4.0 * fa * bx2;
+    if (fc > 0.0) cf = sqrt(0.5 * (fb + sqrt(fc)) / q[o->idn]);
+    else          cf = sqrt(0.5 * fb / q[o->idn]);
+    *cm = q[o->ivx] - cf;
+    *cp = q[o->ivx] + cf;
+  } else if (o->c.eqsys == AMUN_EQSYS_HD) {
     double cs;
     f[o->idn] = u[o->imx];
     f[o->imx] = q[o->ivx] * u[o->imx];
@@ -2050,7 +2209,12 @@ static double maxspeed_cell(const ocfg* o, const double* q)
   if (o->sr) return 1.0;
   vv = (q[o->ivx] * q[o->ivx] + q[o->ivy] * q[o->ivy]) + q[o->ivz] * q[o->ivz];
   v  = sqrt(vv);
-  if (o->c.eqsys == AMUN_EQSYS_HD) {
+  if (o->iso && !o->mhd) {              /* maxspeed_hd_iso 1546-1570 */
+    c = o->csnd;
+  } else if (o->iso) {                  /* maxspeed_mhd_iso 2438-2510 */
+    bb = (q[o->ibx] * q[o->ibx] + q[o->iby] * q[o->iby]) + q[o->ibz] * q[o->ibz];
+    c  = sqrt(o->csnd2 + bb / q[o->idn]);
+  } else if (o->c.eqsys == AMUN_EQSYS_HD) {
     c = sqrt(o->c.gamma * q[o->ipr] / q[o->idn]);
   } else {
     bb = (q[o->ibx] * q[o->ibx] + q[o->iby] * q[o->iby]) + q[o->ibz] * q[o->ibz];
@@ -2870,8 +3034,546 @@ static int riemann_check(const amun_cfg* c)
   return 300;
 }
 
+/* riemann_hd_iso_roe (schemes.F90:939-1057) with esystem_roe_hd_iso
+ * (equations.F90:1655-1738): Roe averages of density and velocity, the
+ * eigenvalues v -+ csnd and v, the upwinded sum over the waves in
+ * increasing (v >= 0) or decreasing order */
+static void riemann_hd_iso_roe(const ocfg* o, double* ql, double* qr, double* f)
+{
+  const int idn = o->idn, ivx = o->ivx, ivy = o->ivy, ivz = o->ivz, nv = o->nv;
+  double ul[NVMAX], ur[NVMAX], fl[NVMAX], fr[NVMAX], qi[NVMAX], ci[NVMAX], al[NVMAX];
+  double li[NVMAX][NVMAX], ri[NVMAX][NVMAX];
+  double sdl, sdr, sds, cm, cp, ch;
+  int p, q;
+  prim2cons(o, ql, ul);
+  prim2cons(o, qr, ur);
+  fluxspeed(o, ql, ul, fl, &cm, &cp);
+  fluxspeed(o, qr, ur, fr, &cm, &cp);
+  sdl = sqrt(ql[idn]);
+  sdr = sqrt(qr[idn]);
+  sds = sdl + sdr;
+  qi[idn] = sdl * sdr;
+  qi[ivx] = (sdl * ql[ivx] + sdr * qr[ivx]) / sds;
+  qi[ivy] = (sdl * ql[ivy] + sdr * qr[ivy]) / sds;
+  qi[ivz] = (sdl * ql[ivz] + sdr * qr[ivz]) / sds;
+  /* esystem_roe_hd_iso: l(wave, var), r(wave, var) */
+  ch = 0.5 / o->csnd;
+  for (p = 0; p < nv; p++)
+    for (q = 0; q < nv; q++) li[p][q] = ri[p][q] = 0.0;
+  ci[0] = qi[ivx] - o->csnd;
+  ci[1] = qi[ivx];
+  ci[2] = qi[ivx];
+  ci[3] = qi[ivx] + o->csnd;
+  li[ivx][0] = -ch;
+  li[ivy][1] = 1.0;
+  li[ivz][2] = 1.0;
+  li[ivx][3] = ch;
+  li[idn][0] = ch * ci[3];
+  li[idn][1] = -qi[ivy];
+  li[idn][2] = -qi[ivz];
+  li[idn][3] = -ch * ci[0];
+  ri[0][idn] = 1.0;
+  ri[1][ivy] = 1.0;
+  ri[2][ivz] = 1.0;
+  ri[3][idn] = 1.0;
+  ri[0][ivx] = ci[0];
+  ri[0][ivy] = qi[ivy];
+  ri[0][ivz] = qi[ivz];
+  ri[3][ivx] = ci[3];
+  ri[3][ivy] = qi[ivy];
+  ri[3][ivz] = qi[ivz];
+  if (ci[0] >= 0.0) {
+    for (p = 0; p < nv; p++) f[p] = fl[p];
+  } else if (ci[nv - 1] <= 0.0) {
+    for (p = 0; p < nv; p++) f[p] = fr[p];
+  } else {
+    /* al(1:nv) += li(p, 1:nv) (ur(p) - ul(p)), p = 1..nv: li is l(p, wave)
+     * with the variable first, as evroe(1, var, wave) */
+    for (q = 0; q < nv; q++) al[q] = 0.0;
+    for (p = 0; p < nv; p++)
+      for (q = 0; q < nv; q++) al[q] = al[q] + li[p][q] * (ur[p] - ul[p]);
+    for (q = 0; q < nv; q++) f[q] = 0.5 * (fl[q] + fr[q]);
+    if (qi[ivx] >= 0.0) {
+      for (p = 0; p < nv; p++) {
+        const double xx = -0.5 * al[p] * fabs(ci[p]);
+        for (q = 0; q < nv; q++) f[q] = f[q] + xx * ri[p][q];
+      }
+    } else {
+      for (p = nv - 1; p >= 0; p--) {
+        const double xx = -0.5 * al[p] * fabs(ci[p]);
+        for (q = 0; q < nv; q++) f[q] = f[q] + xx * ri[p][q];
+      }
+    }
+  }
+}
+
+/* riemann_mhd_iso_hlld (schemes.F90:2007-2364): the GLM pre-step, the HLL
+ * bounds, the contact speed sm from the HLL state's momentum, the Alfven
+ * speeds sm -+ sqrt(Bx^2 / rho*), and the branch tree over Bx^2 (against
+ * 1e-8 of the outer waves' scale, > 0, = 0) and the degeneracies */
+static void riemann_mhd_iso_hlld(const ocfg* o, double* ql, double* qr, double* f)
+{
+  const int idn = o->idn, imx = o->imx, imy = o->imy, imz = o->imz;
+  const int ibx = o->ibx, iby = o->iby, ibz = o->ibz, ibp = o->ibp, nv = o->nv;
+  double ul[NVMAX], ur[NVMAX], fl[NVMAX], fr[NVMAX], wl[NVMAX], wr[NVMAX];
+  double wcl[NVMAX], wcr[NVMAX], ui[NVMAX];
+  double clm, clp, crm, crp, sl, sr, sm, sml, smr, srml, slmm, srmm;
+  double bx, bp, b2, dn, dnl, dnr, dvl, dvr;
+  int p;
+  bx = 0.5 * ((qr[ibx] + ql[ibx]) - (qr[ibp] - ql[ibp]) / o->cmax);
+  bp = 0.5 * ((qr[ibp] + ql[ibp]) - (qr[ibx] - ql[ibx]) * o->cmax);
+  ql[ibx] = bx; qr[ibx] = bx;
+  ql[ibp] = bp; qr[ibp] = bp;
+  prim2cons(o, ql, ul);
+  prim2cons(o, qr, ur);
+  fluxspeed(o, ql, ul, fl, &clm, &clp);
+  fluxspeed(o, qr, ur, fr, &crm, &crp);
+  sl = fmin_r(clm, crm);
+  sr = fmax_r(clp, crp);
+  if (sl >= 0.0) {
+    for (p = 0; p < nv; p++) f[p] = fl[p];
+    return;
+  }
+  if (sr <= 0.0) {
+    for (p = 0; p < nv; p++) f[p] = fr[p];
+    return;
+  }
+  srml = sr - sl;
+  for (p = 0; p < nv; p++) {
+    wl[p] = sl * ul[p] - fl[p];
+    wr[p] = sr * ur[p] - fr[p];
+  }
+  dn = wr[idn] - wl[idn];
+  sm = (wr[imx] - wl[imx]) / dn;
+  bx = ql[ibx];
+  b2 = ql[ibx] * qr[ibx];
+  slmm = sl - sm;
+  srmm = sr - sm;
+  dnl = wl[idn] / slmm;
+  dnr = wr[idn] / srmm;
+#define STAR_L()                                                               \
+  ui[idn] = dnl;                                                               \
+  ui[imx] = dnl * sm;                                                          \
+  ui[imy] = dnl * (slmm * wl[imy] - bx * wl[iby]) / dvl;                       \
+  ui[imz] = dnl * (slmm * wl[imz] - bx * wl[ibz]) / dvl;                       \
+  ui[ibx] = bx;                                                                \
+  ui[iby] = (wl[idn] * wl[iby] - bx * wl[imy]) / dvl;                          \
+  ui[ibz] = (wl[idn] * wl[ibz] - bx * wl[imz]) / dvl;                          \
+  ui[ibp] = ul[ibp]
+#define STAR_R()                                                               \
+  ui[idn] = dnr;                                                               \
+  ui[imx] = dnr * sm;                                                          \
+  ui[imy] = dnr * (srmm * wr[imy] - bx * wr[iby]) / dvr;                       \
+  ui[imz] = dnr * (srmm * wr[imz] - bx * wr[ibz]) / dvr;                       \
+  ui[ibx] = bx;                                                                \
+  ui[iby] = (wr[idn] * wr[iby] - bx * wr[imy]) / dvr;                          \
+  ui[ibz] = (wr[idn] * wr[ibz] - bx * wr[imz]) / dvr;                          \
+  ui[ibp] = ur[ibp]
+  if (b2 > 1.0e-08 * fmax_r(dnl * (sl * sl), dnr * (sr * sr))) {
+    sml = sm - sqrt(b2 / dnl);
+    smr = sm + sqrt(b2 / dnr);
+    dvl = slmm * wl[idn] - b2;
+    dvr = srmm * wr[idn] - b2;
+    if (fmin_r(dvl, dvr) > 0.0) {
+      if (sml >= 0.0) {
+        STAR_L();
+        for (p = 0; p < nv; p++) f[p] = sl * ui[p] - wl[p];
+      } else if (smr <= 0.0) {
+        STAR_R();
+        for (p = 0; p < nv; p++) f[p] = sr * ui[p] - wr[p];
+      } else {
+        STAR_L();
+        for (p = 0; p < nv; p++) wcl[p] = (sml - sl) * ui[p] + wl[p];
+        STAR_R();
+        for (p = 0; p < nv; p++) wcr[p] = (smr - sr) * ui[p] + wr[p];
+        for (p = 0; p < nv; p++) f[p] = (sml * wcr[p] - smr * wcl[p]) / (smr - sml);
+      }
+    } else {
+      if (dvl > 0.0) {
+        STAR_L();
+        if (sml >= 0.0) {
+          for (p = 0; p < nv; p++) f[p] = sl * ui[p] - wl[p];
+        } else {
+          for (p = 0; p < nv; p++) wcl[p] = (sml - sl) * ui[p] + wl[p];
+          for (p = 0; p < nv; p++) f[p] = (sml * wr[p] - sr * wcl[p]) / (sr - sml);
+        }
+      } else if (dvr > 0.0) {
+        STAR_R();
+        if (smr <= 0.0) {
+          for (p = 0; p < nv; p++) f[p] = sr * ui[p] - wr[p];
+        } else {
+          for (p = 0; p < nv; p++) wcr[p] = (smr - sr) * ui[p] + wr[p];
+          for (p = 0; p < nv; p++) f[p] = (sl * wcr[p] - smr * wl[p]) / (smr - sl);
+        }
+      } else {
+        for (p = 0; p < nv; p++) f[p] = (sl * wr[p] - sr * wl[p]) / srml;
+      }
+    }
+  } else if (b2 > 0.0) {
+    for (p = 0; p < nv; p++) f[p] = (sl * wr[p] - sr * wl[p]) / srml;
+  } else {
+    if (sm > 0.0) {
+      ui[idn] = dnl;
+      ui[imx] = dnl * sm;
+      ui[imy] = wl[imy] / slmm;
+      ui[imz] = wl[imz] / slmm;
+      ui[ibx] = 0.0;
+      ui[iby] = wl[iby] / slmm;
+      ui[ibz] = wl[ibz] / slmm;
+      ui[ibp] = ul[ibp];
+      for (p = 0; p < nv; p++) f[p] = sl * ui[p] - wl[p];
+    } else if (sm < 0.0) {
+      ui[idn] = dnr;
+      ui[imx] = dnr * sm;
+      ui[imy] = wr[imy] / srmm;
+      ui[imz] = wr[imz] / srmm;
+      ui[ibx] = 0.0;
+      ui[iby] = wr[iby] / srmm;
+      ui[ibz] = wr[ibz] / srmm;
+      ui[ibp] = ur[ibp];
+      for (p = 0; p < nv; p++) f[p] = sr * ui[p] - wr[p];
+    } else {
+      for (p = 0; p < nv; p++) f[p] = (sl * wr[p] - sr * wl[p]) / srml;
+    }
+  }
+#undef STAR_L
+#undef STAR_R
+}
+
+/* esystem_roe_mhd_iso, equations.F90:2552-2791: the eigenvalues c[8] and
+ * the left (l[var][wave]) and right (r[wave][var]) eigenvectors of the
+ * isothermal MHD Roe matrix at the averaged state q (x and y the magnetic
+ * corrections).  Waves 4 and 8 and the components ibx and ibp stay zero;
+ * nothing but c is set when c[0] >= 0 or c[6] <= 0 (returns 0 then). */
+static int esystem_roe_mhd_iso(const ocfg* o, double x, double y, const double* q, double* c,
+                               double (*l)[NVMAX], double (*r)[NVMAX])
+{
+  const int idn = o->idn, ivx = o->ivx, ivy = o->ivy, ivz = o->ivz;
+  const int iby = o->iby, ibz = o->ibz, ibx = o->ibx, nv = o->nv;
+  double di, btsq, bt_starsq, casq, twid_csq, ct2, tsum, tdif, cf2_cs2, cfsq, cf, cssq, cs, ca;
+  double bt, bt_star, bet2, bet3, bet2_star, bet3_star, bet_starsq, alpha_f, alpha_s;
+  double sqrtd, s, twid_c, qf, qs, af_prime, as_prime, norm, cff, css, af, as, afpb, aspb;
+  double q2_star, q3_star, vqstr;
+  int p, k;
+  di = 1.0 / q[idn];
+  casq = q[ibx] * q[ibx] * di;
+  ca = sqrt(casq);
+  btsq = q[iby] * q[iby] + q[ibz] * q[ibz];
+  bt_starsq = btsq * y;
+  twid_csq = o->csnd2 + x;
+  ct2 = bt_starsq * di;
+  tsum = (casq + ct2) + twid_csq;
+  tdif = (casq + ct2) - twid_csq;
+  cf2_cs2 = sqrt(tdif * tdif + 4.0 * twid_csq * ct2);
+  cfsq = 0.5 * (tsum + cf2_cs2);
+  cf = sqrt(cfsq);
+  cssq = twid_csq * casq / cfsq;
+  cs = sqrt(cssq);
+  c[0] = q[ivx] - cf; c[1] = q[ivx] - ca; c[2] = q[ivx] - cs; c[3] = q[ivx];
+  c[4] = q[ivx] + cs; c[5] = q[ivx] + ca; c[6] = q[ivx] + cf; c[7] = c[6];
+  if (c[0] >= 0.0 || c[6] <= 0.0) return 0;
+  for (p = 0; p < nv; p++)
+    for (k = 0; k < nv; k++) { l[p][k] = 0.0; r[p][k] = 0.0; }
+  bt = sqrt(btsq);
+  bt_star = sqrt(bt_starsq);
+  if (bt == 0.0) { bet2 = 1.0; bet3 = 0.0; }
+  else { bet2 = q[iby] / bt; bet3 = q[ibz] / bt; }
+  bet2_star = bet2 / sqrt(y);
+  bet3_star = bet3 / sqrt(y);
+  bet_starsq = bet2_star * bet2_star + bet3_star * bet3_star;
+  if ((cfsq - cssq) == 0.0) { alpha_f = 1.0; alpha_s = 0.0; }
+  else if ((twid_csq - cssq) <= 0.0) { alpha_f = 0.0; alpha_s = 1.0; }
+  else if ((cfsq - twid_csq) <= 0.0) { alpha_f = 1.0; alpha_s = 0.0; }
+  else {
+    alpha_f = sqrt((twid_csq - cssq) / (cfsq - cssq));
+    alpha_s = sqrt((cfsq - twid_csq) / (cfsq - cssq));
+  }
+  sqrtd = sqrt(q[idn]);
+  s = copysign(1.0, q[ibx]);
+  twid_c = sqrt(twid_csq);
+  qf = cf * alpha_f * s;
+  qs = cs * alpha_s * s;
+  af_prime = twid_c * alpha_f / sqrtd;
+  as_prime = twid_c * alpha_s / sqrtd;
+  /* right eigenvectors r[wave][var] */
+  r[0][idn] = alpha_f;
+  r[0][ivx] = alpha_f * c[0];
+  r[0][ivy] = alpha_f * q[ivy] + qs * bet2_star;
+  r[0][ivz] = alpha_f * q[ivz] + qs * bet3_star;
+  r[0][iby] = as_prime * bet2_star;
+  r[0][ibz] = as_prime * bet3_star;
+  r[1][ivy] = -bet3;
+  r[1][ivz] = bet2;
+  r[1][iby] = -(bet3 * s / sqrtd);
+  r[1][ibz] = bet2 * s / sqrtd;
+  r[2][idn] = alpha_s;
+  r[2][ivx] = alpha_s * c[2];
+  r[2][ivy] = alpha_s * q[ivy] - qf * bet2_star;
+  r[2][ivz] = alpha_s * q[ivz] - qf * bet3_star;
+  r[2][iby] = -(af_prime * bet2_star);
+  r[2][ibz] = -(af_prime * bet3_star);
+  r[4][idn] = alpha_s;
+  r[4][ivx] = alpha_s * c[4];
+  r[4][ivy] = alpha_s * q[ivy] + qf * bet2_star;
+  r[4][ivz] = alpha_s * q[ivz] + qf * bet3_star;
+  r[4][iby] = r[2][iby];
+  r[4][ibz] = r[2][ibz];
+  r[5][ivy] = bet3;
+  r[5][ivz] = -bet2;
+  r[5][iby] = r[1][iby];
+  r[5][ibz] = r[1][ibz];
+  r[6][idn] = alpha_f;
+  r[6][ivx] = alpha_f * c[6];
+  r[6][ivy] = alpha_f * q[ivy] - qs * bet2_star;
+  r[6][ivz] = alpha_f * q[ivz] - qs * bet3_star;
+  r[6][iby] = r[0][iby];
+  r[6][ibz] = r[0][ibz];
+  /* left eigenvectors l[var][wave] */
+  norm = 0.5 / twid_csq;
+  cff = norm * alpha_f * cf;
+  css = norm * alpha_s * cs;
+  qf = qf * norm;
+  qs = qs * norm;
+  af = norm * af_prime * q[idn];
+  as = norm * as_prime * q[idn];
+  afpb = norm * af_prime * bt_star;
+  aspb = norm * as_prime * bt_star;
+  q2_star = bet2_star / bet_starsq;
+  q3_star = bet3_star / bet_starsq;
+  vqstr = q[ivy] * q2_star + q[ivz] * q3_star;
+  l[idn][0] = (cff * c[6] - qs * vqstr) - aspb;
+  l[ivx][0] = -cff;
+  l[ivy][0] = qs * q2_star;
+  l[ivz][0] = qs * q3_star;
+  l[iby][0] = as * q2_star;
+  l[ibz][0] = as * q3_star;
+  l[idn][1] = 0.5 * (q[ivy] * bet3 - q[ivz] * bet2);
+  l[ivy][1] = -(0.5 * bet3);
+  l[ivz][1] = 0.5 * bet2;
+  l[iby][1] = -(0.5 * sqrtd * bet3 * s);
+  l[ibz][1] = 0.5 * sqrtd * bet2 * s;
+  l[idn][2] = (css * c[4] + qf * vqstr) + afpb;
+  l[ivx][2] = -css;
+  l[ivy][2] = -(qf * q2_star);
+  l[ivz][2] = -(qf * q3_star);
+  l[iby][2] = -(af * q2_star);
+  l[ibz][2] = -(af * q3_star);
+  l[idn][4] = (-(css * c[2]) - qf * vqstr) + afpb;
+  l[ivx][4] = css;
+  l[ivy][4] = -l[ivy][2];
+  l[ivz][4] = -l[ivz][2];
+  l[iby][4] = l[iby][2];
+  l[ibz][4] = l[ibz][2];
+  l[idn][5] = -l[idn][1];
+  l[ivy][5] = -l[ivy][1];
+  l[ivz][5] = -l[ivz][1];
+  l[iby][5] = l[iby][1];
+  l[ibz][5] = l[ibz][1];
+  l[idn][6] = (-(cff * c[0]) + qs * vqstr) - aspb;
+  l[ivx][6] = cff;
+  l[ivy][6] = -l[ivy][0];
+  l[ivz][6] = -l[ivz][0];
+  l[iby][6] = l[iby][0];
+  l[ibz][6] = l[ibz][0];
+  return 1;
+}
+
+/* riemann_mhd_iso_roe, schemes.F90:2778-2906: the GLM pre-step, Roe
+ * averages with the magnetic corrections x, y, the eigensystem above and
+ * the flux 1/2 (fl + fr) minus the wave contributions, added from the left
+ * wave for a non-negative Roe velocity and from the right one otherwise;
+ * the Bx and psi fluxes are the left ones */
+static void riemann_mhd_iso_roe(const ocfg* o, double* ql, double* qr, double* f)
+{
+  const int idn = o->idn, ivx = o->ivx, ivy = o->ivy, ivz = o->ivz;
+  const int ibx = o->ibx, iby = o->iby, ibz = o->ibz, ibp = o->ibp, nv = o->nv;
+  const int fv[6] = {o->idn, o->imx, o->imy, o->imz, o->iby, o->ibz};
+  double ul[NVMAX], ur[NVMAX], fl[NVMAX], fr[NVMAX], qi[NVMAX], ci[NVMAX], al[NVMAX];
+  double li[NVMAX][NVMAX], ri[NVMAX][NVMAX], sdl, sdr, sds, xx, yy, bx, bp, cm_, cp_;
+  int p, k, m;
+  bx = 0.5 * ((qr[ibx] + ql[ibx]) - (qr[ibp] - ql[ibp]) / o->cmax);
+  bp = 0.5 * ((qr[ibp] + ql[ibp]) - (qr[ibx] - ql[ibx]) * o->cmax);
+  ql[ibx] = bx; qr[ibx] = bx;
+  ql[ibp] = bp; qr[ibp] = bp;
+  prim2cons(o, ql, ul);
+  prim2cons(o, qr, ur);
+  fluxspeed(o, ql, ul, fl, &cm_, &cp_);
+  fluxspeed(o, qr, ur, fr, &cm_, &cp_);
+  sdl = sqrt(ql[idn]);
+  sdr = sqrt(qr[idn]);
+  sds = sdl + sdr;
+  for (p = 0; p < nv; p++) qi[p] = 0.0;
+  qi[idn] = sdl * sdr;
+  qi[ivx] = (sdl * ql[ivx] + sdr * qr[ivx]) / sds;
+  qi[ivy] = (sdl * ql[ivy] + sdr * qr[ivy]) / sds;
+  qi[ivz] = (sdl * ql[ivz] + sdr * qr[ivz]) / sds;
+  qi[ibx] = ql[ibx];
+  qi[iby] = (sdr * ql[iby] + sdl * qr[iby]) / sds;
+  qi[ibz] = (sdr * ql[ibz] + sdl * qr[ibz]) / sds;
+  qi[ibp] = ql[ibp];
+  xx = 0.5 * ((ql[iby] - qr[iby]) * (ql[iby] - qr[iby]) + (ql[ibz] - qr[ibz]) * (ql[ibz] - qr[ibz]))
+     / (sds * sds);
+  yy = 0.5 * (ql[idn] + qr[idn]) / qi[idn];
+  if (!esystem_roe_mhd_iso(o, xx, yy, qi, ci, li, ri)) {
+    const double* fs = ci[0] >= 0.0 ? fl : fr;
+    for (p = 0; p < nv; p++) f[p] = fs[p];
+    return;
+  }
+  f[ibx] = fl[ibx];
+  f[ibp] = fl[ibp];
+  for (k = 0; k < nv; k++) al[k] = 0.0;
+  for (p = 0; p < nv; p++)
+    for (k = 0; k < nv; k++) al[k] = al[k] + li[p][k] * (ur[p] - ul[p]);
+  for (m = 0; m < 6; m++) f[fv[m]] = 0.5 * (fl[fv[m]] + fr[fv[m]]);
+  for (k = 0; k < nv; k++) {
+    const int w = qi[ivx] >= 0.0 ? k : nv - 1 - k;
+    xx = -(0.5 * al[w] * fabs(ci[w]));
+    for (m = 0; m < 6; m++) f[fv[m]] = f[fv[m]] + xx * ri[w][fv[m]];
+  }
+}
+
+/* riemann_mhd_iso_hlldm, schemes.F90:2388-2777: HLLD with one star density
+ * for both sides, the HLL state's (wr - wl)(idn) / (sr - sl) (Mignone's
+ * variant): Alfven speeds sm -+ sqrt(Bx^2 / rho*), the threshold against
+ * 1e-8 max(sl^2, sr^2) and the same branch tree as riemann_mhd_iso_hlld */
+static void riemann_mhd_iso_hlldm(const ocfg* o, double* ql, double* qr, double* f)
+{
+  const int idn = o->idn, imx = o->imx, imy = o->imy, imz = o->imz;
+  const int ibx = o->ibx, iby = o->iby, ibz = o->ibz, ibp = o->ibp, nv = o->nv;
+  double ul[NVMAX], ur[NVMAX], fl[NVMAX], fr[NVMAX], wl[NVMAX], wr[NVMAX];
+  double wcl[NVMAX], wcr[NVMAX], ui[NVMAX];
+  double clm, clp, crm, crp, sl, sr, sm, sml, smr, srml, slmm, srmm;
+  double bx, bp, b2, dn, dvl, dvr, ca, ca2;
+  int p;
+  bx = 0.5 * ((qr[ibx] + ql[ibx]) - (qr[ibp] - ql[ibp]) / o->cmax);
+  bp = 0.5 * ((qr[ibp] + ql[ibp]) - (qr[ibx] - ql[ibx]) * o->cmax);
+  ql[ibx] = bx; qr[ibx] = bx;
+  ql[ibp] = bp; qr[ibp] = bp;
+  prim2cons(o, ql, ul);
+  prim2cons(o, qr, ur);
+  fluxspeed(o, ql, ul, fl, &clm, &clp);
+  fluxspeed(o, qr, ur, fr, &crm, &crp);
+  sl = fmin_r(clm, crm);
+  sr = fmax_r(clp, crp);
+  if (sl >= 0.0) {
+    for (p = 0; p < nv; p++) f[p] = fl[p];
+    return;
+  }
+  if (sr <= 0.0) {
+    for (p = 0; p < nv; p++) f[p] = fr[p];
+    return;
+  }
+  srml = sr - sl;
+  for (p = 0; p < nv; p++) {
+    wl[p] = sl * ul[p] - fl[p];
+    wr[p] = sr * ur[p] - fr[p];
+  }
+  dn = wr[idn] - wl[idn];
+  sm = (wr[imx] - wl[imx]) / dn;
+  bx = ql[ibx];
+  b2 = ql[ibx] * qr[ibx];
+  slmm = sl - sm;
+  srmm = sr - sm;
+  dn = dn / srml;
+  ca2 = b2 / dn;
+#define STAR_L()                                                               \
+  ui[idn] = dn;                                                                \
+  ui[imx] = dn * sm;                                                           \
+  ui[imy] = dn * (slmm * wl[imy] - bx * wl[iby]) / dvl;                        \
+  ui[imz] = dn * (slmm * wl[imz] - bx * wl[ibz]) / dvl;                        \
+  ui[ibx] = bx;                                                                \
+  ui[iby] = (slmm * dn * wl[iby] - bx * wl[imy]) / dvl;                        \
+  ui[ibz] = (slmm * dn * wl[ibz] - bx * wl[imz]) / dvl;                        \
+  ui[ibp] = ul[ibp]
+#define STAR_R()                                                               \
+  ui[idn] = dn;                                                                \
+  ui[imx] = dn * sm;                                                           \
+  ui[imy] = dn * (srmm * wr[imy] - bx * wr[iby]) / dvr;                        \
+  ui[imz] = dn * (srmm * wr[imz] - bx * wr[ibz]) / dvr;                        \
+  ui[ibx] = bx;                                                                \
+  ui[iby] = (srmm * dn * wr[iby] - bx * wr[imy]) / dvr;                        \
+  ui[ibz] = (srmm * dn * wr[ibz] - bx * wr[imz]) / dvr;                        \
+  ui[ibp] = ur[ibp]
+  if (ca2 > 1.0e-08 * fmax_r(sl * sl, sr * sr)) {
+    ca = sqrt(ca2);
+    sml = sm - ca;
+    smr = sm + ca;
+    dvl = slmm * slmm * dn - b2;
+    dvr = srmm * srmm * dn - b2;
+    if (fmin_r(dvl, dvr) > 0.0) {
+      if (sml >= 0.0) {
+        STAR_L();
+        for (p = 0; p < nv; p++) f[p] = sl * ui[p] - wl[p];
+      } else if (smr <= 0.0) {
+        STAR_R();
+        for (p = 0; p < nv; p++) f[p] = sr * ui[p] - wr[p];
+      } else {
+        STAR_L();
+        for (p = 0; p < nv; p++) wcl[p] = (sml - sl) * ui[p] + wl[p];
+        STAR_R();
+        for (p = 0; p < nv; p++) wcr[p] = (smr - sr) * ui[p] + wr[p];
+        for (p = 0; p < nv; p++) f[p] = (sml * wcr[p] - smr * wcl[p]) / (smr - sml);
+      }
+    } else if (dvl > 0.0) {
+      STAR_L();
+      if (sml >= 0.0) {
+        for (p = 0; p < nv; p++) f[p] = sl * ui[p] - wl[p];
+      } else {
+        for (p = 0; p < nv; p++) wcl[p] = (sml - sl) * ui[p] + wl[p];
+        for (p = 0; p < nv; p++) f[p] = (sml * wr[p] - sr * wcl[p]) / (sr - sml);
+      }
+    } else if (dvr > 0.0) {
+      STAR_R();
+      if (smr <= 0.0) {
+        for (p = 0; p < nv; p++) f[p] = sr * ui[p] - wr[p];
+      } else {
+        for (p = 0; p < nv; p++) wcr[p] = (smr - sr) * ui[p] + wr[p];
+        for (p = 0; p < nv; p++) f[p] = (sl * wcr[p] - smr * wl[p]) / (smr - sl);
+      }
+    } else {
+      for (p = 0; p < nv; p++) f[p] = (sl * wr[p] - sr * wl[p]) / srml;
+    }
+  } else if (b2 > 0.0) {
+    for (p = 0; p < nv; p++) f[p] = (sl * wr[p] - sr * wl[p]) / srml;
+  } else if (sm > 0.0) {
+    ui[idn] = dn;
+    ui[imx] = dn * sm;
+    ui[imy] = wl[imy] / slmm;
+    ui[imz] = wl[imz] / slmm;
+    ui[ibx] = 0.0;
+    ui[iby] = wl[iby] / slmm;
+    ui[ibz] = wl[ibz] / slmm;
+    ui[ibp] = ul[ibp];
+    for (p = 0; p < nv; p++) f[p] = sl * ui[p] - wl[p];
+  } else if (sm < 0.0) {
+    ui[idn] = dn;
+    ui[imx] = dn * sm;
+    ui[imy] = wr[imy] / srmm;
+    ui[imz] = wr[imz] / srmm;
+    ui[ibx] = 0.0;
+    ui[iby] = wr[iby] / srmm;
+    ui[ibz] = wr[ibz] / srmm;
+    ui[ibp] = ur[ibp];
+    for (p = 0; p < nv; p++) f[p] = sr * ui[p] - wr[p];
+  } else {
+    for (p = 0; p < nv; p++) f[p] = (sl * wr[p] - sr * wl[p]) / srml;
+  }
+#undef STAR_L
+#undef STAR_R
+}
+
 static void riemann(const ocfg* o, double* ql, double* qr, double* f)
 {
+  if (o->iso) {
+    /* schemes.F90:163-310: HD iso takes "roe", MHD iso "hlld", every other
+     * name the HLL solver */
+    if (!o->mhd && o->c.riemann_solver == AMUN_RS_ROE) riemann_hd_iso_roe(o, ql, qr, f);
+    else if (o->mhd && o->c.riemann_solver == AMUN_RS_HLLD) riemann_mhd_iso_hlld(o, ql, qr, f);
+    else if (o->mhd && o->c.riemann_solver == AMUN_RS_HLLDM) riemann_mhd_iso_hlldm(o, ql, qr, f);
+    else if (o->mhd && o->c.riemann_solver == AMUN_RS_ROE) riemann_mhd_iso_roe(o, ql, qr, f);
+    else riemann_hll(o, ql, qr, f);
+    return;
+  }
   if (o->sr) {
     if (o->c.riemann_solver == AMUN_RS_HLLC) riemann_srmhd_hllc(o, ql, qr, f);
     else riemann_hll(o, ql, qr, f);               /* riemann_srmhd_adi_hll */

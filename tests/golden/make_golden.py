@@ -407,6 +407,47 @@ def main_glm():
         json.dump(index, fh, indent=1)
 
 
+# (name, ndims, reconstruction, ghosts, steps): runs with mlp_limiting on
+MLP_CASES = (("mlp_tvd_otnoisy2d_32_8", 2, "tvd", 2, 8),
+             ("mlp_weno5z_otnoisy2d_32_8", 2, "weno5z", 4, 8),
+             ("mlp_mgp_otnoisy2d_32_4", 2, "mgp", 4, 4),
+             ("mlp_tvd_noisy3d_16_3", 3, "tvd", 2, 3))
+
+
+def mlp_params(nd, rec, ng):
+    p = Params(ndims=nd, equation_system="mhd", reconstruction=rec, limiter="mc",
+               riemann_solver="hlld", time_advance="rk3", gamma=5.0 / 3.0, cfl=0.3, ncells=8,
+               nghosts=ng, xblocks=4 if nd == 2 else 2, yblocks=4 if nd == 2 else 2,
+               zblocks=2)
+    p.mlp_limiting = "on"
+    return p
+
+
+def mlp_state(name, p):
+    return noisy3d(p) if "noisy3d" in name else ot_noisy(p)
+
+
+def main_mlp():
+    """The multi-dimensional limiting process (mlp_limiting,
+    interpolations.F90:1171-1321) after TVD, WENO5-Z and MGP reconstructions:
+    short runs of the reference (own generator)."""
+    with open(os.path.join(HERE, "index.json")) as fh:
+        index = json.load(fh)
+    for name, nd, rec, ng, nsteps in MLP_CASES:
+        p = mlp_params(nd, rec, ng)
+        u0 = mlp_state(name, p)
+        u, tr = ol.ref_run(p, u0, nsteps)
+        st = 3 if nd == 2 else 2
+        sub = u[:, :, ::st, ::st] if nd == 2 else u[:, ::st, ::st, ::st]
+        np.savez_compressed(os.path.join(HERE, name + ".npz"),
+                            params=np.array(p.to_params_in()), ndims=nd, nsteps=nsteps,
+                            stride=st, trace=tr, sub=sub, sha_u0=sha(u0), sha_u=sha(u))
+        index[name] = {"steps": nsteps, "sha_u": sha(u), "dt_last": float(tr[-1, 0])}
+        print(name, "dt", tr[-1, 0])
+    with open(os.path.join(HERE, "index.json"), "w") as fh:
+        json.dump(index, fh, indent=1)
+
+
 def main_hllc_mhd():
     """riemann_mhd_adi_hllc through update_flux (zero limiter), random states
     with weak, zero and strong Bx, and an Orszag-Tang run (own generator)."""
@@ -884,6 +925,9 @@ def main():
     if sys.argv[1:] == ["refine"]:
         main_refine()
         sys.exit(0)
+    if sys.argv[1:] == ["mlp"]:
+        main_mlp()
+        return
     if sys.argv[1:] == ["glm"]:
         main_glm()
         return

@@ -66,13 +66,12 @@ def test_config_errors(field, value, code):
 
 
 @pytest.mark.parametrize("key,value", [
-    ("mlp_limiting", "on"), ("fix_unphysical_cells", "on"), ("viscosity", "1.0e-3"),
-    ("resistivity", "1.0e-3"),
+    ("fix_unphysical_cells", "on"), ("viscosity", "1.0e-3"), ("resistivity", "1.0e-3"),
 ])
 def test_unsupported_terms_are_refused(key, value):
-    """Parameters of terms outside the device path (MLP limiting, the
-    unphysical-cell fix, the viscous/resistive sources) are carried into
-    amun_cfg and refused with 600 instead of being ignored."""
+    """Parameters of terms outside the device path (the unphysical-cell fix,
+    the viscous/resistive sources) are carried into amun_cfg and refused with
+    600 instead of being ignored."""
     p = Params.from_text(f"reconstruction = tvd\nriemann_solver = hllc\n{key} = {value}\n",
                          ndims=2)
     assert key not in p.extra
@@ -136,3 +135,18 @@ def test_bench_refuses_several_gpus():
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stderr
+
+
+def test_mlp_limiting_ghost_layers():
+    """With two ghost layers the multi-dimensional limiting is built for the
+    TVD and MGP reconstructions only (the reference's process reads the face
+    state of each block's outermost cell); others need nghosts >= 3."""
+    lib = solver.load_library()
+    for rec, ng, code in (("weno3", 2, 600), ("limo3", 2, 600), ("weno3", 3, 0),
+                          ("tvd", 2, 0)):
+        p = Params(ndims=2, equation_system="mhd", riemann_solver="hlld", reconstruction=rec,
+                   nghosts=ng, mlp_limiting="on")
+        cfg = p.to_cfg()
+        h = ctypes.c_void_p()
+        rc = lib.amun_dev_init(ctypes.byref(cfg), ctypes.byref(h))
+        assert rc == code or (code == 0 and rc >= 1000), (rec, ng, rc)

@@ -201,13 +201,86 @@ def test_c5_gradient():
     +-eps perturbations cross kinks of the MC limiter and branch switches of
     HLLD, and the central difference, which averages across them, differs
     from the one-sided derivative by up to ~1e-2 in some directions: reported
-    here, and checked to 1e-5 on the smooth configurations below."""
+    here, explained direction by direction by the one-sided differences of
+    the next test, and checked to 1e-5 on the smooth configurations below."""
     p = c5_params(256)
     e_fw, e_fd = _directional(p, 300, 32, autograd=True)
     print(f"C5 32 directions: adjoint vs forward mode max {e_fw.max():.2e}; adjoint vs "
           f"central differences median {np.median(e_fd):.2e}, max {e_fd.max():.2e}")
     assert e_fw.max() <= 1e-12
     assert np.median(e_fd) <= 1e-3
+
+
+def test_c5_gradient_vs_one_sided_differences():
+    """Why central differences miss the C5 adjoint: over 300 steps the
+    discrete map has kinks (the MC limiter's min/sign switches, the HLLD
+    region tests, the positivity halvings) wherever a perturbation moves a
+    face across a branch boundary.  Along each of 32 seeded directions the
+    one-sided differences (J(+eps) - J(0)) / eps and (J(0) - J(-eps)) / eps
+    (step sizes held) agree to O(eps J'') where no kink lies within +-eps,
+    and differ by the kink's slope jump where one does; the central
+    difference is their average.  At eps = 1e-7 (a perturbation of ~1e-9 per
+    cell) every direction whose one-sided differences agree to 1e-5 matches
+    the adjoint to <= 1e-5 (the floor there is J's rounding, ~1e-17 / 1e-7
+    relative to a directional derivative of ~1e-5); at eps = 1e-6 kinks lie
+    within reach of more directions.  Measured: at 1e-7, 25 of 32 directions
+    without a kink, within 7.1e-6 of the adjoint; at 1e-6, 10 of 32, within
+    7.4e-6; the kinked ones up to 4e-5 and 2e-2."""
+    p = c5_params(256)
+    v, b, pr = ot_fields(p)
+    u0 = orszag_tang_2d(p)
+    dx = p.dx()
+    dvol = dx[0] * dx[1]
+    g1 = p.gamma - 1.0
+
+    def u_of(rho, g):
+        q = np.zeros_like(u0)
+        q[0], q[1:4], q[4:7], q[8] = rho, v, b, pr
+        pp = Params(**{k: x for k, x in p.__dict__.items() if k != "extra"})
+        pp.gamma = g
+        return prim_to_cons(pp, q), pp
+
+    with Solver(p) as s:
+        s.upload(u0)
+        s.tape_forward(300, 20)
+        dt, cm = s.tape_steps()
+        uN = s.download()
+        vv = uN[1:4] / uN[0]
+        lam = np.zeros_like(uN)
+        lam[1:4] = vv * dvol
+        lam[0] = -0.5 * (vv * vv).sum(0) * dvol
+        lam0, gg = s.tape_backward(lam)
+    grho = lam0[0] + (lam0[1:4] * v).sum(0) + lam0[8] * 0.5 * (v * v).sum(0)
+    ggam = gg - float((lam0[8] * pr).sum()) / g1 ** 2
+
+    def J(rho, g):
+        uu, pp = u_of(rho, g)
+        with Solver(pp) as f:
+            f.upload(uu)
+            f.run_fixed(dt, cm)
+            un = f.download()
+        return 0.5 * ((un[1:4] ** 2).sum(0) / un[0]).sum() * dvol
+
+    j0 = J(u0[0], p.gamma)
+    rng = np.random.default_rng(20261202)
+    dirs = [(rng.normal(size=u0.shape[1:]) * 0.01, float(rng.normal()) * 0.01)
+            for _ in range(32)]
+    for eps, min_smooth in ((1e-7, 20), (1e-6, 6)):
+        smooth, errs = [], []
+        for drho, dg in dirs:
+            adj = float((grho * drho).sum()) + ggam * dg
+            jp = J(u0[0] + eps * drho, p.gamma + eps * dg)
+            jm = J(u0[0] - eps * drho, p.gamma - eps * dg)
+            mism = abs((jp - j0) - (j0 - jm)) / eps / abs(adj)
+            err = abs((jp - jm) / (2 * eps) - adj) / abs(adj)
+            smooth.append(mism <= 1e-5)
+            errs.append(err)
+        smooth, errs = np.array(smooth), np.array(errs)
+        print(f"eps {eps:.0e}: {smooth.sum()} of 32 directions without a kink within +-eps, "
+              f"central vs adjoint max {errs[smooth].max():.2e} there; the {(~smooth).sum()} "
+              f"others up to {errs[~smooth].max() if (~smooth).any() else 0.0:.2e}")
+        assert smooth.sum() >= min_smooth
+        assert errs[smooth].max() <= 1e-5
 
 
 def test_c5_gradient_fp32_fluxes():

@@ -96,8 +96,32 @@ def test_one_step_vs_oracle(case):
     assert err.max() <= 1e-12
 
 
+@pytest.mark.parametrize("nd", [2, 3])
+@pytest.mark.parametrize("extra", [{"mlp_limiting": "on"}, {"glm_source_terms": "heglm"}])
+def test_one_step_vs_oracle_mlp_glm(nd, extra):
+    """mlp_limiting and the HEGLM source terms on a three-level mesh: the
+    process reads the virtual blocks (and the fine-face fluxes of the flux
+    correction apply it too); the source terms take each level's cell size."""
+    p, meta, u0 = _case(nd)
+    for k, v in extra.items():
+        setattr(p, k, v)
+    print(f"[case] {nd}D {extra} blocks={len(meta)}", flush=True)
+    o = ol.OracleAmr(p, meta)
+    o.set_blocks(u0)
+    tr = o.run(1)
+    uo = o.get_blocks()
+    with AmrSolver(p, meta) as s:
+        s.upload(u0)
+        dt, cmax = s.advance()
+        ug = s.download()
+    err = relerr(ug.reshape(-1, p.nv), uo.reshape(-1, p.nv))
+    print(f"  max rel {err.max():.2e}")
+    assert abs(dt - tr[0, 0]) <= 1e-13 * tr[0, 0]
+    assert err.max() <= 1e-12
+
+
 @pytest.mark.parametrize("name", ["amr_mhd2d_blast_10", "amr_mhd3d_blast_4",
-                                  "amr_limo3_mhd2d_blast_6"])
+                                  "amr_limo3_mhd2d_blast_6", "amr_heglm_mhd2d_blast_6"])
 def test_reference_runs(name):
     """The reference's own three-level blast runs (golden fixtures)."""
     g = np.load(os.path.join(GOLD, name + ".npz"), allow_pickle=False)

@@ -1035,3 +1035,72 @@ def test_glm_source_terms_refused_for_gradients():
         s.upload(_mhd_state(p))
         with pytest.raises(AmunError):
             s.tape_forward(2, 1)
+
+
+# ------------------------------------------------ multi-dimensional limiting
+@pytest.mark.parametrize("nd", [2, 3])
+@pytest.mark.parametrize("eq,rs", [("hd", "hllc"), ("mhd", "hlld")])
+@pytest.mark.parametrize("rec", ["tvd", "weno5z", "mgp", "crweno5z"])
+def test_mlp_limiting_one_step(nd, eq, rs, rec):
+    """mlp_limiting (interpolations.F90:1171-1321) after the TVD, WENO5-Z,
+    MGP and compact face states: one step against the oracle; the process
+    changes the step by far more than the tolerance."""
+    p = Params(ndims=nd, equation_system=eq, riemann_solver=rs, reconstruction=rec,
+               limiter="mc", time_advance="rk3", ncells=8, nghosts=2 if rec == "tvd" else 4,
+               xblocks=2, yblocks=3, zblocks=2, cfl=0.3, gamma=1.4 if eq == "hd" else 5.0 / 3.0,
+               xmin=-0.5, xmax=0.5, ymin=-0.75, ymax=0.75, zmin=0.0, zmax=1.0,
+               mlp_limiting="on")
+    u0 = _hd_noisy_state(p, jump=10.0) if eq == "hd" else _mhd_state(p, noise=0.05)
+    dt, cmax, ug, tr, uo = one_step(p, u0)
+    assert abs(dt - tr[0]) <= 1e-13 * tr[0]
+    e = relerr(ug, uo)
+    p.mlp_limiting = "off"
+    o = ol.OracleRun(p)
+    o.set_state(u0)
+    o.run(1)
+    d = relerr(o.get_state(), uo)
+    print(f"  vs oracle max {e.max():.2e}; the process changes the step by {d.max():.2e}")
+    assert e.max() <= (1e-11 if eq == "hd" else 1e-12) and d.max() >= 1e-8
+
+
+@pytest.mark.parametrize("name", ["mlp_tvd_otnoisy2d_32_8", "mlp_weno5z_otnoisy2d_32_8",
+                                  "mlp_mgp_otnoisy2d_32_4", "mlp_tvd_noisy3d_16_3"])
+def test_mlp_limiting_golden_runs(name):
+    """The reference's own runs with mlp_limiting on (dt trace and a
+    subsample of the final state), FP64; the mixed mode within 1e-5."""
+    import os
+    import sys
+    gold = os.path.join(os.path.dirname(__file__), "golden")
+    sys.path.insert(0, gold)
+    from make_golden import mlp_state
+    g = np.load(os.path.join(gold, name + ".npz"))
+    p = Params.from_text(str(g["params"]), ndims=int(g["ndims"]))
+    assert p.mlp_limiting == "on"
+    u0 = mlp_state(name, p)
+    st = int(g["stride"])
+    for prec, bound in (("fp64", 1e-10), ("mixed", 1e-5)):
+        p.precision = prec
+        print(f"[case] {name} {prec}", flush=True)
+        with Solver(p) as s:
+            s.upload(u0)
+            dts = s.run(int(g["nsteps"]))
+            u = s.download()
+        sub = u[:, ::st, ::st, ::st] if p.ndims == 3 else u[:, :, ::st, ::st]
+        e = relerr(sub, g["sub"]).max()
+        print(f"  dt rel {np.abs(dts / g['trace'][:, 0] - 1).max():.2e}, state max rel {e:.2e}")
+        assert e <= bound
+        if prec == "fp64":
+            assert np.abs(dts - g["trace"][:, 0]).max() <= 1e-12 * g["trace"][:, 0].max()
+
+
+def test_mlp_limiting_with_domain_sides():
+    """The process at non-periodic sides reads the boundary ghost blocks:
+    one step with reflective and outflow sides against the oracle."""
+    p = Params(ndims=2, equation_system="mhd", riemann_solver="hlld", reconstruction="tvd",
+               limiter="mc", time_advance="rk3", ncells=8, nghosts=2, xblocks=2, yblocks=3,
+               cfl=0.3, xmin=-0.5, xmax=0.5, ymin=-0.75, ymax=0.75, mlp_limiting="on",
+               xlbndry="reflective", xubndry="reflective", ylbndry="outflow",
+               yubndry="outflow")
+    u0 = _mhd_state(p, noise=0.05)
+    dt, cmax, ug, tr, uo = one_step(p, u0)
+    assert relerr(ug, uo).max() <= 1e-12

@@ -553,3 +553,29 @@ def test_glm_source_terms_amr_golden():
     tr = o.run(int(r["nsteps"]))
     np.testing.assert_array_equal(tr[:, 0], r["trace"][:, 0])
     assert sha(o.get_blocks()) == str(r["sha_u"])
+
+
+# ----------------------------------------------- multi-dimensional limiting
+@pytest.mark.parametrize("name", ["mlp_tvd_otnoisy2d_32_8", "mlp_weno5z_otnoisy2d_32_8",
+                                  "mlp_mgp_otnoisy2d_32_4", "mlp_tvd_noisy3d_16_3"])
+def test_mlp_limiting_golden(name):
+    """mlp_limiting (interpolations.F90:1171-1321) after the TVD, WENO5-Z and
+    MGP face states, 2D and 3D: the oracle's run is the reference's bit for
+    bit, and the same run without the process is not."""
+    import sys
+    sys.path.insert(0, GOLD)
+    from make_golden import mlp_state
+    r = load(name)
+    p = params_of(r)
+    u0 = mlp_state(name, p)
+    assert sha(u0) == str(r["sha_u0"])
+    o = ol.OracleRun(p)
+    o.set_state(u0)
+    tr = o.run(int(r["nsteps"]))
+    np.testing.assert_array_equal(tr[:, 0], r["trace"][:, 0])
+    assert sha(o.get_state()) == str(r["sha_u"])
+    p.mlp_limiting = "off"
+    o = ol.OracleRun(p)
+    o.set_state(u0)
+    o.run(int(r["nsteps"]))
+    assert sha(o.get_state()) != str(r["sha_u"])
