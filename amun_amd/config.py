@@ -26,7 +26,7 @@ RECON = {"tvd": 1, "weno3": 2, "limo3": 3, "ppm": 4, "crweno5z": 8, "crweno5-z":
 LIMITER = {"zero": 0, "mm": 1, "minmod": 1, "mc": 2, "monotonized_central": 2,
            "sb": 3, "superbee": 3, "vl": 4, "vanleer": 4, "va": 5,
            "vanalbada": 5}
-RIEMANN = {"hll": 1, "hllc": 2, "hlld": 3, "roe": 4}
+RIEMANN = {"hll": 1, "hllc": 2, "hlld": 3, "roe": 4, "hlldm": 5, "hlld-m": 5}
 INTEGRATOR = {"euler": 0, "rk2": 1, "rk3": 2, "ssprk(m,2)": 3, "rk3.4": 4, "ssprk(4,3)": 4,
               "rk3.5": 5, "ssprk(5,3)": 5}
 PRECISION = {"fp64": 0, "mixed": 1}
@@ -108,7 +108,10 @@ def refine_cfg(p: "Params", minlev: int = 0, maxlev: int = 0) -> AmunRefineCfg:
     c.vortmax = num("vortmax", 0.1)
     c.jabsmin = num("jabsmin", 1.0e-3)
     c.jabsmax = num("jabsmax", 0.1)
-    c.varmask = sum(1 << i for i, n in enumerate(PVARS[eq]) if n in names)
+    pv = PVARS[eq]
+    if EOS.get(p.equation_of_state.lower(), 1) == 2:
+        pv = pv[:-1]      # the isothermal systems have no pressure (equations.F90:410-414)
+    c.varmask = sum(1 << i for i, n in enumerate(pv) if n in names)
     c.vort = int("vort" in names)
     c.jabs = int("jabs" in names and eq != 1)
     c.minlev, c.maxlev = int(minlev), int(maxlev)

@@ -203,7 +203,7 @@ __device__ __forceinline__ void face_flux(const RunConst& rc, const T* __restric
                                           const T* mlps, const T* mlpg, T* f) {
   constexpr int NV = P::nv;
   using RC = Recon<P, REC, ND, DIR, T>;
-  const EosConst<T> e{T(rc.gamma), T(rc.gammam1), T(rc.gammam1i)};
+  const EosConst<T> e{T(rc.gamma), T(rc.gammam1), T(rc.gammam1i), T(rc.csnd), T(rc.csnd2)};
   auto coords = [&](int cc, int& i, int& j, int& k) {
     if constexpr (DIR == 0) { i = cc; j = c1; k = c2; }
     else if constexpr (DIR == 1) { i = c1; j = cc; k = c2; }

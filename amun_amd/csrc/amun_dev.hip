@@ -85,27 +85,47 @@ struct amun_ctx {
 
 static int amr_fill(amun_ctx* c, int qcur, int slot);
 
+// calls f(P{}) with the physics of the configuration's equation system
+template <class F>
+static void by_system(const amun_cfg& cfg, F&& f) {
+  const bool iso = cfg.eos == AMUN_EOS_ISO;
+  if (cfg.eqsys == AMUN_EQSYS_HD) {
+    if (iso) f(amun::HDI{});
+    else f(amun::HD{});
+  } else if (cfg.eqsys == AMUN_EQSYS_MHD) {
+    if (iso) f(amun::MHDI{});
+    else f(amun::MHD{});
+  } else {
+    f(amun::SRMHD{});
+  }
+}
+
+// the pressure (energy) variable, -1 for the isothermal systems
+static int pres_index(const amun_ctx* c) {
+  return c->cfg.eos == AMUN_EOS_ISO ? -1 : c->nv - 1;
+}
+
+// variables of a configuration validate() accepts
+static int cfg_nv(const amun_cfg& c) {
+  const int nv = c.eqsys == AMUN_EQSYS_HD ? 5 : 9;
+  return c.eos == AMUN_EOS_ISO ? nv - 1 : nv;
+}
+
 // the ghost blocks of the non-periodic sides from the primitive state Q[qcur]
 static int bnd_fill(amun_ctx* c, int qcur) {
   if (!c->bnd.on) return 0;
   const long long n = (long long)c->bnd.n * c->rc.ncell;
   const dim3 grid((unsigned)((n + 255) / 256)), bl(256);
-  const int ibx = c->nv == 9 ? 4 : -1;
-  if (c->qsize == sizeof(double)) {
-    if (c->nv == 9)
-      hipLaunchKernelGGL((k_bnd_fill<9, double>), grid, bl, 0, c->stream, c->bnd.task, c->bnd.n,
-                         static_cast<double*>(c->Q[qcur]), c->rc, 1, ibx);
+  const int ibx = c->cfg.eqsys != AMUN_EQSYS_HD ? 4 : -1;
+  by_system(c->cfg, [&](auto tag) {
+    using P = decltype(tag);
+    if (c->qsize == sizeof(double))
+      hipLaunchKernelGGL((k_bnd_fill<P::nv, double>), grid, bl, 0, c->stream, c->bnd.task,
+                         c->bnd.n, static_cast<double*>(c->Q[qcur]), c->rc, 1, ibx);
     else
-      hipLaunchKernelGGL((k_bnd_fill<5, double>), grid, bl, 0, c->stream, c->bnd.task, c->bnd.n,
-                         static_cast<double*>(c->Q[qcur]), c->rc, 1, ibx);
-  } else {
-    if (c->nv == 9)
-      hipLaunchKernelGGL((k_bnd_fill<9, float>), grid, bl, 0, c->stream, c->bnd.task, c->bnd.n,
-                         static_cast<float*>(c->Q[qcur]), c->rc, 1, ibx);
-    else
-      hipLaunchKernelGGL((k_bnd_fill<5, float>), grid, bl, 0, c->stream, c->bnd.task, c->bnd.n,
-                         static_cast<float*>(c->Q[qcur]), c->rc, 1, ibx);
-  }
+      hipLaunchKernelGGL((k_bnd_fill<P::nv, float>), grid, bl, 0, c->stream, c->bnd.task,
+                         c->bnd.n, static_cast<float*>(c->Q[qcur]), c->rc, 1, ibx);
+  });
   HIP_TRY(hipGetLastError());
   return 0;
 }
@@ -115,8 +135,9 @@ namespace {
 bool pick(Dispatch& d, const amun_cfg& c) {
   // one kernel-table unit per reconstruction, equation system and precision
   const int prec = c.precision == AMUN_PREC_MIXED ? AMUN_PREC_MIXED : AMUN_PREC_FP64;
-#define AMUN_PICK_CASE(name, rec, eq, pr) \
-  if (c.reconstruction == (rec) && c.eqsys == (eq) && prec == (pr)) return name(d, c);
+  const int eq = c.eqsys + (c.eos == AMUN_EOS_ISO ? EQ_ISO : 0);
+#define AMUN_PICK_CASE(name, rec, eqk, pr) \
+  if (c.reconstruction == (rec) && eq == (eqk) && prec == (pr)) return name(d, c);
   AMUN_INST_LIST(AMUN_PICK_CASE)
 #undef AMUN_PICK_CASE
   return false;
@@ -127,7 +148,15 @@ int validate(const amun_cfg& c, int* nv) {
   if (c.eqsys == AMUN_EQSYS_HD) *nv = 5;
   else if (c.eqsys == AMUN_EQSYS_MHD || c.eqsys == AMUN_EQSYS_SRMHD) *nv = 9;
   else return fail(100, "unsupported equation_system");
-  if (c.eos != AMUN_EOS_ADI) return fail(110, "unsupported equation_of_state");
+  if (c.eos != AMUN_EOS_ADI && c.eos != AMUN_EOS_ISO)
+    return fail(110, "unsupported equation_of_state");
+  if (c.eos == AMUN_EOS_ISO) {
+    // equations.F90:399-830: isothermal HD and MHD, no energy equation
+    if (c.eqsys == AMUN_EQSYS_SRMHD)
+      return fail(110, "the relativistic system is adiabatic only");
+    if (!(c.csnd > 0.0)) return fail(500, "csnd must be positive");
+    *nv -= 1;
+  }
   switch (c.reconstruction) {
   case AMUN_REC_TVD: case AMUN_REC_WENO5Z: case AMUN_REC_WENO5YC: case AMUN_REC_WENO5NS:
   case AMUN_REC_MP5: case AMUN_REC_MP7: case AMUN_REC_GP: case AMUN_REC_WENO3:
@@ -150,8 +179,10 @@ int validate(const amun_cfg& c, int* nv) {
     if (c.primitive_solver < AMUN_C2P_1DW || c.primitive_solver > AMUN_C2P_2DWU)
       return fail(110, "unsupported primitive_solver");
   } else {
+    const bool iso = c.eos == AMUN_EOS_ISO;
     bool rs_ok = c.riemann_solver == AMUN_RS_HLL || c.riemann_solver == AMUN_RS_HLLC ||
                  (c.riemann_solver == AMUN_RS_HLLD && c.eqsys == AMUN_EQSYS_MHD) ||
+                 (c.riemann_solver == AMUN_RS_HLLDM && c.eqsys == AMUN_EQSYS_MHD && iso) ||
                  c.riemann_solver == AMUN_RS_ROE;
     if (!rs_ok) return fail(300, "riemann_solver does not match the equation system");
   }
@@ -288,6 +319,8 @@ RunConst make_rc(const amun_cfg& cfg) {
   rc.gamma = cfg.gamma;
   rc.gammam1 = cfg.gamma - 1.0;
   rc.gammam1i = 1.0 / rc.gammam1;
+  rc.csnd = cfg.csnd;
+  rc.csnd2 = cfg.csnd * cfg.csnd;                 // equations.F90:849
   // initialize_evolution (evolution.F90:166-200): SSPRK(m,2) with m clamped
   // to [2, 9], SSPRK(4,3) and SSPRK(5,3) step at a multiple of the CFL
   // number, and the psi decay follows the scaled number
@@ -355,17 +388,10 @@ static void mlp_launch_t(amun_ctx* c, int qcur) {
 
 static int mlp_launch(amun_ctx* c, int qcur) {
   const bool f64 = c->qsize == sizeof(double);
-  switch (c->cfg.eqsys) {
-  case AMUN_EQSYS_HD:
-    f64 ? mlp_launch_t<amun::HD, double>(c, qcur) : mlp_launch_t<amun::HD, float>(c, qcur);
-    break;
-  case AMUN_EQSYS_MHD:
-    f64 ? mlp_launch_t<amun::MHD, double>(c, qcur) : mlp_launch_t<amun::MHD, float>(c, qcur);
-    break;
-  default:
-    f64 ? mlp_launch_t<amun::SRMHD, double>(c, qcur) : mlp_launch_t<amun::SRMHD, float>(c, qcur);
-    break;
-  }
+  by_system(c->cfg, [&](auto tag) {
+    using P = decltype(tag);
+    f64 ? mlp_launch_t<P, double>(c, qcur) : mlp_launch_t<P, float>(c, qcur);
+  });
   HIP_TRY(hipGetLastError());
   return 0;
 }
@@ -390,6 +416,7 @@ void amun_dev_default_cfg(amun_cfg* c) {
   c->time_advance = AMUN_TA_RK2;
   c->precision = AMUN_PREC_FP64;
   c->gamma = 5.0 / 3.0;
+  c->csnd = 1.0;
   c->cfl = 0.5;
   c->eps = 2.220446049250313e-16;
   c->alpha = 2.0;
@@ -575,7 +602,7 @@ static int init_device(amun_ctx* c, const std::vector<int32_t>& nbr,
     ALLOC(c->mlps, (size_t)rc.nblk * nv * nd * rc.ncell * c->qsize);
     ALLOC(c->mlpg, (size_t)rc.nblk * nv * nd * gs * c->qsize);
   }
-  if (cfg->glm_source_terms > 0 && nv == 9) {
+  if (cfg->glm_source_terms > 0 && cfg->eqsys != AMUN_EQSYS_HD) {
     // the eight GLM source products of every cell (k_glm_src)
     ALLOC(c->src, (size_t)rc.nblk * 8 * rc.ncell * c->qsize);
   }
@@ -1064,7 +1091,7 @@ static int download_blocks_amr(amun_ctx* c, double* u, double* q) {
   if (e != hipSuccess) return fail(1000 + (int)e, "download_blocks");
   std::vector<double> h(c->nstate);
   HIP_TRY(hipMemcpy(h.data(), c->U0, c->nstate * sizeof(double), hipMemcpyDeviceToHost));
-  const bool mhd = nv == 9;
+  const bool mhd = c->cfg.eqsys != AMUN_EQSYS_HD, iso = c->cfg.eos == AMUN_EOS_ISO;
   const double g1 = c->cfg.gamma - 1.0;
   for (int b = 0; b < rc.nblk; b++)
     for (int k = 0; k < km; k++)
@@ -1084,7 +1111,10 @@ static int download_blocks_amr(amun_ctx* c, double* u, double* q) {
             uo[2] = qq[0] * qq[2];
             uo[3] = qq[0] * qq[3];
             const double ek = 0.5 * ((uo[1] * qq[1] + uo[2] * qq[2]) + uo[3] * qq[3]);
-            if (mhd) {
+            if (iso) {
+              // prim2cons_hd_iso / _mhd_iso (equations.F90:1360, 2220)
+              for (int v = 4; v < nv; v++) uo[v] = qq[v];
+            } else if (mhd) {
               for (int v = 4; v < 8; v++) uo[v] = qq[v];
               const double em = 0.5 * ((qq[4] * qq[4] + qq[5] * qq[5]) + qq[6] * qq[6]);
               uo[8] = (qq[8] / g1 + ek) + em;
@@ -1273,7 +1303,7 @@ static int amr_fill(amun_ctx* c, int qcur, int slot) {
                          A.rtask, A.nr, Q, c->rc, c->nv);
     if (A.np)
       hipLaunchKernelGGL(k_amr_prolong<double>, dim3((A.np + 255) / 256), bl, 0, c->stream,
-                         A.ptask, A.np, Q, c->nbr, c->rc, c->nv, c->nv - 1, c->st);
+                         A.ptask, A.np, Q, c->nbr, c->rc, c->nv, pres_index(c), c->st);
   } else {
     float* Q = static_cast<float*>(c->Q[qcur]);
     if (A.nr)
@@ -1281,7 +1311,7 @@ static int amr_fill(amun_ctx* c, int qcur, int slot) {
                          A.rtask, A.nr, Q, c->rc, c->nv);
     if (A.np)
       hipLaunchKernelGGL(k_amr_prolong<float>, dim3((A.np + 255) / 256), bl, 0, c->stream,
-                         A.ptask, A.np, Q, c->nbr, c->rc, c->nv, c->nv - 1, c->st);
+                         A.ptask, A.np, Q, c->nbr, c->rc, c->nv, pres_index(c), c->st);
   }
   // the slopes of the multi-dimensional limiting read the filled virtual
   // blocks and feed the fine-face fluxes
@@ -1305,26 +1335,19 @@ static int glm_sources(amun_ctx* c, int qcur) {
   const long long n = (long long)rc.nblk * rc.ncell;
   const dim3 grid((unsigned)((n + 255) / 256)), bl(256);
   const int* blev = c->amr.on ? c->amr.blev : nullptr;
-  const bool sr = c->cfg.eqsys == AMUN_EQSYS_SRMHD;
-  if (c->qsize == sizeof(double)) {
-    const double* Q = static_cast<const double*>(c->Q[qcur]);
-    double* S = static_cast<double*>(c->src);
-    if (sr)
-      hipLaunchKernelGGL((k_glm_src<amun::SRMHD, double>), grid, bl, 0, c->stream, Q, c->nbr, rc,
-                         c->rs, c->brun, blev, S);
-    else
-      hipLaunchKernelGGL((k_glm_src<amun::MHD, double>), grid, bl, 0, c->stream, Q, c->nbr, rc,
-                         c->rs, c->brun, blev, S);
-  } else {
-    const float* Q = static_cast<const float*>(c->Q[qcur]);
-    float* S = static_cast<float*>(c->src);
-    if (sr)
-      hipLaunchKernelGGL((k_glm_src<amun::SRMHD, float>), grid, bl, 0, c->stream, Q, c->nbr, rc,
-                         c->rs, c->brun, blev, S);
-    else
-      hipLaunchKernelGGL((k_glm_src<amun::MHD, float>), grid, bl, 0, c->stream, Q, c->nbr, rc,
-                         c->rs, c->brun, blev, S);
-  }
+  by_system(c->cfg, [&](auto tag) {
+    using P = decltype(tag);
+    if constexpr (P::mhd) {
+      if (c->qsize == sizeof(double))
+        hipLaunchKernelGGL((k_glm_src<P, double>), grid, bl, 0, c->stream,
+                           static_cast<const double*>(c->Q[qcur]), c->nbr, rc, c->rs, c->brun,
+                           blev, static_cast<double*>(c->src));
+      else
+        hipLaunchKernelGGL((k_glm_src<P, float>), grid, bl, 0, c->stream,
+                           static_cast<const float*>(c->Q[qcur]), c->nbr, rc, c->rs, c->brun,
+                           blev, static_cast<float*>(c->src));
+    }
+  });
   HIP_TRY(hipGetLastError());
   return 0;
 }
@@ -1744,9 +1767,9 @@ static int statistics_range(amun_ctx* c, int r0, int nr, double out[21]) {
       wsum += (c->amr.on ? std::ldexp(1.0, -c->rc.nd * blev[b]) : 1.0) * c->rc.ncell;
   }
   // without a magnetic field the reference reports zeros for |B|, psi, M_A
-  const bool mhd = c->nv == 9;
+  const bool mhd = c->cfg.eqsys != AMUN_EQSYS_HD, iso = c->cfg.eos == AMUN_EOS_ISO;
   for (int m = 0; m < 7; m++) {
-    const bool zero = !mhd && (m == 3 || m == 4 || m == 6);
+    const bool zero = (!mhd && (m == 3 || m == 4 || m == 6)) || (iso && m == 1);
     out[3 * m] = zero ? 0.0 : sum[m] / wsum;
     out[3 * m + 1] = zero ? 0.0 : lo[m];
     out[3 * m + 2] = zero ? 0.0 : hi[m];
@@ -1982,7 +2005,7 @@ int amun_dev_amr_remesh(amun_ctx* c, int32_t nblk, const int32_t* meta) {
   if (e == hipSuccess) {
     const long long cells = (long long)nblk * n->rc.ncell;
     const dim3 grid((unsigned)((cells + 255) / 256)), bl(256);
-    const int ien = c->nv - 1;
+    const int ien = pres_index(c);
     if (c->qsize == sizeof(double))
       hipLaunchKernelGGL(k_remesh<double>, grid, bl, 0, n->stream, dtask, dkids, nblk, c->U0,
                          c->C, static_cast<const double*>(c->Q[c->qcur]), c->nbr, c->rc, n->U0,
@@ -2517,7 +2540,7 @@ int amun_dev_unit_riemann(const amun_cfg* cfg, int32_t n, double cmax, const dou
   int r = unit_setup(cfg, d, rc);
   if (r) return r;
   if (n < 1 || !ql || !qr || !f) return fail(500, "bad argument");
-  const int nv = (cfg->eqsys == AMUN_EQSYS_HD) ? 5 : 9;
+  const int nv = cfg_nv(*cfg);
   const size_t m = (size_t)n * nv;
   DevBufs B;
   double *dl, *dr, *df;
@@ -2541,7 +2564,7 @@ int amun_dev_unit_cons2prim(const amun_cfg* cfg, int32_t n, const double* u, dou
   int r = unit_setup(cfg, d, rc);
   if (r) return r;
   if (n < 1 || !u || !q) return fail(500, "bad argument");
-  const int nv = (cfg->eqsys == AMUN_EQSYS_HD) ? 5 : 9;
+  const int nv = cfg_nv(*cfg);
   const size_t m = (size_t)n * nv;
   DevBufs B;
   double *du, *dq;
@@ -2602,7 +2625,7 @@ int amun_dev_unit_eqs(const amun_cfg* cfg, int32_t n, double cmax, const double*
   int r = unit_setup(cfg, d, rc);
   if (r) return r;
   if (n < 1 || !q || !u || !q2 || !f || !cm || !cp) return fail(500, "bad argument");
-  const int nv = (cfg->eqsys == AMUN_EQSYS_HD) ? 5 : 9;
+  const int nv = cfg_nv(*cfg);
   const size_t m = (size_t)n * nv;
   DevBufs B;
   double *dq, *du, *dq2, *df, *dm, *dp;

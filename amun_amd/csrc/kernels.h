@@ -94,6 +94,7 @@ struct RunConst {
   int limiter;
   int prol_limiter;                 // prolongation of multi-level ghost fills
   double gamma, gammam1, gammam1i;
+  double csnd, csnd2;               // isothermal sound speed and its square
   double cfl, eps, decay;
   double cgp[5];                    // Gaussian-process weights (prepare_gp)
   double kappa, kbeta;              // MP limiter of the GP reconstruction
@@ -199,12 +200,24 @@ enum {
   REC_WENO5NS = AMUN_REC_WENO5NS, REC_MP5 = AMUN_REC_MP5, REC_MP7 = AMUN_REC_MP7,
   REC_GP = AMUN_REC_GP
 };
-enum { RS_HLL = AMUN_RS_HLL, RS_HLLC = AMUN_RS_HLLC, RS_HLLD = AMUN_RS_HLLD, RS_ROE = AMUN_RS_ROE };
+enum { RS_HLL = AMUN_RS_HLL, RS_HLLC = AMUN_RS_HLLC, RS_HLLD = AMUN_RS_HLLD, RS_ROE = AMUN_RS_ROE,
+       RS_HLLDM = AMUN_RS_HLLDM };
+// kernel-table keys of the isothermal systems: the equation system + 16
+enum { EQ_ISO = 16, EQ_HD_ISO = AMUN_EQSYS_HD + EQ_ISO, EQ_MHD_ISO = AMUN_EQSYS_MHD + EQ_ISO };
 
 template <class P, int RS, typename T>
 __device__ __forceinline__ void riemann(const EosConst<T>& e, const SrConst& sc, T cmax,
                                         T cmax2, T* ql, T* qr, T* f) {
-  if constexpr (P::sr && RS == RS_HLLC) riemann_sr_hllc<P, T>(sc, cmax, cmax2, ql, qr, f);
+  if constexpr (P::iso && !P::mhd && RS == RS_ROE)
+    riemann_roe_hd_iso<P, T>(e, cmax, cmax2, ql, qr, f);
+  else if constexpr (P::iso && P::mhd && RS == RS_HLLD)
+    riemann_hlld_iso<P, false, T>(e, cmax, cmax2, ql, qr, f);
+  else if constexpr (P::iso && P::mhd && RS == RS_HLLDM)
+    riemann_hlld_iso<P, true, T>(e, cmax, cmax2, ql, qr, f);
+  else if constexpr (P::iso && P::mhd && RS == RS_ROE)
+    riemann_roe_mhd_iso<P, T>(e, cmax, cmax2, ql, qr, f);
+  else if constexpr (P::iso) riemann_hll<P, T>(e, cmax, cmax2, ql, qr, f);
+  else if constexpr (P::sr && RS == RS_HLLC) riemann_sr_hllc<P, T>(sc, cmax, cmax2, ql, qr, f);
   else if constexpr (P::sr) riemann_sr_hll<P, T>(sc, cmax, cmax2, ql, qr, f);
   else if constexpr (RS == RS_HLLD) riemann_hlld<P, T>(e, cmax, cmax2, ql, qr, f);
   else if constexpr (RS == RS_HLLC && P::mhd) riemann_hllc_mhd<P, T>(e, cmax, cmax2, ql, qr, f);
@@ -1049,8 +1062,8 @@ __global__ __launch_bounds__(256, sweep_min_waves<T>()) void k_sweep(SweepArgs a
   const T cmax = T(cmaxd), cmax2 = T(cmaxd * cmaxd);
   const double dt = dt_of(rc, R, a.slot, a.dtnext);
   const double ds = a.fac * dt;
-  const EosConst<T> e{T(rc.gamma), T(rc.gammam1), T(rc.gammam1i)};
-  const EosConst<double> ed{rc.gamma, rc.gammam1, rc.gammam1i};
+  const EosConst<T> e{T(rc.gamma), T(rc.gammam1), T(rc.gammam1i), T(rc.csnd), T(rc.csnd2)};
+  const EosConst<double> ed{rc.gamma, rc.gammam1, rc.gammam1i, rc.csnd, rc.csnd2};
   const T* __restrict__ Qp = static_cast<const T*>(a.Q);
   T* __restrict__ Dp = static_cast<T*>(a.D);
   T* __restrict__ Qo = static_cast<T*>(a.Qout);
@@ -1446,7 +1459,7 @@ __global__ __launch_bounds__(256) void k_prim(const double* __restrict__ U,
   constexpr int NV = P::nv;
   const long long n = (long long)nb * rc.ncell;
   const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const EosConst<double> e{rc.gamma, rc.gammam1, rc.gammam1i};
+  const EosConst<double> e{rc.gamma, rc.gammam1, rc.gammam1i, rc.csnd, rc.csnd2};
   double smax = 0.0;
   int run = 0;
   if (gid < n) {
@@ -1533,13 +1546,14 @@ __global__ __launch_bounds__(256) void k_integrals(const double* __restrict__ U,
     acc[1] += u[P::imx];
     acc[2] += u[P::imy];
     acc[3] += u[P::imz];
-    acc[4] += u[P::ien];
+    // the isothermal systems carry no energy (integrals.F90:355, 457)
+    if constexpr (!P::iso) acc[4] += u[P::ien];
     double ek = (u[P::imx] * u[P::imx] + u[P::imy] * u[P::imy] + u[P::imz] * u[P::imz]) / u[P::idn];
     acc[5] += ek;
     double em = 0.0;
     if constexpr (P::mhd) em = u[P::ibx] * u[P::ibx] + u[P::iby] * u[P::iby] + u[P::ibz] * u[P::ibz];
     acc[6] += em;
-    acc[7] += u[P::ien] - 0.5 * ek - 0.5 * em;
+    if constexpr (!P::iso) acc[7] += u[P::ien] - 0.5 * ek - 0.5 * em;
   }
 #pragma unroll
   for (int m = 0; m < 8; m++) sh[m][threadIdx.x] = acc[m];
@@ -1584,8 +1598,16 @@ __global__ __launch_bounds__(256) void k_stats(const void* Qv, RunConst rc, doub
       psi = q[P::ibp];
     }
     const double sv = sqrt(q[P::idn]) * vel;
-    const double t[7] = {q[P::idn], q[P::ipr], vel, mag, psi, sv / sqrt(rc.gamma * q[P::ipr]),
-                         sv / tmax(eps, mag)};
+    // isothermal: no pressure, the sonic Mach number |v| / csnd
+    // (integrals.F90:383-423)
+    double pr = 0.0, ms;
+    if constexpr (P::iso) {
+      ms = vel / rc.csnd;
+    } else {
+      pr = q[P::ipr];
+      ms = sv / sqrt(rc.gamma * q[P::ipr]);
+    }
+    const double t[7] = {q[P::idn], pr, vel, mag, psi, ms, sv / tmax(eps, mag)};
 #pragma unroll
     for (int m = 0; m < 7; m++) {
       s[m] += w * t[m];
@@ -1650,7 +1672,7 @@ __global__ void k_unit_riemann(int n, double cmaxd, const double* __restrict__ q
   constexpr int NV = P::nv;
   const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   if (i >= n) return;
-  const EosConst<T> e{T(rc.gamma), T(rc.gammam1), T(rc.gammam1i)};
+  const EosConst<T> e{T(rc.gamma), T(rc.gammam1), T(rc.gammam1i), T(rc.csnd), T(rc.csnd2)};
   T a[NV], b[NV], g[NV];
 #pragma unroll
   for (int v = 0; v < NV; v++) {
@@ -1711,7 +1733,7 @@ __global__ void k_unit_c2p(int n, const double* __restrict__ u, double* __restri
     const int cs = cons2prim_sr<P>(rc.sc, ud, qd);
     if (cs) c2p_count(st, cs);
   } else {
-    const EosConst<double> ed{rc.gamma, rc.gammam1, rc.gammam1i};
+    const EosConst<double> ed{rc.gamma, rc.gammam1, rc.gammam1i, rc.csnd, rc.csnd2};
     cons2prim<P, double>(ed, ud, qd);
   }
 #pragma unroll
@@ -1729,8 +1751,8 @@ __global__ void k_unit_eqs(int n, double cmaxd, const double* __restrict__ q,
   constexpr int NV = P::nv;
   const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   if (i >= n) return;
-  const EosConst<T> e{T(rc.gamma), T(rc.gammam1), T(rc.gammam1i)};
-  const EosConst<double> ed{rc.gamma, rc.gammam1, rc.gammam1i};
+  const EosConst<T> e{T(rc.gamma), T(rc.gammam1), T(rc.gammam1i), T(rc.csnd), T(rc.csnd2)};
+  const EosConst<double> ed{rc.gamma, rc.gammam1, rc.gammam1i, rc.csnd, rc.csnd2};
   T qq[NV], uu[NV], ff[NV], m, pp;
 #pragma unroll
   for (int v = 0; v < NV; v++) qq[v] = T(q[(size_t)i * NV + v]);
@@ -1834,7 +1856,7 @@ void fill(Dispatch& d) {
   d.fineflux[0] = k_amr_fineflux<P, REC, RS, ND, 0, T>;
   d.fineflux[1] = k_amr_fineflux<P, REC, RS, ND, 1, T>;
   if constexpr (ND == 3) d.fineflux[2] = k_amr_fineflux<P, REC, RS, ND, 2, T>;
-  if constexpr (REC == REC_TVD && !P::sr) {
+  if constexpr (REC == REC_TVD && !P::sr && !P::iso) {
     // the face Jacobians in the arithmetic of the forward fluxes (T)
     d.has_adj = true;
     d.adj_prim = k_adj_prim<P>;
@@ -1877,6 +1899,17 @@ bool pick_eq(Dispatch& d, const amun_cfg& c) {
     // (schemes.F90:479-498)
     if (c.riemann_solver == AMUN_RS_HLLC) return pick_nd<SRMHD, RS_HLLC, REC, T>(d, c.ndims);
     return pick_nd<SRMHD, RS_HLL, REC, T>(d, c.ndims);
+  } else if constexpr (EQ == EQ_HD_ISO) {
+    // "roe" selects riemann_hd_iso_roe, every other name HLL
+    // (schemes.F90:163-176)
+    if (c.riemann_solver == AMUN_RS_ROE) return pick_nd<HDI, RS_ROE, REC, T>(d, c.ndims);
+    return pick_nd<HDI, RS_HLL, REC, T>(d, c.ndims);
+  } else if constexpr (EQ == EQ_MHD_ISO) {
+    // "hlld", "hlldm", "roe", every other name HLL (schemes.F90:201-225)
+    if (c.riemann_solver == AMUN_RS_HLLD) return pick_nd<MHDI, RS_HLLD, REC, T>(d, c.ndims);
+    if (c.riemann_solver == AMUN_RS_HLLDM) return pick_nd<MHDI, RS_HLLDM, REC, T>(d, c.ndims);
+    if (c.riemann_solver == AMUN_RS_ROE) return pick_nd<MHDI, RS_ROE, REC, T>(d, c.ndims);
+    return pick_nd<MHDI, RS_HLL, REC, T>(d, c.ndims);
   }
   return false;
 }

@@ -27,6 +27,7 @@ struct HD {
   static constexpr int ibx = -1, iby = -1, ibz = -1, ibp = -1;
   static constexpr bool mhd = false;
   static constexpr bool sr = false;
+  static constexpr bool iso = false;
 };
 
 struct MHD {
@@ -37,12 +38,38 @@ struct MHD {
   static constexpr int ipr = 8, ien = 8;
   static constexpr bool mhd = true;
   static constexpr bool sr = false;
+  static constexpr bool iso = false;
+};
+
+// the isothermal systems (equations.F90:399-830): no energy equation, the
+// pressure csnd^2 rho
+struct HDI {
+  static constexpr int nv = 4;
+  static constexpr int idn = 0, ivx = 1, ivy = 2, ivz = 3;
+  static constexpr int imx = 1, imy = 2, imz = 3;
+  static constexpr int ipr = -1, ien = -1;
+  static constexpr int ibx = -1, iby = -1, ibz = -1, ibp = -1;
+  static constexpr bool mhd = false;
+  static constexpr bool sr = false;
+  static constexpr bool iso = true;
+};
+
+struct MHDI {
+  static constexpr int nv = 8;
+  static constexpr int idn = 0, ivx = 1, ivy = 2, ivz = 3;
+  static constexpr int imx = 1, imy = 2, imz = 3;
+  static constexpr int ibx = 4, iby = 5, ibz = 6, ibp = 7;
+  static constexpr int ipr = -1, ien = -1;
+  static constexpr bool mhd = true;
+  static constexpr bool sr = false;
+  static constexpr bool iso = true;
 };
 
 // constants that every kernel needs, passed by value
 template <typename T>
 struct EosConst {
   T gamma, gammam1, gammam1i;
+  T csnd, csnd2;                  // isothermal sound speed and its square
 };
 
 template <typename T> __device__ __forceinline__ T tsqrt(T x);
@@ -432,7 +459,15 @@ __device__ __forceinline__ void prim2cons(const EosConst<T>& e, const T* q, T* u
   u[P::imx] = q[P::idn] * q[P::ivx];
   u[P::imy] = q[P::idn] * q[P::ivy];
   u[P::imz] = q[P::idn] * q[P::ivz];
-  if constexpr (!P::mhd) {
+  if constexpr (P::iso) {
+    // prim2cons_hd_iso / prim2cons_mhd_iso (equations.F90:1360, 2220)
+    if constexpr (P::mhd) {
+      u[P::ibx] = q[P::ibx];
+      u[P::iby] = q[P::iby];
+      u[P::ibz] = q[P::ibz];
+      u[P::ibp] = q[P::ibp];
+    }
+  } else if constexpr (!P::mhd) {
     T ek = T(0.5) * (u[P::imx] * q[P::ivx] + u[P::imy] * q[P::ivy] + u[P::imz] * q[P::ivz]);
     T ei = e.gammam1i * q[P::ipr];
     u[P::ien] = ei + ek;
@@ -455,7 +490,15 @@ __device__ __forceinline__ void cons2prim(const EosConst<T>& e, const T* u, T* q
   q[P::ivx] = u[P::imx] * rdn;
   q[P::ivy] = u[P::imy] * rdn;
   q[P::ivz] = u[P::imz] * rdn;
-  if constexpr (!P::mhd) {
+  if constexpr (P::iso) {
+    // cons2prim_hd_iso / cons2prim_mhd_iso (equations.F90:1421, 2285)
+    if constexpr (P::mhd) {
+      q[P::ibx] = u[P::ibx];
+      q[P::iby] = u[P::iby];
+      q[P::ibz] = u[P::ibz];
+      q[P::ibp] = u[P::ibp];
+    }
+  } else if constexpr (!P::mhd) {
     T ek = T(0.5) * (u[P::imx] * q[P::ivx] + u[P::imy] * q[P::ivy] + u[P::imz] * q[P::ivz]);
     T ei = u[P::ien] - ek;
     q[P::ipr] = e.gammam1 * ei;
@@ -479,7 +522,13 @@ __device__ __forceinline__ T maxspeed(const EosConst<T>& e, const T* q) {
   T vv = (q[P::ivx] * q[P::ivx] + q[P::ivy] * q[P::ivy]) + q[P::ivz] * q[P::ivz];
   T v = tsqrt(vv);
   T c;
-  if constexpr (!P::mhd) {
+  if constexpr (P::iso && !P::mhd) {
+    c = e.csnd;                     // maxspeed_hd_iso (equations.F90:1546-1570)
+  } else if constexpr (P::iso) {
+    // maxspeed_mhd_iso (equations.F90:2438-2510)
+    T bb = (q[P::ibx] * q[P::ibx] + q[P::iby] * q[P::iby]) + q[P::ibz] * q[P::ibz];
+    c = tsqrt(e.csnd2 + bb / q[P::idn]);
+  } else if constexpr (!P::mhd) {
     c = tsqrt(e.gamma * q[P::ipr] / q[P::idn]);
   } else {
     T bb = (q[P::ibx] * q[P::ibx] + q[P::iby] * q[P::iby]) + q[P::ibz] * q[P::ibz];
@@ -491,7 +540,41 @@ __device__ __forceinline__ T maxspeed(const EosConst<T>& e, const T* q) {
 template <class P, typename T>
 __device__ __forceinline__ void fluxspeed(const EosConst<T>& e, T cmax2, const T* q,
                                           const T* u, T* f, T& cm, T& cp) {
-  if constexpr (!P::mhd) {
+  if constexpr (P::iso && !P::mhd) {
+    // fluxspeed_hd_iso (equations.F90:1484-1542)
+    f[P::idn] = u[P::imx];
+    f[P::imx] = q[P::ivx] * u[P::imx];
+    f[P::imy] = q[P::ivx] * u[P::imy];
+    f[P::imz] = q[P::ivx] * u[P::imz];
+    f[P::imx] = f[P::imx] + e.csnd2 * q[P::idn];
+    cm = q[P::ivx] - e.csnd;
+    cp = q[P::ivx] + e.csnd;
+  } else if constexpr (P::iso) {
+    // fluxspeed_mhd_iso (equations.F90:2352-2435)
+    T bx2 = q[P::ibx] * q[P::ibx];
+    T by2 = q[P::iby] * q[P::iby];
+    T bz2 = q[P::ibz] * q[P::ibz];
+    T bb = (bx2 + by2) + bz2;
+    T pt = e.csnd2 * q[P::idn] + T(0.5) * bb;
+    f[P::idn] = u[P::imx];
+    f[P::imx] = q[P::ivx] * u[P::imx] - bx2;
+    f[P::imy] = q[P::ivx] * u[P::imy] - q[P::ibx] * q[P::iby];
+    f[P::imz] = q[P::ivx] * u[P::imz] - q[P::ibx] * q[P::ibz];
+    f[P::imx] = f[P::imx] + pt;
+    f[P::ibx] = q[P::ibp];
+    f[P::iby] = q[P::ivx] * q[P::iby] - q[P::ibx] * q[P::ivy];
+    f[P::ibz] = q[P::ivx] * q[P::ibz] - q[P::ibx] * q[P::ivz];
+    f[P::ibp] = cmax2 * q[P::ibx];
+    T fa = e.csnd2 * q[P::idn];
+    T fb = fa + bb;
+    T fc = fb * fb - T(4) * fa * bx2;
+    T cf;
+    const T rdn = rcp(q[P::idn]);
+    if (fc > T(0)) cf = tsqrt(T(0.5) * (fb + tsqrt(fc)) * rdn);
+    else           cf = tsqrt(T(0.5) * fb * rdn);
+    cm = q[P::ivx] - cf;
+    cp = q[P::ivx] + cf;
+  } else if constexpr (!P::mhd) {
     f[P::idn] = u[P::imx];
     f[P::imx] = q[P::ivx] * u[P::imx];
     f[P::imy] = q[P::ivx] * u[P::imy];
@@ -698,6 +781,382 @@ __device__ __forceinline__ void riemann_roe_hd(const EosConst<T>& e, T cmax, T c
     const T xx = -(T(0.5) * al[k] * tabs(c[k]));
 #pragma unroll
     for (int p = 0; p < nv; p++) f[p] = f[p] + xx * r[k][p];
+  }
+}
+
+// riemann_hd_iso_roe (schemes.F90:939-1057) with esystem_roe_hd_iso
+// (equations.F90:1655-1738): the Roe-averaged velocity, the waves v -+ csnd
+// and v, summed from the left wave for a non-negative velocity
+template <class P, typename T>
+__device__ __forceinline__ void riemann_roe_hd_iso(const EosConst<T>& e, T cmax, T cmax2,
+                                                   T* ql, T* qr, T* f) {
+  static_assert(P::iso && !P::mhd, "the isothermal hydrodynamic Roe solver");
+  constexpr int nv = P::nv;
+  T ul[nv], ur[nv], fl[nv], fr[nv];
+  T cm, cp;
+  prim2cons<P>(e, ql, ul);
+  prim2cons<P>(e, qr, ur);
+  fluxspeed<P>(e, cmax2, ql, ul, fl, cm, cp);
+  fluxspeed<P>(e, cmax2, qr, ur, fr, cm, cp);
+  const T sdl = tsqrt(ql[P::idn]), sdr = tsqrt(qr[P::idn]), sds = sdl + sdr;
+  const T rsds = rcp(sds);
+  const T vx = (sdl * ql[P::ivx] + sdr * qr[P::ivx]) * rsds;
+  const T vy = (sdl * ql[P::ivy] + sdr * qr[P::ivy]) * rsds;
+  const T vz = (sdl * ql[P::ivz] + sdr * qr[P::ivz]) * rsds;
+  const T ch = T(0.5) * rcp(e.csnd);
+  const T c[4] = {vx - e.csnd, vx, vx, vx + e.csnd};
+  if (c[0] >= T(0)) {
+#pragma unroll
+    for (int p = 0; p < nv; p++) f[p] = fl[p];
+    return;
+  }
+  if (c[3] <= T(0)) {
+#pragma unroll
+    for (int p = 0; p < nv; p++) f[p] = fr[p];
+    return;
+  }
+  T du[nv];
+#pragma unroll
+  for (int p = 0; p < nv; p++) du[p] = ur[p] - ul[p];
+  // wave strengths al_k = sum_p l(p, k) du(p)
+  T al[4];
+  al[0] = (ch * c[3]) * du[P::idn] + (-ch) * du[P::imx];
+  al[1] = (-vy) * du[P::idn] + du[P::imy];
+  al[2] = (-vz) * du[P::idn] + du[P::imz];
+  al[3] = (-(ch * c[0])) * du[P::idn] + ch * du[P::imx];
+  const T r[4][4] = {{T(1), c[0], vy, vz},
+                     {T(0), T(0), T(1), T(0)},
+                     {T(0), T(0), T(0), T(1)},
+                     {T(1), c[3], vy, vz}};
+#pragma unroll
+  for (int p = 0; p < nv; p++) f[p] = T(0.5) * (fl[p] + fr[p]);
+  const bool fwd = vx >= T(0);
+#pragma unroll
+  for (int m = 0; m < 4; m++) {
+    const int k = fwd ? m : 3 - m;
+    const T xx = -(T(0.5) * al[k] * tabs(c[k]));
+#pragma unroll
+    for (int p = 0; p < nv; p++) f[p] = f[p] + xx * r[k][p];
+  }
+}
+
+// riemann_mhd_iso_hlld (schemes.F90:2007-2364, M = false) and
+// riemann_mhd_iso_hlldm (2388-2777, M = true): the GLM pre-step, the HLL
+// bounds, the contact speed from the HLL state's momentum and the branch
+// tree over Bx^2 (against 1e-8 of the outer waves' scale, > 0, = 0) and the
+// degeneracies.  HLLD: star densities wl(idn) / (sl - sm) and wr(idn) /
+// (sr - sm), Alfven speeds sm -+ sqrt(Bx^2 / rho*); HLLD-M: the HLL
+// state's density on both sides
+template <class P, bool M, typename T>
+__device__ __forceinline__ void riemann_hlld_iso(const EosConst<T>& e, T cmax, T cmax2,
+                                                 T* ql, T* qr, T* f) {
+  static_assert(P::iso && P::mhd, "the isothermal MHD HLLD solvers");
+  constexpr int nv = P::nv;
+  T ul[nv], ur[nv], fl[nv], fr[nv], wl[nv], wr[nv], ui[nv];
+  T clm, clp, crm, crp;
+  glm_upwind<P>(ql, qr, cmax);
+  prim2cons<P>(e, ql, ul);
+  prim2cons<P>(e, qr, ur);
+  fluxspeed<P>(e, cmax2, ql, ul, fl, clm, clp);
+  fluxspeed<P>(e, cmax2, qr, ur, fr, crm, crp);
+  const T sl = tmin(clm, crm), sr = tmax(clp, crp);
+  if (sl >= T(0)) {
+#pragma unroll
+    for (int p = 0; p < nv; p++) f[p] = fl[p];
+    return;
+  }
+  if (sr <= T(0)) {
+#pragma unroll
+    for (int p = 0; p < nv; p++) f[p] = fr[p];
+    return;
+  }
+  const T srml = sr - sl;
+#pragma unroll
+  for (int p = 0; p < nv; p++) {
+    wl[p] = sl * ul[p] - fl[p];
+    wr[p] = sr * ur[p] - fr[p];
+  }
+  const T dn = wr[P::idn] - wl[P::idn];
+  const T sm = dv(wr[P::imx] - wl[P::imx], dn);
+  const T bx = ql[P::ibx];
+  const T b2 = ql[P::ibx] * qr[P::ibx];
+  const T slmm = sl - sm, srmm = sr - sm;
+  // star densities and the coefficients of wl/wr(iby, ibz) in the star field
+  T dnl, dnr, cl, cr;
+  bool alfven;
+  if constexpr (M) {
+    dnl = dnr = dv(dn, srml);
+    cl = slmm * dnl;
+    cr = srmm * dnr;
+    alfven = dv(b2, dnl) > T(1.0e-08) * tmax(sl * sl, sr * sr);
+  } else {
+    dnl = dv(wl[P::idn], slmm);
+    dnr = dv(wr[P::idn], srmm);
+    cl = wl[P::idn];
+    cr = wr[P::idn];
+    alfven = b2 > T(1.0e-08) * tmax(dnl * (sl * sl), dnr * (sr * sr));
+  }
+  auto hll = [&]() {
+    const T r = rcp(srml);
+#pragma unroll
+    for (int p = 0; p < nv; p++) f[p] = (sl * wr[p] - sr * wl[p]) * r;
+  };
+  if (alfven) {
+    T sml, smr, dvl, dvr;
+    if constexpr (M) {
+      const T ca = tsqrt(dv(b2, dnl));
+      sml = sm - ca;
+      smr = sm + ca;
+      dvl = slmm * slmm * dnl - b2;
+      dvr = srmm * srmm * dnr - b2;
+    } else {
+      sml = sm - tsqrt(dv(b2, dnl));
+      smr = sm + tsqrt(dv(b2, dnr));
+      dvl = slmm * wl[P::idn] - b2;
+      dvr = srmm * wr[P::idn] - b2;
+    }
+    auto star_l = [&]() {
+      const T r = rcp(dvl);
+      ui[P::idn] = dnl;
+      ui[P::imx] = dnl * sm;
+      ui[P::imy] = dnl * (slmm * wl[P::imy] - bx * wl[P::iby]) * r;
+      ui[P::imz] = dnl * (slmm * wl[P::imz] - bx * wl[P::ibz]) * r;
+      ui[P::ibx] = bx;
+      ui[P::iby] = (cl * wl[P::iby] - bx * wl[P::imy]) * r;
+      ui[P::ibz] = (cl * wl[P::ibz] - bx * wl[P::imz]) * r;
+      ui[P::ibp] = ul[P::ibp];
+    };
+    auto star_r = [&]() {
+      const T r = rcp(dvr);
+      ui[P::idn] = dnr;
+      ui[P::imx] = dnr * sm;
+      ui[P::imy] = dnr * (srmm * wr[P::imy] - bx * wr[P::iby]) * r;
+      ui[P::imz] = dnr * (srmm * wr[P::imz] - bx * wr[P::ibz]) * r;
+      ui[P::ibx] = bx;
+      ui[P::iby] = (cr * wr[P::iby] - bx * wr[P::imy]) * r;
+      ui[P::ibz] = (cr * wr[P::ibz] - bx * wr[P::imz]) * r;
+      ui[P::ibp] = ur[P::ibp];
+    };
+    if (tmin(dvl, dvr) > T(0)) {
+      if (sml >= T(0)) {
+        star_l();
+#pragma unroll
+        for (int p = 0; p < nv; p++) f[p] = sl * ui[p] - wl[p];
+      } else if (smr <= T(0)) {
+        star_r();
+#pragma unroll
+        for (int p = 0; p < nv; p++) f[p] = sr * ui[p] - wr[p];
+      } else {
+        T wcl[nv];
+        star_l();
+#pragma unroll
+        for (int p = 0; p < nv; p++) wcl[p] = (sml - sl) * ui[p] + wl[p];
+        star_r();
+        const T r = rcp(smr - sml);
+#pragma unroll
+        for (int p = 0; p < nv; p++) {
+          const T wcr = (smr - sr) * ui[p] + wr[p];
+          f[p] = (sml * wcr - smr * wcl[p]) * r;
+        }
+      }
+    } else if (dvl > T(0)) {
+      star_l();
+      if (sml >= T(0)) {
+#pragma unroll
+        for (int p = 0; p < nv; p++) f[p] = sl * ui[p] - wl[p];
+      } else {
+        const T r = rcp(sr - sml);
+#pragma unroll
+        for (int p = 0; p < nv; p++) {
+          const T wcl = (sml - sl) * ui[p] + wl[p];
+          f[p] = (sml * wr[p] - sr * wcl) * r;
+        }
+      }
+    } else if (dvr > T(0)) {
+      star_r();
+      if (smr <= T(0)) {
+#pragma unroll
+        for (int p = 0; p < nv; p++) f[p] = sr * ui[p] - wr[p];
+      } else {
+        const T r = rcp(smr - sl);
+#pragma unroll
+        for (int p = 0; p < nv; p++) {
+          const T wcr = (smr - sr) * ui[p] + wr[p];
+          f[p] = (sl * wcr - smr * wl[p]) * r;
+        }
+      }
+    } else {
+      hll();
+    }
+  } else if (b2 > T(0)) {
+    hll();
+  } else if (sm > T(0)) {
+    const T r = rcp(slmm);
+    ui[P::idn] = dnl;
+    ui[P::imx] = dnl * sm;
+    ui[P::imy] = wl[P::imy] * r;
+    ui[P::imz] = wl[P::imz] * r;
+    ui[P::ibx] = T(0);
+    ui[P::iby] = wl[P::iby] * r;
+    ui[P::ibz] = wl[P::ibz] * r;
+    ui[P::ibp] = ul[P::ibp];
+#pragma unroll
+    for (int p = 0; p < nv; p++) f[p] = sl * ui[p] - wl[p];
+  } else if (sm < T(0)) {
+    const T r = rcp(srmm);
+    ui[P::idn] = dnr;
+    ui[P::imx] = dnr * sm;
+    ui[P::imy] = wr[P::imy] * r;
+    ui[P::imz] = wr[P::imz] * r;
+    ui[P::ibx] = T(0);
+    ui[P::iby] = wr[P::iby] * r;
+    ui[P::ibz] = wr[P::ibz] * r;
+    ui[P::ibp] = ur[P::ibp];
+#pragma unroll
+    for (int p = 0; p < nv; p++) f[p] = sr * ui[p] - wr[p];
+  } else {
+    hll();
+  }
+}
+
+// riemann_mhd_iso_roe (schemes.F90:2778-2906) with the eigensystem of
+// esystem_roe_mhd_iso (equations.F90:2552-2791) at the Roe average with the
+// magnetic corrections x, y: six waves (the fourth and the eighth carry
+// nothing), wave strengths from the left eigenvectors applied to the
+// conserved jump, summed from the left wave for a non-negative Roe velocity
+// and from the right one otherwise; the Bx and psi fluxes are the left ones
+template <class P, typename T>
+__device__ __forceinline__ void riemann_roe_mhd_iso(const EosConst<T>& e, T cmax, T cmax2,
+                                                    T* ql, T* qr, T* f) {
+  static_assert(P::iso && P::mhd, "the isothermal MHD Roe solver");
+  constexpr int nv = P::nv;
+  constexpr int idn = P::idn, ivx = P::ivx, ivy = P::ivy, ivz = P::ivz;
+  constexpr int ibx = P::ibx, iby = P::iby, ibz = P::ibz;
+  constexpr int imx = P::imx, imy = P::imy, imz = P::imz;
+  T ul[nv], ur[nv], fl[nv], fr[nv];
+  T cm, cp;
+  glm_upwind<P>(ql, qr, cmax);
+  prim2cons<P>(e, ql, ul);
+  prim2cons<P>(e, qr, ur);
+  fluxspeed<P>(e, cmax2, ql, ul, fl, cm, cp);
+  fluxspeed<P>(e, cmax2, qr, ur, fr, cm, cp);
+  const T sdl = tsqrt(ql[idn]), sdr = tsqrt(qr[idn]), sds = sdl + sdr;
+  const T dn = sdl * sdr;
+  const T vx = (sdl * ql[ivx] + sdr * qr[ivx]) / sds;
+  const T vy = (sdl * ql[ivy] + sdr * qr[ivy]) / sds;
+  const T vz = (sdl * ql[ivz] + sdr * qr[ivz]) / sds;
+  const T bx = ql[ibx];
+  const T by = (sdr * ql[iby] + sdl * qr[iby]) / sds;
+  const T bz = (sdr * ql[ibz] + sdl * qr[ibz]) / sds;
+  const T dby = ql[iby] - qr[iby], dbz = ql[ibz] - qr[ibz];
+  const T x = T(0.5) * (dby * dby + dbz * dbz) / (sds * sds);
+  const T y = T(0.5) * (ql[idn] + qr[idn]) / dn;
+  // esystem_roe_mhd_iso
+  const T di = T(1) / dn;
+  const T casq = bx * bx * di, ca = tsqrt(casq);
+  const T btsq = by * by + bz * bz;
+  const T bt_starsq = btsq * y;
+  const T twid_csq = e.csnd2 + x;
+  const T ct2 = bt_starsq * di;
+  const T tsum = (casq + ct2) + twid_csq, tdif = (casq + ct2) - twid_csq;
+  const T cfsq = T(0.5) * (tsum + tsqrt(tdif * tdif + T(4) * twid_csq * ct2));
+  const T cf = tsqrt(cfsq);
+  const T cssq = twid_csq * casq / cfsq, cs = tsqrt(cssq);
+  const T c[8] = {vx - cf, vx - ca, vx - cs, vx, vx + cs, vx + ca, vx + cf, vx + cf};
+  if (c[0] >= T(0)) {
+#pragma unroll
+    for (int p = 0; p < nv; p++) f[p] = fl[p];
+    return;
+  }
+  if (c[6] <= T(0)) {
+#pragma unroll
+    for (int p = 0; p < nv; p++) f[p] = fr[p];
+    return;
+  }
+  const T bt = tsqrt(btsq), bt_star = tsqrt(bt_starsq);
+  T bet2 = T(1), bet3 = T(0);
+  if (!(bt == T(0))) {
+    bet2 = by / bt;
+    bet3 = bz / bt;
+  }
+  const T bet2_star = bet2 / tsqrt(y), bet3_star = bet3 / tsqrt(y);
+  const T bet_starsq = bet2_star * bet2_star + bet3_star * bet3_star;
+  T alpha_f, alpha_s;
+  if ((cfsq - cssq) == T(0)) { alpha_f = T(1); alpha_s = T(0); }
+  else if ((twid_csq - cssq) <= T(0)) { alpha_f = T(0); alpha_s = T(1); }
+  else if ((cfsq - twid_csq) <= T(0)) { alpha_f = T(1); alpha_s = T(0); }
+  else {
+    alpha_f = tsqrt((twid_csq - cssq) / (cfsq - cssq));
+    alpha_s = tsqrt((cfsq - twid_csq) / (cfsq - cssq));
+  }
+  const T sqrtd = tsqrt(dn);
+  const T s = tsign(T(1), bx);
+  const T twid_c = tsqrt(twid_csq);
+  T qf = cf * alpha_f * s, qs = cs * alpha_s * s;
+  const T af_prime = twid_c * alpha_f / sqrtd, as_prime = twid_c * alpha_s / sqrtd;
+  // right eigenvectors R[wave][m] over the flux components m: idn, imx, imy,
+  // imz, iby, ibz
+  T R[8][6];
+#pragma unroll
+  for (int k = 0; k < 8; k++)
+#pragma unroll
+    for (int m = 0; m < 6; m++) R[k][m] = T(0);
+  R[0][0] = alpha_f; R[2][0] = alpha_s; R[4][0] = alpha_s; R[6][0] = alpha_f;
+  R[0][1] = alpha_f * c[0]; R[2][1] = alpha_s * c[2];
+  R[4][1] = alpha_s * c[4]; R[6][1] = alpha_f * c[6];
+  R[0][2] = alpha_f * vy + qs * bet2_star; R[1][2] = -bet3;
+  R[2][2] = alpha_s * vy - qf * bet2_star; R[4][2] = alpha_s * vy + qf * bet2_star;
+  R[5][2] = bet3; R[6][2] = alpha_f * vy - qs * bet2_star;
+  R[0][3] = alpha_f * vz + qs * bet3_star; R[1][3] = bet2;
+  R[2][3] = alpha_s * vz - qf * bet3_star; R[4][3] = alpha_s * vz + qf * bet3_star;
+  R[5][3] = -bet2; R[6][3] = alpha_f * vz - qs * bet3_star;
+  R[0][4] = as_prime * bet2_star; R[1][4] = -(bet3 * s / sqrtd);
+  R[2][4] = -(af_prime * bet2_star);
+  R[4][4] = R[2][4]; R[5][4] = R[1][4]; R[6][4] = R[0][4];
+  R[0][5] = as_prime * bet3_star; R[1][5] = bet2 * s / sqrtd;
+  R[2][5] = -(af_prime * bet3_star);
+  R[4][5] = R[2][5]; R[5][5] = R[1][5]; R[6][5] = R[0][5];
+  // left eigenvectors applied to the conserved jump: al[wave]
+  const T norm = T(0.5) / twid_csq;
+  const T cff = norm * alpha_f * cf, css = norm * alpha_s * cs;
+  qf = qf * norm;
+  qs = qs * norm;
+  const T af = norm * af_prime * dn, as = norm * as_prime * dn;
+  const T afpb = norm * af_prime * bt_star, aspb = norm * as_prime * bt_star;
+  const T q2_star = bet2_star / bet_starsq, q3_star = bet3_star / bet_starsq;
+  const T vqstr = vy * q2_star + vz * q3_star;
+  const T d0 = ur[idn] - ul[idn], d1 = ur[imx] - ul[imx], d2 = ur[imy] - ul[imy];
+  const T d3 = ur[imz] - ul[imz], d5 = ur[iby] - ul[iby], d6 = ur[ibz] - ul[ibz];
+  auto lsum = [&](T l0, T l1, T l2, T l3, T l5, T l6) {
+    return ((((l0 * d0 + l1 * d1) + l2 * d2) + l3 * d3) + l5 * d5) + l6 * d6;
+  };
+  T al[8];
+  al[0] = lsum((cff * c[6] - qs * vqstr) - aspb, -cff, qs * q2_star, qs * q3_star,
+               as * q2_star, as * q3_star);
+  const T l20 = T(0.5) * (vy * bet3 - vz * bet2), l22 = -(T(0.5) * bet3), l23 = T(0.5) * bet2;
+  const T l25 = -(T(0.5) * sqrtd * bet3 * s), l26 = T(0.5) * sqrtd * bet2 * s;
+  al[1] = lsum(l20, T(0), l22, l23, l25, l26);
+  const T l32 = -(qf * q2_star), l33 = -(qf * q3_star);
+  const T l35 = -(af * q2_star), l36 = -(af * q3_star);
+  al[2] = lsum((css * c[4] + qf * vqstr) + afpb, -css, l32, l33, l35, l36);
+  al[3] = T(0);
+  al[4] = lsum((-(css * c[2]) - qf * vqstr) + afpb, css, -l32, -l33, l35, l36);
+  al[5] = lsum(-l20, T(0), -l22, -l23, l25, l26);
+  al[6] = lsum((-(cff * c[0]) + qs * vqstr) - aspb, cff, -(qs * q2_star), -(qs * q3_star),
+               as * q2_star, as * q3_star);
+  al[7] = T(0);
+  constexpr int fv[6] = {idn, imx, imy, imz, iby, ibz};
+#pragma unroll
+  for (int m = 0; m < 6; m++) f[fv[m]] = T(0.5) * (fl[fv[m]] + fr[fv[m]]);
+  f[ibx] = fl[ibx];
+  f[P::ibp] = fl[P::ibp];
+  const bool fwd = vx >= T(0);
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    const int w = fwd ? k : 7 - k;
+    const T xx = -(T(0.5) * al[w] * tabs(c[w]));
+#pragma unroll
+    for (int m = 0; m < 6; m++) f[fv[m]] = f[fv[m]] + xx * R[w][m];
   }
 }
 

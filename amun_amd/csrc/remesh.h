@@ -25,10 +25,12 @@ enum { REMESH_COPY = 0, REMESH_PROLONG = 1, REMESH_RESTRICT = 2 };
 
 // prim2cons_hd_adi / prim2cons_mhd_adi (equations.F90:1762, 2815) of one
 // cell of an adiabatic HD (nv = 5) or MHD (nv = 9) primitive pool, variable
-// by variable as the reference orders the operations
+// by variable as the reference orders the operations; the isothermal
+// systems (ien < 0, nv = 4 or 8: prim2cons_hd_iso / _mhd_iso, 1360, 2220)
+// carry no energy
 template <typename T>
 __device__ __forceinline__ void ghost_cons(const T* __restrict__ Q, size_t b, size_t c,
-                                           size_t ncell, int nv, double gammam1i,
+                                           size_t ncell, int nv, int ien, double gammam1i,
                                            double* u) {
 #pragma clang fp contract(off)
   double q[9];
@@ -37,6 +39,10 @@ __device__ __forceinline__ void ghost_cons(const T* __restrict__ Q, size_t b, si
   u[1] = q[0] * q[1];
   u[2] = q[0] * q[2];
   u[3] = q[0] * q[3];
+  if (ien < 0) {
+    for (int v = 4; v < nv; v++) u[v] = q[v];
+    return;
+  }
   const double ek = 0.5 * ((u[1] * q[1] + u[2] * q[2]) + u[3] * q[3]);
   if (nv == 9) {
     for (int v = 4; v < 8; v++) u[v] = q[v];
@@ -135,7 +141,7 @@ k_remesh(const int4* __restrict__ task, const int* __restrict__ kids, int nblk,
         for (int v = 0; v < nv; v++) dst[v] = U0o[pb + (size_t)v * ncell + c];
       } else {
         const size_t pk = cell_index<sizeof(T)>(nbro, rc, P, q[0], q[1], q[2]);
-        ghost_cons<T>(Qo, pk >> 32, pk & 0xffffffffull, ncell, nv, rc.gammam1i, dst);
+        ghost_cons<T>(Qo, pk >> 32, pk & 0xffffffffull, ncell, nv, ien, rc.gammam1i, dst);
       }
     }
   const int cc = lidx<8>(nc, nd, ci, cj, ck);

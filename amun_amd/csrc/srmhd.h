@@ -30,6 +30,7 @@ struct SRMHD {
   static constexpr int ipr = 8, ien = 8;
   static constexpr bool mhd = true;
   static constexpr bool sr = true;
+  static constexpr bool iso = false;
 };
 
 // constants of the primitive solver ("tolerance", "nr_maxit", "nr_extra",

@@ -35,7 +35,7 @@ module amun_dev_mod
   integer(c_int32_t), parameter :: AMUN_LIM_MC = 2, AMUN_LIM_SUPERBEE = 3
   integer(c_int32_t), parameter :: AMUN_LIM_VANLEER = 4, AMUN_LIM_VANALBADA = 5
   integer(c_int32_t), parameter :: AMUN_RS_HLL = 1, AMUN_RS_HLLC = 2
-  integer(c_int32_t), parameter :: AMUN_RS_HLLD = 3, AMUN_RS_ROE = 4
+  integer(c_int32_t), parameter :: AMUN_RS_HLLD = 3, AMUN_RS_ROE = 4, AMUN_RS_HLLDM = 5
   integer(c_int32_t), parameter :: AMUN_TA_EULER = 0, AMUN_TA_RK2 = 1
   integer(c_int32_t), parameter :: AMUN_TA_RK3 = 2, AMUN_TA_SSPRK_M2 = 3
   integer(c_int32_t), parameter :: AMUN_TA_SSPRK43 = 4, AMUN_TA_SSPRK53 = 5

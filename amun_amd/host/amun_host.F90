@@ -167,6 +167,8 @@ program amun_host
     cfg%riemann_solver = AMUN_RS_HLLD
   case("roe", "ROE")
     cfg%riemann_solver = AMUN_RS_ROE
+  case("hlldm", "hlld-m", "HLLDM", "HLLD-M")
+    cfg%riemann_solver = AMUN_RS_HLLDM
   case default
     cfg%riemann_solver = -1
   end select

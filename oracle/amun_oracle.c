@@ -3028,6 +3028,9 @@ static int riemann_check(const amun_cfg* c)
   if (c->riemann_solver == AMUN_RS_ROE &&
       (c->eqsys == AMUN_EQSYS_HD || c->eqsys == AMUN_EQSYS_MHD)) return 0;
   if (c->riemann_solver == AMUN_RS_HLLD && c->eqsys == AMUN_EQSYS_MHD) return 0;
+  /* HLLD-M is the isothermal MHD system's (schemes.F90:219-221) */
+  if (c->riemann_solver == AMUN_RS_HLLDM && c->eqsys == AMUN_EQSYS_MHD &&
+      c->eos == AMUN_EOS_ISO) return 0;
   /* SRMHD: "hllc" selects riemann_srmhd_adi_hllc, any other name HLL
    * (schemes.F90:479-498) */
   if (c->eqsys == AMUN_EQSYS_SRMHD) return 0;
@@ -4471,6 +4474,7 @@ int orc_get_state(void* h, double* uglob, double* qglob)
 /* the statistics of store_integrals (integrals.F90:254-485) of the current
  * primitive state: volume-weighted mean, minimum and maximum of density,
  * pressure, |v|, |B|, psi, the sonic Mach number sqrt(rho)|v|/sqrt(gamma p)
+ * (|v| / csnd for the isothermal systems, whose pressure entries stay 0)
  * and the Alfvenic one sqrt(rho)|v|/max(eps, |B|), in that order, out[21];
  * per-block sums times the block's cell volume, added in block order */
 int orc_statistics(void* h, double* out)
@@ -4483,6 +4487,7 @@ int orc_statistics(void* h, double* out)
   int b, i, j, k, m;
   for (m = 0; m < 7; m++) { av[m] = 0.0; mn[m] = big; mx[m] = -big; }
   if (!o->mhd) { mn[3] = mx[3] = mn[4] = mx[4] = mn[6] = mx[6] = 0.0; }
+  if (o->iso) mn[1] = mx[1] = 0.0;     /* no pressure (integrals.F90:320-323) */
   for (b = 0; b < g->nblk; b++) {
     const double* qb = g->q + b * bs;
     double dvol, s[7] = {0, 0, 0, 0, 0, 0, 0}, lo[7], hi[7];
@@ -4503,14 +4508,16 @@ int orc_statistics(void* h, double* out)
             mag = sqrt((q[o->ibx] * q[o->ibx] + q[o->iby] * q[o->iby]) + q[o->ibz] * q[o->ibz]);
           sqd = sqrt(q[o->idn]);
           t[0] = q[o->idn];
-          t[1] = q[o->ipr];
+          t[1] = o->iso ? 0.0 : q[o->ipr];
           t[2] = vel;
           t[3] = mag;
           t[4] = o->mhd ? q[o->ibp] : 0.0;
-          t[5] = sqd * vel / sqrt(o->c.gamma * q[o->ipr]);
+          /* the isothermal Mach number is |v| / csnd (integrals.F90:418-423) */
+          t[5] = o->iso ? vel / o->csnd : sqd * vel / sqrt(o->c.gamma * q[o->ipr]);
           t[6] = sqd * vel / fmax_r(eps, mag);
           for (m = 0; m < 7; m++) {
             if (!o->mhd && (m == 3 || m == 4 || m == 6)) continue;
+            if (o->iso && m == 1) continue;
             s[m] += t[m];
             lo[m] = fmin_r(lo[m], t[m]);
             hi[m] = fmax_r(hi[m], t[m]);
@@ -4518,6 +4525,7 @@ int orc_statistics(void* h, double* out)
         }
     for (m = 0; m < 7; m++) {
       if (!o->mhd && (m == 3 || m == 4 || m == 6)) continue;
+      if (o->iso && m == 1) continue;
       av[m] = av[m] + s[m] * dvol;
       mn[m] = fmin_r(mn[m], lo[m]);
       mx[m] = fmax_r(mx[m], hi[m]);

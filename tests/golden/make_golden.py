@@ -910,6 +910,142 @@ def main_stats():
     np.savez_compressed(os.path.join(HERE, "stats_ref.npz"), **out)
 
 
+ISO_FLUX = (("hd", "hll"), ("hd", "roe"), ("mhd", "hll"), ("mhd", "hlld"), ("mhd", "hlldm"),
+            ("mhd", "roe"))
+# (name, ndims, equation system, reconstruction, Riemann solver, steps)
+ISO_RUNS = (("iso_hd2d_tvd_roe_blast_20", 2, "hd", "tvd", "roe", 20),
+            ("iso_hd2d_tvd_hll_blast_20", 2, "hd", "tvd", "hll", 20),
+            ("iso_hd3d_weno5z_hll_noisy_4", 3, "hd", "weno5z", "hll", 4),
+            ("iso_hd3d_mp5_roe_noisy_4", 3, "hd", "mp5", "roe", 4),
+            ("iso_mhd2d_tvd_hlld_ot_20", 2, "mhd", "tvd", "hlld", 20),
+            ("iso_mhd2d_tvd_hlldm_ot_20", 2, "mhd", "tvd", "hlldm", 20),
+            ("iso_mhd2d_tvd_roe_ot_20", 2, "mhd", "tvd", "roe", 20),
+            ("iso_mhd2d_tvd_hll_ot_20", 2, "mhd", "tvd", "hll", 20),
+            ("iso_mhd3d_weno5z_hlld_noisy_4", 3, "mhd", "weno5z", "hlld", 4),
+            ("iso_mhd3d_weno5z_roe_noisy_4", 3, "mhd", "weno5z", "roe", 4))
+ISO_CSND = 0.7
+
+
+def iso_params(nd, eq, rec, rs):
+    return Params(ndims=nd, equation_system=eq, equation_of_state="iso", csnd=ISO_CSND,
+                  reconstruction=rec, limiter="mc" if rec == "tvd" else "mm",
+                  riemann_solver=rs, time_advance="rk2" if eq == "hd" else "rk3",
+                  cfl=0.4 if nd == 2 else 0.3, ncells=16 if nd == 2 else 8,
+                  nghosts=2 if rec == "tvd" else 4, xblocks=2, yblocks=2, zblocks=2,
+                  xmin=-0.5, xmax=0.5, ymin=-0.5, ymax=0.5, zmin=-0.5, zmax=0.5)
+
+
+def iso_state(name, p, seed=6):
+    """Seeded isothermal states: a 10:1 density blast at rest (HD, r < 0.1),
+    the Orszag-Tang field with 2 % noise (2D MHD), or the smooth fully
+    three-dimensional fields of noisy3d without pressure (3D)."""
+    nz, ny, nx = p.shape
+    z, y, x = problems.cell_centres(p)
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    xi = [problems.uniform(seed, nz * ny * nx, stream=s).reshape(nz, ny, nx) - 0.5
+          for s in range(40, 48)]
+    q = np.zeros((p.nv, nz, ny, nx))
+    tp = 2.0 * np.pi
+    if "blast" in name:
+        r = np.sqrt(X ** 2 + Y ** 2)
+        q[0] = np.where(r < 0.1, 10.0, 1.0) * (1.0 + 0.02 * xi[0])
+        q[1] = 0.01 * xi[1]
+        q[2] = 0.01 * xi[2]
+    elif "_ot_" in name:
+        b0 = 1.0 / np.sqrt(4.0 * np.pi)
+        q[0] = 1.0 + 0.02 * xi[0]
+        q[1] = -np.sin(tp * Y) + 0.02 * xi[1]
+        q[2] = np.sin(tp * X) + 0.02 * xi[2]
+        q[4] = -b0 * np.sin(tp * Y) + 0.02 * b0 * xi[3]
+        q[5] = b0 * np.sin(2.0 * tp * X)
+    else:
+        a = 0.02
+        q[0] = 1.0 + 0.2 * np.sin(tp * (X + Y + Z)) + a * xi[0]
+        q[1] = 0.5 * np.sin(tp * Y) + 0.3 * np.cos(tp * Z) + a * xi[1]
+        q[2] = 0.5 * np.sin(tp * Z) + 0.3 * np.cos(tp * X) + a * xi[2]
+        q[3] = 0.5 * np.sin(tp * X) + 0.3 * np.cos(tp * Y) + a * xi[3]
+        if p.nv == 8:
+            q[4] = 0.3 * np.sin(tp * (Y + Z)) + 0.2 + a * xi[4]
+            q[5] = 0.3 * np.sin(tp * (Z + X)) - 0.1 + a * xi[5]
+            q[6] = 0.3 * np.sin(tp * (X + Y)) + 0.15 + a * xi[6]
+            q[7] = a * xi[7]
+    return problems.prim_to_cons(p, q)
+
+
+def iso_stat_case(name):
+    nd, eq = (2, "hd") if name == "hd2d_iso" else (2, "mhd")
+    p = iso_params(nd, eq, "tvd", "roe" if eq == "hd" else "hlld")
+    p.xblocks, p.yblocks = 3, 2
+    return p, iso_state("iso_blast" if eq == "hd" else "iso_ot_", p, seed=3)
+
+
+def main_iso():
+    """The isothermal HD and MHD systems (equations.F90:399-830): equation of
+    state vectors, every Riemann solver through update_flux (zero limiter,
+    random states with weak, zero and strong Bx and fast flows), runs of TVD,
+    WENO5-Z and MP5 in 2D and 3D, and the integrals and statistics files
+    (own generator)."""
+    rng = np.random.default_rng(20261211)
+    for eq, nv in (("hd", 4), ("mhd", 8)):
+        pe = Params(ndims=3, equation_system=eq, equation_of_state="iso", csnd=ISO_CSND)
+        q = np.zeros((64, nv))
+        q[:, 0] = rng.uniform(0.1, 3.0, 64)
+        q[:, 1:4] = rng.normal(size=(64, 3))
+        if nv == 8:
+            q[:, 4:8] = rng.normal(size=(64, 4))
+            q[:5, 4] = 0.0
+        u, q2, f, cm, cp = ol.ref_eqs(pe, q, cmax=1.7)
+        np.savez_compressed(os.path.join(HERE, f"unit_eqs_{eq}_iso.npz"), q=q, u=u, q2=q2,
+                            f=f, cm=cm, cp=cp, cmax=1.7, params=np.array(pe.to_params_in()))
+    for eq, rs in ISO_FLUX:
+        pf = Params(ndims=2, equation_system=eq, equation_of_state="iso", csnd=ISO_CSND,
+                    reconstruction="tvd", limiter="zero", riemann_solver=rs, ncells=8,
+                    nghosts=2)
+        nv = pf.nv
+        qb = np.zeros((4, 1, 12, 12, nv))
+        qb[..., 0] = rng.uniform(0.2, 2.0, qb.shape[:-1])
+        qb[..., 1:4] = rng.normal(scale=1.0, size=qb.shape[:-1] + (3,))
+        if nv == 8:
+            qb[..., 4:7] = rng.normal(size=qb.shape[:-1] + (3,))
+            qb[..., 7] = rng.normal(scale=0.1, size=qb.shape[:-1])
+            qb[1, ..., 4] = 1e-6 * rng.normal(size=qb.shape[1:-1])   # weak Bx
+            qb[2, ..., 4] = 0.0                                       # Bx* = 0
+            qb[2, ..., 7] = 0.0
+            qb[3, ..., 1] = qb[3, ..., 1] * 5.0                       # fast flows
+        else:
+            qb[1, ..., 1] = qb[1, ..., 1] * 6.0                       # supersonic
+            qb[2, ..., 1:4] = 0.0                                     # at rest
+            qb[3, ..., 1] = 1e-3 * rng.normal(size=qb.shape[1:-1])   # Roe v ~ 0
+        f = ol.ref_update_flux(pf, qb, cmax=2.3)
+        np.savez_compressed(os.path.join(HERE, f"unit_flux_{eq}_iso_{rs}.npz"), q=qb, f=f,
+                            cmax=2.3, params=np.array(pf.to_params_in()))
+        print("unit_flux", eq, rs)
+    with open(os.path.join(HERE, "index.json")) as fh:
+        index = json.load(fh)
+    for name, nd, eq, rec, rs, nsteps in ISO_RUNS:
+        p = iso_params(nd, eq, rec, rs)
+        u0 = iso_state(name, p)
+        u, tr = ol.ref_run(p, u0, nsteps)
+        sub = u[:, ::2, ::2, ::2] if nd == 3 else u[:, :, ::3, ::3]
+        np.savez_compressed(os.path.join(HERE, name + ".npz"),
+                            params=np.array(p.to_params_in()), ndims=nd, nsteps=nsteps,
+                            stride=2 if nd == 3 else 3, trace=tr, sub=sub, sha_u0=sha(u0),
+                            sha_u=sha(u))
+        index[name] = {"steps": nsteps, "sha_u": sha(u), "dt_last": float(tr[-1, 0])}
+        print(name, "dt", tr[-1, 0])
+    with open(os.path.join(HERE, "index.json"), "w") as fh:
+        json.dump(index, fh, indent=1)
+    out = {}
+    for name in ("hd2d_iso", "mhd2d_iso"):
+        p, u0 = iso_stat_case(name)
+        ints, st = ol.ref_statistics(p, u0, 3)
+        out["ints_" + name], out["stats_" + name] = ints, st
+        out["params_" + name] = np.array(p.to_params_in())
+        out["sha_u0_" + name] = np.array(sha(u0))
+        print(name, st[:3])
+    np.savez_compressed(os.path.join(HERE, "stats_iso_ref.npz"), **out)
+
+
 def main():
     if not (ol.ref_available(2) and ol.ref_available(3)):
         sys.exit("reference binaries missing: run oracle/build_ref.sh first")
@@ -925,6 +1061,9 @@ def main():
     if sys.argv[1:] == ["refine"]:
         main_refine()
         sys.exit(0)
+    if sys.argv[1:] == ["iso"]:
+        main_iso()
+        return
     if sys.argv[1:] == ["mlp"]:
         main_mlp()
         return

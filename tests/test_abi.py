@@ -96,6 +96,38 @@ def test_glm_source_terms_carried(value, code):
     assert lib.amun_dev_init(ctypes.byref(cfg), ctypes.byref(h)) == 500
 
 
+@pytest.mark.parametrize("text,code", [
+    ("equation_system = srmhd\nequation_of_state = iso\n", 110),
+    ("equation_system = mhd\nequation_of_state = iso\ncsnd = 0.0\n", 500),
+    ("equation_system = mhd\nequation_of_state = adi\nriemann_solver = hlldm\n", 300),
+    ("equation_system = hd\nequation_of_state = iso\nriemann_solver = hlld\n", 300),
+])
+def test_isothermal_configuration_errors(text, code):
+    """The isothermal systems are HD and MHD (equations.F90:399-830), with a
+    positive sound speed; HLLD-M belongs to isothermal MHD
+    (schemes.F90:219-221)."""
+    p = Params.from_text(text, ndims=2)
+    cfg = p.to_cfg()
+    lib = solver.load_library()
+    h = ctypes.c_void_p()
+    assert lib.amun_dev_init(ctypes.byref(cfg), ctypes.byref(h)) == code
+    assert lib.amun_dev_last_error()
+
+
+def test_isothermal_configuration_carried():
+    """equation_of_state and csnd reach amun_cfg; four and eight variables."""
+    p = Params.from_text("equation_system = mhd\nequation_of_state = isothermal\n"
+                         "csnd = 0.35\nriemann_solver = hlld-m\n", ndims=2)
+    cfg = p.to_cfg()
+    assert cfg.eos == 2 and cfg.csnd == 0.35 and cfg.riemann_solver == 5
+    assert p.nv == 8
+    assert Params(equation_system="hd", equation_of_state="iso").nv == 4
+    lib = solver.load_library()
+    d = AmunCfg()
+    lib.amun_dev_default_cfg(ctypes.byref(d))
+    assert d.csnd == 1.0                             # equations.F90:176
+
+
 def test_weno_needs_four_ghosts():
     p = c2_params(32)
     p.nghosts = 2
