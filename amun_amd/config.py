@@ -17,7 +17,7 @@ from dataclasses import dataclass, field, fields
 
 # constants mirrored from include/amun_dev.h
 EQSYS = {"hd": 1, "hydrodynamic": 1, "mhd": 2, "magnetohydrodynamic": 2,
-         "srmhd": 4}
+         "srhd": 3, "srmhd": 4}
 EOS = {"adi": 1, "adiabatic": 1, "iso": 2, "isothermal": 2}
 RECON = {"tvd": 1, "weno3": 2, "limo3": 3, "ppm": 4, "crweno5z": 8, "crweno5-z": 8,
          "crweno5yc": 9, "crweno5-yc": 9, "crweno5ns": 10, "crweno5-ns": 10, "crmp5": 13, "mgp": 21,
@@ -65,6 +65,7 @@ class AmunCfg(ctypes.Structure):
         ("glm_source_terms", ctypes.c_int32), ("reserved_cfg", ctypes.c_int32),
         ("viscosity", ctypes.c_double), ("resistivity", ctypes.c_double),
         ("csnd", ctypes.c_double),
+        ("states_4vec", ctypes.c_int32), ("reserved_cfg2", ctypes.c_int32),
     ]
 
 
@@ -83,6 +84,7 @@ class AmunRefineCfg(ctypes.Structure):
 
 # primitive variable names of equations.F90:410-414, 513-521
 PVARS = {1: ["dens", "velx", "vely", "velz", "pres"],
+         3: ["dens", "velx", "vely", "velz", "pres"],
          2: ["dens", "velx", "vely", "velz", "magx", "magy", "magz", "bpsi", "pres"],
          4: ["dens", "velx", "vely", "velz", "magx", "magy", "magz", "bpsi", "pres"]}
 
@@ -141,6 +143,7 @@ class Params:
     equation_system: str = "hd"
     equation_of_state: str = "adi"
     csnd: float = 1.0
+    state_variables: str = "primitive"
     reconstruction: str = "tvd"
     limiter: str = "mm"
     riemann_solver: str = "hlld"
@@ -193,7 +196,7 @@ class Params:
     @property
     def nv(self) -> int:
         iso = EOS.get(self.equation_of_state.lower(), 1) == 2
-        return (5 if EQSYS[self.equation_system.lower()] == 1 else 9) - (1 if iso else 0)
+        return (5 if EQSYS[self.equation_system.lower()] in (1, 3) else 9) - (1 if iso else 0)
 
     @property
     def shape(self):
@@ -255,6 +258,7 @@ class Params:
             str(self.glm_source_terms).strip().lower(), 0)
         c.viscosity, c.resistivity = float(self.viscosity), float(self.resistivity)
         c.csnd = float(self.csnd)
+        c.states_4vec = int(self.state_variables.lower() in ("4vec", "4-vector"))
         c.prol_limiter = {"mm": 1, "minmod": 1, "mc": 2, "monotonized_central": 2,
                           "sb": 3, "superbee": 3, "vl": 4, "vanleer": 4}.get(
                               self.prolongation_limiter.lower(), 0)
@@ -269,10 +273,10 @@ class Params:
 
     def to_params_in(self) -> str:
         """Text of an AMUN params.in with these settings (periodic, one level)."""
-        eq = {1: "hd", 2: "mhd", 4: "srmhd"}[EQSYS[self.equation_system.lower()]]
+        eq = {1: "hd", 2: "mhd", 3: "srhd", 4: "srmhd"}[EQSYS[self.equation_system.lower()]]
         lines = [
             f"equation_system = {eq}", f"equation_of_state = {self.equation_of_state}",
-            f"csnd = {self.csnd!r}",
+            f"csnd = {self.csnd!r}", f"state_variables = {self.state_variables}",
             f"reconstruction = {self.reconstruction}", f"limiter = {self.limiter}",
             f"riemann_solver = {self.riemann_solver}",
             f"time_advance = {self.time_advance}",

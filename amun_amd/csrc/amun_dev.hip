@@ -95,9 +95,21 @@ static void by_system(const amun_cfg& cfg, F&& f) {
   } else if (cfg.eqsys == AMUN_EQSYS_MHD) {
     if (iso) f(amun::MHDI{});
     else f(amun::MHD{});
+  } else if (cfg.eqsys == AMUN_EQSYS_SRHD) {
+    f(amun::SRHD{});
   } else {
     f(amun::SRMHD{});
   }
+}
+
+// the systems with a magnetic field (and GLM psi)
+static bool has_field(const amun_cfg& c) {
+  return c.eqsys == AMUN_EQSYS_MHD || c.eqsys == AMUN_EQSYS_SRMHD;
+}
+
+// the special-relativistic systems
+static bool relativistic(const amun_cfg& c) {
+  return c.eqsys == AMUN_EQSYS_SRHD || c.eqsys == AMUN_EQSYS_SRMHD;
 }
 
 // the pressure (energy) variable, -1 for the isothermal systems
@@ -107,7 +119,7 @@ static int pres_index(const amun_ctx* c) {
 
 // variables of a configuration validate() accepts
 static int cfg_nv(const amun_cfg& c) {
-  const int nv = c.eqsys == AMUN_EQSYS_HD ? 5 : 9;
+  const int nv = has_field(c) ? 9 : 5;
   return c.eos == AMUN_EOS_ISO ? nv - 1 : nv;
 }
 
@@ -116,7 +128,7 @@ static int bnd_fill(amun_ctx* c, int qcur) {
   if (!c->bnd.on) return 0;
   const long long n = (long long)c->bnd.n * c->rc.ncell;
   const dim3 grid((unsigned)((n + 255) / 256)), bl(256);
-  const int ibx = c->cfg.eqsys != AMUN_EQSYS_HD ? 4 : -1;
+  const int ibx = has_field(c->cfg) ? 4 : -1;
   by_system(c->cfg, [&](auto tag) {
     using P = decltype(tag);
     if (c->qsize == sizeof(double))
@@ -145,15 +157,15 @@ bool pick(Dispatch& d, const amun_cfg& c) {
 
 int validate(const amun_cfg& c, int* nv) {
   if (c.ndims != 2 && c.ndims != 3) return fail(500, "ndims must be 2 or 3");
-  if (c.eqsys == AMUN_EQSYS_HD) *nv = 5;
+  if (c.eqsys == AMUN_EQSYS_HD || c.eqsys == AMUN_EQSYS_SRHD) *nv = 5;
   else if (c.eqsys == AMUN_EQSYS_MHD || c.eqsys == AMUN_EQSYS_SRMHD) *nv = 9;
   else return fail(100, "unsupported equation_system");
   if (c.eos != AMUN_EOS_ADI && c.eos != AMUN_EOS_ISO)
     return fail(110, "unsupported equation_of_state");
   if (c.eos == AMUN_EOS_ISO) {
     // equations.F90:399-830: isothermal HD and MHD, no energy equation
-    if (c.eqsys == AMUN_EQSYS_SRMHD)
-      return fail(110, "the relativistic system is adiabatic only");
+    if (relativistic(c))
+      return fail(110, "the relativistic systems are adiabatic only");
     if (!(c.csnd > 0.0)) return fail(500, "csnd must be positive");
     *nv -= 1;
   }
@@ -173,7 +185,7 @@ int validate(const amun_cfg& c, int* nv) {
   if ((c.reconstruction == AMUN_REC_GP || c.reconstruction == AMUN_REC_MGP) && !(c.sgp > 0.0))
     return fail(202, "sgp must be positive");
   if (c.limiter < 0 || c.limiter > 5) return fail(200, "unsupported limiter");
-  if (c.eqsys == AMUN_EQSYS_SRMHD) {
+  if (relativistic(c)) {
     if (!(c.tolerance > 0.0) || c.nr_maxit < 1 || c.nr_extra < 0 || !(c.pmin >= 0.0))
       return fail(500, "tolerance, nr_maxit, nr_extra or pmin out of range");
     if (c.primitive_solver < AMUN_C2P_1DW || c.primitive_solver > AMUN_C2P_2DWU)
@@ -194,6 +206,8 @@ int validate(const amun_cfg& c, int* nv) {
     return fail(600, "fix_positivity / clip_extrema are not supported on the device path");
   if (c.fix_unphysical_cells)
     return fail(600, "fix_unphysical_cells is not supported on the device path");
+  if (c.states_4vec && relativistic(c))
+    return fail(600, "state_variables = 4vec is not supported on the device path");
   if (c.glm_source_terms < 0 || c.glm_source_terms > 2)
     return fail(500, "glm_source_terms must be 0 (none), 1 (eglm) or 2 (heglm)");
   if (c.viscosity != 0.0 || c.resistivity != 0.0)
@@ -233,8 +247,7 @@ int validate(const amun_cfg& c, int* nv) {
     if ((a == AMUN_BND_PERIODIC) != (b == AMUN_BND_PERIODIC))
       return fail(600, "an axis is periodic on both sides or on neither");
   }
-  if ((c.bnd_type[0] || c.bnd_type[2] || (c.ndims == 3 && c.bnd_type[4])) &&
-      c.eqsys == AMUN_EQSYS_SRMHD)
+  if ((c.bnd_type[0] || c.bnd_type[2] || (c.ndims == 3 && c.bnd_type[4])) && relativistic(c))
     return fail(600, "non-periodic sides are built for the HD and MHD systems");
   if (!(c.gamma > 1.0) || !(c.cfl > 0.0)) return fail(500, "gamma must exceed 1 and cfl be positive");
   return 0;
@@ -602,7 +615,7 @@ static int init_device(amun_ctx* c, const std::vector<int32_t>& nbr,
     ALLOC(c->mlps, (size_t)rc.nblk * nv * nd * rc.ncell * c->qsize);
     ALLOC(c->mlpg, (size_t)rc.nblk * nv * nd * gs * c->qsize);
   }
-  if (cfg->glm_source_terms > 0 && cfg->eqsys != AMUN_EQSYS_HD) {
+  if (cfg->glm_source_terms > 0 && has_field(*cfg)) {
     // the eight GLM source products of every cell (k_glm_src)
     ALLOC(c->src, (size_t)rc.nblk * 8 * rc.ncell * c->qsize);
   }
@@ -694,8 +707,8 @@ int amun_dev_init_amr(const amun_cfg* cfg, int32_t nblk, const int32_t* meta, am
   int nv = 0;
   int r = validate(*cfg, &nv);
   if (r) return r;
-  if (cfg->eqsys == AMUN_EQSYS_SRMHD)
-    return fail(600, "multi-level meshes are not supported for the relativistic system");
+  if (relativistic(*cfg))
+    return fail(600, "multi-level meshes are not supported for the relativistic systems");
   if (cfg->bnd_type[0] || cfg->bnd_type[2] || (cfg->ndims == 3 && cfg->bnd_type[4]))
     return fail(600, "multi-level meshes are built for periodic domains");
   if (is_compact(cfg->reconstruction))
@@ -1091,7 +1104,7 @@ static int download_blocks_amr(amun_ctx* c, double* u, double* q) {
   if (e != hipSuccess) return fail(1000 + (int)e, "download_blocks");
   std::vector<double> h(c->nstate);
   HIP_TRY(hipMemcpy(h.data(), c->U0, c->nstate * sizeof(double), hipMemcpyDeviceToHost));
-  const bool mhd = c->cfg.eqsys != AMUN_EQSYS_HD, iso = c->cfg.eos == AMUN_EOS_ISO;
+  const bool mhd = has_field(c->cfg), iso = c->cfg.eos == AMUN_EOS_ISO;
   const double g1 = c->cfg.gamma - 1.0;
   for (int b = 0; b < rc.nblk; b++)
     for (int k = 0; k < km; k++)
@@ -1767,7 +1780,7 @@ static int statistics_range(amun_ctx* c, int r0, int nr, double out[21]) {
       wsum += (c->amr.on ? std::ldexp(1.0, -c->rc.nd * blev[b]) : 1.0) * c->rc.ncell;
   }
   // without a magnetic field the reference reports zeros for |B|, psi, M_A
-  const bool mhd = c->cfg.eqsys != AMUN_EQSYS_HD, iso = c->cfg.eos == AMUN_EOS_ISO;
+  const bool mhd = has_field(c->cfg), iso = c->cfg.eos == AMUN_EOS_ISO;
   for (int m = 0; m < 7; m++) {
     const bool zero = (!mhd && (m == 3 || m == 4 || m == 6)) || (iso && m == 1);
     out[3 * m] = zero ? 0.0 : sum[m] / wsum;

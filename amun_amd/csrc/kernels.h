@@ -1899,6 +1899,11 @@ bool pick_eq(Dispatch& d, const amun_cfg& c) {
     // (schemes.F90:479-498)
     if (c.riemann_solver == AMUN_RS_HLLC) return pick_nd<SRMHD, RS_HLLC, REC, T>(d, c.ndims);
     return pick_nd<SRMHD, RS_HLL, REC, T>(d, c.ndims);
+  } else if constexpr (EQ == AMUN_EQSYS_SRHD) {
+    // "hllc" selects riemann_srhd_adi_hllc, every other name HLL
+    // (schemes.F90:400-430)
+    if (c.riemann_solver == AMUN_RS_HLLC) return pick_nd<SRHD, RS_HLLC, REC, T>(d, c.ndims);
+    return pick_nd<SRHD, RS_HLL, REC, T>(d, c.ndims);
   } else if constexpr (EQ == EQ_HD_ISO) {
     // "roe" selects riemann_hd_iso_roe, every other name HLL
     // (schemes.F90:163-176)

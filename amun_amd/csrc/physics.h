@@ -518,8 +518,18 @@ __device__ __forceinline__ void cons2prim(const EosConst<T>& e, const T* u, T* q
 // maxspeed_srmhd_adi (5116-5163) is the speed of light
 template <class P, typename T>
 __device__ __forceinline__ T maxspeed(const EosConst<T>& e, const T* q) {
-  if constexpr (P::sr) return T(1);
+  if constexpr (P::sr && P::mhd) return T(1);
   T vv = (q[P::ivx] * q[P::ivx] + q[P::ivy] * q[P::ivy]) + q[P::ivz] * q[P::ivz];
+  if constexpr (P::sr) {
+    // maxspeed_srhd_adi (equations.F90:3746-3820): (|v| + c) / (1 + s)
+    const T gx = e.gammam1 / e.gamma;
+    const T ww = q[P::idn] + q[P::ipr] / gx;
+    const T c2 = e.gamma * q[P::ipr] / ww;
+    const T ss = c2 * (T(1) - vv) / (T(1) - c2);
+    const T fc = T(1) + ss;
+    const T cc = tsqrt(ss * (fc - vv));
+    return (tsqrt(vv) + cc) / fc;
+  }
   T v = tsqrt(vv);
   T c;
   if constexpr (P::iso && !P::mhd) {

@@ -33,6 +33,20 @@ struct SRMHD {
   static constexpr bool iso = false;
 };
 
+// the special-relativistic HD system (equations.F90:537-600): nv = 5, no
+// field; its equation of state, primitive solvers and HLLC are the !mhd
+// branches below (equations.F90:3478-4566, schemes.F90:4656-4968)
+struct SRHD {
+  static constexpr int nv = 5;
+  static constexpr int idn = 0, ivx = 1, ivy = 2, ivz = 3;
+  static constexpr int imx = 1, imy = 2, imz = 3;
+  static constexpr int ibx = -1, iby = -1, ibz = -1, ibp = -1;
+  static constexpr int ipr = 4, ien = 4;
+  static constexpr bool mhd = false;
+  static constexpr bool sr = true;
+  static constexpr bool iso = false;
+};
+
 // constants of the primitive solver ("tolerance", "nr_maxit", "nr_extra",
 // "pmin"; equations.F90:184, 196-201)
 struct SrConst {
@@ -356,6 +370,19 @@ __device__ __forceinline__ int alg_quartic(const double* a, double* x) {
 // prim2cons_srmhd_adi, equations.F90:4590-4653
 template <class P, typename T>
 __device__ __forceinline__ void prim2cons_sr(T rgammaxi, const T* q, T* u) {
+  if constexpr (!P::mhd) {
+    // prim2cons_srhd_adi, equations.F90:3478-3532
+    const T vv = (q[P::ivx] * q[P::ivx] + q[P::ivy] * q[P::ivy]) + q[P::ivz] * q[P::ivz];
+    const T vm = T(1) - vv;
+    const T vs = tsqrt(vm);
+    const T ww = (q[P::idn] + q[P::ipr] * rgammaxi) * rcp(vm);
+    u[P::idn] = q[P::idn] * rcp(vs);
+    u[P::imx] = ww * q[P::ivx];
+    u[P::imy] = ww * q[P::ivy];
+    u[P::imz] = ww * q[P::ivz];
+    u[P::ien] = (ww - q[P::ipr]) - u[P::idn];
+    return;
+  } else {
   const T vv = (q[P::ivx] * q[P::ivx] + q[P::ivy] * q[P::ivy]) + q[P::ivz] * q[P::ivz];
   const T bb = (q[P::ibx] * q[P::ibx] + q[P::iby] * q[P::iby]) + q[P::ibz] * q[P::ibz];
   const T vb = (q[P::ivx] * q[P::ibx] + q[P::ivy] * q[P::iby]) + q[P::ivz] * q[P::ibz];
@@ -372,6 +399,7 @@ __device__ __forceinline__ void prim2cons_sr(T rgammaxi, const T* q, T* u) {
   u[P::ibz] = q[P::ibz];
   u[P::ibp] = q[P::ibp];
   u[P::ien] = ((wt - q[P::ipr]) - u[P::idn]) - T(0.5) * (vm * bb + vb * vb);
+  }
 }
 
 // characteristic speeds of fluxspeed_srmhd_adi (equations.F90:4895-5083), in
@@ -458,6 +486,25 @@ __device__ __forceinline__ void sr_speeds(const SrConst& s, double rho, double v
 template <class P, typename T>
 __device__ __forceinline__ void fluxspeed_sr(const SrConst& s, T cmax2, const T* q, const T* u,
                                              T* f, T& cm, T& cp) {
+  if constexpr (!P::mhd) {
+    // fluxspeed_srhd_adi, equations.F90:3665-3731 (speeds in double)
+    f[P::idn] = u[P::idn] * q[P::ivx];
+    f[P::imx] = u[P::imx] * q[P::ivx] + q[P::ipr];
+    f[P::imy] = u[P::imy] * q[P::ivx];
+    f[P::imz] = u[P::imz] * q[P::ivx];
+    f[P::ien] = u[P::imx] - f[P::idn];
+    const double dn = q[P::idn], pr = q[P::ipr];
+    const double vx = q[P::ivx], vy = q[P::ivy], vz = q[P::ivz];
+    const double ww = dn + pr * s.rgammaxi;
+    const double c2 = s.gamma * pr / ww;
+    const double vv = (vx * vx + vy * vy) + vz * vz;
+    const double ss = c2 * (1.0 - vv) / (1.0 - c2);
+    const double fc = 1.0 + ss;
+    const double cc = sqrt(ss * (fc - vx * vx));
+    cm = T((vx - cc) / fc);
+    cp = T((vx + cc) / fc);
+    return;
+  } else {
   const T vv = (q[P::ivx] * q[P::ivx] + q[P::ivy] * q[P::ivy]) + q[P::ivz] * q[P::ivz];
   const T bb = (q[P::ibx] * q[P::ibx] + q[P::iby] * q[P::iby]) + q[P::ibz] * q[P::ibz];
   const T vb = (q[P::ivx] * q[P::ibx] + q[P::ivy] * q[P::iby]) + q[P::ivz] * q[P::ibz];
@@ -491,6 +538,7 @@ __device__ __forceinline__ void fluxspeed_sr(const SrConst& s, T cmax2, const T*
   }
   cm = T(dcm);
   cp = T(dcp);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -683,11 +731,183 @@ __device__ __forceinline__ int nr_iterate_2d(const SrConst& s, double mm, double
   return err >= s.tol ? C2P_NOCONV : 0;
 }
 
+
+// ---------------------------------------------------------------------------
+// primitive recovery of the relativistic HD system (equations.F90:3550-4566)
+// ---------------------------------------------------------------------------
+// nr_function_srhd_adi_1d (3853-3881): F(W) and dF/dW
+__device__ __forceinline__ void nr_function_srhd(const SrConst& s, double mm, double en,
+                                                 double dn, double w, double& f, double* df) {
+  const double gx = s.gammaxi;
+  const double vv = mm / (w * w), vm = 1.0 - vv, vs = sqrt(vm);
+  f = ((1.0 - gx * vm) * w + gx * dn * vs) - en;
+  if (df) *df = 1.0 - gx * (1.0 + (1.0 - dn / (vs * w)) * vv);
+}
+
+// nr_iterate_srhd_adi_1dw (3911-4091): brackets from sqrt(|m|^2 + D^2) +
+// gammaxi pmin and E + pmin (moved up while F keeps its sign), a secant
+// start, bracketed Newton.  Where the lower bracket is an exact root the
+// reference leaves v^2 undefined; here it is |m|^2 / W^2 of that root.
+__device__ __forceinline__ int nr_iterate_srhd_1dw(const SrConst& s, double mm, double en,
+                                                   double dn, double& w, double& vv) {
+  double wl = sqrt(mm + dn * dn), wu, fl, fu;
+  if (!(en > wl)) return C2P_FAIL;
+  wl = wl + s.gammaxi * s.pmin;
+  wu = en + s.pmin;
+  nr_function_srhd(s, mm, en, dn, wl, fl, nullptr);
+  if (!(fl < 0.0)) {
+    if (fl == 0.0) {
+      w = wl;
+      vv = mm / (wl * wl);
+      return 0;
+    }
+    return C2P_FAIL;
+  }
+  if (wu <= wl) wu = 2.0 * wl;
+  nr_function_srhd(s, mm, en, dn, wu, fu, nullptr);
+  int it = s.nrmax;
+  bool keep = fl * fu > 0.0;
+  while (keep) {
+    it = it - 1;
+    wl = wu;
+    fl = fu;
+    wu = 2.0 * wu;
+    nr_function_srhd(s, mm, en, dn, wu, fu, nullptr);
+    keep = (fl * fu > 0.0) && it > 0;
+  }
+  if (!(it > 0 && fu >= 0.0)) return C2P_FAIL;
+  w = wl - fl * (wu - wl) / (fu - fl);
+  it = s.nrmax;
+  int cn = s.nrext;
+  keep = true;
+  double err = 0.0;
+  while (keep) {
+    double f, df;
+    nr_function_srhd(s, mm, en, dn, w, f, &df);
+    if (f > fl && f < 0.0) { wl = w; fl = f; }
+    if (f < fu && f > 0.0) { wu = w; fu = f; }
+    const double dw = f / df;
+    w = w - dw;
+    err = fabs(dw / w);
+    if (err < s.tol) {
+      keep = cn > 0;
+      cn = cn - 1;
+    } else {
+      keep = it > 0;
+    }
+    if (w < wl || w > wu) w = 0.5 * (wl + wu);
+    it = it - 1;
+  }
+  vv = mm / (w * w);
+  return err >= s.tol ? C2P_NOCONV : 0;
+}
+
+// nr_iterate_srhd_adi_2dwv (4128-4330) and _2dwu (4360-4566): the 1D(W)
+// brackets, a secant start, then Newton on (W, v^2) or (W, u^2) with the
+// 2x2 system solved by its inverse; an iterate below the lower bracket or
+// with an unphysical speed fails
+template <bool WU>
+__device__ __forceinline__ int nr_iterate_srhd_2d(const SrConst& s, double mm, double en,
+                                                  double dn, double& w, double& vv) {
+  const double gx = s.gammaxi;
+  double wl = sqrt(mm + dn * dn) + gx * s.pmin, wu = en + s.pmin, fl, fu;
+  int it = s.nrmax;
+  bool keep = wl >= wu;
+  while (keep) {
+    wu = 2.0 * wu;
+    it = it - 1;
+    keep = (wl >= wu) && it > 0;
+  }
+  if (it <= 0) return C2P_FAIL;
+  nr_function_srhd(s, mm, en, dn, wl, fl, nullptr);
+  nr_function_srhd(s, mm, en, dn, wu, fu, nullptr);
+  keep = fl * fu > 0.0;
+  it = s.nrmax;
+  while (keep) {
+    wl = wu;
+    fl = fu;
+    wu = 2.0 * wu;
+    nr_function_srhd(s, mm, en, dn, wu, fu, nullptr);
+    it = it - 1;
+    keep = (fl * fu > 0.0) && it > 0;
+  }
+  if (it <= 0) return C2P_FAIL;
+  w = wl - fl * (wu - wl) / (fu - fl);
+  double x = WU ? mm / (w * w - mm) : mm / (w * w), err = 0.0;
+  it = s.nrmax;
+  int cn = s.nrext;
+  keep = true;
+  while (keep) {
+    const double gd = gx * dn, ww = w * w;
+    double f, g, dfw, dfx, dgw, dgx;
+    if constexpr (!WU) {
+      const double vm = 1.0 - x, vs = sqrt(vm), gv = 1.0 - gx * vm;
+      f = (gv * w - en) + gd * vs;
+      g = x * ww - mm;
+      dfw = gv;
+      dfx = gx * w - 0.5 * gd / vs;
+      dgw = 2.0 * x * w;
+      dgx = ww;
+    } else {
+      const double up = 1.0 + x, gm = sqrt(up);
+      f = ((up - gx) * w - up * en) + gm * gd;
+      g = x * ww - up * mm;
+      dfw = up - gx;
+      dfx = (w - en) + 0.5 * gd / gm;
+      dgw = 2.0 * x * w;
+      dgx = ww - mm;
+    }
+    const double det = dfw * dgx - dfx * dgw;
+    const double jfw = dgx / det, jgw = -dfx / det, jfx = -dgw / det, jgx = dfw / det;
+    const double dw = f * jfw + g * jgw, dx = f * jfx + g * jgx;
+    w = w - dw;
+    x = x - dx;
+    if (w < wl) return C2P_FAIL;
+    if (WU ? x < 0.0 : (x < 0.0 || x >= 1.0)) return C2P_FAIL;
+    err = tmax(fabs(dw / w), fabs(dx));
+    if (err < s.tol) {
+      keep = cn > 0;
+      cn = cn - 1;
+    } else {
+      keep = it > 0;
+    }
+    it = it - 1;
+  }
+  vv = WU ? x / (1.0 + x) : x;
+  return err >= s.tol ? C2P_NOCONV : 0;
+}
+
+// cons2prim_srhd_adi (3550-3639): the pressure is w - E as recovered (no
+// floor); a failed recovery writes p = 0 and keeps the rest of q
+template <class P>
+__device__ __forceinline__ int cons2prim_srhd(const SrConst& s, const double* u, double* q) {
+  const double mm = (u[P::imx] * u[P::imx] + u[P::imy] * u[P::imy]) + u[P::imz] * u[P::imz];
+  const double en = u[P::ien] + u[P::idn];
+  const double dn = u[P::idn];
+  double w = 0.0, vv = 0.0;
+  const int st = s.solver == AMUN_C2P_2DWV   ? nr_iterate_srhd_2d<false>(s, mm, en, dn, w, vv)
+                 : s.solver == AMUN_C2P_2DWU ? nr_iterate_srhd_2d<true>(s, mm, en, dn, w, vv)
+                                             : nr_iterate_srhd_1dw(s, mm, en, dn, w, vv);
+  if (st & C2P_FAIL) {
+    q[P::ipr] = 0.0;
+    return st;
+  }
+  const double vs = sqrt(1.0 - vv);
+  q[P::idn] = dn * vs;
+  q[P::ivx] = u[P::imx] / w;
+  q[P::ivy] = u[P::imy] / w;
+  q[P::ivz] = u[P::imz] / w;
+  q[P::ipr] = w - en;
+  return st;
+}
+
 // cons2prim_srmhd_adi, equations.F90:4671-4790.  q must hold the previous
 // primitive state on entry: a cell whose bracketing fails gets p = 0 and
 // keeps the rest, as the reference leaves its q array.
 template <class P>
 __device__ __forceinline__ int cons2prim_sr(const SrConst& s, const double* u, double* q) {
+  if constexpr (!P::mhd) return cons2prim_srhd<P>(s, u, q);
+  else {
   const double mm = (u[P::imx] * u[P::imx] + u[P::imy] * u[P::imy]) + u[P::imz] * u[P::imz];
   const double mb = (u[P::imx] * u[P::ibx] + u[P::imy] * u[P::iby]) + u[P::imz] * u[P::ibz];
   const double bb = (u[P::ibx] * u[P::ibx] + u[P::iby] * u[P::iby]) + u[P::ibz] * u[P::ibz];
@@ -720,6 +940,7 @@ __device__ __forceinline__ int cons2prim_sr(const SrConst& s, const double* u, d
     st |= C2P_PNEG;
   }
   return st;
+  }
 }
 
 // riemann_srmhd_adi_hll (schemes.F90:5244-5355): the GLM pre-step, then HLL
@@ -754,6 +975,78 @@ __device__ __forceinline__ void riemann_sr_hll(const SrConst& s, T cmax, T cmax2
   }
 }
 
+// riemann_srhd_adi_hllc (schemes.F90:4775-4968, Mignone & Bodo 2005): the
+// HLL average state and flux, the contact speed as the smaller root of
+// F_E x^2 - (E + F_m) x + m = 0 of that state (in FP64), the star state on
+// the side of the contact; no root, a root outside (sl, sr) or a
+// non-positive star pressure keep the HLL flux
+template <class P, typename T>
+__device__ __forceinline__ void riemann_srhd_hllc(const SrConst& s, T* ql, T* qr, T* f) {
+  constexpr int nv = P::nv;
+  constexpr int idn = P::idn, imx = P::imx, imy = P::imy, imz = P::imz, ien = P::ien;
+  T ul[nv], ur[nv], fl[nv], fr[nv], wl[nv], wr[nv], uh[nv], fh[nv];
+  T clm, clp, crm, crp;
+  const T rgx = T(s.rgammaxi);
+  prim2cons_sr<P>(rgx, ql, ul);
+  prim2cons_sr<P>(rgx, qr, ur);
+  fluxspeed_sr<P>(s, T(0), ql, ul, fl, clm, clp);
+  fluxspeed_sr<P>(s, T(0), qr, ur, fr, crm, crp);
+  const T sl = tmin(clm, crm), sr = tmax(clp, crp);
+  if (sl >= T(0)) {
+#pragma unroll
+    for (int p = 0; p < nv; p++) f[p] = fl[p];
+    return;
+  }
+  if (sr <= T(0)) {
+#pragma unroll
+    for (int p = 0; p < nv; p++) f[p] = fr[p];
+    return;
+  }
+  const T srml = sr - sl;
+#pragma unroll
+  for (int p = 0; p < nv; p++) {
+    wl[p] = sl * ul[p] - fl[p];
+    wr[p] = sr * ur[p] - fr[p];
+    uh[p] = (wr[p] - wl[p]) / srml;
+    fh[p] = (sl * wr[p] - sr * wl[p]) / srml;
+    f[p] = fh[p];                           // the flux of every fallback
+  }
+  wl[ien] = wl[ien] + wl[idn];
+  wr[ien] = wr[ien] + wr[idn];
+  double a[3], x[2];
+  a[0] = double(uh[imx]);
+  a[1] = double(-((fh[imx] + uh[ien]) + uh[idn]));
+  a[2] = double(fh[ien] + fh[idn]);
+  if (alg_quadratic(a, x) < 1) return;
+  const T sm = T(x[0]);
+  if (sm <= sl || sm >= sr) return;
+  const T pr = fh[imx] - (fh[ien] + fh[idn]) * sm;
+  if (pr <= T(0)) return;
+  if (sm == T(0)) {
+    f[idn] = T(0);
+    f[imx] = pr;
+    f[imy] = T(0);
+    f[imz] = T(0);
+    f[ien] = T(0);
+    return;
+  }
+  const bool left = sm > T(0);
+  const T sx = left ? sl : sr;
+  const T* w = left ? wl : wr;
+  const T* uq = left ? ul : ur;
+  const T* fq = left ? fl : fr;
+  const T dv = sx - sm;
+  T us[nv];
+  us[idn] = w[idn] / dv;
+  us[imy] = w[imy] / dv;
+  us[imz] = w[imz] / dv;
+  us[ien] = (w[ien] + pr * sm) / dv;
+  us[imx] = (us[ien] + pr) * sm;
+  us[ien] = us[ien] - us[idn];
+#pragma unroll
+  for (int p = 0; p < nv; p++) f[p] = fq[p] + sx * (us[p] - uq[p]);
+}
+
 // riemann_srmhd_adi_hllc (schemes.F90:5382-5700, Mignone & Bodo 2006): the
 // HLL average state and flux, the contact speed as the smaller root of the
 // quadratic of the average state (solved in FP64), the star state of the
@@ -761,6 +1054,10 @@ __device__ __forceinline__ void riemann_sr_hll(const SrConst& s, T cmax, T cmax2
 template <class P, typename T>
 __device__ __forceinline__ void riemann_sr_hllc(const SrConst& s, T cmax, T cmax2, T* ql,
                                                 T* qr, T* f) {
+  if constexpr (!P::mhd) {
+    riemann_srhd_hllc<P, T>(s, ql, qr, f);
+    return;
+  } else {
   constexpr int nv = P::nv;
   constexpr int idn = P::idn, imx = P::imx, imy = P::imy, imz = P::imz, ien = P::ien;
   constexpr int ibx = P::ibx, iby = P::iby, ibz = P::ibz, ibp = P::ibp, ivx = P::ivx;
@@ -860,6 +1157,7 @@ __device__ __forceinline__ void riemann_sr_hllc(const SrConst& s, T cmax, T cmax
     us[ibp] = qq[ibp];
 #pragma unroll
     for (int p = 0; p < nv; p++) f[p] = fq[p] + sx * (us[p] - uq[p]);
+  }
   }
 }
 

@@ -16,7 +16,7 @@ module amun_dev_mod
   implicit none
 
   integer(c_int32_t), parameter :: AMUN_EQSYS_HD = 1, AMUN_EQSYS_MHD = 2
-  integer(c_int32_t), parameter :: AMUN_EQSYS_SRMHD = 4
+  integer(c_int32_t), parameter :: AMUN_EQSYS_SRHD = 3, AMUN_EQSYS_SRMHD = 4
   integer(c_int32_t), parameter :: AMUN_EOS_ADI = 1
   integer(c_int32_t), parameter :: AMUN_EOS_ISO = 2
   integer(c_int32_t), parameter :: AMUN_REC_TVD = 1, AMUN_REC_WENO5Z = 5
@@ -64,6 +64,7 @@ module amun_dev_mod
     integer(c_int32_t) :: glm_source_terms, reserved_cfg
     real(c_double)     :: viscosity, resistivity
     real(c_double)     :: csnd
+    integer(c_int32_t) :: states_4vec, reserved_cfg2
   end type amun_cfg
 
   interface

@@ -58,6 +58,8 @@ program amun_host
     cfg%eqsys = AMUN_EQSYS_HD
   case("mhd", "MHD", "magnetohydrodynamic", "MAGNETOHYDRODYNAMIC")
     cfg%eqsys = AMUN_EQSYS_MHD
+  case("srhd", "SRHD")
+    cfg%eqsys = AMUN_EQSYS_SRHD
   case("srmhd", "SRMHD")
     cfg%eqsys = AMUN_EQSYS_SRMHD
   case default
@@ -239,7 +241,7 @@ program amun_host
   fin     = get_str("input", "")
   fout    = get_str("output", "")
 
-  nv   = merge(5, 9, cfg%eqsys == AMUN_EQSYS_HD)
+  nv   = merge(5, 9, cfg%eqsys == AMUN_EQSYS_HD .or. cfg%eqsys == AMUN_EQSYS_SRHD)
   if (cfg%eos == AMUN_EOS_ISO) nv = nv - 1
   nc   = cfg%ncells
   ng   = cfg%nghosts
@@ -306,7 +308,7 @@ program amun_host
   write(*,"(a,es22.15)") "last dt = ", dt
   write(*,"(a,8es22.14)") "integrals = ", ints
   write(*,"(a,21es16.8)") "statistics = ", stats
-  if (cfg%eqsys == AMUN_EQSYS_SRMHD) then
+  if (cfg%eqsys == AMUN_EQSYS_SRMHD .or. cfg%eqsys == AMUN_EQSYS_SRHD) then
     iret = amun_dev_c2p_events(ctx, c2p)
     write(*,"(a,3(1x,i0))") "primitive recovery failed / p<=0 / not converged =", c2p
   end if
@@ -466,7 +468,7 @@ contains
     real(real64), intent(in) :: q(nx,ny,nz,nv)
     real(real64) :: ek(nx,ny,nz), g
     g = cfg%gamma
-    if (cfg%eqsys == AMUN_EQSYS_SRMHD) then
+    if (cfg%eqsys == AMUN_EQSYS_SRMHD .or. cfg%eqsys == AMUN_EQSYS_SRHD) then
       call cons_from_prim_sr(q)
       return
     end if
@@ -490,11 +492,29 @@ contains
     end if
   end subroutine cons_from_prim
 
-! prim2cons_srmhd_adi (equations.F90:4590-4653) on the global grid
+! prim2cons_srmhd_adi (equations.F90:4590-4653) and prim2cons_srhd_adi
+! (3478-3532) on the global grid
   subroutine cons_from_prim_sr(q)
     real(real64), intent(in) :: q(nx,ny,nz,nv)
     real(real64) :: vv, bb, vb, vm, wt, gx
     gx = (cfg%gamma - 1.0d0) / cfg%gamma
+    if (nv == 5) then
+      do k = 1, nz
+        do j = 1, ny
+          do i = 1, nx
+            vv = (q(i,j,k,2)**2 + q(i,j,k,3)**2) + q(i,j,k,4)**2
+            vm = 1.0d0 - vv
+            wt = (q(i,j,k,1) + q(i,j,k,5) / gx) / vm
+            ug(i,j,k,1) = q(i,j,k,1) / sqrt(vm)
+            do p = 2, 4
+              ug(i,j,k,p) = wt * q(i,j,k,p)
+            end do
+            ug(i,j,k,5) = (wt - q(i,j,k,5)) - ug(i,j,k,1)
+          end do
+        end do
+      end do
+      return
+    end if
     do k = 1, nz
       do j = 1, ny
         do i = 1, nx
@@ -516,8 +536,8 @@ contains
     end do
   end subroutine cons_from_prim_sr
 
-! relativistic magnetised blast: rho = 1, v = 0, p = 1 for r < 0.1 else
-! 0.01, Bx = 0.5, psi = 0
+! relativistic (magnetised) blast: rho = 1, v = 0, p = 1 for r < 0.1 else
+! 0.01, Bx = 0.5 and psi = 0 with a field
   subroutine setup_srblast()
     real(real64), allocatable :: q(:,:,:,:)
     real(real64) :: x, y, dx, dy
@@ -531,8 +551,8 @@ contains
           x = cfg%xmin + (i - 0.5d0) * dx
           y = cfg%ymin + (j - 0.5d0) * dy
           q(i,j,k,1) = 1.0d0
-          q(i,j,k,5) = 0.5d0
-          q(i,j,k,9) = merge(1.0d0, 0.01d0, sqrt(x * x + y * y) < 0.1d0)
+          if (nv == 9) q(i,j,k,5) = 0.5d0
+          q(i,j,k,nv) = merge(1.0d0, 0.01d0, sqrt(x * x + y * y) < 0.1d0)
         end do
       end do
     end do

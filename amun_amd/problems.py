@@ -48,10 +48,21 @@ def cell_centres(p: Params):
 
 
 def prim_to_cons(p: Params, q: np.ndarray) -> np.ndarray:
-    """prim2cons of equations.F90:1762 (HD) / 2815 (MHD) / 4590 (SRMHD) /
+    """prim2cons of equations.F90:1762 (HD) / 2815 (MHD) / 3478 (SRHD) / 4590 (SRMHD) /
     1360, 2220 (isothermal HD / MHD, 4 or 8 variables), array form."""
     g = p.gamma
     u = np.empty_like(q)
+    if EQSYS[p.equation_system.lower()] == 3:
+        # prim2cons_srhd_adi (equations.F90:3478-3532)
+        gammaxi = (g - 1.0) / g
+        vv = (q[1] * q[1] + q[2] * q[2]) + q[3] * q[3]
+        vm = 1.0 - vv
+        vs = np.sqrt(vm)
+        ww = (q[0] + q[4] / gammaxi) / vm
+        u[0] = q[0] / vs
+        u[1:4] = ww * q[1:4]
+        u[4] = (ww - q[4]) - u[0]
+        return u
     if EQSYS[p.equation_system.lower()] == 4:
         gammaxi = (g - 1.0) / g
         vv = (q[1] * q[1] + q[2] * q[2]) + q[3] * q[3]

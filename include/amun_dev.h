@@ -35,6 +35,7 @@ extern "C" {
 /* equation systems: equations.F90:319-830 (parameter "equation_system") */
 #define AMUN_EQSYS_HD      1
 #define AMUN_EQSYS_MHD     2
+#define AMUN_EQSYS_SRHD    3   /* special-relativistic HD, equations.F90:537-600 */
 #define AMUN_EQSYS_SRMHD   4   /* special-relativistic MHD, equations.F90:673-760 */
 /* equations of state: equations.F90:345-395 (parameter "equation_of_state") */
 #define AMUN_EOS_ADI       1
@@ -165,6 +166,11 @@ typedef struct amun_cfg {
   /* "csnd": the sound speed of the isothermal systems (equations.F90:176,
    * 847-851; 1 by default) */
   double  csnd;
+  /* "state_variables" of the relativistic systems (schemes.F90:133, 383-400):
+   * 0 = primitive (default), 1 = "4vec" (reconstruction of rho W and the
+   * four-velocity; refused with 600 on the device path) */
+  int32_t states_4vec;
+  int32_t reserved_cfg2;
 } amun_cfg;
 
 /* Refinement criterion of the leaf blocks (initialize_refinement,

@@ -64,6 +64,7 @@ typedef struct {
   int idn, ivx, ivy, ivz, imx, imy, imz, ibx, iby, ibz, ibp, ipr, ien;
   int    mhd, sr;          /* magnetic field present / relativistic system   */
   int    iso;              /* isothermal equation of state (no energy)       */
+  int    srhd;             /* special-relativistic HD (o->sr is set too)      */
   double csnd, csnd2;      /* isothermal sound speed and its square          */
   double gammam1, gammam1i, gammaxi;
   double cflx;             /* cfl as adjusted by initialize_evolution        */
@@ -120,6 +121,14 @@ static int oc_setup(ocfg* o, const amun_cfg* c)
     else o->ibx = o->iby = o->ibz = o->ibp = -1;
     o->ipr = o->ien = -1;
   } else if (c->eqsys == AMUN_EQSYS_HD) {
+    o->nv = 5;
+    o->idn = 0; o->ivx = 1; o->ivy = 2; o->ivz = 3;
+    o->imx = 1; o->imy = 2; o->imz = 3;
+    o->ibx = o->iby = o->ibz = o->ibp = -1;
+    o->ipr = o->ien = 4;
+  } else if (c->eqsys == AMUN_EQSYS_SRHD) {
+    /* equations.F90:537-600: the special-relativistic HD system */
+    o->sr = o->srhd = 1;
     o->nv = 5;
     o->idn = 0; o->ivx = 1; o->ivy = 2; o->ivz = 3;
     o->imx = 1; o->imy = 2; o->imz = 3;
@@ -1958,6 +1967,250 @@ static void cons2prim_sr(ocfg* o, const double* u, double* q)
   }
 }
 
+
+/* ---------------------------------------------------------------------- */
+/* the special-relativistic HD system (equations.F90:3478-4566)            */
+/* ---------------------------------------------------------------------- */
+/* prim2cons_srhd_adi, equations.F90:3478-3532 */
+static void prim2cons_srhd(const ocfg* o, const double* q, double* u)
+{
+  double vv, vm, vs, ww;
+  vv = (q[o->ivx] * q[o->ivx] + q[o->ivy] * q[o->ivy]) + q[o->ivz] * q[o->ivz];
+  vm = 1.0 - vv;
+  vs = sqrt(vm);
+  ww = (q[o->idn] + q[o->ipr] / o->gammaxi) / vm;
+  u[o->idn] = q[o->idn] / vs;
+  u[o->imx] = ww * q[o->ivx];
+  u[o->imy] = ww * q[o->ivy];
+  u[o->imz] = ww * q[o->ivz];
+  u[o->ien] = (ww - q[o->ipr]) - u[o->idn];
+}
+
+/* nr_function_srhd_adi_1d, equations.F90:3853-3881 */
+static void nr_function_srhd(const ocfg* o, double mm, double en, double dn, double w,
+                             double* f, double* df)
+{
+  const double gx = o->gammaxi;
+  double vv, vm, vs;
+  vv = mm / (w * w);
+  vm = 1.0 - vv;
+  vs = sqrt(vm);
+  *f = ((1.0 - gx * vm) * w + gx * dn * vs) - en;
+  if (df) *df = 1.0 - gx * (1.0 + (1.0 - dn / (vs * w)) * vv);
+}
+
+/* nr_iterate_srhd_adi_1dw, equations.F90:3911-4091: brackets from
+ * sqrt(|m|^2 + D^2) + gammaxi pmin and E + pmin (doubled until the signs
+ * differ), a secant start and bracketed Newton.  Where the lower bracket is
+ * an exact root the reference leaves v^2 undefined; here it is |m|^2 / W^2
+ * of that root. */
+static int nr_iterate_srhd_1dw(const ocfg* o, double mm, double en, double dn, double* w,
+                               double* vv, int* noconv)
+{
+  const double tol = o->c.tolerance, pmin = o->c.pmin, gx = o->gammaxi;
+  double wl, wu, fl, fu, f, df, dw, err = 0.0;
+  int it, cn, keep;
+  *noconv = 0;
+  wl = sqrt(mm + dn * dn);
+  if (!(en > wl)) return 0;
+  wl = wl + gx * pmin;
+  wu = en + pmin;
+  nr_function_srhd(o, mm, en, dn, wl, &fl, NULL);
+  if (fl < 0.0) {
+    if (wu <= wl) wu = 2.0 * wl;
+    nr_function_srhd(o, mm, en, dn, wu, &fu, NULL);
+    it = o->c.nr_maxit;
+    keep = fl * fu > 0.0;
+    while (keep) {
+      it = it - 1;
+      wl = wu;
+      fl = fu;
+      wu = 2.0 * wu;
+      nr_function_srhd(o, mm, en, dn, wu, &fu, NULL);
+      keep = (fl * fu > 0.0) && it > 0;
+    }
+    if (!(it > 0 && fu >= 0.0)) return 0;
+    *w = wl - fl * (wu - wl) / (fu - fl);
+    keep = 1;
+    it = o->c.nr_maxit;
+    cn = o->c.nr_extra;
+    while (keep) {
+      nr_function_srhd(o, mm, en, dn, *w, &f, &df);
+      if (f > fl && f < 0.0) { wl = *w; fl = f; }
+      if (f < fu && f > 0.0) { wu = *w; fu = f; }
+      dw = f / df;
+      *w = *w - dw;
+      err = fabs(dw / *w);
+      if (err < tol) {
+        keep = cn > 0;
+        cn = cn - 1;
+      } else {
+        keep = it > 0;
+      }
+      if (*w < wl || *w > wu) *w = 0.5 * (wl + wu);
+      it = it - 1;
+    }
+    if (err >= tol) *noconv = 1;
+    *vv = mm / (*w * *w);
+    return 1;
+  }
+  if (fl == 0.0) {
+    *w = wl;
+    *vv = mm / (wl * wl);
+    return 1;
+  }
+  return 0;
+}
+
+/* nr_iterate_srhd_adi_2dwv (equations.F90:4128-4330) and _2dwu (4360-4566):
+ * brackets as the 1D(W) solver's (the lower one fixed, the upper doubled
+ * while not above it, then both moved up while the signs agree), a secant
+ * start, then Newton on (W, v^2) or (W, u^2); an iterate below the lower
+ * bracket or with an unphysical speed fails */
+static int nr_iterate_srhd_2d(const ocfg* o, int wu_form, double mm, double en, double dn,
+                              double* w, double* vv, int* noconv)
+{
+  const double tol = o->c.tolerance, pmin = o->c.pmin, gx = o->gammaxi;
+  double wl, wu, fl, fu, x, ww, f, g, dfw, dfx, dgw, dgx, det, jfw, jgw, jfx, jgx, dw, dx;
+  double err = 0.0;
+  int it, cn, keep;
+  *noconv = 0;
+  wl = sqrt(mm + dn * dn) + gx * pmin;
+  wu = en + pmin;
+  keep = wl >= wu;
+  it = o->c.nr_maxit;
+  while (keep) {
+    wu = 2.0 * wu;
+    it = it - 1;
+    keep = (wl >= wu) && it > 0;
+  }
+  if (it <= 0) return 0;
+  nr_function_srhd(o, mm, en, dn, wl, &fl, NULL);
+  nr_function_srhd(o, mm, en, dn, wu, &fu, NULL);
+  keep = fl * fu > 0.0;
+  it = o->c.nr_maxit;
+  while (keep) {
+    wl = wu;
+    fl = fu;
+    wu = 2.0 * wu;
+    nr_function_srhd(o, mm, en, dn, wu, &fu, NULL);
+    it = it - 1;
+    keep = (fl * fu > 0.0) && it > 0;
+  }
+  if (it <= 0) return 0;
+  *w = wl - fl * (wu - wl) / (fu - fl);
+  x = wu_form ? mm / (*w * *w - mm) : mm / (*w * *w);
+  keep = 1;
+  it = o->c.nr_maxit;
+  cn = o->c.nr_extra;
+  while (keep) {
+    const double gd = gx * dn;
+    ww = *w * *w;
+    if (!wu_form) {
+      const double vm = 1.0 - x, vs = sqrt(vm), gv = 1.0 - gx * vm;
+      f = (gv * *w - en) + gd * vs;
+      g = x * ww - mm;
+      dfw = gv;
+      dfx = gx * *w - 0.5 * gd / vs;
+      dgw = 2.0 * x * *w;
+      dgx = ww;
+    } else {
+      const double up = 1.0 + x, gm = sqrt(up);
+      f = ((up - gx) * *w - up * en) + gm * gd;
+      g = x * ww - up * mm;
+      dfw = up - gx;
+      dfx = (*w - en) + 0.5 * gd / gm;
+      dgw = 2.0 * x * *w;
+      dgx = ww - mm;
+    }
+    det = dfw * dgx - dfx * dgw;
+    jfw = dgx / det;
+    jgw = -dfx / det;
+    jfx = -dgw / det;
+    jgx = dfw / det;
+    dw = f * jfw + g * jgw;
+    dx = f * jfx + g * jgx;
+    *w = *w - dw;
+    x = x - dx;
+    if (*w < wl) return 0;
+    if (wu_form ? x < 0.0 : (x < 0.0 || x >= 1.0)) return 0;
+    err = fmax_r(fabs(dw / *w), fabs(dx));
+    if (err < tol) {
+      keep = cn > 0;
+      cn = cn - 1;
+    } else {
+      keep = it > 0;
+    }
+    it = it - 1;
+  }
+  *vv = wu_form ? x / (1.0 + x) : x;
+  if (err >= tol) *noconv = 1;
+  return 1;
+}
+
+/* cons2prim_srhd_adi, equations.F90:3550-3639.  On failure only the
+ * pressure is written (0), the other entries of q keep what they held. */
+static void cons2prim_srhd(ocfg* o, const double* u, double* q)
+{
+  double mm, en, dn, w = 0.0, vv = 0.0, vm, vs;
+  int info, noconv;
+  mm = (u[o->imx] * u[o->imx] + u[o->imy] * u[o->imy]) + u[o->imz] * u[o->imz];
+  en = u[o->ien] + u[o->idn];
+  dn = u[o->idn];
+  if (o->c.primitive_solver == AMUN_C2P_2DWV || o->c.primitive_solver == AMUN_C2P_2DWU)
+    info = nr_iterate_srhd_2d(o, o->c.primitive_solver == AMUN_C2P_2DWU, mm, en, dn, &w, &vv,
+                              &noconv);
+  else
+    info = nr_iterate_srhd_1dw(o, mm, en, dn, &w, &vv, &noconv);
+  if (noconv) o->c2p_noconv++;
+  if (info) {
+    vm = 1.0 - vv;
+    vs = sqrt(vm);
+    q[o->idn] = dn * vs;
+    q[o->ivx] = u[o->imx] / w;
+    q[o->ivy] = u[o->imy] / w;
+    q[o->ivz] = u[o->imz] / w;
+    q[o->ipr] = w - en;
+  } else {
+    o->c2p_fail++;
+    q[o->ipr] = 0.0;
+  }
+}
+
+/* fluxspeed_srhd_adi, equations.F90:3665-3731 */
+static void fluxspeed_srhd(const ocfg* o, const double* q, const double* u, double* f,
+                           double* cm, double* cp)
+{
+  double ww, c2, vv, ss, fc, cc;
+  f[o->idn] = u[o->idn] * q[o->ivx];
+  f[o->imx] = u[o->imx] * q[o->ivx] + q[o->ipr];
+  f[o->imy] = u[o->imy] * q[o->ivx];
+  f[o->imz] = u[o->imz] * q[o->ivx];
+  f[o->ien] = u[o->imx] - f[o->idn];
+  ww = q[o->idn] + q[o->ipr] / o->gammaxi;
+  c2 = o->c.gamma * q[o->ipr] / ww;
+  vv = (q[o->ivx] * q[o->ivx] + q[o->ivy] * q[o->ivy]) + q[o->ivz] * q[o->ivz];
+  ss = c2 * (1.0 - vv) / (1.0 - c2);
+  fc = 1.0 + ss;
+  cc = sqrt(ss * (fc - q[o->ivx] * q[o->ivx]));
+  *cm = (q[o->ivx] - cc) / fc;
+  *cp = (q[o->ivx] + cc) / fc;
+}
+
+/* maxspeed_srhd_adi, equations.F90:3746-3820, for one cell */
+static double maxspeed_srhd(const ocfg* o, const double* q)
+{
+  double vv, v, ww, c2, ss, fc, cc;
+  vv = (q[o->ivx] * q[o->ivx] + q[o->ivy] * q[o->ivy]) + q[o->ivz] * q[o->ivz];
+  v = sqrt(vv);
+  ww = q[o->idn] + q[o->ipr] / o->gammaxi;
+  c2 = o->c.gamma * q[o->ipr] / ww;
+  ss = c2 * (1.0 - vv) / (1.0 - c2);
+  fc = 1.0 + ss;
+  cc = sqrt(ss * (fc - vv));
+  return (v + cc) / fc;
+}
+
 /* fluxspeed_srmhd_adi, equations.F90:4821-5101 (Mignone & Bodo 2006 fluxes;
  * speeds from the quartic, quadratic or bi-quadratic characteristic
  * polynomial) */
@@ -2067,6 +2320,7 @@ static void fluxspeed_sr(const ocfg* o, const double* q, const double* u,
 static void prim2cons(const ocfg* o, const double* q, double* u)
 {
   double ek, ei, em;
+  if (o->srhd) { prim2cons_srhd(o, q, u); return; }
   if (o->sr) { prim2cons_sr(o, q, u); return; }
   u[o->idn] = q[o->idn];
   u[o->imx] = q[o->idn] * q[o->ivx];
@@ -2098,6 +2352,7 @@ static void prim2cons(const ocfg* o, const double* q, double* u)
 static void cons2prim(ocfg* o, const double* u, double* q)
 {
   double ek, ei, em;
+  if (o->srhd) { cons2prim_srhd(o, u, q); return; }
   if (o->sr) { cons2prim_sr(o, u, q); return; }
   q[o->idn] = u[o->idn];
   q[o->ivx] = u[o->imx] / u[o->idn];
@@ -2130,6 +2385,7 @@ static void fluxspeed(const ocfg* o, const double* q, const double* u,
                       double* f, double* cm, double* cp)
 {
   const double g = o->c.gamma;
+  if (o->srhd) { fluxspeed_srhd(o, q, u, f, cm, cp); return; }
   if (o->sr) { fluxspeed_sr(o, q, u, f, cm, cp); return; }
   if (o->iso && !o->mhd) {              /* fluxspeed_hd_iso 1484-1542 */
     f[o->idn] = u[o->imx];
@@ -2206,6 +2462,7 @@ static void fluxspeed(const ocfg* o, const double* q, const double* u,
 static double maxspeed_cell(const ocfg* o, const double* q)
 {
   double vv, v, c, bb;
+  if (o->srhd) return maxspeed_srhd(o, q);
   if (o->sr) return 1.0;
   vv = (q[o->ivx] * q[o->ivx] + q[o->ivy] * q[o->ivy]) + q[o->ivz] * q[o->ivz];
   v  = sqrt(vv);
@@ -3033,7 +3290,7 @@ static int riemann_check(const amun_cfg* c)
       c->eos == AMUN_EOS_ISO) return 0;
   /* SRMHD: "hllc" selects riemann_srmhd_adi_hllc, any other name HLL
    * (schemes.F90:479-498) */
-  if (c->eqsys == AMUN_EQSYS_SRMHD) return 0;
+  if (c->eqsys == AMUN_EQSYS_SRMHD || c->eqsys == AMUN_EQSYS_SRHD) return 0;
   return 300;
 }
 
@@ -3565,6 +3822,89 @@ static void riemann_mhd_iso_hlldm(const ocfg* o, double* ql, double* qr, double*
 #undef STAR_R
 }
 
+/* riemann_srhd_adi_hllc, schemes.F90:4775-4968 (Mignone & Bodo 2005): the
+ * HLL average state and flux, the contact speed as the smaller root of
+ * F_E x^2 - (E + F_m) x + m = 0 of that state, the star state on the side
+ * of the contact; no root, a root outside (sl, sr) or a non-positive star
+ * pressure return the HLL flux */
+static void riemann_srhd_hllc(const ocfg* o, double* ql, double* qr, double* f)
+{
+  const int idn = o->idn, imx = o->imx, imy = o->imy, imz = o->imz, ien = o->ien;
+  const int nv = o->nv;
+  double ul[NVMAX], ur[NVMAX], fl[NVMAX], fr[NVMAX], wl[NVMAX], wr[NVMAX];
+  double uh[NVMAX], fh[NVMAX], us[NVMAX], a[3], x[2];
+  double clm, clp, crm, crp, sl, sr, srml, sm, pr, dv;
+  int p, nr;
+  prim2cons(o, ql, ul);
+  prim2cons(o, qr, ur);
+  fluxspeed(o, ql, ul, fl, &clm, &clp);
+  fluxspeed(o, qr, ur, fr, &crm, &crp);
+  sl = fmin_r(clm, crm);
+  sr = fmax_r(clp, crp);
+  if (sl >= 0.0) {
+    for (p = 0; p < nv; p++) f[p] = fl[p];
+    return;
+  }
+  if (sr <= 0.0) {
+    for (p = 0; p < nv; p++) f[p] = fr[p];
+    return;
+  }
+  srml = sr - sl;
+  for (p = 0; p < nv; p++) {
+    wl[p] = sl * ul[p] - fl[p];
+    wr[p] = sr * ur[p] - fr[p];
+  }
+  for (p = 0; p < nv; p++) {
+    uh[p] = (wr[p] - wl[p]) / srml;
+    fh[p] = (sl * wr[p] - sr * wl[p]) / srml;
+  }
+  wl[ien] = wl[ien] + wl[idn];
+  wr[ien] = wr[ien] + wr[idn];
+  a[0] = uh[imx];
+  a[1] = -((fh[imx] + uh[ien]) + uh[idn]);
+  a[2] = fh[ien] + fh[idn];
+  nr = alg_quadratic(a, x);
+  if (nr < 1) {
+    for (p = 0; p < nv; p++) f[p] = fh[p];
+    return;
+  }
+  sm = x[0];
+  if (sm <= sl || sm >= sr) {
+    for (p = 0; p < nv; p++) f[p] = fh[p];
+    return;
+  }
+  pr = fh[imx] - (fh[ien] + fh[idn]) * sm;
+  if (pr <= 0.0) {
+    for (p = 0; p < nv; p++) f[p] = fh[p];
+    return;
+  }
+  if (sm > 0.0) {
+    dv = sl - sm;
+    us[idn] = wl[idn] / dv;
+    us[imy] = wl[imy] / dv;
+    us[imz] = wl[imz] / dv;
+    us[ien] = (wl[ien] + pr * sm) / dv;
+    us[imx] = (us[ien] + pr) * sm;
+    us[ien] = us[ien] - us[idn];
+    for (p = 0; p < nv; p++) f[p] = fl[p] + sl * (us[p] - ul[p]);
+  } else if (sm < 0.0) {
+    dv = sr - sm;
+    us[idn] = wr[idn] / dv;
+    us[imy] = wr[imy] / dv;
+    us[imz] = wr[imz] / dv;
+    us[ien] = (wr[ien] + pr * sm) / dv;
+    us[imx] = (us[ien] + pr) * sm;
+    us[ien] = us[ien] - us[idn];
+    for (p = 0; p < nv; p++) f[p] = fr[p] + sr * (us[p] - ur[p]);
+  } else {
+    f[idn] = 0.0;
+    f[imx] = pr;
+    f[imy] = 0.0;
+    f[imz] = 0.0;
+    f[ien] = 0.0;
+  }
+}
+
 static void riemann(const ocfg* o, double* ql, double* qr, double* f)
 {
   if (o->iso) {
@@ -3575,6 +3915,13 @@ static void riemann(const ocfg* o, double* ql, double* qr, double* f)
     else if (o->mhd && o->c.riemann_solver == AMUN_RS_HLLDM) riemann_mhd_iso_hlldm(o, ql, qr, f);
     else if (o->mhd && o->c.riemann_solver == AMUN_RS_ROE) riemann_mhd_iso_roe(o, ql, qr, f);
     else riemann_hll(o, ql, qr, f);
+    return;
+  }
+  if (o->srhd) {
+    /* schemes.F90:400-430: "hllc" selects riemann_srhd_adi_hllc, every
+     * other name HLL */
+    if (o->c.riemann_solver == AMUN_RS_HLLC) riemann_srhd_hllc(o, ql, qr, f);
+    else riemann_hll(o, ql, qr, f);               /* riemann_srhd_adi_hll */
     return;
   }
   if (o->sr) {

@@ -1046,6 +1046,126 @@ def main_iso():
     np.savez_compressed(os.path.join(HERE, "stats_iso_ref.npz"), **out)
 
 
+# (name, ndims, reconstruction, Riemann solver, primitive solver, steps)
+SRHD_RUNS = (("srhd_hd2d_tvd_hllc_1dw_blast_20", 2, "tvd", "hllc", "1dw", 20),
+             ("srhd_hd2d_tvd_hll_2dwv_blast_20", 2, "tvd", "hll", "2dwv", 20),
+             ("srhd_hd2d_mp5_hllc_2dwu_blast_10", 2, "mp5", "hllc", "2dwu", 10),
+             ("srhd_hd3d_weno5z_hllc_1dw_noisy_4", 3, "weno5z", "hllc", "1dw", 4),
+             ("srhd_hd3d_tvd_hll_2dwu_noisy_4", 3, "tvd", "hll", "2dwu", 4))
+
+
+def srhd_states(rng, n):
+    """Random special-relativistic HD primitive states (n, 5): |v| < 0.99,
+    densities and pressures over decades; the first rows at rest."""
+    q = np.zeros((n, 5))
+    q[:, 0] = 10.0 ** rng.uniform(-2, 1, n)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    q[:, 1:4] = d * rng.uniform(0.0, 0.99, n)[:, None]
+    q[:, 4] = 10.0 ** rng.uniform(-3, 1, n)
+    q[: n // 16, 1:4] = 0.0
+    return q
+
+
+def srhd_params(nd, rec, rs, c2p):
+    return Params(ndims=nd, equation_system="srhd", reconstruction=rec,
+                  limiter="mc" if rec == "tvd" else "mm", riemann_solver=rs,
+                  time_advance="rk2" if nd == 2 else "rk3", gamma=4.0 / 3.0,
+                  cfl=0.4 if nd == 2 else 0.3, ncells=16 if nd == 2 else 8,
+                  nghosts=2 if rec == "tvd" else 4, xblocks=2, yblocks=2, zblocks=2,
+                  xmin=-0.5, xmax=0.5, ymin=-0.5, ymax=0.5, zmin=-0.5, zmax=0.5,
+                  tolerance=1.0e-12, nr_maxit=50, primitive_solver=c2p)
+
+
+def srhd_state(name, p, seed=6):
+    """Relativistic HD states: a cylindrical blast (rho = 1, p = 1 inside
+    r < 0.1, 0.01 outside, 1 % seeded density noise and velocities of 1e-3)
+    or smooth fully three-dimensional flows with |v| up to about 0.6."""
+    nz, ny, nx = p.shape
+    z, y, x = problems.cell_centres(p)
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    xi = [problems.uniform(seed, nz * ny * nx, stream=s).reshape(nz, ny, nx) - 0.5
+          for s in range(50, 55)]
+    q = np.zeros((5, nz, ny, nx))
+    tp = 2.0 * np.pi
+    if "blast" in name:
+        r = np.sqrt(X ** 2 + Y ** 2)
+        q[0] = 1.0 + 0.02 * xi[0]
+        q[1] = 2e-3 * xi[1]
+        q[2] = 2e-3 * xi[2]
+        q[4] = np.where(r < 0.1, 1.0, 0.01)
+    else:
+        a = 0.02
+        q[0] = 1.0 + 0.2 * np.sin(tp * (X + Y + Z)) + a * xi[0]
+        q[1] = 0.3 * np.sin(tp * Y) + 0.2 * np.cos(tp * Z) + a * xi[1]
+        q[2] = 0.3 * np.sin(tp * Z) + 0.2 * np.cos(tp * X) + a * xi[2]
+        q[3] = 0.3 * np.sin(tp * X) + 0.2 * np.cos(tp * Y) + a * xi[3]
+        q[4] = 1.0 + 0.1 * np.cos(tp * (X - Y + Z)) + a * xi[4]
+    return problems.prim_to_cons(p, q)
+
+
+def main_srhd():
+    """The special-relativistic HD system (equations.F90:3478-4566,
+    schemes.F90:4433-4968): equation-of-state vectors with the three
+    primitive solvers (random, at-rest and failing states), HLL and HLLC
+    through update_flux, runs of TVD, MP5 and WENO5-Z in 2D and 3D, and the
+    statistics file (own generator)."""
+    rng = np.random.default_rng(20261212)
+    out = {}
+    pe = Params(ndims=2, equation_system="srhd", riemann_solver="hll", gamma=4.0 / 3.0,
+                tolerance=1.0e-12, nr_maxit=50)
+    q = srhd_states(rng, 400)
+    u, q2, f, cm, cp = ol.ref_eqs(pe, q, cmax=0.8)
+    out.update(q=q, u=u, q2=q2, f=f, cm=cm, cp=cp, cmax=0.8,
+               params=np.array(pe.to_params_in()))
+    # conserved states for the primitive solvers: the physical ones, very
+    # fast flows (W up to ~70), and unphysical ones (E below sqrt(m^2 + D^2))
+    fast = srhd_states(rng, 32)
+    d = rng.normal(size=(32, 3))
+    fast[:, 1:4] = d / np.linalg.norm(d, axis=1)[:, None] * 0.9999
+    ub = np.concatenate([u, ol.ref_eqs(pe, fast, cmax=0.8)[0]])
+    ub[:16, 4] = -0.5 * ub[:16, 0]          # E + D below sqrt(m^2 + D^2)
+    ub[16:32, 1:4] *= 1.0 + 1e-3
+    for ps in ("1dw", "2dwv", "2dwu"):
+        pp = Params(ndims=2, equation_system="srhd", riemann_solver="hll", gamma=4.0 / 3.0,
+                    tolerance=1.0e-12, nr_maxit=50, primitive_solver=ps)
+        q0 = np.full_like(ub, -7.0)
+        qb, _ = ol.ref_cons2prim(pp, ub, q0)
+        out["ub"], out["q0"], out["qb_" + ps] = ub, q0, qb
+    np.savez_compressed(os.path.join(HERE, "unit_eqs_srhd.npz"), **out)
+    for rs in ("hll", "hllc"):
+        pf = Params(ndims=2, equation_system="srhd", reconstruction="tvd", limiter="zero",
+                    riemann_solver=rs, gamma=4.0 / 3.0, ncells=8, nghosts=2,
+                    tolerance=1.0e-12, nr_maxit=50)
+        qb = srhd_states(rng, 3 * 144).reshape(3, 1, 12, 12, 5)
+        f = ol.ref_update_flux(pf, qb, cmax=1.0)
+        np.savez_compressed(os.path.join(HERE, f"unit_flux_srhd_{rs}.npz"), q=qb, f=f,
+                            cmax=1.0, params=np.array(pf.to_params_in()))
+        print("unit_flux srhd", rs)
+    with open(os.path.join(HERE, "index.json")) as fh:
+        index = json.load(fh)
+    for name, nd, rec, rs, c2p, nsteps in SRHD_RUNS:
+        p = srhd_params(nd, rec, rs, c2p)
+        u0 = srhd_state(name, p)
+        u, tr = ol.ref_run(p, u0, nsteps)
+        sub = u[:, ::2, ::2, ::2] if nd == 3 else u[:, :, ::3, ::3]
+        np.savez_compressed(os.path.join(HERE, name + ".npz"),
+                            params=np.array(p.to_params_in()), ndims=nd, nsteps=nsteps,
+                            stride=2 if nd == 3 else 3, trace=tr, sub=sub, sha_u0=sha(u0),
+                            sha_u=sha(u))
+        index[name] = {"steps": nsteps, "sha_u": sha(u), "dt_last": float(tr[-1, 0])}
+        print(name, "dt", tr[-1, 0])
+    with open(os.path.join(HERE, "index.json"), "w") as fh:
+        json.dump(index, fh, indent=1)
+    p = srhd_params(2, "tvd", "hllc", "1dw")
+    p.xblocks, p.yblocks = 3, 2
+    u0 = srhd_state("blast", p, seed=3)
+    ints, st = ol.ref_statistics(p, u0, 3)
+    np.savez_compressed(os.path.join(HERE, "stats_srhd_ref.npz"), ints=ints, stats=st,
+                        params=np.array(p.to_params_in()), sha_u0=np.array(sha(u0)))
+    print("stats srhd", st[:6])
+
+
 def main():
     if not (ol.ref_available(2) and ol.ref_available(3)):
         sys.exit("reference binaries missing: run oracle/build_ref.sh first")
@@ -1061,6 +1181,9 @@ def main():
     if sys.argv[1:] == ["refine"]:
         main_refine()
         sys.exit(0)
+    if sys.argv[1:] == ["srhd"]:
+        main_srhd()
+        return
     if sys.argv[1:] == ["iso"]:
         main_iso()
         return

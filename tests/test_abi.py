@@ -128,6 +128,21 @@ def test_isothermal_configuration_carried():
     assert d.csnd == 1.0                             # equations.F90:176
 
 
+def test_four_vector_states_are_refused():
+    """state_variables = 4vec (the relativistic systems' four-velocity
+    reconstruction, schemes.F90:4481-4517) reaches amun_cfg and is refused
+    with 600; for the other systems the reference ignores it."""
+    p = Params.from_text("equation_system = srhd\nriemann_solver = hllc\n"
+                         "state_variables = 4vec\n", ndims=2)
+    assert "state_variables" not in p.extra
+    cfg = p.to_cfg()
+    assert cfg.states_4vec == 1
+    lib = solver.load_library()
+    h = ctypes.c_void_p()
+    assert lib.amun_dev_init(ctypes.byref(cfg), ctypes.byref(h)) == 600
+    assert b"4vec" in lib.amun_dev_last_error()
+
+
 def test_weno_needs_four_ghosts():
     p = c2_params(32)
     p.nghosts = 2
