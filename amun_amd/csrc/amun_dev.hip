@@ -33,6 +33,8 @@ struct amun_ctx {
   int nspart = 0;              // entries of spart (workgroups of >= 64 lanes)
   double* ugp = nullptr;       // MGP weights (RunConst::ugp points here)
   void* src = nullptr;         // GLM source products per stage (element size qsize)
+  void* vsrc = nullptr;        // viscous and resistive source products per stage
+  int vis = 0;                 // 1 viscosity, 2 resistivity (SweepArgs::vis)
   void* mlps = nullptr;        // multi-dimensional limiting: slopes of the interior cells
   void* mlpg = nullptr;        // and of the cells beyond the pencil ends (k_mlp)
   int qcur = 0;                // Q[qcur] holds the primitive state of U0
@@ -210,8 +212,8 @@ int validate(const amun_cfg& c, int* nv) {
     return fail(600, "state_variables = 4vec is not supported on the device path");
   if (c.glm_source_terms < 0 || c.glm_source_terms > 2)
     return fail(500, "glm_source_terms must be 0 (none), 1 (eglm) or 2 (heglm)");
-  if (c.viscosity != 0.0 || c.resistivity != 0.0)
-    return fail(600, "viscosity / resistivity sources are not supported on the device path");
+  if (!(c.viscosity >= 0.0) || !(c.resistivity >= 0.0))
+    return fail(500, "viscosity and resistivity must be non-negative");
   if (c.ncells < 4 || c.ncells > 64 || c.ncells % 4)
     return fail(500, "ncells must be a multiple of 4 in [4, 64] (4x4 brick layout)");
   // interior formulas at the faces of cells -1 and ncells (the reference's
@@ -334,6 +336,10 @@ RunConst make_rc(const amun_cfg& cfg) {
   rc.gammam1i = 1.0 / rc.gammam1;
   rc.csnd = cfg.csnd;
   rc.csnd2 = cfg.csnd * cfg.csnd;                 // equations.F90:849
+  // the viscous time-step term takes both coefficients whatever the
+  // system (evolution.F90:478-483)
+  rc.nu = cfg.viscosity;
+  rc.eta = cfg.resistivity;
   // initialize_evolution (evolution.F90:166-200): SSPRK(m,2) with m clamped
   // to [2, 9], SSPRK(4,3) and SSPRK(5,3) step at a multiple of the CFL
   // number, and the psi decay follows the scaled number
@@ -614,6 +620,11 @@ static int init_device(amun_ctx* c, const std::vector<int32_t>& nbr,
     const int gs = 3 * (nd == 3 ? rc.nc * rc.nc : rc.nc);
     ALLOC(c->mlps, (size_t)rc.nblk * nv * nd * rc.ncell * c->qsize);
     ALLOC(c->mlpg, (size_t)rc.nblk * nv * nd * gs * c->qsize);
+  }
+  c->vis = (cfg->viscosity > 0.0 ? 1 : 0) | (cfg->resistivity > 0.0 && has_field(*cfg) ? 2 : 0);
+  if (c->vis) {
+    // the nine viscous and resistive source products of every cell (k_visc_src)
+    ALLOC(c->vsrc, (size_t)rc.nblk * 9 * rc.ncell * c->qsize);
   }
   if (cfg->glm_source_terms > 0 && has_field(*cfg)) {
     // the eight GLM source products of every cell (k_glm_src)
@@ -1000,6 +1011,7 @@ void amun_dev_finalize(amun_ctx* c) {
   hipFree(c->rc.fsl);
   hipFree(c->ugp);
   hipFree(c->src);
+  hipFree(c->vsrc);
   hipFree(c->mlps);
   hipFree(c->mlpg);
   hipFree(c->rc.fsr);
@@ -1365,6 +1377,35 @@ static int glm_sources(amun_ctx* c, int qcur) {
   return 0;
 }
 
+// the viscous and resistive source products of the stage input Q[qcur]
+// (k_visc_src)
+static int visc_sources(amun_ctx* c, int qcur) {
+  const RunConst& rc = c->rc;
+  const long long n = (long long)rc.nblk * rc.ncell;
+  const dim3 grid((unsigned)((n + 255) / 256)), bl(256);
+  const int* blev = c->amr.on ? c->amr.blev : nullptr;
+  by_system(c->cfg, [&](auto tag) {
+    using P = decltype(tag);
+    auto go = [&](auto ndtag, auto ttag) {
+      constexpr int ND = decltype(ndtag)::value;
+      using T = decltype(ttag);
+      hipLaunchKernelGGL((k_visc_src<P, ND, T>), grid, bl, 0, c->stream,
+                         static_cast<const T*>(c->Q[qcur]), c->nbr, rc, c->rs, c->brun, blev,
+                         c->vis, static_cast<T*>(c->vsrc));
+    };
+    const bool f64 = c->qsize == sizeof(double);
+    if (rc.nd == 3) {
+      if (f64) go(std::integral_constant<int, 3>{}, double{});
+      else go(std::integral_constant<int, 3>{}, float{});
+    } else {
+      if (f64) go(std::integral_constant<int, 2>{}, double{});
+      else go(std::integral_constant<int, 2>{}, float{});
+    }
+  });
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
 static int enqueue_step(amun_ctx* c, double dtnext, int slot, int qcur, hipEvent_t* ev,
                         double* rec = nullptr) {
   Dispatch& d = dispatch_of(c);
@@ -1394,7 +1435,13 @@ static int enqueue_step(amun_ctx* c, double dtnext, int slot, int qcur, hipEvent
       int r = glm_sources(c, qcur);
       if (r) return r;
     }
+    if (c->vsrc) {
+      int r = visc_sources(c, qcur);
+      if (r) return r;
+    }
     SweepArgs a;
+    a.vsrc = c->vsrc;
+    a.vis = c->vis;
     a.mlps = c->mlps;
     a.mlpg = c->mlpg;
     a.src = c->src;
@@ -2092,6 +2139,7 @@ static int tape_check(amun_ctx* c) {
   if (!c->disp.has_adj)
     return fail(600, "gradients need TVD reconstruction and the HD or MHD system");
   if (c->src) return fail(600, "gradients do not include the GLM source terms");
+  if (c->vsrc) return fail(600, "gradients do not include the viscous and resistive sources");
   if (c->cfg.mlp_limiting)
     return fail(600, "gradients do not include the multi-dimensional limiting");
   return 0;

@@ -95,6 +95,7 @@ struct RunConst {
   int prol_limiter;                 // prolongation of multi-level ghost fills
   double gamma, gammam1, gammam1i;
   double csnd, csnd2;               // isothermal sound speed and its square
+  double nu, eta;                   // viscosity and resistivity (sources.F90:52-58)
   double cfl, eps, decay;
   double cgp[5];                    // Gaussian-process weights (prepare_gp)
   double kappa, kbeta;              // MP limiter of the GP reconstruction
@@ -118,7 +119,7 @@ __device__ __forceinline__ double dt_of(const RunConst& rc, const RunState* R,
                                        int slot, double dtnext) {
   if (R->fix_dt > 0.0) return R->fix_dt;
   double cmax = cmax_of(R, slot);
-  double dtn = rc.cfl * R->dxmin / fmax(cmax + 2.0 * 0.0 / R->dxmin, kTiny);
+  double dtn = rc.cfl * R->dxmin / fmax(cmax + 2.0 * fmax(rc.nu, rc.eta) / R->dxmin, kTiny);
   return dtnext > 0.0 ? fmin(dtn, dtnext) : R->dt;
 }
 
@@ -270,6 +271,10 @@ struct SweepArgs {
   // k_glm_src, and the kind (1 EGLM, 2 HEGLM)
   const void* src;
   int glm;
+  // viscous and resistive source terms (null when off): the nine products
+  // per cell of k_visc_src, and which are on (1 viscosity, 2 resistivity)
+  const void* vsrc;
+  int vis;
   // multi-dimensional limiting (null when off): the slopes of k_mlp for the
   // interior cells (S) and the cells beyond the pencil ends (G)
   const void* mlps;
@@ -1226,6 +1231,19 @@ __global__ __launch_bounds__(256, sweep_min_waves<T>()) void k_sweep(SweepArgs a
             if constexpr (DIR == 0) du = -dxi * dfx;
             else if constexpr (kDin) du = sd[(((c - 1) & 3) * NV + v) * wg + tid] - dxi * dfx;
             else du = dpre[v] - dxi * dfx;
+            if constexpr (kLast) {
+              // viscous source terms after the completed increment
+              // (update_sources, sources.F90:340-467; products of k_visc_src)
+              if (a.vsrc && (a.vis & 1)) {
+#pragma clang fp contract(off)
+                const T* V = static_cast<const T*>(a.vsrc) + (size_t)b * 9 * rc.ncell + (bt - bb);
+                const size_t n1 = (size_t)rc.ncell;
+                if (v == P::imx) du = du + V[0];
+                if (v == P::imy) du = du + V[n1];
+                if (v == P::imz) du = du + V[2 * n1];
+                if (v == P::ien) du = du + V[3 * n1];
+              }
+            }
             if constexpr (kLast && P::mhd) {
               // GLM source terms after the completed increment
               // (update_sources, sources.F90:479-539; products from k_glm_src)
@@ -1243,6 +1261,16 @@ __global__ __launch_bounds__(256, sweep_min_waves<T>()) void k_sweep(SweepArgs a
                   if (v == P::ibz) du = du - Sp[6 * n1];
                   if (v == P::ien) du = du - Sp[7 * n1];
                 }
+              }
+              // resistive source terms after the GLM ones (sources.F90:543-598)
+              if (a.vsrc && (a.vis & 2)) {
+#pragma clang fp contract(off)
+                const T* V = static_cast<const T*>(a.vsrc) + (size_t)b * 9 * rc.ncell + (bt - bb);
+                const size_t n1 = (size_t)rc.ncell;
+                if (v == P::ibx) du = du + V[4 * n1];
+                if (v == P::iby) du = du + V[5 * n1];
+                if (v == P::ibz) du = du + V[6 * n1];
+                if (v == P::ien) du = (du + V[7 * n1]) + V[8 * n1];
               }
             }
             if constexpr (!kLast) {

@@ -151,12 +151,13 @@ typedef struct amun_cfg {
   /* "mlp_limiting" (1 = on): the multi-dimensional limiting process after
    * every reconstruction (interpolations.F90:204, 844, 955, 1140,
    * 1171-1321); "glm_source_terms" eglm/heglm = 1/2 (sources.F90:128-141,
-   * 479-539) for the magnetic systems.  Terms outside the device path, which
-   * amun_dev_init refuses with 600 when not at their default (off / 0):
+   * 479-539) for the magnetic systems; the "viscosity" and "resistivity"
+   * coefficients (>= 0; sources.F90:145-149, 340-598) of the viscous and
+   * resistive source terms and their dt term (evolution.F90:478-483).
+   * Outside the device path, refused with 600 when on:
    * "fix_unphysical_cells" of update_primitive_variables (equations.F90:905,
-   * 1168), and the "viscosity" and "resistivity" sources with their dt term
-   * (sources.F90:145-149, evolution.F90:478-483).  Gradients
-   * (amun_dev_tape_*) refuse mlp_limiting and the GLM source terms. */
+   * 1168).  Gradients (amun_dev_tape_*) refuse mlp_limiting and the source
+   * terms. */
   int32_t mlp_limiting;
   int32_t fix_unphysical_cells;
   int32_t glm_source_terms;

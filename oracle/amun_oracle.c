@@ -4396,7 +4396,8 @@ static void update_increment(const ocfg* o, const double* f, double* du)
   update_increment_dx(o, f, du, o->adxi, o->adyi, o->adzi);
 }
 
-/* new_time_step, evolution.F90:379-493 (single level, no viscosity) */
+/* new_time_step, evolution.F90:379-493: dt = cfl dx_min / max(cmax + 2 max(nu, eta) / dx_min,
+ * eps) */
 static void new_time_step(ogrid* g, double dtnext)
 {
   ocfg* o = &g->o;
@@ -4423,8 +4424,127 @@ static void new_time_step(ogrid* g, double dtnext)
     dxmin = fmin_r(o->adx, o->ady);
     if (o->nd == 3) dxmin = fmin_r(dxmin, o->adz);
   }
-  g->dtn = o->cflx * dxmin / fmax_r(o->cmax + 2.0 * 0.0 / dxmin, eps);
+  g->dtn = o->cflx * dxmin /
+           fmax_r(o->cmax + 2.0 * fmax_r(o->c.viscosity, o->c.resistivity) / dxmin, eps);
   if (dtnext > 0.0) g->dt = fmin_r(g->dtn, dtnext);
+}
+
+/* derivative_1st (operators.F90:538-615) of variable v at cell c along d:
+ * the central difference of the interior cells */
+static double d1c(const double* q, size_t c, size_t st, int v, double h)
+{
+  return 0.5 * (q[c + st + v] - q[c - st + v]) / h;
+}
+
+/* the viscous stress of update_sources (sources.F90:340-384) at cell c:
+ * velocity gradients g[a][d] = d v_a / d x_d (zero beyond ndims, as
+ * gradient leaves them), gc = nu rho, fc = 2 gc,
+ *   t(a,a) = fc (g(a,a) - div v / 3),  t(a,b) = gc (g(a,b) + g(b,a)) */
+static void visc_tau(const ocfg* o, const double* q, size_t c, const size_t st[3],
+                     const double h[3], double t[3][3])
+{
+  double g[3][3], gc, fc, divv;
+  int a, d;
+  for (a = 0; a < 3; a++)
+    for (d = 0; d < 3; d++) g[a][d] = d < o->nd ? d1c(q, c, st[d], o->ivx + a, h[d]) : 0.0;
+  gc = o->c.viscosity * q[c + o->idn];
+  fc = 2.0 * gc;
+  divv = ((g[0][0] + g[1][1]) + g[2][2]) / 3.0;
+  t[0][0] = fc * (g[0][0] - divv);
+  t[1][1] = fc * (g[1][1] - divv);
+  t[2][2] = fc * (g[2][2] - divv);
+  t[0][1] = gc * (g[0][1] + g[1][0]);
+  t[0][2] = gc * (g[0][2] + g[2][0]);
+  t[1][2] = gc * (g[1][2] + g[2][1]);
+  t[1][0] = t[0][1];
+  t[2][0] = t[0][2];
+  t[2][1] = t[1][2];
+}
+
+/* the viscous source terms of update_sources (sources.F90:340-467) on the
+ * interior cells of one block: du(m_a) += div t(a,:), and with an energy
+ * equation du(E) += div (t v); the divergences are derivative_1st's central
+ * differences summed over the directions in order (operators.F90:181-243).
+ * Every system, when viscosity > 0. */
+static void viscous_sources(const ocfg* o, const double* q, double* du, const double h[3])
+{
+  const int nd = o->nd, nv = o->nv;
+  const size_t st[3] = {(size_t)nv, (size_t)o->im * nv, (size_t)o->im * o->jm * nv};
+  int i, j, k, d, a, b;
+  if (!(o->c.viscosity > 0.0)) return;
+  for (k = o->kb; k <= o->ke; k++)
+    for (j = o->jb; j <= o->je; j++)
+      for (i = o->ib; i <= o->ie; i++) {
+        const size_t c = QI(o, 0, i, j, k);
+        double tp[3][3][3], tm[3][3][3], gp[3], gm[3], s;
+        for (d = 0; d < nd; d++) {
+          visc_tau(o, q, c + st[d], st, h, tp[d]);
+          visc_tau(o, q, c - st[d], st, h, tm[d]);
+        }
+        for (a = 0; a < 3; a++) {
+          s = 0.0;
+          for (d = 0; d < nd; d++) s = s + 0.5 * (tp[d][a][d] - tm[d][a][d]) / h[d];
+          du[c + o->imx + a] = du[c + o->imx + a] + s;
+        }
+        if (o->ien >= 0) {
+          s = 0.0;
+          for (d = 0; d < nd; d++) {
+            const double* vp = q + c + st[d] + o->ivx;
+            const double* vm = q + c - st[d] + o->ivx;
+            b = d;
+            gp[b] = (vp[0] * tp[d][b][0] + vp[1] * tp[d][b][1]) + vp[2] * tp[d][b][2];
+            gm[b] = (vm[0] * tm[d][b][0] + vm[1] * tm[d][b][1]) + vm[2] * tm[d][b][2];
+            s = s + 0.5 * (gp[b] - gm[b]) / h[d];
+          }
+          du[c + o->ien] = du[c + o->ien] + s;
+        }
+      }
+}
+
+/* the resistive source terms of update_sources (sources.F90:543-598) on
+ * the interior cells of one block, for the systems with a magnetic field:
+ * du(B) += eta lap B (derivative_2nd's ((u(i+1) + u(i-1)) - 2 u(i)) / dh^2
+ * summed over the directions, operators.F90:453-711), and with an energy
+ * equation du(E) += B . (eta lap B), then du(E) += eta |curl B|^2 (curl
+ * from derivative_1st, operators.F90:328-434) */
+static void resistive_sources(const ocfg* o, const double* q, double* du, const double h[3])
+{
+  const int nd = o->nd, nv = o->nv, bx = o->ibx;
+  const size_t st[3] = {(size_t)nv, (size_t)o->im * nv, (size_t)o->im * o->jm * nv};
+  const double eta = o->c.resistivity;
+  int i, j, k, d, a;
+  if (!(eta > 0.0) || o->ibx < 0) return;
+  for (k = o->kb; k <= o->ke; k++)
+    for (j = o->jb; j <= o->je; j++)
+      for (i = o->ib; i <= o->ie; i++) {
+        const size_t c = QI(o, 0, i, j, k);
+        const double* qc = q + c;
+        double el[3], cv[3], db;
+        for (a = 0; a < 3; a++) {
+          double lap = 0.0;
+          for (d = 0; d < nd; d++) {
+            const double w = ((q[c + st[d] + bx + a] + q[c - st[d] + bx + a]) - 2.0 * qc[bx + a]) /
+                             (h[d] * h[d]);
+            lap = d == 0 ? w : lap + w;
+          }
+          el[a] = eta * lap;
+          du[c + bx + a] = du[c + bx + a] + el[a];
+        }
+        if (o->ien >= 0) {
+          du[c + o->ien] = du[c + o->ien] +
+                           ((qc[bx] * el[0] + qc[bx + 1] * el[1]) + qc[bx + 2] * el[2]);
+          if (nd == 3) {
+            cv[0] = d1c(q, c, st[1], bx + 2, h[1]) - d1c(q, c, st[2], bx + 1, h[2]);
+            cv[1] = d1c(q, c, st[2], bx, h[2]) - d1c(q, c, st[0], bx + 2, h[0]);
+          } else {
+            cv[0] = d1c(q, c, st[1], bx + 2, h[1]);
+            cv[1] = -d1c(q, c, st[0], bx + 2, h[0]);
+          }
+          cv[2] = d1c(q, c, st[0], bx + 1, h[0]) - d1c(q, c, st[1], bx, h[1]);
+          db = (cv[0] * cv[0] + cv[1] * cv[1]) + cv[2] * cv[2];
+          du[c + o->ien] = du[c + o->ien] + eta * db;
+        }
+      }
 }
 
 /* The GLM source terms of update_sources (sources.F90:479-539) on the
@@ -4490,12 +4610,16 @@ static void stage2(ogrid* g, int kind, double ds, double* dst, const double* a0,
     if (g->amr) {
       const double* lv = g->lvd[g->lev[b]];
       update_increment_dx(o, g->fall + b * bs * o->nd, g->du, lv[3], lv[4], lv[5]);
+      viscous_sources(o, g->q + b * bs, g->du, lv);
       glm_sources(o, g->q + b * bs, g->du, lv);
+      resistive_sources(o, g->q + b * bs, g->du, lv);
     } else {
       const double h[3] = {o->adx, o->ady, o->adz};
       update_flux(o, g->q + b * bs, g->f, g->qs);
       update_increment(o, g->f, g->du);
+      viscous_sources(o, g->q + b * bs, g->du, h);
       glm_sources(o, g->q + b * bs, g->du, h);
+      resistive_sources(o, g->q + b * bs, g->du, h);
     }
     for (n = 0; n < bs; n++) {
       double v;

@@ -1166,6 +1166,154 @@ def main_srhd():
     print("stats srhd", st[:6])
 
 
+# (name, ndims, equation system, reconstruction, viscosity, resistivity, steps)
+VISC_RUNS = (("visc_hd2d_tvd_blast_10", 2, "hd", "tvd", 2e-3, 0.0, 10),
+             ("visc_mhd2d_tvd_ot_10", 2, "mhd", "tvd", 1e-3, 0.0, 10),
+             ("resist_mhd2d_tvd_ot_10", 2, "mhd", "tvd", 0.0, 2e-3, 10),
+             ("viscres_mhd3d_weno5z_noisy_4", 3, "mhd", "weno5z", 1e-3, 1e-3, 4),
+             ("viscres_mhd2d_iso_ot_10", 2, "mhdiso", "tvd", 1e-3, 2e-3, 10),
+             ("visc_hd3d_weno5z_noisy_4", 3, "hd", "weno5z", 3e-3, 0.0, 4))
+
+
+def visc_params(nd, eq, rec, nu, eta):
+    iso = eq.endswith("iso")
+    eqs = eq[:-3] if iso else eq
+    p = Params(ndims=nd, equation_system=eqs, reconstruction=rec,
+               limiter="mc" if rec == "tvd" else "mm",
+               riemann_solver="hllc" if eqs == "hd" else "hlld", time_advance="rk3",
+               gamma=1.4 if eqs == "hd" else 5.0 / 3.0, cfl=0.4 if nd == 2 else 0.3,
+               ncells=16 if nd == 2 else 8, nghosts=2 if rec == "tvd" else 4, xblocks=2,
+               yblocks=2, zblocks=2, viscosity=nu, resistivity=eta)
+    if iso:
+        p.equation_of_state, p.csnd = "iso", 0.8
+    return p
+
+
+def visc_state(name, p):
+    if "blast" in name:
+        return problems.blast_hd_2d(p, seed=9)
+    if p.equation_of_state == "iso":
+        return iso_state("iso" + name[name.index("_ot_"):] if "_ot_" in name else name, p)
+    if "_ot_" in name:
+        return ot_noisy(p)
+    if p.equation_system == "hd":
+        return srhd_state("noisy", p)        # smooth 3D fields, |v| < 0.6
+    return noisy3d(p)
+
+
+def main_visc():
+    """The viscous and resistive source terms of update_sources
+    (sources.F90:340-598) and their time-step limit (evolution.F90:478-483):
+    2D and 3D HD and MHD runs, adiabatic and isothermal (own generator)."""
+    with open(os.path.join(HERE, "index.json")) as fh:
+        index = json.load(fh)
+    for name, nd, eq, rec, nu, eta, nsteps in VISC_RUNS:
+        p = visc_params(nd, eq, rec, nu, eta)
+        u0 = visc_state(name, p)
+        u, tr = ol.ref_run(p, u0, nsteps)
+        sub = u[:, ::2, ::2, ::2] if nd == 3 else u[:, :, ::3, ::3]
+        np.savez_compressed(os.path.join(HERE, name + ".npz"),
+                            params=np.array(p.to_params_in()), ndims=nd, nsteps=nsteps,
+                            stride=2 if nd == 3 else 3, trace=tr, sub=sub, sha_u0=sha(u0),
+                            sha_u=sha(u))
+        index[name] = {"steps": nsteps, "sha_u": sha(u), "dt_last": float(tr[-1, 0])}
+        print(name, "dt", tr[-1, 0])
+    with open(os.path.join(HERE, "index.json"), "w") as fh:
+        json.dump(index, fh, indent=1)
+
+
+# (name, ndims, equation system, reconstruction, steps): gravity of the
+# Rayleigh-Taylor problem
+GRAV_RUNS = (("grav_hd2d_tvd_rt_10", 2, "hd", "tvd", 10),
+             ("grav_mhd2d_tvd_rt_10", 2, "mhd", "tvd", 10),
+             ("grav_hdiso2d_tvd_rt_10", 2, "hdiso", "tvd", 10),
+             ("grav_hd3d_weno5z_rt_4", 3, "hd", "weno5z", 4))
+# (name, ndims, equation system, reconstruction, Riemann solver, steps):
+# the four-velocity reconstruction of the relativistic systems
+V4_RUNS = (("v4_srhd2d_tvd_hllc_blast_20", 2, "srhd", "tvd", "hllc", 20),
+           ("v4_srhd3d_weno5z_hll_noisy_4", 3, "srhd", "weno5z", "hll", 4),
+           ("v4_srmhd2d_tvd_hllc_blast_10", 2, "srmhd", "tvd", "hllc", 10))
+
+
+def grav_params(nd, eq, rec):
+    iso = eq.endswith("iso")
+    eqs = eq[:-3] if iso else eq
+    p = Params(ndims=nd, equation_system=eqs, reconstruction=rec,
+               limiter="mc" if rec == "tvd" else "mm",
+               riemann_solver="hllc" if eqs == "hd" else "hlld", time_advance="rk2",
+               gamma=1.4 if eqs == "hd" else 5.0 / 3.0, cfl=0.4 if nd == 2 else 0.3,
+               ncells=16 if nd == 2 else 8, nghosts=2 if rec == "tvd" else 4, xblocks=2,
+               yblocks=2, zblocks=2, xmin=-0.5, xmax=0.5, ymin=-0.5, ymax=0.5, zmin=-0.5,
+               zmax=0.5, ylbndry="reflective", yubndry="reflective")
+    if iso:
+        p.equation_of_state, p.csnd = "iso", 1.0
+    p.extra["problem"] = "rt"
+    p.extra["gacc"] = "-0.5"
+    return p
+
+
+def grav_state(p, seed=6):
+    """Heavy fluid (rho 2) above light (rho 1) with the interface displaced
+    by 0.02 cos(2 pi x) (times cos(2 pi z) in 3D), 2 % density noise,
+    p = 2.5 - 0.5 rho y where there is a pressure, Bx = 0.05 where there is
+    a field."""
+    nz, ny, nx = p.shape
+    z, y, x = problems.cell_centres(p)
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    xi = problems.uniform(seed, nz * ny * nx, stream=60).reshape(nz, ny, nx) - 0.5
+    q = np.zeros((p.nv, nz, ny, nx))
+    yi = 0.02 * np.cos(2.0 * np.pi * X) * (np.cos(2.0 * np.pi * Z) if p.ndims == 3 else 1.0)
+    q[0] = np.where(Y > yi, 2.0, 1.0) * (1.0 + 0.02 * xi)
+    if p.nv in (5, 9):
+        q[-1] = 2.5 - 0.5 * q[0] * Y
+    if p.nv >= 8:
+        q[4] = 0.05
+    return problems.prim_to_cons(p, q)
+
+
+def v4_params(nd, eq, rec, rs):
+    p = srhd_params(nd, rec, rs, "1dw")
+    p.equation_system = eq
+    if eq == "srmhd":
+        p.gamma = 5.0 / 3.0
+    p.state_variables = "4vec"
+    return p
+
+
+def v4_state(name, p):
+    if eq_of(p) == "srmhd":
+        return problems.srmhd_blast_2d(p, noise=0.01, vnoise=0.05)
+    return srhd_state(name, p)
+
+
+def eq_of(p):
+    return p.equation_system.lower()
+
+
+def main_grav4v():
+    """Gravity of the Rayleigh-Taylor problem (sources.F90:288-335,
+    gravity.F90:141-150, 283-330) and the relativistic systems'
+    four-velocity reconstruction (state_variables = 4vec,
+    schemes.F90:4481-4517, 5040-5076): short runs (own generator)."""
+    with open(os.path.join(HERE, "index.json")) as fh:
+        index = json.load(fh)
+    runs = [(n, grav_params(nd, eq, rec), None, st) for n, nd, eq, rec, st in GRAV_RUNS]
+    runs += [(n, v4_params(nd, eq, rec, rs), n, st) for n, nd, eq, rec, rs, st in V4_RUNS]
+    for name, p, v4, nsteps in runs:
+        u0 = grav_state(p) if v4 is None else v4_state(name, p)
+        u, tr = ol.ref_run(p, u0, nsteps)
+        nd = p.ndims
+        sub = u[:, ::2, ::2, ::2] if nd == 3 else u[:, :, ::3, ::3]
+        np.savez_compressed(os.path.join(HERE, name + ".npz"),
+                            params=np.array(p.to_params_in()), ndims=nd, nsteps=nsteps,
+                            stride=2 if nd == 3 else 3, trace=tr, sub=sub, sha_u0=sha(u0),
+                            sha_u=sha(u))
+        index[name] = {"steps": nsteps, "sha_u": sha(u), "dt_last": float(tr[-1, 0])}
+        print(name, "dt", tr[-1, 0])
+    with open(os.path.join(HERE, "index.json"), "w") as fh:
+        json.dump(index, fh, indent=1)
+
+
 def main():
     if not (ol.ref_available(2) and ol.ref_available(3)):
         sys.exit("reference binaries missing: run oracle/build_ref.sh first")
@@ -1181,6 +1329,12 @@ def main():
     if sys.argv[1:] == ["refine"]:
         main_refine()
         sys.exit(0)
+    if sys.argv[1:] == ["grav4v"]:
+        main_grav4v()
+        return
+    if sys.argv[1:] == ["visc"]:
+        main_visc()
+        return
     if sys.argv[1:] == ["srhd"]:
         main_srhd()
         return

@@ -65,13 +65,11 @@ def test_config_errors(field, value, code):
     assert lib.amun_dev_last_error()
 
 
-@pytest.mark.parametrize("key,value", [
-    ("fix_unphysical_cells", "on"), ("viscosity", "1.0e-3"), ("resistivity", "1.0e-3"),
-])
+@pytest.mark.parametrize("key,value", [("fix_unphysical_cells", "on")])
 def test_unsupported_terms_are_refused(key, value):
-    """Parameters of terms outside the device path (the unphysical-cell fix,
-    the viscous/resistive sources) are carried into amun_cfg and refused with
-    600 instead of being ignored."""
+    """Parameters of terms outside the device path (the unphysical-cell fix)
+    are carried into amun_cfg and refused with 600 instead of being
+    ignored."""
     p = Params.from_text(f"reconstruction = tvd\nriemann_solver = hllc\n{key} = {value}\n",
                          ndims=2)
     assert key not in p.extra
@@ -141,6 +139,20 @@ def test_four_vector_states_are_refused():
     h = ctypes.c_void_p()
     assert lib.amun_dev_init(ctypes.byref(cfg), ctypes.byref(h)) == 600
     assert b"4vec" in lib.amun_dev_last_error()
+
+
+@pytest.mark.parametrize("key", ["viscosity", "resistivity"])
+def test_viscous_coefficients_carried(key):
+    """viscosity and resistivity reach amun_cfg; negative values are refused
+    with 500 (zero switches the terms off, as sources.F90:340, 543)."""
+    p = Params.from_text(f"equation_system = mhd\nriemann_solver = hlld\n{key} = 2.5e-3\n",
+                         ndims=2)
+    cfg = p.to_cfg()
+    assert getattr(cfg, key) == 2.5e-3
+    setattr(cfg, key, -1.0)
+    lib = solver.load_library()
+    h = ctypes.c_void_p()
+    assert lib.amun_dev_init(ctypes.byref(cfg), ctypes.byref(h)) == 500
 
 
 def test_weno_needs_four_ghosts():
