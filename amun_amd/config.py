@@ -66,6 +66,8 @@ class AmunCfg(ctypes.Structure):
         ("viscosity", ctypes.c_double), ("resistivity", ctypes.c_double),
         ("csnd", ctypes.c_double),
         ("states_4vec", ctypes.c_int32), ("reserved_cfg2", ctypes.c_int32),
+        ("gravity", ctypes.c_int32), ("reserved_cfg3", ctypes.c_int32),
+        ("gacc", ctypes.c_double),
     ]
 
 
@@ -259,6 +261,10 @@ class Params:
         c.viscosity, c.resistivity = float(self.viscosity), float(self.resistivity)
         c.csnd = float(self.csnd)
         c.states_4vec = int(self.state_variables.lower() in ("4vec", "4-vector"))
+        # gravity.F90:141-150: the Rayleigh-Taylor problem turns gravity on
+        c.gravity = int(str(self.extra.get("problem", "")).strip().lower()
+                        in ("rt", "rayleightaylor", "rayleigh-taylor"))
+        c.gacc = float(str(self.extra.get("gacc", "-1.0")).replace("d", "e").replace("D", "e"))
         c.prol_limiter = {"mm": 1, "minmod": 1, "mc": 2, "monotonized_central": 2,
                           "sb": 3, "superbee": 3, "vl": 4, "vanleer": 4}.get(
                               self.prolongation_limiter.lower(), 0)

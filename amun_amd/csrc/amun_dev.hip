@@ -34,6 +34,7 @@ struct amun_ctx {
   double* ugp = nullptr;       // MGP weights (RunConst::ugp points here)
   void* src = nullptr;         // GLM source products per stage (element size qsize)
   void* vsrc = nullptr;        // viscous and resistive source products per stage
+  void* q4 = nullptr;          // four-velocity states of the stage input (relativistic)
   int vis = 0;                 // 1 viscosity, 2 resistivity (SweepArgs::vis)
   void* mlps = nullptr;        // multi-dimensional limiting: slopes of the interior cells
   void* mlpg = nullptr;        // and of the cells beyond the pencil ends (k_mlp)
@@ -208,8 +209,6 @@ int validate(const amun_cfg& c, int* nv) {
     return fail(600, "fix_positivity / clip_extrema are not supported on the device path");
   if (c.fix_unphysical_cells)
     return fail(600, "fix_unphysical_cells is not supported on the device path");
-  if (c.states_4vec && relativistic(c))
-    return fail(600, "state_variables = 4vec is not supported on the device path");
   if (c.glm_source_terms < 0 || c.glm_source_terms > 2)
     return fail(500, "glm_source_terms must be 0 (none), 1 (eglm) or 2 (heglm)");
   if (!(c.viscosity >= 0.0) || !(c.resistivity >= 0.0))
@@ -340,6 +339,9 @@ RunConst make_rc(const amun_cfg& cfg) {
   // system (evolution.F90:478-483)
   rc.nu = cfg.viscosity;
   rc.eta = cfg.resistivity;
+  rc.grav = cfg.gravity ? 1 : 0;
+  rc.gacc = cfg.gacc;
+  rc.v4 = cfg.states_4vec && relativistic(cfg) ? 1 : 0;
   // initialize_evolution (evolution.F90:166-200): SSPRK(m,2) with m clamped
   // to [2, 9], SSPRK(4,3) and SSPRK(5,3) step at a multiple of the CFL
   // number, and the psi decay follows the scaled number
@@ -396,7 +398,7 @@ static void mlp_launch_t(amun_ctx* c, int qcur) {
   const int nd = rc.nd, npen = nd == 3 ? rc.nc * rc.nc : rc.nc;
   const long long n = (long long)rc.nblk * (rc.ncell + nd * 3 * npen);
   const dim3 grid((unsigned)((n + 255) / 256)), bl(256);
-  const T* Q = static_cast<const T*>(c->Q[qcur]);
+  const T* Q = static_cast<const T*>(c->q4 ? c->q4 : c->Q[qcur]);
   T* S = static_cast<T*>(c->mlps);
   T* G = static_cast<T*>(c->mlpg);
   if (nd == 3)
@@ -622,6 +624,7 @@ static int init_device(amun_ctx* c, const std::vector<int32_t>& nbr,
     ALLOC(c->mlpg, (size_t)rc.nblk * nv * nd * gs * c->qsize);
   }
   c->vis = (cfg->viscosity > 0.0 ? 1 : 0) | (cfg->resistivity > 0.0 && has_field(*cfg) ? 2 : 0);
+  if (rc.v4) ALLOC(c->q4, (size_t)rc.nblk * nv * rc.ncell * c->qsize);
   if (c->vis) {
     // the nine viscous and resistive source products of every cell (k_visc_src)
     ALLOC(c->vsrc, (size_t)rc.nblk * 9 * rc.ncell * c->qsize);
@@ -1012,6 +1015,7 @@ void amun_dev_finalize(amun_ctx* c) {
   hipFree(c->ugp);
   hipFree(c->src);
   hipFree(c->vsrc);
+  hipFree(c->q4);
   hipFree(c->mlps);
   hipFree(c->mlpg);
   hipFree(c->rc.fsr);
@@ -1377,6 +1381,27 @@ static int glm_sources(amun_ctx* c, int qcur) {
   return 0;
 }
 
+// the four-velocity states of the stage input Q[qcur] (k_to_4vec)
+static int to_4vec(amun_ctx* c, int qcur) {
+  const long long n = (long long)c->rc.nblk * c->rc.ncell;
+  const dim3 grid((unsigned)((n + 255) / 256)), bl(256);
+  by_system(c->cfg, [&](auto tag) {
+    using P = decltype(tag);
+    if constexpr (P::sr) {
+      if (c->qsize == sizeof(double))
+        hipLaunchKernelGGL((k_to_4vec<P, double>), grid, bl, 0, c->stream,
+                           static_cast<const double*>(c->Q[qcur]),
+                           static_cast<double*>(c->q4), n, c->rc.ncell);
+      else
+        hipLaunchKernelGGL((k_to_4vec<P, float>), grid, bl, 0, c->stream,
+                           static_cast<const float*>(c->Q[qcur]), static_cast<float*>(c->q4),
+                           n, c->rc.ncell);
+    }
+  });
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
 // the viscous and resistive source products of the stage input Q[qcur]
 // (k_visc_src)
 static int visc_sources(amun_ctx* c, int qcur) {
@@ -1426,6 +1451,10 @@ static int enqueue_step(amun_ctx* c, double dtnext, int slot, int qcur, hipEvent
         int r = bnd_fill(c, qcur);
         if (r) return r;
       }
+      if (c->q4) {
+        int r = to_4vec(c, qcur);
+        if (r) return r;
+      }
       if (c->mlps) {
         int r = mlp_launch(c, qcur);
         if (r) return r;
@@ -1442,6 +1471,7 @@ static int enqueue_step(amun_ctx* c, double dtnext, int slot, int qcur, hipEvent
     SweepArgs a;
     a.vsrc = c->vsrc;
     a.vis = c->vis;
+    a.Qold = c->Q[qcur];
     a.mlps = c->mlps;
     a.mlpg = c->mlpg;
     a.src = c->src;
@@ -1449,7 +1479,7 @@ static int enqueue_step(amun_ctx* c, double dtnext, int slot, int qcur, hipEvent
     a.blev = c->amr.on ? c->amr.blev : nullptr;
     a.ffix = c->amr.on ? c->amr.ffix : nullptr;
     a.FB = c->amr.on ? c->amr.FB : nullptr;
-    a.Q = c->Q[qcur];
+    a.Q = c->q4 ? c->q4 : c->Q[qcur];
     a.U0 = c->U0;
     a.U1 = c->U1;
     a.Uout = stages[s].to_u0 ? c->U0 : c->U1;
@@ -1475,7 +1505,8 @@ static int enqueue_step(amun_ctx* c, double dtnext, int slot, int qcur, hipEvent
       if (d.compact[dir]) {
         const long long nt = (long long)rc.nblk * (rc.nd == 3 ? rc.nc * rc.nc : rc.nc) * c->nv;
         hipLaunchKernelGGL(d.compact[dir], dim3((unsigned)((nt + 63) / 64)), dim3(64), 0,
-                           c->stream, (const void*)c->Q[qcur], c->nbr, rc, c->st);
+                           c->stream, (const void*)(c->q4 ? c->q4 : c->Q[qcur]), c->nbr, rc,
+                           c->st);
       }
       a.swz = c->swz[dir];
       const dim3 grid((unsigned)((nthreads + wg - 1) / wg)), block(wg);

@@ -96,6 +96,9 @@ struct RunConst {
   double gamma, gammam1, gammam1i;
   double csnd, csnd2;               // isothermal sound speed and its square
   double nu, eta;                   // viscosity and resistivity (sources.F90:52-58)
+  double gacc;                      // uniform gravity (0, gacc, 0) when grav
+  int grav;
+  int v4;                           // relativistic: four-velocity reconstruction
   double cfl, eps, decay;
   double cgp[5];                    // Gaussian-process weights (prepare_gp)
   double kappa, kbeta;              // MP limiter of the GP reconstruction
@@ -275,6 +278,10 @@ struct SweepArgs {
   // per cell of k_visc_src, and which are on (1 viscosity, 2 resistivity)
   const void* vsrc;
   int vis;
+  // the primitive state of the stage input: Q itself, or the primitives
+  // behind the four-velocity states of Q (the recovery keeps them for a
+  // failed cell; gravity reads them)
+  const void* Qold;
   // multi-dimensional limiting (null when off): the slopes of k_mlp for the
   // interior cells (S) and the cells beyond the pencil ends (G)
   const void* mlps;
@@ -1191,6 +1198,12 @@ __global__ __launch_bounds__(256, sweep_min_waves<T>()) void k_sweep(SweepArgs a
         mlp_apply<P, ND, DIR, T>(rc, Qp, static_cast<const T*>(a.mlps),
                                  static_cast<const T*>(a.mlpg), a.nbr, b, c, t, ci, cj, ck,
                                  c == -1 && rc.ng == 2, flc, frc);
+      if constexpr (P::sr) {
+        if (rc.v4) {
+          states_from_4vec<P, T>(flc);
+          states_from_4vec<P, T>(frc);
+        }
+      }
       if (c >= 0) {
         // face c-1/2: left state from cell c-1, right state from cell c
         T ql[NV], qr[NV], fr[NV], f[NV];
@@ -1232,6 +1245,22 @@ __global__ __launch_bounds__(256, sweep_min_waves<T>()) void k_sweep(SweepArgs a
             else if constexpr (kDin) du = sd[(((c - 1) & 3) * NV + v) * wg + tid] - dxi * dfx;
             else du = dpre[v] - dxi * dfx;
             if constexpr (kLast) {
+              // gravity (update_sources, sources.F90:288-335): du(m) += rho g,
+              // du(E) += rho g . v with g = (0, gacc, 0), zero terms included
+              if (rc.grav) {
+#pragma clang fp contract(off)
+                const T* Qg = static_cast<const T*>(a.Qold);
+                const size_t n1 = (size_t)rc.ncell;
+                const T rho = Qg[bt + (size_t)P::idn * n1];
+                const T gx = rho * T(0), gy = rho * T(rc.gacc), gz = rho * T(0);
+                if (v == P::imx) du = du + gx;
+                if (v == P::imy) du = du + gy;
+                if (ND == 3 && v == P::imz) du = du + gz;
+                if (v == P::ien) {
+                  du = (du + gx * Qg[bt + (size_t)P::ivx * n1]) + gy * Qg[bt + (size_t)P::ivy * n1];
+                  if (ND == 3) du = du + gz * Qg[bt + (size_t)P::ivz * n1];
+                }
+              }
               // viscous source terms after the completed increment
               // (update_sources, sources.F90:340-467; products of k_visc_src)
               if (a.vsrc && (a.vis & 1)) {
@@ -1305,7 +1334,7 @@ __global__ __launch_bounds__(256, sweep_min_waves<T>()) void k_sweep(SweepArgs a
             if constexpr (P::sr) {
               const int cs = cons2prim_sr<P>(rc.sc, unew, qn);
               if (cs) {
-                c2p_keep_old<P, T>(cs, qn, Qp, bt, rc.ncell);
+                c2p_keep_old<P, T>(cs, qn, static_cast<const T*>(a.Qold), bt, rc.ncell);
                 c2p_count(a.st, cs);
               }
             } else {

@@ -732,6 +732,40 @@ __device__ __forceinline__ int nr_iterate_2d(const SrConst& s, double mm, double
 }
 
 
+// the face states of the four-velocity reconstruction turned back into rho
+// and v (schemes.F90:4505-4517, 5064-5076): W = sqrt(1 + |W v|^2)
+template <class P, typename T>
+__device__ __forceinline__ void states_from_4vec(T* f) {
+  const T vm = tsqrt(T(1) + ((f[P::ivx] * f[P::ivx] + f[P::ivy] * f[P::ivy]) +
+                             f[P::ivz] * f[P::ivz]));
+  f[P::idn] = f[P::idn] / vm;
+  f[P::ivx] = f[P::ivx] / vm;
+  f[P::ivy] = f[P::ivy] / vm;
+  f[P::ivz] = f[P::ivz] / vm;
+}
+
+// the four-velocity states of the primitive pool (schemes.F90:4481-4496,
+// 5040-5056): rho W and W v with W = 1 / sqrt(1 - v^2), the rest copied
+template <class P, typename T>
+static __global__ void __launch_bounds__(256)
+k_to_4vec(const T* __restrict__ Q, T* __restrict__ Q4, long long ncells, int ncell) {
+  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ncells) return;
+  const long long b = g / ncell, c = g % ncell;
+  const size_t base = (size_t)b * P::nv * ncell + c;
+  T q[P::nv];
+#pragma unroll
+  for (int v = 0; v < P::nv; v++) q[v] = Q[base + (size_t)v * ncell];
+  const T vm = tsqrt(T(1) - ((q[P::ivx] * q[P::ivx] + q[P::ivy] * q[P::ivy]) +
+                             q[P::ivz] * q[P::ivz]));
+  q[P::idn] = q[P::idn] / vm;
+  q[P::ivx] = q[P::ivx] / vm;
+  q[P::ivy] = q[P::ivy] / vm;
+  q[P::ivz] = q[P::ivz] / vm;
+#pragma unroll
+  for (int v = 0; v < P::nv; v++) Q4[base + (size_t)v * ncell] = q[v];
+}
+
 // ---------------------------------------------------------------------------
 // primitive recovery of the relativistic HD system (equations.F90:3550-4566)
 // ---------------------------------------------------------------------------

@@ -65,6 +65,8 @@ module amun_dev_mod
     real(c_double)     :: viscosity, resistivity
     real(c_double)     :: csnd
     integer(c_int32_t) :: states_4vec, reserved_cfg2
+    integer(c_int32_t) :: gravity, reserved_cfg3
+    real(c_double)     :: gacc
   end type amun_cfg
 
   interface

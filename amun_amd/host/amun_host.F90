@@ -238,6 +238,15 @@ program amun_host
   tmax    = get_real("tmax", 1.0d30)
   seed    = int(get_int("seed", 20261202), int64)
   problem = get_str("problem", "blast")
+  ! gravity.F90:141-150: the Rayleigh-Taylor problem turns gravity on, with
+  ! the acceleration (0, gacc, 0)
+  select case(trim(problem))
+  case("rt", "rayleightaylor", "rayleigh-taylor")
+    cfg%gravity = 1
+  case default
+    cfg%gravity = 0
+  end select
+  cfg%gacc = get_real("gacc", -1.0d0)
   fin     = get_str("input", "")
   fout    = get_str("output", "")
 

@@ -114,8 +114,9 @@ def main():
     for k, (n, av, mn, mx) in sorted(st.items(), key=lambda kv: -kv[1][0] * kv[1][1]):
         lines.append(f"{k[:66]:66s} {n:6d} {av:9.1f} {mn:9.1f} {mx:9.1f}")
     lines.append("")
-    lines.append("HBM traffic per launch (calibrated), GB; registers per lane (VGPR, AGPR, "
-                 "SGPR), scratch and LDS bytes per workgroup:")
+    lines.append("HBM traffic per launch (calibrated), GB; the register, scratch and LDS "
+                 "fields of rocprofv3's dispatch records (allocation granules and static LDS; "
+                 "the compiler's resource report gives the registers):")
     rd = wr = 0.0
     for k in sweeps:
         r = _mean(fc.get(k, {}).get("FETCH_SIZE", [])) * 1024.0 * fcal / 1e9

@@ -172,6 +172,12 @@ typedef struct amun_cfg {
    * four-velocity; refused with 600 on the device path) */
   int32_t states_4vec;
   int32_t reserved_cfg2;
+  /* uniform gravity (0, "gacc", 0) of the Rayleigh-Taylor problem
+   * (gravity.F90:141-150, 283-330; "gacc" -1 by default): on for
+   * problem = rt, rayleightaylor or rayleigh-taylor */
+  int32_t gravity;
+  int32_t reserved_cfg3;
+  double  gacc;
 } amun_cfg;
 
 /* Refinement criterion of the leaf blocks (initialize_refinement,

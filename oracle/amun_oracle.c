@@ -3968,7 +3968,42 @@ static void update_flux(ocfg* o, const double* q, double* f, double* qs)
   }
 
   memset(f, 0, ncell * nv * nd * sizeof(double));
-  reconstruct_interfaces(o, q, qs);
+  if (o->sr && o->c.states_4vec) {
+    /* state_variables = 4vec (schemes.F90:4481-4517, 5040-5076): rho W
+     * and the four-velocity W v are reconstructed, the face states turned
+     * back into rho and v with W = sqrt(1 + |W v|^2) */
+    static double* qq = NULL;
+    static size_t nqq = 0;
+    size_t n, m;
+    if (nqq < ncell * nv) {
+      free(qq);
+      nqq = ncell * nv;
+      qq = (double*)malloc(nqq * sizeof(double));
+    }
+    for (n = 0; n < ncell; n++) {
+      const double* qc = q + n * nv;
+      double* qo = qq + n * nv;
+      const double vm = sqrt(1.0 - ((qc[o->ivx] * qc[o->ivx] + qc[o->ivy] * qc[o->ivy]) +
+                                    qc[o->ivz] * qc[o->ivz]));
+      for (p = 0; p < nv; p++) qo[p] = qc[p];
+      qo[o->idn] = qc[o->idn] / vm;
+      qo[o->ivx] = qc[o->ivx] / vm;
+      qo[o->ivy] = qc[o->ivy] / vm;
+      qo[o->ivz] = qc[o->ivz] / vm;
+    }
+    reconstruct_interfaces(o, qq, qs);
+    for (m = 0; m < (size_t)nd * 2 * ncell; m++) {
+      double* qf = qs + m * nv;
+      const double vm = sqrt(1.0 + ((qf[o->ivx] * qf[o->ivx] + qf[o->ivy] * qf[o->ivy]) +
+                                    qf[o->ivz] * qf[o->ivz]));
+      qf[o->idn] = qf[o->idn] / vm;
+      qf[o->ivx] = qf[o->ivx] / vm;
+      qf[o->ivy] = qf[o->ivy] / vm;
+      qf[o->ivz] = qf[o->ivz] / vm;
+    }
+  } else {
+    reconstruct_interfaces(o, q, qs);
+  }
 
 #define QSV(d, s, i, j, k, p) qs[((size_t)((d) * 2 + (s)) * ncell + ((size_t)(k) * jm + (j)) * im + (i)) * nv + (p)]
   for (d = 0; d < nd; d++) {
@@ -4429,6 +4464,34 @@ static void new_time_step(ogrid* g, double dtnext)
   if (dtnext > 0.0) g->dt = fmin_r(g->dtn, dtnext);
 }
 
+/* the gravitational source terms of update_sources (sources.F90:288-335)
+ * for the uniform acceleration (0, gacc, 0) of the Rayleigh-Taylor problem
+ * (gravity.F90:283-330): du(m) += rho g, du(E) += rho g . v, in the
+ * reference's order (the zero components included) */
+static void gravity_sources(const ocfg* o, const double* q, double* du)
+{
+  int i, j, k;
+  if (!o->c.gravity) return;
+  for (k = o->kb; k <= o->ke; k++)
+    for (j = o->jb; j <= o->je; j++)
+      for (i = o->ib; i <= o->ie; i++) {
+        const size_t c = QI(o, 0, i, j, k);
+        const double* qc = q + c;
+        double* dc = du + c;
+        const double gx = qc[o->idn] * 0.0, gy = qc[o->idn] * o->c.gacc;
+        const double gz = qc[o->idn] * 0.0;
+        dc[o->imx] = dc[o->imx] + gx;
+        dc[o->imy] = dc[o->imy] + gy;
+        if (o->nd == 3) dc[o->imz] = dc[o->imz] + gz;
+        if (o->ien >= 0) {
+          if (o->nd == 3)
+            dc[o->ien] = ((dc[o->ien] + gx * qc[o->ivx]) + gy * qc[o->ivy]) + gz * qc[o->ivz];
+          else
+            dc[o->ien] = (dc[o->ien] + gx * qc[o->ivx]) + gy * qc[o->ivy];
+        }
+      }
+}
+
 /* derivative_1st (operators.F90:538-615) of variable v at cell c along d:
  * the central difference of the interior cells */
 static double d1c(const double* q, size_t c, size_t st, int v, double h)
@@ -4610,6 +4673,7 @@ static void stage2(ogrid* g, int kind, double ds, double* dst, const double* a0,
     if (g->amr) {
       const double* lv = g->lvd[g->lev[b]];
       update_increment_dx(o, g->fall + b * bs * o->nd, g->du, lv[3], lv[4], lv[5]);
+      gravity_sources(o, g->q + b * bs, g->du);
       viscous_sources(o, g->q + b * bs, g->du, lv);
       glm_sources(o, g->q + b * bs, g->du, lv);
       resistive_sources(o, g->q + b * bs, g->du, lv);
@@ -4617,6 +4681,7 @@ static void stage2(ogrid* g, int kind, double ds, double* dst, const double* a0,
       const double h[3] = {o->adx, o->ady, o->adz};
       update_flux(o, g->q + b * bs, g->f, g->qs);
       update_increment(o, g->f, g->du);
+      gravity_sources(o, g->q + b * bs, g->du);
       viscous_sources(o, g->q + b * bs, g->du, h);
       glm_sources(o, g->q + b * bs, g->du, h);
       resistive_sources(o, g->q + b * bs, g->du, h);

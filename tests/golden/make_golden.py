@@ -1255,19 +1255,24 @@ def grav_params(nd, eq, rec):
 def grav_state(p, seed=6):
     """Heavy fluid (rho 2) above light (rho 1) with the interface displaced
     by 0.02 cos(2 pi x) (times cos(2 pi z) in 3D), 2 % density noise,
-    p = 2.5 - 0.5 rho y where there is a pressure, Bx = 0.05 where there is
-    a field."""
+    velocities of 1e-2, p = 2.5 - 0.5 rho y where there is a pressure, and
+    B = (0.05, 0.03, 0.02) where there is a field.  The gas moves and every
+    field component is non-zero: at v = 0 with By = Bz = 0 the HLLD contact
+    speed is exactly zero or not depending on round-off, where the
+    reference's branches do not join continuously."""
     nz, ny, nx = p.shape
     z, y, x = problems.cell_centres(p)
     Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
-    xi = problems.uniform(seed, nz * ny * nx, stream=60).reshape(nz, ny, nx) - 0.5
+    xi = [problems.uniform(seed, nz * ny * nx, stream=s).reshape(nz, ny, nx) - 0.5
+          for s in range(60, 64)]
     q = np.zeros((p.nv, nz, ny, nx))
     yi = 0.02 * np.cos(2.0 * np.pi * X) * (np.cos(2.0 * np.pi * Z) if p.ndims == 3 else 1.0)
-    q[0] = np.where(Y > yi, 2.0, 1.0) * (1.0 + 0.02 * xi)
+    q[0] = np.where(Y > yi, 2.0, 1.0) * (1.0 + 0.02 * xi[0])
+    q[1:4] = 0.02 * np.stack(xi[1:4])
     if p.nv in (5, 9):
         q[-1] = 2.5 - 0.5 * q[0] * Y
     if p.nv >= 8:
-        q[4] = 0.05
+        q[4], q[5], q[6] = 0.05, 0.03, 0.02
     return problems.prim_to_cons(p, q)
 
 

@@ -126,19 +126,19 @@ def test_isothermal_configuration_carried():
     assert d.csnd == 1.0                             # equations.F90:176
 
 
-def test_four_vector_states_are_refused():
+def test_four_vector_states_and_gravity_carried():
     """state_variables = 4vec (the relativistic systems' four-velocity
-    reconstruction, schemes.F90:4481-4517) reaches amun_cfg and is refused
-    with 600; for the other systems the reference ignores it."""
+    reconstruction, schemes.F90:4481-4517) and the Rayleigh-Taylor gravity
+    (problem = rt, gacc; gravity.F90:141-150, 283-330) reach amun_cfg."""
     p = Params.from_text("equation_system = srhd\nriemann_solver = hllc\n"
                          "state_variables = 4vec\n", ndims=2)
     assert "state_variables" not in p.extra
     cfg = p.to_cfg()
-    assert cfg.states_4vec == 1
-    lib = solver.load_library()
-    h = ctypes.c_void_p()
-    assert lib.amun_dev_init(ctypes.byref(cfg), ctypes.byref(h)) == 600
-    assert b"4vec" in lib.amun_dev_last_error()
+    assert cfg.states_4vec == 1 and cfg.gravity == 0
+    p = Params.from_text("problem = rt\ngacc = -0.25\n", ndims=2)
+    cfg = p.to_cfg()
+    assert cfg.gravity == 1 and cfg.gacc == -0.25
+    assert Params.from_text("problem = blast\n", ndims=2).to_cfg().gravity == 0
 
 
 @pytest.mark.parametrize("key", ["viscosity", "resistivity"])
