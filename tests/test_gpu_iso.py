@@ -291,3 +291,28 @@ def test_iso_uniform_state_is_preserved():
         s.run(3)
         u = s.download()
     assert relerr(u, u0).max() <= 1e-14
+
+
+@pytest.mark.parametrize("nd", [2, 3])
+def test_iso_amr_remesh_vs_oracle(nd):
+    """A mesh update of an isothermal MHD run (random refine / keep /
+    derefine flags through update_mesh's selection): the prolongations take
+    the ghost cells as prim2cons of the isothermal primitives and keep only
+    the density positive; within round-off of the oracle's update."""
+    p, meta, u0 = _iso_amr_case(nd, "mhd", "hlld")
+    o = ol.OracleAmr(p, meta)
+    o.set_blocks(u0)
+    o.run(2)
+    uo = o.get_blocks()
+    crit = np.random.default_rng(3 + nd).integers(-1, 2, len(meta))
+    meta2 = mesh.update_leaves(p, meta, crit, 1, 3)
+    assert len(meta2) != len(meta)
+    ref = o.remesh(meta2).get_blocks()
+    print(f"[case] iso remesh {nd}D blocks {len(meta)} -> {len(meta2)}", flush=True)
+    with AmrSolver(p, meta) as s:
+        s.upload(uo)
+        s.remesh(meta2)
+        ud = s.download()
+    err = relerr(ud.reshape(-1, p.nv), ref.reshape(-1, p.nv), axis0=False).max()
+    print(f"  max rel {err:.2e}")
+    assert err <= 1e-13
