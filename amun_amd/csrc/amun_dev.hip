@@ -36,6 +36,7 @@ struct amun_ctx {
   void* vsrc = nullptr;        // viscous and resistive source products per stage
   void* q4 = nullptr;          // four-velocity states of the stage input (relativistic)
   int vis = 0;                 // 1 viscosity, 2 resistivity (SweepArgs::vis)
+  int fuse = 0;                // one k_stage launch per stage (amun_dev_set_stage_fusion)
   void* mlps = nullptr;        // multi-dimensional limiting: slopes of the interior cells
   void* mlpg = nullptr;        // and of the cells beyond the pencil ends (k_mlp)
   int qcur = 0;                // Q[qcur] holds the primitive state of U0
@@ -1313,7 +1314,7 @@ static std::vector<StageDef> stage_table(const amun_ctx* c) {
 }
 
 static int kernels_per_step(const amun_ctx* c) {
-  return stages_per_step(c) * c->rc.nd + 1;
+  return stages_per_step(c) * (c->fuse ? 1 : c->rc.nd) + 1;
 }
 
 // Enqueue one whole step for cmax slot `slot` with the primitive state in
@@ -1499,7 +1500,19 @@ static int enqueue_step(amun_ctx* c, double dtnext, int slot, int qcur, hipEvent
     a.ca = stages[s].ca;
     a.cb = stages[s].cb;
     a.dtnext = dtnext;
-    for (int dir = 0; dir < rc.nd; dir++) {
+    if (c->fuse) {
+      // every workgroup of 256 lanes holds whole blocks in each direction
+      a.first = (s == 0);
+      a.swz = c->swz[0];
+      const dim3 grid((unsigned)((nthreads + 255) / 256)), block(256);
+      const size_t lds = std::max(sweep_lds(c, 0, 256), sweep_lds(c, 1, 256));
+      hipLaunchKernelGGL(d.stage, grid, block, lds, c->stream, a, rc);
+      if (a.last)
+        hipLaunchKernelGGL(k_reduce_speed, dim3(1), dim3(1024), 0, c->stream, c->spart,
+                           (int)grid.x, c->rs, slot ^ 1, rc.nruns);
+      if (ev) HIP_TRY(hipEventRecord(*ev++, c->stream));
+    }
+    for (int dir = 0; dir < rc.nd && !c->fuse; dir++) {
       const int wg = c->wg[dir];
       a.first = (s == 0 && dir == 0);
       if (d.compact[dir]) {
@@ -1670,8 +1683,28 @@ int amun_dev_set_graph(amun_ctx* c, int32_t enable) {
   return 0;
 }
 
+int amun_dev_set_stage_fusion(amun_ctx* c, int32_t on) {
+  if (!c) return fail(500, "null argument");
+  if (on) {
+    const int npen = c->rc.nd == 3 ? c->rc.nc * c->rc.nc : c->rc.nc;
+    if (!c->disp.stage)
+      return fail(600, "the fused stage is built for the TVD and WENO5-Z reconstructions of "
+                       "the non-relativistic systems");
+    if (256 % npen)
+      return fail(600, "the fused stage needs whole blocks per workgroup of 256 lanes "
+                       "(ncells^(ndims-1) dividing 256)");
+  }
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->fuse = on ? 1 : 0;
+  drop_graphs(c);
+  return 0;
+}
+
 int amun_dev_autotune(amun_ctx* c, int32_t nsteps, int32_t* choice) {
   if (!c || nsteps < 1) return fail(500, "bad argument");
+  if (c->fuse)
+    return fail(600, "autotune chooses the per-direction launches: call it with the fused "
+                     "stage off");
   const int nd = c->rc.nd, nr = c->rc.nruns;
   HIP_TRY(hipStreamSynchronize(c->stream));
   // the candidates run real steps on the live state, which is restored

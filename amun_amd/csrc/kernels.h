@@ -1036,8 +1036,12 @@ constexpr int sweep_min_waves() { return sizeof(T) == 4 ? 2 : 1; }
 template <typename T>
 constexpr bool sweep_chunk_d() { return sizeof(T) == 8; }
 
+// the sweep along DIR of one workgroup: its lanes walk the pencils
+// workgroup_index(swz) * blockDim.x + threadIdx.x along DIR.  Every thread
+// of the workgroup must call it (the last sweep reduces the maximum speed
+// over the workgroup).
 template <class P, int REC, int RS, int ND, int DIR, typename T>
-__global__ __launch_bounds__(256, sweep_min_waves<T>()) void k_sweep(SweepArgs a, RunConst rc) {
+__device__ __forceinline__ void sweep_body(const SweepArgs& a, const RunConst& rc) {
   constexpr int NV = P::nv;
   constexpr bool kLast = (DIR == ND - 1);
   using RC = Recon<P, REC, ND, DIR, T>;
@@ -1370,6 +1374,30 @@ __global__ __launch_bounds__(256, sweep_min_waves<T>()) void k_sweep(SweepArgs a
 
   if constexpr (kLast) {
     if (a.last) wg_max_speed(a.rs, a.spart, run, a.slot ^ 1, smax);
+  }
+}
+
+template <class P, int REC, int RS, int ND, int DIR, typename T>
+__global__ __launch_bounds__(256, sweep_min_waves<T>()) void k_sweep(SweepArgs a, RunConst rc) {
+  sweep_body<P, REC, RS, ND, DIR, T>(a, rc);
+}
+
+// One launch per stage: when a workgroup's pencils are whole blocks in
+// every direction (npen divides the workgroup size), the sweeps of one
+// block depend only on each other through its own increment D (each
+// direction reads the stage input Q, including the neighbours' halo, which
+// no sweep of the stage writes), so one workgroup runs its blocks' x, y (and
+// z) sweeps in turn with a workgroup barrier between them; D stays in the
+// caches between the directions.
+template <class P, int REC, int RS, int ND, typename T>
+__global__ __launch_bounds__(256, sweep_min_waves<T>()) void k_stage(SweepArgs a, RunConst rc) {
+  sweep_body<P, REC, RS, ND, 0, T>(a, rc);
+  __syncthreads();
+  a.first = 0;
+  sweep_body<P, REC, RS, ND, 1, T>(a, rc);
+  if constexpr (ND == 3) {
+    __syncthreads();
+    sweep_body<P, REC, RS, ND, 2, T>(a, rc);
   }
 }
 
@@ -1855,6 +1883,7 @@ using namespace amun_dev;
 // kernel instantiations selected for one configuration
 struct Dispatch {
   SweepFn sweep[3] = {nullptr, nullptr, nullptr};
+  SweepFn stage = nullptr;   // all directions of a stage in one launch (k_stage)
   void (*compact[3])(const void*, const int*, RunConst, DevState*) = {nullptr, nullptr, nullptr};
   void (*prim)(const double*, void*, RunConst, DevState*, RunState*, const int*, int, int,
                int) = nullptr;
@@ -1899,6 +1928,9 @@ void fill(Dispatch& d) {
   d.sweep[0] = k_sweep<P, REC, RS, ND, 0, T>;
   d.sweep[1] = k_sweep<P, REC, RS, ND, 1, T>;
   if constexpr (ND == 3) d.sweep[2] = k_sweep<P, REC, RS, ND, 2, T>;
+  // the fused stage for the reconstructions of the production workloads
+  if constexpr ((REC == REC_TVD || REC == REC_WENO5Z) && !P::sr)
+    d.stage = k_stage<P, REC, RS, ND, T>;
   d.prim = k_prim<P, T>;
   d.integ = k_integrals<P>;
   d.stats = k_stats<P, T>;

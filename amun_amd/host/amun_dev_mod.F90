@@ -174,6 +174,13 @@ module amun_dev_mod
       integer(c_int64_t), dimension(3), intent(out)  :: n
     end function
 
+    integer(c_int) function amun_dev_set_stage_fusion(ctx, enable) &
+                                       bind(C, name="amun_dev_set_stage_fusion")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value        :: ctx
+      integer(c_int32_t), value :: enable
+    end function
+
     type(c_ptr) function amun_dev_last_error_c() &
                                              bind(C, name="amun_dev_last_error")
       import :: c_ptr

@@ -103,7 +103,7 @@ def main():
     wc, _ = counters(a.write)
     vc, vres = counters(a.valu)
     res.update(vres)
-    sweeps = sorted(k for k in st if "k_sweep" in k)
+    sweeps = sorted(k for k in st if "k_sweep" in k or "k_stage" in k)
     lines = [f"# rocprofv3 summary: 3D MHD-GLM WENO5-Z+HLLD SSPRK(3,3), {a.size}^3 "
              f"({cells // 4096} blocks of 16^3), precision {a.precision}"]
     lines += ["# " + c for c in a.commands.split("\n") if c.strip()]
@@ -142,13 +142,14 @@ def main():
         lines.append(f"  {k[:62]:62s} fp64 {a64:7.2f} GFLOP  fp32 {a32:7.2f} GFLOP  "
                      f"VALU-active/wave-cycles {act / cyc if cyc else 0:5.2f}  "
                      f"wait-any/wave-cycles {wait / cyc if cyc else 0:5.2f}")
-    # a step is three stages of one launch per direction
+    # a step is three stages of one launch per direction (k_sweep) or one
+    # launch each (k_stage)
     nst = 3
     step_us = sum(st[k][1] for k in sweeps) * nst
     step_bytes = (rd + wr) * 1e9 * nst
     bpz = 8 * 9 * 8
     lines.append("")
-    lines.append(f"step ({3 * len(sweeps)} sweep launches): {step_us:.1f} us; HBM traffic "
+    lines.append(f"step ({3 * len(sweeps)} launches): {step_us:.1f} us; HBM traffic "
                  f"{step_bytes / 1e9:.2f} GB = {step_bytes / cells:.0f} B/zone-update "
                  f"(algorithmic {bpz} B) -> {step_bytes / step_us / 1e3:.0f} GB/s measured; "
                  f"algorithmic {bpz * cells / step_us / 1e3:.0f} GB/s = "

@@ -35,7 +35,7 @@ EXPORTS = (
     "amun_dev_unit_eqs", "amun_dev_tape_forward", "amun_dev_tape_steps",
     "amun_dev_tape_backward", "amun_dev_tape_tangent", "amun_dev_run_fixed",
     "amun_dev_init_amr", "amun_dev_amr_info", "amun_dev_statistics",
-    "amun_dev_statistics_run",
+    "amun_dev_statistics_run", "amun_dev_set_stage_fusion",
 )
 
 
@@ -96,6 +96,7 @@ def load_library(path: str = LIB_PATH):
     L.amun_dev_integrals_run.argtypes = [vp, ctypes.c_int32, _P]
     L.amun_dev_set_graph.argtypes = [vp, ctypes.c_int32]
     L.amun_dev_autotune.argtypes = [vp, ctypes.c_int32, i32p]
+    L.amun_dev_set_stage_fusion.argtypes = [vp, ctypes.c_int32]
     L.amun_dev_set_launch.argtypes = [vp, i32p]
     L.amun_dev_tape_forward.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_double]
     L.amun_dev_tape_steps.argtypes = [vp, _P, _P]
@@ -240,6 +241,12 @@ class Solver:
     def set_graph(self, enable: bool = True):
         """Capture steps into HIP graphs and replay them."""
         self._check(self.lib.amun_dev_set_graph(self.h, int(enable)))
+
+    def set_stage_fusion(self, enable: bool = True):
+        """One launch per stage: the x, y (and z) sweeps of a stage in one
+        kernel whose workgroups hold whole blocks (bitwise the same
+        results)."""
+        self._check(self.lib.amun_dev_set_stage_fusion(self.h, int(enable)))
 
     def autotune(self, nsteps: int = 3):
         """Time every launch shape on the live state (which is restored) and

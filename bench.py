@@ -148,6 +148,12 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the mixed-precision leg reported beside the FP64 value")
+    ap.add_argument("--launch", default="fused", choices=["fused", "split"],
+                    help="one launch per stage (x, y, z sweeps in one kernel, "
+                         "amun_dev_set_stage_fusion) or one launch per sweep direction; "
+                         "the other is reported beside the value")
+    ap.add_argument("--no-other-launch", action="store_true",
+                    help="skip the FP64 leg of the other launch mode")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-c3", action="store_true",
                     help="skip the batched/AMR legs (64 KH grids, 6224-leaf AMR mesh)")
@@ -189,11 +195,14 @@ def main():
     print(f"[bench] building {args.size}^3 initial state", file=sys.stderr, flush=True)
     u0 = initial_state(p)
 
-    def measure(params):
+    def measure(params, launch):
+        fused = launch == "fused"
         s = Solver(params)
+        if fused:
+            s.set_stage_fusion(True)
         s.upload(u0)
-        print(f"[bench] {params.precision}: warmup {args.warmup} steps", file=sys.stderr,
-              flush=True)
+        tag = params.precision + (" fused" if fused else "")
+        print(f"[bench] {tag}: warmup {args.warmup} steps", file=sys.stderr, flush=True)
         s.run(args.warmup)
         s.sync()
         sync_all()
@@ -202,18 +211,27 @@ def main():
         s.sync()
         sync_all()
         wall = time.perf_counter() - t0
-        print(f"[bench] {params.precision}: {args.steps} steps: wall {wall:.3f} s, "
+        print(f"[bench] {tag}: {args.steps} steps: wall {wall:.3f} s, "
               f"device {dev_ms:.1f} ms", file=sys.stderr, flush=True)
         # per-kernel averages over a few more steps (same stream, HIP events)
         _, kms = s.timed_run(3, per_kernel=True)
         s.close()
         return wall, dev_ms, kms
 
-    wall, dev_ms, kms = measure(p)
+    wall, dev_ms, kms = measure(p, args.launch)
+    # the same step in the other launch mode (bitwise the same results)
+    other_name = "split" if args.launch == "fused" else "fused"
+    other = None
+    if not args.no_other_launch:
+        wo, do, ko = measure(p, other_name)
+        other = {"value": cells * args.steps / wo, "ms_per_step": wo / args.steps * 1e3,
+                 "device_ms_per_step": do / args.steps,
+                 f"speedup_of_{args.launch}": wo / wall,
+                 "kernel_ms_per_step": [round(float(x), 4) for x in ko]}
     mixed = None
     if args.precision == "fp64" and not args.no_mixed:
         pm = c2_params(args.size, precision="mixed")
-        wm, dm, km = measure(pm)
+        wm, dm, km = measure(pm, args.launch)
         mixed = {"value": cells * args.steps / wm, "ms_per_step": wm / args.steps * 1e3,
                  "speedup_vs_fp64": wall / wm,
                  "roofline_frac": bytes_per_zone_update(pm) * cells / (dm / args.steps * 1e-3)
@@ -226,7 +244,8 @@ def main():
     achieved = bpz * cells / (step_ms * 1e-3) / 1e9
     traffic = None
     valu = None
-    tf = os.path.join(ROOT, "profiles", f"traffic_c2_{args.size}_{args.precision}.json")
+    tf = os.path.join(ROOT, "profiles",
+                      f"traffic_c2_{args.size}_{args.precision}_{args.launch}.json")
     if os.path.exists(tf):
         prof = json.load(open(tf))
         traffic = prof.get("hbm_bytes_per_step")
@@ -246,7 +265,9 @@ def main():
         roofline = {"bound": "valu", "achieved": valu["achieved"], "peak": valu["peak"],
                     "unit": valu["unit"], "frac": valu["frac"], "hbm": hbm}
     roofline.update({"traffic": traffic,
-                     "kernel": "one SSPRK(3,3) step = 9 k_sweep launches + k_end_step",
+                     "kernel": ("one SSPRK(3,3) step = 3 k_stage launches + k_end_step"
+                                if args.launch == "fused" else
+                                "one SSPRK(3,3) step = 9 k_sweep launches + k_end_step"),
                      "bytes_per_zone_update": bpz,
                      "device_ms_per_step": step_ms,
                      "kernel_ms_per_step": [round(float(x), 4) for x in kms]})
@@ -265,11 +286,13 @@ def main():
         "data": "synthetic (seeded random-phase Fourier v and B, rho=1, p=0.6)",
         "config": {"workload": f"3D MHD-GLM WENO5-Z+HLLD SSPRK(3,3) {args.size}^3 "
                                f"as {p.nblocks} blocks of 16^3, CFL 0.3, periodic",
-                   "precision": args.precision, "cells": cells},
+                   "precision": args.precision, "launch": args.launch, "cells": cells},
         "roofline": roofline,
     }
     if valu is not None:
         out["valu"] = valu
+    if other is not None:
+        out[other_name] = other
     if mixed is not None:
         out["mixed"] = mixed
     if not args.no_c3:

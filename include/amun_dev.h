@@ -278,6 +278,15 @@ int  amun_dev_set_graph(amun_ctx* ctx, int32_t enable);
  * be NULL) receives ndims pairs (workgroup size, XCD grouping).
  * amun_dev_set_launch sets such pairs directly. */
 int  amun_dev_autotune(amun_ctx* ctx, int32_t nsteps, int32_t* choice);
+
+/* One launch per stage (no reference counterpart; the reference's evolve_*
+ * calls update_fluxes once per stage, evolution.F90:628-1208): with
+ * enable != 0 every stage runs its x, y (and z) sweeps in one kernel whose
+ * workgroups hold whole blocks, with a workgroup barrier between the
+ * directions.  For the TVD and WENO5-Z reconstructions of the HD and MHD
+ * systems when ncells^(ndims-1) divides 256 (600 otherwise); the results
+ * are bitwise those of the per-direction launches. */
+int  amun_dev_set_stage_fusion(amun_ctx* ctx, int32_t enable);
 int  amun_dev_set_launch(amun_ctx* ctx, const int32_t* choice);
 
 /* Release every device allocation of the context (finalize_* routines). */
