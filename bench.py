@@ -92,24 +92,44 @@ def cpu_baseline(p_full):
                                    f"{n1} steps, FP64, {secs1:.2f} s on one host thread"}}
 
 
-def c3_legs(steps: int, warmup: int):
+def c3_legs(steps: int, warmup: int, launch: str):
     """The batched/AMR configuration beside the headline: the 64-grid KH
     batch (one block pool, per-run dt) and the 3-level 3D AMR mesh of 6224
-    leaves of 16^3, FP64 and mixed, steps replayed from captured graphs;
-    zone-updates/s over all interior cells of the batch or leaves."""
-    from amun_amd.config import c3_batch_params
+    leaves of 16^3, FP64 and mixed, and the 2D HD 128^2 grid (launch-bound),
+    steps replayed from captured graphs; zone-updates/s over all interior
+    cells.  Each is timed with one launch per sweep direction and with one
+    launch per stage; `value` and `ms_per_step` follow `launch`."""
+    from amun_amd.config import c1_params, c3_batch_params
     from amun_amd import mesh
-    from amun_amd.problems import kelvin_helmholtz_2d
-    from amun_amd.solver import AmrSolver, Batch
+    from amun_amd.problems import initial_state, kelvin_helmholtz_2d
+    from amun_amd.solver import AmrSolver, Batch, Solver
 
-    def timed(s):
-        s.set_graph(True)
-        s.run(warmup)
-        s.sync()
-        t0 = time.perf_counter()
-        s.run(steps)
-        s.sync()
-        return time.perf_counter() - t0
+    def timed(make, upload, fused, nsteps):
+        with make() as s:
+            upload(s)
+            if fused:
+                s.set_stage_fusion(True)
+            s.set_graph(True)
+            s.run(warmup)
+            s.sync()
+            t0 = time.perf_counter()
+            s.run(nsteps)
+            s.sync()
+            return time.perf_counter() - t0, s.info() if hasattr(s, "info") else {}
+
+    def leg(make, upload, cells, nsteps=steps, **extra):
+        ws, info = timed(make, upload, False, nsteps)
+        wf, _ = timed(make, upload, True, nsteps)
+        w = wf if launch == "fused" else ws
+        return {"value": cells * nsteps / w, "unit": "zone-updates/s",
+                "ms_per_step": w / nsteps * 1e3, "ms_per_step_split": ws / nsteps * 1e3,
+                "ms_per_step_fused": wf / nsteps * 1e3, "cells": cells, **extra}, info
+
+    def upload_batch(runs):
+        def up(b):
+            for r, p in enumerate(runs):
+                b.upload(r, kelvin_helmholtz_2d(p, seed=r))
+        return up
 
     out = {}
     for prec in ("fp64", "mixed"):
@@ -117,25 +137,22 @@ def c3_legs(steps: int, warmup: int):
         cells = sum(p.ncells_total for p in runs)
         print(f"[bench] c3 batch {prec}: {len(runs)} runs, {cells} cells", file=sys.stderr,
               flush=True)
-        with Batch(common, runs) as b:
-            for r, p in enumerate(runs):
-                b.upload(r, kelvin_helmholtz_2d(p, seed=r))
-            wall = timed(b)
-        out[f"batch_{prec}"] = {"value": cells * steps / wall, "unit": "zone-updates/s",
-                                "ms_per_step": wall / steps * 1e3, "cells": cells,
-                                "runs": len(runs)}
+        out[f"batch_{prec}"], _ = leg(lambda: Batch(common, runs), upload_batch(runs), cells,
+                                      runs=len(runs))
         p, meta = mesh.c3_amr_params(prec)
         leaves = len(meta)
         cells = leaves * p.ncells ** 3
         print(f"[bench] c3 amr {prec}: {leaves} leaves, {cells} cells", file=sys.stderr,
               flush=True)
-        with AmrSolver(p, meta) as s:
-            s.upload(mesh.block_state(p, meta, mesh.blast_prim(p, r0=0.2)))
-            info = s.info()
-            wall = timed(s)
-        out[f"amr_{prec}"] = {"value": cells * steps / wall, "unit": "zone-updates/s",
-                              "ms_per_step": wall / steps * 1e3, "cells": cells,
-                              "leaves": leaves, "virtual_blocks": info["virtual_blocks"]}
+        u0 = mesh.block_state(p, meta, mesh.blast_prim(p, r0=0.2))
+        out[f"amr_{prec}"], info = leg(lambda: AmrSolver(p, meta), lambda s: s.upload(u0),
+                                       cells, leaves=leaves)
+        out[f"amr_{prec}"]["virtual_blocks"] = info["virtual_blocks"]
+    p1 = c1_params()
+    u1 = initial_state(p1)
+    print("[bench] c1 2D HD 128^2 fp64", file=sys.stderr, flush=True)
+    out["c1_fp64"], _ = leg(lambda: Solver(p1), lambda s: s.upload(u1), p1.ncells_total,
+                            nsteps=20 * steps)
     return out
 
 
@@ -296,7 +313,7 @@ def main():
     if mixed is not None:
         out["mixed"] = mixed
     if not args.no_c3:
-        out["c3"] = c3_legs(args.steps, args.warmup)
+        out["c3"] = c3_legs(args.steps, args.warmup, args.launch)
     if not args.no_cpu_baseline:
         print("[bench] CPU baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(p)
