@@ -287,6 +287,14 @@ program amun_host
                                    ": ", trim(amun_dev_last_error())
     stop 2
   end if
+  ! one launch per stage (bitwise the per-direction launches' results) where
+  ! the library builds it; the configurations it does not cover return 600
+  ! and keep the per-direction launches
+  str = get_str("stage_fusion", "on")
+  if (trim(str) == "on" .or. trim(str) == "yes" .or. trim(str) == "true") then
+    iret = amun_dev_set_stage_fusion(ctx, 1)
+    if (iret /= 0 .and. iret /= 600) call die("set_stage_fusion")
+  end if
   iret = amun_dev_block_coords(ctx, coords)
   call scatter()
   iret = amun_dev_upload_blocks(ctx, u)
